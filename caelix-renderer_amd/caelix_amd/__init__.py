@@ -1,0 +1,197 @@
+"""Python binding of libcaelix_amd.so (the MI355X-native Caelix hot path) over its C ABI.
+
+torch is used only as plumbing: device memory (uint8 tensors) and the current HIP stream. It is
+imported before the library is loaded, so that the library binds to the HIP runtime torch already
+loaded (both carry the soname libamdhip64.so.7) and device pointers are shared. A missing library is
+an error: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+from . import layout
+
+PACKAGE_DIR = Path(__file__).resolve().parent.parent
+LIBRARY_PATH = PACKAGE_DIR / "libcaelix_amd.so"
+
+CX_OK = 0
+
+_lib = None
+
+
+class CxError(RuntimeError):
+    pass
+
+
+def _torch():
+    import torch  # noqa: F401  (loads torch's HIP runtime first)
+    return torch
+
+
+class CxLightBvhBuild(ctypes.Structure):
+    _fields_ = [
+        ("leaves", ctypes.c_void_p),
+        ("nodes", ctypes.c_void_p),
+        ("header", ctypes.c_void_p),
+        ("state", ctypes.c_void_p),
+        ("scratch", ctypes.c_void_p),
+        ("leaf_capacity", ctypes.c_uint32),
+        ("node_capacity", ctypes.c_uint32),
+    ]
+
+
+# Exported C symbols and their signatures (include/caelix_amd/caelix_amd.h).
+_P = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+_U64 = ctypes.c_uint64
+_INT = ctypes.c_int
+SIGNATURES = {
+    "cx_set_log_callback": (None, [_P, _P]),
+    "cx_version_string": (ctypes.c_char_p, []),
+    "cx_light_bvh_scratch_bytes": (_U64, [_U32]),
+    "cx_light_bvh_scratch_init": (_INT, [_P, _U32, _P]),
+    "cx_light_bvh_begin": (_INT, [_P, _P, _U32, _U32, _U32, _P]),
+    "cx_light_bvh_build": (_INT, [ctypes.POINTER(CxLightBvhBuild), _P]),
+    "cx_radix_sort_state_bytes": (_U64, [_U32]),
+    "cx_radix_sort_pairs": (_INT, [_P, _P, _P, _P, _P, _P, _U32, _P, _P]),
+    "cx_profile_enable": (None, [_INT]),
+    "cx_profile_reset": (None, []),
+    "cx_profile_read": (_INT, [_U32, _P, _P, _P, _P]),
+}
+
+
+def load(path: Path | str | None = None):
+    """Loads libcaelix_amd.so once (after torch) and declares the C signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    _torch()
+    lib_path = Path(path) if path is not None else LIBRARY_PATH
+    if not lib_path.exists():
+        raise CxError(f"{lib_path} is missing: build it with caelix-renderer_amd/build.py "
+                      "(there is no CPU fallback)")
+    lib = ctypes.CDLL(str(lib_path))
+    for name, (restype, argtypes) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    _lib = lib
+    return lib
+
+
+def check(code: int, what: str) -> None:
+    if code != CX_OK:
+        raise CxError(f"{what} failed with code {code}")
+
+
+def stream_handle(stream=None) -> int:
+    torch = _torch()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def device_bytes(nbytes: int, device="cuda"):
+    torch = _torch()
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+
+def to_device(array: np.ndarray, device="cuda"):
+    torch = _torch()
+    raw = np.ascontiguousarray(array).view(np.uint8).reshape(-1)
+    return torch.from_numpy(raw.copy()).to(device)
+
+
+def to_host(tensor, dtype=np.uint8) -> np.ndarray:
+    raw = tensor.detach().cpu().numpy().view(np.uint8).reshape(-1)
+    return raw.view(dtype)
+
+
+class LightBvhBuilder:
+    """Device buffers of one light tree and its build (cx_light_bvh_build), sized for a leaf
+    capacity: leaves, nodes, header, state, scratch. Mirrors the buffers LightTree owns for one tree
+    (lib/src/light/light_tree.h:37-101)."""
+
+    def __init__(self, leaf_capacity: int, node_capacity: int | None = None, device="cuda"):
+        if leaf_capacity <= 0 or leaf_capacity % 64 != 0:
+            raise ValueError("leaf capacity must be a positive multiple of 64")
+        self.lib = load()
+        self.leaf_capacity = leaf_capacity
+        self.node_capacity = node_capacity if node_capacity is not None else layout.node_capacity_for(leaf_capacity)
+        self.leaves = device_bytes(leaf_capacity * 64, device)
+        self.nodes = device_bytes(self.node_capacity * 224, device)
+        self.header = device_bytes(64, device)
+        self.state = device_bytes(64, device)
+        self.scratch_bytes = int(self.lib.cx_light_bvh_scratch_bytes(leaf_capacity))
+        self.scratch = device_bytes(self.scratch_bytes, device)
+        self.header.zero_()
+        self.state.zero_()
+        self.nodes.zero_()
+        check(self.lib.cx_light_bvh_scratch_init(self.scratch.data_ptr(), leaf_capacity, stream_handle()),
+              "cx_light_bvh_scratch_init")
+        self._desc = CxLightBvhBuild(self.leaves.data_ptr(), self.nodes.data_ptr(), self.header.data_ptr(),
+                                     self.state.data_ptr(), self.scratch.data_ptr(), leaf_capacity,
+                                     self.node_capacity)
+
+    def upload(self, leaves: np.ndarray, building: int = 1) -> None:
+        """Writes the leaves (LEAF records) and sets state.leaves_found / building."""
+        n = len(leaves)
+        if n > 0:
+            self.leaves[: n * 64].copy_(to_device(leaves.view(np.uint8), self.leaves.device))
+        state = np.zeros(1, layout.STATE)
+        state["building"] = building
+        state["leaves_found"] = n
+        host = to_host(self.state).view(layout.STATE).copy()
+        host["building"] = building
+        host["leaves_found"] = n
+        self.state.copy_(to_device(host.view(np.uint8), self.state.device))
+
+    def build(self, stream=None) -> None:
+        check(self.lib.cx_light_bvh_build(ctypes.byref(self._desc), stream_handle(stream)), "cx_light_bvh_build")
+
+    def read(self) -> dict:
+        """Copies every output back: header, state, nodes, leaves, scratch (numpy)."""
+        _torch().cuda.synchronize()
+        return {
+            "header": to_host(self.header).view(layout.HEADER)[0].copy(),
+            "state": to_host(self.state).view(layout.STATE)[0].copy(),
+            "nodes": to_host(self.nodes).view(layout.NODE).copy(),
+            "leaves": to_host(self.leaves).view(layout.LEAF).copy(),
+            "scratch": to_host(self.scratch).copy(),
+        }
+
+
+def radix_sort_pairs(keys, values, stream=None):
+    """Sorts device uint32 (key, value) tensors in place (stable by key) with the onesweep kernels."""
+    torch = _torch()
+    lib = load()
+    n = keys.numel()
+    capacity = max(n, 1)
+    keys1 = torch.empty_like(keys)
+    values1 = torch.empty_like(values)
+    count = torch.tensor([n], dtype=torch.int32, device=keys.device)
+    state = device_bytes(int(lib.cx_radix_sort_state_bytes(capacity)), keys.device)
+    check(lib.cx_radix_sort_pairs(keys.data_ptr(), values.data_ptr(), keys1.data_ptr(), values1.data_ptr(),
+                                  count.data_ptr(), None, capacity, state.data_ptr(), stream_handle(stream)),
+          "cx_radix_sort_pairs")
+
+
+def profile_enable(enabled: bool) -> None:
+    load().cx_profile_enable(1 if enabled else 0)
+
+
+def profile_reset() -> None:
+    load().cx_profile_reset()
+
+
+def profile_read() -> dict:
+    lib = load()
+    max_stages = 64
+    names = (ctypes.c_char_p * max_stages)()
+    ms = (ctypes.c_double * max_stages)()
+    launches = (ctypes.c_uint32 * max_stages)()
+    count = ctypes.c_uint32()
+    check(lib.cx_profile_read(max_stages, names, ms, launches, ctypes.byref(count)), "cx_profile_read")
+    return {names[i].decode(): (ms[i], launches[i]) for i in range(min(count.value, max_stages))}

@@ -1,0 +1,968 @@
+// Light BVH build for gfx950 (the build half of LightTree::RecordTree,
+// lib/src/light/light_tree_record.cpp:271-308; kernels shaders/light/bvh/*.hlsl).
+//
+// Launch sequence (one stream, 9 kernels + 4 sort passes):
+//   bounds     grid-stride min/max of the leaf box centres; writes the centres (12 B per leaf, read
+//              again by the Morton kernel instead of the 64-B leaves); the last workgroup reduces the
+//              partials, writes leaf_count / leaves_dropped / bounds and resets the build's counters
+//   morton     30-bit key + leaf index per leaf, the four 8-bit digit histograms of the onesweep
+//              sort, the cleared arrival counters and gap-cover array, the tile status rows
+//   sort x4    onesweep passes (radix_sort.hip); sorted pairs end in sort buffer 0
+//   karras     binary radix tree (Karras 2012) as the reference builds it, plus +1/-1 at the two ends
+//              of every node's leaf range in the gap-cover array and the node's split gap
+//   aggregate  bottom-up union with agent-scope acq_rel arrival counters
+//   cover      inclusive scan of the gap-cover array (decoupled look-back): cover[g] = the number of
+//              binary nodes whose range spans the gap between sorted leaves g and g + 1
+//   flags      depth(i) = cover[split(i)] - 1 (the nodes spanning node i's split gap are i and its
+//              ancestors), even-depth flags, their exclusive scan = 4-wide node indices, and the
+//              header (the reference's caelix_light_bvh_finish) in the last tile
+//   wide       4-wide nodes from the even-depth binary nodes
+// The reference computes the depth by walking every node up to the root (O(n * depth) dependent
+// loads, caelix_light_bvh_depth.hlsl:7-30); the gap cover gives the same depths with two atomics per
+// node and one scan.
+#include "device_common.h"
+#include "radix_sort.h"
+#include "runtime.h"
+
+namespace cx {
+namespace {
+
+constexpr uint32_t kBvhThreads = 256;
+constexpr uint32_t kBoundsMaxBlocks = 1024;
+constexpr uint32_t kScanThreads = 256;
+constexpr uint32_t kScanItems = 16;
+constexpr uint32_t kScanTile = kScanThreads * kScanItems;
+
+// Scan tile status (64-bit): flag in bits 32..33, signed 32-bit value in bits 0..31.
+constexpr uint64_t kScanAggregate = 1ull << 32;
+constexpr uint64_t kScanPrefix = 2ull << 32;
+constexpr uint64_t kScanFlagMask = 3ull << 32;
+
+// Build control words (u32) at the start of the sort-state region of the scratch.
+enum : uint32_t
+{
+    kCtlBoundsTicket = 0,
+    kCtlLeafCount = 1,
+    kCtlCoverTiles = 2,
+    kCtlFlagsTiles = 3,
+    kCtlWords = 64,
+};
+
+struct Scratch
+{
+    uint8_t* base;
+    uint32_t capacity;
+    __host__ __device__ uint32_t* u32(uint32_t offset) const { return reinterpret_cast<uint32_t*>(base + uint64_t(offset) * capacity); }
+    __host__ __device__ uint32_t* keys(uint32_t buffer) const { return u32(kLtScratchKeys + buffer * kLtScratchSortBufferStride); }
+    __host__ __device__ uint32_t* values(uint32_t buffer) const { return u32(kLtScratchValues + buffer * kLtScratchSortBufferStride); }
+    __host__ __device__ float* aggregates() const { return reinterpret_cast<float*>(base + uint64_t(kLtScratchAggregates) * capacity); }
+    // Free arrays reused by this build: the leaf centres live in the aggregates array until the
+    // aggregation; the gap-cover diffs in `wide` until the flags scan; after the sort, sort buffer 1
+    // holds the cover counts (keys) and every node's split gap (values).
+    __host__ __device__ float* centres() const { return aggregates(); }
+    __host__ __device__ int32_t* cover_diff() const { return reinterpret_cast<int32_t*>(u32(kLtScratchWide)); }
+    __host__ __device__ int32_t* cover() const { return reinterpret_cast<int32_t*>(keys(1)); }
+    __host__ __device__ uint32_t* split() const { return values(1); }
+    __host__ __device__ uint8_t* state_base() const { return base + uint64_t(kLtScratchSortState) * capacity; }
+    __host__ __device__ uint32_t* ctl() const { return reinterpret_cast<uint32_t*>(state_base()); }
+    __host__ __device__ float* bounds_partials() const { return reinterpret_cast<float*>(state_base() + kCtlWords * 4); }
+    __host__ __device__ uint64_t* cover_status() const
+    {
+        return reinterpret_cast<uint64_t*>(state_base() + kCtlWords * 4 + kBoundsMaxBlocks * 32);
+    }
+    __host__ __device__ uint32_t scan_tiles() const { return DivUp(capacity, kScanTile); }
+    __host__ __device__ uint64_t* flags_status() const { return cover_status() + scan_tiles(); }
+    __host__ __device__ uint8_t* sort_base() const
+    {
+        const uint64_t offset = kCtlWords * 4 + kBoundsMaxBlocks * 32 + uint64_t(2) * scan_tiles() * 8;
+        return state_base() + ((offset + 255) & ~uint64_t(255));
+    }
+    __host__ __device__ SortState sort() const { return SortStateAt(sort_base(), capacity); }
+    static uint64_t Bytes(uint32_t capacity)
+    {
+        const uint32_t tiles = DivUp(capacity, kScanTile);
+        const uint64_t offset = kCtlWords * 4 + kBoundsMaxBlocks * 32 + uint64_t(2) * tiles * 8;
+        return uint64_t(kLtScratchSortState) * capacity + ((offset + 255) & ~uint64_t(255)) +
+               SortStateBytes(capacity);
+    }
+};
+
+struct Build
+{
+    LightBvhLeaf* leaves;
+    LightBvhNode* nodes;
+    LightTreeHeader* header;
+    LightTreeState* state;
+    Scratch scratch;
+    uint32_t leaf_capacity;
+    uint32_t node_capacity;
+};
+
+CX_DEV bool Building(const Build& b)
+{
+    return b.state->building != 0;
+}
+
+struct Summary
+{
+    float face[6];
+    float lo[3];
+    float hi[3];
+};
+
+CX_DEV Summary LoadSummary(const float* p)
+{
+    const float4 a = reinterpret_cast<const float4*>(p)[0];
+    const float4 b = reinterpret_cast<const float4*>(p)[1];
+    const float4 c = reinterpret_cast<const float4*>(p)[2];
+    Summary s;
+    s.face[0] = a.x;
+    s.face[1] = a.y;
+    s.face[2] = a.z;
+    s.face[3] = a.w;
+    s.face[4] = b.x;
+    s.face[5] = b.y;
+    s.lo[0] = b.z;
+    s.lo[1] = b.w;
+    s.lo[2] = c.x;
+    s.hi[0] = c.y;
+    s.hi[1] = c.z;
+    s.hi[2] = c.w;
+    return s;
+}
+
+// Coherent load of a binary aggregate written in the same launch (agent-scope word loads).
+CX_DEV Summary LoadSummaryCoherent(const float* p)
+{
+    const uint64_t* q = reinterpret_cast<const uint64_t*>(p);
+    uint64_t w[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+    {
+        w[i] = LoadRelaxed(q + i);
+    }
+    Summary s;
+    __builtin_memcpy(&s, w, sizeof(s));
+    return s;
+}
+
+CX_DEV void StoreSummaryCoherent(float* p, const Summary& s)
+{
+    uint64_t w[6];
+    __builtin_memcpy(w, &s, sizeof(s));
+    uint64_t* q = reinterpret_cast<uint64_t*>(p);
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+    {
+        StoreRelaxed(q + i, w[i]);
+    }
+}
+
+CX_DEV void StoreSummary(float* p, const Summary& s)
+{
+    float4* q = reinterpret_cast<float4*>(p);
+    q[0] = make_float4(s.face[0], s.face[1], s.face[2], s.face[3]);
+    q[1] = make_float4(s.face[4], s.face[5], s.lo[0], s.lo[1]);
+    q[2] = make_float4(s.lo[2], s.hi[0], s.hi[1], s.hi[2]);
+}
+
+CX_DEV Summary EmptySummary()
+{
+    Summary s;
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+    {
+        s.face[f] = 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        s.lo[k] = 0.0f;
+        s.hi[k] = 0.0f;
+    }
+    return s;
+}
+
+CX_DEV float Power(const Summary& s)
+{
+    return s.face[0] + s.face[1] + s.face[2] + s.face[3] + s.face[4] + s.face[5];
+}
+
+// LtUnion (shaders/common/caelix_light_tree_layout.hlsl:315-334).
+CX_DEV Summary Union(const Summary& a, const Summary& b)
+{
+    if (Power(a) <= 0.0f)
+    {
+        return b;
+    }
+    if (Power(b) <= 0.0f)
+    {
+        return a;
+    }
+    Summary s;
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+    {
+        s.face[f] = a.face[f] + b.face[f];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        s.lo[k] = fminf(a.lo[k], b.lo[k]);
+        s.hi[k] = fmaxf(a.hi[k], b.hi[k]);
+    }
+    return s;
+}
+
+// LtBinaryChildSummary (caelix_light_bvh_common.hlsl:94-121).
+CX_DEV Summary ChildSummary(const Build& b, uint32_t child, bool coherent)
+{
+    if ((child & kLtChildLeafBit) != 0)
+    {
+        const uint32_t leaf = b.scratch.values(0)[child & ~kLtChildLeafBit];
+        return LoadSummary(b.leaves[leaf].summary.face);
+    }
+    const float* p = b.scratch.aggregates() + uint64_t(child) * 12;
+    return coherent ? LoadSummaryCoherent(p) : LoadSummary(p);
+}
+
+// ------------------------------------------------------------------------------------------------
+// bounds (caelix_light_bvh_bounds.hlsl:10-53)
+__global__ void __launch_bounds__(kBvhThreads) BoundsKernel(Build b)
+{
+    if (!Building(b))
+    {
+        return;
+    }
+    __shared__ float lds[6][kBvhThreads / kWave];
+    __shared__ bool last;
+    const uint32_t t = threadIdx.x;
+    const uint32_t found = b.state->leaves_found;
+    const uint32_t n = min(found, b.leaf_capacity);
+    float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f};
+    float hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    float* centres = b.scratch.centres();
+    for (uint32_t i = blockIdx.x * kBvhThreads + t; i < n; i += gridDim.x * kBvhThreads)
+    {
+        const float* leaf = b.leaves[i].summary.lo;
+        const float2 l01 = *reinterpret_cast<const float2*>(leaf);      // lo.x, lo.y
+        const float2 l2h0 = *reinterpret_cast<const float2*>(leaf + 2); // lo.z, hi.x
+        const float2 h12 = *reinterpret_cast<const float2*>(leaf + 4);  // hi.y, hi.z
+        const float c0 = 0.5f * (l01.x + l2h0.y);
+        const float c1 = 0.5f * (l01.y + h12.x);
+        const float c2 = 0.5f * (l2h0.x + h12.y);
+        centres[3 * uint64_t(i)] = c0;
+        centres[3 * uint64_t(i) + 1] = c1;
+        centres[3 * uint64_t(i) + 2] = c2;
+        lo[0] = fminf(lo[0], c0);
+        lo[1] = fminf(lo[1], c1);
+        lo[2] = fminf(lo[2], c2);
+        hi[0] = fmaxf(hi[0], c0);
+        hi[1] = fmaxf(hi[1], c1);
+        hi[2] = fmaxf(hi[2], c2);
+    }
+#pragma unroll
+    for (int offset = kWave / 2; offset > 0; offset >>= 1)
+    {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+        {
+            lo[k] = fminf(lo[k], __shfl_xor(lo[k], offset, kWave));
+            hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], offset, kWave));
+        }
+    }
+    const uint32_t wave = t / kWave;
+    if (Lane() == 0)
+    {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+        {
+            lds[k][wave] = lo[k];
+            lds[3 + k][wave] = hi[k];
+        }
+    }
+    __syncthreads();
+    float* partials = b.scratch.bounds_partials();
+    if (t < 6)
+    {
+        float v = lds[t][0];
+        for (uint32_t w = 1; w < kBvhThreads / kWave; ++w)
+        {
+            v = t < 3 ? fminf(v, lds[t][w]) : fmaxf(v, lds[t][w]);
+        }
+        partials[blockIdx.x * 8 + t] = v;
+    }
+    __threadfence();
+    __syncthreads();
+    uint32_t* ctl = b.scratch.ctl();
+    if (t == 0)
+    {
+        last = AtomicAddAcqRel(&ctl[kCtlBoundsTicket], 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last)
+    {
+        return;
+    }
+    __threadfence();
+    // The last workgroup: reduce the partials, write the header/state, reset the build's counters.
+    float v = t < 3 ? 3.0e38f : -3.0e38f;
+    if (t < 6)
+    {
+        for (uint32_t g = 0; g < gridDim.x; ++g)
+        {
+            const float p = __uint_as_float(LoadRelaxed(reinterpret_cast<const uint32_t*>(partials) + g * 8 + t));
+            v = t < 3 ? fminf(v, p) : fmaxf(v, p);
+        }
+        b.state->bounds[t] = v;
+    }
+    if (t == 0)
+    {
+        b.header->leaf_count = n;
+        b.state->leaves_dropped = found - n;
+        ctl[kCtlLeafCount] = n;
+        ctl[kCtlCoverTiles] = 0;
+        ctl[kCtlFlagsTiles] = 0;
+        ctl[kCtlBoundsTicket] = 0;
+    }
+    const SortState s = b.scratch.sort();
+    for (uint32_t i = t; i < kSortCtlBytes / 4; i += kBvhThreads)
+    {
+        s.ctl[i] = 0;
+    }
+    for (uint32_t i = t; i < kSortHistBytes / 4; i += kBvhThreads)
+    {
+        s.hist[i] = 0;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// morton (caelix_light_bvh_morton.hlsl:5-28) + the sort's digit histograms and status clear.
+// One workgroup per sort tile (4096 leaves).
+__global__ void __launch_bounds__(kSortThreads) MortonKernel(Build b)
+{
+    if (!Building(b))
+    {
+        return;
+    }
+    __shared__ uint32_t lds_hist[kSortPasses * kSortRadix];
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < kSortPasses * kSortRadix; i += kSortThreads)
+    {
+        lds_hist[i] = 0;
+    }
+    const SortState s = b.scratch.sort();
+    SortClearTileStatus(s, blockIdx.x);
+    if (t == 0)
+    {
+        const uint32_t tile = blockIdx.x;
+        if (tile < b.scratch.scan_tiles())
+        {
+            b.scratch.cover_status()[tile] = 0;
+            b.scratch.flags_status()[tile] = 0;
+        }
+    }
+    __syncthreads();
+    const uint32_t n = b.header->leaf_count;
+    const float lo0 = b.state->bounds[0];
+    const float lo1 = b.state->bounds[1];
+    const float lo2 = b.state->bounds[2];
+    const float e0 = fmaxf(b.state->bounds[3] - lo0, 1e-6f);
+    const float e1 = fmaxf(b.state->bounds[4] - lo1, 1e-6f);
+    const float e2 = fmaxf(b.state->bounds[5] - lo2, 1e-6f);
+    const float* centres = b.scratch.centres();
+    uint32_t* keys = b.scratch.keys(0);
+    uint32_t* values = b.scratch.values(0);
+    uint32_t* counter = b.scratch.u32(kLtScratchCounter);
+    int32_t* diff = b.scratch.cover_diff();
+    const uint32_t base = blockIdx.x * kSortTile;
+#pragma unroll 4
+    for (int k = 0; k < kSortItems; ++k)
+    {
+        const uint32_t i = base + k * kSortThreads + t;
+        if (i >= b.leaf_capacity)
+        {
+            break;
+        }
+        counter[i] = 0;
+        diff[i] = 0;
+        if (i < n)
+        {
+            const float c0 = centres[3 * uint64_t(i)];
+            const float c1 = centres[3 * uint64_t(i) + 1];
+            const float c2 = centres[3 * uint64_t(i) + 2];
+            // clamp((c - lo) / extent * 1024, 0, 1023) with an IEEE division, truncated to uint.
+            const uint32_t x = uint32_t(fminf(fmaxf(__fdiv_rn(c0 - lo0, e0) * 1024.0f, 0.0f), 1023.0f));
+            const uint32_t y = uint32_t(fminf(fmaxf(__fdiv_rn(c1 - lo1, e1) * 1024.0f, 0.0f), 1023.0f));
+            const uint32_t z = uint32_t(fminf(fmaxf(__fdiv_rn(c2 - lo2, e2) * 1024.0f, 0.0f), 1023.0f));
+            const uint32_t key = Morton3(x, y, z);
+            keys[i] = key;
+            values[i] = i;
+            SortCountKey(lds_hist, key);
+        }
+    }
+    __syncthreads();
+    SortFlushHistogram(lds_hist, s.hist);
+}
+
+// ------------------------------------------------------------------------------------------------
+// karras (caelix_light_bvh_karras.hlsl:8-96)
+CX_DEV uint32_t Clz(uint32_t x)
+{
+    return x == 0 ? 32u : uint32_t(__clz(int(x)));
+}
+
+CX_DEV int Delta(const uint32_t* keys, int i, int j, int n, uint32_t key_i)
+{
+    if (j < 0 || j >= n)
+    {
+        return -1;
+    }
+    const uint32_t key_j = keys[j];
+    if (key_i == key_j)
+    {
+        return 32 + int(Clz(uint32_t(i) ^ uint32_t(j)));
+    }
+    return int(Clz(key_i ^ key_j));
+}
+
+__global__ void __launch_bounds__(kBvhThreads) KarrasKernel(Build b)
+{
+    const uint32_t index = blockIdx.x * kBvhThreads + threadIdx.x;
+    if (index + 1 >= b.leaf_capacity || !Building(b))
+    {
+        return;
+    }
+    const int n = int(b.header->leaf_count);
+    const int i = int(index);
+    if (i >= n - 1)
+    {
+        return;
+    }
+    uint32_t* parent = b.scratch.u32(kLtScratchParent);
+    if (i == 0)
+    {
+        parent[0] = kLtNone;
+    }
+    const uint32_t* keys = b.scratch.keys(0);
+    const uint32_t key_i = keys[i];
+    const int d = Delta(keys, i, i + 1, n, key_i) - Delta(keys, i, i - 1, n, key_i) >= 0 ? 1 : -1;
+    const int delta_min = Delta(keys, i, i - d, n, key_i);
+    int length_max = 2;
+    while (Delta(keys, i, i + length_max * d, n, key_i) > delta_min)
+    {
+        length_max *= 2;
+    }
+    int length = 0;
+    for (int step = length_max / 2; step >= 1; step /= 2)
+    {
+        if (Delta(keys, i, i + (length + step) * d, n, key_i) > delta_min)
+        {
+            length += step;
+        }
+    }
+    const int j = i + length * d;
+    const int delta_node = Delta(keys, i, j, n, key_i);
+    int split = 0;
+    int divisor = 2;
+    int step2 = (length + divisor - 1) / divisor;
+    while (true)
+    {
+        if (Delta(keys, i, i + (split + step2) * d, n, key_i) > delta_node)
+        {
+            split += step2;
+        }
+        if (step2 == 1)
+        {
+            break;
+        }
+        divisor *= 2;
+        step2 = (length + divisor - 1) / divisor;
+    }
+    const int gamma = i + split * d + min(d, 0);
+    const uint32_t left = min(i, j) == gamma ? (kLtChildLeafBit | uint32_t(gamma)) : uint32_t(gamma);
+    const uint32_t right = max(i, j) == gamma + 1 ? (kLtChildLeafBit | uint32_t(gamma + 1)) : uint32_t(gamma + 1);
+    b.scratch.u32(kLtScratchLeft)[i] = left;
+    b.scratch.u32(kLtScratchRight)[i] = right;
+    b.scratch.u32((left & kLtChildLeafBit) != 0 ? kLtScratchSortedLeafParent : kLtScratchParent)
+        [left & ~kLtChildLeafBit] = uint32_t(i);
+    b.scratch.u32((right & kLtChildLeafBit) != 0 ? kLtScratchSortedLeafParent : kLtScratchParent)
+        [right & ~kLtChildLeafBit] = uint32_t(i);
+    // Gap cover: node i spans the gaps min(i, j) .. max(i, j) - 1; its split gap is gamma.
+    int32_t* diff = b.scratch.cover_diff();
+    AtomicAddRelaxed(&diff[min(i, j)], 1);
+    AtomicAddRelaxed(&diff[max(i, j)], -1);
+    b.scratch.split()[i] = uint32_t(gamma);
+}
+
+// ------------------------------------------------------------------------------------------------
+// aggregate (caelix_light_bvh_aggregate.hlsl:30-55)
+__global__ void __launch_bounds__(kBvhThreads) AggregateKernel(Build b)
+{
+    const uint32_t j = blockIdx.x * kBvhThreads + threadIdx.x;
+    if (j >= b.leaf_capacity || !Building(b))
+    {
+        return;
+    }
+    const uint32_t n = b.header->leaf_count;
+    if (j >= n || n < 2)
+    {
+        return;
+    }
+    uint32_t* counter = b.scratch.u32(kLtScratchCounter);
+    const uint32_t* left = b.scratch.u32(kLtScratchLeft);
+    const uint32_t* right = b.scratch.u32(kLtScratchRight);
+    const uint32_t* parent = b.scratch.u32(kLtScratchParent);
+    uint32_t node = b.scratch.u32(kLtScratchSortedLeafParent)[j];
+    for (uint32_t depth = 0; depth < 128 && node != kLtNone; ++depth)
+    {
+        if (AtomicAddAcqRel(&counter[node], 1u) == 0)
+        {
+            return;
+        }
+        const Summary s = Union(ChildSummary(b, left[node], true), ChildSummary(b, right[node], true));
+        StoreSummaryCoherent(b.scratch.aggregates() + uint64_t(node) * 12, s);
+        node = parent[node];
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Single-pass scans with decoupled look-back over 4096-element tiles (64-bit tile status).
+// Returns the exclusive prefix of the tile (sum of all earlier tiles), publishing `tile_sum`.
+CX_DEV int32_t ScanLookBack(uint64_t* status, uint32_t tile, int32_t tile_sum, int32_t* lds_prefix)
+{
+    if (threadIdx.x == 0)
+    {
+        int32_t prefix = 0;
+        if (tile == 0)
+        {
+            StoreRelaxed(&status[0], kScanPrefix | uint32_t(tile_sum));
+        }
+        else
+        {
+            StoreRelaxed(&status[tile], kScanAggregate | uint32_t(tile_sum));
+            int64_t p = int64_t(tile) - 1;
+            while (true)
+            {
+                const uint64_t word = LoadRelaxed(&status[p]);
+                const uint64_t flag = word & kScanFlagMask;
+                if (flag == 0)
+                {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                prefix += int32_t(uint32_t(word));
+                if (flag == kScanPrefix)
+                {
+                    break;
+                }
+                --p;
+            }
+            StoreRelaxed(&status[tile], kScanPrefix | uint32_t(prefix + tile_sum));
+        }
+        *lds_prefix = prefix;
+    }
+    __syncthreads();
+    return *lds_prefix;
+}
+
+// Thread-blocked block scan: thread t holds items [t * 16, t * 16 + 16) of the tile. Converts `v`
+// into exclusive prefixes within the tile; returns the tile total.
+CX_DEV int32_t TileExclusiveScan(int32_t (&v)[kScanItems], int32_t* lds_waves)
+{
+    int32_t sum = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k)
+    {
+        const int32_t x = v[k];
+        v[k] = sum;
+        sum += x;
+    }
+    int32_t total;
+    const int32_t before = BlockExclusiveSum<kScanThreads>(sum, lds_waves, total);
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k)
+    {
+        v[k] += before;
+    }
+    return total;
+}
+
+// cover[g] = sum of diff[0 .. g] for the gaps g < n - 1.
+__global__ void __launch_bounds__(kScanThreads) CoverScanKernel(Build b)
+{
+    if (!Building(b))
+    {
+        return;
+    }
+    __shared__ uint32_t lds_tile;
+    __shared__ int32_t lds_waves[kScanThreads / kWave];
+    __shared__ int32_t lds_prefix;
+    uint32_t* ctl = b.scratch.ctl();
+    if (threadIdx.x == 0)
+    {
+        lds_tile = AtomicAddRelaxed(&ctl[kCtlCoverTiles], 1u);
+    }
+    __syncthreads();
+    const uint32_t tile = lds_tile;
+    const uint32_t n = ctl[kCtlLeafCount];
+    const uint32_t gaps = n > 0 ? n - 1 : 0;
+    const uint32_t first = tile * kScanTile;
+    if (first >= gaps)
+    {
+        return;
+    }
+    const int32_t* diff = b.scratch.cover_diff();
+    int32_t* cover = b.scratch.cover();
+    int32_t v[kScanItems];
+    const uint32_t base = first + threadIdx.x * kScanItems;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k)
+    {
+        v[k] = base + k < gaps ? diff[base + k] : 0;
+    }
+    int32_t own[kScanItems];
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k)
+    {
+        own[k] = v[k];
+    }
+    const int32_t total = TileExclusiveScan(v, lds_waves);
+    const int32_t prefix = ScanLookBack(b.scratch.cover_status(), tile, total, &lds_prefix);
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k)
+    {
+        if (base + k < gaps)
+        {
+            cover[base + k] = prefix + v[k] + own[k];
+        }
+    }
+}
+
+// Depth, even-depth flags and their exclusive scan (the reference's depth kernel + scan over the
+// leaf capacity, caelix_light_bvh_depth.hlsl:7-30), then the header in the last tile
+// (caelix_light_bvh_finish.hlsl:8-63).
+__global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Build b)
+{
+    if (!Building(b))
+    {
+        return;
+    }
+    __shared__ uint32_t lds_tile;
+    __shared__ int32_t lds_waves[kScanThreads / kWave];
+    __shared__ int32_t lds_prefix;
+    uint32_t* ctl = b.scratch.ctl();
+    if (threadIdx.x == 0)
+    {
+        lds_tile = AtomicAddRelaxed(&ctl[kCtlFlagsTiles], 1u);
+    }
+    __syncthreads();
+    const uint32_t tile = lds_tile;
+    const uint32_t n = ctl[kCtlLeafCount];
+    const uint32_t capacity = b.leaf_capacity;
+    const uint32_t first = tile * kScanTile;
+    const int32_t* cover = b.scratch.cover();
+    const uint32_t* split = b.scratch.split();
+    uint32_t* counter = b.scratch.u32(kLtScratchCounter);
+    uint32_t* wide = b.scratch.u32(kLtScratchWide);
+    int32_t v[kScanItems];
+    const uint32_t base = first + threadIdx.x * kScanItems;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k)
+    {
+        const uint32_t i = base + k;
+        int32_t flag = 0;
+        if (i + 1 < n)
+        {
+            const uint32_t depth = uint32_t(cover[split[i]] - 1);
+            counter[i] = depth;
+            flag = (depth & 1u) == 0 ? 1 : 0;
+        }
+        v[k] = flag;
+    }
+    const int32_t total = TileExclusiveScan(v, lds_waves);
+    const int32_t prefix = ScanLookBack(b.scratch.flags_status(), tile, total, &lds_prefix);
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k)
+    {
+        if (base + k < capacity)
+        {
+            wide[base + k] = uint32_t(prefix + v[k]);
+        }
+    }
+    if (tile + 1 != gridDim.x || threadIdx.x != 0)
+    {
+        return;
+    }
+    // Finish: the header. n = 1: one node with the leaf in slot 0. n >= 2: the binary root's summary;
+    // more 4-wide nodes than the capacity leave the tree empty with the overflow flag.
+    const uint32_t node_total = uint32_t(prefix + total);
+    Summary root = EmptySummary();
+    uint32_t root_index = kLtNone;
+    uint32_t node_count = 0;
+    uint32_t overflow = 0;
+    if (n == 1)
+    {
+        LightBvhLeaf* leaf = b.leaves;
+        root = LoadSummary(leaf->summary.face);
+        LightBvhNode* node = b.nodes;
+        uint4* zero = reinterpret_cast<uint4*>(node);
+        for (int q = 0; q < 12; ++q)
+        {
+            zero[q] = make_uint4(0, 0, 0, 0);
+        }
+        for (int f = 0; f < 6; ++f)
+        {
+            node->child_face[0][f] = root.face[f];
+        }
+        for (int k = 0; k < 3; ++k)
+        {
+            node->child_lo[0][k] = root.lo[k];
+            node->child_hi[0][k] = root.hi[k];
+        }
+        node->child[0] = kLtChildLeafBit | 0u;
+        node->child[1] = kLtNone;
+        node->child[2] = kLtNone;
+        node->child[3] = kLtNone;
+        node->parent = kLtNone;
+        node->parent_slot = 0;
+        leaf->parent = 0;
+        root_index = 0;
+        node_count = 1;
+    }
+    else if (n >= 2)
+    {
+        node_count = node_total;
+        if (node_count > b.node_capacity)
+        {
+            overflow = 1;
+        }
+        else
+        {
+            root = LoadSummary(b.scratch.aggregates());
+            root_index = 0;
+        }
+    }
+    StoreSummary(b.header->summary.face, root);
+    b.header->root = root_index;
+    b.header->leaf_count = overflow != 0 ? 0u : n;
+    b.header->node_count = node_count;
+    b.header->pad = 0;
+    b.state->node_overflow = overflow;
+    b.state->builds = b.state->builds + 1;
+}
+
+// ------------------------------------------------------------------------------------------------
+// wide (caelix_light_bvh_wide.hlsl:10-100)
+__global__ void __launch_bounds__(kBvhThreads) WideKernel(Build b)
+{
+    const uint32_t i = blockIdx.x * kBvhThreads + threadIdx.x;
+    if (i + 1 >= b.leaf_capacity || !Building(b))
+    {
+        return;
+    }
+    const uint32_t n = b.scratch.ctl()[kCtlLeafCount];
+    const uint32_t* counter = b.scratch.u32(kLtScratchCounter);
+    if (i + 1 >= n || (counter[i] & 1u) != 0)
+    {
+        return;
+    }
+    const uint32_t* wide = b.scratch.u32(kLtScratchWide);
+    const uint32_t w = wide[i];
+    if (w >= b.node_capacity)
+    {
+        return;
+    }
+    const uint32_t* left = b.scratch.u32(kLtScratchLeft);
+    const uint32_t* right = b.scratch.u32(kLtScratchRight);
+    uint32_t slots[4] = {kLtNone, kLtNone, kLtNone, kLtNone};
+    uint32_t count = 0;
+    const uint32_t children[2] = {left[i], right[i]};
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+    {
+        const uint32_t child = children[c];
+        if ((child & kLtChildLeafBit) != 0)
+        {
+            slots[count++] = child;
+        }
+        else
+        {
+            slots[count++] = left[child];
+            slots[count++] = right[child];
+        }
+    }
+    LightBvhNode* node = b.nodes + w;
+    uint32_t child_words[4] = {kLtNone, kLtNone, kLtNone, kLtNone};
+    float faces[24];
+    float bounds[24];
+#pragma unroll
+    for (uint32_t s = 0; s < 4; ++s)
+    {
+        Summary summary = EmptySummary();
+        if (s < count)
+        {
+            const uint32_t slot = slots[s];
+            summary = ChildSummary(b, slot, false);
+            if ((slot & kLtChildLeafBit) != 0)
+            {
+                const uint32_t leaf = b.scratch.values(0)[slot & ~kLtChildLeafBit];
+                child_words[s] = kLtChildLeafBit | leaf;
+                b.leaves[leaf].parent = (w << kLtParentSlotBits) | s;
+            }
+            else
+            {
+                const uint32_t child_node = wide[slot];
+                child_words[s] = child_node;
+                if (child_node < b.node_capacity)
+                {
+                    *reinterpret_cast<uint2*>(&b.nodes[child_node].parent) = make_uint2(w, s);
+                }
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+        {
+            faces[6 * s + f] = summary.face[f];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+        {
+            bounds[3 * s + k] = summary.lo[k];
+            bounds[12 + 3 * s + k] = summary.hi[k];
+        }
+    }
+    float4* face_out = reinterpret_cast<float4*>(node->child_face[0]);
+    float4* bound_out = reinterpret_cast<float4*>(node->child_lo[0]);
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+    {
+        face_out[q] = make_float4(faces[4 * q], faces[4 * q + 1], faces[4 * q + 2], faces[4 * q + 3]);
+        bound_out[q] = make_float4(bounds[4 * q], bounds[4 * q + 1], bounds[4 * q + 2], bounds[4 * q + 3]);
+    }
+    *reinterpret_cast<uint4*>(node->child) = make_uint4(child_words[0], child_words[1], child_words[2], child_words[3]);
+    if (i == 0)
+    {
+        *reinterpret_cast<uint2*>(&node->parent) = make_uint2(kLtNone, 0u);
+    }
+}
+
+// caelix_light_bvh_begin.hlsl:9-17
+__global__ void BeginKernel(const LightControl* control, LightTreeState* state, uint32_t tree, uint32_t reason,
+                            uint32_t pass_recorded)
+{
+    const bool committed = pass_recorded != 0 && control != nullptr && control->pass_result == kLtPassOk &&
+                           ((control->trees_changed >> tree) & 1u) != 0;
+    state->building = (reason != 0 || committed) ? 1u : 0u;
+}
+
+__global__ void ScratchInitKernel(uint32_t* ctl)
+{
+    if (threadIdx.x < kCtlWords)
+    {
+        ctl[threadIdx.x] = 0;
+    }
+}
+
+}  // namespace
+}  // namespace cx
+
+extern "C" {
+
+CX_API uint64_t cx_light_bvh_scratch_bytes(uint32_t leaf_capacity)
+{
+    return cx::Scratch::Bytes(leaf_capacity);
+}
+
+CX_API int cx_light_bvh_scratch_init(void* scratch, uint32_t leaf_capacity, void* stream)
+{
+    if (scratch == nullptr || leaf_capacity == 0 || leaf_capacity % 64 != 0)
+    {
+        return cx::ArgError("cx_light_bvh_scratch_init: NULL scratch or a leaf capacity that is not a multiple of 64");
+    }
+    const cx::Scratch s{static_cast<uint8_t*>(scratch), leaf_capacity};
+    cx::ScratchInitKernel<<<1, 64, 0, static_cast<hipStream_t>(stream)>>>(s.ctl());
+    return cx::LaunchResult("cx_light_bvh_scratch_init");
+}
+
+CX_API int cx_light_bvh_begin(LightControl* control, LightTreeState* state, uint32_t tree, uint32_t reason,
+                              uint32_t pass_recorded, void* stream)
+{
+    if (state == nullptr || tree >= kLtTrees || (pass_recorded != 0 && control == nullptr))
+    {
+        return cx::ArgError("cx_light_bvh_begin: NULL state, tree >= 2, or a recorded pass without control");
+    }
+    cx::BeginKernel<<<1, 1, 0, static_cast<hipStream_t>(stream)>>>(control, state, tree, reason, pass_recorded);
+    return cx::LaunchResult("cx_light_bvh_begin");
+}
+
+CX_API int cx_light_bvh_build(const CxLightBvhBuild* build, void* stream)
+{
+    if (build == nullptr || build->leaves == nullptr || build->nodes == nullptr || build->header == nullptr ||
+        build->state == nullptr || build->scratch == nullptr)
+    {
+        return cx::ArgError("cx_light_bvh_build: a buffer is NULL");
+    }
+    if (build->leaf_capacity == 0 || build->leaf_capacity % 64 != 0 || build->leaf_capacity >= (1u << 30))
+    {
+        return cx::ArgError("cx_light_bvh_build: leaf capacity %u is not a positive multiple of 64 below 2^30",
+                            build->leaf_capacity);
+    }
+    if (build->node_capacity == 0)
+    {
+        return cx::ArgError("cx_light_bvh_build: node capacity 0");
+    }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    cx::Build b;
+    b.leaves = build->leaves;
+    b.nodes = build->nodes;
+    b.header = build->header;
+    b.state = build->state;
+    b.scratch = cx::Scratch{static_cast<uint8_t*>(build->scratch), build->leaf_capacity};
+    b.leaf_capacity = build->leaf_capacity;
+    b.node_capacity = build->node_capacity;
+    const uint32_t capacity = build->leaf_capacity;
+    const uint32_t per_item_blocks = cx::DivUp(capacity, cx::kBvhThreads);
+    const uint32_t tiles = cx::SortTiles(capacity);
+    {
+        cx::StageScope scope(s, "lt_bounds");
+        cx::BoundsKernel<<<std::min(tiles, cx::kBoundsMaxBlocks), cx::kBvhThreads, 0, s>>>(b);
+    }
+    {
+        cx::StageScope scope(s, "lt_morton");
+        cx::MortonKernel<<<tiles, cx::kSortThreads, 0, s>>>(b);
+    }
+    const cx::SortState sort = b.scratch.sort();
+    const uint32_t* gate = &build->state->building;
+    const uint32_t* count = &build->header->leaf_count;
+    for (uint32_t pass = 0; pass < cx::kSortPasses; ++pass)
+    {
+        const uint32_t in = pass & 1u;
+        cx::LaunchSortPass(b.scratch.keys(in), b.scratch.values(in), b.scratch.keys(1 - in), b.scratch.values(1 - in),
+                           count, gate, sort, pass, s);
+    }
+    {
+        cx::StageScope scope(s, "lt_karras");
+        cx::KarrasKernel<<<per_item_blocks, cx::kBvhThreads, 0, s>>>(b);
+    }
+    {
+        cx::StageScope scope(s, "lt_aggregate");
+        cx::AggregateKernel<<<per_item_blocks, cx::kBvhThreads, 0, s>>>(b);
+    }
+    const uint32_t scan_tiles = b.scratch.scan_tiles();
+    {
+        cx::StageScope scope(s, "lt_cover_scan");
+        cx::CoverScanKernel<<<scan_tiles, cx::kScanThreads, 0, s>>>(b);
+    }
+    {
+        cx::StageScope scope(s, "lt_flags_scan");
+        cx::FlagsScanKernel<<<scan_tiles, cx::kScanThreads, 0, s>>>(b);
+    }
+    {
+        cx::StageScope scope(s, "lt_wide");
+        cx::WideKernel<<<per_item_blocks, cx::kBvhThreads, 0, s>>>(b);
+    }
+    return cx::LaunchResult("cx_light_bvh_build");
+}
+
+}  // extern "C"

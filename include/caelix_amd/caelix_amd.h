@@ -1,0 +1,112 @@
+/*
+ * C ABI of the MI355X-native Caelix hot path: light-tree build, light-tree sample/pdf, hash-table
+ * brick lookup, brick DDA + NEE trace. Plain pointers and sizes; device pointers are HIP device
+ * addresses; `stream` is a hipStream_t (NULL = the default stream). Every entry point is
+ * exception-free and returns a CaelixResult code (include/caelix/caelix.h:115-122 in the
+ * reference); argument errors log once through the log callback and change nothing, as the
+ * reference's scene entry points do (lib/src/scene/scene_light.cpp:18-38,
+ * lib/src/abi/caelix.cpp:83-111).
+ *
+ * Work is stream-ordered and asynchronous: an entry point only enqueues kernels, like the
+ * reference's library only records into the host's command buffer (caelix.h:15-18). The caller owns
+ * every buffer. Calls on one stream are externally synchronised (caelix.h:19-23).
+ */
+#ifndef CAELIX_AMD_H
+#define CAELIX_AMD_H
+
+#include <stdint.h>
+
+#include "caelix_amd/layout.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define CX_API __attribute__((visibility("default")))
+#else
+#define CX_API
+#endif
+
+/* Result codes: the values of the reference's CaelixResult (caelix.h:115-122); CX_ERROR_HIP takes
+ * the place of CAELIX_ERROR_VULKAN. */
+typedef enum CxResult {
+    CX_OK = 0,
+    CX_ERROR_INVALID_ARGUMENT = 1,
+    CX_ERROR_UNSUPPORTED = 2,
+    CX_ERROR_HIP = 3,
+    CX_ERROR_OUT_OF_MEMORY = 4
+} CxResult;
+
+/* Log callback (caelix.h:128-139): level 0 debug .. 3 error. NULL disables logging. */
+typedef void (*CxLogCallback)(int level, const char* message, void* user_data);
+CX_API void cx_set_log_callback(CxLogCallback callback, void* user_data);
+
+/* Library version string and the number of gfx950 kernels built in. */
+CX_API const char* cx_version_string(void);
+
+/* ------------------------------------------------------------------------------------------
+ * Light-tree build (the BVH half of LightTree::RecordTree, lib/src/light/light_tree_record.cpp:
+ * 230-309, after the gather): bounds -> Morton -> radix sort -> Karras -> aggregate -> depth ->
+ * scan -> 4-wide collapse -> finish, one tree per call.
+ *
+ * Inputs in device memory, in the reference's formats (light_tree_layout.h):
+ *   leaves[leaf_capacity]  LightBvhLeaf; the first state->leaves_found hold the gathered leaves
+ *   state                  LightTreeState; `building` gates the build as caelix_light_bvh_begin
+ *                          sets it (0: every kernel returns at once), `leaves_found` as the
+ *                          gather's scan wrote it
+ * Outputs: header (LightTreeHeader), state (leaves_dropped, bounds, node_overflow, builds),
+ * nodes[node_capacity] (LightBvhNode), leaves[].parent, and the scratch arrays in the reference's
+ * scratch layout (sorted keys/values in sort buffer 0, left/right/parent/sorted_leaf_parent,
+ * depths in `counter`, scanned 4-wide indices in `wide`, binary aggregates).
+ * ------------------------------------------------------------------------------------------ */
+typedef struct CxLightBvhBuild {
+    LightBvhLeaf* leaves;
+    LightBvhNode* nodes;
+    LightTreeHeader* header;
+    LightTreeState* state;
+    void* scratch;          /* cx_light_bvh_scratch_bytes(leaf_capacity) bytes */
+    uint32_t leaf_capacity; /* a multiple of 64, >= 64 */
+    uint32_t node_capacity;
+} CxLightBvhBuild;
+
+/* Scratch bytes for a leaf capacity: the reference's arrays (88 bytes per leaf,
+ * LightBvhScratchBytes, light_tree_layout.h:434-439) plus the onesweep sort's tile state. */
+CX_API uint64_t cx_light_bvh_scratch_bytes(uint32_t leaf_capacity);
+/* Prepares freshly allocated scratch (zeroes the build's control words). Call once per
+ * allocation, before the first build; builds leave the control words ready for the next one. */
+CX_API int cx_light_bvh_scratch_init(void* scratch, uint32_t leaf_capacity, void* stream);
+/* caelix_light_bvh_begin (shaders/light/bvh/caelix_light_bvh_begin.hlsl:9-17): sets
+ * state->building from `reason` and the committed group pass recorded in `control`
+ * (LightControl, may be NULL when pass_recorded is 0). */
+CX_API int cx_light_bvh_begin(LightControl* control, LightTreeState* state, uint32_t tree, uint32_t reason,
+                              uint32_t pass_recorded, void* stream);
+/* The build (light_tree_record.cpp:271-308). */
+CX_API int cx_light_bvh_build(const CxLightBvhBuild* build, void* stream);
+
+/* Stable LSD radix sort of (uint32 key, uint32 value) pairs (the reference's
+ * caelix_light_bvh_sort_histogram + caelix_light_bvh_scan + caelix_light_bvh_sort_scatter passes,
+ * shaders/light/bvh/caelix_light_bvh_sort.hlsl:1-38), as 4 onesweep passes of 8 bits. The count is
+ * read from device memory (count_address, like LightSortPush); gate_address (nullable) skips the
+ * sort when the u32 there is 0. Result in keys0/values0; keys1/values1 are the ping-pong buffers.
+ * `sort_state` holds cx_radix_sort_state_bytes(capacity) bytes; capacity >= the count. */
+CX_API uint64_t cx_radix_sort_state_bytes(uint32_t capacity);
+CX_API int cx_radix_sort_pairs(uint32_t* keys0, uint32_t* values0, uint32_t* keys1, uint32_t* values1,
+                               const uint32_t* count_address, const uint32_t* gate_address, uint32_t capacity,
+                               void* sort_state, void* stream);
+
+/* ------------------------------------------------------------------------------------------
+ * Stage timing: when enabled, entry points bracket every kernel with HIP events on the stream it
+ * runs on. cx_profile_read synchronises those events and returns per-stage sums (milliseconds)
+ * and launch counts since the last reset; names are static strings.
+ * ------------------------------------------------------------------------------------------ */
+CX_API void cx_profile_enable(int enabled);
+CX_API void cx_profile_reset(void);
+CX_API int cx_profile_read(uint32_t max_stages, const char** names, double* total_ms, uint32_t* launches,
+                           uint32_t* stage_count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CAELIX_AMD_H */

@@ -1,0 +1,178 @@
+"""GPU parity of the light-tree build (cx_light_bvh_build) and the onesweep sort
+(cx_radix_sort_pairs) against the oracle: bit-exact for Morton codes, sorted order, Karras topology,
+depths, 4-wide indices, aggregates, nodes, leaf parents, header and state."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import lt_checks
+from caelix_amd import layout, synthetic
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_and_oracle(cx, oracle, leaves, capacity, node_capacity=None, found=None, builds=0):
+    node_capacity = node_capacity if node_capacity is not None else layout.node_capacity_for(capacity)
+    n = len(leaves)
+    print(f"light bvh build: n={n} capacity={capacity} node_capacity={node_capacity} found={found}", flush=True)
+    builder = cx.LightBvhBuilder(capacity, node_capacity)
+    builder.upload(leaves)
+    if found is not None or builds:
+        import torch
+        st = cx.to_host(builder.state).view(layout.STATE).copy()
+        st["leaves_found"] = n if found is None else found
+        st["builds"] = builds
+        builder.state.copy_(cx.to_device(st.view(np.uint8)))
+        torch.cuda.synchronize()
+    builder.build()
+    gpu = builder.read()
+    state = np.zeros(1, layout.STATE)
+    state["building"] = 1
+    state["leaves_found"] = n if found is None else found
+    state["builds"] = builds
+    ref = oracle.light_bvh_build(leaves.view(np.uint8), capacity, node_capacity, state.view(np.uint8))
+    return gpu, ref
+
+
+def compare(gpu, ref, capacity, n):
+    g_scratch, r_scratch = gpu["scratch"], ref["scratch"]
+    for name, offset, count in [
+        ("keys", layout.SCRATCH_KEYS, n),
+        ("values", layout.SCRATCH_VALUES, n),
+        ("left", layout.SCRATCH_LEFT, max(n - 1, 0)),
+        ("right", layout.SCRATCH_RIGHT, max(n - 1, 0)),
+        ("parent", layout.SCRATCH_PARENT, max(n - 1, 0)),
+        ("sorted_leaf_parent", layout.SCRATCH_SORTED_LEAF_PARENT, n if n >= 2 else 0),
+        ("counter", layout.SCRATCH_COUNTER, capacity),
+        ("wide", layout.SCRATCH_WIDE, capacity),
+    ]:
+        g = layout.scratch_u32(g_scratch, offset, capacity)[:count]
+        r = layout.scratch_u32(r_scratch, offset, capacity)[:count]
+        bad = np.nonzero(g != r)[0]
+        assert bad.size == 0, f"{name}: {bad.size} mismatches, first at {bad[:5]}: gpu {g[bad[:5]]} oracle {r[bad[:5]]}"
+    if n >= 2:
+        g_agg = layout.scratch_aggregates(g_scratch, capacity)[: n - 1]
+        r_agg = layout.scratch_aggregates(r_scratch, capacity)[: n - 1]
+        assert g_agg.tobytes() == r_agg.tobytes(), "aggregates differ"
+    g_header = gpu["header"]
+    r_header = ref["header"].view(layout.HEADER)[0]
+    assert g_header.tobytes() == r_header.tobytes(), f"header: gpu {g_header} oracle {r_header}"
+    g_state = gpu["state"]
+    r_state = ref["state"].view(layout.STATE)[0]
+    for field in ("building", "leaves_found", "leaves_dropped", "builds", "node_overflow", "bounds"):
+        assert np.array_equal(g_state[field], r_state[field]), field
+    node_count = int(r_header["node_count"]) if int(r_state["node_overflow"]) == 0 else 0
+    g_nodes = gpu["nodes"][:node_count]
+    r_nodes = ref["nodes"].view(layout.NODE)[:node_count]
+    for field in ("child_face", "child_lo", "child_hi", "child", "parent", "parent_slot"):
+        if node_count == 0:
+            break
+        bad = np.nonzero(np.any((g_nodes[field] != r_nodes[field]).reshape(node_count, -1), axis=1))[0]
+        assert bad.size == 0, f"nodes.{field}: {bad.size} mismatches, first {bad[:5]}"
+    g_parent = gpu["leaves"]["parent"][:n]
+    r_parent = ref["leaves"].view(layout.LEAF)["parent"][:n]
+    assert np.array_equal(g_parent, r_parent), "leaf parents differ"
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 64, 1000, 4095, 4096, 4097, 70000])
+def test_build_matches_oracle_small(cx, oracle, n):
+    leaves = synthetic.light_leaves(n, 100 + n, extent=2048.0, clusters=8, sigma=16.0)
+    capacity = max(64, (n + 63) // 64 * 64)
+    gpu, ref = gpu_and_oracle(cx, oracle, leaves, capacity)
+    compare(gpu, ref, capacity, n)
+    if n > 0:
+        h = gpu["header"]
+        lt_checks.check_bvh(h, gpu["state"], gpu["nodes"][: int(h["node_count"])], gpu["leaves"][:n])
+
+
+def test_build_capacity_above_count_and_dropped(cx, oracle):
+    leaves = synthetic.light_leaves(5000, 7)
+    gpu, ref = gpu_and_oracle(cx, oracle, leaves, capacity=8192)
+    compare(gpu, ref, 8192, 5000)
+    # found above the capacity: 64 leaves dropped (the reference's 4,194,240 cap in miniature)
+    gpu, ref = gpu_and_oracle(cx, oracle, leaves[:4096], capacity=4096, found=4160)
+    compare(gpu, ref, 4096, 4096)
+    assert int(gpu["state"]["leaves_dropped"]) == 64
+
+
+def test_build_node_overflow_and_rebuild_counts(cx, oracle):
+    leaves = synthetic.light_leaves(3000, 8)
+    gpu, ref = gpu_and_oracle(cx, oracle, leaves, capacity=3008, node_capacity=100, builds=5)
+    compare(gpu, ref, 3008, 3000)
+    assert int(gpu["state"]["node_overflow"]) == 1
+    assert int(gpu["state"]["builds"]) == 6
+
+
+def test_build_gate_closed_changes_nothing(cx):
+    leaves = synthetic.light_leaves(1000, 9)
+    builder = cx.LightBvhBuilder(1024)
+    builder.upload(leaves, building=0)
+    builder.build()
+    out = builder.read()
+    assert int(out["header"]["leaf_count"]) == 0
+    assert int(out["state"]["builds"]) == 0
+
+
+def test_repeated_builds_reuse_scratch(cx, oracle):
+    builder = cx.LightBvhBuilder(16384)
+    for seed, n in [(11, 16000), (12, 300), (13, 16384)]:
+        leaves = synthetic.light_leaves(n, seed)
+        builder.upload(leaves)
+        builder.build()
+        gpu = builder.read()
+        state = np.zeros(1, layout.STATE)
+        state["building"] = 1
+        state["leaves_found"] = n
+        ref = oracle.light_bvh_build(leaves.view(np.uint8), 16384, builder.node_capacity, state.view(np.uint8))
+        assert gpu["header"].tobytes()[:60] == ref["header"].tobytes()[:60]
+        n_nodes = int(gpu["header"]["node_count"])
+        g_nodes = gpu["nodes"][:n_nodes]
+        r_nodes = ref["nodes"].view(layout.NODE)[:n_nodes]
+        assert np.array_equal(g_nodes["child"], r_nodes["child"])
+
+
+def test_radix_sort_matches_std_sort_on_reference_inputs(cx):
+    import torch
+
+    for n, keys in synthetic.reference_sort_inputs().items():
+        print(f"radix sort reference inputs n={n}", flush=True)
+        values = np.arange(n, dtype=np.uint32)
+        k = torch.from_numpy(keys.view(np.int32).copy()).cuda()
+        v = torch.from_numpy(values.view(np.int32).copy()).cuda()
+        cx.radix_sort_pairs(k, v)
+        torch.cuda.synchronize()
+        order = np.lexsort((values, keys))
+        assert np.array_equal(k.cpu().numpy().view(np.uint32), keys[order]), n
+        assert np.array_equal(v.cpu().numpy().view(np.uint32), values[order]), n
+
+
+def test_radix_sort_full_32bit_keys(cx, oracle):
+    import torch
+
+    rng = np.random.default_rng(5)
+    n = 1 << 20
+    keys = rng.integers(0, 2**32, size=n, dtype=np.uint64).astype(np.uint32)
+    keys[::7] = keys[0]  # many duplicates
+    values = np.arange(n, dtype=np.uint32)
+    k = torch.from_numpy(keys.view(np.int32).copy()).cuda()
+    v = torch.from_numpy(values.view(np.int32).copy()).cuda()
+    cx.radix_sort_pairs(k, v)
+    torch.cuda.synchronize()
+    rk, rv = oracle.radix_sort_pairs(keys, values)
+    assert np.array_equal(k.cpu().numpy().view(np.uint32), rk)
+    assert np.array_equal(v.cpu().numpy().view(np.uint32), rv)
+
+
+def test_rebuild_stress_size_matches_oracle(cx, oracle):
+    """The 4,194,304-leaf rebuild workload (half uniform in 8192^3, half in 64 clusters of sigma 32)
+    against the oracle, at full size."""
+    n = 1 << 22
+    leaves = synthetic.light_leaves(n, 2024, extent=8192.0, clusters=64, sigma=32.0)
+    gpu, ref = gpu_and_oracle(cx, oracle, leaves, capacity=n)
+    compare(gpu, ref, n, n)
+    # the reference's own leaf cap: 64 of the 4,194,304 leaves dropped
+    capped = 65535 * 64
+    gpu, ref = gpu_and_oracle(cx, oracle, leaves[:capped], capacity=capped, found=n)
+    compare(gpu, ref, capped, capped)
+    assert int(gpu["state"]["leaves_dropped"]) == 64

@@ -1,0 +1,136 @@
+"""CPU: pins the light-tree build oracle (oracle/light_bvh_oracle.cpp) against checks that do not
+restate its formulas: brute-force Morton interleave, numpy lexsort (= std::sort of (key, value)
+pairs, the reference test's own oracle), a top-down radix-tree construction, float64 range sums,
+a parent-walk depth, and the reference test's CheckBvh traversal."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import lt_checks
+from caelix_amd import layout, synthetic
+
+
+def run_oracle(oracle, leaves, capacity=None, node_capacity=None, found=None, building=1, builds=0):
+    n = len(leaves)
+    capacity = capacity if capacity is not None else max(64, (n + 63) // 64 * 64)
+    node_capacity = node_capacity if node_capacity is not None else layout.node_capacity_for(capacity)
+    state = np.zeros(1, layout.STATE)
+    state["building"] = building
+    state["leaves_found"] = n if found is None else found
+    state["builds"] = builds
+    out = oracle.light_bvh_build(leaves.view(np.uint8), capacity, node_capacity, state.view(np.uint8))
+    out["header_rec"] = out["header"].view(layout.HEADER)[0]
+    out["state_rec"] = out["state"].view(layout.STATE)[0]
+    out["nodes_rec"] = out["nodes"].view(layout.NODE)
+    out["leaves_rec"] = out["leaves"].view(layout.LEAF)
+    out["capacity"] = capacity
+    return out
+
+
+def test_morton_matches_bruteforce_interleave(oracle):
+    rng = np.random.default_rng(1)
+    cells = rng.integers(0, 1024, size=(2000, 3)).astype(np.uint32)
+    cells[:8] = [[0, 0, 0], [1023, 1023, 1023], [1, 0, 0], [0, 1, 0], [0, 0, 1], [512, 0, 0], [0, 512, 0], [0, 0, 512]]
+    expected = lt_checks.morton_bruteforce(cells[:, 0], cells[:, 1], cells[:, 2])
+    got = np.array([oracle.morton3(int(x), int(y), int(z)) for x, y, z in cells], np.uint32)
+    assert np.array_equal(got, expected)
+    assert oracle.morton3(1, 0, 0) == 1 and oracle.morton3(0, 1, 0) == 2 and oracle.morton3(0, 0, 1) == 4
+
+
+def test_oracle_radix_sort_matches_std_sort_on_reference_inputs(oracle):
+    for n, keys in synthetic.reference_sort_inputs().items():
+        values = np.arange(n, dtype=np.uint32)
+        k, v = oracle.radix_sort_pairs(keys, values)
+        order = np.lexsort((values, keys))
+        assert np.array_equal(k, keys[order]), n
+        assert np.array_equal(v, values[order]), n
+
+
+@pytest.mark.parametrize("n,seed", [(2, 1), (3, 2), (17, 3), (1000, 4), (5000, 5)])
+def test_oracle_build_matches_independent_checks(oracle, n, seed):
+    leaves = synthetic.light_leaves(n, seed, extent=512.0, clusters=4, sigma=6.0)
+    out = run_oracle(oracle, leaves)
+    cap = out["capacity"]
+    scratch = out["scratch"]
+    header = out["header_rec"]
+    assert int(header["leaf_count"]) == n
+
+    # bounds of the centres
+    centre = 0.5 * (leaves["summary"]["lo"] + leaves["summary"]["hi"])
+    assert np.array_equal(out["state_rec"]["bounds"][:3], centre.min(axis=0))
+    assert np.array_equal(out["state_rec"]["bounds"][3:], centre.max(axis=0))
+
+    # Morton keys from float32 quantisation, interleaved bit by bit
+    lo = out["state_rec"]["bounds"][:3]
+    extent = np.maximum(out["state_rec"]["bounds"][3:] - lo, np.float32(1e-6))
+    q = np.clip((centre - lo) / extent * np.float32(1024.0), 0.0, 1023.0).astype(np.uint32)
+    codes = lt_checks.morton_bruteforce(q[:, 0], q[:, 1], q[:, 2])
+
+    # sorted pairs = std::sort of (code, leaf index)
+    order = np.lexsort((np.arange(n), codes))
+    keys0 = layout.scratch_u32(scratch, layout.SCRATCH_KEYS, cap)[:n]
+    values0 = layout.scratch_u32(scratch, layout.SCRATCH_VALUES, cap)[:n]
+    assert np.array_equal(keys0, codes[order])
+    assert np.array_equal(values0, order.astype(np.uint32))
+
+    # Karras topology = top-down radix tree
+    left, right, parent, leaf_parent = lt_checks.top_down_radix_tree(codes[order])
+    assert np.array_equal(layout.scratch_u32(scratch, layout.SCRATCH_LEFT, cap)[: n - 1], left)
+    assert np.array_equal(layout.scratch_u32(scratch, layout.SCRATCH_RIGHT, cap)[: n - 1], right)
+    assert np.array_equal(layout.scratch_u32(scratch, layout.SCRATCH_PARENT, cap)[: n - 1], parent)
+    assert np.array_equal(layout.scratch_u32(scratch, layout.SCRATCH_SORTED_LEAF_PARENT, cap)[:n], leaf_parent)
+
+    # depths by parent walk; 4-wide indices = exclusive scan of even-depth flags over the capacity
+    depth = lt_checks.depths_by_walk(parent)
+    assert np.array_equal(layout.scratch_u32(scratch, layout.SCRATCH_COUNTER, cap)[: n - 1], depth)
+    flags = np.zeros(cap, np.uint32)
+    flags[: n - 1] = (depth % 2 == 0)
+    wide = np.concatenate([[0], np.cumsum(flags)[:-1]]).astype(np.uint32)
+    assert np.array_equal(layout.scratch_u32(scratch, layout.SCRATCH_WIDE, cap), wide)
+    assert int(header["node_count"]) == int(flags.sum())
+
+    # aggregates: float64 range sums and exact boxes
+    sorted_summaries = leaves["summary"][order]
+    first, last, sums = lt_checks.range_sums(sorted_summaries, left, right)
+    agg = layout.scratch_aggregates(scratch, cap)[: n - 1]
+    assert np.allclose(agg["face"].astype(np.float64), sums, rtol=1e-5, atol=1e-4)
+    for i in range(n - 1):
+        a, b = first[i], last[i]
+        assert np.array_equal(agg["lo"][i], sorted_summaries["lo"][a:b + 1].min(axis=0))
+        assert np.array_equal(agg["hi"][i], sorted_summaries["hi"][a:b + 1].max(axis=0))
+
+    # the reference test's structure check
+    nodes = out["nodes_rec"][: int(header["node_count"])]
+    lt_checks.check_bvh(header, out["state_rec"], nodes, out["leaves_rec"][:n])
+
+
+def test_oracle_empty_single_and_gate(oracle):
+    empty = run_oracle(oracle, synthetic.light_leaves(0, 1))
+    h = empty["header_rec"]
+    assert int(h["root"]) == layout.LT_NONE and int(h["leaf_count"]) == 0 and int(h["node_count"]) == 0
+    assert int(empty["state_rec"]["builds"]) == 1
+
+    one = run_oracle(oracle, synthetic.light_leaves(1, 2))
+    h = one["header_rec"]
+    assert int(h["root"]) == 0 and int(h["leaf_count"]) == 1 and int(h["node_count"]) == 1
+    node = one["nodes_rec"][0]
+    assert list(node["child"]) == [layout.LT_CHILD_LEAF_BIT, layout.LT_NONE, layout.LT_NONE, layout.LT_NONE]
+    assert int(one["leaves_rec"][0]["parent"]) == 0
+    lt_checks.check_bvh(h, one["state_rec"], one["nodes_rec"][:1], one["leaves_rec"][:1])
+
+    gated = run_oracle(oracle, synthetic.light_leaves(100, 3), building=0, builds=7)
+    assert int(gated["state_rec"]["builds"]) == 7
+    assert int(gated["header_rec"]["leaf_count"]) == 0
+
+
+def test_oracle_dropped_leaves_and_node_overflow(oracle):
+    leaves = synthetic.light_leaves(200, 4)
+    dropped = run_oracle(oracle, leaves[:128], capacity=128, found=200)
+    assert int(dropped["header_rec"]["leaf_count"]) == 128
+    assert int(dropped["state_rec"]["leaves_dropped"]) == 72
+    overflow = run_oracle(oracle, leaves, capacity=256, node_capacity=8)
+    assert int(overflow["state_rec"]["node_overflow"]) == 1
+    assert int(overflow["header_rec"]["leaf_count"]) == 0
+    assert int(overflow["header_rec"]["root"]) == layout.LT_NONE
+    assert int(overflow["header_rec"]["node_count"]) > 8
