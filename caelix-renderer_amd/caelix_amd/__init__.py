@@ -43,6 +43,19 @@ class CxLightBvhBuild(ctypes.Structure):
     ]
 
 
+class CxHashTable(ctypes.Structure):
+    _fields_ = [
+        ("keys", ctypes.c_void_p),
+        ("stack", ctypes.c_void_p),
+        ("counters", ctypes.c_void_p),
+        ("payloads", ctypes.c_void_p),
+        ("slot_count", ctypes.c_uint32),
+        ("payload_stride", ctypes.c_uint32),
+        ("probe_limit", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+    ]
+
+
 # Exported C symbols and their signatures (include/caelix_amd/caelix_amd.h).
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32
@@ -57,6 +70,12 @@ SIGNATURES = {
     "cx_light_bvh_build": (_INT, [ctypes.POINTER(CxLightBvhBuild), _P]),
     "cx_radix_sort_state_bytes": (_U64, [_U32]),
     "cx_radix_sort_pairs": (_INT, [_P, _P, _P, _P, _P, _P, _U32, _P, _P]),
+    "cx_hash_fingerprint": (_INT, [_P, _U32, _U32, _P, _P]),
+    "cx_hash_lookup": (_INT, [ctypes.POINTER(CxHashTable), _P, _U32, _U32, _U32, _U32, _P, _P, _P]),
+    "cx_hash_maintain": (_INT, [ctypes.POINTER(CxHashTable), _U32, _U32, _U32, _U32, _U32, _U32, _P, _P, _P]),
+    "cx_hash_build_scratch_bytes": (_U64, [_U32]),
+    "cx_hash_build_bricks": (_INT, [_P, _U32, _P, _U32, _P, _P, _P]),
+    "cx_hash_get_bricks": (_INT, [_P, _U32, _P, _P, _U32, _P, _P, _P]),
     "cx_profile_enable": (None, [_INT]),
     "cx_profile_reset": (None, []),
     "cx_profile_read": (_INT, [_U32, _P, _P, _P, _P]),
@@ -195,3 +214,88 @@ def profile_read() -> dict:
     count = ctypes.c_uint32()
     check(lib.cx_profile_read(max_stages, names, ms, launches, ctypes.byref(count)), "cx_profile_read")
     return {names[i].decode(): (ms[i], launches[i]) for i in range(min(count.value, max_stages))}
+
+
+class BrickHashTable:
+    """A bulk-built brick table (cx_hash_build_bricks) over int32 (x, y, z) brick keys at a load
+    factor: slot_count = ceil(count / load)."""
+
+    def __init__(self, keys_xyz, load: float = 0.7, slot_count: int | None = None, stream=None):
+        torch = _torch()
+        self.lib = load_library = load()
+        self.count = int(keys_xyz.shape[0])
+        self.slot_count = int(slot_count) if slot_count is not None else max(int(np.ceil(self.count / load)), self.count + 1)
+        device = keys_xyz.device
+        self.keys_xyz = keys_xyz.contiguous()
+        self.slots = torch.empty(self.slot_count * 4, dtype=torch.int32, device=device)
+        self.info = torch.zeros(4, dtype=torch.int32, device=device)
+        scratch = device_bytes(int(load_library.cx_hash_build_scratch_bytes(self.count)), device)
+        check(load_library.cx_hash_build_bricks(self.keys_xyz.data_ptr(), self.count, self.slots.data_ptr(),
+                                                self.slot_count, scratch.data_ptr(), self.info.data_ptr(),
+                                                stream_handle(stream)), "cx_hash_build_bricks")
+        self._scratch = scratch
+
+    def get(self, keys_xyz, with_probes: bool = False, stream=None):
+        torch = _torch()
+        n = int(keys_xyz.shape[0])
+        payloads = torch.empty(n, dtype=torch.int32, device=keys_xyz.device)
+        probes = torch.empty(n, dtype=torch.int32, device=keys_xyz.device) if with_probes else None
+        check(self.lib.cx_hash_get_bricks(self.slots.data_ptr(), self.slot_count, self.info.data_ptr(),
+                                          keys_xyz.contiguous().data_ptr(), n, payloads.data_ptr(),
+                                          probes.data_ptr() if probes is not None else None,
+                                          stream_handle(stream)), "cx_hash_get_bricks")
+        return payloads, probes
+
+
+class DynamicHashTable:
+    """The reference's table (HashLookupOrInsert, maintenance) over a payload pool of `capacity`
+    payloads of `stride` bytes: power-of-two key slots, 4 per payload (hash_table.cpp:47), 8-probe
+    window."""
+
+    def __init__(self, capacity: int, key_words: int, stride: int | None = None, device="cuda"):
+        torch = _torch()
+        self.lib = load()
+        self.capacity = capacity
+        self.key_words = key_words
+        self.stride = stride if stride is not None else 8 + 4 * key_words + 4
+        self.stride = (self.stride + 3) // 4 * 4
+        slots = 1
+        while slots < 4 * capacity:
+            slots *= 2
+        self.slot_count = slots
+        self.keys = torch.zeros(slots * 4, dtype=torch.int32, device=device)
+        self.stack = torch.zeros(1 + capacity, dtype=torch.int32, device=device)
+        self.counters = torch.zeros(16, dtype=torch.int32, device=device)
+        self.payloads = torch.zeros(capacity * self.stride // 4, dtype=torch.int32, device=device)
+        self.active = torch.zeros(capacity, dtype=torch.int32, device=device)
+        self.dispatch = torch.zeros(4, dtype=torch.int32, device=device)
+        self.desc = CxHashTable(self.keys.data_ptr(), self.stack.data_ptr(), self.counters.data_ptr(),
+                                self.payloads.data_ptr(), slots, self.stride, 8, 0)
+        self.initialised = False
+
+    def maintain(self, frame: int, max_age: int = 128, items_per_group: int = 64):
+        check(self.lib.cx_hash_maintain(ctypes.byref(self.desc), self.capacity, self.key_words, frame, max_age,
+                                        items_per_group, 0 if self.initialised else 1, self.active.data_ptr(),
+                                        self.dispatch.data_ptr(), stream_handle()), "cx_hash_maintain")
+        self.initialised = True
+        return self.counters_host()
+
+    def lookup(self, words, frame: int, insert: bool):
+        """words: int32 tensor [count, key_words]. Returns (results, payloads) tensors."""
+        torch = _torch()
+        n = int(words.shape[0])
+        results = torch.empty(n, dtype=torch.int32, device=words.device)
+        payloads = torch.empty(n, dtype=torch.int32, device=words.device)
+        check(self.lib.cx_hash_lookup(ctypes.byref(self.desc), words.contiguous().data_ptr(), self.key_words, n,
+                                      frame, 1 if insert else 0, results.data_ptr(), payloads.data_ptr(),
+                                      stream_handle()), "cx_hash_lookup")
+        return results, payloads
+
+    def counters_host(self) -> dict:
+        c = to_host(self.counters, np.uint32)
+        names = ["live", "active", "inserted", "pending", "failed_slot", "failed_payload", "evicted",
+                 "rebuild_dropped"]
+        return {name: int(c[i]) for i, name in enumerate(names)}
+
+    def top(self) -> int:
+        return int(to_host(self.stack, np.int32)[0])

@@ -48,6 +48,41 @@ enum : uint32_t
     kCtlWords = 64,
 };
 
+// The bottom-up tree build runs in levels of nested leaf blocks: a level-1 workgroup owns 2048
+// sorted leaves, a level-L workgroup the 32 level-(L-1) blocks below it.
+constexpr uint32_t kTreeThreads = 256;
+constexpr uint32_t kTreeTile = 2048;
+constexpr uint32_t kTreeFanout = 32;
+constexpr uint32_t kTopsCap = 256;
+
+__host__ __device__ inline uint64_t TreeTile(uint32_t level)
+{
+    uint64_t t = kTreeTile;
+    for (uint32_t l = 1; l < level; ++l)
+    {
+        t *= kTreeFanout;
+    }
+    return t;
+}
+__host__ __device__ inline uint32_t TreeLevels(uint32_t capacity)
+{
+    uint32_t level = 1;
+    while (TreeTile(level) < capacity)
+    {
+        ++level;
+    }
+    return level;
+}
+__host__ __device__ inline uint32_t TreeBlocks(uint32_t capacity, uint32_t level)
+{
+    return uint32_t((uint64_t(capacity) + TreeTile(level) - 1) / TreeTile(level));
+}
+__host__ __device__ inline uint64_t TopsBytes(uint32_t capacity, uint32_t level)
+{
+    const uint64_t blocks = TreeBlocks(capacity, level);
+    return ((blocks * kTopsCap * 8 + blocks * 4) + 255) & ~uint64_t(255);
+}
+
 struct Scratch
 {
     uint8_t* base;
@@ -78,12 +113,35 @@ struct Scratch
         return state_base() + ((offset + 255) & ~uint64_t(255));
     }
     __host__ __device__ SortState sort() const { return SortStateAt(sort_base(), capacity); }
+    // Tops of the tree levels: per level-L workgroup, up to kTopsCap (first, last) leaf ranges of
+    // finished elements whose parent gap is on the workgroup's boundary, and their count.
+    __host__ __device__ uint8_t* tops_base() const
+    {
+        const uint64_t offset = uint64_t(sort_base() - base) + SortStateBytes(capacity);
+        return base + ((offset + 255) & ~uint64_t(255));
+    }
+    __host__ __device__ uint2* tops(uint32_t level) const
+    {
+        uint8_t* p = tops_base();
+        for (uint32_t l = 1; l < level; ++l)
+        {
+            p += TopsBytes(capacity, l);
+        }
+        return reinterpret_cast<uint2*>(p);
+    }
+    __host__ __device__ uint32_t* tops_count(uint32_t level) const
+    {
+        return reinterpret_cast<uint32_t*>(tops(level) + uint64_t(TreeBlocks(capacity, level)) * kTopsCap);
+    }
     static uint64_t Bytes(uint32_t capacity)
     {
-        const uint32_t tiles = DivUp(capacity, kScanTile);
-        const uint64_t offset = kCtlWords * 4 + kBoundsMaxBlocks * 32 + uint64_t(2) * tiles * 8;
-        return uint64_t(kLtScratchSortState) * capacity + ((offset + 255) & ~uint64_t(255)) +
-               SortStateBytes(capacity);
+        const Scratch s{nullptr, capacity};
+        uint64_t bytes = uint64_t(s.tops_base() - static_cast<uint8_t*>(nullptr));
+        for (uint32_t l = 1; l <= TreeLevels(capacity); ++l)
+        {
+            bytes += TopsBytes(capacity, l);
+        }
+        return bytes;
     }
 };
 
@@ -306,13 +364,44 @@ __global__ void __launch_bounds__(kBvhThreads) BoundsKernel(Build b)
     }
     __threadfence();
     // The last workgroup: reduce the partials, write the header/state, reset the build's counters.
-    float v = t < 3 ? 3.0e38f : -3.0e38f;
+    float plo[3] = {3.0e38f, 3.0e38f, 3.0e38f};
+    float phi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    for (uint32_t g = t; g < gridDim.x; g += kBvhThreads)
+    {
+        const uint32_t* p = reinterpret_cast<const uint32_t*>(partials) + g * 8;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+        {
+            plo[k] = fminf(plo[k], __uint_as_float(LoadRelaxed(p + k)));
+            phi[k] = fmaxf(phi[k], __uint_as_float(LoadRelaxed(p + 3 + k)));
+        }
+    }
+#pragma unroll
+    for (int offset = kWave / 2; offset > 0; offset >>= 1)
+    {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+        {
+            plo[k] = fminf(plo[k], __shfl_xor(plo[k], offset, kWave));
+            phi[k] = fmaxf(phi[k], __shfl_xor(phi[k], offset, kWave));
+        }
+    }
+    if (Lane() == 0)
+    {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+        {
+            lds[k][wave] = plo[k];
+            lds[3 + k][wave] = phi[k];
+        }
+    }
+    __syncthreads();
     if (t < 6)
     {
-        for (uint32_t g = 0; g < gridDim.x; ++g)
+        float v = lds[t][0];
+        for (uint32_t w = 1; w < kBvhThreads / kWave; ++w)
         {
-            const float p = __uint_as_float(LoadRelaxed(reinterpret_cast<const uint32_t*>(partials) + g * 8 + t));
-            v = t < 3 ? fminf(v, p) : fmaxf(v, p);
+            v = t < 3 ? fminf(v, lds[t][w]) : fmaxf(v, lds[t][w]);
         }
         b.state->bounds[t] = v;
     }
@@ -384,7 +473,7 @@ __global__ void __launch_bounds__(kSortThreads) MortonKernel(Build b)
         {
             break;
         }
-        counter[i] = 0;
+        counter[i] = i + 1 < n ? kLtNone : 0u;  // gap slots of the tree build; 0 past the last gap
         diff[i] = 0;
         if (i < n)
         {
@@ -406,123 +495,216 @@ __global__ void __launch_bounds__(kSortThreads) MortonKernel(Build b)
 }
 
 // ------------------------------------------------------------------------------------------------
-// karras (caelix_light_bvh_karras.hlsl:8-96)
+// The binary radix tree and its aggregation in one bottom-up pass.
+//
+// The tree is the reference's (caelix_light_bvh_karras.hlsl:8-96): the binary radix tree over the
+// sorted keys, equal keys ordered by sorted position, so the key of sorted position i is
+// (code << 32 | i) and delta(g) = the common prefix length of positions g and g + 1 (32 + clz of
+// the positions for equal codes). Instead of a binary search per node, every element (a leaf, or a
+// node with leaf range [a, b]) climbs to its parent: the parent's split is the adjacent gap with the
+// longer common prefix, b if delta(b) > delta(a - 1), else a - 1 (Apetrei 2014). Karras numbers a
+// node by the range end that is not its parent's split: a left child [a, g] is node g, a right child
+// [g + 1, b] is node g + 1, the root is node 0. The first element to arrive at a gap parks its far
+// range end in slot[gap] and stops; the second builds the parent: range, children, parent links, and
+// the union of the two summaries (left, then right: caelix_light_bvh_aggregate.hlsl:30-55), then
+// climbs on. Slot arrivals are workgroup-scope atomics: a level-L workgroup only arrives at gaps
+// inside its own block of sorted leaves, so both arrivals of a gap happen in one workgroup of one
+// launch, or the earlier one in an earlier launch. An element whose parent gap is on the block's
+// boundary is handed to the next level as a "top". Also written per node: its split gap and +1/-1 at
+// its range ends in the gap-cover array (for the depths).
+
 CX_DEV uint32_t Clz(uint32_t x)
 {
     return x == 0 ? 32u : uint32_t(__clz(int(x)));
 }
 
-CX_DEV int Delta(const uint32_t* keys, int i, int j, int n, uint32_t key_i)
+// delta(g) of the gap between sorted positions g and g + 1; -1 outside [0, n - 1).
+CX_DEV int GapDelta(const uint32_t* keys, int g, int n)
 {
-    if (j < 0 || j >= n)
+    if (g < 0 || g + 1 >= n)
     {
         return -1;
     }
-    const uint32_t key_j = keys[j];
-    if (key_i == key_j)
+    const uint32_t k0 = keys[g];
+    const uint32_t k1 = keys[g + 1];
+    if (k0 == k1)
     {
-        return 32 + int(Clz(uint32_t(i) ^ uint32_t(j)));
+        return 32 + int(Clz(uint32_t(g) ^ uint32_t(g + 1)));
     }
-    return int(Clz(key_i ^ key_j));
+    return int(Clz(k0 ^ k1));
 }
 
-__global__ void __launch_bounds__(kBvhThreads) KarrasKernel(Build b)
+CX_DEV Summary ElementSummary(const Build& b, uint32_t element)
 {
-    const uint32_t index = blockIdx.x * kBvhThreads + threadIdx.x;
-    if (index + 1 >= b.leaf_capacity || !Building(b))
+    if ((element & kLtChildLeafBit) != 0)
     {
-        return;
+        const uint32_t leaf = b.scratch.values(0)[element & ~kLtChildLeafBit];
+        return LoadSummary(b.leaves[leaf].summary.face);
     }
-    const int n = int(b.header->leaf_count);
-    const int i = int(index);
-    if (i >= n - 1)
+    const uint64_t* q = reinterpret_cast<const uint64_t*>(b.scratch.aggregates() + uint64_t(element) * 12);
+    uint64_t w[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
     {
-        return;
+        w[i] = __hip_atomic_load(q + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
-    uint32_t* parent = b.scratch.u32(kLtScratchParent);
-    if (i == 0)
+    Summary s;
+    __builtin_memcpy(&s, w, sizeof(s));
+    return s;
+}
+
+CX_DEV void StoreAggregate(const Build& b, uint32_t node, const Summary& s)
+{
+    uint64_t w[6];
+    __builtin_memcpy(w, &s, sizeof(s));
+    uint64_t* q = reinterpret_cast<uint64_t*>(b.scratch.aggregates() + uint64_t(node) * 12);
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
     {
-        parent[0] = kLtNone;
+        __hip_atomic_store(q + i, w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
+}
+
+// Climbs from a finished element with leaf range [a, last] (a leaf when a == last) inside the
+// block [block_lo, block_hi). Returns when the element arrives first at a gap, becomes the root, or
+// has its parent gap on the block boundary (then it is appended to the block's tops).
+CX_DEV void Climb(const Build& b, int n, uint32_t a, uint32_t last, uint64_t block_lo, uint64_t block_hi,
+                  uint2* tops, uint32_t* lds_top_count)
+{
     const uint32_t* keys = b.scratch.keys(0);
-    const uint32_t key_i = keys[i];
-    const int d = Delta(keys, i, i + 1, n, key_i) - Delta(keys, i, i - 1, n, key_i) >= 0 ? 1 : -1;
-    const int delta_min = Delta(keys, i, i - d, n, key_i);
-    int length_max = 2;
-    while (Delta(keys, i, i + length_max * d, n, key_i) > delta_min)
-    {
-        length_max *= 2;
-    }
-    int length = 0;
-    for (int step = length_max / 2; step >= 1; step /= 2)
-    {
-        if (Delta(keys, i, i + (length + step) * d, n, key_i) > delta_min)
-        {
-            length += step;
-        }
-    }
-    const int j = i + length * d;
-    const int delta_node = Delta(keys, i, j, n, key_i);
-    int split = 0;
-    int divisor = 2;
-    int step2 = (length + divisor - 1) / divisor;
+    uint32_t* slot = b.scratch.u32(kLtScratchCounter);
+    uint32_t* left_out = b.scratch.u32(kLtScratchLeft);
+    uint32_t* right_out = b.scratch.u32(kLtScratchRight);
+    uint32_t* parent_out = b.scratch.u32(kLtScratchParent);
+    uint32_t* leaf_parent_out = b.scratch.u32(kLtScratchSortedLeafParent);
+    int32_t* diff = b.scratch.cover_diff();
+    uint32_t* split_out = b.scratch.split();
     while (true)
     {
-        if (Delta(keys, i, i + (split + step2) * d, n, key_i) > delta_node)
+        if (a == 0 && last == uint32_t(n - 1))
         {
-            split += step2;
+            return;  // the root (node 0) is finished
         }
-        if (step2 == 1)
+        const int d_left = GapDelta(keys, int(a) - 1, n);
+        const int d_right = GapDelta(keys, int(last), n);
+        const bool is_left_child = d_right > d_left;
+        const uint32_t gap = is_left_child ? last : a - 1;
+        if (uint64_t(gap) < block_lo || uint64_t(gap) + 1 >= block_hi)
         {
-            break;
-        }
-        divisor *= 2;
-        step2 = (length + divisor - 1) / divisor;
-    }
-    const int gamma = i + split * d + min(d, 0);
-    const uint32_t left = min(i, j) == gamma ? (kLtChildLeafBit | uint32_t(gamma)) : uint32_t(gamma);
-    const uint32_t right = max(i, j) == gamma + 1 ? (kLtChildLeafBit | uint32_t(gamma + 1)) : uint32_t(gamma + 1);
-    b.scratch.u32(kLtScratchLeft)[i] = left;
-    b.scratch.u32(kLtScratchRight)[i] = right;
-    b.scratch.u32((left & kLtChildLeafBit) != 0 ? kLtScratchSortedLeafParent : kLtScratchParent)
-        [left & ~kLtChildLeafBit] = uint32_t(i);
-    b.scratch.u32((right & kLtChildLeafBit) != 0 ? kLtScratchSortedLeafParent : kLtScratchParent)
-        [right & ~kLtChildLeafBit] = uint32_t(i);
-    // Gap cover: node i spans the gaps min(i, j) .. max(i, j) - 1; its split gap is gamma.
-    int32_t* diff = b.scratch.cover_diff();
-    AtomicAddRelaxed(&diff[min(i, j)], 1);
-    AtomicAddRelaxed(&diff[max(i, j)], -1);
-    b.scratch.split()[i] = uint32_t(gamma);
-}
-
-// ------------------------------------------------------------------------------------------------
-// aggregate (caelix_light_bvh_aggregate.hlsl:30-55)
-__global__ void __launch_bounds__(kBvhThreads) AggregateKernel(Build b)
-{
-    const uint32_t j = blockIdx.x * kBvhThreads + threadIdx.x;
-    if (j >= b.leaf_capacity || !Building(b))
-    {
-        return;
-    }
-    const uint32_t n = b.header->leaf_count;
-    if (j >= n || n < 2)
-    {
-        return;
-    }
-    uint32_t* counter = b.scratch.u32(kLtScratchCounter);
-    const uint32_t* left = b.scratch.u32(kLtScratchLeft);
-    const uint32_t* right = b.scratch.u32(kLtScratchRight);
-    const uint32_t* parent = b.scratch.u32(kLtScratchParent);
-    uint32_t node = b.scratch.u32(kLtScratchSortedLeafParent)[j];
-    for (uint32_t depth = 0; depth < 128 && node != kLtNone; ++depth)
-    {
-        if (AtomicAddAcqRel(&counter[node], 1u) == 0)
-        {
+            const uint32_t k = atomicAdd(lds_top_count, 1u);
+            if (k < kTopsCap)
+            {
+                tops[k] = make_uint2(a, last);
+            }
             return;
         }
-        const Summary s = Union(ChildSummary(b, left[node], true), ChildSummary(b, right[node], true));
-        StoreSummaryCoherent(b.scratch.aggregates() + uint64_t(node) * 12, s);
-        node = parent[node];
+        const uint32_t far_end = is_left_child ? a : last;
+        const uint32_t other = __hip_atomic_exchange(&slot[gap], far_end, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (other == kLtNone)
+        {
+            return;  // the sibling finishes the parent
+        }
+        // The parent: range [pa, pb], split `gap`, node index by its own parent gap.
+        const uint32_t pa = is_left_child ? a : other;
+        const uint32_t pb = is_left_child ? other : last;
+        const uint32_t left = pa == gap ? (kLtChildLeafBit | gap) : gap;
+        const uint32_t right = pb == gap + 1 ? (kLtChildLeafBit | (gap + 1)) : gap + 1;
+        uint32_t node;
+        if (pa == 0 && pb == uint32_t(n - 1))
+        {
+            node = 0;
+            parent_out[0] = kLtNone;
+        }
+        else
+        {
+            const bool parent_is_left = GapDelta(keys, int(pb), n) > GapDelta(keys, int(pa) - 1, n);
+            node = parent_is_left ? pb : pa;
+        }
+        left_out[node] = left;
+        right_out[node] = right;
+        ((left & kLtChildLeafBit) != 0 ? leaf_parent_out : parent_out)[left & ~kLtChildLeafBit] = node;
+        ((right & kLtChildLeafBit) != 0 ? leaf_parent_out : parent_out)[right & ~kLtChildLeafBit] = node;
+        split_out[node] = gap;
+        AtomicAddRelaxed(&diff[pa], 1);
+        AtomicAddRelaxed(&diff[pb], -1);
+        StoreAggregate(b, node, Union(ElementSummary(b, left), ElementSummary(b, right)));
+        a = pa;
+        last = pb;
+    }
+}
+
+// Level 1: the workgroup's own 2048 sorted leaves. Level L > 1: the tops of its 32 sub-blocks.
+__global__ void __launch_bounds__(kTreeThreads) TreeLevelKernel(Build b, uint32_t level)
+{
+    if (!Building(b))
+    {
+        return;
+    }
+    __shared__ uint32_t lds_top_count;
+    __shared__ uint32_t lds_offsets[kTreeFanout + 1];
+    const uint32_t t = threadIdx.x;
+    const int n = int(b.scratch.ctl()[kCtlLeafCount]);
+    const uint64_t tile = TreeTile(level);
+    const uint64_t block_lo = uint64_t(blockIdx.x) * tile;
+    const uint64_t block_hi = block_lo + tile;
+    uint2* tops = b.scratch.tops(level) + uint64_t(blockIdx.x) * kTopsCap;
+    if (t == 0)
+    {
+        lds_top_count = 0;
+    }
+    __syncthreads();
+    if (n >= 2 && block_lo < uint64_t(n))
+    {
+        if (level == 1)
+        {
+            for (uint32_t k = t; k < kTreeTile; k += kTreeThreads)
+            {
+                const uint64_t j = block_lo + k;
+                if (j < uint64_t(n))
+                {
+                    Climb(b, n, uint32_t(j), uint32_t(j), block_lo, block_hi, tops, &lds_top_count);
+                }
+            }
+        }
+        else
+        {
+            const uint32_t child_first = blockIdx.x * kTreeFanout;
+            const uint32_t child_blocks = TreeBlocks(b.leaf_capacity, level - 1);
+            const uint32_t* counts = b.scratch.tops_count(level - 1);
+            if (t == 0)
+            {
+                uint32_t sum = 0;
+                for (uint32_t c = 0; c < kTreeFanout; ++c)
+                {
+                    lds_offsets[c] = sum;
+                    const uint32_t child = child_first + c;
+                    sum += child < child_blocks ? min(counts[child], kTopsCap) : 0u;
+                }
+                lds_offsets[kTreeFanout] = sum;
+            }
+            __syncthreads();
+            const uint2* child_tops = b.scratch.tops(level - 1);
+            const uint32_t total = lds_offsets[kTreeFanout];
+            for (uint32_t k = t; k < total; k += kTreeThreads)
+            {
+                uint32_t c = 0;
+                while (lds_offsets[c + 1] <= k)
+                {
+                    ++c;
+                }
+                const uint2 e = child_tops[uint64_t(child_first + c) * kTopsCap + (k - lds_offsets[c])];
+                Climb(b, n, e.x, e.y, block_lo, block_hi, tops, &lds_top_count);
+            }
+        }
+    }
+    __syncthreads();
+    if (t == 0)
+    {
+        b.scratch.tops_count(level)[blockIdx.x] = lds_top_count;
+        if (lds_top_count > kTopsCap)
+        {
+            b.state->pad0[0] = 1;  // never expected: tops are bounded by twice the tree depth
+        }
     }
 }
 
@@ -941,13 +1123,10 @@ CX_API int cx_light_bvh_build(const CxLightBvhBuild* build, void* stream)
         cx::LaunchSortPass(b.scratch.keys(in), b.scratch.values(in), b.scratch.keys(1 - in), b.scratch.values(1 - in),
                            count, gate, sort, pass, s);
     }
+    for (uint32_t level = 1; level <= cx::TreeLevels(capacity); ++level)
     {
-        cx::StageScope scope(s, "lt_karras");
-        cx::KarrasKernel<<<per_item_blocks, cx::kBvhThreads, 0, s>>>(b);
-    }
-    {
-        cx::StageScope scope(s, "lt_aggregate");
-        cx::AggregateKernel<<<per_item_blocks, cx::kBvhThreads, 0, s>>>(b);
+        cx::StageScope scope(s, "lt_tree");
+        cx::TreeLevelKernel<<<cx::TreeBlocks(capacity, level), cx::kTreeThreads, 0, s>>>(b, level);
     }
     const uint32_t scan_tiles = b.scratch.scan_tiles();
     {

@@ -96,6 +96,56 @@ CX_API int cx_radix_sort_pairs(uint32_t* keys0, uint32_t* values0, uint32_t* key
                                void* sort_state, void* stream);
 
 /* ------------------------------------------------------------------------------------------
+ * Hash table (lib/src/rhi/hash_table.{h,cpp}, shaders/common/caelix_hash_table.hlsl,
+ * shaders/hash/*.hlsl). The table maps 64-bit key fingerprints of 1..8 key words to payload indices
+ * in 16-byte HashKeySlot slots. The home slot of a key is key_lo mod slot_count (the reference's
+ * key_lo & key_mask for a power-of-two slot count); a lookup probes probe_limit slots from home
+ * (kHashProbes = 8 in the reference's form).
+ * ------------------------------------------------------------------------------------------ */
+typedef struct CxHashTable {
+    void* keys;     /* HashKeySlot[slot_count] */
+    void* stack;    /* free stack: int top, uint entries[payload capacity] (lookups that insert) */
+    void* counters; /* HashCounters (lookups that insert, maintenance) */
+    void* payloads; /* the user's pool: payload i at payloads + i * payload_stride */
+    uint32_t slot_count;
+    uint32_t payload_stride;
+    uint32_t probe_limit;
+    uint32_t pad;
+} CxHashTable;
+
+/* HashFingerprint (caelix_hash_table.hlsl:113-128) of `count` keys of `word_count` words each:
+ * out[2i] = key_lo, out[2i+1] = key_hi. */
+CX_API int cx_hash_fingerprint(const uint32_t* words, uint32_t word_count, uint32_t count, uint32_t* out,
+                               void* stream);
+/* One thread per key: HashLookup (insert 0) or HashLookupOrInsert (insert 1) with the frame counter
+ * (caelix_hash_table.hlsl:217-230); results[i] = kHash* code, payloads[i] = payload or kHashNone.
+ * Inserts count into the counters (HashCountResult). */
+CX_API int cx_hash_lookup(const CxHashTable* table, const uint32_t* words, uint32_t word_count, uint32_t count,
+                          uint32_t frame, uint32_t insert, uint32_t* results, uint32_t* payloads, void* stream);
+/* HashTable::RecordMaintenance (hash_table.cpp:79-130): [init on the first call] begin, key fill,
+ * maintain (evict payloads with frame - touched > max_age, rebuild the keys of the live ones, active
+ * list), finish (dispatch = {x, y, 1, active count}). `active` holds payload_capacity u32,
+ * `dispatch` 4 u32. */
+CX_API int cx_hash_maintain(const CxHashTable* table, uint32_t payload_capacity, uint32_t key_words, uint32_t frame,
+                            uint32_t max_age, uint32_t items_per_group, uint32_t initialise, uint32_t* active,
+                            uint32_t* dispatch, void* stream);
+
+/* Bulk brick table: every key (3 int32 brick coordinates) of `keys_xyz` inserted at once into
+ * `slots` (any slot_count > count, e.g. count / 0.7 for load 0.7), payload = the key's index. The
+ * layout is deterministic: ordered linear probing, as a sequential insert in (home slot, key index)
+ * order places the keys. info[0] = probe limit (longest displacement + 1), info[1] = longest
+ * displacement, info[2] = keys wrapped past the last slot. No counterpart in the reference, whose
+ * keys are inserted one lookup at a time (HashLookupOrInsert); the slot format and probing are its. */
+CX_API uint64_t cx_hash_build_scratch_bytes(uint32_t count);
+CX_API int cx_hash_build_bricks(const int32_t* keys_xyz, uint32_t count, HashKeySlot* slots, uint32_t slot_count,
+                                void* scratch, uint32_t* info, void* stream);
+/* Lookups in a bulk-built table: payloads[i] = the brick's index or kHashNone; probes[i] (nullable) =
+ * slots read. */
+CX_API int cx_hash_get_bricks(const HashKeySlot* slots, uint32_t slot_count, const uint32_t* info,
+                              const int32_t* keys_xyz, uint32_t count, uint32_t* payloads, uint32_t* probes,
+                              void* stream);
+
+/* ------------------------------------------------------------------------------------------
  * Stage timing: when enabled, entry points bracket every kernel with HIP events on the stream it
  * runs on. cx_profile_read synchronises those events and returns per-stage sums (milliseconds)
  * and launch counts since the last reset; names are static strings.

@@ -45,6 +45,14 @@ def load():
     lib.oracle_exclusive_scan.argtypes = [_P, _U32]
     lib.oracle_lt_build.restype = None
     lib.oracle_lt_build.argtypes = [_P, _U32, _U32, _P, _P, _P, _P]
+    lib.oracle_splitmix64.restype = ctypes.c_uint64
+    lib.oracle_splitmix64.argtypes = [ctypes.c_uint64]
+    lib.oracle_hash_fingerprint.restype = None
+    lib.oracle_hash_fingerprint.argtypes = [_P, _U32, _U32, _P]
+    lib.oracle_hash_build_bricks.restype = _U32
+    lib.oracle_hash_build_bricks.argtypes = [_P, _U32, _P, _U32]
+    lib.oracle_hash_get_brick.restype = _U32
+    lib.oracle_hash_get_brick.argtypes = [_P, _U32, _U32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P]
     _lib = lib
     return lib
 
@@ -82,3 +90,30 @@ def light_bvh_build(leaves_u8: np.ndarray, leaf_capacity: int, node_capacity: in
     load().oracle_lt_build(_ptr(leaves), leaf_capacity, node_capacity, _ptr(header), _ptr(state), _ptr(scratch),
                            _ptr(nodes))
     return {"header": header, "state": state, "nodes": nodes, "leaves": leaves, "scratch": scratch}
+
+
+def splitmix64(x: int) -> int:
+    return int(load().oracle_splitmix64(x))
+
+
+def hash_fingerprint(words: np.ndarray) -> np.ndarray:
+    """words [count, word_count] uint32 -> [count, 2] (key_lo, key_hi)."""
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    out = np.zeros((w.shape[0], 2), np.uint32)
+    load().oracle_hash_fingerprint(_ptr(w), w.shape[1], w.shape[0], _ptr(out))
+    return out
+
+
+def hash_build_bricks(keys_xyz: np.ndarray, slot_count: int) -> tuple[np.ndarray, int]:
+    """Sequential ordered insert; returns (slots [slot_count, 4] uint32, longest displacement)."""
+    k = np.ascontiguousarray(keys_xyz, dtype=np.int32)
+    slots = np.zeros((slot_count, 4), np.uint32)
+    longest = load().oracle_hash_build_bricks(_ptr(k), k.shape[0], _ptr(slots), slot_count)
+    return slots, int(longest)
+
+
+def hash_get_brick(slots: np.ndarray, probe_limit: int, x: int, y: int, z: int) -> tuple[int, int]:
+    probes = ctypes.c_uint32()
+    s = np.ascontiguousarray(slots, dtype=np.uint32)
+    payload = load().oracle_hash_get_brick(_ptr(s), s.shape[0], probe_limit, x, y, z, ctypes.byref(probes))
+    return int(payload), int(probes.value)
