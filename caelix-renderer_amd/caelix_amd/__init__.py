@@ -220,11 +220,12 @@ class BrickHashTable:
     """A bulk-built brick table (cx_hash_build_bricks) over int32 (x, y, z) brick keys at a load
     factor: slot_count = ceil(count / load)."""
 
-    def __init__(self, keys_xyz, load: float = 0.7, slot_count: int | None = None, stream=None):
+    def __init__(self, keys_xyz, load_factor: float = 0.7, slot_count: int | None = None, stream=None):
         torch = _torch()
         self.lib = load_library = load()
         self.count = int(keys_xyz.shape[0])
-        self.slot_count = int(slot_count) if slot_count is not None else max(int(np.ceil(self.count / load)), self.count + 1)
+        self.slot_count = (int(slot_count) if slot_count is not None
+                           else max(int(np.ceil(self.count / load_factor)), self.count + 1))
         device = keys_xyz.device
         self.keys_xyz = keys_xyz.contiguous()
         self.slots = torch.empty(self.slot_count * 4, dtype=torch.int32, device=device)
