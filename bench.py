@@ -30,14 +30,16 @@ HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per leaf of each light-tree stage (reads + writes the stage must make):
 #   bounds 24 (box lo/hi read), morton 32 (centre read 12 + key/value write 8 + counter/cover clears 8
-#   + ...), sort pass 16 (key/value read + write), karras 20, aggregate 148, cover+flags scans 16,
-#   wide 212 (224-byte nodes for n/2 nodes = 112, slot summaries 96, leaf parent 4).
+#   + ...), sort pass 16 (key/value read + write), tree level 1 = Karras 20 + aggregate 148 (the
+#   fused bottom-up build does both; the upper levels finish the few nodes that span level-1 blocks),
+#   gather (leaf summaries into sorted order: value 4 + leaf 48 + write 48 = 100, outside the 520
+#   canonical bytes), cover+flags scans 16, wide 212 (224-byte nodes for n/2 nodes = 112, slot summaries 96, leaf
+#   parent 4).
 STAGE_BYTES = {
     "lt_bounds": 24.0,
     "lt_morton": 32.0,
     "sort_pass": 16.0,
-    "lt_karras": 20.0,
-    "lt_aggregate": 148.0,
+    "lt_tree_l1": 168.0,
     "lt_cover_scan": 8.0,
     "lt_flags_scan": 8.0,
     "lt_wide": 212.0,

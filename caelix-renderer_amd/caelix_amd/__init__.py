@@ -56,6 +56,62 @@ class CxHashTable(ctypes.Structure):
     ]
 
 
+class CxScene(ctypes.Structure):
+    _fields_ = [
+        ("keys", ctypes.c_void_p), ("info", ctypes.c_void_p), ("occupancy", ctypes.c_void_p),
+        ("payload", ctypes.c_void_p), ("slots", ctypes.c_void_p), ("slot_count", ctypes.c_uint32),
+        ("bricks", ctypes.c_uint32), ("hash_info", ctypes.c_void_p), ("brick_bits", ctypes.c_void_p),
+        ("super_bits", ctypes.c_void_p), ("grid_min", ctypes.c_int32 * 3), ("grid_dim", ctypes.c_uint32),
+        ("group_shift", ctypes.c_uint32), ("encoding", ctypes.c_uint32), ("materials", ctypes.c_void_p),
+        ("light_codes", ctypes.c_void_p), ("emission_scale", ctypes.c_float), ("pad", ctypes.c_uint32),
+    ]
+
+
+class CxLightScene(ctypes.Structure):
+    _fields_ = [
+        ("heap", ctypes.c_void_p), ("heap_units", ctypes.c_uint32), ("leaf_capacity", ctypes.c_uint32),
+        ("record_offset", ctypes.c_void_p), ("ordinal_of_brick", ctypes.c_void_p), ("leaves", ctypes.c_void_p),
+        ("state", ctypes.c_void_p), ("totals", ctypes.c_void_p), ("scratch", ctypes.c_void_p),
+    ]
+
+
+class CxLightTreeView(ctypes.Structure):
+    _fields_ = [
+        ("headers", ctypes.c_void_p), ("nodes", ctypes.c_void_p * 2), ("leaves", ctypes.c_void_p * 2),
+        ("heap", ctypes.c_void_p), ("record_offset", ctypes.c_void_p), ("ordinal_of_brick", ctypes.c_void_p),
+        ("group_shift", ctypes.c_uint32), ("mis", ctypes.c_uint32),
+    ]
+
+
+class CxCamera(ctypes.Structure):
+    _fields_ = [
+        ("position", ctypes.c_float * 3), ("forward", ctypes.c_float * 3), ("right", ctypes.c_float * 3),
+        ("up", ctypes.c_float * 3), ("tan_half_fov_y", ctypes.c_float), ("aspect", ctypes.c_float),
+        ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+    ]
+
+
+class CxTraceParams(ctypes.Structure):
+    _fields_ = [("bounces", ctypes.c_uint32), ("frame_seed", ctypes.c_uint32), ("spp", ctypes.c_uint32),
+                ("pad", ctypes.c_uint32)]
+
+
+class CxTraceOutput(ctypes.Structure):
+    _fields_ = [("color", ctypes.c_void_p), ("radiance", ctypes.c_void_p), ("attrib", ctypes.c_void_p),
+                ("counters", ctypes.c_void_p)]
+
+
+def camera_struct(cam: dict) -> CxCamera:
+    c = CxCamera()
+    for name in ("position", "forward", "right", "up"):
+        getattr(c, name)[:] = [float(v) for v in cam[name]]
+    c.tan_half_fov_y = float(cam["tan_half_fov_y"])
+    c.aspect = float(cam["aspect"])
+    c.width = int(cam["width"])
+    c.height = int(cam["height"])
+    return c
+
+
 # Exported C symbols and their signatures (include/caelix_amd/caelix_amd.h).
 _P = ctypes.c_void_p
 _U32 = ctypes.c_uint32
@@ -76,6 +132,14 @@ SIGNATURES = {
     "cx_hash_build_scratch_bytes": (_U64, [_U32]),
     "cx_hash_build_bricks": (_INT, [_P, _U32, _P, _U32, _P, _P, _P]),
     "cx_hash_get_bricks": (_INT, [_P, _U32, _P, _P, _U32, _P, _P, _P]),
+    "cx_scene_finalize": (_INT, [ctypes.POINTER(CxScene), _P]),
+    "cx_scene_light_scratch_bytes": (_U64, [_U32]),
+    "cx_scene_light_records": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightScene), _P]),
+    "cx_trace_frame": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightTreeView), ctypes.POINTER(CxCamera),
+                              ctypes.POINTER(CxTraceParams), ctypes.POINTER(CxTraceOutput), _P]),
+    "cx_trace_rays": (_INT, [ctypes.POINTER(CxScene), _P, _U32, _P, _P]),
+    "cx_light_sample": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightTreeView), _P, _P, _U32, _P, _P, _P]),
+    "cx_light_face_pdf": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightTreeView), _P, _U32, _P, _P]),
     "cx_profile_enable": (None, [_INT]),
     "cx_profile_reset": (None, []),
     "cx_profile_read": (_INT, [_U32, _P, _P, _P, _P]),
@@ -300,3 +364,165 @@ class DynamicHashTable:
 
     def top(self) -> int:
         return int(to_host(self.stack, np.int32)[0])
+
+
+class TraceScene:
+    """A scene resident on the GPU, ready to trace: bricks, bulk hash table, finalized record words,
+    resident bitmaps, brick light records, and the light tree over the light bricks (tree 0; tree 1
+    empty). Built with the library's kernels from SceneData (caelix_amd.scene)."""
+
+    def __init__(self, data, load_factor: float = 0.7, mis: bool = True, stream=None):
+        torch = _torch()
+        lib = self.lib = load()
+        dev = "cuda"
+        self.data = data
+        n = self.bricks = int(len(data.keys))
+        self.keys = torch.from_numpy(np.ascontiguousarray(data.keys, np.int32)).to(dev)
+        self.payload = torch.from_numpy(np.ascontiguousarray(data.payload).view(np.int16)).to(dev)
+        self.materials = torch.from_numpy(np.ascontiguousarray(data.materials, np.float32)).to(dev)
+        self.light_codes = torch.from_numpy(np.ascontiguousarray(data.light_codes).view(np.int32)).to(dev)
+        self.hash = BrickHashTable(self.keys, load_factor=load_factor, stream=stream)
+        self.info = torch.zeros(n * 2, dtype=torch.int32, device=dev)
+        self.occupancy = torch.zeros(n * 8, dtype=torch.int64, device=dev)
+        dim = int(data.grid_dim)
+        cells = dim ** 3
+        self.brick_bits = torch.zeros((cells + 31) // 32, dtype=torch.int32, device=dev)
+        self.super_bits = torch.zeros((cells // 512 + 31) // 32 + 1, dtype=torch.int32, device=dev)
+        sc = self.scene = CxScene()
+        sc.keys = self.keys.data_ptr()
+        sc.info = self.info.data_ptr()
+        sc.occupancy = self.occupancy.data_ptr()
+        sc.payload = self.payload.data_ptr()
+        sc.slots = self.hash.slots.data_ptr()
+        sc.slot_count = self.hash.slot_count
+        sc.bricks = n
+        sc.hash_info = self.hash.info.data_ptr()
+        sc.brick_bits = self.brick_bits.data_ptr()
+        sc.super_bits = self.super_bits.data_ptr()
+        sc.grid_min[:] = list(data.grid_min)
+        sc.grid_dim = dim
+        sc.group_shift = int(data.group_shift)
+        sc.encoding = 1
+        sc.materials = self.materials.data_ptr()
+        sc.light_codes = self.light_codes.data_ptr()
+        sc.emission_scale = 1.0
+        h = stream_handle(stream)
+        check(lib.cx_scene_finalize(ctypes.byref(sc), h), "cx_scene_finalize")
+        # Light pass, growing the heap when it does not fit (as the reference grows its light heap).
+        self.light_scratch = device_bytes(int(lib.cx_scene_light_scratch_bytes(n)), dev)
+        self.record_offset = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
+        self.ordinal_of_brick = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
+        self.totals = torch.zeros(4, dtype=torch.int32, device=dev)
+        heap_units = max(64, n * 8)
+        while True:
+            leaf_capacity = max(64, (n + 63) // 64 * 64)
+            self.tree = LightBvhBuilder(leaf_capacity)
+            self.heap = torch.zeros(heap_units * 4, dtype=torch.int32, device=dev)
+            ls = self.light = CxLightScene()
+            ls.heap = self.heap.data_ptr()
+            ls.heap_units = heap_units
+            ls.leaf_capacity = leaf_capacity
+            ls.record_offset = self.record_offset.data_ptr()
+            ls.ordinal_of_brick = self.ordinal_of_brick.data_ptr()
+            ls.leaves = self.tree.leaves.data_ptr()
+            ls.state = self.tree.state.data_ptr()
+            ls.totals = self.totals.data_ptr()
+            ls.scratch = self.light_scratch.data_ptr()
+            check(lib.cx_scene_light_records(ctypes.byref(sc), ctypes.byref(ls), h), "cx_scene_light_records")
+            totals = self.totals.cpu().numpy().view(np.uint32)
+            if totals[3] == 0:
+                break
+            heap_units = int(totals[1]) + 64
+        self.light_bricks = int(totals[0])
+        self.light_faces = int(totals[2])
+        # The light tree over the light bricks (building = 1, leaves_found written by the light pass).
+        st = to_host(self.tree.state).view(layout.STATE).copy()
+        st["building"] = 1
+        self.tree.state.copy_(to_device(st.view(np.uint8), self.tree.state.device))
+        self.tree.build(stream)
+        self.headers = torch.zeros(32, dtype=torch.int32, device=dev)  # 2 x LightTreeHeader
+        self.headers[:16].copy_(self.tree.header.view(torch.int32))
+        empty = np.zeros(1, layout.HEADER)
+        empty["root"] = layout.LT_NONE
+        self.headers[16:].copy_(torch.from_numpy(empty.view(np.int32).copy()).to(dev))
+        self.empty_nodes = torch.zeros(56, dtype=torch.int32, device=dev)
+        self.empty_leaves = torch.zeros(16, dtype=torch.int32, device=dev)
+        v = self.view = CxLightTreeView()
+        v.headers = self.headers.data_ptr()
+        v.nodes[0] = self.tree.nodes.data_ptr()
+        v.nodes[1] = self.empty_nodes.data_ptr()
+        v.leaves[0] = self.tree.leaves.data_ptr()
+        v.leaves[1] = self.empty_leaves.data_ptr()
+        v.heap = self.heap.data_ptr()
+        v.record_offset = self.record_offset.data_ptr()
+        v.ordinal_of_brick = self.ordinal_of_brick.data_ptr()
+        v.group_shift = int(data.group_shift)
+        v.mis = 1 if mis else 0
+        torch.cuda.synchronize()
+
+    def trace_frame(self, camera: dict, bounces: int, seed: int, spp: int = 1, color=None, radiance=None,
+                    attrib=None, counters=None, stream=None):
+        torch = _torch()
+        w, h = int(camera["width"]), int(camera["height"])
+        color = color if color is not None else torch.empty(w * h * 4, dtype=torch.float32, device="cuda")
+        radiance = radiance if radiance is not None else torch.empty(w * h * 4, dtype=torch.float32, device="cuda")
+        attrib = attrib if attrib is not None else torch.empty(w * h, dtype=torch.int32, device="cuda")
+        counters = counters if counters is not None else torch.zeros(8, dtype=torch.int64, device="cuda")
+        cam = camera_struct(camera)
+        params = CxTraceParams(bounces, seed, spp, 0)
+        out = CxTraceOutput(color.data_ptr(), radiance.data_ptr(), attrib.data_ptr(), counters.data_ptr())
+        check(self.lib.cx_trace_frame(ctypes.byref(self.scene), ctypes.byref(self.view), ctypes.byref(cam),
+                                      ctypes.byref(params), ctypes.byref(out), stream_handle(stream)), "cx_trace_frame")
+        return color, radiance, attrib, counters
+
+    def trace_rays(self, rays: np.ndarray) -> np.ndarray:
+        torch = _torch()
+        r = torch.from_numpy(np.ascontiguousarray(rays, np.float32)).cuda()
+        n = len(rays)
+        hits = torch.empty(max(n, 1) * 9, dtype=torch.int32, device="cuda")
+        check(self.lib.cx_trace_rays(ctypes.byref(self.scene), r.data_ptr(), n, hits.data_ptr(), stream_handle()),
+              "cx_trace_rays")
+        return hits.cpu().numpy().view(np.uint32).reshape(-1, 9)[:n]
+
+    def light_sample(self, receivers: np.ndarray, seeds: np.ndarray):
+        torch = _torch()
+        n = len(receivers)
+        r = torch.from_numpy(np.ascontiguousarray(receivers, np.float32)).cuda()
+        sd = torch.from_numpy(np.ascontiguousarray(seeds, np.uint32).view(np.int32)).cuda()
+        out = torch.empty(max(n, 1) * 9, dtype=torch.float32, device="cuda")
+        ok = torch.empty(max(n, 1), dtype=torch.int32, device="cuda")
+        check(self.lib.cx_light_sample(ctypes.byref(self.scene), ctypes.byref(self.view), r.data_ptr(), sd.data_ptr(),
+                                       n, out.data_ptr(), ok.data_ptr(), stream_handle()), "cx_light_sample")
+        return out.cpu().numpy().reshape(-1, 9)[:n], ok.cpu().numpy()[:n]
+
+    def light_face_pdf(self, queries: np.ndarray) -> np.ndarray:
+        torch = _torch()
+        n = len(queries)
+        q = torch.from_numpy(np.ascontiguousarray(queries, np.uint32).view(np.int32)).cuda()
+        pdf = torch.empty(max(n, 1), dtype=torch.float32, device="cuda")
+        check(self.lib.cx_light_face_pdf(ctypes.byref(self.scene), ctypes.byref(self.view), q.data_ptr(), n,
+                                         pdf.data_ptr(), stream_handle()), "cx_light_face_pdf")
+        return pdf.cpu().numpy()[:n]
+
+    def host_arrays(self) -> dict:
+        """Every device array of the scene and its light tree, copied to numpy (for the oracle)."""
+        _torch().cuda.synchronize()
+        return {
+            "keys": self.keys.cpu().numpy(),
+            "info": self.info.cpu().numpy().view(np.uint32),
+            "occupancy": self.occupancy.cpu().numpy().view(np.uint64),
+            "payload": self.payload.cpu().numpy().view(np.uint16),
+            "slots": self.hash.slots.cpu().numpy().view(np.uint32),
+            "hash_info": self.hash.info.cpu().numpy().view(np.uint32),
+            "brick_bits": self.brick_bits.cpu().numpy().view(np.uint32),
+            "super_bits": self.super_bits.cpu().numpy().view(np.uint32),
+            "materials": self.materials.cpu().numpy(),
+            "light_codes": self.light_codes.cpu().numpy().view(np.uint32),
+            "heap": self.heap.cpu().numpy().view(np.uint32),
+            "record_offset": self.record_offset.cpu().numpy().view(np.uint32),
+            "ordinal_of_brick": self.ordinal_of_brick.cpu().numpy().view(np.uint32),
+            "headers": self.headers.cpu().numpy().view(np.uint8),
+            "nodes": self.tree.nodes.cpu().numpy().view(np.uint8),
+            "leaves": self.tree.leaves.cpu().numpy().view(np.uint8),
+            "totals": self.totals.cpu().numpy().view(np.uint32),
+        }

@@ -1,0 +1,199 @@
+"""Seeded synthetic voxel scenes for the trace workloads (host side, numpy).
+
+terrain(world, seed, ...) builds a value-noise heightfield world of 8^3 bricks with uint8 material ids
+stored as block-ID-encoded opaque blocks (0x8000 | id, record_layout.h:27-34):
+
+  small   world 128^3: all 16^3 = 4,096 brick cells resident, about 30 % solid voxels, 512 emissive
+          bricks with emission uniform in [0.1, 10], 256 x 256 pixels
+  1080p   world 4096^3: 512^2 brick columns x 4 surface bricks = 1,048,576 resident bricks, 8-octave
+          terrain, 65,536 emissive bricks with emission log-uniform in [0.01, 100], 1920 x 1080
+
+Materials: ids 1..127 diffuse (no emission), ids 128..255 emissive (the emissive bricks' solid voxels
+use one of them each); albedo and emission are float16 values (the reference's material table holds
+halves, caelix_material.hlsl:90-113). The light threshold is the smallest emission luminance, so every
+emissive material is a light code.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+OPAQUE = 0x8000
+
+
+@dataclass
+class SceneData:
+    keys: np.ndarray         # int32 [n, 3] brick coordinates
+    payload: np.ndarray      # uint16 [n, 512] block values, voxel x + 8y + 64z
+    materials: np.ndarray    # float32 [65536, 8]
+    light_codes: np.ndarray  # uint32 [2048]
+    grid_min: tuple
+    grid_dim: int
+    group_shift: int
+    camera: dict
+    emissive_bricks: np.ndarray
+
+
+def value_noise(size: int, seed: int, octaves: int, base_period: int) -> np.ndarray:
+    """Sum of `octaves` value-noise layers on a size x size grid, in [0, 1): lattice values at every
+    `period` cells (period halves per octave), smoothstep-interpolated, amplitude halves per octave."""
+    rng = np.random.default_rng(seed)
+    total = np.zeros((size, size), np.float64)
+    norm = 0.0
+    amplitude = 1.0
+    period = base_period
+    coords = np.arange(size, dtype=np.float64)
+    for _ in range(octaves):
+        cells = size // period + 2
+        lattice = rng.uniform(0.0, 1.0, size=(cells, cells))
+        f = coords / period
+        i = np.floor(f).astype(np.int64)
+        t = f - i
+        s = t * t * (3.0 - 2.0 * t)
+        a = lattice[i[:, None], i[None, :]]
+        b = lattice[i[:, None] + 1, i[None, :]]
+        c = lattice[i[:, None], i[None, :] + 1]
+        d = lattice[i[:, None] + 1, i[None, :] + 1]
+        sx = s[:, None]
+        sz = s[None, :]
+        total += amplitude * ((a * (1 - sx) + b * sx) * (1 - sz) + (c * (1 - sx) + d * sx) * sz)
+        norm += amplitude
+        amplitude *= 0.5
+        period = max(period // 2, 1)
+    return total / norm
+
+
+def make_materials(seed: int, emission: tuple[float, float], log_uniform: bool):
+    rng = np.random.default_rng(seed)
+    table = np.zeros((65536, 8), np.float32)
+    ids = np.arange(1, 256)
+    albedo = rng.uniform(0.2, 0.9, size=(255, 3)).astype(np.float16).astype(np.float32)
+    table[OPAQUE | ids, 0:3] = albedo
+    lo, hi = emission
+    n_e = 128
+    if log_uniform:
+        e = np.exp(rng.uniform(np.log(lo), np.log(hi), size=n_e))
+    else:
+        e = rng.uniform(lo, hi, size=n_e)
+    tint = rng.uniform(0.7, 1.0, size=(n_e, 3))
+    em = (e[:, None] * tint).astype(np.float16).astype(np.float32)
+    table[OPAQUE | np.arange(128, 256), 4:7] = em
+    lum = em[:, 0] * np.float32(0.2126) + em[:, 1] * np.float32(0.7152) + em[:, 2] * np.float32(0.0722)
+    threshold = lum.min()
+    codes = np.zeros(65536, bool)
+    all_lum = table[:, 4] * np.float32(0.2126) + table[:, 5] * np.float32(0.7152) + table[:, 6] * np.float32(0.0722)
+    codes[(all_lum >= threshold) & (all_lum > 0)] = True
+    # word i holds codes 32i .. 32i+31, bit b = code 32i + b
+    words = np.zeros(2048, np.uint32)
+    for b in range(32):
+        words |= codes.reshape(2048, 32)[:, b].astype(np.uint32) << np.uint32(b)
+    return table, words
+
+
+def look_at(position, target, width, height, fov_y_deg=60.0):
+    p = np.asarray(position, np.float64)
+    f = np.asarray(target, np.float64) - p
+    f /= np.linalg.norm(f)
+    up0 = np.array([0.0, 1.0, 0.0])
+    r = np.cross(f, up0)
+    r /= np.linalg.norm(r)
+    u = np.cross(r, f)
+    return {"position": p.astype(np.float32), "forward": f.astype(np.float32), "right": r.astype(np.float32),
+            "up": u.astype(np.float32), "tan_half_fov_y": np.float32(np.tan(np.radians(fov_y_deg) / 2)),
+            "aspect": np.float32(width / height), "width": int(width), "height": int(height)}
+
+
+def terrain(world: int, seed: int, octaves: int, bricks_per_column: int | None, emissive: int,
+            emission: tuple[float, float], log_uniform: bool, fill: float, relief: float,
+            width: int, height: int, base_period: int | None = None, chunk: int = 1 << 16) -> SceneData:
+    """A heightfield world of `world`^3 voxels. bricks_per_column None: every brick cell of the grid is
+    resident; else that many bricks per column, from the column's highest surface brick down."""
+    dim = world // 8
+    base_period = base_period if base_period is not None else max(world // 4, 8)
+    h = value_noise(world, seed, octaves, base_period)
+    heights = np.clip(np.round(fill * world + (h - 0.5) * relief * world), 1, world - 1).astype(np.int32)
+    # heights[x, z]
+    col_max = heights.reshape(dim, 8, dim, 8).max(axis=(1, 3))  # [bx, bz]
+    if bricks_per_column is None:
+        bx, by, bz = np.meshgrid(np.arange(dim), np.arange(dim), np.arange(dim), indexing="ij")
+        keys = np.stack([bx.ravel(), by.ravel(), bz.ravel()], axis=1).astype(np.int32)
+    else:
+        top = (col_max - 1) // 8
+        bx, bz = np.meshgrid(np.arange(dim), np.arange(dim), indexing="ij")
+        cols = []
+        for k in range(bricks_per_column):
+            cols.append(np.stack([bx.ravel(), np.maximum(top.ravel() - k, 0), bz.ravel()], axis=1))
+        keys = np.concatenate(cols).astype(np.int32)
+        keys = np.unique(keys, axis=0)
+    rng = np.random.default_rng(seed + 1)
+    n = len(keys)
+    payload = np.zeros((n, 512), np.uint16)
+    v = np.arange(8)
+    for start in range(0, n, chunk):
+        k = keys[start:start + chunk]
+        xs = k[:, 0:1] * 8 + v[None, :]  # [m, 8]
+        ys = k[:, 1:2] * 8 + v[None, :]
+        zs = k[:, 2:3] * 8 + v[None, :]
+        hz = heights[xs[:, None, :], zs[:, :, None]]  # [m, z, x]
+        solid = ys[:, None, :, None] < hz[:, :, None, :]  # [m, z, y, x]
+        depth = hz[:, :, None, :] - ys[:, None, :, None]
+        mat = 1 + ((xs[:, None, None, :] * 7 + zs[:, :, None, None] * 13 + depth * 3) % 127)
+        blocks = np.where(solid, OPAQUE | mat, 0).astype(np.uint16)
+        payload[start:start + chunk] = blocks.reshape(len(k), 512)
+    # Emissive bricks: resident bricks with solid voxels and an open top (the surface bricks).
+    solid_count = (payload != 0).sum(axis=1)
+    surface = np.nonzero((solid_count > 0) & (solid_count < 512))[0]
+    chosen = rng.choice(surface, size=min(emissive, len(surface)), replace=False)
+    emissive_ids = rng.integers(128, 256, size=len(chosen))
+    for c, e in zip(chosen, emissive_ids):
+        row = payload[c]
+        row[row != 0] = OPAQUE | e
+    table, light_codes = make_materials(seed + 2, emission, log_uniform)
+    centre = world / 2
+    peak = float(heights.max())
+    camera = look_at((centre * 0.55, peak + 0.35 * world, centre * 0.35),
+                     (centre, float(np.median(heights)), centre * 1.1), width, height)
+    return SceneData(keys=keys, payload=payload, materials=table, light_codes=light_codes, grid_min=(0, 0, 0),
+                     grid_dim=dim, group_shift=5, camera=camera, emissive_bricks=np.sort(chosen))
+
+
+def small_scene(seed: int = 1) -> SceneData:
+    """4,096 bricks in 128^3, ~30 % solid, 512 emissive bricks (emission U[0.1, 10]), 256 x 256."""
+    return terrain(128, seed, octaves=5, bricks_per_column=None, emissive=512, emission=(0.1, 10.0),
+                   log_uniform=False, fill=0.3, relief=0.6, width=256, height=256, base_period=16)
+
+
+def hd_scene(seed: int = 2) -> SceneData:
+    """1,048,576 bricks in 4096^3 (8-octave terrain), 65,536 emissive (log-U[0.01, 100]), 1920 x 1080."""
+    return terrain(4096, seed, octaves=8, bricks_per_column=4, emissive=65536, emission=(0.01, 100.0),
+                   log_uniform=True, fill=0.45, relief=0.35, width=1920, height=1080)
+
+
+def furnace_box(size_bricks: int = 8, wall: int = 8, emission: float = 1.0, albedo: float = 0.5,
+                width: int = 128, height: int = 128) -> SceneData:
+    """A closed box room: every voxel outside [wall, world - wall)^3 is one opaque material with the
+    given grey emission and albedo, the inside is air, the camera sits at the centre. Every direction
+    from inside sees the same radiance, so the path tracer's expected radiance after B bounces has the
+    closed form emission * (1 - albedo^B) / (1 - albedo)."""
+    world = size_bricks * 8
+    dim = size_bricks
+    bx, by, bz = np.meshgrid(np.arange(dim), np.arange(dim), np.arange(dim), indexing="ij")
+    keys = np.stack([bx.ravel(), by.ravel(), bz.ravel()], axis=1).astype(np.int32)
+    block = OPAQUE | 200
+    v = np.arange(512)
+    lx, ly, lz = v & 7, (v >> 3) & 7, v >> 6
+    x = keys[:, 0:1] * 8 + lx[None, :]
+    y = keys[:, 1:2] * 8 + ly[None, :]
+    z = keys[:, 2:3] * 8 + lz[None, :]
+    inside = ((x >= wall) & (x < world - wall) & (y >= wall) & (y < world - wall) & (z >= wall) & (z < world - wall))
+    payload = np.where(inside, 0, block).astype(np.uint16)
+    table = np.zeros((65536, 8), np.float32)
+    table[block, 0:3] = np.float32(albedo)
+    table[block, 4:7] = np.float32(emission)
+    light_codes = np.zeros(2048, np.uint32)
+    light_codes[block >> 5] |= np.uint32(1) << np.uint32(block & 31)
+    c = world / 2
+    camera = look_at((c + 0.37, c + 0.21, c + 0.13), (c + 10.0, c - 3.0, c + 7.0), width, height, fov_y_deg=90.0)
+    return SceneData(keys=keys, payload=payload, materials=table, light_codes=light_codes, grid_min=(0, 0, 0),
+                     grid_dim=dim, group_shift=5, camera=camera, emissive_bricks=np.arange(len(keys)))

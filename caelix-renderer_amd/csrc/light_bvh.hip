@@ -41,7 +41,7 @@ constexpr uint64_t kScanFlagMask = 3ull << 32;
 // Build control words (u32) at the start of the sort-state region of the scratch.
 enum : uint32_t
 {
-    kCtlBoundsTicket = 0,
+    kCtlUnused = 0,
     kCtlLeafCount = 1,
     kCtlCoverTiles = 2,
     kCtlFlagsTiles = 3,
@@ -136,12 +136,12 @@ struct Scratch
     static uint64_t Bytes(uint32_t capacity)
     {
         const Scratch s{nullptr, capacity};
-        uint64_t bytes = uint64_t(s.tops_base() - static_cast<uint8_t*>(nullptr));
+        uint8_t* p = s.tops_base();
         for (uint32_t l = 1; l <= TreeLevels(capacity); ++l)
         {
-            bytes += TopsBytes(capacity, l);
+            p += TopsBytes(capacity, l);
         }
-        return bytes;
+        return uint64_t(p - static_cast<uint8_t*>(nullptr));
     }
 };
 
@@ -285,27 +285,59 @@ CX_DEV Summary ChildSummary(const Build& b, uint32_t child, bool coherent)
 }
 
 // ------------------------------------------------------------------------------------------------
-// bounds (caelix_light_bvh_bounds.hlsl:10-53)
+// bounds (caelix_light_bvh_bounds.hlsl:10-53), first half: per-workgroup min/max of the leaf box
+// centres (the partials) and the centres themselves. Workgroup 0 also resets the build's counters
+// and the sort histograms for the kernels that follow. The Morton kernel reduces the partials, so no
+// workgroup of this kernel waits on another.
+CX_DEV void MinMax3(float (&lo)[3], float (&hi)[3], float* lds6x4, uint32_t wave)
+{
+#pragma unroll
+    for (int offset = kWave / 2; offset > 0; offset >>= 1)
+    {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+        {
+            lo[k] = fminf(lo[k], __shfl_xor(lo[k], offset, kWave));
+            hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], offset, kWave));
+        }
+    }
+    if (Lane() == 0)
+    {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+        {
+            lds6x4[k * 4 + wave] = lo[k];
+            lds6x4[(3 + k) * 4 + wave] = hi[k];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        lo[k] = fminf(fminf(lds6x4[k * 4], lds6x4[k * 4 + 1]), fminf(lds6x4[k * 4 + 2], lds6x4[k * 4 + 3]));
+        hi[k] = fmaxf(fmaxf(lds6x4[(3 + k) * 4], lds6x4[(3 + k) * 4 + 1]),
+                      fmaxf(lds6x4[(3 + k) * 4 + 2], lds6x4[(3 + k) * 4 + 3]));
+    }
+}
+
 __global__ void __launch_bounds__(kBvhThreads) BoundsKernel(Build b)
 {
     if (!Building(b))
     {
         return;
     }
-    __shared__ float lds[6][kBvhThreads / kWave];
-    __shared__ bool last;
+    __shared__ float lds[6 * 4];
     const uint32_t t = threadIdx.x;
-    const uint32_t found = b.state->leaves_found;
-    const uint32_t n = min(found, b.leaf_capacity);
+    const uint32_t n = min(b.state->leaves_found, b.leaf_capacity);
     float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f};
     float hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
     float* centres = b.scratch.centres();
     for (uint32_t i = blockIdx.x * kBvhThreads + t; i < n; i += gridDim.x * kBvhThreads)
     {
         const float* leaf = b.leaves[i].summary.lo;
-        const float2 l01 = *reinterpret_cast<const float2*>(leaf);      // lo.x, lo.y
-        const float2 l2h0 = *reinterpret_cast<const float2*>(leaf + 2); // lo.z, hi.x
-        const float2 h12 = *reinterpret_cast<const float2*>(leaf + 4);  // hi.y, hi.z
+        const float2 l01 = *reinterpret_cast<const float2*>(leaf);       // lo.x, lo.y
+        const float2 l2h0 = *reinterpret_cast<const float2*>(leaf + 2);  // lo.z, hi.x
+        const float2 h12 = *reinterpret_cast<const float2*>(leaf + 4);   // hi.y, hi.z
         const float c0 = 0.5f * (l01.x + l2h0.y);
         const float c1 = 0.5f * (l01.y + h12.x);
         const float c2 = 0.5f * (l2h0.x + h12.y);
@@ -319,122 +351,44 @@ __global__ void __launch_bounds__(kBvhThreads) BoundsKernel(Build b)
         hi[1] = fmaxf(hi[1], c1);
         hi[2] = fmaxf(hi[2], c2);
     }
-#pragma unroll
-    for (int offset = kWave / 2; offset > 0; offset >>= 1)
-    {
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-        {
-            lo[k] = fminf(lo[k], __shfl_xor(lo[k], offset, kWave));
-            hi[k] = fmaxf(hi[k], __shfl_xor(hi[k], offset, kWave));
-        }
-    }
-    const uint32_t wave = t / kWave;
-    if (Lane() == 0)
-    {
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-        {
-            lds[k][wave] = lo[k];
-            lds[3 + k][wave] = hi[k];
-        }
-    }
-    __syncthreads();
+    MinMax3(lo, hi, lds, t / kWave);
     float* partials = b.scratch.bounds_partials();
     if (t < 6)
     {
-        float v = lds[t][0];
-        for (uint32_t w = 1; w < kBvhThreads / kWave; ++w)
+        partials[blockIdx.x * 8 + t] = t < 3 ? lo[t] : hi[t - 3];
+    }
+    if (blockIdx.x == 0)
+    {
+        uint32_t* ctl = b.scratch.ctl();
+        if (t == 0)
         {
-            v = t < 3 ? fminf(v, lds[t][w]) : fmaxf(v, lds[t][w]);
+            ctl[kCtlCoverTiles] = 0;
+            ctl[kCtlFlagsTiles] = 0;
         }
-        partials[blockIdx.x * 8 + t] = v;
-    }
-    __threadfence();
-    __syncthreads();
-    uint32_t* ctl = b.scratch.ctl();
-    if (t == 0)
-    {
-        last = AtomicAddAcqRel(&ctl[kCtlBoundsTicket], 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last)
-    {
-        return;
-    }
-    __threadfence();
-    // The last workgroup: reduce the partials, write the header/state, reset the build's counters.
-    float plo[3] = {3.0e38f, 3.0e38f, 3.0e38f};
-    float phi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
-    for (uint32_t g = t; g < gridDim.x; g += kBvhThreads)
-    {
-        const uint32_t* p = reinterpret_cast<const uint32_t*>(partials) + g * 8;
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
+        const SortState s = b.scratch.sort();
+        for (uint32_t i = t; i < kSortCtlBytes / 4; i += kBvhThreads)
         {
-            plo[k] = fminf(plo[k], __uint_as_float(LoadRelaxed(p + k)));
-            phi[k] = fmaxf(phi[k], __uint_as_float(LoadRelaxed(p + 3 + k)));
+            s.ctl[i] = 0;
         }
-    }
-#pragma unroll
-    for (int offset = kWave / 2; offset > 0; offset >>= 1)
-    {
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
+        for (uint32_t i = t; i < kSortHistBytes / 4; i += kBvhThreads)
         {
-            plo[k] = fminf(plo[k], __shfl_xor(plo[k], offset, kWave));
-            phi[k] = fmaxf(phi[k], __shfl_xor(phi[k], offset, kWave));
+            s.hist[i] = 0;
         }
-    }
-    if (Lane() == 0)
-    {
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-        {
-            lds[k][wave] = plo[k];
-            lds[3 + k][wave] = phi[k];
-        }
-    }
-    __syncthreads();
-    if (t < 6)
-    {
-        float v = lds[t][0];
-        for (uint32_t w = 1; w < kBvhThreads / kWave; ++w)
-        {
-            v = t < 3 ? fminf(v, lds[t][w]) : fmaxf(v, lds[t][w]);
-        }
-        b.state->bounds[t] = v;
-    }
-    if (t == 0)
-    {
-        b.header->leaf_count = n;
-        b.state->leaves_dropped = found - n;
-        ctl[kCtlLeafCount] = n;
-        ctl[kCtlCoverTiles] = 0;
-        ctl[kCtlFlagsTiles] = 0;
-        ctl[kCtlBoundsTicket] = 0;
-    }
-    const SortState s = b.scratch.sort();
-    for (uint32_t i = t; i < kSortCtlBytes / 4; i += kBvhThreads)
-    {
-        s.ctl[i] = 0;
-    }
-    for (uint32_t i = t; i < kSortHistBytes / 4; i += kBvhThreads)
-    {
-        s.hist[i] = 0;
     }
 }
 
 // ------------------------------------------------------------------------------------------------
 // morton (caelix_light_bvh_morton.hlsl:5-28) + the sort's digit histograms and status clear.
-// One workgroup per sort tile (4096 leaves).
-__global__ void __launch_bounds__(kSortThreads) MortonKernel(Build b)
+// One workgroup per sort tile (4096 leaves). Every workgroup first reduces the bounds partials (the
+// second half of caelix_light_bvh_bounds); workgroup 0 writes leaf_count, leaves_dropped and bounds.
+__global__ void __launch_bounds__(kSortThreads) MortonKernel(Build b, uint32_t bound_blocks)
 {
     if (!Building(b))
     {
         return;
     }
     __shared__ uint32_t lds_hist[kSortPasses * kSortRadix];
+    __shared__ float lds_minmax[6 * 4];
     const uint32_t t = threadIdx.x;
     for (uint32_t i = t; i < kSortPasses * kSortRadix; i += kSortThreads)
     {
@@ -451,14 +405,39 @@ __global__ void __launch_bounds__(kSortThreads) MortonKernel(Build b)
             b.scratch.flags_status()[tile] = 0;
         }
     }
-    __syncthreads();
-    const uint32_t n = b.header->leaf_count;
-    const float lo0 = b.state->bounds[0];
-    const float lo1 = b.state->bounds[1];
-    const float lo2 = b.state->bounds[2];
-    const float e0 = fmaxf(b.state->bounds[3] - lo0, 1e-6f);
-    const float e1 = fmaxf(b.state->bounds[4] - lo1, 1e-6f);
-    const float e2 = fmaxf(b.state->bounds[5] - lo2, 1e-6f);
+    const uint32_t found = b.state->leaves_found;
+    const uint32_t n = min(found, b.leaf_capacity);
+    float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f};
+    float hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    const float* partials = b.scratch.bounds_partials();
+    for (uint32_t g = t; g < bound_blocks; g += kSortThreads)
+    {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+        {
+            lo[k] = fminf(lo[k], partials[g * 8 + k]);
+            hi[k] = fmaxf(hi[k], partials[g * 8 + 3 + k]);
+        }
+    }
+    MinMax3(lo, hi, lds_minmax, t / kWave);
+    if (blockIdx.x == 0 && t == 0)
+    {
+        b.header->leaf_count = n;
+        b.state->leaves_dropped = found - n;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+        {
+            b.state->bounds[k] = lo[k];
+            b.state->bounds[3 + k] = hi[k];
+        }
+        b.scratch.ctl()[kCtlLeafCount] = n;
+    }
+    const float lo0 = lo[0];
+    const float lo1 = lo[1];
+    const float lo2 = lo[2];
+    const float e0 = fmaxf(hi[0] - lo0, 1e-6f);
+    const float e1 = fmaxf(hi[1] - lo1, 1e-6f);
+    const float e2 = fmaxf(hi[2] - lo2, 1e-6f);
     const float* centres = b.scratch.centres();
     uint32_t* keys = b.scratch.keys(0);
     uint32_t* values = b.scratch.values(0);
@@ -565,76 +544,93 @@ CX_DEV void StoreAggregate(const Build& b, uint32_t node, const Summary& s)
     }
 }
 
-// Climbs from a finished element with leaf range [a, last] (a leaf when a == last) inside the
-// block [block_lo, block_hi). Returns when the element arrives first at a gap, becomes the root, or
-// has its parent gap on the block boundary (then it is appended to the block's tops).
-CX_DEV void Climb(const Build& b, int n, uint32_t a, uint32_t last, uint64_t block_lo, uint64_t block_hi,
-                  uint2* tops, uint32_t* lds_top_count)
+// One climb step of a finished element with leaf range [a, last] (a leaf when a == last) inside the
+// block [block_lo, block_hi). Returns true when the element's parent is built (then [a, last] is the
+// parent's range and the caller climbs on); false when the element arrived first at its gap, is the
+// root, or has its parent gap on the block boundary (then it is appended to the block's tops).
+template <bool kLds>
+CX_DEV bool ClimbStep(const Build& b, int n, uint32_t& a, uint32_t& last, uint64_t block_lo, uint64_t block_hi,
+                      uint2* tops, uint32_t* lds_top_count, uint32_t* lds_slot, int32_t* lds_diff)
 {
     const uint32_t* keys = b.scratch.keys(0);
-    uint32_t* slot = b.scratch.u32(kLtScratchCounter);
-    uint32_t* left_out = b.scratch.u32(kLtScratchLeft);
-    uint32_t* right_out = b.scratch.u32(kLtScratchRight);
+    if (a == 0 && last == uint32_t(n - 1))
+    {
+        return false;  // the root (node 0) is finished
+    }
+    const int d_left = GapDelta(keys, int(a) - 1, n);
+    const int d_right = GapDelta(keys, int(last), n);
+    const bool is_left_child = d_right > d_left;
+    const uint32_t gap = is_left_child ? last : a - 1;
+    if (uint64_t(gap) < block_lo || uint64_t(gap) + 1 >= block_hi)
+    {
+        const uint32_t k = atomicAdd(lds_top_count, 1u);
+        if (k < kTopsCap)
+        {
+            tops[k] = make_uint2(a, last);
+        }
+        return false;
+    }
+    const uint32_t far_end = is_left_child ? a : last;
+    // Level 1 keeps the gap slots of its block in LDS (flushed to the global slots at the end of the
+    // launch); a higher level uses the global slots, where an earlier level may have parked a value.
+    uint32_t other;
+    if constexpr (kLds)
+    {
+        other = __hip_atomic_exchange(&lds_slot[gap - uint32_t(block_lo)], far_end, __ATOMIC_ACQ_REL,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    else
+    {
+        other = __hip_atomic_exchange(b.scratch.u32(kLtScratchCounter) + gap, far_end, __ATOMIC_ACQ_REL,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    if (other == kLtNone)
+    {
+        return false;  // the sibling finishes the parent
+    }
+    // The parent: range [pa, pb], split `gap`, node index by its own parent gap.
+    const uint32_t pa = is_left_child ? a : other;
+    const uint32_t pb = is_left_child ? other : last;
+    const uint32_t left = pa == gap ? (kLtChildLeafBit | gap) : gap;
+    const uint32_t right = pb == gap + 1 ? (kLtChildLeafBit | (gap + 1)) : gap + 1;
     uint32_t* parent_out = b.scratch.u32(kLtScratchParent);
     uint32_t* leaf_parent_out = b.scratch.u32(kLtScratchSortedLeafParent);
-    int32_t* diff = b.scratch.cover_diff();
-    uint32_t* split_out = b.scratch.split();
-    while (true)
+    uint32_t node;
+    if (pa == 0 && pb == uint32_t(n - 1))
     {
-        if (a == 0 && last == uint32_t(n - 1))
-        {
-            return;  // the root (node 0) is finished
-        }
-        const int d_left = GapDelta(keys, int(a) - 1, n);
-        const int d_right = GapDelta(keys, int(last), n);
-        const bool is_left_child = d_right > d_left;
-        const uint32_t gap = is_left_child ? last : a - 1;
-        if (uint64_t(gap) < block_lo || uint64_t(gap) + 1 >= block_hi)
-        {
-            const uint32_t k = atomicAdd(lds_top_count, 1u);
-            if (k < kTopsCap)
-            {
-                tops[k] = make_uint2(a, last);
-            }
-            return;
-        }
-        const uint32_t far_end = is_left_child ? a : last;
-        const uint32_t other = __hip_atomic_exchange(&slot[gap], far_end, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (other == kLtNone)
-        {
-            return;  // the sibling finishes the parent
-        }
-        // The parent: range [pa, pb], split `gap`, node index by its own parent gap.
-        const uint32_t pa = is_left_child ? a : other;
-        const uint32_t pb = is_left_child ? other : last;
-        const uint32_t left = pa == gap ? (kLtChildLeafBit | gap) : gap;
-        const uint32_t right = pb == gap + 1 ? (kLtChildLeafBit | (gap + 1)) : gap + 1;
-        uint32_t node;
-        if (pa == 0 && pb == uint32_t(n - 1))
-        {
-            node = 0;
-            parent_out[0] = kLtNone;
-        }
-        else
-        {
-            const bool parent_is_left = GapDelta(keys, int(pb), n) > GapDelta(keys, int(pa) - 1, n);
-            node = parent_is_left ? pb : pa;
-        }
-        left_out[node] = left;
-        right_out[node] = right;
-        ((left & kLtChildLeafBit) != 0 ? leaf_parent_out : parent_out)[left & ~kLtChildLeafBit] = node;
-        ((right & kLtChildLeafBit) != 0 ? leaf_parent_out : parent_out)[right & ~kLtChildLeafBit] = node;
-        split_out[node] = gap;
-        AtomicAddRelaxed(&diff[pa], 1);
-        AtomicAddRelaxed(&diff[pb], -1);
-        StoreAggregate(b, node, Union(ElementSummary(b, left), ElementSummary(b, right)));
-        a = pa;
-        last = pb;
+        node = 0;
+        parent_out[0] = kLtNone;
     }
+    else
+    {
+        const bool parent_is_left = GapDelta(keys, int(pb), n) > GapDelta(keys, int(pa) - 1, n);
+        node = parent_is_left ? pb : pa;
+    }
+    b.scratch.u32(kLtScratchLeft)[node] = left;
+    b.scratch.u32(kLtScratchRight)[node] = right;
+    ((left & kLtChildLeafBit) != 0 ? leaf_parent_out : parent_out)[left & ~kLtChildLeafBit] = node;
+    ((right & kLtChildLeafBit) != 0 ? leaf_parent_out : parent_out)[right & ~kLtChildLeafBit] = node;
+    b.scratch.split()[node] = gap;
+    // Range ends lie in this workgroup's block: no other workgroup of this launch touches them.
+    if constexpr (kLds)
+    {
+        atomicAdd(&lds_diff[pa - uint32_t(block_lo)], 1);
+        atomicAdd(&lds_diff[pb - uint32_t(block_lo)], -1);
+    }
+    else
+    {
+        int32_t* diff = b.scratch.cover_diff();
+        __hip_atomic_fetch_add(&diff[pa], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&diff[pb], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    StoreAggregate(b, node, Union(ElementSummary(b, left), ElementSummary(b, right)));
+    a = pa;
+    last = pb;
+    return true;
 }
 
 // Level 1: the workgroup's own 2048 sorted leaves. Level L > 1: the tops of its 32 sub-blocks.
-__global__ void __launch_bounds__(kTreeThreads) TreeLevelKernel(Build b, uint32_t level)
+__global__ void __launch_bounds__(1024) TreeLevelKernel(Build b, uint32_t level)
 {
     if (!Building(b))
     {
@@ -642,6 +638,8 @@ __global__ void __launch_bounds__(kTreeThreads) TreeLevelKernel(Build b, uint32_
     }
     __shared__ uint32_t lds_top_count;
     __shared__ uint32_t lds_offsets[kTreeFanout + 1];
+    __shared__ uint32_t lds_slot[kTreeTile];
+    __shared__ int32_t lds_diff[kTreeTile];
     const uint32_t t = threadIdx.x;
     const int n = int(b.scratch.ctl()[kCtlLeafCount]);
     const uint64_t tile = TreeTile(level);
@@ -657,12 +655,47 @@ __global__ void __launch_bounds__(kTreeThreads) TreeLevelKernel(Build b, uint32_
     {
         if (level == 1)
         {
-            for (uint32_t k = t; k < kTreeTile; k += kTreeThreads)
+            // Gap slots and gap-cover diffs of the block in LDS.
+            for (uint32_t k = t; k < kTreeTile; k += blockDim.x)
             {
-                const uint64_t j = block_lo + k;
-                if (j < uint64_t(n))
+                lds_slot[k] = kLtNone;
+                lds_diff[k] = 0;
+            }
+            __syncthreads();
+            // Each lane climbs one element at a time and takes its next leaf as soon as the climb
+            // ends, so a long climb holds up only its own lane.
+            uint32_t next = t;
+            uint32_t a = 0;
+            uint32_t last = 0;
+            bool live = false;
+            while (true)
+            {
+                if (!live)
                 {
-                    Climb(b, n, uint32_t(j), uint32_t(j), block_lo, block_hi, tops, &lds_top_count);
+                    const uint64_t j = block_lo + next;
+                    if (next >= kTreeTile || j >= uint64_t(n))
+                    {
+                        break;
+                    }
+                    a = uint32_t(j);
+                    last = a;
+                    next += blockDim.x;
+                }
+                live = ClimbStep<true>(b, n, a, last, block_lo, block_hi, tops, &lds_top_count, lds_slot, lds_diff);
+            }
+            __syncthreads();
+            uint32_t* slots = b.scratch.u32(kLtScratchCounter);
+            int32_t* diff = b.scratch.cover_diff();
+            for (uint32_t k = t; k < kTreeTile; k += blockDim.x)
+            {
+                const uint64_t g = block_lo + k;
+                if (g + 1 < uint64_t(n))
+                {
+                    slots[g] = lds_slot[k];
+                }
+                if (g < uint64_t(n))
+                {
+                    diff[g] = lds_diff[k];
                 }
             }
         }
@@ -685,15 +718,29 @@ __global__ void __launch_bounds__(kTreeThreads) TreeLevelKernel(Build b, uint32_
             __syncthreads();
             const uint2* child_tops = b.scratch.tops(level - 1);
             const uint32_t total = lds_offsets[kTreeFanout];
-            for (uint32_t k = t; k < total; k += kTreeThreads)
+            uint32_t next = t;
+            uint32_t a = 0;
+            uint32_t last = 0;
+            bool live = false;
+            while (true)
             {
-                uint32_t c = 0;
-                while (lds_offsets[c + 1] <= k)
+                if (!live)
                 {
-                    ++c;
+                    if (next >= total)
+                    {
+                        break;
+                    }
+                    uint32_t c = 0;
+                    while (lds_offsets[c + 1] <= next)
+                    {
+                        ++c;
+                    }
+                    const uint2 e = child_tops[uint64_t(child_first + c) * kTopsCap + (next - lds_offsets[c])];
+                    a = e.x;
+                    last = e.y;
+                    next += blockDim.x;
                 }
-                const uint2 e = child_tops[uint64_t(child_first + c) * kTopsCap + (k - lds_offsets[c])];
-                Climb(b, n, e.x, e.y, block_lo, block_hi, tops, &lds_top_count);
+                live = ClimbStep<false>(b, n, a, last, block_lo, block_hi, tops, &lds_top_count, nullptr, nullptr);
             }
         }
     }
@@ -1106,13 +1153,14 @@ CX_API int cx_light_bvh_build(const CxLightBvhBuild* build, void* stream)
     const uint32_t capacity = build->leaf_capacity;
     const uint32_t per_item_blocks = cx::DivUp(capacity, cx::kBvhThreads);
     const uint32_t tiles = cx::SortTiles(capacity);
+    const uint32_t bound_blocks = std::min(tiles, cx::kBoundsMaxBlocks);
     {
         cx::StageScope scope(s, "lt_bounds");
-        cx::BoundsKernel<<<std::min(tiles, cx::kBoundsMaxBlocks), cx::kBvhThreads, 0, s>>>(b);
+        cx::BoundsKernel<<<bound_blocks, cx::kBvhThreads, 0, s>>>(b);
     }
     {
         cx::StageScope scope(s, "lt_morton");
-        cx::MortonKernel<<<tiles, cx::kSortThreads, 0, s>>>(b);
+        cx::MortonKernel<<<tiles, cx::kSortThreads, 0, s>>>(b, bound_blocks);
     }
     const cx::SortState sort = b.scratch.sort();
     const uint32_t* gate = &build->state->building;
@@ -1123,10 +1171,13 @@ CX_API int cx_light_bvh_build(const CxLightBvhBuild* build, void* stream)
         cx::LaunchSortPass(b.scratch.keys(in), b.scratch.values(in), b.scratch.keys(1 - in), b.scratch.values(1 - in),
                            count, gate, sort, pass, s);
     }
+    static const char* const kTreeStage[] = {"lt_tree_l1", "lt_tree_l2", "lt_tree_l3", "lt_tree_l4", "lt_tree_l5",
+                                             "lt_tree_l6"};
     for (uint32_t level = 1; level <= cx::TreeLevels(capacity); ++level)
     {
-        cx::StageScope scope(s, "lt_tree");
-        cx::TreeLevelKernel<<<cx::TreeBlocks(capacity, level), cx::kTreeThreads, 0, s>>>(b, level);
+        cx::StageScope scope(s, kTreeStage[std::min(level, 6u) - 1]);
+        const uint32_t threads = level == 1 ? cx::kTreeThreads : 1024u;
+        cx::TreeLevelKernel<<<cx::TreeBlocks(capacity, level), threads, 0, s>>>(b, level);
     }
     const uint32_t scan_tiles = b.scratch.scan_tiles();
     {

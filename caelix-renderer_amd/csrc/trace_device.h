@@ -1,0 +1,544 @@
+// Device code of the trace for gfx950: the brick DDA (the reference's
+// shaders/render/caelix_brick_trace.hlsl:203-451), the world traversal that finds bricks through the
+// hash table, the light tree's sample and pdf walks (shaders/light/caelix_light_tree.hlsl), the MIS
+// weight (shaders/render/caelix_light_nee.hlsl), the random streams (shaders/render/caelix_random.hlsl)
+// and the opaque-diffuse subset of the path tracer (shaders/render/caelix_path_trace.hlsl:276-452,
+// 737-978).
+//
+// Floating point: IEEE single operations in source order (no contraction; __fdiv_rn / __fsqrt_rn),
+// so the CPU restatement reproduces every decision. sin/cos are the Cephes single-precision
+// polynomials (S. Moshier, Cephes Math Library, sinf.c / cosf.c) with Cody-Waite reduction, not the
+// device library's.
+#pragma once
+#include "device_common.h"
+#include "hash_device.h"
+
+namespace cx {
+
+// ---- scene -----------------------------------------------------------------------------------------
+struct SceneView
+{
+    const int32_t* keys;        // [bricks][3] brick coordinates
+    const uint2* info;          // [bricks] record words 0 and 1
+    const uint64_t* occupancy;  // [bricks][8] micro occupancy of the coarse cells
+    const uint16_t* payload;    // [bricks][512] block values, voxel x + 8y + 64z
+    const HashKeySlot* slots;
+    uint32_t slot_count;
+    uint32_t pow2;
+    const uint32_t* hash_info;  // [0] probe limit
+    const uint32_t* brick_bits;  // grid_dim^3 bits: brick cell resident
+    const uint32_t* super_bits;  // (grid_dim / 8)^3 bits: any resident brick in the 8^3 super cell
+    int32_t grid_min[3];         // brick coordinate of grid cell 0
+    uint32_t grid_dim;           // bricks per axis, a multiple of 8
+    const float4* materials;     // 2 per 16-bit block value: (albedo, metallic), (emission, smoothness)
+    const uint32_t* light_codes; // 65536 bits: the block value is a light material
+    float emission_scale;
+    uint32_t encoding;
+};
+
+struct LightView
+{
+    const LightTreeHeader* headers;  // [2]
+    const LightBvhNode* nodes[2];
+    const LightBvhLeaf* leaves[2];
+    const uint32_t* heap;             // brick light records, 16-byte units
+    const uint32_t* record_offset;    // light ordinal -> record offset (units)
+    const uint32_t* ordinal_of_brick; // brick record -> light ordinal or kLtNone
+    uint32_t group_shift;             // a group is (1 << group_shift)^3 bricks
+    uint32_t mis;
+};
+
+struct CameraView
+{
+    float position[3];
+    float forward[3];
+    float right[3];
+    float up[3];
+    float tan_half_fov_y;
+    float aspect;
+    uint32_t width;
+    uint32_t height;
+};
+
+constexpr float kPi = 3.1415926535f;
+constexpr float kTwoPi = 6.283185307f;
+constexpr float kRayTMax = 1e30f;
+constexpr float kPushOff = 0.002f;
+constexpr float kNearField = 8.0f;
+constexpr uint32_t kBrickMaxSteps = 22;
+
+CX_DEV float Div(float a, float b) { return __fdiv_rn(a, b); }
+
+// Correctly rounded single-precision square root. The hardware v_sqrt_f32 is within 1 ulp (and
+// __fsqrt_rn lowers to it alone on gfx950); the exact answer is s or one of its neighbours, and the
+// signs of the fma residuals x - s_down * s and x - s_up * s say which: if x <= s_down * s the true
+// root lies below the midpoint of s_down and s, if x > s_up * s above the midpoint of s and s_up.
+// Inputs below 2^-96 are scaled by 2^32 first (the root by 2^-16 after) so that the residuals stay
+// normal. Zero, infinity and NaN pass through (their residuals are NaN or not of the tested sign).
+CX_DEV float Sqrt(float a)
+{
+    const bool tiny = a < 0x1p-96f;
+    const float x = tiny ? a * 0x1p+32f : a;
+    float s = __builtin_amdgcn_sqrtf(x);
+    const float down = __uint_as_float(__float_as_uint(s) - 1u);
+    const float up = __uint_as_float(__float_as_uint(s) + 1u);
+    const float r_down = __builtin_fmaf(-down, s, x);
+    const float r_up = __builtin_fmaf(-up, s, x);
+    s = r_down <= 0.0f ? down : s;
+    s = r_up > 0.0f ? up : s;
+    return tiny ? s * 0x1p-16f : s;
+}
+
+struct V3
+{
+    float x, y, z;
+};
+CX_DEV V3 Make(float x, float y, float z) { return V3{x, y, z}; }
+CX_DEV V3 Add(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+CX_DEV V3 Sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+CX_DEV V3 Mul(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
+CX_DEV V3 MulV(V3 a, V3 b) { return V3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+CX_DEV float Dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+CX_DEV float Get(V3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+CX_DEV void Set(V3& v, int i, float s)
+{
+    if (i == 0)
+    {
+        v.x = s;
+    }
+    else if (i == 1)
+    {
+        v.y = s;
+    }
+    else
+    {
+        v.z = s;
+    }
+}
+// v * (1 / sqrt(max(FLT_MIN, dot(v, v)))) (SafeNormalize, caelix_shading.hlsl:110-114, with an IEEE
+// reciprocal square root).
+CX_DEV V3 SafeNormalize(V3 v)
+{
+    const float dp3 = fmaxf(1.175494351e-38f, Dot(v, v));
+    return Mul(v, Div(1.0f, Sqrt(dp3)));
+}
+
+// ---- Cephes sinf / cosf ------------------------------------------------------------------------
+CX_DEV float PolySin(float x, float z)
+{
+    return ((-1.9515295891e-4f * z + 8.3321608736e-3f) * z - 1.6666654611e-1f) * z * x + x;
+}
+CX_DEV float PolyCos(float z)
+{
+    return ((2.443315711809948e-5f * z - 1.388731625493765e-3f) * z + 4.166664568298827e-2f) * z * z - 0.5f * z +
+           1.0f;
+}
+// sin and cos of a >= 0 (a < 2^24).
+CX_DEV void SinCos(float a, float& s, float& c)
+{
+    int j = int(a * 1.27323954473516f);  // 4 / pi
+    float y = float(j);
+    if ((j & 1) != 0)
+    {
+        j += 1;
+        y += 1.0f;
+    }
+    j &= 7;
+    const float x = ((a - y * 0.78515625f) - y * 2.4187564849853515625e-4f) - y * 3.77489497744594108e-8f;
+    const float z = x * x;
+    const float ps = PolySin(x, z);
+    const float pc = PolyCos(z);
+    // sin
+    int js = j;
+    float sign_s = 1.0f;
+    if (js > 3)
+    {
+        sign_s = -1.0f;
+        js -= 4;
+    }
+    s = (js == 1 || js == 2) ? pc : ps;
+    s = sign_s < 0.0f ? -s : s;
+    // cos
+    int jc = j;
+    float sign_c = 1.0f;
+    if (jc > 3)
+    {
+        jc -= 4;
+        sign_c = -sign_c;
+    }
+    if (jc > 1)
+    {
+        sign_c = -sign_c;
+    }
+    c = (jc == 1 || jc == 2) ? ps : pc;
+    c = sign_c < 0.0f ? -c : c;
+}
+
+// ---- random streams (caelix_random.hlsl) ---------------------------------------------------------
+CX_DEV uint32_t PcgHash(uint32_t value)
+{
+    const uint32_t state = value * 747796405u + 2891336453u;
+    const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+CX_DEV uint32_t InitRandom(uint32_t px, uint32_t py, uint32_t seed)
+{
+    const uint32_t state = PcgHash(px ^ (py << 16) ^ (seed * 0x9E3779B9u));
+    return state != 0u ? state : 0x9E3779B9u;
+}
+CX_DEV uint32_t RandomU32(uint32_t& state)
+{
+    state ^= state << 13;
+    state ^= state >> 17;
+    state ^= state << 5;
+    return state;
+}
+CX_DEV float RandomFloat01(uint32_t& state)
+{
+    return float(RandomU32(state) >> 8) * (1.0f / 16777216.0f);
+}
+CX_DEV V3 RandomUnitVector(uint32_t& state)
+{
+    const float u = RandomFloat01(state);
+    const float v = RandomFloat01(state);
+    const float a = u * kTwoPi;
+    const float y = v * 2.0f - 1.0f;
+    const float sy = Sqrt(fmaxf(1.0f - y * y, 0.0f));
+    float s, c;
+    SinCos(a, s, c);
+    return Make(s * sy, y, c * sy);
+}
+// The light tree's PCG stream (caelix_light_tree.hlsl:65-76).
+CX_DEV uint32_t LightRandom(uint32_t& rng)
+{
+    rng = rng * 747796405u + 2891336453u;
+    const uint32_t word = ((rng >> ((rng >> 28) + 4u)) ^ rng) * 277803737u;
+    return (word >> 22) ^ word;
+}
+CX_DEV float LightRandomFloat(uint32_t& rng)
+{
+    return float(LightRandom(rng) >> 8) * (1.0f / 16777216.0f);
+}
+
+// ---- brick DDA (caelix_brick_trace.hlsl) --------------------------------------------------------
+struct BrickHit
+{
+    float t;
+    uint32_t attrib;  // block << 16 | cell << 6 | face flags; 0 = miss
+    uint32_t steps;   // DDA steps taken
+};
+
+CX_DEV float SafeInvAxis(float d)
+{
+    if (fabsf(d) >= 1e-20f)
+    {
+        return Div(1.0f, d);
+    }
+    return d < 0.0f ? Div(-1.0f, 1e-20f) : Div(1.0f, 1e-20f);
+}
+
+CX_DEV uint32_t ReadBlock(const uint16_t* payload, int x, int y, int z)
+{
+    return payload[x + 8 * y + 64 * z];
+}
+
+CX_DEV bool IsOpaque(uint32_t block, uint32_t encoding)
+{
+    return (block & (encoding == kEncodingPacked ? kPackedOpaqueMask : kBlockIdOpaqueMask)) != 0;
+}
+CX_DEV uint32_t FaceBits(uint32_t block, uint32_t encoding)
+{
+    return encoding == kEncodingPacked ? ((block >> kPackedFaceShift) & 0x3Fu) : ((block >> kBlockIdFaceShift) & 0x3Fu);
+}
+
+// CaelixTraceBrickRay (:299-368).
+CX_DEV BrickHit TraceBrickRay(const uint64_t* occupancy, const uint16_t* payload, uint32_t encoding, V3 entry,
+                              V3 dir, float entry_t, uint32_t entry_normal_coarse)
+{
+    const V3 inv = Make(SafeInvAxis(dir.x), SafeInvAxis(dir.y), SafeInvAxis(dir.z));
+    const V3 boundary = Make(dir.x >= 0.0f ? 1.0f : 0.0f, dir.y >= 0.0f ? 1.0f : 0.0f, dir.z >= 0.0f ? 1.0f : 0.0f);
+    const V3 t_start = MulV(Sub(boundary, entry), inv);
+    const float hi = 8.0f - 0.0001f;
+    const V3 origin = Make(fminf(fmaxf(entry.x, 0.0f), hi), fminf(fmaxf(entry.y, 0.0f), hi), fminf(fmaxf(entry.z, 0.0f), hi));
+    int cx_ = int(floorf(origin.x));
+    int cy = int(floorf(origin.y));
+    int cz = int(floorf(origin.z));
+    int px = cx_, py = cy, pz = cz;
+    BrickHit hit{0.0f, 0u, 0u};
+    for (uint32_t step = 0; step < kBrickMaxSteps; ++step)
+    {
+        hit.steps = step + 1;
+        if (uint32_t(cx_) >= 8u || uint32_t(cy) >= 8u || uint32_t(cz) >= 8u)
+        {
+            break;
+        }
+        const uint32_t coarse_bit = uint32_t((cx_ >> 2) | ((cy >> 2) << 1) | ((cz >> 2) << 2));
+        bool advance_from_jump = false;
+        if ((entry_normal_coarse & (1u << coarse_bit)) == 0u)
+        {
+            // VRTJump: the far corner of the empty coarse cell, then one step.
+            cx_ = dir.x < 0.0f ? (cx_ & ~3) : (cx_ | 3);
+            cy = dir.y < 0.0f ? (cy & ~3) : (cy | 3);
+            cz = dir.z < 0.0f ? (cz & ~3) : (cz | 3);
+            advance_from_jump = true;
+        }
+        else
+        {
+            const uint64_t occ = occupancy[coarse_bit];
+            const uint32_t micro = uint32_t((cx_ & 3) | ((cy & 3) << 2) | ((cz & 3) << 4));
+            if (((occ >> micro) & 1ull) != 0ull)
+            {
+                const uint32_t block = ReadBlock(payload, cx_, cy, cz);
+                float T = 0.0f;
+                uint32_t flags;
+                if (step == 0u)
+                {
+                    flags = entry_normal_coarse >> 26;
+                }
+                else
+                {
+                    // CaelixBrickRayNormalFlags from the previous cell.
+                    const V3 side = Add(t_start, MulV(Make(float(px), float(py), float(pz)), inv));
+                    T = fminf(fminf(side.x, side.y), side.z);
+                    const float lim = T + 0.0001f;
+                    const uint32_t x_use = side.x <= lim ? 0xFFFFFFFFu : 0u;
+                    const uint32_t y_use = (side.y <= lim ? 0xFFFFFFFFu : 0u) & ~x_use;
+                    const uint32_t z_use = ~(x_use | y_use);
+                    const uint32_t xf = dir.x > 0.0f ? 0x02u : 0x01u;
+                    const uint32_t yf = dir.y > 0.0f ? 0x08u : 0x04u;
+                    const uint32_t zf = dir.z > 0.0f ? 0x20u : 0x10u;
+                    flags = (xf & x_use) | (yf & y_use) | (zf & z_use);
+                }
+                bool terminate = IsOpaque(block, encoding);
+                if (!terminate && !(step == 0u && entry_t == 0.0f))
+                {
+                    terminate = (FaceBits(block, encoding) & flags) != 0u;
+                }
+                if (terminate)
+                {
+                    hit.attrib = (block << 16) | (uint32_t(cx_ | (cy << 3) | (cz << 6)) << 6) | flags;
+                    hit.t = entry_t + T;
+                    return hit;
+                }
+            }
+        }
+        if (!advance_from_jump)
+        {
+            px = cx_;
+            py = cy;
+            pz = cz;
+        }
+        else
+        {
+            px = cx_;
+            py = cy;
+            pz = cz;
+        }
+        // VRTStep
+        const V3 side = Add(t_start, MulV(Make(float(cx_), float(cy), float(cz)), inv));
+        const float tmin = fminf(fminf(side.x, side.y), side.z) + 0.0001f;
+        const V3 pos = Add(origin, Mul(dir, tmin));
+        cx_ = int(floorf(pos.x));
+        cy = int(floorf(pos.y));
+        cz = int(floorf(pos.z));
+    }
+    hit.attrib = 0u;
+    hit.t = 0.0f;
+    return hit;
+}
+
+// CaelixTraceBrickPrimitiveCore (:376-451): slab test against the tight bounds, entry face, DDA.
+CX_DEV BrickHit TraceBrick(const SceneView& s, uint32_t record, V3 o, V3 dir, float t_current, V3 brick_origin)
+{
+    const uint2 info = s.info[record];
+    const uint32_t tb = info.y;
+    const V3 tmin_box = Make(float(tb & 7u), float((tb >> 3) & 7u), float((tb >> 6) & 7u));
+    const V3 tmax_box = Make(float((tb >> 9) & 7u) + 1.0f, float((tb >> 12) & 7u) + 1.0f, float((tb >> 15) & 7u) + 1.0f);
+    const V3 aabb_min = Add(brick_origin, tmin_box);
+    const V3 aabb_max = Add(brick_origin, tmax_box);
+    const V3 inv = Make(Div(1.0f, dir.x), Div(1.0f, dir.y), Div(1.0f, dir.z));
+    const V3 t0 = MulV(Sub(aabb_min, o), inv);
+    const V3 t1 = MulV(Sub(aabb_max, o), inv);
+    const V3 tn = Make(fminf(t0.x, t1.x), fminf(t0.y, t1.y), fminf(t0.z, t1.z));
+    const V3 tf = Make(fmaxf(t0.x, t1.x), fmaxf(t0.y, t1.y), fmaxf(t0.z, t1.z));
+    const float largest_tmin = fmaxf(fmaxf(tn.x, tn.y), tn.z);
+    const float smallest_tmax = fminf(fminf(tf.x, tf.y), tf.z);
+    BrickHit miss{0.0f, 0u, 0u};
+    if (largest_tmin > smallest_tmax || smallest_tmax < 0.0f || largest_tmin > t_current)
+    {
+        return miss;
+    }
+    const float t = fmaxf(0.0f, largest_tmin);
+    const V3 entry = Sub(Add(o, Mul(dir, t)), brick_origin);
+    uint32_t flags;
+    if (largest_tmin == tn.x)
+    {
+        flags = dir.x > 0.0f ? 0x02u : 0x01u;
+    }
+    else if (largest_tmin == tn.y)
+    {
+        flags = dir.y > 0.0f ? 0x08u : 0x04u;
+    }
+    else
+    {
+        flags = dir.z > 0.0f ? 0x20u : 0x10u;
+    }
+    const uint32_t coarse = (info.x >> kRecordCoarseShift) & kRecordCoarseMask;
+    return TraceBrickRay(s.occupancy + uint64_t(record) * 8, s.payload + uint64_t(record) * 512, s.encoding, entry,
+                         dir, t, (flags << 26) | coarse);
+}
+
+// ---- world traversal ------------------------------------------------------------------------------
+struct WorldHit
+{
+    float t;
+    uint32_t attrib;
+    uint32_t record;   // brick record of the hit, kHashNone for a miss
+    int32_t brick[3];  // brick coordinate of the hit
+    uint32_t cells;    // brick cells visited
+    uint32_t probes;   // hash slots read
+    uint32_t steps;    // brick DDA steps
+};
+
+CX_DEV bool BitSet(const uint32_t* bits, uint64_t index)
+{
+    return ((bits[index >> 5] >> (index & 31u)) & 1u) != 0u;
+}
+
+// The ray [o, o + t d], t in [0, t_max], through the brick grid: a DDA over the 8^3-brick super
+// cells and, inside a resident super cell, over its bricks; a resident brick is found through the
+// hash table and traced with the brick DDA. The first hit is the closest (bricks are disjoint and
+// visited in ray order). `any_hit`: a shadow ray (any hit with t < t_max).
+CX_DEV WorldHit TraceWorld(const SceneView& s, V3 o, V3 dir, float t_max)
+{
+    WorldHit result{0.0f, 0u, kHashNone, {0, 0, 0}, 0u, 0u, 0u};
+    const float g0[3] = {float(s.grid_min[0] * 8), float(s.grid_min[1] * 8), float(s.grid_min[2] * 8)};
+    const float extent = float(s.grid_dim * 8);
+    const float ov[3] = {o.x, o.y, o.z};
+    const float dv[3] = {dir.x, dir.y, dir.z};
+    float inv[3];
+    int step[3];
+    float t_enter = 0.0f;
+    float t_exit = t_max;
+    for (int a = 0; a < 3; ++a)
+    {
+        inv[a] = SafeInvAxis(dv[a]);
+        step[a] = dv[a] >= 0.0f ? 1 : -1;
+        const float t0 = (g0[a] - ov[a]) * inv[a];
+        const float t1 = (g0[a] + extent - ov[a]) * inv[a];
+        t_enter = fmaxf(t_enter, fminf(t0, t1));
+        t_exit = fminf(t_exit, fmaxf(t0, t1));
+    }
+    if (t_enter > t_exit)
+    {
+        return result;
+    }
+    const int super_dim = int(s.grid_dim >> 3);
+    // Super cell of the entry point.
+    int sc[3];
+    for (int a = 0; a < 3; ++a)
+    {
+        const float p = ov[a] + dv[a] * t_enter;
+        int c = int(floorf((p - g0[a]) * (1.0f / 64.0f)));
+        sc[a] = min(max(c, 0), super_dim - 1);
+    }
+    float t_cell = t_enter;
+    while (true)
+    {
+        // Exit of the current super cell (plane crossings (k * 64 + g0 - o) * inv).
+        float t_next[3];
+        for (int a = 0; a < 3; ++a)
+        {
+            const int k = sc[a] + (step[a] > 0 ? 1 : 0);
+            t_next[a] = (float(k * 64) + g0[a] - ov[a]) * inv[a];
+        }
+        const float t_leave = fminf(fminf(t_next[0], t_next[1]), t_next[2]);
+        const uint64_t super_index = (uint64_t(sc[2]) * super_dim + sc[1]) * super_dim + sc[0];
+        if (BitSet(s.super_bits, super_index))
+        {
+            // The bricks of this super cell along the ray, from the entry t_cell.
+            int bc[3];
+            for (int a = 0; a < 3; ++a)
+            {
+                const float p = ov[a] + dv[a] * t_cell;
+                int c = int(floorf((p - g0[a]) * 0.125f));
+                bc[a] = min(max(c, sc[a] * 8), sc[a] * 8 + 7);
+            }
+            while (true)
+            {
+                result.cells += 1;
+                const uint64_t brick_index = (uint64_t(bc[2]) * s.grid_dim + bc[1]) * s.grid_dim + bc[0];
+                if (BitSet(s.brick_bits, brick_index))
+                {
+                    const int32_t key[3] = {s.grid_min[0] + bc[0], s.grid_min[1] + bc[1], s.grid_min[2] + bc[2]};
+                    uint32_t probes = 0;
+                    const uint32_t record = HashGet(s.slots, s.slot_count, s.pow2, s.hash_info[0],
+                                                    BrickFingerprint(key[0], key[1], key[2]), &probes);
+                    result.probes += probes;
+                    if (record != kHashNone)
+                    {
+                        const V3 origin = Make(float(key[0] * 8), float(key[1] * 8), float(key[2] * 8));
+                        const BrickHit h = TraceBrick(s, record, o, dir, t_max, origin);
+                        result.steps += h.steps;
+                        if (h.attrib != 0u && h.t < t_max)
+                        {
+                            result.t = h.t;
+                            result.attrib = h.attrib;
+                            result.record = record;
+                            result.brick[0] = key[0];
+                            result.brick[1] = key[1];
+                            result.brick[2] = key[2];
+                            return result;
+                        }
+                    }
+                }
+                // Next brick cell.
+                float tb[3];
+                for (int a = 0; a < 3; ++a)
+                {
+                    const int k = bc[a] + (step[a] > 0 ? 1 : 0);
+                    tb[a] = (float(k * 8) + g0[a] - ov[a]) * inv[a];
+                }
+                int axis = 0;
+                if (tb[1] < tb[axis])
+                {
+                    axis = 1;
+                }
+                if (tb[2] < tb[axis])
+                {
+                    axis = 2;
+                }
+                if (tb[axis] > t_max)
+                {
+                    return result;
+                }
+                bc[axis] += step[axis];
+                if (bc[axis] < sc[axis] * 8 || bc[axis] > sc[axis] * 8 + 7)
+                {
+                    break;  // left the super cell
+                }
+            }
+        }
+        int axis = 0;
+        if (t_next[1] < t_next[axis])
+        {
+            axis = 1;
+        }
+        if (t_next[2] < t_next[axis])
+        {
+            axis = 2;
+        }
+        if (t_leave > t_exit)
+        {
+            return result;
+        }
+        sc[axis] += step[axis];
+        if (sc[axis] < 0 || sc[axis] >= super_dim)
+        {
+            return result;
+        }
+        t_cell = t_next[axis];
+    }
+}
+
+}  // namespace cx

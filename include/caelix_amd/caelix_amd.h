@@ -146,6 +146,122 @@ CX_API int cx_hash_get_bricks(const HashKeySlot* slots, uint32_t slot_count, con
                               void* stream);
 
 /* ------------------------------------------------------------------------------------------
+ * Scene: resident bricks in the split record format of layout.h (info words, micro occupancy,
+ * payload; record index = the brick's index in `keys`), found through a bulk-built hash table, and
+ * bitmaps of resident bricks / 8^3-brick super cells over a grid of grid_dim^3 bricks from
+ * grid_min. One entity at the identity frame; groups of (1 << group_shift)^3 bricks. Materials: 2
+ * float4 per 16-bit block value, (albedo rgb, metallic) and (emission rgb, smoothness) (the
+ * reference's FP16 hot entry, caelix_material.hlsl:90-113, as float); light_codes: 65536 bits, the
+ * block values whose emission luminance reaches the light threshold.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct CxScene {
+    const int32_t* keys;      /* [bricks][3] */
+    uint32_t* info;           /* [bricks][2] record words 0 and 1 (written by cx_scene_finalize) */
+    void* occupancy;          /* [bricks][8] uint64 micro occupancy (written by cx_scene_finalize) */
+    const uint16_t* payload;  /* [bricks][512] block values, voxel x + 8y + 64z */
+    const HashKeySlot* slots; /* cx_hash_build_bricks over keys */
+    uint32_t slot_count;
+    uint32_t bricks;
+    const uint32_t* hash_info; /* cx_hash_build_bricks info (probe limit) */
+    uint32_t* brick_bits;      /* grid_dim^3 bits */
+    uint32_t* super_bits;      /* (grid_dim / 8)^3 bits */
+    int32_t grid_min[3];
+    uint32_t grid_dim;         /* a multiple of 8 */
+    uint32_t group_shift;
+    uint32_t encoding;         /* kEncodingPacked or kEncodingBlockId */
+    const float* materials;    /* [65536][8] */
+    const uint32_t* light_codes;
+    float emission_scale;
+    uint32_t pad;
+} CxScene;
+
+/* The record producer's opaque subset (shaders/scene/caelix_finalize.hlsl:112-361): word 0 (group
+ * position, coarse byte), word 1 bounds, micro occupancy; and the resident bitmaps. */
+CX_API int cx_scene_finalize(const CxScene* scene, void* stream);
+
+/* The light pass (the group pass of shaders/light/records/*.hlsl for one entity, plus the gather of
+ * caelix_light_gather_emit.hlsl): light face counts into word 1, brick light records in the heap
+ * (BrickLightRecordHeader + entries), one LightBvhLeaf per light brick (ordinal order) with
+ * group = brick record, entry = light ordinal; state->leaves_found = light bricks. totals[0] light
+ * bricks, [1] heap units needed, [2] light faces, [3] 1 when the heap was too small (grow to
+ * totals[1] units and run again, as the reference grows its heap, light_tree_record.cpp:37-70). */
+typedef struct CxLightScene {
+    uint32_t* heap;
+    uint32_t heap_units;
+    uint32_t leaf_capacity;
+    uint32_t* record_offset;    /* [bricks] by light ordinal */
+    uint32_t* ordinal_of_brick; /* [bricks] */
+    LightBvhLeaf* leaves;       /* [leaf_capacity] */
+    LightTreeState* state;      /* nullable */
+    uint32_t* totals;           /* 4 u32 */
+    void* scratch;              /* cx_scene_light_scratch_bytes(bricks) */
+} CxLightScene;
+CX_API uint64_t cx_scene_light_scratch_bytes(uint32_t bricks);
+CX_API int cx_scene_light_records(const CxScene* scene, const CxLightScene* light, void* stream);
+
+/* The light tree as the trace reads it: two tree headers (tree 1 may be empty, root kLtNone), nodes
+ * and leaves per tree, and the brick light records. mis: balance-heuristic MIS on both sides. */
+typedef struct CxLightTreeView {
+    const LightTreeHeader* headers; /* [2] */
+    const LightBvhNode* nodes[2];
+    const LightBvhLeaf* leaves[2];
+    const uint32_t* heap;
+    const uint32_t* record_offset;
+    const uint32_t* ordinal_of_brick;
+    uint32_t group_shift;
+    uint32_t mis;
+} CxLightTreeView;
+
+/* Pinhole camera (caelix_frame.hlsl:164-171): unit forward/right/up. */
+typedef struct CxCamera {
+    float position[3];
+    float forward[3];
+    float right[3];
+    float up[3];
+    float tan_half_fov_y;
+    float aspect;
+    uint32_t width;
+    uint32_t height;
+} CxCamera;
+
+typedef struct CxTraceParams {
+    uint32_t bounces;    /* bounce_count */
+    uint32_t frame_seed;
+    uint32_t spp;        /* samples per pixel; sample s uses frame_seed + s */
+    uint32_t pad;
+} CxTraceParams;
+
+/* Per-pixel outputs (nullable except counters): color = emission + albedo * radiance (rgba, a =
+ * primary hit t), radiance = TracePath's demodulated radiance, accumulated over the samples;
+ * counters: [0] rays (primary + bounce), [1] shadow rays, [2] brick cells visited, [3] hash probes,
+ * [4] brick DDA steps (u64 each). */
+typedef struct CxTraceOutput {
+    float* color;
+    float* radiance;
+    uint32_t* attrib; /* primary hit attribute (0 = miss) */
+    uint64_t* counters;
+} CxTraceOutput;
+
+/* One frame: primary ray, brick DDA, light NEE + MIS, `bounces` bounces, for every pixel
+ * (shaders/render/caelix_path_trace.hlsl:1050-, TracePath :737-978), opaque Lambertian subset. */
+CX_API int cx_trace_frame(const CxScene* scene, const CxLightTreeView* lights, const CxCamera* camera,
+                          const CxTraceParams* params, const CxTraceOutput* out, void* stream);
+
+/* Rays through the world (TraceScene / TraceShadow semantics): rays[i] = origin(3), direction(3),
+ * t_max; hits[i] = {t (float bits), attrib, record, brick x, y, z, cells, probes, steps}. */
+CX_API int cx_trace_rays(const CxScene* scene, const float* rays, uint32_t count, uint32_t* hits, void* stream);
+
+/* LightTreeSample (caelix_light_tree.hlsl:749-767) for receivers[i] = x(3), n(3) with PCG state
+ * seeds[i]; samples[i] = {position(3), normal(3), pdf, block, face} (floats; block/face as bits);
+ * ok[i] = 1 when a point was drawn. */
+CX_API int cx_light_sample(const CxScene* scene, const CxLightTreeView* lights, const float* receivers,
+                           const uint32_t* seeds, uint32_t count, float* samples, uint32_t* ok, void* stream);
+/* The area pdf of a light face for a receiver (LightTreeFindLightFace x LightTreeBrickPdf):
+ * queries[i] = {x(3), n(3) as floats; record, voxel x, y, z, face, block as u32}; pdf[i]. */
+CX_API int cx_light_face_pdf(const CxScene* scene, const CxLightTreeView* lights, const uint32_t* queries,
+                             uint32_t count, float* pdf, void* stream);
+
+/* ------------------------------------------------------------------------------------------
  * Stage timing: when enabled, entry points bracket every kernel with HIP events on the stream it
  * runs on. cx_profile_read synchronises those events and returns per-stage sums (milliseconds)
  * and launch counts since the last reset; names are static strings.

@@ -117,3 +117,209 @@ def hash_get_brick(slots: np.ndarray, probe_limit: int, x: int, y: int, z: int) 
     s = np.ascontiguousarray(slots, dtype=np.uint32)
     payload = load().oracle_hash_get_brick(_ptr(s), s.shape[0], probe_limit, x, y, z, ctypes.byref(probes))
     return int(payload), int(probes.value)
+
+
+# ------------------------------------------------------------------------------------------------
+# Trace side: scenes, DDA, light sampling, path tracer (oracle/trace_oracle.cpp).
+
+class _Scene(ctypes.Structure):
+    _fields_ = [
+        ("keys", _P), ("info", _P), ("occupancy", _P), ("payload", _P), ("slots", _P), ("slot_count", _U32),
+        ("bricks", _U32), ("hash_info", _P), ("brick_bits", _P), ("super_bits", _P),
+        ("grid_min", ctypes.c_int32 * 3), ("grid_dim", _U32), ("group_shift", _U32), ("encoding", _U32),
+        ("materials", _P), ("light_codes", _P), ("emission_scale", ctypes.c_float), ("pad", _U32),
+    ]
+
+
+class _TreeView(ctypes.Structure):
+    _fields_ = [
+        ("headers", _P), ("nodes", _P * 2), ("leaves", _P * 2), ("heap", _P), ("record_offset", _P),
+        ("ordinal_of_brick", _P), ("group_shift", _U32), ("mis", _U32),
+    ]
+
+
+class _Camera(ctypes.Structure):
+    _fields_ = [
+        ("position", ctypes.c_float * 3), ("forward", ctypes.c_float * 3), ("right", ctypes.c_float * 3),
+        ("up", ctypes.c_float * 3), ("tan_half_fov_y", ctypes.c_float), ("aspect", ctypes.c_float),
+        ("width", _U32), ("height", _U32),
+    ]
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("bounces", _U32), ("frame_seed", _U32), ("spp", _U32), ("pad", _U32)]
+
+
+def _trace_lib():
+    lib = load()
+    if not hasattr(lib, "_trace_bound"):
+        S, T = ctypes.POINTER(_Scene), ctypes.POINTER(_TreeView)
+        lib.oracle_scene_finalize.restype = None
+        lib.oracle_scene_finalize.argtypes = [S]
+        lib.oracle_scene_light_records.restype = _U32
+        lib.oracle_scene_light_records.argtypes = [S, _P, _U32, _P, _P, _P, _U32, _P]
+        lib.oracle_trace_rays.restype = None
+        lib.oracle_trace_rays.argtypes = [S, _P, _U32, _P]
+        lib.oracle_light_sample.restype = None
+        lib.oracle_light_sample.argtypes = [S, T, _P, _P, _U32, _P, _P]
+        lib.oracle_light_face_pdf.restype = None
+        lib.oracle_light_face_pdf.argtypes = [S, T, _P, _U32, _P]
+        lib.oracle_trace_frame.restype = None
+        lib.oracle_trace_frame.argtypes = [S, T, ctypes.POINTER(_Camera), ctypes.POINTER(_Params), _U32, _U32, _P, _P,
+                                           _P, _P]
+        lib._trace_bound = True
+    return lib
+
+
+def camera_struct(cam: dict) -> _Camera:
+    c = _Camera()
+    for name in ("position", "forward", "right", "up"):
+        getattr(c, name)[:] = [float(v) for v in cam[name]]
+    c.tan_half_fov_y = float(cam["tan_half_fov_y"])
+    c.aspect = float(cam["aspect"])
+    c.width = int(cam["width"])
+    c.height = int(cam["height"])
+    return c
+
+
+class OracleScene:
+    """A scene held in host arrays for the oracle. `arrays` holds the same named arrays as the GPU
+    scene (keys, info, occupancy, payload, slots, hash_info, brick_bits, super_bits, materials,
+    light_codes, heap, record_offset, ordinal_of_brick, headers, nodes, leaves); missing ones are
+    produced by the oracle itself with build()."""
+
+    def __init__(self, arrays: dict, grid_min, grid_dim: int, group_shift: int, mis: bool = True):
+        self.a = {k: np.ascontiguousarray(v) for k, v in arrays.items()}
+        self.grid_min = tuple(int(v) for v in grid_min)
+        self.grid_dim = int(grid_dim)
+        self.group_shift = int(group_shift)
+        self.mis = mis
+        self._refresh()
+
+    def _refresh(self):
+        a = self.a
+        s = self.scene = _Scene()
+        s.keys = _ptr(a["keys"])
+        s.info = _ptr(a["info"])
+        s.occupancy = _ptr(a["occupancy"])
+        s.payload = _ptr(a["payload"])
+        s.slots = _ptr(a["slots"])
+        s.slot_count = a["slots"].size // 4
+        s.bricks = a["keys"].size // 3
+        s.hash_info = _ptr(a["hash_info"])
+        s.brick_bits = _ptr(a["brick_bits"])
+        s.super_bits = _ptr(a["super_bits"])
+        s.grid_min[:] = list(self.grid_min)
+        s.grid_dim = self.grid_dim
+        s.group_shift = self.group_shift
+        s.encoding = 1
+        s.materials = _ptr(a["materials"])
+        s.light_codes = _ptr(a["light_codes"])
+        s.emission_scale = 1.0
+        if "headers" in a:
+            v = self.view = _TreeView()
+            v.headers = _ptr(a["headers"])
+            v.nodes[0] = _ptr(a["nodes"])
+            v.nodes[1] = _ptr(a["empty_nodes"])
+            v.leaves[0] = _ptr(a["leaves"])
+            v.leaves[1] = _ptr(a["empty_leaves"])
+            v.heap = _ptr(a["heap"])
+            v.record_offset = _ptr(a["record_offset"])
+            v.ordinal_of_brick = _ptr(a["ordinal_of_brick"])
+            v.group_shift = self.group_shift
+            v.mis = 1 if self.mis else 0
+
+    @classmethod
+    def from_gpu_arrays(cls, arrays: dict, data, mis: bool = True) -> "OracleScene":
+        a = dict(arrays)
+        a["empty_nodes"] = np.zeros(NODE_BYTES, np.uint8)
+        a["empty_leaves"] = np.zeros(LEAF_BYTES, np.uint8)
+        return cls(a, data.grid_min, data.grid_dim, data.group_shift, mis)
+
+    @classmethod
+    def build(cls, data, slot_count: int, mis: bool = True) -> "OracleScene":
+        """Everything from SceneData on the CPU: the ordered-insert hash table, finalize, the light
+        pass and the light-tree build, all in the oracle."""
+        n = len(data.keys)
+        dim = int(data.grid_dim)
+        cells = dim ** 3
+        slots, longest = hash_build_bricks(data.keys, slot_count)
+        a = {
+            "keys": np.ascontiguousarray(data.keys, np.int32),
+            "info": np.zeros(2 * n, np.uint32),
+            "occupancy": np.zeros(8 * n, np.uint64),
+            "payload": np.ascontiguousarray(data.payload, np.uint16),
+            "slots": slots,
+            "hash_info": np.array([longest + 1, longest, 0, 0], np.uint32),
+            "brick_bits": np.zeros((cells + 31) // 32, np.uint32),
+            "super_bits": np.zeros((cells // 512 + 31) // 32 + 1, np.uint32),
+            "materials": np.ascontiguousarray(data.materials, np.float32),
+            "light_codes": np.ascontiguousarray(data.light_codes, np.uint32),
+        }
+        self = cls(a, data.grid_min, dim, data.group_shift, mis)
+        lib = _trace_lib()
+        lib.oracle_scene_finalize(ctypes.byref(self.scene))
+        leaf_capacity = max(64, (n + 63) // 64 * 64)
+        node_capacity = leaf_capacity // 8 * 5 + 64
+        heap_units = max(64, n * 8)
+        while True:
+            heap = np.zeros(heap_units * 4, np.uint32)
+            record_offset = np.zeros(max(n, 1), np.uint32)
+            ordinal_of_brick = np.zeros(max(n, 1), np.uint32)
+            leaves = np.zeros(leaf_capacity * LEAF_BYTES, np.uint8)
+            totals = np.zeros(4, np.uint32)
+            lib.oracle_scene_light_records(ctypes.byref(self.scene), _ptr(heap), heap_units, _ptr(record_offset),
+                                           _ptr(ordinal_of_brick), _ptr(leaves), leaf_capacity, _ptr(totals))
+            if totals[3] == 0:
+                break
+            heap_units = int(totals[1]) + 64
+        state = np.zeros(STATE_BYTES, np.uint8)
+        st = state.view(np.uint32)
+        st[0] = 1  # building
+        st[1] = totals[0]  # leaves_found
+        tree = light_bvh_build(leaves, leaf_capacity, node_capacity, state)
+        headers = np.zeros(2 * HEADER_BYTES, np.uint8)
+        headers[:HEADER_BYTES] = tree["header"]
+        headers[HEADER_BYTES:].view(np.uint32)[12] = 0xFFFFFFFF  # tree 1: empty (root = none)
+        a.update(heap=heap, record_offset=record_offset, ordinal_of_brick=ordinal_of_brick, headers=headers,
+                 nodes=tree["nodes"], leaves=tree["leaves"], totals=totals,
+                 empty_nodes=np.zeros(NODE_BYTES, np.uint8), empty_leaves=np.zeros(LEAF_BYTES, np.uint8))
+        self.a = a
+        self._refresh()
+        return self
+
+    def trace_rays(self, rays: np.ndarray) -> np.ndarray:
+        r = np.ascontiguousarray(rays, np.float32)
+        hits = np.zeros((len(r), 9), np.uint32)
+        _trace_lib().oracle_trace_rays(ctypes.byref(self.scene), _ptr(r), len(r), _ptr(hits))
+        return hits
+
+    def light_sample(self, receivers: np.ndarray, seeds: np.ndarray):
+        r = np.ascontiguousarray(receivers, np.float32)
+        sd = np.ascontiguousarray(seeds, np.uint32)
+        out = np.zeros((len(r), 9), np.float32)
+        ok = np.zeros(len(r), np.uint32)
+        _trace_lib().oracle_light_sample(ctypes.byref(self.scene), ctypes.byref(self.view), _ptr(r), _ptr(sd), len(r),
+                                         _ptr(out), _ptr(ok))
+        return out, ok
+
+    def light_face_pdf(self, queries: np.ndarray) -> np.ndarray:
+        q = np.ascontiguousarray(queries, np.uint32)
+        pdf = np.zeros(len(q), np.float32)
+        _trace_lib().oracle_light_face_pdf(ctypes.byref(self.scene), ctypes.byref(self.view), _ptr(q), len(q),
+                                           _ptr(pdf))
+        return pdf
+
+    def trace_frame(self, camera: dict, bounces: int, seed: int, spp: int = 1, rows=None):
+        cam = camera_struct(camera)
+        prm = _Params(bounces, seed, spp, 0)
+        r0, r1 = rows if rows is not None else (0, cam.height)
+        count = (r1 - r0) * cam.width
+        color = np.zeros(count * 4, np.float64)
+        radiance = np.zeros(count * 4, np.float64)
+        attrib = np.zeros(count, np.uint32)
+        counters = np.zeros(5, np.uint64)
+        _trace_lib().oracle_trace_frame(ctypes.byref(self.scene), ctypes.byref(self.view), ctypes.byref(cam),
+                                        ctypes.byref(prm), r0, r1, _ptr(color), _ptr(radiance), _ptr(attrib),
+                                        _ptr(counters))
+        return color.reshape(-1, 4), radiance.reshape(-1, 4), attrib, counters

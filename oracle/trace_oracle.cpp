@@ -1,0 +1,1481 @@
+// ORACLE — test infrastructure only. Sequential CPU restatement of the trace side of the hot path:
+// the record producer's opaque subset (shaders/scene/caelix_finalize.hlsl:112-361), the light pass
+// (shaders/light/records/caelix_light_tree_build.hlsl:93-283, caelix_light_gather_emit.hlsl:36-86),
+// the brick DDA (shaders/render/caelix_brick_trace.hlsl:203-451), the world traversal through the hash
+// table, the light tree's sample and pdf (shaders/light/caelix_light_tree.hlsl:85-777), the MIS weight
+// (shaders/render/caelix_light_nee.hlsl:20-34) and the path tracer
+// (shaders/render/caelix_path_trace.hlsl:276-452, 737-978) for opaque Lambertian voxels.
+//
+// Every float operation is one IEEE binary32 operation in source order (-ffp-contract=off), as in the
+// gfx950 kernels, so decisions (DDA steps, draws) match; the path tracer accumulates radiance in
+// float64. sin/cos: the Cephes single-precision polynomials (S. Moshier, Cephes Math Library).
+//
+// Pinning (tests/):
+//   brick/world DDA   brute force: the first opaque voxel along the ray by exact rational stepping
+//                     on axis-aligned and grazing-free random rays; step counts bounded (<= 22);
+//                     bricks found by hash agree with a dict of the scene
+//   finalize words    numpy recomputation of occupancy bits, coarse byte and tight bounds
+//   light records     numpy recomputation of open light faces, families, counts, box
+//   light sampling    sum of pdf over all light faces = 1 per receiver (exhaustive), draw pdf = face
+//                     pdf, histogram of draws vs pdf (chi-square), as the reference test
+//                     (tests/light_tree_test.cpp:888-1068)
+//   path tracer       one-bounce ceiling energy against a closed form is not reachable for these
+//                     scenes: parity unpinned beyond the pieces above (documented)
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "caelix_amd/caelix_amd.h"
+#include "oracle_common.h"
+
+namespace {
+
+struct V
+{
+    float x, y, z;
+};
+V Mk(float x, float y, float z) { return V{x, y, z}; }
+V Add(V a, V b) { return V{a.x + b.x, a.y + b.y, a.z + b.z}; }
+V Sub(V a, V b) { return V{a.x - b.x, a.y - b.y, a.z - b.z}; }
+V Mul(V a, float s) { return V{a.x * s, a.y * s, a.z * s}; }
+V MulV(V a, V b) { return V{a.x * b.x, a.y * b.y, a.z * b.z}; }
+float Dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+float Comp(V v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+void SetComp(V& v, int i, float s)
+{
+    (i == 0 ? v.x : (i == 1 ? v.y : v.z)) = s;
+}
+V SafeNormalize(V v)
+{
+    const float dp3 = std::fmax(1.175494351e-38f, Dot(v, v));
+    return Mul(v, 1.0f / std::sqrt(dp3));
+}
+
+constexpr float kPi = 3.1415926535f;
+constexpr float kTwoPi = 6.283185307f;
+constexpr float kRayTMax = 1e30f;
+constexpr float kPushOff = 0.002f;
+constexpr uint32_t kNoneU = 0xFFFFFFFFu;
+
+void SinCos(float a, float& s, float& c)
+{
+    int j = int(a * 1.27323954473516f);
+    float y = float(j);
+    if ((j & 1) != 0)
+    {
+        j += 1;
+        y += 1.0f;
+    }
+    j &= 7;
+    const float x = ((a - y * 0.78515625f) - y * 2.4187564849853515625e-4f) - y * 3.77489497744594108e-8f;
+    const float z = x * x;
+    const float ps = ((-1.9515295891e-4f * z + 8.3321608736e-3f) * z - 1.6666654611e-1f) * z * x + x;
+    const float pc = ((2.443315711809948e-5f * z - 1.388731625493765e-3f) * z + 4.166664568298827e-2f) * z * z - 0.5f * z + 1.0f;
+    int js = j;
+    bool neg_s = false;
+    if (js > 3)
+    {
+        neg_s = true;
+        js -= 4;
+    }
+    s = (js == 1 || js == 2) ? pc : ps;
+    if (neg_s)
+    {
+        s = -s;
+    }
+    int jc = j;
+    bool neg_c = false;
+    if (jc > 3)
+    {
+        jc -= 4;
+        neg_c = !neg_c;
+    }
+    if (jc > 1)
+    {
+        neg_c = !neg_c;
+    }
+    c = (jc == 1 || jc == 2) ? ps : pc;
+    if (neg_c)
+    {
+        c = -c;
+    }
+}
+
+uint32_t PcgHash(uint32_t value)
+{
+    const uint32_t state = value * 747796405u + 2891336453u;
+    const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+uint32_t InitRandom(uint32_t px, uint32_t py, uint32_t seed)
+{
+    const uint32_t state = PcgHash(px ^ (py << 16) ^ (seed * 0x9E3779B9u));
+    return state != 0u ? state : 0x9E3779B9u;
+}
+uint32_t RandomU32(uint32_t& state)
+{
+    state ^= state << 13;
+    state ^= state >> 17;
+    state ^= state << 5;
+    return state;
+}
+float RandomFloat01(uint32_t& state)
+{
+    return float(RandomU32(state) >> 8) * (1.0f / 16777216.0f);
+}
+V RandomUnitVector(uint32_t& state)
+{
+    const float u = RandomFloat01(state);
+    const float v = RandomFloat01(state);
+    const float a = u * kTwoPi;
+    const float y = v * 2.0f - 1.0f;
+    const float sy = std::sqrt(std::fmax(1.0f - y * y, 0.0f));
+    float s, c;
+    SinCos(a, s, c);
+    return Mk(s * sy, y, c * sy);
+}
+uint32_t LightRandom(uint32_t& rng)
+{
+    rng = rng * 747796405u + 2891336453u;
+    const uint32_t word = ((rng >> ((rng >> 28) + 4u)) ^ rng) * 277803737u;
+    return (word >> 22) ^ word;
+}
+float LightRandomFloat(uint32_t& rng)
+{
+    return float(LightRandom(rng) >> 8) * (1.0f / 16777216.0f);
+}
+
+// ---- hash ----
+uint64_t SplitMix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+void BrickKey(int32_t x, int32_t y, int32_t z, uint32_t& lo, uint32_t& hi)
+{
+    uint64_t h = SplitMix64(0 ^ ((uint64_t(uint32_t(y)) << 32) | uint32_t(x)));
+    h = SplitMix64(h ^ uint64_t(uint32_t(z)));
+    lo = std::max(uint32_t(h), 1u);
+    hi = uint32_t(h >> 32) != 0 ? uint32_t(h >> 32) : 1u;
+}
+uint32_t HashGet(const CxScene& s, int32_t x, int32_t y, int32_t z, uint32_t& probes)
+{
+    uint32_t lo, hi;
+    BrickKey(x, y, z, lo, hi);
+    const bool pow2 = (s.slot_count & (s.slot_count - 1)) == 0;
+    uint32_t slot = pow2 ? (lo & (s.slot_count - 1)) : (lo % s.slot_count);
+    const uint32_t limit = s.hash_info[0];
+    for (uint32_t p = 0; p < limit; ++p)
+    {
+        const HashKeySlot& k = s.slots[slot];
+        if (k.key_lo == 0)
+        {
+            probes = p + 1;
+            return kNoneU;
+        }
+        if (k.key_lo == lo && k.key_hi == hi)
+        {
+            probes = p + 1;
+            return k.payload;
+        }
+        slot = slot + 1 == s.slot_count ? 0 : slot + 1;
+    }
+    probes = limit;
+    return kNoneU;
+}
+
+bool Opaque(uint32_t block, uint32_t encoding)
+{
+    return (block & (encoding == kEncodingPacked ? kPackedOpaqueMask : kBlockIdOpaqueMask)) != 0;
+}
+uint32_t FaceBits(uint32_t block, uint32_t encoding)
+{
+    return encoding == kEncodingPacked ? ((block >> kPackedFaceShift) & 0x3Fu) : ((block >> kBlockIdFaceShift) & 0x3Fu);
+}
+
+// ---- brick DDA ----
+struct BrickHit
+{
+    float t;
+    uint32_t attrib;
+    uint32_t steps;
+};
+
+float SafeInvAxis(float d)
+{
+    if (std::fabs(d) >= 1e-20f)
+    {
+        return 1.0f / d;
+    }
+    return d < 0.0f ? -1.0f / 1e-20f : 1.0f / 1e-20f;
+}
+
+BrickHit TraceBrickRay(const uint64_t* occ, const uint16_t* payload, uint32_t encoding, V entry, V dir, float entry_t,
+                       uint32_t entry_normal_coarse)
+{
+    const V inv = Mk(SafeInvAxis(dir.x), SafeInvAxis(dir.y), SafeInvAxis(dir.z));
+    const V boundary = Mk(dir.x >= 0.0f ? 1.0f : 0.0f, dir.y >= 0.0f ? 1.0f : 0.0f, dir.z >= 0.0f ? 1.0f : 0.0f);
+    const V t_start = MulV(Sub(boundary, entry), inv);
+    const float hi = 8.0f - 0.0001f;
+    const V origin = Mk(std::fmin(std::fmax(entry.x, 0.0f), hi), std::fmin(std::fmax(entry.y, 0.0f), hi),
+                        std::fmin(std::fmax(entry.z, 0.0f), hi));
+    int cell[3] = {int(std::floor(origin.x)), int(std::floor(origin.y)), int(std::floor(origin.z))};
+    int prev[3] = {cell[0], cell[1], cell[2]};
+    BrickHit hit{0.0f, 0u, 0u};
+    for (uint32_t step = 0; step < 22; ++step)
+    {
+        hit.steps = step + 1;
+        if (uint32_t(cell[0]) >= 8u || uint32_t(cell[1]) >= 8u || uint32_t(cell[2]) >= 8u)
+        {
+            break;
+        }
+        const uint32_t coarse_bit = uint32_t((cell[0] >> 2) | ((cell[1] >> 2) << 1) | ((cell[2] >> 2) << 2));
+        if ((entry_normal_coarse & (1u << coarse_bit)) == 0u)
+        {
+            // VRTJump (caelix_brick_trace.hlsl:286-289), then VRTStep.
+            cell[0] = dir.x < 0.0f ? (cell[0] & ~3) : (cell[0] | 3);
+            cell[1] = dir.y < 0.0f ? (cell[1] & ~3) : (cell[1] | 3);
+            cell[2] = dir.z < 0.0f ? (cell[2] & ~3) : (cell[2] | 3);
+        }
+        else
+        {
+            const uint32_t micro = uint32_t((cell[0] & 3) | ((cell[1] & 3) << 2) | ((cell[2] & 3) << 4));
+            if (((occ[coarse_bit] >> micro) & 1ull) != 0ull)
+            {
+                const uint32_t block = payload[cell[0] + 8 * cell[1] + 64 * cell[2]];
+                float T = 0.0f;
+                uint32_t flags;
+                if (step == 0)
+                {
+                    flags = entry_normal_coarse >> 26;
+                }
+                else
+                {
+                    const V side = Add(t_start, MulV(Mk(float(prev[0]), float(prev[1]), float(prev[2])), inv));
+                    T = std::fmin(std::fmin(side.x, side.y), side.z);
+                    const float lim = T + 0.0001f;
+                    const uint32_t x_use = side.x <= lim ? 0xFFFFFFFFu : 0u;
+                    const uint32_t y_use = (side.y <= lim ? 0xFFFFFFFFu : 0u) & ~x_use;
+                    const uint32_t z_use = ~(x_use | y_use);
+                    flags = ((dir.x > 0.0f ? 0x02u : 0x01u) & x_use) | ((dir.y > 0.0f ? 0x08u : 0x04u) & y_use) |
+                            ((dir.z > 0.0f ? 0x20u : 0x10u) & z_use);
+                }
+                bool terminate = Opaque(block, encoding);
+                if (!terminate && !(step == 0 && entry_t == 0.0f))
+                {
+                    terminate = (FaceBits(block, encoding) & flags) != 0u;
+                }
+                if (terminate)
+                {
+                    hit.attrib = (block << 16) | (uint32_t(cell[0] | (cell[1] << 3) | (cell[2] << 6)) << 6) | flags;
+                    hit.t = entry_t + T;
+                    return hit;
+                }
+            }
+        }
+        prev[0] = cell[0];
+        prev[1] = cell[1];
+        prev[2] = cell[2];
+        const V side = Add(t_start, MulV(Mk(float(cell[0]), float(cell[1]), float(cell[2])), inv));
+        const float tmin = std::fmin(std::fmin(side.x, side.y), side.z) + 0.0001f;
+        const V pos = Add(origin, Mul(dir, tmin));
+        cell[0] = int(std::floor(pos.x));
+        cell[1] = int(std::floor(pos.y));
+        cell[2] = int(std::floor(pos.z));
+    }
+    hit.attrib = 0u;
+    hit.t = 0.0f;
+    return hit;
+}
+
+BrickHit TraceBrick(const CxScene& s, uint32_t record, V o, V dir, float t_current, V brick_origin)
+{
+    const uint32_t tb = s.info[2 * uint64_t(record) + 1];
+    const V tmin_box = Mk(float(tb & 7u), float((tb >> 3) & 7u), float((tb >> 6) & 7u));
+    const V tmax_box = Mk(float((tb >> 9) & 7u) + 1.0f, float((tb >> 12) & 7u) + 1.0f, float((tb >> 15) & 7u) + 1.0f);
+    const V aabb_min = Add(brick_origin, tmin_box);
+    const V aabb_max = Add(brick_origin, tmax_box);
+    const V inv = Mk(1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z);
+    const V t0 = MulV(Sub(aabb_min, o), inv);
+    const V t1 = MulV(Sub(aabb_max, o), inv);
+    const V tn = Mk(std::fmin(t0.x, t1.x), std::fmin(t0.y, t1.y), std::fmin(t0.z, t1.z));
+    const V tf = Mk(std::fmax(t0.x, t1.x), std::fmax(t0.y, t1.y), std::fmax(t0.z, t1.z));
+    const float largest_tmin = std::fmax(std::fmax(tn.x, tn.y), tn.z);
+    const float smallest_tmax = std::fmin(std::fmin(tf.x, tf.y), tf.z);
+    if (largest_tmin > smallest_tmax || smallest_tmax < 0.0f || largest_tmin > t_current)
+    {
+        return BrickHit{0.0f, 0u, 0u};
+    }
+    const float t = std::fmax(0.0f, largest_tmin);
+    const V entry = Sub(Add(o, Mul(dir, t)), brick_origin);
+    uint32_t flags;
+    if (largest_tmin == tn.x)
+    {
+        flags = dir.x > 0.0f ? 0x02u : 0x01u;
+    }
+    else if (largest_tmin == tn.y)
+    {
+        flags = dir.y > 0.0f ? 0x08u : 0x04u;
+    }
+    else
+    {
+        flags = dir.z > 0.0f ? 0x20u : 0x10u;
+    }
+    const uint32_t coarse = (s.info[2 * uint64_t(record)] >> kRecordCoarseShift) & kRecordCoarseMask;
+    const uint64_t* occ = static_cast<const uint64_t*>(s.occupancy) + uint64_t(record) * 8;
+    return TraceBrickRay(occ, s.payload + uint64_t(record) * 512, s.encoding, entry, dir, t, (flags << 26) | coarse);
+}
+
+struct WorldHit
+{
+    float t;
+    uint32_t attrib;
+    uint32_t record;
+    int32_t brick[3];
+    uint32_t cells, probes, steps;
+};
+
+bool Bit(const uint32_t* bits, uint64_t i)
+{
+    return ((bits[i >> 5] >> (i & 31u)) & 1u) != 0u;
+}
+
+// Super-cell DDA, then brick DDA inside resident super cells; plane crossings (k * size + g0 - o) * inv.
+WorldHit TraceWorld(const CxScene& s, V o, V dir, float t_max)
+{
+    WorldHit r{0.0f, 0u, kNoneU, {0, 0, 0}, 0, 0, 0};
+    const float g0[3] = {float(s.grid_min[0] * 8), float(s.grid_min[1] * 8), float(s.grid_min[2] * 8)};
+    const float extent = float(s.grid_dim * 8);
+    const float ov[3] = {o.x, o.y, o.z};
+    const float dv[3] = {dir.x, dir.y, dir.z};
+    float inv[3];
+    int step[3];
+    float t_enter = 0.0f;
+    float t_exit = t_max;
+    for (int a = 0; a < 3; ++a)
+    {
+        inv[a] = SafeInvAxis(dv[a]);
+        step[a] = dv[a] >= 0.0f ? 1 : -1;
+        const float t0 = (g0[a] - ov[a]) * inv[a];
+        const float t1 = (g0[a] + extent - ov[a]) * inv[a];
+        t_enter = std::fmax(t_enter, std::fmin(t0, t1));
+        t_exit = std::fmin(t_exit, std::fmax(t0, t1));
+    }
+    if (t_enter > t_exit)
+    {
+        return r;
+    }
+    const int sd = int(s.grid_dim >> 3);
+    int sc[3];
+    for (int a = 0; a < 3; ++a)
+    {
+        const float p = ov[a] + dv[a] * t_enter;
+        sc[a] = std::min(std::max(int(std::floor((p - g0[a]) * (1.0f / 64.0f))), 0), sd - 1);
+    }
+    float t_cell = t_enter;
+    while (true)
+    {
+        float t_next[3];
+        for (int a = 0; a < 3; ++a)
+        {
+            const int k = sc[a] + (step[a] > 0 ? 1 : 0);
+            t_next[a] = (float(k * 64) + g0[a] - ov[a]) * inv[a];
+        }
+        const float t_leave = std::fmin(std::fmin(t_next[0], t_next[1]), t_next[2]);
+        if (Bit(s.super_bits, (uint64_t(sc[2]) * sd + sc[1]) * sd + sc[0]))
+        {
+            int bc[3];
+            for (int a = 0; a < 3; ++a)
+            {
+                const float p = ov[a] + dv[a] * t_cell;
+                bc[a] = std::min(std::max(int(std::floor((p - g0[a]) * 0.125f)), sc[a] * 8), sc[a] * 8 + 7);
+            }
+            while (true)
+            {
+                r.cells += 1;
+                if (Bit(s.brick_bits, (uint64_t(bc[2]) * s.grid_dim + bc[1]) * s.grid_dim + bc[0]))
+                {
+                    const int32_t key[3] = {s.grid_min[0] + bc[0], s.grid_min[1] + bc[1], s.grid_min[2] + bc[2]};
+                    uint32_t probes = 0;
+                    const uint32_t record = HashGet(s, key[0], key[1], key[2], probes);
+                    r.probes += probes;
+                    if (record != kNoneU)
+                    {
+                        const BrickHit h = TraceBrick(s, record, o, dir, t_max,
+                                                      Mk(float(key[0] * 8), float(key[1] * 8), float(key[2] * 8)));
+                        r.steps += h.steps;
+                        if (h.attrib != 0u && h.t < t_max)
+                        {
+                            r.t = h.t;
+                            r.attrib = h.attrib;
+                            r.record = record;
+                            r.brick[0] = key[0];
+                            r.brick[1] = key[1];
+                            r.brick[2] = key[2];
+                            return r;
+                        }
+                    }
+                }
+                float tb[3];
+                for (int a = 0; a < 3; ++a)
+                {
+                    const int k = bc[a] + (step[a] > 0 ? 1 : 0);
+                    tb[a] = (float(k * 8) + g0[a] - ov[a]) * inv[a];
+                }
+                int axis = 0;
+                if (tb[1] < tb[axis])
+                {
+                    axis = 1;
+                }
+                if (tb[2] < tb[axis])
+                {
+                    axis = 2;
+                }
+                if (tb[axis] > t_max)
+                {
+                    return r;
+                }
+                bc[axis] += step[axis];
+                if (bc[axis] < sc[axis] * 8 || bc[axis] > sc[axis] * 8 + 7)
+                {
+                    break;
+                }
+            }
+        }
+        int axis = 0;
+        if (t_next[1] < t_next[axis])
+        {
+            axis = 1;
+        }
+        if (t_next[2] < t_next[axis])
+        {
+            axis = 2;
+        }
+        if (t_leave > t_exit)
+        {
+            return r;
+        }
+        sc[axis] += step[axis];
+        if (sc[axis] < 0 || sc[axis] >= sd)
+        {
+            return r;
+        }
+        t_cell = t_next[axis];
+    }
+}
+
+// ---- light tree ----
+struct Summ
+{
+    float face[6];
+    V lo, hi;
+};
+float Power6(const float* f)
+{
+    return f[0] + f[1] + f[2] + f[3] + f[4] + f[5];
+}
+Summ SlotOf(const LightBvhNode& n, uint32_t s)
+{
+    Summ r;
+    for (int f = 0; f < 6; ++f)
+    {
+        r.face[f] = n.child_face[s][f];
+    }
+    r.lo = Mk(n.child_lo[s][0], n.child_lo[s][1], n.child_lo[s][2]);
+    r.hi = Mk(n.child_hi[s][0], n.child_hi[s][1], n.child_hi[s][2]);
+    return r;
+}
+void EmitterCosines(V d_lo, V d_hi, float c[6])
+{
+    const V closest = Mk(std::fmin(std::fmax(0.0f, d_lo.x), d_hi.x), std::fmin(std::fmax(0.0f, d_lo.y), d_hi.y),
+                         std::fmin(std::fmax(0.0f, d_lo.z), d_hi.z));
+    const V c2 = MulV(closest, closest);
+    const float all2 = c2.x + c2.y + c2.z;
+    for (int a = 0; a < 3; ++a)
+    {
+        const float others = all2 - Comp(c2, a);
+        const float dh = Comp(d_hi, a);
+        const float dl = Comp(d_lo, a);
+        c[2 * a] = dh > 0.0f ? dh / std::sqrt(dh * dh + others) : 0.0f;
+        c[2 * a + 1] = dl < 0.0f ? -dl / std::sqrt(dl * dl + others) : 0.0f;
+    }
+}
+float Importance(V x, V n, V lo, V hi, const float face[6])
+{
+    float c[6];
+    EmitterCosines(Sub(x, hi), Sub(x, lo), c);
+    float e = 0.0f;
+    for (int f = 0; f < 6; ++f)
+    {
+        e += face[f] * c[f];
+    }
+    if (e <= 0.0f)
+    {
+        return 0.0f;
+    }
+    const V centre = Mul(Add(lo, hi), 0.5f);
+    const V half = Mul(Sub(hi, lo), 0.5f);
+    float cr = 1.0f;
+    if (n.x != 0.0f || n.y != 0.0f || n.z != 0.0f)
+    {
+        const float m = Dot(n, Sub(centre, x)) + Dot(Mk(std::fabs(n.x), std::fabs(n.y), std::fabs(n.z)), half);
+        if (m <= 0.0f)
+        {
+            return 0.0f;
+        }
+        const V gap = Mk(std::fmax(std::fmax(lo.x - x.x, x.x - hi.x), 0.0f), std::fmax(std::fmax(lo.y - x.y, x.y - hi.y), 0.0f),
+                         std::fmax(std::fmax(lo.z - x.z, x.z - hi.z), 0.0f));
+        const float dmin = std::sqrt(Dot(gap, gap));
+        cr = dmin > 0.0f ? std::fmin(1.0f, m / dmin) : 1.0f;
+    }
+    const V tc = Sub(x, centre);
+    const float d2 = std::fmax(Dot(tc, tc), std::fmax(1.0f, Dot(half, half)));
+    return (e * cr) / d2;
+}
+struct Pick
+{
+    float sum_i = 0.0f, sum_p = 0.0f, weight_i = 0.0f, weight_p = 0.0f;
+    uint32_t index_i = kNoneU, index_p = kNoneU;
+};
+void PickAdd(Pick& p, float importance, float power, uint32_t index, float u)
+{
+    if (importance > 0.0f)
+    {
+        p.sum_i += importance;
+        if (u * p.sum_i < importance)
+        {
+            p.weight_i = importance;
+            p.index_i = index;
+        }
+    }
+    if (power > 0.0f)
+    {
+        p.sum_p += power;
+        if (u * p.sum_p < power)
+        {
+            p.weight_p = power;
+            p.index_p = index;
+        }
+    }
+}
+uint32_t PickEnd(const Pick& p, float& probability)
+{
+    if (p.sum_i > 0.0f)
+    {
+        probability = p.weight_i / p.sum_i;
+        return p.index_i;
+    }
+    probability = p.sum_p > 0.0f ? p.weight_p / p.sum_p : 0.0f;
+    return p.sum_p > 0.0f ? p.index_p : kNoneU;
+}
+float PickProbability(float sum_i, float sum_p, float importance, float power)
+{
+    if (sum_i > 0.0f)
+    {
+        return importance / sum_i;
+    }
+    return sum_p > 0.0f ? power / sum_p : 0.0f;
+}
+Summ TreeSummary(const CxLightTreeView& L, uint32_t k, uint32_t& root)
+{
+    root = L.headers[k].root;
+    Summ e;
+    if (root == kNoneU)
+    {
+        std::memset(&e, 0, sizeof(e));
+        return e;
+    }
+    for (int f = 0; f < 6; ++f)
+    {
+        e.face[f] = L.headers[k].summary.face[f];
+    }
+    e.lo = Mk(L.headers[k].summary.lo[0], L.headers[k].summary.lo[1], L.headers[k].summary.lo[2]);
+    e.hi = Mk(L.headers[k].summary.hi[0], L.headers[k].summary.hi[1], L.headers[k].summary.hi[2]);
+    return e;
+}
+uint32_t ChildCount(const uint32_t* c)
+{
+    return uint32_t(c[0] != kNoneU) + uint32_t(c[1] != kNoneU) + uint32_t(c[2] != kNoneU) + uint32_t(c[3] != kNoneU);
+}
+uint32_t ChooseLeaf(const CxLightTreeView& L, V x, V n, uint32_t& rng, float& pdf, uint32_t& tree)
+{
+    Pick pick;
+    uint32_t roots[2];
+    for (uint32_t k = 0; k < 2; ++k)
+    {
+        const Summ s = TreeSummary(L, k, roots[k]);
+        PickAdd(pick, Importance(x, n, s.lo, s.hi, s.face), Power6(s.face), k, LightRandomFloat(rng));
+    }
+    float probability;
+    const uint32_t k = PickEnd(pick, probability);
+    tree = k;
+    if (k == kNoneU)
+    {
+        return kNoneU;
+    }
+    pdf *= probability;
+    uint32_t node = roots[k];
+    for (uint32_t depth = 0; depth < 64; ++depth)
+    {
+        const LightBvhNode& nd = L.nodes[k][node];
+        uint32_t chosen;
+        if (ChildCount(nd.child) == 1)
+        {
+            chosen = nd.child[0];
+        }
+        else
+        {
+            Pick p;
+            for (uint32_t s = 0; s < 4; ++s)
+            {
+                if (nd.child[s] == kNoneU)
+                {
+                    continue;
+                }
+                const Summ slot = SlotOf(nd, s);
+                PickAdd(p, Importance(x, n, slot.lo, slot.hi, slot.face), Power6(slot.face), s, LightRandomFloat(rng));
+            }
+            float prob;
+            const uint32_t s = PickEnd(p, prob);
+            if (s == kNoneU)
+            {
+                return kNoneU;
+            }
+            pdf *= prob;
+            chosen = nd.child[s];
+        }
+        if ((chosen & kLtChildLeafBit) != 0)
+        {
+            return chosen & ~kLtChildLeafBit;
+        }
+        node = chosen;
+    }
+    return kNoneU;
+}
+float SlotProbability(const LightBvhNode& nd, V x, V n, uint32_t slot)
+{
+    if (ChildCount(nd.child) == 1)
+    {
+        return 1.0f;
+    }
+    float sum_i = 0.0f, sum_p = 0.0f, it = 0.0f, pt = 0.0f;
+    for (uint32_t s = 0; s < 4; ++s)
+    {
+        if (nd.child[s] == kNoneU)
+        {
+            continue;
+        }
+        const Summ sm = SlotOf(nd, s);
+        const float i = Importance(x, n, sm.lo, sm.hi, sm.face);
+        const float p = Power6(sm.face);
+        sum_i += i > 0.0f ? i : 0.0f;
+        sum_p += p > 0.0f ? p : 0.0f;
+        if (s == slot)
+        {
+            it = i;
+            pt = p;
+        }
+    }
+    return PickProbability(sum_i, sum_p, it, pt);
+}
+float TreeProbability(const CxLightTreeView& L, V x, V n, uint32_t target)
+{
+    float sum_i = 0.0f, sum_p = 0.0f, it = 0.0f, pt = 0.0f;
+    for (uint32_t k = 0; k < 2; ++k)
+    {
+        uint32_t root;
+        const Summ s = TreeSummary(L, k, root);
+        const float i = Importance(x, n, s.lo, s.hi, s.face);
+        const float p = Power6(s.face);
+        sum_i += i > 0.0f ? i : 0.0f;
+        sum_p += p > 0.0f ? p : 0.0f;
+        if (k == target)
+        {
+            it = i;
+            pt = p;
+        }
+    }
+    return PickProbability(sum_i, sum_p, it, pt);
+}
+float LeafProbability(const CxLightTreeView& L, uint32_t k, uint32_t leaf, V x, V n)
+{
+    if (L.headers[k].root == kNoneU)
+    {
+        return 0.0f;
+    }
+    const uint32_t parent = L.leaves[k][leaf].parent;
+    if (parent == kNoneU)
+    {
+        return 0.0f;
+    }
+    uint32_t node = parent >> 2;
+    uint32_t slot = parent & 3u;
+    float probability = 1.0f;
+    for (uint32_t depth = 0; depth < 64; ++depth)
+    {
+        const LightBvhNode& nd = L.nodes[k][node];
+        probability *= SlotProbability(nd, x, n, slot);
+        if (nd.parent == kNoneU)
+        {
+            return probability * TreeProbability(L, x, n, k);
+        }
+        slot = nd.parent_slot;
+        node = nd.parent;
+    }
+    return 0.0f;
+}
+struct Rec
+{
+    float face[6];
+    uint32_t lo[3], hi[3];
+    uint32_t pos, entry_count, counts_lo, counts_hi;
+    const uint32_t* entries;
+};
+Rec LoadRec(const CxLightTreeView& L, uint32_t ordinal)
+{
+    const uint32_t* r = L.heap + uint64_t(L.record_offset[ordinal]) * 4;
+    Rec rec;
+    std::memcpy(rec.face, r, 24);
+    for (int a = 0; a < 3; ++a)
+    {
+        rec.lo[a] = (r[6] >> (10 * a)) & 0x3FFu;
+        rec.hi[a] = (r[7] >> (10 * a)) & 0x3FFu;
+    }
+    rec.pos = r[8];
+    rec.entry_count = r[9];
+    rec.counts_lo = r[10];
+    rec.counts_hi = r[11];
+    rec.entries = r + 12;
+    return rec;
+}
+uint32_t FamilyCount(const Rec& r, uint32_t f)
+{
+    return ((f < 3 ? r.counts_lo : r.counts_hi) >> (10 * (f % 3))) & 0x3FFu;
+}
+void FamilyWeights(const Rec& r, V x, float w[6], float& sum)
+{
+    float c[6];
+    EmitterCosines(Sub(x, Mk(float(r.hi[0]), float(r.hi[1]), float(r.hi[2]))),
+                   Sub(x, Mk(float(r.lo[0]), float(r.lo[1]), float(r.lo[2]))), c);
+    sum = 0.0f;
+    for (int f = 0; f < 6; ++f)
+    {
+        w[f] = r.face[f] * c[f];
+        sum += w[f];
+    }
+    if (sum <= 0.0f)
+    {
+        sum = 0.0f;
+        for (int g = 0; g < 6; ++g)
+        {
+            w[g] = r.face[g];
+            sum += w[g];
+        }
+    }
+}
+V GroupOrigin(const CxLightTreeView& L, const int32_t* key)
+{
+    const int32_t g = int32_t(L.group_shift);
+    return Mk(float((key[0] >> g) << g) * 8.0f, float((key[1] >> g) << g) * 8.0f, float((key[2] >> g) << g) * 8.0f);
+}
+V FaceDirection(uint32_t f)
+{
+    const float sign = (f & 1) != 0 ? -1.0f : 1.0f;
+    const uint32_t axis = f >> 1;
+    return Mk(axis == 0 ? sign : 0.0f, axis == 1 ? sign : 0.0f, axis == 2 ? sign : 0.0f);
+}
+struct LightPoint
+{
+    V position, normal;
+    uint32_t block, face;
+    float pdf;
+};
+bool LightTreeSample(const CxScene& s, const CxLightTreeView& L, V x, V n, uint32_t& rng, LightPoint& p)
+{
+    p = LightPoint{Mk(0, 0, 0), Mk(0, 0, 0), 0u, 0u, 0.0f};
+    float pdf_brick = 1.0f;
+    uint32_t tree;
+    const uint32_t leaf = ChooseLeaf(L, x, n, rng, pdf_brick, tree);
+    if (leaf == kNoneU)
+    {
+        return false;
+    }
+    const LightBvhLeaf& lf = L.leaves[tree][leaf];
+    const Rec record = LoadRec(L, lf.entry);
+    const int32_t* key = s.keys + uint64_t(lf.group) * 3;
+    const V origin = GroupOrigin(L, key);
+    float w[6];
+    float sum;
+    FamilyWeights(record, Sub(x, origin), w, sum);
+    if (sum <= 0.0f)
+    {
+        return false;
+    }
+    const float u = LightRandomFloat(rng) * sum;
+    uint32_t f = kNoneU;
+    uint32_t last = 0;
+    float cumulative = 0.0f;
+    for (uint32_t g = 0; g < 6; ++g)
+    {
+        if (w[g] > 0.0f)
+        {
+            last = g;
+            cumulative += w[g];
+            if (f == kNoneU && u < cumulative)
+            {
+                f = g;
+            }
+        }
+    }
+    f = f == kNoneU ? last : f;
+    uint32_t first = 0;
+    for (uint32_t h = 0; h < f; ++h)
+    {
+        first += FamilyCount(record, h);
+    }
+    const uint32_t count = FamilyCount(record, f);
+    if (count == 0 || w[f] <= 0.0f)
+    {
+        return false;
+    }
+    const uint32_t index = std::min(uint32_t(LightRandomFloat(rng) * float(count)), count - 1u);
+    const uint32_t entry = record.entries[first + index];
+    const uint32_t vi = entry & 0x1FFu;
+    const uint32_t mask = (1u << L.group_shift) - 1u;
+    const uint32_t b[3] = {record.pos & mask, (record.pos >> L.group_shift) & mask, record.pos >> (2 * L.group_shift)};
+    V local = Add(origin, Mk(float(b[0] * 8 + (vi & 7)), float(b[1] * 8 + ((vi >> 3) & 7)), float(b[2] * 8 + (vi >> 6))));
+    const float a = LightRandomFloat(rng);
+    const float c = LightRandomFloat(rng);
+    const int axis = int(f >> 1);
+    SetComp(local, axis, Comp(local, axis) + ((f & 1) != 0 ? 0.0f : 1.0f));
+    SetComp(local, (axis + 1) % 3, Comp(local, (axis + 1) % 3) + a);
+    SetComp(local, (axis + 2) % 3, Comp(local, (axis + 2) % 3) + c);
+    p.position = local;
+    p.normal = FaceDirection(f);
+    p.block = entry >> 16;
+    p.face = f;
+    p.pdf = ((w[f] / sum) / float(count)) * pdf_brick;
+    return true;
+}
+float LightFacePdf(const CxScene& s, const CxLightTreeView& L, V x, V n, uint32_t record, const int32_t voxel[3],
+                   uint32_t face, uint32_t block, bool& found)
+{
+    found = false;
+    const uint32_t code = block & 0xFFFFu;
+    if (((s.light_codes[code >> 5] >> (code & 31u)) & 1u) == 0u || face > 5u)
+    {
+        return 0.0f;
+    }
+    const uint32_t ordinal = L.ordinal_of_brick[record];
+    if (ordinal == kNoneU)
+    {
+        return 0.0f;
+    }
+    const Rec rec = LoadRec(L, ordinal);
+    const V origin = GroupOrigin(L, s.keys + uint64_t(record) * 3);
+    float w[6];
+    float sum;
+    FamilyWeights(rec, Sub(x, origin), w, sum);
+    const uint32_t count = FamilyCount(rec, face);
+    if (sum <= 0.0f || w[face] <= 0.0f || count == 0u)
+    {
+        return 0.0f;
+    }
+    uint32_t first = 0;
+    for (uint32_t g = 0; g < face; ++g)
+    {
+        first += FamilyCount(rec, g);
+    }
+    const uint32_t vi = uint32_t((voxel[0] & 7) | ((voxel[1] & 7) << 3) | ((voxel[2] & 7) << 6));
+    bool hit = false;
+    for (uint32_t e = 0; e < count; ++e)  // linear scan: the entries are sorted, any search finds it
+    {
+        if ((rec.entries[first + e] & 0x1FFu) == vi)
+        {
+            hit = true;
+            break;
+        }
+    }
+    if (!hit)
+    {
+        return 0.0f;
+    }
+    found = true;
+    const float in_brick = (w[face] / sum) / float(count);
+    if (ordinal >= L.headers[0].leaf_count)
+    {
+        found = false;
+        return 0.0f;
+    }
+    return LeafProbability(L, 0, ordinal, x, n) * in_brick;
+}
+float MisWeight(float p_area, V n_r, V w, float d, float cos_l, bool nee_side)
+{
+    const float p_sa = cos_l > 0.0f ? (p_area * (d * d)) / cos_l : 0.0f;
+    const float p_bsdf = std::fmax(0.0f, Dot(n_r, w)) * (1.0f / 3.1415926535f);
+    if (!(cos_l > 0.0f) || !(p_sa <= 3.0e38f))
+    {
+        return nee_side ? 1.0f : 0.0f;
+    }
+    const float sum = p_sa + p_bsdf;
+    if (!(sum > 0.0f))
+    {
+        return nee_side ? 0.0f : 1.0f;
+    }
+    return (nee_side ? p_sa : p_bsdf) / sum;
+}
+
+// ---- path tracer ----
+V MatAlbedo(const CxScene& s, uint32_t b)
+{
+    const float* m = s.materials + uint64_t(b) * 8;
+    return Mk(m[0], m[1], m[2]);
+}
+V MatEmission(const CxScene& s, uint32_t b)
+{
+    const float* m = s.materials + uint64_t(b) * 8 + 4;
+    return Mul(Mk(m[0], m[1], m[2]), s.emission_scale);
+}
+float Luminance(V c)
+{
+    return c.x * 0.2126f + c.y * 0.7152f + c.z * 0.0722f;
+}
+uint32_t LowBit(uint32_t v)
+{
+    return uint32_t(__builtin_ctz(v));
+}
+struct Counters
+{
+    uint64_t rays = 0, shadow = 0, cells = 0, probes = 0, steps = 0;
+};
+WorldHit Trace(const CxScene& s, V o, V d, float t_max, Counters& c)
+{
+    const WorldHit h = TraceWorld(s, o, d, t_max);
+    c.cells += h.cells;
+    c.probes += h.probes;
+    c.steps += h.steps;
+    return h;
+}
+struct Shade
+{
+    V albedo, emission, origin, direction;
+};
+Shade ShadeHit(const CxScene& s, uint32_t block, V hp, V normal, uint32_t& rng)
+{
+    Shade sh;
+    sh.emission = MatEmission(s, block);
+    sh.albedo = MatAlbedo(s, block);
+    (void)RandomFloat01(rng);
+    V d = SafeNormalize(Add(normal, RandomUnitVector(rng)));
+    d = Dot(d, d) > 0.0f ? d : normal;
+    sh.origin = Add(hp, Mul(normal, kPushOff));
+    sh.direction = d;
+    return sh;
+}
+V NextEvent(const CxScene& s, const CxLightTreeView& L, V x_r, V n_r, uint32_t& light_rng, Counters& c)
+{
+    const uint32_t seed = RandomU32(light_rng);
+    V contribution = Mk(0, 0, 0);
+    V direction = Mk(0, 0, 0);
+    float t_max = 0.0f;
+    uint32_t rng = seed;
+    LightPoint p;
+    if (LightTreeSample(s, L, x_r, n_r, rng, p) && p.pdf > 0.0f)
+    {
+        const V d = Sub(p.position, x_r);
+        const float d2 = Dot(d, d);
+        const float distance = std::sqrt(d2);
+        if (distance >= 8.0f)
+        {
+            const V w = Mk(d.x / distance, d.y / distance, d.z / distance);
+            const float cos_r = Dot(n_r, w);
+            const float cos_l = -Dot(p.normal, w);
+            if (cos_r > 0.0f && cos_l > 0.0f)
+            {
+                float scale = ((cos_r * (1.0f / kPi)) * (cos_l / d2)) / p.pdf;
+                if (L.mis != 0u)
+                {
+                    scale *= MisWeight(p.pdf, n_r, w, distance, cos_l, true);
+                }
+                direction = w;
+                t_max = distance - 2.0f * kPushOff;
+                contribution = Mul(MatEmission(s, p.block), scale);
+            }
+        }
+    }
+    const float w_lamp = Luminance(contribution) * std::fmax(0.05f, 1.0f);
+    const float w_sum = 0.0f + w_lamp;
+    V radiance = Mk(0, 0, 0);
+    if (w_sum > 0.0f)
+    {
+        c.shadow += 1;
+        if (t_max > 0.0f)
+        {
+            const WorldHit h = Trace(s, x_r, direction, t_max, c);
+            if (h.attrib == 0u)
+            {
+                radiance = Mul(contribution, w_sum / w_lamp);
+            }
+        }
+    }
+    return radiance;
+}
+float LampHitWeight(const CxScene& s, const CxLightTreeView& L, uint32_t segment_light, const WorldHit& hit, V origin,
+                    V direction, V segment_normal)
+{
+    if (segment_light == 0u || hit.t < 8.0f)
+    {
+        return 1.0f;
+    }
+    const V n = FaceDirection(LowBit(hit.attrib & 0x3Fu));
+    const V hp = Add(origin, Mul(direction, hit.t));
+    const V inside = Sub(hp, Mul(n, 0.25f));
+    const int32_t voxel[3] = {int32_t(std::floor(inside.x)), int32_t(std::floor(inside.y)), int32_t(std::floor(inside.z))};
+    bool found;
+    const float p_area = LightFacePdf(s, L, origin, segment_normal, hit.record, voxel, LowBit(hit.attrib & 0x3Fu),
+                                      (hit.attrib >> 16) & 0xFFFFu, found);
+    if (!found)
+    {
+        return 1.0f;
+    }
+    if ((segment_light & 2u) == 0u)
+    {
+        return 0.0f;
+    }
+    return MisWeight(p_area, segment_normal, direction, hit.t, -Dot(n, direction), false);
+}
+
+struct DV
+{
+    double x = 0, y = 0, z = 0;
+};
+void AddD(DV& a, V b)
+{
+    a.x += double(b.x);
+    a.y += double(b.y);
+    a.z += double(b.z);
+}
+
+// One pixel sample; radiance accumulated in float64 (the float32 terms are the kernel's).
+void TracePixel(const CxScene& s, const CxLightTreeView& L, const CxCamera& cam, uint32_t bounces, uint32_t seed,
+                uint32_t px, uint32_t py, DV& radiance, V& albedo, V& emission, float& t_hit, uint32_t& attrib,
+                Counters& c)
+{
+    const float sx = float(px) + 0.5f;
+    const float sy = float(py) + 0.5f;
+    const float ndc_x = (sx / float(cam.width)) * 2.0f - 1.0f;
+    const float ndc_y = 1.0f - (sy / float(cam.height)) * 2.0f;
+    const V right = Mk(cam.right[0], cam.right[1], cam.right[2]);
+    const V up = Mk(cam.up[0], cam.up[1], cam.up[2]);
+    const V fwd = Mk(cam.forward[0], cam.forward[1], cam.forward[2]);
+    const V dir = SafeNormalize(Add(Add(Mul(right, ndc_x * cam.tan_half_fov_y * cam.aspect), Mul(up, ndc_y * cam.tan_half_fov_y)), fwd));
+    const V origin = Mk(cam.position[0], cam.position[1], cam.position[2]);
+    albedo = Mk(0, 0, 0);
+    emission = Mk(0, 0, 0);
+    t_hit = 0.0f;
+    attrib = 0u;
+    c.rays += 1;
+    const WorldHit primary = Trace(s, origin, dir, kRayTMax, c);
+    if (primary.attrib == 0u)
+    {
+        return;
+    }
+    t_hit = primary.t;
+    attrib = primary.attrib;
+    uint32_t rng = InitRandom(px, py, seed);
+    uint32_t light_rng = InitRandom(px, py, seed ^ 0x7F4A7C15u);
+    const uint32_t mis_bit = L.mis != 0u ? 2u : 0u;
+    uint32_t segment_light = 0u;
+    const V primary_normal = FaceDirection(LowBit(primary.attrib & 0x3Fu));
+    const Shade ps = ShadeHit(s, (primary.attrib >> 16) & 0xFFFFu, Add(origin, Mul(dir, primary.t)), primary_normal, rng);
+    albedo = ps.albedo;
+    emission = ps.emission;
+    V throughput = Mk(1, 1, 1);
+    V segment_normal = primary_normal;
+    if (bounces >= 1u)
+    {
+        AddD(radiance, MulV(NextEvent(s, L, ps.origin, primary_normal, light_rng, c), throughput));
+        segment_light = 1u | mis_bit;
+    }
+    V bo = ps.origin;
+    V bd = ps.direction;
+    uint32_t bounce_index = 1;
+    while (bounce_index <= bounces)
+    {
+        c.rays += 1;
+        const WorldHit hit = Trace(s, bo, bd, kRayTMax, c);
+        if (hit.attrib == 0u)
+        {
+            break;
+        }
+        const V vn = FaceDirection(LowBit(hit.attrib & 0x3Fu));
+        const Shade sh = ShadeHit(s, (hit.attrib >> 16) & 0xFFFFu, Add(bo, Mul(bd, hit.t)), vn, rng);
+        bounce_index += 1;
+        float ew = 1.0f;
+        if (segment_light != 0u && (sh.emission.x > 0.0f || sh.emission.y > 0.0f || sh.emission.z > 0.0f))
+        {
+            ew = LampHitWeight(s, L, segment_light, hit, bo, bd, segment_normal);
+        }
+        AddD(radiance, MulV(sh.emission, Mul(throughput, ew)));
+        throughput = MulV(throughput, Mk(sh.albedo.x / 1.0f, sh.albedo.y / 1.0f, sh.albedo.z / 1.0f));
+        segment_light = 0u;
+        if (bounce_index <= bounces)
+        {
+            segment_light = 1u | mis_bit;
+            AddD(radiance, MulV(NextEvent(s, L, sh.origin, vn, light_rng, c), Mul(throughput, 1.0f)));
+        }
+        const float cp = std::fmax(throughput.x, std::fmax(throughput.y, throughput.z));
+        if (cp <= 0.0f || cp < RandomFloat01(rng))
+        {
+            break;
+        }
+        throughput = Mul(throughput, 1.0f / cp);
+        bo = sh.origin;
+        bd = sh.direction;
+        segment_normal = vn;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Record words 0/1 (bounds), micro occupancy, bitmaps (caelix_finalize.hlsl's opaque subset).
+void oracle_scene_finalize(CxScene* s)
+{
+    const uint64_t cells = uint64_t(s->grid_dim) * s->grid_dim * s->grid_dim;
+    std::memset(s->brick_bits, 0, ((cells + 31) / 32) * 4);
+    std::memset(s->super_bits, 0, ((cells / 512 + 31) / 32) * 4);
+    uint64_t* occ_all = static_cast<uint64_t*>(s->occupancy);
+    for (uint32_t b = 0; b < s->bricks; ++b)
+    {
+        uint64_t occ[8] = {};
+        uint32_t lo[3] = {7, 7, 7}, hi[3] = {0, 0, 0};
+        bool any = false;
+        for (uint32_t v = 0; v < 512; ++v)
+        {
+            if (s->payload[uint64_t(b) * 512 + v] == 0)
+            {
+                continue;
+            }
+            const uint32_t x = v & 7u, y = (v >> 3) & 7u, z = v >> 6;
+            occ[(x >> 2) | ((y >> 2) << 1) | ((z >> 2) << 2)] |= 1ull << ((x & 3u) | ((y & 3u) << 2) | ((z & 3u) << 4));
+            any = true;
+            lo[0] = std::min(lo[0], x);
+            lo[1] = std::min(lo[1], y);
+            lo[2] = std::min(lo[2], z);
+            hi[0] = std::max(hi[0], x);
+            hi[1] = std::max(hi[1], y);
+            hi[2] = std::max(hi[2], z);
+        }
+        uint32_t coarse = 0;
+        for (int c = 0; c < 8; ++c)
+        {
+            occ_all[uint64_t(b) * 8 + c] = occ[c];
+            coarse |= occ[c] != 0 ? (1u << c) : 0u;
+        }
+        const int32_t* key = s->keys + uint64_t(b) * 3;
+        const uint32_t mask = (1u << s->group_shift) - 1u;
+        const uint32_t pos = (uint32_t(key[0]) & mask) | ((uint32_t(key[1]) & mask) << s->group_shift) |
+                             ((uint32_t(key[2]) & mask) << (2 * s->group_shift));
+        s->info[2 * uint64_t(b)] = (pos & kRecordPosMask) | (coarse << kRecordCoarseShift);
+        s->info[2 * uint64_t(b) + 1] =
+            any ? (lo[0] | (lo[1] << 3) | (lo[2] << 6) | (hi[0] << 9) | (hi[1] << 12) | (hi[2] << 15)) : 0u;
+        const int32_t g[3] = {key[0] - s->grid_min[0], key[1] - s->grid_min[1], key[2] - s->grid_min[2]};
+        if (any && uint32_t(g[0]) < s->grid_dim && uint32_t(g[1]) < s->grid_dim && uint32_t(g[2]) < s->grid_dim)
+        {
+            const uint64_t i = (uint64_t(g[2]) * s->grid_dim + g[1]) * s->grid_dim + g[0];
+            s->brick_bits[i >> 5] |= 1u << (i & 31u);
+            const uint32_t sd = s->grid_dim >> 3;
+            const uint64_t j = (uint64_t(g[2] >> 3) * sd + (g[1] >> 3)) * sd + (g[0] >> 3);
+            s->super_bits[j >> 5] |= 1u << (j & 31u);
+        }
+    }
+}
+
+// The light pass: word 1 light face counts, records, leaves, ordinals. Returns the light bricks.
+uint32_t oracle_scene_light_records(CxScene* s, uint32_t* heap, uint32_t heap_units, uint32_t* record_offset,
+                                    uint32_t* ordinal_of_brick, LightBvhLeaf* leaves, uint32_t leaf_capacity,
+                                    uint32_t* totals)
+{
+    auto block_at = [&](uint32_t record, int x, int y, int z) -> uint32_t {
+        return s->payload[uint64_t(record) * 512 + x + 8 * y + 64 * z];
+    };
+    auto open_faces = [&](uint32_t record, uint32_t v, uint32_t& block) -> uint32_t {
+        block = s->payload[uint64_t(record) * 512 + v];
+        if (block == 0u || ((s->light_codes[block >> 5] >> (block & 31u)) & 1u) == 0u)
+        {
+            return 0u;
+        }
+        const int p0[3] = {int(v & 7u), int((v >> 3) & 7u), int(v >> 6)};
+        uint32_t open = 0;
+        for (uint32_t f = 0; f < 6; ++f)
+        {
+            int p[3] = {p0[0], p0[1], p0[2]};
+            const int axis = int(f >> 1);
+            const int sign = (f & 1u) ? -1 : 1;
+            p[axis] += sign;
+            uint32_t nb;
+            if (p[axis] >= 0 && p[axis] < 8)
+            {
+                nb = block_at(record, p[0], p[1], p[2]);
+            }
+            else
+            {
+                int32_t k[3] = {s->keys[3 * uint64_t(record)], s->keys[3 * uint64_t(record) + 1], s->keys[3 * uint64_t(record) + 2]};
+                k[axis] += sign;
+                p[axis] = (p[axis] + 8) & 7;
+                uint32_t probes;
+                const uint32_t other = HashGet(*s, k[0], k[1], k[2], probes);
+                nb = other == kNoneU ? 0u : block_at(other, p[0], p[1], p[2]);
+            }
+            if (!Opaque(nb, s->encoding))
+            {
+                open |= 1u << f;
+            }
+        }
+        return open;
+    };
+    uint32_t ordinal = 0;
+    uint32_t offset = 0;
+    uint32_t faces_total = 0;
+    totals[3] = 0;
+    for (uint32_t b = 0; b < s->bricks; ++b)
+    {
+        uint32_t opens[512];
+        uint32_t blocks[512];
+        uint32_t count = 0;
+        for (uint32_t v = 0; v < 512; ++v)
+        {
+            opens[v] = open_faces(b, v, blocks[v]);
+            count += uint32_t(__builtin_popcount(opens[v]));
+        }
+        count = std::min(count, 0xFFFu);
+        s->info[2 * uint64_t(b) + 1] = (s->info[2 * uint64_t(b) + 1] & 0x3FFFFu) | (count << 18);
+        if (count == 0)
+        {
+            ordinal_of_brick[b] = kNoneU;
+            continue;
+        }
+        const uint32_t units = (48 + count * 4 + 15) / 16;
+        if (uint64_t(offset) + units > heap_units)
+        {
+            ordinal_of_brick[b] = kNoneU;
+            totals[3] = 1;
+            offset += units;
+            ordinal += 1;
+            continue;
+        }
+        uint32_t* rec = heap + uint64_t(offset) * 4;
+        uint32_t family_count[6] = {};
+        for (uint32_t v = 0; v < 512; ++v)
+        {
+            for (uint32_t f = 0; f < 6; ++f)
+            {
+                family_count[f] += (opens[v] >> f) & 1u;
+            }
+        }
+        uint32_t base[6];
+        uint32_t written = 0;
+        for (uint32_t f = 0; f < 6; ++f)
+        {
+            base[f] = written;
+            written += family_count[f];
+        }
+        // Face sums: per run of 8 consecutive voxels, then the 64 runs in order.
+        float face[6] = {};
+        uint32_t cursor[6] = {};
+        uint32_t lo[3] = {8, 8, 8}, hi[3] = {0, 0, 0};
+        for (uint32_t run = 0; run < 64; ++run)
+        {
+            float sums[6] = {};
+            for (uint32_t i = 0; i < 8; ++i)
+            {
+                const uint32_t v = run * 8 + i;
+                if (opens[v] == 0)
+                {
+                    continue;
+                }
+                const float* e = s->materials + uint64_t(blocks[v]) * 8 + 4;
+                const float weight = e[0] * 0.2126f + e[1] * 0.7152f + e[2] * 0.0722f;
+                for (uint32_t f = 0; f < 6; ++f)
+                {
+                    if ((opens[v] >> f) & 1u)
+                    {
+                        rec[12 + base[f] + cursor[f]++] = v | (blocks[v] << 16);
+                        sums[f] += weight;
+                    }
+                }
+                const uint32_t x = v & 7u, y = (v >> 3) & 7u, z = v >> 6;
+                lo[0] = std::min(lo[0], x);
+                lo[1] = std::min(lo[1], y);
+                lo[2] = std::min(lo[2], z);
+                hi[0] = std::max(hi[0], x + 1);
+                hi[1] = std::max(hi[1], y + 1);
+                hi[2] = std::max(hi[2], z + 1);
+            }
+            for (uint32_t f = 0; f < 6; ++f)
+            {
+                face[f] += sums[f];
+            }
+        }
+        const int32_t* key = s->keys + uint64_t(b) * 3;
+        const int32_t g = int32_t(s->group_shift);
+        const uint32_t mask = (1u << s->group_shift) - 1u;
+        const uint32_t bx = uint32_t(key[0]) & mask, by = uint32_t(key[1]) & mask, bz = uint32_t(key[2]) & mask;
+        std::memcpy(rec, face, 24);
+        rec[6] = (bx * 8 + lo[0]) | ((by * 8 + lo[1]) << 10) | ((bz * 8 + lo[2]) << 20);
+        rec[7] = (bx * 8 + hi[0]) | ((by * 8 + hi[1]) << 10) | ((bz * 8 + hi[2]) << 20);
+        rec[8] = bx | (by << s->group_shift) | (bz << (2 * s->group_shift));
+        rec[9] = written;
+        rec[10] = family_count[0] | (family_count[1] << 10) | (family_count[2] << 20);
+        rec[11] = family_count[3] | (family_count[4] << 10) | (family_count[5] << 20);
+        for (uint32_t w = 12 + written; w < ((12 + written + 3) / 4) * 4; ++w)
+        {
+            rec[w] = 0u;
+        }
+        record_offset[ordinal] = offset;
+        ordinal_of_brick[b] = ordinal;
+        faces_total += written;
+        if (ordinal < leaf_capacity)
+        {
+            LightBvhLeaf& leaf = leaves[ordinal];
+            std::memcpy(leaf.summary.face, face, 24);
+            const float o[3] = {float((key[0] >> g) << g) * 8.0f, float((key[1] >> g) << g) * 8.0f, float((key[2] >> g) << g) * 8.0f};
+            const uint32_t bb[3] = {bx, by, bz};
+            for (int a = 0; a < 3; ++a)
+            {
+                leaf.summary.lo[a] = o[a] + float(bb[a] * 8 + lo[a]);
+                leaf.summary.hi[a] = o[a] + float(bb[a] * 8 + hi[a]);
+            }
+            leaf.group = b;
+            leaf.entry = ordinal;
+            leaf.pad = 0;
+            leaf.parent = kNoneU;
+        }
+        offset += units;
+        ordinal += 1;
+    }
+    totals[0] = ordinal;
+    totals[1] = offset;
+    totals[2] = faces_total;
+    return ordinal;
+}
+
+void oracle_trace_rays(const CxScene* s, const float* rays, uint32_t count, uint32_t* hits)
+{
+    for (uint32_t i = 0; i < count; ++i)
+    {
+        const float* r = rays + uint64_t(i) * 7;
+        const WorldHit h = TraceWorld(*s, Mk(r[0], r[1], r[2]), Mk(r[3], r[4], r[5]), r[6]);
+        uint32_t* o = hits + uint64_t(i) * 9;
+        std::memcpy(&o[0], &h.t, 4);
+        o[1] = h.attrib;
+        o[2] = h.record;
+        o[3] = uint32_t(h.brick[0]);
+        o[4] = uint32_t(h.brick[1]);
+        o[5] = uint32_t(h.brick[2]);
+        o[6] = h.cells;
+        o[7] = h.probes;
+        o[8] = h.steps;
+    }
+}
+
+void oracle_light_sample(const CxScene* s, const CxLightTreeView* L, const float* receivers, const uint32_t* seeds,
+                         uint32_t count, float* samples, uint32_t* ok)
+{
+    for (uint32_t i = 0; i < count; ++i)
+    {
+        const float* r = receivers + uint64_t(i) * 6;
+        uint32_t rng = seeds[i];
+        LightPoint p;
+        const bool drawn = LightTreeSample(*s, *L, Mk(r[0], r[1], r[2]), Mk(r[3], r[4], r[5]), rng, p);
+        float* o = samples + uint64_t(i) * 9;
+        o[0] = p.position.x;
+        o[1] = p.position.y;
+        o[2] = p.position.z;
+        o[3] = p.normal.x;
+        o[4] = p.normal.y;
+        o[5] = p.normal.z;
+        o[6] = p.pdf;
+        std::memcpy(&o[7], &p.block, 4);
+        std::memcpy(&o[8], &p.face, 4);
+        ok[i] = drawn ? 1u : 0u;
+    }
+}
+
+void oracle_light_face_pdf(const CxScene* s, const CxLightTreeView* L, const uint32_t* queries, uint32_t count, float* pdf)
+{
+    for (uint32_t i = 0; i < count; ++i)
+    {
+        const uint32_t* q = queries + uint64_t(i) * 12;
+        float f[6];
+        std::memcpy(f, q, 24);
+        const int32_t voxel[3] = {int32_t(q[7]), int32_t(q[8]), int32_t(q[9])};
+        bool found;
+        pdf[i] = LightFacePdf(*s, *L, Mk(f[0], f[1], f[2]), Mk(f[3], f[4], f[5]), q[6], voxel, q[10], q[11], found);
+    }
+}
+
+// A frame over pixel rows [row_begin, row_end): color (rgb + t, float64 rgb) and radiance.
+void oracle_trace_frame(const CxScene* s, const CxLightTreeView* L, const CxCamera* cam, const CxTraceParams* prm,
+                        uint32_t row_begin, uint32_t row_end, double* color, double* radiance_out, uint32_t* attrib_out,
+                        uint64_t* counters)
+{
+    uint64_t rays = 0, shadow = 0, cells = 0, probes = 0, steps = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : rays, shadow, cells, probes, steps)
+    for (int64_t py = row_begin; py < int64_t(row_end); ++py)
+    {
+        Counters local;
+        for (uint32_t px = 0; px < cam->width; ++px)
+        {
+            DV col, rad;
+            float t = 0.0f;
+            uint32_t attrib = 0;
+            for (uint32_t sample = 0; sample < prm->spp; ++sample)
+            {
+                DV r;
+                V albedo, emission;
+                TracePixel(*s, *L, *cam, prm->bounces, prm->frame_seed + sample, px, uint32_t(py), r, albedo, emission, t,
+                           attrib, local);
+                col.x += double(emission.x) + double(albedo.x) * r.x;
+                col.y += double(emission.y) + double(albedo.y) * r.y;
+                col.z += double(emission.z) + double(albedo.z) * r.z;
+                rad.x += r.x;
+                rad.y += r.y;
+                rad.z += r.z;
+            }
+            const uint64_t i = uint64_t(py - row_begin) * cam->width + px;
+            color[4 * i] = col.x;
+            color[4 * i + 1] = col.y;
+            color[4 * i + 2] = col.z;
+            color[4 * i + 3] = double(t);
+            radiance_out[4 * i] = rad.x;
+            radiance_out[4 * i + 1] = rad.y;
+            radiance_out[4 * i + 2] = rad.z;
+            radiance_out[4 * i + 3] = 0.0;
+            attrib_out[i] = attrib;
+        }
+        rays += local.rays;
+        shadow += local.shadow;
+        cells += local.cells;
+        probes += local.probes;
+        steps += local.steps;
+    }
+    Counters c;
+    c.rays = rays;
+    c.shadow = shadow;
+    c.cells = cells;
+    c.probes = probes;
+    c.steps = steps;
+    counters[0] = c.rays;
+    counters[1] = c.shadow;
+    counters[2] = c.cells;
+    counters[3] = c.probes;
+    counters[4] = c.steps;
+}
+
+}  // extern "C"

@@ -1,0 +1,182 @@
+"""GPU parity of the trace side (libcaelix_amd through its C ABI) against the oracle on the same
+scenes: record words, bitmaps, light records and the light tree bit-exact; world DDA hits, light
+draws and face pdfs bit-exact; the path-traced frame's hit attributes, depths and ray/step counters
+exact and its radiance within 2e-5 relative of the oracle's float64 accumulation (the kernels add the
+same float32 terms in float32); the furnace closed form on the GPU path."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import trace_checks as tc
+from caelix_amd import scene
+
+pytestmark = pytest.mark.gpu
+
+SMALL = scene.small_scene()
+LOAD = 0.7
+
+
+def slot_count(data) -> int:
+    return int(np.ceil(len(data.keys) / LOAD))
+
+
+@pytest.fixture(scope="module")
+def pair(cx, oracle):
+    gpu = cx.TraceScene(SMALL, load_factor=LOAD)
+    assert gpu.hash.slot_count == slot_count(SMALL)
+    arrays = gpu.host_arrays()
+    ref = oracle.OracleScene.build(SMALL, slot_count(SMALL))
+    return gpu, arrays, ref
+
+
+def test_scene_records_bit_exact(pair):
+    gpu, got, ref = pair
+    want = ref.a
+    assert np.array_equal(got["slots"], want["slots"].reshape(-1))
+    assert got["hash_info"][0] == want["hash_info"][0]
+    for name in ("info", "occupancy", "brick_bits", "super_bits", "record_offset", "ordinal_of_brick"):
+        assert np.array_equal(got[name], want[name]), name
+    assert np.array_equal(got["totals"][:3], want["totals"][:3])
+    units = int(want["totals"][1])
+    assert np.array_equal(got["heap"][: units * 4], want["heap"][: units * 4])
+
+
+def test_light_tree_bit_exact(pair):
+    gpu, got, ref = pair
+    want = ref.a
+    assert np.array_equal(got["headers"], want["headers"])
+    node_count = int(want["headers"][:64].view(np.uint32)[14])
+    assert np.array_equal(got["nodes"][: node_count * 224], want["nodes"][: node_count * 224])
+    leaves = int(want["totals"][0])
+    assert np.array_equal(got["leaves"][: leaves * 64], want["leaves"][: leaves * 64])
+
+
+def _rays(solid, rng, count):
+    pts = []
+    while len(pts) < count:
+        p = rng.uniform(0.5, 127.5, size=3)
+        c = np.floor(p).astype(int)
+        if not solid[c[0], c[1], c[2]]:
+            pts.append(p)
+    d = rng.normal(size=(count, 3))
+    d[: count // 8] = np.eye(3)[rng.integers(0, 3, count // 8)] * rng.choice([-1, 1], size=(count // 8, 1))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    t_max = rng.choice([1e30, 5.0, 40.0], size=(count, 1))
+    return np.concatenate([np.array(pts), d, t_max], axis=1).astype(np.float32)
+
+
+def test_trace_rays_bit_exact(pair):
+    gpu, _, ref = pair
+    solid = tc.dense_blocks(SMALL) != 0
+    rays = _rays(solid, np.random.default_rng(3), 20000)
+    # rays that start outside the grid too
+    rays[:500, 0:3] = np.random.default_rng(4).uniform(-50, 180, size=(500, 3))
+    got = gpu.trace_rays(rays)
+    want = ref.trace_rays(rays)
+    assert (want[:, 1] != 0).mean() > 0.1
+    bad = np.nonzero((got != want).any(axis=1))[0]
+    assert len(bad) == 0, (bad[:5], got[bad[:3]], want[bad[:3]])
+
+
+def _receivers(solid, rng, count):
+    pts = []
+    while len(pts) < count:
+        p = rng.uniform(1.0, 127.0, size=3)
+        c = np.floor(p).astype(int)
+        if not solid[c[0], c[1], c[2]]:
+            pts.append(p)
+    n = rng.normal(size=(count, 3))
+    n[: count // 4] = 0.0
+    n[count // 4: count // 2] = [0.0, 1.0, 0.0]
+    n /= np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-30)
+    return np.concatenate([np.array(pts), n], axis=1).astype(np.float32)
+
+
+def test_light_sample_bit_exact(pair):
+    gpu, _, ref = pair
+    rng = np.random.default_rng(8)
+    recv = _receivers(tc.dense_blocks(SMALL) != 0, rng, 20000)
+    seeds = rng.integers(0, 2 ** 32, size=len(recv), dtype=np.uint64).astype(np.uint32)
+    got, ok = gpu.light_sample(recv, seeds)
+    want, ok_want = ref.light_sample(recv, seeds)
+    assert np.array_equal(ok, ok_want.astype(ok.dtype)) and ok.all()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_light_face_pdf_bit_exact(pair):
+    gpu, _, ref = pair
+    rng = np.random.default_rng(10)
+    recv = _receivers(tc.dense_blocks(SMALL) != 0, rng, 4)
+    open_ = tc.open_light_faces(SMALL)
+    b, v, f = np.nonzero(open_)
+    vox = SMALL.keys[b].astype(np.int64) * 8 + np.stack([v & 7, (v >> 3) & 7, v >> 6], axis=1)
+    for r in recv:
+        q = np.zeros((len(b), 12), np.uint32)
+        q[:, 0:6] = r.view(np.uint32)
+        q[:, 6] = b
+        q[:, 7:10] = vox.astype(np.int32).view(np.uint32)
+        q[:, 10] = f
+        q[:, 11] = SMALL.payload[b, v]
+        # plus queries that are not light faces (wrong face, non-light block): pdf 0
+        q[:50, 10] = (q[:50, 10] + 1) % 6
+        got = gpu.light_face_pdf(q)
+        want = ref.light_face_pdf(q)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+        assert abs(want.astype(np.float64).sum() - 1.0) < 0.05
+
+
+@pytest.mark.parametrize("bounces,spp,mis", [(0, 1, True), (1, 1, True), (2, 1, True), (2, 2, False), (4, 1, True)])
+def test_trace_frame_matches_oracle(cx, pair, bounces, spp, mis):
+    gpu, _, ref = pair
+    gpu.view.mis = 1 if mis else 0
+    ref.view.mis = 1 if mis else 0
+    try:
+        color, radiance, attrib, counters = gpu.trace_frame(SMALL.camera, bounces, 1234, spp)
+        w_color, w_rad, w_attrib, w_counters = ref.trace_frame(SMALL.camera, bounces, 1234, spp)
+    finally:
+        gpu.view.mis = 1
+        ref.view.mis = 1
+    color = color.cpu().numpy().reshape(-1, 4).astype(np.float64)
+    radiance = radiance.cpu().numpy().reshape(-1, 4).astype(np.float64)
+    attrib = attrib.cpu().numpy().view(np.uint32)
+    counters = counters.cpu().numpy().view(np.uint64)[:5]
+    assert np.array_equal(attrib, w_attrib)
+    assert np.array_equal(color[:, 3], w_color[:, 3])  # primary depth, float32 in both
+    assert np.array_equal(counters, w_counters), (counters, w_counters)
+    tol = 2e-5 * (1.0 + np.abs(w_rad[:, :3]))
+    assert np.all(np.abs(radiance[:, :3] - w_rad[:, :3]) <= tol)
+    tol = 2e-5 * (1.0 + np.abs(w_color[:, :3]))
+    assert np.all(np.abs(color[:, :3] - w_color[:, :3]) <= tol)
+
+
+@pytest.mark.parametrize("bounces", [1, 3])
+def test_trace_frame_furnace_closed_form(cx, oracle, bounces):
+    data = scene.furnace_box(width=128, height=128)
+    gpu = cx.TraceScene(data, load_factor=LOAD)
+    spp = 8
+    _, radiance, attrib, _ = gpu.trace_frame(data.camera, bounces, 99, spp)
+    r = radiance.cpu().numpy().reshape(-1, 4)[:, 0].astype(np.float64) / spp
+    assert np.all(attrib.cpu().numpy() != 0)
+    expected = (1 - 0.5 ** bounces) / (1 - 0.5)
+    stderr = r.std() / np.sqrt(len(r))
+    assert abs(r.mean() - expected) < 5 * stderr + 1e-4, (r.mean(), expected, stderr)
+
+
+def test_trace_frame_deterministic_and_seeded(pair):
+    gpu, _, _ = pair
+    a = gpu.trace_frame(SMALL.camera, 2, 5, 1)[1].cpu().numpy()
+    b = gpu.trace_frame(SMALL.camera, 2, 5, 1)[1].cpu().numpy()
+    c = gpu.trace_frame(SMALL.camera, 2, 6, 1)[1].cpu().numpy()
+    assert np.array_equal(a, b)
+    assert not np.array_equal(a, c)
+
+
+def test_trace_rejects_bad_arguments(cx, pair):
+    gpu, _, _ = pair
+    cam = dict(SMALL.camera)
+    cam["width"] = 0
+    with pytest.raises(RuntimeError):
+        gpu.trace_frame(cam, 1, 0, 1)
+    with pytest.raises(RuntimeError):
+        gpu.trace_frame(SMALL.camera, 1, 0, 0)

@@ -1,0 +1,136 @@
+"""Independent checks for the trace side (numpy, float64): a dense voxel world, a brute-force voxel
+walk, and recomputations of the finalize words and the brick light records straight from the
+block payloads. None of these share code with the oracle or the kernels."""
+from __future__ import annotations
+
+import numpy as np
+
+OPAQUE = 0x8000
+FACE_AXIS = np.array([0, 0, 1, 1, 2, 2])
+FACE_SIGN = np.array([1, -1, 1, -1, 1, -1])
+
+
+def dense_blocks(data) -> np.ndarray:
+    """blocks[x, y, z] (uint16) of the whole grid; bricks that are not resident are air."""
+    dim = data.grid_dim
+    world = np.zeros((dim * 8, dim * 8, dim * 8), np.uint16)
+    g = np.asarray(data.keys, np.int64) - np.asarray(data.grid_min, np.int64)[None, :]
+    v = np.arange(512)
+    lx, ly, lz = v & 7, (v >> 3) & 7, v >> 6
+    x = g[:, 0:1] * 8 + lx[None, :]
+    y = g[:, 1:2] * 8 + ly[None, :]
+    z = g[:, 2:3] * 8 + lz[None, :]
+    world[x, y, z] = data.payload
+    return world
+
+
+def is_light(blocks: np.ndarray, light_codes: np.ndarray) -> np.ndarray:
+    b = blocks.astype(np.int64)
+    return ((light_codes[b >> 5] >> (b & 31).astype(np.uint32)) & 1).astype(bool) & (b != 0)
+
+
+def walk_first_solid(solid: np.ndarray, o, d, max_steps: int = 4096):
+    """Amanatides-Woo in float64 from origin o along d through the unit voxel grid `solid`.
+    Returns (voxel or None, t_entry, entry_axis, tie) where tie is the smallest gap between two
+    consecutive plane crossings seen (a small gap means the float32 DDA may cut a corner)."""
+    n = np.array(solid.shape)
+    o = np.asarray(o, np.float64)
+    d = np.asarray(d, np.float64)
+    inv = np.where(np.abs(d) > 1e-30, 1.0 / np.where(d == 0, 1.0, d), np.inf)
+    t_enter, t_exit = 0.0, np.inf
+    for a in range(3):
+        if d[a] == 0.0:
+            if o[a] < 0 or o[a] >= n[a]:
+                return None, 0.0, -1, np.inf
+            continue
+        t0 = (0.0 - o[a]) * inv[a]
+        t1 = (n[a] - o[a]) * inv[a]
+        t_enter = max(t_enter, min(t0, t1))
+        t_exit = min(t_exit, max(t0, t1))
+    if t_enter > t_exit:
+        return None, 0.0, -1, np.inf
+    p = o + d * t_enter
+    cell = np.clip(np.floor(p).astype(np.int64), 0, n - 1)
+    step = np.where(d >= 0, 1, -1)
+    t_next = np.where(d != 0, ((cell + (step > 0)) - o) * inv, np.inf)
+    t_delta = np.abs(inv)
+    t = t_enter
+    axis = -1
+    tie = np.inf
+    for _ in range(max_steps):
+        if solid[cell[0], cell[1], cell[2]]:
+            return cell.copy(), t, axis, tie
+        order = np.argsort(t_next)
+        tie = min(tie, t_next[order[1]] - t_next[order[0]])
+        axis = int(order[0])
+        t = t_next[axis]
+        cell[axis] += step[axis]
+        if cell[axis] < 0 or cell[axis] >= n[axis]:
+            return None, t, axis, tie
+        t_next[axis] += t_delta[axis]
+    return None, t, axis, tie
+
+
+def finalize_expected(data):
+    """(info words [n, 2] with the light count left 0, occupancy [n, 8] uint64, brick_bits, super_bits)."""
+    p = np.asarray(data.payload)
+    n = len(p)
+    solid = p != 0
+    v = np.arange(512)
+    x, y, z = v & 7, (v >> 3) & 7, v >> 6
+    coarse_of = (x >> 2) | ((y >> 2) << 1) | ((z >> 2) << 2)
+    micro_of = (x & 3) | ((y & 3) << 2) | ((z & 3) << 4)
+    occ = np.zeros((n, 8), np.uint64)
+    for c in range(8):
+        sel = coarse_of == c
+        bits = (np.uint64(1) << micro_of[sel].astype(np.uint64))[None, :]
+        occ[:, c] = np.bitwise_or.reduce(np.where(solid[:, sel], bits, np.uint64(0)), axis=1)
+    coarse = np.zeros(n, np.uint32)
+    for c in range(8):
+        coarse |= (occ[:, c] != 0).astype(np.uint32) << np.uint32(c)
+    keys = np.asarray(data.keys, np.int64)
+    g = data.group_shift
+    mask = (1 << g) - 1
+    pos = (keys[:, 0] & mask) | ((keys[:, 1] & mask) << g) | ((keys[:, 2] & mask) << (2 * g))
+    w0 = (pos & 0xFFFF).astype(np.uint32) | (coarse << np.uint32(16))
+    any_ = solid.any(axis=1)
+    big = 99
+    lo = [np.where(solid, c[None, :], big).min(axis=1) for c in (x, y, z)]
+    hi = [np.where(solid, c[None, :], -1).max(axis=1) for c in (x, y, z)]
+    w1 = (lo[0] | (lo[1] << 3) | (lo[2] << 6) | (hi[0] << 9) | (hi[1] << 12) | (hi[2] << 15))
+    w1 = np.where(any_, w1, 0).astype(np.uint32)
+    dim = data.grid_dim
+    gk = keys - np.asarray(data.grid_min, np.int64)[None, :]
+    cells = dim ** 3
+    brick_bits = np.zeros((cells + 31) // 32, np.uint32)
+    super_bits = np.zeros((cells // 512 + 31) // 32 + 1, np.uint32)
+    for i in np.nonzero(any_)[0]:
+        c = (gk[i, 2] * dim + gk[i, 1]) * dim + gk[i, 0]
+        brick_bits[c >> 5] |= np.uint32(1) << np.uint32(c & 31)
+        sd = dim >> 3
+        s = ((gk[i, 2] >> 3) * sd + (gk[i, 1] >> 3)) * sd + (gk[i, 0] >> 3)
+        super_bits[s >> 5] |= np.uint32(1) << np.uint32(s & 31)
+    return np.stack([w0, w1], axis=1), occ, brick_bits, super_bits
+
+
+def open_light_faces(data) -> np.ndarray:
+    """open[n, 512, 6] bool: voxel v of brick i is a light and its face f borders a voxel that is not
+    opaque (air, or a brick that is not resident)."""
+    world = dense_blocks(data)
+    light = is_light(world, np.asarray(data.light_codes, np.uint32))
+    opaque = (world & OPAQUE) != 0
+    padded = np.zeros(tuple(s + 2 for s in opaque.shape), bool)
+    padded[1:-1, 1:-1, 1:-1] = opaque
+    g = np.asarray(data.keys, np.int64) - np.asarray(data.grid_min, np.int64)[None, :]
+    v = np.arange(512)
+    x = g[:, 0:1] * 8 + (v & 7)[None, :]
+    y = g[:, 1:2] * 8 + ((v >> 3) & 7)[None, :]
+    z = g[:, 2:3] * 8 + (v >> 6)[None, :]
+    out = np.zeros((len(g), 512, 6), bool)
+    lit = light[x, y, z]
+    for f in range(6):
+        dx = [0, 0, 0]
+        dx[FACE_AXIS[f]] = FACE_SIGN[f]
+        nb = padded[x + 1 + dx[0], y + 1 + dx[1], z + 1 + dx[2]]
+        out[:, :, f] = lit & ~nb
+    return out
