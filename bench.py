@@ -1,16 +1,23 @@
 """Benchmark of the Caelix hot path on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload lighttree]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload trace|lighttree]
 
-lighttree  full light-tree rebuild (bounds, Morton, 4 onesweep passes, Karras, aggregate, depth scans,
-           4-wide collapse, header) over 4,194,304 emissive brick leaves: half uniform in an 8192^3
-           box, half in 64 Gaussian clusters (sigma 32 voxels), emission log-uniform in [0.01, 100].
-           One step = one rebuild from leaves resident in HBM. value = Mprims/s (leaves per second).
+trace      (default) DDA + NEE at 1920 x 1080: 1,048,576 resident 8^3 bricks in a 4096^3 world (8-octave
+           value-noise terrain, 4 surface bricks per column), bulk hash table at load 0.7, 65,536
+           emissive bricks with emission log-uniform in [0.01, 100], 1 spp, 2 bounces, NEE through the
+           light tree at the primary and bounce-1 vertices, float32 throughout. One step = one frame
+           (the frame seed changes per step). value = Mrays/s: primary + bounce + shadow rays traced
+           per second. The light-tree rebuild below is measured in the same run as a second figure.
+lighttree  full light-tree rebuild over 4,194,304 emissive brick leaves (half uniform in an 8192^3
+           box, half in 64 Gaussian clusters, sigma 32 voxels, emission log-uniform in [0.01, 100]).
+           One step = one rebuild from leaves resident in HBM. value = Mprims/s.
 
-Prints one JSON line. The roofline object is for the dominant kernel of a step (largest share of
-the step's time), from HIP events around each launch on the stream it runs on; its algorithmic bytes
-per leaf are listed in STAGE_BYTES. cpu_baseline times the oracle (the sequential C++ restatement of
-the reference's kernels) on a bounded sample of the same workload on the host, single-threaded.
+Prints one JSON line. `roofline` is for the dominant kernel of a step, timed with HIP events around
+its launches on the stream it runs on. Its algorithmic bytes: for the trace, the demand bytes per ray
+(every global load the kernels make, at its size) counted by the oracle's instrumented traversal on a
+band of the same frame's pixels, times the rays of the frame; for the light tree, the per-leaf bytes
+of STAGE_BYTES. `cpu_baseline` times the oracle (the CPU restatement of the same algorithms) on a
+bounded sample of the same workload on the host.
 """
 from __future__ import annotations
 
@@ -29,12 +36,11 @@ for _p in (REPO, REPO / "caelix-renderer_amd"):
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per leaf of each light-tree stage (reads + writes the stage must make):
-#   bounds 24 (box lo/hi read), morton 32 (centre read 12 + key/value write 8 + counter/cover clears 8
-#   + ...), sort pass 16 (key/value read + write), tree level 1 = Karras 20 + aggregate 148 (the
-#   fused bottom-up build does both; the upper levels finish the few nodes that span level-1 blocks),
-#   gather (leaf summaries into sorted order: value 4 + leaf 48 + write 48 = 100, outside the 520
-#   canonical bytes), cover+flags scans 16, wide 212 (224-byte nodes for n/2 nodes = 112, slot summaries 96, leaf
-#   parent 4).
+#   bounds 24 (box lo/hi read), morton 32 (centre read 12 + key/value write 8 + counter/cover clears
+#   8 + partials), sort pass 16 (key/value read + write), tree level 1 = Karras 20 + aggregate 148
+#   (the fused bottom-up build does both; the upper levels finish the few nodes that span level-1
+#   blocks), cover + flags scans 16, wide 212 (224-byte nodes for n/2 nodes = 112, slot summaries
+#   96, leaf parent 4).
 STAGE_BYTES = {
     "lt_bounds": 24.0,
     "lt_morton": 32.0,
@@ -46,9 +52,167 @@ STAGE_BYTES = {
 }
 BUILD_BYTES_PER_LEAF = 520.0
 
+TRACE_BOUNCES = 2
+TRACE_SPP = 1
 
-def lighttree_bench(args) -> dict:
+
+def _profile(run, steps: int) -> dict:
+    """{stage: (total ms, launches)} over `steps` calls of run()."""
+    import torch
+
+    import caelix_amd
+
+    caelix_amd.profile_enable(True)
+    caelix_amd.profile_reset()
+    for i in range(steps):
+        run(i)
+    torch.cuda.synchronize()
+    stages = caelix_amd.profile_read()
+    caelix_amd.profile_enable(False)
+    return stages
+
+
+def _time_steps(run, steps: int, warmup: int, stream) -> float:
+    import torch
+
+    for i in range(warmup):
+        run(i)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for i in range(steps):
+        run(warmup + i)
+    end.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / steps
+
+
+# ------------------------------------------------------------------------------------------------
+# DDA + NEE at 1080p
+
+
+def trace_bench(args) -> dict:
     import numpy as np
+    import torch
+
+    import caelix_amd
+    from caelix_amd import scene
+
+    t0 = time.perf_counter()
+    data = scene.hd_scene()
+    gen_s = time.perf_counter() - t0
+    ts = caelix_amd.TraceScene(data, load_factor=0.7)
+    cam = data.camera
+    w, h = cam["width"], cam["height"]
+    stream = torch.cuda.current_stream()
+    color = torch.empty(w * h * 4, dtype=torch.float32, device="cuda")
+    radiance = torch.empty(w * h * 4, dtype=torch.float32, device="cuda")
+    attrib = torch.empty(w * h, dtype=torch.int32, device="cuda")
+    counters = torch.zeros(8, dtype=torch.int64, device="cuda")
+
+    def frame(i):
+        ts.trace_frame(cam, TRACE_BOUNCES, 1000 + i, TRACE_SPP, color, radiance, attrib, counters, stream)
+
+    for i in range(args.warmup):
+        frame(i)
+    torch.cuda.synchronize()
+    before = counters.cpu().numpy().copy()
+    ms = _time_steps(lambda i: frame(args.warmup + i), args.steps, 0, stream)
+    after = counters.cpu().numpy()
+    per_frame = (after - before) / args.steps
+    rays_per_frame = float(per_frame[0] + per_frame[1])
+    value = rays_per_frame / (ms * 1e-3) / 1e6
+
+    stages = _profile(frame, 3)
+    kernel_ms = stages["trace_frame"][0] / stages["trace_frame"][1]
+
+    # Demand bytes per ray and a parity check, from the oracle on a band of rows of the last frame.
+    import oracle
+
+    seed = 1000 + args.warmup + args.steps - 1
+    frame(args.warmup + args.steps - 1)  # the last timed frame again, into the output buffers
+    torch.cuda.synchronize()
+    o = oracle.OracleScene.from_gpu_arrays(ts.host_arrays(), data)
+    band = (h // 2 - 24, h // 2 + 24)
+    o_color, o_rad, o_attrib, o_counters = o.trace_frame(cam, TRACE_BOUNCES, seed, TRACE_SPP, rows=band)
+    lo, hi = band[0] * w, band[1] * w
+    g_attrib = attrib.cpu().numpy().view(np.uint32)[lo:hi]
+    g_rad = radiance.cpu().numpy().reshape(-1, 4)[lo:hi, :3].astype(np.float64)
+    rel = np.abs(g_rad - o_rad[:, :3]) / (1.0 + np.abs(o_rad[:, :3]))
+    band_rays = float(o_counters[0] + o_counters[1])
+    bytes_per_ray = float(o_counters[5]) / band_rays
+    frame_bytes = bytes_per_ray * rays_per_frame
+    achieved = frame_bytes / (kernel_ms * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "kernel": "trace_frame",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "kernel_ms": round(kernel_ms, 4),
+        "demand_bytes_per_ray": round(bytes_per_ray, 1),
+    }
+    result = {
+        "metric": "Mrays/s DDA+NEE @1080p (primary + bounce + shadow rays)",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "ms_per_step": round(ms, 4),
+        "dtype": "f32",
+        "config": {
+            "workload": "DDA+NEE 1920x1080, 1 spp, 2 bounces; 1,048,576 bricks in 4096^3 (8-octave value-noise "
+                        "terrain), hash load 0.7, 65,536 emissive bricks, emission log-uniform [0.01, 100]",
+            "bricks": int(len(data.keys)),
+            "light_bricks": ts.light_bricks,
+            "light_faces": ts.light_faces,
+            "slots": ts.hash.slot_count,
+            "rays_per_frame": int(rays_per_frame),
+            "shadow_rays_per_frame": int(per_frame[1]),
+            "scene_generation_s": round(gen_s, 1),
+        },
+        "roofline": roofline,
+        "parity": {
+            "rows": list(band),
+            "attrib_exact": bool(np.array_equal(g_attrib, o_attrib)),
+            "max_rel_radiance": float(rel.max()) if len(rel) else 0.0,
+        },
+        "stages": {k: {"avg_ms": round(v[0] / v[1], 5), "launches_per_step": round(v[1] / 3, 2)}
+                   for k, v in stages.items()},
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = trace_cpu_baseline(o, cam, seed)
+    return result
+
+
+def trace_cpu_baseline(o, cam, seed) -> dict:
+    """The oracle's path tracer (OpenMP over rows) on bands of 64 rows of the same frame, repeated
+    over the next bands until at least 10 s have passed."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    rays = 0
+    t0 = time.perf_counter()
+    rows = 0
+    r0 = 0
+    while True:
+        band = (r0, min(r0 + 64, cam["height"]))
+        _, _, _, c = o.trace_frame(cam, TRACE_BOUNCES, seed, TRACE_SPP, rows=band)
+        rays += int(c[0] + c[1])
+        rows += band[1] - band[0]
+        r0 = band[1] % cam["height"]
+        if time.perf_counter() - t0 >= 10.0:
+            break
+    seconds = time.perf_counter() - t0
+    return {"value": round(rays / seconds / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"oracle path tracer on {rows} rows of the same 1080p frame ({rays} rays), "
+                      f"{threads} OpenMP threads"}
+
+
+# ------------------------------------------------------------------------------------------------
+# Light-tree rebuild
+
+
+def lighttree_bench(args, with_cpu: bool) -> dict:
     import torch
 
     import caelix_amd
@@ -59,60 +223,36 @@ def lighttree_bench(args) -> dict:
     builder = caelix_amd.LightBvhBuilder(n)
     builder.upload(leaves)
     stream = torch.cuda.current_stream()
-    for _ in range(args.warmup):
+
+    def build(_):
         builder.build(stream)
-    torch.cuda.synchronize()
-    start = torch.cuda.Event(enable_timing=True)
-    end = torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    for _ in range(args.steps):
-        builder.build(stream)
-    end.record(stream)
-    torch.cuda.synchronize()
-    ms = start.elapsed_time(end) / args.steps
+
+    ms = _time_steps(build, args.steps, args.warmup, stream)
     out = builder.read()
     assert int(out["header"]["leaf_count"]) == n and int(out["state"]["node_overflow"]) == 0
-
-    # Per-kernel durations (HIP events around each launch), after the timed region.
-    caelix_amd.profile_enable(True)
-    caelix_amd.profile_reset()
     profile_steps = max(3, min(args.steps, 10))
-    for _ in range(profile_steps):
-        builder.build(stream)
-    torch.cuda.synchronize()
-    stages = caelix_amd.profile_read()
-    caelix_amd.profile_enable(False)
-    per_stage = {name: {"avg_ms": total / launches, "launches_per_step": launches / profile_steps}
+    stages = _profile(build, profile_steps)
+    per_stage = {name: {"avg_ms": round(total / launches, 5), "launches_per_step": launches / profile_steps}
                  for name, (total, launches) in stages.items()}
     dominant = max(stages, key=lambda k: stages[k][0])
     avg_ms = stages[dominant][0] / stages[dominant][1]
     achieved = STAGE_BYTES.get(dominant, 0.0) * n / (avg_ms * 1e-3) / 1e9
-    roofline = {
-        "bound": "hbm",
-        "kernel": dominant,
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
-    }
     pipeline_gbs = BUILD_BYTES_PER_LEAF * n / (ms * 1e-3) / 1e9
-    value = n / (ms * 1e-3) / 1e6
     result = {
         "metric": "light-tree build Mprims/s (4,194,304 emissive leaves, full rebuild)",
-        "value": round(value, 2),
+        "value": round(n / (ms * 1e-3) / 1e6, 2),
         "unit": "Mprims/s",
         "ms_per_step": round(ms, 4),
         "dtype": "f32+u32",
         "config": {"workload": "light-tree rebuild, 4,194,304 leaves (half uniform in 8192^3, half in 64 "
                                "clusters sigma 32), u32 Morton keys", "leaves": n},
-        "roofline": roofline,
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "pipeline_roofline": {"bytes_per_leaf": BUILD_BYTES_PER_LEAF, "achieved": round(pipeline_gbs, 1),
-                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                              "frac": round(pipeline_gbs / HBM_PEAK_GBS, 4)},
-        "stages": {k: {kk: round(vv, 5) for kk, vv in v.items()} for k, v in per_stage.items()},
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(pipeline_gbs / HBM_PEAK_GBS, 4)},
+        "stages": per_stage,
     }
-    if not args.no_cpu_baseline:
+    if with_cpu:
         result["cpu_baseline"] = lighttree_cpu_baseline(leaves)
     return result
 
@@ -148,17 +288,25 @@ def main() -> int:
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=20)
     parser.add_argument("--warmup", type=int, default=3)
-    parser.add_argument("--workload", default="lighttree", choices=["lighttree"])
+    parser.add_argument("--workload", default="trace", choices=["trace", "lighttree"])
     parser.add_argument("--no-cpu-baseline", action="store_true")
+    parser.add_argument("--no-lighttree", action="store_true", help="trace only, without the light-tree figure")
     args = parser.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the hot path does not span GPUs (see DESIGN.md on sharding); "
-              "run it with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the hot path runs on one GPU and this build has no multi-GPU "
+              "path (the design for sharding it is documented); run it with --gpus 1", file=sys.stderr)
         return 2
     import torch
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    result = lighttree_bench(args)
+    if args.workload == "trace":
+        result = trace_bench(args)
+        if not args.no_lighttree:
+            lt = lighttree_bench(args, with_cpu=False)
+            result["lighttree"] = {k: lt[k] for k in ("value", "unit", "ms_per_step", "roofline",
+                                                      "pipeline_roofline", "config")}
+    else:
+        result = lighttree_bench(args, with_cpu=not args.no_cpu_baseline)
     line = {
         "metric": result["metric"],
         "value": result["value"],
@@ -175,7 +323,7 @@ def main() -> int:
         "config": result["config"],
         "roofline": result["roofline"],
     }
-    for key in ("cpu_baseline", "pipeline_roofline", "stages"):
+    for key in ("cpu_baseline", "parity", "lighttree", "pipeline_roofline", "stages"):
         if key in result:
             line[key] = result[key]
     print(json.dumps(line))

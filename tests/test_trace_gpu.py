@@ -143,7 +143,7 @@ def test_trace_frame_matches_oracle(cx, pair, bounces, spp, mis):
     counters = counters.cpu().numpy().view(np.uint64)[:5]
     assert np.array_equal(attrib, w_attrib)
     assert np.array_equal(color[:, 3], w_color[:, 3])  # primary depth, float32 in both
-    assert np.array_equal(counters, w_counters), (counters, w_counters)
+    assert np.array_equal(counters, w_counters[:5]), (counters, w_counters)
     tol = 2e-5 * (1.0 + np.abs(w_rad[:, :3]))
     assert np.all(np.abs(radiance[:, :3] - w_rad[:, :3]) <= tol)
     tol = 2e-5 * (1.0 + np.abs(w_color[:, :3]))
