@@ -1,6 +1,6 @@
 """Benchmark of the Caelix hot path on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload trace|lighttree]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload trace|lighttree|copy]
 
 trace      (default) DDA + NEE at 1920 x 1080: 1,048,576 resident 8^3 bricks in a 4096^3 world (8-octave
            value-noise terrain, 4 surface bricks per column), bulk hash table at load 0.7, 65,536
@@ -11,6 +11,7 @@ trace      (default) DDA + NEE at 1920 x 1080: 1,048,576 resident 8^3 bricks in 
 lighttree  full light-tree rebuild over 4,194,304 emissive brick leaves (half uniform in an 8192^3
            box, half in 64 Gaussian clusters, sigma 32 voxels, emission log-uniform in [0.01, 100]).
            One step = one rebuild from leaves resident in HBM. value = Mprims/s.
+copy       a plain 1 GiB device copy: the calibration run for the HBM byte counters of rocprofv3.
 
 Prints one JSON line. `roofline` is for the dominant kernel of a step, timed with HIP events around
 its launches on the stream it runs on. Its algorithmic bytes: for the trace, the demand bytes per ray
@@ -54,6 +55,18 @@ BUILD_BYTES_PER_LEAF = 520.0
 
 TRACE_BOUNCES = 2
 TRACE_SPP = 1
+
+# HBM bytes per launch of the dominant kernels, measured with rocprofv3 --pmc (FETCH_SIZE, then
+# WRITE_SIZE) and scaled by the same counters on a 1 GiB copy; null when not measured.
+TRAFFIC_FILE = REPO / "profiles" / "hbm_traffic.json"
+
+
+def _traffic(kernel: str):
+    try:
+        entry = json.loads(TRAFFIC_FILE.read_text()).get(kernel)
+    except (OSError, ValueError):
+        return None
+    return int(entry["bytes"]) if entry else None
 
 
 def _profile(run, steps: int) -> dict:
@@ -151,9 +164,10 @@ def trace_bench(args) -> dict:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": _traffic("trace_frame"),
         "kernel_ms": round(kernel_ms, 4),
         "demand_bytes_per_ray": round(bytes_per_ray, 1),
+        "algorithmic_bytes": int(frame_bytes),
     }
     result = {
         "metric": "Mrays/s DDA+NEE @1080p (primary + bounce + shadow rays)",
@@ -247,7 +261,8 @@ def lighttree_bench(args, with_cpu: bool) -> dict:
         "config": {"workload": "light-tree rebuild, 4,194,304 leaves (half uniform in 8192^3, half in 64 "
                                "clusters sigma 32), u32 Morton keys", "leaves": n},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(dominant),
+                     "algorithmic_bytes": int(STAGE_BYTES.get(dominant, 0.0) * n)},
         "pipeline_roofline": {"bytes_per_leaf": BUILD_BYTES_PER_LEAF, "achieved": round(pipeline_gbs, 1),
                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(pipeline_gbs / HBM_PEAK_GBS, 4)},
         "stages": per_stage,
@@ -283,12 +298,29 @@ def lighttree_cpu_baseline(leaves) -> dict:
                       f"{runs} runs, single thread"}
 
 
+def copy_bench(args) -> dict:
+    """A plain device copy of 1 GiB (torch's copy kernel): the calibration run for the HBM byte
+    counters (FETCH_SIZE / WRITE_SIZE per dispatch against 1 GiB read and 1 GiB written)."""
+    import torch
+
+    n = 1 << 30
+    src = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+    dst = torch.empty_like(src)
+    stream = torch.cuda.current_stream()
+    ms = _time_steps(lambda _: dst.copy_(src), args.steps, args.warmup, stream)
+    gbs = 2 * n / (ms * 1e-3) / 1e9
+    return {"metric": "device copy GB/s (1 GiB read + 1 GiB written)", "value": round(gbs, 1), "unit": "GB/s",
+            "ms_per_step": round(ms, 4), "dtype": "u8", "config": {"workload": "copy 1 GiB", "bytes": n},
+            "roofline": {"bound": "hbm", "kernel": "copy", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": 2 * n}}
+
+
 def main() -> int:
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=20)
     parser.add_argument("--warmup", type=int, default=3)
-    parser.add_argument("--workload", default="trace", choices=["trace", "lighttree"])
+    parser.add_argument("--workload", default="trace", choices=["trace", "lighttree", "copy"])
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-lighttree", action="store_true", help="trace only, without the light-tree figure")
     args = parser.parse_args()
@@ -305,6 +337,8 @@ def main() -> int:
             lt = lighttree_bench(args, with_cpu=False)
             result["lighttree"] = {k: lt[k] for k in ("value", "unit", "ms_per_step", "roofline",
                                                       "pipeline_roofline", "config")}
+    elif args.workload == "copy":
+        result = copy_bench(args)
     else:
         result = lighttree_bench(args, with_cpu=not args.no_cpu_baseline)
     line = {

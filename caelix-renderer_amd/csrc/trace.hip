@@ -82,8 +82,13 @@ CX_DEV void AddCounter(unsigned long long* c, uint32_t v)
     }
 }
 
-// One wave per 8x8 pixel tile.
-__global__ void __launch_bounds__(64) FrameKernel(SceneView s, LightView l, CameraView cam, TraceParams prm,
+// One wave per 8x8 pixel tile. The path state and the world DDA need ~174 VGPRs unconstrained (2
+// waves per SIMD); capping at 5 waves per SIMD (96 VGPRs, some spilled to scratch) hides more of
+// the dependent hash/record/occupancy load latency: 1.6x the rays per second of the uncapped
+// kernel at 1080p, more than the 4- and 8-wave caps.
+constexpr int kFrameWavesPerSimd = 5;
+
+__global__ void __launch_bounds__(64, kFrameWavesPerSimd) FrameKernel(SceneView s, LightView l, CameraView cam, TraceParams prm,
                                                   uint32_t spp, FrameOut out)
 {
     const uint32_t px = blockIdx.x * 8 + (threadIdx.x & 7u);
