@@ -137,10 +137,11 @@ def trace_bench(args) -> dict:
     rays_per_frame = float(per_frame[0] + per_frame[1])
     value = rays_per_frame / (ms * 1e-3) / 1e6
 
-    stages = _profile(frame, 3)
-    kernel_ms = stages["trace_frame"][0] / stages["trace_frame"][1]
+    profile_frames = 3
+    stages = _profile(frame, profile_frames)
 
-    # Demand bytes per ray and a parity check, from the oracle on a band of rows of the last frame.
+    # Demand bytes per ray and stage and a parity check, from the oracle on a band of rows of the
+    # last timed frame.
     import oracle
 
     seed = 1000 + args.warmup + args.steps - 1
@@ -153,21 +154,53 @@ def trace_bench(args) -> dict:
     g_attrib = attrib.cpu().numpy().view(np.uint32)[lo:hi]
     g_rad = radiance.cpu().numpy().reshape(-1, 4)[lo:hi, :3].astype(np.float64)
     rel = np.abs(g_rad - o_rad[:, :3]) / (1.0 + np.abs(o_rad[:, :3]))
-    band_rays = float(o_counters[0] + o_counters[1])
-    bytes_per_ray = float(o_counters[5]) / band_rays
-    frame_bytes = bytes_per_ray * rays_per_frame
-    achieved = frame_bytes / (kernel_ms * 1e-3) / 1e9
+    band_pixels = float((band[1] - band[0]) * w)
+    band_bounce = float(o_counters[0]) - band_pixels
+    band_shadow = float(o_counters[1])
+    pixels = float(w * h)
+    frame_bounce = float(per_frame[0]) - pixels
+    frame_shadow = float(per_frame[1])
+    bytes_per_ray = float(o_counters[5]) / float(o_counters[0] + o_counters[1])
+    # Algorithmic bytes of each stage per frame: the scene data it loads (the oracle's demand bytes of
+    # that kind of work, per ray of that kind, times the frame's rays of that kind) plus the path
+    # state the wavefront moves (per ray: primary hit 16; shade reads hit 16 and writes radiance,
+    # albedo, emission, throughput, state 16 each, ray 32, queue 4, shadow entry 48; extend reads
+    # queue 4 + ray 32 and writes the hit 16; shadow reads the entry 48 and updates radiance 32;
+    # bounce shade reads hit, state, ray, throughput, radiance 96 and writes 116; finish 84).
+    per_kind = [float(o_counters[6 + k]) for k in range(5)]
+    algorithmic = {
+        "trace_primary": per_kind[0] / band_pixels * pixels + 16.0 * pixels,
+        "trace_extend": per_kind[1] / max(band_bounce, 1.0) * frame_bounce + 52.0 * frame_bounce,
+        "trace_shadow": per_kind[2] / max(band_shadow, 1.0) * frame_shadow + 80.0 * frame_shadow,
+        "trace_shade_primary": per_kind[3] / band_pixels * pixels + 196.0 * pixels,
+        "trace_shade_bounce": per_kind[4] / max(band_bounce, 1.0) * frame_bounce + 260.0 * frame_bounce,
+        "trace_finish": 84.0 * pixels,
+    }
+    stage_table = {}
+    for name, (total_ms, launches) in stages.items():
+        per_frame_ms = total_ms / profile_frames
+        gb = algorithmic.get(name, 0.0) / 1e9
+        stage_table[name] = {"ms_per_frame": round(per_frame_ms, 4), "launches_per_frame": launches / profile_frames,
+                             "algorithmic_GB": round(gb, 3),
+                             "GB_per_s": round(gb / (per_frame_ms * 1e-3), 1) if per_frame_ms > 0 else 0.0}
+    dominant = max(stages, key=lambda k: stages[k][0])
+    kernel_ms = stages[dominant][0] / stages[dominant][1]
+    launch_bytes = algorithmic.get(dominant, 0.0) / (stages[dominant][1] / profile_frames)
+    achieved = launch_bytes / (kernel_ms * 1e-3) / 1e9
+    frame_bytes = sum(algorithmic.values())
     roofline = {
         "bound": "hbm",
-        "kernel": "trace_frame",
+        "kernel": dominant,
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": _traffic("trace_frame"),
+        "traffic": _traffic(dominant),
         "kernel_ms": round(kernel_ms, 4),
+        "algorithmic_bytes": int(launch_bytes),
+        "frame_algorithmic_bytes": int(frame_bytes),
+        "frame_GB_per_s": round(frame_bytes / (ms * 1e-3) / 1e9, 1),
         "demand_bytes_per_ray": round(bytes_per_ray, 1),
-        "algorithmic_bytes": int(frame_bytes),
     }
     result = {
         "metric": "Mrays/s DDA+NEE @1080p (primary + bounce + shadow rays)",
@@ -192,8 +225,7 @@ def trace_bench(args) -> dict:
             "attrib_exact": bool(np.array_equal(g_attrib, o_attrib)),
             "max_rel_radiance": float(rel.max()) if len(rel) else 0.0,
         },
-        "stages": {k: {"avg_ms": round(v[0] / v[1], 5), "launches_per_step": round(v[1] / 3, 2)}
-                   for k, v in stages.items()},
+        "stages": stage_table,
     }
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = trace_cpu_baseline(o, cam, seed)

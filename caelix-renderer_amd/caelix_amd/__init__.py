@@ -98,7 +98,7 @@ class CxTraceParams(ctypes.Structure):
 
 class CxTraceOutput(ctypes.Structure):
     _fields_ = [("color", ctypes.c_void_p), ("radiance", ctypes.c_void_p), ("attrib", ctypes.c_void_p),
-                ("counters", ctypes.c_void_p)]
+                ("counters", ctypes.c_void_p), ("workspace", ctypes.c_void_p), ("workspace_bytes", ctypes.c_uint64)]
 
 
 def camera_struct(cam: dict) -> CxCamera:
@@ -135,6 +135,7 @@ SIGNATURES = {
     "cx_scene_finalize": (_INT, [ctypes.POINTER(CxScene), _P]),
     "cx_scene_light_scratch_bytes": (_U64, [_U32]),
     "cx_scene_light_records": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightScene), _P]),
+    "cx_trace_workspace_bytes": (_U64, [_U32, _U32]),
     "cx_trace_frame": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightTreeView), ctypes.POINTER(CxCamera),
                               ctypes.POINTER(CxTraceParams), ctypes.POINTER(CxTraceOutput), _P]),
     "cx_trace_rays": (_INT, [ctypes.POINTER(CxScene), _P, _U32, _P, _P]),
@@ -470,7 +471,12 @@ class TraceScene:
         counters = counters if counters is not None else torch.zeros(8, dtype=torch.int64, device="cuda")
         cam = camera_struct(camera)
         params = CxTraceParams(bounces, seed, spp, 0)
-        out = CxTraceOutput(color.data_ptr(), radiance.data_ptr(), attrib.data_ptr(), counters.data_ptr())
+        key = (w, h)
+        if getattr(self, "_workspace_key", None) != key:
+            self.workspace = device_bytes(int(self.lib.cx_trace_workspace_bytes(max(w, 1), max(h, 1))), "cuda")
+            self._workspace_key = key
+        out = CxTraceOutput(color.data_ptr(), radiance.data_ptr(), attrib.data_ptr(), counters.data_ptr(),
+                            self.workspace.data_ptr(), self.workspace.numel())
         check(self.lib.cx_trace_frame(ctypes.byref(self.scene), ctypes.byref(self.view), ctypes.byref(cam),
                                       ctypes.byref(params), ctypes.byref(out), stream_handle(stream)), "cx_trace_frame")
         return color, radiance, attrib, counters

@@ -60,14 +60,6 @@ CX_DEV Shade ShadeHit(const SceneView& s, uint32_t block, V3 hit_position, V3 no
     return sh;
 }
 
-struct PathCounters
-{
-    uint32_t rays;        // primary + bounce rays traced
-    uint32_t shadow_rays;
-    uint32_t cells;
-    uint32_t probes;
-    uint32_t steps;
-};
 
 CX_DEV WorldHit Trace(const SceneView& s, V3 o, V3 d, float t_max, PathCounters& c)
 {
@@ -78,14 +70,17 @@ CX_DEV WorldHit Trace(const SceneView& s, V3 o, V3 d, float t_max, PathCounters&
     return h;
 }
 
-// NextEventAtVertex (:356-413) without the sun: the lamp candidate (LampCandidate, :301-333) and its
-// shadow ray.
-CX_DEV V3 NextEvent(const SceneView& s, const LightView& L, V3 x_r, V3 n_r, uint32_t& light_rng, PathCounters& c)
+// NextEventAtVertex (:356-413) without the sun: the lamp candidate (LampCandidate, :301-333). Returns
+// false when no shadow ray is counted (the candidate's weight is 0); else `direction` and `t_max`
+// give the shadow ray from x_r (t_max 0: counted but not traced) and `visible` the radiance it adds
+// when it reaches the light unoccluded.
+CX_DEV bool NeeCandidate(const SceneView& s, const LightView& L, V3 x_r, V3 n_r, uint32_t& light_rng, V3& direction,
+                         float& t_max, V3& visible)
 {
     const uint32_t point_seed = RandomU32(light_rng);
     V3 contribution = Make(0.0f, 0.0f, 0.0f);
-    V3 direction = Make(0.0f, 0.0f, 0.0f);
-    float t_max = 0.0f;
+    direction = Make(0.0f, 0.0f, 0.0f);
+    t_max = 0.0f;
     uint32_t rng = point_seed;
     LightPoint p;
     if (LightTreeSample(s, L, x_r, n_r, rng, p) && p.pdf > 0.0f)
@@ -113,17 +108,12 @@ CX_DEV V3 NextEvent(const SceneView& s, const LightView& L, V3 x_r, V3 n_r, uint
     }
     const float w_lamp = Luminance(contribution) * fmaxf(0.05f, 1.0f);
     const float w_sum = 0.0f + w_lamp;
-    V3 radiance = Make(0.0f, 0.0f, 0.0f);
-    if (w_sum > 0.0f)
+    if (!(w_sum > 0.0f))
     {
-        c.shadow_rays += 1;
-        const WorldHit h = t_max > 0.0f ? Trace(s, x_r, direction, t_max, c) : WorldHit{0.0f, 0u, kHashNone, {0, 0, 0}, 0, 0, 0};
-        if (t_max > 0.0f && h.attrib == 0u)
-        {
-            radiance = Mul(contribution, Div(w_sum, w_lamp));
-        }
+        return false;
     }
-    return radiance;
+    visible = Mul(contribution, Div(w_sum, w_lamp));
+    return true;
 }
 
 // LampHitWeight (:425-452).
@@ -154,26 +144,9 @@ CX_DEV float LampHitWeight(const SceneView& s, const LightView& L, uint32_t segm
     return MisWeight(p_area, segment_normal, direction, hit.t, cos_l, false);
 }
 
-struct PixelResult
+// CaelixPrimaryDirection (caelix_frame.hlsl:164-171): the pinhole ray through the pixel centre.
+CX_DEV V3 PrimaryDirection(const CameraView& cam, uint32_t px, uint32_t py)
 {
-    V3 radiance;  // the demodulated stochastic radiance (TracePath's `stochastic`)
-    V3 albedo;    // of the primary hit
-    V3 emission;  // of the primary hit
-    float t;      // primary hit distance (0 = miss)
-    uint32_t attrib;
-};
-
-// The main kernel's per-pixel work (:1050-) and TracePath for a primary ray.
-CX_DEV PixelResult TracePixel(const SceneView& s, const LightView& L, const CameraView& cam, const TraceParams& prm,
-                              uint32_t px, uint32_t py, PathCounters& c)
-{
-    PixelResult out;
-    out.radiance = Make(0.0f, 0.0f, 0.0f);
-    out.albedo = Make(0.0f, 0.0f, 0.0f);
-    out.emission = Make(0.0f, 0.0f, 0.0f);
-    out.t = 0.0f;
-    out.attrib = 0u;
-    // CaelixPrimaryDirection (caelix_frame.hlsl:164-171), pixel centre.
     const float sx = float(px) + 0.5f;
     const float sy = float(py) + 0.5f;
     const float ndc_x = Div(sx, float(cam.width)) * 2.0f - 1.0f;
@@ -181,75 +154,13 @@ CX_DEV PixelResult TracePixel(const SceneView& s, const LightView& L, const Came
     const V3 right = Make(cam.right[0], cam.right[1], cam.right[2]);
     const V3 up = Make(cam.up[0], cam.up[1], cam.up[2]);
     const V3 forward = Make(cam.forward[0], cam.forward[1], cam.forward[2]);
-    const V3 dir = SafeNormalize(Add(Add(Mul(right, ndc_x * cam.tan_half_fov_y * cam.aspect), Mul(up, ndc_y * cam.tan_half_fov_y)),
-                                     forward));
-    const V3 origin = Make(cam.position[0], cam.position[1], cam.position[2]);
-    c.rays += 1;
-    const WorldHit primary = Trace(s, origin, dir, kRayTMax, c);
-    if (primary.attrib == 0u)
-    {
-        return out;  // the sky: black
-    }
-    out.t = primary.t;
-    out.attrib = primary.attrib;
-    uint32_t rng = InitRandom(px, py, prm.frame_seed);
-    uint32_t light_rng = InitRandom(px, py, prm.frame_seed ^ 0x7F4A7C15u);
-    const uint32_t mis_bit = L.mis != 0u ? kSegmentLampMis : 0u;
-    uint32_t segment_light = 0u;
-    const uint32_t surface_block = (primary.attrib >> 16) & 0xFFFFu;
-    const V3 primary_normal = FaceNormal(primary.attrib);
-    const Shade primary_shade = ShadeHit(s, surface_block, Add(origin, Mul(dir, primary.t)), primary_normal, rng);
-    out.albedo = primary_shade.albedo;
-    out.emission = primary_shade.emission;
-    V3 radiance = Make(0.0f, 0.0f, 0.0f);
-    V3 throughput = Make(1.0f, 1.0f, 1.0f);  // 1 / max(0.001, k), k = 1
-    V3 segment_normal = primary_normal;
-    if (prm.bounces >= 1u)
-    {
-        radiance = Add(radiance, MulV(NextEvent(s, L, primary_shade.bounce_origin, primary_normal, light_rng, c), throughput));
-        segment_light = kSegmentLampNee | mis_bit;
-    }
-    V3 bounce_origin = primary_shade.bounce_origin;
-    V3 bounce_direction = primary_shade.bounce_direction;
-    uint32_t bounce_index = 1;
-    while (bounce_index <= prm.bounces)
-    {
-        c.rays += 1;
-        const WorldHit hit = Trace(s, bounce_origin, bounce_direction, kRayTMax, c);
-        if (hit.attrib == 0u)
-        {
-            break;  // a miss sees the black sky
-        }
-        const uint32_t hit_block = (hit.attrib >> 16) & 0xFFFFu;
-        const V3 vertex_normal = FaceNormal(hit.attrib);
-        const Shade shade = ShadeHit(s, hit_block, Add(bounce_origin, Mul(bounce_direction, hit.t)), vertex_normal, rng);
-        bounce_index += 1;
-        float emission_weight = 1.0f;
-        if (segment_light != 0u && (shade.emission.x > 0.0f || shade.emission.y > 0.0f || shade.emission.z > 0.0f))
-        {
-            emission_weight = LampHitWeight(s, L, segment_light, hit, bounce_origin, bounce_direction, segment_normal);
-        }
-        radiance = Add(radiance, MulV(shade.emission, Mul(throughput, emission_weight)));
-        throughput = MulV(throughput, Make(Div(shade.albedo.x, 1.0f), Div(shade.albedo.y, 1.0f), Div(shade.albedo.z, 1.0f)));
-        segment_light = 0u;
-        if (bounce_index <= prm.bounces)
-        {
-            segment_light = kSegmentLampNee | mis_bit;
-            radiance = Add(radiance, MulV(NextEvent(s, L, shade.bounce_origin, vertex_normal, light_rng, c),
-                                          Mul(throughput, 1.0f)));
-        }
-        const float cp = fmaxf(throughput.x, fmaxf(throughput.y, throughput.z));
-        if (cp <= 0.0f || cp < RandomFloat01(rng))
-        {
-            break;
-        }
-        throughput = Mul(throughput, Div(1.0f, cp));
-        bounce_origin = shade.bounce_origin;
-        bounce_direction = shade.bounce_direction;
-        segment_normal = vertex_normal;
-    }
-    out.radiance = radiance;
-    return out;
+    return SafeNormalize(Add(Add(Mul(right, ndc_x * cam.tan_half_fov_y * cam.aspect), Mul(up, ndc_y * cam.tan_half_fov_y)),
+                             forward));
+}
+
+CX_DEV uint32_t FaceIndex(uint32_t attrib)
+{
+    return uint32_t(__ffs(int(attrib & 0x3Fu)) - 1);
 }
 
 }  // namespace cx

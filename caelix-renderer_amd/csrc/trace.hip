@@ -82,54 +82,500 @@ CX_DEV void AddCounter(unsigned long long* c, uint32_t v)
     }
 }
 
-// One wave per 8x8 pixel tile. The path state and the world DDA need ~174 VGPRs unconstrained (2
-// waves per SIMD); capping at 5 waves per SIMD (96 VGPRs, some spilled to scratch) hides more of
-// the dependent hash/record/occupancy load latency: 1.6x the rays per second of the uncapped
-// kernel at 1080p, more than the 4- and 8-wave caps.
-constexpr int kFrameWavesPerSimd = 5;
-
-__global__ void __launch_bounds__(64, kFrameWavesPerSimd) FrameKernel(SceneView s, LightView l, CameraView cam, TraceParams prm,
-                                                  uint32_t spp, FrameOut out)
+CX_DEV void FlushCounters(unsigned long long* counters, const PathCounters& c)
 {
-    const uint32_t px = blockIdx.x * 8 + (threadIdx.x & 7u);
-    const uint32_t py = blockIdx.y * 8 + (threadIdx.x >> 3);
-    const bool inside = px < cam.width && py < cam.height;
-    PathCounters c{0, 0, 0, 0, 0};
-    if (inside)
+    AddCounter(&counters[0], c.rays);
+    AddCounter(&counters[1], c.shadow_rays);
+    AddCounter(&counters[2], c.cells);
+    AddCounter(&counters[3], c.probes);
+    AddCounter(&counters[4], c.steps);
+}
+
+// ------------------------------------------------------------------------------------------------
+// The frame as a wavefront: per sample, PrimaryTrace -> PrimaryShade -> Shadow, then per bounce
+// Extend -> BounceShade -> Shadow, then Finish. Each path (one per pixel, numbered so that a wave's
+// 64 paths are one 8x8 pixel tile) keeps its state in the workspace between launches; the shade
+// stages append the paths that go on and their shadow rays to queues (wave-aggregated atomic
+// appends), so the trace launches run only live rays and hold only the traversal's registers. Each
+// path does exactly the operations of TracePath (caelix_path_trace.hlsl:737-978) in the same order:
+// the radiance terms are added in path order (NEE at vertex 0, emission at vertex 1, NEE at vertex
+// 1, ...), because every shadow launch completes before the next bounce's shade launch.
+
+constexpr uint32_t kWaveThreads = 256;
+
+
+
+struct Workspace
+{
+    float4* ray_o;      // [paths] next ray origin
+    float4* ray_d;      // [paths] next ray direction
+    uint4* hit;         // [paths] t bits, attrib, brick record, -
+    float4* throughput; // [paths]
+    float4* radiance;   // [paths] this sample's radiance
+    uint4* state;       // [paths] rng, light rng, segment light | bounce << 8 | segment face << 16, -
+    float4* albedo;     // [paths] primary albedo, t
+    float4* emission;   // [paths] primary emission, attrib bits
+    float4* shadow_o;   // [paths] shadow ray origin, t_max
+    float4* shadow_d;   // [paths] shadow ray direction, path bits
+    float4* shadow_add; // [paths] radiance the shadow ray adds when unoccluded
+    uint32_t* queue[2]; // [paths] path queues (ping-pong by bounce parity)
+    uint32_t* ctl;      // [0], [1] queue lengths, [2] shadow queue length
+};
+
+uint32_t TilesX(uint32_t width)
+{
+    return (width + 7) / 8;
+}
+uint32_t PathCount(uint32_t width, uint32_t height)
+{
+    return TilesX(width) * ((height + 7) / 8) * 64;
+}
+uint64_t Align256(uint64_t x)
+{
+    return (x + 255) & ~uint64_t(255);
+}
+uint64_t WorkspaceBytes(uint32_t paths)
+{
+    return 11 * Align256(uint64_t(paths) * 16) + 2 * Align256(uint64_t(paths) * 4) + 256;
+}
+Workspace CarveWorkspace(void* base, uint32_t paths)
+{
+    uint8_t* p = static_cast<uint8_t*>(base);
+    auto take = [&](uint64_t bytes) {
+        uint8_t* r = p;
+        p += Align256(bytes);
+        return r;
+    };
+    Workspace w;
+    const uint64_t v = uint64_t(paths) * 16;
+    w.ray_o = reinterpret_cast<float4*>(take(v));
+    w.ray_d = reinterpret_cast<float4*>(take(v));
+    w.hit = reinterpret_cast<uint4*>(take(v));
+    w.throughput = reinterpret_cast<float4*>(take(v));
+    w.radiance = reinterpret_cast<float4*>(take(v));
+    w.state = reinterpret_cast<uint4*>(take(v));
+    w.albedo = reinterpret_cast<float4*>(take(v));
+    w.emission = reinterpret_cast<float4*>(take(v));
+    w.shadow_o = reinterpret_cast<float4*>(take(v));
+    w.shadow_d = reinterpret_cast<float4*>(take(v));
+    w.shadow_add = reinterpret_cast<float4*>(take(v));
+    w.queue[0] = reinterpret_cast<uint32_t*>(take(uint64_t(paths) * 4));
+    w.queue[1] = reinterpret_cast<uint32_t*>(take(uint64_t(paths) * 4));
+    w.ctl = reinterpret_cast<uint32_t*>(take(256));
+    return w;
+}
+
+CX_DEV void PathPixel(const CameraView& cam, uint32_t path, uint32_t& px, uint32_t& py)
+{
+    const uint32_t tiles_x = (cam.width + 7) / 8;
+    const uint32_t tile = path >> 6;
+    const uint32_t within = path & 63u;
+    px = (tile % tiles_x) * 8 + (within & 7u);
+    py = (tile / tiles_x) * 8 + (within >> 3);
+}
+
+// Appends `value` to queue[*count ...] for every lane with `active`; one atomic per wave.
+CX_DEV uint32_t WaveAppend(uint32_t* count, bool active)
+{
+    const uint64_t mask = Ballot(active);
+    if (mask == 0)
     {
-        V3 color = Make(0.0f, 0.0f, 0.0f);
-        V3 radiance = Make(0.0f, 0.0f, 0.0f);
-        float t = 0.0f;
-        uint32_t attrib = 0;
-        for (uint32_t sample = 0; sample < spp; ++sample)
+        return 0;
+    }
+    uint32_t base = 0;
+    const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+    if (Lane() == uint32_t(leader))
+    {
+        base = AtomicAddRelaxed(count, uint32_t(PopC(mask)));
+    }
+    base = __shfl(base, leader);
+    return base + uint32_t(PopC(mask & LanesBelow()));
+}
+
+CX_DEV void EnqueueShadow(const Workspace& w, uint32_t path, bool active, V3 origin, V3 direction, float t_max, V3 add)
+{
+    const uint32_t slot = WaveAppend(&w.ctl[2], active);
+    if (active)
+    {
+        w.shadow_o[slot] = make_float4(origin.x, origin.y, origin.z, t_max);
+        w.shadow_d[slot] = make_float4(direction.x, direction.y, direction.z, __uint_as_float(path));
+        w.shadow_add[slot] = make_float4(add.x, add.y, add.z, 0.0f);
+    }
+}
+
+CX_DEV uint32_t PackState(uint32_t segment_light, uint32_t bounce, uint32_t face)
+{
+    return segment_light | (bounce << 8) | (face << 16);
+}
+
+// ---- persistent traversal ----------------------------------------------------------------------
+// Control words: [0], [1] path queue lengths, [2] shadow queue length, [8] shadow cursor, [9] bounce
+// cursor. A wave takes 64 consecutive items of its queue per atomic and hands
+// them to its idle lanes; every lane advances its ray one cell per iteration and takes the next item
+// when its ray ends, so a wave's lanes stay busy until the queue is drained.
+constexpr uint32_t kCtlShadowCursor = 8;
+constexpr uint32_t kCtlBounceCursor = 9;
+
+// Camera rays of every path and their first hit, one path per lane: the rays of a wave are one 8x8
+// pixel tile and stay coherent, so they run in lockstep without refills (faster here than the
+// persistent form).
+__global__ void __launch_bounds__(kWaveThreads) PrimaryTraceKernel(SceneView s, CameraView cam, Workspace w,
+                                                                   uint32_t paths, unsigned long long* counters)
+{
+    const uint32_t path = blockIdx.x * kWaveThreads + threadIdx.x;
+    PathCounters c{0, 0, 0, 0, 0};
+    if (path < paths)
+    {
+        uint32_t px, py;
+        PathPixel(cam, path, px, py);
+        uint4 h = make_uint4(0u, 0u, kHashNone, 0u);
+        if (px < cam.width && py < cam.height)
         {
-            TraceParams p = prm;
-            p.frame_seed = prm.frame_seed + sample;
-            const PixelResult r = TracePixel(s, l, cam, p, px, py, c);
-            color = Add(color, Add(r.emission, MulV(r.albedo, r.radiance)));
-            radiance = Add(radiance, r.radiance);
-            t = r.t;
-            attrib = r.attrib;
+            const V3 dir = PrimaryDirection(cam, px, py);
+            const V3 origin = Make(cam.position[0], cam.position[1], cam.position[2]);
+            c.rays += 1;
+            const WorldHit hit = Trace(s, origin, dir, kRayTMax, c);
+            h = make_uint4(__float_as_uint(hit.t), hit.attrib, hit.record, 0u);
         }
-        const uint64_t pixel = uint64_t(py) * cam.width + px;
-        if (out.color != nullptr)
+        w.hit[path] = h;
+    }
+    FlushCounters(counters, c);
+}
+
+__global__ void __launch_bounds__(64) ResetKernel(Workspace w)
+{
+    w.ctl[threadIdx.x] = 0u;
+}
+
+// Bounce rays: item -> path of queue[in].
+struct BounceSource
+{
+    Workspace w;
+    uint32_t in;
+    CX_DEV uint32_t Count() const { return w.ctl[in]; }
+    CX_DEV uint32_t* Cursor() const { return &w.ctl[kCtlBounceCursor]; }
+    // The queues the following shade launch appends to, and the shadow cursor, start empty.
+    CX_DEV void Prepare() const
+    {
+        w.ctl[1 - in] = 0u;
+        w.ctl[2] = 0u;
+        w.ctl[kCtlShadowCursor] = 0u;
+    }
+    CX_DEV bool Load(uint32_t item, V3& o, V3& d, float& t_max, PathCounters& c) const
+    {
+        const uint32_t path = w.queue[in][item];
+        const float4 o4 = w.ray_o[path];
+        const float4 d4 = w.ray_d[path];
+        o = Make(o4.x, o4.y, o4.z);
+        d = Make(d4.x, d4.y, d4.z);
+        t_max = kRayTMax;
+        c.rays += 1;
+        return true;
+    }
+    CX_DEV void Finish(uint32_t item, const WorldHit& h) const
+    {
+        w.hit[w.queue[in][item]] = make_uint4(__float_as_uint(h.t), h.attrib, h.record, 0u);
+    }
+};
+
+// Shadow rays: an unoccluded one adds its radiance to its path (one shadow ray per path per launch).
+struct ShadowSource
+{
+    Workspace w;
+    CX_DEV uint32_t Count() const { return w.ctl[2]; }
+    CX_DEV uint32_t* Cursor() const { return &w.ctl[kCtlShadowCursor]; }
+    CX_DEV void Prepare() const {}
+    CX_DEV bool Load(uint32_t item, V3& o, V3& d, float& t_max, PathCounters&) const
+    {
+        const float4 o4 = w.shadow_o[item];
+        const float4 d4 = w.shadow_d[item];
+        o = Make(o4.x, o4.y, o4.z);
+        d = Make(d4.x, d4.y, d4.z);
+        t_max = o4.w;
+        return true;
+    }
+    CX_DEV void Finish(uint32_t item, const WorldHit& h) const
+    {
+        if (h.attrib != 0u)
         {
-            out.color[pixel] = make_float4(color.x, color.y, color.z, t);
+            return;
         }
-        if (out.radiance != nullptr)
+        const uint32_t path = __float_as_uint(w.shadow_d[item].w);
+        const float4 a = w.shadow_add[item];
+        float4 r = w.radiance[path];
+        r.x = r.x + a.x;
+        r.y = r.y + a.y;
+        r.z = r.z + a.z;
+        w.radiance[path] = r;
+    }
+};
+
+template <class Source>
+__global__ void __launch_bounds__(kWaveThreads) PersistentTraceKernel(SceneView s, Source src,
+                                                                      unsigned long long* counters)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        src.Prepare();
+    }
+    PathCounters c{0, 0, 0, 0, 0};
+    const uint32_t count = src.Count();
+    uint32_t* cursor = src.Cursor();
+    WorldRay r;
+    r.phase = kRayDone;
+    uint32_t item = 0;
+    bool busy = false;
+    uint32_t chunk_next = 0;  // wave-uniform: next item of the wave's chunk
+    uint32_t chunk_end = 0;   // wave-uniform: end of the wave's chunk
+    bool drained = false;     // wave-uniform: the queue has no more chunks
+    while (true)
+    {
+        const uint64_t idle = Ballot(!busy);
+        if (idle != 0 && !drained)
         {
-            out.radiance[pixel] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
+            if (chunk_next == chunk_end)
+            {
+                uint32_t base = 0;
+                if (Lane() == 0)
+                {
+                    base = AtomicAddRelaxed(cursor, 64u);
+                }
+                base = __shfl(base, 0);
+                chunk_next = min(base, count);
+                chunk_end = min(base + 64u, count);
+                drained = chunk_next == chunk_end;
+            }
+            const uint32_t take = min(uint32_t(PopC(idle)), chunk_end - chunk_next);
+            const uint32_t rank = uint32_t(PopC(idle & LanesBelow()));
+            if (!busy && rank < take)
+            {
+                item = chunk_next + rank;
+                V3 o, d;
+                float t_max;
+                if (src.Load(item, o, d, t_max, c))
+                {
+                    WorldRayBegin(s, o, d, t_max, r);
+                    busy = r.phase != kRayDone;
+                    if (!busy)
+                    {
+                        src.Finish(item, WorldHit{0.0f, 0u, kHashNone, {0, 0, 0}, 0u, 0u, 0u});
+                    }
+                }
+            }
+            chunk_next += take;
         }
-        if (out.attrib != nullptr)
+        if (Ballot(busy) == 0)
         {
-            out.attrib[pixel] = attrib;
+            if (drained)
+            {
+                break;
+            }
+            continue;
+        }
+        if (busy)
+        {
+            WorldHit h;
+            WorldRayStep(s, r, h, c);
+            if (r.phase == kRayDone)
+            {
+                src.Finish(item, h);
+                busy = false;
+            }
         }
     }
-    AddCounter(&out.counters[0], c.rays);
-    AddCounter(&out.counters[1], c.shadow_rays);
-    AddCounter(&out.counters[2], c.cells);
-    AddCounter(&out.counters[3], c.probes);
-    AddCounter(&out.counters[4], c.steps);
+    FlushCounters(counters, c);
+}
+
+// The primary vertex: outputs of the hit, the first shade, the NEE at vertex 0 and the first
+// bounce ray (queue 0).
+__global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, LightView L, CameraView cam,
+                                                                   TraceParams prm, Workspace w, uint32_t paths,
+                                                                   unsigned long long* counters)
+{
+    const uint32_t path = blockIdx.x * kWaveThreads + threadIdx.x;
+    PathCounters c{0, 0, 0, 0, 0};
+    bool shadow = false;
+    bool go_on = false;
+    V3 shadow_origin = Make(0.0f, 0.0f, 0.0f), shadow_dir = Make(0.0f, 0.0f, 0.0f), shadow_add = Make(0.0f, 0.0f, 0.0f);
+    float shadow_t = 0.0f;
+    if (path < paths)
+    {
+        const uint4 h = w.hit[path];
+        w.radiance[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (h.y == 0u)
+        {
+            w.albedo[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            w.emission[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        else
+        {
+            uint32_t px, py;
+            PathPixel(cam, path, px, py);
+            const float t = __uint_as_float(h.x);
+            const V3 dir = PrimaryDirection(cam, px, py);
+            const V3 origin = Make(cam.position[0], cam.position[1], cam.position[2]);
+            uint32_t rng = InitRandom(px, py, prm.frame_seed);
+            uint32_t light_rng = InitRandom(px, py, prm.frame_seed ^ 0x7F4A7C15u);
+            const uint32_t mis_bit = L.mis != 0u ? kSegmentLampMis : 0u;
+            const uint32_t face = FaceIndex(h.y);
+            const V3 normal = FaceDirection(face);
+            const Shade sh = ShadeHit(s, (h.y >> 16) & 0xFFFFu, Add(origin, Mul(dir, t)), normal, rng);
+            w.albedo[path] = make_float4(sh.albedo.x, sh.albedo.y, sh.albedo.z, t);
+            w.emission[path] = make_float4(sh.emission.x, sh.emission.y, sh.emission.z, __uint_as_float(h.y));
+            const V3 throughput = Make(1.0f, 1.0f, 1.0f);
+            uint32_t segment_light = 0u;
+            if (prm.bounces >= 1u)
+            {
+                V3 visible;
+                if (NeeCandidate(s, L, sh.bounce_origin, normal, light_rng, shadow_dir, shadow_t, visible))
+                {
+                    c.shadow_rays += 1;
+                    shadow = shadow_t > 0.0f;
+                    shadow_origin = sh.bounce_origin;
+                    shadow_add = MulV(visible, throughput);
+                }
+                segment_light = kSegmentLampNee | mis_bit;
+                go_on = true;
+            }
+            w.throughput[path] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+            w.state[path] = make_uint4(rng, light_rng, PackState(segment_light, 1u, face), 0u);
+            w.ray_o[path] = make_float4(sh.bounce_origin.x, sh.bounce_origin.y, sh.bounce_origin.z, 0.0f);
+            w.ray_d[path] = make_float4(sh.bounce_direction.x, sh.bounce_direction.y, sh.bounce_direction.z, 0.0f);
+        }
+    }
+    EnqueueShadow(w, path, shadow, shadow_origin, shadow_dir, shadow_t, shadow_add);
+    const uint32_t slot = WaveAppend(&w.ctl[0], go_on);
+    if (go_on)
+    {
+        w.queue[0][slot] = path;
+    }
+    FlushCounters(counters, c);
+}
+
+// One iteration of TracePath's bounce loop for the paths in queue[(bounce - 1) & 1].
+__global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, LightView L, TraceParams prm, Workspace w,
+                                                                  uint32_t bounce, uint32_t paths,
+                                                                  unsigned long long* counters)
+{
+    const uint32_t i = blockIdx.x * kWaveThreads + threadIdx.x;
+    const uint32_t in = (bounce - 1) & 1u;
+    if (i == 0)
+    {
+        w.ctl[kCtlBounceCursor] = 0u;  // for the next bounce's rays
+    }
+    PathCounters c{0, 0, 0, 0, 0};
+    const uint32_t count = w.ctl[in];
+    bool shadow = false;
+    bool go_on = false;
+    uint32_t path = 0;
+    V3 shadow_origin = Make(0.0f, 0.0f, 0.0f), shadow_dir = Make(0.0f, 0.0f, 0.0f), shadow_add = Make(0.0f, 0.0f, 0.0f);
+    float shadow_t = 0.0f;
+    if (i < count && i < paths)
+    {
+        path = w.queue[in][i];
+        const uint4 h = w.hit[path];
+        if (h.y != 0u)
+        {
+            uint4 st = w.state[path];
+            uint32_t rng = st.x;
+            uint32_t light_rng = st.y;
+            uint32_t segment_light = st.z & 0xFFu;
+            uint32_t bounce_index = (st.z >> 8) & 0xFFu;
+            const uint32_t segment_face = st.z >> 16;
+            const float4 o4 = w.ray_o[path];
+            const float4 d4 = w.ray_d[path];
+            const V3 origin = Make(o4.x, o4.y, o4.z);
+            const V3 dir = Make(d4.x, d4.y, d4.z);
+            const float4 t4 = w.throughput[path];
+            V3 throughput = Make(t4.x, t4.y, t4.z);
+            const float4 r4 = w.radiance[path];
+            V3 radiance = Make(r4.x, r4.y, r4.z);
+            WorldHit hit;
+            hit.t = __uint_as_float(h.x);
+            hit.attrib = h.y;
+            hit.record = h.z;
+            const uint32_t face = FaceIndex(h.y);
+            const V3 vertex_normal = FaceDirection(face);
+            const Shade sh = ShadeHit(s, (h.y >> 16) & 0xFFFFu, Add(origin, Mul(dir, hit.t)), vertex_normal, rng);
+            bounce_index += 1;
+            float emission_weight = 1.0f;
+            if (segment_light != 0u && (sh.emission.x > 0.0f || sh.emission.y > 0.0f || sh.emission.z > 0.0f))
+            {
+                emission_weight = LampHitWeight(s, L, segment_light, hit, origin, dir, FaceDirection(segment_face));
+            }
+            radiance = Add(radiance, MulV(sh.emission, Mul(throughput, emission_weight)));
+            throughput = MulV(throughput, Make(Div(sh.albedo.x, 1.0f), Div(sh.albedo.y, 1.0f), Div(sh.albedo.z, 1.0f)));
+            segment_light = 0u;
+            if (bounce_index <= prm.bounces)
+            {
+                segment_light = kSegmentLampNee | (L.mis != 0u ? kSegmentLampMis : 0u);
+                V3 visible;
+                if (NeeCandidate(s, L, sh.bounce_origin, vertex_normal, light_rng, shadow_dir, shadow_t, visible))
+                {
+                    c.shadow_rays += 1;
+                    shadow = shadow_t > 0.0f;
+                    shadow_origin = sh.bounce_origin;
+                    shadow_add = MulV(visible, Mul(throughput, 1.0f));
+                }
+            }
+            const float cp = fmaxf(throughput.x, fmaxf(throughput.y, throughput.z));
+            if (!(cp <= 0.0f || cp < RandomFloat01(rng)))
+            {
+                throughput = Mul(throughput, Div(1.0f, cp));
+                go_on = bounce_index <= prm.bounces;
+                w.ray_o[path] = make_float4(sh.bounce_origin.x, sh.bounce_origin.y, sh.bounce_origin.z, 0.0f);
+                w.ray_d[path] = make_float4(sh.bounce_direction.x, sh.bounce_direction.y, sh.bounce_direction.z, 0.0f);
+            }
+            w.radiance[path] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
+            w.throughput[path] = make_float4(throughput.x, throughput.y, throughput.z, 0.0f);
+            w.state[path] = make_uint4(rng, light_rng, PackState(segment_light, bounce_index, face), 0u);
+        }
+    }
+    EnqueueShadow(w, path, shadow, shadow_origin, shadow_dir, shadow_t, shadow_add);
+    const uint32_t slot = WaveAppend(&w.ctl[1 - in], go_on);
+    if (go_on)
+    {
+        w.queue[1 - in][slot] = path;
+    }
+    FlushCounters(counters, c);
+}
+
+// color += emission + albedo * radiance, radiance += radiance, t and attrib of the last sample.
+__global__ void __launch_bounds__(kWaveThreads) FinishKernel(CameraView cam, Workspace w, uint32_t paths, uint32_t sample,
+                                                             FrameOut out)
+{
+    const uint32_t path = blockIdx.x * kWaveThreads + threadIdx.x;
+    if (path >= paths)
+    {
+        return;
+    }
+    uint32_t px, py;
+    PathPixel(cam, path, px, py);
+    if (px >= cam.width || py >= cam.height)
+    {
+        return;
+    }
+    const float4 a = w.albedo[path];
+    const float4 e = w.emission[path];
+    const float4 r = w.radiance[path];
+    const V3 x = Add(Make(e.x, e.y, e.z), MulV(Make(a.x, a.y, a.z), Make(r.x, r.y, r.z)));
+    const uint64_t pixel = uint64_t(py) * cam.width + px;
+    if (out.color != nullptr)
+    {
+        float4 c = sample == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : out.color[pixel];
+        c = make_float4(c.x + x.x, c.y + x.y, c.z + x.z, a.w);
+        out.color[pixel] = c;
+    }
+    if (out.radiance != nullptr)
+    {
+        float4 c = sample == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : out.radiance[pixel];
+        c = make_float4(c.x + r.x, c.y + r.y, c.z + r.z, 0.0f);
+        out.radiance[pixel] = c;
+    }
+    if (out.attrib != nullptr)
+    {
+        out.attrib[pixel] = __float_as_uint(e.w);
+    }
 }
 
 __global__ void __launch_bounds__(256) RaysKernel(SceneView s, const float* rays, uint32_t count, uint32_t* hits)
@@ -220,10 +666,36 @@ int CheckLights(const CxLightTreeView* l, const char* what)
     return CX_OK;
 }
 
+// Workgroups of a persistent kernel: as many as fit on the device at once.
+template <class Kernel>
+uint32_t PersistentBlocks(Kernel kernel)
+{
+    static uint32_t blocks = 0;
+    if (blocks == 0)
+    {
+        int device = 0;
+        int cus = 0;
+        int per_cu = 0;
+        if (hipGetDevice(&device) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kWaveThreads, 0) != hipSuccess)
+        {
+            return 0;
+        }
+        blocks = uint32_t(std::max(1, per_cu) * std::max(1, cus));
+    }
+    return blocks;
+}
+
 }  // namespace
 }  // namespace cx
 
 extern "C" {
+
+CX_API uint64_t cx_trace_workspace_bytes(uint32_t width, uint32_t height)
+{
+    return cx::WorkspaceBytes(cx::PathCount(width, height));
+}
 
 CX_API int cx_trace_frame(const CxScene* scene, const CxLightTreeView* lights, const CxCamera* camera,
                           const CxTraceParams* params, const CxTraceOutput* out, void* stream)
@@ -244,14 +716,72 @@ CX_API int cx_trace_frame(const CxScene* scene, const CxLightTreeView* lights, c
     {
         return cx::ArgError("cx_trace_frame: empty image or 0 samples per pixel");
     }
+    if (params->bounces > 255)
+    {
+        return cx::ArgError("cx_trace_frame: more than 255 bounces");
+    }
+    if (uint64_t(camera->width) * camera->height > (uint64_t(1) << 28))
+    {
+        return cx::ArgError("cx_trace_frame: image larger than 2^28 pixels");
+    }
+    const uint32_t paths = cx::PathCount(camera->width, camera->height);
+    if (out->workspace == nullptr || out->workspace_bytes < cx::WorkspaceBytes(paths))
+    {
+        return cx::ArgError("cx_trace_frame: workspace missing or smaller than cx_trace_workspace_bytes");
+    }
     const hipStream_t s = static_cast<hipStream_t>(stream);
-    cx::TraceParams p{params->bounces, params->frame_seed, lights->mis, 0};
+    const cx::SceneView sv = cx::SceneOf(*scene);
+    const cx::LightView lv = cx::LightsOf(*lights);
+    const cx::CameraView cam = cx::CameraOf(*camera);
+    const cx::Workspace w = cx::CarveWorkspace(out->workspace, paths);
     cx::FrameOut o{reinterpret_cast<float4*>(out->color), reinterpret_cast<float4*>(out->radiance), out->attrib,
                    reinterpret_cast<unsigned long long*>(out->counters)};
-    const dim3 grid(cx::DivUp(camera->width, 8), cx::DivUp(camera->height, 8));
-    cx::StageScope scope(s, "trace_frame");
-    cx::FrameKernel<<<grid, 64, 0, s>>>(cx::SceneOf(*scene), cx::LightsOf(*lights), cx::CameraOf(*camera), p,
-                                        params->spp, o);
+    const uint32_t blocks = cx::DivUp(paths, cx::kWaveThreads);
+    const uint32_t trace_blocks_bounce = cx::PersistentBlocks(cx::PersistentTraceKernel<cx::BounceSource>);
+    const uint32_t trace_blocks_shadow = cx::PersistentBlocks(cx::PersistentTraceKernel<cx::ShadowSource>);
+    if (trace_blocks_bounce == 0 || trace_blocks_shadow == 0)
+    {
+        return cx::HipResult(hipErrorInvalidDeviceFunction, "cx_trace_frame: occupancy query");
+    }
+    const cx::ShadowSource shadow_source{w};
+    for (uint32_t sample = 0; sample < params->spp; ++sample)
+    {
+        const cx::TraceParams p{params->bounces, params->frame_seed + sample, lights->mis, 0};
+        cx::ResetKernel<<<1, 64, 0, s>>>(w);
+        {
+            cx::StageScope scope(s, "trace_primary");
+            cx::PrimaryTraceKernel<<<blocks, cx::kWaveThreads, 0, s>>>(sv, cam, w, paths, o.counters);
+        }
+        {
+            cx::StageScope scope(s, "trace_shade_primary");
+            cx::PrimaryShadeKernel<<<blocks, cx::kWaveThreads, 0, s>>>(sv, lv, cam, p, w, paths, o.counters);
+        }
+        {
+            cx::StageScope scope(s, "trace_shadow");
+            cx::PersistentTraceKernel<<<trace_blocks_shadow, cx::kWaveThreads, 0, s>>>(sv, shadow_source, o.counters);
+        }
+        for (uint32_t bounce = 1; bounce <= params->bounces; ++bounce)
+        {
+            {
+                cx::StageScope scope(s, "trace_extend");
+                cx::PersistentTraceKernel<<<trace_blocks_bounce, cx::kWaveThreads, 0, s>>>(
+                    sv, cx::BounceSource{w, (bounce - 1) & 1u}, o.counters);
+            }
+            {
+                cx::StageScope scope(s, "trace_shade_bounce");
+                cx::BounceShadeKernel<<<blocks, cx::kWaveThreads, 0, s>>>(sv, lv, p, w, bounce, paths, o.counters);
+            }
+            {
+                cx::StageScope scope(s, "trace_shadow");
+                cx::PersistentTraceKernel<<<trace_blocks_shadow, cx::kWaveThreads, 0, s>>>(sv, shadow_source,
+                                                                                            o.counters);
+            }
+        }
+        {
+            cx::StageScope scope(s, "trace_finish");
+            cx::FinishKernel<<<blocks, cx::kWaveThreads, 0, s>>>(cam, w, paths, sample, o);
+        }
+    }
     return cx::LaunchResult("cx_trace_frame");
 }
 

@@ -5,7 +5,8 @@
 // and the opaque-diffuse subset of the path tracer (shaders/render/caelix_path_trace.hlsl:276-452,
 // 737-978).
 //
-// Floating point: IEEE single operations in source order (no contraction; __fdiv_rn / __fsqrt_rn),
+// Floating point: IEEE single operations in source order (no contraction; IEEE division and a
+// correctly rounded square root),
 // so the CPU restatement reproduces every decision. sin/cos are the Cephes single-precision
 // polynomials (S. Moshier, Cephes Math Library, sinf.c / cosf.c) with Cody-Waite reduction, not the
 // device library's.
@@ -251,11 +252,11 @@ CX_DEV uint32_t FaceBits(uint32_t block, uint32_t encoding)
     return encoding == kEncodingPacked ? ((block >> kPackedFaceShift) & 0x3Fu) : ((block >> kBlockIdFaceShift) & 0x3Fu);
 }
 
-// CaelixTraceBrickRay (:299-368).
+// CaelixTraceBrickRay (:299-368). `inv` = SafeInvAxis of each component of dir (a function of the
+// ray only, so callers compute it once per ray).
 CX_DEV BrickHit TraceBrickRay(const uint64_t* occupancy, const uint16_t* payload, uint32_t encoding, V3 entry,
-                              V3 dir, float entry_t, uint32_t entry_normal_coarse)
+                              V3 dir, V3 inv, float entry_t, uint32_t entry_normal_coarse)
 {
-    const V3 inv = Make(SafeInvAxis(dir.x), SafeInvAxis(dir.y), SafeInvAxis(dir.z));
     const V3 boundary = Make(dir.x >= 0.0f ? 1.0f : 0.0f, dir.y >= 0.0f ? 1.0f : 0.0f, dir.z >= 0.0f ? 1.0f : 0.0f);
     const V3 t_start = MulV(Sub(boundary, entry), inv);
     const float hi = 8.0f - 0.0001f;
@@ -347,8 +348,24 @@ CX_DEV BrickHit TraceBrickRay(const uint64_t* occupancy, const uint16_t* payload
     return hit;
 }
 
+// The two reciprocals of a ray direction the brick test uses: 1 / d (the slab test,
+// caelix_brick_trace.hlsl:376-451, unguarded) and SafeInvAxis (the brick DDA).
+struct RayInv
+{
+    V3 raw;
+    V3 safe;
+};
+CX_DEV RayInv InverseOf(V3 dir)
+{
+    RayInv r;
+    r.raw = Make(Div(1.0f, dir.x), Div(1.0f, dir.y), Div(1.0f, dir.z));
+    r.safe = Make(SafeInvAxis(dir.x), SafeInvAxis(dir.y), SafeInvAxis(dir.z));
+    return r;
+}
+
 // CaelixTraceBrickPrimitiveCore (:376-451): slab test against the tight bounds, entry face, DDA.
-CX_DEV BrickHit TraceBrick(const SceneView& s, uint32_t record, V3 o, V3 dir, float t_current, V3 brick_origin)
+CX_DEV BrickHit TraceBrick(const SceneView& s, uint32_t record, V3 o, V3 dir, const RayInv& ri, float t_current,
+                           V3 brick_origin)
 {
     const uint2 info = s.info[record];
     const uint32_t tb = info.y;
@@ -356,9 +373,8 @@ CX_DEV BrickHit TraceBrick(const SceneView& s, uint32_t record, V3 o, V3 dir, fl
     const V3 tmax_box = Make(float((tb >> 9) & 7u) + 1.0f, float((tb >> 12) & 7u) + 1.0f, float((tb >> 15) & 7u) + 1.0f);
     const V3 aabb_min = Add(brick_origin, tmin_box);
     const V3 aabb_max = Add(brick_origin, tmax_box);
-    const V3 inv = Make(Div(1.0f, dir.x), Div(1.0f, dir.y), Div(1.0f, dir.z));
-    const V3 t0 = MulV(Sub(aabb_min, o), inv);
-    const V3 t1 = MulV(Sub(aabb_max, o), inv);
+    const V3 t0 = MulV(Sub(aabb_min, o), ri.raw);
+    const V3 t1 = MulV(Sub(aabb_max, o), ri.raw);
     const V3 tn = Make(fminf(t0.x, t1.x), fminf(t0.y, t1.y), fminf(t0.z, t1.z));
     const V3 tf = Make(fmaxf(t0.x, t1.x), fmaxf(t0.y, t1.y), fmaxf(t0.z, t1.z));
     const float largest_tmin = fmaxf(fmaxf(tn.x, tn.y), tn.z);
@@ -385,7 +401,7 @@ CX_DEV BrickHit TraceBrick(const SceneView& s, uint32_t record, V3 o, V3 dir, fl
     }
     const uint32_t coarse = (info.x >> kRecordCoarseShift) & kRecordCoarseMask;
     return TraceBrickRay(s.occupancy + uint64_t(record) * 8, s.payload + uint64_t(record) * 512, s.encoding, entry,
-                         dir, t, (flags << 26) | coarse);
+                         dir, ri.safe, t, (flags << 26) | coarse);
 }
 
 // ---- world traversal ------------------------------------------------------------------------------
@@ -412,6 +428,7 @@ CX_DEV bool BitSet(const uint32_t* bits, uint64_t index)
 CX_DEV WorldHit TraceWorld(const SceneView& s, V3 o, V3 dir, float t_max)
 {
     WorldHit result{0.0f, 0u, kHashNone, {0, 0, 0}, 0u, 0u, 0u};
+    const RayInv ri = InverseOf(dir);
     const float g0[3] = {float(s.grid_min[0] * 8), float(s.grid_min[1] * 8), float(s.grid_min[2] * 8)};
     const float extent = float(s.grid_dim * 8);
     const float ov[3] = {o.x, o.y, o.z};
@@ -478,7 +495,7 @@ CX_DEV WorldHit TraceWorld(const SceneView& s, V3 o, V3 dir, float t_max)
                     if (record != kHashNone)
                     {
                         const V3 origin = Make(float(key[0] * 8), float(key[1] * 8), float(key[2] * 8));
-                        const BrickHit h = TraceBrick(s, record, o, dir, t_max, origin);
+                        const BrickHit h = TraceBrick(s, record, o, dir, ri, t_max, origin);
                         result.steps += h.steps;
                         if (h.attrib != 0u && h.t < t_max)
                         {
@@ -538,6 +555,242 @@ CX_DEV WorldHit TraceWorld(const SceneView& s, V3 o, V3 dir, float t_max)
             return result;
         }
         t_cell = t_next[axis];
+    }
+}
+
+// Work counters of a lane (summed over a wave, then added to the frame's counters).
+struct PathCounters
+{
+    uint32_t rays;        // primary + bounce rays traced
+    uint32_t shadow_rays;
+    uint32_t cells;
+    uint32_t probes;
+    uint32_t steps;
+};
+
+// ---- resumable world traversal ------------------------------------------------------------------
+// TraceWorld as a state machine that advances one brick cell (or one super cell) per step, so that a
+// persistent kernel can give a lane a new ray as soon as its ray ends instead of leaving it idle
+// until the longest ray of its wave ends. Every float value is computed by the same expression as in
+// TraceWorld, so both give the same hit and the same cell / probe / step counts.
+constexpr uint32_t kRayDone = 0;     // finished
+constexpr uint32_t kRaySuper = 1;    // at the top of the super-cell loop for sc
+constexpr uint32_t kRayBricks = 2;   // at brick cell bc inside the resident super cell sc
+constexpr uint32_t kRayAdvance = 3;  // leaving super cell sc
+
+struct WorldRay
+{
+    V3 o;
+    V3 dir;
+    RayInv ri;
+    float t_max;
+    float t_exit;
+    float t_cell;
+    int sc0, sc1, sc2;
+    int bc0, bc1, bc2;
+    uint32_t phase;
+};
+
+CX_DEV float GridOrigin(const SceneView& s, int a)
+{
+    return float(s.grid_min[a] * 8);
+}
+
+CX_DEV int StepOf(float d)
+{
+    return d >= 0.0f ? 1 : -1;
+}
+
+// Starts a ray; phase kRayDone when it misses the grid.
+CX_DEV void WorldRayBegin(const SceneView& s, V3 o, V3 dir, float t_max, WorldRay& r)
+{
+    r.o = o;
+    r.dir = dir;
+    r.ri = InverseOf(dir);
+    r.t_max = t_max;
+    const float extent = float(s.grid_dim * 8);
+    float t_enter = 0.0f;
+    float t_exit = t_max;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        const float g0 = GridOrigin(s, a);
+        const float inv = Get(r.ri.safe, a);
+        const float t0 = (g0 - Get(o, a)) * inv;
+        const float t1 = (g0 + extent - Get(o, a)) * inv;
+        t_enter = fmaxf(t_enter, fminf(t0, t1));
+        t_exit = fminf(t_exit, fmaxf(t0, t1));
+    }
+    r.t_exit = t_exit;
+    r.t_cell = t_enter;
+    if (t_enter > t_exit)
+    {
+        r.phase = kRayDone;
+        return;
+    }
+    const int super_dim = int(s.grid_dim >> 3);
+    int sc[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        const float p = Get(o, a) + Get(dir, a) * t_enter;
+        const int c = int(floorf((p - GridOrigin(s, a)) * (1.0f / 64.0f)));
+        sc[a] = min(max(c, 0), super_dim - 1);
+    }
+    r.sc0 = sc[0];
+    r.sc1 = sc[1];
+    r.sc2 = sc[2];
+    r.phase = kRaySuper;
+}
+
+// One step of the ray: the super-cell test, at most one brick cell (with its brick DDA) and the
+// super-cell advance when the ray leaves it. On a hit or a miss the phase becomes kRayDone and `hit`
+// holds the result; the counters count this step's work.
+CX_DEV void WorldRayStep(const SceneView& s, WorldRay& r, WorldHit& hit, PathCounters& c)
+{
+    hit = WorldHit{0.0f, 0u, kHashNone, {0, 0, 0}, 0u, 0u, 0u};
+    const int super_dim = int(s.grid_dim >> 3);
+    const int st0 = StepOf(r.dir.x), st1 = StepOf(r.dir.y), st2 = StepOf(r.dir.z);
+    if (r.phase == kRaySuper)
+    {
+        const uint64_t super_index = (uint64_t(r.sc2) * super_dim + r.sc1) * super_dim + r.sc0;
+        if (BitSet(s.super_bits, super_index))
+        {
+            const int sc[3] = {r.sc0, r.sc1, r.sc2};
+            int bc[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+            {
+                const float p = Get(r.o, a) + Get(r.dir, a) * r.t_cell;
+                const int cc = int(floorf((p - GridOrigin(s, a)) * 0.125f));
+                bc[a] = min(max(cc, sc[a] * 8), sc[a] * 8 + 7);
+            }
+            r.bc0 = bc[0];
+            r.bc1 = bc[1];
+            r.bc2 = bc[2];
+            r.phase = kRayBricks;
+        }
+        else
+        {
+            r.phase = kRayAdvance;
+        }
+    }
+    if (r.phase == kRayBricks)
+    {
+        c.cells += 1;
+        const uint64_t brick_index = (uint64_t(r.bc2) * s.grid_dim + r.bc1) * s.grid_dim + r.bc0;
+        if (BitSet(s.brick_bits, brick_index))
+        {
+            const int32_t key0 = s.grid_min[0] + r.bc0;
+            const int32_t key1 = s.grid_min[1] + r.bc1;
+            const int32_t key2 = s.grid_min[2] + r.bc2;
+            uint32_t probes = 0;
+            const uint32_t record = HashGet(s.slots, s.slot_count, s.pow2, s.hash_info[0],
+                                            BrickFingerprint(key0, key1, key2), &probes);
+            c.probes += probes;
+            if (record != kHashNone)
+            {
+                const V3 origin = Make(float(key0 * 8), float(key1 * 8), float(key2 * 8));
+                const BrickHit h = TraceBrick(s, record, r.o, r.dir, r.ri, r.t_max, origin);
+                c.steps += h.steps;
+                if (h.attrib != 0u && h.t < r.t_max)
+                {
+                    hit.t = h.t;
+                    hit.attrib = h.attrib;
+                    hit.record = record;
+                    hit.brick[0] = key0;
+                    hit.brick[1] = key1;
+                    hit.brick[2] = key2;
+                    r.phase = kRayDone;
+                    return;
+                }
+            }
+        }
+        const float tb0 = (float((r.bc0 + (st0 > 0 ? 1 : 0)) * 8) + GridOrigin(s, 0) - r.o.x) * r.ri.safe.x;
+        const float tb1 = (float((r.bc1 + (st1 > 0 ? 1 : 0)) * 8) + GridOrigin(s, 1) - r.o.y) * r.ri.safe.y;
+        const float tb2 = (float((r.bc2 + (st2 > 0 ? 1 : 0)) * 8) + GridOrigin(s, 2) - r.o.z) * r.ri.safe.z;
+        int axis = 0;
+        float t_axis = tb0;
+        if (tb1 < t_axis)
+        {
+            axis = 1;
+            t_axis = tb1;
+        }
+        if (tb2 < t_axis)
+        {
+            axis = 2;
+            t_axis = tb2;
+        }
+        if (t_axis > r.t_max)
+        {
+            r.phase = kRayDone;
+            return;
+        }
+        bool left;
+        if (axis == 0)
+        {
+            r.bc0 += st0;
+            left = r.bc0 < r.sc0 * 8 || r.bc0 > r.sc0 * 8 + 7;
+        }
+        else if (axis == 1)
+        {
+            r.bc1 += st1;
+            left = r.bc1 < r.sc1 * 8 || r.bc1 > r.sc1 * 8 + 7;
+        }
+        else
+        {
+            r.bc2 += st2;
+            left = r.bc2 < r.sc2 * 8 || r.bc2 > r.sc2 * 8 + 7;
+        }
+        if (!left)
+        {
+            return;
+        }
+        r.phase = kRayAdvance;
+    }
+    if (r.phase == kRayAdvance)
+    {
+        const float tn0 = (float((r.sc0 + (st0 > 0 ? 1 : 0)) * 64) + GridOrigin(s, 0) - r.o.x) * r.ri.safe.x;
+        const float tn1 = (float((r.sc1 + (st1 > 0 ? 1 : 0)) * 64) + GridOrigin(s, 1) - r.o.y) * r.ri.safe.y;
+        const float tn2 = (float((r.sc2 + (st2 > 0 ? 1 : 0)) * 64) + GridOrigin(s, 2) - r.o.z) * r.ri.safe.z;
+        const float t_leave = fminf(fminf(tn0, tn1), tn2);
+        int axis = 0;
+        float t_axis = tn0;
+        if (tn1 < t_axis)
+        {
+            axis = 1;
+            t_axis = tn1;
+        }
+        if (tn2 < t_axis)
+        {
+            axis = 2;
+            t_axis = tn2;
+        }
+        if (t_leave > r.t_exit)
+        {
+            r.phase = kRayDone;
+            return;
+        }
+        int moved;
+        if (axis == 0)
+        {
+            moved = r.sc0 += st0;
+        }
+        else if (axis == 1)
+        {
+            moved = r.sc1 += st1;
+        }
+        else
+        {
+            moved = r.sc2 += st2;
+        }
+        if (moved < 0 || moved >= super_dim)
+        {
+            r.phase = kRayDone;
+            return;
+        }
+        r.t_cell = t_axis;
+        r.phase = kRaySuper;
     }
 }
 

@@ -240,10 +240,17 @@ typedef struct CxTraceOutput {
     float* radiance;
     uint32_t* attrib; /* primary hit attribute (0 = miss) */
     uint64_t* counters;
+    void* workspace;          /* device memory of cx_trace_workspace_bytes(width, height) bytes */
+    uint64_t workspace_bytes; /* its size */
 } CxTraceOutput;
 
-/* One frame: primary ray, brick DDA, light NEE + MIS, `bounces` bounces, for every pixel
- * (shaders/render/caelix_path_trace.hlsl:1050-, TracePath :737-978), opaque Lambertian subset. */
+/* Device workspace of cx_trace_frame for a width x height image: the path state, the path queues
+ * and the shadow-ray queue of the wavefront (about 184 bytes per pixel, 8x8-tile padded). */
+CX_API uint64_t cx_trace_workspace_bytes(uint32_t width, uint32_t height);
+
+/* One frame: primary ray, brick DDA, light NEE + MIS, `bounces` bounces (at most 255), for every
+ * pixel (shaders/render/caelix_path_trace.hlsl:1050-, TracePath :737-978), opaque Lambertian subset,
+ * as a wavefront of trace / shade / shadow launches over compacted path queues. */
 CX_API int cx_trace_frame(const CxScene* scene, const CxLightTreeView* lights, const CxCamera* camera,
                           const CxTraceParams* params, const CxTraceOutput* out, void* stream);
 

@@ -318,7 +318,7 @@ class OracleScene:
         color = np.zeros(count * 4, np.float64)
         radiance = np.zeros(count * 4, np.float64)
         attrib = np.zeros(count, np.uint32)
-        counters = np.zeros(6, np.uint64)
+        counters = np.zeros(11, np.uint64)
         _trace_lib().oracle_trace_frame(ctypes.byref(self.scene), ctypes.byref(self.view), ctypes.byref(cam),
                                         ctypes.byref(prm), r0, r1, _ptr(color), _ptr(radiance), _ptr(attrib),
                                         _ptr(counters))

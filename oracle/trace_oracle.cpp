@@ -161,9 +161,12 @@ void BrickKey(int32_t x, int32_t y, int32_t z, uint32_t& lo, uint32_t& hi)
     lo = std::max(uint32_t(h), 1u);
     hi = uint32_t(h >> 32) != 0 ? uint32_t(h >> 32) : 1u;
 }
-// Demand bytes: every global load the kernels make, counted at its size (no cache model). Used to
-// state the algorithmic bytes per ray of the trace; one counter per OpenMP thread.
-thread_local uint64_t g_demand = 0;
+// Demand bytes: every global load the kernels make, counted at its size (no cache model), per kind
+// of work: 0 primary-ray traversal, 1 bounce-ray traversal, 2 shadow-ray traversal, 3 shading at the
+// primary vertex (shade + NEE draw), 4 shading at bounce vertices (shade + MIS pdf + NEE draw). Used
+// to state the algorithmic bytes of each stage of the trace; one set per OpenMP thread.
+thread_local uint64_t g_demand[5] = {};
+thread_local int g_kind = 0;
 
 uint32_t HashGet(const CxScene& s, int32_t x, int32_t y, int32_t z, uint32_t& probes)
 {
@@ -175,7 +178,7 @@ uint32_t HashGet(const CxScene& s, int32_t x, int32_t y, int32_t z, uint32_t& pr
     for (uint32_t p = 0; p < limit; ++p)
     {
         const HashKeySlot& k = s.slots[slot];
-        g_demand += 16;
+        g_demand[g_kind] += 16;
         if (k.key_lo == 0)
         {
             probes = p + 1;
@@ -248,11 +251,11 @@ BrickHit TraceBrickRay(const uint64_t* occ, const uint16_t* payload, uint32_t en
         else
         {
             const uint32_t micro = uint32_t((cell[0] & 3) | ((cell[1] & 3) << 2) | ((cell[2] & 3) << 4));
-            g_demand += 8;
+            g_demand[g_kind] += 8;
             if (((occ[coarse_bit] >> micro) & 1ull) != 0ull)
             {
                 const uint32_t block = payload[cell[0] + 8 * cell[1] + 64 * cell[2]];
-                g_demand += 2;
+                g_demand[g_kind] += 2;
                 float T = 0.0f;
                 uint32_t flags;
                 if (step == 0)
@@ -301,7 +304,7 @@ BrickHit TraceBrickRay(const uint64_t* occ, const uint16_t* payload, uint32_t en
 BrickHit TraceBrick(const CxScene& s, uint32_t record, V o, V dir, float t_current, V brick_origin)
 {
     const uint32_t tb = s.info[2 * uint64_t(record) + 1];
-    g_demand += 8;
+    g_demand[g_kind] += 8;
     const V tmin_box = Mk(float(tb & 7u), float((tb >> 3) & 7u), float((tb >> 6) & 7u));
     const V tmax_box = Mk(float((tb >> 9) & 7u) + 1.0f, float((tb >> 12) & 7u) + 1.0f, float((tb >> 15) & 7u) + 1.0f);
     const V aabb_min = Add(brick_origin, tmin_box);
@@ -348,7 +351,7 @@ struct WorldHit
 
 bool Bit(const uint32_t* bits, uint64_t i)
 {
-    g_demand += 4;
+    g_demand[g_kind] += 4;
     return ((bits[i >> 5] >> (i & 31u)) & 1u) != 0u;
 }
 
@@ -488,7 +491,7 @@ float Power6(const float* f)
 }
 Summ SlotOf(const LightBvhNode& n, uint32_t s)
 {
-    g_demand += 48;
+    g_demand[g_kind] += 48;
     Summ r;
     for (int f = 0; f < 6; ++f)
     {
@@ -592,7 +595,7 @@ float PickProbability(float sum_i, float sum_p, float importance, float power)
 Summ TreeSummary(const CxLightTreeView& L, uint32_t k, uint32_t& root)
 {
     root = L.headers[k].root;
-    g_demand += 64;
+    g_demand[g_kind] += 64;
     Summ e;
     if (root == kNoneU)
     {
@@ -609,7 +612,7 @@ Summ TreeSummary(const CxLightTreeView& L, uint32_t k, uint32_t& root)
 }
 uint32_t ChildCount(const uint32_t* c)
 {
-    g_demand += 16;
+    g_demand[g_kind] += 16;
     return uint32_t(c[0] != kNoneU) + uint32_t(c[1] != kNoneU) + uint32_t(c[2] != kNoneU) + uint32_t(c[3] != kNoneU);
 }
 uint32_t ChooseLeaf(const CxLightTreeView& L, V x, V n, uint32_t& rng, float& pdf, uint32_t& tree)
@@ -719,7 +722,7 @@ float LeafProbability(const CxLightTreeView& L, uint32_t k, uint32_t leaf, V x, 
         return 0.0f;
     }
     const uint32_t parent = L.leaves[k][leaf].parent;
-    g_demand += 4;
+    g_demand[g_kind] += 4;
     if (parent == kNoneU)
     {
         return 0.0f;
@@ -737,7 +740,7 @@ float LeafProbability(const CxLightTreeView& L, uint32_t k, uint32_t leaf, V x, 
         }
         slot = nd.parent_slot;
         node = nd.parent;
-        g_demand += 8;
+        g_demand[g_kind] += 8;
     }
     return 0.0f;
 }
@@ -751,7 +754,7 @@ struct Rec
 Rec LoadRec(const CxLightTreeView& L, uint32_t ordinal)
 {
     const uint32_t* r = L.heap + uint64_t(L.record_offset[ordinal]) * 4;
-    g_demand += 4 + 48;
+    g_demand[g_kind] += 4 + 48;
     Rec rec;
     std::memcpy(rec.face, r, 24);
     for (int a = 0; a < 3; ++a)
@@ -819,7 +822,7 @@ bool LightTreeSample(const CxScene& s, const CxLightTreeView& L, V x, V n, uint3
         return false;
     }
     const LightBvhLeaf& lf = L.leaves[tree][leaf];
-    g_demand += 64 + 12;
+    g_demand[g_kind] += 64 + 12;
     const Rec record = LoadRec(L, lf.entry);
     const int32_t* key = s.keys + uint64_t(lf.group) * 3;
     const V origin = GroupOrigin(L, key);
@@ -859,7 +862,7 @@ bool LightTreeSample(const CxScene& s, const CxLightTreeView& L, V x, V n, uint3
     }
     const uint32_t index = std::min(uint32_t(LightRandomFloat(rng) * float(count)), count - 1u);
     const uint32_t entry = record.entries[first + index];
-    g_demand += 4;
+    g_demand[g_kind] += 4;
     const uint32_t vi = entry & 0x1FFu;
     const uint32_t mask = (1u << L.group_shift) - 1u;
     const uint32_t b[3] = {record.pos & mask, (record.pos >> L.group_shift) & mask, record.pos >> (2 * L.group_shift)};
@@ -882,13 +885,13 @@ float LightFacePdf(const CxScene& s, const CxLightTreeView& L, V x, V n, uint32_
 {
     found = false;
     const uint32_t code = block & 0xFFFFu;
-    g_demand += 4;
+    g_demand[g_kind] += 4;
     if (((s.light_codes[code >> 5] >> (code & 31u)) & 1u) == 0u || face > 5u)
     {
         return 0.0f;
     }
     const uint32_t ordinal = L.ordinal_of_brick[record];
-    g_demand += 4 + 12;
+    g_demand[g_kind] += 4 + 12;
     if (ordinal == kNoneU)
     {
         return 0.0f;
@@ -912,7 +915,7 @@ float LightFacePdf(const CxScene& s, const CxLightTreeView& L, V x, V n, uint32_
     bool hit = false;
     for (uint32_t e = 0; e < count; ++e)  // linear scan: the entries are sorted, any search finds it
     {
-        g_demand += 4;
+        g_demand[g_kind] += 4;
         if ((rec.entries[first + e] & 0x1FFu) == vi)
         {
             hit = true;
@@ -986,7 +989,7 @@ struct Shade
 Shade ShadeHit(const CxScene& s, uint32_t block, V hp, V normal, uint32_t& rng)
 {
     Shade sh;
-    g_demand += 32;
+    g_demand[g_kind] += 32;
     sh.emission = MatEmission(s, block);
     sh.albedo = MatAlbedo(s, block);
     (void)RandomFloat01(rng);
@@ -1024,7 +1027,7 @@ V NextEvent(const CxScene& s, const CxLightTreeView& L, V x_r, V n_r, uint32_t& 
                 direction = w;
                 t_max = distance - 2.0f * kPushOff;
                 contribution = Mul(MatEmission(s, p.block), scale);
-                g_demand += 16;
+                g_demand[g_kind] += 16;
             }
         }
     }
@@ -1036,7 +1039,10 @@ V NextEvent(const CxScene& s, const CxLightTreeView& L, V x_r, V n_r, uint32_t& 
         c.shadow += 1;
         if (t_max > 0.0f)
         {
+            const int kind = g_kind;
+            g_kind = 2;
             const WorldHit h = Trace(s, x_r, direction, t_max, c);
+            g_kind = kind;
             if (h.attrib == 0u)
             {
                 radiance = Mul(contribution, w_sum / w_lamp);
@@ -1100,7 +1106,9 @@ void TracePixel(const CxScene& s, const CxLightTreeView& L, const CxCamera& cam,
     t_hit = 0.0f;
     attrib = 0u;
     c.rays += 1;
+    g_kind = 0;
     const WorldHit primary = Trace(s, origin, dir, kRayTMax, c);
+    g_kind = 3;
     if (primary.attrib == 0u)
     {
         return;
@@ -1128,7 +1136,9 @@ void TracePixel(const CxScene& s, const CxLightTreeView& L, const CxCamera& cam,
     while (bounce_index <= bounces)
     {
         c.rays += 1;
+        g_kind = 1;
         const WorldHit hit = Trace(s, bo, bd, kRayTMax, c);
+        g_kind = 4;
         if (hit.attrib == 0u)
         {
             break;
@@ -1443,18 +1453,23 @@ void oracle_light_face_pdf(const CxScene* s, const CxLightTreeView* L, const uin
 }
 
 // A frame over pixel rows [row_begin, row_end): color (rgb + t, float64 rgb) and radiance. counters:
-// rays (primary + bounce), shadow rays, brick cells visited, hash probes, brick DDA steps, and the
-// demand bytes of every global load the kernels make for those pixels.
+// rays (primary + bounce), shadow rays, brick cells visited, hash probes, brick DDA steps, the demand
+// bytes of every global load the kernels make for those pixels, and that total split by kind of work
+// (primary, bounce and shadow traversal, primary-vertex and bounce-vertex shading).
 void oracle_trace_frame(const CxScene* s, const CxLightTreeView* L, const CxCamera* cam, const CxTraceParams* prm,
                         uint32_t row_begin, uint32_t row_end, double* color, double* radiance_out, uint32_t* attrib_out,
                         uint64_t* counters)
 {
-    uint64_t rays = 0, shadow = 0, cells = 0, probes = 0, steps = 0, demand = 0;
-#pragma omp parallel for schedule(dynamic, 1) reduction(+ : rays, shadow, cells, probes, steps, demand)
+    uint64_t rays = 0, shadow = 0, cells = 0, probes = 0, steps = 0;
+    uint64_t d0 = 0, d1 = 0, d2 = 0, d3 = 0, d4 = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : rays, shadow, cells, probes, steps, d0, d1, d2, d3, d4)
     for (int64_t py = row_begin; py < int64_t(row_end); ++py)
     {
         Counters local;
-        g_demand = 0;
+        for (int k = 0; k < 5; ++k)
+        {
+            g_demand[k] = 0;
+        }
         for (uint32_t px = 0; px < cam->width; ++px)
         {
             DV col, rad;
@@ -1489,7 +1504,11 @@ void oracle_trace_frame(const CxScene* s, const CxLightTreeView* L, const CxCame
         cells += local.cells;
         probes += local.probes;
         steps += local.steps;
-        demand += g_demand;
+        d0 += g_demand[0];
+        d1 += g_demand[1];
+        d2 += g_demand[2];
+        d3 += g_demand[3];
+        d4 += g_demand[4];
     }
     Counters c;
     c.rays = rays;
@@ -1502,7 +1521,12 @@ void oracle_trace_frame(const CxScene* s, const CxLightTreeView* L, const CxCame
     counters[2] = c.cells;
     counters[3] = c.probes;
     counters[4] = c.steps;
-    counters[5] = demand;
+    counters[5] = d0 + d1 + d2 + d3 + d4;
+    counters[6] = d0;
+    counters[7] = d1;
+    counters[8] = d2;
+    counters[9] = d3;
+    counters[10] = d4;
 }
 
 }  // extern "C"

@@ -180,3 +180,16 @@ def test_trace_rejects_bad_arguments(cx, pair):
         gpu.trace_frame(cam, 1, 0, 1)
     with pytest.raises(RuntimeError):
         gpu.trace_frame(SMALL.camera, 1, 0, 0)
+
+
+def test_trace_frame_odd_image_size(pair):
+    """An image that is not a multiple of the 8x8 path tiles: padding paths trace nothing."""
+    gpu, _, ref = pair
+    cam = dict(SMALL.camera)
+    cam["width"], cam["height"] = 100, 37
+    color, radiance, attrib, counters = gpu.trace_frame(cam, 2, 77, 1)
+    w_color, w_rad, w_attrib, w_counters = ref.trace_frame(cam, 2, 77, 1)
+    assert np.array_equal(attrib.cpu().numpy().view(np.uint32), w_attrib)
+    assert np.array_equal(counters.cpu().numpy().view(np.uint64)[:5], w_counters[:5])
+    rad = radiance.cpu().numpy().reshape(-1, 4)[:, :3].astype(np.float64)
+    assert np.all(np.abs(rad - w_rad[:, :3]) <= 2e-5 * (1.0 + np.abs(w_rad[:, :3])))
