@@ -148,31 +148,62 @@ CX_DEV uint32_t HashFind(const HashTableView& t, uint2 key, const uint32_t* word
 }
 
 // A read-only lookup of a bulk-built table (no touch, no pending states): the payload or kHashNone.
-// Probes until the key, an empty slot or the probe limit.
+// Probes until the key, an empty slot or the probe limit; `probes_out` = the slots that sequential
+// probing examines. The slots are read four at a time (independent loads of consecutive slots,
+// usually one or two 64-byte lines), so the typical lookup at load 0.7 (about 2.2 probes) costs one
+// memory round trip; the home slot is computed once (one integer division for a table whose size
+// is not a power of two).
 CX_DEV uint32_t HashGet(const HashKeySlot* keys, uint32_t slot_count, uint32_t pow2, uint32_t probe_limit, uint2 key,
                         uint32_t* probes_out = nullptr)
 {
-    HashTableView t{};
-    t.slot_count = slot_count;
-    t.pow2 = pow2;
-    for (uint32_t probe = 0; probe < probe_limit; ++probe)
+    const uint32_t home = pow2 != 0 ? (key.x & (slot_count - 1)) : (key.x % slot_count);
+    for (uint32_t base = 0; base < probe_limit; base += 4)
     {
-        const uint4 s = *reinterpret_cast<const uint4*>(keys + HashSlotIndex(t, key.x, probe));
-        if (s.x == 0u)
+        uint4 s[4];
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j)
         {
-            if (probes_out != nullptr)
+            // home + base + j < 2 * slot_count + 3: two subtractions wrap it when slot_count >= 4.
+            uint32_t index = home + base + j;
+            if (pow2 != 0)
             {
-                *probes_out = probe + 1;
+                index &= slot_count - 1;
             }
-            return kHashNone;
+            else if (slot_count >= 4)
+            {
+                index = index >= slot_count ? index - slot_count : index;
+                index = index >= slot_count ? index - slot_count : index;
+            }
+            else
+            {
+                index %= slot_count;
+            }
+            s[j] = *reinterpret_cast<const uint4*>(keys + index);
         }
-        if (s.x == key.x && s.y == key.y)
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j)
         {
-            if (probes_out != nullptr)
+            const uint32_t probe = base + j;
+            if (probe >= probe_limit)
             {
-                *probes_out = probe + 1;
+                break;
             }
-            return s.z;
+            if (s[j].x == 0u)
+            {
+                if (probes_out != nullptr)
+                {
+                    *probes_out = probe + 1;
+                }
+                return kHashNone;
+            }
+            if (s[j].x == key.x && s[j].y == key.y)
+            {
+                if (probes_out != nullptr)
+                {
+                    *probes_out = probe + 1;
+                }
+                return s[j].z;
+            }
         }
     }
     if (probes_out != nullptr)
