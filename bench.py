@@ -7,7 +7,8 @@ trace      (default) DDA + NEE at 1920 x 1080: 1,048,576 resident 8^3 bricks in 
            emissive bricks with emission log-uniform in [0.01, 100], 1 spp, 2 bounces, NEE through the
            light tree at the primary and bounce-1 vertices, float32 throughout. One step = one frame
            (the frame seed changes per step). value = Mrays/s: primary + bounce + shadow rays traced
-           per second. The light-tree rebuild below is measured in the same run as a second figure.
+           per second. The light-tree rebuild below and the differentiable form (adjoint / forward
+           time at 512x512, 4 spp, 3 bounces, 262,144 bricks) are measured in the same run.
 lighttree  full light-tree rebuild over 4,194,304 emissive brick leaves (half uniform in an 8192^3
            box, half in 64 Gaussian clusters, sigma 32 voxels, emission log-uniform in [0.01, 100]).
            One step = one rebuild from leaves resident in HBM. value = Mprims/s.
@@ -330,6 +331,79 @@ def lighttree_cpu_baseline(leaves) -> dict:
                       f"{runs} runs, single thread"}
 
 
+def diff_bench(args) -> dict:
+    """The differentiable form: 512x512, 4 spp, 3 bounces on the terrain scaled to 262,144 bricks
+    (16,384 emitters). Loss = squared error to a target rendered with the parameters perturbed by
+    seeded N(0, 0.05); gradients of the 768 albedos and 16,384 emission scales by the path-replay
+    adjoint. Reports forward and adjoint time (HIP events) and a central finite-difference check
+    (h = 1e-3, roulette off) of the two largest albedo and emission-scale gradients."""
+    import numpy as np
+    import torch
+
+    import caelix_amd
+    from caelix_amd import scene
+    from caelix_amd.diff import DifferentiableRenderer, TRACE_NO_RUSSIAN_ROULETTE
+
+    data = scene.diff_scene()
+    ts = caelix_amd.TraceScene(data, load_factor=0.7)
+    cam = data.camera
+    bounces, spp, seed = 3, 4, 7
+    stream = torch.cuda.current_stream()
+    rng = np.random.default_rng(11)
+    emitters = data.emissive_bricks
+    r = DifferentiableRenderer(ts, cam, bounces, spp, seed, emitters)
+    a0 = r.initial_albedo()
+    s0 = torch.ones(len(emitters), dtype=torch.float32, device="cuda")
+    a_t = (a0 + torch.from_numpy(rng.normal(0.0, 0.05, size=(256, 3)).astype(np.float32)).cuda()).clamp(0.01, 0.99)
+    s_t = s0 + torch.from_numpy(rng.normal(0.0, 0.05, size=len(emitters)).astype(np.float32)).cuda()
+    with torch.no_grad():
+        target = r.render(a_t, s_t).clone()
+    r.set_parameters(a0, s0)
+    steps = max(3, min(args.steps, 10))
+    fwd_ms = _time_steps(lambda _: r.forward_image(), steps, 1, stream)
+    image = r.forward_image().clone()
+    grad_image = 2.0 * (image - target)
+    adj_ms = _time_steps(lambda _: r.adjoint(grad_image), steps, 1, stream)
+
+    # finite differences at full size, roulette off
+    rf = DifferentiableRenderer(ts, cam, bounces, spp, seed, emitters, flags=TRACE_NO_RUSSIAN_ROULETTE)
+    with torch.no_grad():
+        target_f = rf.render(a_t, s_t).clone()
+    rf.set_parameters(a0, s0)
+    base = rf.forward_image().clone()
+    ga, gs = rf.adjoint(2.0 * (base - target_f))
+    ga = ga.double().cpu().numpy()
+    gs = gs.double().cpu().numpy()
+
+    def loss(a, sc):
+        rf.set_parameters(a, sc)
+        return float(((rf.forward_image().double() - target_f.double()) ** 2).sum())
+
+    h = 1e-3
+    checks = []
+    for flat in np.argsort(-np.abs(ga).ravel())[:2]:
+        m, c = divmod(int(flat), 3)
+        up, dn = a0.clone(), a0.clone()
+        up[m, c] += h
+        dn[m, c] -= h
+        fd = (loss(up, s0) - loss(dn, s0)) / (2 * h)
+        checks.append({"param": f"albedo[{m}][{c}]", "fd": fd, "adjoint": float(ga[m, c]),
+                       "rel_err": abs(fd - ga[m, c]) / max(abs(fd), 1e-30)})
+    for e in np.argsort(-np.abs(gs))[:2]:
+        up, dn = s0.clone(), s0.clone()
+        up[int(e)] += h
+        dn[int(e)] -= h
+        fd = (loss(a0, up) - loss(a0, dn)) / (2 * h)
+        checks.append({"param": f"scale[{int(e)}]", "fd": fd, "adjoint": float(gs[e]),
+                       "rel_err": abs(fd - gs[e]) / max(abs(fd), 1e-30)})
+    r.set_parameters(a0, s0)
+    return {"forward_ms": round(fwd_ms, 4), "adjoint_ms": round(adj_ms, 4), "adjoint_over_forward": round(adj_ms / fwd_ms, 3),
+            "config": {"workload": "differentiable: 512x512, 4 spp, 3 bounces, 262,144 bricks (2048^3 terrain), "
+                                   "16,384 emitters; 768 albedo + 16,384 emission-scale gradients",
+                       "bricks": int(len(data.keys)), "emitters": int(len(emitters))},
+            "fd_check": checks}
+
+
 def copy_bench(args) -> dict:
     """A plain device copy of 1 GiB (torch's copy kernel): the calibration run for the HBM byte
     counters (FETCH_SIZE / WRITE_SIZE per dispatch against 1 GiB read and 1 GiB written)."""
@@ -355,6 +429,7 @@ def main() -> int:
     parser.add_argument("--workload", default="trace", choices=["trace", "lighttree", "copy"])
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-lighttree", action="store_true", help="trace only, without the light-tree figure")
+    parser.add_argument("--no-adjoint", action="store_true", help="without the differentiable-form figure")
     args = parser.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the hot path runs on one GPU and this build has no multi-GPU "
@@ -365,6 +440,8 @@ def main() -> int:
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.workload == "trace":
         result = trace_bench(args)
+        if not args.no_adjoint:
+            result["adjoint"] = diff_bench(args)
         if not args.no_lighttree:
             lt = lighttree_bench(args, with_cpu=False)
             result["lighttree"] = {k: lt[k] for k in ("value", "unit", "ms_per_step", "roofline",
@@ -389,7 +466,7 @@ def main() -> int:
         "config": result["config"],
         "roofline": result["roofline"],
     }
-    for key in ("cpu_baseline", "parity", "lighttree", "pipeline_roofline", "stages"):
+    for key in ("cpu_baseline", "parity", "lighttree", "adjoint", "pipeline_roofline", "stages"):
         if key in result:
             line[key] = result[key]
     print(json.dumps(line))

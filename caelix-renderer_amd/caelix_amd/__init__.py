@@ -64,6 +64,7 @@ class CxScene(ctypes.Structure):
         ("super_bits", ctypes.c_void_p), ("grid_min", ctypes.c_int32 * 3), ("grid_dim", ctypes.c_uint32),
         ("group_shift", ctypes.c_uint32), ("encoding", ctypes.c_uint32), ("materials", ctypes.c_void_p),
         ("light_codes", ctypes.c_void_p), ("emission_scale", ctypes.c_float), ("pad", ctypes.c_uint32),
+        ("brick_emission_scale", ctypes.c_void_p),
     ]
 
 
@@ -93,7 +94,14 @@ class CxCamera(ctypes.Structure):
 
 class CxTraceParams(ctypes.Structure):
     _fields_ = [("bounces", ctypes.c_uint32), ("frame_seed", ctypes.c_uint32), ("spp", ctypes.c_uint32),
-                ("pad", ctypes.c_uint32)]
+                ("flags", ctypes.c_uint32)]
+
+
+TRACE_NO_RUSSIAN_ROULETTE = 1
+
+
+class CxTraceGradients(ctypes.Structure):
+    _fields_ = [("d_color", ctypes.c_void_p), ("grad_albedo", ctypes.c_void_p), ("grad_scale", ctypes.c_void_p)]
 
 
 class CxTraceOutput(ctypes.Structure):
@@ -138,6 +146,9 @@ SIGNATURES = {
     "cx_trace_workspace_bytes": (_U64, [_U32, _U32]),
     "cx_trace_frame": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightTreeView), ctypes.POINTER(CxCamera),
                               ctypes.POINTER(CxTraceParams), ctypes.POINTER(CxTraceOutput), _P]),
+    "cx_trace_frame_adjoint": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightTreeView), ctypes.POINTER(CxCamera),
+                                      ctypes.POINTER(CxTraceParams), ctypes.POINTER(CxTraceGradients),
+                                      ctypes.POINTER(CxTraceOutput), _P]),
     "cx_trace_rays": (_INT, [ctypes.POINTER(CxScene), _P, _U32, _P, _P]),
     "cx_light_sample": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightTreeView), _P, _P, _U32, _P, _P, _P]),
     "cx_light_face_pdf": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightTreeView), _P, _U32, _P, _P]),
@@ -461,8 +472,39 @@ class TraceScene:
         v.mis = 1 if mis else 0
         torch.cuda.synchronize()
 
+    def set_brick_scale(self, scale):
+        """Per-brick emission scales (a float32 device tensor of `bricks` values) or None (1)."""
+        self.brick_scale = scale
+        self.scene.brick_emission_scale = scale.data_ptr() if scale is not None else None
+
+    def _output(self, camera, color, radiance, attrib, counters):
+        w, h = int(camera["width"]), int(camera["height"])
+        key = (w, h)
+        if getattr(self, "_workspace_key", None) != key:
+            self.workspace = device_bytes(int(self.lib.cx_trace_workspace_bytes(max(w, 1), max(h, 1))), "cuda")
+            self._workspace_key = key
+        return CxTraceOutput(color.data_ptr() if color is not None else None,
+                             radiance.data_ptr() if radiance is not None else None,
+                             attrib.data_ptr() if attrib is not None else None, counters.data_ptr(),
+                             self.workspace.data_ptr(), self.workspace.numel())
+
+    def trace_frame_adjoint(self, camera: dict, bounces: int, seed: int, spp: int, d_color, grad_albedo, grad_scale,
+                            flags: int = 0, counters=None, stream=None):
+        """Adds the path-replay gradients for d_color = dL/dI (rgba per pixel, float32 device tensor)
+        to grad_albedo ([256, 4] float32) and grad_scale ([bricks] float32)."""
+        torch = _torch()
+        counters = counters if counters is not None else torch.zeros(8, dtype=torch.int64, device="cuda")
+        cam = camera_struct(camera)
+        params = CxTraceParams(bounces, seed, spp, flags)
+        grads = CxTraceGradients(d_color.data_ptr(), grad_albedo.data_ptr(), grad_scale.data_ptr())
+        out = self._output(camera, None, None, None, counters)
+        check(self.lib.cx_trace_frame_adjoint(ctypes.byref(self.scene), ctypes.byref(self.view), ctypes.byref(cam),
+                                              ctypes.byref(params), ctypes.byref(grads), ctypes.byref(out),
+                                              stream_handle(stream)), "cx_trace_frame_adjoint")
+        return counters
+
     def trace_frame(self, camera: dict, bounces: int, seed: int, spp: int = 1, color=None, radiance=None,
-                    attrib=None, counters=None, stream=None):
+                    attrib=None, counters=None, stream=None, flags: int = 0):
         torch = _torch()
         w, h = int(camera["width"]), int(camera["height"])
         color = color if color is not None else torch.empty(w * h * 4, dtype=torch.float32, device="cuda")
@@ -470,13 +512,8 @@ class TraceScene:
         attrib = attrib if attrib is not None else torch.empty(w * h, dtype=torch.int32, device="cuda")
         counters = counters if counters is not None else torch.zeros(8, dtype=torch.int64, device="cuda")
         cam = camera_struct(camera)
-        params = CxTraceParams(bounces, seed, spp, 0)
-        key = (w, h)
-        if getattr(self, "_workspace_key", None) != key:
-            self.workspace = device_bytes(int(self.lib.cx_trace_workspace_bytes(max(w, 1), max(h, 1))), "cuda")
-            self._workspace_key = key
-        out = CxTraceOutput(color.data_ptr(), radiance.data_ptr(), attrib.data_ptr(), counters.data_ptr(),
-                            self.workspace.data_ptr(), self.workspace.numel())
+        params = CxTraceParams(bounces, seed, spp, flags)
+        out = self._output(camera, color, radiance, attrib, counters)
         check(self.lib.cx_trace_frame(ctypes.byref(self.scene), ctypes.byref(self.view), ctypes.byref(cam),
                                       ctypes.byref(params), ctypes.byref(out), stream_handle(stream)), "cx_trace_frame")
         return color, radiance, attrib, counters

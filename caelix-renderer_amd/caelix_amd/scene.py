@@ -197,3 +197,10 @@ def furnace_box(size_bricks: int = 8, wall: int = 8, emission: float = 1.0, albe
     camera = look_at((c + 0.37, c + 0.21, c + 0.13), (c + 10.0, c - 3.0, c + 7.0), width, height, fov_y_deg=90.0)
     return SceneData(keys=keys, payload=payload, materials=table, light_codes=light_codes, grid_min=(0, 0, 0),
                      grid_dim=dim, group_shift=5, camera=camera, emissive_bricks=np.arange(len(keys)))
+
+
+def diff_scene(seed: int = 5) -> SceneData:
+    """The differentiable workload: the 1080p terrain scaled to 262,144 bricks (2048^3, 256^2 columns x
+    4 bricks), 16,384 emissive bricks (the emitters), emission log-uniform in [0.01, 100], 512 x 512."""
+    return terrain(2048, seed, octaves=8, bricks_per_column=4, emissive=16384, emission=(0.01, 100.0),
+                   log_uniform=True, fill=0.45, relief=0.35, width=512, height=512)

@@ -415,6 +415,7 @@ struct LightPoint
     uint32_t block;
     uint32_t face;
     float pdf;
+    uint32_t record;  // the brick record of the light
 };
 
 CX_DEV V3 FaceDirection(uint32_t f)
@@ -432,6 +433,7 @@ CX_DEV bool LightTreeSample(const SceneView& s, const LightView& L, V3 x, V3 n, 
     p.block = 0;
     p.face = 0;
     p.pdf = 0.0f;
+    p.record = kLtNone;
     float pdf_brick = 1.0f;
     uint32_t tree;
     const uint32_t leaf = ChooseLeaf(L, x, n, rng, pdf_brick, tree);
@@ -498,6 +500,7 @@ CX_DEV bool LightTreeSample(const SceneView& s, const LightView& L, V3 x, V3 n, 
     p.normal = FaceDirection(f);
     p.block = entry >> kLtFaceEntryBlockShift;
     p.face = f;
+    p.record = brick_record;
     p.pdf = Div(Div(w[f], sum), float(count)) * pdf_brick;
     return true;
 }

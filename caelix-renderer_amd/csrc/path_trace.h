@@ -15,7 +15,7 @@ struct TraceParams
     uint32_t bounces;
     uint32_t frame_seed;
     uint32_t mis;
-    uint32_t pad;
+    uint32_t roulette;  // 1: Russian roulette (the reference's), 0: paths run all bounces
 };
 
 struct Shade
@@ -73,15 +73,18 @@ CX_DEV WorldHit Trace(const SceneView& s, V3 o, V3 d, float t_max, PathCounters&
 // NextEventAtVertex (:356-413) without the sun: the lamp candidate (LampCandidate, :301-333). Returns
 // false when no shadow ray is counted (the candidate's weight is 0); else `direction` and `t_max`
 // give the shadow ray from x_r (t_max 0: counted but not traced) and `visible` the radiance it adds
-// when it reaches the light unoccluded.
+// when it reaches the light unoccluded. `unscaled` is `visible` without the light brick's emission
+// scale and `light_record` that brick (for the adjoint).
 CX_DEV bool NeeCandidate(const SceneView& s, const LightView& L, V3 x_r, V3 n_r, uint32_t& light_rng, V3& direction,
-                         float& t_max, V3& visible)
+                         float& t_max, V3& visible, V3& unscaled, uint32_t& light_record)
 {
     const uint32_t point_seed = RandomU32(light_rng);
     V3 contribution = Make(0.0f, 0.0f, 0.0f);
     direction = Make(0.0f, 0.0f, 0.0f);
     t_max = 0.0f;
     uint32_t rng = point_seed;
+    unscaled = Make(0.0f, 0.0f, 0.0f);
+    light_record = kHashNone;
     LightPoint p;
     if (LightTreeSample(s, L, x_r, n_r, rng, p) && p.pdf > 0.0f)
     {
@@ -102,7 +105,9 @@ CX_DEV bool NeeCandidate(const SceneView& s, const LightView& L, V3 x_r, V3 n_r,
                 }
                 direction = w;
                 t_max = distance - 2.0f * kPushOff;
-                contribution = Mul(BlockEmission(s, p.block), scale);
+                unscaled = Mul(BlockEmission(s, p.block), scale);
+                contribution = Mul(Mul(BlockEmission(s, p.block), BrickScale(s, p.record)), scale);
+                light_record = p.record;
             }
         }
     }

@@ -29,6 +29,7 @@ SceneView SceneOf(const CxScene& c)
     s.light_codes = c.light_codes;
     s.emission_scale = c.emission_scale;
     s.encoding = c.encoding;
+    s.brick_scale = c.brick_emission_scale;
     return s;
 }
 
@@ -117,9 +118,25 @@ struct Workspace
     float4* emission;   // [paths] primary emission, attrib bits
     float4* shadow_o;   // [paths] shadow ray origin, t_max
     float4* shadow_d;   // [paths] shadow ray direction, path bits
-    float4* shadow_add; // [paths] radiance the shadow ray adds when unoccluded
+    float4* shadow_add; // [paths] radiance the shadow ray adds when unoccluded, light record bits
+    float4* adjoint;    // [paths] adjoint: dL/d(radiance term) (rgb)
+    float4* shadow_grad; // [paths] adjoint: the shadow ray's radiance without the emission scale, roulette
     uint32_t* queue[2]; // [paths] path queues (ping-pong by bounce parity)
     uint32_t* ctl;      // [0], [1] queue lengths, [2] shadow queue length
+    float* partials;    // [kGradRows][768] adjoint: per-workgroup albedo gradient sums
+};
+
+// Albedo gradients: 256 materials x rgb, summed per workgroup in LDS and then in workgroup rows of
+// the workspace (no global atomics on the few hot materials); one final kernel sums the rows.
+constexpr uint32_t kGradEntries = 768;
+constexpr uint32_t kGradRowsMin = 4096;
+
+struct GradView
+{
+    const float4* d_color;  // null: the forward frame
+    float* grad_albedo;     // [256][4]
+    float* grad_scale;      // [bricks]
+    float inv_spp;
 };
 
 uint32_t TilesX(uint32_t width)
@@ -134,9 +151,14 @@ uint64_t Align256(uint64_t x)
 {
     return (x + 255) & ~uint64_t(255);
 }
+uint32_t GradRows(uint32_t paths)
+{
+    return std::max(kGradRowsMin, (paths + 255) / 256);
+}
 uint64_t WorkspaceBytes(uint32_t paths)
 {
-    return 11 * Align256(uint64_t(paths) * 16) + 2 * Align256(uint64_t(paths) * 4) + 256;
+    return 13 * Align256(uint64_t(paths) * 16) + 2 * Align256(uint64_t(paths) * 4) + 256 +
+           Align256(uint64_t(GradRows(paths)) * kGradEntries * 4);
 }
 Workspace CarveWorkspace(void* base, uint32_t paths)
 {
@@ -159,9 +181,12 @@ Workspace CarveWorkspace(void* base, uint32_t paths)
     w.shadow_o = reinterpret_cast<float4*>(take(v));
     w.shadow_d = reinterpret_cast<float4*>(take(v));
     w.shadow_add = reinterpret_cast<float4*>(take(v));
+    w.adjoint = reinterpret_cast<float4*>(take(v));
+    w.shadow_grad = reinterpret_cast<float4*>(take(v));
     w.queue[0] = reinterpret_cast<uint32_t*>(take(uint64_t(paths) * 4));
     w.queue[1] = reinterpret_cast<uint32_t*>(take(uint64_t(paths) * 4));
     w.ctl = reinterpret_cast<uint32_t*>(take(256));
+    w.partials = reinterpret_cast<float*>(take(uint64_t(GradRows(paths)) * kGradEntries * 4));
     return w;
 }
 
@@ -192,20 +217,112 @@ CX_DEV uint32_t WaveAppend(uint32_t* count, bool active)
     return base + uint32_t(PopC(mask & LanesBelow()));
 }
 
-CX_DEV void EnqueueShadow(const Workspace& w, uint32_t path, bool active, V3 origin, V3 direction, float t_max, V3 add)
+CX_DEV void EnqueueShadow(const Workspace& w, uint32_t path, bool active, V3 origin, V3 direction, float t_max, V3 add,
+                          uint32_t light, V3 unscaled, float roulette, bool adjoint)
 {
     const uint32_t slot = WaveAppend(&w.ctl[2], active);
     if (active)
     {
         w.shadow_o[slot] = make_float4(origin.x, origin.y, origin.z, t_max);
         w.shadow_d[slot] = make_float4(direction.x, direction.y, direction.z, __uint_as_float(path));
-        w.shadow_add[slot] = make_float4(add.x, add.y, add.z, 0.0f);
+        w.shadow_add[slot] = make_float4(add.x, add.y, add.z, __uint_as_float(light));
+        if (adjoint)
+        {
+            w.shadow_grad[slot] = make_float4(unscaled.x, unscaled.y, unscaled.z, roulette);
+        }
     }
 }
 
 CX_DEV uint32_t PackState(uint32_t segment_light, uint32_t bounce, uint32_t face)
 {
     return segment_light | (bounce << 8) | (face << 16);
+}
+
+// ---- adjoint helpers ---------------------------------------------------------------------------
+// The bounce vertices of a path whose albedos enter its later terms: material ids (block & 0xFF) of
+// vertices 1..3 in bytes 0..2, their count in byte 3.
+CX_DEV uint32_t VertexCount(uint32_t verts)
+{
+    return verts >> 24;
+}
+CX_DEV uint32_t VertexId(uint32_t verts, uint32_t j)
+{
+    return (verts >> (8 * j)) & 0xFFu;
+}
+CX_DEV uint32_t AppendVertex(uint32_t verts, uint32_t id)
+{
+    const uint32_t n = VertexCount(verts);
+    return n >= 3 ? verts : ((verts & 0x00FFFFFFu) | (id << (8 * n)) | ((n + 1) << 24));
+}
+CX_DEV V3 MaterialAlbedo(const SceneView& s, uint32_t id)
+{
+    return BlockAlbedo(s, kBlockIdOpaqueMask | id);
+}
+
+CX_DEV void LdsGradAdd(float* lds, uint32_t id, V3 v)
+{
+    atomicAdd(&lds[id * 3 + 0], v.x);
+    atomicAdd(&lds[id * 3 + 1], v.y);
+    atomicAdd(&lds[id * 3 + 2], v.z);
+}
+
+// A term T = base * prod_{j < n} albedo(vertex j) (per channel): adds adj * dT/d(albedo of vertex j)
+// = adj * base * prod_{i != j} albedo(vertex i) for every j, and returns T's albedo product.
+CX_DEV V3 AddAlbedoTerms(const SceneView& s, float* lds, V3 adj, V3 base, uint32_t verts, uint32_t n)
+{
+    V3 a[3];
+    V3 product = Make(1.0f, 1.0f, 1.0f);
+#pragma unroll
+    for (uint32_t j = 0; j < 3; ++j)
+    {
+        a[j] = j < n ? MaterialAlbedo(s, VertexId(verts, j)) : Make(1.0f, 1.0f, 1.0f);
+        product = MulV(product, a[j]);
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < 3; ++j)
+    {
+        if (j < n)
+        {
+            V3 others = base;
+#pragma unroll
+            for (uint32_t i = 0; i < 3; ++i)
+            {
+                if (i != j)
+                {
+                    others = MulV(others, a[i]);
+                }
+            }
+            LdsGradAdd(lds, VertexId(verts, j), MulV(adj, others));
+        }
+    }
+    return product;
+}
+
+CX_DEV void ScaleGradAdd(const GradView& g, uint32_t record, V3 adj, V3 d_term)
+{
+    const float v = adj.x * d_term.x + adj.y * d_term.y + adj.z * d_term.z;
+    if (v != 0.0f)
+    {
+        atomicAdd(&g.grad_scale[record], v);
+    }
+}
+
+CX_DEV void GradRowsBegin(float* lds)
+{
+    for (uint32_t i = threadIdx.x; i < kGradEntries; i += blockDim.x)
+    {
+        lds[i] = 0.0f;
+    }
+    __syncthreads();
+}
+CX_DEV void GradRowsEnd(const Workspace& w, float* lds)
+{
+    __syncthreads();
+    float* row = w.partials + uint64_t(blockIdx.x) * kGradEntries;
+    for (uint32_t i = threadIdx.x; i < kGradEntries; i += blockDim.x)
+    {
+        row[i] += lds[i];
+    }
 }
 
 // ---- persistent traversal ----------------------------------------------------------------------
@@ -272,7 +389,7 @@ struct BounceSource
         c.rays += 1;
         return true;
     }
-    CX_DEV void Finish(uint32_t item, const WorldHit& h) const
+    CX_DEV void Finish(uint32_t item, const WorldHit& h, float*) const
     {
         w.hit[w.queue[in][item]] = make_uint4(__float_as_uint(h.t), h.attrib, h.record, 0u);
     }
@@ -282,6 +399,8 @@ struct BounceSource
 struct ShadowSource
 {
     Workspace w;
+    SceneView sv;
+    GradView g;
     CX_DEV uint32_t Count() const { return w.ctl[2]; }
     CX_DEV uint32_t* Cursor() const { return &w.ctl[kCtlShadowCursor]; }
     CX_DEV void Prepare() const {}
@@ -294,7 +413,7 @@ struct ShadowSource
         t_max = o4.w;
         return true;
     }
-    CX_DEV void Finish(uint32_t item, const WorldHit& h) const
+    CX_DEV void Finish(uint32_t item, const WorldHit& h, float* lds) const
     {
         if (h.attrib != 0u)
         {
@@ -307,13 +426,32 @@ struct ShadowSource
         r.y = r.y + a.y;
         r.z = r.z + a.z;
         w.radiance[path] = r;
+        if (g.d_color != nullptr)
+        {
+            // T = unscaled * scale(light) * roulette * prod_{j <= k} albedo(vertex j)
+            const float4 sg = w.shadow_grad[item];
+            const uint32_t light = __float_as_uint(a.w);
+            const float4 adj4 = w.adjoint[path];
+            const V3 adj = Make(adj4.x, adj4.y, adj4.z);
+            const uint32_t verts = w.state[path].w;
+            const V3 unscaled = Mul(Make(sg.x, sg.y, sg.z), sg.w);
+            const V3 base = Mul(unscaled, BrickScale(sv, light));
+            const V3 product = AddAlbedoTerms(sv, lds, adj, base, verts, VertexCount(verts));
+            ScaleGradAdd(g, light, adj, MulV(unscaled, product));
+        }
     }
 };
 
 template <class Source>
 __global__ void __launch_bounds__(kWaveThreads) PersistentTraceKernel(SceneView s, Source src,
-                                                                      unsigned long long* counters)
+                                                                      unsigned long long* counters, Workspace w,
+                                                                      bool adjoint)
 {
+    __shared__ float lds_grad[kGradEntries];
+    if (adjoint)
+    {
+        GradRowsBegin(lds_grad);
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0)
     {
         src.Prepare();
@@ -358,7 +496,7 @@ __global__ void __launch_bounds__(kWaveThreads) PersistentTraceKernel(SceneView 
                     busy = r.phase != kRayDone;
                     if (!busy)
                     {
-                        src.Finish(item, WorldHit{0.0f, 0u, kHashNone, {0, 0, 0}, 0u, 0u, 0u});
+                        src.Finish(item, WorldHit{0.0f, 0u, kHashNone, {0, 0, 0}, 0u, 0u, 0u}, lds_grad);
                     }
                 }
             }
@@ -378,25 +516,32 @@ __global__ void __launch_bounds__(kWaveThreads) PersistentTraceKernel(SceneView 
             WorldRayStep(s, r, h, c);
             if (r.phase == kRayDone)
             {
-                src.Finish(item, h);
+                src.Finish(item, h, lds_grad);
                 busy = false;
             }
         }
+    }
+    if (adjoint)
+    {
+        GradRowsEnd(w, lds_grad);
     }
     FlushCounters(counters, c);
 }
 
 // The primary vertex: outputs of the hit, the first shade, the NEE at vertex 0 and the first
-// bounce ray (queue 0).
+// bounce ray (queue 0). Adjoint: the path's adjoint dL/dI / spp * albedo, and the primary emission's
+// emission-scale gradient.
 __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, LightView L, CameraView cam,
                                                                    TraceParams prm, Workspace w, uint32_t paths,
-                                                                   unsigned long long* counters)
+                                                                   GradView g, unsigned long long* counters)
 {
     const uint32_t path = blockIdx.x * kWaveThreads + threadIdx.x;
     PathCounters c{0, 0, 0, 0, 0};
     bool shadow = false;
     bool go_on = false;
     V3 shadow_origin = Make(0.0f, 0.0f, 0.0f), shadow_dir = Make(0.0f, 0.0f, 0.0f), shadow_add = Make(0.0f, 0.0f, 0.0f);
+    V3 shadow_unscaled = Make(0.0f, 0.0f, 0.0f);
+    uint32_t shadow_light = 0;
     float shadow_t = 0.0f;
     if (path < paths)
     {
@@ -420,14 +565,24 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
             const uint32_t face = FaceIndex(h.y);
             const V3 normal = FaceDirection(face);
             const Shade sh = ShadeHit(s, (h.y >> 16) & 0xFFFFu, Add(origin, Mul(dir, t)), normal, rng);
+            const V3 emission = Mul(sh.emission, BrickScale(s, h.z));
             w.albedo[path] = make_float4(sh.albedo.x, sh.albedo.y, sh.albedo.z, t);
-            w.emission[path] = make_float4(sh.emission.x, sh.emission.y, sh.emission.z, __uint_as_float(h.y));
+            w.emission[path] = make_float4(emission.x, emission.y, emission.z, __uint_as_float(h.y));
+            if (g.d_color != nullptr)
+            {
+                const float4 d = g.d_color[uint64_t(py) * cam.width + px];
+                const V3 dc = Mul(Make(d.x, d.y, d.z), g.inv_spp);
+                const V3 adj = MulV(dc, sh.albedo);
+                w.adjoint[path] = make_float4(adj.x, adj.y, adj.z, 0.0f);
+                ScaleGradAdd(g, h.z, dc, sh.emission);  // color = emission * scale + albedo * radiance
+            }
             const V3 throughput = Make(1.0f, 1.0f, 1.0f);
             uint32_t segment_light = 0u;
             if (prm.bounces >= 1u)
             {
                 V3 visible;
-                if (NeeCandidate(s, L, sh.bounce_origin, normal, light_rng, shadow_dir, shadow_t, visible))
+                if (NeeCandidate(s, L, sh.bounce_origin, normal, light_rng, shadow_dir, shadow_t, visible, shadow_unscaled,
+                                 shadow_light))
                 {
                     c.shadow_rays += 1;
                     shadow = shadow_t > 0.0f;
@@ -437,13 +592,14 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
                 segment_light = kSegmentLampNee | mis_bit;
                 go_on = true;
             }
-            w.throughput[path] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+            w.throughput[path] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);  // w: the roulette's normalisation
             w.state[path] = make_uint4(rng, light_rng, PackState(segment_light, 1u, face), 0u);
             w.ray_o[path] = make_float4(sh.bounce_origin.x, sh.bounce_origin.y, sh.bounce_origin.z, 0.0f);
             w.ray_d[path] = make_float4(sh.bounce_direction.x, sh.bounce_direction.y, sh.bounce_direction.z, 0.0f);
         }
     }
-    EnqueueShadow(w, path, shadow, shadow_origin, shadow_dir, shadow_t, shadow_add);
+    EnqueueShadow(w, path, shadow, shadow_origin, shadow_dir, shadow_t, shadow_add, shadow_light, shadow_unscaled, 1.0f,
+                  g.d_color != nullptr);
     const uint32_t slot = WaveAppend(&w.ctl[0], go_on);
     if (go_on)
     {
@@ -452,11 +608,18 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
     FlushCounters(counters, c);
 }
 
-// One iteration of TracePath's bounce loop for the paths in queue[(bounce - 1) & 1].
+// One iteration of TracePath's bounce loop for the paths in queue[(bounce - 1) & 1]. Adjoint: the
+// gradients of the emission term of this vertex.
 __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, LightView L, TraceParams prm, Workspace w,
-                                                                  uint32_t bounce, uint32_t paths,
+                                                                  uint32_t bounce, uint32_t paths, GradView g,
                                                                   unsigned long long* counters)
 {
+    __shared__ float lds_grad[kGradEntries];
+    const bool adjoint = g.d_color != nullptr;
+    if (adjoint)
+    {
+        GradRowsBegin(lds_grad);
+    }
     const uint32_t i = blockIdx.x * kWaveThreads + threadIdx.x;
     const uint32_t in = (bounce - 1) & 1u;
     if (i == 0)
@@ -469,7 +632,10 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, L
     bool go_on = false;
     uint32_t path = 0;
     V3 shadow_origin = Make(0.0f, 0.0f, 0.0f), shadow_dir = Make(0.0f, 0.0f, 0.0f), shadow_add = Make(0.0f, 0.0f, 0.0f);
+    V3 shadow_unscaled = Make(0.0f, 0.0f, 0.0f);
+    uint32_t shadow_light = 0;
     float shadow_t = 0.0f;
+    float shadow_roulette = 1.0f;
     if (i < count && i < paths)
     {
         path = w.queue[in][i];
@@ -482,12 +648,14 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, L
             uint32_t segment_light = st.z & 0xFFu;
             uint32_t bounce_index = (st.z >> 8) & 0xFFu;
             const uint32_t segment_face = st.z >> 16;
+            uint32_t verts = st.w;
             const float4 o4 = w.ray_o[path];
             const float4 d4 = w.ray_d[path];
             const V3 origin = Make(o4.x, o4.y, o4.z);
             const V3 dir = Make(d4.x, d4.y, d4.z);
             const float4 t4 = w.throughput[path];
             V3 throughput = Make(t4.x, t4.y, t4.z);
+            float roulette = t4.w;
             const float4 r4 = w.radiance[path];
             V3 radiance = Make(r4.x, r4.y, r4.z);
             WorldHit hit;
@@ -495,87 +663,161 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, L
             hit.attrib = h.y;
             hit.record = h.z;
             const uint32_t face = FaceIndex(h.y);
+            const uint32_t block = (h.y >> 16) & 0xFFFFu;
             const V3 vertex_normal = FaceDirection(face);
-            const Shade sh = ShadeHit(s, (h.y >> 16) & 0xFFFFu, Add(origin, Mul(dir, hit.t)), vertex_normal, rng);
+            const Shade sh = ShadeHit(s, block, Add(origin, Mul(dir, hit.t)), vertex_normal, rng);
+            const V3 emission = Mul(sh.emission, BrickScale(s, h.z));
             bounce_index += 1;
             float emission_weight = 1.0f;
             if (segment_light != 0u && (sh.emission.x > 0.0f || sh.emission.y > 0.0f || sh.emission.z > 0.0f))
             {
                 emission_weight = LampHitWeight(s, L, segment_light, hit, origin, dir, FaceDirection(segment_face));
             }
-            radiance = Add(radiance, MulV(sh.emission, Mul(throughput, emission_weight)));
+            radiance = Add(radiance, MulV(emission, Mul(throughput, emission_weight)));
+            if (adjoint && (sh.emission.x > 0.0f || sh.emission.y > 0.0f || sh.emission.z > 0.0f))
+            {
+                // T = emission * scale * weight * roulette * prod_{j < k} albedo(vertex j)
+                const float4 adj4 = w.adjoint[path];
+                const V3 adj = Make(adj4.x, adj4.y, adj4.z);
+                const V3 base = Mul(emission, emission_weight * roulette);
+                const V3 product = AddAlbedoTerms(s, lds_grad, adj, base, verts, VertexCount(verts));
+                ScaleGradAdd(g, h.z, adj, MulV(Mul(sh.emission, emission_weight * roulette), product));
+            }
             throughput = MulV(throughput, Make(Div(sh.albedo.x, 1.0f), Div(sh.albedo.y, 1.0f), Div(sh.albedo.z, 1.0f)));
+            verts = AppendVertex(verts, block & 0xFFu);
             segment_light = 0u;
             if (bounce_index <= prm.bounces)
             {
                 segment_light = kSegmentLampNee | (L.mis != 0u ? kSegmentLampMis : 0u);
                 V3 visible;
-                if (NeeCandidate(s, L, sh.bounce_origin, vertex_normal, light_rng, shadow_dir, shadow_t, visible))
+                if (NeeCandidate(s, L, sh.bounce_origin, vertex_normal, light_rng, shadow_dir, shadow_t, visible,
+                                 shadow_unscaled, shadow_light))
                 {
                     c.shadow_rays += 1;
                     shadow = shadow_t > 0.0f;
                     shadow_origin = sh.bounce_origin;
                     shadow_add = MulV(visible, Mul(throughput, 1.0f));
+                    shadow_roulette = roulette;
                 }
             }
             const float cp = fmaxf(throughput.x, fmaxf(throughput.y, throughput.z));
-            if (!(cp <= 0.0f || cp < RandomFloat01(rng)))
+            const float u = RandomFloat01(rng);
+            const bool survive = prm.roulette != 0u ? !(cp <= 0.0f || cp < u) : !(cp <= 0.0f);
+            if (survive)
             {
-                throughput = Mul(throughput, Div(1.0f, cp));
+                if (prm.roulette != 0u)
+                {
+                    const float inv_cp = Div(1.0f, cp);
+                    throughput = Mul(throughput, inv_cp);
+                    roulette = roulette * inv_cp;
+                }
                 go_on = bounce_index <= prm.bounces;
                 w.ray_o[path] = make_float4(sh.bounce_origin.x, sh.bounce_origin.y, sh.bounce_origin.z, 0.0f);
                 w.ray_d[path] = make_float4(sh.bounce_direction.x, sh.bounce_direction.y, sh.bounce_direction.z, 0.0f);
             }
             w.radiance[path] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
-            w.throughput[path] = make_float4(throughput.x, throughput.y, throughput.z, 0.0f);
-            w.state[path] = make_uint4(rng, light_rng, PackState(segment_light, bounce_index, face), 0u);
+            w.throughput[path] = make_float4(throughput.x, throughput.y, throughput.z, roulette);
+            w.state[path] = make_uint4(rng, light_rng, PackState(segment_light, bounce_index, face), verts);
         }
     }
-    EnqueueShadow(w, path, shadow, shadow_origin, shadow_dir, shadow_t, shadow_add);
+    EnqueueShadow(w, path, shadow, shadow_origin, shadow_dir, shadow_t, shadow_add, shadow_light, shadow_unscaled,
+                  shadow_roulette, adjoint);
     const uint32_t slot = WaveAppend(&w.ctl[1 - in], go_on);
     if (go_on)
     {
         w.queue[1 - in][slot] = path;
     }
     FlushCounters(counters, c);
+    if (adjoint)
+    {
+        GradRowsEnd(w, lds_grad);
+    }
 }
 
 // color += emission + albedo * radiance, radiance += radiance, t and attrib of the last sample.
+// Adjoint: the primary albedo's gradient dL/dI / spp * radiance.
 __global__ void __launch_bounds__(kWaveThreads) FinishKernel(CameraView cam, Workspace w, uint32_t paths, uint32_t sample,
-                                                             FrameOut out)
+                                                             FrameOut out, GradView g)
 {
+    __shared__ float lds_grad[kGradEntries];
+    const bool adjoint = g.d_color != nullptr;
+    if (adjoint)
+    {
+        GradRowsBegin(lds_grad);
+    }
     const uint32_t path = blockIdx.x * kWaveThreads + threadIdx.x;
-    if (path >= paths)
+    uint32_t px = 0, py = 0;
+    if (path < paths)
+    {
+        PathPixel(cam, path, px, py);
+    }
+    if (path < paths && px < cam.width && py < cam.height)
+    {
+        const float4 a = w.albedo[path];
+        const float4 e = w.emission[path];
+        const float4 r = w.radiance[path];
+        const uint64_t pixel = uint64_t(py) * cam.width + px;
+        if (adjoint)
+        {
+            const uint32_t attrib = __float_as_uint(e.w);
+            if (attrib != 0u)
+            {
+                const float4 d = g.d_color[pixel];
+                LdsGradAdd(lds_grad, (attrib >> 16) & 0xFFu,
+                           MulV(Mul(Make(d.x, d.y, d.z), g.inv_spp), Make(r.x, r.y, r.z)));
+            }
+        }
+        else
+        {
+            const V3 x = Add(Make(e.x, e.y, e.z), MulV(Make(a.x, a.y, a.z), Make(r.x, r.y, r.z)));
+            if (out.color != nullptr)
+            {
+                float4 c = sample == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : out.color[pixel];
+                c = make_float4(c.x + x.x, c.y + x.y, c.z + x.z, a.w);
+                out.color[pixel] = c;
+            }
+            if (out.radiance != nullptr)
+            {
+                float4 c = sample == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : out.radiance[pixel];
+                c = make_float4(c.x + r.x, c.y + r.y, c.z + r.z, 0.0f);
+                out.radiance[pixel] = c;
+            }
+            if (out.attrib != nullptr)
+            {
+                out.attrib[pixel] = __float_as_uint(e.w);
+            }
+        }
+    }
+    if (adjoint)
+    {
+        GradRowsEnd(w, lds_grad);
+    }
+}
+
+// Zeroes the albedo gradient rows of the workspace.
+__global__ void __launch_bounds__(256) ClearRowsKernel(float* rows, uint64_t count)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i < count)
+    {
+        rows[i] = 0.0f;
+    }
+}
+
+// grad_albedo[id][c] += sum over the rows (one thread per entry, a fixed order: deterministic).
+__global__ void __launch_bounds__(256) ReduceRowsKernel(const float* rows, uint32_t row_count, float* grad_albedo)
+{
+    const uint32_t e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= kGradEntries)
     {
         return;
     }
-    uint32_t px, py;
-    PathPixel(cam, path, px, py);
-    if (px >= cam.width || py >= cam.height)
+    float sum = 0.0f;
+    for (uint32_t r = 0; r < row_count; ++r)
     {
-        return;
+        sum += rows[uint64_t(r) * kGradEntries + e];
     }
-    const float4 a = w.albedo[path];
-    const float4 e = w.emission[path];
-    const float4 r = w.radiance[path];
-    const V3 x = Add(Make(e.x, e.y, e.z), MulV(Make(a.x, a.y, a.z), Make(r.x, r.y, r.z)));
-    const uint64_t pixel = uint64_t(py) * cam.width + px;
-    if (out.color != nullptr)
-    {
-        float4 c = sample == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : out.color[pixel];
-        c = make_float4(c.x + x.x, c.y + x.y, c.z + x.z, a.w);
-        out.color[pixel] = c;
-    }
-    if (out.radiance != nullptr)
-    {
-        float4 c = sample == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : out.radiance[pixel];
-        c = make_float4(c.x + r.x, c.y + r.y, c.z + r.z, 0.0f);
-        out.radiance[pixel] = c;
-    }
-    if (out.attrib != nullptr)
-    {
-        out.attrib[pixel] = __float_as_uint(e.w);
-    }
+    grad_albedo[(e / 3) * 4 + e % 3] += sum;
 }
 
 __global__ void __launch_bounds__(256) RaysKernel(SceneView s, const float* rays, uint32_t count, uint32_t* hits)
@@ -687,6 +929,122 @@ uint32_t PersistentBlocks(Kernel kernel)
     return blocks;
 }
 
+// The frame (grads null) or its adjoint replay (grads set): the same launches, in the same order.
+int RunFrame(const CxScene* scene, const CxLightTreeView* lights, const CxCamera* camera, const CxTraceParams* params,
+             const CxTraceGradients* grads, const CxTraceOutput* out, void* stream, const char* what)
+{
+    if (int r = CheckScene(scene, what); r != CX_OK)
+    {
+        return r;
+    }
+    if (int r = CheckLights(lights, what); r != CX_OK)
+    {
+        return r;
+    }
+    if (scene->light_codes == nullptr || camera == nullptr || params == nullptr || out == nullptr || out->counters == nullptr)
+    {
+        return ArgError("cx_trace_frame: NULL light codes, camera, params or counters");
+    }
+    if (camera->width == 0 || camera->height == 0 || params->spp == 0)
+    {
+        return ArgError("cx_trace_frame: empty image or 0 samples per pixel");
+    }
+    if (params->bounces > 255)
+    {
+        return ArgError("cx_trace_frame: more than 255 bounces");
+    }
+    if (grads != nullptr && params->bounces > 4)
+    {
+        return ArgError("cx_trace_frame_adjoint: at most 4 bounces (3 albedo vertices per path)");
+    }
+    if ((params->flags & ~CX_TRACE_NO_RUSSIAN_ROULETTE) != 0)
+    {
+        return ArgError("cx_trace_frame: unknown flags");
+    }
+    if (uint64_t(camera->width) * camera->height > (uint64_t(1) << 28))
+    {
+        return ArgError("cx_trace_frame: image larger than 2^28 pixels");
+    }
+    const uint32_t paths = PathCount(camera->width, camera->height);
+    if (out->workspace == nullptr || out->workspace_bytes < WorkspaceBytes(paths))
+    {
+        return ArgError("cx_trace_frame: workspace missing or smaller than cx_trace_workspace_bytes");
+    }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    const SceneView sv = SceneOf(*scene);
+    const LightView lv = LightsOf(*lights);
+    const CameraView cam = CameraOf(*camera);
+    const Workspace w = CarveWorkspace(out->workspace, paths);
+    FrameOut o{reinterpret_cast<float4*>(out->color), reinterpret_cast<float4*>(out->radiance), out->attrib,
+               reinterpret_cast<unsigned long long*>(out->counters)};
+    GradView g{nullptr, nullptr, nullptr, 1.0f / float(params->spp)};
+    const bool adjoint = grads != nullptr;
+    if (adjoint)
+    {
+        g.d_color = reinterpret_cast<const float4*>(grads->d_color);
+        g.grad_albedo = grads->grad_albedo;
+        g.grad_scale = grads->grad_scale;
+    }
+    const uint32_t blocks = DivUp(paths, kWaveThreads);
+    const uint32_t rows = GradRows(paths);
+    const uint32_t bounce_blocks = std::min(PersistentBlocks(PersistentTraceKernel<BounceSource>), rows);
+    const uint32_t shadow_blocks = std::min(PersistentBlocks(PersistentTraceKernel<ShadowSource>), rows);
+    if (bounce_blocks == 0 || shadow_blocks == 0)
+    {
+        return HipResult(hipErrorInvalidDeviceFunction, "cx_trace_frame: occupancy query");
+    }
+    if (adjoint)
+    {
+        const uint64_t n = uint64_t(rows) * kGradEntries;
+        ClearRowsKernel<<<uint32_t(DivUp(n, uint64_t(256))), 256, 0, s>>>(w.partials, n);
+    }
+    const ShadowSource shadow_source{w, sv, g};
+    for (uint32_t sample = 0; sample < params->spp; ++sample)
+    {
+        const TraceParams p{params->bounces, params->frame_seed + sample, lights->mis,
+                            (params->flags & CX_TRACE_NO_RUSSIAN_ROULETTE) != 0 ? 0u : 1u};
+        ResetKernel<<<1, 64, 0, s>>>(w);
+        {
+            StageScope scope(s, adjoint ? "adjoint_trace" : "trace_primary");
+            PrimaryTraceKernel<<<blocks, kWaveThreads, 0, s>>>(sv, cam, w, paths, o.counters);
+        }
+        {
+            StageScope scope(s, adjoint ? "adjoint_shade" : "trace_shade_primary");
+            PrimaryShadeKernel<<<blocks, kWaveThreads, 0, s>>>(sv, lv, cam, p, w, paths, g, o.counters);
+        }
+        {
+            StageScope scope(s, adjoint ? "adjoint_trace" : "trace_shadow");
+            PersistentTraceKernel<<<shadow_blocks, kWaveThreads, 0, s>>>(sv, shadow_source, o.counters, w, adjoint);
+        }
+        for (uint32_t bounce = 1; bounce <= params->bounces; ++bounce)
+        {
+            {
+                StageScope scope(s, adjoint ? "adjoint_trace" : "trace_extend");
+                PersistentTraceKernel<<<bounce_blocks, kWaveThreads, 0, s>>>(
+                    sv, BounceSource{w, (bounce - 1) & 1u}, o.counters, w, false);
+            }
+            {
+                StageScope scope(s, adjoint ? "adjoint_shade" : "trace_shade_bounce");
+                BounceShadeKernel<<<blocks, kWaveThreads, 0, s>>>(sv, lv, p, w, bounce, paths, g, o.counters);
+            }
+            {
+                StageScope scope(s, adjoint ? "adjoint_trace" : "trace_shadow");
+                PersistentTraceKernel<<<shadow_blocks, kWaveThreads, 0, s>>>(sv, shadow_source, o.counters, w,
+                                                                             adjoint);
+            }
+        }
+        {
+            StageScope scope(s, adjoint ? "adjoint_shade" : "trace_finish");
+            FinishKernel<<<blocks, kWaveThreads, 0, s>>>(cam, w, paths, sample, o, g);
+        }
+    }
+    if (adjoint)
+    {
+        ReduceRowsKernel<<<DivUp(kGradEntries, 256u), 256, 0, s>>>(w.partials, rows, g.grad_albedo);
+    }
+    return LaunchResult(what);
+}
+
 }  // namespace
 }  // namespace cx
 
@@ -700,89 +1058,18 @@ CX_API uint64_t cx_trace_workspace_bytes(uint32_t width, uint32_t height)
 CX_API int cx_trace_frame(const CxScene* scene, const CxLightTreeView* lights, const CxCamera* camera,
                           const CxTraceParams* params, const CxTraceOutput* out, void* stream)
 {
-    if (int r = cx::CheckScene(scene, "cx_trace_frame"); r != CX_OK)
+    return cx::RunFrame(scene, lights, camera, params, nullptr, out, stream, "cx_trace_frame");
+}
+
+CX_API int cx_trace_frame_adjoint(const CxScene* scene, const CxLightTreeView* lights, const CxCamera* camera,
+                                  const CxTraceParams* params, const CxTraceGradients* grads,
+                                  const CxTraceOutput* out, void* stream)
+{
+    if (grads == nullptr || grads->d_color == nullptr || grads->grad_albedo == nullptr || grads->grad_scale == nullptr)
     {
-        return r;
+        return cx::ArgError("cx_trace_frame_adjoint: NULL d_color or gradient buffer");
     }
-    if (int r = cx::CheckLights(lights, "cx_trace_frame"); r != CX_OK)
-    {
-        return r;
-    }
-    if (scene->light_codes == nullptr || camera == nullptr || params == nullptr || out == nullptr || out->counters == nullptr)
-    {
-        return cx::ArgError("cx_trace_frame: NULL light codes, camera, params or counters");
-    }
-    if (camera->width == 0 || camera->height == 0 || params->spp == 0)
-    {
-        return cx::ArgError("cx_trace_frame: empty image or 0 samples per pixel");
-    }
-    if (params->bounces > 255)
-    {
-        return cx::ArgError("cx_trace_frame: more than 255 bounces");
-    }
-    if (uint64_t(camera->width) * camera->height > (uint64_t(1) << 28))
-    {
-        return cx::ArgError("cx_trace_frame: image larger than 2^28 pixels");
-    }
-    const uint32_t paths = cx::PathCount(camera->width, camera->height);
-    if (out->workspace == nullptr || out->workspace_bytes < cx::WorkspaceBytes(paths))
-    {
-        return cx::ArgError("cx_trace_frame: workspace missing or smaller than cx_trace_workspace_bytes");
-    }
-    const hipStream_t s = static_cast<hipStream_t>(stream);
-    const cx::SceneView sv = cx::SceneOf(*scene);
-    const cx::LightView lv = cx::LightsOf(*lights);
-    const cx::CameraView cam = cx::CameraOf(*camera);
-    const cx::Workspace w = cx::CarveWorkspace(out->workspace, paths);
-    cx::FrameOut o{reinterpret_cast<float4*>(out->color), reinterpret_cast<float4*>(out->radiance), out->attrib,
-                   reinterpret_cast<unsigned long long*>(out->counters)};
-    const uint32_t blocks = cx::DivUp(paths, cx::kWaveThreads);
-    const uint32_t trace_blocks_bounce = cx::PersistentBlocks(cx::PersistentTraceKernel<cx::BounceSource>);
-    const uint32_t trace_blocks_shadow = cx::PersistentBlocks(cx::PersistentTraceKernel<cx::ShadowSource>);
-    if (trace_blocks_bounce == 0 || trace_blocks_shadow == 0)
-    {
-        return cx::HipResult(hipErrorInvalidDeviceFunction, "cx_trace_frame: occupancy query");
-    }
-    const cx::ShadowSource shadow_source{w};
-    for (uint32_t sample = 0; sample < params->spp; ++sample)
-    {
-        const cx::TraceParams p{params->bounces, params->frame_seed + sample, lights->mis, 0};
-        cx::ResetKernel<<<1, 64, 0, s>>>(w);
-        {
-            cx::StageScope scope(s, "trace_primary");
-            cx::PrimaryTraceKernel<<<blocks, cx::kWaveThreads, 0, s>>>(sv, cam, w, paths, o.counters);
-        }
-        {
-            cx::StageScope scope(s, "trace_shade_primary");
-            cx::PrimaryShadeKernel<<<blocks, cx::kWaveThreads, 0, s>>>(sv, lv, cam, p, w, paths, o.counters);
-        }
-        {
-            cx::StageScope scope(s, "trace_shadow");
-            cx::PersistentTraceKernel<<<trace_blocks_shadow, cx::kWaveThreads, 0, s>>>(sv, shadow_source, o.counters);
-        }
-        for (uint32_t bounce = 1; bounce <= params->bounces; ++bounce)
-        {
-            {
-                cx::StageScope scope(s, "trace_extend");
-                cx::PersistentTraceKernel<<<trace_blocks_bounce, cx::kWaveThreads, 0, s>>>(
-                    sv, cx::BounceSource{w, (bounce - 1) & 1u}, o.counters);
-            }
-            {
-                cx::StageScope scope(s, "trace_shade_bounce");
-                cx::BounceShadeKernel<<<blocks, cx::kWaveThreads, 0, s>>>(sv, lv, p, w, bounce, paths, o.counters);
-            }
-            {
-                cx::StageScope scope(s, "trace_shadow");
-                cx::PersistentTraceKernel<<<trace_blocks_shadow, cx::kWaveThreads, 0, s>>>(sv, shadow_source,
-                                                                                            o.counters);
-            }
-        }
-        {
-            cx::StageScope scope(s, "trace_finish");
-            cx::FinishKernel<<<blocks, cx::kWaveThreads, 0, s>>>(cam, w, paths, sample, o);
-        }
-    }
-    return cx::LaunchResult("cx_trace_frame");
+    return cx::RunFrame(scene, lights, camera, params, grads, out, stream, "cx_trace_frame_adjoint");
 }
 
 CX_API int cx_trace_rays(const CxScene* scene, const float* rays, uint32_t count, uint32_t* hits, void* stream)

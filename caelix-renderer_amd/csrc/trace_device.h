@@ -35,7 +35,13 @@ struct SceneView
     const uint32_t* light_codes; // 65536 bits: the block value is a light material
     float emission_scale;
     uint32_t encoding;
+    const float* brick_scale;    // [bricks] emission scale per brick, or null (1)
 };
+
+CX_DEV float BrickScale(const SceneView& s, uint32_t record)
+{
+    return s.brick_scale != nullptr ? s.brick_scale[record] : 1.0f;
+}
 
 struct LightView
 {

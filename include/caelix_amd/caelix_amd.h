@@ -173,6 +173,10 @@ typedef struct CxScene {
     const uint32_t* light_codes;
     float emission_scale;
     uint32_t pad;
+    /* [bricks] per-brick emission scale (the emitters' parameters of the differentiable form), or NULL
+     * for 1. Multiplies the emission the path tracer adds; the light tree samples with the unscaled
+     * emission. */
+    const float* brick_emission_scale;
 } CxScene;
 
 /* The record producer's opaque subset (shaders/scene/caelix_finalize.hlsl:112-361): word 0 (group
@@ -228,8 +232,13 @@ typedef struct CxTraceParams {
     uint32_t bounces;    /* bounce_count */
     uint32_t frame_seed;
     uint32_t spp;        /* samples per pixel; sample s uses frame_seed + s */
-    uint32_t pad;
+    uint32_t flags;      /* CX_TRACE_* */
 } CxTraceParams;
+
+/* No Russian roulette: paths run all `bounces` (the roulette's random number is still drawn, so the
+ * other decisions stay the same). The estimator is then a smooth function of the albedos and the
+ * emission scales at fixed random numbers, which central finite differences can check. */
+#define CX_TRACE_NO_RUSSIAN_ROULETTE 1u
 
 /* Per-pixel outputs (nullable except counters): color = emission + albedo * radiance (rgba, a =
  * primary hit t), radiance = TracePath's demodulated radiance, accumulated over the samples;
@@ -253,6 +262,25 @@ CX_API uint64_t cx_trace_workspace_bytes(uint32_t width, uint32_t height);
  * as a wavefront of trace / shade / shadow launches over compacted path queues. */
 CX_API int cx_trace_frame(const CxScene* scene, const CxLightTreeView* lights, const CxCamera* camera,
                           const CxTraceParams* params, const CxTraceOutput* out, void* stream);
+
+/* The adjoint of cx_trace_frame (a path-replay adjoint, the differentiable form): for the image
+ * I = color / spp of the same scene, camera and params, and d_color = dL/dI (rgba per pixel, a
+ * ignored), adds dL/d(albedo) and dL/d(emission scale) to the gradient buffers. The replay traces
+ * the same rays and draws the same lights from the same random streams as the forward frame (the
+ * decisions are recomputed, not stored; the path state between the wavefront's launches is the
+ * per-bounce checkpoint); sampling pdfs, MIS weights and the roulette's normalisation are held
+ * constant (detached). Albedo parameters are the material entries of the opaque blocks 0x8000 | id,
+ * id < 256: grad_albedo[id] (rgb + pad, 256 float4); emission scales are per brick record:
+ * grad_scale[record] (scene->bricks floats). The caller zeroes both; the adds are float atomics.
+ * No counterpart in the reference. */
+typedef struct CxTraceGradients {
+    const float* d_color; /* [height][width][4] */
+    float* grad_albedo;   /* [256][4] */
+    float* grad_scale;    /* [bricks] */
+} CxTraceGradients;
+CX_API int cx_trace_frame_adjoint(const CxScene* scene, const CxLightTreeView* lights, const CxCamera* camera,
+                                  const CxTraceParams* params, const CxTraceGradients* grads,
+                                  const CxTraceOutput* out, void* stream);
 
 /* Rays through the world (TraceScene / TraceShadow semantics): rays[i] = origin(3), direction(3),
  * t_max; hits[i] = {t (float bits), attrib, record, brick x, y, z, cells, probes, steps}. */

@@ -128,6 +128,7 @@ class _Scene(ctypes.Structure):
         ("bricks", _U32), ("hash_info", _P), ("brick_bits", _P), ("super_bits", _P),
         ("grid_min", ctypes.c_int32 * 3), ("grid_dim", _U32), ("group_shift", _U32), ("encoding", _U32),
         ("materials", _P), ("light_codes", _P), ("emission_scale", ctypes.c_float), ("pad", _U32),
+        ("brick_emission_scale", _P),
     ]
 
 
@@ -147,7 +148,10 @@ class _Camera(ctypes.Structure):
 
 
 class _Params(ctypes.Structure):
-    _fields_ = [("bounces", _U32), ("frame_seed", _U32), ("spp", _U32), ("pad", _U32)]
+    _fields_ = [("bounces", _U32), ("frame_seed", _U32), ("spp", _U32), ("flags", _U32)]
+
+
+NO_RUSSIAN_ROULETTE = 1
 
 
 def _trace_lib():
@@ -167,6 +171,9 @@ def _trace_lib():
         lib.oracle_trace_frame.restype = None
         lib.oracle_trace_frame.argtypes = [S, T, ctypes.POINTER(_Camera), ctypes.POINTER(_Params), _U32, _U32, _P, _P,
                                            _P, _P]
+        lib.oracle_trace_frame_adjoint.restype = None
+        lib.oracle_trace_frame_adjoint.argtypes = [S, T, ctypes.POINTER(_Camera), ctypes.POINTER(_Params), _U32, _U32,
+                                                   _P, _P, _P]
         lib._trace_bound = True
     return lib
 
@@ -216,6 +223,7 @@ class OracleScene:
         s.materials = _ptr(a["materials"])
         s.light_codes = _ptr(a["light_codes"])
         s.emission_scale = 1.0
+        s.brick_emission_scale = _ptr(a["brick_scale"]) if "brick_scale" in a else None
         if "headers" in a:
             v = self.view = _TreeView()
             v.headers = _ptr(a["headers"])
@@ -310,9 +318,35 @@ class OracleScene:
                                            _ptr(pdf))
         return pdf
 
-    def trace_frame(self, camera: dict, bounces: int, seed: int, spp: int = 1, rows=None):
+    def set_parameters(self, albedo=None, brick_scale=None):
+        """albedo [256, 3] float32: the albedo of the opaque materials 0x8000 | id; brick_scale
+        [bricks] float32: the emission scale of every brick (None: 1)."""
+        if albedo is not None:
+            m = self.a["materials"].reshape(65536, 8)
+            m[0x8000:0x8100, 0:3] = np.asarray(albedo, np.float32)
+        if brick_scale is not None:
+            self.a["brick_scale"] = np.ascontiguousarray(brick_scale, np.float32)
+        elif "brick_scale" in self.a:
+            del self.a["brick_scale"]
+        self._refresh()
+
+    def trace_frame_adjoint(self, camera: dict, bounces: int, seed: int, spp: int, d_color: np.ndarray, flags: int = 0,
+                            rows=None):
+        """float64 path-replay gradients: (grad_albedo [256, 3], grad_scale [bricks])."""
         cam = camera_struct(camera)
-        prm = _Params(bounces, seed, spp, 0)
+        prm = _Params(bounces, seed, spp, flags)
+        r0, r1 = rows if rows is not None else (0, cam.height)
+        d = np.ascontiguousarray(d_color, np.float32).reshape(-1)
+        assert d.size == cam.width * cam.height * 4
+        ga = np.zeros(256 * 3, np.float64)
+        gs = np.zeros(self.a["keys"].size // 3, np.float64)
+        _trace_lib().oracle_trace_frame_adjoint(ctypes.byref(self.scene), ctypes.byref(self.view), ctypes.byref(cam),
+                                                ctypes.byref(prm), r0, r1, _ptr(d), _ptr(ga), _ptr(gs))
+        return ga.reshape(256, 3), gs
+
+    def trace_frame(self, camera: dict, bounces: int, seed: int, spp: int = 1, rows=None, flags: int = 0):
+        cam = camera_struct(camera)
+        prm = _Params(bounces, seed, spp, flags)
         r0, r1 = rows if rows is not None else (0, cam.height)
         count = (r1 - r0) * cam.width
         color = np.zeros(count * 4, np.float64)

@@ -22,9 +22,12 @@
 //   path tracer       one-bounce ceiling energy against a closed form is not reachable for these
 //                     scenes: parity unpinned beyond the pieces above (documented)
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <vector>
+
+#include <omp.h>
 
 #include "caelix_amd/caelix_amd.h"
 #include "oracle_common.h"
@@ -810,10 +813,11 @@ struct LightPoint
     V position, normal;
     uint32_t block, face;
     float pdf;
+    uint32_t record;  // the light's brick record
 };
 bool LightTreeSample(const CxScene& s, const CxLightTreeView& L, V x, V n, uint32_t& rng, LightPoint& p)
 {
-    p = LightPoint{Mk(0, 0, 0), Mk(0, 0, 0), 0u, 0u, 0.0f};
+    p = LightPoint{Mk(0, 0, 0), Mk(0, 0, 0), 0u, 0u, 0.0f, kNoneU};
     float pdf_brick = 1.0f;
     uint32_t tree;
     const uint32_t leaf = ChooseLeaf(L, x, n, rng, pdf_brick, tree);
@@ -877,6 +881,7 @@ bool LightTreeSample(const CxScene& s, const CxLightTreeView& L, V x, V n, uint3
     p.normal = FaceDirection(f);
     p.block = entry >> 16;
     p.face = f;
+    p.record = lf.group;
     p.pdf = ((w[f] / sum) / float(count)) * pdf_brick;
     return true;
 }
@@ -999,8 +1004,24 @@ Shade ShadeHit(const CxScene& s, uint32_t block, V hp, V normal, uint32_t& rng)
     sh.direction = d;
     return sh;
 }
-V NextEvent(const CxScene& s, const CxLightTreeView& L, V x_r, V n_r, uint32_t& light_rng, Counters& c)
+float BrickScale(const CxScene& s, uint32_t record)
 {
+    return s.brick_emission_scale != nullptr ? s.brick_emission_scale[record] : 1.0f;
+}
+
+// What the adjoint needs of one NEE: the radiance it adds without the light brick's emission scale,
+// that brick, and whether it was added (unoccluded).
+struct NeeTerm
+{
+    bool visible = false;
+    V unscaled;
+    uint32_t light = kNoneU;
+};
+
+V NextEvent(const CxScene& s, const CxLightTreeView& L, V x_r, V n_r, uint32_t& light_rng, Counters& c, NeeTerm& term)
+{
+    term = NeeTerm{};
+    term.unscaled = Mk(0, 0, 0);
     const uint32_t seed = RandomU32(light_rng);
     V contribution = Mk(0, 0, 0);
     V direction = Mk(0, 0, 0);
@@ -1026,7 +1047,9 @@ V NextEvent(const CxScene& s, const CxLightTreeView& L, V x_r, V n_r, uint32_t& 
                 }
                 direction = w;
                 t_max = distance - 2.0f * kPushOff;
-                contribution = Mul(MatEmission(s, p.block), scale);
+                term.unscaled = Mul(MatEmission(s, p.block), scale);
+                term.light = p.record;
+                contribution = Mul(Mul(MatEmission(s, p.block), BrickScale(s, p.record)), scale);
                 g_demand[g_kind] += 16;
             }
         }
@@ -1046,6 +1069,7 @@ V NextEvent(const CxScene& s, const CxLightTreeView& L, V x_r, V n_r, uint32_t& 
             if (h.attrib == 0u)
             {
                 radiance = Mul(contribution, w_sum / w_lamp);
+                term.visible = true;
             }
         }
     }
@@ -1087,11 +1111,58 @@ void AddD(DV& a, V b)
     a.z += double(b.z);
 }
 
-// One pixel sample; radiance accumulated in float64 (the float32 terms are the kernel's).
-void TracePixel(const CxScene& s, const CxLightTreeView& L, const CxCamera& cam, uint32_t bounces, uint32_t seed,
-                uint32_t px, uint32_t py, DV& radiance, V& albedo, V& emission, float& t_hit, uint32_t& attrib,
-                Counters& c)
+// The gradient accumulators of the adjoint (float64): dL/dI / spp of the pixel, and the sums.
+struct Adjoint
 {
+    double dcol[3];
+    double* grad_albedo;  // [256][3]
+    double* grad_scale;   // [bricks]
+};
+
+// A term T = base * prod_j albedo(ids[j]) (per channel, float64): adds adj * dT/d(albedo of vertex j)
+// for every vertex and returns the product of the albedos.
+void AdjointAlbedoTerms(const CxScene& s, Adjoint& g, const double adj[3], const double base[3],
+                        const std::vector<uint32_t>& ids, double product[3])
+{
+    std::vector<std::array<double, 3>> a(ids.size());
+    for (int c = 0; c < 3; ++c)
+    {
+        product[c] = 1.0;
+    }
+    for (size_t j = 0; j < ids.size(); ++j)
+    {
+        const V al = MatAlbedo(s, 0x8000u | ids[j]);
+        a[j] = {double(al.x), double(al.y), double(al.z)};
+        for (int c = 0; c < 3; ++c)
+        {
+            product[c] *= a[j][c];
+        }
+    }
+    for (size_t j = 0; j < ids.size(); ++j)
+    {
+        for (int c = 0; c < 3; ++c)
+        {
+            double others = base[c];
+            for (size_t i = 0; i < ids.size(); ++i)
+            {
+                if (i != j)
+                {
+                    others *= a[i][c];
+                }
+            }
+            g.grad_albedo[ids[j] * 3 + c] += adj[c] * others;
+        }
+    }
+}
+
+// One pixel sample; radiance accumulated in float64 (the float32 terms are the kernel's). With
+// `adjoint`, also the path-replay gradients of the sample (sampling pdfs, MIS weights and the
+// roulette's normalisation held constant).
+void TracePixel(const CxScene& s, const CxLightTreeView& L, const CxCamera& cam, uint32_t bounces, uint32_t seed,
+                uint32_t flags, uint32_t px, uint32_t py, DV& radiance, V& albedo, V& emission, float& t_hit,
+                uint32_t& attrib, Counters& c, Adjoint* adjoint = nullptr)
+{
+    const bool roulette = (flags & CX_TRACE_NO_RUSSIAN_ROULETTE) == 0u;
     const float sx = float(px) + 0.5f;
     const float sy = float(py) + 0.5f;
     const float ndc_x = (sx / float(cam.width)) * 2.0f - 1.0f;
@@ -1122,12 +1193,47 @@ void TracePixel(const CxScene& s, const CxLightTreeView& L, const CxCamera& cam,
     const V primary_normal = FaceDirection(LowBit(primary.attrib & 0x3Fu));
     const Shade ps = ShadeHit(s, (primary.attrib >> 16) & 0xFFFFu, Add(origin, Mul(dir, primary.t)), primary_normal, rng);
     albedo = ps.albedo;
-    emission = ps.emission;
+    emission = Mul(ps.emission, BrickScale(s, primary.record));
+    double adj[3] = {0, 0, 0};
+    double rr = 1.0;
+    std::vector<uint32_t> ids;
+    if (adjoint != nullptr)
+    {
+        const double e[3] = {ps.emission.x, ps.emission.y, ps.emission.z};
+        const double a[3] = {ps.albedo.x, ps.albedo.y, ps.albedo.z};
+        double v = 0.0;
+        for (int k = 0; k < 3; ++k)
+        {
+            v += adjoint->dcol[k] * e[k];
+            adj[k] = adjoint->dcol[k] * a[k];
+        }
+        adjoint->grad_scale[primary.record] += v;
+    }
+    // An unoccluded NEE term: unscaled * scale(light) * rr * prod(albedos of ids).
+    auto nee_adjoint = [&](const NeeTerm& term) {
+        if (adjoint == nullptr || !term.visible)
+        {
+            return;
+        }
+        const double se = BrickScale(s, term.light);
+        const double u[3] = {double(term.unscaled.x) * rr, double(term.unscaled.y) * rr, double(term.unscaled.z) * rr};
+        const double base[3] = {u[0] * se, u[1] * se, u[2] * se};
+        double product[3];
+        AdjointAlbedoTerms(s, *adjoint, adj, base, ids, product);
+        double v = 0.0;
+        for (int k = 0; k < 3; ++k)
+        {
+            v += adj[k] * u[k] * product[k];
+        }
+        adjoint->grad_scale[term.light] += v;
+    };
     V throughput = Mk(1, 1, 1);
     V segment_normal = primary_normal;
     if (bounces >= 1u)
     {
-        AddD(radiance, MulV(NextEvent(s, L, ps.origin, primary_normal, light_rng, c), throughput));
+        NeeTerm term;
+        AddD(radiance, MulV(NextEvent(s, L, ps.origin, primary_normal, light_rng, c, term), throughput));
+        nee_adjoint(term);
         segment_light = 1u | mis_bit;
     }
     V bo = ps.origin;
@@ -1144,30 +1250,69 @@ void TracePixel(const CxScene& s, const CxLightTreeView& L, const CxCamera& cam,
             break;
         }
         const V vn = FaceDirection(LowBit(hit.attrib & 0x3Fu));
-        const Shade sh = ShadeHit(s, (hit.attrib >> 16) & 0xFFFFu, Add(bo, Mul(bd, hit.t)), vn, rng);
+        const uint32_t block = (hit.attrib >> 16) & 0xFFFFu;
+        const Shade sh = ShadeHit(s, block, Add(bo, Mul(bd, hit.t)), vn, rng);
+        const V scaled_emission = Mul(sh.emission, BrickScale(s, hit.record));
         bounce_index += 1;
         float ew = 1.0f;
         if (segment_light != 0u && (sh.emission.x > 0.0f || sh.emission.y > 0.0f || sh.emission.z > 0.0f))
         {
             ew = LampHitWeight(s, L, segment_light, hit, bo, bd, segment_normal);
         }
-        AddD(radiance, MulV(sh.emission, Mul(throughput, ew)));
+        AddD(radiance, MulV(scaled_emission, Mul(throughput, ew)));
+        if (adjoint != nullptr && (sh.emission.x > 0.0f || sh.emission.y > 0.0f || sh.emission.z > 0.0f))
+        {
+            const double w = double(ew) * rr;
+            const double base[3] = {scaled_emission.x * w, scaled_emission.y * w, scaled_emission.z * w};
+            double product[3];
+            AdjointAlbedoTerms(s, *adjoint, adj, base, ids, product);
+            double v = 0.0;
+            const double e[3] = {sh.emission.x, sh.emission.y, sh.emission.z};
+            for (int k = 0; k < 3; ++k)
+            {
+                v += adj[k] * e[k] * w * product[k];
+            }
+            adjoint->grad_scale[hit.record] += v;
+        }
         throughput = MulV(throughput, Mk(sh.albedo.x / 1.0f, sh.albedo.y / 1.0f, sh.albedo.z / 1.0f));
+        if (ids.size() < 3)
+        {
+            ids.push_back(block & 0xFFu);
+        }
         segment_light = 0u;
         if (bounce_index <= bounces)
         {
             segment_light = 1u | mis_bit;
-            AddD(radiance, MulV(NextEvent(s, L, sh.origin, vn, light_rng, c), Mul(throughput, 1.0f)));
+            NeeTerm term;
+            AddD(radiance, MulV(NextEvent(s, L, sh.origin, vn, light_rng, c, term), Mul(throughput, 1.0f)));
+            nee_adjoint(term);
         }
         const float cp = std::fmax(throughput.x, std::fmax(throughput.y, throughput.z));
-        if (cp <= 0.0f || cp < RandomFloat01(rng))
+        if (cp <= 0.0f)
         {
             break;
         }
-        throughput = Mul(throughput, 1.0f / cp);
+        const float u = RandomFloat01(rng);
+        if (roulette)
+        {
+            if (cp < u)
+            {
+                break;
+            }
+            const float inv_cp = 1.0f / cp;
+            throughput = Mul(throughput, inv_cp);
+            rr *= double(inv_cp);
+        }
         bo = sh.origin;
         bd = sh.direction;
         segment_normal = vn;
+    }
+    if (adjoint != nullptr)
+    {
+        const uint32_t id = (primary.attrib >> 16) & 0xFFu;
+        adjoint->grad_albedo[id * 3 + 0] += adjoint->dcol[0] * radiance.x;
+        adjoint->grad_albedo[id * 3 + 1] += adjoint->dcol[1] * radiance.y;
+        adjoint->grad_albedo[id * 3 + 2] += adjoint->dcol[2] * radiance.z;
     }
 }
 
@@ -1479,8 +1624,8 @@ void oracle_trace_frame(const CxScene* s, const CxLightTreeView* L, const CxCame
             {
                 DV r;
                 V albedo, emission;
-                TracePixel(*s, *L, *cam, prm->bounces, prm->frame_seed + sample, px, uint32_t(py), r, albedo, emission, t,
-                           attrib, local);
+                TracePixel(*s, *L, *cam, prm->bounces, prm->frame_seed + sample, prm->flags, px, uint32_t(py), r, albedo,
+                           emission, t, attrib, local);
                 col.x += double(emission.x) + double(albedo.x) * r.x;
                 col.y += double(emission.y) + double(albedo.y) * r.y;
                 col.z += double(emission.z) + double(albedo.z) * r.z;
@@ -1527,6 +1672,55 @@ void oracle_trace_frame(const CxScene* s, const CxLightTreeView* L, const CxCame
     counters[8] = d2;
     counters[9] = d3;
     counters[10] = d4;
+}
+
+// The adjoint of oracle_trace_frame over pixel rows [row_begin, row_end): for d_color = dL/dI of the
+// whole image (rgba per pixel, I = color / spp), adds dL/d(albedo of material id) to
+// grad_albedo[id][3] and dL/d(emission scale of brick) to grad_scale[brick], in float64.
+void oracle_trace_frame_adjoint(const CxScene* s, const CxLightTreeView* L, const CxCamera* cam,
+                                const CxTraceParams* prm, uint32_t row_begin, uint32_t row_end, const float* d_color,
+                                double* grad_albedo, double* grad_scale)
+{
+    const int threads = omp_get_max_threads();
+    std::vector<std::vector<double>> albedo_parts(threads, std::vector<double>(256 * 3, 0.0));
+    std::vector<std::vector<double>> scale_parts(threads, std::vector<double>(s->bricks, 0.0));
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t py = row_begin; py < int64_t(row_end); ++py)
+    {
+        const int t = omp_get_thread_num();
+        Counters local;
+        for (uint32_t px = 0; px < cam->width; ++px)
+        {
+            const float* d = d_color + (uint64_t(py) * cam->width + px) * 4;
+            Adjoint adj;
+            for (int k = 0; k < 3; ++k)
+            {
+                adj.dcol[k] = double(d[k]) / double(prm->spp);
+            }
+            adj.grad_albedo = albedo_parts[t].data();
+            adj.grad_scale = scale_parts[t].data();
+            for (uint32_t sample = 0; sample < prm->spp; ++sample)
+            {
+                DV r;
+                V albedo, emission;
+                float th;
+                uint32_t attrib;
+                TracePixel(*s, *L, *cam, prm->bounces, prm->frame_seed + sample, prm->flags, px, uint32_t(py), r, albedo,
+                           emission, th, attrib, local, &adj);
+            }
+        }
+    }
+    for (int t = 0; t < threads; ++t)
+    {
+        for (int i = 0; i < 256 * 3; ++i)
+        {
+            grad_albedo[i] += albedo_parts[t][i];
+        }
+        for (uint32_t b = 0; b < s->bricks; ++b)
+        {
+            grad_scale[b] += scale_parts[t][b];
+        }
+    }
 }
 
 }  // extern "C"
