@@ -132,6 +132,9 @@ SIGNATURES = {
     "cx_light_bvh_scratch_init": (_INT, [_P, _U32, _P]),
     "cx_light_bvh_begin": (_INT, [_P, _P, _U32, _U32, _U32, _P]),
     "cx_light_bvh_build": (_INT, [ctypes.POINTER(CxLightBvhBuild), _P]),
+    "cx_light_bvh_batch_bytes": (_U64, [_U32]),
+    "cx_light_bvh_batch_prepare": (_INT, [ctypes.POINTER(CxLightBvhBuild), _U32, _P, _P]),
+    "cx_light_bvh_build_batch": (_INT, [_P, _U32, _U32, _P]),
     "cx_radix_sort_state_bytes": (_U64, [_U32]),
     "cx_radix_sort_pairs": (_INT, [_P, _P, _P, _P, _P, _P, _U32, _P, _P]),
     "cx_hash_fingerprint": (_INT, [_P, _U32, _U32, _P, _P]),
@@ -256,6 +259,25 @@ class LightBvhBuilder:
             "leaves": to_host(self.leaves).view(layout.LEAF).copy(),
             "scratch": to_host(self.scratch).copy(),
         }
+
+
+class LightBvhBatch:
+    """Many light trees built by one sequence of launches (cx_light_bvh_build_batch): the builders'
+    leaves, states and buffers stay theirs; build() rebuilds every tree whose state is building."""
+
+    def __init__(self, builders, stream=None):
+        self.lib = load()
+        self.builders = list(builders)
+        n = len(self.builders)
+        array = (CxLightBvhBuild * n)(*[b._desc for b in self.builders])
+        self.descriptors = device_bytes(int(self.lib.cx_light_bvh_batch_bytes(n)), "cuda")
+        check(self.lib.cx_light_bvh_batch_prepare(array, n, self.descriptors.data_ptr(), stream_handle(stream)),
+              "cx_light_bvh_batch_prepare")
+        self.max_capacity = max(b.leaf_capacity for b in self.builders)
+
+    def build(self, stream=None) -> None:
+        check(self.lib.cx_light_bvh_build_batch(self.descriptors.data_ptr(), len(self.builders), self.max_capacity,
+                                                stream_handle(stream)), "cx_light_bvh_build_batch")
 
 
 def radix_sort_pairs(keys, values, stream=None):

@@ -22,6 +22,8 @@
 // node and one scan.
 #include "device_common.h"
 #include "radix_sort.h"
+
+#include <vector>
 #include "runtime.h"
 
 namespace cx {
@@ -159,6 +161,25 @@ struct Build
 CX_DEV bool Building(const Build& b)
 {
     return b.state->building != 0;
+}
+
+// The builds a launch works on: one (the plain entry point) or one per scene of a batch, selected by
+// blockIdx.y. Every kernel sizes its own part of the grid from its build's capacity, so a batch
+// launch covers the largest build and the workgroups beyond a smaller build's grid return at once.
+struct OneBuild
+{
+    Build b;
+    CX_DEV const Build& Get() const { return b; }
+};
+struct BuildArray
+{
+    const Build* builds;
+    CX_DEV const Build& Get() const { return builds[blockIdx.y]; }
+};
+
+__host__ __device__ inline uint32_t BoundBlocks(uint32_t capacity)
+{
+    return SortTiles(capacity) < kBoundsMaxBlocks ? SortTiles(capacity) : kBoundsMaxBlocks;
 }
 
 struct Summary
@@ -320,9 +341,12 @@ CX_DEV void MinMax3(float (&lo)[3], float (&hi)[3], float* lds6x4, uint32_t wave
     }
 }
 
-__global__ void __launch_bounds__(kBvhThreads) BoundsKernel(Build b)
+template <class Set>
+__global__ void __launch_bounds__(kBvhThreads) BoundsKernel(Set set)
 {
-    if (!Building(b))
+    const Build& b = set.Get();
+    const uint32_t grid = BoundBlocks(b.leaf_capacity);
+    if (blockIdx.x >= grid || !Building(b))
     {
         return;
     }
@@ -332,7 +356,7 @@ __global__ void __launch_bounds__(kBvhThreads) BoundsKernel(Build b)
     float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f};
     float hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
     float* centres = b.scratch.centres();
-    for (uint32_t i = blockIdx.x * kBvhThreads + t; i < n; i += gridDim.x * kBvhThreads)
+    for (uint32_t i = blockIdx.x * kBvhThreads + t; i < n; i += grid * kBvhThreads)
     {
         const float* leaf = b.leaves[i].summary.lo;
         const float2 l01 = *reinterpret_cast<const float2*>(leaf);       // lo.x, lo.y
@@ -381,9 +405,12 @@ __global__ void __launch_bounds__(kBvhThreads) BoundsKernel(Build b)
 // morton (caelix_light_bvh_morton.hlsl:5-28) + the sort's digit histograms and status clear.
 // One workgroup per sort tile (4096 leaves). Every workgroup first reduces the bounds partials (the
 // second half of caelix_light_bvh_bounds); workgroup 0 writes leaf_count, leaves_dropped and bounds.
-__global__ void __launch_bounds__(kSortThreads) MortonKernel(Build b, uint32_t bound_blocks)
+template <class Set>
+__global__ void __launch_bounds__(kSortThreads) MortonKernel(Set set)
 {
-    if (!Building(b))
+    const Build& b = set.Get();
+    const uint32_t bound_blocks = BoundBlocks(b.leaf_capacity);
+    if (blockIdx.x >= SortTiles(b.leaf_capacity) || !Building(b))
     {
         return;
     }
@@ -630,9 +657,11 @@ CX_DEV bool ClimbStep(const Build& b, int n, uint32_t& a, uint32_t& last, uint64
 }
 
 // Level 1: the workgroup's own 2048 sorted leaves. Level L > 1: the tops of its 32 sub-blocks.
-__global__ void __launch_bounds__(1024) TreeLevelKernel(Build b, uint32_t level)
+template <class Set>
+__global__ void __launch_bounds__(1024) TreeLevelKernel(Set set, uint32_t level)
 {
-    if (!Building(b))
+    const Build& b = set.Get();
+    if (level > TreeLevels(b.leaf_capacity) || blockIdx.x >= TreeBlocks(b.leaf_capacity, level) || !Building(b))
     {
         return;
     }
@@ -818,9 +847,11 @@ CX_DEV int32_t TileExclusiveScan(int32_t (&v)[kScanItems], int32_t* lds_waves)
 }
 
 // cover[g] = sum of diff[0 .. g] for the gaps g < n - 1.
-__global__ void __launch_bounds__(kScanThreads) CoverScanKernel(Build b)
+template <class Set>
+__global__ void __launch_bounds__(kScanThreads) CoverScanKernel(Set set)
 {
-    if (!Building(b))
+    const Build& b = set.Get();
+    if (blockIdx.x >= b.scratch.scan_tiles() || !Building(b))
     {
         return;
     }
@@ -871,9 +902,11 @@ __global__ void __launch_bounds__(kScanThreads) CoverScanKernel(Build b)
 // Depth, even-depth flags and their exclusive scan (the reference's depth kernel + scan over the
 // leaf capacity, caelix_light_bvh_depth.hlsl:7-30), then the header in the last tile
 // (caelix_light_bvh_finish.hlsl:8-63).
-__global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Build b)
+template <class Set>
+__global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Set set)
 {
-    if (!Building(b))
+    const Build& b = set.Get();
+    if (blockIdx.x >= b.scratch.scan_tiles() || !Building(b))
     {
         return;
     }
@@ -919,7 +952,7 @@ __global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Build b)
             wide[base + k] = uint32_t(prefix + v[k]);
         }
     }
-    if (tile + 1 != gridDim.x || threadIdx.x != 0)
+    if (tile + 1 != b.scratch.scan_tiles() || threadIdx.x != 0)
     {
         return;
     }
@@ -983,8 +1016,10 @@ __global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Build b)
 
 // ------------------------------------------------------------------------------------------------
 // wide (caelix_light_bvh_wide.hlsl:10-100)
-__global__ void __launch_bounds__(kBvhThreads) WideKernel(Build b)
+template <class Set>
+__global__ void __launch_bounds__(kBvhThreads) WideKernel(Set set)
 {
+    const Build& b = set.Get();
     const uint32_t i = blockIdx.x * kBvhThreads + threadIdx.x;
     if (i + 1 >= b.leaf_capacity || !Building(b))
     {
@@ -1077,6 +1112,63 @@ __global__ void __launch_bounds__(kBvhThreads) WideKernel(Build b)
 }
 
 // caelix_light_bvh_begin.hlsl:9-17
+// One onesweep pass of the build's sort (keys / values ping-pong in the scratch).
+template <class Set>
+__global__ void __launch_bounds__(kSortThreads) BuildSortPassKernel(Set set, uint32_t pass)
+{
+    const Build& b = set.Get();
+    if (blockIdx.x >= SortTiles(b.leaf_capacity))
+    {
+        return;
+    }
+    const uint32_t in = pass & 1u;
+    SortPassTile(b.scratch.keys(in), b.scratch.values(in), b.scratch.keys(1 - in), b.scratch.values(1 - in),
+                 &b.header->leaf_count, &b.state->building, b.scratch.sort(), pass);
+}
+
+// The build sequence (light_tree_record.cpp:271-308) for a set of builds; `capacity` = the largest
+// leaf capacity of the set, `count` = the number of builds (grid y).
+template <class Set>
+void RecordBuild(const Set& set, uint32_t capacity, uint32_t count, hipStream_t s)
+{
+    const uint32_t per_item_blocks = DivUp(capacity, kBvhThreads);
+    const uint32_t tiles = SortTiles(capacity);
+    {
+        StageScope scope(s, "lt_bounds");
+        BoundsKernel<<<dim3(BoundBlocks(capacity), count), kBvhThreads, 0, s>>>(set);
+    }
+    {
+        StageScope scope(s, "lt_morton");
+        MortonKernel<<<dim3(tiles, count), kSortThreads, 0, s>>>(set);
+    }
+    for (uint32_t pass = 0; pass < kSortPasses; ++pass)
+    {
+        StageScope scope(s, "sort_pass");
+        BuildSortPassKernel<<<dim3(tiles, count), kSortThreads, 0, s>>>(set, pass);
+    }
+    static const char* const kTreeStage[] = {"lt_tree_l1", "lt_tree_l2", "lt_tree_l3", "lt_tree_l4", "lt_tree_l5",
+                                             "lt_tree_l6"};
+    for (uint32_t level = 1; level <= TreeLevels(capacity); ++level)
+    {
+        StageScope scope(s, kTreeStage[std::min(level, 6u) - 1]);
+        const uint32_t threads = level == 1 ? kTreeThreads : 1024u;
+        TreeLevelKernel<<<dim3(TreeBlocks(capacity, level), count), threads, 0, s>>>(set, level);
+    }
+    const uint32_t scan_tiles = DivUp(capacity, kScanTile);
+    {
+        StageScope scope(s, "lt_cover_scan");
+        CoverScanKernel<<<dim3(scan_tiles, count), kScanThreads, 0, s>>>(set);
+    }
+    {
+        StageScope scope(s, "lt_flags_scan");
+        FlagsScanKernel<<<dim3(scan_tiles, count), kScanThreads, 0, s>>>(set);
+    }
+    {
+        StageScope scope(s, "lt_wide");
+        WideKernel<<<dim3(per_item_blocks, count), kBvhThreads, 0, s>>>(set);
+    }
+}
+
 __global__ void BeginKernel(const LightControl* control, LightTreeState* state, uint32_t tree, uint32_t reason,
                             uint32_t pass_recorded)
 {
@@ -1094,6 +1186,34 @@ __global__ void ScratchInitKernel(uint32_t* ctl)
 }
 
 }  // namespace
+}  // namespace cx
+
+namespace cx {
+// Validates a build description and converts it (argument errors log and return, changing nothing).
+int CheckBuild(const CxLightBvhBuild* build, const char* what, Build& b)
+{
+    if (build == nullptr || build->leaves == nullptr || build->nodes == nullptr || build->header == nullptr ||
+        build->state == nullptr || build->scratch == nullptr)
+    {
+        return ArgError("%s: a buffer is NULL", what);
+    }
+    if (build->leaf_capacity == 0 || build->leaf_capacity % 64 != 0 || build->leaf_capacity >= (1u << 30))
+    {
+        return ArgError("%s: leaf capacity %u is not a positive multiple of 64 below 2^30", what, build->leaf_capacity);
+    }
+    if (build->node_capacity == 0)
+    {
+        return ArgError("%s: node capacity 0", what);
+    }
+    b.leaves = build->leaves;
+    b.nodes = build->nodes;
+    b.header = build->header;
+    b.state = build->state;
+    b.scratch = Scratch{static_cast<uint8_t*>(build->scratch), build->leaf_capacity};
+    b.leaf_capacity = build->leaf_capacity;
+    b.node_capacity = build->node_capacity;
+    return CX_OK;
+}
 }  // namespace cx
 
 extern "C" {
@@ -1127,72 +1247,53 @@ CX_API int cx_light_bvh_begin(LightControl* control, LightTreeState* state, uint
 
 CX_API int cx_light_bvh_build(const CxLightBvhBuild* build, void* stream)
 {
-    if (build == nullptr || build->leaves == nullptr || build->nodes == nullptr || build->header == nullptr ||
-        build->state == nullptr || build->scratch == nullptr)
+    cx::Build b;
+    if (int r = cx::CheckBuild(build, "cx_light_bvh_build", b); r != CX_OK)
     {
-        return cx::ArgError("cx_light_bvh_build: a buffer is NULL");
-    }
-    if (build->leaf_capacity == 0 || build->leaf_capacity % 64 != 0 || build->leaf_capacity >= (1u << 30))
-    {
-        return cx::ArgError("cx_light_bvh_build: leaf capacity %u is not a positive multiple of 64 below 2^30",
-                            build->leaf_capacity);
-    }
-    if (build->node_capacity == 0)
-    {
-        return cx::ArgError("cx_light_bvh_build: node capacity 0");
+        return r;
     }
     const hipStream_t s = static_cast<hipStream_t>(stream);
-    cx::Build b;
-    b.leaves = build->leaves;
-    b.nodes = build->nodes;
-    b.header = build->header;
-    b.state = build->state;
-    b.scratch = cx::Scratch{static_cast<uint8_t*>(build->scratch), build->leaf_capacity};
-    b.leaf_capacity = build->leaf_capacity;
-    b.node_capacity = build->node_capacity;
-    const uint32_t capacity = build->leaf_capacity;
-    const uint32_t per_item_blocks = cx::DivUp(capacity, cx::kBvhThreads);
-    const uint32_t tiles = cx::SortTiles(capacity);
-    const uint32_t bound_blocks = std::min(tiles, cx::kBoundsMaxBlocks);
-    {
-        cx::StageScope scope(s, "lt_bounds");
-        cx::BoundsKernel<<<bound_blocks, cx::kBvhThreads, 0, s>>>(b);
-    }
-    {
-        cx::StageScope scope(s, "lt_morton");
-        cx::MortonKernel<<<tiles, cx::kSortThreads, 0, s>>>(b, bound_blocks);
-    }
-    const cx::SortState sort = b.scratch.sort();
-    const uint32_t* gate = &build->state->building;
-    const uint32_t* count = &build->header->leaf_count;
-    for (uint32_t pass = 0; pass < cx::kSortPasses; ++pass)
-    {
-        const uint32_t in = pass & 1u;
-        cx::LaunchSortPass(b.scratch.keys(in), b.scratch.values(in), b.scratch.keys(1 - in), b.scratch.values(1 - in),
-                           count, gate, sort, pass, s);
-    }
-    static const char* const kTreeStage[] = {"lt_tree_l1", "lt_tree_l2", "lt_tree_l3", "lt_tree_l4", "lt_tree_l5",
-                                             "lt_tree_l6"};
-    for (uint32_t level = 1; level <= cx::TreeLevels(capacity); ++level)
-    {
-        cx::StageScope scope(s, kTreeStage[std::min(level, 6u) - 1]);
-        const uint32_t threads = level == 1 ? cx::kTreeThreads : 1024u;
-        cx::TreeLevelKernel<<<cx::TreeBlocks(capacity, level), threads, 0, s>>>(b, level);
-    }
-    const uint32_t scan_tiles = b.scratch.scan_tiles();
-    {
-        cx::StageScope scope(s, "lt_cover_scan");
-        cx::CoverScanKernel<<<scan_tiles, cx::kScanThreads, 0, s>>>(b);
-    }
-    {
-        cx::StageScope scope(s, "lt_flags_scan");
-        cx::FlagsScanKernel<<<scan_tiles, cx::kScanThreads, 0, s>>>(b);
-    }
-    {
-        cx::StageScope scope(s, "lt_wide");
-        cx::WideKernel<<<per_item_blocks, cx::kBvhThreads, 0, s>>>(b);
-    }
+    cx::RecordBuild(cx::OneBuild{b}, build->leaf_capacity, 1, s);
     return cx::LaunchResult("cx_light_bvh_build");
+}
+
+CX_API uint64_t cx_light_bvh_batch_bytes(uint32_t count)
+{
+    return uint64_t(count) * sizeof(cx::Build);
+}
+
+CX_API int cx_light_bvh_batch_prepare(const CxLightBvhBuild* builds, uint32_t count, void* descriptors, void* stream)
+{
+    if (builds == nullptr || descriptors == nullptr || count == 0)
+    {
+        return cx::ArgError("cx_light_bvh_batch_prepare: no builds or no descriptor buffer");
+    }
+    std::vector<cx::Build> host(count);
+    for (uint32_t i = 0; i < count; ++i)
+    {
+        if (int r = cx::CheckBuild(&builds[i], "cx_light_bvh_batch_prepare", host[i]); r != CX_OK)
+        {
+            return r;
+        }
+    }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    return cx::HipResult(hipMemcpyAsync(descriptors, host.data(), host.size() * sizeof(cx::Build), hipMemcpyHostToDevice, s),
+                         "cx_light_bvh_batch_prepare");
+}
+
+CX_API int cx_light_bvh_build_batch(const void* descriptors, uint32_t count, uint32_t max_leaf_capacity, void* stream)
+{
+    if (descriptors == nullptr || count == 0 || count > 65535)
+    {
+        return cx::ArgError("cx_light_bvh_build_batch: no descriptors or %u builds (1..65535)", count);
+    }
+    if (max_leaf_capacity == 0 || max_leaf_capacity % 64 != 0 || max_leaf_capacity >= (1u << 30))
+    {
+        return cx::ArgError("cx_light_bvh_build_batch: largest leaf capacity %u invalid", max_leaf_capacity);
+    }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    cx::RecordBuild(cx::BuildArray{static_cast<const cx::Build*>(descriptors)}, max_leaf_capacity, count, s);
+    return cx::LaunchResult("cx_light_bvh_build_batch");
 }
 
 }  // extern "C"

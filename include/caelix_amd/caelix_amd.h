@@ -84,6 +84,18 @@ CX_API int cx_light_bvh_begin(LightControl* control, LightTreeState* state, uint
 /* The build (light_tree_record.cpp:271-308). */
 CX_API int cx_light_bvh_build(const CxLightBvhBuild* build, void* stream);
 
+/* The batched form of the build: `count` builds (each its own leaves, nodes, header, state,
+ * scratch and capacity, as for cx_light_bvh_build) in one sequence of launches whose grid covers
+ * every build (grid y = build), so many small trees (chunk scenes) cost one build's launches.
+ * cx_light_bvh_batch_prepare copies the build descriptions to `descriptors` (device memory of
+ * cx_light_bvh_batch_bytes(count) bytes); cx_light_bvh_build_batch records only kernel launches (it
+ * can be captured in a hipGraph) for `max_leaf_capacity` = the largest capacity of the builds. Each
+ * tree is the one cx_light_bvh_build makes. No counterpart in the reference (one RecordTree per
+ * tree, light_tree_record.cpp:230-309). */
+CX_API uint64_t cx_light_bvh_batch_bytes(uint32_t count);
+CX_API int cx_light_bvh_batch_prepare(const CxLightBvhBuild* builds, uint32_t count, void* descriptors, void* stream);
+CX_API int cx_light_bvh_build_batch(const void* descriptors, uint32_t count, uint32_t max_leaf_capacity, void* stream);
+
 /* Stable LSD radix sort of (uint32 key, uint32 value) pairs (the reference's
  * caelix_light_bvh_sort_histogram + caelix_light_bvh_scan + caelix_light_bvh_sort_scatter passes,
  * shaders/light/bvh/caelix_light_bvh_sort.hlsl:1-38), as 4 onesweep passes of 8 bits. The count is

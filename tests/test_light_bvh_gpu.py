@@ -176,3 +176,31 @@ def test_rebuild_stress_size_matches_oracle(cx, oracle):
     gpu, ref = gpu_and_oracle(cx, oracle, leaves[:capped], capacity=capped, found=n)
     compare(gpu, ref, capped, capped)
     assert int(gpu["state"]["leaves_dropped"]) == 64
+
+
+def test_batch_build_matches_oracle_per_tree(cx, oracle):
+    """40 trees of uneven sizes in one batched build (cx_light_bvh_build_batch): each is the oracle's
+    tree for its own leaves, and rebuilding the batch gives the same trees again."""
+    rng = np.random.default_rng(77)
+    sizes = [0, 1, 2, 3, 63, 64, 65, 4095, 4096, 4097, 9000] + list(rng.integers(5, 3000, size=29))
+    builders, leaves_of = [], []
+    for k, n in enumerate(sizes):
+        leaves = synthetic.light_leaves(int(n), 500 + k, extent=1024.0, clusters=4, sigma=20.0)
+        capacity = max(64, (int(n) + 63) // 64 * 64)
+        b = cx.LightBvhBuilder(capacity)
+        b.upload(leaves)
+        builders.append(b)
+        leaves_of.append((leaves, capacity))
+    batch = cx.LightBvhBatch(builders)
+    print(f"batch build: {len(builders)} trees, largest capacity {batch.max_capacity}", flush=True)
+    for rebuild in range(2):
+        batch.build()
+        for b, (leaves, capacity) in zip(builders, leaves_of):
+            n = len(leaves)
+            state = np.zeros(1, layout.STATE)
+            state["building"] = 1
+            state["leaves_found"] = n
+            state["builds"] = rebuild
+            ref = oracle.light_bvh_build(leaves.view(np.uint8), capacity, layout.node_capacity_for(capacity),
+                                         state.view(np.uint8))
+            compare(b.read(), ref, capacity, n)
