@@ -7,8 +7,9 @@ trace      (default) DDA + NEE at 1920 x 1080: 1,048,576 resident 8^3 bricks in 
            emissive bricks with emission log-uniform in [0.01, 100], 1 spp, 2 bounces, NEE through the
            light tree at the primary and bounce-1 vertices, float32 throughout. One step = one frame
            (the frame seed changes per step). value = Mrays/s: primary + bounce + shadow rays traced
-           per second. The light-tree rebuild below and the differentiable form (adjoint / forward
-           time at 512x512, 4 spp, 3 bounces, 262,144 bricks) are measured in the same run.
+           per second. The light-tree rebuild below, the differentiable form (adjoint / forward
+           time at 512x512, 4 spp, 3 bounces, 262,144 bricks) and the batched form (256 chunk scenes,
+           batched vs launch-per-scene) are measured in the same run.
 lighttree  full light-tree rebuild over 4,194,304 emissive brick leaves (half uniform in an 8192^3
            box, half in 64 Gaussian clusters, sigma 32 voxels, emission log-uniform in [0.01, 100]).
            One step = one rebuild from leaves resident in HBM. value = Mprims/s.
@@ -404,6 +405,60 @@ def diff_bench(args) -> dict:
             "fd_check": checks}
 
 
+def batch_bench(args) -> dict:
+    """The batched form: 256 chunk scenes (brick counts log-uniform in [64, 65536] as far as the
+    64^3-brick chunk's terrain holds them, 5 % emissive), each step rebuilding every scene's light
+    tree and tracing a 256x256, 1 spp, 2-bounce frame of every scene. Timed three ways on the same
+    scenes: launch-per-scene (one build and one cx_trace_frame per scene), batched (one batched build
+    and one cx_trace_batch), and the batched step captured in a hipGraph and replayed."""
+    import numpy as np
+    import torch
+
+    from caelix_amd import scene
+    from caelix_amd.batch import PlainScenes, TraceBatch
+
+    t0 = time.perf_counter()
+    datas = scene.chunk_batch(256)
+    gen_s = time.perf_counter() - t0
+    stream = torch.cuda.current_stream()
+    batch = TraceBatch(datas)
+    plain = PlainScenes(datas)
+    steps = max(2, min(args.steps, 5))
+    plain_ms = _time_steps(lambda i: plain.step(2, 50 + i, 1), steps, 1, stream)
+
+    def batched(i):
+        batch.build_trees()
+        batch.trace(2, 50 + i, 1)
+
+    batch_ms = _time_steps(batched, steps, 1, stream)
+    side = torch.cuda.Stream()
+    side.wait_stream(stream)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(side):
+        batched(0)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(graph, stream=side):
+            batched(0)
+    stream.wait_stream(side)
+    graph_ms = _time_steps(lambda _: graph.replay(), steps, 1, stream)
+    # the rays of one step
+    batch.counters.zero_()
+    batched(0)
+    torch.cuda.synchronize()
+    c = batch.counters.cpu().numpy()
+    rays = float(c[0] + c[1])
+    bricks = np.array([len(d.keys) for d in datas])
+    return {
+        "plain_ms": round(plain_ms, 3), "batched_ms": round(batch_ms, 3), "graph_ms": round(graph_ms, 3),
+        "speedup_batched": round(plain_ms / batch_ms, 2), "speedup_graph": round(plain_ms / graph_ms, 2),
+        "rays_per_step": int(rays), "graph_Mrays_per_s": round(rays / (graph_ms * 1e-3) / 1e6, 1),
+        "config": {"workload": "256 chunk scenes (64^3-brick terrain chunks), 5 % emissive, light-tree build + "
+                               "256x256 1 spp 2-bounce frame each", "scenes": len(datas),
+                   "bricks_total": int(bricks.sum()), "bricks_min": int(bricks.min()), "bricks_max": int(bricks.max()),
+                   "light_bricks_total": int(batch.light_bricks.sum()), "scene_generation_s": round(gen_s, 1)},
+    }
+
+
 def copy_bench(args) -> dict:
     """A plain device copy of 1 GiB (torch's copy kernel): the calibration run for the HBM byte
     counters (FETCH_SIZE / WRITE_SIZE per dispatch against 1 GiB read and 1 GiB written)."""
@@ -430,6 +485,7 @@ def main() -> int:
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-lighttree", action="store_true", help="trace only, without the light-tree figure")
     parser.add_argument("--no-adjoint", action="store_true", help="without the differentiable-form figure")
+    parser.add_argument("--no-batch", action="store_true", help="without the batched-form figure")
     args = parser.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the hot path runs on one GPU and this build has no multi-GPU "
@@ -442,6 +498,8 @@ def main() -> int:
         result = trace_bench(args)
         if not args.no_adjoint:
             result["adjoint"] = diff_bench(args)
+        if not args.no_batch:
+            result["batch"] = batch_bench(args)
         if not args.no_lighttree:
             lt = lighttree_bench(args, with_cpu=False)
             result["lighttree"] = {k: lt[k] for k in ("value", "unit", "ms_per_step", "roofline",
@@ -466,7 +524,7 @@ def main() -> int:
         "config": result["config"],
         "roofline": result["roofline"],
     }
-    for key in ("cpu_baseline", "parity", "lighttree", "adjoint", "pipeline_roofline", "stages"):
+    for key in ("cpu_baseline", "parity", "lighttree", "adjoint", "batch", "pipeline_roofline", "stages"):
         if key in result:
             line[key] = result[key]
     print(json.dumps(line))

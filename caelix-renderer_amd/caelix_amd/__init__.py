@@ -64,7 +64,7 @@ class CxScene(ctypes.Structure):
         ("super_bits", ctypes.c_void_p), ("grid_min", ctypes.c_int32 * 3), ("grid_dim", ctypes.c_uint32),
         ("group_shift", ctypes.c_uint32), ("encoding", ctypes.c_uint32), ("materials", ctypes.c_void_p),
         ("light_codes", ctypes.c_void_p), ("emission_scale", ctypes.c_float), ("pad", ctypes.c_uint32),
-        ("brick_emission_scale", ctypes.c_void_p),
+        ("brick_emission_scale", ctypes.c_void_p), ("scene_count", ctypes.c_uint32), ("pad2", ctypes.c_uint32),
     ]
 
 
@@ -80,7 +80,8 @@ class CxLightTreeView(ctypes.Structure):
     _fields_ = [
         ("headers", ctypes.c_void_p), ("nodes", ctypes.c_void_p * 2), ("leaves", ctypes.c_void_p * 2),
         ("heap", ctypes.c_void_p), ("record_offset", ctypes.c_void_p), ("ordinal_of_brick", ctypes.c_void_p),
-        ("group_shift", ctypes.c_uint32), ("mis", ctypes.c_uint32),
+        ("group_shift", ctypes.c_uint32), ("mis", ctypes.c_uint32), ("leaf_base", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
     ]
 
 
@@ -152,6 +153,12 @@ SIGNATURES = {
     "cx_trace_frame_adjoint": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightTreeView), ctypes.POINTER(CxCamera),
                                       ctypes.POINTER(CxTraceParams), ctypes.POINTER(CxTraceGradients),
                                       ctypes.POINTER(CxTraceOutput), _P]),
+    "cx_trace_batch_bytes": (_U64, [_U32]),
+    "cx_trace_batch_workspace_bytes": (_U64, [_U32, _U32, _U32]),
+    "cx_trace_batch_prepare": (_INT, [ctypes.POINTER(CxLightTreeView), ctypes.POINTER(CxCamera), _U32, _P, _P]),
+    "cx_trace_batch": (_INT, [ctypes.POINTER(CxScene), _P, _U32, _U32, _U32, ctypes.POINTER(CxTraceParams),
+                              ctypes.POINTER(CxTraceOutput), _P]),
+    "cx_hash_build_keys": (_INT, [_P, _U32, _U32, _P, _U32, _P, _P, _P]),
     "cx_trace_rays": (_INT, [ctypes.POINTER(CxScene), _P, _U32, _P, _P]),
     "cx_light_sample": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightTreeView), _P, _P, _U32, _P, _P, _P]),
     "cx_light_face_pdf": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightTreeView), _P, _U32, _P, _P]),
@@ -212,15 +219,20 @@ class LightBvhBuilder:
     capacity: leaves, nodes, header, state, scratch. Mirrors the buffers LightTree owns for one tree
     (lib/src/light/light_tree.h:37-101)."""
 
-    def __init__(self, leaf_capacity: int, node_capacity: int | None = None, device="cuda"):
+    def __init__(self, leaf_capacity: int, node_capacity: int | None = None, device="cuda", leaves=None, header=None):
+        """`leaves`: an existing uint8 device tensor view of at least leaf_capacity * 64 bytes to build
+        over (a scene's slice of a batch's leaves), else the builder allocates its own; `header`: a
+        64-byte uint8 view to write the tree header into (the first header of a light tree view)."""
         if leaf_capacity <= 0 or leaf_capacity % 64 != 0:
             raise ValueError("leaf capacity must be a positive multiple of 64")
         self.lib = load()
         self.leaf_capacity = leaf_capacity
         self.node_capacity = node_capacity if node_capacity is not None else layout.node_capacity_for(leaf_capacity)
-        self.leaves = device_bytes(leaf_capacity * 64, device)
+        if leaves is not None and leaves.numel() < leaf_capacity * 64:
+            raise ValueError("the leaves view is smaller than the leaf capacity")
+        self.leaves = leaves if leaves is not None else device_bytes(leaf_capacity * 64, device)
         self.nodes = device_bytes(self.node_capacity * 224, device)
-        self.header = device_bytes(64, device)
+        self.header = header if header is not None else device_bytes(64, device)
         self.state = device_bytes(64, device)
         self.scratch_bytes = int(self.lib.cx_light_bvh_scratch_bytes(leaf_capacity))
         self.scratch = device_bytes(self.scratch_bytes, device)
@@ -450,7 +462,8 @@ class TraceScene:
         heap_units = max(64, n * 8)
         while True:
             leaf_capacity = max(64, (n + 63) // 64 * 64)
-            self.tree = LightBvhBuilder(leaf_capacity)
+            self.headers = torch.zeros(32, dtype=torch.int32, device=dev)  # 2 x LightTreeHeader
+            self.tree = LightBvhBuilder(leaf_capacity, header=self.headers[:16].view(torch.uint8))
             self.heap = torch.zeros(heap_units * 4, dtype=torch.int32, device=dev)
             ls = self.light = CxLightScene()
             ls.heap = self.heap.data_ptr()
@@ -474,8 +487,6 @@ class TraceScene:
         st["building"] = 1
         self.tree.state.copy_(to_device(st.view(np.uint8), self.tree.state.device))
         self.tree.build(stream)
-        self.headers = torch.zeros(32, dtype=torch.int32, device=dev)  # 2 x LightTreeHeader
-        self.headers[:16].copy_(self.tree.header.view(torch.int32))
         empty = np.zeros(1, layout.HEADER)
         empty["root"] = layout.LT_NONE
         self.headers[16:].copy_(torch.from_numpy(empty.view(np.int32).copy()).to(dev))

@@ -204,3 +204,70 @@ def diff_scene(seed: int = 5) -> SceneData:
     4 bricks), 16,384 emissive bricks (the emitters), emission log-uniform in [0.01, 100], 512 x 512."""
     return terrain(2048, seed, octaves=8, bricks_per_column=4, emissive=16384, emission=(0.01, 100.0),
                    log_uniform=True, fill=0.45, relief=0.35, width=512, height=512)
+
+
+def chunk_scene(seed: int, bricks: int, grid: int = 64, emissive_fraction: float = 0.05, width: int = 256,
+                height: int = 256, materials=None) -> SceneData:
+    """One chunk scene of the batched workload: a value-noise terrain in a grid^3-brick cell (grid a
+    multiple of 8), exactly `bricks` resident bricks taken layer by layer from the surface down (the
+    surface bricks of all columns first, a random subset of columns for a partial layer), about
+    `emissive_fraction` of them emissive (at least one), camera from the seed. `materials` (table,
+    light codes) is shared by the scenes of a batch; by default the batch palette."""
+    world = grid * 8
+    rng = np.random.default_rng(seed)
+    h = value_noise(world, seed, 5, max(world // 4, 8))
+    heights = np.clip(np.round(0.45 * world + (h - 0.5) * 0.4 * world), 1, world - 1).astype(np.int32)
+    col_max = heights.reshape(grid, 8, grid, 8).max(axis=(1, 3))
+    top = (col_max - 1) // 8
+    bx, bz = np.meshgrid(np.arange(grid), np.arange(grid), indexing="ij")
+    cols = np.stack([bx.ravel(), bz.ravel()], axis=1)
+    keys = []
+    remaining = int(bricks)
+    layer = 0
+    while remaining > 0 and layer < grid:
+        ys = top.ravel() - layer
+        ok = np.nonzero(ys >= 0)[0]
+        if len(ok) == 0:
+            break
+        take = ok if len(ok) <= remaining else rng.choice(ok, size=remaining, replace=False)
+        keys.append(np.stack([cols[take, 0], ys[take], cols[take, 1]], axis=1))
+        remaining -= len(take)
+        layer += 1
+    keys = np.concatenate(keys).astype(np.int32)
+    order = np.lexsort((keys[:, 0], keys[:, 1], keys[:, 2]))
+    keys = keys[order]
+    n = len(keys)
+    v = np.arange(8)
+    xs = keys[:, 0:1] * 8 + v[None, :]
+    ys = keys[:, 1:2] * 8 + v[None, :]
+    zs = keys[:, 2:3] * 8 + v[None, :]
+    hz = heights[xs[:, None, :], zs[:, :, None]]
+    solid = ys[:, None, :, None] < hz[:, :, None, :]
+    depth = hz[:, :, None, :] - ys[:, None, :, None]
+    mat = 1 + ((xs[:, None, None, :] * 7 + zs[:, :, None, None] * 13 + depth * 3) % 127)
+    payload = np.where(solid, OPAQUE | mat, 0).astype(np.uint16).reshape(n, 512)
+    solid_count = (payload != 0).sum(axis=1)
+    candidates = np.nonzero(solid_count > 0)[0]
+    emissive = max(1, int(round(emissive_fraction * n)))
+    chosen = rng.choice(candidates, size=min(emissive, len(candidates)), replace=False)
+    emissive_ids = rng.integers(128, 256, size=len(chosen))
+    for c, e in zip(chosen, emissive_ids):
+        row = payload[c]
+        row[row != 0] = OPAQUE | e
+    table, light_codes = materials if materials is not None else make_materials(4242, (0.01, 100.0), True)
+    peak = float(heights.max())
+    c = world / 2
+    camera = look_at((c * 0.3 + rng.uniform(-8, 8), peak + 0.3 * world, c * 0.2 + rng.uniform(-8, 8)),
+                     (c, float(np.median(heights)), c * 1.2), width, height)
+    return SceneData(keys=keys, payload=payload, materials=table, light_codes=light_codes, grid_min=(0, 0, 0),
+                     grid_dim=grid, group_shift=5, camera=camera, emissive_bricks=np.sort(chosen))
+
+
+def chunk_batch(count: int = 256, seed: int = 9, low: int = 64, high: int = 65536, grid: int = 64,
+                emissive_fraction: float = 0.05, width: int = 256, height: int = 256) -> list:
+    """The batched workload: `count` chunk scenes with brick counts log-uniform in [low, high]."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(low), np.log(high), size=count)).astype(np.int64)
+    palette = make_materials(4242, (0.01, 100.0), True)
+    return [chunk_scene(seed * 1000 + i, int(sizes[i]), grid, emissive_fraction, width, height, palette)
+            for i in range(count)]

@@ -314,8 +314,8 @@ uint64_t BulkScratchBytes(uint32_t count, BulkScratch* out, uint8_t* base)
     return offset;
 }
 
-__global__ void BulkPrepareKernel(const int32_t* keys_xyz, uint32_t count, uint32_t slot_count, uint32_t pow2,
-                                  BulkScratch s, uint32_t tiles)
+__global__ void BulkPrepareKernel(const uint32_t* words, uint32_t word_count, uint32_t count, uint32_t slot_count,
+                                  uint32_t pow2, BulkScratch s, uint32_t tiles)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < tiles)
@@ -332,8 +332,7 @@ __global__ void BulkPrepareKernel(const int32_t* keys_xyz, uint32_t count, uint3
     {
         return;
     }
-    const uint2 key = BrickFingerprint(keys_xyz[3 * uint64_t(i)], keys_xyz[3 * uint64_t(i) + 1],
-                                       keys_xyz[3 * uint64_t(i) + 2]);
+    const uint2 key = Fingerprint(words + uint64_t(i) * word_count, word_count);
     s.homes[0][i] = pow2 != 0 ? (key.x & (slot_count - 1)) : (key.x % slot_count);
     s.index[0][i] = i;
     s.key_lo[i] = key.x;
@@ -649,13 +648,24 @@ CX_API uint64_t cx_hash_build_scratch_bytes(uint32_t count)
 CX_API int cx_hash_build_bricks(const int32_t* keys_xyz, uint32_t count, HashKeySlot* slots, uint32_t slot_count,
                                 void* scratch, uint32_t* info, void* stream)
 {
-    if (keys_xyz == nullptr || slots == nullptr || scratch == nullptr || info == nullptr)
+    return cx_hash_build_keys(reinterpret_cast<const uint32_t*>(keys_xyz), 3, count, slots, slot_count, scratch, info,
+                              stream);
+}
+
+CX_API int cx_hash_build_keys(const uint32_t* words, uint32_t word_count, uint32_t count, HashKeySlot* slots,
+                              uint32_t slot_count, void* scratch, uint32_t* info, void* stream)
+{
+    if (words == nullptr || slots == nullptr || scratch == nullptr || info == nullptr)
     {
-        return cx::ArgError("cx_hash_build_bricks: a buffer is NULL");
+        return cx::ArgError("cx_hash_build_keys: a buffer is NULL");
+    }
+    if (word_count == 0 || word_count > 8)
+    {
+        return cx::ArgError("cx_hash_build_keys: %u key words (1..8)", word_count);
     }
     if (slot_count == 0 || slot_count > (1u << 31) || count >= slot_count)
     {
-        return cx::ArgError("cx_hash_build_bricks: %u keys do not fit %u slots (need count < slots <= 2^31)", count,
+        return cx::ArgError("cx_hash_build_keys: %u keys do not fit %u slots (need count < slots <= 2^31)", count,
                             slot_count);
     }
     const hipStream_t s = static_cast<hipStream_t>(stream);
@@ -667,8 +677,8 @@ CX_API int cx_hash_build_bricks(const int32_t* keys_xyz, uint32_t count, HashKey
     cx::StageScope scope(s, "hash_build");
     cx::FillKeysKernel<<<std::min(cx::DivUp(slot_count, 256), 4096u), 256, 0, s>>>(reinterpret_cast<uint4*>(slots),
                                                                                   slot_count);
-    cx::BulkPrepareKernel<<<cx::DivUp(std::max(n, tiles), 256), 256, 0, s>>>(keys_xyz, count, slot_count, pow2, b,
-                                                                            tiles);
+    cx::BulkPrepareKernel<<<cx::DivUp(std::max(n, tiles), 256), 256, 0, s>>>(words, word_count, count, slot_count, pow2,
+                                                                            b, tiles);
     if (count > 0)
     {
         // count lives in ctl[1] (set by the prepare kernel) for the sort's device-side count.
@@ -683,7 +693,7 @@ CX_API int cx_hash_build_bricks(const int32_t* keys_xyz, uint32_t count, HashKey
         cx::BulkPlaceKernel<<<tiles, 256, 0, s>>>(count, slot_count, b, slots);
     }
     cx::BulkWrapKernel<<<1, 1024, 0, s>>>(count, slot_count, b, slots, info);
-    return cx::LaunchResult("cx_hash_build_bricks");
+    return cx::LaunchResult("cx_hash_build_keys");
 }
 
 CX_API int cx_hash_get_bricks(const HashKeySlot* slots, uint32_t slot_count, const uint32_t* info,

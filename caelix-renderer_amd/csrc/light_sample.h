@@ -445,7 +445,7 @@ CX_DEV bool LightTreeSample(const SceneView& s, const LightView& L, V3 x, V3 n, 
     const uint32_t brick_record = lf.group;
     const uint32_t ordinal = lf.entry;
     const Record record = LoadRecord(L, ordinal);
-    const int32_t* key = s.keys + uint64_t(brick_record) * 3;
+    const int32_t* key = BrickXyz(s, brick_record);
     const V3 origin = GroupOrigin(s, L, key[0], key[1], key[2]);
     const V3 x_group = Sub(x, origin);
     float w[6];
@@ -523,7 +523,7 @@ CX_DEV float LightFacePdf(const SceneView& s, const LightView& L, V3 x, V3 n, ui
         return 0.0f;
     }
     const Record rec = LoadRecord(L, ordinal);
-    const int32_t* key = s.keys + uint64_t(record) * 3;
+    const int32_t* key = BrickXyz(s, record);
     const V3 origin = GroupOrigin(s, L, key[0], key[1], key[2]);
     float w[6];
     float sum;
@@ -566,13 +566,15 @@ CX_DEV float LightFacePdf(const SceneView& s, const LightView& L, V3 x, V3 n, ui
     }
     found = true;
     const float in_brick = Div(Div(w[face], sum), float(count));
-    // The brick's leaf: tree 0 (the static tree), leaf = the light ordinal when it is below the count.
-    if (ordinal >= L.headers[0].leaf_count)
+    // The brick's leaf: tree 0 (the static tree), leaf = the light ordinal - leaf_base when it is
+    // below the count.
+    const uint32_t leaf = ordinal - L.leaf_base;
+    if (leaf >= L.headers[0].leaf_count)
     {
         found = false;  // no leaf: LampHitWeight gives the full emission
         return 0.0f;
     }
-    return LeafProbability(L, 0, ordinal, x, n) * in_brick;
+    return LeafProbability(L, 0, leaf, x, n) * in_brick;
 }
 
 // LightNeeMisWeight (caelix_light_nee.hlsl:20-34).

@@ -61,9 +61,9 @@ CX_DEV Shade ShadeHit(const SceneView& s, uint32_t block, V3 hit_position, V3 no
 }
 
 
-CX_DEV WorldHit Trace(const SceneView& s, V3 o, V3 d, float t_max, PathCounters& c)
+CX_DEV WorldHit Trace(const SceneView& s, V3 o, V3 d, float t_max, PathCounters& c, uint32_t scene = 0)
 {
-    const WorldHit h = TraceWorld(s, o, d, t_max);
+    const WorldHit h = TraceWorld(s, o, d, t_max, scene);
     c.cells += h.cells;
     c.probes += h.probes;
     c.steps += h.steps;

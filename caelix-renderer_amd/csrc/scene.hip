@@ -36,7 +36,27 @@ struct SceneArgs
     uint32_t encoding;
     const float4* materials;
     const uint32_t* light_codes;
+    uint32_t key_words;  // 3: (x, y, z); 4: (scene, x, y, z) for a batch of chunk scenes
 };
+
+// The (x, y, z) brick coordinates of a record and its scene (0 for one scene).
+CX_DEV const int32_t* BrickXyz(const SceneArgs& s, uint32_t record)
+{
+    return s.keys + uint64_t(record) * s.key_words + (s.key_words - 3);
+}
+CX_DEV uint32_t BrickScene(const SceneArgs& s, uint32_t record)
+{
+    return s.key_words == 4 ? uint32_t(s.keys[uint64_t(record) * 4]) : 0u;
+}
+CX_DEV uint2 KeyFingerprint(uint32_t key_words, uint32_t scene, int32_t x, int32_t y, int32_t z)
+{
+    if (key_words == 4)
+    {
+        const uint32_t w[4] = {scene, uint32_t(x), uint32_t(y), uint32_t(z)};
+        return Fingerprint(w, 4);
+    }
+    return BrickFingerprint(x, y, z);
+}
 
 CX_DEV bool Opaque(uint32_t block, uint32_t encoding)
 {
@@ -106,7 +126,8 @@ __global__ void __launch_bounds__(64) FinalizeKernel(SceneArgs s)
         {
             coarse |= lds_occ[c] != 0ull ? (1u << c) : 0u;
         }
-        const int32_t* key = s.keys + uint64_t(brick) * 3;
+        const int32_t* key = BrickXyz(s, brick);
+        const uint64_t scene = BrickScene(s, brick);
         const uint32_t mask = (1u << s.group_shift) - 1u;
         const uint32_t pos = (uint32_t(key[0]) & mask) | ((uint32_t(key[1]) & mask) << s.group_shift) |
                              ((uint32_t(key[2]) & mask) << (2 * s.group_shift));
@@ -118,10 +139,12 @@ __global__ void __launch_bounds__(64) FinalizeKernel(SceneArgs s)
         const int32_t gz = key[2] - s.grid_min[2];
         if (any_voxel && uint32_t(gx) < s.grid_dim && uint32_t(gy) < s.grid_dim && uint32_t(gz) < s.grid_dim)
         {
-            const uint64_t index = (uint64_t(gz) * s.grid_dim + uint32_t(gy)) * s.grid_dim + uint32_t(gx);
+            const uint64_t cells = uint64_t(s.grid_dim) * s.grid_dim * s.grid_dim;
+            const uint64_t index = scene * cells + (uint64_t(gz) * s.grid_dim + uint32_t(gy)) * s.grid_dim + uint32_t(gx);
             atomicOr(&s.brick_bits[index >> 5], 1u << (index & 31u));
             const uint32_t sd = s.grid_dim >> 3;
-            const uint64_t super_index = (uint64_t(gz >> 3) * sd + uint32_t(gy >> 3)) * sd + uint32_t(gx >> 3);
+            const uint64_t super_index = scene * (cells / 512) + (uint64_t(gz >> 3) * sd + uint32_t(gy >> 3)) * sd +
+                                         uint32_t(gx >> 3);
             atomicOr(&s.super_bits[super_index >> 5], 1u << (super_index & 31u));
         }
     }
@@ -144,11 +167,12 @@ CX_DEV uint32_t NeighbourBlock(const SceneArgs& s, uint32_t record, int x, int y
     {
         return s.payload[uint64_t(record) * 512 + p[0] + 8 * p[1] + 64 * p[2]];
     }
-    const int32_t* key = s.keys + uint64_t(record) * 3;
+    const int32_t* key = BrickXyz(s, record);
     int32_t nk[3] = {key[0], key[1], key[2]};
     nk[axis] += sign;
     p[axis] = (p[axis] + 8) & 7;
-    const uint32_t other = HashGet(s.slots, s.slot_count, s.pow2, s.hash_info[0], BrickFingerprint(nk[0], nk[1], nk[2]));
+    const uint32_t other = HashGet(s.slots, s.slot_count, s.pow2, s.hash_info[0],
+                                   KeyFingerprint(s.key_words, BrickScene(s, record), nk[0], nk[1], nk[2]));
     if (other == kHashNone)
     {
         return 0u;
@@ -318,7 +342,7 @@ __global__ void __launch_bounds__(64) LightRecordKernel(SceneArgs s, LightArgs l
             }
             face[f] = sum;
         }
-        const int32_t* key = s.keys + uint64_t(record) * 3;
+        const int32_t* key = BrickXyz(s, record);
         const int32_t g = int32_t(s.group_shift);
         const uint32_t mask = (1u << s.group_shift) - 1u;
         const uint32_t bx = uint32_t(key[0]) & mask, by = uint32_t(key[1]) & mask, bz = uint32_t(key[2]) & mask;
@@ -487,6 +511,7 @@ SceneArgs ArgsOf(const CxScene& c)
     s.encoding = c.encoding;
     s.materials = reinterpret_cast<const float4*>(c.materials);
     s.light_codes = c.light_codes;
+    s.key_words = c.scene_count > 1 ? 4u : 3u;
     return s;
 }
 
@@ -507,7 +532,8 @@ CX_API int cx_scene_finalize(const CxScene* scene, void* stream)
         return cx::ArgError("cx_scene_finalize: grid_dim must be a positive multiple of 8, group_shift in 1..7");
     }
     const hipStream_t s = static_cast<hipStream_t>(stream);
-    const uint64_t cells = uint64_t(scene->grid_dim) * scene->grid_dim * scene->grid_dim;
+    const uint64_t scenes = scene->scene_count > 1 ? scene->scene_count : 1;
+    const uint64_t cells = uint64_t(scene->grid_dim) * scene->grid_dim * scene->grid_dim * scenes;
     const uint64_t supers = cells / 512;
     cx::StageScope scope(s, "scene_finalize");
     cx::FillKernel<<<2048, 256, 0, s>>>(scene->brick_bits, (cells + 31) / 32);

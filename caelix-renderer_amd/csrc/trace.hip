@@ -4,6 +4,8 @@
 #include "path_trace.h"
 #include "runtime.h"
 
+#include <vector>
+
 namespace cx {
 namespace {
 
@@ -30,6 +32,9 @@ SceneView SceneOf(const CxScene& c)
     s.emission_scale = c.emission_scale;
     s.encoding = c.encoding;
     s.brick_scale = c.brick_emission_scale;
+    s.key_words = c.scene_count > 1 ? 4u : 3u;
+    s.scene_cells = uint64_t(c.grid_dim) * c.grid_dim * c.grid_dim;
+    s.scene_supers = s.scene_cells / 512;
     return s;
 }
 
@@ -46,6 +51,7 @@ LightView LightsOf(const CxLightTreeView& v)
     l.ordinal_of_brick = v.ordinal_of_brick;
     l.group_shift = v.group_shift;
     l.mis = v.mis;
+    l.leaf_base = v.leaf_base;
     return l;
 }
 
@@ -189,6 +195,32 @@ Workspace CarveWorkspace(void* base, uint32_t paths)
     w.partials = reinterpret_cast<float*>(take(uint64_t(GradRows(paths)) * kGradEntries * 4));
     return w;
 }
+
+// The scenes of a launch. OneScene: the plain frame. SceneBatch: chunk scenes of one image size whose
+// paths are numbered scene-major (paths_per_scene each); their light trees and cameras are arrays
+// in device memory and their outputs are images of pixels_per_scene pixels, scene-major.
+struct OneScene
+{
+    LightView light;
+    CameraView camera;
+    CX_DEV uint32_t SceneOf(uint32_t) const { return 0; }
+    CX_DEV uint32_t Local(uint32_t path) const { return path; }
+    CX_DEV const LightView& Light(uint32_t) const { return light; }
+    CX_DEV const CameraView& Camera(uint32_t) const { return camera; }
+    CX_DEV uint64_t PixelBase(uint32_t) const { return 0; }
+};
+struct SceneBatch
+{
+    const LightView* lights;
+    const CameraView* cameras;
+    uint32_t paths_per_scene;
+    uint32_t pixels_per_scene;
+    CX_DEV uint32_t SceneOf(uint32_t path) const { return path / paths_per_scene; }
+    CX_DEV uint32_t Local(uint32_t path) const { return path % paths_per_scene; }
+    CX_DEV const LightView& Light(uint32_t scene) const { return lights[scene]; }
+    CX_DEV const CameraView& Camera(uint32_t scene) const { return cameras[scene]; }
+    CX_DEV uint64_t PixelBase(uint32_t scene) const { return uint64_t(scene) * pixels_per_scene; }
+};
 
 CX_DEV void PathPixel(const CameraView& cam, uint32_t path, uint32_t& px, uint32_t& py)
 {
@@ -336,22 +368,25 @@ constexpr uint32_t kCtlBounceCursor = 9;
 // Camera rays of every path and their first hit, one path per lane: the rays of a wave are one 8x8
 // pixel tile and stay coherent, so they run in lockstep without refills (faster here than the
 // persistent form).
-__global__ void __launch_bounds__(kWaveThreads) PrimaryTraceKernel(SceneView s, CameraView cam, Workspace w,
+template <class Views>
+__global__ void __launch_bounds__(kWaveThreads) PrimaryTraceKernel(SceneView s, Views views, Workspace w,
                                                                    uint32_t paths, unsigned long long* counters)
 {
     const uint32_t path = blockIdx.x * kWaveThreads + threadIdx.x;
     PathCounters c{0, 0, 0, 0, 0};
     if (path < paths)
     {
+        const uint32_t scene = views.SceneOf(path);
+        const CameraView& cam = views.Camera(scene);
         uint32_t px, py;
-        PathPixel(cam, path, px, py);
+        PathPixel(cam, views.Local(path), px, py);
         uint4 h = make_uint4(0u, 0u, kHashNone, 0u);
         if (px < cam.width && py < cam.height)
         {
             const V3 dir = PrimaryDirection(cam, px, py);
             const V3 origin = Make(cam.position[0], cam.position[1], cam.position[2]);
             c.rays += 1;
-            const WorldHit hit = Trace(s, origin, dir, kRayTMax, c);
+            const WorldHit hit = Trace(s, origin, dir, kRayTMax, c, scene);
             h = make_uint4(__float_as_uint(hit.t), hit.attrib, hit.record, 0u);
         }
         w.hit[path] = h;
@@ -365,10 +400,12 @@ __global__ void __launch_bounds__(64) ResetKernel(Workspace w)
 }
 
 // Bounce rays: item -> path of queue[in].
+template <class Views>
 struct BounceSource
 {
     Workspace w;
     uint32_t in;
+    Views views;
     CX_DEV uint32_t Count() const { return w.ctl[in]; }
     CX_DEV uint32_t* Cursor() const { return &w.ctl[kCtlBounceCursor]; }
     // The queues the following shade launch appends to, and the shadow cursor, start empty.
@@ -378,9 +415,10 @@ struct BounceSource
         w.ctl[2] = 0u;
         w.ctl[kCtlShadowCursor] = 0u;
     }
-    CX_DEV bool Load(uint32_t item, V3& o, V3& d, float& t_max, PathCounters& c) const
+    CX_DEV bool Load(uint32_t item, V3& o, V3& d, float& t_max, uint32_t& scene, PathCounters& c) const
     {
         const uint32_t path = w.queue[in][item];
+        scene = views.SceneOf(path);
         const float4 o4 = w.ray_o[path];
         const float4 d4 = w.ray_d[path];
         o = Make(o4.x, o4.y, o4.z);
@@ -396,18 +434,21 @@ struct BounceSource
 };
 
 // Shadow rays: an unoccluded one adds its radiance to its path (one shadow ray per path per launch).
+template <class Views>
 struct ShadowSource
 {
     Workspace w;
+    Views views;
     SceneView sv;
     GradView g;
     CX_DEV uint32_t Count() const { return w.ctl[2]; }
     CX_DEV uint32_t* Cursor() const { return &w.ctl[kCtlShadowCursor]; }
     CX_DEV void Prepare() const {}
-    CX_DEV bool Load(uint32_t item, V3& o, V3& d, float& t_max, PathCounters&) const
+    CX_DEV bool Load(uint32_t item, V3& o, V3& d, float& t_max, uint32_t& scene, PathCounters&) const
     {
         const float4 o4 = w.shadow_o[item];
         const float4 d4 = w.shadow_d[item];
+        scene = views.SceneOf(__float_as_uint(d4.w));
         o = Make(o4.x, o4.y, o4.z);
         d = Make(d4.x, d4.y, d4.z);
         t_max = o4.w;
@@ -490,9 +531,10 @@ __global__ void __launch_bounds__(kWaveThreads) PersistentTraceKernel(SceneView 
                 item = chunk_next + rank;
                 V3 o, d;
                 float t_max;
-                if (src.Load(item, o, d, t_max, c))
+                uint32_t scene = 0;
+                if (src.Load(item, o, d, t_max, scene, c))
                 {
-                    WorldRayBegin(s, o, d, t_max, r);
+                    WorldRayBegin(s, o, d, t_max, scene, r);
                     busy = r.phase != kRayDone;
                     if (!busy)
                     {
@@ -531,9 +573,10 @@ __global__ void __launch_bounds__(kWaveThreads) PersistentTraceKernel(SceneView 
 // The primary vertex: outputs of the hit, the first shade, the NEE at vertex 0 and the first
 // bounce ray (queue 0). Adjoint: the path's adjoint dL/dI / spp * albedo, and the primary emission's
 // emission-scale gradient.
-__global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, LightView L, CameraView cam,
-                                                                   TraceParams prm, Workspace w, uint32_t paths,
-                                                                   GradView g, unsigned long long* counters)
+template <class Views>
+__global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, Views views, TraceParams prm,
+                                                                   Workspace w, uint32_t paths, GradView g,
+                                                                   unsigned long long* counters)
 {
     const uint32_t path = blockIdx.x * kWaveThreads + threadIdx.x;
     PathCounters c{0, 0, 0, 0, 0};
@@ -554,8 +597,11 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
         }
         else
         {
+            const uint32_t scene = views.SceneOf(path);
+            const LightView& L = views.Light(scene);
+            const CameraView& cam = views.Camera(scene);
             uint32_t px, py;
-            PathPixel(cam, path, px, py);
+            PathPixel(cam, views.Local(path), px, py);
             const float t = __uint_as_float(h.x);
             const V3 dir = PrimaryDirection(cam, px, py);
             const V3 origin = Make(cam.position[0], cam.position[1], cam.position[2]);
@@ -570,7 +616,7 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
             w.emission[path] = make_float4(emission.x, emission.y, emission.z, __uint_as_float(h.y));
             if (g.d_color != nullptr)
             {
-                const float4 d = g.d_color[uint64_t(py) * cam.width + px];
+                const float4 d = g.d_color[views.PixelBase(scene) + uint64_t(py) * cam.width + px];
                 const V3 dc = Mul(Make(d.x, d.y, d.z), g.inv_spp);
                 const V3 adj = MulV(dc, sh.albedo);
                 w.adjoint[path] = make_float4(adj.x, adj.y, adj.z, 0.0f);
@@ -610,7 +656,8 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
 
 // One iteration of TracePath's bounce loop for the paths in queue[(bounce - 1) & 1]. Adjoint: the
 // gradients of the emission term of this vertex.
-__global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, LightView L, TraceParams prm, Workspace w,
+template <class Views>
+__global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, Views views, TraceParams prm, Workspace w,
                                                                   uint32_t bounce, uint32_t paths, GradView g,
                                                                   unsigned long long* counters)
 {
@@ -642,6 +689,7 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, L
         const uint4 h = w.hit[path];
         if (h.y != 0u)
         {
+            const LightView& L = views.Light(views.SceneOf(path));
             uint4 st = w.state[path];
             uint32_t rng = st.x;
             uint32_t light_rng = st.y;
@@ -736,7 +784,8 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, L
 
 // color += emission + albedo * radiance, radiance += radiance, t and attrib of the last sample.
 // Adjoint: the primary albedo's gradient dL/dI / spp * radiance.
-__global__ void __launch_bounds__(kWaveThreads) FinishKernel(CameraView cam, Workspace w, uint32_t paths, uint32_t sample,
+template <class Views>
+__global__ void __launch_bounds__(kWaveThreads) FinishKernel(Views views, Workspace w, uint32_t paths, uint32_t sample,
                                                              FrameOut out, GradView g)
 {
     __shared__ float lds_grad[kGradEntries];
@@ -747,16 +796,18 @@ __global__ void __launch_bounds__(kWaveThreads) FinishKernel(CameraView cam, Wor
     }
     const uint32_t path = blockIdx.x * kWaveThreads + threadIdx.x;
     uint32_t px = 0, py = 0;
+    const uint32_t scene = path < paths ? views.SceneOf(path) : 0u;
+    const CameraView& cam = views.Camera(scene);
     if (path < paths)
     {
-        PathPixel(cam, path, px, py);
+        PathPixel(cam, views.Local(path), px, py);
     }
     if (path < paths && px < cam.width && py < cam.height)
     {
         const float4 a = w.albedo[path];
         const float4 e = w.emission[path];
         const float4 r = w.radiance[path];
-        const uint64_t pixel = uint64_t(py) * cam.width + px;
+        const uint64_t pixel = views.PixelBase(scene) + uint64_t(py) * cam.width + px;
         if (adjoint)
         {
             const uint32_t attrib = __float_as_uint(e.w);
@@ -929,7 +980,95 @@ uint32_t PersistentBlocks(Kernel kernel)
     return blocks;
 }
 
-// The frame (grads null) or its adjoint replay (grads set): the same launches, in the same order.
+// The launches of a frame (g.d_color null) or of its adjoint replay (set), in the same order, for the
+// scenes of `views` (paths = all of their paths).
+template <class Views>
+void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const CxTraceParams* params, uint32_t mis,
+                 const GradView& g, const Workspace& w, const FrameOut& o, hipStream_t s)
+{
+    const bool adjoint = g.d_color != nullptr;
+    const uint32_t blocks = DivUp(paths, kWaveThreads);
+    const uint32_t rows = GradRows(paths);
+    const uint32_t bounce_blocks = std::min(PersistentBlocks(PersistentTraceKernel<BounceSource<Views>>), rows);
+    const uint32_t shadow_blocks = std::min(PersistentBlocks(PersistentTraceKernel<ShadowSource<Views>>), rows);
+    if (adjoint)
+    {
+        const uint64_t n = uint64_t(rows) * kGradEntries;
+        ClearRowsKernel<<<uint32_t(DivUp(n, uint64_t(256))), 256, 0, s>>>(w.partials, n);
+    }
+    const ShadowSource<Views> shadow_source{w, views, sv, g};
+    for (uint32_t sample = 0; sample < params->spp; ++sample)
+    {
+        const TraceParams p{params->bounces, params->frame_seed + sample, mis,
+                            (params->flags & CX_TRACE_NO_RUSSIAN_ROULETTE) != 0 ? 0u : 1u};
+        ResetKernel<<<1, 64, 0, s>>>(w);
+        {
+            StageScope scope(s, adjoint ? "adjoint_trace" : "trace_primary");
+            PrimaryTraceKernel<<<blocks, kWaveThreads, 0, s>>>(sv, views, w, paths, o.counters);
+        }
+        {
+            StageScope scope(s, adjoint ? "adjoint_shade" : "trace_shade_primary");
+            PrimaryShadeKernel<<<blocks, kWaveThreads, 0, s>>>(sv, views, p, w, paths, g, o.counters);
+        }
+        {
+            StageScope scope(s, adjoint ? "adjoint_trace" : "trace_shadow");
+            PersistentTraceKernel<<<shadow_blocks, kWaveThreads, 0, s>>>(sv, shadow_source, o.counters, w, adjoint);
+        }
+        for (uint32_t bounce = 1; bounce <= params->bounces; ++bounce)
+        {
+            {
+                StageScope scope(s, adjoint ? "adjoint_trace" : "trace_extend");
+                PersistentTraceKernel<<<bounce_blocks, kWaveThreads, 0, s>>>(
+                    sv, BounceSource<Views>{w, (bounce - 1) & 1u, views}, o.counters, w, false);
+            }
+            {
+                StageScope scope(s, adjoint ? "adjoint_shade" : "trace_shade_bounce");
+                BounceShadeKernel<<<blocks, kWaveThreads, 0, s>>>(sv, views, p, w, bounce, paths, g, o.counters);
+            }
+            {
+                StageScope scope(s, adjoint ? "adjoint_trace" : "trace_shadow");
+                PersistentTraceKernel<<<shadow_blocks, kWaveThreads, 0, s>>>(sv, shadow_source, o.counters, w,
+                                                                             adjoint);
+            }
+        }
+        {
+            StageScope scope(s, adjoint ? "adjoint_shade" : "trace_finish");
+            FinishKernel<<<blocks, kWaveThreads, 0, s>>>(views, w, paths, sample, o, g);
+        }
+    }
+    if (adjoint)
+    {
+        ReduceRowsKernel<<<DivUp(kGradEntries, 256u), 256, 0, s>>>(w.partials, rows, g.grad_albedo);
+    }
+}
+
+int CheckParams(const CxTraceParams* params, const CxTraceOutput* out, uint32_t width, uint32_t height, bool adjoint,
+                const char* what)
+{
+    if (params == nullptr || out == nullptr || out->counters == nullptr)
+    {
+        return ArgError("%s: NULL params, output or counters", what);
+    }
+    if (width == 0 || height == 0 || params->spp == 0)
+    {
+        return ArgError("%s: empty image or 0 samples per pixel", what);
+    }
+    if (params->bounces > 255)
+    {
+        return ArgError("%s: more than 255 bounces", what);
+    }
+    if (adjoint && params->bounces > 4)
+    {
+        return ArgError("%s: at most 4 bounces (3 albedo vertices per path)", what);
+    }
+    if ((params->flags & ~CX_TRACE_NO_RUSSIAN_ROULETTE) != 0)
+    {
+        return ArgError("%s: unknown flags", what);
+    }
+    return CX_OK;
+}
+
+// The frame (grads null) or its adjoint replay (grads set) of one scene.
 int RunFrame(const CxScene* scene, const CxLightTreeView* lights, const CxCamera* camera, const CxTraceParams* params,
              const CxTraceGradients* grads, const CxTraceOutput* out, void* stream, const char* what)
 {
@@ -941,108 +1080,51 @@ int RunFrame(const CxScene* scene, const CxLightTreeView* lights, const CxCamera
     {
         return r;
     }
-    if (scene->light_codes == nullptr || camera == nullptr || params == nullptr || out == nullptr || out->counters == nullptr)
+    if (scene->light_codes == nullptr || camera == nullptr)
     {
-        return ArgError("cx_trace_frame: NULL light codes, camera, params or counters");
+        return ArgError("%s: NULL light codes or camera", what);
     }
-    if (camera->width == 0 || camera->height == 0 || params->spp == 0)
+    if (scene->scene_count > 1)
     {
-        return ArgError("cx_trace_frame: empty image or 0 samples per pixel");
+        return ArgError("%s: a batch of scenes goes through cx_trace_batch", what);
     }
-    if (params->bounces > 255)
+    if (int r = CheckParams(params, out, camera->width, camera->height, grads != nullptr, what); r != CX_OK)
     {
-        return ArgError("cx_trace_frame: more than 255 bounces");
-    }
-    if (grads != nullptr && params->bounces > 4)
-    {
-        return ArgError("cx_trace_frame_adjoint: at most 4 bounces (3 albedo vertices per path)");
-    }
-    if ((params->flags & ~CX_TRACE_NO_RUSSIAN_ROULETTE) != 0)
-    {
-        return ArgError("cx_trace_frame: unknown flags");
+        return r;
     }
     if (uint64_t(camera->width) * camera->height > (uint64_t(1) << 28))
     {
-        return ArgError("cx_trace_frame: image larger than 2^28 pixels");
+        return ArgError("%s: image larger than 2^28 pixels", what);
     }
     const uint32_t paths = PathCount(camera->width, camera->height);
     if (out->workspace == nullptr || out->workspace_bytes < WorkspaceBytes(paths))
     {
-        return ArgError("cx_trace_frame: workspace missing or smaller than cx_trace_workspace_bytes");
+        return ArgError("%s: workspace missing or smaller than cx_trace_workspace_bytes", what);
     }
     const hipStream_t s = static_cast<hipStream_t>(stream);
-    const SceneView sv = SceneOf(*scene);
-    const LightView lv = LightsOf(*lights);
-    const CameraView cam = CameraOf(*camera);
     const Workspace w = CarveWorkspace(out->workspace, paths);
     FrameOut o{reinterpret_cast<float4*>(out->color), reinterpret_cast<float4*>(out->radiance), out->attrib,
                reinterpret_cast<unsigned long long*>(out->counters)};
     GradView g{nullptr, nullptr, nullptr, 1.0f / float(params->spp)};
-    const bool adjoint = grads != nullptr;
-    if (adjoint)
+    if (grads != nullptr)
     {
         g.d_color = reinterpret_cast<const float4*>(grads->d_color);
         g.grad_albedo = grads->grad_albedo;
         g.grad_scale = grads->grad_scale;
     }
-    const uint32_t blocks = DivUp(paths, kWaveThreads);
-    const uint32_t rows = GradRows(paths);
-    const uint32_t bounce_blocks = std::min(PersistentBlocks(PersistentTraceKernel<BounceSource>), rows);
-    const uint32_t shadow_blocks = std::min(PersistentBlocks(PersistentTraceKernel<ShadowSource>), rows);
-    if (bounce_blocks == 0 || shadow_blocks == 0)
+    if (PersistentBlocks(PersistentTraceKernel<BounceSource<OneScene>>) == 0 ||
+        PersistentBlocks(PersistentTraceKernel<ShadowSource<OneScene>>) == 0)
     {
         return HipResult(hipErrorInvalidDeviceFunction, "cx_trace_frame: occupancy query");
     }
-    if (adjoint)
-    {
-        const uint64_t n = uint64_t(rows) * kGradEntries;
-        ClearRowsKernel<<<uint32_t(DivUp(n, uint64_t(256))), 256, 0, s>>>(w.partials, n);
-    }
-    const ShadowSource shadow_source{w, sv, g};
-    for (uint32_t sample = 0; sample < params->spp; ++sample)
-    {
-        const TraceParams p{params->bounces, params->frame_seed + sample, lights->mis,
-                            (params->flags & CX_TRACE_NO_RUSSIAN_ROULETTE) != 0 ? 0u : 1u};
-        ResetKernel<<<1, 64, 0, s>>>(w);
-        {
-            StageScope scope(s, adjoint ? "adjoint_trace" : "trace_primary");
-            PrimaryTraceKernel<<<blocks, kWaveThreads, 0, s>>>(sv, cam, w, paths, o.counters);
-        }
-        {
-            StageScope scope(s, adjoint ? "adjoint_shade" : "trace_shade_primary");
-            PrimaryShadeKernel<<<blocks, kWaveThreads, 0, s>>>(sv, lv, cam, p, w, paths, g, o.counters);
-        }
-        {
-            StageScope scope(s, adjoint ? "adjoint_trace" : "trace_shadow");
-            PersistentTraceKernel<<<shadow_blocks, kWaveThreads, 0, s>>>(sv, shadow_source, o.counters, w, adjoint);
-        }
-        for (uint32_t bounce = 1; bounce <= params->bounces; ++bounce)
-        {
-            {
-                StageScope scope(s, adjoint ? "adjoint_trace" : "trace_extend");
-                PersistentTraceKernel<<<bounce_blocks, kWaveThreads, 0, s>>>(
-                    sv, BounceSource{w, (bounce - 1) & 1u}, o.counters, w, false);
-            }
-            {
-                StageScope scope(s, adjoint ? "adjoint_shade" : "trace_shade_bounce");
-                BounceShadeKernel<<<blocks, kWaveThreads, 0, s>>>(sv, lv, p, w, bounce, paths, g, o.counters);
-            }
-            {
-                StageScope scope(s, adjoint ? "adjoint_trace" : "trace_shadow");
-                PersistentTraceKernel<<<shadow_blocks, kWaveThreads, 0, s>>>(sv, shadow_source, o.counters, w,
-                                                                             adjoint);
-            }
-        }
-        {
-            StageScope scope(s, adjoint ? "adjoint_shade" : "trace_finish");
-            FinishKernel<<<blocks, kWaveThreads, 0, s>>>(cam, w, paths, sample, o, g);
-        }
-    }
-    if (adjoint)
-    {
-        ReduceRowsKernel<<<DivUp(kGradEntries, 256u), 256, 0, s>>>(w.partials, rows, g.grad_albedo);
-    }
+    RecordFrame(SceneOf(*scene), OneScene{LightsOf(*lights), CameraOf(*camera)}, paths, params, lights->mis, g, w, o, s);
     return LaunchResult(what);
+}
+
+// Per-scene device views of a batch: the light tree views, then the cameras.
+uint64_t BatchViewBytes(uint32_t scenes)
+{
+    return Align256(uint64_t(scenes) * sizeof(LightView)) + uint64_t(scenes) * sizeof(CameraView);
 }
 
 }  // namespace
@@ -1070,6 +1152,90 @@ CX_API int cx_trace_frame_adjoint(const CxScene* scene, const CxLightTreeView* l
         return cx::ArgError("cx_trace_frame_adjoint: NULL d_color or gradient buffer");
     }
     return cx::RunFrame(scene, lights, camera, params, grads, out, stream, "cx_trace_frame_adjoint");
+}
+
+CX_API uint64_t cx_trace_batch_bytes(uint32_t scenes)
+{
+    return cx::BatchViewBytes(scenes);
+}
+
+CX_API uint64_t cx_trace_batch_workspace_bytes(uint32_t width, uint32_t height, uint32_t scenes)
+{
+    return cx::WorkspaceBytes(uint32_t(std::min<uint64_t>(uint64_t(cx::PathCount(width, height)) * scenes, 0xFFFFFFFFull)));
+}
+
+CX_API int cx_trace_batch_prepare(const CxLightTreeView* lights, const CxCamera* cameras, uint32_t scenes,
+                                  void* device_views, void* stream)
+{
+    if (lights == nullptr || cameras == nullptr || device_views == nullptr || scenes == 0)
+    {
+        return cx::ArgError("cx_trace_batch_prepare: no scenes or a NULL array");
+    }
+    std::vector<uint8_t> host(cx::BatchViewBytes(scenes), 0);
+    cx::LightView* lv = reinterpret_cast<cx::LightView*>(host.data());
+    cx::CameraView* cv = reinterpret_cast<cx::CameraView*>(host.data() + cx::Align256(uint64_t(scenes) * sizeof(cx::LightView)));
+    for (uint32_t i = 0; i < scenes; ++i)
+    {
+        if (int r = cx::CheckLights(&lights[i], "cx_trace_batch_prepare"); r != CX_OK)
+        {
+            return r;
+        }
+        if (cameras[i].width != cameras[0].width || cameras[i].height != cameras[0].height)
+        {
+            return cx::ArgError("cx_trace_batch_prepare: the cameras of a batch share one image size");
+        }
+        lv[i] = cx::LightsOf(lights[i]);
+        cv[i] = cx::CameraOf(cameras[i]);
+    }
+    return cx::HipResult(hipMemcpyAsync(device_views, host.data(), host.size(), hipMemcpyHostToDevice,
+                                        static_cast<hipStream_t>(stream)),
+                         "cx_trace_batch_prepare");
+}
+
+CX_API int cx_trace_batch(const CxScene* scene, const void* device_views, uint32_t scenes, uint32_t width,
+                          uint32_t height, const CxTraceParams* params, const CxTraceOutput* out, void* stream)
+{
+    const char* what = "cx_trace_batch";
+    if (int r = cx::CheckScene(scene, what); r != CX_OK)
+    {
+        return r;
+    }
+    if (scene->light_codes == nullptr || device_views == nullptr || scenes == 0 ||
+        scenes != (scene->scene_count > 1 ? scene->scene_count : 1u))
+    {
+        return cx::ArgError("%s: NULL light codes or views, or scenes != scene->scene_count", what);
+    }
+    if (int r = cx::CheckParams(params, out, width, height, false, what); r != CX_OK)
+    {
+        return r;
+    }
+    const uint64_t per_scene = cx::PathCount(width, height);
+    if (per_scene * scenes > (uint64_t(1) << 30))
+    {
+        return cx::ArgError("%s: more than 2^30 paths", what);
+    }
+    const uint32_t paths = uint32_t(per_scene * scenes);
+    if (out->workspace == nullptr || out->workspace_bytes < cx::WorkspaceBytes(paths))
+    {
+        return cx::ArgError("%s: workspace missing or smaller than cx_trace_batch_workspace_bytes", what);
+    }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    const cx::Workspace w = cx::CarveWorkspace(out->workspace, paths);
+    cx::FrameOut o{reinterpret_cast<float4*>(out->color), reinterpret_cast<float4*>(out->radiance), out->attrib,
+                   reinterpret_cast<unsigned long long*>(out->counters)};
+    const uint8_t* base = static_cast<const uint8_t*>(device_views);
+    const cx::SceneBatch views{reinterpret_cast<const cx::LightView*>(base),
+                               reinterpret_cast<const cx::CameraView*>(base + cx::Align256(uint64_t(scenes) * sizeof(cx::LightView))),
+                               uint32_t(per_scene), width * height};
+    if (cx::PersistentBlocks(cx::PersistentTraceKernel<cx::BounceSource<cx::SceneBatch>>) == 0 ||
+        cx::PersistentBlocks(cx::PersistentTraceKernel<cx::ShadowSource<cx::SceneBatch>>) == 0)
+    {
+        return cx::HipResult(hipErrorInvalidDeviceFunction, "cx_trace_batch: occupancy query");
+    }
+    const cx::GradView g{nullptr, nullptr, nullptr, 1.0f / float(params->spp)};
+    // every scene's light view carries its mis flag; the launch takes the first one's
+    cx::RecordFrame(cx::SceneOf(*scene), views, paths, params, 1u, g, w, o, s);
+    return cx::LaunchResult(what);
 }
 
 CX_API int cx_trace_rays(const CxScene* scene, const float* rays, uint32_t count, uint32_t* hits, void* stream)

@@ -36,7 +36,16 @@ struct SceneView
     float emission_scale;
     uint32_t encoding;
     const float* brick_scale;    // [bricks] emission scale per brick, or null (1)
+    uint32_t key_words;          // 3: keys (x, y, z); 4: (scene, x, y, z), a batch of chunk scenes
+    uint64_t scene_cells;        // brick-bitmap bits per scene (grid_dim^3)
+    uint64_t scene_supers;       // super-bitmap bits per scene ((grid_dim / 8)^3)
 };
+
+// The (x, y, z) brick coordinates of a record.
+CX_DEV const int32_t* BrickXyz(const SceneView& s, uint32_t record)
+{
+    return s.keys + uint64_t(record) * s.key_words + (s.key_words - 3);
+}
 
 CX_DEV float BrickScale(const SceneView& s, uint32_t record)
 {
@@ -53,6 +62,7 @@ struct LightView
     const uint32_t* ordinal_of_brick; // brick record -> light ordinal or kLtNone
     uint32_t group_shift;             // a group is (1 << group_shift)^3 bricks
     uint32_t mis;
+    uint32_t leaf_base;               // light ordinal of leaf 0 of tree 0
 };
 
 struct CameraView
@@ -427,11 +437,22 @@ CX_DEV bool BitSet(const uint32_t* bits, uint64_t index)
     return ((bits[index >> 5] >> (index & 31u)) & 1u) != 0u;
 }
 
+// The hash key of brick (x, y, z) of a scene: the 3 coordinates, or (scene, x, y, z) in a batch.
+CX_DEV uint2 SceneBrickFingerprint(const SceneView& s, uint32_t scene, int32_t x, int32_t y, int32_t z)
+{
+    if (s.key_words == 4)
+    {
+        const uint32_t w[4] = {scene, uint32_t(x), uint32_t(y), uint32_t(z)};
+        return Fingerprint(w, 4);
+    }
+    return BrickFingerprint(x, y, z);
+}
+
 // The ray [o, o + t d], t in [0, t_max], through the brick grid: a DDA over the 8^3-brick super
 // cells and, inside a resident super cell, over its bricks; a resident brick is found through the
 // hash table and traced with the brick DDA. The first hit is the closest (bricks are disjoint and
 // visited in ray order). `any_hit`: a shadow ray (any hit with t < t_max).
-CX_DEV WorldHit TraceWorld(const SceneView& s, V3 o, V3 dir, float t_max)
+CX_DEV WorldHit TraceWorld(const SceneView& s, V3 o, V3 dir, float t_max, uint32_t scene = 0)
 {
     WorldHit result{0.0f, 0u, kHashNone, {0, 0, 0}, 0u, 0u, 0u};
     const RayInv ri = InverseOf(dir);
@@ -476,7 +497,7 @@ CX_DEV WorldHit TraceWorld(const SceneView& s, V3 o, V3 dir, float t_max)
             t_next[a] = (float(k * 64) + g0[a] - ov[a]) * inv[a];
         }
         const float t_leave = fminf(fminf(t_next[0], t_next[1]), t_next[2]);
-        const uint64_t super_index = (uint64_t(sc[2]) * super_dim + sc[1]) * super_dim + sc[0];
+        const uint64_t super_index = scene * s.scene_supers + (uint64_t(sc[2]) * super_dim + sc[1]) * super_dim + sc[0];
         if (BitSet(s.super_bits, super_index))
         {
             // The bricks of this super cell along the ray, from the entry t_cell.
@@ -490,13 +511,13 @@ CX_DEV WorldHit TraceWorld(const SceneView& s, V3 o, V3 dir, float t_max)
             while (true)
             {
                 result.cells += 1;
-                const uint64_t brick_index = (uint64_t(bc[2]) * s.grid_dim + bc[1]) * s.grid_dim + bc[0];
+                const uint64_t brick_index = scene * s.scene_cells + (uint64_t(bc[2]) * s.grid_dim + bc[1]) * s.grid_dim + bc[0];
                 if (BitSet(s.brick_bits, brick_index))
                 {
                     const int32_t key[3] = {s.grid_min[0] + bc[0], s.grid_min[1] + bc[1], s.grid_min[2] + bc[2]};
                     uint32_t probes = 0;
                     const uint32_t record = HashGet(s.slots, s.slot_count, s.pow2, s.hash_info[0],
-                                                    BrickFingerprint(key[0], key[1], key[2]), &probes);
+                                                    SceneBrickFingerprint(s, scene, key[0], key[1], key[2]), &probes);
                     result.probes += probes;
                     if (record != kHashNone)
                     {
@@ -595,6 +616,7 @@ struct WorldRay
     int sc0, sc1, sc2;
     int bc0, bc1, bc2;
     uint32_t phase;
+    uint32_t scene;
 };
 
 CX_DEV float GridOrigin(const SceneView& s, int a)
@@ -608,8 +630,9 @@ CX_DEV int StepOf(float d)
 }
 
 // Starts a ray; phase kRayDone when it misses the grid.
-CX_DEV void WorldRayBegin(const SceneView& s, V3 o, V3 dir, float t_max, WorldRay& r)
+CX_DEV void WorldRayBegin(const SceneView& s, V3 o, V3 dir, float t_max, uint32_t scene, WorldRay& r)
 {
+    r.scene = scene;
     r.o = o;
     r.dir = dir;
     r.ri = InverseOf(dir);
@@ -659,7 +682,8 @@ CX_DEV void WorldRayStep(const SceneView& s, WorldRay& r, WorldHit& hit, PathCou
     const int st0 = StepOf(r.dir.x), st1 = StepOf(r.dir.y), st2 = StepOf(r.dir.z);
     if (r.phase == kRaySuper)
     {
-        const uint64_t super_index = (uint64_t(r.sc2) * super_dim + r.sc1) * super_dim + r.sc0;
+        const uint64_t super_index =
+            r.scene * s.scene_supers + (uint64_t(r.sc2) * super_dim + r.sc1) * super_dim + r.sc0;
         if (BitSet(s.super_bits, super_index))
         {
             const int sc[3] = {r.sc0, r.sc1, r.sc2};
@@ -684,7 +708,8 @@ CX_DEV void WorldRayStep(const SceneView& s, WorldRay& r, WorldHit& hit, PathCou
     if (r.phase == kRayBricks)
     {
         c.cells += 1;
-        const uint64_t brick_index = (uint64_t(r.bc2) * s.grid_dim + r.bc1) * s.grid_dim + r.bc0;
+        const uint64_t brick_index =
+            r.scene * s.scene_cells + (uint64_t(r.bc2) * s.grid_dim + r.bc1) * s.grid_dim + r.bc0;
         if (BitSet(s.brick_bits, brick_index))
         {
             const int32_t key0 = s.grid_min[0] + r.bc0;
@@ -692,7 +717,7 @@ CX_DEV void WorldRayStep(const SceneView& s, WorldRay& r, WorldHit& hit, PathCou
             const int32_t key2 = s.grid_min[2] + r.bc2;
             uint32_t probes = 0;
             const uint32_t record = HashGet(s.slots, s.slot_count, s.pow2, s.hash_info[0],
-                                            BrickFingerprint(key0, key1, key2), &probes);
+                                            SceneBrickFingerprint(s, r.scene, key0, key1, key2), &probes);
             c.probes += probes;
             if (record != kHashNone)
             {

@@ -149,6 +149,10 @@ CX_API int cx_hash_maintain(const CxHashTable* table, uint32_t payload_capacity,
  * displacement, info[2] = keys wrapped past the last slot. No counterpart in the reference, whose
  * keys are inserted one lookup at a time (HashLookupOrInsert); the slot format and probing are its. */
 CX_API uint64_t cx_hash_build_scratch_bytes(uint32_t count);
+/* The same bulk build over keys of `word_count` uint32 words each (1..8; the fingerprint of
+ * caelix_hash_table.hlsl:113-128): the batched scenes key their bricks by (scene, x, y, z). */
+CX_API int cx_hash_build_keys(const uint32_t* words, uint32_t word_count, uint32_t count, HashKeySlot* slots,
+                              uint32_t slot_count, void* scratch, uint32_t* info, void* stream);
 CX_API int cx_hash_build_bricks(const int32_t* keys_xyz, uint32_t count, HashKeySlot* slots, uint32_t slot_count,
                                 void* scratch, uint32_t* info, void* stream);
 /* Lookups in a bulk-built table: payloads[i] = the brick's index or kHashNone; probes[i] (nullable) =
@@ -189,6 +193,11 @@ typedef struct CxScene {
      * for 1. Multiplies the emission the path tracer adds; the light tree samples with the unscaled
      * emission. */
     const float* brick_emission_scale;
+    /* A batch of chunk scenes when > 1: keys are [bricks][4] = (scene, x, y, z), the hash table is
+     * keyed by those 4 words (cx_hash_build_keys), the bitmaps hold scene_count grids of grid_dim^3
+     * (and (grid_dim / 8)^3) bits, scene-major. 0 or 1: one scene, keys [bricks][3]. */
+    uint32_t scene_count;
+    uint32_t pad2;
 } CxScene;
 
 /* The record producer's opaque subset (shaders/scene/caelix_finalize.hlsl:112-361): word 0 (group
@@ -226,6 +235,8 @@ typedef struct CxLightTreeView {
     const uint32_t* ordinal_of_brick;
     uint32_t group_shift;
     uint32_t mis;
+    uint32_t leaf_base; /* light ordinal of leaf 0 of tree 0 (a scene's slice of a batch's leaves), else 0 */
+    uint32_t pad;
 } CxLightTreeView;
 
 /* Pinhole camera (caelix_frame.hlsl:164-171): unit forward/right/up. */
@@ -293,6 +304,22 @@ typedef struct CxTraceGradients {
 CX_API int cx_trace_frame_adjoint(const CxScene* scene, const CxLightTreeView* lights, const CxCamera* camera,
                                   const CxTraceParams* params, const CxTraceGradients* grads,
                                   const CxTraceOutput* out, void* stream);
+
+/* The batched form of the frame: `scenes` chunk scenes of one batched CxScene (scene_count =
+ * scenes), each with its own light tree view and camera (one image size), traced as one wavefront:
+ * the paths of every scene go through the same launches (scene s owns paths [s * P, (s + 1) * P),
+ * P = the 8x8-tile-padded pixel count), so the whole batch costs one frame's launches. Outputs are
+ * the per-scene images, scene-major (width * height pixels each). cx_trace_batch_prepare copies the
+ * views and cameras to `device_views` (cx_trace_batch_bytes(scenes) bytes of device memory);
+ * cx_trace_batch records only kernel launches (it can be captured in a hipGraph). The workspace is
+ * cx_trace_batch_workspace_bytes(width, height, scenes) bytes. Each scene's image is the one
+ * cx_trace_frame makes of it alone. No counterpart in the reference. */
+CX_API uint64_t cx_trace_batch_bytes(uint32_t scenes);
+CX_API uint64_t cx_trace_batch_workspace_bytes(uint32_t width, uint32_t height, uint32_t scenes);
+CX_API int cx_trace_batch_prepare(const CxLightTreeView* lights, const CxCamera* cameras, uint32_t scenes,
+                                  void* device_views, void* stream);
+CX_API int cx_trace_batch(const CxScene* scene, const void* device_views, uint32_t scenes, uint32_t width,
+                          uint32_t height, const CxTraceParams* params, const CxTraceOutput* out, void* stream);
 
 /* Rays through the world (TraceScene / TraceShadow semantics): rays[i] = origin(3), direction(3),
  * t_max; hits[i] = {t (float bits), attrib, record, brick x, y, z, cells, probes, steps}. */

@@ -128,14 +128,14 @@ class _Scene(ctypes.Structure):
         ("bricks", _U32), ("hash_info", _P), ("brick_bits", _P), ("super_bits", _P),
         ("grid_min", ctypes.c_int32 * 3), ("grid_dim", _U32), ("group_shift", _U32), ("encoding", _U32),
         ("materials", _P), ("light_codes", _P), ("emission_scale", ctypes.c_float), ("pad", _U32),
-        ("brick_emission_scale", _P),
+        ("brick_emission_scale", _P), ("scene_count", _U32), ("pad2", _U32),
     ]
 
 
 class _TreeView(ctypes.Structure):
     _fields_ = [
         ("headers", _P), ("nodes", _P * 2), ("leaves", _P * 2), ("heap", _P), ("record_offset", _P),
-        ("ordinal_of_brick", _P), ("group_shift", _U32), ("mis", _U32),
+        ("ordinal_of_brick", _P), ("group_shift", _U32), ("mis", _U32), ("leaf_base", _U32), ("pad", _U32),
     ]
 
 

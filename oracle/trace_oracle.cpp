@@ -933,12 +933,13 @@ float LightFacePdf(const CxScene& s, const CxLightTreeView& L, V x, V n, uint32_
     }
     found = true;
     const float in_brick = (w[face] / sum) / float(count);
-    if (ordinal >= L.headers[0].leaf_count)
+    const uint32_t leaf = ordinal - L.leaf_base;
+    if (leaf >= L.headers[0].leaf_count)
     {
         found = false;
         return 0.0f;
     }
-    return LeafProbability(L, 0, ordinal, x, n) * in_brick;
+    return LeafProbability(L, 0, leaf, x, n) * in_brick;
 }
 float MisWeight(float p_area, V n_r, V w, float d, float cos_l, bool nee_side)
 {
