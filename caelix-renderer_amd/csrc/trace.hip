@@ -486,7 +486,7 @@ struct ShadowSource
 template <class Source>
 __global__ void __launch_bounds__(kWaveThreads) PersistentTraceKernel(SceneView s, Source src,
                                                                       unsigned long long* counters, Workspace w,
-                                                                      bool adjoint)
+                                                                      bool adjoint, uint32_t chunk)
 {
     __shared__ float lds_grad[kGradEntries];
     if (adjoint)
@@ -517,11 +517,11 @@ __global__ void __launch_bounds__(kWaveThreads) PersistentTraceKernel(SceneView 
                 uint32_t base = 0;
                 if (Lane() == 0)
                 {
-                    base = AtomicAddRelaxed(cursor, 64u);
+                    base = AtomicAddRelaxed(cursor, chunk);
                 }
                 base = __shfl(base, 0);
                 chunk_next = min(base, count);
-                chunk_end = min(base + 64u, count);
+                chunk_end = min(base + chunk, count);
                 drained = chunk_next == chunk_end;
             }
             const uint32_t take = min(uint32_t(PopC(idle)), chunk_end - chunk_next);
@@ -959,6 +959,30 @@ int CheckLights(const CxLightTreeView* l, const char* what)
     return CX_OK;
 }
 
+// Tunables of the trace (cx_trace_set_tuning): the ray-queue depth a wave claims per refill, and the
+// persistent traversal's workgroups per CU (0: as many as fit).
+CxTraceTuning g_tuning{64u, 0u};
+
+uint32_t DeviceCUs()
+{
+    static int cus = 0;
+    if (cus == 0)
+    {
+        int device = 0;
+        if (hipGetDevice(&device) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        {
+            cus = 1;
+        }
+    }
+    return uint32_t(cus);
+}
+
+uint32_t TunedBlocks(uint32_t fit)
+{
+    return g_tuning.workgroups_per_cu == 0 ? fit : std::min(fit, g_tuning.workgroups_per_cu * DeviceCUs());
+}
+
 // Workgroups of a persistent kernel: as many as fit on the device at once.
 template <class Kernel>
 uint32_t PersistentBlocks(Kernel kernel)
@@ -989,8 +1013,9 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
     const bool adjoint = g.d_color != nullptr;
     const uint32_t blocks = DivUp(paths, kWaveThreads);
     const uint32_t rows = GradRows(paths);
-    const uint32_t bounce_blocks = std::min(PersistentBlocks(PersistentTraceKernel<BounceSource<Views>>), rows);
-    const uint32_t shadow_blocks = std::min(PersistentBlocks(PersistentTraceKernel<ShadowSource<Views>>), rows);
+    const uint32_t bounce_blocks = std::min(TunedBlocks(PersistentBlocks(PersistentTraceKernel<BounceSource<Views>>)), rows);
+    const uint32_t shadow_blocks = std::min(TunedBlocks(PersistentBlocks(PersistentTraceKernel<ShadowSource<Views>>)), rows);
+    const uint32_t chunk = g_tuning.queue_chunk;
     if (adjoint)
     {
         const uint64_t n = uint64_t(rows) * kGradEntries;
@@ -1012,14 +1037,15 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
         }
         {
             StageScope scope(s, adjoint ? "adjoint_trace" : "trace_shadow");
-            PersistentTraceKernel<<<shadow_blocks, kWaveThreads, 0, s>>>(sv, shadow_source, o.counters, w, adjoint);
+            PersistentTraceKernel<<<shadow_blocks, kWaveThreads, 0, s>>>(sv, shadow_source, o.counters, w, adjoint,
+                                                                         chunk);
         }
         for (uint32_t bounce = 1; bounce <= params->bounces; ++bounce)
         {
             {
                 StageScope scope(s, adjoint ? "adjoint_trace" : "trace_extend");
                 PersistentTraceKernel<<<bounce_blocks, kWaveThreads, 0, s>>>(
-                    sv, BounceSource<Views>{w, (bounce - 1) & 1u, views}, o.counters, w, false);
+                    sv, BounceSource<Views>{w, (bounce - 1) & 1u, views}, o.counters, w, false, chunk);
             }
             {
                 StageScope scope(s, adjoint ? "adjoint_shade" : "trace_shade_bounce");
@@ -1028,7 +1054,7 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
             {
                 StageScope scope(s, adjoint ? "adjoint_trace" : "trace_shadow");
                 PersistentTraceKernel<<<shadow_blocks, kWaveThreads, 0, s>>>(sv, shadow_source, o.counters, w,
-                                                                             adjoint);
+                                                                             adjoint, chunk);
             }
         }
         {
@@ -1152,6 +1178,24 @@ CX_API int cx_trace_frame_adjoint(const CxScene* scene, const CxLightTreeView* l
         return cx::ArgError("cx_trace_frame_adjoint: NULL d_color or gradient buffer");
     }
     return cx::RunFrame(scene, lights, camera, params, grads, out, stream, "cx_trace_frame_adjoint");
+}
+
+CX_API int cx_trace_set_tuning(const CxTraceTuning* tuning)
+{
+    if (tuning == nullptr || tuning->queue_chunk == 0 || tuning->queue_chunk > 4096 || tuning->workgroups_per_cu > 64)
+    {
+        return cx::ArgError("cx_trace_set_tuning: queue_chunk in 1..4096, workgroups_per_cu in 0..64");
+    }
+    cx::g_tuning = *tuning;
+    return CX_OK;
+}
+
+CX_API void cx_trace_get_tuning(CxTraceTuning* tuning)
+{
+    if (tuning != nullptr)
+    {
+        *tuning = cx::g_tuning;
+    }
 }
 
 CX_API uint64_t cx_trace_batch_bytes(uint32_t scenes)

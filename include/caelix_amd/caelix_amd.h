@@ -305,6 +305,17 @@ CX_API int cx_trace_frame_adjoint(const CxScene* scene, const CxLightTreeView* l
                                   const CxTraceParams* params, const CxTraceGradients* grads,
                                   const CxTraceOutput* out, void* stream);
 
+/* Tunables of cx_trace_frame / cx_trace_batch (process-wide; set between frames): queue_chunk = the
+ * ray-queue depth a wave claims per refill of the persistent traversal (default 64), and
+ * workgroups_per_cu = its persistent workgroups per CU (0 = as many as fit, the default). The
+ * autotuner of the Python package picks them per workload. They change speed only, never results. */
+typedef struct CxTraceTuning {
+    uint32_t queue_chunk;
+    uint32_t workgroups_per_cu;
+} CxTraceTuning;
+CX_API int cx_trace_set_tuning(const CxTraceTuning* tuning);
+CX_API void cx_trace_get_tuning(CxTraceTuning* tuning);
+
 /* The batched form of the frame: `scenes` chunk scenes of one batched CxScene (scene_count =
  * scenes), each with its own light tree view and camera (one image size), traced as one wavefront:
  * the paths of every scene go through the same launches (scene s owns paths [s * P, (s + 1) * P),

@@ -89,3 +89,25 @@ def test_batch_graph_replays_match_eager(batch_and_plain):
         torch.cuda.synchronize()
         for a, b in zip(batch.images(), eager):
             assert torch.equal(a, b), replay
+
+
+def test_tuning_changes_speed_not_results(batch_and_plain):
+    from caelix_amd import autotune
+
+    datas, batch, plain = batch_and_plain
+    before = autotune.get_tuning()
+    try:
+        batch.build_trees()
+        batch.trace(2, 23, 1)
+        reference = [t.clone() for t in batch.images()]
+        for chunk, per_cu in [(32, 0), (256, 2), (64, 4)]:
+            autotune.set_tuning(chunk, per_cu)
+            batch.trace(2, 23, 1)
+            for a, b in zip(batch.images(), reference):
+                assert np.array_equal(a.cpu().numpy(), b.cpu().numpy()), (chunk, per_cu)
+        with pytest.raises(RuntimeError):
+            autotune.set_tuning(0, 0)
+        result = autotune.tune("test-batch", lambda: batch.trace(2, 23, 1), reps=1, use_cache=False)
+        assert result["queue_chunk"] in autotune.QUEUE_CHUNKS and len(result["trials"]) == 12
+    finally:
+        autotune.set_tuning(before["queue_chunk"], before["workgroups_per_cu"])
