@@ -50,10 +50,10 @@ enum : uint32_t
     kCtlWords = 64,
 };
 
-// The bottom-up tree build runs in levels of nested leaf blocks: a level-1 workgroup owns 2048
-// sorted leaves, a level-L workgroup the 32 level-(L-1) blocks below it.
+// The bottom-up tree build runs in levels of nested leaf blocks: a level-1 workgroup owns 512
+// sorted leaves (and works in LDS), a level-L workgroup the 32 level-(L-1) blocks below it.
 constexpr uint32_t kTreeThreads = 256;
-constexpr uint32_t kTreeTile = 2048;
+constexpr uint32_t kTreeTile = 512;
 constexpr uint32_t kTreeFanout = 32;
 constexpr uint32_t kTopsCap = 256;
 
@@ -656,7 +656,239 @@ CX_DEV bool ClimbStep(const Build& b, int n, uint32_t& a, uint32_t& last, uint64
     return true;
 }
 
-// Level 1: the workgroup's own 2048 sorted leaves. Level L > 1: the tops of its 32 sub-blocks.
+// ---- level 1 in LDS ------------------------------------------------------------------------------
+// A level-1 workgroup keeps everything of its block of kTreeTile sorted leaves in LDS: the keys of
+// the block and its two neighbours, the leaf summaries (gathered once, in sorted order), the gap
+// slots and cover diffs, and the outputs of the nodes it builds (children, split, parent links,
+// aggregates), which it writes out with coalesced stores at the end. The climb is ClimbStep's.
+constexpr uint32_t kUnset = 0xFFFFFFFEu;  // a staged output the block did not write
+
+struct Level1Lds
+{
+    uint32_t keys[kTreeTile + 2];  // sorted keys [block_lo - 1, block_hi]
+    float leaf[kTreeTile][12];     // summaries of the block's sorted leaves
+    float agg[kTreeTile][12];      // aggregates of the nodes built here (node - block_lo)
+    uint32_t slot[kTreeTile];
+    int32_t diff[kTreeTile];
+    uint32_t left[kTreeTile];
+    uint32_t right[kTreeTile];
+    uint32_t split[kTreeTile];
+    uint32_t parent[kTreeTile];       // parent of internal node block_lo + k
+    uint32_t leaf_parent[kTreeTile];  // parent of sorted leaf block_lo + k
+};
+
+CX_DEV int GapDeltaLds(const Level1Lds& l, int g, int n, int lo)
+{
+    if (g < 0 || g + 1 >= n)
+    {
+        return -1;
+    }
+    const uint32_t k0 = l.keys[g - (lo - 1)];
+    const uint32_t k1 = l.keys[g + 1 - (lo - 1)];
+    if (k0 == k1)
+    {
+        return 32 + int(Clz(uint32_t(g) ^ uint32_t(g + 1)));
+    }
+    return int(Clz(k0 ^ k1));
+}
+
+CX_DEV Summary LdsSummary(const float (&v)[12])
+{
+    Summary s;
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+    {
+        s.face[f] = v[f];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        s.lo[k] = v[6 + k];
+        s.hi[k] = v[9 + k];
+    }
+    return s;
+}
+
+CX_DEV void LdsStore(float (&v)[12], const Summary& s)
+{
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+    {
+        v[f] = s.face[f];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+        v[6 + k] = s.lo[k];
+        v[9 + k] = s.hi[k];
+    }
+}
+
+CX_DEV Summary Level1Summary(const Level1Lds& l, uint32_t element, uint32_t lo)
+{
+    return (element & kLtChildLeafBit) != 0 ? LdsSummary(l.leaf[(element & ~kLtChildLeafBit) - lo])
+                                            : LdsSummary(l.agg[element - lo]);
+}
+
+// ClimbStep for level 1: every arrival is in LDS.
+CX_DEV bool ClimbStepLds(Level1Lds& l, int n, uint32_t& a, uint32_t& last, uint32_t lo, uint32_t hi, uint2* tops,
+                         uint32_t* lds_top_count)
+{
+    if (a == 0 && last == uint32_t(n - 1))
+    {
+        return false;  // the root (node 0) is finished
+    }
+    const int d_left = GapDeltaLds(l, int(a) - 1, n, int(lo));
+    const int d_right = GapDeltaLds(l, int(last), n, int(lo));
+    const bool is_left_child = d_right > d_left;
+    const uint32_t gap = is_left_child ? last : a - 1;
+    if (gap < lo || gap + 1 >= hi || (a == 0 && !is_left_child))
+    {
+        const uint32_t k = atomicAdd(lds_top_count, 1u);
+        if (k < kTopsCap)
+        {
+            tops[k] = make_uint2(a, last);
+        }
+        return false;
+    }
+    const uint32_t far_end = is_left_child ? a : last;
+    const uint32_t other =
+        __hip_atomic_exchange(&l.slot[gap - lo], far_end, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (other == kLtNone)
+    {
+        return false;  // the sibling finishes the parent
+    }
+    const uint32_t pa = is_left_child ? a : other;
+    const uint32_t pb = is_left_child ? other : last;
+    const uint32_t left = pa == gap ? (kLtChildLeafBit | gap) : gap;
+    const uint32_t right = pb == gap + 1 ? (kLtChildLeafBit | (gap + 1)) : gap + 1;
+    uint32_t node;
+    if (pa == 0 && pb == uint32_t(n - 1))
+    {
+        node = 0;
+        l.parent[0] = kLtNone;  // the block holds the whole tree (lo = 0)
+    }
+    else
+    {
+        const bool parent_is_left = GapDeltaLds(l, int(pb), n, int(lo)) > GapDeltaLds(l, int(pa) - 1, n, int(lo));
+        node = parent_is_left ? pb : pa;
+    }
+    l.left[node - lo] = left;
+    l.right[node - lo] = right;
+    l.split[node - lo] = gap;
+    ((left & kLtChildLeafBit) != 0 ? l.leaf_parent : l.parent)[(left & ~kLtChildLeafBit) - lo] = node;
+    ((right & kLtChildLeafBit) != 0 ? l.leaf_parent : l.parent)[(right & ~kLtChildLeafBit) - lo] = node;
+    atomicAdd(&l.diff[pa - lo], 1);
+    atomicAdd(&l.diff[pb - lo], -1);
+    const Summary s = Union(Level1Summary(l, left, lo), Level1Summary(l, right, lo));
+    LdsStore(l.agg[node - lo], s);
+    a = pa;
+    last = pb;
+    return true;
+}
+
+CX_DEV void TreeLevel1(const Build& b, int n, uint32_t lo, uint32_t hi, uint2* tops, uint32_t* lds_top_count,
+                       Level1Lds& l)
+{
+    const uint32_t t = threadIdx.x;
+    const uint32_t* keys = b.scratch.keys(0);
+    const uint32_t* values = b.scratch.values(0);
+    for (uint32_t k = t; k < kTreeTile + 2; k += blockDim.x)
+    {
+        const int64_t j = int64_t(lo) - 1 + k;
+        l.keys[k] = (j >= 0 && j < n) ? keys[j] : 0u;
+    }
+    for (uint32_t k = t; k < kTreeTile; k += blockDim.x)
+    {
+        l.slot[k] = kLtNone;
+        l.diff[k] = 0;
+        l.left[k] = kUnset;
+        l.parent[k] = kUnset;
+        l.leaf_parent[k] = kUnset;
+    }
+    // The leaf summaries in sorted order: 3 x 16-byte loads per leaf, all issued up front.
+    for (uint32_t k = t; k < kTreeTile; k += blockDim.x)
+    {
+        if (lo + k < uint32_t(n))
+        {
+            const float4* src = reinterpret_cast<const float4*>(b.leaves[values[lo + k]].summary.face);
+            const float4 p = src[0];
+            const float4 q = src[1];
+            const float4 r = src[2];
+            float* d = l.leaf[k];
+            d[0] = p.x, d[1] = p.y, d[2] = p.z, d[3] = p.w;
+            d[4] = q.x, d[5] = q.y, d[6] = q.z, d[7] = q.w;
+            d[8] = r.x, d[9] = r.y, d[10] = r.z, d[11] = r.w;
+        }
+    }
+    __syncthreads();
+    uint32_t next = t;
+    uint32_t a = 0;
+    uint32_t last = 0;
+    bool live = false;
+    while (true)
+    {
+        if (!live)
+        {
+            if (next >= kTreeTile || lo + next >= uint32_t(n))
+            {
+                break;
+            }
+            a = lo + next;
+            last = a;
+            next += blockDim.x;
+        }
+        live = ClimbStepLds(l, n, a, last, lo, hi, tops, lds_top_count);
+    }
+    __syncthreads();
+    // Coalesced write-out of the block.
+    uint32_t* slots = b.scratch.u32(kLtScratchCounter);
+    int32_t* diff = b.scratch.cover_diff();
+    uint32_t* left_out = b.scratch.u32(kLtScratchLeft);
+    uint32_t* right_out = b.scratch.u32(kLtScratchRight);
+    uint32_t* split_out = b.scratch.split();
+    uint32_t* parent_out = b.scratch.u32(kLtScratchParent);
+    uint32_t* leaf_parent_out = b.scratch.u32(kLtScratchSortedLeafParent);
+    float* agg_out = b.scratch.aggregates();
+    for (uint32_t k = t; k < kTreeTile; k += blockDim.x)
+    {
+        const uint32_t g = lo + k;
+        if (g + 1 < uint32_t(n))
+        {
+            slots[g] = l.slot[k];
+        }
+        if (g < uint32_t(n))
+        {
+            diff[g] = l.diff[k];
+        }
+        if (l.left[k] != kUnset)
+        {
+            left_out[g] = l.left[k];
+            right_out[g] = l.right[k];
+            split_out[g] = l.split[k];
+        }
+        if (l.parent[k] != kUnset)
+        {
+            parent_out[g] = l.parent[k];
+        }
+        if (l.leaf_parent[k] != kUnset)
+        {
+            leaf_parent_out[g] = l.leaf_parent[k];
+        }
+    }
+    // aggregates: 12 floats per node, as float4 triples over consecutive threads
+    for (uint32_t q = t; q < kTreeTile * 3; q += blockDim.x)
+    {
+        const uint32_t k = q / 3;
+        if (l.left[k] != kUnset)
+        {
+            const float* v = &l.agg[k][(q % 3) * 4];
+            reinterpret_cast<float4*>(agg_out + uint64_t(lo + k) * 12)[q % 3] = make_float4(v[0], v[1], v[2], v[3]);
+        }
+    }
+}
+
+// Level 1: the workgroup's own 512 sorted leaves. Level L > 1: the tops of its 32 sub-blocks.
 template <class Set>
 __global__ void __launch_bounds__(1024) TreeLevelKernel(Set set, uint32_t level)
 {
@@ -667,8 +899,7 @@ __global__ void __launch_bounds__(1024) TreeLevelKernel(Set set, uint32_t level)
     }
     __shared__ uint32_t lds_top_count;
     __shared__ uint32_t lds_offsets[kTreeFanout + 1];
-    __shared__ uint32_t lds_slot[kTreeTile];
-    __shared__ int32_t lds_diff[kTreeTile];
+    __shared__ Level1Lds lds_level1;
     const uint32_t t = threadIdx.x;
     const int n = int(b.scratch.ctl()[kCtlLeafCount]);
     const uint64_t tile = TreeTile(level);
@@ -684,49 +915,7 @@ __global__ void __launch_bounds__(1024) TreeLevelKernel(Set set, uint32_t level)
     {
         if (level == 1)
         {
-            // Gap slots and gap-cover diffs of the block in LDS.
-            for (uint32_t k = t; k < kTreeTile; k += blockDim.x)
-            {
-                lds_slot[k] = kLtNone;
-                lds_diff[k] = 0;
-            }
-            __syncthreads();
-            // Each lane climbs one element at a time and takes its next leaf as soon as the climb
-            // ends, so a long climb holds up only its own lane.
-            uint32_t next = t;
-            uint32_t a = 0;
-            uint32_t last = 0;
-            bool live = false;
-            while (true)
-            {
-                if (!live)
-                {
-                    const uint64_t j = block_lo + next;
-                    if (next >= kTreeTile || j >= uint64_t(n))
-                    {
-                        break;
-                    }
-                    a = uint32_t(j);
-                    last = a;
-                    next += blockDim.x;
-                }
-                live = ClimbStep<true>(b, n, a, last, block_lo, block_hi, tops, &lds_top_count, lds_slot, lds_diff);
-            }
-            __syncthreads();
-            uint32_t* slots = b.scratch.u32(kLtScratchCounter);
-            int32_t* diff = b.scratch.cover_diff();
-            for (uint32_t k = t; k < kTreeTile; k += blockDim.x)
-            {
-                const uint64_t g = block_lo + k;
-                if (g + 1 < uint64_t(n))
-                {
-                    slots[g] = lds_slot[k];
-                }
-                if (g < uint64_t(n))
-                {
-                    diff[g] = lds_diff[k];
-                }
-            }
+            TreeLevel1(b, n, uint32_t(block_lo), uint32_t(block_hi), tops, &lds_top_count, lds_level1);
         }
         else
         {
