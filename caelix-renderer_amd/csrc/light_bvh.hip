@@ -52,7 +52,7 @@ enum : uint32_t
 
 // The bottom-up tree build runs in levels of nested leaf blocks: a level-1 workgroup owns 512
 // sorted leaves (and works in LDS), a level-L workgroup the 32 level-(L-1) blocks below it.
-constexpr uint32_t kTreeThreads = 256;
+constexpr uint32_t kTreeThreads = 512;  // level 1: one lane per leaf
 constexpr uint32_t kTreeTile = 512;
 constexpr uint32_t kTreeFanout = 32;
 constexpr uint32_t kTopsCap = 256;
@@ -665,7 +665,7 @@ constexpr uint32_t kUnset = 0xFFFFFFFEu;  // a staged output the block did not w
 
 struct Level1Lds
 {
-    uint32_t keys[kTreeTile + 2];  // sorted keys [block_lo - 1, block_hi]
+    int32_t delta[kTreeTile + 1];  // delta of the gaps [block_lo - 1, block_hi - 1]
     float leaf[kTreeTile][12];     // summaries of the block's sorted leaves
     float agg[kTreeTile][12];      // aggregates of the nodes built here (node - block_lo)
     uint32_t slot[kTreeTile];
@@ -677,19 +677,9 @@ struct Level1Lds
     uint32_t leaf_parent[kTreeTile];  // parent of sorted leaf block_lo + k
 };
 
-CX_DEV int GapDeltaLds(const Level1Lds& l, int g, int n, int lo)
+CX_DEV int GapDeltaLds(const Level1Lds& l, int g, int lo)
 {
-    if (g < 0 || g + 1 >= n)
-    {
-        return -1;
-    }
-    const uint32_t k0 = l.keys[g - (lo - 1)];
-    const uint32_t k1 = l.keys[g + 1 - (lo - 1)];
-    if (k0 == k1)
-    {
-        return 32 + int(Clz(uint32_t(g) ^ uint32_t(g + 1)));
-    }
-    return int(Clz(k0 ^ k1));
+    return l.delta[g - (lo - 1)];
 }
 
 CX_DEV Summary LdsSummary(const float (&v)[12])
@@ -730,16 +720,16 @@ CX_DEV Summary Level1Summary(const Level1Lds& l, uint32_t element, uint32_t lo)
                                             : LdsSummary(l.agg[element - lo]);
 }
 
-// ClimbStep for level 1: every arrival is in LDS.
-CX_DEV bool ClimbStepLds(Level1Lds& l, int n, uint32_t& a, uint32_t& last, uint32_t lo, uint32_t hi, uint2* tops,
-                         uint32_t* lds_top_count)
+// ClimbStep for level 1: every arrival is in LDS; the climbing lane carries its element's summary.
+CX_DEV bool ClimbStepLds(Level1Lds& l, int n, uint32_t& a, uint32_t& last, Summary& own, uint32_t lo, uint32_t hi,
+                         uint2* tops, uint32_t* lds_top_count)
 {
     if (a == 0 && last == uint32_t(n - 1))
     {
         return false;  // the root (node 0) is finished
     }
-    const int d_left = GapDeltaLds(l, int(a) - 1, n, int(lo));
-    const int d_right = GapDeltaLds(l, int(last), n, int(lo));
+    const int d_left = GapDeltaLds(l, int(a) - 1, int(lo));
+    const int d_right = GapDeltaLds(l, int(last), int(lo));
     const bool is_left_child = d_right > d_left;
     const uint32_t gap = is_left_child ? last : a - 1;
     if (gap < lo || gap + 1 >= hi || (a == 0 && !is_left_child))
@@ -770,7 +760,7 @@ CX_DEV bool ClimbStepLds(Level1Lds& l, int n, uint32_t& a, uint32_t& last, uint3
     }
     else
     {
-        const bool parent_is_left = GapDeltaLds(l, int(pb), n, int(lo)) > GapDeltaLds(l, int(pa) - 1, n, int(lo));
+        const bool parent_is_left = GapDeltaLds(l, int(pb), int(lo)) > GapDeltaLds(l, int(pa) - 1, int(lo));
         node = parent_is_left ? pb : pa;
     }
     l.left[node - lo] = left;
@@ -780,8 +770,9 @@ CX_DEV bool ClimbStepLds(Level1Lds& l, int n, uint32_t& a, uint32_t& last, uint3
     ((right & kLtChildLeafBit) != 0 ? l.leaf_parent : l.parent)[(right & ~kLtChildLeafBit) - lo] = node;
     atomicAdd(&l.diff[pa - lo], 1);
     atomicAdd(&l.diff[pb - lo], -1);
-    const Summary s = Union(Level1Summary(l, left, lo), Level1Summary(l, right, lo));
-    LdsStore(l.agg[node - lo], s);
+    const Summary sibling = Level1Summary(l, is_left_child ? right : left, lo);
+    own = is_left_child ? Union(own, sibling) : Union(sibling, own);
+    LdsStore(l.agg[node - lo], own);
     a = pa;
     last = pb;
     return true;
@@ -793,10 +784,18 @@ CX_DEV void TreeLevel1(const Build& b, int n, uint32_t lo, uint32_t hi, uint2* t
     const uint32_t t = threadIdx.x;
     const uint32_t* keys = b.scratch.keys(0);
     const uint32_t* values = b.scratch.values(0);
-    for (uint32_t k = t; k < kTreeTile + 2; k += blockDim.x)
+    // delta(g) of the gaps g = lo - 1 .. hi - 1 (-1 outside [0, n - 1)), as GapDelta.
+    for (uint32_t k = t; k < kTreeTile + 1; k += blockDim.x)
     {
-        const int64_t j = int64_t(lo) - 1 + k;
-        l.keys[k] = (j >= 0 && j < n) ? keys[j] : 0u;
+        const int64_t g = int64_t(lo) - 1 + k;
+        int d = -1;
+        if (g >= 0 && g + 1 < n)
+        {
+            const uint32_t k0 = keys[g];
+            const uint32_t k1 = keys[g + 1];
+            d = k0 == k1 ? 32 + int(Clz(uint32_t(g) ^ uint32_t(g + 1))) : int(Clz(k0 ^ k1));
+        }
+        l.delta[k] = d;
     }
     for (uint32_t k = t; k < kTreeTile; k += blockDim.x)
     {
@@ -825,6 +824,7 @@ CX_DEV void TreeLevel1(const Build& b, int n, uint32_t lo, uint32_t hi, uint2* t
     uint32_t next = t;
     uint32_t a = 0;
     uint32_t last = 0;
+    Summary own;
     bool live = false;
     while (true)
     {
@@ -836,9 +836,10 @@ CX_DEV void TreeLevel1(const Build& b, int n, uint32_t lo, uint32_t hi, uint2* t
             }
             a = lo + next;
             last = a;
+            own = LdsSummary(l.leaf[next]);
             next += blockDim.x;
         }
-        live = ClimbStepLds(l, n, a, last, lo, hi, tops, lds_top_count);
+        live = ClimbStepLds(l, n, a, last, own, lo, hi, tops, lds_top_count);
     }
     __syncthreads();
     // Coalesced write-out of the block.
@@ -1116,20 +1117,35 @@ __global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Set set)
     const uint32_t* split = b.scratch.split();
     uint32_t* counter = b.scratch.u32(kLtScratchCounter);
     uint32_t* wide = b.scratch.u32(kLtScratchWide);
-    int32_t v[kScanItems];
+    // All loads of the thread's items first (splits, then the covers they index), stores after: a
+    // store between them would order every later load behind it.
     const uint32_t base = first + threadIdx.x * kScanItems;
+    uint32_t at[kScanItems];
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k)
     {
-        const uint32_t i = base + k;
-        int32_t flag = 0;
-        if (i + 1 < n)
+        at[k] = base + k + 1 < n ? split[base + k] : 0u;  // gap 0 stands in for items past the gaps
+    }
+    uint32_t depth[kScanItems];
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k)
+    {
+        const int32_t c = cover[at[k]];
+        depth[k] = base + k + 1 < n ? uint32_t(c - 1) : 1u;
+    }
+    int32_t v[kScanItems];
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k)
+    {
+        v[k] = (depth[k] & 1u) == 0 ? 1 : 0;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k)
+    {
+        if (base + k + 1 < n)
         {
-            const uint32_t depth = uint32_t(cover[split[i]] - 1);
-            counter[i] = depth;
-            flag = (depth & 1u) == 0 ? 1 : 0;
+            counter[base + k] = depth[k];
         }
-        v[k] = flag;
     }
     const int32_t total = TileExclusiveScan(v, lds_waves);
     const int32_t prefix = ScanLookBack(b.scratch.flags_status(), tile, total, &lds_prefix);
@@ -1228,50 +1244,54 @@ __global__ void __launch_bounds__(kBvhThreads) WideKernel(Set set)
     }
     const uint32_t* left = b.scratch.u32(kLtScratchLeft);
     const uint32_t* right = b.scratch.u32(kLtScratchRight);
-    uint32_t slots[4] = {kLtNone, kLtNone, kLtNone, kLtNone};
-    uint32_t count = 0;
+    const uint32_t* values = b.scratch.values(0);
+    // Every load is issued before the first store, in four dependent rounds: the children, the
+    // grandchildren, the leaf ids / wide indices of the slots, the slot summaries.
     const uint32_t children[2] = {left[i], right[i]};
+    uint32_t grand[4];
 #pragma unroll
     for (int c = 0; c < 2; ++c)
     {
-        const uint32_t child = children[c];
-        if ((child & kLtChildLeafBit) != 0)
+        const bool inner = (children[c] & kLtChildLeafBit) == 0;
+        const uint32_t at = inner ? children[c] : i;  // node i stands in for a leaf child
+        const uint32_t l = left[at];
+        const uint32_t r = right[at];
+        grand[2 * c] = inner ? l : children[c];
+        grand[2 * c + 1] = inner ? r : kLtNone;
+    }
+    // The slots in order: a leaf child takes one, an internal child its two children.
+    const bool inner0 = (children[0] & kLtChildLeafBit) == 0;
+    const bool inner1 = (children[1] & kLtChildLeafBit) == 0;
+    const uint32_t count = 2u + uint32_t(inner0) + uint32_t(inner1);
+    const uint32_t slots[4] = {grand[0], inner0 ? grand[1] : grand[2], inner0 ? grand[2] : grand[3],
+                               inner0 ? grand[3] : kLtNone};
+    // Leaf ids and wide indices, loaded unconditionally (index 0 stands in for the other kind).
+    uint32_t child_words[4] = {kLtNone, kLtNone, kLtNone, kLtNone};
+#pragma unroll
+    for (uint32_t s = 0; s < 4; ++s)
+    {
+        const bool used = s < count;
+        const bool leaf = (slots[s] & kLtChildLeafBit) != 0;
+        const uint32_t leaf_id = values[used && leaf ? slots[s] & ~kLtChildLeafBit : 0u];
+        const uint32_t wide_id = wide[used && !leaf ? slots[s] : 0u];
+        if (used)
         {
-            slots[count++] = child;
-        }
-        else
-        {
-            slots[count++] = left[child];
-            slots[count++] = right[child];
+            child_words[s] = leaf ? kLtChildLeafBit | leaf_id : wide_id;
         }
     }
-    LightBvhNode* node = b.nodes + w;
-    uint32_t child_words[4] = {kLtNone, kLtNone, kLtNone, kLtNone};
     float faces[24];
     float bounds[24];
 #pragma unroll
     for (uint32_t s = 0; s < 4; ++s)
     {
-        Summary summary = EmptySummary();
-        if (s < count)
+        const bool used = s < count;
+        const bool leaf = (slots[s] & kLtChildLeafBit) != 0;
+        const float* p = used && leaf ? b.leaves[child_words[s] & ~kLtChildLeafBit].summary.face
+                                      : b.scratch.aggregates() + (used ? uint64_t(slots[s]) * 12 : 0u);
+        Summary summary = LoadSummary(p);
+        if (!used)
         {
-            const uint32_t slot = slots[s];
-            summary = ChildSummary(b, slot, false);
-            if ((slot & kLtChildLeafBit) != 0)
-            {
-                const uint32_t leaf = b.scratch.values(0)[slot & ~kLtChildLeafBit];
-                child_words[s] = kLtChildLeafBit | leaf;
-                b.leaves[leaf].parent = (w << kLtParentSlotBits) | s;
-            }
-            else
-            {
-                const uint32_t child_node = wide[slot];
-                child_words[s] = child_node;
-                if (child_node < b.node_capacity)
-                {
-                    *reinterpret_cast<uint2*>(&b.nodes[child_node].parent) = make_uint2(w, s);
-                }
-            }
+            summary = EmptySummary();
         }
 #pragma unroll
         for (int f = 0; f < 6; ++f)
@@ -1283,6 +1303,22 @@ __global__ void __launch_bounds__(kBvhThreads) WideKernel(Set set)
         {
             bounds[3 * s + k] = summary.lo[k];
             bounds[12 + 3 * s + k] = summary.hi[k];
+        }
+    }
+    LightBvhNode* node = b.nodes + w;
+#pragma unroll
+    for (uint32_t s = 0; s < 4; ++s)
+    {
+        if (s < count)
+        {
+            if ((slots[s] & kLtChildLeafBit) != 0)
+            {
+                b.leaves[child_words[s] & ~kLtChildLeafBit].parent = (w << kLtParentSlotBits) | s;
+            }
+            else if (child_words[s] < b.node_capacity)
+            {
+                *reinterpret_cast<uint2*>(&b.nodes[child_words[s]].parent) = make_uint2(w, s);
+            }
         }
     }
     float4* face_out = reinterpret_cast<float4*>(node->child_face[0]);
