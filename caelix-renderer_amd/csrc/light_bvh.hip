@@ -47,6 +47,7 @@ enum : uint32_t
     kCtlLeafCount = 1,
     kCtlCoverTiles = 2,
     kCtlFlagsTiles = 3,
+    kCtlWideNodes = 4,  // 4-wide nodes of the build (set by the flags scan)
     kCtlWords = 64,
 };
 
@@ -95,11 +96,13 @@ struct Scratch
     __host__ __device__ float* aggregates() const { return reinterpret_cast<float*>(base + uint64_t(kLtScratchAggregates) * capacity); }
     // Free arrays reused by this build: the leaf centres live in the aggregates array until the
     // aggregation; the gap-cover diffs in `wide` until the flags scan; after the sort, sort buffer 1
-    // holds the cover counts (keys) and every node's split gap (values).
+    // holds the cover counts (keys) and every node's split gap (values), and the flags scan replaces
+    // the consumed split gaps by the binary node of every 4-wide node (wide_source).
     __host__ __device__ float* centres() const { return aggregates(); }
     __host__ __device__ int32_t* cover_diff() const { return reinterpret_cast<int32_t*>(u32(kLtScratchWide)); }
     __host__ __device__ int32_t* cover() const { return reinterpret_cast<int32_t*>(keys(1)); }
     __host__ __device__ uint32_t* split() const { return values(1); }
+    __host__ __device__ uint32_t* wide_source() const { return values(1); }
     __host__ __device__ uint8_t* state_base() const { return base + uint64_t(kLtScratchSortState) * capacity; }
     __host__ __device__ uint32_t* ctl() const { return reinterpret_cast<uint32_t*>(state_base()); }
     __host__ __device__ float* bounds_partials() const { return reinterpret_cast<float*>(state_base() + kCtlWords * 4); }
@@ -571,55 +574,70 @@ CX_DEV void StoreAggregate(const Build& b, uint32_t node, const Summary& s)
     }
 }
 
-// One climb step of a finished element with leaf range [a, last] (a leaf when a == last) inside the
-// block [block_lo, block_hi). Returns true when the element's parent is built (then [a, last] is the
-// parent's range and the caller climbs on); false when the element arrived first at its gap, is the
-// root, or has its parent gap on the block boundary (then it is appended to the block's tops).
-template <bool kLds>
-CX_DEV bool ClimbStep(const Build& b, int n, uint32_t& a, uint32_t& last, uint64_t block_lo, uint64_t block_hi,
-                      uint2* tops, uint32_t* lds_top_count, uint32_t* lds_slot, int32_t* lds_diff)
+// An element climbing at level L > 1: leaf range [a, last] (a leaf when a == last), the deltas of
+// its two boundary gaps (d_left = delta(a - 1), d_right = delta(last)) and its summary, all carried
+// by the lane, so a climb step only fetches what belongs to the sibling.
+struct Climb
+{
+    uint32_t a;
+    uint32_t last;
+    int d_left;
+    int d_right;
+    Summary own;
+};
+
+// The element id of [a, last] given its boundary deltas: a leaf, else the internal node numbered
+// by its parent gap (a left child carries the index of its right end).
+CX_DEV uint32_t ElementOf(uint32_t a, uint32_t last, int d_left, int d_right)
+{
+    if (a == last)
+    {
+        return kLtChildLeafBit | a;
+    }
+    return d_right > d_left ? last : a;
+}
+
+// One climb step of a finished element inside the block [block_lo, block_hi). Returns true when the
+// element's parent is built (the lane then holds the parent and climbs on); false when the element
+// arrived first at its gap, is the root, or has its parent gap on the block boundary (then it is
+// appended to the block's tops). A higher level uses the global gap slots, where an earlier level
+// may have parked a value.
+CX_DEV bool ClimbStep(const Build& b, int n, Climb& e, uint64_t block_lo, uint64_t block_hi, uint2* tops,
+                      uint32_t* lds_top_count)
 {
     const uint32_t* keys = b.scratch.keys(0);
-    if (a == 0 && last == uint32_t(n - 1))
+    if (e.a == 0 && e.last == uint32_t(n - 1))
     {
         return false;  // the root (node 0) is finished
     }
-    const int d_left = GapDelta(keys, int(a) - 1, n);
-    const int d_right = GapDelta(keys, int(last), n);
-    const bool is_left_child = d_right > d_left;
-    const uint32_t gap = is_left_child ? last : a - 1;
+    const bool is_left_child = e.d_right > e.d_left;
+    const uint32_t gap = is_left_child ? e.last : e.a - 1;
     if (uint64_t(gap) < block_lo || uint64_t(gap) + 1 >= block_hi)
     {
         const uint32_t k = atomicAdd(lds_top_count, 1u);
         if (k < kTopsCap)
         {
-            tops[k] = make_uint2(a, last);
+            tops[k] = make_uint2(e.a, e.last);
         }
         return false;
     }
-    const uint32_t far_end = is_left_child ? a : last;
-    // Level 1 keeps the gap slots of its block in LDS (flushed to the global slots at the end of the
-    // launch); a higher level uses the global slots, where an earlier level may have parked a value.
-    uint32_t other;
-    if constexpr (kLds)
-    {
-        other = __hip_atomic_exchange(&lds_slot[gap - uint32_t(block_lo)], far_end, __ATOMIC_ACQ_REL,
-                                      __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-    else
-    {
-        other = __hip_atomic_exchange(b.scratch.u32(kLtScratchCounter) + gap, far_end, __ATOMIC_ACQ_REL,
-                                      __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
+    const uint32_t far_end = is_left_child ? e.a : e.last;
+    const uint32_t other = __hip_atomic_exchange(b.scratch.u32(kLtScratchCounter) + gap, far_end, __ATOMIC_ACQ_REL,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
     if (other == kLtNone)
     {
         return false;  // the sibling finishes the parent
     }
-    // The parent: range [pa, pb], split `gap`, node index by its own parent gap.
-    const uint32_t pa = is_left_child ? a : other;
-    const uint32_t pb = is_left_child ? other : last;
+    // The parent: range [pa, pb], split `gap`; the sibling's summary and its far boundary delta are
+    // the only loads.
+    const uint32_t pa = is_left_child ? e.a : other;
+    const uint32_t pb = is_left_child ? other : e.last;
     const uint32_t left = pa == gap ? (kLtChildLeafBit | gap) : gap;
     const uint32_t right = pb == gap + 1 ? (kLtChildLeafBit | (gap + 1)) : gap + 1;
+    const int d_far = is_left_child ? GapDelta(keys, int(pb), n) : GapDelta(keys, int(pa) - 1, n);
+    const Summary sibling = ElementSummary(b, is_left_child ? right : left);
+    const int d_left = is_left_child ? e.d_left : d_far;
+    const int d_right = is_left_child ? d_far : e.d_right;
     uint32_t* parent_out = b.scratch.u32(kLtScratchParent);
     uint32_t* leaf_parent_out = b.scratch.u32(kLtScratchSortedLeafParent);
     uint32_t node;
@@ -630,8 +648,7 @@ CX_DEV bool ClimbStep(const Build& b, int n, uint32_t& a, uint32_t& last, uint64
     }
     else
     {
-        const bool parent_is_left = GapDelta(keys, int(pb), n) > GapDelta(keys, int(pa) - 1, n);
-        node = parent_is_left ? pb : pa;
+        node = d_right > d_left ? pb : pa;
     }
     b.scratch.u32(kLtScratchLeft)[node] = left;
     b.scratch.u32(kLtScratchRight)[node] = right;
@@ -639,20 +656,15 @@ CX_DEV bool ClimbStep(const Build& b, int n, uint32_t& a, uint32_t& last, uint64
     ((right & kLtChildLeafBit) != 0 ? leaf_parent_out : parent_out)[right & ~kLtChildLeafBit] = node;
     b.scratch.split()[node] = gap;
     // Range ends lie in this workgroup's block: no other workgroup of this launch touches them.
-    if constexpr (kLds)
-    {
-        atomicAdd(&lds_diff[pa - uint32_t(block_lo)], 1);
-        atomicAdd(&lds_diff[pb - uint32_t(block_lo)], -1);
-    }
-    else
-    {
-        int32_t* diff = b.scratch.cover_diff();
-        __hip_atomic_fetch_add(&diff[pa], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(&diff[pb], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-    StoreAggregate(b, node, Union(ElementSummary(b, left), ElementSummary(b, right)));
-    a = pa;
-    last = pb;
+    int32_t* diff = b.scratch.cover_diff();
+    __hip_atomic_fetch_add(&diff[pa], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&diff[pb], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    e.own = is_left_child ? Union(e.own, sibling) : Union(sibling, e.own);
+    StoreAggregate(b, node, e.own);
+    e.a = pa;
+    e.last = pb;
+    e.d_left = d_left;
+    e.d_right = d_right;
     return true;
 }
 
@@ -938,8 +950,7 @@ __global__ void __launch_bounds__(1024) TreeLevelKernel(Set set, uint32_t level)
             const uint2* child_tops = b.scratch.tops(level - 1);
             const uint32_t total = lds_offsets[kTreeFanout];
             uint32_t next = t;
-            uint32_t a = 0;
-            uint32_t last = 0;
+            Climb e;
             bool live = false;
             while (true)
             {
@@ -954,12 +965,16 @@ __global__ void __launch_bounds__(1024) TreeLevelKernel(Set set, uint32_t level)
                     {
                         ++c;
                     }
-                    const uint2 e = child_tops[uint64_t(child_first + c) * kTopsCap + (next - lds_offsets[c])];
-                    a = e.x;
-                    last = e.y;
+                    const uint2 top = child_tops[uint64_t(child_first + c) * kTopsCap + (next - lds_offsets[c])];
+                    const uint32_t* keys = b.scratch.keys(0);
+                    e.a = top.x;
+                    e.last = top.y;
+                    e.d_left = GapDelta(keys, int(e.a) - 1, n);
+                    e.d_right = GapDelta(keys, int(e.last), n);
+                    e.own = ElementSummary(b, ElementOf(e.a, e.last, e.d_left, e.d_right));
                     next += blockDim.x;
                 }
-                live = ClimbStep<false>(b, n, a, last, block_lo, block_hi, tops, &lds_top_count, nullptr, nullptr);
+                live = ClimbStep(b, n, e, block_lo, block_hi, tops, &lds_top_count);
             }
         }
     }
@@ -1149,12 +1164,19 @@ __global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Set set)
     }
     const int32_t total = TileExclusiveScan(v, lds_waves);
     const int32_t prefix = ScanLookBack(b.scratch.flags_status(), tile, total, &lds_prefix);
+    // The binary node of each 4-wide node. Slot w <= item index, and every earlier tile has read its
+    // split gaps before publishing, so these stores only replace consumed split gaps.
+    uint32_t* source = b.scratch.wide_source();
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k)
     {
         if (base + k < capacity)
         {
             wide[base + k] = uint32_t(prefix + v[k]);
+        }
+        if (base + k + 1 < n && (depth[k] & 1u) == 0)
+        {
+            source[prefix + v[k]] = base + k;
         }
     }
     if (tile + 1 != b.scratch.scan_tiles() || threadIdx.x != 0)
@@ -1164,6 +1186,7 @@ __global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Set set)
     // Finish: the header. n = 1: one node with the leaf in slot 0. n >= 2: the binary root's summary;
     // more 4-wide nodes than the capacity leave the tree empty with the overflow flag.
     const uint32_t node_total = uint32_t(prefix + total);
+    ctl[kCtlWideNodes] = n >= 2 ? node_total : 0u;
     Summary root = EmptySummary();
     uint32_t root_index = kLtNone;
     uint32_t node_count = 0;
@@ -1220,28 +1243,24 @@ __global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Set set)
 }
 
 // ------------------------------------------------------------------------------------------------
-// wide (caelix_light_bvh_wide.hlsl:10-100)
+// wide (caelix_light_bvh_wide.hlsl:10-100), one lane per 4-wide node: the reference's lanes are the
+// binary nodes, of which the odd-depth half returns at once; here the flags scan lists the
+// even-depth ones, so every lane builds a node.
 template <class Set>
 __global__ void __launch_bounds__(kBvhThreads) WideKernel(Set set)
 {
     const Build& b = set.Get();
-    const uint32_t i = blockIdx.x * kBvhThreads + threadIdx.x;
-    if (i + 1 >= b.leaf_capacity || !Building(b))
+    const uint32_t w = blockIdx.x * kBvhThreads + threadIdx.x;
+    if (w + 1 >= b.leaf_capacity || !Building(b))
     {
         return;
     }
-    const uint32_t n = b.scratch.ctl()[kCtlLeafCount];
-    const uint32_t* counter = b.scratch.u32(kLtScratchCounter);
-    if (i + 1 >= n || (counter[i] & 1u) != 0)
+    if (w >= b.scratch.ctl()[kCtlWideNodes] || w >= b.node_capacity)
     {
         return;
     }
+    const uint32_t i = b.scratch.wide_source()[w];  // the binary node (an even-depth one)
     const uint32_t* wide = b.scratch.u32(kLtScratchWide);
-    const uint32_t w = wide[i];
-    if (w >= b.node_capacity)
-    {
-        return;
-    }
     const uint32_t* left = b.scratch.u32(kLtScratchLeft);
     const uint32_t* right = b.scratch.u32(kLtScratchRight);
     const uint32_t* values = b.scratch.values(0);
