@@ -31,7 +31,7 @@ namespace {
 
 constexpr uint32_t kBvhThreads = 256;
 constexpr uint32_t kBoundsMaxBlocks = 1024;
-constexpr uint32_t kScanThreads = 256;
+constexpr uint32_t kScanThreads = 1024;
 constexpr uint32_t kScanItems = 16;
 constexpr uint32_t kScanTile = kScanThreads * kScanItems;
 
@@ -994,51 +994,90 @@ __global__ void __launch_bounds__(1024) TreeLevelKernel(Set set, uint32_t level)
 // Returns the exclusive prefix of the tile (sum of all earlier tiles), publishing `tile_sum`.
 CX_DEV int32_t ScanLookBack(uint64_t* status, uint32_t tile, int32_t tile_sum, int32_t* lds_prefix)
 {
-    if (threadIdx.x == 0)
+    if (threadIdx.x < kWave)
     {
+        const uint32_t lane = threadIdx.x;
         int32_t prefix = 0;
         if (tile == 0)
         {
-            StoreRelaxed(&status[0], kScanPrefix | uint32_t(tile_sum));
+            if (lane == 0)
+            {
+                StoreRelaxed(&status[0], kScanPrefix | uint32_t(tile_sum));
+            }
         }
         else
         {
-            StoreRelaxed(&status[tile], kScanAggregate | uint32_t(tile_sum));
+            if (lane == 0)
+            {
+                StoreRelaxed(&status[tile], kScanAggregate | uint32_t(tile_sum));
+            }
+            // The wave reads 64 predecessors per round trip: lane j reads tile p - j. Aggregates up
+            // to the first prefix are summed; a not yet published tile ends the round early.
             int64_t p = int64_t(tile) - 1;
             while (true)
             {
-                const uint64_t word = LoadRelaxed(&status[p]);
+                const int64_t q = p - int64_t(lane);
+                const uint64_t word = q >= 0 ? LoadRelaxed(&status[q]) : kScanPrefix;
                 const uint64_t flag = word & kScanFlagMask;
-                if (flag == 0)
-                {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                prefix += int32_t(uint32_t(word));
-                if (flag == kScanPrefix)
+                const uint64_t waiting = Ballot(flag == 0);
+                const uint64_t done = Ballot(flag == kScanPrefix);
+                const uint64_t stop = waiting | done;
+                const uint32_t first = stop != 0 ? uint32_t(__ffsll(static_cast<unsigned long long>(stop)) - 1) : kWave;
+                const bool finished = first < kWave && ((done >> first) & 1u) != 0;
+                const bool take = lane < first || (finished && lane == first);
+                prefix += WaveSum(take ? int32_t(uint32_t(word)) : 0);
+                if (finished)
                 {
                     break;
                 }
-                --p;
+                p -= first;
+                if (first < kWave)
+                {
+                    __builtin_amdgcn_s_sleep(1);
+                }
             }
-            StoreRelaxed(&status[tile], kScanPrefix | uint32_t(prefix + tile_sum));
+            if (lane == 0)
+            {
+                StoreRelaxed(&status[tile], kScanPrefix | uint32_t(prefix + tile_sum));
+            }
         }
-        *lds_prefix = prefix;
+        if (lane == 0)
+        {
+            *lds_prefix = prefix;
+        }
     }
     __syncthreads();
     return *lds_prefix;
 }
 
-// Thread-blocked block scan: thread t holds items [t * 16, t * 16 + 16) of the tile. Converts `v`
-// into exclusive prefixes within the tile; returns the tile total.
-CX_DEV int32_t TileExclusiveScan(int32_t (&v)[kScanItems], int32_t* lds_waves)
+// Tile scan. Item k of thread t is tile element k * kScanThreads + t (striped, so every load and
+// store of a wave is one contiguous run); the scan transposes through LDS to thread-blocked order
+// (thread t: elements [t * 16, t * 16 + 16)), scans there and transposes back. One pad word per 16
+// keeps both orders free of bank conflicts. Converts `v` into exclusive prefixes within the tile and
+// returns the tile total.
+constexpr uint32_t kScanLdsWords = kScanTile + kScanTile / 16;
+
+CX_DEV uint32_t ScanPad(uint32_t j)
 {
+    return j + (j >> 4);
+}
+
+CX_DEV int32_t TileExclusiveScan(int32_t (&v)[kScanItems], int32_t* lds_items, int32_t* lds_waves)
+{
+    const uint32_t t = threadIdx.x;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k)
+    {
+        lds_items[ScanPad(k * kScanThreads + t)] = v[k];
+    }
+    __syncthreads();
+    int32_t w[kScanItems];
     int32_t sum = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k)
     {
-        const int32_t x = v[k];
-        v[k] = sum;
+        const int32_t x = lds_items[ScanPad(t * kScanItems + k)];
+        w[k] = sum;
         sum += x;
     }
     int32_t total;
@@ -1046,7 +1085,13 @@ CX_DEV int32_t TileExclusiveScan(int32_t (&v)[kScanItems], int32_t* lds_waves)
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k)
     {
-        v[k] += before;
+        lds_items[ScanPad(t * kScanItems + k)] = w[k] + before;  // this thread's own slots
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k)
+    {
+        v[k] = lds_items[ScanPad(k * kScanThreads + t)];
     }
     return total;
 }
@@ -1063,6 +1108,7 @@ __global__ void __launch_bounds__(kScanThreads) CoverScanKernel(Set set)
     __shared__ uint32_t lds_tile;
     __shared__ int32_t lds_waves[kScanThreads / kWave];
     __shared__ int32_t lds_prefix;
+    __shared__ int32_t lds_items[kScanLdsWords];
     uint32_t* ctl = b.scratch.ctl();
     if (threadIdx.x == 0)
     {
@@ -1080,11 +1126,12 @@ __global__ void __launch_bounds__(kScanThreads) CoverScanKernel(Set set)
     const int32_t* diff = b.scratch.cover_diff();
     int32_t* cover = b.scratch.cover();
     int32_t v[kScanItems];
-    const uint32_t base = first + threadIdx.x * kScanItems;
+    const uint32_t base = first + threadIdx.x;  // item k: base + k * kScanThreads
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k)
     {
-        v[k] = base + k < gaps ? diff[base + k] : 0;
+        const uint32_t g = base + k * kScanThreads;
+        v[k] = g < gaps ? diff[g] : 0;
     }
     int32_t own[kScanItems];
 #pragma unroll
@@ -1092,14 +1139,15 @@ __global__ void __launch_bounds__(kScanThreads) CoverScanKernel(Set set)
     {
         own[k] = v[k];
     }
-    const int32_t total = TileExclusiveScan(v, lds_waves);
+    const int32_t total = TileExclusiveScan(v, lds_items, lds_waves);
     const int32_t prefix = ScanLookBack(b.scratch.cover_status(), tile, total, &lds_prefix);
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k)
     {
-        if (base + k < gaps)
+        const uint32_t g = base + k * kScanThreads;
+        if (g < gaps)
         {
-            cover[base + k] = prefix + v[k] + own[k];
+            cover[g] = prefix + v[k] + own[k];
         }
     }
 }
@@ -1118,6 +1166,7 @@ __global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Set set)
     __shared__ uint32_t lds_tile;
     __shared__ int32_t lds_waves[kScanThreads / kWave];
     __shared__ int32_t lds_prefix;
+    __shared__ int32_t lds_items[kScanLdsWords];
     uint32_t* ctl = b.scratch.ctl();
     if (threadIdx.x == 0)
     {
@@ -1134,19 +1183,20 @@ __global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Set set)
     uint32_t* wide = b.scratch.u32(kLtScratchWide);
     // All loads of the thread's items first (splits, then the covers they index), stores after: a
     // store between them would order every later load behind it.
-    const uint32_t base = first + threadIdx.x * kScanItems;
+    const uint32_t base = first + threadIdx.x;  // item k: base + k * kScanThreads
     uint32_t at[kScanItems];
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k)
     {
-        at[k] = base + k + 1 < n ? split[base + k] : 0u;  // gap 0 stands in for items past the gaps
+        const uint32_t i = base + k * kScanThreads;
+        at[k] = i + 1 < n ? split[i] : 0u;  // gap 0 stands in for items past the gaps
     }
     uint32_t depth[kScanItems];
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k)
     {
         const int32_t c = cover[at[k]];
-        depth[k] = base + k + 1 < n ? uint32_t(c - 1) : 1u;
+        depth[k] = base + k * kScanThreads + 1 < n ? uint32_t(c - 1) : 1u;
     }
     int32_t v[kScanItems];
 #pragma unroll
@@ -1157,12 +1207,13 @@ __global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Set set)
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k)
     {
-        if (base + k + 1 < n)
+        const uint32_t i = base + k * kScanThreads;
+        if (i + 1 < n)
         {
-            counter[base + k] = depth[k];
+            counter[i] = depth[k];
         }
     }
-    const int32_t total = TileExclusiveScan(v, lds_waves);
+    const int32_t total = TileExclusiveScan(v, lds_items, lds_waves);
     const int32_t prefix = ScanLookBack(b.scratch.flags_status(), tile, total, &lds_prefix);
     // The binary node of each 4-wide node. Slot w <= item index, and every earlier tile has read its
     // split gaps before publishing, so these stores only replace consumed split gaps.
@@ -1170,13 +1221,14 @@ __global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Set set)
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k)
     {
-        if (base + k < capacity)
+        const uint32_t i = base + k * kScanThreads;
+        if (i < capacity)
         {
-            wide[base + k] = uint32_t(prefix + v[k]);
+            wide[i] = uint32_t(prefix + v[k]);
         }
-        if (base + k + 1 < n && (depth[k] & 1u) == 0)
+        if (i + 1 < n && (depth[k] & 1u) == 0)
         {
-            source[prefix + v[k]] = base + k;
+            source[prefix + v[k]] = i;
         }
     }
     if (tile + 1 != b.scratch.scan_tiles() || threadIdx.x != 0)

@@ -15,6 +15,7 @@ constexpr int kSortWaves = kSortThreads / kWave;
 constexpr int kSortRadix = 256;
 constexpr int kSortPasses = 4;
 constexpr int kSortSlices = 8;  // histogram copies, one per XCD (blockIdx % 8)
+constexpr int kSortLookBack = 4;  // predecessor tiles read per look-back round trip
 
 // Sort state bytes: control words, 8 x 4 x 256 histogram slices, 4 x tiles x 256 tile status.
 constexpr uint64_t kSortCtlBytes = 256;
@@ -199,25 +200,54 @@ CX_DEV void SortPassTile(const uint32_t* __restrict__ keys_in, const uint32_t* _
     uint32_t unused;
     const uint32_t digit_base = BlockExclusiveSum<kSortThreads>(total_d, lds_scan, unused);
     const uint32_t digit_start = BlockExclusiveSum<kSortThreads>(count, lds_scan, unused);
+    // Look-back, kSortLookBack predecessors per round trip: aggregates are summed up to the first
+    // prefix; a tile not yet published ends the round there.
     uint32_t prefix = 0;
     if (tile > 0)
     {
         int64_t p = int64_t(tile) - 1;
         while (true)
         {
-            const uint32_t word = LoadRelaxed(&status[uint64_t(p) * kSortRadix + t]);
-            const uint32_t flag = word & kFlagMask;
-            if (flag == 0)
+            uint32_t word[kSortLookBack];
+#pragma unroll
+            for (int j = 0; j < kSortLookBack; ++j)
             {
-                __builtin_amdgcn_s_sleep(1);
-                continue;
+                word[j] = p - j >= 0 ? LoadRelaxed(&status[uint64_t(p - j) * kSortRadix + t]) : kFlagPrefix;
             }
-            prefix += word & kCountMask;
-            if (flag == kFlagPrefix)
+            int used = kSortLookBack;
+            bool finished = false;
+#pragma unroll
+            for (int j = kSortLookBack - 1; j >= 0; --j)
+            {
+                const uint32_t flag = word[j] & kFlagMask;
+                if (flag == 0)
+                {
+                    used = j;
+                    finished = false;
+                }
+                else if (flag == kFlagPrefix)
+                {
+                    used = j;
+                    finished = true;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kSortLookBack; ++j)
+            {
+                if (j < used || (finished && j == used))
+                {
+                    prefix += word[j] & kCountMask;
+                }
+            }
+            if (finished)
             {
                 break;
             }
-            --p;
+            p -= used;
+            if (used < kSortLookBack)
+            {
+                __builtin_amdgcn_s_sleep(1);
+            }
         }
         StoreRelaxed(&status[uint64_t(tile) * kSortRadix + t], kFlagPrefix | (prefix + count));
     }
