@@ -313,7 +313,8 @@ CX_DEV Summary ChildSummary(const Build& b, uint32_t child, bool coherent)
 // centres (the partials) and the centres themselves. Workgroup 0 also resets the build's counters
 // and the sort histograms for the kernels that follow. The Morton kernel reduces the partials, so no
 // workgroup of this kernel waits on another.
-CX_DEV void MinMax3(float (&lo)[3], float (&hi)[3], float* lds6x4, uint32_t wave)
+template <int kWaves>
+CX_DEV void MinMax3(float (&lo)[3], float (&hi)[3], float* lds, uint32_t wave)  // lds: 6 x kWaves floats
 {
 #pragma unroll
     for (int offset = kWave / 2; offset > 0; offset >>= 1)
@@ -330,17 +331,24 @@ CX_DEV void MinMax3(float (&lo)[3], float (&hi)[3], float* lds6x4, uint32_t wave
 #pragma unroll
         for (int k = 0; k < 3; ++k)
         {
-            lds6x4[k * 4 + wave] = lo[k];
-            lds6x4[(3 + k) * 4 + wave] = hi[k];
+            lds[k * kWaves + wave] = lo[k];
+            lds[(3 + k) * kWaves + wave] = hi[k];
         }
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 3; ++k)
     {
-        lo[k] = fminf(fminf(lds6x4[k * 4], lds6x4[k * 4 + 1]), fminf(lds6x4[k * 4 + 2], lds6x4[k * 4 + 3]));
-        hi[k] = fmaxf(fmaxf(lds6x4[(3 + k) * 4], lds6x4[(3 + k) * 4 + 1]),
-                      fmaxf(lds6x4[(3 + k) * 4 + 2], lds6x4[(3 + k) * 4 + 3]));
+        float l = lds[k * kWaves];
+        float h = lds[(3 + k) * kWaves];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w)
+        {
+            l = fminf(l, lds[k * kWaves + w]);
+            h = fmaxf(h, lds[(3 + k) * kWaves + w]);
+        }
+        lo[k] = l;
+        hi[k] = h;
     }
 }
 
@@ -353,7 +361,7 @@ __global__ void __launch_bounds__(kBvhThreads) BoundsKernel(Set set)
     {
         return;
     }
-    __shared__ float lds[6 * 4];
+    __shared__ float lds[6 * (kBvhThreads / kWave)];
     const uint32_t t = threadIdx.x;
     const uint32_t n = min(b.state->leaves_found, b.leaf_capacity);
     float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f};
@@ -378,7 +386,7 @@ __global__ void __launch_bounds__(kBvhThreads) BoundsKernel(Set set)
         hi[1] = fmaxf(hi[1], c1);
         hi[2] = fmaxf(hi[2], c2);
     }
-    MinMax3(lo, hi, lds, t / kWave);
+    MinMax3<kBvhThreads / kWave>(lo, hi, lds, t / kWave);
     float* partials = b.scratch.bounds_partials();
     if (t < 6)
     {
@@ -418,7 +426,7 @@ __global__ void __launch_bounds__(kSortThreads) MortonKernel(Set set)
         return;
     }
     __shared__ uint32_t lds_hist[kSortPasses * kSortRadix];
-    __shared__ float lds_minmax[6 * 4];
+    __shared__ float lds_minmax[6 * (kSortThreads / kWave)];
     const uint32_t t = threadIdx.x;
     for (uint32_t i = t; i < kSortPasses * kSortRadix; i += kSortThreads)
     {
@@ -449,7 +457,7 @@ __global__ void __launch_bounds__(kSortThreads) MortonKernel(Set set)
             hi[k] = fmaxf(hi[k], partials[g * 8 + 3 + k]);
         }
     }
-    MinMax3(lo, hi, lds_minmax, t / kWave);
+    MinMax3<kSortThreads / kWave>(lo, hi, lds_minmax, t / kWave);
     if (blockIdx.x == 0 && t == 0)
     {
         b.header->leaf_count = n;
@@ -1410,7 +1418,7 @@ __global__ void __launch_bounds__(kBvhThreads) WideKernel(Set set)
 // caelix_light_bvh_begin.hlsl:9-17
 // One onesweep pass of the build's sort (keys / values ping-pong in the scratch).
 template <class Set>
-__global__ void __launch_bounds__(kSortThreads) BuildSortPassKernel(Set set, uint32_t pass)
+__global__ void __launch_bounds__(kSortThreads, kSortWavesPerSimd) BuildSortPassKernel(Set set, uint32_t pass)
 {
     const Build& b = set.Get();
     if (blockIdx.x >= SortTiles(b.leaf_capacity))

@@ -1,6 +1,6 @@
 // Onesweep LSD radix sort of (uint32 key, uint32 value) pairs for gfx950: 8-bit digits, 4 passes.
 // One upfront kernel builds the four digit histograms (folded into the light BVH's Morton kernel),
-// then each pass ranks a 4096-element tile with wave64 ballots, finds the tile's global digit
+// then each pass ranks an 8192-element tile with wave64 ballots, finds the tile's global digit
 // offsets by decoupled look-back over earlier tiles, stages the tile in LDS in digit order and
 // writes it out coalesced. Stable: equal digits keep their input order.
 #pragma once
@@ -8,14 +8,15 @@
 
 namespace cx {
 
-constexpr int kSortThreads = 256;
+constexpr int kSortThreads = 512;
 constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 elements
+constexpr int kSortTile = kSortThreads * kSortItems;  // 8192 elements
 constexpr int kSortWaves = kSortThreads / kWave;
 constexpr int kSortRadix = 256;
 constexpr int kSortPasses = 4;
 constexpr int kSortSlices = 8;  // histogram copies, one per XCD (blockIdx % 8)
 constexpr int kSortLookBack = 4;  // predecessor tiles read per look-back round trip
+constexpr int kSortWavesPerSimd = 4;  // two workgroups per CU: at most 128 VGPRs per lane
 
 // Sort state bytes: control words, 8 x 4 x 256 histogram slices, 4 x tiles x 256 tile status.
 constexpr uint64_t kSortCtlBytes = 256;
@@ -100,7 +101,7 @@ CX_DEV void SortClearTileStatus(const SortState& s, uint32_t tile)
     }
 }
 
-// One onesweep pass over 4096-element tiles. Element order inside a tile: wave w owns elements
+// One onesweep pass over 8192-element tiles. Element order inside a tile: wave w owns elements
 // [w * 1024, (w + 1) * 1024); item k of lane l is element w * 1024 + k * 64 + l. Ranking walks the
 // items in order, so (wave, item, lane) is the input order and equal digits stay stable.
 // The body of a pass for one workgroup of kSortThreads threads (kernels of one sort, or of one
@@ -182,28 +183,42 @@ CX_DEV void SortPassTile(const uint32_t* __restrict__ keys_in, const uint32_t* _
     }
     __syncthreads();
 
-    // Thread t owns digit t: wave offsets, tile count, look-back.
-    const uint32_t c0 = lds_wave_count[0][t];
-    const uint32_t c1 = lds_wave_count[1][t];
-    const uint32_t c2 = lds_wave_count[2][t];
-    const uint32_t c3 = lds_wave_count[3][t];
-    const uint32_t count = c0 + c1 + c2 + c3;
+    // Thread t < 256 owns digit t: wave offsets (in place: only the owner reads its column), tile
+    // count, look-back.
+    const bool owner = t < uint32_t(kSortRadix);
+    uint32_t count = 0;
+    if (owner)
+    {
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w)
+        {
+            const uint32_t c = lds_wave_count[w][t];
+            lds_wave_count[w][t] = count;
+            count += c;
+        }
+    }
     uint32_t* status = s.status + (uint64_t(pass) * s.tiles) * kSortRadix;
-    StoreRelaxed(&status[uint64_t(tile) * kSortRadix + t], (tile == 0 ? kFlagPrefix : kFlagAggregate) | count);
+    if (owner)
+    {
+        StoreRelaxed(&status[uint64_t(tile) * kSortRadix + t], (tile == 0 ? kFlagPrefix : kFlagAggregate) | count);
+    }
     // Digit bases of this pass: the exclusive scan over digits of the summed histogram slices.
     uint32_t total_d = 0;
-#pragma unroll
-    for (int sl = 0; sl < kSortSlices; ++sl)
+    if (owner)
     {
-        total_d += s.hist[(sl * kSortPasses + pass) * kSortRadix + t];
+#pragma unroll
+        for (int sl = 0; sl < kSortSlices; ++sl)
+        {
+            total_d += s.hist[(sl * kSortPasses + pass) * kSortRadix + t];
+        }
     }
     uint32_t unused;
     const uint32_t digit_base = BlockExclusiveSum<kSortThreads>(total_d, lds_scan, unused);
-    const uint32_t digit_start = BlockExclusiveSum<kSortThreads>(count, lds_scan, unused);
+    const uint32_t digit_start = BlockExclusiveSum<kSortThreads>(owner ? count : 0u, lds_scan, unused);
     // Look-back, kSortLookBack predecessors per round trip: aggregates are summed up to the first
     // prefix; a tile not yet published ends the round there.
     uint32_t prefix = 0;
-    if (tile > 0)
+    if (owner && tile > 0)
     {
         int64_t p = int64_t(tile) - 1;
         while (true)
@@ -251,12 +266,11 @@ CX_DEV void SortPassTile(const uint32_t* __restrict__ keys_in, const uint32_t* _
         }
         StoreRelaxed(&status[uint64_t(tile) * kSortRadix + t], kFlagPrefix | (prefix + count));
     }
-    lds_wave_count[0][t] = 0;
-    lds_wave_count[1][t] = c0;
-    lds_wave_count[2][t] = c0 + c1;
-    lds_wave_count[3][t] = c0 + c1 + c2;
-    lds_digit_start[t] = digit_start;
-    lds_destination[t] = digit_base + prefix;
+    if (owner)
+    {
+        lds_digit_start[t] = digit_start;
+        lds_destination[t] = digit_base + prefix;
+    }
     __syncthreads();
 
     // Stage the tile in LDS in digit order, then write it out: consecutive LDS slots of one digit

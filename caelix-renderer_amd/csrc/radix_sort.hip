@@ -52,7 +52,7 @@ __global__ void __launch_bounds__(kSortThreads) SortHistogramKernel(const uint32
     SortFlushHistogram(lds_hist, s.hist);
 }
 
-__global__ void __launch_bounds__(kSortThreads) SortPassKernel(const uint32_t* __restrict__ keys_in,
+__global__ void __launch_bounds__(kSortThreads, kSortWavesPerSimd) SortPassKernel(const uint32_t* __restrict__ keys_in,
                                                                const uint32_t* __restrict__ values_in,
                                                                uint32_t* __restrict__ keys_out,
                                                                uint32_t* __restrict__ values_out,
