@@ -741,16 +741,16 @@ CX_DEV Summary Level1Summary(const Level1Lds& l, uint32_t element, uint32_t lo)
 }
 
 // ClimbStep for level 1: every arrival is in LDS; the climbing lane carries its element's summary.
-CX_DEV bool ClimbStepLds(Level1Lds& l, int n, uint32_t& a, uint32_t& last, Summary& own, uint32_t lo, uint32_t hi,
-                         uint2* tops, uint32_t* lds_top_count)
+CX_DEV bool ClimbStepLds(Level1Lds& l, int n, Climb& e, uint32_t lo, uint32_t hi, uint2* tops,
+                         uint32_t* lds_top_count)
 {
+    const uint32_t a = e.a;
+    const uint32_t last = e.last;
     if (a == 0 && last == uint32_t(n - 1))
     {
         return false;  // the root (node 0) is finished
     }
-    const int d_left = GapDeltaLds(l, int(a) - 1, int(lo));
-    const int d_right = GapDeltaLds(l, int(last), int(lo));
-    const bool is_left_child = d_right > d_left;
+    const bool is_left_child = e.d_right > e.d_left;
     const uint32_t gap = is_left_child ? last : a - 1;
     if (gap < lo || gap + 1 >= hi || (a == 0 && !is_left_child))
     {
@@ -772,6 +772,11 @@ CX_DEV bool ClimbStepLds(Level1Lds& l, int n, uint32_t& a, uint32_t& last, Summa
     const uint32_t pb = is_left_child ? other : last;
     const uint32_t left = pa == gap ? (kLtChildLeafBit | gap) : gap;
     const uint32_t right = pb == gap + 1 ? (kLtChildLeafBit | (gap + 1)) : gap + 1;
+    // The sibling's far boundary delta and summary are the only reads.
+    const int d_far = is_left_child ? GapDeltaLds(l, int(pb), int(lo)) : GapDeltaLds(l, int(pa) - 1, int(lo));
+    const Summary sibling = Level1Summary(l, is_left_child ? right : left, lo);
+    const int d_left = is_left_child ? e.d_left : d_far;
+    const int d_right = is_left_child ? d_far : e.d_right;
     uint32_t node;
     if (pa == 0 && pb == uint32_t(n - 1))
     {
@@ -780,8 +785,7 @@ CX_DEV bool ClimbStepLds(Level1Lds& l, int n, uint32_t& a, uint32_t& last, Summa
     }
     else
     {
-        const bool parent_is_left = GapDeltaLds(l, int(pb), int(lo)) > GapDeltaLds(l, int(pa) - 1, int(lo));
-        node = parent_is_left ? pb : pa;
+        node = d_right > d_left ? pb : pa;
     }
     l.left[node - lo] = left;
     l.right[node - lo] = right;
@@ -790,11 +794,12 @@ CX_DEV bool ClimbStepLds(Level1Lds& l, int n, uint32_t& a, uint32_t& last, Summa
     ((right & kLtChildLeafBit) != 0 ? l.leaf_parent : l.parent)[(right & ~kLtChildLeafBit) - lo] = node;
     atomicAdd(&l.diff[pa - lo], 1);
     atomicAdd(&l.diff[pb - lo], -1);
-    const Summary sibling = Level1Summary(l, is_left_child ? right : left, lo);
-    own = is_left_child ? Union(own, sibling) : Union(sibling, own);
-    LdsStore(l.agg[node - lo], own);
-    a = pa;
-    last = pb;
+    e.own = is_left_child ? Union(e.own, sibling) : Union(sibling, e.own);
+    LdsStore(l.agg[node - lo], e.own);
+    e.a = pa;
+    e.last = pb;
+    e.d_left = d_left;
+    e.d_right = d_right;
     return true;
 }
 
@@ -842,9 +847,7 @@ CX_DEV void TreeLevel1(const Build& b, int n, uint32_t lo, uint32_t hi, uint2* t
     }
     __syncthreads();
     uint32_t next = t;
-    uint32_t a = 0;
-    uint32_t last = 0;
-    Summary own;
+    Climb e;
     bool live = false;
     while (true)
     {
@@ -854,12 +857,14 @@ CX_DEV void TreeLevel1(const Build& b, int n, uint32_t lo, uint32_t hi, uint2* t
             {
                 break;
             }
-            a = lo + next;
-            last = a;
-            own = LdsSummary(l.leaf[next]);
+            e.a = lo + next;
+            e.last = e.a;
+            e.d_left = l.delta[next];       // gap e.a - 1
+            e.d_right = l.delta[next + 1];  // gap e.a
+            e.own = LdsSummary(l.leaf[next]);
             next += blockDim.x;
         }
-        live = ClimbStepLds(l, n, a, last, own, lo, hi, tops, lds_top_count);
+        live = ClimbStepLds(l, n, e, lo, hi, tops, lds_top_count);
     }
     __syncthreads();
     // Coalesced write-out of the block.
