@@ -48,12 +48,14 @@ STAGE_BYTES = {
     "lt_bounds": 24.0,
     "lt_morton": 32.0,
     "sort_pass": 16.0,
+    "sort64_pass": 16.0,
     "lt_tree_l1": 168.0,
     "lt_cover_scan": 8.0,
     "lt_flags_scan": 8.0,
     "lt_wide": 212.0,
 }
 BUILD_BYTES_PER_LEAF = 520.0
+BUILD_BYTES_PER_LEAF_KEYS64 = 580.0  # 7 key-only passes of 16 B + 8 B wider keys in place of 4 pair passes
 
 TRACE_BOUNCES = 2
 TRACE_SPP = 1
@@ -301,6 +303,22 @@ def lighttree_bench(args, with_cpu: bool) -> dict:
                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(pipeline_gbs / HBM_PEAK_GBS, 4)},
         "stages": per_stage,
     }
+    # The 64-bit key form of the same rebuild (code << 32 | leaf sorted as one key): the same tree.
+    def build64(_):
+        builder.build(stream, keys64=True)
+
+    ms64 = _time_steps(build64, args.steps, args.warmup, stream)
+    out64 = builder.read()
+    assert out64["header"].tobytes() == out["header"].tobytes()
+    stages64 = _profile(build64, profile_steps)
+    gbs64 = BUILD_BYTES_PER_LEAF_KEYS64 * n / (ms64 * 1e-3) / 1e9
+    result["keys64"] = {
+        "value": round(n / (ms64 * 1e-3) / 1e6, 2), "unit": "Mprims/s", "ms_per_step": round(ms64, 4),
+        "pipeline_roofline": {"bytes_per_leaf": BUILD_BYTES_PER_LEAF_KEYS64, "achieved": round(gbs64, 1),
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs64 / HBM_PEAK_GBS, 4)},
+        "stages": {name: {"avg_ms": round(total / launches, 5), "launches_per_step": launches / profile_steps}
+                   for name, (total, launches) in stages64.items() if name in ("lt_morton", "sort64_pass")},
+    }
     if with_cpu:
         result["cpu_baseline"] = lighttree_cpu_baseline(leaves)
     return result
@@ -510,7 +528,7 @@ def main() -> int:
         if not args.no_lighttree:
             lt = lighttree_bench(args, with_cpu=False)
             result["lighttree"] = {k: lt[k] for k in ("value", "unit", "ms_per_step", "roofline",
-                                                      "pipeline_roofline", "config")}
+                                                      "pipeline_roofline", "config", "keys64")}
     elif args.workload == "copy":
         result = copy_bench(args)
     else:
@@ -531,7 +549,7 @@ def main() -> int:
         "config": result["config"],
         "roofline": result["roofline"],
     }
-    for key in ("cpu_baseline", "parity", "lighttree", "adjoint", "batch", "pipeline_roofline", "stages"):
+    for key in ("cpu_baseline", "parity", "lighttree", "adjoint", "batch", "pipeline_roofline", "keys64", "stages"):
         if key in result:
             line[key] = result[key]
     print(json.dumps(line))

@@ -136,8 +136,11 @@ SIGNATURES = {
     "cx_light_bvh_batch_bytes": (_U64, [_U32]),
     "cx_light_bvh_batch_prepare": (_INT, [ctypes.POINTER(CxLightBvhBuild), _U32, _P, _P]),
     "cx_light_bvh_build_batch": (_INT, [_P, _U32, _U32, _P]),
+    "cx_light_bvh_build_keys64": (_INT, [ctypes.POINTER(CxLightBvhBuild), _P]),
     "cx_radix_sort_state_bytes": (_U64, [_U32]),
     "cx_radix_sort_pairs": (_INT, [_P, _P, _P, _P, _P, _P, _U32, _P, _P]),
+    "cx_radix_sort_keys64_state_bytes": (_U64, [_U32]),
+    "cx_radix_sort_keys64": (_INT, [_P, _P, _P, _P, _U32, _P, _P]),
     "cx_hash_fingerprint": (_INT, [_P, _U32, _U32, _P, _P]),
     "cx_hash_lookup": (_INT, [ctypes.POINTER(CxHashTable), _P, _U32, _U32, _U32, _U32, _P, _P, _P]),
     "cx_hash_maintain": (_INT, [ctypes.POINTER(CxHashTable), _U32, _U32, _U32, _U32, _U32, _U32, _P, _P, _P]),
@@ -258,8 +261,13 @@ class LightBvhBuilder:
         host["leaves_found"] = n
         self.state.copy_(to_device(host.view(np.uint8), self.state.device))
 
-    def build(self, stream=None) -> None:
-        check(self.lib.cx_light_bvh_build(ctypes.byref(self._desc), stream_handle(stream)), "cx_light_bvh_build")
+    def build(self, stream=None, keys64: bool = False) -> None:
+        """The build; keys64 sorts code << 32 | leaf as one 64-bit key (the same tree)."""
+        if keys64:
+            check(self.lib.cx_light_bvh_build_keys64(ctypes.byref(self._desc), stream_handle(stream)),
+                  "cx_light_bvh_build_keys64")
+        else:
+            check(self.lib.cx_light_bvh_build(ctypes.byref(self._desc), stream_handle(stream)), "cx_light_bvh_build")
 
     def read(self) -> dict:
         """Copies every output back: header, state, nodes, leaves, scratch (numpy)."""
@@ -305,6 +313,21 @@ def radix_sort_pairs(keys, values, stream=None):
     check(lib.cx_radix_sort_pairs(keys.data_ptr(), values.data_ptr(), keys1.data_ptr(), values1.data_ptr(),
                                   count.data_ptr(), None, capacity, state.data_ptr(), stream_handle(stream)),
           "cx_radix_sort_pairs")
+
+
+def radix_sort_keys64(keys, stream=None):
+    """Sorts a device int64 tensor of uint64 key bit patterns (bit 63 clear) in place, stable, with
+    the 7-pass onesweep kernels."""
+    torch = _torch()
+    lib = load()
+    n = keys.numel()
+    capacity = max(n, 1)
+    keys1 = torch.empty_like(keys)
+    count = torch.tensor([n], dtype=torch.int32, device=keys.device)
+    state = device_bytes(int(lib.cx_radix_sort_keys64_state_bytes(capacity)), keys.device)
+    check(lib.cx_radix_sort_keys64(keys.data_ptr(), keys1.data_ptr(), count.data_ptr(), None, capacity,
+                                   state.data_ptr(), stream_handle(stream)), "cx_radix_sort_keys64")
+    keys.copy_(keys1)
 
 
 def profile_enable(enabled: bool) -> None:

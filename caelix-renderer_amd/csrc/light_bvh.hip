@@ -118,11 +118,22 @@ struct Scratch
         return state_base() + ((offset + 255) & ~uint64_t(255));
     }
     __host__ __device__ SortState sort() const { return SortStateAt(sort_base(), capacity); }
+    // The 64-bit key variant's sort: its state at the same place (the region holds the larger of the
+    // two layouts), its keys (Morton code << 32 | leaf) over sort buffer `buffer` (keys and values
+    // side by side, 8 bytes per leaf).
+    __host__ __device__ SortState sort64() const { return SortStateOf<SortKeys64>(sort_base(), capacity); }
+    __host__ __device__ uint64_t* keys64(uint32_t buffer) const
+    {
+        return reinterpret_cast<uint64_t*>(u32(kLtScratchKeys + buffer * kLtScratchSortBufferStride));
+    }
     // Tops of the tree levels: per level-L workgroup, up to kTopsCap (first, last) leaf ranges of
     // finished elements whose parent gap is on the workgroup's boundary, and their count.
     __host__ __device__ uint8_t* tops_base() const
     {
-        const uint64_t offset = uint64_t(sort_base() - base) + SortStateBytes(capacity);
+        const uint64_t state = SortStateBytesOf<SortPairs32>(capacity) > SortStateBytesOf<SortKeys64>(capacity)
+                                   ? SortStateBytesOf<SortPairs32>(capacity)
+                                   : SortStateBytesOf<SortKeys64>(capacity);
+        const uint64_t offset = uint64_t(sort_base() - base) + state;
         return base + ((offset + 255) & ~uint64_t(255));
     }
     __host__ __device__ uint2* tops(uint32_t level) const
@@ -352,7 +363,7 @@ CX_DEV void MinMax3(float (&lo)[3], float (&hi)[3], float* lds, uint32_t wave)  
     }
 }
 
-template <class Set>
+template <class Set, class Keys>
 __global__ void __launch_bounds__(kBvhThreads) BoundsKernel(Set set)
 {
     const Build& b = set.Get();
@@ -400,12 +411,12 @@ __global__ void __launch_bounds__(kBvhThreads) BoundsKernel(Set set)
             ctl[kCtlCoverTiles] = 0;
             ctl[kCtlFlagsTiles] = 0;
         }
-        const SortState s = b.scratch.sort();
+        const SortState s = SortStateOf<Keys>(b.scratch.sort_base(), b.leaf_capacity);
         for (uint32_t i = t; i < kSortCtlBytes / 4; i += kBvhThreads)
         {
             s.ctl[i] = 0;
         }
-        for (uint32_t i = t; i < kSortHistBytes / 4; i += kBvhThreads)
+        for (uint32_t i = t; i < SortHistBytesOf<Keys>() / 4; i += kBvhThreads)
         {
             s.hist[i] = 0;
         }
@@ -414,9 +425,11 @@ __global__ void __launch_bounds__(kBvhThreads) BoundsKernel(Set set)
 
 // ------------------------------------------------------------------------------------------------
 // morton (caelix_light_bvh_morton.hlsl:5-28) + the sort's digit histograms and status clear.
-// One workgroup per sort tile (4096 leaves). Every workgroup first reduces the bounds partials (the
-// second half of caelix_light_bvh_bounds); workgroup 0 writes leaf_count, leaves_dropped and bounds.
-template <class Set>
+// One workgroup per 8192 leaves. Every workgroup first reduces the bounds partials (the second half
+// of caelix_light_bvh_bounds); workgroup 0 writes leaf_count, leaves_dropped and bounds. The keys:
+// (code, leaf) pairs in sort buffer 0 (SortPairs32), or code << 32 | leaf in sort buffer 1
+// (SortKeys64, whose tiles are half as long: two status rows per workgroup).
+template <class Set, class Keys>
 __global__ void __launch_bounds__(kSortThreads) MortonKernel(Set set)
 {
     const Build& b = set.Get();
@@ -425,15 +438,21 @@ __global__ void __launch_bounds__(kSortThreads) MortonKernel(Set set)
     {
         return;
     }
-    __shared__ uint32_t lds_hist[kSortPasses * kSortRadix];
+    constexpr uint32_t kHistWords = Keys::kPasses * Keys::kRadix;
+    constexpr uint32_t kStatusRows = kSortTile / (Keys::kThreads * Keys::kItems);  // key tiles per workgroup
+    __shared__ uint32_t lds_hist[kHistWords];
     __shared__ float lds_minmax[6 * (kSortThreads / kWave)];
     const uint32_t t = threadIdx.x;
-    for (uint32_t i = t; i < kSortPasses * kSortRadix; i += kSortThreads)
+    for (uint32_t i = t; i < kHistWords; i += kSortThreads)
     {
         lds_hist[i] = 0;
     }
-    const SortState s = b.scratch.sort();
-    SortClearTileStatus(s, blockIdx.x);
+    const SortState s = SortStateOf<Keys>(b.scratch.sort_base(), b.leaf_capacity);
+#pragma unroll
+    for (uint32_t r = 0; r < kStatusRows; ++r)
+    {
+        SortClearTileStatusOf<Keys>(s, blockIdx.x * kStatusRows + r);
+    }
     if (t == 0)
     {
         const uint32_t tile = blockIdx.x;
@@ -502,13 +521,22 @@ __global__ void __launch_bounds__(kSortThreads) MortonKernel(Set set)
             const uint32_t y = uint32_t(fminf(fmaxf(__fdiv_rn(c1 - lo1, e1) * 1024.0f, 0.0f), 1023.0f));
             const uint32_t z = uint32_t(fminf(fmaxf(__fdiv_rn(c2 - lo2, e2) * 1024.0f, 0.0f), 1023.0f));
             const uint32_t key = Morton3(x, y, z);
-            keys[i] = key;
-            values[i] = i;
-            SortCountKey(lds_hist, key);
+            if constexpr (Keys::kValues)
+            {
+                keys[i] = key;
+                values[i] = i;
+                SortCountKeyOf<Keys>(lds_hist, key);
+            }
+            else
+            {
+                const uint64_t wide_key = (uint64_t(key) << 32) | i;
+                b.scratch.keys64(1)[i] = wide_key;
+                SortCountKeyOf<Keys>(lds_hist, wide_key);
+            }
         }
     }
     __syncthreads();
-    SortFlushHistogram(lds_hist, s.hist);
+    SortFlushHistogramOf<Keys>(lds_hist, s.hist);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1435,25 +1463,59 @@ __global__ void __launch_bounds__(kSortThreads, kSortWavesPerSimd) BuildSortPass
                  &b.header->leaf_count, &b.state->building, b.scratch.sort(), pass);
 }
 
-// The build sequence (light_tree_record.cpp:271-308) for a set of builds; `capacity` = the largest
-// leaf capacity of the set, `count` = the number of builds (grid y).
+// One pass of the 64-bit key variant's sort: the keys (code << 32 | leaf) start in sort buffer 1,
+// pass p reads buffer 1 - p % 2 and writes the other; the last pass writes the sorted codes and
+// leaves as the tree kernels read them (keys(0), values(0)).
 template <class Set>
+__global__ void __launch_bounds__(SortKeys64::kThreads, kSortWavesPerSimd) BuildSort64PassKernel(Set set,
+                                                                                                   uint32_t pass)
+{
+    const Build& b = set.Get();
+    if (blockIdx.x >= SortTilesOf<SortKeys64>(b.leaf_capacity))
+    {
+        return;
+    }
+    const uint32_t in = 1u - (pass & 1u);
+    const bool last = pass + 1 == uint32_t(SortKeys64::kPasses);
+    const SortSink<SortKeys64> out{last ? nullptr : b.scratch.keys64(1u - in), nullptr, b.scratch.keys(0),
+                                   b.scratch.values(0)};
+    SortPassTileOf<SortKeys64>(b.scratch.keys64(in), nullptr, out, &b.header->leaf_count, &b.state->building,
+                               b.scratch.sort64(), pass);
+}
+
+// The build sequence (light_tree_record.cpp:271-308) for a set of builds; `capacity` = the largest
+// leaf capacity of the set, `count` = the number of builds (grid y). Keys = SortPairs32 (the
+// reference's uint32 Morton keys with leaf values) or SortKeys64 (code << 32 | leaf, sorted as one
+// 64-bit key); both give the same order, hence the same tree.
+template <class Keys, class Set>
 void RecordBuild(const Set& set, uint32_t capacity, uint32_t count, hipStream_t s)
 {
     const uint32_t per_item_blocks = DivUp(capacity, kBvhThreads);
     const uint32_t tiles = SortTiles(capacity);
     {
         StageScope scope(s, "lt_bounds");
-        BoundsKernel<<<dim3(BoundBlocks(capacity), count), kBvhThreads, 0, s>>>(set);
+        BoundsKernel<Set, Keys><<<dim3(BoundBlocks(capacity), count), kBvhThreads, 0, s>>>(set);
     }
     {
         StageScope scope(s, "lt_morton");
-        MortonKernel<<<dim3(tiles, count), kSortThreads, 0, s>>>(set);
+        MortonKernel<Set, Keys><<<dim3(tiles, count), kSortThreads, 0, s>>>(set);
     }
-    for (uint32_t pass = 0; pass < kSortPasses; ++pass)
+    if constexpr (Keys::kValues)
     {
-        StageScope scope(s, "sort_pass");
-        BuildSortPassKernel<<<dim3(tiles, count), kSortThreads, 0, s>>>(set, pass);
+        for (uint32_t pass = 0; pass < kSortPasses; ++pass)
+        {
+            StageScope scope(s, "sort_pass");
+            BuildSortPassKernel<<<dim3(tiles, count), kSortThreads, 0, s>>>(set, pass);
+        }
+    }
+    else
+    {
+        for (uint32_t pass = 0; pass < uint32_t(SortKeys64::kPasses); ++pass)
+        {
+            StageScope scope(s, "sort64_pass");
+            BuildSort64PassKernel<<<dim3(SortTilesOf<SortKeys64>(capacity), count), SortKeys64::kThreads, 0, s>>>(
+                set, pass);
+        }
     }
     static const char* const kTreeStage[] = {"lt_tree_l1", "lt_tree_l2", "lt_tree_l3", "lt_tree_l4", "lt_tree_l5",
                                              "lt_tree_l6"};
@@ -1562,8 +1624,20 @@ CX_API int cx_light_bvh_build(const CxLightBvhBuild* build, void* stream)
         return r;
     }
     const hipStream_t s = static_cast<hipStream_t>(stream);
-    cx::RecordBuild(cx::OneBuild{b}, build->leaf_capacity, 1, s);
+    cx::RecordBuild<cx::SortPairs32>(cx::OneBuild{b}, build->leaf_capacity, 1, s);
     return cx::LaunchResult("cx_light_bvh_build");
+}
+
+CX_API int cx_light_bvh_build_keys64(const CxLightBvhBuild* build, void* stream)
+{
+    cx::Build b;
+    if (int r = cx::CheckBuild(build, "cx_light_bvh_build_keys64", b); r != CX_OK)
+    {
+        return r;
+    }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    cx::RecordBuild<cx::SortKeys64>(cx::OneBuild{b}, build->leaf_capacity, 1, s);
+    return cx::LaunchResult("cx_light_bvh_build_keys64");
 }
 
 CX_API uint64_t cx_light_bvh_batch_bytes(uint32_t count)
@@ -1601,7 +1675,8 @@ CX_API int cx_light_bvh_build_batch(const void* descriptors, uint32_t count, uin
         return cx::ArgError("cx_light_bvh_build_batch: largest leaf capacity %u invalid", max_leaf_capacity);
     }
     const hipStream_t s = static_cast<hipStream_t>(stream);
-    cx::RecordBuild(cx::BuildArray{static_cast<const cx::Build*>(descriptors)}, max_leaf_capacity, count, s);
+    cx::RecordBuild<cx::SortPairs32>(cx::BuildArray{static_cast<const cx::Build*>(descriptors)}, max_leaf_capacity,
+                                     count, s);
     return cx::LaunchResult("cx_light_bvh_build_batch");
 }
 

@@ -96,6 +96,13 @@ CX_API uint64_t cx_light_bvh_batch_bytes(uint32_t count);
 CX_API int cx_light_bvh_batch_prepare(const CxLightBvhBuild* builds, uint32_t count, void* descriptors, void* stream);
 CX_API int cx_light_bvh_build_batch(const void* descriptors, uint32_t count, uint32_t max_leaf_capacity, void* stream);
 
+/* The build with 64-bit sort keys: code << 32 | leaf index, sorted as one key by
+ * cx_radix_sort_keys64's passes (7 x 9 bits) in place of the (code, leaf) pair sort. The order, and
+ * so every output (tree, nodes, leaves, header, scratch arrays), is the one cx_light_bvh_build makes.
+ * No counterpart in the reference, whose keys are 32-bit (light_tree_record.cpp:271-308); it is the
+ * wide-key form for callers whose keys do not fit 32 bits. */
+CX_API int cx_light_bvh_build_keys64(const CxLightBvhBuild* build, void* stream);
+
 /* Stable LSD radix sort of (uint32 key, uint32 value) pairs (the reference's
  * caelix_light_bvh_sort_histogram + caelix_light_bvh_scan + caelix_light_bvh_sort_scatter passes,
  * shaders/light/bvh/caelix_light_bvh_sort.hlsl:1-38), as 4 onesweep passes of 8 bits. The count is
@@ -106,6 +113,14 @@ CX_API uint64_t cx_radix_sort_state_bytes(uint32_t capacity);
 CX_API int cx_radix_sort_pairs(uint32_t* keys0, uint32_t* values0, uint32_t* keys1, uint32_t* values1,
                                const uint32_t* count_address, const uint32_t* gate_address, uint32_t capacity,
                                void* sort_state, void* stream);
+/* Stable LSD radix sort of uint64 keys alone (bits 0..62: 7 onesweep passes of 9 bits; keys with
+ * bit 63 set sort as if it were clear), count and gate as for cx_radix_sort_pairs. The result is in
+ * keys1 (odd pass count); keys0 is overwritten. `sort_state` holds
+ * cx_radix_sort_keys64_state_bytes(capacity) bytes. No counterpart in the reference (32-bit keys,
+ * caelix_light_bvh_sort.hlsl:1-38). */
+CX_API uint64_t cx_radix_sort_keys64_state_bytes(uint32_t capacity);
+CX_API int cx_radix_sort_keys64(uint64_t* keys0, uint64_t* keys1, const uint32_t* count_address,
+                                const uint32_t* gate_address, uint32_t capacity, void* sort_state, void* stream);
 
 /* ------------------------------------------------------------------------------------------
  * Hash table (lib/src/rhi/hash_table.{h,cpp}, shaders/common/caelix_hash_table.hlsl,

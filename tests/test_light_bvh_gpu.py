@@ -1,6 +1,7 @@
-"""GPU parity of the light-tree build (cx_light_bvh_build) and the onesweep sort
-(cx_radix_sort_pairs) against the oracle: bit-exact for Morton codes, sorted order, Karras topology,
-depths, 4-wide indices, aggregates, nodes, leaf parents, header and state."""
+"""GPU parity of the light-tree build (cx_light_bvh_build, and its 64-bit key form
+cx_light_bvh_build_keys64) and the onesweep sorts (cx_radix_sort_pairs, cx_radix_sort_keys64)
+against the oracle: bit-exact for Morton codes, sorted order, Karras topology, depths, 4-wide
+indices, aggregates, nodes, leaf parents, header and state."""
 from __future__ import annotations
 
 import numpy as np
@@ -12,10 +13,11 @@ from caelix_amd import layout, synthetic
 pytestmark = pytest.mark.gpu
 
 
-def gpu_and_oracle(cx, oracle, leaves, capacity, node_capacity=None, found=None, builds=0):
+def gpu_and_oracle(cx, oracle, leaves, capacity, node_capacity=None, found=None, builds=0, keys64=False):
     node_capacity = node_capacity if node_capacity is not None else layout.node_capacity_for(capacity)
     n = len(leaves)
-    print(f"light bvh build: n={n} capacity={capacity} node_capacity={node_capacity} found={found}", flush=True)
+    print(f"light bvh build: n={n} capacity={capacity} node_capacity={node_capacity} found={found} "
+          f"keys64={keys64}", flush=True)
     builder = cx.LightBvhBuilder(capacity, node_capacity)
     builder.upload(leaves)
     if found is not None or builds:
@@ -25,7 +27,7 @@ def gpu_and_oracle(cx, oracle, leaves, capacity, node_capacity=None, found=None,
         st["builds"] = builds
         builder.state.copy_(cx.to_device(st.view(np.uint8)))
         torch.cuda.synchronize()
-    builder.build()
+    builder.build(keys64=keys64)
     gpu = builder.read()
     state = np.zeros(1, layout.STATE)
     state["building"] = 1
@@ -84,6 +86,16 @@ def test_build_matches_oracle_small(cx, oracle, n):
     if n > 0:
         h = gpu["header"]
         lt_checks.check_bvh(h, gpu["state"], gpu["nodes"][: int(h["node_count"])], gpu["leaves"][:n])
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 64, 1000, 4095, 4097, 70000])
+def test_build_keys64_matches_oracle_small(cx, oracle, n):
+    """The 64-bit key form (code << 32 | leaf sorted as one key) makes the oracle's tree too, with
+    the same sorted codes and leaves in the scratch."""
+    leaves = synthetic.light_leaves(n, 300 + n, extent=2048.0, clusters=8, sigma=16.0)
+    capacity = max(64, (n + 63) // 64 * 64)
+    gpu, ref = gpu_and_oracle(cx, oracle, leaves, capacity, keys64=True)
+    compare(gpu, ref, capacity, n)
 
 
 def test_build_capacity_above_count_and_dropped(cx, oracle):
@@ -162,6 +174,60 @@ def test_radix_sort_full_32bit_keys(cx, oracle):
     rk, rv = oracle.radix_sort_pairs(keys, values)
     assert np.array_equal(k.cpu().numpy().view(np.uint32), rk)
     assert np.array_equal(v.cpu().numpy().view(np.uint32), rv)
+
+
+def test_radix_sort_keys64_matches_stable_sort(cx):
+    """cx_radix_sort_keys64 against numpy's stable sort of the same uint64 keys: random 63-bit
+    keys with many duplicates, keys that differ only in the low word or only in the high word, and
+    sizes around the 4096-key tile."""
+    import torch
+
+    rng = np.random.default_rng(64)
+    cases = {}
+    for n in [1, 2, 100, 4095, 4096, 4097, 1 << 20]:
+        keys = rng.integers(0, 2**63, size=n, dtype=np.uint64)
+        keys[::5] = keys[0]
+        cases[f"random{n}"] = keys
+    n = 300000
+    cases["low_word"] = (np.uint64(7) << np.uint64(40)) | rng.integers(0, 2**32, size=n, dtype=np.uint64)
+    cases["high_word"] = rng.integers(0, 2**31, size=n, dtype=np.uint64) << np.uint64(32)
+    cases["code_leaf"] = (rng.integers(0, 2**30, size=n, dtype=np.uint64) << np.uint64(32)) | np.arange(
+        n, dtype=np.uint64)
+    for name, keys in cases.items():
+        print(f"radix sort keys64 {name} n={len(keys)}", flush=True)
+        k = torch.from_numpy(keys.view(np.int64).copy()).cuda()
+        cx.radix_sort_keys64(k)
+        torch.cuda.synchronize()
+        assert np.array_equal(k.cpu().numpy().view(np.uint64), np.sort(keys, kind="stable")), name
+
+
+def test_rebuild_stress_keys64_matches_u32_build(cx):
+    """At the full 4,194,304-leaf rebuild size, the 64-bit key build writes exactly what the
+    32-bit key build (checked against the oracle in the next test) writes."""
+    n = 1 << 22
+    leaves = synthetic.light_leaves(n, 2024, extent=8192.0, clusters=64, sigma=32.0)
+    outs = []
+    for keys64 in (False, True):
+        print(f"light bvh stress build keys64={keys64}", flush=True)
+        builder = cx.LightBvhBuilder(n)
+        builder.upload(leaves)
+        builder.build(keys64=keys64)
+        outs.append(builder.read())
+    a, b = outs
+    assert a["header"].tobytes() == b["header"].tobytes()
+    assert a["state"].tobytes() == b["state"].tobytes()
+    count = int(a["header"]["node_count"])
+    for field in ("child_face", "child_lo", "child_hi", "child", "parent", "parent_slot"):
+        assert np.array_equal(a["nodes"][field][:count], b["nodes"][field][:count]), field
+    assert a["leaves"].tobytes() == b["leaves"].tobytes()
+    for name, offset, count in [("keys", layout.SCRATCH_KEYS, n), ("values", layout.SCRATCH_VALUES, n),
+                                ("left", layout.SCRATCH_LEFT, n - 1), ("counter", layout.SCRATCH_COUNTER, n),
+                                ("wide", layout.SCRATCH_WIDE, n)]:
+        ga = layout.scratch_u32(a["scratch"], offset, n)[:count]
+        gb = layout.scratch_u32(b["scratch"], offset, n)[:count]
+        assert np.array_equal(ga, gb), name
+    assert layout.scratch_aggregates(a["scratch"], n)[: n - 1].tobytes() == \
+        layout.scratch_aggregates(b["scratch"], n)[: n - 1].tobytes()
 
 
 def test_rebuild_stress_size_matches_oracle(cx, oracle):
