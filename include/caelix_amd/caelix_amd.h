@@ -291,8 +291,9 @@ typedef struct CxTraceOutput {
     uint64_t workspace_bytes; /* its size */
 } CxTraceOutput;
 
-/* Device workspace of cx_trace_frame for a width x height image: the path state, the path queues
- * and the shadow-ray queue of the wavefront (about 184 bytes per pixel, 8x8-tile padded). */
+/* Device workspace of cx_trace_frame (and cx_trace_frame_adjoint) for a width x height image: the
+ * path state, the path queues, the shadow rays and the adjoint's gradient rows of the wavefront
+ * (about 230 bytes per pixel, 8x8-tile padded; 0.44 GiB at 1920 x 1080). */
 CX_API uint64_t cx_trace_workspace_bytes(uint32_t width, uint32_t height);
 
 /* One frame: primary ray, brick DDA, light NEE + MIS, `bounces` bounces (at most 255), for every
