@@ -65,6 +65,7 @@ class CxScene(ctypes.Structure):
         ("group_shift", ctypes.c_uint32), ("encoding", ctypes.c_uint32), ("materials", ctypes.c_void_p),
         ("light_codes", ctypes.c_void_p), ("emission_scale", ctypes.c_float), ("pad", ctypes.c_uint32),
         ("brick_emission_scale", ctypes.c_void_p), ("scene_count", ctypes.c_uint32), ("pad2", ctypes.c_uint32),
+        ("occupied_box", ctypes.c_void_p),
     ]
 
 
@@ -457,6 +458,7 @@ class TraceScene:
         cells = dim ** 3
         self.brick_bits = torch.zeros((cells + 31) // 32, dtype=torch.int32, device=dev)
         self.super_bits = torch.zeros((cells // 512 + 31) // 32 + 1, dtype=torch.int32, device=dev)
+        self.occupied_box = torch.zeros(6, dtype=torch.int32, device=dev)
         sc = self.scene = CxScene()
         sc.keys = self.keys.data_ptr()
         sc.info = self.info.data_ptr()
@@ -475,6 +477,7 @@ class TraceScene:
         sc.materials = self.materials.data_ptr()
         sc.light_codes = self.light_codes.data_ptr()
         sc.emission_scale = 1.0
+        sc.occupied_box = self.occupied_box.data_ptr()
         h = stream_handle(stream)
         check(lib.cx_scene_finalize(ctypes.byref(sc), h), "cx_scene_finalize")
         # Light pass, growing the heap when it does not fit (as the reference grows its light heap).

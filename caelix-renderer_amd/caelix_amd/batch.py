@@ -59,6 +59,7 @@ class TraceBatch:
         self.occupancy = torch.zeros(n * 8, dtype=torch.int64, device=dev)
         self.brick_bits = torch.zeros((S * cells + 31) // 32, dtype=torch.int32, device=dev)
         self.super_bits = torch.zeros((S * (cells // 512) + 31) // 32 + 1, dtype=torch.int32, device=dev)
+        self.occupied_box = torch.zeros(S * 6, dtype=torch.int32, device=dev)
         sc = self.scene = CxScene()
         sc.keys = self.keys.data_ptr()
         sc.info = self.info.data_ptr()
@@ -78,6 +79,7 @@ class TraceBatch:
         sc.light_codes = self.light_codes.data_ptr()
         sc.emission_scale = 1.0
         sc.scene_count = S
+        sc.occupied_box = self.occupied_box.data_ptr()
         h = stream_handle()
         check(lib.cx_scene_finalize(ctypes.byref(sc), h), "cx_scene_finalize")
         # the light pass over all scenes: ordinals and leaves scene-major

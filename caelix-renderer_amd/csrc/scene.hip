@@ -37,6 +37,7 @@ struct SceneArgs
     const float4* materials;
     const uint32_t* light_codes;
     uint32_t key_words;  // 3: (x, y, z); 4: (scene, x, y, z) for a batch of chunk scenes
+    int32_t* box;        // per scene: super-cell box of the bricks with a voxel, or null
 };
 
 // The (x, y, z) brick coordinates of a record and its scene (0 for one scene).
@@ -61,6 +62,15 @@ CX_DEV uint2 KeyFingerprint(uint32_t key_words, uint32_t scene, int32_t x, int32
 CX_DEV bool Opaque(uint32_t block, uint32_t encoding)
 {
     return (block & (encoding == kEncodingPacked ? kPackedOpaqueMask : kBlockIdOpaqueMask)) != 0;
+}
+
+// Empty boxes (lo = INT32_MAX, hi = INT32_MIN) for the finalize kernel's atomic min / max.
+__global__ void BoxInitKernel(int32_t* box, uint32_t scenes)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < scenes * 6; i += gridDim.x * blockDim.x)
+    {
+        box[i] = i % 6 < 3 ? INT32_MAX : INT32_MIN;
+    }
 }
 
 __global__ void FillKernel(uint32_t* p, uint64_t words)
@@ -146,6 +156,16 @@ __global__ void __launch_bounds__(64) FinalizeKernel(SceneArgs s)
             const uint64_t super_index = scene * (cells / 512) + (uint64_t(gz >> 3) * sd + uint32_t(gy >> 3)) * sd +
                                          uint32_t(gx >> 3);
             atomicOr(&s.super_bits[super_index >> 5], 1u << (super_index & 31u));
+            if (s.box != nullptr)
+            {
+                int32_t* box = s.box + scene * 6;
+                atomicMin(&box[0], gx >> 3);
+                atomicMin(&box[1], gy >> 3);
+                atomicMin(&box[2], gz >> 3);
+                atomicMax(&box[3], gx >> 3);
+                atomicMax(&box[4], gy >> 3);
+                atomicMax(&box[5], gz >> 3);
+            }
         }
     }
 }
@@ -512,6 +532,7 @@ SceneArgs ArgsOf(const CxScene& c)
     s.materials = reinterpret_cast<const float4*>(c.materials);
     s.light_codes = c.light_codes;
     s.key_words = c.scene_count > 1 ? 4u : 3u;
+    s.box = c.occupied_box;
     return s;
 }
 
@@ -538,6 +559,11 @@ CX_API int cx_scene_finalize(const CxScene* scene, void* stream)
     cx::StageScope scope(s, "scene_finalize");
     cx::FillKernel<<<2048, 256, 0, s>>>(scene->brick_bits, (cells + 31) / 32);
     cx::FillKernel<<<256, 256, 0, s>>>(scene->super_bits, (supers + 31) / 32);
+    if (scene->occupied_box != nullptr)
+    {
+        cx::BoxInitKernel<<<uint32_t(cx::DivUp(scenes * 6, uint64_t(256))), 256, 0, s>>>(scene->occupied_box,
+                                                                                        uint32_t(scenes));
+    }
     if (scene->bricks > 0)
     {
         cx::FinalizeKernel<<<scene->bricks, 64, 0, s>>>(cx::ArgsOf(*scene));

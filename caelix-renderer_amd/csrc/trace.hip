@@ -22,6 +22,7 @@ SceneView SceneOf(const CxScene& c)
     s.hash_info = c.hash_info;
     s.brick_bits = c.brick_bits;
     s.super_bits = c.super_bits;
+    s.box = c.occupied_box;
     for (int k = 0; k < 3; ++k)
     {
         s.grid_min[k] = c.grid_min[k];

@@ -29,6 +29,7 @@ struct SceneView
     const uint32_t* hash_info;  // [0] probe limit
     const uint32_t* brick_bits;  // grid_dim^3 bits: brick cell resident
     const uint32_t* super_bits;  // (grid_dim / 8)^3 bits: any resident brick in the 8^3 super cell
+    const int32_t* box;          // per scene: super-cell box of the resident bricks (lo, hi), or null
     int32_t grid_min[3];         // brick coordinate of grid cell 0
     uint32_t grid_dim;           // bricks per axis, a multiple of 8
     const float4* materials;     // 2 per 16-bit block value: (albedo, metallic), (emission, smoothness)
@@ -452,6 +453,33 @@ CX_DEV uint2 SceneBrickFingerprint(const SceneView& s, uint32_t scene, int32_t x
 // cells and, inside a resident super cell, over its bricks; a resident brick is found through the
 // hash table and traced with the brick DDA. The first hit is the closest (bricks are disjoint and
 // visited in ray order). `any_hit`: a shadow ray (any hit with t < t_max).
+// Ends the traversal where the ray leaves the box of super cells holding resident bricks, widened by
+// one super cell within the grid; t_exit < t_enter when the ray misses it. No resident brick lies
+// outside the box, and its planes are super-cell planes computed as the super-cell exits are
+// ((k * 64 + grid origin - o) * inv), so every resident super cell is reached exactly as before.
+CX_DEV void ClipToOccupied(const SceneView& s, uint32_t scene, const float (&ov)[3], const float (&inv)[3],
+                           float t_enter, float& t_exit)
+{
+    const int32_t* b = s.box + uint64_t(scene) * 6;
+    const int super_dim = int(s.grid_dim >> 3);
+    float box_enter = t_enter;
+    float box_exit = t_exit;
+    bool empty = false;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        const int lo = max(b[a], 1) - 1;
+        const int hi = min(b[3 + a], super_dim - 2) + 1;
+        empty = empty || b[a] > b[3 + a];
+        const float g0 = float(s.grid_min[a] * 8);
+        const float t0 = (float(lo * 64) + g0 - ov[a]) * inv[a];
+        const float t1 = (float((hi + 1) * 64) + g0 - ov[a]) * inv[a];
+        box_enter = fmaxf(box_enter, fminf(t0, t1));
+        box_exit = fminf(box_exit, fmaxf(t0, t1));
+    }
+    t_exit = empty || box_enter > box_exit ? -1.0f : box_exit;
+}
+
 CX_DEV WorldHit TraceWorld(const SceneView& s, V3 o, V3 dir, float t_max, uint32_t scene = 0)
 {
     WorldHit result{0.0f, 0u, kHashNone, {0, 0, 0}, 0u, 0u, 0u};
@@ -472,6 +500,10 @@ CX_DEV WorldHit TraceWorld(const SceneView& s, V3 o, V3 dir, float t_max, uint32
         const float t1 = (g0[a] + extent - ov[a]) * inv[a];
         t_enter = fmaxf(t_enter, fminf(t0, t1));
         t_exit = fminf(t_exit, fmaxf(t0, t1));
+    }
+    if (s.box != nullptr && t_enter <= t_exit)
+    {
+        ClipToOccupied(s, scene, ov, inv, t_enter, t_exit);
     }
     if (t_enter > t_exit)
     {
@@ -649,6 +681,12 @@ CX_DEV void WorldRayBegin(const SceneView& s, V3 o, V3 dir, float t_max, uint32_
         const float t1 = (g0 + extent - Get(o, a)) * inv;
         t_enter = fmaxf(t_enter, fminf(t0, t1));
         t_exit = fminf(t_exit, fmaxf(t0, t1));
+    }
+    if (s.box != nullptr && t_enter <= t_exit)
+    {
+        const float ov[3] = {o.x, o.y, o.z};
+        const float inv[3] = {r.ri.safe.x, r.ri.safe.y, r.ri.safe.z};
+        ClipToOccupied(s, scene, ov, inv, t_enter, t_exit);
     }
     r.t_exit = t_exit;
     r.t_cell = t_enter;

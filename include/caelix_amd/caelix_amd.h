@@ -213,6 +213,11 @@ typedef struct CxScene {
      * (and (grid_dim / 8)^3) bits, scene-major. 0 or 1: one scene, keys [bricks][3]. */
     uint32_t scene_count;
     uint32_t pad2;
+    /* [max(scene_count, 1)][6], or NULL: per scene the box of super cells (lo x, y, z, hi x, y, z,
+     * inclusive) that hold a brick with a voxel, written by cx_scene_finalize. The traversal then ends
+     * a ray where it leaves that box widened by one super cell (no resident brick lies outside; hits
+     * and counts are unchanged). NULL: every ray is walked to the end of the grid. */
+    int32_t* occupied_box;
 } CxScene;
 
 /* The record producer's opaque subset (shaders/scene/caelix_finalize.hlsl:112-361): word 0 (group

@@ -128,7 +128,7 @@ class _Scene(ctypes.Structure):
         ("bricks", _U32), ("hash_info", _P), ("brick_bits", _P), ("super_bits", _P),
         ("grid_min", ctypes.c_int32 * 3), ("grid_dim", _U32), ("group_shift", _U32), ("encoding", _U32),
         ("materials", _P), ("light_codes", _P), ("emission_scale", ctypes.c_float), ("pad", _U32),
-        ("brick_emission_scale", _P), ("scene_count", _U32), ("pad2", _U32),
+        ("brick_emission_scale", _P), ("scene_count", _U32), ("pad2", _U32), ("occupied_box", _P),
     ]
 
 

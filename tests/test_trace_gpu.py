@@ -182,6 +182,33 @@ def test_trace_rejects_bad_arguments(cx, pair):
         gpu.trace_frame(SMALL.camera, 1, 0, 0)
 
 
+def test_occupied_box_clip_changes_nothing(cx, oracle):
+    """The same bricks in a grid four times wider: cx_scene_finalize's occupied super-cell box is the
+    box of the bricks, and ending rays where they leave it (widened by one super cell) changes no hit,
+    no counter and no pixel: the same results as walking the whole grid, and as the oracle."""
+    import dataclasses
+
+    big = dataclasses.replace(SMALL, grid_min=(-24, -16, -20), grid_dim=64)
+    gpu = cx.TraceScene(big, load_factor=LOAD)
+    keys = np.asarray(big.keys)
+    has_voxel = (np.asarray(big.payload).reshape(len(keys), -1) != 0).any(axis=1)
+    sc = (keys[has_voxel] - np.asarray(big.grid_min)) // 8
+    assert np.array_equal(gpu.occupied_box.cpu().numpy(), np.concatenate([sc.min(axis=0), sc.max(axis=0)]))
+    solid = tc.dense_blocks(SMALL) != 0
+    rays = _rays(solid, np.random.default_rng(13), 20000)
+    rays[:2000, 0:3] = np.random.default_rng(14).uniform(-200, 330, size=(2000, 3))
+    clipped_rays = gpu.trace_rays(rays)
+    clipped_frame = [a.cpu().numpy() for a in gpu.trace_frame(SMALL.camera, 2, 9, 1)]
+    gpu.scene.occupied_box = None
+    whole_rays = gpu.trace_rays(rays)
+    whole_frame = [a.cpu().numpy() for a in gpu.trace_frame(SMALL.camera, 2, 9, 1)]
+    assert np.array_equal(clipped_rays, whole_rays)
+    for a, b in zip(clipped_frame, whole_frame):
+        assert a.tobytes() == b.tobytes()
+    ref = oracle.OracleScene.build(big, slot_count(big))
+    assert np.array_equal(clipped_rays, ref.trace_rays(rays))
+
+
 def test_trace_frame_odd_image_size(pair):
     """An image that is not a multiple of the 8x8 path tiles: padding paths trace nothing."""
     gpu, _, ref = pair
