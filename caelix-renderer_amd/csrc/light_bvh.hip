@@ -31,6 +31,7 @@ namespace {
 
 constexpr uint32_t kBvhThreads = 256;
 constexpr uint32_t kBoundsMaxBlocks = 1024;
+constexpr uint32_t kBoundsUnroll = 4;
 constexpr uint32_t kScanThreads = 1024;
 constexpr uint32_t kScanItems = 16;
 constexpr uint32_t kScanTile = kScanThreads * kScanItems;
@@ -48,6 +49,7 @@ enum : uint32_t
     kCtlCoverTiles = 2,
     kCtlFlagsTiles = 3,
     kCtlWideNodes = 4,  // 4-wide nodes of the build (set by the flags scan)
+    kCtlTreeOverflow = 5,  // nonzero: an open list or an upper level's LDS tables overflowed
     kCtlWords = 64,
 };
 
@@ -56,7 +58,9 @@ enum : uint32_t
 constexpr uint32_t kTreeThreads = 512;  // level 1: one lane per leaf
 constexpr uint32_t kTreeTile = 512;
 constexpr uint32_t kTreeFanout = 32;
-constexpr uint32_t kTopsCap = 256;
+constexpr uint32_t kOpenCap = 256;   // open elements per block (bounded by twice the tree depth)
+constexpr uint32_t kOpenWords = 18;  // one open element: range, boundary deltas, child words, summary
+constexpr uint32_t kMaxClimb = 128;  // climb steps per element and level (the tree depth is below 64)
 
 __host__ __device__ inline uint64_t TreeTile(uint32_t level)
 {
@@ -80,10 +84,10 @@ __host__ __device__ inline uint32_t TreeBlocks(uint32_t capacity, uint32_t level
 {
     return uint32_t((uint64_t(capacity) + TreeTile(level) - 1) / TreeTile(level));
 }
-__host__ __device__ inline uint64_t TopsBytes(uint32_t capacity, uint32_t level)
+__host__ __device__ inline uint64_t OpenBytes(uint32_t capacity, uint32_t level)
 {
     const uint64_t blocks = TreeBlocks(capacity, level);
-    return ((blocks * kTopsCap * 8 + blocks * 4) + 255) & ~uint64_t(255);
+    return ((blocks * kOpenCap * kOpenWords * 4 + blocks * 4) + 255) & ~uint64_t(255);
 }
 
 struct Scratch
@@ -126,9 +130,12 @@ struct Scratch
     {
         return reinterpret_cast<uint64_t*>(u32(kLtScratchKeys + buffer * kLtScratchSortBufferStride));
     }
-    // Tops of the tree levels: per level-L workgroup, up to kTopsCap (first, last) leaf ranges of
-    // finished elements whose parent gap is on the workgroup's boundary, and their count.
-    __host__ __device__ uint8_t* tops_base() const
+    // The wide collapse's inputs, written while the binary tree is built: per binary node its
+    // child words (`kids`: a leaf child as kLtChildLeafBit | leaf id, an internal child as its node
+    // index) and the 4 slots of the 4-wide node it becomes if its depth is even (`quad`: the child
+    // words of its grandchildren, a leaf child in place of its absent children, kLtNone padding).
+    // 24 bytes per leaf of capacity, at a fixed offset (no per-level layout walk).
+    __host__ __device__ uint8_t* wide_inputs_base() const
     {
         const uint64_t state = SortStateBytesOf<SortPairs32>(capacity) > SortStateBytesOf<SortKeys64>(capacity)
                                    ? SortStateBytesOf<SortPairs32>(capacity)
@@ -136,26 +143,31 @@ struct Scratch
         const uint64_t offset = uint64_t(sort_base() - base) + state;
         return base + ((offset + 255) & ~uint64_t(255));
     }
-    __host__ __device__ uint2* tops(uint32_t level) const
+    __host__ __device__ uint4* quad() const { return reinterpret_cast<uint4*>(wide_inputs_base()); }
+    __host__ __device__ uint2* kids() const { return reinterpret_cast<uint2*>(quad() + capacity); }
+    // Open lists of the tree levels: per level-L block, up to kOpenCap elements whose parent the
+    // block did not build (kOpenWords words each), and their count.
+    __host__ __device__ uint8_t* open_base() const { return reinterpret_cast<uint8_t*>(kids() + capacity); }
+    __host__ __device__ uint32_t* open(uint32_t level) const
     {
-        uint8_t* p = tops_base();
+        uint8_t* p = open_base();
         for (uint32_t l = 1; l < level; ++l)
         {
-            p += TopsBytes(capacity, l);
+            p += OpenBytes(capacity, l);
         }
-        return reinterpret_cast<uint2*>(p);
+        return reinterpret_cast<uint32_t*>(p);
     }
-    __host__ __device__ uint32_t* tops_count(uint32_t level) const
+    __host__ __device__ uint32_t* open_count(uint32_t level) const
     {
-        return reinterpret_cast<uint32_t*>(tops(level) + uint64_t(TreeBlocks(capacity, level)) * kTopsCap);
+        return open(level) + uint64_t(TreeBlocks(capacity, level)) * kOpenCap * kOpenWords;
     }
     static uint64_t Bytes(uint32_t capacity)
     {
         const Scratch s{nullptr, capacity};
-        uint8_t* p = s.tops_base();
+        uint8_t* p = s.open_base();
         for (uint32_t l = 1; l <= TreeLevels(capacity); ++l)
         {
-            p += TopsBytes(capacity, l);
+            p += OpenBytes(capacity, l);
         }
         return uint64_t(p - static_cast<uint8_t*>(nullptr));
     }
@@ -378,24 +390,44 @@ __global__ void __launch_bounds__(kBvhThreads) BoundsKernel(Set set)
     float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f};
     float hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
     float* centres = b.scratch.centres();
-    for (uint32_t i = blockIdx.x * kBvhThreads + t; i < n; i += grid * kBvhThreads)
+    // kBoundsUnroll leaves per thread and round, all loaded before any store (a store would order
+    // the later loads behind it); a leaf past n loads leaf n - 1 and is not counted.
+    const uint32_t stride = grid * kBvhThreads;
+    for (uint32_t first = blockIdx.x * kBvhThreads + t; first < n; first += stride * kBoundsUnroll)
     {
-        const float* leaf = b.leaves[i].summary.lo;
-        const float2 l01 = *reinterpret_cast<const float2*>(leaf);       // lo.x, lo.y
-        const float2 l2h0 = *reinterpret_cast<const float2*>(leaf + 2);  // lo.z, hi.x
-        const float2 h12 = *reinterpret_cast<const float2*>(leaf + 4);   // hi.y, hi.z
-        const float c0 = 0.5f * (l01.x + l2h0.y);
-        const float c1 = 0.5f * (l01.y + h12.x);
-        const float c2 = 0.5f * (l2h0.x + h12.y);
-        centres[3 * uint64_t(i)] = c0;
-        centres[3 * uint64_t(i) + 1] = c1;
-        centres[3 * uint64_t(i) + 2] = c2;
-        lo[0] = fminf(lo[0], c0);
-        lo[1] = fminf(lo[1], c1);
-        lo[2] = fminf(lo[2], c2);
-        hi[0] = fmaxf(hi[0], c0);
-        hi[1] = fmaxf(hi[1], c1);
-        hi[2] = fmaxf(hi[2], c2);
+        float2 l01[kBoundsUnroll];
+        float2 l2h0[kBoundsUnroll];
+        float2 h12[kBoundsUnroll];
+#pragma unroll
+        for (uint32_t j = 0; j < kBoundsUnroll; ++j)
+        {
+            const uint32_t i = min(first + j * stride, n - 1);
+            const float* leaf = b.leaves[i].summary.lo;
+            l01[j] = *reinterpret_cast<const float2*>(leaf);       // lo.x, lo.y
+            l2h0[j] = *reinterpret_cast<const float2*>(leaf + 2);  // lo.z, hi.x
+            h12[j] = *reinterpret_cast<const float2*>(leaf + 4);   // hi.y, hi.z
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kBoundsUnroll; ++j)
+        {
+            const uint32_t i = first + j * stride;
+            if (i >= n)
+            {
+                break;
+            }
+            const float c0 = 0.5f * (l01[j].x + l2h0[j].y);
+            const float c1 = 0.5f * (l01[j].y + h12[j].x);
+            const float c2 = 0.5f * (l2h0[j].x + h12[j].y);
+            centres[3 * uint64_t(i)] = c0;
+            centres[3 * uint64_t(i) + 1] = c1;
+            centres[3 * uint64_t(i) + 2] = c2;
+            lo[0] = fminf(lo[0], c0);
+            lo[1] = fminf(lo[1], c1);
+            lo[2] = fminf(lo[2], c2);
+            hi[0] = fmaxf(hi[0], c0);
+            hi[1] = fmaxf(hi[1], c1);
+            hi[2] = fmaxf(hi[2], c2);
+        }
     }
     MinMax3<kBvhThreads / kWave>(lo, hi, lds, t / kWave);
     float* partials = b.scratch.bounds_partials();
@@ -410,6 +442,7 @@ __global__ void __launch_bounds__(kBvhThreads) BoundsKernel(Set set)
         {
             ctl[kCtlCoverTiles] = 0;
             ctl[kCtlFlagsTiles] = 0;
+            ctl[kCtlTreeOverflow] = 0;
         }
         const SortState s = SortStateOf<Keys>(b.scratch.sort_base(), b.leaf_capacity);
         for (uint32_t i = t; i < kSortCtlBytes / 4; i += kBvhThreads)
@@ -509,8 +542,13 @@ __global__ void __launch_bounds__(kSortThreads) MortonKernel(Set set)
         {
             break;
         }
-        counter[i] = i + 1 < n ? kLtNone : 0u;  // gap slots of the tree build; 0 past the last gap
-        diff[i] = 0;
+        // The depths replace counter[i] for i < n - 1 (flags scan) and level 1 writes diff[i] for
+        // i < n; past them the arrays are zero.
+        if (i + 1 >= n)
+        {
+            counter[i] = 0u;
+            diff[i] = 0;
+        }
         if (i < n)
         {
             const float c0 = centres[3 * uint64_t(i)];
@@ -549,78 +587,64 @@ __global__ void __launch_bounds__(kSortThreads) MortonKernel(Set set)
 // node with leaf range [a, b]) climbs to its parent: the parent's split is the adjacent gap with the
 // longer common prefix, b if delta(b) > delta(a - 1), else a - 1 (Apetrei 2014). Karras numbers a
 // node by the range end that is not its parent's split: a left child [a, g] is node g, a right child
-// [g + 1, b] is node g + 1, the root is node 0. The first element to arrive at a gap parks its far
-// range end in slot[gap] and stops; the second builds the parent: range, children, parent links, and
-// the union of the two summaries (left, then right: caelix_light_bvh_aggregate.hlsl:30-55), then
-// climbs on. Slot arrivals are workgroup-scope atomics: a level-L workgroup only arrives at gaps
-// inside its own block of sorted leaves, so both arrivals of a gap happen in one workgroup of one
-// launch, or the earlier one in an earlier launch. An element whose parent gap is on the block's
-// boundary is handed to the next level as a "top". Also written per node: its split gap and +1/-1 at
-// its range ends in the gap-cover array (for the depths).
+// [g + 1, b] is node g + 1, the root is node 0. The first element to arrive at a gap parks and stops;
+// the second builds the parent: range, children, parent links, and the union of the two summaries
+// (left, then right: caelix_light_bvh_aggregate.hlsl:30-55), then climbs on. Also written per node:
+// its split gap, +1/-1 at its range ends in the gap-cover array (for the depths), its child words and
+// its 4-wide slot list (for the wide collapse).
+//
+// The climb runs in levels of nested blocks of sorted leaves, each level-L workgroup in LDS: every
+// arrival at a gap inside the block happens in the block's workgroup, so the arrivals, parked
+// elements and exchanges are LDS operations and a climb step makes no global round trip. An element
+// whose parent is not built inside its block (its parent gap is on the block boundary, or it parked
+// and its sibling reaches outside the block) is "open": the workgroup writes it, whole (range,
+// boundary deltas, child words, summary), to its block's open list, and the level above starts from
+// the open lists of its 32 sub-blocks. Kernel boundaries order the levels.
 
 CX_DEV uint32_t Clz(uint32_t x)
 {
     return x == 0 ? 32u : uint32_t(__clz(int(x)));
 }
 
-// delta(g) of the gap between sorted positions g and g + 1; -1 outside [0, n - 1).
-CX_DEV int GapDelta(const uint32_t* keys, int g, int n)
-{
-    if (g < 0 || g + 1 >= n)
-    {
-        return -1;
-    }
-    const uint32_t k0 = keys[g];
-    const uint32_t k1 = keys[g + 1];
-    if (k0 == k1)
-    {
-        return 32 + int(Clz(uint32_t(g) ^ uint32_t(g + 1)));
-    }
-    return int(Clz(k0 ^ k1));
-}
-
-CX_DEV Summary ElementSummary(const Build& b, uint32_t element)
-{
-    if ((element & kLtChildLeafBit) != 0)
-    {
-        const uint32_t leaf = b.scratch.values(0)[element & ~kLtChildLeafBit];
-        return LoadSummary(b.leaves[leaf].summary.face);
-    }
-    const uint64_t* q = reinterpret_cast<const uint64_t*>(b.scratch.aggregates() + uint64_t(element) * 12);
-    uint64_t w[6];
-#pragma unroll
-    for (int i = 0; i < 6; ++i)
-    {
-        w[i] = __hip_atomic_load(q + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-    Summary s;
-    __builtin_memcpy(&s, w, sizeof(s));
-    return s;
-}
-
-CX_DEV void StoreAggregate(const Build& b, uint32_t node, const Summary& s)
-{
-    uint64_t w[6];
-    __builtin_memcpy(w, &s, sizeof(s));
-    uint64_t* q = reinterpret_cast<uint64_t*>(b.scratch.aggregates() + uint64_t(node) * 12);
-#pragma unroll
-    for (int i = 0; i < 6; ++i)
-    {
-        __hip_atomic_store(q + i, w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-}
-
-// An element climbing at level L > 1: leaf range [a, last] (a leaf when a == last), the deltas of
-// its two boundary gaps (d_left = delta(a - 1), d_right = delta(last)) and its summary, all carried
-// by the lane, so a climb step only fetches what belongs to the sibling.
+// An element climbing: leaf range [a, last] (a leaf when a == last), the deltas of its two boundary
+// gaps (d_left = delta(a - 1), d_right = delta(last)), its summary, and what it contributes to its
+// parent's 4-wide slots: a leaf its own child word (and kLtNone), an internal node its two child
+// words. 72 bytes, also the format of an open-list record (kOpenWords words).
 struct Climb
 {
     uint32_t a;
     uint32_t last;
     int d_left;
     int d_right;
+    uint32_t kid0;
+    uint32_t kid1;
     Summary own;
 };
+static_assert(sizeof(Climb) == kOpenWords * 4, "open record layout");
+
+CX_DEV void StoreClimb(uint32_t* dst, const Climb& e)  // 8-byte aligned
+{
+    uint64_t w[kOpenWords / 2];
+    __builtin_memcpy(w, &e, sizeof(e));
+#pragma unroll
+    for (uint32_t i = 0; i < kOpenWords / 2; ++i)
+    {
+        reinterpret_cast<uint64_t*>(dst)[i] = w[i];
+    }
+}
+
+CX_DEV Climb LoadClimb(const uint32_t* src)
+{
+    uint64_t w[kOpenWords / 2];
+#pragma unroll
+    for (uint32_t i = 0; i < kOpenWords / 2; ++i)
+    {
+        w[i] = reinterpret_cast<const uint64_t*>(src)[i];
+    }
+    Climb e;
+    __builtin_memcpy(&e, w, sizeof(e));
+    return e;
+}
 
 // The element id of [a, last] given its boundary deltas: a leaf, else the internal node numbered
 // by its parent gap (a left child carries the index of its right end).
@@ -633,211 +657,266 @@ CX_DEV uint32_t ElementOf(uint32_t a, uint32_t last, int d_left, int d_right)
     return d_right > d_left ? last : a;
 }
 
-// One climb step of a finished element inside the block [block_lo, block_hi). Returns true when the
-// element's parent is built (the lane then holds the parent and climbs on); false when the element
-// arrived first at its gap, is the root, or has its parent gap on the block boundary (then it is
-// appended to the block's tops). A higher level uses the global gap slots, where an earlier level
-// may have parked a value.
-CX_DEV bool ClimbStep(const Build& b, int n, Climb& e, uint64_t block_lo, uint64_t block_hi, uint2* tops,
-                      uint32_t* lds_top_count)
+// The 4 slots of the 4-wide node made from a binary node whose children contribute (l0, l1) and
+// (r0, r1), in order; kLtNone pads.
+CX_DEV uint4 QuadOf(uint32_t l0, uint32_t l1, uint32_t r0, uint32_t r1)
 {
-    const uint32_t* keys = b.scratch.keys(0);
-    if (e.a == 0 && e.last == uint32_t(n - 1))
+    return l1 == kLtNone ? make_uint4(l0, r0, r1, kLtNone) : make_uint4(l0, l1, r0, r1);
+}
+
+// The parent of the left child `l` and the right child `r` split at `gap`: range, boundary deltas,
+// child words and summary (left, then right: caelix_light_bvh_aggregate.hlsl:30-55).
+CX_DEV uint32_t WordOf(const Climb& e)
+{
+    return e.a == e.last ? e.kid0 : ElementOf(e.a, e.last, e.d_left, e.d_right);
+}
+
+CX_DEV Climb ParentOf(const Climb& l, const Climb& r)
+{
+    Climb p;
+    p.a = l.a;
+    p.last = r.last;
+    p.d_left = l.d_left;
+    p.d_right = r.d_right;
+    p.kid0 = WordOf(l);
+    p.kid1 = WordOf(r);
+    p.own = Union(l.own, r.own);
+    return p;
+}
+
+// The node index of a built element (the root is node 0).
+CX_DEV uint32_t NodeOf(const Climb& p, int n)
+{
+    return p.a == 0 && p.last == uint32_t(n - 1) ? 0u : ElementOf(p.a, p.last, p.d_left, p.d_right);
+}
+
+// The outputs of the node `p` built from `l` and `r` at `gap` that go to global memory: split gap,
+// the leaf children's parent words, aggregate and 4-wide slot list; the internal children's parent
+// words and the cover diffs go through the callbacks (LDS at level 1). Returns the node index and the
+// reference-format child words (kLtChildLeafBit | sorted position for a leaf child).
+template <class InternalParent, class CoverAdd>
+CX_DEV uint32_t WriteNode(const Build& b, int n, const Climb& l, const Climb& r, const Climb& p, uint32_t gap,
+                          InternalParent&& internal_parent, CoverAdd&& cover_add, uint32_t& left, uint32_t& right)
+{
+    const uint32_t node = NodeOf(p, n);
+    if (node == 0)
     {
-        return false;  // the root (node 0) is finished
+        internal_parent(0u, kLtNone);
     }
-    const bool is_left_child = e.d_right > e.d_left;
-    const uint32_t gap = is_left_child ? e.last : e.a - 1;
-    if (uint64_t(gap) < block_lo || uint64_t(gap) + 1 >= block_hi)
-    {
-        const uint32_t k = atomicAdd(lds_top_count, 1u);
-        if (k < kTopsCap)
-        {
-            tops[k] = make_uint2(e.a, e.last);
-        }
-        return false;
-    }
-    const uint32_t far_end = is_left_child ? e.a : e.last;
-    const uint32_t other = __hip_atomic_exchange(b.scratch.u32(kLtScratchCounter) + gap, far_end, __ATOMIC_ACQ_REL,
-                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (other == kLtNone)
-    {
-        return false;  // the sibling finishes the parent
-    }
-    // The parent: range [pa, pb], split `gap`; the sibling's summary and its far boundary delta are
-    // the only loads.
-    const uint32_t pa = is_left_child ? e.a : other;
-    const uint32_t pb = is_left_child ? other : e.last;
-    const uint32_t left = pa == gap ? (kLtChildLeafBit | gap) : gap;
-    const uint32_t right = pb == gap + 1 ? (kLtChildLeafBit | (gap + 1)) : gap + 1;
-    const int d_far = is_left_child ? GapDelta(keys, int(pb), n) : GapDelta(keys, int(pa) - 1, n);
-    const Summary sibling = ElementSummary(b, is_left_child ? right : left);
-    const int d_left = is_left_child ? e.d_left : d_far;
-    const int d_right = is_left_child ? d_far : e.d_right;
-    uint32_t* parent_out = b.scratch.u32(kLtScratchParent);
+    left = l.a == l.last ? (kLtChildLeafBit | gap) : gap;
+    right = r.a == r.last ? (kLtChildLeafBit | (gap + 1)) : gap + 1;
     uint32_t* leaf_parent_out = b.scratch.u32(kLtScratchSortedLeafParent);
-    uint32_t node;
-    if (pa == 0 && pb == uint32_t(n - 1))
+    if ((left & kLtChildLeafBit) != 0)
     {
-        node = 0;
-        parent_out[0] = kLtNone;
+        leaf_parent_out[gap] = node;
     }
     else
     {
-        node = d_right > d_left ? pb : pa;
+        internal_parent(left, node);
     }
-    b.scratch.u32(kLtScratchLeft)[node] = left;
-    b.scratch.u32(kLtScratchRight)[node] = right;
-    ((left & kLtChildLeafBit) != 0 ? leaf_parent_out : parent_out)[left & ~kLtChildLeafBit] = node;
-    ((right & kLtChildLeafBit) != 0 ? leaf_parent_out : parent_out)[right & ~kLtChildLeafBit] = node;
+    if ((right & kLtChildLeafBit) != 0)
+    {
+        leaf_parent_out[gap + 1] = node;
+    }
+    else
+    {
+        internal_parent(right, node);
+    }
     b.scratch.split()[node] = gap;
-    // Range ends lie in this workgroup's block: no other workgroup of this launch touches them.
-    int32_t* diff = b.scratch.cover_diff();
-    __hip_atomic_fetch_add(&diff[pa], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    __hip_atomic_fetch_add(&diff[pb], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    e.own = is_left_child ? Union(e.own, sibling) : Union(sibling, e.own);
-    StoreAggregate(b, node, e.own);
-    e.a = pa;
-    e.last = pb;
-    e.d_left = d_left;
-    e.d_right = d_right;
-    return true;
+    cover_add(p.a, p.last);
+    StoreSummary(b.scratch.aggregates() + uint64_t(node) * 12, p.own);
+    b.scratch.quad()[node] = QuadOf(l.kid0, l.kid1, r.kid0, r.kid1);
+    return node;
 }
 
-// ---- level 1 in LDS ------------------------------------------------------------------------------
-// A level-1 workgroup keeps everything of its block of kTreeTile sorted leaves in LDS: the keys of
-// the block and its two neighbours, the leaf summaries (gathered once, in sorted order), the gap
-// slots and cover diffs, and the outputs of the nodes it builds (children, split, parent links,
-// aggregates), which it writes out with coalesced stores at the end. The climb is ClimbStep's.
-constexpr uint32_t kUnset = 0xFFFFFFFEu;  // a staged output the block did not write
+// How a climb step ended.
+enum : uint32_t
+{
+    kClimbOn = 0,   // the parent is built; the lane holds it and climbs on
+    kClimbRoot,     // the root is finished
+    kClimbOpen,     // the parent gap is on the block boundary
+    kClimbParked,   // arrived first at its parent gap
+};
+
+CX_DEV bool IsRoot(const Climb& e, int n)
+{
+    return e.a == 0 && e.last == uint32_t(n - 1);
+}
+
+// The parent gap of a non-root element and whether it lies inside the block [lo, hi).
+CX_DEV uint32_t ParentGap(const Climb& e, bool& is_left_child)
+{
+    is_left_child = e.d_right > e.d_left;
+    return is_left_child ? e.last : e.a - 1;
+}
+
+CX_DEV bool GapInside(uint32_t gap, uint64_t lo, uint64_t hi)
+{
+    return gap != kLtNone && uint64_t(gap) >= lo && uint64_t(gap) + 1 < hi;
+}
+
+// Appends an open element to the block's open list.
+CX_DEV void EmitOpen(const Build& b, uint32_t* list, uint32_t* lds_count, const Climb& e)
+{
+    const uint32_t k = atomicAdd(lds_count, 1u);
+    if (k < kOpenCap)
+    {
+        StoreClimb(list + uint64_t(k) * kOpenWords, e);
+    }
+    else
+    {
+        AtomicAddRelaxed(&b.scratch.ctl()[kCtlTreeOverflow], 1u);
+    }
+}
+
+// ---- level 1 ------------------------------------------------------------------------------------
+// A level-1 workgroup (one lane per leaf) keeps its block of kTreeTile sorted leaves in LDS: the gap
+// deltas of the block and its two neighbours, the leaf ids, the gap slots and cover diffs, the
+// children of the nodes it builds (written out with coalesced stores at the end), and one parked
+// summary per element position. An element that arrives first at its gap parks its summary at its
+// position p (leaf p or node p) for the sibling to read. Leaf p and node p never need the slot at
+// once: node p is built only after leaf p's parent (node p is that parent or an ancestor of it), and
+// leaf p's parked summary is read before its parent is built.
+//
+// The climb runs in stages of kClimbStage steps. Half the elements stop at every step, so after a
+// few steps most lanes of a wave idle while the wave still issues every instruction of a step; at
+// the end of a stage the elements still climbing park their summary and queue their range, and the
+// next stage hands them to the lowest lanes. Any element's record can be rebuilt from LDS (range,
+// the delta table, the parked summary, the children), so the open elements are found after the climb
+// from the slots (a parked element whose sibling did not arrive) and the two boundary tops, and are
+// written in position order. 39.4 KB of LDS: four workgroups per CU.
+constexpr uint32_t kUnset = 0xFFFFFFFEu;     // a staged output the block did not write
+constexpr uint32_t kResolved = 0xFFFFFFFDu;  // a gap slot whose parent is built
+constexpr uint32_t kClimbStage = kMaxClimb;
 
 struct Level1Lds
 {
-    int32_t delta[kTreeTile + 1];  // delta of the gaps [block_lo - 1, block_hi - 1]
-    float leaf[kTreeTile][12];     // summaries of the block's sorted leaves
-    float agg[kTreeTile][12];      // aggregates of the nodes built here (node - block_lo)
-    uint32_t slot[kTreeTile];
+    float park[kTreeTile][12];     // summary of the parked element at position lo + k
+    uint32_t slot[kTreeTile];      // far range end of the first arrival at gap lo + k
     int32_t diff[kTreeTile];
     uint32_t left[kTreeTile];
     uint32_t right[kTreeTile];
-    uint32_t split[kTreeTile];
-    uint32_t parent[kTreeTile];       // parent of internal node block_lo + k
-    uint32_t leaf_parent[kTreeTile];  // parent of sorted leaf block_lo + k
+    uint32_t leaf_id[kTreeTile];   // leaf index (sorted value) of sorted leaf block_lo + k
+    uint2 queue[kTreeTile];        // ranges of the elements climbing on; then the open elements by start
+    int8_t delta[kTreeTile + 4];   // delta of the gaps [block_lo - 1, block_hi - 1] (-1 .. 63)
+    uint2 top[2];                  // open elements whose parent gap is block_lo - 1 / block_hi - 1
+    uint32_t queue_count[2];
+    uint32_t wave_open[kTreeTile / kWave];
 };
-
-CX_DEV int GapDeltaLds(const Level1Lds& l, int g, int lo)
-{
-    return l.delta[g - (lo - 1)];
-}
 
 CX_DEV Summary LdsSummary(const float (&v)[12])
 {
+    const float4 a = *reinterpret_cast<const float4*>(&v[0]);
+    const float4 b = *reinterpret_cast<const float4*>(&v[4]);
+    const float4 c = *reinterpret_cast<const float4*>(&v[8]);
     Summary s;
-#pragma unroll
-    for (int f = 0; f < 6; ++f)
-    {
-        s.face[f] = v[f];
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-    {
-        s.lo[k] = v[6 + k];
-        s.hi[k] = v[9 + k];
-    }
+    s.face[0] = a.x, s.face[1] = a.y, s.face[2] = a.z, s.face[3] = a.w;
+    s.face[4] = b.x, s.face[5] = b.y, s.lo[0] = b.z, s.lo[1] = b.w;
+    s.lo[2] = c.x, s.hi[0] = c.y, s.hi[1] = c.z, s.hi[2] = c.w;
     return s;
 }
 
 CX_DEV void LdsStore(float (&v)[12], const Summary& s)
 {
-#pragma unroll
-    for (int f = 0; f < 6; ++f)
-    {
-        v[f] = s.face[f];
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-    {
-        v[6 + k] = s.lo[k];
-        v[9 + k] = s.hi[k];
-    }
+    *reinterpret_cast<float4*>(&v[0]) = make_float4(s.face[0], s.face[1], s.face[2], s.face[3]);
+    *reinterpret_cast<float4*>(&v[4]) = make_float4(s.face[4], s.face[5], s.lo[0], s.lo[1]);
+    *reinterpret_cast<float4*>(&v[8]) = make_float4(s.lo[2], s.hi[0], s.hi[1], s.hi[2]);
 }
 
-CX_DEV Summary Level1Summary(const Level1Lds& l, uint32_t element, uint32_t lo)
+// delta(g) from the table (g in [lo - 1, hi - 1]).
+CX_DEV int DeltaLds(const Level1Lds& l, uint32_t g, uint32_t lo)
 {
-    return (element & kLtChildLeafBit) != 0 ? LdsSummary(l.leaf[(element & ~kLtChildLeafBit) - lo])
-                                            : LdsSummary(l.agg[element - lo]);
+    return int(l.delta[g + 1 - lo]);
 }
 
-// ClimbStep for level 1: every arrival is in LDS; the climbing lane carries its element's summary.
-CX_DEV bool ClimbStepLds(Level1Lds& l, int n, Climb& e, uint32_t lo, uint32_t hi, uint2* tops,
-                         uint32_t* lds_top_count)
+// The child word of a reference-format child (kLtChildLeafBit | sorted position, or a node index).
+CX_DEV uint32_t ChildWordLds(const Level1Lds& l, uint32_t child, uint32_t lo)
 {
-    const uint32_t a = e.a;
-    const uint32_t last = e.last;
-    if (a == 0 && last == uint32_t(n - 1))
+    return (child & kLtChildLeafBit) != 0 ? kLtChildLeafBit | l.leaf_id[(child & ~kLtChildLeafBit) - lo] : child;
+}
+
+// The record of the element [a, last] of the block, rebuilt from LDS (its summary parked).
+CX_DEV Climb ElementLds(const Level1Lds& l, uint32_t a, uint32_t last, uint32_t lo)
+{
+    Climb e;
+    e.a = a;
+    e.last = last;
+    e.d_left = DeltaLds(l, a - 1, lo);
+    e.d_right = DeltaLds(l, last, lo);
+    const uint32_t id = ElementOf(a, last, e.d_left, e.d_right);
+    const uint32_t at = (id & ~kLtChildLeafBit) - lo;
+    e.own = LdsSummary(l.park[at]);
+    if ((id & kLtChildLeafBit) != 0)
     {
-        return false;  // the root (node 0) is finished
-    }
-    const bool is_left_child = e.d_right > e.d_left;
-    const uint32_t gap = is_left_child ? last : a - 1;
-    if (gap < lo || gap + 1 >= hi || (a == 0 && !is_left_child))
-    {
-        const uint32_t k = atomicAdd(lds_top_count, 1u);
-        if (k < kTopsCap)
-        {
-            tops[k] = make_uint2(a, last);
-        }
-        return false;
-    }
-    const uint32_t far_end = is_left_child ? a : last;
-    const uint32_t other =
-        __hip_atomic_exchange(&l.slot[gap - lo], far_end, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (other == kLtNone)
-    {
-        return false;  // the sibling finishes the parent
-    }
-    const uint32_t pa = is_left_child ? a : other;
-    const uint32_t pb = is_left_child ? other : last;
-    const uint32_t left = pa == gap ? (kLtChildLeafBit | gap) : gap;
-    const uint32_t right = pb == gap + 1 ? (kLtChildLeafBit | (gap + 1)) : gap + 1;
-    // The sibling's far boundary delta and summary are the only reads.
-    const int d_far = is_left_child ? GapDeltaLds(l, int(pb), int(lo)) : GapDeltaLds(l, int(pa) - 1, int(lo));
-    const Summary sibling = Level1Summary(l, is_left_child ? right : left, lo);
-    const int d_left = is_left_child ? e.d_left : d_far;
-    const int d_right = is_left_child ? d_far : e.d_right;
-    uint32_t node;
-    if (pa == 0 && pb == uint32_t(n - 1))
-    {
-        node = 0;
-        l.parent[0] = kLtNone;  // the block holds the whole tree (lo = 0)
+        e.kid0 = kLtChildLeafBit | l.leaf_id[at];
+        e.kid1 = kLtNone;
     }
     else
     {
-        node = d_right > d_left ? pb : pa;
+        e.kid0 = ChildWordLds(l, l.left[at], lo);
+        e.kid1 = ChildWordLds(l, l.right[at], lo);
     }
-    l.left[node - lo] = left;
-    l.right[node - lo] = right;
-    l.split[node - lo] = gap;
-    ((left & kLtChildLeafBit) != 0 ? l.leaf_parent : l.parent)[(left & ~kLtChildLeafBit) - lo] = node;
-    ((right & kLtChildLeafBit) != 0 ? l.leaf_parent : l.parent)[(right & ~kLtChildLeafBit) - lo] = node;
-    atomicAdd(&l.diff[pa - lo], 1);
-    atomicAdd(&l.diff[pb - lo], -1);
-    e.own = is_left_child ? Union(e.own, sibling) : Union(sibling, e.own);
-    LdsStore(l.agg[node - lo], e.own);
-    e.a = pa;
-    e.last = pb;
-    e.d_left = d_left;
-    e.d_right = d_right;
-    return true;
+    return e;
 }
 
-CX_DEV void TreeLevel1(const Build& b, int n, uint32_t lo, uint32_t hi, uint2* tops, uint32_t* lds_top_count,
+CX_DEV void ParkLds(Level1Lds& l, const Climb& e, uint32_t lo)
+{
+    const uint32_t id = ElementOf(e.a, e.last, e.d_left, e.d_right);
+    LdsStore(l.park[(id & ~kLtChildLeafBit) - lo], e.own);
+}
+
+// One level-1 climb step.
+CX_DEV uint32_t ClimbStepLds(Level1Lds& l, int n, Climb& e, uint32_t lo, uint32_t hi)
+{
+    if (IsRoot(e, n))
+    {
+        ParkLds(l, e, lo);  // node 0's aggregate, for the write-out
+        return kClimbRoot;
+    }
+    bool is_left_child;
+    const uint32_t gap = ParentGap(e, is_left_child);
+    if (!GapInside(gap, lo, hi))
+    {
+        ParkLds(l, e, lo);
+        l.top[is_left_child ? 1 : 0] = make_uint2(e.a, e.last);
+        return kClimbOpen;
+    }
+    // Park the summary at the element's position before arriving; harmless when the sibling is
+    // already there. The release and acquire fences order the LDS accesses only (the parked summary,
+    // children and leaf ids all live in LDS), so the exchange does not wait for global stores.
+    ParkLds(l, e, lo);
+    const uint32_t far_end = is_left_child ? e.a : e.last;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    const uint32_t other =
+        __hip_atomic_exchange(&l.slot[gap - lo], far_end, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (other == kLtNone)
+    {
+        return kClimbParked;  // the sibling finishes the parent
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    l.slot[gap - lo] = kResolved;
+    const Climb s = is_left_child ? ElementLds(l, gap + 1, other, lo) : ElementLds(l, other, gap, lo);
+    const Climb& lc = is_left_child ? e : s;
+    const Climb& rc = is_left_child ? s : e;
+    const Climb parent = ParentOf(lc, rc);
+    // The node's children in LDS (indexed by the node, which lies in the block) and its cover diffs;
+    // everything else about it is written out after the climb.
+    const uint32_t node = NodeOf(parent, n);
+    l.left[node - lo] = lc.a == lc.last ? (kLtChildLeafBit | gap) : gap;
+    l.right[node - lo] = rc.a == rc.last ? (kLtChildLeafBit | (gap + 1)) : gap + 1;
+    atomicAdd(&l.diff[parent.a - lo], 1);
+    atomicAdd(&l.diff[parent.last - lo], -1);
+    e = parent;
+    return kClimbOn;
+}
+
+CX_DEV void TreeLevel1(const Build& b, int n, uint32_t lo, uint32_t hi, uint32_t* open, uint32_t* open_count,
                        Level1Lds& l)
 {
     const uint32_t t = threadIdx.x;
     const uint32_t* keys = b.scratch.keys(0);
     const uint32_t* values = b.scratch.values(0);
-    // delta(g) of the gaps g = lo - 1 .. hi - 1 (-1 outside [0, n - 1)), as GapDelta.
+    // delta(g) of the gaps g = lo - 1 .. hi - 1 (-1 outside [0, n - 1)).
     for (uint32_t k = t; k < kTreeTile + 1; k += blockDim.x)
     {
         const int64_t g = int64_t(lo) - 1 + k;
@@ -848,185 +927,400 @@ CX_DEV void TreeLevel1(const Build& b, int n, uint32_t lo, uint32_t hi, uint2* t
             const uint32_t k1 = keys[g + 1];
             d = k0 == k1 ? 32 + int(Clz(uint32_t(g) ^ uint32_t(g + 1))) : int(Clz(k0 ^ k1));
         }
-        l.delta[k] = d;
+        l.delta[k] = int8_t(d);
     }
-    for (uint32_t k = t; k < kTreeTile; k += blockDim.x)
-    {
-        l.slot[k] = kLtNone;
-        l.diff[k] = 0;
-        l.left[k] = kUnset;
-        l.parent[k] = kUnset;
-        l.leaf_parent[k] = kUnset;
-    }
-    // The leaf summaries in sorted order: 3 x 16-byte loads per leaf, all issued up front.
-    for (uint32_t k = t; k < kTreeTile; k += blockDim.x)
-    {
-        if (lo + k < uint32_t(n))
-        {
-            const float4* src = reinterpret_cast<const float4*>(b.leaves[values[lo + k]].summary.face);
-            const float4 p = src[0];
-            const float4 q = src[1];
-            const float4 r = src[2];
-            float* d = l.leaf[k];
-            d[0] = p.x, d[1] = p.y, d[2] = p.z, d[3] = p.w;
-            d[4] = q.x, d[5] = q.y, d[6] = q.z, d[7] = q.w;
-            d[8] = r.x, d[9] = r.y, d[10] = r.z, d[11] = r.w;
-        }
-    }
-    __syncthreads();
-    uint32_t next = t;
+    // One leaf per lane (blockDim.x == kTreeTile): its id and summary.
+    const uint32_t leaves = min(uint32_t(n) - lo, kTreeTile);
+    const bool has_leaf = t < leaves;
+    const uint32_t leaf = has_leaf ? values[lo + t] : 0u;
     Climb e;
-    bool live = false;
-    while (true)
+    if (has_leaf)
     {
-        if (!live)
-        {
-            if (next >= kTreeTile || lo + next >= uint32_t(n))
-            {
-                break;
-            }
-            e.a = lo + next;
-            e.last = e.a;
-            e.d_left = l.delta[next];       // gap e.a - 1
-            e.d_right = l.delta[next + 1];  // gap e.a
-            e.own = LdsSummary(l.leaf[next]);
-            next += blockDim.x;
-        }
-        live = ClimbStepLds(l, n, e, lo, hi, tops, lds_top_count);
+        e.own = LoadSummary(b.leaves[leaf].summary.face);
+    }
+    l.slot[t] = kLtNone;
+    l.diff[t] = 0;
+    l.left[t] = kUnset;
+    l.leaf_id[t] = leaf;
+    if (t < 2)
+    {
+        l.top[t] = make_uint2(kLtNone, kLtNone);
+        l.queue_count[t] = 0;
     }
     __syncthreads();
-    // Coalesced write-out of the block.
-    uint32_t* slots = b.scratch.u32(kLtScratchCounter);
-    int32_t* diff = b.scratch.cover_diff();
-    uint32_t* left_out = b.scratch.u32(kLtScratchLeft);
-    uint32_t* right_out = b.scratch.u32(kLtScratchRight);
-    uint32_t* split_out = b.scratch.split();
-    uint32_t* parent_out = b.scratch.u32(kLtScratchParent);
-    uint32_t* leaf_parent_out = b.scratch.u32(kLtScratchSortedLeafParent);
-    float* agg_out = b.scratch.aggregates();
-    for (uint32_t k = t; k < kTreeTile; k += blockDim.x)
+    bool climbing = has_leaf;
+    if (has_leaf)
     {
-        const uint32_t g = lo + k;
-        if (g + 1 < uint32_t(n))
+        e.a = lo + t;
+        e.last = e.a;
+        e.d_left = DeltaLds(l, e.a - 1, lo);
+        e.d_right = DeltaLds(l, e.a, lo);
+        e.kid0 = kLtChildLeafBit | leaf;
+        e.kid1 = kLtNone;
+    }
+    for (uint32_t stage = 0; stage < kMaxClimb / kClimbStage; ++stage)
+    {
+        uint32_t* count = &l.queue_count[stage & 1u];
+        if (climbing)
         {
-            slots[g] = l.slot[k];
+            uint32_t end = kClimbOn;
+            for (uint32_t k = 0; k < kClimbStage && end == kClimbOn; ++k)
+            {
+                end = ClimbStepLds(l, n, e, lo, hi);
+            }
+            if (end == kClimbOn)
+            {
+                ParkLds(l, e, lo);
+                l.queue[atomicAdd(count, 1u)] = make_uint2(e.a, e.last);
+            }
         }
-        if (g < uint32_t(n))
+        __syncthreads();
+        const uint32_t survivors = *count;
+        if (survivors == 0)
         {
-            diff[g] = l.diff[k];
+            break;
         }
-        if (l.left[k] != kUnset)
+        if (stage + 1 == kMaxClimb / kClimbStage)
         {
-            left_out[g] = l.left[k];
-            right_out[g] = l.right[k];
-            split_out[g] = l.split[k];
+            if (t == 0)
+            {
+                AtomicAddRelaxed(&b.scratch.ctl()[kCtlTreeOverflow], 1u);  // never expected
+            }
+            break;
         }
-        if (l.parent[k] != kUnset)
+        climbing = t < survivors;
+        if (climbing)
         {
-            parent_out[g] = l.parent[k];
+            const uint2 range = l.queue[t];
+            e = ElementLds(l, range.x, range.y, lo);
         }
-        if (l.leaf_parent[k] != kUnset)
+        if (t == 0)
         {
-            leaf_parent_out[g] = l.leaf_parent[k];
+            l.queue_count[(stage + 1) & 1u] = 0;
+        }
+        __syncthreads();
+    }
+    // Open elements: parked at a gap the block did not resolve, or the tops. queue[k] = the open
+    // element starting at position lo + k.
+    l.queue[t] = make_uint2(kLtNone, kLtNone);
+    __syncthreads();
+    if (lo + t + 1 < uint32_t(n) && t + 1 < kTreeTile)
+    {
+        const uint32_t g = lo + t;
+        const uint32_t v = l.slot[t];
+        if (v != kLtNone && v != kResolved)
+        {
+            const uint2 range = v <= g ? make_uint2(v, g) : make_uint2(g + 1, v);
+            l.queue[range.x - lo] = range;
         }
     }
-    // aggregates: 12 floats per node, as float4 triples over consecutive threads
+    if (t < 2 && l.top[t].x != kLtNone)
+    {
+        l.queue[l.top[t].x - lo] = l.top[t];
+    }
+    __syncthreads();
+    const uint2 mine = l.queue[t];
+    const bool is_open = mine.x != kLtNone;
+    const uint64_t ballot = Ballot(is_open);
+    const uint32_t wave = t / kWave;
+    if (Lane() == 0)
+    {
+        l.wave_open[wave] = uint32_t(__popcll(ballot));
+    }
+    __syncthreads();
+    uint32_t rank = uint32_t(__popcll(ballot & ((uint64_t(1) << Lane()) - 1)));
+    for (uint32_t w = 0; w < wave; ++w)
+    {
+        rank += l.wave_open[w];
+    }
+    if (is_open && rank < kOpenCap)
+    {
+        StoreClimb(open + uint64_t(rank) * kOpenWords, ElementLds(l, mine.x, mine.y, lo));
+    }
+    if (t == 0)
+    {
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < kTreeTile / kWave; ++w)
+        {
+            total += l.wave_open[w];
+        }
+        if (total > kOpenCap)
+        {
+            AtomicAddRelaxed(&b.scratch.ctl()[kCtlTreeOverflow], 1u);
+        }
+        *open_count = min(total, kOpenCap);
+    }
+    // Write-out of the nodes the block built, node lo + t by lane t: children, child words, split gap
+    // (the left child's index: a leaf at the split, or a left child numbered by its range end), the
+    // 4-wide slot list, the parent words of the children and the aggregate (parked at the node's
+    // position: node p is the last element to use position p).
+    int32_t* diff = b.scratch.cover_diff();
+    const uint32_t g = lo + t;
+    if (g < uint32_t(n))
+    {
+        diff[g] = l.diff[t];
+    }
+    const uint32_t left = l.left[t];
+    const bool built = left != kUnset;
+    if (built)
+    {
+        const uint32_t right = l.right[t];
+        b.scratch.u32(kLtScratchLeft)[g] = left;
+        b.scratch.u32(kLtScratchRight)[g] = right;
+        const uint32_t kid0 = ChildWordLds(l, left, lo);
+        const uint32_t kid1 = ChildWordLds(l, right, lo);
+        b.scratch.kids()[g] = make_uint2(kid0, kid1);
+        b.scratch.split()[g] = left & ~kLtChildLeafBit;
+        uint32_t l0 = kid0;
+        uint32_t l1 = kLtNone;
+        uint32_t r0 = kid1;
+        uint32_t r1 = kLtNone;
+        uint32_t* parent_out = b.scratch.u32(kLtScratchParent);
+        uint32_t* leaf_parent_out = b.scratch.u32(kLtScratchSortedLeafParent);
+        if ((left & kLtChildLeafBit) == 0)
+        {
+            l0 = ChildWordLds(l, l.left[left - lo], lo);
+            l1 = ChildWordLds(l, l.right[left - lo], lo);
+            parent_out[left] = g;
+        }
+        else
+        {
+            leaf_parent_out[left & ~kLtChildLeafBit] = g;
+        }
+        if ((right & kLtChildLeafBit) == 0)
+        {
+            r0 = ChildWordLds(l, l.left[right - lo], lo);
+            r1 = ChildWordLds(l, l.right[right - lo], lo);
+            parent_out[right] = g;
+        }
+        else
+        {
+            leaf_parent_out[right & ~kLtChildLeafBit] = g;
+        }
+        b.scratch.quad()[g] = QuadOf(l0, l1, r0, r1);
+        if (g == 0)
+        {
+            parent_out[0] = kLtNone;  // the root (the block holds the whole tree)
+        }
+    }
+    // Aggregates, as float4 triples over consecutive lanes.
+    float* agg_out = b.scratch.aggregates();
     for (uint32_t q = t; q < kTreeTile * 3; q += blockDim.x)
     {
         const uint32_t k = q / 3;
         if (l.left[k] != kUnset)
         {
-            const float* v = &l.agg[k][(q % 3) * 4];
+            const float* v = &l.park[k][(q % 3) * 4];
             reinterpret_cast<float4*>(agg_out + uint64_t(lo + k) * 12)[q % 3] = make_float4(v[0], v[1], v[2], v[3]);
         }
     }
 }
 
-// Level 1: the workgroup's own 512 sorted leaves. Level L > 1: the tops of its 32 sub-blocks.
 template <class Set>
-__global__ void __launch_bounds__(1024) TreeLevelKernel(Set set, uint32_t level)
+__global__ void __launch_bounds__(kTreeThreads) TreeLevel1Kernel(Set set)
+{
+    const Build& b = set.Get();
+    if (blockIdx.x >= TreeBlocks(b.leaf_capacity, 1) || !Building(b))
+    {
+        return;
+    }
+    __shared__ Level1Lds lds;
+    const int n = int(b.scratch.ctl()[kCtlLeafCount]);
+    const uint32_t lo = blockIdx.x * kTreeTile;
+    uint32_t* count = b.scratch.open_count(1) + blockIdx.x;
+    if (n >= 2 && lo < uint32_t(n))
+    {
+        TreeLevel1(b, n, lo, lo + kTreeTile, b.scratch.open(1) + uint64_t(blockIdx.x) * kOpenCap * kOpenWords,
+                   count, lds);
+    }
+    else if (threadIdx.x == 0)
+    {
+        *count = 0;
+    }
+}
+
+
+// ---- levels 2 and up ----------------------------------------------------------------------------
+// A level-L workgroup starts from the open elements of its 32 sub-blocks. Arrivals go through an LDS
+// hash table keyed by gap (the first arrival of a gap claims the entry; linear probing) whose value
+// is the parked element's far range end and pool index; the parked element's record is in the LDS
+// pool. Every arrival writes its record to a fresh pool entry before exchanging (so the sibling can
+// read it once the exchange publishes it); the second arrival builds the parent.
+constexpr uint32_t kUpperThreads = 1024;
+constexpr uint32_t kHashEntries = 2048;  // power of two
+constexpr uint32_t kPoolEntries = 1536;
+constexpr uint32_t kHashEmpty = 0xFFFFFFFFu;
+constexpr uint64_t kArrivalNone = ~uint64_t(0);      // no arrival yet; else the first's pool slot
+constexpr uint64_t kArrivalResolved = uint64_t(1) << 62;  // | left child's slot << 16 | right child's slot
+
+struct UpperLds
+{
+    uint32_t key[kHashEntries];
+    uint64_t arrival[kHashEntries];
+    uint32_t pool[kPoolEntries][kOpenWords];
+    uint32_t offsets[kTreeFanout + 1];
+    uint32_t pool_count;
+    uint32_t open_count;
+};
+
+CX_DEV uint32_t HashFindOrInsert(UpperLds& l, uint32_t gap)
+{
+    uint32_t h = (gap * 0x9E3779B1u) >> (32 - 11);
+    static_assert(kHashEntries == 1u << 11, "hash bits");
+    for (uint32_t probe = 0; probe < kHashEntries; ++probe)
+    {
+        const uint32_t k = atomicCAS(&l.key[h], kHashEmpty, gap);
+        if (k == kHashEmpty || k == gap)
+        {
+            return h;
+        }
+        h = (h + 1) & (kHashEntries - 1);
+    }
+    return kHashEmpty;
+}
+
+template <class Set>
+__global__ void __launch_bounds__(kUpperThreads) TreeUpperKernel(Set set, uint32_t level)
 {
     const Build& b = set.Get();
     if (level > TreeLevels(b.leaf_capacity) || blockIdx.x >= TreeBlocks(b.leaf_capacity, level) || !Building(b))
     {
         return;
     }
-    __shared__ uint32_t lds_top_count;
-    __shared__ uint32_t lds_offsets[kTreeFanout + 1];
-    __shared__ Level1Lds lds_level1;
+    __shared__ UpperLds l;
     const uint32_t t = threadIdx.x;
     const int n = int(b.scratch.ctl()[kCtlLeafCount]);
     const uint64_t tile = TreeTile(level);
     const uint64_t block_lo = uint64_t(blockIdx.x) * tile;
     const uint64_t block_hi = block_lo + tile;
-    uint2* tops = b.scratch.tops(level) + uint64_t(blockIdx.x) * kTopsCap;
-    if (t == 0)
+    for (uint32_t k = t; k < kHashEntries; k += kUpperThreads)
     {
-        lds_top_count = 0;
+        l.key[k] = kHashEmpty;
+        l.arrival[k] = kArrivalNone;
+    }
+    const uint32_t child_first = blockIdx.x * kTreeFanout;
+    const uint32_t child_blocks = TreeBlocks(b.leaf_capacity, level - 1);
+    if (t < kWave)
+    {
+        // Offsets of the sub-blocks' open lists: one count per lane, a wave prefix sum.
+        const uint32_t child = child_first + t;
+        const uint32_t count = t < kTreeFanout && child < child_blocks ? b.scratch.open_count(level - 1)[child] : 0u;
+        uint32_t inclusive = count;
+#pragma unroll
+        for (int offset = 1; offset < kWave; offset <<= 1)
+        {
+            const uint32_t up = __shfl_up(inclusive, offset, kWave);
+            inclusive += Lane() >= uint32_t(offset) ? up : 0u;
+        }
+        if (t <= kTreeFanout)
+        {
+            l.offsets[t] = inclusive - count;
+        }
+        if (t == 0)
+        {
+            l.pool_count = 0;
+            l.open_count = 0;
+        }
     }
     __syncthreads();
+    uint32_t* open = b.scratch.open(level) + uint64_t(blockIdx.x) * kOpenCap * kOpenWords;
     if (n >= 2 && block_lo < uint64_t(n))
     {
-        if (level == 1)
+        const uint32_t total = l.offsets[kTreeFanout];
+        const uint32_t* child_open = b.scratch.open(level - 1);
+        for (uint32_t next = t; next < total; next += kUpperThreads)
         {
-            TreeLevel1(b, n, uint32_t(block_lo), uint32_t(block_hi), tops, &lds_top_count, lds_level1);
-        }
-        else
-        {
-            const uint32_t child_first = blockIdx.x * kTreeFanout;
-            const uint32_t child_blocks = TreeBlocks(b.leaf_capacity, level - 1);
-            const uint32_t* counts = b.scratch.tops_count(level - 1);
-            if (t == 0)
+            uint32_t c = 0;
+            while (l.offsets[c + 1] <= next)
             {
-                uint32_t sum = 0;
-                for (uint32_t c = 0; c < kTreeFanout; ++c)
-                {
-                    lds_offsets[c] = sum;
-                    const uint32_t child = child_first + c;
-                    sum += child < child_blocks ? min(counts[child], kTopsCap) : 0u;
-                }
-                lds_offsets[kTreeFanout] = sum;
+                ++c;
             }
-            __syncthreads();
-            const uint2* child_tops = b.scratch.tops(level - 1);
-            const uint32_t total = lds_offsets[kTreeFanout];
-            uint32_t next = t;
-            Climb e;
-            bool live = false;
-            while (true)
+            Climb e = LoadClimb(child_open + (uint64_t(child_first + c) * kOpenCap + (next - l.offsets[c])) * kOpenWords);
+            // The climb touches LDS only; the nodes it builds are written out after it.
+            for (uint32_t step = 0;; ++step)
             {
-                if (!live)
+                if (IsRoot(e, n))
                 {
-                    if (next >= total)
-                    {
-                        break;
-                    }
-                    uint32_t c = 0;
-                    while (lds_offsets[c + 1] <= next)
-                    {
-                        ++c;
-                    }
-                    const uint2 top = child_tops[uint64_t(child_first + c) * kTopsCap + (next - lds_offsets[c])];
-                    const uint32_t* keys = b.scratch.keys(0);
-                    e.a = top.x;
-                    e.last = top.y;
-                    e.d_left = GapDelta(keys, int(e.a) - 1, n);
-                    e.d_right = GapDelta(keys, int(e.last), n);
-                    e.own = ElementSummary(b, ElementOf(e.a, e.last, e.d_left, e.d_right));
-                    next += blockDim.x;
+                    break;
                 }
-                live = ClimbStep(b, n, e, block_lo, block_hi, tops, &lds_top_count);
+                if (step == kMaxClimb)
+                {
+                    AtomicAddRelaxed(&b.scratch.ctl()[kCtlTreeOverflow], 1u);  // never expected
+                    break;
+                }
+                bool is_left_child;
+                const uint32_t gap = ParentGap(e, is_left_child);
+                if (!GapInside(gap, block_lo, block_hi))
+                {
+                    EmitOpen(b, open, &l.open_count, e);
+                    break;
+                }
+                const uint32_t entry = HashFindOrInsert(l, gap);
+                const uint32_t slot = atomicAdd(&l.pool_count, 1u);
+                if (entry == kHashEmpty || slot >= kPoolEntries)
+                {
+                    AtomicAddRelaxed(&b.scratch.ctl()[kCtlTreeOverflow], 1u);
+                    break;
+                }
+                StoreClimb(l.pool[slot], e);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+                const uint64_t other = __hip_atomic_exchange(&l.arrival[entry], uint64_t(slot), __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (other == kArrivalNone)
+                {
+                    break;  // parked: the sibling finishes the parent
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+                // Resolved: the pool slots of the left and the right child, for the write-out.
+                const uint32_t first = uint32_t(other);
+                const uint32_t left_slot = is_left_child ? slot : first;
+                const uint32_t right_slot = is_left_child ? first : slot;
+                l.arrival[entry] = kArrivalResolved | (uint64_t(left_slot) << 16) | right_slot;
+                const Climb sibling = LoadClimb(l.pool[first]);
+                e = is_left_child ? ParentOf(e, sibling) : ParentOf(sibling, e);
             }
         }
     }
     __syncthreads();
+    // Write-out: every node built here (from its children's pool records, in parallel), and the
+    // parked elements whose sibling did not arrive, which stay open.
+    uint32_t* left_out = b.scratch.u32(kLtScratchLeft);
+    uint32_t* right_out = b.scratch.u32(kLtScratchRight);
+    uint32_t* parent_out = b.scratch.u32(kLtScratchParent);
+    uint2* kids_out = b.scratch.kids();
+    int32_t* diff = b.scratch.cover_diff();
+    for (uint32_t k = t; k < kHashEntries; k += kUpperThreads)
+    {
+        const uint64_t v = l.arrival[k];
+        if (v == kArrivalNone)
+        {
+            continue;
+        }
+        if ((v & kArrivalResolved) == 0)
+        {
+            EmitOpen(b, open, &l.open_count, LoadClimb(l.pool[uint32_t(v)]));
+            continue;
+        }
+        const Climb lc = LoadClimb(l.pool[uint32_t(v >> 16) & 0xFFFFu]);
+        const Climb rc = LoadClimb(l.pool[uint32_t(v) & 0xFFFFu]);
+        const Climb parent = ParentOf(lc, rc);
+        uint32_t left;
+        uint32_t right;
+        const uint32_t node = WriteNode(
+            b, n, lc, rc, parent, l.key[k], [&](uint32_t child, uint32_t nd) { parent_out[child] = nd; },
+            [&](uint32_t pa, uint32_t pb) {
+                __hip_atomic_fetch_add(&diff[pa], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __hip_atomic_fetch_add(&diff[pb], -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            },
+            left, right);
+        left_out[node] = left;
+        right_out[node] = right;
+        kids_out[node] = make_uint2(parent.kid0, parent.kid1);
+    }
+    __syncthreads();
     if (t == 0)
     {
-        b.scratch.tops_count(level)[blockIdx.x] = lds_top_count;
-        if (lds_top_count > kTopsCap)
-        {
-            b.state->pad0[0] = 1;  // never expected: tops are bounded by twice the tree depth
-        }
+        b.scratch.open_count(level)[blockIdx.x] = min(l.open_count, kOpenCap);
     }
 }
 
@@ -1316,7 +1610,10 @@ __global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Set set)
     else if (n >= 2)
     {
         node_count = node_total;
-        if (node_count > b.node_capacity)
+        // More 4-wide nodes than the capacity (the reference's overflow), or a tree level whose
+        // open list or LDS tables overflowed (never expected: both hold twice the tree depth per
+        // block or more): the tree is left empty with the overflow flag.
+        if (node_count > b.node_capacity || ctl[kCtlTreeOverflow] != 0)
         {
             overflow = 1;
         }
@@ -1336,117 +1633,112 @@ __global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Set set)
 }
 
 // ------------------------------------------------------------------------------------------------
-// wide (caelix_light_bvh_wide.hlsl:10-100), one lane per 4-wide node: the reference's lanes are the
-// binary nodes, of which the odd-depth half returns at once; here the flags scan lists the
-// even-depth ones, so every lane builds a node.
+// wide (caelix_light_bvh_wide.hlsl:10-100): the reference's lanes are the binary nodes, of which
+// the odd-depth half returns at once. Here the flags scan lists the even-depth ones, two lanes build
+// one 4-wide node (lane h: slots 2h and 2h + 1) from the slot list the tree build wrote for its
+// binary node, and a workgroup's kWideRecords consecutive nodes are staged in LDS and written out
+// with coalesced stores. Three dependent rounds of loads: the binary node, its slot words, then per
+// slot the leaf record or the binary aggregate and the 4-wide index. The parent words of a node (bytes 208..215) belong to the lane that builds
+// its parent, so the write-out covers bytes 0..207 of each record.
+constexpr uint32_t kWideThreads = 256;
+constexpr uint32_t kWideRecords = kWideThreads / 2;
+constexpr uint32_t kWideRecordChunks = 13;  // 16-byte chunks of bytes 0..207
+constexpr uint32_t kWideRecordFloats = kWideRecordChunks * 4;
+
 template <class Set>
-__global__ void __launch_bounds__(kBvhThreads) WideKernel(Set set)
+__global__ void __launch_bounds__(kWideThreads) WideKernel(Set set)
 {
     const Build& b = set.Get();
-    const uint32_t w = blockIdx.x * kBvhThreads + threadIdx.x;
-    if (w + 1 >= b.leaf_capacity || !Building(b))
+    if (!Building(b))
     {
         return;
     }
-    if (w >= b.scratch.ctl()[kCtlWideNodes] || w >= b.node_capacity)
+    const uint32_t total = min(b.scratch.ctl()[kCtlWideNodes], b.node_capacity);
+    const uint32_t w0 = blockIdx.x * kWideRecords;
+    if (w0 >= total)
     {
         return;
     }
-    const uint32_t i = b.scratch.wide_source()[w];  // the binary node (an even-depth one)
-    const uint32_t* wide = b.scratch.u32(kLtScratchWide);
-    const uint32_t* left = b.scratch.u32(kLtScratchLeft);
-    const uint32_t* right = b.scratch.u32(kLtScratchRight);
-    const uint32_t* values = b.scratch.values(0);
-    // Every load is issued before the first store, in four dependent rounds: the children, the
-    // grandchildren, the leaf ids / wide indices of the slots, the slot summaries.
-    const uint32_t children[2] = {left[i], right[i]};
-    uint32_t grand[4];
-#pragma unroll
-    for (int c = 0; c < 2; ++c)
+    __shared__ float4 lds_records[kWideRecords * kWideRecordChunks];
+    const uint32_t r = threadIdx.x >> 1;
+    const uint32_t h = threadIdx.x & 1u;
+    const uint32_t w = w0 + r;
+    if (w < total)
     {
-        const bool inner = (children[c] & kLtChildLeafBit) == 0;
-        const uint32_t at = inner ? children[c] : i;  // node i stands in for a leaf child
-        const uint32_t l = left[at];
-        const uint32_t r = right[at];
-        grand[2 * c] = inner ? l : children[c];
-        grand[2 * c + 1] = inner ? r : kLtNone;
-    }
-    // The slots in order: a leaf child takes one, an internal child its two children.
-    const bool inner0 = (children[0] & kLtChildLeafBit) == 0;
-    const bool inner1 = (children[1] & kLtChildLeafBit) == 0;
-    const uint32_t count = 2u + uint32_t(inner0) + uint32_t(inner1);
-    const uint32_t slots[4] = {grand[0], inner0 ? grand[1] : grand[2], inner0 ? grand[2] : grand[3],
-                               inner0 ? grand[3] : kLtNone};
-    // Leaf ids and wide indices, loaded unconditionally (index 0 stands in for the other kind).
-    uint32_t child_words[4] = {kLtNone, kLtNone, kLtNone, kLtNone};
+        const uint32_t i = b.scratch.wide_source()[w];  // the binary node (an even-depth one)
+        const uint2 words = reinterpret_cast<const uint2*>(b.scratch.quad() + i)[h];
+        const uint32_t slot_word[2] = {words.x, words.y};
+        const uint32_t* wide = b.scratch.u32(kLtScratchWide);
+        // All loads first: per slot the 4-wide index (internal) and the summary.
+        uint32_t child_word[2];
+        Summary summary[2];
 #pragma unroll
-    for (uint32_t s = 0; s < 4; ++s)
-    {
-        const bool used = s < count;
-        const bool leaf = (slots[s] & kLtChildLeafBit) != 0;
-        const uint32_t leaf_id = values[used && leaf ? slots[s] & ~kLtChildLeafBit : 0u];
-        const uint32_t wide_id = wide[used && !leaf ? slots[s] : 0u];
-        if (used)
+        for (uint32_t j = 0; j < 2; ++j)
         {
-            child_words[s] = leaf ? kLtChildLeafBit | leaf_id : wide_id;
-        }
-    }
-    float faces[24];
-    float bounds[24];
-#pragma unroll
-    for (uint32_t s = 0; s < 4; ++s)
-    {
-        const bool used = s < count;
-        const bool leaf = (slots[s] & kLtChildLeafBit) != 0;
-        const float* p = used && leaf ? b.leaves[child_words[s] & ~kLtChildLeafBit].summary.face
-                                      : b.scratch.aggregates() + (used ? uint64_t(slots[s]) * 12 : 0u);
-        Summary summary = LoadSummary(p);
-        if (!used)
-        {
-            summary = EmptySummary();
-        }
-#pragma unroll
-        for (int f = 0; f < 6; ++f)
-        {
-            faces[6 * s + f] = summary.face[f];
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-        {
-            bounds[3 * s + k] = summary.lo[k];
-            bounds[12 + 3 * s + k] = summary.hi[k];
-        }
-    }
-    LightBvhNode* node = b.nodes + w;
-#pragma unroll
-    for (uint32_t s = 0; s < 4; ++s)
-    {
-        if (s < count)
-        {
-            if ((slots[s] & kLtChildLeafBit) != 0)
+            const uint32_t word = slot_word[j];
+            const bool used = word != kLtNone;
+            const bool leaf = used && (word & kLtChildLeafBit) != 0;
+            child_word[j] = used && !leaf ? wide[word] : word;
+            const float* p = leaf ? b.leaves[word & ~kLtChildLeafBit].summary.face
+                                  : b.scratch.aggregates() + (used ? uint64_t(word) * 12 : 0u);
+            summary[j] = LoadSummary(p);
+            if (!used)
             {
-                b.leaves[child_words[s] & ~kLtChildLeafBit].parent = (w << kLtParentSlotBits) | s;
-            }
-            else if (child_words[s] < b.node_capacity)
-            {
-                *reinterpret_cast<uint2*>(&b.nodes[child_words[s]].parent) = make_uint2(w, s);
+                summary[j] = EmptySummary();
             }
         }
-    }
-    float4* face_out = reinterpret_cast<float4*>(node->child_face[0]);
-    float4* bound_out = reinterpret_cast<float4*>(node->child_lo[0]);
+        float* rec = reinterpret_cast<float*>(lds_records + r * kWideRecordChunks);
 #pragma unroll
-    for (int q = 0; q < 6; ++q)
-    {
-        face_out[q] = make_float4(faces[4 * q], faces[4 * q + 1], faces[4 * q + 2], faces[4 * q + 3]);
-        bound_out[q] = make_float4(bounds[4 * q], bounds[4 * q + 1], bounds[4 * q + 2], bounds[4 * q + 3]);
+        for (uint32_t j = 0; j < 2; ++j)
+        {
+            const uint32_t s = 2 * h + j;
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+            {
+                rec[6 * s + f] = summary[j].face[f];
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+            {
+                rec[24 + 3 * s + k] = summary[j].lo[k];
+                rec[36 + 3 * s + k] = summary[j].hi[k];
+            }
+            reinterpret_cast<uint32_t*>(rec)[48 + s] = child_word[j];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < 2; ++j)
+        {
+            const uint32_t s = 2 * h + j;
+            const uint32_t word = child_word[j];
+            if (word == kLtNone)
+            {
+                continue;
+            }
+            if ((word & kLtChildLeafBit) != 0)
+            {
+                b.leaves[word & ~kLtChildLeafBit].parent = (w << kLtParentSlotBits) | s;
+            }
+            else if (word < b.node_capacity)
+            {
+                *reinterpret_cast<uint2*>(&b.nodes[word].parent) = make_uint2(w, s);
+            }
+        }
+        if (w == 0 && h == 0)
+        {
+            *reinterpret_cast<uint2*>(&b.nodes[0].parent) = make_uint2(kLtNone, 0u);  // the root
+        }
     }
-    *reinterpret_cast<uint4*>(node->child) = make_uint4(child_words[0], child_words[1], child_words[2], child_words[3]);
-    if (i == 0)
+    __syncthreads();
+    const uint32_t records = min(kWideRecords, total - w0);
+    uint8_t* out = reinterpret_cast<uint8_t*>(b.nodes + w0);
+    for (uint32_t c = threadIdx.x; c < records * kWideRecordChunks; c += kWideThreads)
     {
-        *reinterpret_cast<uint2*>(&node->parent) = make_uint2(kLtNone, 0u);
+        const uint32_t rec = c / kWideRecordChunks;
+        const uint32_t part = c - rec * kWideRecordChunks;
+        *reinterpret_cast<float4*>(out + uint64_t(rec) * sizeof(LightBvhNode) + part * 16) = lds_records[c];
     }
 }
+
 
 // caelix_light_bvh_begin.hlsl:9-17
 // One onesweep pass of the build's sort (keys / values ping-pong in the scratch).
@@ -1490,7 +1782,6 @@ __global__ void __launch_bounds__(SortKeys64::kThreads, kSortWavesPerSimd) Build
 template <class Keys, class Set>
 void RecordBuild(const Set& set, uint32_t capacity, uint32_t count, hipStream_t s)
 {
-    const uint32_t per_item_blocks = DivUp(capacity, kBvhThreads);
     const uint32_t tiles = SortTiles(capacity);
     {
         StageScope scope(s, "lt_bounds");
@@ -1519,11 +1810,14 @@ void RecordBuild(const Set& set, uint32_t capacity, uint32_t count, hipStream_t 
     }
     static const char* const kTreeStage[] = {"lt_tree_l1", "lt_tree_l2", "lt_tree_l3", "lt_tree_l4", "lt_tree_l5",
                                              "lt_tree_l6"};
-    for (uint32_t level = 1; level <= TreeLevels(capacity); ++level)
+    {
+        StageScope scope(s, kTreeStage[0]);
+        TreeLevel1Kernel<<<dim3(TreeBlocks(capacity, 1), count), kTreeThreads, 0, s>>>(set);
+    }
+    for (uint32_t level = 2; level <= TreeLevels(capacity); ++level)
     {
         StageScope scope(s, kTreeStage[std::min(level, 6u) - 1]);
-        const uint32_t threads = level == 1 ? kTreeThreads : 1024u;
-        TreeLevelKernel<<<dim3(TreeBlocks(capacity, level), count), threads, 0, s>>>(set, level);
+        TreeUpperKernel<<<dim3(TreeBlocks(capacity, level), count), kUpperThreads, 0, s>>>(set, level);
     }
     const uint32_t scan_tiles = DivUp(capacity, kScanTile);
     {
@@ -1536,7 +1830,7 @@ void RecordBuild(const Set& set, uint32_t capacity, uint32_t count, hipStream_t 
     }
     {
         StageScope scope(s, "lt_wide");
-        WideKernel<<<dim3(per_item_blocks, count), kBvhThreads, 0, s>>>(set);
+        WideKernel<<<dim3(DivUp(capacity, kWideRecords), count), kWideThreads, 0, s>>>(set);
     }
 }
 
