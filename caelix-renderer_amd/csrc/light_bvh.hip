@@ -1,16 +1,19 @@
 // Light BVH build for gfx950 (the build half of LightTree::RecordTree,
 // lib/src/light/light_tree_record.cpp:271-308; kernels shaders/light/bvh/*.hlsl).
 //
-// Launch sequence (one stream, 9 kernels + 4 sort passes):
-//   bounds     grid-stride min/max of the leaf box centres; writes the centres (12 B per leaf, read
-//              again by the Morton kernel instead of the 64-B leaves); the last workgroup reduces the
-//              partials, writes leaf_count / leaves_dropped / bounds and resets the build's counters
-//   morton     30-bit key + leaf index per leaf, the four 8-bit digit histograms of the onesweep
-//              sort, the cleared arrival counters and gap-cover array, the tile status rows
-//   sort x4    onesweep passes (radix_sort.hip); sorted pairs end in sort buffer 0
-//   karras     binary radix tree (Karras 2012) as the reference builds it, plus +1/-1 at the two ends
-//              of every node's leaf range in the gap-cover array and the node's split gap
-//   aggregate  bottom-up union with agent-scope acq_rel arrival counters
+// Launch sequence (one stream; 4 sort passes, or 7 with 64-bit keys):
+//   bounds     grid-stride min/max of the leaf box centres into per-workgroup partials; writes the
+//              centres (12 B per leaf, read again by the Morton kernel instead of the 64-B leaves);
+//              workgroup 0 resets the build's counters and the sort histograms
+//   morton     reduces the partials (leaf_count, leaves_dropped, bounds), 30-bit key + leaf index per
+//              leaf, the onesweep sort's digit histograms and tile status rows
+//   sort       onesweep passes (radix_sort.h); sorted pairs end in sort buffer 0
+//   tree L1    per block of 512 sorted leaves, in LDS: the binary radix tree as the reference builds
+//              it (Karras 2012 numbering) and its bottom-up aggregation, by one climb from the leaves;
+//              per node its children, parent links, split gap, +1/-1 at the two ends of its range in
+//              the gap-cover array, aggregate and 4-wide slot list; the block's open elements (whose
+//              parent lies outside it) go to its open list
+//   tree L2+   per block of 32 sub-blocks, in LDS: the same climb from the sub-blocks' open elements
 //   cover      inclusive scan of the gap-cover array (decoupled look-back): cover[g] = the number of
 //              binary nodes whose range spans the gap between sorted leaves g and g + 1
 //   flags      depth(i) = cover[split(i)] - 1 (the nodes spanning node i's split gap are i and its
@@ -31,7 +34,6 @@ namespace {
 
 constexpr uint32_t kBvhThreads = 256;
 constexpr uint32_t kBoundsMaxBlocks = 1024;
-constexpr uint32_t kBoundsUnroll = 4;
 constexpr uint32_t kScanThreads = 1024;
 constexpr uint32_t kScanItems = 16;
 constexpr uint32_t kScanTile = kScanThreads * kScanItems;
@@ -100,13 +102,11 @@ struct Scratch
     __host__ __device__ float* aggregates() const { return reinterpret_cast<float*>(base + uint64_t(kLtScratchAggregates) * capacity); }
     // Free arrays reused by this build: the leaf centres live in the aggregates array until the
     // aggregation; the gap-cover diffs in `wide` until the flags scan; after the sort, sort buffer 1
-    // holds the cover counts (keys) and every node's split gap (values), and the flags scan replaces
-    // the consumed split gaps by the binary node of every 4-wide node (wide_source).
+    // holds the cover counts (keys) and every node's split gap (values).
     __host__ __device__ float* centres() const { return aggregates(); }
     __host__ __device__ int32_t* cover_diff() const { return reinterpret_cast<int32_t*>(u32(kLtScratchWide)); }
     __host__ __device__ int32_t* cover() const { return reinterpret_cast<int32_t*>(keys(1)); }
     __host__ __device__ uint32_t* split() const { return values(1); }
-    __host__ __device__ uint32_t* wide_source() const { return values(1); }
     __host__ __device__ uint8_t* state_base() const { return base + uint64_t(kLtScratchSortState) * capacity; }
     __host__ __device__ uint32_t* ctl() const { return reinterpret_cast<uint32_t*>(state_base()); }
     __host__ __device__ float* bounds_partials() const { return reinterpret_cast<float*>(state_base() + kCtlWords * 4); }
@@ -390,44 +390,24 @@ __global__ void __launch_bounds__(kBvhThreads) BoundsKernel(Set set)
     float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f};
     float hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
     float* centres = b.scratch.centres();
-    // kBoundsUnroll leaves per thread and round, all loaded before any store (a store would order
-    // the later loads behind it); a leaf past n loads leaf n - 1 and is not counted.
-    const uint32_t stride = grid * kBvhThreads;
-    for (uint32_t first = blockIdx.x * kBvhThreads + t; first < n; first += stride * kBoundsUnroll)
+    for (uint32_t i = blockIdx.x * kBvhThreads + t; i < n; i += grid * kBvhThreads)
     {
-        float2 l01[kBoundsUnroll];
-        float2 l2h0[kBoundsUnroll];
-        float2 h12[kBoundsUnroll];
-#pragma unroll
-        for (uint32_t j = 0; j < kBoundsUnroll; ++j)
-        {
-            const uint32_t i = min(first + j * stride, n - 1);
-            const float* leaf = b.leaves[i].summary.lo;
-            l01[j] = *reinterpret_cast<const float2*>(leaf);       // lo.x, lo.y
-            l2h0[j] = *reinterpret_cast<const float2*>(leaf + 2);  // lo.z, hi.x
-            h12[j] = *reinterpret_cast<const float2*>(leaf + 4);   // hi.y, hi.z
-        }
-#pragma unroll
-        for (uint32_t j = 0; j < kBoundsUnroll; ++j)
-        {
-            const uint32_t i = first + j * stride;
-            if (i >= n)
-            {
-                break;
-            }
-            const float c0 = 0.5f * (l01[j].x + l2h0[j].y);
-            const float c1 = 0.5f * (l01[j].y + h12[j].x);
-            const float c2 = 0.5f * (l2h0[j].x + h12[j].y);
-            centres[3 * uint64_t(i)] = c0;
-            centres[3 * uint64_t(i) + 1] = c1;
-            centres[3 * uint64_t(i) + 2] = c2;
-            lo[0] = fminf(lo[0], c0);
-            lo[1] = fminf(lo[1], c1);
-            lo[2] = fminf(lo[2], c2);
-            hi[0] = fmaxf(hi[0], c0);
-            hi[1] = fmaxf(hi[1], c1);
-            hi[2] = fmaxf(hi[2], c2);
-        }
+        const float* leaf = b.leaves[i].summary.lo;
+        const float2 l01 = *reinterpret_cast<const float2*>(leaf);       // lo.x, lo.y
+        const float2 l2h0 = *reinterpret_cast<const float2*>(leaf + 2);  // lo.z, hi.x
+        const float2 h12 = *reinterpret_cast<const float2*>(leaf + 4);   // hi.y, hi.z
+        const float c0 = 0.5f * (l01.x + l2h0.y);
+        const float c1 = 0.5f * (l01.y + h12.x);
+        const float c2 = 0.5f * (l2h0.x + h12.y);
+        centres[3 * uint64_t(i)] = c0;
+        centres[3 * uint64_t(i) + 1] = c1;
+        centres[3 * uint64_t(i) + 2] = c2;
+        lo[0] = fminf(lo[0], c0);
+        lo[1] = fminf(lo[1], c1);
+        lo[2] = fminf(lo[2], c2);
+        hi[0] = fmaxf(hi[0], c0);
+        hi[1] = fmaxf(hi[1], c1);
+        hi[2] = fmaxf(hi[2], c2);
     }
     MinMax3<kBvhThreads / kWave>(lo, hi, lds, t / kWave);
     float* partials = b.scratch.bounds_partials();
@@ -1550,9 +1530,6 @@ __global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Set set)
     }
     const int32_t total = TileExclusiveScan(v, lds_items, lds_waves);
     const int32_t prefix = ScanLookBack(b.scratch.flags_status(), tile, total, &lds_prefix);
-    // The binary node of each 4-wide node. Slot w <= item index, and every earlier tile has read its
-    // split gaps before publishing, so these stores only replace consumed split gaps.
-    uint32_t* source = b.scratch.wide_source();
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k)
     {
@@ -1560,10 +1537,6 @@ __global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Set set)
         if (i < capacity)
         {
             wide[i] = uint32_t(prefix + v[k]);
-        }
-        if (i + 1 < n && (depth[k] & 1u) == 0)
-        {
-            source[prefix + v[k]] = i;
         }
     }
     if (tile + 1 != b.scratch.scan_tiles() || threadIdx.x != 0)
@@ -1633,17 +1606,17 @@ __global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Set set)
 }
 
 // ------------------------------------------------------------------------------------------------
-// wide (caelix_light_bvh_wide.hlsl:10-100): the reference's lanes are the binary nodes, of which
-// the odd-depth half returns at once. Here the flags scan lists the even-depth ones, two lanes build
-// one 4-wide node (lane h: slots 2h and 2h + 1) from the slot list the tree build wrote for its
-// binary node, and a workgroup's kWideRecords consecutive nodes are staged in LDS and written out
-// with coalesced stores. Three dependent rounds of loads: the binary node, its slot words, then per
-// slot the leaf record or the binary aggregate and the 4-wide index. The parent words of a node (bytes 208..215) belong to the lane that builds
-// its parent, so the write-out covers bytes 0..207 of each record.
+// wide (caelix_light_bvh_wide.hlsl:10-100), one lane per binary node as in the reference: the
+// odd-depth lanes stop after the first round of loads. Two dependent rounds: the node's depth, 4-wide
+// index and slot list (written by the tree build), all coalesced; then per slot the leaf record, or
+// the 4-wide index and binary aggregate. The 4-wide nodes of a workgroup's binary nodes are
+// consecutive (4-wide index = exclusive scan of the even-depth flags); the first kWideStaged of them
+// are staged in LDS and written out with coalesced stores, any beyond directly. The parent words of a
+// node (bytes 208..215) belong to the lane that builds its parent, so a record write covers bytes
+// 0..207.
 constexpr uint32_t kWideThreads = 256;
-constexpr uint32_t kWideRecords = kWideThreads / 2;
+constexpr uint32_t kWideStaged = 192;
 constexpr uint32_t kWideRecordChunks = 13;  // 16-byte chunks of bytes 0..207
-constexpr uint32_t kWideRecordFloats = kWideRecordChunks * 4;
 
 template <class Set>
 __global__ void __launch_bounds__(kWideThreads) WideKernel(Set set)
@@ -1653,63 +1626,80 @@ __global__ void __launch_bounds__(kWideThreads) WideKernel(Set set)
     {
         return;
     }
-    const uint32_t total = min(b.scratch.ctl()[kCtlWideNodes], b.node_capacity);
-    const uint32_t w0 = blockIdx.x * kWideRecords;
-    if (w0 >= total)
+    const uint32_t* ctl = b.scratch.ctl();
+    const uint32_t n = ctl[kCtlLeafCount];
+    const uint32_t total = min(ctl[kCtlWideNodes], b.node_capacity);
+    const uint32_t i0 = blockIdx.x * kWideThreads;
+    if (total == 0 || i0 + 1 >= n)
     {
         return;
     }
-    __shared__ float4 lds_records[kWideRecords * kWideRecordChunks];
-    const uint32_t r = threadIdx.x >> 1;
-    const uint32_t h = threadIdx.x & 1u;
-    const uint32_t w = w0 + r;
-    if (w < total)
+    __shared__ float4 lds_records[kWideStaged * kWideRecordChunks];
+    __shared__ uint32_t lds_end;
+    const uint32_t t = threadIdx.x;
+    const uint32_t i = i0 + t;
+    const uint32_t at = i + 1 < n ? i : i0;
+    const uint32_t* wide = b.scratch.u32(kLtScratchWide);
+    const uint32_t depth = b.scratch.u32(kLtScratchCounter)[at];
+    const uint32_t w = wide[at];
+    const uint32_t w0 = wide[i0];
+    const uint4 q = b.scratch.quad()[at];
+    if (t == 0)
     {
-        const uint32_t i = b.scratch.wide_source()[w];  // the binary node (an even-depth one)
-        const uint2 words = reinterpret_cast<const uint2*>(b.scratch.quad() + i)[h];
-        const uint32_t slot_word[2] = {words.x, words.y};
-        const uint32_t* wide = b.scratch.u32(kLtScratchWide);
+        lds_end = w0;
+    }
+    __syncthreads();
+    const bool even = i + 1 < n && (depth & 1u) == 0 && w < total;
+    if (even)
+    {
+        atomicMax(&lds_end, w + 1);
+        const uint32_t slot_word[4] = {q.x, q.y, q.z, q.w};
         // All loads first: per slot the 4-wide index (internal) and the summary.
-        uint32_t child_word[2];
-        Summary summary[2];
+        uint32_t child_word[4];
+        Summary summary[4];
 #pragma unroll
-        for (uint32_t j = 0; j < 2; ++j)
+        for (uint32_t s = 0; s < 4; ++s)
         {
-            const uint32_t word = slot_word[j];
+            const uint32_t word = slot_word[s];
             const bool used = word != kLtNone;
             const bool leaf = used && (word & kLtChildLeafBit) != 0;
-            child_word[j] = used && !leaf ? wide[word] : word;
+            child_word[s] = used && !leaf ? wide[word] : word;
             const float* p = leaf ? b.leaves[word & ~kLtChildLeafBit].summary.face
                                   : b.scratch.aggregates() + (used ? uint64_t(word) * 12 : 0u);
-            summary[j] = LoadSummary(p);
+            summary[s] = LoadSummary(p);
             if (!used)
             {
-                summary[j] = EmptySummary();
+                summary[s] = EmptySummary();
             }
         }
-        float* rec = reinterpret_cast<float*>(lds_records + r * kWideRecordChunks);
+        const uint32_t r = w - w0;
+        float staged[kWideRecordChunks * 4];
 #pragma unroll
-        for (uint32_t j = 0; j < 2; ++j)
+        for (uint32_t s = 0; s < 4; ++s)
         {
-            const uint32_t s = 2 * h + j;
 #pragma unroll
             for (int f = 0; f < 6; ++f)
             {
-                rec[6 * s + f] = summary[j].face[f];
+                staged[6 * s + f] = summary[s].face[f];
             }
 #pragma unroll
             for (int k = 0; k < 3; ++k)
             {
-                rec[24 + 3 * s + k] = summary[j].lo[k];
-                rec[36 + 3 * s + k] = summary[j].hi[k];
+                staged[24 + 3 * s + k] = summary[s].lo[k];
+                staged[36 + 3 * s + k] = summary[s].hi[k];
             }
-            reinterpret_cast<uint32_t*>(rec)[48 + s] = child_word[j];
+            staged[48 + s] = __uint_as_float(child_word[s]);
+        }
+        float4* dst = r < kWideStaged ? lds_records + r * kWideRecordChunks : reinterpret_cast<float4*>(b.nodes + w);
+#pragma unroll
+        for (uint32_t c = 0; c < kWideRecordChunks; ++c)
+        {
+            dst[c] = make_float4(staged[4 * c], staged[4 * c + 1], staged[4 * c + 2], staged[4 * c + 3]);
         }
 #pragma unroll
-        for (uint32_t j = 0; j < 2; ++j)
+        for (uint32_t s = 0; s < 4; ++s)
         {
-            const uint32_t s = 2 * h + j;
-            const uint32_t word = child_word[j];
+            const uint32_t word = child_word[s];
             if (word == kLtNone)
             {
                 continue;
@@ -1723,15 +1713,15 @@ __global__ void __launch_bounds__(kWideThreads) WideKernel(Set set)
                 *reinterpret_cast<uint2*>(&b.nodes[word].parent) = make_uint2(w, s);
             }
         }
-        if (w == 0 && h == 0)
+        if (w == 0)
         {
             *reinterpret_cast<uint2*>(&b.nodes[0].parent) = make_uint2(kLtNone, 0u);  // the root
         }
     }
     __syncthreads();
-    const uint32_t records = min(kWideRecords, total - w0);
+    const uint32_t records = min(lds_end - w0, kWideStaged);
     uint8_t* out = reinterpret_cast<uint8_t*>(b.nodes + w0);
-    for (uint32_t c = threadIdx.x; c < records * kWideRecordChunks; c += kWideThreads)
+    for (uint32_t c = t; c < records * kWideRecordChunks; c += kWideThreads)
     {
         const uint32_t rec = c / kWideRecordChunks;
         const uint32_t part = c - rec * kWideRecordChunks;
@@ -1830,7 +1820,7 @@ void RecordBuild(const Set& set, uint32_t capacity, uint32_t count, hipStream_t 
     }
     {
         StageScope scope(s, "lt_wide");
-        WideKernel<<<dim3(DivUp(capacity, kWideRecords), count), kWideThreads, 0, s>>>(set);
+        WideKernel<<<dim3(DivUp(capacity, kWideThreads), count), kWideThreads, 0, s>>>(set);
     }
 }
 
