@@ -1,7 +1,7 @@
 """Builds the gfx950 library libcaelix_amd.so in-tree with hipcc.
 
-Every .hip file under csrc/ is compiled for gfx950 only and linked into one shared library with a C
-ABI (include/caelix_amd/caelix_amd.h). Floating point keeps IEEE single operations in source order
+Every .hip file under csrc/ is compiled for gfx950 only, every .cpp file (host code: the .cxw reader)
+with g++, and all are linked into one shared library with a C ABI (include/caelix_amd/caelix_amd.h). Floating point keeps IEEE single operations in source order
 (-ffp-contract=off, no fast math), which the bit-exact parity of the light-tree build relies on.
 """
 from __future__ import annotations
@@ -44,15 +44,30 @@ def _compile(source: Path) -> Path:
     return obj
 
 
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-pthread", "-Wall", "-I", str(REPO / "include")]
+
+
+def _compile_host(source: Path) -> Path:
+    obj = BUILD / (source.stem + ".o")
+    cmd = ["g++", *HOST_FLAGS, "-c", str(source), "-o", str(obj)]
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"g++ failed for {source.name}:\n{proc.stdout}\n{proc.stderr}")
+    return obj
+
+
 def build(verbose: bool = True) -> Path:
-    """Compiles every csrc/*.hip for gfx950 and links libcaelix_amd.so; returns its path."""
+    """Compiles every csrc/*.hip for gfx950 and every csrc/*.cpp (host code) and links libcaelix_amd.so;
+    returns its path."""
     BUILD.mkdir(exist_ok=True)
     sources = sorted(CSRC.glob("*.hip"))
-    workers = min(8, len(sources)) or 1
+    host_sources = sorted(CSRC.glob("*.cpp"))
+    workers = min(8, len(sources) + len(host_sources)) or 1
     with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as pool:
-        objects = list(pool.map(_compile, sources))
+        objects = list(pool.map(_compile, sources)) + list(pool.map(_compile_host, host_sources))
+    sources = sources + host_sources
     tmp = LIBRARY.with_suffix(".so.tmp")
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objects)]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objects), "-lz", "-pthread"]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         raise RuntimeError(f"hipcc link failed:\n{proc.stdout}\n{proc.stderr}")

@@ -166,6 +166,14 @@ SIGNATURES = {
     "cx_trace_rays": (_INT, [ctypes.POINTER(CxScene), _P, _U32, _P, _P]),
     "cx_light_sample": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightTreeView), _P, _P, _U32, _P, _P, _P]),
     "cx_light_face_pdf": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightTreeView), _P, _U32, _P, _P]),
+    "cx_cxw_open": (_INT, [ctypes.c_char_p, _P, _P]),
+    "cx_cxw_close": (None, [_P]),
+    "cx_cxw_world": (_INT, [_P, _P]),
+    "cx_cxw_entity": (_INT, [_P, _U32, _P]),
+    "cx_cxw_sector": (_INT, [_P, _U64, _P]),
+    "cx_cxw_decode": (_INT, [_P, _U64, _P, _P, _P, _P, _P, _P]),
+    "cx_cxw_decode_all": (_INT, [_P, _U32, _P, _P, _P]),
+    "cx_cxw_copy_decoded": (_INT, [_P, _P, _P, _P, _P, _U64]),
     "cx_profile_enable": (None, [_INT]),
     "cx_profile_reset": (None, []),
     "cx_profile_read": (_INT, [_U32, _P, _P, _P, _P]),

@@ -368,6 +368,90 @@ CX_API int cx_light_face_pdf(const CxScene* scene, const CxLightTreeView* lights
                              uint32_t count, float* pdf, void* stream);
 
 /* ------------------------------------------------------------------------------------------
+ * .cxw world files (format version 8 and older): the reader of the bricks the scene build takes.
+ * Replaces the reference's Rust reader host-rs/caelix-cxw/src/lib.rs:1-656 (CxwFile::open,
+ * ::world, ::sectors, ::decode, ::decode_all; Error variants lib.rs:69-109; read rules of
+ * parse_index lib.rs:314-413, region lib.rs:415-452, decode_payload lib.rs:515-656). Host code:
+ * the file is memory-mapped, sectors are inflated with zlib (raw deflate) on host threads.
+ * Format errors return CX_ERROR_INVALID_ARGUMENT and fill *error (kind = a CxCxwErrorKind, with
+ * the fields of the matching reference variant); a NULL error pointer is allowed.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct CxCxw CxCxw; /* an open file (opaque) */
+
+typedef enum CxCxwErrorKind {
+    CX_CXW_NONE = 0,
+    CX_CXW_IO = 1,
+    CX_CXW_BAD_MAGIC = 2,           /* actual = the magic */
+    CX_CXW_UNSUPPORTED_VERSION = 3, /* actual = the version */
+    CX_CXW_BAD_FLAGS = 4,           /* actual = the flags */
+    CX_CXW_TRUNCATED = 5,           /* what, offset (a payload offset when what starts "payload") */
+    CX_CXW_BAD_REGION_SIZE = 6,     /* coord, expected, actual */
+    CX_CXW_BAD_SECTOR_MAGIC = 7,    /* actual = the magic */
+    CX_CXW_BAD_COUNTS = 8,          /* expected = capacity, actual = count, offset = free count */
+    CX_CXW_INFLATE = 9,
+    CX_CXW_DUPLICATE_SECTOR = 10,   /* coord */
+    CX_CXW_BAD_PAYLOAD = 11         /* coord, what = the reason */
+} CxCxwErrorKind;
+
+typedef struct CxCxwError {
+    int32_t kind;
+    int32_t coord[3];
+    uint64_t offset;
+    uint64_t expected;
+    uint64_t actual;
+    char what[64];
+    char message[192];
+} CxCxwError;
+
+typedef struct CxCxwWorld {
+    uint16_t version;
+    uint8_t has_player_pose; /* version 7+ world record with a pose */
+    uint8_t pad;
+    float pose_position[3];
+    float pose_rotation[4]; /* quaternion x, y, z, w */
+    uint32_t entity_count;
+    uint64_t sector_count;  /* every sector of every entity */
+} CxCxwWorld;
+
+typedef struct CxCxwEntity { /* EntityInfo (lib.rs:111-133), after the version migrations */
+    uint8_t guid[16];
+    float position[3];
+    float rotation[4];
+    float velocity[6];      /* linear xyz, angular xyz; zero before version 4 */
+    uint32_t sector_count;
+    uint16_t require_update_flags; /* raw, as stored */
+    uint8_t flags;          /* bit 0 HasBody, bit 1 Static */
+    uint8_t is_protected;
+} CxCxwEntity;
+
+typedef struct CxCxwSector { /* SectorRef (lib.rs:185-192) */
+    uint32_t entity;
+    int32_t coord[3];
+    uint64_t region_offset;
+    uint32_t region_size;
+    uint32_t pad;
+} CxCxwSector;
+
+CX_API int cx_cxw_open(const char* path, CxCxw** out, CxCxwError* error);
+CX_API void cx_cxw_close(CxCxw* file);
+CX_API int cx_cxw_world(const CxCxw* file, CxCxwWorld* out);
+CX_API int cx_cxw_entity(const CxCxw* file, uint32_t index, CxCxwEntity* out);
+/* Sectors in entity order, then sector index order. */
+CX_API int cx_cxw_sector(const CxCxw* file, uint64_t index, CxCxwSector* out);
+/* One sector's non-air bricks in ascending brick index order: keys[3 * i] = entity-local brick key
+ * (sector coord * 16 + brick in sector), blocks[512 * i + x + 8y + 64z] = raw value, summaries[i] =
+ * the preview word; buffers of 4096 bricks. A brick whose 512 values are all 0 is dropped. */
+CX_API int cx_cxw_decode(const CxCxw* file, uint64_t sector, int32_t* keys, uint16_t* blocks, uint32_t* summaries,
+                         uint32_t* brick_count, uint64_t* voxels, CxCxwError* error);
+/* Decodes every sector on `threads` host threads (0 = one per core) and keeps the results in the
+ * handle; stops at the first error. Then cx_cxw_copy_decoded writes all bricks in sector order
+ * (entity_of_brick and summaries may be NULL). */
+CX_API int cx_cxw_decode_all(CxCxw* file, uint32_t threads, uint64_t* brick_count, uint64_t* voxels,
+                             CxCxwError* error);
+CX_API int cx_cxw_copy_decoded(const CxCxw* file, int32_t* keys, uint16_t* blocks, uint32_t* summaries,
+                               uint32_t* entity_of_brick, uint64_t capacity_bricks);
+
+/* ------------------------------------------------------------------------------------------
  * Stage timing: when enabled, entry points bracket every kernel with HIP events on the stream it
  * runs on. cx_profile_read synchronises those events and returns per-stage sums (milliseconds)
  * and launch counts since the last reset; names are static strings.
