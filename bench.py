@@ -1,6 +1,6 @@
 """Benchmark of the Caelix hot path on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload trace|lighttree|copy]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload trace|lighttree|copy|cxw]
 
 trace      (default) DDA + NEE at 1920 x 1080: 1,048,576 resident 8^3 bricks in a 4096^3 world (8-octave
            value-noise terrain, 4 surface bricks per column), bulk hash table at load 0.7, 65,536
@@ -14,6 +14,8 @@ lighttree  full light-tree rebuild over 4,194,304 emissive brick leaves (half un
            box, half in 64 Gaussian clusters, sigma 32 voxels, emission log-uniform in [0.01, 100]).
            One step = one rebuild from leaves resident in HBM. value = Mprims/s.
 copy       a plain 1 GiB device copy: the calibration run for the HBM byte counters of rocprofv3.
+cxw        the 1080p terrain world loaded from a .cxw file: open + raw-deflate decode of every sector
+           on the host cores (value = bricks/s), then the GPU scene build from the decoded bricks.
 
 Prints one JSON line. `roofline` is for the dominant kernel of a step, timed with HIP events around
 its launches on the stream it runs on. Its algorithmic bytes: for the trace, the demand bytes per ray
@@ -501,12 +503,65 @@ def copy_bench(args) -> dict:
                          "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": 2 * n}}
 
 
+def cxw_bench(args) -> dict:
+    """The 1080p terrain world (1,048,576 bricks) written to a .cxw file (raw deflate, level 1), then
+    timed: open (index parse) + decode of every sector on the host cores, and the GPU scene build from
+    the decoded bricks (upload, finalize, bulk hash, light records, light tree). One step = one load.
+    value = decoded bricks per second of open + decode. Host-side work: no HBM roofline applies to the
+    decode; the scene build's kernels are the trace workload's set-up kernels."""
+    import tempfile
+
+    import torch
+
+    import caelix_amd
+    from caelix_amd import cxw, scene
+
+    data = scene.hd_scene()
+    threads = min(os.cpu_count() or 1, 16)
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as tmp:
+        path = Path(tmp) / "terrain_1080p.cxw"
+        t0 = time.perf_counter()
+        cxw.write(path, [cxw.WEntity(cxw.sectors_from_bricks(data.keys, data.payload))], level=1)
+        write_s = time.perf_counter() - t0
+        file_bytes = path.stat().st_size
+        times = []
+        decoded = None
+        for i in range(args.warmup + args.steps):
+            t0 = time.perf_counter()
+            with cxw.CxwFile(path) as f:
+                decoded = f.decode_all(threads)
+                sectors = len(f.sectors)
+            if i >= args.warmup:
+                times.append(time.perf_counter() - t0)
+        bricks = len(decoded["keys"])
+        decode_s = sorted(times)[len(times) // 2]
+        loaded = cxw.load_scene(path, data.materials, data.light_codes, data.camera, threads=threads)
+    torch.cuda.synchronize()
+    build_times = []
+    for i in range(2):
+        t0 = time.perf_counter()
+        ts = caelix_amd.TraceScene(loaded, load_factor=0.7)
+        torch.cuda.synchronize()
+        build_times.append(time.perf_counter() - t0)
+        del ts
+    inflated = bricks * 1024
+    return {"metric": "cxw load: decoded bricks/s (open + decode on host threads)",
+            "value": round(bricks / decode_s, 1), "unit": "bricks/s", "ms_per_step": round(decode_s * 1e3, 3),
+            "dtype": "u16", "config": {"workload": "1080p terrain world as .cxw: 1,048,576 bricks, 16^3-brick "
+                                                   "sectors, raw deflate level 1", "bricks": bricks,
+                                       "sectors": sectors, "file_bytes": file_bytes, "threads": threads},
+            "roofline": None,
+            "cxw": {"decode_s": round(decode_s, 4), "inflated_GB_per_s": round(inflated / decode_s / 1e9, 3),
+                    "file_MB_per_s": round(file_bytes / decode_s / 1e6, 1), "write_s": round(write_s, 2),
+                    "scene_build_s": round(min(build_times), 4), "voxels": decoded["voxels"]}}
+
+
 def main() -> int:
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=20)
     parser.add_argument("--warmup", type=int, default=3)
-    parser.add_argument("--workload", default="trace", choices=["trace", "lighttree", "copy"])
+    parser.add_argument("--workload", default="trace", choices=["trace", "lighttree", "copy", "cxw"])
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--no-lighttree", action="store_true", help="trace only, without the light-tree figure")
     parser.add_argument("--no-adjoint", action="store_true", help="without the differentiable-form figure")
@@ -531,6 +586,8 @@ def main() -> int:
                                                       "pipeline_roofline", "config", "keys64")}
     elif args.workload == "copy":
         result = copy_bench(args)
+    elif args.workload == "cxw":
+        result = cxw_bench(args)
     else:
         result = lighttree_bench(args, with_cpu=not args.no_cpu_baseline)
     line = {
@@ -549,7 +606,8 @@ def main() -> int:
         "config": result["config"],
         "roofline": result["roofline"],
     }
-    for key in ("cpu_baseline", "parity", "lighttree", "adjoint", "batch", "pipeline_roofline", "keys64", "stages"):
+    for key in ("cpu_baseline", "parity", "lighttree", "adjoint", "batch", "pipeline_roofline", "keys64", "stages",
+                "cxw"):
         if key in result:
             line[key] = result[key]
     print(json.dumps(line))
