@@ -96,10 +96,39 @@ class CxCamera(ctypes.Structure):
 
 class CxTraceParams(ctypes.Structure):
     _fields_ = [("bounces", ctypes.c_uint32), ("frame_seed", ctypes.c_uint32), ("spp", ctypes.c_uint32),
-                ("flags", ctypes.c_uint32)]
+                ("flags", ctypes.c_uint32), ("sun_dir", ctypes.c_float * 3), ("sun_solid_angle", ctypes.c_float),
+                ("sun_radiance", ctypes.c_float * 3), ("sun_cos_radius", ctypes.c_float),
+                ("sky_zenith", ctypes.c_float * 3), ("pad0", ctypes.c_float), ("sky_horizon", ctypes.c_float * 3),
+                ("pad1", ctypes.c_float), ("sky_ground", ctypes.c_float * 3), ("pad2", ctypes.c_float)]
 
 
 TRACE_NO_RUSSIAN_ROULETTE = 1
+TRACE_SKY = 2
+TRACE_NO_SUN_NEE = 4
+
+
+def sky_params(sun_dir=(0.0, 1.0, 0.0), sun_radiance=(0.0, 0.0, 0.0), sun_radius_deg: float = 0.5,
+               zenith=(0.0, 0.0, 0.0), horizon=(0.0, 0.0, 0.0), ground=(0.0, 0.0, 0.0)) -> dict:
+    """The sky fields of CxTraceParams: sun_dir normalised, the cone's solid angle 2 pi (1 - cos r) and
+    cos r computed in double (as the reference's host does)."""
+    d = np.asarray(sun_dir, np.float64)
+    d = d / np.linalg.norm(d)
+    r = np.radians(sun_radius_deg)
+    return {"sun_dir": [float(v) for v in d], "sun_radiance": [float(v) for v in sun_radiance],
+            "sun_solid_angle": float(2.0 * np.pi * (1.0 - np.cos(r))), "sun_cos_radius": float(np.cos(r)),
+            "sky_zenith": [float(v) for v in zenith], "sky_horizon": [float(v) for v in horizon],
+            "sky_ground": [float(v) for v in ground]}
+
+
+def apply_sky(params: CxTraceParams, sky: dict | None) -> None:
+    """Sets the CX_TRACE_SKY flag and the sky fields of `params` (sky None: black sky, no sun)."""
+    if sky is None:
+        return
+    params.flags |= TRACE_SKY
+    for name in ("sun_dir", "sun_radiance", "sky_zenith", "sky_horizon", "sky_ground"):
+        getattr(params, name)[:] = sky[name]
+    params.sun_solid_angle = sky["sun_solid_angle"]
+    params.sun_cos_radius = sky["sun_cos_radius"]
 
 
 class CxTraceGradients(ctypes.Structure):
@@ -571,7 +600,7 @@ class TraceScene:
         return counters
 
     def trace_frame(self, camera: dict, bounces: int, seed: int, spp: int = 1, color=None, radiance=None,
-                    attrib=None, counters=None, stream=None, flags: int = 0):
+                    attrib=None, counters=None, stream=None, flags: int = 0, sky: dict | None = None):
         torch = _torch()
         w, h = int(camera["width"]), int(camera["height"])
         color = color if color is not None else torch.empty(w * h * 4, dtype=torch.float32, device="cuda")
@@ -580,6 +609,7 @@ class TraceScene:
         counters = counters if counters is not None else torch.zeros(8, dtype=torch.int64, device="cuda")
         cam = camera_struct(camera)
         params = CxTraceParams(bounces, seed, spp, flags)
+        apply_sky(params, sky)
         out = self._output(camera, color, radiance, attrib, counters)
         check(self.lib.cx_trace_frame(ctypes.byref(self.scene), ctypes.byref(self.view), ctypes.byref(cam),
                                       ctypes.byref(params), ctypes.byref(out), stream_handle(stream)), "cx_trace_frame")

@@ -199,6 +199,49 @@ def furnace_box(size_bricks: int = 8, wall: int = 8, emission: float = 1.0, albe
                      grid_dim=dim, group_shift=5, camera=camera, emissive_bricks=np.arange(len(keys)))
 
 
+def sun_floor(size_bricks: int = 16, albedo: float = 123 / 255, lamp: str = "below", lamp_emission: float = 20.0,
+              width: int = 64, height: int = 64) -> SceneData:
+    """A flat floor under an open sky: size_bricks^2 bricks in one layer whose voxels y in [4, 8) are one
+    grey opaque material (top face at y = 8), plus one lamp: "below" a 2^3 emissive cube under the
+    floor (voxels y 0..1 of the central floor brick; every face it opens lies below the floor top, so
+    no point on the floor can receive its light), "above" a 4^3 emissive cube in the air over the
+    floor's centre (voxels y 18..21). The camera looks (almost) straight down at the floor's centre
+    from y = 120 with a 30 degree field of view, so every primary ray hits the floor or the lamp."""
+    dim = size_bricks
+    world = dim * 8
+    bx, bz = np.meshgrid(np.arange(dim), np.arange(dim), indexing="ij")
+    keys = np.stack([bx.ravel(), np.zeros(dim * dim, np.int64), bz.ravel()], axis=1).astype(np.int32)
+    floor_block, lamp_block = OPAQUE | 1, OPAQUE | 200
+    v = np.arange(512)
+    lx, ly, lz = v & 7, (v >> 3) & 7, v >> 6
+    payload = np.where(ly[None, :] >= 4, floor_block, 0).repeat(len(keys), axis=0).astype(np.uint16)
+    centre = dim // 2
+    cube = (lx >= 3) & (lx < 5) & (lz >= 3) & (lz < 5)
+    centre_index = int(np.nonzero((keys[:, 0] == centre) & (keys[:, 2] == centre))[0][0])
+    if lamp == "below":
+        payload[centre_index, cube & (ly < 2)] = lamp_block
+        emissive = np.array([centre_index])
+    elif lamp == "above":
+        lamp_payload = np.zeros((1, 512), np.uint16)
+        cube4 = (lx >= 2) & (lx < 6) & (lz >= 2) & (lz < 6) & (ly >= 2) & (ly < 6)
+        lamp_payload[0, cube4] = lamp_block
+        keys = np.concatenate([keys, np.array([[centre, 2, centre]], np.int32)])
+        payload = np.concatenate([payload, lamp_payload])
+        emissive = np.array([len(keys) - 1])
+    else:
+        raise ValueError(f"unknown lamp placement {lamp!r}")
+    table = np.zeros((65536, 8), np.float32)
+    table[floor_block, 0:3] = np.float32(albedo)
+    table[lamp_block, 0:3] = np.float32(0.5)
+    table[lamp_block, 4:7] = np.float32(lamp_emission)
+    light_codes = np.zeros(2048, np.uint32)
+    light_codes[lamp_block >> 5] |= np.uint32(1) << np.uint32(lamp_block & 31)
+    c = world / 2
+    camera = look_at((c + 0.013, 120.0, c + 0.021), (c, 8.0, c), width, height, fov_y_deg=30.0)
+    return SceneData(keys=keys, payload=payload, materials=table, light_codes=light_codes, grid_min=(0, 0, 0),
+                     grid_dim=dim, group_shift=5, camera=camera, emissive_bricks=emissive)
+
+
 def diff_scene(seed: int = 5) -> SceneData:
     """The differentiable workload: the 1080p terrain scaled to 262,144 bricks (2048^3, 256^2 columns x
     4 bricks), 16,384 emissive bricks (the emitters), emission log-uniform in [0.01, 100], 512 x 512."""

@@ -1,7 +1,9 @@
 // The path tracer of one pixel for gfx950 (shaders/render/caelix_path_trace.hlsl:276-452, 737-978;
 // shaders/render/caelix_shading.hlsl:267-307), the subset the synthetic workloads use: opaque
 // Lambertian voxels (metallic 0, smoothness 0), light NEE through the light tree with MIS on both
-// sides, no sun, no sky light (a miss adds nothing), no fog, no glass, bounce NEE probability 1.
+// sides, bounce NEE probability 1, no fog, no glass; optionally (CX_TRACE_SKY) the procedural sky on
+// misses and the sun, whose NEE shares the vertex's one shadow ray with the lamp (one-sample RIS);
+// else a black sky and no sun.
 #pragma once
 #include "light_sample.h"
 
@@ -10,13 +12,99 @@ namespace cx {
 constexpr uint32_t kSegmentLampNee = 1u;
 constexpr uint32_t kSegmentLampMis = 2u;
 
+// The sky and the sun (the frame constants sky_zenith/horizon/ground, sun_dir, sun_radiance,
+// sun_cos_radius, sun_solid_angle of caelix_frame.hlsl).
+struct SkyParams
+{
+    uint32_t on;   // the procedural sky on misses and the sun disk
+    uint32_t sun;  // the sun's NEE runs: `on`, kRenderFlagSunNee and a sun radiance above 0 (caelix_path_trace.hlsl:742)
+    V3 sun_dir;
+    float sun_solid_angle;
+    V3 sun_radiance;
+    float sun_cos_radius;
+    V3 zenith;
+    V3 horizon;
+    V3 ground;
+};
+
 struct TraceParams
 {
     uint32_t bounces;
     uint32_t frame_seed;
     uint32_t mis;
     uint32_t roulette;  // 1: Russian roulette (the reference's), 0: paths run all bounces
+    SkyParams sky;
 };
+
+CX_DEV float Saturate(float x)
+{
+    return fminf(fmaxf(x, 0.0f), 1.0f);
+}
+// HLSL lerp(a, b, t) = a + t (b - a).
+CX_DEV V3 Lerp(V3 a, V3 b, float t)
+{
+    return Add(a, Mul(Sub(b, a), t));
+}
+// ProceduralSky (caelix_shading.hlsl:364-371): above the horizon lerp(horizon, zenith, sqrt(y)),
+// below it lerp(horizon, ground, sqrt(-y)); no sun disk.
+CX_DEV V3 ProceduralSky(const SkyParams& k, V3 d)
+{
+    if (d.y >= 0.0f)
+    {
+        return Lerp(k.horizon, k.zenith, Sqrt(Saturate(d.y)));
+    }
+    return Lerp(k.horizon, k.ground, Sqrt(Saturate(-d.y)));
+}
+// SunDisk (caelix_shading.hlsl:394-398).
+CX_DEV V3 SunDisk(const SkyParams& k, V3 d)
+{
+    return Dot(d, k.sun_dir) > k.sun_cos_radius ? k.sun_radiance : Make(0.0f, 0.0f, 0.0f);
+}
+// SkyRadiance (:400-404): what a primary miss shows.
+CX_DEV V3 SkyRadiance(const SkyParams& k, V3 d)
+{
+    return Add(ProceduralSky(k, d), SunDisk(k, d));
+}
+// SkyDiffuseMiss (:428-437) of a bounce ray: every segment leaves a vertex that ran the NEE
+// (segment_sees_sun = !nee, caelix_path_trace.hlsl:809, 903), so it sees the sun disk exactly when the
+// sun's NEE is off.
+CX_DEV V3 SkyMiss(const SkyParams& k, V3 d)
+{
+    const V3 sun = k.sun != 0u ? Make(0.0f, 0.0f, 0.0f) : SunDisk(k, d);
+    return Add(ProceduralSky(k, d), sun);
+}
+// SampleSunDirection (:467-485): uniform in the sun cone, cos theta = 1 - u1 (1 - cos r), phi =
+// 2 pi u2, 1 - cos r from the solid angle; the Duff et al. 2017 basis around sun_dir. sin / cos: the
+// Cephes polynomials of RandomUnitVector.
+CX_DEV V3 SampleSunDirection(const SkyParams& k, uint32_t& rng)
+{
+    const float one_minus_cos_r = k.sun_solid_angle * (1.0f / kTwoPi);
+    const float a = RandomFloat01(rng) * one_minus_cos_r;
+    const float phi = RandomFloat01(rng) * kTwoPi;
+    const float cos_theta = 1.0f - a;
+    const float sin_theta = Sqrt(fmaxf(a * (2.0f - a), 0.0f));
+    const V3 w = k.sun_dir;
+    const float sign_z = w.z >= 0.0f ? 1.0f : -1.0f;
+    const float p = Div(-1.0f, sign_z + w.z);
+    const float q = w.x * w.y * p;
+    const V3 t = Make(1.0f + sign_z * w.x * w.x * p, sign_z * q, -sign_z * w.x);
+    const V3 b = Make(q, sign_z + w.y * w.y * p, -w.y);
+    float sp, cp;
+    SinCos(phi, sp, cp);
+    return SafeNormalize(Add(Add(Mul(t, cp * sin_theta), Mul(b, sp * sin_theta)), Mul(w, cos_theta)));
+}
+// SunCandidate (caelix_path_trace.hlsl:276-290): two numbers of `rng`; the demodulated radiance of
+// the Lambert lobe (0 below the horizon of `normal`).
+CX_DEV V3 SunCandidate(const SkyParams& k, V3 normal, uint32_t& rng, V3& direction)
+{
+    direction = SampleSunDirection(k, rng);
+    const float cos_l = Dot(normal, direction);
+    if (!(cos_l > 0.0f))
+    {
+        return Make(0.0f, 0.0f, 0.0f);
+    }
+    return Mul(k.sun_radiance, k.sun_solid_angle * cos_l * (1.0f / kPi));
+}
 
 struct Shade
 {
@@ -70,13 +158,16 @@ CX_DEV WorldHit Trace(const SceneView& s, V3 o, V3 d, float t_max, PathCounters&
     return h;
 }
 
-// NextEventAtVertex (:356-413) without the sun: the lamp candidate (LampCandidate, :301-333). Returns
-// false when no shadow ray is counted (the candidate's weight is 0); else `direction` and `t_max`
-// give the shadow ray from x_r (t_max 0: counted but not traced) and `visible` the radiance it adds
-// when it reaches the light unoccluded. `unscaled` is `visible` without the light brick's emission
-// scale and `light_record` that brick (for the adjoint).
-CX_DEV bool NeeCandidate(const SceneView& s, const LightView& L, V3 x_r, V3 n_r, uint32_t& light_rng, V3& direction,
-                         float& t_max, V3& visible, V3& unscaled, uint32_t& light_record)
+// NextEventAtVertex (:356-413): the lamp candidate (LampCandidate, :301-333) and, with the sun's NEE,
+// the sun candidate (two numbers of nee_rng) and the choice (a third), kept in proportion to their
+// unoccluded luminance; the kept sun candidate is drawn again from its saved state. Returns false when
+// no shadow ray is counted (both weights 0); else `direction` and `t_max` give the shadow ray from
+// x_r (t_max 0: counted but not traced) and `visible` the radiance it adds when it reaches its light
+// unoccluded. `unscaled` is `visible` without the light brick's emission scale and `light_record` that
+// brick (for the adjoint, which runs without the sun).
+CX_DEV bool NeeCandidate(const SceneView& s, const LightView& L, const SkyParams& k, V3 x_r, V3 n_r,
+                         uint32_t& light_rng, uint32_t& nee_rng, V3& direction, float& t_max, V3& visible, V3& unscaled,
+                         uint32_t& light_record)
 {
     const uint32_t point_seed = RandomU32(light_rng);
     V3 contribution = Make(0.0f, 0.0f, 0.0f);
@@ -112,10 +203,29 @@ CX_DEV bool NeeCandidate(const SceneView& s, const LightView& L, V3 x_r, V3 n_r,
         }
     }
     const float w_lamp = Luminance(contribution) * fmaxf(0.05f, 1.0f);
-    const float w_sum = 0.0f + w_lamp;
+    const uint32_t sun_start = nee_rng;
+    float w_sun = 0.0f;
+    float u_choice = 0.0f;
+    if (k.sun != 0u)
+    {
+        V3 sun_direction;
+        w_sun = Luminance(SunCandidate(k, n_r, nee_rng, sun_direction)) * fmaxf(0.05f, 1.0f);
+        u_choice = RandomFloat01(nee_rng);
+    }
+    const float w_sum = w_sun + w_lamp;
     if (!(w_sum > 0.0f))
     {
         return false;
+    }
+    if (u_choice * w_sum < w_sun)
+    {
+        uint32_t replay = sun_start;
+        const V3 sun = SunCandidate(k, n_r, replay, direction);
+        t_max = kRayTMax;
+        visible = Mul(sun, Div(w_sum, w_sun));
+        unscaled = Make(0.0f, 0.0f, 0.0f);
+        light_record = kHashNone;
+        return true;
     }
     visible = Mul(contribution, Div(w_sum, w_lamp));
     return true;

@@ -4,6 +4,7 @@
 #include "path_trace.h"
 #include "runtime.h"
 
+#include <cmath>
 #include <vector>
 
 namespace cx {
@@ -129,6 +130,7 @@ struct Workspace
     float4* adjoint;    // [paths] adjoint: dL/d(radiance term) (rgb)
     float4* shadow_grad; // [paths] adjoint: the shadow ray's radiance without the emission scale, roulette
     uint32_t* queue[2]; // [paths] path queues (ping-pong by bounce parity)
+    uint32_t* nee_rng;  // [paths] the sun's random stream (read and written only with the sun's NEE)
     uint32_t* ctl;      // [0], [1] queue lengths, [2] shadow queue length
     float* partials;    // [kGradRows][768] adjoint: per-workgroup albedo gradient sums
 };
@@ -164,7 +166,7 @@ uint32_t GradRows(uint32_t paths)
 }
 uint64_t WorkspaceBytes(uint32_t paths)
 {
-    return 13 * Align256(uint64_t(paths) * 16) + 2 * Align256(uint64_t(paths) * 4) + 256 +
+    return 13 * Align256(uint64_t(paths) * 16) + 3 * Align256(uint64_t(paths) * 4) + 256 +
            Align256(uint64_t(GradRows(paths)) * kGradEntries * 4);
 }
 Workspace CarveWorkspace(void* base, uint32_t paths)
@@ -192,6 +194,7 @@ Workspace CarveWorkspace(void* base, uint32_t paths)
     w.shadow_grad = reinterpret_cast<float4*>(take(v));
     w.queue[0] = reinterpret_cast<uint32_t*>(take(uint64_t(paths) * 4));
     w.queue[1] = reinterpret_cast<uint32_t*>(take(uint64_t(paths) * 4));
+    w.nee_rng = reinterpret_cast<uint32_t*>(take(uint64_t(paths) * 4));
     w.ctl = reinterpret_cast<uint32_t*>(take(256));
     w.partials = reinterpret_cast<float*>(take(uint64_t(GradRows(paths)) * kGradEntries * 4));
     return w;
@@ -593,8 +596,18 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
         w.radiance[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (h.y == 0u)
         {
+            // A primary miss: the sky with the sun disk in the sky's frame (the composite's
+            // kRenderFlagSkyOnMiss, caelix_composite.hlsl:130-133), else black.
+            V3 sky = Make(0.0f, 0.0f, 0.0f);
+            if (prm.sky.on != 0u)
+            {
+                const CameraView& cam = views.Camera(views.SceneOf(path));
+                uint32_t px, py;
+                PathPixel(cam, views.Local(path), px, py);
+                sky = SkyRadiance(prm.sky, PrimaryDirection(cam, px, py));
+            }
             w.albedo[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            w.emission[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            w.emission[path] = make_float4(sky.x, sky.y, sky.z, 0.0f);
         }
         else
         {
@@ -608,6 +621,7 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
             const V3 origin = Make(cam.position[0], cam.position[1], cam.position[2]);
             uint32_t rng = InitRandom(px, py, prm.frame_seed);
             uint32_t light_rng = InitRandom(px, py, prm.frame_seed ^ 0x7F4A7C15u);
+            uint32_t nee_rng = InitRandom(px, py, prm.frame_seed ^ 0x68E31DA4u);
             const uint32_t mis_bit = L.mis != 0u ? kSegmentLampMis : 0u;
             const uint32_t face = FaceIndex(h.y);
             const V3 normal = FaceDirection(face);
@@ -628,8 +642,8 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
             if (prm.bounces >= 1u)
             {
                 V3 visible;
-                if (NeeCandidate(s, L, sh.bounce_origin, normal, light_rng, shadow_dir, shadow_t, visible, shadow_unscaled,
-                                 shadow_light))
+                if (NeeCandidate(s, L, prm.sky, sh.bounce_origin, normal, light_rng, nee_rng, shadow_dir, shadow_t, visible,
+                                 shadow_unscaled, shadow_light))
                 {
                     c.shadow_rays += 1;
                     shadow = shadow_t > 0.0f;
@@ -641,6 +655,10 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
             }
             w.throughput[path] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);  // w: the roulette's normalisation
             w.state[path] = make_uint4(rng, light_rng, PackState(segment_light, 1u, face), 0u);
+            if (prm.sky.sun != 0u)
+            {
+                w.nee_rng[path] = nee_rng;
+            }
             w.ray_o[path] = make_float4(sh.bounce_origin.x, sh.bounce_origin.y, sh.bounce_origin.z, 0.0f);
             w.ray_d[path] = make_float4(sh.bounce_direction.x, sh.bounce_direction.y, sh.bounce_direction.z, 0.0f);
         }
@@ -688,12 +706,22 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
     {
         path = w.queue[in][i];
         const uint4 h = w.hit[path];
+        if (h.y == 0u && prm.sky.on != 0u)
+        {
+            // A bounce miss: the sky the diffuse segment receives (SkyDiffuseMiss, caelix_path_trace.hlsl:840-846).
+            const float4 d4 = w.ray_d[path];
+            const float4 t4 = w.throughput[path];
+            const float4 r4 = w.radiance[path];
+            const V3 add = MulV(SkyMiss(prm.sky, Make(d4.x, d4.y, d4.z)), Make(t4.x, t4.y, t4.z));
+            w.radiance[path] = make_float4(r4.x + add.x, r4.y + add.y, r4.z + add.z, 0.0f);
+        }
         if (h.y != 0u)
         {
             const LightView& L = views.Light(views.SceneOf(path));
             uint4 st = w.state[path];
             uint32_t rng = st.x;
             uint32_t light_rng = st.y;
+            uint32_t nee_rng = prm.sky.sun != 0u ? w.nee_rng[path] : 0u;
             uint32_t segment_light = st.z & 0xFFu;
             uint32_t bounce_index = (st.z >> 8) & 0xFFu;
             const uint32_t segment_face = st.z >> 16;
@@ -739,8 +767,8 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
             {
                 segment_light = kSegmentLampNee | (L.mis != 0u ? kSegmentLampMis : 0u);
                 V3 visible;
-                if (NeeCandidate(s, L, sh.bounce_origin, vertex_normal, light_rng, shadow_dir, shadow_t, visible,
-                                 shadow_unscaled, shadow_light))
+                if (NeeCandidate(s, L, prm.sky, sh.bounce_origin, vertex_normal, light_rng, nee_rng, shadow_dir, shadow_t,
+                                 visible, shadow_unscaled, shadow_light))
                 {
                     c.shadow_rays += 1;
                     shadow = shadow_t > 0.0f;
@@ -767,6 +795,10 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
             w.radiance[path] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
             w.throughput[path] = make_float4(throughput.x, throughput.y, throughput.z, roulette);
             w.state[path] = make_uint4(rng, light_rng, PackState(segment_light, bounce_index, face), verts);
+            if (prm.sky.sun != 0u)
+            {
+                w.nee_rng[path] = nee_rng;
+            }
         }
     }
     EnqueueShadow(w, path, shadow, shadow_origin, shadow_dir, shadow_t, shadow_add, shadow_light, shadow_unscaled,
@@ -1005,6 +1037,29 @@ uint32_t PersistentBlocks(Kernel kernel)
     return blocks;
 }
 
+// The sky constants of `params`: all zero without CX_TRACE_SKY; the sun's NEE runs unless
+// CX_TRACE_NO_SUN_NEE and when a component of its radiance is above 0 (caelix_path_trace.hlsl:742).
+SkyParams SkyOf(const CxTraceParams* params)
+{
+    SkyParams k{};
+    if ((params->flags & CX_TRACE_SKY) == 0)
+    {
+        return k;
+    }
+    auto v = [](const float* x) { return V3{x[0], x[1], x[2]}; };
+    k.on = 1u;
+    const bool radiant = params->sun_radiance[0] > 0.0f || params->sun_radiance[1] > 0.0f || params->sun_radiance[2] > 0.0f;
+    k.sun = (params->flags & CX_TRACE_NO_SUN_NEE) == 0 && radiant ? 1u : 0u;
+    k.sun_dir = v(params->sun_dir);
+    k.sun_solid_angle = params->sun_solid_angle;
+    k.sun_radiance = v(params->sun_radiance);
+    k.sun_cos_radius = params->sun_cos_radius;
+    k.zenith = v(params->sky_zenith);
+    k.horizon = v(params->sky_horizon);
+    k.ground = v(params->sky_ground);
+    return k;
+}
+
 // The launches of a frame (g.d_color null) or of its adjoint replay (set), in the same order, for the
 // scenes of `views` (paths = all of their paths).
 template <class Views>
@@ -1026,7 +1081,7 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
     for (uint32_t sample = 0; sample < params->spp; ++sample)
     {
         const TraceParams p{params->bounces, params->frame_seed + sample, mis,
-                            (params->flags & CX_TRACE_NO_RUSSIAN_ROULETTE) != 0 ? 0u : 1u};
+                            (params->flags & CX_TRACE_NO_RUSSIAN_ROULETTE) != 0 ? 0u : 1u, SkyOf(params)};
         ResetKernel<<<1, 64, 0, s>>>(w);
         {
             StageScope scope(s, adjoint ? "adjoint_trace" : "trace_primary");
@@ -1088,9 +1143,32 @@ int CheckParams(const CxTraceParams* params, const CxTraceOutput* out, uint32_t 
     {
         return ArgError("%s: at most 4 bounces (3 albedo vertices per path)", what);
     }
-    if ((params->flags & ~CX_TRACE_NO_RUSSIAN_ROULETTE) != 0)
+    if ((params->flags & ~(CX_TRACE_NO_RUSSIAN_ROULETTE | CX_TRACE_SKY | CX_TRACE_NO_SUN_NEE)) != 0)
     {
         return ArgError("%s: unknown flags", what);
+    }
+    if (adjoint && (params->flags & CX_TRACE_SKY) != 0)
+    {
+        Log(3, "%s: the adjoint has no sky or sun (CX_TRACE_SKY)", what);
+        return CX_ERROR_UNSUPPORTED;
+    }
+    if ((params->flags & CX_TRACE_SKY) != 0)
+    {
+        const float* d = params->sun_dir;
+        const float n2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        if (!(std::fabs(n2 - 1.0f) < 1e-3f) || !(params->sun_solid_angle >= 0.0f) ||
+            !(params->sun_solid_angle <= 4.0f * 3.14159265f) || !(params->sun_cos_radius >= -1.0f) ||
+            !(params->sun_cos_radius <= 1.0f))
+        {
+            return ArgError("%s: sun_dir not a unit vector, or sun solid angle / cos radius out of range", what);
+        }
+        for (int c = 0; c < 3; ++c)
+        {
+            if (!(params->sun_radiance[c] >= 0.0f) || !(params->sun_radiance[c] < 3.0e38f))
+            {
+                return ArgError("%s: sun radiance negative or not finite", what);
+            }
+        }
     }
     return CX_OK;
 }

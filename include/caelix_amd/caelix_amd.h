@@ -276,7 +276,32 @@ typedef struct CxTraceParams {
     uint32_t frame_seed;
     uint32_t spp;        /* samples per pixel; sample s uses frame_seed + s */
     uint32_t flags;      /* CX_TRACE_* */
+    /* Sky and sun, read with CX_TRACE_SKY (the frame constants of shaders/common/caelix_frame.hlsl:
+     * sun_dir, sun_radiance, sun_cos_radius, sun_solid_angle, sky_zenith/horizon/ground). */
+    float sun_dir[3];        /* unit vector toward the sun */
+    float sun_solid_angle;   /* 2 pi (1 - cos r) of the sun cone (computed in double by the host) */
+    float sun_radiance[3];   /* radiance inside the sun disk, >= 0; all 0: no sun (its NEE off) */
+    float sun_cos_radius;    /* cos r */
+    float sky_zenith[3];
+    float pad0;
+    float sky_horizon[3];
+    float pad1;
+    float sky_ground[3];
+    float pad2;
 } CxTraceParams;
+
+/* The procedural sky on misses and the sun (caelix_shading.hlsl:364-403, 453-485;
+ * caelix_path_trace.hlsl:276-290, 356-413, 811-846): a primary miss shows SkyRadiance (the sky plus
+ * the sun disk; the color output, radiance 0); a bounce ray that misses adds throughput x
+ * ProceduralSky, plus the sun disk when the sun's NEE is off; with the sun's NEE every NEE vertex
+ * draws a sun candidate next to the lamp candidate and keeps one in proportion to its unoccluded
+ * luminance (one-sample RIS, one shadow ray). Without it the sky is black and there is no sun. Not
+ * supported by cx_trace_frame_adjoint (CX_ERROR_UNSUPPORTED). */
+#define CX_TRACE_SKY 2u
+
+/* With CX_TRACE_SKY: no sun NEE (the reference's kRenderFlagSunNee off); bounce rays that reach the
+ * sun disk see its radiance instead. */
+#define CX_TRACE_NO_SUN_NEE 4u
 
 /* No Russian roulette: paths run all `bounces` (the roulette's random number is still drawn, so the
  * other decisions stay the same). The estimator is then a smooth function of the albedos and the

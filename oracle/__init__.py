@@ -148,10 +148,29 @@ class _Camera(ctypes.Structure):
 
 
 class _Params(ctypes.Structure):
-    _fields_ = [("bounces", _U32), ("frame_seed", _U32), ("spp", _U32), ("flags", _U32)]
+    _F3 = ctypes.c_float * 3
+    _fields_ = [("bounces", _U32), ("frame_seed", _U32), ("spp", _U32), ("flags", _U32), ("sun_dir", _F3),
+                ("sun_solid_angle", ctypes.c_float), ("sun_radiance", _F3), ("sun_cos_radius", ctypes.c_float),
+                ("sky_zenith", _F3), ("pad0", ctypes.c_float), ("sky_horizon", _F3), ("pad1", ctypes.c_float),
+                ("sky_ground", _F3), ("pad2", ctypes.c_float)]
 
 
 NO_RUSSIAN_ROULETTE = 1
+SKY = 2
+NO_SUN_NEE = 4
+
+
+def _params(bounces: int, seed: int, spp: int, flags: int, sky: dict | None) -> _Params:
+    """The trace parameters; `sky` (keys sun_dir, sun_radiance, sun_solid_angle, sun_cos_radius,
+    sky_zenith, sky_horizon, sky_ground) sets the SKY flag and the sky fields."""
+    prm = _Params(bounces, seed, spp, flags)
+    if sky is not None:
+        prm.flags |= SKY
+        for name in ("sun_dir", "sun_radiance", "sky_zenith", "sky_horizon", "sky_ground"):
+            getattr(prm, name)[:] = [float(v) for v in sky[name]]
+        prm.sun_solid_angle = float(sky["sun_solid_angle"])
+        prm.sun_cos_radius = float(sky["sun_cos_radius"])
+    return prm
 
 
 def _trace_lib():
@@ -344,9 +363,10 @@ class OracleScene:
                                                 ctypes.byref(prm), r0, r1, _ptr(d), _ptr(ga), _ptr(gs))
         return ga.reshape(256, 3), gs
 
-    def trace_frame(self, camera: dict, bounces: int, seed: int, spp: int = 1, rows=None, flags: int = 0):
+    def trace_frame(self, camera: dict, bounces: int, seed: int, spp: int = 1, rows=None, flags: int = 0,
+                    sky: dict | None = None):
         cam = camera_struct(camera)
-        prm = _Params(bounces, seed, spp, flags)
+        prm = _params(bounces, seed, spp, flags, sky)
         r0, r1 = rows if rows is not None else (0, cam.height)
         count = (r1 - r0) * cam.width
         color = np.zeros(count * 4, np.float64)

@@ -1010,6 +1010,100 @@ float BrickScale(const CxScene& s, uint32_t record)
     return s.brick_emission_scale != nullptr ? s.brick_emission_scale[record] : 1.0f;
 }
 
+// The procedural sky and the sun of CX_TRACE_SKY (the frame constants sky_zenith/horizon/ground,
+// sun_dir, sun_radiance, sun_cos_radius, sun_solid_angle of shaders/render/caelix_frame.hlsl).
+struct Sky
+{
+    bool on = false;
+    bool sun = false;  // the sun's NEE: `on`, kRenderFlagSunNee, a sun radiance above 0 (caelix_path_trace.hlsl:742)
+    V sun_dir, sun_radiance, zenith, horizon, ground;
+    float sun_solid_angle = 0.0f, sun_cos_radius = 1.0f;
+};
+Sky SkyOf(const CxTraceParams& p)
+{
+    Sky k;
+    k.sun_dir = k.sun_radiance = k.zenith = k.horizon = k.ground = Mk(0, 0, 0);
+    if ((p.flags & CX_TRACE_SKY) == 0u)
+    {
+        return k;
+    }
+    k.on = true;
+    k.sun = (p.flags & CX_TRACE_NO_SUN_NEE) == 0u &&
+            (p.sun_radiance[0] > 0.0f || p.sun_radiance[1] > 0.0f || p.sun_radiance[2] > 0.0f);
+    k.sun_dir = Mk(p.sun_dir[0], p.sun_dir[1], p.sun_dir[2]);
+    k.sun_radiance = Mk(p.sun_radiance[0], p.sun_radiance[1], p.sun_radiance[2]);
+    k.zenith = Mk(p.sky_zenith[0], p.sky_zenith[1], p.sky_zenith[2]);
+    k.horizon = Mk(p.sky_horizon[0], p.sky_horizon[1], p.sky_horizon[2]);
+    k.ground = Mk(p.sky_ground[0], p.sky_ground[1], p.sky_ground[2]);
+    k.sun_solid_angle = p.sun_solid_angle;
+    k.sun_cos_radius = p.sun_cos_radius;
+    return k;
+}
+float Saturate(float x)
+{
+    return std::fmin(std::fmax(x, 0.0f), 1.0f);
+}
+// lerp(a, b, t) = a + t (b - a), per component.
+V Lerp(V a, V b, float t)
+{
+    return Add(a, Mul(Sub(b, a), t));
+}
+// ProceduralSky (shaders/render/caelix_shading.hlsl:364-371).
+V ProceduralSky(const Sky& k, V d)
+{
+    if (d.y >= 0.0f)
+    {
+        return Lerp(k.horizon, k.zenith, std::sqrt(Saturate(d.y)));
+    }
+    return Lerp(k.horizon, k.ground, std::sqrt(Saturate(-d.y)));
+}
+// SunDisk (caelix_shading.hlsl:394-398).
+V SunDisk(const Sky& k, V d)
+{
+    return Dot(d, k.sun_dir) > k.sun_cos_radius ? k.sun_radiance : Mk(0, 0, 0);
+}
+// SkyRadiance (caelix_shading.hlsl:400-404), the primary miss (caelix_composite.hlsl:130-133).
+V SkyRadiance(const Sky& k, V d)
+{
+    return Add(ProceduralSky(k, d), SunDisk(k, d));
+}
+// SkyDiffuseMiss (caelix_shading.hlsl:428-437) with the procedural sky; a bounce segment leaves a
+// vertex that ran the NEE, so segment_sees_sun = !sun (caelix_path_trace.hlsl:809, 903).
+V SkyMiss(const Sky& k, V d)
+{
+    return Add(ProceduralSky(k, d), k.sun ? Mk(0, 0, 0) : SunDisk(k, d));
+}
+// SampleSunDirection (caelix_shading.hlsl:467-485): uniform in the cone of solid angle
+// sun_solid_angle around sun_dir in the branchless basis of Duff et al. 2017.
+V SampleSunDirection(const Sky& k, uint32_t& rng)
+{
+    const float one_minus_cos_r = k.sun_solid_angle * (1.0f / kTwoPi);
+    const float a = RandomFloat01(rng) * one_minus_cos_r;
+    const float phi = RandomFloat01(rng) * kTwoPi;
+    const float cos_theta = 1.0f - a;
+    const float sin_theta = std::sqrt(std::fmax(a * (2.0f - a), 0.0f));
+    const V w = k.sun_dir;
+    const float sign_z = w.z >= 0.0f ? 1.0f : -1.0f;
+    const float p = -1.0f / (sign_z + w.z);
+    const float q = w.x * w.y * p;
+    const V t = Mk(1.0f + sign_z * w.x * w.x * p, sign_z * q, -sign_z * w.x);
+    const V b = Mk(q, sign_z + w.y * w.y * p, -w.y);
+    float sp, cp;
+    SinCos(phi, sp, cp);
+    return SafeNormalize(Add(Add(Mul(t, cp * sin_theta), Mul(b, sp * sin_theta)), Mul(w, cos_theta)));
+}
+// SunCandidate (caelix_path_trace.hlsl:276-290).
+V SunCandidate(const Sky& k, V normal, uint32_t& rng, V& direction)
+{
+    direction = SampleSunDirection(k, rng);
+    const float cos_l = Dot(normal, direction);
+    if (!(cos_l > 0.0f))
+    {
+        return Mk(0, 0, 0);
+    }
+    return Mul(k.sun_radiance, k.sun_solid_angle * cos_l * (1.0f / kPi));
+}
+
 // What the adjoint needs of one NEE: the radiance it adds without the light brick's emission scale,
 // that brick, and whether it was added (unoccluded).
 struct NeeTerm
@@ -1019,7 +1113,10 @@ struct NeeTerm
     uint32_t light = kNoneU;
 };
 
-V NextEvent(const CxScene& s, const CxLightTreeView& L, V x_r, V n_r, uint32_t& light_rng, Counters& c, NeeTerm& term)
+// NextEventAtVertex (caelix_path_trace.hlsl:356-413): the lamp candidate and, with the sun's NEE,
+// the sun candidate and the choice between them (one-sample RIS), then one shadow ray.
+V NextEvent(const CxScene& s, const CxLightTreeView& L, const Sky& k, V x_r, V n_r, uint32_t& light_rng,
+            uint32_t& nee_rng, Counters& c, NeeTerm& term)
 {
     term = NeeTerm{};
     term.unscaled = Mk(0, 0, 0);
@@ -1056,9 +1153,35 @@ V NextEvent(const CxScene& s, const CxLightTreeView& L, V x_r, V n_r, uint32_t& 
         }
     }
     const float w_lamp = Luminance(contribution) * std::fmax(0.05f, 1.0f);
-    const float w_sum = 0.0f + w_lamp;
+    const uint32_t sun_start = nee_rng;
+    float w_sun = 0.0f;
+    float u_choice = 0.0f;
+    if (k.sun)
+    {
+        V sun_direction;
+        w_sun = Luminance(SunCandidate(k, n_r, nee_rng, sun_direction)) * std::fmax(0.05f, 1.0f);
+        u_choice = RandomFloat01(nee_rng);
+    }
+    const float w_sum = w_sun + w_lamp;
     V radiance = Mk(0, 0, 0);
-    if (w_sum > 0.0f)
+    if (w_sum > 0.0f && u_choice * w_sum < w_sun)
+    {
+        // The sun candidate, drawn again from its saved state; its shadow ray runs to kRayTMax.
+        c.shadow += 1;
+        uint32_t replay = sun_start;
+        const V sun = SunCandidate(k, n_r, replay, direction);
+        term = NeeTerm{};
+        term.unscaled = Mk(0, 0, 0);
+        const int kind = g_kind;
+        g_kind = 2;
+        const WorldHit h = Trace(s, x_r, direction, kRayTMax, c);
+        g_kind = kind;
+        if (h.attrib == 0u)
+        {
+            radiance = Mul(sun, w_sum / w_sun);
+        }
+    }
+    else if (w_sum > 0.0f)
     {
         c.shadow += 1;
         if (t_max > 0.0f)
@@ -1160,8 +1283,8 @@ void AdjointAlbedoTerms(const CxScene& s, Adjoint& g, const double adj[3], const
 // `adjoint`, also the path-replay gradients of the sample (sampling pdfs, MIS weights and the
 // roulette's normalisation held constant).
 void TracePixel(const CxScene& s, const CxLightTreeView& L, const CxCamera& cam, uint32_t bounces, uint32_t seed,
-                uint32_t flags, uint32_t px, uint32_t py, DV& radiance, V& albedo, V& emission, float& t_hit,
-                uint32_t& attrib, Counters& c, Adjoint* adjoint = nullptr)
+                uint32_t flags, const Sky& k, uint32_t px, uint32_t py, DV& radiance, V& albedo, V& emission,
+                float& t_hit, uint32_t& attrib, Counters& c, Adjoint* adjoint = nullptr)
 {
     const bool roulette = (flags & CX_TRACE_NO_RUSSIAN_ROULETTE) == 0u;
     const float sx = float(px) + 0.5f;
@@ -1183,12 +1306,17 @@ void TracePixel(const CxScene& s, const CxLightTreeView& L, const CxCamera& cam,
     g_kind = 3;
     if (primary.attrib == 0u)
     {
+        if (k.on)
+        {
+            emission = SkyRadiance(k, dir);
+        }
         return;
     }
     t_hit = primary.t;
     attrib = primary.attrib;
     uint32_t rng = InitRandom(px, py, seed);
     uint32_t light_rng = InitRandom(px, py, seed ^ 0x7F4A7C15u);
+    uint32_t nee_rng = InitRandom(px, py, seed ^ 0x68E31DA4u);
     const uint32_t mis_bit = L.mis != 0u ? 2u : 0u;
     uint32_t segment_light = 0u;
     const V primary_normal = FaceDirection(LowBit(primary.attrib & 0x3Fu));
@@ -1233,7 +1361,7 @@ void TracePixel(const CxScene& s, const CxLightTreeView& L, const CxCamera& cam,
     if (bounces >= 1u)
     {
         NeeTerm term;
-        AddD(radiance, MulV(NextEvent(s, L, ps.origin, primary_normal, light_rng, c, term), throughput));
+        AddD(radiance, MulV(NextEvent(s, L, k, ps.origin, primary_normal, light_rng, nee_rng, c, term), throughput));
         nee_adjoint(term);
         segment_light = 1u | mis_bit;
     }
@@ -1248,6 +1376,10 @@ void TracePixel(const CxScene& s, const CxLightTreeView& L, const CxCamera& cam,
         g_kind = 4;
         if (hit.attrib == 0u)
         {
+            if (k.on)
+            {
+                AddD(radiance, MulV(SkyMiss(k, bd), throughput));
+            }
             break;
         }
         const V vn = FaceDirection(LowBit(hit.attrib & 0x3Fu));
@@ -1285,7 +1417,7 @@ void TracePixel(const CxScene& s, const CxLightTreeView& L, const CxCamera& cam,
         {
             segment_light = 1u | mis_bit;
             NeeTerm term;
-            AddD(radiance, MulV(NextEvent(s, L, sh.origin, vn, light_rng, c, term), Mul(throughput, 1.0f)));
+            AddD(radiance, MulV(NextEvent(s, L, k, sh.origin, vn, light_rng, nee_rng, c, term), Mul(throughput, 1.0f)));
             nee_adjoint(term);
         }
         const float cp = std::fmax(throughput.x, std::fmax(throughput.y, throughput.z));
@@ -1608,6 +1740,7 @@ void oracle_trace_frame(const CxScene* s, const CxLightTreeView* L, const CxCame
 {
     uint64_t rays = 0, shadow = 0, cells = 0, probes = 0, steps = 0;
     uint64_t d0 = 0, d1 = 0, d2 = 0, d3 = 0, d4 = 0;
+    const Sky sky = SkyOf(*prm);
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : rays, shadow, cells, probes, steps, d0, d1, d2, d3, d4)
     for (int64_t py = row_begin; py < int64_t(row_end); ++py)
     {
@@ -1625,8 +1758,8 @@ void oracle_trace_frame(const CxScene* s, const CxLightTreeView* L, const CxCame
             {
                 DV r;
                 V albedo, emission;
-                TracePixel(*s, *L, *cam, prm->bounces, prm->frame_seed + sample, prm->flags, px, uint32_t(py), r, albedo,
-                           emission, t, attrib, local);
+                TracePixel(*s, *L, *cam, prm->bounces, prm->frame_seed + sample, prm->flags, sky, px, uint32_t(py), r,
+                           albedo, emission, t, attrib, local);
                 col.x += double(emission.x) + double(albedo.x) * r.x;
                 col.y += double(emission.y) + double(albedo.y) * r.y;
                 col.z += double(emission.z) + double(albedo.z) * r.z;
@@ -1685,6 +1818,7 @@ void oracle_trace_frame_adjoint(const CxScene* s, const CxLightTreeView* L, cons
     const int threads = omp_get_max_threads();
     std::vector<std::vector<double>> albedo_parts(threads, std::vector<double>(256 * 3, 0.0));
     std::vector<std::vector<double>> scale_parts(threads, std::vector<double>(s->bricks, 0.0));
+    const Sky sky = SkyOf(*prm);
 #pragma omp parallel for schedule(dynamic, 1)
     for (int64_t py = row_begin; py < int64_t(row_end); ++py)
     {
@@ -1706,8 +1840,8 @@ void oracle_trace_frame_adjoint(const CxScene* s, const CxLightTreeView* L, cons
                 V albedo, emission;
                 float th;
                 uint32_t attrib;
-                TracePixel(*s, *L, *cam, prm->bounces, prm->frame_seed + sample, prm->flags, px, uint32_t(py), r, albedo,
-                           emission, th, attrib, local, &adj);
+                TracePixel(*s, *L, *cam, prm->bounces, prm->frame_seed + sample, prm->flags, sky, px, uint32_t(py), r,
+                           albedo, emission, th, attrib, local, &adj);
             }
         }
     }

@@ -1,0 +1,194 @@
+"""Pins the oracle's procedural sky and sun (CX_TRACE_SKY) without the GPU, on a flat floor under an
+open sky where the expected radiance has a closed form:
+
+  sun straight overhead, sun NEE     E[radiance] = L_sun sin^2 r (the cone's cosine-weighted solid
+                                     angle / pi: Omega (1 + cos r) / (2 pi) = 1 - cos^2 r), the same
+                                     for 1 and 4 bounces (the bounce ray leaves into a black sky)
+  sun NEE off                        the bounce ray finds the disk with probability sin^2 r: the same
+                                     expectation (statistical)
+  constant sky C, no sun             radiance = C exactly (every bounce ray misses with throughput 1)
+  graded sky, no sun                 E[radiance] = H + 0.8 (Z - H): a cosine-weighted direction has
+                                     cos theta = sqrt(u), so E[sqrt(cos theta)] = E[u^(1/4)] = 4/5
+  primary misses                     color = ProceduralSky(dir) + the sun disk, recomputed in numpy
+  sun + lamp (one-sample RIS)        E[both] = E[lamp alone] + E[sun alone] (unbiasedness of the
+                                     choice weight w_sum / w_picked; statistical)
+
+The procedural sky and sun follow shaders/render/caelix_shading.hlsl:364-485 and
+caelix_path_trace.hlsl:276-290, 356-413, 737-846."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from caelix_amd import scene
+
+LOAD = 0.7
+
+
+def sky(sun_dir=(0.0, 1.0, 0.0), sun_radiance=(0.0, 0.0, 0.0), radius_deg=0.5, zenith=(0, 0, 0), horizon=(0, 0, 0),
+        ground=(0, 0, 0)) -> dict:
+    d = np.asarray(sun_dir, np.float64)
+    d = d / np.linalg.norm(d)
+    r = np.radians(radius_deg)
+    return {"sun_dir": d, "sun_radiance": sun_radiance, "sun_solid_angle": 2 * np.pi * (1 - np.cos(r)),
+            "sun_cos_radius": np.cos(r), "sky_zenith": zenith, "sky_horizon": horizon, "sky_ground": ground}
+
+
+def build(oracle, data):
+    return oracle.OracleScene.build(data, int(np.ceil(len(data.keys) / LOAD)))
+
+
+@pytest.fixture(scope="module")
+def floor(oracle):
+    data = scene.sun_floor(lamp="below")
+    return data, build(oracle, data)
+
+
+def test_sky_params_helper_matches(cx):
+    """The package's sky_params gives the fields this file's helper gives."""
+    got = cx.sky_params(sun_dir=(1.0, 2.0, 0.5), sun_radiance=(3.0, 2.0, 1.0), sun_radius_deg=1.5,
+                        zenith=(0.1, 0.2, 0.3), horizon=(0.4, 0.5, 0.6), ground=(0.7, 0.8, 0.9))
+    want = sky((1.0, 2.0, 0.5), (3.0, 2.0, 1.0), 1.5, (0.1, 0.2, 0.3), (0.4, 0.5, 0.6), (0.7, 0.8, 0.9))
+    for k, v in want.items():
+        assert np.allclose(np.asarray(got[k], np.float64), np.asarray(v, np.float64), rtol=1e-12, atol=0), k
+
+
+@pytest.mark.parametrize("bounces", [1, 4])
+def test_sun_floor_closed_form(floor, bounces):
+    data, ref = floor
+    L, r_deg, spp = 1000.0, 2.0, 4
+    k = sky(sun_radiance=(L, L, L), radius_deg=r_deg)
+    color, radiance, attrib, counters = ref.trace_frame(data.camera, bounces, 7, spp, sky=k)
+    assert np.all(attrib != 0)
+    albedo = np.float32(123 / 255)
+    rad = radiance[:, 0] / spp
+    want = L * np.sin(np.radians(r_deg)) ** 2
+    # cos_l lies in [cos r, 1]: per-sample relative spread below 2e-4, the mean is far tighter
+    assert abs(rad.mean() - want) < 2e-5 * want, (rad.mean(), want)
+    assert abs(color[:, 0].mean() / spp - float(albedo) * want) < 2e-5 * want
+    assert counters[1] == data.camera["width"] * data.camera["height"] * spp  # one sun shadow ray per pixel
+
+
+def test_sun_floor_bounces_agree_and_zero_without_nee(floor):
+    data, ref = floor
+    k = sky(sun_radiance=(1000.0, 800.0, 600.0), radius_deg=2.0)
+    one = ref.trace_frame(data.camera, 1, 11, 2, sky=k)
+    four = ref.trace_frame(data.camera, 4, 11, 2, sky=k)
+    assert np.array_equal(one[1], four[1]) and np.array_equal(one[0], four[0])
+    zero = ref.trace_frame(data.camera, 0, 11, 2, sky=k)
+    assert np.all(zero[1] == 0.0) and np.all(zero[0][:, :3] == 0.0)
+
+
+def test_sun_disk_seen_by_bounce_rays_without_sun_nee(oracle, floor):
+    data, ref = floor
+    L, r_deg, spp = 100.0, 10.0, 16
+    k = sky(sun_radiance=(L, L, L), radius_deg=r_deg)
+    cam = dict(data.camera)
+    cam["width"] = cam["height"] = 96
+    _, radiance, _, counters = ref.trace_frame(cam, 1, 3, spp, sky=k, flags=oracle.NO_SUN_NEE)
+    assert counters[1] == 0  # no shadow ray: the lamp is out of reach and the sun is not sampled
+    samples = radiance[:, 0] / spp
+    want = L * np.sin(np.radians(r_deg)) ** 2
+    se = samples.std() / np.sqrt(len(samples))
+    assert se > 0 and abs(samples.mean() - want) < 5 * se, (samples.mean(), want, se)
+    # every sample is 0 or L (a bounce ray in the disk): pixel sums are multiples of L
+    sums = radiance[:, 0]
+    assert np.allclose(sums / L, np.round(sums / L), atol=1e-9)
+
+
+@pytest.mark.parametrize("bounces", [1, 3])
+def test_constant_sky_floor_exact(floor, bounces):
+    data, ref = floor
+    c = (0.3, 0.5, 0.7)
+    color, radiance, attrib, counters = ref.trace_frame(data.camera, bounces, 5, 1, sky=sky(zenith=c, horizon=c, ground=c))
+    assert np.array_equal(radiance[:, :3], np.tile(np.float32(c).astype(np.float64), (len(radiance), 1)))
+    albedo = np.float64(np.float32(123 / 255))
+    assert np.allclose(color[:, :3], albedo * np.float32(c).astype(np.float64), rtol=1e-15, atol=0)
+    assert counters[1] == 0  # no lamp in reach, no sun: no shadow ray
+
+
+def test_graded_sky_floor_expectation(floor):
+    data, ref = floor
+    zenith, horizon = np.array([0.2, 0.4, 1.0]), np.array([1.0, 0.8, 0.6])
+    spp = 8
+    cam = dict(data.camera)
+    cam["width"] = cam["height"] = 96
+    _, radiance, _, _ = ref.trace_frame(cam, 2, 21, spp, sky=sky(zenith=zenith, horizon=horizon, ground=(5, 5, 5)))
+    samples = radiance[:, :3] / spp
+    want = horizon + 0.8 * (zenith - horizon)
+    se = samples.std(axis=0) / np.sqrt(len(samples))
+    assert np.all(np.abs(samples.mean(axis=0) - want) < 5 * se + 1e-6), (samples.mean(axis=0), want, se)
+    # without the square root the expectation would be H + (2/3)(Z - H): far outside the band
+    wrong = horizon + (2 / 3) * (zenith - horizon)
+    assert np.all(np.abs(wrong - want) > 50 * se)
+
+
+def _primary_dirs(cam):
+    """The oracle's primary directions in float32, operation by operation."""
+    f32 = np.float32
+    w, h = cam["width"], cam["height"]
+    px, py = np.meshgrid(np.arange(w), np.arange(h))
+    px, py = px.ravel(), py.ravel()
+    ndc_x = (((px.astype(f32) + f32(0.5)) / f32(w)) * f32(2.0)) - f32(1.0)
+    ndc_y = f32(1.0) - ((py.astype(f32) + f32(0.5)) / f32(h)) * f32(2.0)
+    t, a = f32(cam["tan_half_fov_y"]), f32(cam["aspect"])
+    right, up, fwd = (np.asarray(cam[n], f32) for n in ("right", "up", "forward"))
+    sx = (ndc_x * t) * a
+    sy = ndc_y * t
+    v = (right[None, :] * sx[:, None] + up[None, :] * sy[:, None]) + fwd[None, :]
+    dp = np.maximum(f32(1.175494351e-38), (v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]) + v[:, 2] * v[:, 2])
+    return v * (f32(1.0) / np.sqrt(dp))[:, None]
+
+
+def test_primary_miss_shows_sky_and_sun_disk(floor):
+    data, ref = floor
+    sun_dir = np.array([0.3, 0.8, 0.2]) / np.linalg.norm([0.3, 0.8, 0.2])
+    position = np.array([64.0, 40.0, 64.0])
+    cam = scene.look_at(position, position + sun_dir * 100, 64, 64, fov_y_deg=30.0)
+    zenith, horizon, ground = (0.1, 0.3, 0.9), (0.8, 0.7, 0.6), (0.2, 0.1, 0.0)
+    k = sky(sun_dir=sun_dir, sun_radiance=(50.0, 40.0, 30.0), radius_deg=2.0, zenith=zenith, horizon=horizon,
+            ground=ground)
+    color, radiance, attrib, counters = ref.trace_frame(cam, 2, 1, 1, sky=k)
+    assert np.all(attrib == 0) and np.all(radiance == 0.0) and np.all(color[:, 3] == 0.0)
+    assert counters[1] == 0
+    d = _primary_dirs(cam).astype(np.float64)
+    y = d[:, 1:2]
+    want = np.where(y >= 0, np.asarray(horizon) + (np.asarray(zenith) - horizon) * np.sqrt(np.clip(y, 0, 1)),
+                    np.asarray(horizon) + (np.asarray(ground) - horizon) * np.sqrt(np.clip(-y, 0, 1)))
+    cos_sun = d @ np.float32(sun_dir).astype(np.float64)
+    margin = np.abs(cos_sun - np.cos(np.radians(2.0))) > 1e-6
+    in_disk = cos_sun > np.cos(np.radians(2.0))
+    want = want + np.where(in_disk[:, None], np.array([50.0, 40.0, 30.0]), 0.0)
+    assert in_disk.sum() > 20  # the disk covers a few dozen pixels
+    assert np.allclose(color[margin, :3], want[margin], rtol=1e-5, atol=1e-6)
+
+
+def test_sun_and_lamp_choice_is_unbiased(oracle):
+    """One shadow ray per vertex, sun or lamp kept in proportion to its unoccluded luminance and
+    weighted by w_sum / w_kept: the image mean with both lights equals the sum of the means with each
+    alone (the lamp switched off by an emission scale of 0), to five standard errors."""
+    data = scene.sun_floor(lamp="above", lamp_emission=30.0)
+    ref = build(oracle, data)
+    cam = dict(data.camera)
+    cam["width"] = cam["height"] = 96
+    spp = 8
+    k = sky(sun_dir=(0.4, 1.0, 0.3), sun_radiance=(200.0, 200.0, 200.0), radius_deg=3.0)
+    lamp_brick = int(data.emissive_bricks[0])
+
+    def mean_se(**kw):
+        _, rad, _, _ = ref.trace_frame(cam, 1, 41, spp, **kw)
+        px = rad[:, 0] / spp
+        return px.mean(), px.std() / np.sqrt(len(px))
+
+    both, se_b = mean_se(sky=k)
+    lamp, se_l = mean_se()
+    scale = np.ones(len(data.keys), np.float32)
+    scale[lamp_brick] = 0.0
+    ref.set_parameters(brick_scale=scale)
+    try:
+        sun, se_s = mean_se(sky=k)
+    finally:
+        ref.set_parameters(brick_scale=None)
+    tol = 5 * np.sqrt(se_b ** 2 + se_l ** 2 + se_s ** 2)
+    assert lamp > 10 * tol and sun > 10 * tol  # both lights matter
+    assert abs(both - (lamp + sun)) < tol, (both, lamp, sun, tol)
