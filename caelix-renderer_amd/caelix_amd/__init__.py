@@ -99,7 +99,8 @@ class CxTraceParams(ctypes.Structure):
                 ("flags", ctypes.c_uint32), ("sun_dir", ctypes.c_float * 3), ("sun_solid_angle", ctypes.c_float),
                 ("sun_radiance", ctypes.c_float * 3), ("sun_cos_radius", ctypes.c_float),
                 ("sky_zenith", ctypes.c_float * 3), ("pad0", ctypes.c_float), ("sky_horizon", ctypes.c_float * 3),
-                ("pad1", ctypes.c_float), ("sky_ground", ctypes.c_float * 3), ("pad2", ctypes.c_float)]
+                ("pad1", ctypes.c_float), ("sky_ground", ctypes.c_float * 3), ("pad2", ctypes.c_float),
+                ("fog_color", ctypes.c_float * 3), ("fog_density", ctypes.c_float)]
 
 
 TRACE_NO_RUSSIAN_ROULETTE = 1
@@ -120,8 +121,12 @@ def sky_params(sun_dir=(0.0, 1.0, 0.0), sun_radiance=(0.0, 0.0, 0.0), sun_radius
             "sky_ground": [float(v) for v in ground]}
 
 
-def apply_sky(params: CxTraceParams, sky: dict | None) -> None:
-    """Sets the CX_TRACE_SKY flag and the sky fields of `params` (sky None: black sky, no sun)."""
+def apply_sky(params: CxTraceParams, sky: dict | None, fog=None) -> None:
+    """Sets the CX_TRACE_SKY flag and the sky fields of `params` (sky None: black sky, no sun) and
+    the fog of the air, fog = (rgb radiance, density) (None: no fog)."""
+    if fog is not None:
+        params.fog_color[:] = [float(v) for v in fog[0]]
+        params.fog_density = float(fog[1])
     if sky is None:
         return
     params.flags |= TRACE_SKY
@@ -585,13 +590,14 @@ class TraceScene:
                              self.workspace.data_ptr(), self.workspace.numel())
 
     def trace_frame_adjoint(self, camera: dict, bounces: int, seed: int, spp: int, d_color, grad_albedo, grad_scale,
-                            flags: int = 0, counters=None, stream=None):
+                            flags: int = 0, counters=None, stream=None, fog=None):
         """Adds the path-replay gradients for d_color = dL/dI (rgba per pixel, float32 device tensor)
         to grad_albedo ([256, 4] float32) and grad_scale ([bricks] float32)."""
         torch = _torch()
         counters = counters if counters is not None else torch.zeros(8, dtype=torch.int64, device="cuda")
         cam = camera_struct(camera)
         params = CxTraceParams(bounces, seed, spp, flags)
+        apply_sky(params, None, fog)
         grads = CxTraceGradients(d_color.data_ptr(), grad_albedo.data_ptr(), grad_scale.data_ptr())
         out = self._output(camera, None, None, None, counters)
         check(self.lib.cx_trace_frame_adjoint(ctypes.byref(self.scene), ctypes.byref(self.view), ctypes.byref(cam),
@@ -600,7 +606,7 @@ class TraceScene:
         return counters
 
     def trace_frame(self, camera: dict, bounces: int, seed: int, spp: int = 1, color=None, radiance=None,
-                    attrib=None, counters=None, stream=None, flags: int = 0, sky: dict | None = None):
+                    attrib=None, counters=None, stream=None, flags: int = 0, sky: dict | None = None, fog=None):
         torch = _torch()
         w, h = int(camera["width"]), int(camera["height"])
         color = color if color is not None else torch.empty(w * h * 4, dtype=torch.float32, device="cuda")
@@ -609,7 +615,7 @@ class TraceScene:
         counters = counters if counters is not None else torch.zeros(8, dtype=torch.int64, device="cuda")
         cam = camera_struct(camera)
         params = CxTraceParams(bounces, seed, spp, flags)
-        apply_sky(params, sky)
+        apply_sky(params, sky, fog)
         out = self._output(camera, color, radiance, attrib, counters)
         check(self.lib.cx_trace_frame(ctypes.byref(self.scene), ctypes.byref(self.view), ctypes.byref(cam),
                                       ctypes.byref(params), ctypes.byref(out), stream_handle(stream)), "cx_trace_frame")

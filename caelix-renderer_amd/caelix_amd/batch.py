@@ -166,9 +166,10 @@ class TraceBatch:
         """Rebuild every scene's light tree: one batched build (the headers land in the views)."""
         self.tree_batch.build(stream)
 
-    def trace(self, bounces: int, seed: int, spp: int = 1, flags: int = 0, stream=None, sky: dict | None = None):
+    def trace(self, bounces: int, seed: int, spp: int = 1, flags: int = 0, stream=None, sky: dict | None = None,
+              fog=None):
         params = CxTraceParams(bounces, seed, spp, flags)
-        apply_sky(params, sky)
+        apply_sky(params, sky, fog)
         out = CxTraceOutput(self.color.data_ptr(), self.radiance.data_ptr(), self.attrib.data_ptr(),
                             self.counters.data_ptr(), self.workspace.data_ptr(), self.workspace.numel())
         check(self.lib.cx_trace_batch(ctypes.byref(self.scene), self.device_views.data_ptr(), self.count, self.width,
@@ -198,10 +199,10 @@ class PlainScenes:
         self.counters = torch.zeros(8, dtype=torch.int64, device="cuda")
         self.pixels = pixels
 
-    def step(self, bounces: int, seed: int, spp: int = 1, stream=None, sky: dict | None = None) -> None:
+    def step(self, bounces: int, seed: int, spp: int = 1, stream=None, sky: dict | None = None, fog=None) -> None:
         p = self.pixels
         for i, (ts, cam) in enumerate(zip(self.scenes, self.cameras)):
             ts.tree.build(stream)
             ts.trace_frame(cam, bounces, seed, spp, color=self.color[i * p * 4:(i + 1) * p * 4],
                            radiance=self.radiance[i * p * 4:(i + 1) * p * 4], attrib=self.attrib[i * p:(i + 1) * p],
-                           counters=self.counters, stream=stream, sky=sky)
+                           counters=self.counters, stream=stream, sky=sky, fog=fog)

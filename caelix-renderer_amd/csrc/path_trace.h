@@ -34,7 +34,19 @@ struct TraceParams
     uint32_t mis;
     uint32_t roulette;  // 1: Russian roulette (the reference's), 0: paths run all bounces
     SkyParams sky;
+    V3 fog;             // the fog's radiance (fog_color.rgb)
+    float fog_density;  // fog_color.w; 0: no fog
 };
+
+// TraverseSegment (caelix_shading.hlsl:150-175) in the air: a segment of length t that ends on a
+// surface keeps exp(-density t) of the throughput and adds the fog's emission with the throughput
+// before it.
+CX_DEV void FogSegment(const TraceParams& p, float t, V3& throughput, V3& radiance)
+{
+    const float transmission = ExpNonPositive((-p.fog_density) * t);
+    radiance = Add(radiance, Mul(MulV(throughput, p.fog), 1.0f - transmission));
+    throughput = Mul(throughput, transmission);
+}
 
 CX_DEV float Saturate(float x)
 {

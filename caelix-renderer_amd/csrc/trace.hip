@@ -627,8 +627,24 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
             const V3 normal = FaceDirection(face);
             const Shade sh = ShadeHit(s, (h.y >> 16) & 0xFFFFu, Add(origin, Mul(dir, t)), normal, rng);
             const V3 emission = Mul(sh.emission, BrickScale(s, h.z));
-            w.albedo[path] = make_float4(sh.albedo.x, sh.albedo.y, sh.albedo.z, t);
-            w.emission[path] = make_float4(emission.x, emission.y, emission.z, __uint_as_float(h.y));
+            if (prm.fog_density > 0.0f)
+            {
+                // The camera segment in the air (the delta chain's TraverseSegment): its transmission
+                // multiplies what the surface sends, its fog emission is deterministic
+                // (caelix_path_trace.hlsl:1273-1283).
+                V3 chain = Make(1.0f, 1.0f, 1.0f);
+                V3 inscatter = Make(0.0f, 0.0f, 0.0f);
+                FogSegment(prm, t, chain, inscatter);
+                const V3 albedo = MulV(chain, sh.albedo);
+                const V3 deterministic = Add(MulV(chain, emission), inscatter);
+                w.albedo[path] = make_float4(albedo.x, albedo.y, albedo.z, t);
+                w.emission[path] = make_float4(deterministic.x, deterministic.y, deterministic.z, __uint_as_float(h.y));
+            }
+            else
+            {
+                w.albedo[path] = make_float4(sh.albedo.x, sh.albedo.y, sh.albedo.z, t);
+                w.emission[path] = make_float4(emission.x, emission.y, emission.z, __uint_as_float(h.y));
+            }
             if (g.d_color != nullptr)
             {
                 const float4 d = g.d_color[views.PixelBase(scene) + uint64_t(py) * cam.width + px];
@@ -739,6 +755,10 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
             hit.t = __uint_as_float(h.x);
             hit.attrib = h.y;
             hit.record = h.z;
+            if (prm.fog_density > 0.0f)
+            {
+                FogSegment(prm, hit.t, throughput, radiance);
+            }
             const uint32_t face = FaceIndex(h.y);
             const uint32_t block = (h.y >> 16) & 0xFFFFu;
             const V3 vertex_normal = FaceDirection(face);
@@ -1081,7 +1101,8 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
     for (uint32_t sample = 0; sample < params->spp; ++sample)
     {
         const TraceParams p{params->bounces, params->frame_seed + sample, mis,
-                            (params->flags & CX_TRACE_NO_RUSSIAN_ROULETTE) != 0 ? 0u : 1u, SkyOf(params)};
+                            (params->flags & CX_TRACE_NO_RUSSIAN_ROULETTE) != 0 ? 0u : 1u, SkyOf(params),
+                            V3{params->fog_color[0], params->fog_color[1], params->fog_color[2]}, params->fog_density};
         ResetKernel<<<1, 64, 0, s>>>(w);
         {
             StageScope scope(s, adjoint ? "adjoint_trace" : "trace_primary");
@@ -1147,9 +1168,15 @@ int CheckParams(const CxTraceParams* params, const CxTraceOutput* out, uint32_t 
     {
         return ArgError("%s: unknown flags", what);
     }
-    if (adjoint && (params->flags & CX_TRACE_SKY) != 0)
+    if (!(params->fog_density >= 0.0f) || !(params->fog_density < 3.0e38f) || !(params->fog_color[0] >= 0.0f) ||
+        !(params->fog_color[1] >= 0.0f) || !(params->fog_color[2] >= 0.0f) || !(params->fog_color[0] < 3.0e38f) ||
+        !(params->fog_color[1] < 3.0e38f) || !(params->fog_color[2] < 3.0e38f))
     {
-        Log(3, "%s: the adjoint has no sky or sun (CX_TRACE_SKY)", what);
+        return ArgError("%s: fog density or colour negative or not finite", what);
+    }
+    if (adjoint && ((params->flags & CX_TRACE_SKY) != 0 || params->fog_density > 0.0f))
+    {
+        Log(3, "%s: the adjoint has no sky, sun or fog (CX_TRACE_SKY, fog_density)", what);
         return CX_ERROR_UNSUPPORTED;
     }
     if ((params->flags & CX_TRACE_SKY) != 0)

@@ -192,6 +192,27 @@ CX_DEV void SinCos(float a, float& s, float& c)
     c = sign_c < 0.0f ? -c : c;
 }
 
+// ---- Cephes expf -------------------------------------------------------------------------------
+// exp(x) for x <= 0 (the fog's transmission): x = n ln 2 + r with n = floor(x log2 e + 0.5), r by
+// the two-part ln 2, the degree-5 polynomial of Cephes expf (S. Moshier, Cephes Math Library), times
+// 2^n from its exponent bits; 0 below -87, where the result would leave the normal range. exp(0) is
+// exactly 1.
+CX_DEV float ExpNonPositive(float x)
+{
+    if (!(x >= -87.0f))
+    {
+        return 0.0f;
+    }
+    const float fx = floorf(x * 1.44269504088896341f + 0.5f);
+    float r = x - fx * 0.693359375f;
+    r = r - fx * -2.12194440e-4f;
+    const float z = r * r;
+    const float p = ((((1.9875691500e-4f * r + 1.3981999507e-3f) * r + 8.3334519073e-3f) * r + 4.1665795894e-2f) * r +
+                     1.6666665459e-1f) * r + 5.0000001201e-1f;
+    const float e = p * z + r + 1.0f;
+    return e * __int_as_float((int(fx) + 127) << 23);
+}
+
 // ---- random streams (caelix_random.hlsl) ---------------------------------------------------------
 CX_DEV uint32_t PcgHash(uint32_t value)
 {

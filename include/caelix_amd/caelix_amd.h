@@ -288,6 +288,13 @@ typedef struct CxTraceParams {
     float pad1;
     float sky_ground[3];
     float pad2;
+    /* Fog of the air (the frame constant fog_color: rgb radiance, w density; caelix_shading.hlsl:
+     * 150-175): fog_density > 0 turns it on. Every segment in the air that ends on a surface (the
+     * camera ray, every bounce segment) keeps exp(-density t) of its throughput and gains
+     * fog_color (1 - exp(-density t)); misses and shadow rays see no fog. Not supported by
+     * cx_trace_frame_adjoint. */
+    float fog_color[3];
+    float fog_density;
 } CxTraceParams;
 
 /* The procedural sky on misses and the sun (caelix_shading.hlsl:364-403, 453-485;

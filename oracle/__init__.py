@@ -152,7 +152,7 @@ class _Params(ctypes.Structure):
     _fields_ = [("bounces", _U32), ("frame_seed", _U32), ("spp", _U32), ("flags", _U32), ("sun_dir", _F3),
                 ("sun_solid_angle", ctypes.c_float), ("sun_radiance", _F3), ("sun_cos_radius", ctypes.c_float),
                 ("sky_zenith", _F3), ("pad0", ctypes.c_float), ("sky_horizon", _F3), ("pad1", ctypes.c_float),
-                ("sky_ground", _F3), ("pad2", ctypes.c_float)]
+                ("sky_ground", _F3), ("pad2", ctypes.c_float), ("fog_color", _F3), ("fog_density", ctypes.c_float)]
 
 
 NO_RUSSIAN_ROULETTE = 1
@@ -160,10 +160,14 @@ SKY = 2
 NO_SUN_NEE = 4
 
 
-def _params(bounces: int, seed: int, spp: int, flags: int, sky: dict | None) -> _Params:
+def _params(bounces: int, seed: int, spp: int, flags: int, sky: dict | None, fog=None) -> _Params:
     """The trace parameters; `sky` (keys sun_dir, sun_radiance, sun_solid_angle, sun_cos_radius,
-    sky_zenith, sky_horizon, sky_ground) sets the SKY flag and the sky fields."""
+    sky_zenith, sky_horizon, sky_ground) sets the SKY flag and the sky fields, fog = (rgb, density)
+    the fog of the air."""
     prm = _Params(bounces, seed, spp, flags)
+    if fog is not None:
+        prm.fog_color[:] = [float(v) for v in fog[0]]
+        prm.fog_density = float(fog[1])
     if sky is not None:
         prm.flags |= SKY
         for name in ("sun_dir", "sun_radiance", "sky_zenith", "sky_horizon", "sky_ground"):
@@ -364,9 +368,9 @@ class OracleScene:
         return ga.reshape(256, 3), gs
 
     def trace_frame(self, camera: dict, bounces: int, seed: int, spp: int = 1, rows=None, flags: int = 0,
-                    sky: dict | None = None):
+                    sky: dict | None = None, fog=None):
         cam = camera_struct(camera)
-        prm = _params(bounces, seed, spp, flags, sky)
+        prm = _params(bounces, seed, spp, flags, sky, fog)
         r0, r1 = rows if rows is not None else (0, cam.height)
         count = (r1 - r0) * cam.width
         color = np.zeros(count * 4, np.float64)

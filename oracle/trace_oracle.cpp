@@ -1010,10 +1010,43 @@ float BrickScale(const CxScene& s, uint32_t record)
     return s.brick_emission_scale != nullptr ? s.brick_emission_scale[record] : 1.0f;
 }
 
+// exp(x) for x <= 0: the Cephes expf polynomial (S. Moshier, Cephes Math Library) with the
+// two-part ln 2 and 2^n from exponent bits, in the kernels' operation order; 0 below -87.
+float ExpNonPositive(float x)
+{
+    if (!(x >= -87.0f))
+    {
+        return 0.0f;
+    }
+    const float fx = std::floor(x * 1.44269504088896341f + 0.5f);
+    float r = x - fx * 0.693359375f;
+    r = r - fx * -2.12194440e-4f;
+    const float z = r * r;
+    const float p = ((((1.9875691500e-4f * r + 1.3981999507e-3f) * r + 8.3334519073e-3f) * r + 4.1665795894e-2f) * r +
+                     1.6666665459e-1f) * r + 5.0000001201e-1f;
+    const float e = p * z + r + 1.0f;
+    const uint32_t bits = uint32_t(int(fx) + 127) << 23;
+    float scale;
+    std::memcpy(&scale, &bits, 4);
+    return e * scale;
+}
+
+// TraverseSegment (shaders/render/caelix_shading.hlsl:150-175) in the air with fog: the segment of
+// length t keeps exp(-density t) of the throughput and adds fog (1 - exp(-density t)) times the
+// throughput before it.
+void FogSegment(V fog, float density, float t, V& throughput, V& radiance)
+{
+    const float transmission = ExpNonPositive((-density) * t);
+    radiance = Add(radiance, Mul(MulV(throughput, fog), 1.0f - transmission));
+    throughput = Mul(throughput, transmission);
+}
+
 // The procedural sky and the sun of CX_TRACE_SKY (the frame constants sky_zenith/horizon/ground,
 // sun_dir, sun_radiance, sun_cos_radius, sun_solid_angle of shaders/render/caelix_frame.hlsl).
 struct Sky
 {
+    V fog;                     // the fog of the air (fog_color rgb, density; density 0: none)
+    float fog_density = 0.0f;
     bool on = false;
     bool sun = false;  // the sun's NEE: `on`, kRenderFlagSunNee, a sun radiance above 0 (caelix_path_trace.hlsl:742)
     V sun_dir, sun_radiance, zenith, horizon, ground;
@@ -1023,6 +1056,8 @@ Sky SkyOf(const CxTraceParams& p)
 {
     Sky k;
     k.sun_dir = k.sun_radiance = k.zenith = k.horizon = k.ground = Mk(0, 0, 0);
+    k.fog = Mk(p.fog_color[0], p.fog_color[1], p.fog_color[2]);
+    k.fog_density = p.fog_density;
     if ((p.flags & CX_TRACE_SKY) == 0u)
     {
         return k;
@@ -1323,6 +1358,15 @@ void TracePixel(const CxScene& s, const CxLightTreeView& L, const CxCamera& cam,
     const Shade ps = ShadeHit(s, (primary.attrib >> 16) & 0xFFFFu, Add(origin, Mul(dir, primary.t)), primary_normal, rng);
     albedo = ps.albedo;
     emission = Mul(ps.emission, BrickScale(s, primary.record));
+    if (k.fog_density > 0.0f)
+    {
+        // The camera segment: its transmission multiplies what the surface sends, its fog emission
+        // is deterministic (caelix_path_trace.hlsl:1273-1283).
+        V chain = Mk(1, 1, 1), inscatter = Mk(0, 0, 0);
+        FogSegment(k.fog, k.fog_density, primary.t, chain, inscatter);
+        albedo = MulV(chain, ps.albedo);
+        emission = Add(MulV(chain, emission), inscatter);
+    }
     double adj[3] = {0, 0, 0};
     double rr = 1.0;
     std::vector<uint32_t> ids;
@@ -1381,6 +1425,13 @@ void TracePixel(const CxScene& s, const CxLightTreeView& L, const CxCamera& cam,
                 AddD(radiance, MulV(SkyMiss(k, bd), throughput));
             }
             break;
+        }
+        if (k.fog_density > 0.0f)
+        {
+            // The segment's fog, added in float32 as the kernels add it to their float32 radiance.
+            V fog_add = Mk(0, 0, 0);
+            FogSegment(k.fog, k.fog_density, hit.t, throughput, fog_add);
+            AddD(radiance, fog_add);
         }
         const V vn = FaceDirection(LowBit(hit.attrib & 0x3Fu));
         const uint32_t block = (hit.attrib >> 16) & 0xFFFFu;

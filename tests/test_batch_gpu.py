@@ -39,16 +39,18 @@ def test_batched_trees_equal_single_trees(batch_and_plain):
         assert got["leaves"]["summary"][:n].tobytes() == want["leaves"]["summary"][:n].tobytes()
 
 
-@pytest.mark.parametrize("bounces,spp,sky", [(0, 1, False), (2, 1, False), (3, 2, False), (2, 1, True)])
-def test_batched_frames_equal_single_frames(cx, batch_and_plain, bounces, spp, sky):
+@pytest.mark.parametrize("bounces,spp,outdoor", [(0, 1, False), (2, 1, False), (3, 2, False), (2, 1, True)])
+def test_batched_frames_equal_single_frames(cx, batch_and_plain, bounces, spp, outdoor):
+    """Also under the procedural sky, the sun and fog."""
     datas, batch, plain = batch_and_plain
     k = cx.sky_params(sun_dir=(0.3, 0.9, 0.2), sun_radiance=(40.0, 36.0, 30.0), sun_radius_deg=3.0,
-                      zenith=(0.2, 0.3, 0.6), horizon=(0.6, 0.6, 0.7), ground=(0.1, 0.1, 0.1)) if sky else None
+                      zenith=(0.2, 0.3, 0.6), horizon=(0.6, 0.6, 0.7), ground=(0.1, 0.1, 0.1)) if outdoor else None
+    fog = ((0.3, 0.3, 0.4), 0.01) if outdoor else None
     batch.counters.zero_()
     plain.counters.zero_()
     batch.build_trees()
-    batch.trace(bounces, 17, spp, sky=k)
-    plain.step(bounces, 17, spp, sky=k)
+    batch.trace(bounces, 17, spp, sky=k, fog=fog)
+    plain.step(bounces, 17, spp, sky=k, fog=fog)
     color, radiance, attrib = batch.images()
     p = plain.pixels
     for i in range(len(datas)):

@@ -1,8 +1,8 @@
-"""GPU parity of the procedural sky and sun (CX_TRACE_SKY) through the C ABI: the small terrain's
-frame with a graded sky, a sun and its lamps against the oracle (hit attributes, depths and ray /
-shadow-ray / step counters exact, color and radiance within 2e-5 relative), with and without the sun's
-NEE; the sun-floor closed form L sin^2 r and the constant-sky floor (radiance = C exactly) on the GPU
-path; the adjoint refuses the flag."""
+"""GPU parity of the procedural sky and sun (CX_TRACE_SKY) and of the fog through the C ABI: the
+small terrain's frame with a graded sky, a sun, fog and its lamps against the oracle (hit attributes,
+depths and ray / shadow-ray / step counters exact, color and radiance within 2e-5 relative), with and
+without the sun's NEE; the sun-floor closed form L sin^2 r, the constant-sky floor (radiance = C
+exactly) and the isothermal fogged furnace on the GPU path; the adjoint refuses sky and fog."""
 from __future__ import annotations
 
 import numpy as np
@@ -123,3 +123,53 @@ def test_sky_rejected_by_adjoint_and_bad_sun(cx, pair):
     bad = outdoor(cx, sun_radiance=(-1.0, 1.0, 1.0))
     with pytest.raises(RuntimeError):
         gpu.trace_frame(SMALL.camera, 1, 1, 1, sky=bad)
+
+
+FOG = ((0.35, 0.4, 0.5), 0.004)
+
+
+@pytest.mark.parametrize("bounces,with_sky", [(0, False), (2, False), (3, True)])
+def test_fog_frame_matches_oracle(cx, pair, bounces, with_sky):
+    gpu, ref = pair
+    k = outdoor(cx) if with_sky else None
+    print(f"fog frame bounces={bounces} sky={with_sky}", flush=True)
+    got = gpu.trace_frame(SMALL.camera, bounces, 555, 1, sky=k, fog=FOG)
+    want = ref.trace_frame(SMALL.camera, bounces, 555, 1, sky=k, fog=FOG)
+    compare(got, want)
+    plain = ref.trace_frame(SMALL.camera, bounces, 555, 1, sky=k)
+    hit = want[2] != 0
+    assert np.all(np.abs(want[0][hit, :3] - plain[0][hit, :3]).max(axis=1) > 0)  # the fog shows on every hit
+
+
+def test_fog_isothermal_furnace_gpu(cx, oracle):
+    data = scene.furnace_box(width=64, height=64)
+    data.light_codes = np.zeros_like(data.light_codes)
+    data.emissive_bricks = np.zeros(0, np.int64)
+    gpu = cx.TraceScene(data, load_factor=LOAD)
+    assert gpu.light_bricks == 0
+    F = 2.0
+    fog = ((F, F, F), 0.02)
+    color, radiance, attrib, counters = gpu.trace_frame(data.camera, 24, 8, 1, flags=cx.TRACE_NO_RUSSIAN_ROULETTE,
+                                                        fog=fog)
+    c = color.cpu().numpy().reshape(-1, 4)[:, :3].astype(np.float64)
+    assert np.all(attrib.cpu().numpy() != 0) and int(counters.cpu().numpy()[1]) == 0
+    assert np.all(np.abs(c - F) < 1e-5 * F), np.abs(c - F).max()
+    ref = oracle.OracleScene.build(data, slot_count(data))
+    compare(gpu.trace_frame(data.camera, 1, 8, 1, flags=cx.TRACE_NO_RUSSIAN_ROULETTE, fog=fog),
+            ref.trace_frame(data.camera, 1, 8, 1, flags=oracle.NO_RUSSIAN_ROULETTE, fog=fog))
+
+
+def test_fog_rejected_by_adjoint_and_bad_fog(cx, pair):
+    import torch
+
+    gpu, _ = pair
+    w, h = SMALL.camera["width"], SMALL.camera["height"]
+    d_color = torch.ones(w * h * 4, dtype=torch.float32, device="cuda")
+    ga = torch.zeros(256 * 4, dtype=torch.float32, device="cuda")
+    gs = torch.zeros(len(SMALL.keys), dtype=torch.float32, device="cuda")
+    with pytest.raises(RuntimeError):
+        gpu.trace_frame_adjoint(SMALL.camera, 2, 1, 1, d_color, ga, gs, fog=FOG)
+    with pytest.raises(RuntimeError):
+        gpu.trace_frame(SMALL.camera, 1, 1, 1, fog=((0.1, 0.1, 0.1), -1.0))
+    with pytest.raises(RuntimeError):
+        gpu.trace_frame(SMALL.camera, 1, 1, 1, fog=((0.1, -0.1, 0.1), 0.01))

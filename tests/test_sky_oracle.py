@@ -13,8 +13,19 @@ open sky where the expected radiance has a closed form:
   sun + lamp (one-sample RIS)        E[both] = E[lamp alone] + E[sun alone] (unbiasedness of the
                                      choice weight w_sum / w_picked; statistical)
 
+and the fog of the air (density s, radiance F):
+
+  camera segment, 0 bounces          color = F (1 - exp(-s t)) at the pixel's depth t
+  camera segment + constant sky C    color = F (1 - T) + T albedo C, radiance = C (bounce rays miss:
+                                     no fog on them)
+  isothermal fogged furnace          walls of emission E and albedo a in fog of radiance F = E / (1 - a):
+                                     every segment adds theta F (1 - a T) and passes theta a T on, so
+                                     the pixel is F (1 - a T_0 prod_j a T_j), F within float rounding at
+                                     24 bounces without roulette, whatever the segment lengths
+
 The procedural sky and sun follow shaders/render/caelix_shading.hlsl:364-485 and
-caelix_path_trace.hlsl:276-290, 356-413, 737-846."""
+caelix_path_trace.hlsl:276-290, 356-413, 737-846; the fog caelix_shading.hlsl:150-175 and
+caelix_path_trace.hlsl:1273-1283."""
 from __future__ import annotations
 
 import numpy as np
@@ -192,3 +203,67 @@ def test_sun_and_lamp_choice_is_unbiased(oracle):
     tol = 5 * np.sqrt(se_b ** 2 + se_l ** 2 + se_s ** 2)
     assert lamp > 10 * tol and sun > 10 * tol  # both lights matter
     assert abs(both - (lamp + sun)) < tol, (both, lamp, sun, tol)
+
+
+FOG = (np.array([0.5, 0.6, 0.7]), 0.01)
+
+
+@pytest.mark.parametrize("density", [1e-4, 3e-3, 0.01, 0.04])
+def test_fog_camera_segment_closed_form(floor, density):
+    """Also pins the exponential (density x depth from 0.011 to 5) to 1e-6 relative."""
+    data, ref = floor
+    color, radiance, attrib, _ = ref.trace_frame(data.camera, 0, 3, 1, fog=(FOG[0], density))
+    assert np.all(attrib != 0) and np.all(radiance == 0.0)
+    t = color[:, 3]
+    assert t.min() > 100.0  # the camera is 112 voxels above the floor
+    want = FOG[0][None, :] * (1.0 - np.exp(-density * t))[:, None]
+    # 1 - T is taken in float32: a few ulps of T (6e-8 each) absolute, then 1e-6 relative
+    assert np.all(np.abs(color[:, :3] - want) <= 1e-6 * want + 3e-7 * FOG[0][None, :])
+
+
+def test_fog_camera_segment_with_constant_sky(floor):
+    data, ref = floor
+    c = np.array([0.3, 0.5, 0.7])
+    color, radiance, _, _ = ref.trace_frame(data.camera, 1, 3, 1, sky=sky(zenith=c, horizon=c, ground=c), fog=FOG)
+    assert np.array_equal(radiance[:, :3], np.tile(np.float32(c).astype(np.float64), (len(radiance), 1)))
+    T = np.exp(-FOG[1] * color[:, 3])[:, None]
+    albedo = float(np.float32(123 / 255))
+    want = FOG[0][None, :] * (1.0 - T) + T * albedo * c[None, :]
+    assert np.allclose(color[:, :3], want, rtol=2e-6, atol=0)
+
+
+def isothermal_furnace():
+    """The furnace box with walls of emission 1 and albedo 0.5 that are not lights (no NEE)."""
+    data = scene.furnace_box(width=64, height=64)
+    data.light_codes = np.zeros_like(data.light_codes)
+    data.emissive_bricks = np.zeros(0, np.int64)
+    return data
+
+
+def test_fog_isothermal_furnace(oracle):
+    data = isothermal_furnace()
+    ref = build(oracle, data)
+    F = 1.0 / (1.0 - 0.5)
+    fog = ((F, F, F), 0.02)
+    color, radiance, attrib, counters = ref.trace_frame(data.camera, 24, 8, 1, flags=oracle.NO_RUSSIAN_ROULETTE, fog=fog)
+    assert np.all(attrib != 0) and counters[1] == 0
+    assert np.all(np.abs(color[:, :3] - F) < 1e-5 * F), np.abs(color[:, :3] - F).max()
+
+
+def test_fog_isothermal_furnace_one_bounce(oracle):
+    """One bounce: radiance R = F (1 - a T1) and color = F (1 - a^2 T0 T1), T0 = exp(-s depth). Both
+    outputs give the same bounce-segment transmission T1, inside [exp(-s * the box diagonal), 1];
+    without the fog R would be E = 1 and color 1.5."""
+    data = isothermal_furnace()
+    ref = build(oracle, data)
+    a, s_ = 0.5, 0.02
+    F = 1.0 / (1.0 - a)
+    color, radiance, _, _ = ref.trace_frame(data.camera, 1, 8, 1, flags=oracle.NO_RUSSIAN_ROULETTE,
+                                            fog=((F, F, F), s_))
+    T0 = np.exp(-s_ * color[:, 3])
+    t1_from_radiance = (F - radiance[:, 0]) / (F * a)
+    t1_from_color = (F - color[:, 0]) / (F * a * a * T0)
+    assert np.allclose(t1_from_radiance, t1_from_color, rtol=2e-5, atol=2e-6)
+    diagonal = 48 * np.sqrt(3.0)
+    assert np.all(t1_from_radiance > np.exp(-s_ * diagonal) - 1e-6) and np.all(t1_from_radiance < 1.0)
+    assert t1_from_radiance.mean() < 0.8  # the bounce segments lose a visible share to the fog
