@@ -515,7 +515,7 @@ class TraceScene:
         sc.grid_min[:] = list(data.grid_min)
         sc.grid_dim = dim
         sc.group_shift = int(data.group_shift)
-        sc.encoding = 1
+        sc.encoding = int(getattr(data, "encoding", 1))  # 0 packed, 1 block-ID
         sc.materials = self.materials.data_ptr()
         sc.light_codes = self.light_codes.data_ptr()
         sc.emission_scale = 1.0

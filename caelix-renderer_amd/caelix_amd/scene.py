@@ -33,6 +33,7 @@ class SceneData:
     group_shift: int
     camera: dict
     emissive_bricks: np.ndarray
+    encoding: int = 1        # block values: 1 block-ID (opaque 0x8000, id 0x1FF, faces << 9), 0 packed
 
 
 def value_noise(size: int, seed: int, octaves: int, base_period: int) -> np.ndarray:
@@ -240,6 +241,30 @@ def sun_floor(size_bricks: int = 16, albedo: float = 123 / 255, lamp: str = "bel
     camera = look_at((c + 0.013, 120.0, c + 0.021), (c, 8.0, c), width, height, fov_y_deg=30.0)
     return SceneData(keys=keys, payload=payload, materials=table, light_codes=light_codes, grid_min=(0, 0, 0),
                      grid_dim=dim, group_shift=5, camera=camera, emissive_bricks=emissive)
+
+
+def glass_world(encoding: int = 1) -> SceneData:
+    """The reference finalize test's glass world (tests/finalize_test.cpp:264-290): bricks (0..2, 0, 0)
+    of opaque floor 0x8123 and (0..2, 1, 0) holding a glass slab x 2..21, y 8..11, z 2..5 in air; the
+    glass is transparent id 0x42 in either encoding."""
+    glass, floor = 0x42, 0x8123
+    keys, payload = [], []
+    v = np.arange(512)
+    lx, ly, lz = v & 7, (v >> 3) & 7, v >> 6
+    for bx in range(3):
+        keys.append((bx, 0, 0))
+        payload.append(np.full(512, floor, np.uint16))
+        gx, gy = bx * 8 + lx, 8 + ly
+        inside = (gx >= 2) & (gx < 22) & (gy >= 8) & (gy < 12) & (lz >= 2) & (lz < 6)
+        keys.append((bx, 1, 0))
+        payload.append(np.where(inside, glass, 0).astype(np.uint16))
+    table = np.zeros((65536, 8), np.float32)
+    table[floor, 0:3] = 0.5
+    table[glass, 0:3] = 0.9
+    camera = look_at((12.0, 20.0, -10.0), (12.0, 9.0, 4.0), 32, 32)
+    return SceneData(keys=np.array(keys, np.int32), payload=np.stack(payload), materials=table,
+                     light_codes=np.zeros(2048, np.uint32), grid_min=(0, 0, 0), grid_dim=8, group_shift=4,
+                     camera=camera, emissive_bricks=np.zeros(0, np.int64), encoding=encoding)
 
 
 def diff_scene(seed: int = 5) -> SceneData:

@@ -1,9 +1,13 @@
 // Scene kernels for gfx950: the record producer and the light pass of the trace workloads.
 //
-// finalize     per brick (one 64-lane workgroup): micro occupancy words, coarse byte, tight bounds
-//              (the opaque subset of shaders/scene/caelix_finalize.hlsl:112-361: a voxel is occupied
-//              when its block is not air), and the resident bits of the brick grid and its 8^3-brick
-//              super cells for the world traversal.
+// finalize     per brick (one 64-lane workgroup, lane = one x row): the rules of
+//              shaders/scene/caelix_finalize.hlsl:205-247 per voxel over its six neighbours (in this
+//              brick, or in the face-neighbour brick found through the hash table, air when absent):
+//              a transparent or air voxel gets the face bits of the neighbours that are not opaque and
+//              of another medium, written back into the payload; a voxel is occupied when it is alive
+//              (fewer than six opaque neighbours), not 0 and an interface (opaque, or a face bit).
+//              Then micro occupancy words, coarse byte, tight bounds and the resident bits of the
+//              brick grid and its 8^3-brick super cells for the world traversal.
 // light pass   per brick: the open faces of its light voxels (a light material code; the neighbour,
 //              in this brick or found through the hash table, not opaque), the light face count into
 //              record word 1 (caelix_finalize.hlsl:206-243), then the brick light records in the
@@ -38,6 +42,7 @@ struct SceneArgs
     const uint32_t* light_codes;
     uint32_t key_words;  // 3: (x, y, z); 4: (scene, x, y, z) for a batch of chunk scenes
     int32_t* box;        // per scene: super-cell box of the bricks with a voxel, or null
+    uint16_t* payload_out;  // = payload: finalize writes the face bits back
 };
 
 // The (x, y, z) brick coordinates of a record and its scene (0 for one scene).
@@ -63,6 +68,21 @@ CX_DEV bool Opaque(uint32_t block, uint32_t encoding)
 {
     return (block & (encoding == kEncodingPacked ? kPackedOpaqueMask : kBlockIdOpaqueMask)) != 0;
 }
+CX_DEV uint32_t IdMask(uint32_t encoding)
+{
+    return encoding == kEncodingPacked ? kPackedIdMask : kBlockIdIdMask;
+}
+// BlockMaterialCode (caelix_record.hlsl:209): an opaque value as it is, a transparent one without its
+// face bits.
+CX_DEV uint32_t MaterialCode(uint32_t block, uint32_t encoding)
+{
+    return Opaque(block, encoding) ? block : (block & IdMask(encoding));
+}
+CX_DEV uint32_t RowBlock(uint4 row, uint32_t x)
+{
+    const uint32_t w = x < 4 ? (x < 2 ? row.x : row.y) : (x < 6 ? row.z : row.w);
+    return (x & 1u) != 0 ? (w >> 16) : (w & 0xFFFFu);
+}
 
 // Empty boxes (lo = INT32_MAX, hi = INT32_MIN) for the finalize kernel's atomic min / max.
 __global__ void BoxInitKernel(int32_t* box, uint32_t scenes)
@@ -86,20 +106,64 @@ __global__ void __launch_bounds__(64) FinalizeKernel(SceneArgs s)
     const uint32_t brick = blockIdx.x;
     const uint32_t lane = threadIdx.x;
     __shared__ unsigned long long lds_occ[8];
+    __shared__ uint4 lds_rows[64];
+    __shared__ uint32_t lds_nbr[6];
     if (lane < 8)
     {
         lds_occ[lane] = 0ull;
     }
+    // The own rows (lane = y + 8 z, 8 blocks of 16 bits each) and the six face-neighbour records.
+    const uint4 own = reinterpret_cast<const uint4*>(s.payload + uint64_t(brick) * 512)[lane];
+    lds_rows[lane] = own;
+    if (lane < 6)
+    {
+        const int32_t* key = BrickXyz(s, brick);
+        int32_t nk[3] = {key[0], key[1], key[2]};
+        nk[lane >> 1] += (lane & 1u) != 0 ? -1 : 1;
+        lds_nbr[lane] = HashGet(s.slots, s.slot_count, s.pow2, s.hash_info[0],
+                                KeyFingerprint(s.key_words, BrickScene(s, brick), nk[0], nk[1], nk[2]));
+    }
     __syncthreads();
-    const uint16_t* payload = s.payload + uint64_t(brick) * 512;
+    const uint32_t y = lane & 7u, z = lane >> 3;
+    auto neighbour_row = [&](uint32_t f, uint32_t ny, uint32_t nz) {
+        const uint32_t record = lds_nbr[f];
+        return record == kHashNone ? make_uint4(0u, 0u, 0u, 0u)
+                                   : reinterpret_cast<const uint4*>(s.payload + uint64_t(record) * 512)[ny + 8 * nz];
+    };
+    const uint4 row_yp = y < 7 ? lds_rows[lane + 1] : neighbour_row(2, 0, z);
+    const uint4 row_yn = y > 0 ? lds_rows[lane - 1] : neighbour_row(3, 7, z);
+    const uint4 row_zp = z < 7 ? lds_rows[lane + 8] : neighbour_row(4, y, 0);
+    const uint4 row_zn = z > 0 ? lds_rows[lane - 8] : neighbour_row(5, y, 7);
+    const uint32_t block_xp = lds_nbr[0] == kHashNone ? 0u : s.payload[uint64_t(lds_nbr[0]) * 512 + 8 * y + 64 * z];
+    const uint32_t block_xn = lds_nbr[1] == kHashNone ? 0u : s.payload[uint64_t(lds_nbr[1]) * 512 + 7 + 8 * y + 64 * z];
+    const uint32_t opaque_mask = s.encoding == kEncodingPacked ? kPackedOpaqueMask : kBlockIdOpaqueMask;
+    const uint32_t id_mask = IdMask(s.encoding);
+    const uint32_t face_shift = s.encoding == kEncodingPacked ? kPackedFaceShift : kBlockIdFaceShift;
     uint32_t lo[3] = {7, 7, 7};
     uint32_t hi[3] = {0, 0, 0};
     bool any = false;
-    for (uint32_t i = 0; i < 8; ++i)
+    for (uint32_t x = 0; x < 8; ++x)
     {
-        const uint32_t v = lane * 8 + i;
-        const uint32_t x = v & 7u, y = (v >> 3) & 7u, z = v >> 6;
-        if (payload[v] != 0u)
+        const uint32_t raw = RowBlock(own, x);
+        const uint32_t v = (raw & opaque_mask) != 0u ? raw : (raw & id_mask);
+        const uint32_t n[6] = {x < 7 ? RowBlock(own, x + 1) : block_xp, x > 0 ? RowBlock(own, x - 1) : block_xn,
+                               RowBlock(row_yp, x), RowBlock(row_yn, x), RowBlock(row_zp, x), RowBlock(row_zn, x)};
+        uint32_t opaque_count = 0;
+        uint32_t faces = 0;
+#pragma unroll
+        for (uint32_t d = 0; d < 6; ++d)
+        {
+            const bool neighbour_opaque = (n[d] & opaque_mask) != 0u;
+            opaque_count += neighbour_opaque ? 1u : 0u;
+            faces |= (!neighbour_opaque && (n[d] & id_mask) != v) ? (1u << d) : 0u;
+        }
+        const bool opaque = (v & opaque_mask) != 0u;
+        const uint32_t result = opaque ? v : (v | (faces << face_shift));
+        if (result != raw)
+        {
+            s.payload_out[uint64_t(brick) * 512 + x + 8 * y + 64 * z] = uint16_t(result);
+        }
+        if (opaque_count < 6 && result != 0u && (opaque || faces != 0u))
         {
             const uint32_t coarse = (x >> 2) | ((y >> 2) << 1) | ((z >> 2) << 2);
             const uint32_t micro = (x & 3u) | ((y & 3u) << 2) | ((z & 3u) << 4);
@@ -200,11 +264,12 @@ CX_DEV uint32_t NeighbourBlock(const SceneArgs& s, uint32_t record, int x, int y
     return s.payload[uint64_t(other) * 512 + p[0] + 8 * p[1] + 64 * p[2]];
 }
 
-// Open light faces of voxel v: bit f set when v is a light voxel and its face f is open.
+// Open light faces of voxel v (LtOpenFaces, caelix_light_tree_build.hlsl:92-111): bit f set when v's
+// material code is a light code and its face f is open.
 CX_DEV uint32_t OpenLightFaces(const SceneArgs& s, uint32_t record, uint32_t v, uint32_t& block)
 {
     block = s.payload[uint64_t(record) * 512 + v];
-    if (block == 0u || !LightCode(s.light_codes, block))
+    if (!LightCode(s.light_codes, MaterialCode(block, s.encoding)))
     {
         return 0u;
     }
@@ -316,7 +381,7 @@ __global__ void __launch_bounds__(64) LightRecordKernel(SceneArgs s, LightArgs l
         {
             continue;
         }
-        const float4 e = s.materials[2 * uint64_t(block) + 1];
+        const float4 e = s.materials[2 * uint64_t(MaterialCode(block, s.encoding)) + 1];
         const float weight = e.x * 0.2126f + e.y * 0.7152f + e.z * 0.0722f;
         for (uint32_t f = 0; f < 6; ++f)
         {
@@ -515,6 +580,7 @@ SceneArgs ArgsOf(const CxScene& c)
     s.info = reinterpret_cast<uint2*>(c.info);
     s.occupancy = reinterpret_cast<uint64_t*>(c.occupancy);
     s.payload = c.payload;
+    s.payload_out = c.payload;
     s.slots = c.slots;
     s.slot_count = c.slot_count;
     s.pow2 = (c.slot_count & (c.slot_count - 1)) == 0 ? 1u : 0u;
@@ -544,9 +610,14 @@ extern "C" {
 CX_API int cx_scene_finalize(const CxScene* scene, void* stream)
 {
     if (scene == nullptr || scene->keys == nullptr || scene->info == nullptr || scene->occupancy == nullptr ||
-        scene->payload == nullptr || scene->brick_bits == nullptr || scene->super_bits == nullptr)
+        scene->payload == nullptr || scene->brick_bits == nullptr || scene->super_bits == nullptr ||
+        scene->slots == nullptr || scene->hash_info == nullptr)
     {
         return cx::ArgError("cx_scene_finalize: a buffer is NULL");
+    }
+    if ((reinterpret_cast<uintptr_t>(scene->payload) & 15u) != 0)
+    {
+        return cx::ArgError("cx_scene_finalize: payload not 16-byte aligned");
     }
     if (scene->grid_dim == 0 || scene->grid_dim % 8 != 0 || scene->group_shift == 0 || scene->group_shift > 7)
     {

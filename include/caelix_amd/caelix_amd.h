@@ -189,7 +189,8 @@ typedef struct CxScene {
     const int32_t* keys;      /* [bricks][3] */
     uint32_t* info;           /* [bricks][2] record words 0 and 1 (written by cx_scene_finalize) */
     void* occupancy;          /* [bricks][8] uint64 micro occupancy (written by cx_scene_finalize) */
-    const uint16_t* payload;  /* [bricks][512] block values, voxel x + 8y + 64z */
+    uint16_t* payload;        /* [bricks][512] block values, voxel x + 8y + 64z, 16-byte aligned;
+                               * cx_scene_finalize writes the face bits of transparent and air voxels */
     const HashKeySlot* slots; /* cx_hash_build_bricks over keys */
     uint32_t slot_count;
     uint32_t bricks;
@@ -220,8 +221,11 @@ typedef struct CxScene {
     int32_t* occupied_box;
 } CxScene;
 
-/* The record producer's opaque subset (shaders/scene/caelix_finalize.hlsl:112-361): word 0 (group
- * position, coarse byte), word 1 bounds, micro occupancy; and the resident bitmaps. */
+/* The record producer (shaders/scene/caelix_finalize.hlsl:112-361) over the bricks and their face
+ * neighbours (found through the hash table: slots and hash_info are read): face bits of transparent and
+ * air voxels written into the payload, word 0 (group position, coarse byte), word 1 bounds and the
+ * micro occupancy of the occupied voxels (alive: fewer than six opaque neighbours; not 0; opaque or a
+ * face bit); and the resident bitmaps. Running it again changes nothing. */
 CX_API int cx_scene_finalize(const CxScene* scene, void* stream);
 
 /* The light pass (the group pass of shaders/light/records/*.hlsl for one entity, plus the gather of

@@ -218,12 +218,14 @@ class OracleScene:
     light_codes, heap, record_offset, ordinal_of_brick, headers, nodes, leaves); missing ones are
     produced by the oracle itself with build()."""
 
-    def __init__(self, arrays: dict, grid_min, grid_dim: int, group_shift: int, mis: bool = True):
+    def __init__(self, arrays: dict, grid_min, grid_dim: int, group_shift: int, mis: bool = True,
+                 encoding: int = 1):
         self.a = {k: np.ascontiguousarray(v) for k, v in arrays.items()}
         self.grid_min = tuple(int(v) for v in grid_min)
         self.grid_dim = int(grid_dim)
         self.group_shift = int(group_shift)
         self.mis = mis
+        self.encoding = int(encoding)  # 0 packed, 1 block-ID (the reference's CaelixSceneDesc::encoding)
         self._refresh()
 
     def _refresh(self):
@@ -242,7 +244,7 @@ class OracleScene:
         s.grid_min[:] = list(self.grid_min)
         s.grid_dim = self.grid_dim
         s.group_shift = self.group_shift
-        s.encoding = 1
+        s.encoding = self.encoding
         s.materials = _ptr(a["materials"])
         s.light_codes = _ptr(a["light_codes"])
         s.emission_scale = 1.0
@@ -265,7 +267,7 @@ class OracleScene:
         a = dict(arrays)
         a["empty_nodes"] = np.zeros(NODE_BYTES, np.uint8)
         a["empty_leaves"] = np.zeros(LEAF_BYTES, np.uint8)
-        return cls(a, data.grid_min, data.grid_dim, data.group_shift, mis)
+        return cls(a, data.grid_min, data.grid_dim, data.group_shift, mis, getattr(data, "encoding", 1))
 
     @classmethod
     def build(cls, data, slot_count: int, mis: bool = True) -> "OracleScene":
@@ -279,7 +281,7 @@ class OracleScene:
             "keys": np.ascontiguousarray(data.keys, np.int32),
             "info": np.zeros(2 * n, np.uint32),
             "occupancy": np.zeros(8 * n, np.uint64),
-            "payload": np.ascontiguousarray(data.payload, np.uint16),
+            "payload": np.array(data.payload, np.uint16, copy=True),  # finalize writes face bits
             "slots": slots,
             "hash_info": np.array([longest + 1, longest, 0, 0], np.uint32),
             "brick_bits": np.zeros((cells + 31) // 32, np.uint32),
@@ -287,7 +289,7 @@ class OracleScene:
             "materials": np.ascontiguousarray(data.materials, np.float32),
             "light_codes": np.ascontiguousarray(data.light_codes, np.uint32),
         }
-        self = cls(a, data.grid_min, dim, data.group_shift, mis)
+        self = cls(a, data.grid_min, dim, data.group_shift, mis, getattr(data, "encoding", 1))
         lib = _trace_lib()
         lib.oracle_scene_finalize(ctypes.byref(self.scene))
         leaf_capacity = max(64, (n + 63) // 64 * 64)

@@ -202,6 +202,10 @@ bool Opaque(uint32_t block, uint32_t encoding)
 {
     return (block & (encoding == kEncodingPacked ? kPackedOpaqueMask : kBlockIdOpaqueMask)) != 0;
 }
+uint32_t MaterialCode(uint32_t block, uint32_t encoding)
+{
+    return Opaque(block, encoding) ? block : (block & (encoding == kEncodingPacked ? kPackedIdMask : kBlockIdIdMask));
+}
 uint32_t FaceBits(uint32_t block, uint32_t encoding)
 {
     return encoding == kEncodingPacked ? ((block >> kPackedFaceShift) & 0x3Fu) : ((block >> kBlockIdFaceShift) & 0x3Fu);
@@ -1504,13 +1508,48 @@ void TracePixel(const CxScene& s, const CxLightTreeView& L, const CxCamera& cam,
 
 extern "C" {
 
-// Record words 0/1 (bounds), micro occupancy, bitmaps (caelix_finalize.hlsl's opaque subset).
+// The finalize kernel (shaders/scene/caelix_finalize.hlsl:205-247, 112-361): per voxel the six
+// neighbours (another brick's through the hash table, air when absent) give the face bits of a
+// transparent or air voxel (a neighbour that is not opaque and of another medium), written into the
+// payload, and the occupancy: alive (fewer than six opaque neighbours), not 0, opaque or a face bit.
+// Then record words 0/1 (bounds), micro occupancy and the bitmaps. The neighbours are read with their
+// face bits masked off, so the order in which bricks are rewritten does not matter.
 void oracle_scene_finalize(CxScene* s)
 {
     const uint64_t cells = uint64_t(s->grid_dim) * s->grid_dim * s->grid_dim;
     std::memset(s->brick_bits, 0, ((cells + 31) / 32) * 4);
     std::memset(s->super_bits, 0, ((cells / 512 + 31) / 32) * 4);
     uint64_t* occ_all = static_cast<uint64_t*>(s->occupancy);
+    const bool packed = s->encoding == kEncodingPacked;
+    const uint32_t opaque_mask = packed ? kPackedOpaqueMask : kBlockIdOpaqueMask;
+    const uint32_t id_mask = packed ? kPackedIdMask : kBlockIdIdMask;
+    const uint32_t face_shift = packed ? kPackedFaceShift : kBlockIdFaceShift;
+    auto block_at = [&](uint32_t record, int x, int y, int z) -> uint32_t {
+        int p[3] = {x, y, z};
+        int32_t k[3] = {s->keys[3 * uint64_t(record)], s->keys[3 * uint64_t(record) + 1], s->keys[3 * uint64_t(record) + 2]};
+        bool outside = false;
+        for (int a = 0; a < 3; ++a)
+        {
+            if (p[a] < 0 || p[a] > 7)
+            {
+                k[a] += p[a] < 0 ? -1 : 1;
+                p[a] = (p[a] + 8) & 7;
+                outside = true;
+            }
+        }
+        uint32_t other = record;
+        if (outside)
+        {
+            uint32_t probes;
+            other = HashGet(*s, k[0], k[1], k[2], probes);
+            if (other == kNoneU)
+            {
+                return 0u;
+            }
+        }
+        return s->payload[uint64_t(other) * 512 + p[0] + 8 * p[1] + 64 * p[2]];
+    };
+    constexpr int kDir[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
     for (uint32_t b = 0; b < s->bricks; ++b)
     {
         uint64_t occ[8] = {};
@@ -1518,19 +1557,37 @@ void oracle_scene_finalize(CxScene* s)
         bool any = false;
         for (uint32_t v = 0; v < 512; ++v)
         {
-            if (s->payload[uint64_t(b) * 512 + v] == 0)
+            const int x = int(v & 7u), y = int((v >> 3) & 7u), z = int(v >> 6);
+            const uint32_t raw = s->payload[uint64_t(b) * 512 + v];
+            const uint32_t own = (raw & opaque_mask) != 0u ? raw : (raw & id_mask);
+            uint32_t opaque_count = 0, faces = 0;
+            for (int d = 0; d < 6; ++d)
+            {
+                const uint32_t n = block_at(b, x + kDir[d][0], y + kDir[d][1], z + kDir[d][2]);
+                if ((n & opaque_mask) != 0u)
+                {
+                    ++opaque_count;
+                }
+                else if ((n & id_mask) != own)
+                {
+                    faces |= 1u << d;
+                }
+            }
+            const bool opaque = (own & opaque_mask) != 0u;
+            const uint32_t result = opaque ? own : (own | (faces << face_shift));
+            s->payload[uint64_t(b) * 512 + v] = uint16_t(result);
+            if (!(opaque_count < 6 && result != 0u && (opaque || faces != 0u)))
             {
                 continue;
             }
-            const uint32_t x = v & 7u, y = (v >> 3) & 7u, z = v >> 6;
             occ[(x >> 2) | ((y >> 2) << 1) | ((z >> 2) << 2)] |= 1ull << ((x & 3u) | ((y & 3u) << 2) | ((z & 3u) << 4));
             any = true;
-            lo[0] = std::min(lo[0], x);
-            lo[1] = std::min(lo[1], y);
-            lo[2] = std::min(lo[2], z);
-            hi[0] = std::max(hi[0], x);
-            hi[1] = std::max(hi[1], y);
-            hi[2] = std::max(hi[2], z);
+            const uint32_t c[3] = {uint32_t(x), uint32_t(y), uint32_t(z)};
+            for (int a = 0; a < 3; ++a)
+            {
+                lo[a] = std::min(lo[a], c[a]);
+                hi[a] = std::max(hi[a], c[a]);
+            }
         }
         uint32_t coarse = 0;
         for (int c = 0; c < 8; ++c)
@@ -1565,9 +1622,11 @@ uint32_t oracle_scene_light_records(CxScene* s, uint32_t* heap, uint32_t heap_un
     auto block_at = [&](uint32_t record, int x, int y, int z) -> uint32_t {
         return s->payload[uint64_t(record) * 512 + x + 8 * y + 64 * z];
     };
+    // LtOpenFaces (caelix_light_tree_build.hlsl:92-111): the light code of the voxel's material code.
     auto open_faces = [&](uint32_t record, uint32_t v, uint32_t& block) -> uint32_t {
         block = s->payload[uint64_t(record) * 512 + v];
-        if (block == 0u || ((s->light_codes[block >> 5] >> (block & 31u)) & 1u) == 0u)
+        const uint32_t code = MaterialCode(block, s->encoding);
+        if (((s->light_codes[code >> 5] >> (code & 31u)) & 1u) == 0u)
         {
             return 0u;
         }
@@ -1660,7 +1719,7 @@ uint32_t oracle_scene_light_records(CxScene* s, uint32_t* heap, uint32_t heap_un
                 {
                     continue;
                 }
-                const float* e = s->materials + uint64_t(blocks[v]) * 8 + 4;
+                const float* e = s->materials + uint64_t(MaterialCode(blocks[v], s->encoding)) * 8 + 4;
                 const float weight = e[0] * 0.2126f + e[1] * 0.7152f + e[2] * 0.0722f;
                 for (uint32_t f = 0; f < 6; ++f)
                 {

@@ -220,3 +220,31 @@ def test_trace_frame_odd_image_size(pair):
     assert np.array_equal(counters.cpu().numpy().view(np.uint64)[:5], w_counters[:5])
     rad = radiance.cpu().numpy().reshape(-1, 4)[:, :3].astype(np.float64)
     assert np.all(np.abs(rad - w_rad[:, :3]) <= 2e-5 * (1.0 + np.abs(w_rad[:, :3])))
+
+
+@pytest.mark.parametrize("encoding", [0, 1])
+def test_glass_world_finalize_and_rays(cx, oracle, encoding):
+    """The reference finalize test's glass world in both block encodings: the finalized payload (face
+    bits of glass and of the air around it), record words, occupancy and bitmaps equal the oracle's, and
+    rays through it (terminating at the transparent faces, caelix_brick_trace.hlsl:275-284) hit the same
+    voxels with the same t bits, faces and step counts."""
+    data = scene.glass_world(encoding)
+    print(f"glass world encoding={encoding}", flush=True)
+    gpu = cx.TraceScene(data, load_factor=LOAD)
+    arrays = gpu.host_arrays()
+    ref = oracle.OracleScene.build(data, slot_count(data))
+    for name in ("payload", "info", "occupancy", "brick_bits", "super_bits"):
+        assert np.array_equal(arrays[name][: ref.a[name].size], ref.a[name]), name
+    assert (arrays["payload"] >> (8 if encoding == 0 else 9) & 0x3F).any()  # face bits were written
+    rng = np.random.default_rng(50 + encoding)
+    count = 20000
+    o = rng.uniform([-4.0, 6.0, -4.0], [28.0, 16.0, 12.0], size=(count, 3))
+    d = rng.normal(size=(count, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, d, np.full((count, 1), 1e30)], axis=1).astype(np.float32)
+    got = gpu.trace_rays(rays)
+    want = ref.trace_rays(rays)
+    glass_hits = ((want[:, 1] >> 16) & 0xFFFF) & (0xC000 if encoding == 0 else 0x8000) == 0
+    assert ((want[:, 1] != 0) & glass_hits).sum() > 1000  # many rays end on a glass or air interface
+    bad = np.nonzero((got != want).any(axis=1))[0]
+    assert len(bad) == 0, (bad[:5], got[bad[:3]], want[bad[:3]])

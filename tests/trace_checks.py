@@ -25,8 +25,11 @@ def dense_blocks(data) -> np.ndarray:
 
 
 def is_light(blocks: np.ndarray, light_codes: np.ndarray) -> np.ndarray:
+    """Light voxels: the material code (an opaque value, a transparent id without face bits) is a light
+    code (block-ID encoding)."""
     b = blocks.astype(np.int64)
-    return ((light_codes[b >> 5] >> (b & 31).astype(np.uint32)) & 1).astype(bool) & (b != 0)
+    b = np.where((b & OPAQUE) != 0, b, b & 0x1FF)
+    return ((light_codes[b >> 5] >> (b & 31).astype(np.uint32)) & 1).astype(bool)
 
 
 def walk_first_solid(solid: np.ndarray, o, d, max_steps: int = 4096):
@@ -71,11 +74,57 @@ def walk_first_solid(solid: np.ndarray, o, d, max_steps: int = 4096):
     return None, t, axis, tie
 
 
-def finalize_expected(data):
-    """(info words [n, 2] with the light count left 0, occupancy [n, 8] uint64, brick_bits, super_bits)."""
-    p = np.asarray(data.payload)
-    n = len(p)
-    solid = p != 0
+ENCODINGS = {0: (0xC000, 0x00FF, 8), 1: (0x8000, 0x01FF, 9)}  # opaque mask, id mask, face shift
+
+
+def finalize_payload(data, encoding: int = 1):
+    """(finalized payload [n, 512] uint16, occupied [n, 512] bool) by the rules of the reference's
+    finalize test (tests/finalize_test.cpp:74-116, Reference()) over each brick and its face neighbours
+    (absent bricks are air): a transparent or air voxel takes the face bits of the neighbours that are
+    not opaque and of another medium; a voxel is occupied when alive (fewer than six opaque neighbours)
+    and not 0 and, as the kernel adds (caelix_finalize.hlsl:244-247), an interface (opaque or a face
+    bit). That last term differs from the test's Reference() only for a transparent voxel with no face."""
+    opaque_mask, id_mask, face_shift = ENCODINGS[encoding]
+    raw = np.asarray(data.payload).astype(np.int64)
+    n = len(raw)
+    keys = np.asarray(data.keys, np.int64)
+    index = {tuple(k): i for i, k in enumerate(keys.tolist())}
+    cube = raw.reshape(n, 8, 8, 8)  # [brick, z, y, x]
+    pad = np.zeros((n, 10, 10, 10), np.int64)
+    pad[:, 1:9, 1:9, 1:9] = cube
+    for axis, sign in ((0, 1), (0, -1), (1, 1), (1, -1), (2, 1), (2, -1)):
+        step = np.zeros(3, np.int64)
+        step[axis] = sign
+        nb = np.array([index.get(tuple(k), -1) for k in (keys + step[None, :]).tolist()], np.int64)
+        ok = nb >= 0
+        src = cube[np.where(ok, nb, 0)] * ok[:, None, None, None]
+        dim = 3 - axis  # array axis of this world axis ([brick, z, y, x])
+        take = 0 if sign > 0 else 7
+        put = 9 if sign > 0 else 0
+        sl = [slice(None), slice(1, 9), slice(1, 9), slice(1, 9)]
+        sl_src = [slice(None), slice(None), slice(None), slice(None)]
+        sl[dim] = put
+        sl_src[dim] = take
+        pad[tuple(sl)] = src[tuple(sl_src)]
+    opaque = (cube & opaque_mask) != 0
+    own = np.where(opaque, cube, cube & id_mask)
+    opaque_count = np.zeros_like(own)
+    faces = np.zeros_like(own)
+    for d, (dx, dy, dz) in enumerate(((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1))):
+        nbv = pad[:, 1 + dz:9 + dz, 1 + dy:9 + dy, 1 + dx:9 + dx]
+        n_opaque = (nbv & opaque_mask) != 0
+        opaque_count += n_opaque
+        faces |= np.where(~n_opaque & ((nbv & id_mask) != own), 1 << d, 0)
+    result = np.where(opaque, own, own | (faces << face_shift))
+    occupied = (opaque_count < 6) & (result != 0) & (opaque | (faces != 0))
+    return result.reshape(n, 512).astype(np.uint16), occupied.reshape(n, 512)
+
+
+def finalize_expected(data, encoding: int = 1):
+    """(info words [n, 2] with the light count left 0, occupancy [n, 8] uint64, brick_bits, super_bits)
+    of the finalized bricks (finalize_payload's occupied voxels)."""
+    _, solid = finalize_payload(data, encoding)
+    n = len(solid)
     v = np.arange(512)
     x, y, z = v & 7, (v >> 3) & 7, v >> 6
     coarse_of = (x >> 2) | ((y >> 2) << 1) | ((z >> 2) << 2)
