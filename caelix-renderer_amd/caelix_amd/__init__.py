@@ -65,7 +65,7 @@ class CxScene(ctypes.Structure):
         ("group_shift", ctypes.c_uint32), ("encoding", ctypes.c_uint32), ("materials", ctypes.c_void_p),
         ("light_codes", ctypes.c_void_p), ("emission_scale", ctypes.c_float), ("pad", ctypes.c_uint32),
         ("brick_emission_scale", ctypes.c_void_p), ("scene_count", ctypes.c_uint32), ("pad2", ctypes.c_uint32),
-        ("occupied_box", ctypes.c_void_p),
+        ("occupied_box", ctypes.c_void_p), ("materials_ext", ctypes.c_void_p),
     ]
 
 
@@ -100,7 +100,8 @@ class CxTraceParams(ctypes.Structure):
                 ("sun_radiance", ctypes.c_float * 3), ("sun_cos_radius", ctypes.c_float),
                 ("sky_zenith", ctypes.c_float * 3), ("pad0", ctypes.c_float), ("sky_horizon", ctypes.c_float * 3),
                 ("pad1", ctypes.c_float), ("sky_ground", ctypes.c_float * 3), ("pad2", ctypes.c_float),
-                ("fog_color", ctypes.c_float * 3), ("fog_density", ctypes.c_float)]
+                ("fog_color", ctypes.c_float * 3), ("fog_density", ctypes.c_float),
+                ("bounces_transparent", ctypes.c_uint32), ("pad3", ctypes.c_uint32 * 3)]
 
 
 TRACE_NO_RUSSIAN_ROULETTE = 1
@@ -493,6 +494,8 @@ class TraceScene:
         self.payload = torch.from_numpy(np.ascontiguousarray(data.payload).view(np.int16)).to(dev)
         self.materials = torch.from_numpy(np.ascontiguousarray(data.materials, np.float32)).to(dev)
         self.light_codes = torch.from_numpy(np.ascontiguousarray(data.light_codes).view(np.int32)).to(dev)
+        ext = getattr(data, "materials_ext", None)
+        self.materials_ext = None if ext is None else torch.from_numpy(np.ascontiguousarray(ext, np.float32)).to(dev)
         self.hash = BrickHashTable(self.keys, load_factor=load_factor, stream=stream)
         self.info = torch.zeros(n * 2, dtype=torch.int32, device=dev)
         self.occupancy = torch.zeros(n * 8, dtype=torch.int64, device=dev)
@@ -520,6 +523,7 @@ class TraceScene:
         sc.light_codes = self.light_codes.data_ptr()
         sc.emission_scale = 1.0
         sc.occupied_box = self.occupied_box.data_ptr()
+        sc.materials_ext = None if self.materials_ext is None else self.materials_ext.data_ptr()
         h = stream_handle(stream)
         check(lib.cx_scene_finalize(ctypes.byref(sc), h), "cx_scene_finalize")
         # Light pass, growing the heap when it does not fit (as the reference grows its light heap).
@@ -606,7 +610,8 @@ class TraceScene:
         return counters
 
     def trace_frame(self, camera: dict, bounces: int, seed: int, spp: int = 1, color=None, radiance=None,
-                    attrib=None, counters=None, stream=None, flags: int = 0, sky: dict | None = None, fog=None):
+                    attrib=None, counters=None, stream=None, flags: int = 0, sky: dict | None = None, fog=None,
+                    bounces_transparent: int = 0):
         torch = _torch()
         w, h = int(camera["width"]), int(camera["height"])
         color = color if color is not None else torch.empty(w * h * 4, dtype=torch.float32, device="cuda")
@@ -616,6 +621,7 @@ class TraceScene:
         cam = camera_struct(camera)
         params = CxTraceParams(bounces, seed, spp, flags)
         apply_sky(params, sky, fog)
+        params.bounces_transparent = int(bounces_transparent)
         out = self._output(camera, color, radiance, attrib, counters)
         check(self.lib.cx_trace_frame(ctypes.byref(self.scene), ctypes.byref(self.view), ctypes.byref(cam),
                                       ctypes.byref(params), ctypes.byref(out), stream_handle(stream)), "cx_trace_frame")

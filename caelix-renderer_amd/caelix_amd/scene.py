@@ -34,6 +34,7 @@ class SceneData:
     camera: dict
     emissive_bricks: np.ndarray
     encoding: int = 1        # block values: 1 block-ID (opaque 0x8000, id 0x1FF, faces << 9), 0 packed
+    materials_ext: np.ndarray | None = None  # float32 [65536, 4] (ior, extinction, 0, 0): the full material model
 
 
 def value_noise(size: int, seed: int, octaves: int, base_period: int) -> np.ndarray:
@@ -265,6 +266,70 @@ def glass_world(encoding: int = 1) -> SceneData:
     return SceneData(keys=np.array(keys, np.int32), payload=np.stack(payload), materials=table,
                      light_codes=np.zeros(2048, np.uint32), grid_min=(0, 0, 0), grid_dim=8, group_shift=4,
                      camera=camera, emissive_bricks=np.zeros(0, np.int64), encoding=encoding)
+
+
+GLASS = 0x42  # the transparent id of the glass scenes
+
+
+def glass_slab(size_bricks: int = 6, ior: float = 1.5, extinction: float = 0.3, tint=(0.9, 0.7, 0.5),
+               smoothness: float = 1.0, floor_emission=(2.0, 2.0, 2.0), width: int = 64, height: int = 64) -> SceneData:
+    """A glass slab on an emissive floor under an open sky: size_bricks^2 floor bricks (voxels y 0..7
+    opaque 0x8123, albedo 0.5, emission floor_emission, not a light code, so no NEE) and above them glass
+    voxels y 8..11 (transparent id 0x42: tint, smoothness, ior, extinction in the extended table; the
+    other codes ior 1.5, extinction 0). The camera looks down at the slab from y = 60 with a 40 degree
+    field of view, so every camera ray meets the slab's top face (y = 12) first."""
+    dim = size_bricks
+    bx, bz = np.meshgrid(np.arange(dim), np.arange(dim), indexing="ij")
+    cols = np.stack([bx.ravel(), bz.ravel()], axis=1)
+    floor = OPAQUE | 0x123
+    v = np.arange(512)
+    ly = (v >> 3) & 7
+    keys = np.concatenate([np.stack([cols[:, 0], np.zeros(len(cols), np.int64), cols[:, 1]], axis=1),
+                           np.stack([cols[:, 0], np.ones(len(cols), np.int64), cols[:, 1]], axis=1)]).astype(np.int32)
+    payload = np.concatenate([np.full((len(cols), 512), floor, np.uint16),
+                              np.tile(np.where(ly < 4, GLASS, 0).astype(np.uint16), (len(cols), 1))])
+    table = np.zeros((65536, 8), np.float32)
+    table[floor, 0:3] = 0.5
+    table[floor, 4:7] = np.asarray(floor_emission, np.float32)
+    table[GLASS, 0:3] = np.asarray(tint, np.float32)
+    table[GLASS, 3] = np.float32(smoothness)
+    ext = np.zeros((65536, 4), np.float32)
+    ext[:, 0] = 1.5
+    ext[GLASS, 0] = np.float32(ior)
+    ext[GLASS, 1] = np.float32(extinction)
+    c = dim * 4.0
+    camera = look_at((c + 3.1, 60.0, c - 2.3), (c, 12.0, c), width, height, fov_y_deg=40.0)
+    return SceneData(keys=keys, payload=payload, materials=table, light_codes=np.zeros(2048, np.uint32),
+                     grid_min=(0, 0, 0), grid_dim=max(8, (dim + 7) // 8 * 8), group_shift=4, camera=camera,
+                     emissive_bricks=np.zeros(0, np.int64), materials_ext=ext)
+
+
+def glass_terrain(seed: int = 1) -> SceneData:
+    """The small terrain with the full material model: opaque materials 1..16 become transparent media
+    (ids 1..16; 1..8 smooth, delta interfaces; 9..16 rough, shaded stochastically; tints, ior 1.2..1.8,
+    extinction 0.02..0.4), materials 17..48 get a specular lobe (smoothness 0.3..1, metallic 0..1,
+    ior 1.2..2); the emissive bricks stay lamps."""
+    data = small_scene(seed)
+    rng = np.random.default_rng(seed + 77)
+    payload = np.asarray(data.payload).copy()
+    ids = payload & 0x7FFF
+    to_glass = ((payload & OPAQUE) != 0) & (ids >= 1) & (ids <= 16)
+    payload[to_glass] = ids[to_glass]
+    table = np.asarray(data.materials).copy()
+    ext = np.zeros((65536, 4), np.float32)
+    ext[:, 0] = 1.5
+    media = np.arange(1, 17)
+    table[media, 0:3] = rng.uniform(0.4, 1.0, size=(16, 3)).astype(np.float16).astype(np.float32)
+    table[media, 3] = np.where(media <= 8, 1.0, rng.uniform(0.2, 0.9, size=16)).astype(np.float32)
+    ext[media, 0] = rng.uniform(1.2, 1.8, size=16).astype(np.float16).astype(np.float32)
+    ext[media, 1] = rng.uniform(0.02, 0.4, size=16).astype(np.float16).astype(np.float32)
+    shiny = OPAQUE | np.arange(17, 49)
+    table[shiny, 3] = rng.uniform(0.3, 1.0, size=32).astype(np.float16).astype(np.float32)
+    table[shiny, 7] = rng.uniform(0.0, 1.0, size=32).astype(np.float16).astype(np.float32)
+    ext[shiny, 0] = rng.uniform(1.2, 2.0, size=32).astype(np.float16).astype(np.float32)
+    return SceneData(keys=data.keys, payload=payload, materials=table, light_codes=data.light_codes,
+                     grid_min=data.grid_min, grid_dim=data.grid_dim, group_shift=data.group_shift, camera=data.camera,
+                     emissive_bricks=data.emissive_bricks, materials_ext=ext)
 
 
 def diff_scene(seed: int = 5) -> SceneData:

@@ -36,6 +36,7 @@ struct TraceParams
     SkyParams sky;
     V3 fog;             // the fog's radiance (fog_color.rgb)
     float fog_density;  // fog_color.w; 0: no fog
+    uint32_t delta_budget;  // bounce_count_transparent: smooth interfaces the camera ray crosses
 };
 
 // TraverseSegment (caelix_shading.hlsl:150-175) in the air: a segment of length t that ends on a
@@ -43,7 +44,7 @@ struct TraceParams
 // before it.
 CX_DEV void FogSegment(const TraceParams& p, float t, V3& throughput, V3& radiance)
 {
-    const float transmission = ExpNonPositive((-p.fog_density) * t);
+    const float transmission = ExpCephes((-p.fog_density) * t);
     radiance = Add(radiance, Mul(MulV(throughput, p.fog), 1.0f - transmission));
     throughput = Mul(throughput, transmission);
 }
@@ -160,6 +161,191 @@ CX_DEV Shade ShadeHit(const SceneView& s, uint32_t block, V3 hit_position, V3 no
     return sh;
 }
 
+
+// ---- the full material model (a scene with the extended material table) -----------------------
+CX_DEV bool BlockOpaque(uint32_t block, uint32_t encoding)
+{
+    return (block & (encoding == kEncodingPacked ? kPackedOpaqueMask : kBlockIdOpaqueMask)) != 0u;
+}
+// CaelixMaterialCode (caelix_material.hlsl:65-70): an opaque value as it is, a transparent one
+// without its face bits (its medium id; 0 = air).
+CX_DEV uint32_t BlockCode(uint32_t block, uint32_t encoding)
+{
+    return BlockOpaque(block, encoding) ? block : (block & (encoding == kEncodingPacked ? kPackedIdMask : kBlockIdIdMask));
+}
+struct Material
+{
+    V3 albedo;
+    float smoothness;
+    V3 emission;
+    float metallic;
+    float ior;
+    float extinction;
+};
+// CaelixGetMaterial (caelix_material.hlsl:84-100) of a table index; without the extended table
+// ior 1.5 and extinction 0 (the reference's default entries).
+CX_DEV Material GetMaterial(const SceneView& s, uint32_t code)
+{
+    const float4 a = s.materials[2 * uint64_t(code)];
+    const float4 e = s.materials[2 * uint64_t(code) + 1];
+    Material m;
+    m.albedo = Make(a.x, a.y, a.z);
+    m.smoothness = a.w;
+    m.emission = Make(e.x, e.y, e.z);
+    m.metallic = e.w;
+    m.ior = 1.5f;
+    m.extinction = 0.0f;
+    if (s.materials_ext != nullptr)
+    {
+        const float4 x = s.materials_ext[code];
+        m.ior = x.x;
+        m.extinction = x.y;
+    }
+    return m;
+}
+// HLSL reflect(i, n) = i - 2 dot(n, i) n and refract(i, n, eta) (0 past the critical angle).
+CX_DEV V3 Reflect(V3 i, V3 n)
+{
+    return Sub(i, Mul(n, 2.0f * Dot(n, i)));
+}
+CX_DEV V3 Refract(V3 i, V3 n, float eta)
+{
+    const float c = Dot(n, i);
+    const float k = 1.0f - eta * eta * (1.0f - c * c);
+    if (k < 0.0f)
+    {
+        return Make(0.0f, 0.0f, 0.0f);
+    }
+    return Sub(Mul(i, eta), Mul(n, eta * c + Sqrt(k)));
+}
+// FresnelReflectAmountOpaque / FresnelReflectAmountTransparent (caelix_shading.hlsl:116-148): Schlick,
+// with the transmitted cosine when n1 > n2 and 1 past the critical angle.
+CX_DEV float Schlick(float r0, float cos_x)
+{
+    const float x = 1.0f - cos_x;
+    const float xx = x * x;
+    return r0 + (1.0f - r0) * xx * xx * x;
+}
+CX_DEV float FresnelOpaque(float n1, float n2, V3 incident, V3 normal)
+{
+    float r0 = Div(n1 - n2, n1 + n2);
+    r0 *= r0;
+    return Schlick(r0, -Dot(normal, incident));
+}
+CX_DEV float FresnelTransparent(float n1, float n2, V3 incident, V3 normal)
+{
+    float r0 = Div(n1 - n2, n1 + n2);
+    r0 *= r0;
+    float cos_x = -Dot(normal, incident);
+    if (n1 > n2)
+    {
+        const float n = Div(n1, n2);
+        const float sin_t2 = n * n * (1.0f - cos_x * cos_x);
+        if (sin_t2 >= 1.0f)
+        {
+            return 1.0f;
+        }
+        cos_x = Sqrt(1.0f - sin_t2);
+    }
+    return Schlick(r0, cos_x);
+}
+struct Interface
+{
+    V3 normal;  // against the incoming ray
+    V3 direction;
+    bool refracted;
+    uint32_t medium;  // after the interface
+};
+// CrossInterface (caelix_shading.hlsl:203-243), random mode: refraction when it exists and a number
+// of `rng` >= the Fresnel reflectance (the number is drawn only when refraction exists).
+CX_DEV Interface CrossInterface(const SceneView& s, uint32_t source, uint32_t destination, float destination_ior, V3 dir,
+                                V3 normal, uint32_t& rng)
+{
+    Interface it;
+    it.normal = SafeNormalize(normal);
+    if (Dot(dir, it.normal) > 0.0f)
+    {
+        it.normal = Make(-it.normal.x, -it.normal.y, -it.normal.z);
+    }
+    const float source_ior = source != 0u ? GetMaterial(s, source).ior : 1.0f;
+    const float target_ior = destination != 0u ? destination_ior : 1.0f;
+    const float eta = Div(source_ior, target_ior);
+    const V3 reflection = Reflect(dir, it.normal);
+    const V3 refraction = Refract(dir, it.normal, eta);
+    const bool can_refract = Dot(refraction, refraction) > 0.000001f;
+    const float fresnel = can_refract ? FresnelTransparent(source_ior, target_ior, dir, it.normal) : 1.0f;
+    it.refracted = can_refract && RandomFloat01(rng) >= fresnel;
+    it.direction = SafeNormalize(it.refracted ? refraction : reflection);
+    it.medium = it.refracted ? destination : source;
+    return it;
+}
+struct ShadeFull
+{
+    V3 albedo;
+    V3 emission;
+    V3 bounce_origin;
+    V3 bounce_direction;
+    bool specular;  // the bounce took the specular lobe (always at a transparent block): no NEE
+};
+// ShadeVoxelHit (caelix_shading.hlsl:267-307). Transparent block: the interface from `medium` into
+// the block's medium, albedo 1, origin pushed to the side of the chosen direction, `medium` updated.
+// Opaque block: specular chance lerp(metallic, 1, fresnel smoothness) against the lobe number, the
+// cosine direction, the specular direction lerp(diffuse, reflect, smoothness), the chosen direction
+// lerp(diffuse, specular, 0 or 1). HLSL lerp(a, b, t) is taken as a + t (b - a).
+CX_DEV ShadeFull ShadeVoxel(const SceneView& s, uint32_t block, V3 hit_position, V3 dir, V3 normal, uint32_t& rng,
+                            uint32_t& medium)
+{
+    const uint32_t code = BlockCode(block, s.encoding);
+    const Material m = GetMaterial(s, code);
+    ShadeFull sh;
+    sh.emission = Mul(m.emission, s.emission_scale);
+    if (!BlockOpaque(block, s.encoding))
+    {
+        const Interface it = CrossInterface(s, medium, code, m.ior, dir, normal, rng);
+        sh.albedo = Make(1.0f, 1.0f, 1.0f);
+        sh.bounce_origin = Add(hit_position, Mul(it.normal, it.refracted ? -kPushOff : kPushOff));
+        sh.bounce_direction = it.direction;
+        sh.specular = true;
+        medium = it.medium;
+        return sh;
+    }
+    sh.albedo = m.albedo;
+    const float fresnel = FresnelOpaque(1.0f, m.ior, dir, normal);
+    const float chance = m.metallic + fresnel * m.smoothness * (1.0f - m.metallic);
+    const float do_specular = RandomFloat01(rng) < chance ? 1.0f : 0.0f;
+    V3 diffuse = SafeNormalize(Add(normal, RandomUnitVector(rng)));
+    diffuse = Dot(diffuse, diffuse) > 0.0f ? diffuse : normal;
+    const V3 specular = SafeNormalize(Lerp(diffuse, Reflect(dir, normal), m.smoothness));
+    sh.bounce_origin = Add(hit_position, Mul(normal, kPushOff));
+    sh.bounce_direction = Lerp(diffuse, specular, do_specular);
+    sh.specular = do_specular > 0.5f;
+    return sh;
+}
+// TraverseSegment (caelix_shading.hlsl:150-175): in the air the fog; inside a medium the throughput
+// keeps exp(-(1 - albedo) t extinction) per channel.
+CX_DEV void MediumSegment(const SceneView& s, const TraceParams& p, uint32_t medium, float t, V3& throughput,
+                          V3& radiance)
+{
+    if (medium == 0u)
+    {
+        if (p.fog_density > 0.0f)
+        {
+            FogSegment(p, t, throughput, radiance);
+        }
+        return;
+    }
+    const Material m = GetMaterial(s, medium);
+    const V3 transmission = Make(ExpCephes((-(1.0f - m.albedo.x)) * t * m.extinction),
+                                 ExpCephes((-(1.0f - m.albedo.y)) * t * m.extinction),
+                                 ExpCephes((-(1.0f - m.albedo.z)) * t * m.extinction));
+    throughput = MulV(throughput, transmission);
+}
+// The sky a segment that misses receives (SkyDiffuseMiss / SkySpecularMiss with the procedural sky):
+// the sun disk when the segment sees the sun.
+CX_DEV V3 SkySegmentMiss(const SkyParams& k, V3 d, bool sees_sun)
+{
+    return Add(ProceduralSky(k, d), sees_sun ? SunDisk(k, d) : Make(0.0f, 0.0f, 0.0f));
+}
 
 CX_DEV WorldHit Trace(const SceneView& s, V3 o, V3 d, float t_max, PathCounters& c, uint32_t scene = 0)
 {

@@ -30,6 +30,7 @@ SceneView SceneOf(const CxScene& c)
     }
     s.grid_dim = c.grid_dim;
     s.materials = reinterpret_cast<const float4*>(c.materials);
+    s.materials_ext = reinterpret_cast<const float4*>(c.materials_ext);
     s.light_codes = c.light_codes;
     s.emission_scale = c.emission_scale;
     s.encoding = c.encoding;
@@ -269,9 +270,12 @@ CX_DEV void EnqueueShadow(const Workspace& w, uint32_t path, bool active, V3 ori
     }
 }
 
-CX_DEV uint32_t PackState(uint32_t segment_light, uint32_t bounce, uint32_t face)
+// state.z: segment light | bounce << 8 | segment face << 16 | medium << 19 | segment sees the sun << 28
+// (the last two only in the full material model).
+CX_DEV uint32_t PackState(uint32_t segment_light, uint32_t bounce, uint32_t face, uint32_t medium = 0u,
+                          bool sees_sun = false)
 {
-    return segment_light | (bounce << 8) | (face << 16);
+    return segment_light | (bounce << 8) | (face << 16) | (medium << 19) | (sees_sun ? (1u << 28) : 0u);
 }
 
 // ---- adjoint helpers ---------------------------------------------------------------------------
@@ -574,10 +578,114 @@ __global__ void __launch_bounds__(kWaveThreads) PersistentTraceKernel(SceneView 
     FlushCounters(counters, c);
 }
 
+// The primary vertex in the full material model: the delta chain of the camera ray (RunDeltaChain,
+// caelix_path_trace.hlsl:613-678; traced here, it is short and rare), then ShadeVoxelHit at the
+// surface that ended it, the NEE unless that shade took the specular lobe, and the first bounce ray.
+// Outputs: albedo = chain transmission x albedo, emission = chain transmission x emission + the
+// fog along the chain (caelix_path_trace.hlsl:1270-1284); a chain that leaves the scene shows the
+// sky in its last direction times its transmission; depth = the chain's length, attrib = the last
+// surface or interface it met.
+CX_DEV void PrimaryGlass(const SceneView& s, const TraceParams& prm, const Workspace& w, uint32_t path, uint32_t scene,
+                         const LightView& L, V3 origin, V3 dir, uint4 h, uint32_t& rng, uint32_t& light_rng,
+                         uint32_t& nee_rng, uint32_t mis_bit, PathCounters& c, bool& shadow, bool& go_on,
+                         V3& shadow_origin, V3& shadow_dir, float& shadow_t, V3& shadow_add, uint32_t& shadow_light,
+                         V3& shadow_unscaled)
+{
+    WorldHit surface{__uint_as_float(h.x), h.y, h.z, {0, 0, 0}, 0u, 0u, 0u};
+    uint32_t geometry = h.y;
+    V3 ray_o = origin;
+    V3 ray_d = dir;
+    V3 chain = Make(1.0f, 1.0f, 1.0f);
+    V3 inscatter = Make(0.0f, 0.0f, 0.0f);
+    float t_virtual = 0.0f;
+    uint32_t medium = 0u;
+    uint32_t interfaces = 0u;
+    bool surface_hit = true;
+    for (;;)
+    {
+        MediumSegment(s, prm, medium, surface.t, chain, inscatter);
+        t_virtual += surface.t;
+        geometry = surface.attrib;
+        if (interfaces >= prm.delta_budget)
+        {
+            break;
+        }
+        const uint32_t block = (surface.attrib >> 16) & 0xFFFFu;
+        if (BlockOpaque(block, s.encoding))
+        {
+            break;
+        }
+        const uint32_t code = BlockCode(block, s.encoding);
+        const Material m = GetMaterial(s, code);
+        if (m.smoothness < 0.99f)
+        {
+            break;
+        }
+        const V3 hp = Add(ray_o, Mul(ray_d, surface.t));
+        const Interface it = CrossInterface(s, medium, code, m.ior, ray_d, FaceNormal(surface.attrib), rng);
+        interfaces += 1u;
+        medium = it.medium;
+        ray_o = Add(hp, Mul(it.normal, it.refracted ? -kPushOff : kPushOff));
+        ray_d = it.direction;
+        c.rays += 1;
+        surface = Trace(s, ray_o, ray_d, kRayTMax, c, scene);
+        if (surface.attrib == 0u)
+        {
+            surface_hit = false;
+            break;
+        }
+    }
+    if (!surface_hit)
+    {
+        const V3 sky = prm.sky.on != 0u ? SkyRadiance(prm.sky, ray_d) : Make(0.0f, 0.0f, 0.0f);
+        const V3 deterministic = Add(MulV(chain, sky), inscatter);
+        w.albedo[path] = make_float4(0.0f, 0.0f, 0.0f, t_virtual);
+        w.emission[path] = make_float4(deterministic.x, deterministic.y, deterministic.z, __uint_as_float(geometry));
+        return;
+    }
+    const uint32_t face = FaceIndex(surface.attrib);
+    const V3 normal = FaceDirection(face);
+    const ShadeFull sh = ShadeVoxel(s, (surface.attrib >> 16) & 0xFFFFu, Add(ray_o, Mul(ray_d, surface.t)), ray_d, normal,
+                                    rng, medium);
+    const V3 emission = Mul(sh.emission, BrickScale(s, surface.record));
+    const V3 albedo = MulV(chain, sh.albedo);
+    const V3 deterministic = Add(MulV(chain, emission), inscatter);
+    w.albedo[path] = make_float4(albedo.x, albedo.y, albedo.z, t_virtual);
+    w.emission[path] = make_float4(deterministic.x, deterministic.y, deterministic.z, __uint_as_float(geometry));
+    uint32_t segment_light = 0u;
+    bool sees_sun = true;
+    if (prm.bounces >= 1u)
+    {
+        if (!sh.specular)
+        {
+            V3 visible;
+            if (NeeCandidate(s, L, prm.sky, sh.bounce_origin, normal, light_rng, nee_rng, shadow_dir, shadow_t, visible,
+                             shadow_unscaled, shadow_light))
+            {
+                c.shadow_rays += 1;
+                shadow = shadow_t > 0.0f;
+                shadow_origin = sh.bounce_origin;
+                shadow_add = visible;
+            }
+            segment_light = kSegmentLampNee | mis_bit;
+            sees_sun = prm.sky.sun == 0u;
+        }
+        go_on = true;
+    }
+    w.throughput[path] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    w.state[path] = make_uint4(rng, light_rng, PackState(segment_light, 1u, face, medium, sees_sun), 0u);
+    if (prm.sky.sun != 0u)
+    {
+        w.nee_rng[path] = nee_rng;
+    }
+    w.ray_o[path] = make_float4(sh.bounce_origin.x, sh.bounce_origin.y, sh.bounce_origin.z, 0.0f);
+    w.ray_d[path] = make_float4(sh.bounce_direction.x, sh.bounce_direction.y, sh.bounce_direction.z, 0.0f);
+}
+
 // The primary vertex: outputs of the hit, the first shade, the NEE at vertex 0 and the first
 // bounce ray (queue 0). Adjoint: the path's adjoint dL/dI / spp * albedo, and the primary emission's
 // emission-scale gradient.
-template <class Views>
+template <class Views, bool Glass>
 __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, Views views, TraceParams prm,
                                                                    Workspace w, uint32_t paths, GradView g,
                                                                    unsigned long long* counters)
@@ -625,6 +733,14 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
             const uint32_t mis_bit = L.mis != 0u ? kSegmentLampMis : 0u;
             const uint32_t face = FaceIndex(h.y);
             const V3 normal = FaceDirection(face);
+            if constexpr (Glass)
+            {
+                PrimaryGlass(s, prm, w, path, views.SceneOf(path), L, origin, dir, h, rng, light_rng, nee_rng, mis_bit,
+                             c, shadow, go_on, shadow_origin, shadow_dir, shadow_t, shadow_add, shadow_light,
+                             shadow_unscaled);
+            }
+            else
+            {
             const Shade sh = ShadeHit(s, (h.y >> 16) & 0xFFFFu, Add(origin, Mul(dir, t)), normal, rng);
             const V3 emission = Mul(sh.emission, BrickScale(s, h.z));
             if (prm.fog_density > 0.0f)
@@ -677,6 +793,7 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
             }
             w.ray_o[path] = make_float4(sh.bounce_origin.x, sh.bounce_origin.y, sh.bounce_origin.z, 0.0f);
             w.ray_d[path] = make_float4(sh.bounce_direction.x, sh.bounce_direction.y, sh.bounce_direction.z, 0.0f);
+            }
         }
     }
     EnqueueShadow(w, path, shadow, shadow_origin, shadow_dir, shadow_t, shadow_add, shadow_light, shadow_unscaled, 1.0f,
@@ -691,7 +808,7 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
 
 // One iteration of TracePath's bounce loop for the paths in queue[(bounce - 1) & 1]. Adjoint: the
 // gradients of the emission term of this vertex.
-template <class Views>
+template <class Views, bool Glass>
 __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, Views views, TraceParams prm, Workspace w,
                                                                   uint32_t bounce, uint32_t paths, GradView g,
                                                                   unsigned long long* counters)
@@ -728,7 +845,11 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
             const float4 d4 = w.ray_d[path];
             const float4 t4 = w.throughput[path];
             const float4 r4 = w.radiance[path];
-            const V3 add = MulV(SkyMiss(prm.sky, Make(d4.x, d4.y, d4.z)), Make(t4.x, t4.y, t4.z));
+            // The full material model keeps whether the segment sees the sun (it does after a specular
+            // vertex, which ran no NEE); else every segment leaves an NEE vertex.
+            const V3 sky = Glass ? SkySegmentMiss(prm.sky, Make(d4.x, d4.y, d4.z), ((w.state[path].z >> 28) & 1u) != 0u)
+                                 : SkyMiss(prm.sky, Make(d4.x, d4.y, d4.z));
+            const V3 add = MulV(sky, Make(t4.x, t4.y, t4.z));
             w.radiance[path] = make_float4(r4.x + add.x, r4.y + add.y, r4.z + add.z, 0.0f);
         }
         if (h.y != 0u)
@@ -740,7 +861,8 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
             uint32_t nee_rng = prm.sky.sun != 0u ? w.nee_rng[path] : 0u;
             uint32_t segment_light = st.z & 0xFFu;
             uint32_t bounce_index = (st.z >> 8) & 0xFFu;
-            const uint32_t segment_face = st.z >> 16;
+            const uint32_t segment_face = (st.z >> 16) & 7u;
+            uint32_t medium = (st.z >> 19) & 0x1FFu;
             uint32_t verts = st.w;
             const float4 o4 = w.ray_o[path];
             const float4 d4 = w.ray_d[path];
@@ -755,14 +877,27 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
             hit.t = __uint_as_float(h.x);
             hit.attrib = h.y;
             hit.record = h.z;
-            if (prm.fog_density > 0.0f)
+            if constexpr (Glass)
+            {
+                MediumSegment(s, prm, medium, hit.t, throughput, radiance);
+            }
+            else if (prm.fog_density > 0.0f)
             {
                 FogSegment(prm, hit.t, throughput, radiance);
             }
             const uint32_t face = FaceIndex(h.y);
             const uint32_t block = (h.y >> 16) & 0xFFFFu;
             const V3 vertex_normal = FaceDirection(face);
-            const Shade sh = ShadeHit(s, block, Add(origin, Mul(dir, hit.t)), vertex_normal, rng);
+            ShadeFull sh;
+            if constexpr (Glass)
+            {
+                sh = ShadeVoxel(s, block, Add(origin, Mul(dir, hit.t)), dir, vertex_normal, rng, medium);
+            }
+            else
+            {
+                const Shade lambert = ShadeHit(s, block, Add(origin, Mul(dir, hit.t)), vertex_normal, rng);
+                sh = ShadeFull{lambert.albedo, lambert.emission, lambert.bounce_origin, lambert.bounce_direction, false};
+            }
             const V3 emission = Mul(sh.emission, BrickScale(s, h.z));
             bounce_index += 1;
             float emission_weight = 1.0f;
@@ -783,9 +918,11 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
             throughput = MulV(throughput, Make(Div(sh.albedo.x, 1.0f), Div(sh.albedo.y, 1.0f), Div(sh.albedo.z, 1.0f)));
             verts = AppendVertex(verts, block & 0xFFu);
             segment_light = 0u;
-            if (bounce_index <= prm.bounces)
+            bool sees_sun = true;
+            if (!sh.specular && bounce_index <= prm.bounces)
             {
                 segment_light = kSegmentLampNee | (L.mis != 0u ? kSegmentLampMis : 0u);
+                sees_sun = prm.sky.sun == 0u;
                 V3 visible;
                 if (NeeCandidate(s, L, prm.sky, sh.bounce_origin, vertex_normal, light_rng, nee_rng, shadow_dir, shadow_t,
                                  visible, shadow_unscaled, shadow_light))
@@ -814,7 +951,10 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
             }
             w.radiance[path] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
             w.throughput[path] = make_float4(throughput.x, throughput.y, throughput.z, roulette);
-            w.state[path] = make_uint4(rng, light_rng, PackState(segment_light, bounce_index, face), verts);
+            w.state[path] = make_uint4(rng, light_rng,
+                                       Glass ? PackState(segment_light, bounce_index, face, medium, sees_sun)
+                                             : PackState(segment_light, bounce_index, face),
+                                       verts);
             if (prm.sky.sun != 0u)
             {
                 w.nee_rng[path] = nee_rng;
@@ -1087,6 +1227,8 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
                  const GradView& g, const Workspace& w, const FrameOut& o, hipStream_t s)
 {
     const bool adjoint = g.d_color != nullptr;
+    // The full material model: the scene's extended material table (or a delta chain) asks for it.
+    const bool glass = sv.materials_ext != nullptr || params->bounces_transparent > 0u;
     const uint32_t blocks = DivUp(paths, kWaveThreads);
     const uint32_t rows = GradRows(paths);
     const uint32_t bounce_blocks = std::min(TunedBlocks(PersistentBlocks(PersistentTraceKernel<BounceSource<Views>>)), rows);
@@ -1102,7 +1244,8 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
     {
         const TraceParams p{params->bounces, params->frame_seed + sample, mis,
                             (params->flags & CX_TRACE_NO_RUSSIAN_ROULETTE) != 0 ? 0u : 1u, SkyOf(params),
-                            V3{params->fog_color[0], params->fog_color[1], params->fog_color[2]}, params->fog_density};
+                            V3{params->fog_color[0], params->fog_color[1], params->fog_color[2]}, params->fog_density,
+                            params->bounces_transparent};
         ResetKernel<<<1, 64, 0, s>>>(w);
         {
             StageScope scope(s, adjoint ? "adjoint_trace" : "trace_primary");
@@ -1110,7 +1253,14 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
         }
         {
             StageScope scope(s, adjoint ? "adjoint_shade" : "trace_shade_primary");
-            PrimaryShadeKernel<<<blocks, kWaveThreads, 0, s>>>(sv, views, p, w, paths, g, o.counters);
+            if (glass)
+            {
+                PrimaryShadeKernel<Views, true><<<blocks, kWaveThreads, 0, s>>>(sv, views, p, w, paths, g, o.counters);
+            }
+            else
+            {
+                PrimaryShadeKernel<Views, false><<<blocks, kWaveThreads, 0, s>>>(sv, views, p, w, paths, g, o.counters);
+            }
         }
         {
             StageScope scope(s, adjoint ? "adjoint_trace" : "trace_shadow");
@@ -1126,7 +1276,16 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
             }
             {
                 StageScope scope(s, adjoint ? "adjoint_shade" : "trace_shade_bounce");
-                BounceShadeKernel<<<blocks, kWaveThreads, 0, s>>>(sv, views, p, w, bounce, paths, g, o.counters);
+                if (glass)
+                {
+                    BounceShadeKernel<Views, true><<<blocks, kWaveThreads, 0, s>>>(sv, views, p, w, bounce, paths, g,
+                                                                                   o.counters);
+                }
+                else
+                {
+                    BounceShadeKernel<Views, false><<<blocks, kWaveThreads, 0, s>>>(sv, views, p, w, bounce, paths, g,
+                                                                                    o.counters);
+                }
             }
             {
                 StageScope scope(s, adjoint ? "adjoint_trace" : "trace_shadow");
@@ -1174,9 +1333,13 @@ int CheckParams(const CxTraceParams* params, const CxTraceOutput* out, uint32_t 
     {
         return ArgError("%s: fog density or colour negative or not finite", what);
     }
-    if (adjoint && ((params->flags & CX_TRACE_SKY) != 0 || params->fog_density > 0.0f))
+    if (params->bounces_transparent > 15)
     {
-        Log(3, "%s: the adjoint has no sky, sun or fog (CX_TRACE_SKY, fog_density)", what);
+        return ArgError("%s: bounces_transparent above 15", what);
+    }
+    if (adjoint && ((params->flags & CX_TRACE_SKY) != 0 || params->fog_density > 0.0f || params->bounces_transparent > 0))
+    {
+        Log(3, "%s: the adjoint has no sky, sun, fog or delta chain", what);
         return CX_ERROR_UNSUPPORTED;
     }
     if ((params->flags & CX_TRACE_SKY) != 0)
@@ -1223,6 +1386,11 @@ int RunFrame(const CxScene* scene, const CxLightTreeView* lights, const CxCamera
     if (int r = CheckParams(params, out, camera->width, camera->height, grads != nullptr, what); r != CX_OK)
     {
         return r;
+    }
+    if (grads != nullptr && scene->materials_ext != nullptr)
+    {
+        Log(3, "%s: the adjoint shades opaque Lambertian voxels only (materials_ext set)", what);
+        return CX_ERROR_UNSUPPORTED;
     }
     if (uint64_t(camera->width) * camera->height > (uint64_t(1) << 28))
     {

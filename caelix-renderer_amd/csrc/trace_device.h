@@ -32,7 +32,8 @@ struct SceneView
     const int32_t* box;          // per scene: super-cell box of the resident bricks (lo, hi), or null
     int32_t grid_min[3];         // brick coordinate of grid cell 0
     uint32_t grid_dim;           // bricks per axis, a multiple of 8
-    const float4* materials;     // 2 per 16-bit block value: (albedo, metallic), (emission, smoothness)
+    const float4* materials;     // 2 per 16-bit block value: (albedo, smoothness), (emission, metallic)
+    const float4* materials_ext; // null, or 1 per block value: (ior, extinction, 0, 0)
     const uint32_t* light_codes; // 65536 bits: the block value is a light material
     float emission_scale;
     uint32_t encoding;
@@ -193,15 +194,19 @@ CX_DEV void SinCos(float a, float& s, float& c)
 }
 
 // ---- Cephes expf -------------------------------------------------------------------------------
-// exp(x) for x <= 0 (the fog's transmission): x = n ln 2 + r with n = floor(x log2 e + 0.5), r by
+// exp(x) (the transmissions of fog and media): x = n ln 2 + r with n = floor(x log2 e + 0.5), r by
 // the two-part ln 2, the degree-5 polynomial of Cephes expf (S. Moshier, Cephes Math Library), times
-// 2^n from its exponent bits; 0 below -87, where the result would leave the normal range. exp(0) is
-// exactly 1.
-CX_DEV float ExpNonPositive(float x)
+// 2^n from its exponent bits; 0 below -87, where the result would leave the normal range, and +inf
+// above 88. exp(0) is exactly 1.
+CX_DEV float ExpCephes(float x)
 {
     if (!(x >= -87.0f))
     {
         return 0.0f;
+    }
+    if (x > 88.0f)
+    {
+        return __int_as_float(0x7F800000);
     }
     const float fx = floorf(x * 1.44269504088896341f + 0.5f);
     float r = x - fx * 0.693359375f;

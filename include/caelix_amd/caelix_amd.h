@@ -181,9 +181,11 @@ CX_API int cx_hash_get_bricks(const HashKeySlot* slots, uint32_t slot_count, con
  * payload; record index = the brick's index in `keys`), found through a bulk-built hash table, and
  * bitmaps of resident bricks / 8^3-brick super cells over a grid of grid_dim^3 bricks from
  * grid_min. One entity at the identity frame; groups of (1 << group_shift)^3 bricks. Materials: 2
- * float4 per 16-bit block value, (albedo rgb, metallic) and (emission rgb, smoothness) (the
- * reference's FP16 hot entry, caelix_material.hlsl:90-113, as float); light_codes: 65536 bits, the
- * block values whose emission luminance reaches the light threshold.
+ * float4 per 16-bit block value, (albedo rgb, smoothness) and (emission rgb, metallic) (the first 16
+ * bytes of the reference's FP16 hot entry, caelix_material.hlsl:1-20, 84-100, as float), looked up by
+ * material code (a transparent value without its face bits); materials_ext optionally the rest,
+ * (ior, extinction, 0, 0); light_codes: 65536 bits, the block values whose emission luminance
+ * reaches the light threshold.
  * ------------------------------------------------------------------------------------------ */
 typedef struct CxScene {
     const int32_t* keys;      /* [bricks][3] */
@@ -219,6 +221,12 @@ typedef struct CxScene {
      * a ray where it leaves that box widened by one super cell (no resident brick lies outside; hits
      * and counts are unchanged). NULL: every ray is walked to the end of the grid. */
     int32_t* occupied_box;
+    /* [65536][4] (ior, extinction, 0, 0) per material code, or NULL. Set: cx_trace_frame shades by the
+     * reference's full rule (ShadeVoxelHit, caelix_shading.hlsl:267-307): the specular lobe of opaque
+     * materials (smoothness, metallic, ior), transparent blocks as interfaces between media (reflect or
+     * refract by Fresnel) whose extinction attenuates the segments inside them, no NEE at specular
+     * vertices. NULL: every block is opaque Lambertian (the benchmarked subset). */
+    const float* materials_ext;
 } CxScene;
 
 /* The record producer (shaders/scene/caelix_finalize.hlsl:112-361) over the bricks and their face
@@ -299,6 +307,12 @@ typedef struct CxTraceParams {
      * cx_trace_frame_adjoint. */
     float fog_color[3];
     float fog_density;
+    /* The delta chain of the camera ray (bounce_count_transparent, 0..15; caelix_path_trace.hlsl:
+     * 595-678): it crosses up to this many smooth transparent interfaces (smoothness >= 0.99) by
+     * reflection or refraction; the first other hit is the primary surface, everything it sends times
+     * the chain's transmission, depth the chain's length. Needs CxScene::materials_ext. */
+    uint32_t bounces_transparent;
+    uint32_t pad3[3];
 } CxTraceParams;
 
 /* The procedural sky on misses and the sun (caelix_shading.hlsl:364-403, 453-485;

@@ -129,6 +129,7 @@ class _Scene(ctypes.Structure):
         ("grid_min", ctypes.c_int32 * 3), ("grid_dim", _U32), ("group_shift", _U32), ("encoding", _U32),
         ("materials", _P), ("light_codes", _P), ("emission_scale", ctypes.c_float), ("pad", _U32),
         ("brick_emission_scale", _P), ("scene_count", _U32), ("pad2", _U32), ("occupied_box", _P),
+        ("materials_ext", _P),
     ]
 
 
@@ -152,7 +153,8 @@ class _Params(ctypes.Structure):
     _fields_ = [("bounces", _U32), ("frame_seed", _U32), ("spp", _U32), ("flags", _U32), ("sun_dir", _F3),
                 ("sun_solid_angle", ctypes.c_float), ("sun_radiance", _F3), ("sun_cos_radius", ctypes.c_float),
                 ("sky_zenith", _F3), ("pad0", ctypes.c_float), ("sky_horizon", _F3), ("pad1", ctypes.c_float),
-                ("sky_ground", _F3), ("pad2", ctypes.c_float), ("fog_color", _F3), ("fog_density", ctypes.c_float)]
+                ("sky_ground", _F3), ("pad2", ctypes.c_float), ("fog_color", _F3), ("fog_density", ctypes.c_float),
+                ("bounces_transparent", _U32), ("pad3", _U32 * 3)]
 
 
 NO_RUSSIAN_ROULETTE = 1
@@ -249,6 +251,7 @@ class OracleScene:
         s.light_codes = _ptr(a["light_codes"])
         s.emission_scale = 1.0
         s.brick_emission_scale = _ptr(a["brick_scale"]) if "brick_scale" in a else None
+        s.materials_ext = _ptr(a["materials_ext"]) if "materials_ext" in a else None
         if "headers" in a:
             v = self.view = _TreeView()
             v.headers = _ptr(a["headers"])
@@ -289,6 +292,8 @@ class OracleScene:
             "materials": np.ascontiguousarray(data.materials, np.float32),
             "light_codes": np.ascontiguousarray(data.light_codes, np.uint32),
         }
+        if getattr(data, "materials_ext", None) is not None:
+            a["materials_ext"] = np.ascontiguousarray(data.materials_ext, np.float32)
         self = cls(a, data.grid_min, dim, data.group_shift, mis, getattr(data, "encoding", 1))
         lib = _trace_lib()
         lib.oracle_scene_finalize(ctypes.byref(self.scene))
@@ -370,9 +375,10 @@ class OracleScene:
         return ga.reshape(256, 3), gs
 
     def trace_frame(self, camera: dict, bounces: int, seed: int, spp: int = 1, rows=None, flags: int = 0,
-                    sky: dict | None = None, fog=None):
+                    sky: dict | None = None, fog=None, bounces_transparent: int = 0):
         cam = camera_struct(camera)
         prm = _params(bounces, seed, spp, flags, sky, fog)
+        prm.bounces_transparent = int(bounces_transparent)
         r0, r1 = rows if rows is not None else (0, cam.height)
         count = (r1 - r0) * cam.width
         color = np.zeros(count * 4, np.float64)

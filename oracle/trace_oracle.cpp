@@ -1014,13 +1014,17 @@ float BrickScale(const CxScene& s, uint32_t record)
     return s.brick_emission_scale != nullptr ? s.brick_emission_scale[record] : 1.0f;
 }
 
-// exp(x) for x <= 0: the Cephes expf polynomial (S. Moshier, Cephes Math Library) with the
-// two-part ln 2 and 2^n from exponent bits, in the kernels' operation order; 0 below -87.
-float ExpNonPositive(float x)
+// exp(x): the Cephes expf polynomial (S. Moshier, Cephes Math Library) with the two-part ln 2 and
+// 2^n from exponent bits, in the kernels' operation order; 0 below -87, +inf above 88.
+float ExpCephes(float x)
 {
     if (!(x >= -87.0f))
     {
         return 0.0f;
+    }
+    if (x > 88.0f)
+    {
+        return INFINITY;
     }
     const float fx = std::floor(x * 1.44269504088896341f + 0.5f);
     float r = x - fx * 0.693359375f;
@@ -1040,7 +1044,7 @@ float ExpNonPositive(float x)
 // throughput before it.
 void FogSegment(V fog, float density, float t, V& throughput, V& radiance)
 {
-    const float transmission = ExpNonPositive((-density) * t);
+    const float transmission = ExpCephes((-density) * t);
     radiance = Add(radiance, Mul(MulV(throughput, fog), 1.0f - transmission));
     throughput = Mul(throughput, transmission);
 }
@@ -1051,6 +1055,8 @@ struct Sky
 {
     V fog;                     // the fog of the air (fog_color rgb, density; density 0: none)
     float fog_density = 0.0f;
+    bool glass = false;        // the full material model (materials_ext set, or a delta chain)
+    uint32_t delta_budget = 0; // bounce_count_transparent
     bool on = false;
     bool sun = false;  // the sun's NEE: `on`, kRenderFlagSunNee, a sun radiance above 0 (caelix_path_trace.hlsl:742)
     V sun_dir, sun_radiance, zenith, horizon, ground;
@@ -1062,6 +1068,7 @@ Sky SkyOf(const CxTraceParams& p)
     k.sun_dir = k.sun_radiance = k.zenith = k.horizon = k.ground = Mk(0, 0, 0);
     k.fog = Mk(p.fog_color[0], p.fog_color[1], p.fog_color[2]);
     k.fog_density = p.fog_density;
+    k.delta_budget = p.bounces_transparent;
     if ((p.flags & CX_TRACE_SKY) == 0u)
     {
         return k;
@@ -1141,6 +1148,161 @@ V SunCandidate(const Sky& k, V normal, uint32_t& rng, V& direction)
         return Mk(0, 0, 0);
     }
     return Mul(k.sun_radiance, k.sun_solid_angle * cos_l * (1.0f / kPi));
+}
+
+// ---- the full material model (shaders/render/caelix_shading.hlsl:116-307) ----------------------
+struct Material
+{
+    V albedo, emission;
+    float smoothness, metallic, ior, extinction;
+};
+// CaelixGetMaterial of a table index (caelix_material.hlsl:84-100); without the extended table ior
+// 1.5 and extinction 0.
+Material GetMaterial(const CxScene& s, uint32_t code)
+{
+    const float* m = s.materials + uint64_t(code) * 8;
+    Material r;
+    r.albedo = Mk(m[0], m[1], m[2]);
+    r.smoothness = m[3];
+    r.emission = Mk(m[4], m[5], m[6]);
+    r.metallic = m[7];
+    r.ior = 1.5f;
+    r.extinction = 0.0f;
+    if (s.materials_ext != nullptr)
+    {
+        r.ior = s.materials_ext[uint64_t(code) * 4];
+        r.extinction = s.materials_ext[uint64_t(code) * 4 + 1];
+    }
+    return r;
+}
+V Reflect(V i, V n)
+{
+    return Sub(i, Mul(n, 2.0f * Dot(n, i)));
+}
+V Refract(V i, V n, float eta)
+{
+    const float c = Dot(n, i);
+    const float k = 1.0f - eta * eta * (1.0f - c * c);
+    if (k < 0.0f)
+    {
+        return Mk(0, 0, 0);
+    }
+    return Sub(Mul(i, eta), Mul(n, eta * c + std::sqrt(k)));
+}
+float Schlick(float r0, float cos_x)
+{
+    const float x = 1.0f - cos_x;
+    const float xx = x * x;
+    return r0 + (1.0f - r0) * xx * xx * x;
+}
+// FresnelReflectAmountOpaque / FresnelReflectAmountTransparent (caelix_shading.hlsl:116-148).
+float FresnelOpaque(float n1, float n2, V incident, V normal)
+{
+    float r0 = (n1 - n2) / (n1 + n2);
+    r0 *= r0;
+    return Schlick(r0, -Dot(normal, incident));
+}
+float FresnelTransparent(float n1, float n2, V incident, V normal)
+{
+    float r0 = (n1 - n2) / (n1 + n2);
+    r0 *= r0;
+    float cos_x = -Dot(normal, incident);
+    if (n1 > n2)
+    {
+        const float n = n1 / n2;
+        const float sin_t2 = n * n * (1.0f - cos_x * cos_x);
+        if (sin_t2 >= 1.0f)
+        {
+            return 1.0f;
+        }
+        cos_x = std::sqrt(1.0f - sin_t2);
+    }
+    return Schlick(r0, cos_x);
+}
+struct Interface
+{
+    V normal, direction;
+    bool refracted;
+    uint32_t medium;
+};
+// CrossInterface (caelix_shading.hlsl:203-243), random mode.
+Interface CrossInterface(const CxScene& s, uint32_t source, uint32_t destination, float destination_ior, V dir, V normal,
+                         uint32_t& rng)
+{
+    Interface it;
+    it.normal = SafeNormalize(normal);
+    if (Dot(dir, it.normal) > 0.0f)
+    {
+        it.normal = Mk(-it.normal.x, -it.normal.y, -it.normal.z);
+    }
+    const float source_ior = source != 0u ? GetMaterial(s, source).ior : 1.0f;
+    const float target_ior = destination != 0u ? destination_ior : 1.0f;
+    const float eta = source_ior / target_ior;
+    const V reflection = Reflect(dir, it.normal);
+    const V refraction = Refract(dir, it.normal, eta);
+    const bool can_refract = Dot(refraction, refraction) > 0.000001f;
+    const float fresnel = can_refract ? FresnelTransparent(source_ior, target_ior, dir, it.normal) : 1.0f;
+    it.refracted = can_refract && RandomFloat01(rng) >= fresnel;
+    it.direction = SafeNormalize(it.refracted ? refraction : reflection);
+    it.medium = it.refracted ? destination : source;
+    return it;
+}
+V LerpV(V a, V b, float t)
+{
+    return Add(a, Mul(Sub(b, a), t));
+}
+struct ShadeFull
+{
+    V albedo, emission, origin, direction;
+    bool specular;
+};
+// ShadeVoxelHit (caelix_shading.hlsl:267-307); lerp(a, b, t) = a + t (b - a).
+ShadeFull ShadeVoxel(const CxScene& s, uint32_t block, V hp, V dir, V normal, uint32_t& rng, uint32_t& medium)
+{
+    g_demand[g_kind] += 32;
+    const uint32_t code = MaterialCode(block, s.encoding);
+    const Material m = GetMaterial(s, code);
+    ShadeFull sh;
+    sh.emission = Mul(m.emission, s.emission_scale);
+    if (!Opaque(block, s.encoding))
+    {
+        const Interface it = CrossInterface(s, medium, code, m.ior, dir, normal, rng);
+        sh.albedo = Mk(1, 1, 1);
+        sh.origin = Add(hp, Mul(it.normal, it.refracted ? -kPushOff : kPushOff));
+        sh.direction = it.direction;
+        sh.specular = true;
+        medium = it.medium;
+        return sh;
+    }
+    sh.albedo = m.albedo;
+    const float fresnel = FresnelOpaque(1.0f, m.ior, dir, normal);
+    const float chance = m.metallic + fresnel * m.smoothness * (1.0f - m.metallic);
+    const float do_specular = RandomFloat01(rng) < chance ? 1.0f : 0.0f;
+    V diffuse = SafeNormalize(Add(normal, RandomUnitVector(rng)));
+    diffuse = Dot(diffuse, diffuse) > 0.0f ? diffuse : normal;
+    const V specular = SafeNormalize(LerpV(diffuse, Reflect(dir, normal), m.smoothness));
+    sh.origin = Add(hp, Mul(normal, kPushOff));
+    sh.direction = LerpV(diffuse, specular, do_specular);
+    sh.specular = do_specular > 0.5f;
+    return sh;
+}
+// TraverseSegment (caelix_shading.hlsl:150-175): fog in the air, exp(-(1 - albedo) t extinction)
+// per channel inside a medium.
+void MediumSegment(const CxScene& s, const Sky& k, uint32_t medium, float t, V& throughput, V& radiance)
+{
+    if (medium == 0u)
+    {
+        if (k.fog_density > 0.0f)
+        {
+            FogSegment(k.fog, k.fog_density, t, throughput, radiance);
+        }
+        return;
+    }
+    const Material m = GetMaterial(s, medium);
+    const V transmission = Mk(ExpCephes((-(1.0f - m.albedo.x)) * t * m.extinction),
+                              ExpCephes((-(1.0f - m.albedo.y)) * t * m.extinction),
+                              ExpCephes((-(1.0f - m.albedo.z)) * t * m.extinction));
+    throughput = MulV(throughput, transmission);
 }
 
 // What the adjoint needs of one NEE: the radiance it adds without the light brick's emission scale,
@@ -1504,6 +1666,175 @@ void TracePixel(const CxScene& s, const CxLightTreeView& L, const CxCamera& cam,
     }
 }
 
+// TracePath in the full material model (caelix_path_trace.hlsl:595-678, 737-978): the delta chain of
+// the camera ray (RunDeltaChain: smooth transparent interfaces crossed up to the budget, every segment
+// through its medium), ShadeVoxelHit at the surface that ended it (albedo and emission times the
+// chain's transmission, plus the fog along it), then the bounce loop with media, specular lobes (no
+// NEE at a specular vertex; its segment sees the sun disk and counts a lamp in full) and the
+// procedural sky on misses. A chain that leaves the scene shows the sky in its last direction times
+// its transmission. t_hit = the chain's length, attrib = the last surface or interface it met.
+void TracePixelGlass(const CxScene& s, const CxLightTreeView& L, const CxCamera& cam, uint32_t bounces, uint32_t seed,
+                     uint32_t flags, const Sky& k, uint32_t px, uint32_t py, DV& radiance, V& albedo, V& emission,
+                     float& t_hit, uint32_t& attrib, Counters& c)
+{
+    const bool roulette = (flags & CX_TRACE_NO_RUSSIAN_ROULETTE) == 0u;
+    const float sx = float(px) + 0.5f;
+    const float sy = float(py) + 0.5f;
+    const float ndc_x = (sx / float(cam.width)) * 2.0f - 1.0f;
+    const float ndc_y = 1.0f - (sy / float(cam.height)) * 2.0f;
+    const V right = Mk(cam.right[0], cam.right[1], cam.right[2]);
+    const V up = Mk(cam.up[0], cam.up[1], cam.up[2]);
+    const V fwd = Mk(cam.forward[0], cam.forward[1], cam.forward[2]);
+    const V dir = SafeNormalize(Add(Add(Mul(right, ndc_x * cam.tan_half_fov_y * cam.aspect), Mul(up, ndc_y * cam.tan_half_fov_y)), fwd));
+    const V origin = Mk(cam.position[0], cam.position[1], cam.position[2]);
+    albedo = Mk(0, 0, 0);
+    emission = Mk(0, 0, 0);
+    t_hit = 0.0f;
+    attrib = 0u;
+    c.rays += 1;
+    g_kind = 0;
+    const WorldHit primary = Trace(s, origin, dir, kRayTMax, c);
+    g_kind = 3;
+    if (primary.attrib == 0u)
+    {
+        if (k.on)
+        {
+            emission = SkyRadiance(k, dir);
+        }
+        return;
+    }
+    uint32_t rng = InitRandom(px, py, seed);
+    uint32_t light_rng = InitRandom(px, py, seed ^ 0x7F4A7C15u);
+    uint32_t nee_rng = InitRandom(px, py, seed ^ 0x68E31DA4u);
+    const uint32_t mis_bit = L.mis != 0u ? 2u : 0u;
+    WorldHit surface = primary;
+    uint32_t geometry = primary.attrib;
+    V ro = origin, rd = dir;
+    V chain = Mk(1, 1, 1), inscatter = Mk(0, 0, 0);
+    float t_virtual = 0.0f;
+    uint32_t medium = 0u, interfaces = 0u;
+    bool surface_hit = true;
+    for (;;)
+    {
+        MediumSegment(s, k, medium, surface.t, chain, inscatter);
+        t_virtual += surface.t;
+        geometry = surface.attrib;
+        if (interfaces >= k.delta_budget)
+        {
+            break;
+        }
+        const uint32_t block = (surface.attrib >> 16) & 0xFFFFu;
+        if (Opaque(block, s.encoding))
+        {
+            break;
+        }
+        const uint32_t code = MaterialCode(block, s.encoding);
+        const Material m = GetMaterial(s, code);
+        if (m.smoothness < 0.99f)
+        {
+            break;
+        }
+        const V hp = Add(ro, Mul(rd, surface.t));
+        const Interface it = CrossInterface(s, medium, code, m.ior, rd, FaceDirection(LowBit(surface.attrib & 0x3Fu)), rng);
+        interfaces += 1u;
+        medium = it.medium;
+        ro = Add(hp, Mul(it.normal, it.refracted ? -kPushOff : kPushOff));
+        rd = it.direction;
+        c.rays += 1;
+        g_kind = 0;
+        surface = Trace(s, ro, rd, kRayTMax, c);
+        g_kind = 3;
+        if (surface.attrib == 0u)
+        {
+            surface_hit = false;
+            break;
+        }
+    }
+    t_hit = t_virtual;
+    attrib = geometry;
+    if (!surface_hit)
+    {
+        const V sky = k.on ? SkyRadiance(k, rd) : Mk(0, 0, 0);
+        emission = Add(MulV(chain, sky), inscatter);
+        return;
+    }
+    const V pn = FaceDirection(LowBit(surface.attrib & 0x3Fu));
+    const ShadeFull ps =
+        ShadeVoxel(s, (surface.attrib >> 16) & 0xFFFFu, Add(ro, Mul(rd, surface.t)), rd, pn, rng, medium);
+    albedo = MulV(chain, ps.albedo);
+    emission = Add(MulV(chain, Mul(ps.emission, BrickScale(s, surface.record))), inscatter);
+    V throughput = Mk(1, 1, 1);
+    V segment_normal = pn;
+    uint32_t segment_light = 0u;
+    bool sees_sun = true;
+    if (bounces >= 1u && !ps.specular)
+    {
+        NeeTerm term;
+        AddD(radiance, MulV(NextEvent(s, L, k, ps.origin, pn, light_rng, nee_rng, c, term), throughput));
+        segment_light = 1u | mis_bit;
+        sees_sun = !k.sun;
+    }
+    V bo = ps.origin;
+    V bd = ps.direction;
+    uint32_t bounce_index = 1;
+    while (bounce_index <= bounces)
+    {
+        c.rays += 1;
+        g_kind = 1;
+        const WorldHit hit = Trace(s, bo, bd, kRayTMax, c);
+        g_kind = 4;
+        if (hit.attrib == 0u)
+        {
+            if (k.on)
+            {
+                AddD(radiance, MulV(Add(ProceduralSky(k, bd), sees_sun ? SunDisk(k, bd) : Mk(0, 0, 0)), throughput));
+            }
+            break;
+        }
+        V segment_add = Mk(0, 0, 0);
+        MediumSegment(s, k, medium, hit.t, throughput, segment_add);
+        AddD(radiance, segment_add);
+        const V vn = FaceDirection(LowBit(hit.attrib & 0x3Fu));
+        const uint32_t block = (hit.attrib >> 16) & 0xFFFFu;
+        const ShadeFull sh = ShadeVoxel(s, block, Add(bo, Mul(bd, hit.t)), bd, vn, rng, medium);
+        const V scaled_emission = Mul(sh.emission, BrickScale(s, hit.record));
+        bounce_index += 1;
+        float ew = 1.0f;
+        if (segment_light != 0u && (sh.emission.x > 0.0f || sh.emission.y > 0.0f || sh.emission.z > 0.0f))
+        {
+            ew = LampHitWeight(s, L, segment_light, hit, bo, bd, segment_normal);
+        }
+        AddD(radiance, MulV(scaled_emission, Mul(throughput, ew)));
+        throughput = MulV(throughput, Mk(sh.albedo.x / 1.0f, sh.albedo.y / 1.0f, sh.albedo.z / 1.0f));
+        segment_light = 0u;
+        sees_sun = true;
+        if (!sh.specular && bounce_index <= bounces)
+        {
+            segment_light = 1u | mis_bit;
+            sees_sun = !k.sun;
+            NeeTerm term;
+            AddD(radiance, MulV(NextEvent(s, L, k, sh.origin, vn, light_rng, nee_rng, c, term), Mul(throughput, 1.0f)));
+        }
+        const float cp = std::fmax(throughput.x, std::fmax(throughput.y, throughput.z));
+        if (cp <= 0.0f)
+        {
+            break;
+        }
+        const float u = RandomFloat01(rng);
+        if (roulette)
+        {
+            if (cp < u)
+            {
+                break;
+            }
+            throughput = Mul(throughput, 1.0f / cp);
+        }
+        bo = sh.origin;
+        bd = sh.direction;
+        segment_normal = vn;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1850,7 +2181,8 @@ void oracle_trace_frame(const CxScene* s, const CxLightTreeView* L, const CxCame
 {
     uint64_t rays = 0, shadow = 0, cells = 0, probes = 0, steps = 0;
     uint64_t d0 = 0, d1 = 0, d2 = 0, d3 = 0, d4 = 0;
-    const Sky sky = SkyOf(*prm);
+    Sky sky = SkyOf(*prm);
+    sky.glass = s->materials_ext != nullptr || prm->bounces_transparent > 0u;
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : rays, shadow, cells, probes, steps, d0, d1, d2, d3, d4)
     for (int64_t py = row_begin; py < int64_t(row_end); ++py)
     {
@@ -1868,8 +2200,16 @@ void oracle_trace_frame(const CxScene* s, const CxLightTreeView* L, const CxCame
             {
                 DV r;
                 V albedo, emission;
-                TracePixel(*s, *L, *cam, prm->bounces, prm->frame_seed + sample, prm->flags, sky, px, uint32_t(py), r,
-                           albedo, emission, t, attrib, local);
+                if (sky.glass)
+                {
+                    TracePixelGlass(*s, *L, *cam, prm->bounces, prm->frame_seed + sample, prm->flags, sky, px,
+                                    uint32_t(py), r, albedo, emission, t, attrib, local);
+                }
+                else
+                {
+                    TracePixel(*s, *L, *cam, prm->bounces, prm->frame_seed + sample, prm->flags, sky, px, uint32_t(py),
+                               r, albedo, emission, t, attrib, local);
+                }
                 col.x += double(emission.x) + double(albedo.x) * r.x;
                 col.y += double(emission.y) + double(albedo.y) * r.y;
                 col.z += double(emission.z) + double(albedo.z) * r.z;

@@ -1,0 +1,109 @@
+"""Pins the oracle's full material model (ShadeVoxelHit's transparent and specular branches, the
+delta chain, media) without the GPU, on a glass slab (thickness 4, transparent id 0x42) lying on an
+emissive floor (emission E, no NEE) under a constant sky C, camera above it, no bounces:
+
+  a camera ray meets the slab's top face at incidence cos_i; it reflects with the Schlick probability
+  F = r0 + (1 - r0)(1 - cos_i)^5, r0 = ((1 - ior)/(1 + ior))^2, and then leaves into the sky: color C,
+  depth = the distance to the top face; or it refracts (Snell: sin_t = sin_i / ior), crosses the glass
+  over L = (4 - push) / cos_t and ends on the floor: color = exp(-(1 - tint) extinction L) E per
+  channel, depth = distance to the top + L. The share of reflections matches the mean F (5 sigma).
+  ior 1: no bending, F = (1 - cos_i)^5, almost never a reflection.
+  Rough glass (smoothness < 1, shaded stochastically) at one bounce gives exactly the colours of the
+  smooth delta chain at none: the same random number decides the same interface.
+
+The full model follows shaders/render/caelix_shading.hlsl:116-307 and caelix_path_trace.hlsl:595-978."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import trace_checks as tc
+from caelix_amd import scene
+
+LOAD = 0.7
+C = np.array([0.0, 0.0, 7.0])
+E = 2.0
+PUSH = 0.002
+
+
+def build(oracle, data):
+    return oracle.OracleScene.build(data, int(np.ceil(len(data.keys) / LOAD)))
+
+
+def constant_sky():
+    return {"sun_dir": (0.0, 1.0, 0.0), "sun_radiance": (0.0, 0.0, 0.0), "sun_solid_angle": 0.0, "sun_cos_radius": 1.0,
+            "sky_zenith": C, "sky_horizon": C, "sky_ground": C}
+
+
+def slab_expectation(data, ior, extinction, tint):
+    """Per pixel (float64): reflection probability, refracted colour, depths (reflected, refracted)."""
+    d = tc.primary_dirs(data.camera).astype(np.float64)
+    cos_i = -d[:, 1]
+    r0 = ((1.0 - ior) / (1.0 + ior)) ** 2
+    F = r0 + (1.0 - r0) * (1.0 - cos_i) ** 5
+    sin_t = np.sqrt(1.0 - cos_i ** 2) / ior
+    cos_t = np.sqrt(1.0 - sin_t ** 2)
+    L = (4.0 - PUSH) / cos_t
+    T = np.exp(-(1.0 - np.asarray(tint))[None, :] * extinction * L[:, None])
+    t_top = (float(data.camera["position"][1]) - 12.0) / cos_i
+    return F, T * E, t_top, t_top + L
+
+
+@pytest.mark.parametrize("ior", [1.5, 1.0])
+def test_glass_slab_closed_form(oracle, ior):
+    tint, extinction = (0.9, 0.7, 0.5), 0.3
+    data = scene.glass_slab(size_bricks=8, ior=ior, extinction=extinction, tint=tint)
+    ref = build(oracle, data)
+    F, refracted, depth_reflected, depth_refracted = slab_expectation(data, ior, extinction, tint)
+    reflections = 0
+    frames = 8
+    for seed in range(frames):
+        color, radiance, attrib, counters = ref.trace_frame(data.camera, 0, 100 + seed, 1, sky=constant_sky(),
+                                                            bounces_transparent=2)
+        assert np.all(attrib != 0) and np.all(radiance == 0.0)
+        refl = color[:, 0] == 0.0
+        reflections += int(refl.sum())
+        assert np.all(color[refl, :3] == C[None, :])
+        assert np.allclose(color[refl, 3], depth_reflected[refl], rtol=1e-5)
+        assert np.allclose(color[~refl, :3], refracted[~refl], rtol=1e-4)
+        assert np.allclose(color[~refl, 3], depth_refracted[~refl], rtol=1e-5)
+        blocks = (attrib >> 16) & 0xFFFF
+        assert np.all(blocks[refl] & 0x1FF == scene.GLASS) and np.all(blocks[~refl] == 0x8123)
+        # one camera ray and one chain ray per pixel
+        assert counters[0] == 2 * len(color)
+    mean, var = F.sum() * frames, (F * (1 - F)).sum() * frames
+    assert abs(reflections - mean) < 5 * np.sqrt(var) + 1, (reflections, mean, np.sqrt(var))
+    if ior == 1.5:
+        assert reflections > 500
+
+
+def test_rough_glass_one_bounce_equals_delta_chain(oracle):
+    smooth = scene.glass_slab(size_bricks=8, smoothness=1.0)
+    rough = scene.glass_slab(size_bricks=8, smoothness=0.5)
+    a = build(oracle, smooth).trace_frame(smooth.camera, 0, 17, 1, sky=constant_sky(), bounces_transparent=2)
+    b = build(oracle, rough).trace_frame(rough.camera, 1, 17, 1, sky=constant_sky(), bounces_transparent=2)
+    assert np.array_equal(a[0][:, :3], b[0][:, :3])
+    # the rough slab is the primary surface: its depth is the distance to the top face
+    _, _, depth_top, _ = slab_expectation(smooth, 1.5, 0.3, (0.9, 0.7, 0.5))
+    assert np.allclose(b[0][:, 3], depth_top, rtol=1e-5)
+
+
+def test_delta_budget_zero_shades_the_glass(oracle):
+    """Budget 0: the slab's top face is the primary surface, shaded stochastically like rough glass."""
+    data = scene.glass_slab(size_bricks=8)
+    ref = build(oracle, data)
+    color, _, attrib, counters = ref.trace_frame(data.camera, 1, 17, 1, sky=constant_sky(), bounces_transparent=0)
+    assert np.all((attrib >> 16) & 0x1FF == scene.GLASS)
+    chain = ref.trace_frame(data.camera, 0, 17, 1, sky=constant_sky(), bounces_transparent=2)
+    assert np.array_equal(color[:, :3], chain[0][:, :3])
+
+
+def test_glass_terrain_renders_finite(oracle):
+    """The small terrain with media, rough glass and specular lobes: finite, and different from the
+    opaque-only render of the same scene."""
+    data = scene.glass_terrain()
+    ref = build(oracle, data)
+    color, radiance, attrib, counters = ref.trace_frame(data.camera, 3, 9, 1, sky=constant_sky(), bounces_transparent=4)
+    assert np.isfinite(color).all() and np.isfinite(radiance).all()
+    media = ((attrib >> 16) & 0x8000) == 0
+    assert (media & (attrib != 0)).sum() > 100  # the chain ends on (rough) glass for some pixels

@@ -183,3 +183,21 @@ def open_light_faces(data) -> np.ndarray:
         nb = padded[x + 1 + dx[0], y + 1 + dx[1], z + 1 + dx[2]]
         out[:, :, f] = lit & ~nb
     return out
+
+
+def primary_dirs(cam):
+    """The primary directions of a pinhole camera in float32, operation by operation as the kernels
+    and the oracle compute them (CaelixPrimaryDirection)."""
+    f32 = np.float32
+    w, h = cam["width"], cam["height"]
+    px, py = np.meshgrid(np.arange(w), np.arange(h))
+    px, py = px.ravel(), py.ravel()
+    ndc_x = (((px.astype(f32) + f32(0.5)) / f32(w)) * f32(2.0)) - f32(1.0)
+    ndc_y = f32(1.0) - ((py.astype(f32) + f32(0.5)) / f32(h)) * f32(2.0)
+    t, a = f32(cam["tan_half_fov_y"]), f32(cam["aspect"])
+    right, up, fwd = (np.asarray(cam[n], f32) for n in ("right", "up", "forward"))
+    sx = (ndc_x * t) * a
+    sy = ndc_y * t
+    v = (right[None, :] * sx[:, None] + up[None, :] * sy[:, None]) + fwd[None, :]
+    dp = np.maximum(f32(1.175494351e-38), (v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]) + v[:, 2] * v[:, 2])
+    return v * (f32(1.0) / np.sqrt(dp))[:, None]
