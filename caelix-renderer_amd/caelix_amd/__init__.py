@@ -107,6 +107,7 @@ class CxTraceParams(ctypes.Structure):
 TRACE_NO_RUSSIAN_ROULETTE = 1
 TRACE_SKY = 2
 TRACE_NO_SUN_NEE = 4
+TRACE_SHADOW_TRANSMISSION = 8
 
 
 def sky_params(sun_dir=(0.0, 1.0, 0.0), sun_radiance=(0.0, 0.0, 0.0), sun_radius_deg: float = 0.5,

@@ -202,13 +202,18 @@ def furnace_box(size_bricks: int = 8, wall: int = 8, emission: float = 1.0, albe
 
 
 def sun_floor(size_bricks: int = 16, albedo: float = 123 / 255, lamp: str = "below", lamp_emission: float = 20.0,
-              width: int = 64, height: int = 64) -> SceneData:
+              width: int = 64, height: int = 64, glass_above=None) -> SceneData:
     """A flat floor under an open sky: size_bricks^2 bricks in one layer whose voxels y in [4, 8) are one
     grey opaque material (top face at y = 8), plus one lamp: "below" a 2^3 emissive cube under the
     floor (voxels y 0..1 of the central floor brick; every face it opens lies below the floor top, so
     no point on the floor can receive its light), "above" a 4^3 emissive cube in the air over the
     floor's centre (voxels y 18..21). The camera looks (almost) straight down at the floor's centre
-    from y = 120 with a 30 degree field of view, so every primary ray hits the floor or the lamp."""
+    from y = 120 with a 30 degree field of view, so every primary ray hits the floor or the lamp.
+    glass_above = (tint, extinction): also a smooth glass slab (id 0x42, ior 1.5) over the whole floor,
+    voxels y 40..43, resident air bricks below and above it (so both of its faces are interfaces), black
+    opaque walls (albedo 0) on the world's border from the floor up to the slab (the gap between them
+    is closed: grazing rays that the slab's underside reflects end there, and no shadow ray leaves
+    under the slab), the vacuum entry for code 0 and the extended material table."""
     dim = size_bricks
     world = dim * 8
     bx, bz = np.meshgrid(np.arange(dim), np.arange(dim), indexing="ij")
@@ -238,10 +243,33 @@ def sun_floor(size_bricks: int = 16, albedo: float = 123 / 255, lamp: str = "bel
     table[lamp_block, 4:7] = np.float32(lamp_emission)
     light_codes = np.zeros(2048, np.uint32)
     light_codes[lamp_block >> 5] |= np.uint32(1) << np.uint32(lamp_block & 31)
+    ext = None
+    if glass_above is not None:
+        tint, extinction = glass_above
+        slab = np.where(ly < 4, 0x42, 0).astype(np.uint16)
+        wall = OPAQUE | 2
+        gx = bx.ravel()[:, None] * 8 + (v & 7)[None, :]
+        gz = bz.ravel()[:, None] * 8 + (v >> 6)[None, :]
+        border = (gx == 0) | (gx == world - 1) | (gz == 0) | (gz == world - 1)
+        for layer in (1, 2, 3, 4, 5, 6):
+            if layer == 5:
+                blocks = np.tile(slab, (dim * dim, 1))
+            elif layer == 6:
+                blocks = np.zeros((dim * dim, 512), np.uint16)
+            else:
+                blocks = np.where(border, wall, 0).astype(np.uint16)
+            keys = np.concatenate([keys, np.stack([bx.ravel(), np.full(dim * dim, layer), bz.ravel()], axis=1).astype(np.int32)])
+            payload = np.concatenate([payload, blocks])
+        vacuum(table)
+        table[0x42, 0:3] = np.asarray(tint, np.float32)
+        table[0x42, 3] = 1.0
+        ext = np.zeros((65536, 4), np.float32)
+        ext[:, 0] = 1.5
+        ext[0x42, 1] = np.float32(extinction)
     c = world / 2
     camera = look_at((c + 0.013, 120.0, c + 0.021), (c, 8.0, c), width, height, fov_y_deg=30.0)
     return SceneData(keys=keys, payload=payload, materials=table, light_codes=light_codes, grid_min=(0, 0, 0),
-                     grid_dim=dim, group_shift=5, camera=camera, emissive_bricks=emissive)
+                     grid_dim=dim, group_shift=5, camera=camera, emissive_bricks=emissive, materials_ext=ext)
 
 
 def glass_world(encoding: int = 1) -> SceneData:
@@ -271,6 +299,13 @@ def glass_world(encoding: int = 1) -> SceneData:
 GLASS = 0x42  # the transparent id of the glass scenes
 
 
+def vacuum(table: np.ndarray) -> None:
+    """Material code 0 (air) as the reference's vacuum entry: albedo 1, smoothness 1 (a glass-to-air
+    exit is a smooth interface the delta chain crosses), no emission."""
+    table[0, 0:3] = 1.0
+    table[0, 3] = 1.0
+
+
 def glass_slab(size_bricks: int = 6, ior: float = 1.5, extinction: float = 0.3, tint=(0.9, 0.7, 0.5),
                smoothness: float = 1.0, floor_emission=(2.0, 2.0, 2.0), width: int = 64, height: int = 64) -> SceneData:
     """A glass slab on an emissive floor under an open sky: size_bricks^2 floor bricks (voxels y 0..7
@@ -289,6 +324,7 @@ def glass_slab(size_bricks: int = 6, ior: float = 1.5, extinction: float = 0.3, 
     payload = np.concatenate([np.full((len(cols), 512), floor, np.uint16),
                               np.tile(np.where(ly < 4, GLASS, 0).astype(np.uint16), (len(cols), 1))])
     table = np.zeros((65536, 8), np.float32)
+    vacuum(table)
     table[floor, 0:3] = 0.5
     table[floor, 4:7] = np.asarray(floor_emission, np.float32)
     table[GLASS, 0:3] = np.asarray(tint, np.float32)
@@ -316,6 +352,7 @@ def glass_terrain(seed: int = 1) -> SceneData:
     to_glass = ((payload & OPAQUE) != 0) & (ids >= 1) & (ids <= 16)
     payload[to_glass] = ids[to_glass]
     table = np.asarray(data.materials).copy()
+    vacuum(table)
     ext = np.zeros((65536, 4), np.float32)
     ext[:, 0] = 1.5
     media = np.arange(1, 17)

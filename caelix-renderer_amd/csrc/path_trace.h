@@ -37,6 +37,7 @@ struct TraceParams
     V3 fog;             // the fog's radiance (fog_color.rgb)
     float fog_density;  // fog_color.w; 0: no fog
     uint32_t delta_budget;  // bounce_count_transparent: smooth interfaces the camera ray crosses
+    uint32_t transmissive;  // kRenderFlagShadowTransmission: the primary vertex's shadow ray passes media
 };
 
 // TraverseSegment (caelix_shading.hlsl:150-175) in the air: a segment of length t that ends on a
@@ -354,6 +355,62 @@ CX_DEV WorldHit Trace(const SceneView& s, V3 o, V3 d, float t_max, PathCounters&
     c.probes += h.probes;
     c.steps += h.steps;
     return h;
+}
+
+// TraceShadowTransmissive (caelix_scene_trace.hlsl:248-266, 401-477) without subsurface: up to 8 hops;
+// an interface voxel is entered (the origin pushed through along the hit normal, the remaining length
+// keeping the ray's end), the segments inside media keep exp(-(1 - albedo) t extinction) (the air: 1,
+// no fog), an opaque voxel or a transmittance below 1e-3 in every channel occludes, a remaining length
+// of 1e29 or more is unbounded (a miss inside a medium adds no segment).
+CX_DEV V3 ShadowGlassTransmittance(const SceneView& s, uint32_t medium, float t)
+{
+    if (medium == 0u)
+    {
+        return Make(1.0f, 1.0f, 1.0f);
+    }
+    const Material m = GetMaterial(s, medium);
+    return Make(ExpCephes((-(1.0f - m.albedo.x)) * t * m.extinction), ExpCephes((-(1.0f - m.albedo.y)) * t * m.extinction),
+                ExpCephes((-(1.0f - m.albedo.z)) * t * m.extinction));
+}
+CX_DEV bool ShadowOccluded(V3 t)
+{
+    return fmaxf(t.x, fmaxf(t.y, t.z)) < 1e-3f;
+}
+CX_DEV V3 ShadowTransmissive(const SceneView& s, V3 origin, V3 dir, float t_max, uint32_t medium, PathCounters& c,
+                             uint32_t scene)
+{
+    V3 transmittance = Make(1.0f, 1.0f, 1.0f);
+    uint32_t cur = medium;
+    float remaining = t_max;
+    const V3 none = Make(0.0f, 0.0f, 0.0f);
+    for (uint32_t hop = 0; hop < 8u; ++hop)
+    {
+        const WorldHit h = Trace(s, origin, dir, remaining, c, scene);
+        if (h.attrib == 0u)
+        {
+            if (cur != 0u && remaining < 1e29f)
+            {
+                transmittance = MulV(transmittance, ShadowGlassTransmittance(s, cur, remaining));
+            }
+            return ShadowOccluded(transmittance) ? none : transmittance;
+        }
+        transmittance = MulV(transmittance, ShadowGlassTransmittance(s, cur, h.t));
+        if (ShadowOccluded(transmittance))
+        {
+            return none;
+        }
+        remaining -= h.t;
+        const uint32_t block = (h.attrib >> 16) & 0xFFFFu;
+        if (BlockOpaque(block, s.encoding))
+        {
+            return none;
+        }
+        cur = BlockCode(block, s.encoding);
+        const V3 offset = Mul(FaceNormal(h.attrib), -kPushOff);
+        origin = Add(Add(origin, Mul(dir, h.t)), offset);
+        remaining -= Dot(offset, dir);
+    }
+    return none;
 }
 
 // NextEventAtVertex (:356-413): the lamp candidate (LampCandidate, :301-333) and, with the sun's NEE,

@@ -270,12 +270,13 @@ CX_DEV void EnqueueShadow(const Workspace& w, uint32_t path, bool active, V3 ori
     }
 }
 
-// state.z: segment light | bounce << 8 | segment face << 16 | medium << 19 | segment sees the sun << 28
-// (the last two only in the full material model).
+// state.z: segment light | bounce << 8 | segment face << 16 | medium << 19 | segment sees the sun << 28 |
+// the transmissive NEE's claim << 29 (the last three only in the full material model).
 CX_DEV uint32_t PackState(uint32_t segment_light, uint32_t bounce, uint32_t face, uint32_t medium = 0u,
-                          bool sees_sun = false)
+                          bool sees_sun = false, bool claim = false)
 {
-    return segment_light | (bounce << 8) | (face << 16) | (medium << 19) | (sees_sun ? (1u << 28) : 0u);
+    return segment_light | (bounce << 8) | (face << 16) | (medium << 19) | (sees_sun ? (1u << 28) : 0u) |
+           (claim ? (1u << 29) : 0u);
 }
 
 // ---- adjoint helpers ---------------------------------------------------------------------------
@@ -654,6 +655,7 @@ CX_DEV void PrimaryGlass(const SceneView& s, const TraceParams& prm, const Works
     w.emission[path] = make_float4(deterministic.x, deterministic.y, deterministic.z, __uint_as_float(geometry));
     uint32_t segment_light = 0u;
     bool sees_sun = true;
+    bool claim = false;
     if (prm.bounces >= 1u)
     {
         if (!sh.specular)
@@ -663,17 +665,32 @@ CX_DEV void PrimaryGlass(const SceneView& s, const TraceParams& prm, const Works
                              shadow_unscaled, shadow_light))
             {
                 c.shadow_rays += 1;
-                shadow = shadow_t > 0.0f;
-                shadow_origin = sh.bounce_origin;
-                shadow_add = visible;
+                if (prm.transmissive != 0u)
+                {
+                    // The shadow ray through the media, traced here (at most 8 hops): its radiance is
+                    // the path's first term.
+                    if (shadow_t > 0.0f)
+                    {
+                        const V3 tr = ShadowTransmissive(s, sh.bounce_origin, shadow_dir, shadow_t, medium, c, scene);
+                        const V3 add = MulV(visible, tr);
+                        w.radiance[path] = make_float4(add.x, add.y, add.z, 0.0f);
+                    }
+                }
+                else
+                {
+                    shadow = shadow_t > 0.0f;
+                    shadow_origin = sh.bounce_origin;
+                    shadow_add = visible;
+                }
             }
             segment_light = kSegmentLampNee | mis_bit;
             sees_sun = prm.sky.sun == 0u;
+            claim = prm.transmissive != 0u;
         }
         go_on = true;
     }
     w.throughput[path] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-    w.state[path] = make_uint4(rng, light_rng, PackState(segment_light, 1u, face, medium, sees_sun), 0u);
+    w.state[path] = make_uint4(rng, light_rng, PackState(segment_light, 1u, face, medium, sees_sun, claim), 0u);
     if (prm.sky.sun != 0u)
     {
         w.nee_rng[path] = nee_rng;
@@ -863,6 +880,7 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
             uint32_t bounce_index = (st.z >> 8) & 0xFFu;
             const uint32_t segment_face = (st.z >> 16) & 7u;
             uint32_t medium = (st.z >> 19) & 0x1FFu;
+            bool claim = ((st.z >> 29) & 1u) != 0u;
             uint32_t verts = st.w;
             const float4 o4 = w.ray_o[path];
             const float4 d4 = w.ray_d[path];
@@ -919,6 +937,20 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
             verts = AppendVertex(verts, block & 0xFFu);
             segment_light = 0u;
             bool sees_sun = true;
+            if (Glass && claim)
+            {
+                // The claim of a transmissive primary NEE (caelix_path_trace.hlsl:895-907): through a
+                // transparent vertex the next segment sees no sun disk and counts a samplable lamp as 0.
+                if (!BlockOpaque(block, s.encoding))
+                {
+                    sees_sun = prm.sky.sun == 0u;
+                    segment_light = kSegmentLampNee;
+                }
+                else
+                {
+                    claim = false;
+                }
+            }
             if (!sh.specular && bounce_index <= prm.bounces)
             {
                 segment_light = kSegmentLampNee | (L.mis != 0u ? kSegmentLampMis : 0u);
@@ -952,7 +984,7 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
             w.radiance[path] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
             w.throughput[path] = make_float4(throughput.x, throughput.y, throughput.z, roulette);
             w.state[path] = make_uint4(rng, light_rng,
-                                       Glass ? PackState(segment_light, bounce_index, face, medium, sees_sun)
+                                       Glass ? PackState(segment_light, bounce_index, face, medium, sees_sun, claim)
                                              : PackState(segment_light, bounce_index, face),
                                        verts);
             if (prm.sky.sun != 0u)
@@ -1228,7 +1260,8 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
 {
     const bool adjoint = g.d_color != nullptr;
     // The full material model: the scene's extended material table (or a delta chain) asks for it.
-    const bool glass = sv.materials_ext != nullptr || params->bounces_transparent > 0u;
+    const bool glass = sv.materials_ext != nullptr || params->bounces_transparent > 0u ||
+                       (params->flags & CX_TRACE_SHADOW_TRANSMISSION) != 0;
     const uint32_t blocks = DivUp(paths, kWaveThreads);
     const uint32_t rows = GradRows(paths);
     const uint32_t bounce_blocks = std::min(TunedBlocks(PersistentBlocks(PersistentTraceKernel<BounceSource<Views>>)), rows);
@@ -1245,7 +1278,7 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
         const TraceParams p{params->bounces, params->frame_seed + sample, mis,
                             (params->flags & CX_TRACE_NO_RUSSIAN_ROULETTE) != 0 ? 0u : 1u, SkyOf(params),
                             V3{params->fog_color[0], params->fog_color[1], params->fog_color[2]}, params->fog_density,
-                            params->bounces_transparent};
+                            params->bounces_transparent, (params->flags & CX_TRACE_SHADOW_TRANSMISSION) != 0 ? 1u : 0u};
         ResetKernel<<<1, 64, 0, s>>>(w);
         {
             StageScope scope(s, adjoint ? "adjoint_trace" : "trace_primary");
@@ -1323,7 +1356,7 @@ int CheckParams(const CxTraceParams* params, const CxTraceOutput* out, uint32_t 
     {
         return ArgError("%s: at most 4 bounces (3 albedo vertices per path)", what);
     }
-    if ((params->flags & ~(CX_TRACE_NO_RUSSIAN_ROULETTE | CX_TRACE_SKY | CX_TRACE_NO_SUN_NEE)) != 0)
+    if ((params->flags & ~(CX_TRACE_NO_RUSSIAN_ROULETTE | CX_TRACE_SKY | CX_TRACE_NO_SUN_NEE | CX_TRACE_SHADOW_TRANSMISSION)) != 0)
     {
         return ArgError("%s: unknown flags", what);
     }
@@ -1337,7 +1370,8 @@ int CheckParams(const CxTraceParams* params, const CxTraceOutput* out, uint32_t 
     {
         return ArgError("%s: bounces_transparent above 15", what);
     }
-    if (adjoint && ((params->flags & CX_TRACE_SKY) != 0 || params->fog_density > 0.0f || params->bounces_transparent > 0))
+    if (adjoint && ((params->flags & (CX_TRACE_SKY | CX_TRACE_SHADOW_TRANSMISSION)) != 0 || params->fog_density > 0.0f ||
+                    params->bounces_transparent > 0))
     {
         Log(3, "%s: the adjoint has no sky, sun, fog or delta chain", what);
         return CX_ERROR_UNSUPPORTED;

@@ -328,6 +328,14 @@ typedef struct CxTraceParams {
  * sun disk see its radiance instead. */
 #define CX_TRACE_NO_SUN_NEE 4u
 
+/* Transmissive shadows at the primary vertex (kRenderFlagShadowTransmission; caelix_scene_trace.hlsl:
+ * 401-477, caelix_path_trace.hlsl:795-930): its shadow ray starts in the vertex's medium and goes
+ * straight through transparent voxels (at most 8 hops), attenuated by each medium's extinction (no
+ * fog), stopped by an opaque voxel or a transmittance below 1e-3; that NEE claims the light it saw:
+ * after a transparent vertex, up to the first opaque one, segments see no sun disk (with sun NEE) and
+ * count a lamp the tree can sample beyond the near field as 0. Implies the full material model. */
+#define CX_TRACE_SHADOW_TRANSMISSION 8u
+
 /* No Russian roulette: paths run all `bounces` (the roulette's random number is still drawn, so the
  * other decisions stay the same). The estimator is then a smooth function of the albedos and the
  * emission scales at fixed random numbers, which central finite differences can check. */

@@ -160,6 +160,7 @@ class _Params(ctypes.Structure):
 NO_RUSSIAN_ROULETTE = 1
 SKY = 2
 NO_SUN_NEE = 4
+SHADOW_TRANSMISSION = 8
 
 
 def _params(bounces: int, seed: int, spp: int, flags: int, sky: dict | None, fog=None) -> _Params:

@@ -1057,6 +1057,7 @@ struct Sky
     float fog_density = 0.0f;
     bool glass = false;        // the full material model (materials_ext set, or a delta chain)
     uint32_t delta_budget = 0; // bounce_count_transparent
+    bool transmissive = false; // kRenderFlagShadowTransmission at the primary vertex
     bool on = false;
     bool sun = false;  // the sun's NEE: `on`, kRenderFlagSunNee, a sun radiance above 0 (caelix_path_trace.hlsl:742)
     V sun_dir, sun_radiance, zenith, horizon, ground;
@@ -1069,6 +1070,7 @@ Sky SkyOf(const CxTraceParams& p)
     k.fog = Mk(p.fog_color[0], p.fog_color[1], p.fog_color[2]);
     k.fog_density = p.fog_density;
     k.delta_budget = p.bounces_transparent;
+    k.transmissive = (p.flags & CX_TRACE_SHADOW_TRANSMISSION) != 0u;
     if ((p.flags & CX_TRACE_SKY) == 0u)
     {
         return k;
@@ -1305,6 +1307,56 @@ void MediumSegment(const CxScene& s, const Sky& k, uint32_t medium, float t, V& 
     throughput = MulV(throughput, transmission);
 }
 
+// TraceShadowTransmissive (caelix_scene_trace.hlsl:248-266, 401-477) without subsurface.
+V ShadowGlassTransmittance(const CxScene& s, uint32_t medium, float t)
+{
+    if (medium == 0u)
+    {
+        return Mk(1, 1, 1);
+    }
+    const Material m = GetMaterial(s, medium);
+    return Mk(ExpCephes((-(1.0f - m.albedo.x)) * t * m.extinction), ExpCephes((-(1.0f - m.albedo.y)) * t * m.extinction),
+              ExpCephes((-(1.0f - m.albedo.z)) * t * m.extinction));
+}
+bool ShadowOccluded(V t)
+{
+    return std::fmax(t.x, std::fmax(t.y, t.z)) < 1e-3f;
+}
+V ShadowTransmissive(const CxScene& s, V origin, V dir, float t_max, uint32_t medium, Counters& c)
+{
+    V transmittance = Mk(1, 1, 1);
+    uint32_t cur = medium;
+    float remaining = t_max;
+    for (uint32_t hop = 0; hop < 8u; ++hop)
+    {
+        const WorldHit h = Trace(s, origin, dir, remaining, c);
+        if (h.attrib == 0u)
+        {
+            if (cur != 0u && remaining < 1e29f)
+            {
+                transmittance = MulV(transmittance, ShadowGlassTransmittance(s, cur, remaining));
+            }
+            return ShadowOccluded(transmittance) ? Mk(0, 0, 0) : transmittance;
+        }
+        transmittance = MulV(transmittance, ShadowGlassTransmittance(s, cur, h.t));
+        if (ShadowOccluded(transmittance))
+        {
+            return Mk(0, 0, 0);
+        }
+        remaining -= h.t;
+        const uint32_t block = (h.attrib >> 16) & 0xFFFFu;
+        if (Opaque(block, s.encoding))
+        {
+            return Mk(0, 0, 0);
+        }
+        cur = MaterialCode(block, s.encoding);
+        const V offset = Mul(FaceDirection(LowBit(h.attrib & 0x3Fu)), -kPushOff);
+        origin = Add(Add(origin, Mul(dir, h.t)), offset);
+        remaining -= Dot(offset, dir);
+    }
+    return Mk(0, 0, 0);
+}
+
 // What the adjoint needs of one NEE: the radiance it adds without the light brick's emission scale,
 // that brick, and whether it was added (unoccluded).
 struct NeeTerm
@@ -1317,7 +1369,7 @@ struct NeeTerm
 // NextEventAtVertex (caelix_path_trace.hlsl:356-413): the lamp candidate and, with the sun's NEE,
 // the sun candidate and the choice between them (one-sample RIS), then one shadow ray.
 V NextEvent(const CxScene& s, const CxLightTreeView& L, const Sky& k, V x_r, V n_r, uint32_t& light_rng,
-            uint32_t& nee_rng, Counters& c, NeeTerm& term)
+            uint32_t& nee_rng, Counters& c, NeeTerm& term, bool transmissive = false, uint32_t medium = 0u)
 {
     term = NeeTerm{};
     term.unscaled = Mk(0, 0, 0);
@@ -1375,12 +1427,19 @@ V NextEvent(const CxScene& s, const CxLightTreeView& L, const Sky& k, V x_r, V n
         term.unscaled = Mk(0, 0, 0);
         const int kind = g_kind;
         g_kind = 2;
-        const WorldHit h = Trace(s, x_r, direction, kRayTMax, c);
-        g_kind = kind;
-        if (h.attrib == 0u)
+        if (transmissive)
         {
-            radiance = Mul(sun, w_sum / w_sun);
+            radiance = MulV(Mul(sun, w_sum / w_sun), ShadowTransmissive(s, x_r, direction, kRayTMax, medium, c));
         }
+        else
+        {
+            const WorldHit h = Trace(s, x_r, direction, kRayTMax, c);
+            if (h.attrib == 0u)
+            {
+                radiance = Mul(sun, w_sum / w_sun);
+            }
+        }
+        g_kind = kind;
     }
     else if (w_sum > 0.0f)
     {
@@ -1389,13 +1448,20 @@ V NextEvent(const CxScene& s, const CxLightTreeView& L, const Sky& k, V x_r, V n
         {
             const int kind = g_kind;
             g_kind = 2;
-            const WorldHit h = Trace(s, x_r, direction, t_max, c);
-            g_kind = kind;
-            if (h.attrib == 0u)
+            if (transmissive)
             {
-                radiance = Mul(contribution, w_sum / w_lamp);
-                term.visible = true;
+                radiance = MulV(Mul(contribution, w_sum / w_lamp), ShadowTransmissive(s, x_r, direction, t_max, medium, c));
             }
+            else
+            {
+                const WorldHit h = Trace(s, x_r, direction, t_max, c);
+                if (h.attrib == 0u)
+                {
+                    radiance = Mul(contribution, w_sum / w_lamp);
+                    term.visible = true;
+                }
+            }
+            g_kind = kind;
         }
     }
     return radiance;
@@ -1767,12 +1833,15 @@ void TracePixelGlass(const CxScene& s, const CxLightTreeView& L, const CxCamera&
     V segment_normal = pn;
     uint32_t segment_light = 0u;
     bool sees_sun = true;
+    bool claim = false;
     if (bounces >= 1u && !ps.specular)
     {
         NeeTerm term;
-        AddD(radiance, MulV(NextEvent(s, L, k, ps.origin, pn, light_rng, nee_rng, c, term), throughput));
+        AddD(radiance, MulV(NextEvent(s, L, k, ps.origin, pn, light_rng, nee_rng, c, term, k.transmissive, medium),
+                            throughput));
         segment_light = 1u | mis_bit;
         sees_sun = !k.sun;
+        claim = k.transmissive;
     }
     V bo = ps.origin;
     V bd = ps.direction;
@@ -1808,6 +1877,19 @@ void TracePixelGlass(const CxScene& s, const CxLightTreeView& L, const CxCamera&
         throughput = MulV(throughput, Mk(sh.albedo.x / 1.0f, sh.albedo.y / 1.0f, sh.albedo.z / 1.0f));
         segment_light = 0u;
         sees_sun = true;
+        if (claim)
+        {
+            // The claim of a transmissive primary NEE (caelix_path_trace.hlsl:895-907).
+            if (!Opaque(block, s.encoding))
+            {
+                sees_sun = !k.sun;
+                segment_light = 1u;
+            }
+            else
+            {
+                claim = false;
+            }
+        }
         if (!sh.specular && bounce_index <= bounces)
         {
             segment_light = 1u | mis_bit;
@@ -2182,7 +2264,8 @@ void oracle_trace_frame(const CxScene* s, const CxLightTreeView* L, const CxCame
     uint64_t rays = 0, shadow = 0, cells = 0, probes = 0, steps = 0;
     uint64_t d0 = 0, d1 = 0, d2 = 0, d3 = 0, d4 = 0;
     Sky sky = SkyOf(*prm);
-    sky.glass = s->materials_ext != nullptr || prm->bounces_transparent > 0u;
+    sky.glass = s->materials_ext != nullptr || prm->bounces_transparent > 0u ||
+                (prm->flags & CX_TRACE_SHADOW_TRANSMISSION) != 0u;
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : rays, shadow, cells, probes, steps, d0, d1, d2, d3, d4)
     for (int64_t py = row_begin; py < int64_t(row_end); ++py)
     {

@@ -2,8 +2,9 @@
 through the C ABI: the glass slab's closed form (Fresnel share, Snell path, Beer transmission, depth)
 on the GPU path and against the oracle; the small terrain with media, rough glass and specular lobes
 under sky, sun and fog against the oracle (hit attributes, depths and counters exact, color and
-radiance within 2e-5 relative) at 0..3 bounces and delta budgets 0 and 4; the adjoint refuses the
-full model."""
+radiance within 2e-5 relative) at 0..3 bounces and delta budgets 0 and 4, also with transmissive
+shadows; the floor under a floating slab with transmissive sun shadows against the oracle and its
+closed form; the adjoint refuses the full model."""
 from __future__ import annotations
 
 import numpy as np
@@ -89,3 +90,41 @@ def test_full_model_rejected_by_adjoint(cx, terrain):
         gpu.trace_frame_adjoint(data.camera, 2, 1, 1, d_color, ga, gs)
     with pytest.raises(RuntimeError):
         gpu.trace_frame(data.camera, 1, 1, 1, bounces_transparent=16)
+
+
+def test_transmissive_shadows_gpu(cx, oracle):
+    """The floor under the floating slab with the sun: transmissive shadows and the claim on the GPU
+    equal the oracle's, and the floor's mean radiance meets the closed form."""
+    from test_glass_oracle import SLAB_EXTINCTION, SLAB_TINT, sun_through_slab
+
+    data = scene.sun_floor(glass_above=(SLAB_TINT, SLAB_EXTINCTION))
+    gpu = cx.TraceScene(data, load_factor=LOAD)
+    ref = oracle.OracleScene.build(data, slot_count(data))
+    L, r_deg = 100.0, 10.0
+    sky = cx.sky_params(sun_dir=(0.0, 1.0, 0.0), sun_radiance=(L, L, L), sun_radius_deg=r_deg)
+    cam = dict(data.camera)
+    cam["width"] = cam["height"] = 96
+    px = []
+    for seed in range(4):
+        got = gpu.trace_frame(cam, 3, 5 + seed, 1, sky=sky, bounces_transparent=2, flags=cx.TRACE_SHADOW_TRANSMISSION)
+        want = ref.trace_frame(cam, 3, 5 + seed, 1, sky=sky, bounces_transparent=2, flags=oracle.SHADOW_TRANSMISSION)
+        compare(got, want)
+        on_floor = ((want[2] >> 16) & 0xFFFF) == 0x8001
+        px.append(got[1].cpu().numpy().reshape(-1, 4)[on_floor, :3].astype(np.float64))
+    px = np.concatenate(px)
+    expect = sun_through_slab(L, r_deg, SLAB_TINT, SLAB_EXTINCTION)
+    se = px.std(axis=0) / np.sqrt(len(px))
+    assert np.all(np.abs(px.mean(axis=0) - expect) < 5 * se + 1e-4 * expect)
+
+
+@pytest.mark.parametrize("bounces", [1, 3])
+def test_glass_terrain_transmissive_matches_oracle(cx, oracle, terrain, bounces):
+    data, gpu, ref = terrain
+    sky = cx.sky_params(sun_dir=(0.3, 0.9, 0.2), sun_radiance=(40.0, 36.0, 30.0), sun_radius_deg=3.0,
+                        zenith=(0.2, 0.3, 0.6), horizon=(0.6, 0.6, 0.7), ground=(0.1, 0.1, 0.1))
+    print(f"glass terrain transmissive bounces={bounces}", flush=True)
+    got = gpu.trace_frame(data.camera, bounces, 77, 1, sky=sky, bounces_transparent=4,
+                          flags=cx.TRACE_SHADOW_TRANSMISSION)
+    want = ref.trace_frame(data.camera, bounces, 77, 1, sky=sky, bounces_transparent=4,
+                           flags=oracle.SHADOW_TRANSMISSION)
+    compare(got, want)
