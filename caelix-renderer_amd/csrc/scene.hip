@@ -219,16 +219,30 @@ __global__ void __launch_bounds__(64) FinalizeKernel(SceneArgs s)
             const uint32_t sd = s.grid_dim >> 3;
             const uint64_t super_index = scene * (cells / 512) + (uint64_t(gz >> 3) * sd + uint32_t(gy >> 3)) * sd +
                                          uint32_t(gx >> 3);
-            atomicOr(&s.super_bits[super_index >> 5], 1u << (super_index & 31u));
+            // Every brick of a super cell, and every brick of the scene for the box, would hit the same
+            // words: the atomics are issued only when they change something. The words only gain bits
+            // and the box only widens during the kernel, and the launch starts from cleared caches, so a
+            // stale read can cost an extra atomic but never skip a needed one.
+            const uint32_t super_bit = 1u << (super_index & 31u);
+            if ((s.super_bits[super_index >> 5] & super_bit) == 0u)
+            {
+                atomicOr(&s.super_bits[super_index >> 5], super_bit);
+            }
             if (s.box != nullptr)
             {
                 int32_t* box = s.box + scene * 6;
-                atomicMin(&box[0], gx >> 3);
-                atomicMin(&box[1], gy >> 3);
-                atomicMin(&box[2], gz >> 3);
-                atomicMax(&box[3], gx >> 3);
-                atomicMax(&box[4], gy >> 3);
-                atomicMax(&box[5], gz >> 3);
+                const int32_t c[3] = {gx >> 3, gy >> 3, gz >> 3};
+                for (int k = 0; k < 3; ++k)
+                {
+                    if (c[k] < box[k])
+                    {
+                        atomicMin(&box[k], c[k]);
+                    }
+                    if (c[k] > box[3 + k])
+                    {
+                        atomicMax(&box[3 + k], c[k]);
+                    }
+                }
             }
         }
     }
