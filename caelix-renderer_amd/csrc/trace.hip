@@ -758,58 +758,58 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
             }
             else
             {
-            const Shade sh = ShadeHit(s, (h.y >> 16) & 0xFFFFu, Add(origin, Mul(dir, t)), normal, rng);
-            const V3 emission = Mul(sh.emission, BrickScale(s, h.z));
-            if (prm.fog_density > 0.0f)
-            {
-                // The camera segment in the air (the delta chain's TraverseSegment): its transmission
-                // multiplies what the surface sends, its fog emission is deterministic
-                // (caelix_path_trace.hlsl:1273-1283).
-                V3 chain = Make(1.0f, 1.0f, 1.0f);
-                V3 inscatter = Make(0.0f, 0.0f, 0.0f);
-                FogSegment(prm, t, chain, inscatter);
-                const V3 albedo = MulV(chain, sh.albedo);
-                const V3 deterministic = Add(MulV(chain, emission), inscatter);
-                w.albedo[path] = make_float4(albedo.x, albedo.y, albedo.z, t);
-                w.emission[path] = make_float4(deterministic.x, deterministic.y, deterministic.z, __uint_as_float(h.y));
-            }
-            else
-            {
-                w.albedo[path] = make_float4(sh.albedo.x, sh.albedo.y, sh.albedo.z, t);
-                w.emission[path] = make_float4(emission.x, emission.y, emission.z, __uint_as_float(h.y));
-            }
-            if (g.d_color != nullptr)
-            {
-                const float4 d = g.d_color[views.PixelBase(scene) + uint64_t(py) * cam.width + px];
-                const V3 dc = Mul(Make(d.x, d.y, d.z), g.inv_spp);
-                const V3 adj = MulV(dc, sh.albedo);
-                w.adjoint[path] = make_float4(adj.x, adj.y, adj.z, 0.0f);
-                ScaleGradAdd(g, h.z, dc, sh.emission);  // color = emission * scale + albedo * radiance
-            }
-            const V3 throughput = Make(1.0f, 1.0f, 1.0f);
-            uint32_t segment_light = 0u;
-            if (prm.bounces >= 1u)
-            {
-                V3 visible;
-                if (NeeCandidate(s, L, prm.sky, sh.bounce_origin, normal, light_rng, nee_rng, shadow_dir, shadow_t, visible,
-                                 shadow_unscaled, shadow_light))
+                const Shade sh = ShadeHit(s, (h.y >> 16) & 0xFFFFu, Add(origin, Mul(dir, t)), normal, rng);
+                const V3 emission = Mul(sh.emission, BrickScale(s, h.z));
+                if (prm.fog_density > 0.0f)
                 {
-                    c.shadow_rays += 1;
-                    shadow = shadow_t > 0.0f;
-                    shadow_origin = sh.bounce_origin;
-                    shadow_add = MulV(visible, throughput);
+                    // The camera segment in the air (the delta chain's TraverseSegment): its transmission
+                    // multiplies what the surface sends, its fog emission is deterministic
+                    // (caelix_path_trace.hlsl:1273-1283).
+                    V3 chain = Make(1.0f, 1.0f, 1.0f);
+                    V3 inscatter = Make(0.0f, 0.0f, 0.0f);
+                    FogSegment(prm, t, chain, inscatter);
+                    const V3 albedo = MulV(chain, sh.albedo);
+                    const V3 deterministic = Add(MulV(chain, emission), inscatter);
+                    w.albedo[path] = make_float4(albedo.x, albedo.y, albedo.z, t);
+                    w.emission[path] = make_float4(deterministic.x, deterministic.y, deterministic.z, __uint_as_float(h.y));
                 }
-                segment_light = kSegmentLampNee | mis_bit;
-                go_on = true;
-            }
-            w.throughput[path] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);  // w: the roulette's normalisation
-            w.state[path] = make_uint4(rng, light_rng, PackState(segment_light, 1u, face), 0u);
-            if (prm.sky.sun != 0u)
-            {
-                w.nee_rng[path] = nee_rng;
-            }
-            w.ray_o[path] = make_float4(sh.bounce_origin.x, sh.bounce_origin.y, sh.bounce_origin.z, 0.0f);
-            w.ray_d[path] = make_float4(sh.bounce_direction.x, sh.bounce_direction.y, sh.bounce_direction.z, 0.0f);
+                else
+                {
+                    w.albedo[path] = make_float4(sh.albedo.x, sh.albedo.y, sh.albedo.z, t);
+                    w.emission[path] = make_float4(emission.x, emission.y, emission.z, __uint_as_float(h.y));
+                }
+                if (g.d_color != nullptr)
+                {
+                    const float4 d = g.d_color[views.PixelBase(scene) + uint64_t(py) * cam.width + px];
+                    const V3 dc = Mul(Make(d.x, d.y, d.z), g.inv_spp);
+                    const V3 adj = MulV(dc, sh.albedo);
+                    w.adjoint[path] = make_float4(adj.x, adj.y, adj.z, 0.0f);
+                    ScaleGradAdd(g, h.z, dc, sh.emission);  // color = emission * scale + albedo * radiance
+                }
+                const V3 throughput = Make(1.0f, 1.0f, 1.0f);
+                uint32_t segment_light = 0u;
+                if (prm.bounces >= 1u)
+                {
+                    V3 visible;
+                    if (NeeCandidate(s, L, prm.sky, sh.bounce_origin, normal, light_rng, nee_rng, shadow_dir, shadow_t, visible,
+                                     shadow_unscaled, shadow_light))
+                    {
+                        c.shadow_rays += 1;
+                        shadow = shadow_t > 0.0f;
+                        shadow_origin = sh.bounce_origin;
+                        shadow_add = MulV(visible, throughput);
+                    }
+                    segment_light = kSegmentLampNee | mis_bit;
+                    go_on = true;
+                }
+                w.throughput[path] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);  // w: the roulette's normalisation
+                w.state[path] = make_uint4(rng, light_rng, PackState(segment_light, 1u, face), 0u);
+                if (prm.sky.sun != 0u)
+                {
+                    w.nee_rng[path] = nee_rng;
+                }
+                w.ray_o[path] = make_float4(sh.bounce_origin.x, sh.bounce_origin.y, sh.bounce_origin.z, 0.0f);
+                w.ray_d[path] = make_float4(sh.bounce_direction.x, sh.bounce_direction.y, sh.bounce_direction.z, 0.0f);
             }
         }
     }
