@@ -1,26 +1,35 @@
 // ORACLE — test infrastructure only. Sequential CPU restatement of the trace side of the hot path:
-// the record producer's opaque subset (shaders/scene/caelix_finalize.hlsl:112-361), the light pass
+// the record producer (shaders/scene/caelix_finalize.hlsl:112-361), the light pass
 // (shaders/light/records/caelix_light_tree_build.hlsl:93-283, caelix_light_gather_emit.hlsl:36-86),
 // the brick DDA (shaders/render/caelix_brick_trace.hlsl:203-451), the world traversal through the hash
 // table, the light tree's sample and pdf (shaders/light/caelix_light_tree.hlsl:85-777), the MIS weight
 // (shaders/render/caelix_light_nee.hlsl:20-34) and the path tracer
-// (shaders/render/caelix_path_trace.hlsl:276-452, 737-978) for opaque Lambertian voxels.
+// (shaders/render/caelix_path_trace.hlsl:276-452, 595-978): opaque Lambertian voxels, and with the
+// extended material table the full material model (shaders/render/caelix_shading.hlsl:116-307: media,
+// glass, specular lobes, the delta chain, transmissive shadows of caelix_scene_trace.hlsl:401-477);
+// the procedural sky and sun, the fog of the air.
 //
 // Every float operation is one IEEE binary32 operation in source order (-ffp-contract=off), as in the
-// gfx950 kernels, so decisions (DDA steps, draws) match; the path tracer accumulates radiance in
-// float64. sin/cos: the Cephes single-precision polynomials (S. Moshier, Cephes Math Library).
+// gfx950 kernels, so decisions (DDA steps, draws, Fresnel choices, roulette) match; the path tracer
+// accumulates radiance in float64. sin/cos and exp: the Cephes single-precision polynomials
+// (S. Moshier, Cephes Math Library).
 //
 // Pinning (tests/):
 //   brick/world DDA   brute force: the first opaque voxel along the ray by exact rational stepping
 //                     on axis-aligned and grazing-free random rays; step counts bounded (<= 22);
 //                     bricks found by hash agree with a dict of the scene
-//   finalize words    numpy recomputation of occupancy bits, coarse byte and tight bounds
+//   finalize          numpy restatement of the reference finalize test's Reference() (face bits,
+//                     alive rule, occupancy, coarse byte, bounds) and its glass scenario's checks
 //   light records     numpy recomputation of open light faces, families, counts, box
 //   light sampling    sum of pdf over all light faces = 1 per receiver (exhaustive), draw pdf = face
 //                     pdf, histogram of draws vs pdf (chi-square), as the reference test
 //                     (tests/light_tree_test.cpp:888-1068)
-//   path tracer       one-bounce ceiling energy against a closed form is not reachable for these
-//                     scenes: parity unpinned beyond the pieces above (documented)
+//   path tracer       closed forms: the furnace box, the sun-lit floor (L sin^2 r), constant and graded
+//                     skies, the fogged camera segment and the isothermal fogged furnace, the glass
+//                     slab (Fresnel share, Snell path, Beer transmission, depth), sun through a
+//                     floating slab with transmissive shadows; RIS unbiasedness (sun + lamp); beyond
+//                     these, radiance at matched streams on terrain scenes is checked against this
+//                     oracle only
 #include <algorithm>
 #include <array>
 #include <cmath>
