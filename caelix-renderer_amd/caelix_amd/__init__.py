@@ -108,6 +108,7 @@ TRACE_NO_RUSSIAN_ROULETTE = 1
 TRACE_SKY = 2
 TRACE_NO_SUN_NEE = 4
 TRACE_SHADOW_TRANSMISSION = 8
+TRACE_DELTA_CHECKERBOARD = 16
 
 
 def sky_params(sun_dir=(0.0, 1.0, 0.0), sun_radiance=(0.0, 0.0, 0.0), sun_radius_deg: float = 0.5,

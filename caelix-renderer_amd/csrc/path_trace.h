@@ -38,6 +38,7 @@ struct TraceParams
     float fog_density;  // fog_color.w; 0: no fog
     uint32_t delta_budget;  // bounce_count_transparent: smooth interfaces the camera ray crosses
     uint32_t transmissive;  // kRenderFlagShadowTransmission: the primary vertex's shadow ray passes media
+    uint32_t checkerboard;  // kRenderFlagDeltaCheckerboard: the chain's first interface split by pixel parity
 };
 
 // TraverseSegment (caelix_shading.hlsl:150-175) in the air: a segment of length t that ends on a
@@ -256,11 +257,15 @@ struct Interface
     V3 direction;
     bool refracted;
     uint32_t medium;  // after the interface
+    float weight;     // 1, or the split's 2 F / 2 (1 - F)
+    bool split_applied;
 };
-// CrossInterface (caelix_shading.hlsl:203-243), random mode: refraction when it exists and a number
-// of `rng` >= the Fresnel reflectance (the number is drawn only when refraction exists).
+// CrossInterface (caelix_shading.hlsl:203-243). split < 0: refraction when it exists and a number of
+// `rng` >= the Fresnel reflectance (the number is drawn only when refraction exists); split 0 / 1: the
+// checkerboard's given branch (reflect / refract), no number, weight 2 F / 2 (1 - F); total internal
+// reflection is never split.
 CX_DEV Interface CrossInterface(const SceneView& s, uint32_t source, uint32_t destination, float destination_ior, V3 dir,
-                                V3 normal, uint32_t& rng)
+                                V3 normal, uint32_t& rng, int split = -1)
 {
     Interface it;
     it.normal = SafeNormalize(normal);
@@ -275,7 +280,22 @@ CX_DEV Interface CrossInterface(const SceneView& s, uint32_t source, uint32_t de
     const V3 refraction = Refract(dir, it.normal, eta);
     const bool can_refract = Dot(refraction, refraction) > 0.000001f;
     const float fresnel = can_refract ? FresnelTransparent(source_ior, target_ior, dir, it.normal) : 1.0f;
-    it.refracted = can_refract && RandomFloat01(rng) >= fresnel;
+    it.refracted = false;
+    it.weight = 1.0f;
+    it.split_applied = false;
+    if (can_refract)
+    {
+        if (split < 0)
+        {
+            it.refracted = RandomFloat01(rng) >= fresnel;
+        }
+        else
+        {
+            it.refracted = split == 1;
+            it.weight = 2.0f * (it.refracted ? 1.0f - fresnel : fresnel);
+            it.split_applied = true;
+        }
+    }
     it.direction = SafeNormalize(it.refracted ? refraction : reflection);
     it.medium = it.refracted ? destination : source;
     return it;

@@ -587,7 +587,7 @@ __global__ void __launch_bounds__(kWaveThreads) PersistentTraceKernel(SceneView 
 // sky in its last direction times its transmission; depth = the chain's length, attrib = the last
 // surface or interface it met.
 CX_DEV void PrimaryGlass(const SceneView& s, const TraceParams& prm, const Workspace& w, uint32_t path, uint32_t scene,
-                         const LightView& L, V3 origin, V3 dir, uint4 h, uint32_t& rng, uint32_t& light_rng,
+                         const LightView& L, V3 origin, V3 dir, uint4 h, bool even_pixel, uint32_t& rng, uint32_t& light_rng,
                          uint32_t& nee_rng, uint32_t mis_bit, PathCounters& c, bool& shadow, bool& go_on,
                          V3& shadow_origin, V3& shadow_dir, float& shadow_t, V3& shadow_add, uint32_t& shadow_light,
                          V3& shadow_unscaled)
@@ -623,7 +623,12 @@ CX_DEV void PrimaryGlass(const SceneView& s, const TraceParams& prm, const Works
             break;
         }
         const V3 hp = Add(ray_o, Mul(ray_d, surface.t));
-        const Interface it = CrossInterface(s, medium, code, m.ior, ray_d, FaceNormal(surface.attrib), rng);
+        const int split = prm.checkerboard != 0u && interfaces == 0u ? (even_pixel ? 1 : 0) : -1;
+        const Interface it = CrossInterface(s, medium, code, m.ior, ray_d, FaceNormal(surface.attrib), rng, split);
+        if (it.split_applied)
+        {
+            chain = Mul(chain, it.weight);
+        }
         interfaces += 1u;
         medium = it.medium;
         ray_o = Add(hp, Mul(it.normal, it.refracted ? -kPushOff : kPushOff));
@@ -752,9 +757,9 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
             const V3 normal = FaceDirection(face);
             if constexpr (Glass)
             {
-                PrimaryGlass(s, prm, w, path, views.SceneOf(path), L, origin, dir, h, rng, light_rng, nee_rng, mis_bit,
-                             c, shadow, go_on, shadow_origin, shadow_dir, shadow_t, shadow_add, shadow_light,
-                             shadow_unscaled);
+                PrimaryGlass(s, prm, w, path, views.SceneOf(path), L, origin, dir, h, ((px + py) & 1u) == 0u, rng,
+                             light_rng, nee_rng, mis_bit, c, shadow, go_on, shadow_origin, shadow_dir, shadow_t,
+                             shadow_add, shadow_light, shadow_unscaled);
             }
             else
             {
@@ -1278,7 +1283,8 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
         const TraceParams p{params->bounces, params->frame_seed + sample, mis,
                             (params->flags & CX_TRACE_NO_RUSSIAN_ROULETTE) != 0 ? 0u : 1u, SkyOf(params),
                             V3{params->fog_color[0], params->fog_color[1], params->fog_color[2]}, params->fog_density,
-                            params->bounces_transparent, (params->flags & CX_TRACE_SHADOW_TRANSMISSION) != 0 ? 1u : 0u};
+                            params->bounces_transparent, (params->flags & CX_TRACE_SHADOW_TRANSMISSION) != 0 ? 1u : 0u,
+                            (params->flags & CX_TRACE_DELTA_CHECKERBOARD) != 0 ? 1u : 0u};
         ResetKernel<<<1, 64, 0, s>>>(w);
         {
             StageScope scope(s, adjoint ? "adjoint_trace" : "trace_primary");
@@ -1356,7 +1362,8 @@ int CheckParams(const CxTraceParams* params, const CxTraceOutput* out, uint32_t 
     {
         return ArgError("%s: at most 4 bounces (3 albedo vertices per path)", what);
     }
-    if ((params->flags & ~(CX_TRACE_NO_RUSSIAN_ROULETTE | CX_TRACE_SKY | CX_TRACE_NO_SUN_NEE | CX_TRACE_SHADOW_TRANSMISSION)) != 0)
+    if ((params->flags & ~(CX_TRACE_NO_RUSSIAN_ROULETTE | CX_TRACE_SKY | CX_TRACE_NO_SUN_NEE | CX_TRACE_SHADOW_TRANSMISSION |
+                           CX_TRACE_DELTA_CHECKERBOARD)) != 0)
     {
         return ArgError("%s: unknown flags", what);
     }

@@ -336,6 +336,12 @@ typedef struct CxTraceParams {
  * count a lamp the tree can sample beyond the near field as 0. Implies the full material model. */
 #define CX_TRACE_SHADOW_TRANSMISSION 8u
 
+/* The delta checkerboard (kRenderFlagDeltaCheckerboard; caelix_path_trace.hlsl:608-660, the split mode
+ * of caelix_shading.hlsl:178-243): the delta chain's first interface takes the branch of the pixel's
+ * parity (px + py even refracts, odd reflects) with the weight 2 (1 - F) or 2 F and draws no random
+ * number; total internal reflection still reflects with weight 1. */
+#define CX_TRACE_DELTA_CHECKERBOARD 16u
+
 /* No Russian roulette: paths run all `bounces` (the roulette's random number is still drawn, so the
  * other decisions stay the same). The estimator is then a smooth function of the albedos and the
  * emission scales at fixed random numbers, which central finite differences can check. */

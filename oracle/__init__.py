@@ -161,6 +161,7 @@ NO_RUSSIAN_ROULETTE = 1
 SKY = 2
 NO_SUN_NEE = 4
 SHADOW_TRANSMISSION = 8
+DELTA_CHECKERBOARD = 16
 
 
 def _params(bounces: int, seed: int, spp: int, flags: int, sky: dict | None, fog=None) -> _Params:

@@ -1067,6 +1067,7 @@ struct Sky
     bool glass = false;        // the full material model (materials_ext set, or a delta chain)
     uint32_t delta_budget = 0; // bounce_count_transparent
     bool transmissive = false; // kRenderFlagShadowTransmission at the primary vertex
+    bool checkerboard = false; // kRenderFlagDeltaCheckerboard: the chain's first interface by pixel parity
     bool on = false;
     bool sun = false;  // the sun's NEE: `on`, kRenderFlagSunNee, a sun radiance above 0 (caelix_path_trace.hlsl:742)
     V sun_dir, sun_radiance, zenith, horizon, ground;
@@ -1080,6 +1081,7 @@ Sky SkyOf(const CxTraceParams& p)
     k.fog_density = p.fog_density;
     k.delta_budget = p.bounces_transparent;
     k.transmissive = (p.flags & CX_TRACE_SHADOW_TRANSMISSION) != 0u;
+    k.checkerboard = (p.flags & CX_TRACE_DELTA_CHECKERBOARD) != 0u;
     if ((p.flags & CX_TRACE_SKY) == 0u)
     {
         return k;
@@ -1235,10 +1237,13 @@ struct Interface
     V normal, direction;
     bool refracted;
     uint32_t medium;
+    float weight;
+    bool split_applied;
 };
-// CrossInterface (caelix_shading.hlsl:203-243), random mode.
+// CrossInterface (caelix_shading.hlsl:203-243): split < 0 random, 0 / 1 the checkerboard's reflect /
+// refract with weight 2 F / 2 (1 - F).
 Interface CrossInterface(const CxScene& s, uint32_t source, uint32_t destination, float destination_ior, V dir, V normal,
-                         uint32_t& rng)
+                         uint32_t& rng, int split = -1)
 {
     Interface it;
     it.normal = SafeNormalize(normal);
@@ -1253,7 +1258,22 @@ Interface CrossInterface(const CxScene& s, uint32_t source, uint32_t destination
     const V refraction = Refract(dir, it.normal, eta);
     const bool can_refract = Dot(refraction, refraction) > 0.000001f;
     const float fresnel = can_refract ? FresnelTransparent(source_ior, target_ior, dir, it.normal) : 1.0f;
-    it.refracted = can_refract && RandomFloat01(rng) >= fresnel;
+    it.refracted = false;
+    it.weight = 1.0f;
+    it.split_applied = false;
+    if (can_refract)
+    {
+        if (split < 0)
+        {
+            it.refracted = RandomFloat01(rng) >= fresnel;
+        }
+        else
+        {
+            it.refracted = split == 1;
+            it.weight = 2.0f * (it.refracted ? 1.0f - fresnel : fresnel);
+            it.split_applied = true;
+        }
+    }
     it.direction = SafeNormalize(it.refracted ? refraction : reflection);
     it.medium = it.refracted ? destination : source;
     return it;
@@ -1810,7 +1830,13 @@ void TracePixelGlass(const CxScene& s, const CxLightTreeView& L, const CxCamera&
             break;
         }
         const V hp = Add(ro, Mul(rd, surface.t));
-        const Interface it = CrossInterface(s, medium, code, m.ior, rd, FaceDirection(LowBit(surface.attrib & 0x3Fu)), rng);
+        const int split = k.checkerboard && interfaces == 0u ? (((px + py) & 1u) == 0u ? 1 : 0) : -1;
+        const Interface it =
+            CrossInterface(s, medium, code, m.ior, rd, FaceDirection(LowBit(surface.attrib & 0x3Fu)), rng, split);
+        if (it.split_applied)
+        {
+            chain = Mul(chain, it.weight);
+        }
         interfaces += 1u;
         medium = it.medium;
         ro = Add(hp, Mul(it.normal, it.refracted ? -kPushOff : kPushOff));

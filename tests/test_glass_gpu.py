@@ -128,3 +128,18 @@ def test_glass_terrain_transmissive_matches_oracle(cx, oracle, terrain, bounces)
     want = ref.trace_frame(data.camera, bounces, 77, 1, sky=sky, bounces_transparent=4,
                            flags=oracle.SHADOW_TRANSMISSION)
     compare(got, want)
+
+
+def test_delta_checkerboard_gpu(cx, oracle):
+    from test_glass_oracle import checkerboard_expectation
+
+    data = scene.glass_slab(size_bricks=8)
+    gpu = cx.TraceScene(data, load_factor=LOAD)
+    ref = oracle.OracleScene.build(data, slot_count(data))
+    got = gpu.trace_frame(data.camera, 0, 3, 1, sky=constant_sky(), bounces_transparent=2,
+                          flags=cx.TRACE_DELTA_CHECKERBOARD)
+    color = compare(got, ref.trace_frame(data.camera, 0, 3, 1, sky=constant_sky(), bounces_transparent=2,
+                                         flags=oracle.DELTA_CHECKERBOARD))
+    want_color, want_depth = checkerboard_expectation(data)
+    assert np.allclose(color[:, :3], want_color, rtol=1e-4, atol=1e-7)
+    assert np.allclose(color[:, 3], want_depth, rtol=1e-5)

@@ -160,3 +160,31 @@ def test_transmissive_sun_shadow_closed_form(oracle):
     # without the transmission the slab blocks the floor's shadow rays
     _, blocked, attrib, _ = ref.trace_frame(cam, 1, 5, 1, sky=sky, bounces_transparent=2)
     assert np.all(blocked[((attrib >> 16) & 0xFFFF) == 0x8001, :3] == 0.0)
+
+
+def checkerboard_expectation(data):
+    """Per pixel (float64) with the delta checkerboard on the slab: px + py even refracts with weight
+    2 (1 - F) (colour 2 (1 - F) T E, depth top + L), odd reflects with weight 2 F (colour 2 F C, depth
+    top)."""
+    F, refracted, depth_reflected, depth_refracted = slab_expectation(data, 1.5, 0.3, (0.9, 0.7, 0.5))
+    w, h = data.camera["width"], data.camera["height"]
+    px, py = np.meshgrid(np.arange(w), np.arange(h))
+    even = ((px + py).ravel() & 1) == 0
+    color = np.where(even[:, None], 2 * (1 - F)[:, None] * refracted, 2 * F[:, None] * C[None, :])
+    depth = np.where(even, depth_refracted, depth_reflected)
+    return color, depth
+
+
+def test_delta_checkerboard_closed_form(oracle):
+    data = scene.glass_slab(size_bricks=8)
+    ref = build(oracle, data)
+    want_color, want_depth = checkerboard_expectation(data)
+    for seed in (1, 2):
+        color, _, attrib, _ = ref.trace_frame(data.camera, 0, seed, 1, sky=constant_sky(), bounces_transparent=2,
+                                              flags=oracle.DELTA_CHECKERBOARD)
+        assert np.allclose(color[:, :3], want_color, rtol=1e-4, atol=1e-7)
+        assert np.allclose(color[:, 3], want_depth, rtol=1e-5)
+    # the checkerboard's mean over pixel pairs is the random mode's expectation per pixel
+    F, refracted, _, _ = slab_expectation(data, 1.5, 0.3, (0.9, 0.7, 0.5))
+    expectation = (1 - F)[:, None] * refracted + F[:, None] * C[None, :]
+    assert abs(want_color.mean() / expectation.mean() - 1) < 1e-2
