@@ -101,7 +101,8 @@ class CxTraceParams(ctypes.Structure):
                 ("sky_zenith", ctypes.c_float * 3), ("pad0", ctypes.c_float), ("sky_horizon", ctypes.c_float * 3),
                 ("pad1", ctypes.c_float), ("sky_ground", ctypes.c_float * 3), ("pad2", ctypes.c_float),
                 ("fog_color", ctypes.c_float * 3), ("fog_density", ctypes.c_float),
-                ("bounces_transparent", ctypes.c_uint32), ("pad3", ctypes.c_uint32 * 3)]
+                ("bounces_transparent", ctypes.c_uint32), ("bounce_nee_probability", ctypes.c_float),
+                ("pad3", ctypes.c_uint32 * 2)]
 
 
 TRACE_NO_RUSSIAN_ROULETTE = 1
@@ -613,7 +614,7 @@ class TraceScene:
 
     def trace_frame(self, camera: dict, bounces: int, seed: int, spp: int = 1, color=None, radiance=None,
                     attrib=None, counters=None, stream=None, flags: int = 0, sky: dict | None = None, fog=None,
-                    bounces_transparent: int = 0):
+                    bounces_transparent: int = 0, bounce_nee_probability: float = 1.0):
         torch = _torch()
         w, h = int(camera["width"]), int(camera["height"])
         color = color if color is not None else torch.empty(w * h * 4, dtype=torch.float32, device="cuda")
@@ -624,6 +625,7 @@ class TraceScene:
         params = CxTraceParams(bounces, seed, spp, flags)
         apply_sky(params, sky, fog)
         params.bounces_transparent = int(bounces_transparent)
+        params.bounce_nee_probability = float(bounce_nee_probability)
         out = self._output(camera, color, radiance, attrib, counters)
         check(self.lib.cx_trace_frame(ctypes.byref(self.scene), ctypes.byref(self.view), ctypes.byref(cam),
                                       ctypes.byref(params), ctypes.byref(out), stream_handle(stream)), "cx_trace_frame")

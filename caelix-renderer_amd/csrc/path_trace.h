@@ -39,7 +39,15 @@ struct TraceParams
     uint32_t delta_budget;  // bounce_count_transparent: smooth interfaces the camera ray crosses
     uint32_t transmissive;  // kRenderFlagShadowTransmission: the primary vertex's shadow ray passes media
     uint32_t checkerboard;  // kRenderFlagDeltaCheckerboard: the chain's first interface split by pixel parity
+    float nee_q;            // bounce_nee_probability (1: every bounce vertex runs its NEE)
 };
+
+// Whether the sun stream (nee_rng) is carried from vertex to vertex: the sun's NEE or the bounce NEE
+// budget draws from it.
+CX_DEV bool KeepsNeeStream(const TraceParams& p)
+{
+    return p.sky.sun != 0u || p.nee_q < 1.0f;
+}
 
 // TraverseSegment (caelix_shading.hlsl:150-175) in the air: a segment of length t that ends on a
 // surface keeps exp(-density t) of the throughput and adds the fog's emission with the throughput

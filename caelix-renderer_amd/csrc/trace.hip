@@ -696,7 +696,7 @@ CX_DEV void PrimaryGlass(const SceneView& s, const TraceParams& prm, const Works
     }
     w.throughput[path] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
     w.state[path] = make_uint4(rng, light_rng, PackState(segment_light, 1u, face, medium, sees_sun, claim), 0u);
-    if (prm.sky.sun != 0u)
+    if (KeepsNeeStream(prm))
     {
         w.nee_rng[path] = nee_rng;
     }
@@ -809,7 +809,7 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
                 }
                 w.throughput[path] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);  // w: the roulette's normalisation
                 w.state[path] = make_uint4(rng, light_rng, PackState(segment_light, 1u, face), 0u);
-                if (prm.sky.sun != 0u)
+                if (KeepsNeeStream(prm))
                 {
                     w.nee_rng[path] = nee_rng;
                 }
@@ -880,7 +880,7 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
             uint4 st = w.state[path];
             uint32_t rng = st.x;
             uint32_t light_rng = st.y;
-            uint32_t nee_rng = prm.sky.sun != 0u ? w.nee_rng[path] : 0u;
+            uint32_t nee_rng = KeepsNeeStream(prm) ? w.nee_rng[path] : 0u;
             uint32_t segment_light = st.z & 0xFFu;
             uint32_t bounce_index = (st.z >> 8) & 0xFFu;
             const uint32_t segment_face = (st.z >> 16) & 7u;
@@ -960,14 +960,16 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
             {
                 segment_light = kSegmentLampNee | (L.mis != 0u ? kSegmentLampMis : 0u);
                 sees_sun = prm.sky.sun == 0u;
+                // The bounce NEE budget: with probability q, scaled by 1 / q (q = 1 draws no number).
+                const bool run = prm.nee_q >= 1.0f || RandomFloat01(nee_rng) < prm.nee_q;
                 V3 visible;
-                if (NeeCandidate(s, L, prm.sky, sh.bounce_origin, vertex_normal, light_rng, nee_rng, shadow_dir, shadow_t,
-                                 visible, shadow_unscaled, shadow_light))
+                if (run && NeeCandidate(s, L, prm.sky, sh.bounce_origin, vertex_normal, light_rng, nee_rng, shadow_dir,
+                                        shadow_t, visible, shadow_unscaled, shadow_light))
                 {
                     c.shadow_rays += 1;
                     shadow = shadow_t > 0.0f;
                     shadow_origin = sh.bounce_origin;
-                    shadow_add = MulV(visible, Mul(throughput, 1.0f));
+                    shadow_add = MulV(visible, Mul(throughput, Div(1.0f, prm.nee_q)));
                     shadow_roulette = roulette;
                 }
             }
@@ -992,7 +994,7 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
                                        Glass ? PackState(segment_light, bounce_index, face, medium, sees_sun, claim)
                                              : PackState(segment_light, bounce_index, face),
                                        verts);
-            if (prm.sky.sun != 0u)
+            if (KeepsNeeStream(prm))
             {
                 w.nee_rng[path] = nee_rng;
             }
@@ -1284,7 +1286,8 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
                             (params->flags & CX_TRACE_NO_RUSSIAN_ROULETTE) != 0 ? 0u : 1u, SkyOf(params),
                             V3{params->fog_color[0], params->fog_color[1], params->fog_color[2]}, params->fog_density,
                             params->bounces_transparent, (params->flags & CX_TRACE_SHADOW_TRANSMISSION) != 0 ? 1u : 0u,
-                            (params->flags & CX_TRACE_DELTA_CHECKERBOARD) != 0 ? 1u : 0u};
+                            (params->flags & CX_TRACE_DELTA_CHECKERBOARD) != 0 ? 1u : 0u,
+                            params->bounce_nee_probability > 0.0f ? params->bounce_nee_probability : 1.0f};
         ResetKernel<<<1, 64, 0, s>>>(w);
         {
             StageScope scope(s, adjoint ? "adjoint_trace" : "trace_primary");
@@ -1376,6 +1379,15 @@ int CheckParams(const CxTraceParams* params, const CxTraceOutput* out, uint32_t 
     if (params->bounces_transparent > 15)
     {
         return ArgError("%s: bounces_transparent above 15", what);
+    }
+    if (!(params->bounce_nee_probability >= 0.0f) || !(params->bounce_nee_probability <= 1.0f))
+    {
+        return ArgError("%s: bounce_nee_probability outside [0, 1]", what);
+    }
+    if (adjoint && params->bounce_nee_probability > 0.0f && params->bounce_nee_probability < 1.0f)
+    {
+        Log(3, "%s: the adjoint runs every bounce NEE (bounce_nee_probability below 1)", what);
+        return CX_ERROR_UNSUPPORTED;
     }
     if (adjoint && ((params->flags & (CX_TRACE_SKY | CX_TRACE_SHADOW_TRANSMISSION)) != 0 || params->fog_density > 0.0f ||
                     params->bounces_transparent > 0))

@@ -312,7 +312,12 @@ typedef struct CxTraceParams {
      * reflection or refraction; the first other hit is the primary surface, everything it sends times
      * the chain's transmission, depth the chain's length. Needs CxScene::materials_ext. */
     uint32_t bounces_transparent;
-    uint32_t pad3[3];
+    /* The bounce NEE budget (bounce_nee_probability, caelix_path_trace.hlsl:909-944): a bounce vertex
+     * runs its NEE with this probability (one number of the sun stream) and scales it by its inverse;
+     * when it does not, its segment still carries the state of a vertex that ran it. 0 or 1: always
+     * (no number drawn). In [0, 1]. Not supported by cx_trace_frame_adjoint below 1. */
+    float bounce_nee_probability;
+    uint32_t pad3[2];
 } CxTraceParams;
 
 /* The procedural sky on misses and the sun (caelix_shading.hlsl:364-403, 453-485;

@@ -154,7 +154,7 @@ class _Params(ctypes.Structure):
                 ("sun_solid_angle", ctypes.c_float), ("sun_radiance", _F3), ("sun_cos_radius", ctypes.c_float),
                 ("sky_zenith", _F3), ("pad0", ctypes.c_float), ("sky_horizon", _F3), ("pad1", ctypes.c_float),
                 ("sky_ground", _F3), ("pad2", ctypes.c_float), ("fog_color", _F3), ("fog_density", ctypes.c_float),
-                ("bounces_transparent", _U32), ("pad3", _U32 * 3)]
+                ("bounces_transparent", _U32), ("bounce_nee_probability", ctypes.c_float), ("pad3", _U32 * 2)]
 
 
 NO_RUSSIAN_ROULETTE = 1
@@ -377,10 +377,12 @@ class OracleScene:
         return ga.reshape(256, 3), gs
 
     def trace_frame(self, camera: dict, bounces: int, seed: int, spp: int = 1, rows=None, flags: int = 0,
-                    sky: dict | None = None, fog=None, bounces_transparent: int = 0):
+                    sky: dict | None = None, fog=None, bounces_transparent: int = 0,
+                    bounce_nee_probability: float = 1.0):
         cam = camera_struct(camera)
         prm = _params(bounces, seed, spp, flags, sky, fog)
         prm.bounces_transparent = int(bounces_transparent)
+        prm.bounce_nee_probability = float(bounce_nee_probability)
         r0, r1 = rows if rows is not None else (0, cam.height)
         count = (r1 - r0) * cam.width
         color = np.zeros(count * 4, np.float64)

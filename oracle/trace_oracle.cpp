@@ -1068,6 +1068,7 @@ struct Sky
     uint32_t delta_budget = 0; // bounce_count_transparent
     bool transmissive = false; // kRenderFlagShadowTransmission at the primary vertex
     bool checkerboard = false; // kRenderFlagDeltaCheckerboard: the chain's first interface by pixel parity
+    float nee_q = 1.0f;        // bounce_nee_probability (0 is taken as 1)
     bool on = false;
     bool sun = false;  // the sun's NEE: `on`, kRenderFlagSunNee, a sun radiance above 0 (caelix_path_trace.hlsl:742)
     V sun_dir, sun_radiance, zenith, horizon, ground;
@@ -1082,6 +1083,7 @@ Sky SkyOf(const CxTraceParams& p)
     k.delta_budget = p.bounces_transparent;
     k.transmissive = (p.flags & CX_TRACE_SHADOW_TRANSMISSION) != 0u;
     k.checkerboard = (p.flags & CX_TRACE_DELTA_CHECKERBOARD) != 0u;
+    k.nee_q = p.bounce_nee_probability > 0.0f ? p.bounce_nee_probability : 1.0f;
     if ((p.flags & CX_TRACE_SKY) == 0u)
     {
         return k;
@@ -1728,9 +1730,14 @@ void TracePixel(const CxScene& s, const CxLightTreeView& L, const CxCamera& cam,
         if (bounce_index <= bounces)
         {
             segment_light = 1u | mis_bit;
-            NeeTerm term;
-            AddD(radiance, MulV(NextEvent(s, L, k, sh.origin, vn, light_rng, nee_rng, c, term), Mul(throughput, 1.0f)));
-            nee_adjoint(term);
+            // The bounce NEE budget (caelix_path_trace.hlsl:909-944): with probability q, scaled by 1 / q.
+            if (k.nee_q >= 1.0f || RandomFloat01(nee_rng) < k.nee_q)
+            {
+                NeeTerm term;
+                AddD(radiance, MulV(NextEvent(s, L, k, sh.origin, vn, light_rng, nee_rng, c, term),
+                                    Mul(throughput, 1.0f / k.nee_q)));
+                nee_adjoint(term);
+            }
         }
         const float cp = std::fmax(throughput.x, std::fmax(throughput.y, throughput.z));
         if (cp <= 0.0f)
@@ -1929,8 +1936,12 @@ void TracePixelGlass(const CxScene& s, const CxLightTreeView& L, const CxCamera&
         {
             segment_light = 1u | mis_bit;
             sees_sun = !k.sun;
-            NeeTerm term;
-            AddD(radiance, MulV(NextEvent(s, L, k, sh.origin, vn, light_rng, nee_rng, c, term), Mul(throughput, 1.0f)));
+            if (k.nee_q >= 1.0f || RandomFloat01(nee_rng) < k.nee_q)
+            {
+                NeeTerm term;
+                AddD(radiance, MulV(NextEvent(s, L, k, sh.origin, vn, light_rng, nee_rng, c, term),
+                                    Mul(throughput, 1.0f / k.nee_q)));
+            }
         }
         const float cp = std::fmax(throughput.x, std::fmax(throughput.y, throughput.z));
         if (cp <= 0.0f)

@@ -248,3 +248,21 @@ def test_glass_world_finalize_and_rays(cx, oracle, encoding):
     assert ((want[:, 1] != 0) & glass_hits).sum() > 1000  # many rays end on a glass or air interface
     bad = np.nonzero((got != want).any(axis=1))[0]
     assert len(bad) == 0, (bad[:5], got[bad[:3]], want[bad[:3]])
+
+
+def test_bounce_nee_budget_gpu(cx, oracle):
+    """bounce_nee_probability 0.5 on the GPU: the oracle's frame (counters exact, radiance 2e-5) and the
+    furnace's closed form."""
+    data = scene.furnace_box(width=96, height=96)
+    gpu = cx.TraceScene(data, load_factor=LOAD)
+    ref = oracle.OracleScene.build(data, slot_count(data))
+    color, radiance, attrib, counters = gpu.trace_frame(data.camera, 3, 17, 4, bounce_nee_probability=0.5)
+    w_color, w_rad, w_attrib, w_counters = ref.trace_frame(data.camera, 3, 17, 4, bounce_nee_probability=0.5)
+    rad = radiance.cpu().numpy().reshape(-1, 4).astype(np.float64)
+    assert np.array_equal(attrib.cpu().numpy().view(np.uint32), w_attrib)
+    assert np.array_equal(counters.cpu().numpy().view(np.uint64)[:5], w_counters[:5])
+    assert np.all(np.abs(rad[:, :3] - w_rad[:, :3]) <= 2e-5 * (1.0 + np.abs(w_rad[:, :3])))
+    per_pixel = rad[:, 0] / 4
+    assert abs(per_pixel.mean() - 1.75) < 5 * per_pixel.std() / np.sqrt(len(per_pixel)) + 1e-4
+    with pytest.raises(RuntimeError):
+        gpu.trace_frame(data.camera, 3, 17, 1, bounce_nee_probability=1.5)

@@ -250,3 +250,22 @@ def test_path_tracer_furnace_closed_form(oracle, mis, bounces):
     assert abs(mean - expected) < 5 * stderr + 1e-4, (mean, expected, stderr)
     # primary hits see the wall's emission and albedo
     np.testing.assert_allclose(color[:, 0], spp * 1.0 + 0.5 * radiance[:, 0], rtol=1e-12, atol=1e-9)
+
+
+@pytest.mark.parametrize("q", [0.5, 0.25])
+def test_bounce_nee_budget_keeps_the_furnace(oracle, q):
+    """The bounce NEE budget (bounce_nee_probability q): a bounce vertex runs its NEE with probability q
+    and scales it by 1 / q, so the furnace's closed form still holds; fewer shadow rays are traced."""
+    data = scene.furnace_box(width=96, height=96)
+    o = oracle.OracleScene.build(data, slot_count(data))
+    spp, bounces = 4, 3
+    _, radiance, _, counters = o.trace_frame(data.camera, bounces, 17, spp, bounce_nee_probability=q)
+    _, _, _, full = o.trace_frame(data.camera, bounces, 17, spp)
+    per_pixel = radiance[:, 0] / spp
+    stderr = per_pixel.std() / np.sqrt(len(per_pixel))
+    expected = (1 - 0.5 ** bounces) / (1 - 0.5)
+    assert abs(per_pixel.mean() - expected) < 5 * stderr + 1e-4, (per_pixel.mean(), expected, stderr)
+    assert counters[1] < full[1]
+    # q = 1 (and 0, taken as 1) draws no number: the plain frame
+    assert np.array_equal(o.trace_frame(data.camera, bounces, 17, 1, bounce_nee_probability=1.0)[1],
+                          o.trace_frame(data.camera, bounces, 17, 1, bounce_nee_probability=0.0)[1])
