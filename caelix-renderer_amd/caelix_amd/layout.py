@@ -29,7 +29,7 @@ HEADER = np.dtype([("summary", SUMMARY), ("root", "<u4"), ("leaf_count", "<u4"),
 assert HEADER.itemsize == 64
 
 STATE = np.dtype([("building", "<u4"), ("leaves_found", "<u4"), ("leaves_dropped", "<u4"), ("builds", "<u4"),
-                  ("node_overflow", "<u4"), ("pad0", "<u4", (3,)), ("bounds", "<f4", (6,)), ("pad1", "<u4", (2,))])
+                  ("node_overflow", "<u4"), ("build_error", "<u4"), ("pad0", "<u4", (2,)), ("bounds", "<f4", (6,)), ("pad1", "<u4", (2,))])
 assert STATE.itemsize == 64
 
 HASH_SLOT = np.dtype([("key_lo", "<u4"), ("key_hi", "<u4"), ("payload", "<u4"), ("pad", "<u4")])

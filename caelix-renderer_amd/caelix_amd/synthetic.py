@@ -63,3 +63,90 @@ def reference_sort_inputs() -> dict[int, np.ndarray]:
         keys = np.where(first % 3 == 0, ((second % 64) * 0x00FFFFF) & 0xFFFFFFFF, second & 0x3FFFFFFF)
         out[n] = keys.astype(np.uint32)
     return out
+
+
+def _morton_cells(codes: np.ndarray) -> np.ndarray:
+    """The (x, y, z) cells of 30-bit Morton codes (bit 3b -> x bit b, 3b + 1 -> y, 3b + 2 -> z)."""
+    codes = codes.astype(np.uint64)
+    cells = np.zeros((len(codes), 3), np.int64)
+    for b in range(10):
+        for axis in range(3):
+            cells[:, axis] |= ((codes >> np.uint64(3 * b + axis)) & np.uint64(1)).astype(np.int64) << b
+    return cells
+
+
+def leaves_at_codes(codes: np.ndarray, seed: int) -> np.ndarray:
+    """Leaves whose box centres quantise to the given Morton codes, provided the codes hold cell 0 and
+    cell 1023 on every axis (then the centre bounds are 0.25 .. 1023.25 and cell X is centre X + 0.25:
+    X * 1024 / 1023 rounds down to X). Boxes are [X, X + 0.5]; face sums log-uniform."""
+    cells = _morton_cells(codes).astype(np.float32)
+    n = len(codes)
+    rng = np.random.default_rng(seed)
+    leaves = np.zeros(n, layout.LEAF)
+    leaves["summary"]["lo"] = cells
+    leaves["summary"]["hi"] = cells + np.float32(0.5)
+    power = np.exp(rng.uniform(np.log(0.01), np.log(100.0), size=n))
+    weights = rng.uniform(0.1, 1.0, size=(n, 6))
+    leaves["summary"]["face"] = (power[:, None] * weights / weights.sum(axis=1, keepdims=True)).astype(np.float32)
+    leaves["group"] = (np.arange(n) // 64).astype(np.uint32)
+    leaves["entry"] = (np.arange(n) % 64).astype(np.uint32)
+    leaves["parent"] = layout.LT_NONE
+    return leaves
+
+
+def degenerate_leaves(kind: str, n: int, seed: int) -> np.ndarray:
+    """Light-tree inputs at the edges of the key distribution (n leaves, in shuffled order):
+    equal      every box centre the same point: all codes 0, the tree is the positions' radix tree
+    caterpillar codes 0 and 2^k (k = 0..29) plus the far corner, most leaves on code 1: a 30-deep spine
+               of single-code groups whose deepest group adds log2(n) levels of equal-key splits
+    cluster    one Gaussian cluster of sigma 0.5 voxels: a few distinct centres, huge equal-key groups
+    """
+    rng = np.random.default_rng(seed)
+    if kind == "equal":
+        codes = np.zeros(n, np.uint32)
+        leaves = leaves_at_codes(codes, seed)
+    elif kind == "caterpillar":
+        spine = np.array([0] + [1 << k for k in range(30)] + [(1 << 30) - 1], np.uint64)
+        sizes = np.maximum(1, (n >> np.arange(2, 2 + len(spine))).astype(np.int64))
+        sizes[1] += n - sizes.sum()
+        codes = np.repeat(spine, sizes).astype(np.uint32)
+        leaves = leaves_at_codes(codes, seed)
+    elif kind == "cluster":
+        return light_leaves(n, seed, extent=8192.0, clusters=1, sigma=0.5, clustered=1.0)
+    else:
+        raise ValueError(kind)
+    return leaves[rng.permutation(n)]
+
+
+def pool_stress_leaves(seed: int) -> np.ndarray:
+    """16,385 leaves whose first 16,384 sorted positions (one upper-level block of 32 sub-blocks of 512)
+    hold the most nodes crossing sub-block boundaries that 30-bit codes allow: the block is cut into 33
+    parts by a 6-bit code prefix, part k (1..31) spanning positions [512k - 256, 512k + 256), so each
+    sub-block boundary lies in the middle of a part; inside a part, 24 suffix bits peel a group of 20
+    leaves off alternately the left and the right end, around a middle group of 32 leaves that
+    straddles the boundary. Every boundary is then crossed by a chain of 25 nested nodes (807 nodes
+    built in the block, 1,614 arrivals). The last leaf, at the far corner, fixes the centre bounds."""
+    codes = []
+    for part in range(33):
+        prefix = part << 24
+        if part in (0, 32):
+            codes.append(np.full(256, prefix, np.uint32))  # a flat part: one equal-key group
+            continue
+        path = 0
+        left_groups, right_groups = [], []
+        for j in range(23, -1, -1):
+            if (23 - j) % 2 == 0:  # left peel: the group has bit j clear, the middle path sets it
+                left_groups.append(prefix | path)
+                path |= 1 << j
+            else:  # right peel: the group sets bit j, the middle path keeps it clear
+                right_groups.append(prefix | path | (1 << j))
+        middle = prefix | path
+        part_codes = [np.full(20, c, np.uint32) for c in left_groups]
+        part_codes.append(np.full(32, middle, np.uint32))
+        part_codes += [np.full(20, c, np.uint32) for c in reversed(right_groups)]  # outermost last
+        codes.append(np.concatenate(part_codes))
+    codes.append(np.array([(1 << 30) - 1], np.uint32))
+    codes = np.concatenate(codes)
+    assert len(codes) == 16385 and np.all(np.diff(codes.astype(np.int64)) >= 0)
+    rng = np.random.default_rng(seed)
+    return leaves_at_codes(codes, seed)[rng.permutation(len(codes))]

@@ -60,9 +60,22 @@ enum : uint32_t
 constexpr uint32_t kTreeThreads = 512;  // level 1: one lane per leaf
 constexpr uint32_t kTreeTile = 512;
 constexpr uint32_t kTreeFanout = 32;
-constexpr uint32_t kOpenCap = 256;   // open elements per block (bounded by twice the tree depth)
 constexpr uint32_t kOpenWords = 18;  // one open element: range, boundary deltas, child words, summary
-constexpr uint32_t kMaxClimb = 128;  // climb steps per element and level (the tree depth is below 64)
+// The bounds every work list below is sized from. Along a root-to-leaf path the deltas of the nodes
+// strictly increase; a delta is clz(k0 ^ k1) in 2..31 for two distinct 30-bit codes, or 32 + clz(i ^ j)
+// in 34..63 for equal codes at sorted positions below 2^30. So no path holds more than kDeltaLevels
+// internal nodes, and:
+//  - the elements of a block whose parent lies outside it hang off the nodes that cross one of the
+//    block's two ends, at most one per such node: at most 2 * kDeltaLevels open elements;
+//  - the nodes an upper-level block builds cross one of the 31 boundaries between its sub-blocks; the
+//    nodes that cross one boundary lie on one path: at most 31 * kDeltaLevels of them; each is
+//    reached by its two children, and each open element that parks takes one more arrival.
+constexpr uint32_t kDeltaLevels = 60;
+constexpr uint32_t kOpenCap = 256;   // open elements per block: >= 2 * kDeltaLevels
+constexpr uint32_t kMaxClimb = 128;  // climb steps per element and level: > kDeltaLevels
+constexpr uint32_t kUpperBuiltBound = (kTreeFanout - 1) * kDeltaLevels;
+constexpr uint32_t kUpperArrivalBound = 2 * kUpperBuiltBound + 2 * kDeltaLevels;  // 3840
+static_assert(kOpenCap >= 2 * kDeltaLevels && kMaxClimb > kDeltaLevels, "work lists below the depth bound");
 
 __host__ __device__ inline uint64_t TreeTile(uint32_t level)
 {
@@ -90,6 +103,16 @@ __host__ __device__ inline uint64_t OpenBytes(uint32_t capacity, uint32_t level)
 {
     const uint64_t blocks = TreeBlocks(capacity, level);
     return ((blocks * kOpenCap * kOpenWords * 4 + blocks * 4) + 255) & ~uint64_t(255);
+}
+
+// An upper-level block keeps its first kPoolEntries element records in LDS and the rest, up to the
+// arrival bound, in its slice of a global spill area (shared by the levels, which run one after
+// another; sized for the level-2 grid, the largest).
+constexpr uint32_t kPoolEntries = 1536;
+constexpr uint32_t kSpillEntries = kUpperArrivalBound - kPoolEntries;
+__host__ __device__ inline uint64_t SpillBytes(uint32_t capacity)
+{
+    return TreeLevels(capacity) >= 2 ? uint64_t(TreeBlocks(capacity, 2)) * kSpillEntries * kOpenWords * 4 : 0u;
 }
 
 struct Scratch
@@ -161,15 +184,12 @@ struct Scratch
     {
         return open(level) + uint64_t(TreeBlocks(capacity, level)) * kOpenCap * kOpenWords;
     }
+    // The upper levels' spill area (after the open lists of every level).
+    __host__ __device__ uint32_t* spill() const { return open(TreeLevels(capacity) + 1); }
     static uint64_t Bytes(uint32_t capacity)
     {
         const Scratch s{nullptr, capacity};
-        uint8_t* p = s.open_base();
-        for (uint32_t l = 1; l <= TreeLevels(capacity); ++l)
-        {
-            p += OpenBytes(capacity, l);
-        }
-        return uint64_t(p - static_cast<uint8_t*>(nullptr));
+        return uint64_t(reinterpret_cast<uint8_t*>(s.spill()) - static_cast<uint8_t*>(nullptr)) + SpillBytes(capacity);
     }
 };
 
@@ -1125,10 +1145,13 @@ __global__ void __launch_bounds__(kTreeThreads) TreeLevel1Kernel(Set set)
 // hash table keyed by gap (the first arrival of a gap claims the entry; linear probing) whose value
 // is the parked element's far range end and pool index; the parked element's record is in the LDS
 // pool. Every arrival writes its record to a fresh pool entry before exchanging (so the sibling can
-// read it once the exchange publishes it); the second arrival builds the parent.
+// read it once the exchange publishes it); the second arrival builds the parent. The hash holds one
+// entry per gap an element arrives at: at most kUpperBuiltBound built nodes plus 2 * kDeltaLevels
+// parked open elements. Pool records past kPoolEntries go to the block's spill slice (global memory,
+// so their exchange fences cover global memory as well).
 constexpr uint32_t kUpperThreads = 1024;
 constexpr uint32_t kHashEntries = 2048;  // power of two
-constexpr uint32_t kPoolEntries = 1536;
+static_assert(kHashEntries >= kUpperBuiltBound + 2 * kDeltaLevels, "gap hash below the depth bound");
 constexpr uint32_t kHashEmpty = 0xFFFFFFFFu;
 constexpr uint64_t kArrivalNone = ~uint64_t(0);      // no arrival yet; else the first's pool slot
 constexpr uint64_t kArrivalResolved = uint64_t(1) << 62;  // | left child's slot << 16 | right child's slot
@@ -1142,6 +1165,23 @@ struct UpperLds
     uint32_t pool_count;
     uint32_t open_count;
 };
+
+CX_DEV void StorePool(UpperLds& l, uint32_t* spill, uint32_t slot, const Climb& e)
+{
+    if (slot < kPoolEntries)
+    {
+        StoreClimb(l.pool[slot], e);
+    }
+    else
+    {
+        StoreClimb(spill + uint64_t(slot - kPoolEntries) * kOpenWords, e);
+    }
+}
+
+CX_DEV Climb LoadPool(const UpperLds& l, const uint32_t* spill, uint32_t slot)
+{
+    return slot < kPoolEntries ? LoadClimb(l.pool[slot]) : LoadClimb(spill + uint64_t(slot - kPoolEntries) * kOpenWords);
+}
 
 CX_DEV uint32_t HashFindOrInsert(UpperLds& l, uint32_t gap)
 {
@@ -1204,6 +1244,7 @@ __global__ void __launch_bounds__(kUpperThreads) TreeUpperKernel(Set set, uint32
     }
     __syncthreads();
     uint32_t* open = b.scratch.open(level) + uint64_t(blockIdx.x) * kOpenCap * kOpenWords;
+    uint32_t* spill = b.scratch.spill() + uint64_t(blockIdx.x) * kSpillEntries * kOpenWords;
     if (n >= 2 && block_lo < uint64_t(n))
     {
         const uint32_t total = l.offsets[kTreeFanout];
@@ -1237,26 +1278,40 @@ __global__ void __launch_bounds__(kUpperThreads) TreeUpperKernel(Set set, uint32
                 }
                 const uint32_t entry = HashFindOrInsert(l, gap);
                 const uint32_t slot = atomicAdd(&l.pool_count, 1u);
-                if (entry == kHashEmpty || slot >= kPoolEntries)
+                if (entry == kHashEmpty || slot >= kUpperArrivalBound)
                 {
-                    AtomicAddRelaxed(&b.scratch.ctl()[kCtlTreeOverflow], 1u);
+                    AtomicAddRelaxed(&b.scratch.ctl()[kCtlTreeOverflow], 1u);  // never expected
                     break;
                 }
-                StoreClimb(l.pool[slot], e);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+                StorePool(l, spill, slot, e);
+                if (slot < kPoolEntries)
+                {
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+                }
+                else
+                {
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                }
                 const uint64_t other = __hip_atomic_exchange(&l.arrival[entry], uint64_t(slot), __ATOMIC_RELAXED,
                                                              __HIP_MEMORY_SCOPE_WORKGROUP);
                 if (other == kArrivalNone)
                 {
                     break;  // parked: the sibling finishes the parent
                 }
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
                 // Resolved: the pool slots of the left and the right child, for the write-out.
                 const uint32_t first = uint32_t(other);
+                if (first < kPoolEntries)
+                {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+                }
+                else
+                {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                }
                 const uint32_t left_slot = is_left_child ? slot : first;
                 const uint32_t right_slot = is_left_child ? first : slot;
                 l.arrival[entry] = kArrivalResolved | (uint64_t(left_slot) << 16) | right_slot;
-                const Climb sibling = LoadClimb(l.pool[first]);
+                const Climb sibling = LoadPool(l, spill, first);
                 e = is_left_child ? ParentOf(e, sibling) : ParentOf(sibling, e);
             }
         }
@@ -1278,11 +1333,11 @@ __global__ void __launch_bounds__(kUpperThreads) TreeUpperKernel(Set set, uint32
         }
         if ((v & kArrivalResolved) == 0)
         {
-            EmitOpen(b, open, &l.open_count, LoadClimb(l.pool[uint32_t(v)]));
+            EmitOpen(b, open, &l.open_count, LoadPool(l, spill, uint32_t(v)));
             continue;
         }
-        const Climb lc = LoadClimb(l.pool[uint32_t(v >> 16) & 0xFFFFu]);
-        const Climb rc = LoadClimb(l.pool[uint32_t(v) & 0xFFFFu]);
+        const Climb lc = LoadPool(l, spill, uint32_t(v >> 16) & 0xFFFFu);
+        const Climb rc = LoadPool(l, spill, uint32_t(v) & 0xFFFFu);
         const Climb parent = ParentOf(lc, rc);
         uint32_t left;
         uint32_t right;
@@ -1551,6 +1606,7 @@ __global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Set set)
     uint32_t root_index = kLtNone;
     uint32_t node_count = 0;
     uint32_t overflow = 0;
+    uint32_t error = 0;
     if (n == 1)
     {
         LightBvhLeaf* leaf = b.leaves;
@@ -1583,14 +1639,13 @@ __global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Set set)
     else if (n >= 2)
     {
         node_count = node_total;
-        // More 4-wide nodes than the capacity (the reference's overflow), or a tree level whose
-        // open list or LDS tables overflowed (never expected: both hold twice the tree depth per
-        // block or more): the tree is left empty with the overflow flag.
-        if (node_count > b.node_capacity || ctl[kCtlTreeOverflow] != 0)
-        {
-            overflow = 1;
-        }
-        else
+        // More 4-wide nodes than the capacity (the reference's overflow: the host grows the node
+        // buffer and builds again), or a tree level past one of its work-list bounds (never
+        // expected: they are sized from the depth bound): the tree is left empty, with the flag
+        // that says which.
+        overflow = node_count > b.node_capacity ? 1u : 0u;
+        error = ctl[kCtlTreeOverflow] != 0 ? 1u : 0u;
+        if (overflow == 0 && error == 0)
         {
             root = LoadSummary(b.scratch.aggregates());
             root_index = 0;
@@ -1598,10 +1653,11 @@ __global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Set set)
     }
     StoreSummary(b.header->summary.face, root);
     b.header->root = root_index;
-    b.header->leaf_count = overflow != 0 ? 0u : n;
+    b.header->leaf_count = overflow != 0 || error != 0 ? 0u : n;
     b.header->node_count = node_count;
     b.header->pad = 0;
     b.state->node_overflow = overflow;
+    b.state->build_error = error;
     b.state->builds = b.state->builds + 1;
 }
 

@@ -55,7 +55,7 @@ CX_API const char* cx_version_string(void);
  *   state                  LightTreeState; `building` gates the build as caelix_light_bvh_begin
  *                          sets it (0: every kernel returns at once), `leaves_found` as the
  *                          gather's scan wrote it
- * Outputs: header (LightTreeHeader), state (leaves_dropped, bounds, node_overflow, builds),
+ * Outputs: header (LightTreeHeader), state (leaves_dropped, bounds, node_overflow, build_error, builds),
  * nodes[node_capacity] (LightBvhNode), leaves[].parent, and the scratch arrays in the reference's
  * scratch layout (sorted keys/values in sort buffer 0, left/right/parent/sorted_leaf_parent,
  * depths in `counter`, scanned 4-wide indices in `wide`, binary aggregates).

@@ -96,7 +96,11 @@ typedef struct LightTreeState {
     uint32_t leaves_dropped; /* of those, above the leaf capacity */
     uint32_t builds;         /* builds so far */
     uint32_t node_overflow;  /* 1 when the last build needed more nodes than the capacity */
-    uint32_t pad0[3];
+    /* Not in the reference (its first pad word): nonzero when the last build exceeded a bound of
+     * this build's own work lists (never expected: each is sized from the tree's depth bound). The
+     * tree is then left empty; unlike node_overflow, a larger node capacity does not help. */
+    uint32_t build_error;
+    uint32_t pad0[2];
     float bounds[6]; /* lo[3], hi[3] of the leaf box centres */
     uint32_t pad1[2];
 } LightTreeState;

@@ -174,3 +174,64 @@ def range_sums(sorted_leaf_summaries: np.ndarray, left: np.ndarray, right: np.nd
     prefix = np.vstack([np.zeros((1, 6)), np.cumsum(faces, axis=0)])
     sums = prefix[last + 1] - prefix[first]
     return first, last, sums
+
+
+def leaf_ranges(left: np.ndarray, right: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """first[i], last[i]: the sorted-leaf range of binary node i, by pointer jumping down the leftmost
+    and rightmost child chains (at most one round per tree level)."""
+    def chase(child_of):
+        cur = child_of.astype(np.int64)
+        inner = (cur & LEAF_BIT) == 0
+        while inner.any():
+            cur[inner] = child_of[cur[inner]]
+            inner = (cur & LEAF_BIT) == 0
+        return cur & ~LEAF_BIT
+    return chase(left), chase(right)
+
+
+def block_demand(left: np.ndarray, right: np.ndarray, n: int, tile: int = 512, fanout: int = 32) -> list[dict]:
+    """What the nested-block tree build asks of its work lists, per level, from a finished binary tree
+    (Karras numbering; n >= 2 leaves): the largest per-block count of open elements (range inside the
+    block, parent range not), of gap-hash entries at levels >= 2 (nodes built there, i.e. inside the
+    block but inside none of its sub-blocks, plus open elements whose parent's split gap lies inside the
+    block and so park) and of pool arrivals (two per built node plus one per parked element)."""
+    first, last = leaf_ranges(left, right)
+    internal = np.arange(n - 1, dtype=np.int64)
+    # Every element: internal nodes 0..n-2, then leaves (sorted positions) as n-1+p.
+    e_first = np.concatenate([first, np.arange(n, dtype=np.int64)])
+    e_last = np.concatenate([last, np.arange(n, dtype=np.int64)])
+    parent = np.full(2 * n - 1, -1, np.int64)
+    for child_of in (left, right):
+        c = child_of.astype(np.int64)
+        is_leaf = (c & LEAF_BIT) != 0
+        parent[np.where(is_leaf, n - 1 + (c & ~LEAF_BIT), c)] = internal
+    has_parent = parent >= 0
+    p = np.where(has_parent, parent, 0)
+    p_first, p_last = e_first[p], e_last[p]
+    lc = left.astype(np.int64)[p]
+    split = np.where((lc & LEAF_BIT) != 0, lc & ~LEAF_BIT, last[np.minimum(lc & ~LEAF_BIT, n - 2)])
+    out = []
+    size = tile
+    level = 1
+    while True:
+        sub = size // fanout
+        inside = (e_first // size) == (e_last // size)
+        parent_inside = has_parent & ((p_first // size) == (p_last // size))
+        is_open = inside & has_parent & ~parent_inside
+        block = e_first // size
+        blocks = (n + size - 1) // size
+        opened = np.bincount(block[is_open], minlength=blocks)
+        row = {"level": level, "tile": size, "blocks": blocks, "open": int(opened.max())}
+        if level >= 2:
+            built = inside[: n - 1] & ((first // sub) != (last // sub))
+            nodes = np.bincount(block[: n - 1][built], minlength=blocks)
+            parks = is_open & ((split // size) == block) & (((split + 1) // size) == block)
+            parked = np.bincount(block[parks], minlength=blocks)
+            row["hash"] = int((nodes + parked).max())
+            row["arrivals"] = int((2 * nodes + parked).max())
+        out.append(row)
+        if size >= n:
+            break
+        size *= fanout
+        level += 1
+    return out

@@ -64,6 +64,7 @@ def compare(gpu, ref, capacity, n):
     r_state = ref["state"].view(layout.STATE)[0]
     for field in ("building", "leaves_found", "leaves_dropped", "builds", "node_overflow", "bounds"):
         assert np.array_equal(g_state[field], r_state[field]), field
+    assert int(g_state["build_error"]) == 0, "a work list of the GPU build passed its bound"
     node_count = int(r_header["node_count"]) if int(r_state["node_overflow"]) == 0 else 0
     g_nodes = gpu["nodes"][:node_count]
     r_nodes = ref["nodes"].view(layout.NODE)[:node_count]
@@ -270,3 +271,29 @@ def test_batch_build_matches_oracle_per_tree(cx, oracle):
             ref = oracle.light_bvh_build(leaves.view(np.uint8), capacity, layout.node_capacity_for(capacity),
                                          state.view(np.uint8))
             compare(b.read(), ref, capacity, n)
+
+
+def test_pool_stress_spills_and_matches_oracle(cx, oracle):
+    """One level-2 block whose 31 sub-block boundaries are each crossed by 25 nested nodes: 1,613 pool
+    arrivals against 1,536 LDS pool entries (the CPU demand test pins the count), so the upper-level
+    build spills records to global memory. The tree is the oracle's, not an empty overflow."""
+    leaves = synthetic.pool_stress_leaves(3)
+    n = len(leaves)
+    capacity = (n + 63) // 64 * 64
+    gpu, ref = gpu_and_oracle(cx, oracle, leaves, capacity)
+    compare(gpu, ref, capacity, n)
+    assert int(gpu["header"]["leaf_count"]) == n and int(gpu["state"]["node_overflow"]) == 0
+    h = gpu["header"]
+    lt_checks.check_bvh(h, gpu["state"], gpu["nodes"][: int(h["node_count"])], gpu["leaves"][:n])
+
+
+@pytest.mark.parametrize("kind", ["equal", "caterpillar", "cluster"])
+def test_degenerate_inputs_at_rebuild_size_match_oracle(cx, oracle, kind):
+    """4,194,304 leaves at the edges of the key distribution (all centres equal; a 30-deep spine over a
+    deep equal-key group, 51 levels; one cluster of sigma 0.5): bit-exact with the oracle, non-empty,
+    no overflow."""
+    n = 1 << 22
+    leaves = synthetic.degenerate_leaves(kind, n, 11)
+    gpu, ref = gpu_and_oracle(cx, oracle, leaves, capacity=n)
+    compare(gpu, ref, n, n)
+    assert int(gpu["header"]["leaf_count"]) == n and int(gpu["header"]["root"]) == 0

@@ -134,3 +134,55 @@ def test_oracle_dropped_leaves_and_node_overflow(oracle):
     assert int(overflow["header_rec"]["leaf_count"]) == 0
     assert int(overflow["header_rec"]["root"]) == layout.LT_NONE
     assert int(overflow["header_rec"]["node_count"]) > 8
+
+
+# The work-list bounds of the nested-block GPU build (caelix-renderer_amd/csrc/light_bvh.hip): open
+# elements per block, gap-hash entries and pool arrivals per upper-level block. They follow from at most
+# 60 internal nodes on a root-to-leaf path (the deltas 2..31 of distinct 30-bit codes and 34..63 of equal
+# codes strictly increase down a path).
+OPEN_CAP, HASH_ENTRIES, POOL_LDS, ARRIVAL_BOUND, DELTA_LEVELS = 256, 2048, 1536, 3840, 60
+
+
+def demand_of(out):
+    cap, n = out["capacity"], int(out["header_rec"]["leaf_count"])
+    left = layout.scratch_u32(out["scratch"], layout.SCRATCH_LEFT, cap)[: n - 1]
+    right = layout.scratch_u32(out["scratch"], layout.SCRATCH_RIGHT, cap)[: n - 1]
+    depth = layout.scratch_u32(out["scratch"], layout.SCRATCH_COUNTER, cap)[: n - 1]
+    return lt_checks.block_demand(left, right, n), int(depth.max())
+
+
+@pytest.mark.parametrize("kind", ["equal", "caterpillar", "cluster"])
+def test_degenerate_inputs_stay_inside_the_work_list_bounds(oracle, kind):
+    """Equal centres, a 30-deep spine over a deep equal-key group, and one tight cluster: the oracle's
+    tree (pinned by the checks above) asks the GPU build's work lists for less than their bounds."""
+    n = 1 << 20
+    out = run_oracle(oracle, synthetic.degenerate_leaves(kind, n, 11))
+    assert int(out["header_rec"]["leaf_count"]) == n
+    demand, depth = demand_of(out)
+    assert depth + 1 <= DELTA_LEVELS
+    if kind == "caterpillar":
+        assert depth >= 45  # the spine and the equal-key group below it
+    for row in demand:
+        assert row["open"] <= 2 * DELTA_LEVELS <= OPEN_CAP, row
+        if row["level"] >= 2:
+            assert row["hash"] <= HASH_ENTRIES and row["arrivals"] <= ARRIVAL_BOUND, row
+
+
+def test_pool_stress_input_needs_the_spill(oracle):
+    """The pool stress leaves cross all 31 sub-block boundaries of one level-2 block with 25 nested
+    nodes each: more pool arrivals than the block's LDS pool holds (the GPU build spills the rest), and
+    still inside the arrival bound. The construction is checked against the top-down radix tree too."""
+    leaves = synthetic.pool_stress_leaves(3)
+    out = run_oracle(oracle, leaves)
+    n = len(leaves)
+    assert int(out["header_rec"]["leaf_count"]) == n
+    demand, depth = demand_of(out)
+    level2 = demand[1]
+    assert level2["level"] == 2
+    assert POOL_LDS < level2["arrivals"] <= ARRIVAL_BOUND, level2
+    assert level2["hash"] == 807  # 31 boundaries x 25 nested nodes + the 32 nodes over the parts
+    cap = out["capacity"]
+    keys = layout.scratch_u32(out["scratch"], layout.SCRATCH_KEYS, cap)[:n]
+    left, right, _, _ = lt_checks.top_down_radix_tree(keys)
+    assert np.array_equal(left, layout.scratch_u32(out["scratch"], layout.SCRATCH_LEFT, cap)[: n - 1])
+    assert np.array_equal(right, layout.scratch_u32(out["scratch"], layout.SCRATCH_RIGHT, cap)[: n - 1])
