@@ -172,6 +172,7 @@ SIGNATURES = {
     "cx_light_bvh_scratch_init": (_INT, [_P, _U32, _P]),
     "cx_light_bvh_begin": (_INT, [_P, _P, _U32, _U32, _U32, _P]),
     "cx_light_bvh_build": (_INT, [ctypes.POINTER(CxLightBvhBuild), _P]),
+    "cx_light_bvh_build_static": (_INT, [ctypes.POINTER(CxLightBvhBuild), _P]),
     "cx_light_bvh_batch_bytes": (_U64, [_U32]),
     "cx_light_bvh_batch_prepare": (_INT, [ctypes.POINTER(CxLightBvhBuild), _U32, _P, _P]),
     "cx_light_bvh_build_batch": (_INT, [_P, _U32, _U32, _P]),

@@ -26,6 +26,7 @@
 #include "device_common.h"
 #include "radix_sort.h"
 
+#include <mutex>
 #include <vector>
 #include "runtime.h"
 
@@ -1826,22 +1827,22 @@ __global__ void __launch_bounds__(SortKeys64::kThreads, kSortWavesPerSimd) Build
 // reference's uint32 Morton keys with leaf values) or SortKeys64 (code << 32 | leaf, sorted as one
 // 64-bit key); both give the same order, hence the same tree.
 template <class Keys, class Set>
-void RecordBuild(const Set& set, uint32_t capacity, uint32_t count, hipStream_t s)
+void RecordBuild(const Set& set, uint32_t capacity, uint32_t count, hipStream_t s, bool timed = true)
 {
     const uint32_t tiles = SortTiles(capacity);
     {
-        StageScope scope(s, "lt_bounds");
+        StageScope scope(s, "lt_bounds", timed);
         BoundsKernel<Set, Keys><<<dim3(BoundBlocks(capacity), count), kBvhThreads, 0, s>>>(set);
     }
     {
-        StageScope scope(s, "lt_morton");
+        StageScope scope(s, "lt_morton", timed);
         MortonKernel<Set, Keys><<<dim3(tiles, count), kSortThreads, 0, s>>>(set);
     }
     if constexpr (Keys::kValues)
     {
         for (uint32_t pass = 0; pass < kSortPasses; ++pass)
         {
-            StageScope scope(s, "sort_pass");
+            StageScope scope(s, "sort_pass", timed);
             BuildSortPassKernel<<<dim3(tiles, count), kSortThreads, 0, s>>>(set, pass);
         }
     }
@@ -1849,7 +1850,7 @@ void RecordBuild(const Set& set, uint32_t capacity, uint32_t count, hipStream_t 
     {
         for (uint32_t pass = 0; pass < uint32_t(SortKeys64::kPasses); ++pass)
         {
-            StageScope scope(s, "sort64_pass");
+            StageScope scope(s, "sort64_pass", timed);
             BuildSort64PassKernel<<<dim3(SortTilesOf<SortKeys64>(capacity), count), SortKeys64::kThreads, 0, s>>>(
                 set, pass);
         }
@@ -1857,25 +1858,25 @@ void RecordBuild(const Set& set, uint32_t capacity, uint32_t count, hipStream_t 
     static const char* const kTreeStage[] = {"lt_tree_l1", "lt_tree_l2", "lt_tree_l3", "lt_tree_l4", "lt_tree_l5",
                                              "lt_tree_l6"};
     {
-        StageScope scope(s, kTreeStage[0]);
+        StageScope scope(s, kTreeStage[0], timed);
         TreeLevel1Kernel<<<dim3(TreeBlocks(capacity, 1), count), kTreeThreads, 0, s>>>(set);
     }
     for (uint32_t level = 2; level <= TreeLevels(capacity); ++level)
     {
-        StageScope scope(s, kTreeStage[std::min(level, 6u) - 1]);
+        StageScope scope(s, kTreeStage[std::min(level, 6u) - 1], timed);
         TreeUpperKernel<<<dim3(TreeBlocks(capacity, level), count), kUpperThreads, 0, s>>>(set, level);
     }
     const uint32_t scan_tiles = DivUp(capacity, kScanTile);
     {
-        StageScope scope(s, "lt_cover_scan");
+        StageScope scope(s, "lt_cover_scan", timed);
         CoverScanKernel<<<dim3(scan_tiles, count), kScanThreads, 0, s>>>(set);
     }
     {
-        StageScope scope(s, "lt_flags_scan");
+        StageScope scope(s, "lt_flags_scan", timed);
         FlagsScanKernel<<<dim3(scan_tiles, count), kScanThreads, 0, s>>>(set);
     }
     {
-        StageScope scope(s, "lt_wide");
+        StageScope scope(s, "lt_wide", timed);
         WideKernel<<<dim3(DivUp(capacity, kWideThreads), count), kWideThreads, 0, s>>>(set);
     }
 }
@@ -1900,6 +1901,43 @@ __global__ void ScratchInitKernel(uint32_t* ctl)
 }  // namespace cx
 
 namespace cx {
+// The build scratch of cx_light_bvh_build_static: one per GPU the library runs on, held until the
+// process ends, shared by every call on that GPU. Its host-side name is not a device address; the
+// address comes from hipGetSymbolAddress, per device.
+constexpr uint32_t kStaticLeafCapacity = 65536;
+constexpr uint64_t kStaticScratchBytes = 11ull << 20;
+static __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+
+constexpr int kStaticMaxDevices = 64;
+std::mutex g_static_mutex;
+void* g_static_scratch[kStaticMaxDevices];
+
+// The device address of the current device's static scratch (looked up on the first call per device).
+int StaticScratch(void** out, const char* what)
+{
+    int device = 0;
+    if (hipError_t e = hipGetDevice(&device); e != hipSuccess)
+    {
+        return HipResult(e, what);
+    }
+    if (device < 0 || device >= kStaticMaxDevices)
+    {
+        return ArgError("%s: device %d above the %d devices the static scratch serves", what, device, kStaticMaxDevices);
+    }
+    std::lock_guard<std::mutex> lock(g_static_mutex);
+    if (g_static_scratch[device] == nullptr)
+    {
+        void* address = nullptr;
+        if (hipError_t e = hipGetSymbolAddress(&address, HIP_SYMBOL(scratch)); e != hipSuccess)
+        {
+            return HipResult(e, what);
+        }
+        g_static_scratch[device] = address;
+    }
+    *out = g_static_scratch[device];
+    return CX_OK;
+}
+
 // Validates a build description and converts it (argument errors log and return, changing nothing).
 int CheckBuild(const CxLightBvhBuild* build, const char* what, Build& b)
 {
@@ -1978,6 +2016,39 @@ CX_API int cx_light_bvh_build_keys64(const CxLightBvhBuild* build, void* stream)
     const hipStream_t s = static_cast<hipStream_t>(stream);
     cx::RecordBuild<cx::SortKeys64>(cx::OneBuild{b}, build->leaf_capacity, 1, s);
     return cx::LaunchResult("cx_light_bvh_build_keys64");
+}
+
+CX_API int cx_light_bvh_build_static(const CxLightBvhBuild* build, void* stream)
+{
+    const char* what = "cx_light_bvh_build_static";
+    if (build == nullptr || build->scratch != nullptr)
+    {
+        return cx::ArgError("%s: no build, or a scratch buffer (this entry point uses its own; pass NULL)", what);
+    }
+    if (build->leaf_capacity > cx::kStaticLeafCapacity ||
+        cx::Scratch::Bytes(build->leaf_capacity) > cx::kStaticScratchBytes)
+    {
+        return cx::ArgError("%s: leaf capacity %u above the static scratch's %u leaves", what, build->leaf_capacity,
+                            cx::kStaticLeafCapacity);
+    }
+    void* static_scratch = nullptr;
+    if (int r = cx::StaticScratch(&static_scratch, what); r != CX_OK)
+    {
+        return r;
+    }
+    CxLightBvhBuild desc = *build;
+    desc.scratch = static_scratch;
+    cx::Build b;
+    if (int r = cx::CheckBuild(&desc, what, b); r != CX_OK)
+    {
+        return r;
+    }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    // The control words sit at an offset that depends on the capacity, over whatever the previous call
+    // left there: clear them with a kernel (no memset node if the caller captures the stream).
+    cx::ScratchInitKernel<<<1, 64, 0, s>>>(b.scratch.ctl());
+    cx::RecordBuild<cx::SortPairs32>(cx::OneBuild{b}, build->leaf_capacity, 1, s, false);
+    return cx::LaunchResult(what);
 }
 
 CX_API uint64_t cx_light_bvh_batch_bytes(uint32_t count)

@@ -27,12 +27,12 @@ bool ProfileEnabled();
 void ProfileBegin(hipStream_t stream, const char* stage);
 void ProfileEnd(hipStream_t stream);
 
-// RAII bracket of one stage.
+// RAII bracket of one stage (`timed` false: never timed, for callers that must not create events).
 struct StageScope
 {
     hipStream_t stream;
     bool on;
-    StageScope(hipStream_t s, const char* stage) : stream(s), on(ProfileEnabled())
+    StageScope(hipStream_t s, const char* stage, bool timed = true) : stream(s), on(timed && ProfileEnabled())
     {
         if (on)
         {

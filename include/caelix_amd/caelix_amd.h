@@ -84,6 +84,16 @@ CX_API int cx_light_bvh_begin(LightControl* control, LightTreeState* state, uint
 /* The build (light_tree_record.cpp:271-308). */
 CX_API int cx_light_bvh_build(const CxLightBvhBuild* build, void* stream);
 
+/* The build without a caller scratch buffer, for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No counterpart in the
+ * reference. `build->scratch` must be NULL: the build uses the library's static scratch of 11 MiB per
+ * GPU (held until the process ends), so leaf_capacity is at most 65,536; a larger one is rejected with
+ * CX_ERROR_INVALID_ARGUMENT. It allocates nothing and launches the kernels of cx_light_bvh_build (one
+ * more to clear the control words), with the same results. Every call on one GPU shares that scratch:
+ * calls that overlap on one GPU (other streams, other threads) race; the scratch keeps the last
+ * call's contents. Stage timing (cx_profile_enable) does not time it. */
+CX_API int cx_light_bvh_build_static(const CxLightBvhBuild* build, void* stream);
+
 /* The batched form of the build: `count` builds (each its own leaves, nodes, header, state,
  * scratch and capacity, as for cx_light_bvh_build) in one sequence of launches whose grid covers
  * every build (grid y = build), so many small trees (chunk scenes) cost one build's launches.
