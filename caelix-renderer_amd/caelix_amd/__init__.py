@@ -205,6 +205,7 @@ SIGNATURES = {
     "cx_trace_rays": (_INT, [ctypes.POINTER(CxScene), _P, _U32, _P, _P]),
     "cx_light_sample": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightTreeView), _P, _P, _U32, _P, _P, _P]),
     "cx_light_face_pdf": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightTreeView), _P, _U32, _P, _P]),
+    "cx_light_mis_weight": (_INT, [_P, _U32, _P, _P]),
     "cx_cxw_open": (_INT, [ctypes.c_char_p, _P, _P]),
     "cx_cxw_close": (None, [_P]),
     "cx_cxw_world": (_INT, [_P, _P]),
@@ -376,6 +377,17 @@ def radix_sort_keys64(keys, stream=None):
     check(lib.cx_radix_sort_keys64(keys.data_ptr(), keys1.data_ptr(), count.data_ptr(), None, capacity,
                                    state.data_ptr(), stream_handle(stream)), "cx_radix_sort_keys64")
     keys.copy_(keys1)
+
+
+def light_mis_weight(queries: np.ndarray, stream=None) -> np.ndarray:
+    """cx_light_mis_weight: queries [n, 9] float32 = (p_area, d, cos_l, n_r, w) -> [n, 2] float32 =
+    (w_nee, w_bsdf)."""
+    torch = _torch()
+    n = len(queries)
+    q = torch.from_numpy(np.ascontiguousarray(queries, np.float32)).cuda()
+    out = torch.empty(max(n, 1) * 2, dtype=torch.float32, device="cuda")
+    check(load().cx_light_mis_weight(q.data_ptr(), n, out.data_ptr(), stream_handle(stream)), "cx_light_mis_weight")
+    return out.cpu().numpy().reshape(-1, 2)[:n]
 
 
 def profile_enable(enabled: bool) -> None:

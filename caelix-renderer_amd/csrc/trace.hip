@@ -1166,6 +1166,21 @@ __global__ void __launch_bounds__(256) FacePdfKernel(SceneView s, LightView l, c
     pdf[i] = LightFacePdf(s, l, x, n, q[6], voxel, q[10], q[11], found);
 }
 
+// LightNeeMisWeight of both sides per query: q = {p_area, d, cos_l, n_r (3), w (3)} -> {w_nee, w_bsdf}.
+__global__ void __launch_bounds__(256) MisWeightKernel(const float* queries, uint32_t count, float* weights)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count)
+    {
+        return;
+    }
+    const float* q = queries + uint64_t(i) * 9;
+    const V3 n = Make(q[3], q[4], q[5]);
+    const V3 w = Make(q[6], q[7], q[8]);
+    weights[2 * uint64_t(i)] = MisWeight(q[0], n, w, q[1], q[2], true);
+    weights[2 * uint64_t(i) + 1] = MisWeight(q[0], n, w, q[1], q[2], false);
+}
+
 int CheckScene(const CxScene* s, const char* what)
 {
     if (s == nullptr || s->keys == nullptr || s->info == nullptr || s->occupancy == nullptr || s->payload == nullptr ||
@@ -1673,6 +1688,21 @@ CX_API int cx_light_face_pdf(const CxScene* scene, const CxLightTreeView* lights
     cx::FacePdfKernel<<<cx::DivUp(count, 256), 256, 0, s>>>(cx::SceneOf(*scene), cx::LightsOf(*lights), queries,
                                                             count, pdf);
     return cx::LaunchResult("cx_light_face_pdf");
+}
+
+CX_API int cx_light_mis_weight(const float* queries, uint32_t count, float* weights, void* stream)
+{
+    if (count == 0)
+    {
+        return CX_OK;
+    }
+    if (queries == nullptr || weights == nullptr)
+    {
+        return cx::ArgError("cx_light_mis_weight: a buffer is NULL");
+    }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    cx::MisWeightKernel<<<cx::DivUp(count, 256), 256, 0, s>>>(queries, count, weights);
+    return cx::LaunchResult("cx_light_mis_weight");
 }
 
 }  // extern "C"

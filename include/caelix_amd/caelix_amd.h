@@ -446,6 +446,12 @@ CX_API int cx_light_sample(const CxScene* scene, const CxLightTreeView* lights, 
 CX_API int cx_light_face_pdf(const CxScene* scene, const CxLightTreeView* lights, const uint32_t* queries,
                              uint32_t count, float* pdf, void* stream);
 
+/* LightNeeMisWeight (shaders/render/caelix_light_nee.hlsl:20-34), the balance heuristic both MIS sides
+ * share, per query: queries[i] = {p_area, d, cos_l, receiver normal (3), unit direction (3)} (9
+ * floats); weights[i] = {NEE side, BSDF side}. A non-finite or degenerate solid-angle density counts
+ * as infinite (NEE 1, BSDF 0); both densities 0 give NEE 0, BSDF 1. */
+CX_API int cx_light_mis_weight(const float* queries, uint32_t count, float* weights, void* stream);
+
 /* ------------------------------------------------------------------------------------------
  * .cxw world files (format version 8 and older): the reader of the bricks the scene build takes.
  * Replaces the reference's Rust reader host-rs/caelix-cxw/src/lib.rs:1-656 (CxwFile::open,

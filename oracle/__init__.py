@@ -195,6 +195,8 @@ def _trace_lib():
         lib.oracle_light_sample.argtypes = [S, T, _P, _P, _U32, _P, _P]
         lib.oracle_light_face_pdf.restype = None
         lib.oracle_light_face_pdf.argtypes = [S, T, _P, _U32, _P]
+        lib.oracle_mis_weight.restype = None
+        lib.oracle_mis_weight.argtypes = [_P, _U32, _P]
         lib.oracle_trace_frame.restype = None
         lib.oracle_trace_frame.argtypes = [S, T, ctypes.POINTER(_Camera), ctypes.POINTER(_Params), _U32, _U32, _P, _P,
                                            _P, _P]
@@ -203,6 +205,15 @@ def _trace_lib():
                                                    _P, _P, _P]
         lib._trace_bound = True
     return lib
+
+
+def mis_weight(queries: np.ndarray) -> np.ndarray:
+    """LightNeeMisWeight of both sides: queries [n, 9] float32 = (p_area, d, cos_l, n_r, w) ->
+    [n, 2] float32 = (w_nee, w_bsdf)."""
+    q = np.ascontiguousarray(queries, np.float32)
+    out = np.zeros((len(q), 2), np.float32)
+    _trace_lib().oracle_mis_weight(_ptr(q), len(q), _ptr(out))
+    return out
 
 
 def camera_struct(cam: dict) -> _Camera:

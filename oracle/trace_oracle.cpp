@@ -2299,6 +2299,19 @@ void oracle_light_face_pdf(const CxScene* s, const CxLightTreeView* L, const uin
     }
 }
 
+// MisWeight of both sides per query: {p_area, d, cos_l, n_r (3), w (3)} -> {w_nee, w_bsdf}.
+void oracle_mis_weight(const float* queries, uint32_t count, float* weights)
+{
+    for (uint32_t i = 0; i < count; ++i)
+    {
+        const float* q = queries + uint64_t(i) * 9;
+        const V n = Mk(q[3], q[4], q[5]);
+        const V w = Mk(q[6], q[7], q[8]);
+        weights[2 * i] = MisWeight(q[0], n, w, q[1], q[2], true);
+        weights[2 * i + 1] = MisWeight(q[0], n, w, q[1], q[2], false);
+    }
+}
+
 // A frame over pixel rows [row_begin, row_end): color (rgb + t, float64 rgb) and radiance. counters:
 // rays (primary + bounce), shadow rays, brick cells visited, hash probes, brick DDA steps, the demand
 // bytes of every global load the kernels make for those pixels, and that total split by kind of work
