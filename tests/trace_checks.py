@@ -201,3 +201,69 @@ def primary_dirs(cam):
     v = (right[None, :] * sx[:, None] + up[None, :] * sy[:, None]) + fwd[None, :]
     dp = np.maximum(f32(1.175494351e-38), (v[:, 0] * v[:, 0] + v[:, 1] * v[:, 1]) + v[:, 2] * v[:, 2])
     return v * (f32(1.0) / np.sqrt(dp))[:, None]
+
+
+def walk_sparse(keys: np.ndarray, payload: np.ndarray, opaque_mask: int, origins: np.ndarray, dirs: np.ndarray,
+                max_steps: int = 8192):
+    """Amanatides-Woo in float64 for many rays at once through a sparse world of bricks (keys [n, 3],
+    payload [n, 512]): the first voxel whose block has a bit of opaque_mask. Returns (voxel [m, 3]
+    int64, t_entry, entry_axis, hit flag, tie), tie = the smallest gap between two consecutive plane
+    crossings on the way (the float32 DDA may cut a corner where it is small)."""
+    keys = np.asarray(keys, np.int64)
+    lo_b = keys.min(axis=0)
+    shape = keys.max(axis=0) - lo_b + 1
+    index = np.full(tuple(shape), -1, np.int64)
+    index[tuple((keys - lo_b).T)] = np.arange(len(keys))
+    box_lo = (lo_b * 8).astype(np.float64)
+    box_hi = ((lo_b + shape) * 8).astype(np.float64)
+    o = np.asarray(origins, np.float64)
+    d = np.asarray(dirs, np.float64)
+    m = len(o)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = np.where(d != 0.0, 1.0 / d, np.inf)
+        t0 = (box_lo[None, :] - o) * inv
+        t1 = (box_hi[None, :] - o) * inv
+    t0 = np.where(d == 0.0, np.where((o >= box_lo) & (o < box_hi), -np.inf, np.inf), t0)
+    t1 = np.where(d == 0.0, np.where((o >= box_lo) & (o < box_hi), np.inf, -np.inf), t1)
+    near = np.minimum(t0, t1)
+    t_enter = np.maximum(near.max(axis=1), 0.0)
+    t_exit = np.maximum(t0, t1).min(axis=1)
+    active = t_enter <= t_exit
+    p = o + d * t_enter[:, None]
+    cell = np.clip(np.floor(p), box_lo, box_hi - 1).astype(np.int64)
+    step = np.where(d >= 0, 1, -1)
+    with np.errstate(invalid="ignore"):
+        t_next = np.where(d != 0, ((cell + (step > 0)) - o) * inv, np.inf)
+    t_delta = np.abs(inv)
+    t = t_enter.copy()
+    # a ray that starts outside the box enters through the face of its last slab
+    axis = np.where(near.max(axis=1) > 0.0, near.argmax(axis=1), -1)
+    tie = np.full(m, np.inf)
+    hit = np.zeros(m, bool)
+    lo_v, hi_v = (lo_b * 8), ((lo_b + shape) * 8)
+    rows = np.arange(m)
+    for _ in range(max_steps):
+        idx = np.nonzero(active)[0]
+        if idx.size == 0:
+            break
+        c = cell[idx]
+        brick = index[tuple(((c >> 3) - lo_b).T)]
+        local = (c[:, 0] & 7) + 8 * (c[:, 1] & 7) + 64 * (c[:, 2] & 7)
+        block = np.where(brick >= 0, payload[np.maximum(brick, 0), local], 0)
+        solid = (block & opaque_mask) != 0
+        hit[idx[solid]] = True
+        active[idx[solid]] = False
+        idx = idx[~solid]
+        tn = t_next[idx]
+        order = np.argsort(tn, axis=1)
+        first = order[:, 0]
+        r = np.arange(len(idx))
+        tie[idx] = np.minimum(tie[idx], tn[r, order[:, 1]] - tn[r, first])
+        axis[idx] = first
+        t[idx] = tn[r, first]
+        cell[idx, first] += step[idx, first]
+        t_next[idx, first] += t_delta[idx, first]
+        out = (cell[idx, first] < lo_v[first]) | (cell[idx, first] >= hi_v[first])
+        active[idx[out]] = False
+    del rows
+    return cell, t, axis, hit, tie
