@@ -103,3 +103,121 @@ def test_terrain_finalize_matches_rules(oracle):
     solid = np.asarray(data.payload) != 0
     assert (solid & ~occupied).sum() > 100000  # most terrain voxels are buried
     assert not (occupied & ~solid).any()  # opaque terrain: air never becomes an interface
+
+
+# ---- the reference finalize test's other scenarios (finalize_test.cpp:319-451), packed encoding ----
+
+import ref_fixtures as rf  # noqa: E402
+
+
+def random_bricks_world():
+    """Scenario 1: RandomBlobs(300) and a RandomBrick per key from mt19937(20261014)."""
+    rng = rf.Mt19937Stream(20261014)
+    keys = rf.random_blobs(rng, 300)
+    return {k: rf.random_brick(rng) for k in keys}, rng
+
+
+def check_rules(o, data, encoding=0):
+    payload, _ = tc.finalize_payload(data, encoding)
+    assert np.array_equal(o.a["payload"].reshape(-1, 512), payload)
+    words, occ, brick_bits, super_bits = tc.finalize_expected(data, encoding)
+    info = o.a["info"].reshape(-1, 2)
+    assert np.array_equal(info[:, 0], words[:, 0])
+    assert np.array_equal(info[:, 1] & 0x3FFFF, words[:, 1])
+    assert np.array_equal(o.a["occupancy"].reshape(-1, 8), occ)
+    assert np.array_equal(o.a["brick_bits"][: len(brick_bits)], brick_bits)
+
+
+def scenario_worlds():
+    """(scenario 1 world, scenario 5 world: 20 bricks removed and 20 others replaced with new draws)."""
+    world, rng = random_bricks_world()
+    keys = sorted(world)
+    order = np.random.default_rng(5).permutation(len(keys))
+    removed = [keys[i] for i in order[:20]]
+    replaced = [keys[i] for i in order[20:40]]
+    updated = dict(world)
+    for k in removed:
+        del updated[k]
+    for k in replaced:
+        updated[k] = rf.random_brick(rng)
+    return world, updated
+
+
+def test_random_bricks_removal_and_replacement(oracle):
+    world, updated = scenario_worlds()
+    for w in (world, updated):
+        data = rf.scene_from_world(w, rf.look_plus_z(0, 0, -100, 8), group_shift=3, lights=False)
+        o = build(oracle, data)
+        check_rules(o, data)
+        # finalizing the finalized payload again changes nothing (the lossless rule)
+        again = rf.scene_from_world(w, rf.look_plus_z(0, 0, -100, 8), group_shift=3, lights=False)
+        again.payload = o.a["payload"].reshape(-1, 512).copy()
+        o2 = build(oracle, again)
+        for name in ("payload", "info", "occupancy", "brick_bits", "super_bits"):
+            assert np.array_equal(o.a[name], o2.a[name]), name
+    assert len(updated) == 280
+
+
+def solid_cube_checks(info, occ_words, keys):
+    index = {tuple(k): i for i, k in enumerate(np.asarray(keys).tolist())}
+    centre = info[index[(16, 0, 32)]]
+    assert centre[0] >> 16 == 0 and centre[1] == 0  # the interior brick: no voxel alive
+    face = info[index[(15, 0, 32)]]
+    assert face[1] & 0x3FFFF == (0 | (0 << 3) | (0 << 6) | (0 << 9) | (7 << 12) | (7 << 15))  # only x = 0 alive
+    b = index[(15, 0, 32)]
+    assert occupied_bit(occ_words, b, 0, 3, 3) and not occupied_bit(occ_words, b, 1, 3, 3)
+
+
+def test_solid_cube_across_group_boundaries(oracle):
+    world = rf.solid_cube_world(7)
+    assert len({(x >> 4, y >> 4, z >> 4) for x, y, z in world}) == 8
+    data = rf.scene_from_world(world, rf.look_plus_z(0, 0, -100, 8), group_shift=4, lights=False)
+    o = build(oracle, data)
+    check_rules(o, data)
+    solid_cube_checks(o.a["info"].reshape(-1, 2), o.a["occupancy"], data.keys)
+
+
+def neighbour_worlds():
+    solid = np.full(512, 0x8001, np.uint16)
+    return {(15, 0, 0): solid.copy()}, {(15, 0, 0): solid.copy(), (16, 0, 0): solid.copy()}
+
+
+def neighbour_checks(alone_info, alone_occ, both_info, both_occ, both_keys):
+    assert occupied_bit(alone_occ, 0, 7, 3, 3) and not occupied_bit(alone_occ, 0, 3, 3, 3)
+    first = [tuple(k) for k in np.asarray(both_keys).tolist()].index((15, 0, 0))
+    assert not occupied_bit(both_occ, first, 7, 3, 3)
+    assert not np.array_equal(both_occ.reshape(-1, 8)[first], alone_occ.reshape(-1, 8)[0])
+    assert (int(both_info[first, 1]) >> 9) & 7 == 7  # max x stays 7 (the edges of the x = 7 face)
+
+
+def test_neighbour_across_a_group_boundary(oracle):
+    alone, both = neighbour_worlds()
+    cam = rf.look_plus_z(0, 0, -100, 8)
+    d1 = rf.scene_from_world(alone, cam, group_shift=4, lights=False)
+    d2 = rf.scene_from_world(both, cam, group_shift=4, lights=False)
+    o1, o2 = build(oracle, d1), build(oracle, d2)
+    check_rules(o2, d2)
+    neighbour_checks(o1.a["info"].reshape(-1, 2), o1.a["occupancy"], o2.a["info"].reshape(-1, 2), o2.a["occupancy"],
+                     d2.keys)
+
+
+@pytest.mark.gpu
+def test_reference_finalize_scenarios_gpu(cx, oracle):
+    """Scenarios 1, 5 (removal, replacement), 2 (solid cube) and 6 (a neighbour in the next group) on the
+    GPU: every record word, occupancy word, payload value and bitmap equals the oracle's."""
+    cam = rf.look_plus_z(0, 0, -100, 8)
+    world, updated = scenario_worlds()
+    alone, both = neighbour_worlds()
+    cases = [(world, 3), (updated, 3), (rf.solid_cube_world(7), 4), (alone, 4), (both, 4)]
+    got = []
+    for w, shift in cases:
+        data = rf.scene_from_world(w, cam, group_shift=shift, lights=False)
+        g = cx.TraceScene(data, load_factor=LOAD).host_arrays()
+        o = build(oracle, data)
+        for name in ("payload", "info", "occupancy", "brick_bits", "super_bits"):
+            assert np.array_equal(g[name], o.a[name]), name
+        got.append((g, data))
+    g, data = got[2]
+    solid_cube_checks(g["info"].reshape(-1, 2), g["occupancy"], data.keys)
+    (g1, _), (g2, d2) = got[3], got[4]
+    neighbour_checks(g1["info"].reshape(-1, 2), g1["occupancy"], g2["info"].reshape(-1, 2), g2["occupancy"], d2.keys)

@@ -223,3 +223,155 @@ def test_gpu_free_stack_exhaustion_and_eviction(cx):
     m = t.maintain(2 + 10, max_age=5)
     assert m["evicted"] == 256 and m["live"] == 0
     assert t.top() == 256
+
+
+# --------------------------------------- the reference test's table checks (hash_table_test.cpp:318-470)
+
+
+def table_keys(t) -> dict:
+    """(key_lo, key_hi) -> the slots holding it."""
+    slots = t.keys.cpu().numpy().view(np.uint32).reshape(-1, 4)
+    out: dict = {}
+    for i in np.nonzero(slots[:, 0])[0]:
+        out.setdefault((int(slots[i, 0]), int(slots[i, 1])), []).append(int(i))
+    return out
+
+
+def payload_headers(t) -> np.ndarray:
+    return t.payloads.cpu().numpy().view(np.uint32).reshape(t.capacity, -1)
+
+
+def check_inserts(oracle, t, keys, key_of, results, payloads) -> dict:
+    """CheckInserts: per key at most one insert, every hit and insert on one payload, every key once in
+    its window, no payload twice, every payload header holding its key. Returns key index -> payload."""
+    r = results.cpu().numpy().view(np.uint32)
+    p = payloads.cpu().numpy().view(np.uint32)
+    assert np.all(r <= K_EMPTY)
+    payload_of: dict = {}
+    inserted = np.zeros(len(keys), np.int64)
+    for i in np.nonzero((r == K_INSERTED) | (r == K_HIT))[0]:
+        assert p[i] < t.capacity
+        assert payload_of.setdefault(int(key_of[i]), int(p[i])) == int(p[i])
+    np.add.at(inserted, key_of[r == K_INSERTED], 1)
+    assert np.all(inserted <= 1)
+    slots = t.keys.cpu().numpy().view(np.uint32).reshape(-1, 4)
+    headers = payload_headers(t)
+    fps = oracle.hash_fingerprint(keys)
+    key_index = {(int(a), int(b)): k for k, (a, b) in enumerate(fps)}
+    seen_payloads = set()
+    for key, where in table_keys(t).items():
+        assert len(where) == 1 and key[1] != 0
+        assert ((where[0] - key[0]) & (t.slot_count - 1)) < 8
+        payload = int(slots[where[0], 2])
+        if payload == HASH_NONE:
+            continue
+        assert payload not in seen_payloads
+        seen_payloads.add(payload)
+        assert headers[payload, 0] in (1, 2)
+        fp = oracle.hash_fingerprint(headers[payload, 2:2 + t.key_words][None, :])[0]
+        assert (int(fp[0]), int(fp[1])) == key
+        k = key_index.get(key)
+        if k is not None and k in payload_of:
+            assert payload_of[k] == payload
+    table = table_keys(t)
+    for k, payload in payload_of.items():
+        where = table[(int(fps[k, 0]), int(fps[k, 1]))]
+        assert int(slots[where[0], 2]) == payload
+    return payload_of
+
+
+def check_partition(t) -> None:
+    """CheckPartition: the free stack's entries and the payloads that are not free partition the pool."""
+    stack = t.stack.cpu().numpy().view(np.int32)
+    top = int(stack[0])
+    assert top >= 0
+    headers = payload_headers(t)
+    seen = np.zeros(t.capacity, np.int64)
+    entries = stack[1:1 + top].view(np.uint32)
+    assert np.all(entries < t.capacity)
+    assert np.all(headers[entries, 0] == 0)
+    np.add.at(seen, entries, 1)
+    seen += headers[:, 0] != 0
+    assert np.all(seen == 1)
+
+
+def check_rebuilt(oracle, t, frame, counters) -> None:
+    """CheckRebuilt: after a maintenance every live payload's key is in the table once, on that payload;
+    the active list holds exactly the payloads touched at frame - 1; live and active count them."""
+    table = table_keys(t)
+    slots = t.keys.cpu().numpy().view(np.uint32).reshape(-1, 4)
+    headers = payload_headers(t)
+    active = t.active.cpu().numpy().view(np.uint32)[: counters["active"]]
+    assert len(set(active.tolist())) == len(active)
+    live_ids = np.nonzero(headers[:, 0] != 0)[0]
+    if len(live_ids):
+        fps = oracle.hash_fingerprint(np.ascontiguousarray(headers[live_ids, 2:2 + t.key_words]))
+        for p, fp in zip(live_ids, fps):
+            where = table[(int(fp[0]), int(fp[1]))]
+            assert len(where) == 1 and int(slots[where[0], 2]) == p
+    is_active = headers[live_ids, 1] == frame - 1
+    assert set(active.tolist()) == set(live_ids[is_active].tolist())
+    assert np.all(headers[active, 0] != 0)
+    assert len(table) == len(live_ids) == counters["live"]
+    assert counters["active"] == int(is_active.sum())
+
+
+@pytest.mark.gpu
+def test_gpu_equal_key_lo_pair(cx, oracle):
+    """Two keys with the same key_lo (so the same home) and different key_hi, found by a birthday search:
+    one dispatch inserts at least one; a second dispatch of the same frame finds it published and inserts
+    the other in the next slot; both survive the maintenance (hash_table_test.cpp:655-704)."""
+    import torch
+
+    rng = np.random.default_rng(31)
+    words = random_words(rng, 400000, 2)
+    fps = oracle.hash_fingerprint(words)
+    order = np.argsort(fps[:, 0], kind="stable")
+    lo = fps[order, 0]
+    dup = np.nonzero((lo[1:] == lo[:-1]) & (fps[order[1:], 1] != fps[order[:-1], 1]))[0]
+    assert len(dup) > 0
+    a, b = order[dup[0]], order[dup[0] + 1]
+    keys = np.stack([words[a], words[b]])
+    t = cx.DynamicHashTable(1024, 2)
+    t.maintain(2)
+    key_of = np.arange(1024) & 1
+    inputs = torch.from_numpy(keys[key_of].view(np.int32)).cuda()
+    first = check_inserts(oracle, t, keys, key_of, *t.lookup(inputs, 2, insert=True))
+    assert len(first) >= 1
+    second = check_inserts(oracle, t, keys, key_of, *t.lookup(inputs, 2, insert=True))
+    assert len(second) == 2 and second[0] != second[1]
+    m = t.maintain(3)
+    check_rebuilt(oracle, t, 3, m)
+    assert m["live"] == 2
+    look, payloads = t.lookup(torch.from_numpy(keys.view(np.int32)).cuda(), 3, insert=False)
+    assert np.all(look.cpu().numpy() == K_HIT)
+    assert payloads.cpu().numpy().tolist() == [second[0], second[1]]
+    check_partition(t)
+
+
+@pytest.mark.gpu
+def test_gpu_random_frames_keep_the_pool_partitioned(cx, oracle):
+    """40 frames of random key subsets from a universe of 6000 five-word keys, up to 12 threads per key
+    (some frames overfill the 2048-payload pool), each followed by a maintenance (max age 6): every
+    dispatch passes CheckInserts, every maintenance CheckRebuilt and CheckPartition
+    (hash_table_test.cpp:806-855)."""
+    import torch
+
+    rng = np.random.default_rng(51)
+    t = cx.DynamicHashTable(2048, 5)
+    universe = random_words(rng, 6000, 5)
+    frame = 2
+    m = t.maintain(frame, max_age=6)
+    for round_ in range(40):
+        distinct = 100 + int(rng.integers(0, 3000))
+        chosen = rng.choice(len(universe), size=distinct, replace=False)
+        keys = universe[chosen]
+        threads = distinct * (1 + int(rng.integers(0, 12)))
+        key_of = rng.integers(0, distinct, size=threads)
+        inputs = torch.from_numpy(np.ascontiguousarray(keys[key_of]).view(np.int32)).cuda()
+        check_inserts(oracle, t, keys, key_of, *t.lookup(inputs, frame, insert=True))
+        frame += 1
+        m = t.maintain(frame, max_age=6)
+        check_rebuilt(oracle, t, frame, m)
+        check_partition(t)
+    assert m["live"] > 0

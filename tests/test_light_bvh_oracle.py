@@ -10,6 +10,9 @@ import pytest
 import lt_checks
 from caelix_amd import layout, synthetic
 
+LEAF_BIT = layout.LT_CHILD_LEAF_BIT
+NONE = layout.LT_NONE
+
 
 def run_oracle(oracle, leaves, capacity=None, node_capacity=None, found=None, building=1, builds=0):
     n = len(leaves)
@@ -186,3 +189,40 @@ def test_pool_stress_input_needs_the_spill(oracle):
     left, right, _, _ = lt_checks.top_down_radix_tree(keys)
     assert np.array_equal(left, layout.scratch_u32(out["scratch"], layout.SCRATCH_LEFT, cap)[: n - 1])
     assert np.array_equal(right, layout.scratch_u32(out["scratch"], layout.SCRATCH_RIGHT, cap)[: n - 1])
+
+
+@pytest.mark.parametrize("n,seed", [(3000, 21), (777, 22)])
+def test_wide_slot_order_from_the_top_down_tree(oracle, n, seed):
+    """Every 4-wide node as caelix_light_bvh_wide.hlsl:10-100 makes it, recomputed from the top-down
+    radix tree (not the oracle's): node w = the exclusive scan of the even-depth flags at binary node i;
+    its slots are i's children in order, an internal child replaced by its two children (left first); a
+    leaf slot holds kLtChildLeafBit | the leaf's index, an internal one its 4-wide index; leaf parents
+    are w << 2 | slot, child nodes' parent links (w, slot)."""
+    leaves = synthetic.light_leaves(n, seed, extent=512.0, clusters=3, sigma=2.0)  # many equal codes
+    out = run_oracle(oracle, leaves)
+    cap = out["capacity"]
+    keys = layout.scratch_u32(out["scratch"], layout.SCRATCH_KEYS, cap)[:n]
+    values = layout.scratch_u32(out["scratch"], layout.SCRATCH_VALUES, cap)[:n]
+    assert len(np.unique(keys)) < n  # equal codes present
+    left, right, parent, _ = lt_checks.top_down_radix_tree(keys)
+    depth = lt_checks.depths_by_walk(parent)
+    even = (depth & 1) == 0
+    wide = np.concatenate([[0], np.cumsum(even)[:-1]]).astype(np.int64)
+    nodes = out["nodes_rec"]
+    leaf_parent = out["leaves_rec"]["parent"]
+    count = int(out["header_rec"]["node_count"])
+    assert count == int(even.sum())
+    for i in np.nonzero(even)[0]:
+        w = int(wide[i])
+        slots = []
+        for c in (int(left[i]), int(right[i])):
+            slots += [c] if c & LEAF_BIT else [int(left[c]), int(right[c])]
+        words = [LEAF_BIT | int(values[s & ~LEAF_BIT]) if s & LEAF_BIT else int(wide[s]) for s in slots]
+        words += [NONE] * (4 - len(words))
+        assert list(nodes[w]["child"]) == words, (i, w)
+        for s, word in enumerate(words[: len(slots)]):
+            if word & LEAF_BIT:
+                assert int(leaf_parent[word & ~LEAF_BIT]) == (w << 2) | s
+            else:
+                assert (int(nodes[word]["parent"]), int(nodes[word]["parent_slot"])) == (w, s)
+    assert int(nodes[0]["parent"]) == NONE
