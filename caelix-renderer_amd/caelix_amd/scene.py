@@ -381,7 +381,8 @@ def chunk_scene(seed: int, bricks: int, grid: int = 64, emissive_fraction: float
     """One chunk scene of the batched workload: a value-noise terrain in a grid^3-brick cell (grid a
     multiple of 8), exactly `bricks` resident bricks taken layer by layer from the surface down (the
     surface bricks of all columns first, a random subset of columns for a partial layer), about
-    `emissive_fraction` of them emissive (at least one), camera from the seed. `materials` (table,
+    `emissive_fraction` of them emissive (at least one; chosen among the bricks with an open face),
+    camera from the seed. `materials` (table,
     light codes) is shared by the scenes of a batch; by default the batch palette."""
     world = grid * 8
     rng = np.random.default_rng(seed)
@@ -416,8 +417,13 @@ def chunk_scene(seed: int, bricks: int, grid: int = 64, emissive_fraction: float
     depth = hz[:, :, None, :] - ys[:, None, :, None]
     mat = 1 + ((xs[:, None, None, :] * 7 + zs[:, :, None, None] * 13 + depth * 3) % 127)
     payload = np.where(solid, OPAQUE | mat, 0).astype(np.uint16).reshape(n, 512)
+    # Emitters among the bricks with an open face: a brick that holds both solid and air voxels has a
+    # solid voxel next to an air voxel, so its emission reaches the scene (a fully solid brick may be
+    # buried and give no light face).
     solid_count = (payload != 0).sum(axis=1)
-    candidates = np.nonzero(solid_count > 0)[0]
+    candidates = np.nonzero((solid_count > 0) & (solid_count < 512))[0]
+    if len(candidates) == 0:
+        candidates = np.nonzero(solid_count > 0)[0]
     emissive = max(1, int(round(emissive_fraction * n)))
     chosen = rng.choice(candidates, size=min(emissive, len(candidates)), replace=False)
     emissive_ids = rng.integers(128, 256, size=len(chosen))
