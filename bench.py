@@ -238,6 +238,25 @@ def trace_bench(args) -> dict:
     return result
 
 
+def host_cpu() -> dict:
+    """The host's CPU model (the first "model name" of /proc/cpuinfo, as lscpu reports it) and the
+    hardware threads this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "host_threads": usable}
+
+
 def trace_cpu_baseline(o, cam, seed) -> dict:
     """The oracle's path tracer (OpenMP over rows) on bands of 64 rows of the same frame, repeated
     over the next bands until at least 10 s have passed."""
@@ -257,7 +276,7 @@ def trace_cpu_baseline(o, cam, seed) -> dict:
     seconds = time.perf_counter() - t0
     return {"value": round(rays / seconds / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"oracle path tracer on {rows} rows of the same 1080p frame ({rays} rays), "
-                      f"{threads} OpenMP threads"}
+                      f"{threads} OpenMP threads", **host_cpu()}
 
 
 # ------------------------------------------------------------------------------------------------
@@ -344,12 +363,14 @@ def lighttree_cpu_baseline(leaves) -> dict:
     while True:
         oracle.light_bvh_build(sub.view(np.uint8), sample, node_capacity, state.view(np.uint8))
         runs += 1
-        if time.perf_counter() - t0 >= 10.0:
+        if time.perf_counter() - t0 >= 8.0:
             break
     seconds = (time.perf_counter() - t0) / runs
+    # The oracle's build is the reference's kernel sequence restated as one sequential pass (each kernel
+    # a loop over its lanes), so it runs on one thread; the trace baseline shows the multi-core oracle.
     return {"value": round(sample / seconds / 1e6, 3), "unit": "Mprims/s", "cores": 1, "kind": "port",
             "sample": f"oracle light-tree build of the first {sample} leaves of the same workload, "
-                      f"{runs} runs, single thread"}
+                      f"{runs} runs, single thread", **host_cpu()}
 
 
 def diff_bench(args) -> dict:
@@ -428,9 +449,10 @@ def diff_bench(args) -> dict:
 def batch_bench(args) -> dict:
     """The batched form: 256 chunk scenes (brick counts log-uniform in [64, 65536] as far as the
     64^3-brick chunk's terrain holds them, 5 % emissive), each step rebuilding every scene's light
-    tree and tracing a 256x256, 1 spp, 2-bounce frame of every scene. Timed three ways on the same
-    scenes: launch-per-scene (one build and one cx_trace_frame per scene), batched (one batched build
-    and one cx_trace_batch), and the batched step captured in a hipGraph and replayed."""
+    tree and tracing a 256x256, 1 spp, 2-bounce frame of every scene. Timed four ways on the same
+    scenes: launch-per-scene (one build and one cx_trace_frame per scene, launched from Python), the
+    same launch-per-scene step captured in a hipGraph and replayed, batched (one batched build and one
+    cx_trace_batch), and the batched step captured in a hipGraph and replayed."""
     import numpy as np
     import torch
 
@@ -465,6 +487,16 @@ def batch_bench(args) -> dict:
             batched(0)
     stream.wait_stream(side)
     graph_ms = _time_steps(lambda _: graph.replay(), steps, 1, stream)
+    # The launch-per-scene step captured in a hipGraph too (the same launches without Python and ctypes
+    # in between): the baseline the batched form is held against without host launch overhead.
+    plain_graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(side):
+        plain.step(2, 50, 1)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(plain_graph, stream=side):
+            plain.step(2, 50, 1)
+    stream.wait_stream(side)
+    plain_graph_ms = _time_steps(lambda _: plain_graph.replay(), steps, 1, stream)
     # the rays of one step
     batch.counters.zero_()
     batched(0)
@@ -474,10 +506,11 @@ def batch_bench(args) -> dict:
     bricks = np.array([len(d.keys) for d in datas])
     return {
         "plain_ms": round(plain_ms, 3), "batched_default_ms": round(batch_ms, 3), "batched_ms": round(tuned_ms, 3),
-        "graph_ms": round(graph_ms, 3),
+        "graph_ms": round(graph_ms, 3), "plain_graph_ms": round(plain_graph_ms, 3),
         "autotuned": {"queue_chunk": tuned["queue_chunk"], "workgroups_per_cu": tuned["workgroups_per_cu"],
                       "trials": len(tuned.get("trials", []))},
         "speedup_batched": round(plain_ms / tuned_ms, 2), "speedup_graph": round(plain_ms / graph_ms, 2),
+        "speedup_graph_vs_plain_graph": round(plain_graph_ms / graph_ms, 2),
         "rays_per_step": int(rays), "graph_Mrays_per_s": round(rays / (graph_ms * 1e-3) / 1e6, 1),
         "config": {"workload": "256 chunk scenes (64^3-brick terrain chunks), 5 % emissive, light-tree build + "
                                "256x256 1 spp 2-bounce frame each", "scenes": len(datas),
@@ -581,9 +614,10 @@ def main() -> int:
         if not args.no_batch:
             result["batch"] = batch_bench(args)
         if not args.no_lighttree:
-            lt = lighttree_bench(args, with_cpu=False)
+            lt = lighttree_bench(args, with_cpu=not args.no_cpu_baseline)
             result["lighttree"] = {k: lt[k] for k in ("value", "unit", "ms_per_step", "roofline",
-                                                      "pipeline_roofline", "config", "keys64")}
+                                                      "pipeline_roofline", "config", "keys64", "cpu_baseline")
+                                   if k in lt}
     elif args.workload == "copy":
         result = copy_bench(args)
     elif args.workload == "cxw":

@@ -131,7 +131,8 @@ struct Workspace
     float4* adjoint;    // [paths] adjoint: dL/d(radiance term) (rgb)
     float4* shadow_grad; // [paths] adjoint: the shadow ray's radiance without the emission scale, roulette
     uint32_t* queue[2]; // [paths] path queues (ping-pong by bounce parity)
-    uint32_t* nee_rng;  // [paths] the sun's random stream (read and written only with the sun's NEE)
+    uint32_t* nee_rng;  // [paths] the sun's random stream, kept whenever something draws from it: the
+                        // sun's NEE, or the bounce NEE budget below 1 (KeepsNeeStream)
     uint32_t* ctl;      // [0], [1] queue lengths, [2] shadow queue length
     float* partials;    // [kGradRows][768] adjoint: per-workgroup albedo gradient sums
 };

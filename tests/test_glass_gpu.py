@@ -143,3 +143,15 @@ def test_delta_checkerboard_gpu(cx, oracle):
     want_color, want_depth = checkerboard_expectation(data)
     assert np.allclose(color[:, :3], want_color, rtol=1e-4, atol=1e-7)
     assert np.allclose(color[:, 3], want_depth, rtol=1e-5)
+
+
+def test_glass_terrain_bounce_budget_under_the_sun_matches_oracle(cx, terrain):
+    """The glass instance of the bounce shading at q = 0.5 under the sky and the sun, whose stream the
+    budget's number shares with the sun candidate: counters exact, radiance within 2e-5."""
+    data, gpu, ref = terrain
+    sky = cx.sky_params(sun_dir=(0.3, 0.9, 0.2), sun_radiance=(40.0, 36.0, 30.0), sun_radius_deg=3.0,
+                        zenith=(0.2, 0.3, 0.6), horizon=(0.6, 0.6, 0.7), ground=(0.1, 0.1, 0.1))
+    for bounces in (2, 3):
+        got = gpu.trace_frame(data.camera, bounces, 55, 2, sky=sky, bounces_transparent=4, bounce_nee_probability=0.5)
+        want = ref.trace_frame(data.camera, bounces, 55, 2, sky=sky, bounces_transparent=4, bounce_nee_probability=0.5)
+        compare(got, want)

@@ -251,3 +251,32 @@ def test_fog_isothermal_furnace_one_bounce(oracle):
     diagonal = 48 * np.sqrt(3.0)
     assert np.all(t1_from_radiance > np.exp(-s_ * diagonal) - 1e-6) and np.all(t1_from_radiance < 1.0)
     assert t1_from_radiance.mean() < 0.8  # the bounce segments lose a visible share to the fog
+
+
+@pytest.mark.parametrize("q", [0.5, 0.25])
+def test_bounce_nee_budget_under_the_sun(oracle, floor, q):
+    """The bounce NEE budget draws its number from the sun's stream only at bounce vertices: on the sun
+    floor (whose bounce rays leave into the black sky) the primary vertex's sun NEE is untouched, so
+    the closed form L sin^2 r holds at q < 1 too."""
+    data, ref = floor
+    L, r_deg, spp = 1000.0, 2.0, 4
+    k = sky(sun_radiance=(L, L, L), radius_deg=r_deg)
+    _, radiance, _, counters = ref.trace_frame(data.camera, 4, 7, spp, sky=k, bounce_nee_probability=q)
+    want = L * np.sin(np.radians(r_deg)) ** 2
+    assert abs(radiance[:, 0].mean() / spp - want) < 2e-5 * want
+
+
+def test_bounce_nee_budget_unbiased_with_sky_sun_and_glass(oracle):
+    """On the glass terrain under the sky and the sun, two bounces: the image at q = 0.5 (bounce vertices
+    run their sun / lamp NEE half the time, scaled by 2) has the mean of the image at q = 1, within 5 sigma
+    of the per-pixel differences of two independent renders."""
+    data = scene.glass_terrain()
+    ref = build(oracle, data)
+    k = sky(sun_dir=(0.3, 0.9, 0.2), sun_radiance=(40.0, 36.0, 30.0), radius_deg=3.0, zenith=(0.2, 0.3, 0.6),
+            horizon=(0.6, 0.6, 0.7), ground=(0.1, 0.1, 0.1))
+    spp = 16
+    half = ref.trace_frame(data.camera, 2, 100, spp, sky=k, bounces_transparent=4, bounce_nee_probability=0.5)
+    full = ref.trace_frame(data.camera, 2, 900, spp, sky=k, bounces_transparent=4)
+    assert half[3][1] < full[3][1]  # fewer shadow rays
+    d = (half[1][:, :3] - full[1][:, :3]).sum(axis=1) / spp
+    assert abs(d.mean()) < 5 * d.std() / np.sqrt(len(d)), (d.mean(), d.std())
