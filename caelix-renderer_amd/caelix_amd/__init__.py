@@ -65,25 +65,45 @@ class CxScene(ctypes.Structure):
         ("group_shift", ctypes.c_uint32), ("encoding", ctypes.c_uint32), ("materials", ctypes.c_void_p),
         ("light_codes", ctypes.c_void_p), ("emission_scale", ctypes.c_float), ("pad", ctypes.c_uint32),
         ("brick_emission_scale", ctypes.c_void_p), ("scene_count", ctypes.c_uint32), ("pad2", ctypes.c_uint32),
-        ("occupied_box", ctypes.c_void_p), ("materials_ext", ctypes.c_void_p),
+        ("occupied_box", ctypes.c_void_p), ("materials_ext", ctypes.c_void_p), ("group_of_brick", ctypes.c_void_p),
+        ("light_group_rows", ctypes.c_void_p),
     ]
 
 
-class CxLightScene(ctypes.Structure):
-    _fields_ = [
-        ("heap", ctypes.c_void_p), ("heap_units", ctypes.c_uint32), ("leaf_capacity", ctypes.c_uint32),
-        ("record_offset", ctypes.c_void_p), ("ordinal_of_brick", ctypes.c_void_p), ("leaves", ctypes.c_void_p),
-        ("state", ctypes.c_void_p), ("totals", ctypes.c_void_p), ("scratch", ctypes.c_void_p),
-    ]
+class CxLightPass(ctypes.Structure):
+    _fields_ = [("descriptor", ctypes.c_void_p), ("heap_units", ctypes.c_uint64), ("staged_flags", ctypes.c_void_p),
+                ("scratch", ctypes.c_void_p), ("group_count", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
 
 
 class CxLightTreeView(ctypes.Structure):
-    _fields_ = [
-        ("headers", ctypes.c_void_p), ("nodes", ctypes.c_void_p * 2), ("leaves", ctypes.c_void_p * 2),
-        ("heap", ctypes.c_void_p), ("record_offset", ctypes.c_void_p), ("ordinal_of_brick", ctypes.c_void_p),
-        ("group_shift", ctypes.c_uint32), ("mis", ctypes.c_uint32), ("leaf_base", ctypes.c_uint32),
-        ("pad", ctypes.c_uint32),
-    ]
+    _fields_ = [("descriptor", ctypes.c_void_p), ("mis", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+
+
+class CxLightOptions(ctypes.Structure):
+    _fields_ = [("enabled", ctypes.c_uint32), ("min_emission", ctypes.c_float), ("heap_bytes", ctypes.c_uint64)]
+
+
+class CxLightStats(ctypes.Structure):
+    _U = ctypes.c_uint32
+    _fields_ = [("enabled", _U), ("light_groups", _U), ("light_bricks", ctypes.c_uint64),
+                ("light_faces", ctypes.c_uint64), ("bricks_dropped", ctypes.c_uint64),
+                ("heap_bytes_used", ctypes.c_uint64), ("heap_bytes_capacity", ctypes.c_uint64), ("passes", _U),
+                ("failed_passes", _U), ("records_recomputed", _U), ("records_copied", _U), ("gpu_ms", ctypes.c_float),
+                ("pad", _U), ("tree_leaves", _U * 2), ("tree_nodes", _U * 2), ("tree_builds", _U * 2),
+                ("tree_leaves_dropped", _U * 2)]
+
+    def as_dict(self) -> dict:
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+
+class CxLightSceneTables(ctypes.Structure):
+    _fields_ = [("group_infos", ctypes.c_void_p), ("entity_infos", ctypes.c_void_p),
+                ("group_capacity", ctypes.c_uint32), ("entity_capacity", ctypes.c_uint32),
+                ("brick_capacity", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
 
 
 class CxCamera(ctypes.Structure):
@@ -188,8 +208,33 @@ SIGNATURES = {
     "cx_hash_build_bricks": (_INT, [_P, _U32, _P, _U32, _P, _P, _P]),
     "cx_hash_get_bricks": (_INT, [_P, _U32, _P, _P, _U32, _P, _P, _P]),
     "cx_scene_finalize": (_INT, [ctypes.POINTER(CxScene), _P]),
-    "cx_scene_light_scratch_bytes": (_U64, [_U32]),
-    "cx_scene_light_records": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightScene), _P]),
+    "cx_light_pass_scratch_bytes": (_U64, [_U32, _U32]),
+    "cx_light_pass": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightPass), _P]),
+    "cx_light_gather": (_INT, [ctypes.POINTER(CxLightPass), _U32, _P, _U32, _P, _P]),
+    "cx_light_gather_entities": (_INT, [ctypes.POINTER(CxLightPass), _U32, _P, _U32, _P, _P, _P]),
+    "cx_light_tree_create": (_INT, [_U32, _U32, ctypes.POINTER(CxLightOptions), _P]),
+    "cx_light_tree_destroy": (None, [_P]),
+    "cx_light_tree_on_entity_added": (_INT, [_P, _U32, _U32]),
+    "cx_light_tree_on_entity_removed": (_INT, [_P, _U32]),
+    "cx_light_tree_on_entity_moved": (_INT, [_P, _U32]),
+    "cx_light_tree_on_entity_motion": (_INT, [_P, _U32, _U32]),
+    "cx_light_tree_on_group_created": (_INT, [_P, _U32]),
+    "cx_light_tree_on_group_released": (_INT, [_P, _U32, _U32]),
+    "cx_light_tree_on_group_bricks_removed": (_INT, [_P, _U32]),
+    "cx_light_tree_on_materials_changed": (_INT, [_P]),
+    "cx_light_tree_set_min_emission": (_INT, [_P, ctypes.c_float]),
+    "cx_light_tree_prepare": (_INT, [_P, _P, ctypes.POINTER(CxLightSceneTables), _P]),
+    "cx_light_tree_stage": (_INT, [_P, _U32, _P]),
+    "cx_light_tree_record_transfers": (_INT, [_P, _P]),
+    "cx_light_tree_record": (_INT, [_P, ctypes.POINTER(CxScene), _U64, _P]),
+    "cx_light_tree_end_flush": (_INT, [_P, _U64]),
+    "cx_light_tree_rebuild": (_INT, [_P, _U32, _P]),
+    "cx_light_tree_recycle": (_INT, [_P, _U64]),
+    "cx_light_tree_get_stats": (None, [_P, ctypes.POINTER(CxLightStats)]),
+    "cx_light_tree_pending": (_U32, [_P]),
+    "cx_light_tree_descriptor": (_P, [_P]),
+    "cx_light_tree_light_codes": (_P, [_P]),
+    "cx_light_tree_group_rows": (_P, [_P]),
     "cx_trace_workspace_bytes": (_U64, [_U32, _U32]),
     "cx_trace_frame": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightTreeView), ctypes.POINTER(CxCamera),
                               ctypes.POINTER(CxTraceParams), ctypes.POINTER(CxTraceOutput), _P]),
@@ -203,8 +248,8 @@ SIGNATURES = {
                               ctypes.POINTER(CxTraceOutput), _P]),
     "cx_hash_build_keys": (_INT, [_P, _U32, _U32, _P, _U32, _P, _P, _P]),
     "cx_trace_rays": (_INT, [ctypes.POINTER(CxScene), _P, _U32, _P, _P]),
-    "cx_light_sample": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightTreeView), _P, _P, _U32, _P, _P, _P]),
-    "cx_light_face_pdf": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightTreeView), _P, _U32, _P, _P]),
+    "cx_light_sample": (_INT, [ctypes.POINTER(CxLightTreeView), _P, _P, _U32, _P, _P, _P]),
+    "cx_light_point_pdf": (_INT, [ctypes.POINTER(CxLightTreeView), _P, _U32, _P, _P]),
     "cx_light_mis_weight": (_INT, [_P, _U32, _P, _P]),
     "cx_cxw_open": (_INT, [ctypes.c_char_p, _P, _P]),
     "cx_cxw_close": (None, [_P]),
@@ -495,6 +540,241 @@ class DynamicHashTable:
         return int(to_host(self.stack, np.int32)[0])
 
 
+def groups_of(keys: np.ndarray, group_shift: int):
+    """The render groups of bricks: (group coordinates [G, k], group_of_brick [n] uint32). A group holds the
+    (1 << group_shift)^3 bricks of one group coordinate key >> group_shift (per coordinate, arithmetic);
+    keys of 4 words (scene, x, y, z) keep the scene word. Groups are numbered in ascending lexicographic
+    order of their coordinates (scene-major for a batch)."""
+    k = np.ascontiguousarray(keys, np.int64).copy()
+    k[:, -3:] >>= int(group_shift)
+    coords, inverse = np.unique(k, axis=0, return_inverse=True)
+    return coords.astype(np.int32), inverse.reshape(-1).astype(np.uint32)
+
+
+GROUP_INFO = np.dtype([("coord", "<i4", (3,)), ("entity", "<u4"), ("slot_row_base", "<u4"), ("reverse_base", "<u4"),
+                       ("aabb_row_base", "<u4"), ("high_water", "<u4")])
+ENTITY_INFO = np.dtype([("grid_min", "<i4", (3,)), ("grid_base", "<u4"), ("grid_dims", "<i4", (3,)), ("life", "<u4"),
+                        ("cur_rows", "<f4", (3, 4)), ("prev_rows", "<f4", (3, 4))])
+assert GROUP_INFO.itemsize == 32 and ENTITY_INFO.itemsize == 128
+
+
+def group_infos(coords: np.ndarray, entity_of_group=None) -> np.ndarray:
+    g = np.zeros(len(coords), GROUP_INFO)
+    g["coord"] = coords[:, -3:]
+    g["entity"] = 0 if entity_of_group is None else np.asarray(entity_of_group, np.uint32)
+    return g
+
+
+def entity_infos(count: int, rows=None) -> np.ndarray:
+    """`count` entities; rows [count, 3, 4] object-to-world (rotation | translation) or None: identity."""
+    e = np.zeros(count, ENTITY_INFO)
+    if rows is None:
+        rows = np.tile(np.eye(3, 4, dtype=np.float32), (count, 1, 1))
+    e["cur_rows"] = rows
+    e["prev_rows"] = rows
+    return e
+
+
+def light_codes_for(materials: np.ndarray, min_emission: float, encoding: int) -> np.ndarray:
+    """The light code set the tree computes (LightTree::RebuildCodes, light_tree.cpp:185-198): a code whose
+    canonical code's emission luminance reaches min_emission. [2048] uint32."""
+    codes = np.arange(65536, dtype=np.uint32)
+    if encoding == layout.ENCODING_PACKED:
+        opaque, ids = codes & np.uint32(layout.PACKED_OPAQUE_MASK), codes & np.uint32(layout.PACKED_ID_MASK)
+    else:
+        opaque, ids = codes & np.uint32(layout.BLOCK_ID_OPAQUE_MASK), codes & np.uint32(layout.BLOCK_ID_ID_MASK)
+    canonical = np.where(opaque != 0, codes, ids)
+    e = np.asarray(materials, np.float32).reshape(65536, 8)[canonical, 4:7]
+    lum = np.float32(0.2126) * e[:, 0] + np.float32(0.7152) * e[:, 1] + np.float32(0.0722) * e[:, 2]
+    bits = (lum >= np.float32(min_emission)).reshape(2048, 32).astype(np.uint32)
+    return (bits << np.arange(32, dtype=np.uint32)).sum(axis=1, dtype=np.uint64).astype(np.uint32)
+
+
+def min_emission_for(data) -> float:
+    """The light threshold that makes the tree's code set the scene's light codes (the smallest emission
+    luminance among them), checked on every canonical code; ValueError when no threshold does."""
+    enc = int(getattr(data, "encoding", 1))
+    want = np.asarray(data.light_codes, np.uint32)
+    bits = ((want[:, None] >> np.arange(32, dtype=np.uint32)) & 1).reshape(-1).astype(bool)
+    e = np.asarray(data.materials, np.float32).reshape(65536, 8)[:, 4:7]
+    lum = np.float32(0.2126) * e[:, 0] + np.float32(0.7152) * e[:, 1] + np.float32(0.0722) * e[:, 2]
+    if not bits.any():
+        return 3.0e38
+    threshold = float(lum[bits].min())
+    if not threshold > 0.0:
+        raise ValueError("a light code with zero emission")
+    got = light_codes_for(data.materials, threshold, enc)
+    got_bits = ((got[:, None] >> np.arange(32, dtype=np.uint32)) & 1).reshape(-1).astype(bool)
+    codes = np.arange(65536, dtype=np.uint32)
+    mask = np.uint32(layout.PACKED_OPAQUE_MASK if enc == layout.ENCODING_PACKED else layout.BLOCK_ID_OPAQUE_MASK)
+    idm = np.uint32(layout.PACKED_ID_MASK if enc == layout.ENCODING_PACKED else layout.BLOCK_ID_ID_MASK)
+    canonical = np.where((codes & mask) != 0, codes, codes & idm) == codes
+    if not np.array_equal(got_bits[canonical], bits[canonical]):
+        raise ValueError("the scene's light codes are not a luminance threshold of its materials")
+    return threshold
+
+
+_hip = None
+
+
+def read_device(address: int, nbytes: int) -> np.ndarray:
+    """Copies nbytes at a device address (memory the library allocated) to a uint8 array."""
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.restype = ctypes.c_int
+        _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    out = np.empty(nbytes, np.uint8)
+    if nbytes > 0:
+        rc = _hip.hipMemcpy(out.ctypes.data, address, nbytes, 2)  # hipMemcpyDeviceToHost
+        if rc != 0:
+            raise CxError(f"hipMemcpy from the device failed ({rc})")
+    return out
+
+
+class LightTree:
+    """The light tree's host side (cx_light_tree_*, the reference's LightTree): hooks, flush steps and
+    stats over the library's adapter. flush() runs prepare, stage, the transfers, the scene's finalize
+    (with the tree's codes and rows), record and end_flush; settle() flushes until nothing is pending (a
+    failed pass retried after the heap grew, a tree rebuilt after its capacity grew)."""
+
+    def __init__(self, group_shift: int, encoding: int, min_emission: float = 0.0, heap_bytes: int = 0):
+        self.lib = load()
+        self.handle = ctypes.c_void_p()
+        opts = CxLightOptions(1, float(min_emission), int(heap_bytes))
+        check(self.lib.cx_light_tree_create(int(group_shift), int(encoding), ctypes.byref(opts),
+                                            ctypes.byref(self.handle)), "cx_light_tree_create")
+        self.frame = 0
+
+    def __del__(self):
+        lib = getattr(self, "lib", None)
+        if lib is not None and getattr(self, "handle", None) and self.handle.value:
+            lib.cx_light_tree_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def hook(self, name: str, *args) -> None:
+        check(getattr(self.lib, "cx_light_tree_" + name)(self.handle, *args), "cx_light_tree_" + name)
+
+    @property
+    def descriptor(self) -> int:
+        return int(self.lib.cx_light_tree_descriptor(self.handle) or 0)
+
+    @property
+    def light_codes(self) -> int:
+        return int(self.lib.cx_light_tree_light_codes(self.handle) or 0)
+
+    @property
+    def group_rows(self) -> int:
+        return int(self.lib.cx_light_tree_group_rows(self.handle) or 0)
+
+    def rebuild(self, tree: int = 0, stream=None) -> None:
+        """The gather and build of one tree now (kernel launches only)."""
+        check(self.lib.cx_light_tree_rebuild(self.handle, int(tree), stream_handle(stream)), "cx_light_tree_rebuild")
+
+    def pending(self) -> bool:
+        return bool(self.lib.cx_light_tree_pending(self.handle))
+
+    def stats(self) -> dict:
+        st = CxLightStats()
+        self.lib.cx_light_tree_get_stats(self.handle, ctypes.byref(st))
+        return st.as_dict()
+
+    def flush(self, scene: CxScene, materials: np.ndarray, tables: CxLightSceneTables, finalize: bool = True,
+              stream=None) -> None:
+        lib, h = self.lib, stream_handle(stream)
+        self.frame += 1
+        requeue = ctypes.c_uint32(0)
+        mats = np.ascontiguousarray(materials, np.float32)
+        check(lib.cx_light_tree_prepare(self.handle, mats.ctypes.data, ctypes.byref(tables), ctypes.byref(requeue)),
+              "cx_light_tree_prepare")
+        finalize = finalize or requeue.value != 0
+        staged = ctypes.c_uint32(0)
+        check(lib.cx_light_tree_stage(self.handle, 1 if finalize else 0, ctypes.byref(staged)), "cx_light_tree_stage")
+        check(lib.cx_light_tree_record_transfers(self.handle, h), "cx_light_tree_record_transfers")
+        scene.light_codes = self.light_codes
+        scene.light_group_rows = self.group_rows
+        if finalize:
+            check(lib.cx_scene_finalize(ctypes.byref(scene), h), "cx_scene_finalize")
+        check(lib.cx_light_tree_record(self.handle, ctypes.byref(scene), self.frame, h), "cx_light_tree_record")
+        check(lib.cx_light_tree_end_flush(self.handle, self.frame), "cx_light_tree_end_flush")
+
+    def recycle(self) -> None:
+        """Waits for the device, then recycles every flushed frame."""
+        _torch().cuda.synchronize()
+        check(self.lib.cx_light_tree_recycle(self.handle, self.frame), "cx_light_tree_recycle")
+
+    def settle(self, scene: CxScene, materials: np.ndarray, tables: CxLightSceneTables, finalize: bool = True,
+               max_flushes: int = 16, stream=None) -> int:
+        """flush + recycle until nothing is pending; returns the flushes made."""
+        for i in range(max_flushes):
+            self.flush(scene, materials, tables, finalize=finalize and i == 0, stream=stream)
+            self.recycle()
+            if not self.pending():
+                return i + 1
+        raise CxError("the light tree still has work after %d flushes" % max_flushes)
+
+
+def light_sample(view: CxLightTreeView, receivers: np.ndarray, seeds: np.ndarray, stream=None):
+    torch = _torch()
+    lib = load()
+    n = len(receivers)
+    r = torch.from_numpy(np.ascontiguousarray(receivers, np.float32)).cuda()
+    sd = torch.from_numpy(np.ascontiguousarray(seeds, np.uint32).view(np.int32)).cuda()
+    out = torch.empty(max(n, 1) * 12, dtype=torch.float32, device="cuda")
+    ok = torch.empty(max(n, 1), dtype=torch.int32, device="cuda")
+    check(lib.cx_light_sample(ctypes.byref(view), r.data_ptr(), sd.data_ptr(), n, out.data_ptr(), ok.data_ptr(),
+                              stream_handle(stream)), "cx_light_sample")
+    return out.cpu().numpy().reshape(-1, 12)[:n], ok.cpu().numpy()[:n]
+
+
+def light_point_pdf(view: CxLightTreeView, queries: np.ndarray, stream=None) -> np.ndarray:
+    torch = _torch()
+    lib = load()
+    n = len(queries)
+    q = torch.from_numpy(np.ascontiguousarray(queries, np.uint32).view(np.int32)).cuda()
+    pdf = torch.empty(max(n, 1), dtype=torch.float32, device="cuda")
+    check(lib.cx_light_point_pdf(ctypes.byref(view), q.data_ptr(), n, pdf.data_ptr(), stream_handle(stream)),
+          "cx_light_point_pdf")
+    return pdf.cpu().numpy()[:n]
+
+
+def point_queries(queries: np.ndarray, group_of_brick: np.ndarray, group_coords: np.ndarray, group_shift: int):
+    """Face queries {x, n, brick record, world voxel, face, block} (12 u32) as LightTreePointPdf queries
+    {x, n, group, group-local voxel, face, block} (identity entity frames)."""
+    q = np.ascontiguousarray(queries, np.uint32).copy()
+    group = np.asarray(group_of_brick, np.uint32)[q[:, 6]]
+    origin = group_coords[group][:, -3:].astype(np.int64) * (8 << group_shift)
+    q[:, 6] = group
+    q[:, 7:10] = (q[:, 7:10].view(np.int32).astype(np.int64) - origin).astype(np.uint32)
+    return q
+
+
+def light_tables(descriptor: int, groups: int, entities: int) -> dict:
+    """The tables a light tree descriptor names, read back as uint8 arrays: descriptor, control, both heaps
+    (to the end of the rows' blocks), rows, motion words, light codes, nodes and leaves of both trees (to
+    their headers' counts)."""
+    d = read_device(descriptor, 128).view(layout.DESCRIPTOR)[0]
+    control = read_device(int(d["control"]), 320).view(layout.CONTROL)[0]
+    rows = read_device(int(d["group_rows"]), 32 * groups).view(layout.ROW)
+    units = int(((rows["block_offset"].astype(np.int64) + rows["block_units"]).max()) if groups else 0)
+    out = {
+        "descriptor": read_device(descriptor, 128),
+        "control": read_device(int(d["control"]), 320),
+        "rows": rows.view(np.uint8).copy(),
+        "motion": read_device(int(d["entity_motion"]), 4 * entities),
+        "codes": read_device(int(d["light_codes"]), 8192),
+        "heap0": read_device(int(d["heap"][0]), 16 * units),
+        "heap1": read_device(int(d["heap"][1]), 16 * units),
+    }
+    for t in range(2):
+        h = control["headers"][t]
+        st = control["states"][t]
+        nodes = int(h["node_count"]) if int(h["root"]) != layout.LT_NONE else 0
+        out[f"nodes{t}"] = read_device(int(d["nodes"][t]), 224 * max(nodes, 1))
+        out[f"leaves{t}"] = read_device(int(d["leaves"][t]), 64 * max(int(h["leaf_count"]), 1))
+    return out
+
+
 class TraceScene:
     """A scene resident on the GPU, ready to trace: bricks, bulk hash table, finalized record words,
     resident bitmaps, brick light records, and the light tree over the light bricks (tree 0; tree 1
@@ -540,56 +820,25 @@ class TraceScene:
         sc.emission_scale = 1.0
         sc.occupied_box = self.occupied_box.data_ptr()
         sc.materials_ext = None if self.materials_ext is None else self.materials_ext.data_ptr()
-        h = stream_handle(stream)
-        check(lib.cx_scene_finalize(ctypes.byref(sc), h), "cx_scene_finalize")
-        # Light pass, growing the heap when it does not fit (as the reference grows its light heap).
-        self.light_scratch = device_bytes(int(lib.cx_scene_light_scratch_bytes(n)), dev)
-        self.record_offset = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
-        self.ordinal_of_brick = torch.zeros(max(n, 1), dtype=torch.int32, device=dev)
-        self.totals = torch.zeros(4, dtype=torch.int32, device=dev)
-        heap_units = max(64, n * 8)
-        while True:
-            leaf_capacity = max(64, (n + 63) // 64 * 64)
-            self.headers = torch.zeros(32, dtype=torch.int32, device=dev)  # 2 x LightTreeHeader
-            self.tree = LightBvhBuilder(leaf_capacity, header=self.headers[:16].view(torch.uint8))
-            self.heap = torch.zeros(heap_units * 4, dtype=torch.int32, device=dev)
-            ls = self.light = CxLightScene()
-            ls.heap = self.heap.data_ptr()
-            ls.heap_units = heap_units
-            ls.leaf_capacity = leaf_capacity
-            ls.record_offset = self.record_offset.data_ptr()
-            ls.ordinal_of_brick = self.ordinal_of_brick.data_ptr()
-            ls.leaves = self.tree.leaves.data_ptr()
-            ls.state = self.tree.state.data_ptr()
-            ls.totals = self.totals.data_ptr()
-            ls.scratch = self.light_scratch.data_ptr()
-            check(lib.cx_scene_light_records(ctypes.byref(sc), ctypes.byref(ls), h), "cx_scene_light_records")
-            totals = self.totals.cpu().numpy().view(np.uint32)
-            if totals[3] == 0:
-                break
-            heap_units = int(totals[1]) + 64
-        self.light_bricks = int(totals[0])
-        self.light_faces = int(totals[2])
-        # The light tree over the light bricks (building = 1, leaves_found written by the light pass).
-        st = to_host(self.tree.state).view(layout.STATE).copy()
-        st["building"] = 1
-        self.tree.state.copy_(to_device(st.view(np.uint8), self.tree.state.device))
-        self.tree.build(stream)
-        empty = np.zeros(1, layout.HEADER)
-        empty["root"] = layout.LT_NONE
-        self.headers[16:].copy_(torch.from_numpy(empty.view(np.int32).copy()).to(dev))
-        self.empty_nodes = torch.zeros(56, dtype=torch.int32, device=dev)
-        self.empty_leaves = torch.zeros(16, dtype=torch.int32, device=dev)
+        # the light tree: one static entity at the identity frame, its groups, the tree's flushes
+        coords, gob = groups_of(data.keys, int(data.group_shift))
+        self.group_coords = coords
+        self.group_of_brick_host = gob
+        self.group_of_brick = torch.from_numpy(gob.view(np.int32)).to(dev)
+        self.group_infos = torch.from_numpy(group_infos(coords).view(np.uint8)).to(dev)
+        self.entity_infos = torch.from_numpy(entity_infos(1).view(np.uint8)).to(dev)
+        sc.group_of_brick = self.group_of_brick.data_ptr()
+        self.lights = LightTree(int(data.group_shift), sc.encoding, min_emission_for(data))
+        self.lights.hook("on_entity_added", 0, 0)
+        for g in range(len(coords)):
+            self.lights.hook("on_group_created", g)
+        self.tables = CxLightSceneTables(self.group_infos.data_ptr(), self.entity_infos.data_ptr(), len(coords), 1, n)
+        self.flushes = self.lights.settle(sc, data.materials, self.tables, stream=stream)
+        st = self.lights.stats()
+        self.light_bricks = int(st["light_bricks"])
+        self.light_faces = int(st["light_faces"])
         v = self.view = CxLightTreeView()
-        v.headers = self.headers.data_ptr()
-        v.nodes[0] = self.tree.nodes.data_ptr()
-        v.nodes[1] = self.empty_nodes.data_ptr()
-        v.leaves[0] = self.tree.leaves.data_ptr()
-        v.leaves[1] = self.empty_leaves.data_ptr()
-        v.heap = self.heap.data_ptr()
-        v.record_offset = self.record_offset.data_ptr()
-        v.ordinal_of_brick = self.ordinal_of_brick.data_ptr()
-        v.group_shift = int(data.group_shift)
+        v.descriptor = self.lights.descriptor
         v.mis = 1 if mis else 0
         torch.cuda.synchronize()
 
@@ -654,24 +903,31 @@ class TraceScene:
         return hits.cpu().numpy().view(np.uint32).reshape(-1, 9)[:n]
 
     def light_sample(self, receivers: np.ndarray, seeds: np.ndarray):
-        torch = _torch()
-        n = len(receivers)
-        r = torch.from_numpy(np.ascontiguousarray(receivers, np.float32)).cuda()
-        sd = torch.from_numpy(np.ascontiguousarray(seeds, np.uint32).view(np.int32)).cuda()
-        out = torch.empty(max(n, 1) * 9, dtype=torch.float32, device="cuda")
-        ok = torch.empty(max(n, 1), dtype=torch.int32, device="cuda")
-        check(self.lib.cx_light_sample(ctypes.byref(self.scene), ctypes.byref(self.view), r.data_ptr(), sd.data_ptr(),
-                                       n, out.data_ptr(), ok.data_ptr(), stream_handle()), "cx_light_sample")
-        return out.cpu().numpy().reshape(-1, 9)[:n], ok.cpu().numpy()[:n]
+        """LightTreeSample per receiver (x, n): ([n, 12] float32 = position, normal, pdf, block, face, group,
+        entry, voxel in group (the last five as u32 bits), ok [n])."""
+        return light_sample(self.view, receivers, seeds)
+
+    def light_point_pdf(self, queries: np.ndarray) -> np.ndarray:
+        """LightTreePointPdf per query (x, n, group, voxel in group, face, block)."""
+        return light_point_pdf(self.view, queries)
 
     def light_face_pdf(self, queries: np.ndarray) -> np.ndarray:
-        torch = _torch()
-        n = len(queries)
-        q = torch.from_numpy(np.ascontiguousarray(queries, np.uint32).view(np.int32)).cuda()
-        pdf = torch.empty(max(n, 1), dtype=torch.float32, device="cuda")
-        check(self.lib.cx_light_face_pdf(ctypes.byref(self.scene), ctypes.byref(self.view), q.data_ptr(), n,
-                                         pdf.data_ptr(), stream_handle()), "cx_light_face_pdf")
-        return pdf.cpu().numpy()[:n]
+        """The face pdf of queries that name a brick record and a world voxel (words 6..9): converted to
+        the group and group-local voxel of LightTreePointPdf."""
+        return self.light_point_pdf(point_queries(queries, self.group_of_brick_host, self.group_coords,
+                                                  int(self.data.group_shift)))
+
+    def light_tables(self) -> dict:
+        """The light tree's tables read back through its descriptor (numpy), for the oracle."""
+        return light_tables(self.lights.descriptor, len(self.group_coords), 1)
+
+    def tree_read(self, tree: int = 0) -> dict:
+        """Tree `tree` as read back: header (HEADER), nodes (NODE) and leaves (LEAF) to their counts."""
+        _torch().cuda.synchronize()
+        t = self.light_tables()
+        control = t["control"].view(layout.CONTROL)[0]
+        return {"header": control["headers"][tree].copy(), "nodes": t["nodes%d" % tree].view(layout.NODE),
+                "leaves": t["leaves%d" % tree].view(layout.LEAF)}
 
     def host_arrays(self) -> dict:
         """Every device array of the scene and its light tree, copied to numpy (for the oracle)."""
@@ -687,11 +943,8 @@ class TraceScene:
             "super_bits": self.super_bits.cpu().numpy().view(np.uint32),
             "materials": self.materials.cpu().numpy(),
             "light_codes": self.light_codes.cpu().numpy().view(np.uint32),
-            "heap": self.heap.cpu().numpy().view(np.uint32),
-            "record_offset": self.record_offset.cpu().numpy().view(np.uint32),
-            "ordinal_of_brick": self.ordinal_of_brick.cpu().numpy().view(np.uint32),
-            "headers": self.headers.cpu().numpy().view(np.uint8),
-            "nodes": self.tree.nodes.cpu().numpy().view(np.uint8),
-            "leaves": self.tree.leaves.cpu().numpy().view(np.uint8),
-            "totals": self.totals.cpu().numpy().view(np.uint32),
+            "group_of_brick": self.group_of_brick_host.copy(),
+            "group_infos": self.group_infos.cpu().numpy(),
+            "entity_infos": self.entity_infos.cpu().numpy(),
+            **{"lt_" + k: v for k, v in self.light_tables().items()},
         }

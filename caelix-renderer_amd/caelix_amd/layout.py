@@ -61,3 +61,32 @@ def scratch_u32(scratch: np.ndarray, offset: int, capacity: int) -> np.ndarray:
 
 def scratch_aggregates(scratch: np.ndarray, capacity: int) -> np.ndarray:
     return scratch[SCRATCH_AGGREGATES * capacity:(SCRATCH_AGGREGATES + 48) * capacity].view(SUMMARY)
+
+
+# Block value encodings (record_layout.h:27-34): packed and block-ID, opaque bits and id masks.
+ENCODING_PACKED = 0
+ENCODING_BLOCK_ID = 1
+PACKED_OPAQUE_MASK = 0xC000
+PACKED_ID_MASK = 0xFF
+BLOCK_ID_OPAQUE_MASK = 0x8000
+BLOCK_ID_ID_MASK = 0x1FF
+
+# The light tree's tables (light_tree_layout.h:138-389).
+CONTROL = np.dtype([("current", "<u4"), ("pass_result", "<u4"), ("needed_units", "<u4"), ("pad", "<u4"),
+                    ("light_groups", "<u4"), ("bricks_dropped", "<u4"), ("light_bricks", "<u8"), ("light_faces", "<u8"),
+                    ("passes", "<u4"), ("failed_passes", "<u4"), ("pass_flags", "<u4"), ("records_recomputed", "<u4"),
+                    ("records_copied", "<u4"), ("trees_changed", "<u4"), ("headers", HEADER, (2,)),
+                    ("states", STATE, (2,))])
+assert CONTROL.itemsize == 320 and CONTROL.fields["headers"][1] == 64 and CONTROL.fields["states"][1] == 192
+ROW = np.dtype([("block_offset", "<u4"), ("block_units", "<u4"), ("generation", "<u4"), ("gpu_flags", "<u4"),
+                ("cpu_flags", "<u4"), ("bricks", "<u4"), ("light_faces", "<u4"), ("pad", "<u4")])
+assert ROW.itemsize == 32
+DESCRIPTOR = np.dtype([("control", "<u8"), ("heap", "<u8", (2,)), ("group_rows", "<u8"), ("group_infos", "<u8"),
+                       ("entity_infos", "<u8"), ("entity_motion", "<u8"), ("grids", "<u8"), ("nodes", "<u8", (2,)),
+                       ("leaves", "<u8", (2,)), ("light_codes", "<u8"), ("tree_headers", "<u8"), ("group_shift", "<u4"),
+                       ("encoding", "<u4"), ("pad", "<u4"), ("group_count", "<u4")])
+assert DESCRIPTOR.itemsize == 128 and DESCRIPTOR.fields["group_count"][1] == 124
+PASS_NO_CHANGE, PASS_OK, PASS_FAILED = 2, 1, 0
+CPU_REBUILD, CPU_RELEASED, CPU_FRESH = 1, 2, 4
+GPU_LIGHT_CHANGED = 1
+RECORD_STALE_BIT = 1 << 30

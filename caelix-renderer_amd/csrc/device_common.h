@@ -113,6 +113,8 @@ CX_DEV T BlockExclusiveSum(T v, T* lds, T& total)
     return before + inclusive - v;
 }
 
+// 30-bit Morton code of a cell with 10-bit coordinates (LtPart1By2 / LtMorton,
+// caelix_light_tree_layout.hlsl:248-261).
 CX_DEV uint32_t Part1By2(uint32_t v)
 {
     v &= 0x3FFu;

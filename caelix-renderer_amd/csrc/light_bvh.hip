@@ -1767,12 +1767,14 @@ __global__ void __launch_bounds__(kWideThreads) WideKernel(Set set)
             }
             else if (word < b.node_capacity)
             {
-                *reinterpret_cast<uint2*>(&b.nodes[word].parent) = make_uint2(w, s);
+                // parent, parent_slot and the two pad words (zero: the node's bytes do not depend on what
+                // the buffer held)
+                *reinterpret_cast<uint4*>(&b.nodes[word].parent) = make_uint4(w, s, 0u, 0u);
             }
         }
         if (w == 0)
         {
-            *reinterpret_cast<uint2*>(&b.nodes[0].parent) = make_uint2(kLtNone, 0u);  // the root
+            *reinterpret_cast<uint4*>(&b.nodes[0].parent) = make_uint4(kLtNone, 0u, 0u, 0u);  // the root
         }
     }
     __syncthreads();

@@ -3,12 +3,12 @@
 // leaf, the in-brick draw of a light face, and the pdf walk up the parent links. Layouts are the
 // reference's (LightTreeHeader, LightBvhNode, LightBvhLeaf, brick light records).
 //
-// The leaf -> brick light record indirection of this build: LightBvhLeaf::entry is the brick's light
-// ordinal, whose record starts at heap unit record_offset[ordinal]; a brick record's ordinal is
-// ordinal_of_brick[record]. (The reference goes leaf -> (group, index entry) -> group row -> block ->
-// record, caelix_light_tree.hlsl:600-680; the records themselves are the same.) Entities are the
-// identity frame.
+// A leaf names its brick as (group, index entry); the group's row gives its light block in the current
+// heap buffer, whose index entry gives the record (caelix_light_tree.hlsl:575-747). Receivers are taken
+// into the group's local voxel frame through its entity's frame (R^T (x - t) - origin) and drawn points
+// back to the world (R local + t).
 #pragma once
+#include "light_block.h"
 #include "trace_device.h"
 
 namespace cx {
@@ -332,7 +332,7 @@ CX_DEV float LeafProbability(const LightView& L, uint32_t k, uint32_t leaf, V3 x
     return 0.0f;
 }
 
-// A brick light record (the reference's BrickLightRecordHeader + entries).
+// A brick light record (BrickLightRecordHeader + entries, light_tree_layout.h:296-334).
 struct Record
 {
     float face[6];
@@ -345,27 +345,38 @@ struct Record
     const uint32_t* entries;
 };
 
-CX_DEV Record LoadRecord(const LightView& L, uint32_t ordinal)
+// A group's light block in the current heap buffer, or null when the group has none (LtGroupRow,
+// caelix_light_tree.hlsl:339-343: row words block_offset and block_units).
+CX_DEV const uint32_t* GroupBlock(const LightView& L, uint32_t group)
 {
-    const uint32_t* r = L.heap + uint64_t(L.record_offset[ordinal]) * 4;
+    const uint2 row = *reinterpret_cast<const uint2*>(&L.rows[group].block_offset);
+    return row.y != 0u ? L.heap + uint64_t(row.x) * 4u : nullptr;
+}
+
+// LtLoadEntryRecord (caelix_light_tree_layout.hlsl:377-401): the record of index entry `entry`.
+CX_DEV Record LoadRecord(const uint32_t* block, uint32_t entry)
+{
+    const uint32_t* r = block + uint64_t(IndexEntry(block, entry)->record_offset) * 4u;
+    const uint4 a = *reinterpret_cast<const uint4*>(r);
+    const uint4 b = *reinterpret_cast<const uint4*>(r + 4);
+    const uint4 c = *reinterpret_cast<const uint4*>(r + 8);
     Record rec;
-#pragma unroll
-    for (int f = 0; f < 6; ++f)
-    {
-        rec.face[f] = __uint_as_float(r[f]);
-    }
-    const uint32_t box_lo = r[6];
-    const uint32_t box_hi = r[7];
-    rec.lo[0] = box_lo & kLtVoxelMask;
-    rec.lo[1] = (box_lo >> kLtVoxelBits) & kLtVoxelMask;
-    rec.lo[2] = (box_lo >> (2 * kLtVoxelBits)) & kLtVoxelMask;
-    rec.hi[0] = box_hi & kLtVoxelMask;
-    rec.hi[1] = (box_hi >> kLtVoxelBits) & kLtVoxelMask;
-    rec.hi[2] = (box_hi >> (2 * kLtVoxelBits)) & kLtVoxelMask;
-    rec.pos = r[8];
-    rec.entry_count = r[9];
-    rec.counts_lo = r[10];
-    rec.counts_hi = r[11];
+    rec.face[0] = __uint_as_float(a.x);
+    rec.face[1] = __uint_as_float(a.y);
+    rec.face[2] = __uint_as_float(a.z);
+    rec.face[3] = __uint_as_float(a.w);
+    rec.face[4] = __uint_as_float(b.x);
+    rec.face[5] = __uint_as_float(b.y);
+    rec.lo[0] = b.z & kLtVoxelMask;
+    rec.lo[1] = (b.z >> kLtVoxelBits) & kLtVoxelMask;
+    rec.lo[2] = (b.z >> (2 * kLtVoxelBits)) & kLtVoxelMask;
+    rec.hi[0] = b.w & kLtVoxelMask;
+    rec.hi[1] = (b.w >> kLtVoxelBits) & kLtVoxelMask;
+    rec.hi[2] = (b.w >> (2 * kLtVoxelBits)) & kLtVoxelMask;
+    rec.pos = c.x;
+    rec.entry_count = c.y;
+    rec.counts_lo = c.z;
+    rec.counts_hi = c.w;
     rec.entries = r + 12;
     return rec;
 }
@@ -402,10 +413,63 @@ CX_DEV void FamilyWeights(const Record& r, V3 x, float w[6], float& sum)
     }
 }
 
-CX_DEV V3 GroupOrigin(const SceneView& s, const LightView& L, int32_t bx, int32_t by, int32_t bz)
+// An entity's current frame: object-to-world rotation rows and translation (LtEntityFrame,
+// caelix_light_tree_layout.hlsl:412-425).
+struct Frame
 {
-    const int32_t g = int32_t(L.group_shift);
-    return Make(float((bx >> g) << g) * 8.0f, float((by >> g) << g) * 8.0f, float((bz >> g) << g) * 8.0f);
+    float r[3][3];
+    V3 t;
+};
+
+// LtGroupFrame (caelix_light_tree.hlsl:346-353) + LtGroupOrigin (caelix_light_tree_layout.hlsl:428-433):
+// the group's entity-local voxel origin, and its entity's frame.
+CX_DEV V3 GroupFrame(const LightView& L, uint32_t group, Frame& frame)
+{
+    const int4 w0 = *reinterpret_cast<const int4*>(L.group_infos[group].coord);
+    const int32_t edge = int32_t(8u << L.group_shift);
+    const EntityInfo* e = L.entity_infos + uint32_t(w0.w);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        const float4 row = *reinterpret_cast<const float4*>(e->cur_rows[i]);
+        frame.r[i][0] = row.x;
+        frame.r[i][1] = row.y;
+        frame.r[i][2] = row.z;
+        Set(frame.t, i, row.w);
+    }
+    return Make(float(w0.x * edge), float(w0.y * edge), float(w0.z * edge));
+}
+
+// R^T (x - t) - origin: a world point in the group's local voxel frame.
+CX_DEV V3 ToGroup(const Frame& f, V3 x, V3 origin)
+{
+    const V3 d = Sub(x, f.t);
+    const V3 local = Make(d.x * f.r[0][0] + d.y * f.r[1][0] + d.z * f.r[2][0],
+                          d.x * f.r[0][1] + d.y * f.r[1][1] + d.z * f.r[2][1],
+                          d.x * f.r[0][2] + d.y * f.r[1][2] + d.z * f.r[2][2]);
+    return Sub(local, origin);
+}
+
+// R v
+CX_DEV V3 Rotate(const Frame& f, V3 v)
+{
+    return Make(f.r[0][0] * v.x + f.r[0][1] * v.y + f.r[0][2] * v.z,
+                f.r[1][0] * v.x + f.r[1][1] * v.y + f.r[1][2] * v.z,
+                f.r[2][0] * v.x + f.r[2][1] * v.y + f.r[2][2] * v.z);
+}
+
+// LtBrickLeaf (caelix_light_tree.hlsl:594-607): the brick's leaf word (tree << 31 | leaf) when that leaf
+// points back at (group, entry), else kLtNone.
+CX_DEV uint32_t BrickLeaf(const LightView& L, const uint32_t* block, uint32_t group, uint32_t entry)
+{
+    const uint32_t word = IndexEntry(block, entry)->bvh_leaf;
+    if (word == kLtNone)
+    {
+        return kLtNone;
+    }
+    const LightBvhLeaf& leaf = L.leaves[word >> kLtBvhLeafTreeShift][word & kLtBvhLeafIndexMask];
+    const uint2 owner = *reinterpret_cast<const uint2*>(&leaf.group);
+    return owner.x == group && owner.y == entry ? word : kLtNone;
 }
 
 struct LightPoint
@@ -413,9 +477,13 @@ struct LightPoint
     V3 position;
     V3 normal;
     uint32_t block;
+    uint32_t voxel;   // the voxel in its brick (x + 8y + 64z)
     uint32_t face;
     float pdf;
-    uint32_t record;  // the brick record of the light
+    uint32_t group;   // the brick: group, index entry
+    uint32_t entry;
+    uint32_t record;  // the brick's record in the brick store (the index entry's dense_id)
+    uint32_t voxel_in_group;  // group-local voxel, x | y << 10 | z << 20
 };
 
 CX_DEV V3 FaceDirection(uint32_t f)
@@ -425,15 +493,19 @@ CX_DEV V3 FaceDirection(uint32_t f)
     return Make(axis == 0 ? sign : 0.0f, axis == 1 ? sign : 0.0f, axis == 2 ? sign : 0.0f);
 }
 
-// LightTreeSample (:749-767) = LightTreeSampleBrick + LightBrickSamplePoint (:647-747).
-CX_DEV bool LightTreeSample(const SceneView& s, const LightView& L, V3 x, V3 n, uint32_t& rng, LightPoint& p)
+// LightTreeSample (:749-767) = LightTreeSampleBrick (:647-670) + LightBrickSamplePoint (:672-747).
+CX_DEV bool LightTreeSample(const LightView& L, V3 x, V3 n, uint32_t& rng, LightPoint& p)
 {
     p.position = Make(0.0f, 0.0f, 0.0f);
     p.normal = Make(0.0f, 0.0f, 0.0f);
     p.block = 0;
+    p.voxel = 0;
     p.face = 0;
     p.pdf = 0.0f;
+    p.group = kLtNone;
+    p.entry = kLtNone;
     p.record = kLtNone;
+    p.voxel_in_group = 0;
     float pdf_brick = 1.0f;
     uint32_t tree;
     const uint32_t leaf = ChooseLeaf(L, x, n, rng, pdf_brick, tree);
@@ -441,13 +513,19 @@ CX_DEV bool LightTreeSample(const SceneView& s, const LightView& L, V3 x, V3 n, 
     {
         return false;
     }
-    const LightBvhLeaf& lf = L.leaves[tree][leaf];
-    const uint32_t brick_record = lf.group;
-    const uint32_t ordinal = lf.entry;
-    const Record record = LoadRecord(L, ordinal);
-    const int32_t* key = BrickXyz(s, brick_record);
-    const V3 origin = GroupOrigin(s, L, key[0], key[1], key[2]);
-    const V3 x_group = Sub(x, origin);
+    const uint2 owner = *reinterpret_cast<const uint2*>(&L.leaves[tree][leaf].group);
+    p.group = owner.x;
+    p.entry = owner.y;
+    const uint32_t* block = GroupBlock(L, owner.x);
+    if (block == nullptr)
+    {
+        return false;
+    }
+    const Record record = LoadRecord(block, owner.y);
+    p.record = IndexEntry(block, owner.y)->dense_id;
+    Frame frame;
+    const V3 origin = GroupFrame(L, owner.x, frame);
+    const V3 x_group = ToGroup(frame, x, origin);
     float w[6];
     float sum;
     FamilyWeights(record, x_group, w, sum);
@@ -486,48 +564,63 @@ CX_DEV bool LightTreeSample(const SceneView& s, const LightView& L, V3 x, V3 n, 
     const uint32_t entry = record.entries[entry_first + entry_index];
     const uint32_t vi = entry & kLtFaceEntryVoxelMask;
     const uint32_t mask = (1u << L.group_shift) - 1u;
-    const uint32_t bx = record.pos & mask;
-    const uint32_t by = (record.pos >> L.group_shift) & mask;
-    const uint32_t bz = record.pos >> (2 * L.group_shift);
-    V3 local = Add(origin, Make(float(bx * 8 + (vi & 7)), float(by * 8 + ((vi >> 3) & 7)), float(bz * 8 + (vi >> 6))));
+    const uint32_t vx = (record.pos & mask) * 8 + (vi & 7);
+    const uint32_t vy = ((record.pos >> L.group_shift) & mask) * 8 + ((vi >> 3) & 7);
+    const uint32_t vz = (record.pos >> (2 * L.group_shift)) * 8 + (vi >> 6);
+    V3 local = Add(origin, Make(float(vx), float(vy), float(vz)));
     const float a = LightRandomFloat(rng);
     const float b = LightRandomFloat(rng);
     const int axis = int(f >> 1);
     Set(local, axis, Get(local, axis) + ((f & 1) != 0 ? 0.0f : 1.0f));
     Set(local, (axis + 1) % 3, Get(local, (axis + 1) % 3) + a);
     Set(local, (axis + 2) % 3, Get(local, (axis + 2) % 3) + b);
-    p.position = local;
-    p.normal = FaceDirection(f);
+    p.position = Add(Rotate(frame, local), frame.t);
+    p.normal = Rotate(frame, FaceDirection(f));
     p.block = entry >> kLtFaceEntryBlockShift;
+    p.voxel = vi;
     p.face = f;
-    p.record = brick_record;
+    p.voxel_in_group = vx | (vy << kLtVoxelBits) | (vz << (2 * kLtVoxelBits));
     p.pdf = Div(Div(w[f], sum), float(count)) * pdf_brick;
     return true;
 }
 
-// LightTreeFindLightFace (:853-932) + LightTreeBrickPdf: the area pdf with which LightTreeSample
-// draws the point on face `face` of voxel `voxel` (world voxel coordinates) of brick `record`, for the
-// receiver (x, n); 0 when it is not a light face the tree can draw. `found` = a light face.
-CX_DEV float LightFacePdf(const SceneView& s, const LightView& L, V3 x, V3 n, uint32_t record, const int32_t voxel[3],
-                          uint32_t face, uint32_t block, bool& found)
+// LightTreePointPdf (:936-949) = LightTreeFindLightFace (:853-932) + LtBrickProbability: the area pdf
+// with which LightTreeSample draws the point on face `face` of the group-local voxel `v` of `group`,
+// whose raw block value is `block`, for the receiver (x, n); 0 when it is not a light face the tree can
+// draw. `found` = a light face of the tree whose brick has a leaf (LightTreeFindLightFace and
+// leaf != kLtNone, the condition of LampHitWeight, caelix_path_trace.hlsl:438-443).
+CX_DEV float LightPointPdf(const LightView& L, V3 x, V3 n, uint32_t group, const uint32_t v[3], uint32_t face,
+                           uint32_t block_value, bool& found)
 {
     found = false;
-    const uint32_t code = block & 0xFFFFu;
-    if (((s.light_codes[code >> 5] >> (code & 31u)) & 1u) == 0u || face > 5u)
+    // BlockMaterialCode (caelix_record.hlsl:209): an opaque value as it is, a transparent one without its
+    // face bits
+    const uint32_t raw = block_value & 0xFFFFu;
+    const bool packed = L.encoding == kEncodingPacked;
+    const uint32_t code = (raw & (packed ? kPackedOpaqueMask : kBlockIdOpaqueMask)) != 0u
+                              ? raw
+                              : (raw & (packed ? kPackedIdMask : kBlockIdIdMask));
+    if (((L.light_codes[code >> 5] >> (code & 31u)) & 1u) == 0u || face > 5u)
     {
         return 0.0f;
     }
-    const uint32_t ordinal = L.ordinal_of_brick[record];
-    if (ordinal == kLtNone)
+    const uint32_t* block = GroupBlock(L, group);
+    if (block == nullptr)
     {
         return 0.0f;
     }
-    const Record rec = LoadRecord(L, ordinal);
-    const int32_t* key = BrickXyz(s, record);
-    const V3 origin = GroupOrigin(s, L, key[0], key[1], key[2]);
+    const uint32_t key = Morton3(v[0] >> 3, v[1] >> 3, v[2] >> 3);
+    const uint32_t entry = FindIndexEntry(block, block[0], key, L.group_shift);
+    if (entry == kLtNone)
+    {
+        return 0.0f;
+    }
+    const Record rec = LoadRecord(block, entry);
+    Frame frame;
+    const V3 origin = GroupFrame(L, group, frame);
     float w[6];
     float sum;
-    FamilyWeights(rec, Sub(x, origin), w, sum);
+    FamilyWeights(rec, ToGroup(frame, x, origin), w, sum);
     const uint32_t count = FamilyCount(rec, face);
     if (sum <= 0.0f || w[face] <= 0.0f || count == 0u)
     {
@@ -538,7 +631,7 @@ CX_DEV float LightFacePdf(const SceneView& s, const LightView& L, V3 x, V3 n, ui
     {
         first += FamilyCount(rec, g);
     }
-    const uint32_t vi = uint32_t((voxel[0] & 7) | ((voxel[1] & 7) << 3) | ((voxel[2] & 7) << 6));
+    const uint32_t vi = (v[0] & 7u) | ((v[1] & 7u) << 3) | ((v[2] & 7u) << 6);
     uint32_t a = 0;
     uint32_t b = count;
     bool hit = false;
@@ -564,17 +657,28 @@ CX_DEV float LightFacePdf(const SceneView& s, const LightView& L, V3 x, V3 n, ui
     {
         return 0.0f;
     }
-    found = true;
     const float in_brick = Div(Div(w[face], sum), float(count));
-    // The brick's leaf: tree 0 (the static tree), leaf = the light ordinal - leaf_base when it is
-    // below the count.
-    const uint32_t leaf = ordinal - L.leaf_base;
-    if (leaf >= L.headers[0].leaf_count)
+    const uint32_t leaf = BrickLeaf(L, block, group, entry);
+    if (leaf == kLtNone)
     {
-        found = false;  // no leaf: LampHitWeight gives the full emission
-        return 0.0f;
+        return 0.0f;  // no draw reaches it: LampHitWeight gives the full emission
     }
-    return LeafProbability(L, 0, leaf, x, n) * in_brick;
+    found = true;
+    return LeafProbability(L, leaf >> kLtBvhLeafTreeShift, leaf & kLtBvhLeafIndexMask, x, n) * in_brick;
+}
+
+// The light face pdf at a hit of the world traversal: brick `record` of the brick store and the world
+// voxel `voxel`. The traversal's bricks belong to entities with the identity frame, so the group-local
+// voxel is the world voxel less the group's origin.
+CX_DEV float LightFacePdf(const SceneView& s, const LightView& L, V3 x, V3 n, uint32_t record, const int32_t voxel[3],
+                          uint32_t face, uint32_t block, bool& found)
+{
+    const uint32_t group = s.group_of_brick[record];
+    const int4 w0 = *reinterpret_cast<const int4*>(L.group_infos[group].coord);
+    const int32_t edge = int32_t(8u << L.group_shift);
+    const uint32_t v[3] = {uint32_t(voxel[0] - w0.x * edge), uint32_t(voxel[1] - w0.y * edge),
+                           uint32_t(voxel[2] - w0.z * edge)};
+    return LightPointPdf(L, x, n, group, v, face, block, found);
 }
 
 // LightNeeMisWeight (caelix_light_nee.hlsl:20-34).

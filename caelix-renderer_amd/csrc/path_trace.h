@@ -460,7 +460,7 @@ CX_DEV bool NeeCandidate(const SceneView& s, const LightView& L, const SkyParams
     unscaled = Make(0.0f, 0.0f, 0.0f);
     light_record = kHashNone;
     LightPoint p;
-    if (LightTreeSample(s, L, x_r, n_r, rng, p) && p.pdf > 0.0f)
+    if (LightTreeSample(L, x_r, n_r, rng, p) && p.pdf > 0.0f)
     {
         const V3 d = Sub(p.position, x_r);
         const float d2 = Dot(d, d);

@@ -38,24 +38,17 @@ SceneView SceneOf(const CxScene& c)
     s.key_words = c.scene_count > 1 ? 4u : 3u;
     s.scene_cells = uint64_t(c.grid_dim) * c.grid_dim * c.grid_dim;
     s.scene_supers = s.scene_cells / 512;
+    s.group_of_brick = c.group_of_brick;
     return s;
 }
 
-LightView LightsOf(const CxLightTreeView& v)
+LightRef LightsOf(const CxLightTreeView& v)
 {
-    LightView l;
-    l.headers = v.headers;
-    l.nodes[0] = v.nodes[0];
-    l.nodes[1] = v.nodes[1];
-    l.leaves[0] = v.leaves[0];
-    l.leaves[1] = v.leaves[1];
-    l.heap = v.heap;
-    l.record_offset = v.record_offset;
-    l.ordinal_of_brick = v.ordinal_of_brick;
-    l.group_shift = v.group_shift;
-    l.mis = v.mis;
-    l.leaf_base = v.leaf_base;
-    return l;
+    LightRef r;
+    r.descriptor = v.descriptor;
+    r.mis = v.mis;
+    r.pad = 0;
+    return r;
 }
 
 CameraView CameraOf(const CxCamera& c)
@@ -207,23 +200,23 @@ Workspace CarveWorkspace(void* base, uint32_t paths)
 // in device memory and their outputs are images of pixels_per_scene pixels, scene-major.
 struct OneScene
 {
-    LightView light;
+    LightRef light;
     CameraView camera;
     CX_DEV uint32_t SceneOf(uint32_t) const { return 0; }
     CX_DEV uint32_t Local(uint32_t path) const { return path; }
-    CX_DEV const LightView& Light(uint32_t) const { return light; }
+    CX_DEV LightView Light(uint32_t) const { return OpenLights(light); }
     CX_DEV const CameraView& Camera(uint32_t) const { return camera; }
     CX_DEV uint64_t PixelBase(uint32_t) const { return 0; }
 };
 struct SceneBatch
 {
-    const LightView* lights;
+    const LightRef* lights;
     const CameraView* cameras;
     uint32_t paths_per_scene;
     uint32_t pixels_per_scene;
     CX_DEV uint32_t SceneOf(uint32_t path) const { return path / paths_per_scene; }
     CX_DEV uint32_t Local(uint32_t path) const { return path % paths_per_scene; }
-    CX_DEV const LightView& Light(uint32_t scene) const { return lights[scene]; }
+    CX_DEV LightView Light(uint32_t scene) const { return OpenLights(lights[scene]); }
     CX_DEV const CameraView& Camera(uint32_t scene) const { return cameras[scene]; }
     CX_DEV uint64_t PixelBase(uint32_t scene) const { return uint64_t(scene) * pixels_per_scene; }
 };
@@ -743,7 +736,7 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
         else
         {
             const uint32_t scene = views.SceneOf(path);
-            const LightView& L = views.Light(scene);
+            const LightView L = views.Light(scene);
             const CameraView& cam = views.Camera(scene);
             uint32_t px, py;
             PathPixel(cam, views.Local(path), px, py);
@@ -877,7 +870,7 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
         }
         if (h.y != 0u)
         {
-            const LightView& L = views.Light(views.SceneOf(path));
+            const LightView L = views.Light(views.SceneOf(path));
             uint4 st = w.state[path];
             uint32_t rng = st.x;
             uint32_t light_rng = st.y;
@@ -1125,7 +1118,7 @@ __global__ void __launch_bounds__(256) RaysKernel(SceneView s, const float* rays
     o[8] = h.steps;
 }
 
-__global__ void __launch_bounds__(256) SampleKernel(SceneView s, LightView l, const float* receivers,
+__global__ void __launch_bounds__(256) SampleKernel(LightRef lights, const float* receivers,
                                                     const uint32_t* seeds, uint32_t count, float* samples,
                                                     uint32_t* ok)
 {
@@ -1134,11 +1127,12 @@ __global__ void __launch_bounds__(256) SampleKernel(SceneView s, LightView l, co
     {
         return;
     }
+    const LightView l = OpenLights(lights);
     const float* r = receivers + uint64_t(i) * 6;
     uint32_t rng = seeds[i];
     LightPoint p;
-    const bool drawn = LightTreeSample(s, l, Make(r[0], r[1], r[2]), Make(r[3], r[4], r[5]), rng, p);
-    float* o = samples + uint64_t(i) * 9;
+    const bool drawn = LightTreeSample(l, Make(r[0], r[1], r[2]), Make(r[3], r[4], r[5]), rng, p);
+    float* o = samples + uint64_t(i) * 12;
     o[0] = p.position.x;
     o[1] = p.position.y;
     o[2] = p.position.z;
@@ -1148,23 +1142,27 @@ __global__ void __launch_bounds__(256) SampleKernel(SceneView s, LightView l, co
     o[6] = p.pdf;
     o[7] = __uint_as_float(p.block);
     o[8] = __uint_as_float(p.face);
+    o[9] = __uint_as_float(p.group);
+    o[10] = __uint_as_float(p.entry);
+    o[11] = __uint_as_float(p.voxel_in_group);
     ok[i] = drawn ? 1u : 0u;
 }
 
-__global__ void __launch_bounds__(256) FacePdfKernel(SceneView s, LightView l, const uint32_t* queries,
-                                                     uint32_t count, float* pdf)
+__global__ void __launch_bounds__(256) PointPdfKernel(LightRef lights, const uint32_t* queries, uint32_t count,
+                                                      float* pdf)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count)
     {
         return;
     }
+    const LightView l = OpenLights(lights);
     const uint32_t* q = queries + uint64_t(i) * 12;
     const V3 x = Make(__uint_as_float(q[0]), __uint_as_float(q[1]), __uint_as_float(q[2]));
     const V3 n = Make(__uint_as_float(q[3]), __uint_as_float(q[4]), __uint_as_float(q[5]));
-    const int32_t voxel[3] = {int32_t(q[7]), int32_t(q[8]), int32_t(q[9])};
+    const uint32_t voxel[3] = {q[7], q[8], q[9]};
     bool found;
-    pdf[i] = LightFacePdf(s, l, x, n, q[6], voxel, q[10], q[11], found);
+    pdf[i] = LightPointPdf(l, x, n, q[6], voxel, q[10], q[11], found);
 }
 
 // LightNeeMisWeight of both sides per query: q = {p_area, d, cos_l, n_r (3), w (3)} -> {w_nee, w_bsdf}.
@@ -1199,10 +1197,9 @@ int CheckScene(const CxScene* s, const char* what)
 
 int CheckLights(const CxLightTreeView* l, const char* what)
 {
-    if (l == nullptr || l->headers == nullptr || l->nodes[0] == nullptr || l->leaves[0] == nullptr || l->heap == nullptr ||
-        l->record_offset == nullptr || l->ordinal_of_brick == nullptr)
+    if (l == nullptr || l->descriptor == nullptr)
     {
-        return ArgError("%s: the light tree view has a NULL buffer", what);
+        return ArgError("%s: no light tree descriptor", what);
     }
     return CX_OK;
 }
@@ -1493,7 +1490,7 @@ int RunFrame(const CxScene* scene, const CxLightTreeView* lights, const CxCamera
 // Per-scene device views of a batch: the light tree views, then the cameras.
 uint64_t BatchViewBytes(uint32_t scenes)
 {
-    return Align256(uint64_t(scenes) * sizeof(LightView)) + uint64_t(scenes) * sizeof(CameraView);
+    return Align256(uint64_t(scenes) * sizeof(LightRef)) + uint64_t(scenes) * sizeof(CameraView);
 }
 
 }  // namespace
@@ -1559,8 +1556,8 @@ CX_API int cx_trace_batch_prepare(const CxLightTreeView* lights, const CxCamera*
         return cx::ArgError("cx_trace_batch_prepare: no scenes or a NULL array");
     }
     std::vector<uint8_t> host(cx::BatchViewBytes(scenes), 0);
-    cx::LightView* lv = reinterpret_cast<cx::LightView*>(host.data());
-    cx::CameraView* cv = reinterpret_cast<cx::CameraView*>(host.data() + cx::Align256(uint64_t(scenes) * sizeof(cx::LightView)));
+    cx::LightRef* lv = reinterpret_cast<cx::LightRef*>(host.data());
+    cx::CameraView* cv = reinterpret_cast<cx::CameraView*>(host.data() + cx::Align256(uint64_t(scenes) * sizeof(cx::LightRef)));
     for (uint32_t i = 0; i < scenes; ++i)
     {
         if (int r = cx::CheckLights(&lights[i], "cx_trace_batch_prepare"); r != CX_OK)
@@ -1611,8 +1608,8 @@ CX_API int cx_trace_batch(const CxScene* scene, const void* device_views, uint32
     cx::FrameOut o{reinterpret_cast<float4*>(out->color), reinterpret_cast<float4*>(out->radiance), out->attrib,
                    reinterpret_cast<unsigned long long*>(out->counters)};
     const uint8_t* base = static_cast<const uint8_t*>(device_views);
-    const cx::SceneBatch views{reinterpret_cast<const cx::LightView*>(base),
-                               reinterpret_cast<const cx::CameraView*>(base + cx::Align256(uint64_t(scenes) * sizeof(cx::LightView))),
+    const cx::SceneBatch views{reinterpret_cast<const cx::LightRef*>(base),
+                               reinterpret_cast<const cx::CameraView*>(base + cx::Align256(uint64_t(scenes) * sizeof(cx::LightRef))),
                                uint32_t(per_scene), width * height};
     if (cx::PersistentBlocks(cx::PersistentTraceKernel<cx::BounceSource<cx::SceneBatch>>) == 0 ||
         cx::PersistentBlocks(cx::PersistentTraceKernel<cx::ShadowSource<cx::SceneBatch>>) == 0)
@@ -1645,15 +1642,14 @@ CX_API int cx_trace_rays(const CxScene* scene, const float* rays, uint32_t count
     return cx::LaunchResult("cx_trace_rays");
 }
 
-CX_API int cx_light_sample(const CxScene* scene, const CxLightTreeView* lights, const float* receivers,
-                           const uint32_t* seeds, uint32_t count, float* samples, uint32_t* ok, void* stream)
+CX_API int cx_light_sample(const CxLightTreeView* lights, const float* receivers, const uint32_t* seeds,
+                           uint32_t count, float* samples, uint32_t* ok, void* stream)
 {
     if (int r = cx::CheckLights(lights, "cx_light_sample"); r != CX_OK)
     {
         return r;
     }
-    if (scene == nullptr || scene->keys == nullptr || receivers == nullptr || seeds == nullptr || samples == nullptr ||
-        ok == nullptr)
+    if (receivers == nullptr || seeds == nullptr || samples == nullptr || ok == nullptr)
     {
         return cx::ArgError("cx_light_sample: a buffer is NULL");
     }
@@ -1663,32 +1659,30 @@ CX_API int cx_light_sample(const CxScene* scene, const CxLightTreeView* lights, 
     }
     const hipStream_t s = static_cast<hipStream_t>(stream);
     cx::StageScope scope(s, "light_sample");
-    cx::SampleKernel<<<cx::DivUp(count, 256), 256, 0, s>>>(cx::SceneOf(*scene), cx::LightsOf(*lights), receivers,
-                                                           seeds, count, samples, ok);
+    cx::SampleKernel<<<cx::DivUp(count, 256), 256, 0, s>>>(cx::LightsOf(*lights), receivers, seeds, count, samples,
+                                                           ok);
     return cx::LaunchResult("cx_light_sample");
 }
 
-CX_API int cx_light_face_pdf(const CxScene* scene, const CxLightTreeView* lights, const uint32_t* queries,
-                             uint32_t count, float* pdf, void* stream)
+CX_API int cx_light_point_pdf(const CxLightTreeView* lights, const uint32_t* queries, uint32_t count, float* pdf,
+                              void* stream)
 {
-    if (int r = cx::CheckLights(lights, "cx_light_face_pdf"); r != CX_OK)
+    if (int r = cx::CheckLights(lights, "cx_light_point_pdf"); r != CX_OK)
     {
         return r;
     }
-    if (scene == nullptr || scene->keys == nullptr || scene->light_codes == nullptr || queries == nullptr ||
-        pdf == nullptr)
+    if (queries == nullptr || pdf == nullptr)
     {
-        return cx::ArgError("cx_light_face_pdf: a buffer is NULL");
+        return cx::ArgError("cx_light_point_pdf: a buffer is NULL");
     }
     if (count == 0)
     {
         return CX_OK;
     }
     const hipStream_t s = static_cast<hipStream_t>(stream);
-    cx::StageScope scope(s, "light_face_pdf");
-    cx::FacePdfKernel<<<cx::DivUp(count, 256), 256, 0, s>>>(cx::SceneOf(*scene), cx::LightsOf(*lights), queries,
-                                                            count, pdf);
-    return cx::LaunchResult("cx_light_face_pdf");
+    cx::StageScope scope(s, "light_point_pdf");
+    cx::PointPdfKernel<<<cx::DivUp(count, 256), 256, 0, s>>>(cx::LightsOf(*lights), queries, count, pdf);
+    return cx::LaunchResult("cx_light_point_pdf");
 }
 
 CX_API int cx_light_mis_weight(const float* queries, uint32_t count, float* weights, void* stream)

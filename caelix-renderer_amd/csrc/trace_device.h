@@ -41,6 +41,7 @@ struct SceneView
     uint32_t key_words;          // 3: keys (x, y, z); 4: (scene, x, y, z), a batch of chunk scenes
     uint64_t scene_cells;        // brick-bitmap bits per scene (grid_dim^3)
     uint64_t scene_supers;       // super-bitmap bits per scene ((grid_dim / 8)^3)
+    const uint32_t* group_of_brick;  // [bricks] group index (the light tree's group rows and infos)
 };
 
 // The (x, y, z) brick coordinates of a record.
@@ -54,18 +55,52 @@ CX_DEV float BrickScale(const SceneView& s, uint32_t record)
     return s.brick_scale != nullptr ? s.brick_scale[record] : 1.0f;
 }
 
+// The light tree as the trace kernels read it, opened from its descriptor (LightTreeOpen /
+// LightTreeView, caelix_light_tree.hlsl:239-287): the tree headers in the control buffer, the nodes and
+// leaves of both trees, the current heap buffer, the group rows, group and entity infos.
 struct LightView
 {
     const LightTreeHeader* headers;  // [2]
     const LightBvhNode* nodes[2];
     const LightBvhLeaf* leaves[2];
-    const uint32_t* heap;             // brick light records, 16-byte units
-    const uint32_t* record_offset;    // light ordinal -> record offset (units)
-    const uint32_t* ordinal_of_brick; // brick record -> light ordinal or kLtNone
-    uint32_t group_shift;             // a group is (1 << group_shift)^3 bricks
+    const uint32_t* heap;            // the current buffer: group light blocks, 16-byte units
+    const GroupLightRow* rows;
+    const GroupInfo* group_infos;
+    const EntityInfo* entity_infos;
+    const uint32_t* light_codes;     // 65536 bits: the block value is a light material
+    uint32_t group_shift;            // a group is (1 << group_shift)^3 bricks
+    uint32_t encoding;               // the block value encoding (kEncoding*)
     uint32_t mis;
-    uint32_t leaf_base;               // light ordinal of leaf 0 of tree 0
 };
+
+// What a kernel receives of a light tree: its descriptor's device address and the MIS switch.
+struct LightRef
+{
+    const LightTreeDescriptor* descriptor;
+    uint32_t mis;
+    uint32_t pad;
+};
+
+CX_DEV LightView OpenLights(const LightRef& r)
+{
+    const LightTreeDescriptor* d = r.descriptor;
+    LightView l;
+    l.headers = reinterpret_cast<const LightTreeHeader*>(d->tree_headers);
+    l.nodes[0] = reinterpret_cast<const LightBvhNode*>(d->nodes[0]);
+    l.nodes[1] = reinterpret_cast<const LightBvhNode*>(d->nodes[1]);
+    l.leaves[0] = reinterpret_cast<const LightBvhLeaf*>(d->leaves[0]);
+    l.leaves[1] = reinterpret_cast<const LightBvhLeaf*>(d->leaves[1]);
+    const uint32_t current = reinterpret_cast<const LightControl*>(d->control)->current & 1u;
+    l.heap = reinterpret_cast<const uint32_t*>(current != 0 ? d->heap[1] : d->heap[0]);
+    l.rows = reinterpret_cast<const GroupLightRow*>(d->group_rows);
+    l.group_infos = reinterpret_cast<const GroupInfo*>(d->group_infos);
+    l.entity_infos = reinterpret_cast<const EntityInfo*>(d->entity_infos);
+    l.light_codes = reinterpret_cast<const uint32_t*>(d->light_codes);
+    l.group_shift = d->group_shift;
+    l.encoding = d->encoding;
+    l.mis = r.mis;
+    return l;
+}
 
 struct CameraView
 {

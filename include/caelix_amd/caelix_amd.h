@@ -237,47 +237,77 @@ typedef struct CxScene {
      * refract by Fresnel) whose extinction attenuates the segments inside them, no NEE at specular
      * vertices. NULL: every block is opaque Lambertian (the benchmarked subset). */
     const float* materials_ext;
+    /* [bricks] the brick's group index (its GroupInfo and GroupLightRow), written by the light tree's
+     * host side: the light pass groups the light bricks by it and the trace finds a hit light face
+     * through it (LightTreeFindLightFace). */
+    const uint32_t* group_of_brick;
+    /* [groups] GroupLightRow, or NULL (FinalizePush::light_group_rows, caelix_finalize.hlsl:52): set, the
+     * finalize keeps the light state of record word 1 (bits 18..30): the open faces of the brick's light
+     * voxels (kLtRecordFaceCountShift) and the stale bit, set with kLtGpuLightChanged in the group's row
+     * when the old or the new count is not 0 (caelix_finalize.hlsl:22-29, 326-340). New bricks start with
+     * word 1 = 0. NULL: those bits are 0. */
+    GroupLightRow* light_group_rows;
 } CxScene;
 
 /* The record producer (shaders/scene/caelix_finalize.hlsl:112-361) over the bricks and their face
  * neighbours (found through the hash table: slots and hash_info are read): face bits of transparent and
  * air voxels written into the payload, word 0 (group position, coarse byte), word 1 bounds and the
  * micro occupancy of the occupied voxels (alive: fewer than six opaque neighbours; not 0; opaque or a
- * face bit); and the resident bitmaps. Running it again changes nothing. */
+ * face bit); and the resident bitmaps; with light_group_rows, the light state of word 1. Running it
+ * again changes nothing but the stale bits and row flags of bricks with light faces. */
 CX_API int cx_scene_finalize(const CxScene* scene, void* stream);
 
-/* The light pass (the group pass of shaders/light/records/*.hlsl for one entity, plus the gather of
- * caelix_light_gather_emit.hlsl): light face counts into word 1, brick light records in the heap
- * (BrickLightRecordHeader + entries), one LightBvhLeaf per light brick (ordinal order) with
- * group = brick record, entry = light ordinal; state->leaves_found = light bricks. totals[0] light
- * bricks, [1] heap units needed, [2] light faces, [3] 1 when the heap was too small (grow to
- * totals[1] units and run again, as the reference grows its heap, light_tree_record.cpp:37-70). */
-typedef struct CxLightScene {
-    uint32_t* heap;
-    uint32_t heap_units;
-    uint32_t leaf_capacity;
-    uint32_t* record_offset;    /* [bricks] by light ordinal */
-    uint32_t* ordinal_of_brick; /* [bricks] */
-    LightBvhLeaf* leaves;       /* [leaf_capacity] */
-    LightTreeState* state;      /* nullable */
-    uint32_t* totals;           /* 4 u32 */
-    void* scratch;              /* cx_scene_light_scratch_bytes(bricks) */
-} CxLightScene;
-CX_API uint64_t cx_scene_light_scratch_bytes(uint32_t bricks);
-CX_API int cx_scene_light_records(const CxScene* scene, const CxLightScene* light, void* stream);
+/* The group pass (shaders/light/records/caelix_light_tree_{size,scan,build,commit}.hlsl), over the tables
+ * the descriptor names (control, heap[2], group rows, group infos, entity infos and motion words, light
+ * codes), as lib/src/light/light_tree_record.cpp:198-228 records it. The staged flags are merged into the
+ * rows; a group is released (kLtCpuReleased), rebuilt (kLtCpuRebuild, or the finalize's
+ * kLtGpuLightChanged) or copied. When a group is released or rebuilt, every group's light block is
+ * written into the heap buffer that is not current, if all blocks fit heap_units: a copied group's block
+ * as it is; a rebuilt group's header, index entries in Morton order of the brick position in the group
+ * and brick light records of its light bricks (record word 1 counts light faces; at most 4096 per group,
+ * the lowest Morton positions; the rest counted as dropped), each record copied from the old block when
+ * the brick is not stale and had an entry there, else recomputed; then rows (generation: the next version
+ * of a rebuilt group, the next life of a released one), statistics and the current buffer are committed
+ * (kLtPassOk), the stale bits of the rebuilt groups' bricks cleared, and trees_changed names the trees of
+ * the changed groups' entities. A pass that does not fit changes nothing but pass_result (kLtPassFailed),
+ * needed_units and the counters; the flags stay for the next pass. Bricks are the scene's, their groups
+ * group_of_brick (the descriptor's group_shift must be the scene's). Groups: at most 2^32 / 8^group_shift. */
+typedef struct CxLightPass {
+    LightTreeDescriptor* descriptor; /* device address; the descriptor and its tables in device memory */
+    uint64_t heap_units;             /* capacity of each heap buffer, in units of 16 bytes */
+    uint32_t* staged_flags;          /* [group_count] kLtCpu* flags staged by the host; cleared by the pass */
+    void* scratch;                   /* cx_light_pass_scratch_bytes(bricks, group_count) bytes */
+    uint32_t group_count;
+    uint32_t pad;
+} CxLightPass;
+CX_API uint64_t cx_light_pass_scratch_bytes(uint32_t bricks, uint32_t groups);
+CX_API int cx_light_pass(const CxScene* scene, const CxLightPass* pass, void* stream);
 
-/* The light tree as the trace reads it: two tree headers (tree 1 may be empty, root kLtNone), nodes
- * and leaves per tree, and the brick light records. mis: balance-heuristic MIS on both sides. */
+/* The gather of tree `tree` (caelix_light_gather_count.hlsl, caelix_light_bvh_scan.hlsl,
+ * caelix_light_gather_emit.hlsl), gated by its state's `building` (cx_light_bvh_begin): per group whose
+ * entity has the tree's motion, its index entries become leaves at the group's offset (an exclusive scan
+ * of the counts over the group indices; state->leaves_found = the total), each the record's box and face
+ * sums moved to world space by the entity's frame (LtWorldSummary: the AABB of the rotated box, projected
+ * face sums), group and entry = the brick's group and index entry; the index entry's bvh_leaf =
+ * tree << 31 | leaf, or kLtNone past leaf_capacity. Then cx_light_bvh_build builds the tree over the
+ * leaves. `pass` names the descriptor and the scratch of the last cx_light_pass. */
+CX_API int cx_light_gather(const CxLightPass* pass, uint32_t tree, LightBvhLeaf* leaves, uint32_t leaf_capacity,
+                           LightTreeState* state, void* stream);
+/* The gather for many trees at once, one per entity (chunk scenes: an entity each, its groups
+ * consecutive): as cx_light_gather, but the bvh_leaf words count leaves from entity_leaf_base[entity] (the
+ * leaf index of the entity's first leaf), so that each entity's leaves form a tree of their own over the
+ * slice leaves + entity_leaf_base[entity]. No counterpart in the reference (two trees per descriptor). */
+CX_API int cx_light_gather_entities(const CxLightPass* pass, uint32_t tree, LightBvhLeaf* leaves,
+                                    uint32_t leaf_capacity, LightTreeState* state, const uint32_t* entity_leaf_base,
+                                    void* stream);
+
+/* The light tree as the trace and the light probes read it: the device address of its 128-byte
+ * descriptor (LightTreeDescriptor, the reference's LightTreeOpen reads it the same way:
+ * shaders/light/caelix_light_tree.hlsl:239-287), and whether the NEE and the BSDF hits weigh each other
+ * by the balance heuristic (the reference's light NEE without CAELIX_LIGHT_NEE_FLAG_NO_MIS). */
 typedef struct CxLightTreeView {
-    const LightTreeHeader* headers; /* [2] */
-    const LightBvhNode* nodes[2];
-    const LightBvhLeaf* leaves[2];
-    const uint32_t* heap;
-    const uint32_t* record_offset;
-    const uint32_t* ordinal_of_brick;
-    uint32_t group_shift;
+    const LightTreeDescriptor* descriptor;
     uint32_t mis;
-    uint32_t leaf_base; /* light ordinal of leaf 0 of tree 0 (a scene's slice of a batch's leaves), else 0 */
     uint32_t pad;
 } CxLightTreeView;
 
@@ -437,20 +467,142 @@ CX_API int cx_trace_batch(const CxScene* scene, const void* device_views, uint32
 CX_API int cx_trace_rays(const CxScene* scene, const float* rays, uint32_t count, uint32_t* hits, void* stream);
 
 /* LightTreeSample (caelix_light_tree.hlsl:749-767) for receivers[i] = x(3), n(3) with PCG state
- * seeds[i]; samples[i] = {position(3), normal(3), pdf, block, face} (floats; block/face as bits);
- * ok[i] = 1 when a point was drawn. */
-CX_API int cx_light_sample(const CxScene* scene, const CxLightTreeView* lights, const float* receivers,
-                           const uint32_t* seeds, uint32_t count, float* samples, uint32_t* ok, void* stream);
-/* The area pdf of a light face for a receiver (LightTreeFindLightFace x LightTreeBrickPdf):
- * queries[i] = {x(3), n(3) as floats; record, voxel x, y, z, face, block as u32}; pdf[i]. */
-CX_API int cx_light_face_pdf(const CxScene* scene, const CxLightTreeView* lights, const uint32_t* queries,
-                             uint32_t count, float* pdf, void* stream);
+ * seeds[i]; samples[i] = {position(3), normal(3), pdf, block, face, group, entry, voxel in group} (12
+ * words: floats, then block, face, the brick (group, index entry) and the group-local voxel
+ * x | y << 10 | z << 20 as bits); ok[i] = 1 when a point was drawn. */
+CX_API int cx_light_sample(const CxLightTreeView* lights, const float* receivers, const uint32_t* seeds,
+                           uint32_t count, float* samples, uint32_t* ok, void* stream);
+/* LightTreePointPdf (caelix_light_tree.hlsl:936-949): the area pdf with which LightTreeSample draws
+ * the point on a light face, for a receiver: queries[i] = {x(3), n(3) as floats; group, group-local
+ * voxel x, y, z, face, raw block value as u32} (12 words); pdf[i] (0 when the face is not a light face
+ * of the tree or its brick has no leaf). */
+CX_API int cx_light_point_pdf(const CxLightTreeView* lights, const uint32_t* queries, uint32_t count, float* pdf,
+                              void* stream);
 
 /* LightNeeMisWeight (shaders/render/caelix_light_nee.hlsl:20-34), the balance heuristic both MIS sides
  * share, per query: queries[i] = {p_area, d, cos_l, receiver normal (3), unit direction (3)} (9
  * floats); weights[i] = {NEE side, BSDF side}. A non-finite or degenerate solid-angle density counts
  * as infinite (NEE 1, BSDF 0); both densities 0 give NEE 0, BSDF 1. */
 CX_API int cx_light_mis_weight(const float* queries, uint32_t count, float* weights, void* stream);
+
+/* ------------------------------------------------------------------------------------------
+ * The light tree's host side (the reference's LightTree, lib/src/light/light_tree.h:37-101,
+ * light_tree.cpp, light_tree_record.cpp): the CPU state mirrored from the scene (entity motions,
+ * group lives and the flags staged for the group pass), the light code set, and the device tables
+ * it owns: control, group rows, staged flags, entity motion words, the two heap buffers, leaves and
+ * nodes of the static and the dynamic tree, the build scratch, the 128-byte descriptor. The scene
+ * calls the hooks when its entities and groups change and, per flush, the steps in this order:
+ *
+ *   cx_light_tree_prepare           CPU work and every allocation; *requeue = 1 when the set of light
+ *                                   codes changed (the scene finalizes every resident brick again)
+ *   cx_light_tree_stage             the staged group flags and entity motion words, for the next
+ *                                   transfers; *staged = 0 when nothing of the tree is recorded
+ *   cx_light_tree_record_transfers  growth copies, zero fills of new tables, the descriptor, the code
+ *                                   set and the staged words (PrepareTransfers + RecordTransfers)
+ *   cx_scene_finalize               the scene's finalize with light_group_rows = the tree's rows
+ *   cx_light_tree_record            the group pass (cx_light_pass), then per tree that changed the
+ *                                   begin, gather and build (cx_light_bvh_begin, cx_light_gather,
+ *                                   cx_light_bvh_build), and a copy of the control to the host
+ *   cx_light_tree_end_flush         closes the flush
+ *
+ * cx_light_tree_recycle(safe_frame) reads the copies of frames <= safe_frame (the caller knows those
+ * frames are done on the GPU), frees replaced buffers and schedules heap and BVH growth: a pass that
+ * did not fit the heap is retried at the next flush after the heap grew (x2, or 1.5 x the need), a
+ * tree whose leaves passed 3/4 of its capacity (or dropped leaves) grows and is rebuilt, a node
+ * overflow grows the node capacity to the leaf capacity. Calls are externally synchronised; all work
+ * is enqueued on the stream given to the recording steps. No GPU memory is allocated outside prepare
+ * and record_transfers (record allocates the pass scratch for a larger brick count).
+ * ------------------------------------------------------------------------------------------ */
+
+/* CaelixLightOptions (caelix.h:629-641): min_emission 0 = 20; heap_bytes 0 = 16 MiB, rounded up to 16
+ * bytes, at most 2^32 x 16 bytes. `enabled` is not read here (a disabled tree is one not created). */
+typedef struct CxLightOptions {
+    uint32_t enabled;
+    float min_emission;
+    uint64_t heap_bytes;
+} CxLightOptions;
+
+/* CaelixLightStats (caelix.h:648-680), 104 bytes: the newest light update read back. */
+typedef struct CxLightStats {
+    uint32_t enabled;
+    uint32_t light_groups;
+    uint64_t light_bricks;
+    uint64_t light_faces;
+    uint64_t bricks_dropped;
+    uint64_t heap_bytes_used;
+    uint64_t heap_bytes_capacity;
+    uint32_t passes;
+    uint32_t failed_passes;
+    uint32_t records_recomputed;
+    uint32_t records_copied;
+    float gpu_ms;
+    uint32_t pad;
+    uint32_t tree_leaves[2];
+    uint32_t tree_nodes[2];
+    uint32_t tree_builds[2];
+    uint32_t tree_leaves_dropped[2];
+} CxLightStats;
+
+/* What the tree reads of the scene at a flush (LightSceneTables, light_tree.h:22-35): the device
+ * tables of groups and entities (GroupInfo, EntityInfo) and their capacities, and the brick capacity
+ * (the group pass's scratch). */
+typedef struct CxLightSceneTables {
+    const GroupInfo* group_infos;
+    const EntityInfo* entity_infos;
+    uint32_t group_capacity;  /* group indices in use or free: [0, group_capacity) */
+    uint32_t entity_capacity; /* entity ids: [0, entity_capacity) */
+    uint32_t brick_capacity;  /* bricks of the CxScene the pass reads */
+    uint32_t pad;
+} CxLightSceneTables;
+
+typedef struct CxLightTree CxLightTree; /* opaque */
+
+/* Entity motions (CAELIX_ENTITY_STATIC / DYNAMIC): the tree that holds an entity's lights. */
+#define CX_ENTITY_STATIC 0u
+#define CX_ENTITY_DYNAMIC 1u
+
+/* LightTree::LightTree (light_tree.cpp:41-86): group_shift 1..7, encoding kEncoding*. */
+CX_API int cx_light_tree_create(uint32_t group_shift, uint32_t encoding, const CxLightOptions* options,
+                                CxLightTree** out);
+/* Waits for the device and frees everything. */
+CX_API void cx_light_tree_destroy(CxLightTree* tree);
+/* The scene hooks (light_tree.cpp:113-178). */
+CX_API int cx_light_tree_on_entity_added(CxLightTree* tree, uint32_t entity, uint32_t motion);
+CX_API int cx_light_tree_on_entity_removed(CxLightTree* tree, uint32_t entity);
+CX_API int cx_light_tree_on_entity_moved(CxLightTree* tree, uint32_t entity);
+CX_API int cx_light_tree_on_entity_motion(CxLightTree* tree, uint32_t entity, uint32_t motion);
+CX_API int cx_light_tree_on_group_created(CxLightTree* tree, uint32_t group);
+CX_API int cx_light_tree_on_group_released(CxLightTree* tree, uint32_t group, uint32_t motion);
+CX_API int cx_light_tree_on_group_bricks_removed(CxLightTree* tree, uint32_t group);
+CX_API int cx_light_tree_on_materials_changed(CxLightTree* tree);
+CX_API int cx_light_tree_set_min_emission(CxLightTree* tree, float min_emission);
+/* The flush steps (light_tree.cpp:198-346, light_tree_record.cpp:102-404). `materials` is the host
+ * copy of the scene's table ([65536][8] floats, as CxScene::materials): a code is a light code when the
+ * luminance of its canonical code's emission (BlockMaterialCode) reaches min_emission. */
+CX_API int cx_light_tree_prepare(CxLightTree* tree, const float* materials, const CxLightSceneTables* tables,
+                                 uint32_t* requeue);
+CX_API int cx_light_tree_stage(CxLightTree* tree, uint32_t finalize_ran, uint32_t* staged);
+CX_API int cx_light_tree_record_transfers(CxLightTree* tree, void* stream);
+CX_API int cx_light_tree_record(CxLightTree* tree, const CxScene* scene, uint64_t frame, void* stream);
+CX_API int cx_light_tree_end_flush(CxLightTree* tree, uint64_t frame);
+/* The gather and build of tree `which` now, from the current blocks (RecordTree with a reason,
+ * light_tree_record.cpp:230-309), outside a flush: only kernel launches (it can be captured in a
+ * hipGraph), for a caller that rebuilds a tree every frame. After a flush that recorded the tree. No
+ * counterpart in the reference's C ABI. */
+CX_API int cx_light_tree_rebuild(CxLightTree* tree, uint32_t which, void* stream);
+CX_API int cx_light_tree_recycle(CxLightTree* tree, uint64_t safe_frame);
+/* LightTree::GetStats (light_tree_record.cpp:406-428); a NULL tree reports enabled 0. */
+CX_API void cx_light_tree_get_stats(const CxLightTree* tree, CxLightStats* out);
+/* 1 while the tree has work for a later flush or results not yet read back: a pass to retry, growth
+ * to make, a tree to rebuild, flags staged by the hooks, a readback not yet recycled. No counterpart in
+ * the reference (its flushes run every frame); a caller that flushes only on change loops on it. */
+CX_API uint32_t cx_light_tree_pending(const CxLightTree* tree);
+/* Device addresses: the descriptor (caelix_scene_get_light_tree, caelix.h:682-688; valid for work
+ * enqueued after the next record_transfers), the light code set and the group rows (the finalize's
+ * light_group_rows). */
+CX_API const LightTreeDescriptor* cx_light_tree_descriptor(const CxLightTree* tree);
+CX_API const uint32_t* cx_light_tree_light_codes(const CxLightTree* tree);
+CX_API GroupLightRow* cx_light_tree_group_rows(const CxLightTree* tree);
 
 /* ------------------------------------------------------------------------------------------
  * .cxw world files (format version 8 and older): the reader of the bricks the scene build takes.

@@ -137,12 +137,48 @@ CX_CONST(uint32_t, kLtPassFailed, 0);
 CX_CONST(uint32_t, kLtPassOk, 1);
 CX_CONST(uint32_t, kLtPassNoChange, 2);
 
-/* Group light block in a heap buffer (units of 16 bytes): header, index entries, records. */
+/* LightControl::pass_flags (light_tree_layout.h:208-212): a group was flagged or released; bit 1 + t: one of
+ * them belongs to tree t. */
+CX_CONST(uint32_t, kLtPassFlagChanged, 1);
+CX_CONST(uint32_t, kLtPassFlagTreeShift, 1);
+
+/* The light row of a group, by group index (32 bytes; light_tree_layout.h:214-250). The host never
+ * writes a row: its per-group flags go through the staged flags (kLtCpu*), merged by the group pass. */
+typedef struct GroupLightRow {
+    uint32_t block_offset; /* units of 16 bytes in the current heap buffer */
+    uint32_t block_units;  /* 0 = no block */
+    uint32_t generation;   /* life << 16 | version */
+    uint32_t gpu_flags;    /* kLtGpuLightChanged */
+    uint32_t cpu_flags;    /* the merged staged flags (kLtCpu*) */
+    uint32_t bricks;       /* light bricks | dropped << 16 */
+    uint32_t light_faces;
+    uint32_t pad;
+} GroupLightRow;
+CX_STATIC_ASSERT(offsetof(GroupLightRow, cpu_flags) == 16);
+CX_STATIC_ASSERT(sizeof(GroupLightRow) == 32);
+CX_CONST(uint32_t, kLtRowBricksMask, 0xFFFF);
+CX_CONST(uint32_t, kLtRowDroppedShift, 16);
+CX_CONST(uint32_t, kLtGenerationLifeShift, 16);
+CX_CONST(uint32_t, kLtGenerationVersionMask, 0xFFFF);
+CX_CONST(uint32_t, kLtGpuLightChanged, 1);
+CX_CONST(uint32_t, kLtCpuRebuild, 1);  /* rebuild the block (bricks removed) */
+CX_CONST(uint32_t, kLtCpuReleased, 2); /* the group index is free: clear the row */
+CX_CONST(uint32_t, kLtCpuFresh, 4);    /* a new life of the group index */
+
+/* Group light block in a heap buffer (units of 16 bytes): header, index entries (Morton order of the
+ * brick position in the group), records (light_tree_layout.h:252-334). */
 CX_CONST(uint32_t, kLtUnitBytes, 16);
 CX_CONST(uint32_t, kLtBlockIndexOffset, 16);
+typedef struct LightBlockHeader {
+    uint32_t record_count; /* index entries = brick light records of the block */
+    uint32_t pad[3];
+} LightBlockHeader;
+CX_STATIC_ASSERT(sizeof(LightBlockHeader) == 16);
+CX_CONST(uint32_t, kLtBvhLeafTreeShift, 31);
+CX_CONST(uint32_t, kLtBvhLeafIndexMask, 0x7FFFFFFF);
 typedef struct LightIndexEntry {
-    uint32_t pos;
-    uint32_t dense_id;
+    uint32_t pos;      /* group-local brick position (x | y << shift | z << 2 shift) */
+    uint32_t dense_id; /* the brick's id in the group (this build: its record in the brick store) */
     uint32_t record_offset; /* units from the block start */
     uint32_t bvh_leaf;      /* tree << 31 | leaf index; kLtNone = none */
 } LightIndexEntry;

@@ -129,15 +129,17 @@ class _Scene(ctypes.Structure):
         ("grid_min", ctypes.c_int32 * 3), ("grid_dim", _U32), ("group_shift", _U32), ("encoding", _U32),
         ("materials", _P), ("light_codes", _P), ("emission_scale", ctypes.c_float), ("pad", _U32),
         ("brick_emission_scale", _P), ("scene_count", _U32), ("pad2", _U32), ("occupied_box", _P),
-        ("materials_ext", _P),
+        ("materials_ext", _P), ("group_of_brick", _P), ("light_group_rows", _P),
     ]
 
 
 class _TreeView(ctypes.Structure):
-    _fields_ = [
-        ("headers", _P), ("nodes", _P * 2), ("leaves", _P * 2), ("heap", _P), ("record_offset", _P),
-        ("ordinal_of_brick", _P), ("group_shift", _U32), ("mis", _U32), ("leaf_base", _U32), ("pad", _U32),
-    ]
+    _fields_ = [("descriptor", _P), ("mis", _U32), ("pad", _U32)]
+
+
+class _Pass(ctypes.Structure):
+    _fields_ = [("descriptor", _P), ("heap_units", ctypes.c_uint64), ("staged_flags", _P), ("scratch", _P),
+                ("group_count", _U32), ("pad", _U32)]
 
 
 class _Camera(ctypes.Structure):
@@ -187,14 +189,16 @@ def _trace_lib():
         S, T = ctypes.POINTER(_Scene), ctypes.POINTER(_TreeView)
         lib.oracle_scene_finalize.restype = None
         lib.oracle_scene_finalize.argtypes = [S]
-        lib.oracle_scene_light_records.restype = _U32
-        lib.oracle_scene_light_records.argtypes = [S, _P, _U32, _P, _P, _P, _U32, _P]
+        lib.oracle_light_pass.restype = None
+        lib.oracle_light_pass.argtypes = [S, ctypes.POINTER(_Pass)]
+        lib.oracle_light_gather.restype = None
+        lib.oracle_light_gather.argtypes = [ctypes.POINTER(_Pass), _U32, _P, _U32, _P, _P]
         lib.oracle_trace_rays.restype = None
         lib.oracle_trace_rays.argtypes = [S, _P, _U32, _P]
         lib.oracle_light_sample.restype = None
-        lib.oracle_light_sample.argtypes = [S, T, _P, _P, _U32, _P, _P]
-        lib.oracle_light_face_pdf.restype = None
-        lib.oracle_light_face_pdf.argtypes = [S, T, _P, _U32, _P]
+        lib.oracle_light_sample.argtypes = [T, _P, _P, _U32, _P, _P]
+        lib.oracle_light_point_pdf.restype = None
+        lib.oracle_light_point_pdf.argtypes = [T, _P, _U32, _P]
         lib.oracle_mis_weight.restype = None
         lib.oracle_mis_weight.argtypes = [_P, _U32, _P]
         lib.oracle_trace_frame.restype = None
@@ -227,20 +231,181 @@ def camera_struct(cam: dict) -> _Camera:
     return c
 
 
+# ------------------------------------------------------------------------------------------------
+# The light tree's tables on the host (the oracle's restatement of what the reference's LightTree
+# holds, lib/src/light/light_tree.h:37-101): a descriptor of host addresses over numpy arrays.
+
+_CONTROL = np.dtype([("current", "<u4"), ("pass_result", "<u4"), ("needed_units", "<u4"), ("pad", "<u4"),
+                     ("light_groups", "<u4"), ("bricks_dropped", "<u4"), ("light_bricks", "<u8"),
+                     ("light_faces", "<u8"), ("passes", "<u4"), ("failed_passes", "<u4"), ("pass_flags", "<u4"),
+                     ("records_recomputed", "<u4"), ("records_copied", "<u4"), ("trees_changed", "<u4"),
+                     ("headers", "u1", (2, 64)), ("states", "u1", (2, 64))])
+_DESCRIPTOR = np.dtype([("control", "<u8"), ("heap", "<u8", (2,)), ("group_rows", "<u8"), ("group_infos", "<u8"),
+                        ("entity_infos", "<u8"), ("entity_motion", "<u8"), ("grids", "<u8"), ("nodes", "<u8", (2,)),
+                        ("leaves", "<u8", (2,)), ("light_codes", "<u8"), ("tree_headers", "<u8"),
+                        ("group_shift", "<u4"), ("encoding", "<u4"), ("pad", "<u4"), ("group_count", "<u4")])
+assert _CONTROL.itemsize == 320 and _DESCRIPTOR.itemsize == 128
+GROUP_INFO = np.dtype([("coord", "<i4", (3,)), ("entity", "<u4"), ("pad", "<u4", (4,))])
+ENTITY_INFO = np.dtype([("pad", "<u4", (8,)), ("cur_rows", "<f4", (3, 4)), ("prev_rows", "<f4", (3, 4))])
+ROW = np.dtype([("block_offset", "<u4"), ("block_units", "<u4"), ("generation", "<u4"), ("gpu_flags", "<u4"),
+                ("cpu_flags", "<u4"), ("bricks", "<u4"), ("light_faces", "<u4"), ("pad", "<u4")])
+CPU_REBUILD, CPU_RELEASED, CPU_FRESH = 1, 2, 4
+PASS_FAILED, PASS_OK, PASS_NO_CHANGE = 0, 1, 2
+
+
+def groups_of(keys: np.ndarray, group_shift: int):
+    """Render groups: a group is the bricks of one coordinate key >> group_shift (arithmetic, on the last
+    three key words); numbered in ascending lexicographic order of the coordinates. (coords, group_of_brick)."""
+    k = np.asarray(keys, np.int64).copy()
+    k[:, -3:] >>= int(group_shift)
+    coords, inverse = np.unique(k, axis=0, return_inverse=True)
+    return coords.astype(np.int32), inverse.reshape(-1).astype(np.uint32)
+
+
+class LightTables:
+    """Control, heaps, rows, group and entity infos, motion words, codes, nodes and leaves of both trees,
+    and the descriptor naming them (host addresses), for the oracle's group pass, gather, build and
+    sampling."""
+
+    def __init__(self, group_coords, group_entity, entity_rows, entity_motion, codes, group_shift: int,
+                 encoding: int, heap_units: int = 1024):
+        G = len(group_coords)
+        self.group_shift = int(group_shift)
+        self.encoding = int(encoding)
+        self.control = np.zeros(1, _CONTROL)
+        for t in range(2):
+            self.control["headers"][0, t].view(np.uint32)[12] = 0xFFFFFFFF  # root none
+        self.rows = np.zeros(G, ROW)
+        self.infos = np.zeros(G, GROUP_INFO)
+        self.infos["coord"] = np.asarray(group_coords)[:, -3:]
+        self.infos["entity"] = np.asarray(group_entity, np.uint32)
+        self.entities = np.zeros(len(entity_rows), ENTITY_INFO)
+        self.entities["cur_rows"] = np.asarray(entity_rows, np.float32)
+        self.entities["prev_rows"] = np.asarray(entity_rows, np.float32)
+        self.motion = np.ascontiguousarray(entity_motion, np.uint32).copy()
+        self.codes = np.ascontiguousarray(codes, np.uint32).copy()
+        self.heap_units = int(heap_units)
+        self.heaps = [np.zeros(self.heap_units * 4, np.uint32), np.zeros(self.heap_units * 4, np.uint32)]
+        self.staged = np.zeros(max(G, 1), np.uint32)
+        self.leaves = [np.zeros(LEAF_BYTES, np.uint8), np.zeros(LEAF_BYTES, np.uint8)]
+        self.nodes = [np.zeros(NODE_BYTES, np.uint8), np.zeros(NODE_BYTES, np.uint8)]
+        self.desc = np.zeros(1, _DESCRIPTOR)
+        self.refresh()
+
+    @property
+    def groups(self) -> int:
+        return len(self.rows)
+
+    def refresh(self) -> None:
+        d = self.desc[0]
+        d["control"] = self.control.ctypes.data
+        d["heap"] = [self.heaps[0].ctypes.data, self.heaps[1].ctypes.data]
+        d["group_rows"] = self.rows.ctypes.data
+        d["group_infos"] = self.infos.ctypes.data
+        d["entity_infos"] = self.entities.ctypes.data
+        d["entity_motion"] = self.motion.ctypes.data
+        d["nodes"] = [self.nodes[0].ctypes.data, self.nodes[1].ctypes.data]
+        d["leaves"] = [self.leaves[0].ctypes.data, self.leaves[1].ctypes.data]
+        d["light_codes"] = self.codes.ctypes.data
+        d["tree_headers"] = self.control.ctypes.data + 64
+        d["group_shift"] = self.group_shift
+        d["encoding"] = self.encoding
+        d["group_count"] = self.groups
+
+    def view(self, mis: bool) -> _TreeView:
+        return _TreeView(self.desc.ctypes.data, 1 if mis else 0, 0)
+
+    def _pass_struct(self) -> _Pass:
+        return _Pass(self.desc.ctypes.data, self.heap_units, self.staged.ctypes.data, None, max(self.groups, 1), 0)
+
+    def grow_heap(self, units: int) -> None:
+        """Both heap buffers grow, their contents kept at the same offsets (LightTree::GrowHeap,
+        light_tree_record.cpp:38-70)."""
+        for i in range(2):
+            h = np.zeros(int(units) * 4, np.uint32)
+            h[: len(self.heaps[i])] = self.heaps[i]
+            self.heaps[i] = h
+        self.heap_units = int(units)
+        self.refresh()
+
+    def run_pass(self, scene) -> int:
+        """oracle_light_pass; a pass that does not fit grows the heap to its need and runs again (the
+        flags stay). Returns the pass result."""
+        while True:
+            _trace_lib().oracle_light_pass(ctypes.byref(scene), ctypes.byref(self._pass_struct()))
+            c = self.control[0]
+            if int(c["pass_result"]) != PASS_FAILED:
+                return int(c["pass_result"])
+            self.grow_heap(max(int(c["needed_units"]), self.heap_units + 1))
+
+    def build_tree(self, tree: int, entity_leaf_base=None) -> dict:
+        """Gather (capacity: the tree's light bricks rounded up to 64) and the reference build of tree
+        `tree`; the header and state land in the control."""
+        rows = self.rows
+        in_tree = self.motion[self.infos["entity"]] & 1 == tree if self.groups else np.zeros(0, bool)
+        count = int((rows["bricks"][in_tree] & 0xFFFF).sum()) if self.groups else 0
+        cap = max(64, (count + 63) // 64 * 64)
+        leaves = np.zeros(cap * LEAF_BYTES, np.uint8)
+        state = np.zeros(STATE_BYTES, np.uint8)
+        state.view(np.uint32)[0] = 1
+        base = None if entity_leaf_base is None else np.ascontiguousarray(entity_leaf_base, np.uint32)
+        _trace_lib().oracle_light_gather(ctypes.byref(self._pass_struct()), tree, _ptr(leaves), cap, _ptr(state),
+                                         None if base is None else _ptr(base))
+        built = light_bvh_build(leaves, cap, cap // 8 * 5 + 64, state)
+        self.control["headers"][0, tree] = built["header"]
+        self.control["states"][0, tree] = built["state"]
+        self.leaves[tree] = built["leaves"]
+        self.nodes[tree] = built["nodes"]
+        self.refresh()
+        return built
+
+    def current_heap(self) -> np.ndarray:
+        return self.heaps[int(self.control[0]["current"]) & 1]
+
+    def blocks(self) -> dict:
+        """The block of every group with one in the current heap buffer: {group: uint32 words}."""
+        h = self.current_heap()
+        out = {}
+        for g in range(self.groups):
+            r = self.rows[g]
+            if int(r["block_units"]):
+                o = int(r["block_offset"]) * 4
+                out[g] = h[o:o + int(r["block_units"]) * 4]
+        return out
+
+    @classmethod
+    def from_readback(cls, lt: dict, group_infos: np.ndarray, entity_infos: np.ndarray, group_shift: int,
+                      encoding: int) -> "LightTables":
+        """The tables of a GPU light tree as read back (light_tables of the product package)."""
+        gi = np.ascontiguousarray(group_infos).view(np.uint8).view(GROUP_INFO)
+        ei = np.ascontiguousarray(entity_infos).view(np.uint8).view(ENTITY_INFO)
+        self = cls(gi["coord"], gi["entity"], ei["cur_rows"], lt["motion"].view(np.uint32), lt["codes"].view(np.uint32),
+                   group_shift, encoding, heap_units=1)
+        self.control = lt["control"].copy().view(_CONTROL)
+        self.rows = lt["rows"].copy().view(ROW)
+        self.heaps = [lt["heap0"].view(np.uint32).copy(), lt["heap1"].view(np.uint32).copy()]
+        self.heap_units = len(self.heaps[0]) // 4
+        self.leaves = [lt["leaves0"].copy(), lt["leaves1"].copy()]
+        self.nodes = [lt["nodes0"].copy(), lt["nodes1"].copy()]
+        self.refresh()
+        return self
+
+
 class OracleScene:
     """A scene held in host arrays for the oracle. `arrays` holds the same named arrays as the GPU
     scene (keys, info, occupancy, payload, slots, hash_info, brick_bits, super_bits, materials,
-    light_codes, heap, record_offset, ordinal_of_brick, headers, nodes, leaves); missing ones are
-    produced by the oracle itself with build()."""
+    light_codes, group_of_brick) and `lt` its light tree's tables; build() produces all of them with the
+    oracle itself."""
 
     def __init__(self, arrays: dict, grid_min, grid_dim: int, group_shift: int, mis: bool = True,
-                 encoding: int = 1):
+                 encoding: int = 1, lt: LightTables | None = None):
         self.a = {k: np.ascontiguousarray(v) for k, v in arrays.items()}
         self.grid_min = tuple(int(v) for v in grid_min)
         self.grid_dim = int(grid_dim)
         self.group_shift = int(group_shift)
         self.mis = mis
         self.encoding = int(encoding)  # 0 packed, 1 block-ID (the reference's CaelixSceneDesc::encoding)
+        self.lt = lt
         self._refresh()
 
     def _refresh(self):
@@ -265,34 +430,31 @@ class OracleScene:
         s.emission_scale = 1.0
         s.brick_emission_scale = _ptr(a["brick_scale"]) if "brick_scale" in a else None
         s.materials_ext = _ptr(a["materials_ext"]) if "materials_ext" in a else None
-        if "headers" in a:
-            v = self.view = _TreeView()
-            v.headers = _ptr(a["headers"])
-            v.nodes[0] = _ptr(a["nodes"])
-            v.nodes[1] = _ptr(a["empty_nodes"])
-            v.leaves[0] = _ptr(a["leaves"])
-            v.leaves[1] = _ptr(a["empty_leaves"])
-            v.heap = _ptr(a["heap"])
-            v.record_offset = _ptr(a["record_offset"])
-            v.ordinal_of_brick = _ptr(a["ordinal_of_brick"])
-            v.group_shift = self.group_shift
-            v.mis = 1 if self.mis else 0
+        s.group_of_brick = _ptr(a["group_of_brick"]) if "group_of_brick" in a else None
+        s.light_group_rows = self.lt.rows.ctypes.data if self.lt is not None else None
+        if self.lt is not None:
+            self.view = self.lt.view(self.mis)
 
     @classmethod
     def from_gpu_arrays(cls, arrays: dict, data, mis: bool = True) -> "OracleScene":
-        a = dict(arrays)
-        a["empty_nodes"] = np.zeros(NODE_BYTES, np.uint8)
-        a["empty_leaves"] = np.zeros(LEAF_BYTES, np.uint8)
-        return cls(a, data.grid_min, data.grid_dim, data.group_shift, mis, getattr(data, "encoding", 1))
+        a = {k: v for k, v in arrays.items() if not k.startswith("lt_")}
+        lt = LightTables.from_readback({k[3:]: v for k, v in arrays.items() if k.startswith("lt_")},
+                                       arrays["group_infos"], arrays["entity_infos"], data.group_shift,
+                                       getattr(data, "encoding", 1))
+        return cls(a, data.grid_min, data.grid_dim, data.group_shift, mis, getattr(data, "encoding", 1), lt)
 
     @classmethod
     def build(cls, data, slot_count: int, mis: bool = True) -> "OracleScene":
-        """Everything from SceneData on the CPU: the ordered-insert hash table, finalize, the light
-        pass and the light-tree build, all in the oracle."""
+        """Everything from SceneData on the CPU: the ordered-insert hash table, finalize with the light
+        state, the group pass (every group created: rebuild | fresh), the gathers and the light-tree
+        builds of both trees (one static entity at the identity frame), all in the oracle."""
         n = len(data.keys)
         dim = int(data.grid_dim)
         cells = dim ** 3
+        shift = int(data.group_shift)
+        enc = getattr(data, "encoding", 1)
         slots, longest = hash_build_bricks(data.keys, slot_count)
+        coords, gob = groups_of(data.keys, shift)
         a = {
             "keys": np.ascontiguousarray(data.keys, np.int32),
             "info": np.zeros(2 * n, np.uint32),
@@ -304,38 +466,19 @@ class OracleScene:
             "super_bits": np.zeros((cells // 512 + 31) // 32 + 1, np.uint32),
             "materials": np.ascontiguousarray(data.materials, np.float32),
             "light_codes": np.ascontiguousarray(data.light_codes, np.uint32),
+            "group_of_brick": gob,
         }
         if getattr(data, "materials_ext", None) is not None:
             a["materials_ext"] = np.ascontiguousarray(data.materials_ext, np.float32)
-        self = cls(a, data.grid_min, dim, data.group_shift, mis, getattr(data, "encoding", 1))
+        lt = LightTables(coords, np.zeros(len(coords), np.uint32), np.eye(3, 4, dtype=np.float32)[None], [0],
+                         data.light_codes, shift, enc, heap_units=max(64, n * 8))
+        self = cls(a, data.grid_min, dim, shift, mis, enc, lt)
         lib = _trace_lib()
         lib.oracle_scene_finalize(ctypes.byref(self.scene))
-        leaf_capacity = max(64, (n + 63) // 64 * 64)
-        node_capacity = leaf_capacity // 8 * 5 + 64
-        heap_units = max(64, n * 8)
-        while True:
-            heap = np.zeros(heap_units * 4, np.uint32)
-            record_offset = np.zeros(max(n, 1), np.uint32)
-            ordinal_of_brick = np.zeros(max(n, 1), np.uint32)
-            leaves = np.zeros(leaf_capacity * LEAF_BYTES, np.uint8)
-            totals = np.zeros(4, np.uint32)
-            lib.oracle_scene_light_records(ctypes.byref(self.scene), _ptr(heap), heap_units, _ptr(record_offset),
-                                           _ptr(ordinal_of_brick), _ptr(leaves), leaf_capacity, _ptr(totals))
-            if totals[3] == 0:
-                break
-            heap_units = int(totals[1]) + 64
-        state = np.zeros(STATE_BYTES, np.uint8)
-        st = state.view(np.uint32)
-        st[0] = 1  # building
-        st[1] = totals[0]  # leaves_found
-        tree = light_bvh_build(leaves, leaf_capacity, node_capacity, state)
-        headers = np.zeros(2 * HEADER_BYTES, np.uint8)
-        headers[:HEADER_BYTES] = tree["header"]
-        headers[HEADER_BYTES:].view(np.uint32)[12] = 0xFFFFFFFF  # tree 1: empty (root = none)
-        a.update(heap=heap, record_offset=record_offset, ordinal_of_brick=ordinal_of_brick, headers=headers,
-                 nodes=tree["nodes"], leaves=tree["leaves"], totals=totals,
-                 empty_nodes=np.zeros(NODE_BYTES, np.uint8), empty_leaves=np.zeros(LEAF_BYTES, np.uint8))
-        self.a = a
+        lt.staged[:] = CPU_REBUILD | CPU_FRESH
+        lt.run_pass(self.scene)
+        for tree in range(2):
+            lt.build_tree(tree)
         self._refresh()
         return self
 
@@ -348,18 +491,29 @@ class OracleScene:
     def light_sample(self, receivers: np.ndarray, seeds: np.ndarray):
         r = np.ascontiguousarray(receivers, np.float32)
         sd = np.ascontiguousarray(seeds, np.uint32)
-        out = np.zeros((len(r), 9), np.float32)
+        out = np.zeros((len(r), 12), np.float32)
         ok = np.zeros(len(r), np.uint32)
-        _trace_lib().oracle_light_sample(ctypes.byref(self.scene), ctypes.byref(self.view), _ptr(r), _ptr(sd), len(r),
-                                         _ptr(out), _ptr(ok))
+        _trace_lib().oracle_light_sample(ctypes.byref(self.view), _ptr(r), _ptr(sd), len(r), _ptr(out), _ptr(ok))
         return out, ok
 
-    def light_face_pdf(self, queries: np.ndarray) -> np.ndarray:
+    def light_point_pdf(self, queries: np.ndarray) -> np.ndarray:
         q = np.ascontiguousarray(queries, np.uint32)
         pdf = np.zeros(len(q), np.float32)
-        _trace_lib().oracle_light_face_pdf(ctypes.byref(self.scene), ctypes.byref(self.view), _ptr(q), len(q),
-                                           _ptr(pdf))
+        _trace_lib().oracle_light_point_pdf(ctypes.byref(self.view), _ptr(q), len(q), _ptr(pdf))
         return pdf
+
+    def point_queries(self, queries: np.ndarray) -> np.ndarray:
+        """Face queries {x, n, brick record, world voxel, face, block} as point queries {x, n, group,
+        group-local voxel, face, block} (identity entity frames)."""
+        q = np.ascontiguousarray(queries, np.uint32).copy()
+        group = self.a["group_of_brick"][q[:, 6]]
+        origin = self.lt.infos["coord"][group].astype(np.int64) * (8 << self.group_shift)
+        q[:, 6] = group
+        q[:, 7:10] = (q[:, 7:10].view(np.int32).astype(np.int64) - origin).astype(np.uint32)
+        return q
+
+    def light_face_pdf(self, queries: np.ndarray) -> np.ndarray:
+        return self.light_point_pdf(self.point_queries(queries))
 
     def set_parameters(self, albedo=None, brick_scale=None):
         """albedo [256, 3] float32: the albedo of the opaque materials 0x8000 | id; brick_scale

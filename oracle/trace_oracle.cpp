@@ -41,7 +41,14 @@
 #include "caelix_amd/caelix_amd.h"
 #include "oracle_common.h"
 
+extern "C" uint32_t oracle_morton3(uint32_t x, uint32_t y, uint32_t z);
+
 namespace {
+
+uint32_t Morton3(uint32_t x, uint32_t y, uint32_t z)
+{
+    return oracle_morton3(x, y, z);
+}
 
 struct V
 {
@@ -608,7 +615,44 @@ float PickProbability(float sum_i, float sum_p, float importance, float power)
     }
     return sum_p > 0.0f ? power / sum_p : 0.0f;
 }
-Summ TreeSummary(const CxLightTreeView& L, uint32_t k, uint32_t& root)
+// The light tree opened from its descriptor (LightTreeOpen / LightTreeView, caelix_light_tree.hlsl:
+// 239-287): the oracle's descriptor holds host addresses of the same tables.
+struct Lights
+{
+    const LightTreeHeader* headers;
+    const LightBvhNode* nodes[2];
+    const LightBvhLeaf* leaves[2];
+    const uint32_t* heap;  // the current buffer
+    const GroupLightRow* rows;
+    const GroupInfo* group_infos;
+    const EntityInfo* entity_infos;
+    const uint32_t* light_codes;
+    uint32_t group_shift;
+    uint32_t encoding;
+    uint32_t mis;
+};
+Lights OpenLights(const CxLightTreeView& v)
+{
+    const LightTreeDescriptor* d = v.descriptor;
+    Lights l;
+    l.headers = reinterpret_cast<const LightTreeHeader*>(d->tree_headers);
+    for (int k = 0; k < 2; ++k)
+    {
+        l.nodes[k] = reinterpret_cast<const LightBvhNode*>(d->nodes[k]);
+        l.leaves[k] = reinterpret_cast<const LightBvhLeaf*>(d->leaves[k]);
+    }
+    const uint32_t current = reinterpret_cast<const LightControl*>(d->control)->current & 1u;
+    l.heap = reinterpret_cast<const uint32_t*>(d->heap[current]);
+    l.rows = reinterpret_cast<const GroupLightRow*>(d->group_rows);
+    l.group_infos = reinterpret_cast<const GroupInfo*>(d->group_infos);
+    l.entity_infos = reinterpret_cast<const EntityInfo*>(d->entity_infos);
+    l.light_codes = reinterpret_cast<const uint32_t*>(d->light_codes);
+    l.group_shift = d->group_shift;
+    l.encoding = d->encoding;
+    l.mis = v.mis;
+    return l;
+}
+Summ TreeSummary(const Lights& L, uint32_t k, uint32_t& root)
 {
     root = L.headers[k].root;
     g_demand[g_kind] += 64;
@@ -631,7 +675,7 @@ uint32_t ChildCount(const uint32_t* c)
     g_demand[g_kind] += 16;
     return uint32_t(c[0] != kNoneU) + uint32_t(c[1] != kNoneU) + uint32_t(c[2] != kNoneU) + uint32_t(c[3] != kNoneU);
 }
-uint32_t ChooseLeaf(const CxLightTreeView& L, V x, V n, uint32_t& rng, float& pdf, uint32_t& tree)
+uint32_t ChooseLeaf(const Lights& L, V x, V n, uint32_t& rng, float& pdf, uint32_t& tree)
 {
     Pick pick;
     uint32_t roots[2];
@@ -712,7 +756,7 @@ float SlotProbability(const LightBvhNode& nd, V x, V n, uint32_t slot)
     }
     return PickProbability(sum_i, sum_p, it, pt);
 }
-float TreeProbability(const CxLightTreeView& L, V x, V n, uint32_t target)
+float TreeProbability(const Lights& L, V x, V n, uint32_t target)
 {
     float sum_i = 0.0f, sum_p = 0.0f, it = 0.0f, pt = 0.0f;
     for (uint32_t k = 0; k < 2; ++k)
@@ -731,7 +775,7 @@ float TreeProbability(const CxLightTreeView& L, V x, V n, uint32_t target)
     }
     return PickProbability(sum_i, sum_p, it, pt);
 }
-float LeafProbability(const CxLightTreeView& L, uint32_t k, uint32_t leaf, V x, V n)
+float LeafProbability(const Lights& L, uint32_t k, uint32_t leaf, V x, V n)
 {
     if (L.headers[k].root == kNoneU)
     {
@@ -767,9 +811,21 @@ struct Rec
     uint32_t pos, entry_count, counts_lo, counts_hi;
     const uint32_t* entries;
 };
-Rec LoadRec(const CxLightTreeView& L, uint32_t ordinal)
+// A group's light block in the current heap buffer, or null (LtGroupRow, caelix_light_tree.hlsl:339-343).
+const uint32_t* GroupBlock(const Lights& L, uint32_t group)
 {
-    const uint32_t* r = L.heap + uint64_t(L.record_offset[ordinal]) * 4;
+    g_demand[g_kind] += 8;
+    const GroupLightRow& row = L.rows[group];
+    return row.block_units != 0u ? L.heap + uint64_t(row.block_offset) * 4 : nullptr;
+}
+const LightIndexEntry& IndexEntryOf(const uint32_t* block, uint32_t entry)
+{
+    return reinterpret_cast<const LightIndexEntry*>(block + 4)[entry];
+}
+// LtLoadEntryRecord (caelix_light_tree_layout.hlsl:377-401).
+Rec LoadRec(const uint32_t* block, uint32_t entry)
+{
+    const uint32_t* r = block + uint64_t(IndexEntryOf(block, entry).record_offset) * 4;
     g_demand[g_kind] += 4 + 48;
     Rec rec;
     std::memcpy(rec.face, r, 24);
@@ -810,10 +866,74 @@ void FamilyWeights(const Rec& r, V x, float w[6], float& sum)
         }
     }
 }
-V GroupOrigin(const CxLightTreeView& L, const int32_t* key)
+// LtEntityFrame + LtGroupOrigin (caelix_light_tree_layout.hlsl:412-433): the group's entity-local voxel
+// origin (its coordinate times 8 << shift, in integers), its entity's rotation rows and translation.
+struct Frame
 {
-    const int32_t g = int32_t(L.group_shift);
-    return Mk(float((key[0] >> g) << g) * 8.0f, float((key[1] >> g) << g) * 8.0f, float((key[2] >> g) << g) * 8.0f);
+    float r[3][3];
+    V t;
+};
+V GroupFrame(const Lights& L, uint32_t group, Frame& f)
+{
+    g_demand[g_kind] += 16 + 48;
+    const GroupInfo& info = L.group_infos[group];
+    const EntityInfo& e = L.entity_infos[info.entity];
+    for (int i = 0; i < 3; ++i)
+    {
+        for (int j = 0; j < 3; ++j)
+        {
+            f.r[i][j] = e.cur_rows[i][j];
+        }
+    }
+    f.t = Mk(e.cur_rows[0][3], e.cur_rows[1][3], e.cur_rows[2][3]);
+    const int32_t edge = int32_t(8u << L.group_shift);
+    return Mk(float(info.coord[0] * edge), float(info.coord[1] * edge), float(info.coord[2] * edge));
+}
+// mul(x - t, R) - origin = R^T (x - t) - origin (caelix_light_tree.hlsl:691).
+V ToGroup(const Frame& f, V x, V origin)
+{
+    const V d = Sub(x, f.t);
+    const V l = Mk(d.x * f.r[0][0] + d.y * f.r[1][0] + d.z * f.r[2][0], d.x * f.r[0][1] + d.y * f.r[1][1] + d.z * f.r[2][1],
+                   d.x * f.r[0][2] + d.y * f.r[1][2] + d.z * f.r[2][2]);
+    return Sub(l, origin);
+}
+// mul(R, v)
+V Rotate(const Frame& f, V v)
+{
+    return Mk(f.r[0][0] * v.x + f.r[0][1] * v.y + f.r[0][2] * v.z, f.r[1][0] * v.x + f.r[1][1] * v.y + f.r[1][2] * v.z,
+              f.r[2][0] * v.x + f.r[2][1] * v.y + f.r[2][2] * v.z);
+}
+uint32_t MortonOfPos(uint32_t pos, uint32_t shift)
+{
+    const uint32_t mask = (1u << shift) - 1u;
+    return Morton3(pos & mask, (pos >> shift) & mask, (pos >> (2 * shift)) & mask);
+}
+// LtFindIndexEntry (caelix_light_tree_layout.hlsl:336-360), restated as a linear scan for the entry
+// with the key (the entries are in strictly increasing Morton order, so a binary search finds the same).
+uint32_t FindIndexEntry(const uint32_t* block, uint32_t record_count, uint32_t key, uint32_t shift)
+{
+    for (uint32_t e = 0; e < record_count; ++e)
+    {
+        g_demand[g_kind] += 4;
+        if (MortonOfPos(IndexEntryOf(block, e).pos, shift) == key)
+        {
+            return e;
+        }
+    }
+    return kNoneU;
+}
+// LtBrickLeaf (caelix_light_tree.hlsl:594-607).
+uint32_t BrickLeaf(const Lights& L, const uint32_t* block, uint32_t group, uint32_t entry)
+{
+    const uint32_t word = IndexEntryOf(block, entry).bvh_leaf;
+    g_demand[g_kind] += 4;
+    if (word == kNoneU)
+    {
+        return kNoneU;
+    }
+    const LightBvhLeaf& leaf = L.leaves[word >> 31][word & 0x7FFFFFFFu];
+    g_demand[g_kind] += 8;
+    return leaf.group == group && leaf.entry == entry ? word : kNoneU;
 }
 V FaceDirection(uint32_t f)
 {
@@ -824,13 +944,17 @@ V FaceDirection(uint32_t f)
 struct LightPoint
 {
     V position, normal;
-    uint32_t block, face;
+    uint32_t block, voxel, face;
     float pdf;
-    uint32_t record;  // the light's brick record
+    uint32_t group, entry;
+    uint32_t record;          // the index entry's dense_id: the brick's record in the brick store
+    uint32_t voxel_in_group;  // x | y << 10 | z << 20
 };
-bool LightTreeSample(const CxScene& s, const CxLightTreeView& L, V x, V n, uint32_t& rng, LightPoint& p)
+// LightTreeSample (caelix_light_tree.hlsl:749-767): LightTreeSampleBrick (:647-670), then
+// LightBrickSamplePoint (:672-747) in the group's local frame, the point moved back to the world.
+bool LightTreeSample(const Lights& L, V x, V n, uint32_t& rng, LightPoint& p)
 {
-    p = LightPoint{Mk(0, 0, 0), Mk(0, 0, 0), 0u, 0u, 0.0f, kNoneU};
+    p = LightPoint{Mk(0, 0, 0), Mk(0, 0, 0), 0u, 0u, 0u, 0.0f, kNoneU, kNoneU, kNoneU, 0u};
     float pdf_brick = 1.0f;
     uint32_t tree;
     const uint32_t leaf = ChooseLeaf(L, x, n, rng, pdf_brick, tree);
@@ -839,13 +963,21 @@ bool LightTreeSample(const CxScene& s, const CxLightTreeView& L, V x, V n, uint3
         return false;
     }
     const LightBvhLeaf& lf = L.leaves[tree][leaf];
-    g_demand[g_kind] += 64 + 12;
-    const Rec record = LoadRec(L, lf.entry);
-    const int32_t* key = s.keys + uint64_t(lf.group) * 3;
-    const V origin = GroupOrigin(L, key);
+    g_demand[g_kind] += 8;
+    p.group = lf.group;
+    p.entry = lf.entry;
+    const uint32_t* block = GroupBlock(L, lf.group);
+    if (block == nullptr)
+    {
+        return false;
+    }
+    const Rec record = LoadRec(block, lf.entry);
+    p.record = IndexEntryOf(block, lf.entry).dense_id;
+    Frame frame;
+    const V origin = GroupFrame(L, lf.group, frame);
     float w[6];
     float sum;
-    FamilyWeights(record, Sub(x, origin), w, sum);
+    FamilyWeights(record, ToGroup(frame, x, origin), w, sum);
     if (sum <= 0.0f)
     {
         return false;
@@ -882,43 +1014,53 @@ bool LightTreeSample(const CxScene& s, const CxLightTreeView& L, V x, V n, uint3
     g_demand[g_kind] += 4;
     const uint32_t vi = entry & 0x1FFu;
     const uint32_t mask = (1u << L.group_shift) - 1u;
-    const uint32_t b[3] = {record.pos & mask, (record.pos >> L.group_shift) & mask, record.pos >> (2 * L.group_shift)};
-    V local = Add(origin, Mk(float(b[0] * 8 + (vi & 7)), float(b[1] * 8 + ((vi >> 3) & 7)), float(b[2] * 8 + (vi >> 6))));
+    const uint32_t v[3] = {(record.pos & mask) * 8 + (vi & 7), ((record.pos >> L.group_shift) & mask) * 8 + ((vi >> 3) & 7),
+                           (record.pos >> (2 * L.group_shift)) * 8 + (vi >> 6)};
+    V local = Add(origin, Mk(float(v[0]), float(v[1]), float(v[2])));
     const float a = LightRandomFloat(rng);
     const float c = LightRandomFloat(rng);
     const int axis = int(f >> 1);
     SetComp(local, axis, Comp(local, axis) + ((f & 1) != 0 ? 0.0f : 1.0f));
     SetComp(local, (axis + 1) % 3, Comp(local, (axis + 1) % 3) + a);
     SetComp(local, (axis + 2) % 3, Comp(local, (axis + 2) % 3) + c);
-    p.position = local;
-    p.normal = FaceDirection(f);
+    p.position = Add(Rotate(frame, local), frame.t);
+    p.normal = Rotate(frame, FaceDirection(f));
     p.block = entry >> 16;
+    p.voxel = vi;
     p.face = f;
-    p.record = lf.group;
+    p.voxel_in_group = v[0] | (v[1] << 10) | (v[2] << 20);
     p.pdf = ((w[f] / sum) / float(count)) * pdf_brick;
     return true;
 }
-float LightFacePdf(const CxScene& s, const CxLightTreeView& L, V x, V n, uint32_t record, const int32_t voxel[3],
-                   uint32_t face, uint32_t block, bool& found)
+// LightTreePointPdf (caelix_light_tree.hlsl:936-949) over LightTreeFindLightFace (:853-932): the area
+// pdf of the draw of face `face` of the group-local voxel v of `group`; `found` = a light face whose brick
+// has a leaf (LampHitWeight's condition, caelix_path_trace.hlsl:438-443).
+float LightPointPdf(const Lights& L, V x, V n, uint32_t group, const uint32_t v[3], uint32_t face, uint32_t block_value,
+                    bool& found)
 {
     found = false;
-    const uint32_t code = block & 0xFFFFu;
+    const uint32_t code = MaterialCode(block_value & 0xFFFFu, L.encoding);
     g_demand[g_kind] += 4;
-    if (((s.light_codes[code >> 5] >> (code & 31u)) & 1u) == 0u || face > 5u)
+    if (((L.light_codes[code >> 5] >> (code & 31u)) & 1u) == 0u || face > 5u)
     {
         return 0.0f;
     }
-    const uint32_t ordinal = L.ordinal_of_brick[record];
-    g_demand[g_kind] += 4 + 12;
-    if (ordinal == kNoneU)
+    const uint32_t* block = GroupBlock(L, group);
+    if (block == nullptr)
     {
         return 0.0f;
     }
-    const Rec rec = LoadRec(L, ordinal);
-    const V origin = GroupOrigin(L, s.keys + uint64_t(record) * 3);
+    const uint32_t entry = FindIndexEntry(block, block[0], Morton3(v[0] >> 3, v[1] >> 3, v[2] >> 3), L.group_shift);
+    if (entry == kNoneU)
+    {
+        return 0.0f;
+    }
+    const Rec rec = LoadRec(block, entry);
+    Frame frame;
+    const V origin = GroupFrame(L, group, frame);
     float w[6];
     float sum;
-    FamilyWeights(rec, Sub(x, origin), w, sum);
+    FamilyWeights(rec, ToGroup(frame, x, origin), w, sum);
     const uint32_t count = FamilyCount(rec, face);
     if (sum <= 0.0f || w[face] <= 0.0f || count == 0u)
     {
@@ -929,7 +1071,7 @@ float LightFacePdf(const CxScene& s, const CxLightTreeView& L, V x, V n, uint32_
     {
         first += FamilyCount(rec, g);
     }
-    const uint32_t vi = uint32_t((voxel[0] & 7) | ((voxel[1] & 7) << 3) | ((voxel[2] & 7) << 6));
+    const uint32_t vi = (v[0] & 7u) | ((v[1] & 7u) << 3) | ((v[2] & 7u) << 6);
     bool hit = false;
     for (uint32_t e = 0; e < count; ++e)  // linear scan: the entries are sorted, any search finds it
     {
@@ -944,15 +1086,27 @@ float LightFacePdf(const CxScene& s, const CxLightTreeView& L, V x, V n, uint32_
     {
         return 0.0f;
     }
-    found = true;
     const float in_brick = (w[face] / sum) / float(count);
-    const uint32_t leaf = ordinal - L.leaf_base;
-    if (leaf >= L.headers[0].leaf_count)
+    const uint32_t leaf = BrickLeaf(L, block, group, entry);
+    if (leaf == kNoneU)
     {
-        found = false;
         return 0.0f;
     }
-    return LeafProbability(L, 0, leaf, x, n) * in_brick;
+    found = true;
+    return LeafProbability(L, leaf >> 31, leaf & 0x7FFFFFFFu, x, n) * in_brick;
+}
+// A hit of the world traversal (identity entity frames): the group of the brick record, the voxel less
+// the group origin.
+float LightFacePdf(const CxScene& s, const Lights& L, V x, V n, uint32_t record, const int32_t voxel[3], uint32_t face,
+                   uint32_t block, bool& found)
+{
+    const uint32_t group = s.group_of_brick[record];
+    const GroupInfo& info = L.group_infos[group];
+    const int32_t edge = int32_t(8u << L.group_shift);
+    const uint32_t v[3] = {uint32_t(voxel[0] - info.coord[0] * edge), uint32_t(voxel[1] - info.coord[1] * edge),
+                           uint32_t(voxel[2] - info.coord[2] * edge)};
+    g_demand[g_kind] += 4 + 16;
+    return LightPointPdf(L, x, n, group, v, face, block, found);
 }
 float MisWeight(float p_area, V n_r, V w, float d, float cos_l, bool nee_side)
 {
@@ -1399,7 +1553,7 @@ struct NeeTerm
 
 // NextEventAtVertex (caelix_path_trace.hlsl:356-413): the lamp candidate and, with the sun's NEE,
 // the sun candidate and the choice between them (one-sample RIS), then one shadow ray.
-V NextEvent(const CxScene& s, const CxLightTreeView& L, const Sky& k, V x_r, V n_r, uint32_t& light_rng,
+V NextEvent(const CxScene& s, const Lights& L, const Sky& k, V x_r, V n_r, uint32_t& light_rng,
             uint32_t& nee_rng, Counters& c, NeeTerm& term, bool transmissive = false, uint32_t medium = 0u)
 {
     term = NeeTerm{};
@@ -1410,7 +1564,7 @@ V NextEvent(const CxScene& s, const CxLightTreeView& L, const Sky& k, V x_r, V n
     float t_max = 0.0f;
     uint32_t rng = seed;
     LightPoint p;
-    if (LightTreeSample(s, L, x_r, n_r, rng, p) && p.pdf > 0.0f)
+    if (LightTreeSample(L, x_r, n_r, rng, p) && p.pdf > 0.0f)
     {
         const V d = Sub(p.position, x_r);
         const float d2 = Dot(d, d);
@@ -1497,7 +1651,7 @@ V NextEvent(const CxScene& s, const CxLightTreeView& L, const Sky& k, V x_r, V n
     }
     return radiance;
 }
-float LampHitWeight(const CxScene& s, const CxLightTreeView& L, uint32_t segment_light, const WorldHit& hit, V origin,
+float LampHitWeight(const CxScene& s, const Lights& L, uint32_t segment_light, const WorldHit& hit, V origin,
                     V direction, V segment_normal)
 {
     if (segment_light == 0u || hit.t < 8.0f)
@@ -1580,7 +1734,7 @@ void AdjointAlbedoTerms(const CxScene& s, Adjoint& g, const double adj[3], const
 // One pixel sample; radiance accumulated in float64 (the float32 terms are the kernel's). With
 // `adjoint`, also the path-replay gradients of the sample (sampling pdfs, MIS weights and the
 // roulette's normalisation held constant).
-void TracePixel(const CxScene& s, const CxLightTreeView& L, const CxCamera& cam, uint32_t bounces, uint32_t seed,
+void TracePixel(const CxScene& s, const Lights& L, const CxCamera& cam, uint32_t bounces, uint32_t seed,
                 uint32_t flags, const Sky& k, uint32_t px, uint32_t py, DV& radiance, V& albedo, V& emission,
                 float& t_hit, uint32_t& attrib, Counters& c, Adjoint* adjoint = nullptr)
 {
@@ -1775,7 +1929,7 @@ void TracePixel(const CxScene& s, const CxLightTreeView& L, const CxCamera& cam,
 // NEE at a specular vertex; its segment sees the sun disk and counts a lamp in full) and the
 // procedural sky on misses. A chain that leaves the scene shows the sky in its last direction times
 // its transmission. t_hit = the chain's length, attrib = the last surface or interface it met.
-void TracePixelGlass(const CxScene& s, const CxLightTreeView& L, const CxCamera& cam, uint32_t bounces, uint32_t seed,
+void TracePixelGlass(const CxScene& s, const Lights& L, const CxCamera& cam, uint32_t bounces, uint32_t seed,
                      uint32_t flags, const Sky& k, uint32_t px, uint32_t py, DV& radiance, V& albedo, V& emission,
                      float& t_hit, uint32_t& attrib, Counters& c)
 {
@@ -1963,9 +2117,208 @@ void TracePixelGlass(const CxScene& s, const CxLightTreeView& L, const CxCamera&
     }
 }
 
+// LtOpenFaces (caelix_light_tree_build.hlsl:92-111): bit f set when voxel v of `record` has a light
+// material code and its face f is open (the neighbour, through the hash table in another brick, is not
+// opaque); `block` = the voxel's value.
+uint32_t OpenLightFaces(const CxScene* s, uint32_t record, uint32_t v, uint32_t& block)
+{
+    block = s->payload[uint64_t(record) * 512 + v];
+    const uint32_t code = MaterialCode(block, s->encoding);
+    if (((s->light_codes[code >> 5] >> (code & 31u)) & 1u) == 0u)
+    {
+        return 0u;
+    }
+    const int p0[3] = {int(v & 7u), int((v >> 3) & 7u), int(v >> 6)};
+    uint32_t open = 0;
+    for (uint32_t f = 0; f < 6; ++f)
+    {
+        int p[3] = {p0[0], p0[1], p0[2]};
+        const int axis = int(f >> 1);
+        const int sign = (f & 1u) ? -1 : 1;
+        p[axis] += sign;
+        uint32_t other = record;
+        if (p[axis] < 0 || p[axis] > 7)
+        {
+            int32_t k[3] = {s->keys[3 * uint64_t(record)], s->keys[3 * uint64_t(record) + 1], s->keys[3 * uint64_t(record) + 2]};
+            k[axis] += sign;
+            p[axis] = (p[axis] + 8) & 7;
+            uint32_t probes;
+            other = HashGet(*s, k[0], k[1], k[2], probes);
+        }
+        const uint32_t nb = other == kNoneU ? 0u : s->payload[uint64_t(other) * 512 + p[0] + 8 * p[1] + 64 * p[2]];
+        if (!Opaque(nb, s->encoding))
+        {
+            open |= 1u << f;
+        }
+    }
+    return open;
+}
+
+uint32_t LightFaceCount(uint32_t word1)
+{
+    return (word1 >> kLtRecordFaceCountShift) & kLtRecordFaceCountMask;
+}
+
+// LtRecordUnits (caelix_light_tree_group.hlsl:209-212).
+uint32_t RecordUnitsOf(uint32_t entries)
+{
+    return (48u + 4u * entries + 15u) / 16u;
+}
+
+// LtBuildBrick's recomputed record (caelix_light_tree_build.hlsl:161-282) at `rec` with room for
+// `units` units: the open faces of the brick's light voxels in voxel order per family, at most the
+// finalize's face count kept (families in order), face weights (luminance of the emission) summed per
+// run of 8 voxels and then over the 64 runs in order, the tight box of the kept voxels (the brick
+// corner when none), zero padding. Returns the entries written.
+uint32_t RecomputeRecord(const CxScene* s, uint32_t record, uint32_t pos, uint32_t shift, uint32_t units, uint32_t* rec)
+{
+    uint32_t opens[512], blocks[512];
+    uint32_t found[6] = {};
+    for (uint32_t v = 0; v < 512; ++v)
+    {
+        opens[v] = OpenLightFaces(s, record, v, blocks[v]);
+        for (uint32_t f = 0; f < 6; ++f)
+        {
+            found[f] += (opens[v] >> f) & 1u;
+        }
+    }
+    uint32_t left = LightFaceCount(s->info[2 * uint64_t(record) + 1]);
+    uint32_t family_count[6], family_base[6], cursor[6] = {};
+    uint32_t written = 0;
+    for (uint32_t f = 0; f < 6; ++f)
+    {
+        family_count[f] = std::min(found[f], left);
+        family_base[f] = written;
+        written += family_count[f];
+        left -= family_count[f];
+    }
+    float face[6] = {};
+    uint32_t lo[3] = {8, 8, 8}, hi[3] = {0, 0, 0};
+    for (uint32_t run = 0; run < 64; ++run)
+    {
+        float sums[6] = {};
+        for (uint32_t i = 0; i < 8; ++i)
+        {
+            const uint32_t v = run * 8 + i;
+            if (opens[v] == 0)
+            {
+                continue;
+            }
+            const float* e = s->materials + uint64_t(MaterialCode(blocks[v], s->encoding)) * 8 + 4;
+            const float weight = e[0] * 0.2126f + e[1] * 0.7152f + e[2] * 0.0722f;
+            bool kept = false;
+            for (uint32_t f = 0; f < 6; ++f)
+            {
+                if ((opens[v] >> f) & 1u)
+                {
+                    if (cursor[f] < family_count[f])
+                    {
+                        rec[12 + family_base[f] + cursor[f]] = v | (blocks[v] << 16);
+                        sums[f] += weight;
+                        kept = true;
+                    }
+                    ++cursor[f];
+                }
+            }
+            if (kept)
+            {
+                const uint32_t c[3] = {v & 7u, (v >> 3) & 7u, v >> 6};
+                for (int a = 0; a < 3; ++a)
+                {
+                    lo[a] = std::min(lo[a], c[a]);
+                    hi[a] = std::max(hi[a], c[a] + 1);
+                }
+            }
+        }
+        for (uint32_t f = 0; f < 6; ++f)
+        {
+            face[f] += sums[f];
+        }
+    }
+    const uint32_t mask = (1u << shift) - 1u;
+    const uint32_t o[3] = {(pos & mask) * 8, ((pos >> shift) & mask) * 8, (pos >> (2 * shift)) * 8};
+    for (int a = 0; a < 3; ++a)
+    {
+        lo[a] = o[a] + (written == 0 ? 0u : lo[a]);
+        hi[a] = o[a] + (written == 0 ? 0u : hi[a]);
+    }
+    std::memcpy(rec, face, 24);
+    rec[6] = lo[0] | (lo[1] << 10) | (lo[2] << 20);
+    rec[7] = hi[0] | (hi[1] << 10) | (hi[2] << 20);
+    rec[8] = pos;
+    rec[9] = written;
+    rec[10] = family_count[0] | (family_count[1] << 10) | (family_count[2] << 20);
+    rec[11] = family_count[3] | (family_count[4] << 10) | (family_count[5] << 20);
+    for (uint32_t e = written; e < (units * 16 - 48) / 4; ++e)
+    {
+        rec[12 + e] = 0u;
+    }
+    return written;
+}
+
+// LtWorldSummary (caelix_light_gather_emit.hlsl:11-34).
+LightSummary WorldSummaryOf(const float face[6], const float lo[3], const float hi[3], const float rows[3][4])
+{
+    LightSummary w;
+    float c[3], h[3];
+    for (int k = 0; k < 3; ++k)
+    {
+        c[k] = 0.5f * (lo[k] + hi[k]);
+        h[k] = 0.5f * (hi[k] - lo[k]);
+    }
+    for (int r = 0; r < 3; ++r)
+    {
+        const float wc = ((rows[r][0] * c[0] + rows[r][1] * c[1]) + rows[r][2] * c[2]) + rows[r][3];
+        const float wh = (std::fabs(rows[r][0]) * h[0] + std::fabs(rows[r][1]) * h[1]) + std::fabs(rows[r][2]) * h[2];
+        w.lo[r] = wc - wh;
+        w.hi[r] = wc + wh;
+    }
+    for (int f = 0; f < 6; ++f)
+    {
+        w.face[f] = 0.0f;
+    }
+    for (int f = 0; f < 6; ++f)
+    {
+        const int axis = f >> 1;
+        const float sign = (f & 1) != 0 ? -1.0f : 1.0f;
+        for (int b = 0; b < 3; ++b)
+        {
+            const float d = sign * rows[b][axis];
+            w.face[2 * b] += face[f] * std::fmax(0.0f, d);
+            w.face[2 * b + 1] += face[f] * std::fmax(0.0f, -d);
+        }
+    }
+    return w;
+}
+
 }  // namespace
 
 extern "C" {
+
+// The finalize's light state (caelix_finalize.hlsl:22-29, 326-340), after oracle_scene_finalize (which
+// leaves word 1's light bits as it found them): per brick the open faces of its light voxels into word 1
+// bits 18..29; when the old count or the new one is not 0 the stale bit (bit 30) is set and the brick's
+// group row gets kLtGpuLightChanged; a set stale bit is kept.
+void oracle_scene_light_state(CxScene* s)
+{
+    for (uint32_t b = 0; b < s->bricks; ++b)
+    {
+        uint32_t faces = 0;
+        for (uint32_t v = 0; v < 512; ++v)
+        {
+            uint32_t block;
+            faces += uint32_t(__builtin_popcount(OpenLightFaces(s, b, v, block)));
+        }
+        const uint32_t old_word1 = s->info[2 * uint64_t(b) + 1];
+        uint32_t word1 = (old_word1 & kRecordBoundsMask) | (faces << kLtRecordFaceCountShift) | (old_word1 & kLtRecordStaleBit);
+        if (LightFaceCount(old_word1) != 0 || faces != 0)
+        {
+            word1 |= kLtRecordStaleBit;
+            s->light_group_rows[s->group_of_brick[b]].gpu_flags |= kLtGpuLightChanged;
+        }
+        s->info[2 * uint64_t(b) + 1] = word1;
+    }
+}
 
 // The finalize kernel (shaders/scene/caelix_finalize.hlsl:205-247, 112-361): per voxel the six
 // neighbours (another brick's through the hash table, air when absent) give the face bits of a
@@ -2059,8 +2412,10 @@ void oracle_scene_finalize(CxScene* s)
         const uint32_t pos = (uint32_t(key[0]) & mask) | ((uint32_t(key[1]) & mask) << s->group_shift) |
                              ((uint32_t(key[2]) & mask) << (2 * s->group_shift));
         s->info[2 * uint64_t(b)] = (pos & kRecordPosMask) | (coarse << kRecordCoarseShift);
+        // the light bits of word 1 stay for the light state (0 without it)
+        const uint32_t light_bits = s->light_group_rows != nullptr ? (s->info[2 * uint64_t(b) + 1] & ~kRecordBoundsMask) : 0u;
         s->info[2 * uint64_t(b) + 1] =
-            any ? (lo[0] | (lo[1] << 3) | (lo[2] << 6) | (hi[0] << 9) | (hi[1] << 12) | (hi[2] << 15)) : 0u;
+            (any ? (lo[0] | (lo[1] << 3) | (lo[2] << 6) | (hi[0] << 9) | (hi[1] << 12) | (hi[2] << 15)) : 0u) | light_bits;
         const int32_t g[3] = {key[0] - s->grid_min[0], key[1] - s->grid_min[1], key[2] - s->grid_min[2]};
         if (any && uint32_t(g[0]) < s->grid_dim && uint32_t(g[1]) < s->grid_dim && uint32_t(g[2]) < s->grid_dim)
         {
@@ -2071,177 +2426,280 @@ void oracle_scene_finalize(CxScene* s)
             s->super_bits[j >> 5] |= 1u << (j & 31u);
         }
     }
+    if (s->light_group_rows != nullptr)
+    {
+        oracle_scene_light_state(s);
+    }
 }
 
-// The light pass: word 1 light face counts, records, leaves, ordinals. Returns the light bricks.
-uint32_t oracle_scene_light_records(CxScene* s, uint32_t* heap, uint32_t heap_units, uint32_t* record_offset,
-                                    uint32_t* ordinal_of_brick, LightBvhLeaf* leaves, uint32_t leaf_capacity,
-                                    uint32_t* totals)
+// The group pass (shaders/light/records/caelix_light_tree_{size,scan,build,commit}.hlsl,
+// caelix_light_tree_group.hlsl:129-243), sequentially over the group indices, on host tables named by a
+// descriptor of host addresses. The reference scans a group's dense ids; here the bricks of group g are
+// those with group_of_brick = g, and a brick's dense id is its record. Per group: the staged flags merged,
+// the mode (released, rebuild: kLtCpuRebuild or kLtGpuLightChanged, else copy); a rebuilt group's light
+// bricks by Morton position (a bitmap; above 4096 the lowest positions kept), each planned as a copy of
+// its old record (not stale, an old block that is not fresh, an entry at its position) or a recompute;
+// sizes; the exclusive scan and the decision; on kLtPassOk the blocks (copied whole, or header, index
+// entries and records), rows, stale bits, statistics and the current buffer.
+void oracle_light_pass(CxScene* s, const CxLightPass* pass)
 {
-    auto block_at = [&](uint32_t record, int x, int y, int z) -> uint32_t {
-        return s->payload[uint64_t(record) * 512 + x + 8 * y + 64 * z];
+    const LightTreeDescriptor* d = pass->descriptor;
+    LightControl* control = reinterpret_cast<LightControl*>(d->control);
+    GroupLightRow* rows = reinterpret_cast<GroupLightRow*>(d->group_rows);
+    const GroupInfo* infos = reinterpret_cast<const GroupInfo*>(d->group_infos);
+    const uint32_t* motion = reinterpret_cast<const uint32_t*>(d->entity_motion);
+    const uint32_t shift = d->group_shift;
+    const uint32_t G = pass->group_count;
+    const uint32_t current = control->current & 1u;
+    const uint32_t* source = reinterpret_cast<const uint32_t*>(d->heap[current]);
+    uint32_t* target = reinterpret_cast<uint32_t*>(d->heap[1 - current]);
+    std::vector<std::vector<uint32_t>> members(G);
+    for (uint32_t b = 0; b < s->bricks; ++b)
+    {
+        members[s->group_of_brick[b]].push_back(b);
+    }
+    struct Plan
+    {
+        uint32_t record, pos, units, old_record;  // old_record kNoneU: recompute
     };
-    // LtOpenFaces (caelix_light_tree_build.hlsl:92-111): the light code of the voxel's material code.
-    auto open_faces = [&](uint32_t record, uint32_t v, uint32_t& block) -> uint32_t {
-        block = s->payload[uint64_t(record) * 512 + v];
-        const uint32_t code = MaterialCode(block, s->encoding);
-        if (((s->light_codes[code >> 5] >> (code & 31u)) & 1u) == 0u)
+    std::vector<uint32_t> mode(G), sizes(G);
+    std::vector<std::vector<Plan>> plans(G);
+    std::vector<uint32_t> dropped(G, 0);
+    uint32_t pass_flags = control->pass_flags;
+    // size kernel (caelix_light_tree_size.hlsl)
+    for (uint32_t g = 0; g < G; ++g)
+    {
+        GroupLightRow& row = rows[g];
+        uint32_t cpu = row.cpu_flags;
+        if (pass->staged_flags[g] != 0)
         {
-            return 0u;
+            cpu = (cpu & kLtCpuFresh) | pass->staged_flags[g];
+            row.cpu_flags = cpu;
+            pass->staged_flags[g] = 0;
         }
-        const int p0[3] = {int(v & 7u), int((v >> 3) & 7u), int(v >> 6)};
-        uint32_t open = 0;
-        for (uint32_t f = 0; f < 6; ++f)
+        mode[g] = (cpu & kLtCpuReleased) != 0 ? 2u
+                  : ((cpu & kLtCpuRebuild) != 0 || (row.gpu_flags & kLtGpuLightChanged) != 0) ? 1u
+                                                                                               : 0u;
+        uint32_t units = mode[g] == 0 ? row.block_units : 0u;
+        if (mode[g] == 1)
         {
-            int p[3] = {p0[0], p0[1], p0[2]};
-            const int axis = int(f >> 1);
-            const int sign = (f & 1u) ? -1 : 1;
-            p[axis] += sign;
-            uint32_t nb;
-            if (p[axis] >= 0 && p[axis] < 8)
+            // LtScanGroup: the light bricks by Morton position, the lowest kLtMaxLightBricksPerGroup kept
+            std::vector<std::pair<uint32_t, uint32_t>> by_morton;  // (Morton, record)
+            for (uint32_t b : members[g])
             {
-                nb = block_at(record, p[0], p[1], p[2]);
+                if (LightFaceCount(s->info[2 * uint64_t(b) + 1]) != 0)
+                {
+                    const int32_t* k = s->keys + 3 * uint64_t(b);
+                    const uint32_t m = (1u << shift) - 1u;
+                    by_morton.emplace_back(Morton3(uint32_t(k[0]) & m, uint32_t(k[1]) & m, uint32_t(k[2]) & m), b);
+                }
+            }
+            std::sort(by_morton.begin(), by_morton.end());
+            if (by_morton.size() > kLtMaxLightBricksPerGroup)
+            {
+                dropped[g] = uint32_t(by_morton.size()) - kLtMaxLightBricksPerGroup;
+                by_morton.resize(kLtMaxLightBricksPerGroup);
+            }
+            const bool old_valid = row.block_units != 0 && (cpu & kLtCpuFresh) == 0;
+            const uint32_t* old = source + uint64_t(row.block_offset) * 4;
+            uint32_t records = 0;
+            for (const auto& [m, b] : by_morton)
+            {
+                const uint32_t word1 = s->info[2 * uint64_t(b) + 1];
+                const int32_t* k = s->keys + 3 * uint64_t(b);
+                const uint32_t mk = (1u << shift) - 1u;
+                Plan p{b, (uint32_t(k[0]) & mk) | ((uint32_t(k[1]) & mk) << shift) | ((uint32_t(k[2]) & mk) << (2 * shift)),
+                       RecordUnitsOf(LightFaceCount(word1)), kNoneU};
+                if ((word1 & kLtRecordStaleBit) == 0 && old_valid)
+                {
+                    const uint32_t e = FindIndexEntry(old, old[0], m, shift);
+                    if (e != kNoneU)
+                    {
+                        p.old_record = IndexEntryOf(old, e).record_offset;
+                        p.units = RecordUnitsOf(old[uint64_t(p.old_record) * 4 + 9]);
+                    }
+                }
+                records += p.units;
+                plans[g].push_back(p);
+            }
+            units = plans[g].empty() ? 0u : 1u + uint32_t(plans[g].size()) + records;
+        }
+        sizes[g] = units;
+        if (mode[g] != 0)
+        {
+            pass_flags |= kLtPassFlagChanged | (1u << (kLtPassFlagTreeShift + (motion[infos[g].entity] & 1u)));
+        }
+    }
+    // scan kernel (caelix_light_tree_scan.hlsl)
+    uint64_t carry = 0;
+    std::vector<uint32_t> offsets(G);
+    for (uint32_t g = 0; g < G; ++g)
+    {
+        offsets[g] = uint32_t(std::min(carry, uint64_t(0xFFFFFFFFu)));
+        carry += sizes[g];
+    }
+    const uint32_t changed = pass_flags & kLtPassFlagChanged;
+    const uint32_t result = changed == 0 ? kLtPassNoChange : (carry <= pass->heap_units ? kLtPassOk : kLtPassFailed);
+    control->pass_result = result;
+    if (changed != 0)
+    {
+        control->needed_units = uint32_t(std::min(carry, uint64_t(0xFFFFFFFFu)));
+        control->records_recomputed = 0;
+        control->records_copied = 0;
+    }
+    control->trees_changed = (pass_flags >> kLtPassFlagTreeShift) & 3u;
+    control->pass_flags = 0;
+    if (result == kLtPassFailed)
+    {
+        control->passes += 1;
+        control->failed_passes += 1;
+        return;
+    }
+    if (result != kLtPassOk)
+    {
+        return;
+    }
+    // build kernel (caelix_light_tree_build.hlsl)
+    uint32_t recomputed = 0, copied = 0;
+    for (uint32_t g = 0; g < G; ++g)
+    {
+        GroupLightRow& row = rows[g];
+        if (mode[g] == 0)
+        {
+            std::memcpy(target + uint64_t(offsets[g]) * 4, source + uint64_t(row.block_offset) * 4,
+                        uint64_t(row.block_units) * 16);
+            row.block_offset = row.block_units != 0 ? offsets[g] : 0u;
+            continue;
+        }
+        if (mode[g] == 2)
+        {
+            row = GroupLightRow{0, 0, ((row.generation >> 16) + 1u) << 16, 0, 0, 0, 0, 0};
+            continue;
+        }
+        const uint32_t kept = uint32_t(plans[g].size());
+        uint32_t* block = target + uint64_t(offsets[g]) * 4;
+        const uint32_t* old = source + uint64_t(row.block_offset) * 4;
+        uint32_t at = 1 + kept;  // record units from the block start
+        uint32_t faces = 0;
+        if (kept != 0)
+        {
+            block[0] = kept;
+            block[1] = block[2] = block[3] = 0;
+        }
+        for (uint32_t r = 0; r < kept; ++r)
+        {
+            const Plan& p = plans[g][r];
+            uint32_t* entry = block + 4 + uint64_t(r) * 4;
+            entry[0] = p.pos;
+            entry[1] = p.record;
+            entry[2] = at;
+            entry[3] = kNoneU;
+            uint32_t* rec = block + uint64_t(at) * 4;
+            if (p.old_record != kNoneU)
+            {
+                std::memcpy(rec, old + uint64_t(p.old_record) * 4, uint64_t(p.units) * 16);
+                faces += rec[9];
+                ++copied;
             }
             else
             {
-                int32_t k[3] = {s->keys[3 * uint64_t(record)], s->keys[3 * uint64_t(record) + 1], s->keys[3 * uint64_t(record) + 2]};
-                k[axis] += sign;
-                p[axis] = (p[axis] + 8) & 7;
-                uint32_t probes;
-                const uint32_t other = HashGet(*s, k[0], k[1], k[2], probes);
-                nb = other == kNoneU ? 0u : block_at(other, p[0], p[1], p[2]);
+                faces += RecomputeRecord(s, p.record, p.pos, shift, p.units, rec);
+                ++recomputed;
             }
-            if (!Opaque(nb, s->encoding))
-            {
-                open |= 1u << f;
-            }
+            at += p.units;
         }
-        return open;
-    };
-    uint32_t ordinal = 0;
-    uint32_t offset = 0;
-    uint32_t faces_total = 0;
-    totals[3] = 0;
-    for (uint32_t b = 0; b < s->bricks; ++b)
-    {
-        uint32_t opens[512];
-        uint32_t blocks[512];
-        uint32_t count = 0;
-        for (uint32_t v = 0; v < 512; ++v)
+        for (uint32_t b : members[g])
         {
-            opens[v] = open_faces(b, v, blocks[v]);
-            count += uint32_t(__builtin_popcount(opens[v]));
+            s->info[2 * uint64_t(b) + 1] &= ~kLtRecordStaleBit;
         }
-        count = std::min(count, 0xFFFu);
-        s->info[2 * uint64_t(b) + 1] = (s->info[2 * uint64_t(b) + 1] & 0x3FFFFu) | (count << 18);
-        if (count == 0)
-        {
-            ordinal_of_brick[b] = kNoneU;
-            continue;
-        }
-        const uint32_t units = (48 + count * 4 + 15) / 16;
-        if (uint64_t(offset) + units > heap_units)
-        {
-            ordinal_of_brick[b] = kNoneU;
-            totals[3] = 1;
-            offset += units;
-            ordinal += 1;
-            continue;
-        }
-        uint32_t* rec = heap + uint64_t(offset) * 4;
-        uint32_t family_count[6] = {};
-        for (uint32_t v = 0; v < 512; ++v)
-        {
-            for (uint32_t f = 0; f < 6; ++f)
-            {
-                family_count[f] += (opens[v] >> f) & 1u;
-            }
-        }
-        uint32_t base[6];
-        uint32_t written = 0;
-        for (uint32_t f = 0; f < 6; ++f)
-        {
-            base[f] = written;
-            written += family_count[f];
-        }
-        // Face sums: per run of 8 consecutive voxels, then the 64 runs in order.
-        float face[6] = {};
-        uint32_t cursor[6] = {};
-        uint32_t lo[3] = {8, 8, 8}, hi[3] = {0, 0, 0};
-        for (uint32_t run = 0; run < 64; ++run)
-        {
-            float sums[6] = {};
-            for (uint32_t i = 0; i < 8; ++i)
-            {
-                const uint32_t v = run * 8 + i;
-                if (opens[v] == 0)
-                {
-                    continue;
-                }
-                const float* e = s->materials + uint64_t(MaterialCode(blocks[v], s->encoding)) * 8 + 4;
-                const float weight = e[0] * 0.2126f + e[1] * 0.7152f + e[2] * 0.0722f;
-                for (uint32_t f = 0; f < 6; ++f)
-                {
-                    if ((opens[v] >> f) & 1u)
-                    {
-                        rec[12 + base[f] + cursor[f]++] = v | (blocks[v] << 16);
-                        sums[f] += weight;
-                    }
-                }
-                const uint32_t x = v & 7u, y = (v >> 3) & 7u, z = v >> 6;
-                lo[0] = std::min(lo[0], x);
-                lo[1] = std::min(lo[1], y);
-                lo[2] = std::min(lo[2], z);
-                hi[0] = std::max(hi[0], x + 1);
-                hi[1] = std::max(hi[1], y + 1);
-                hi[2] = std::max(hi[2], z + 1);
-            }
-            for (uint32_t f = 0; f < 6; ++f)
-            {
-                face[f] += sums[f];
-            }
-        }
-        const int32_t* key = s->keys + uint64_t(b) * 3;
-        const int32_t g = int32_t(s->group_shift);
-        const uint32_t mask = (1u << s->group_shift) - 1u;
-        const uint32_t bx = uint32_t(key[0]) & mask, by = uint32_t(key[1]) & mask, bz = uint32_t(key[2]) & mask;
-        std::memcpy(rec, face, 24);
-        rec[6] = (bx * 8 + lo[0]) | ((by * 8 + lo[1]) << 10) | ((bz * 8 + lo[2]) << 20);
-        rec[7] = (bx * 8 + hi[0]) | ((by * 8 + hi[1]) << 10) | ((bz * 8 + hi[2]) << 20);
-        rec[8] = bx | (by << s->group_shift) | (bz << (2 * s->group_shift));
-        rec[9] = written;
-        rec[10] = family_count[0] | (family_count[1] << 10) | (family_count[2] << 20);
-        rec[11] = family_count[3] | (family_count[4] << 10) | (family_count[5] << 20);
-        for (uint32_t w = 12 + written; w < ((12 + written + 3) / 4) * 4; ++w)
-        {
-            rec[w] = 0u;
-        }
-        record_offset[ordinal] = offset;
-        ordinal_of_brick[b] = ordinal;
-        faces_total += written;
-        if (ordinal < leaf_capacity)
-        {
-            LightBvhLeaf& leaf = leaves[ordinal];
-            std::memcpy(leaf.summary.face, face, 24);
-            const float o[3] = {float((key[0] >> g) << g) * 8.0f, float((key[1] >> g) << g) * 8.0f, float((key[2] >> g) << g) * 8.0f};
-            const uint32_t bb[3] = {bx, by, bz};
-            for (int a = 0; a < 3; ++a)
-            {
-                leaf.summary.lo[a] = o[a] + float(bb[a] * 8 + lo[a]);
-                leaf.summary.hi[a] = o[a] + float(bb[a] * 8 + hi[a]);
-            }
-            leaf.group = b;
-            leaf.entry = ordinal;
-            leaf.pad = 0;
-            leaf.parent = kNoneU;
-        }
-        offset += units;
-        ordinal += 1;
+        const uint32_t units = kept == 0 ? 0u : at;
+        const uint32_t generation = (row.generation & ~0xFFFFu) | ((row.generation + 1u) & 0xFFFFu);
+        row = GroupLightRow{units != 0 ? offsets[g] : 0u, units, generation, 0, 0,
+                            kept | (std::min(dropped[g], 0xFFFFu) << 16), faces, 0};
     }
-    totals[0] = ordinal;
-    totals[1] = offset;
-    totals[2] = faces_total;
-    return ordinal;
+    // commit kernel (caelix_light_tree_commit.hlsl)
+    uint64_t bricks = 0, faces = 0, dropped_total = 0;
+    uint32_t groups = 0;
+    for (uint32_t g = 0; g < G; ++g)
+    {
+        bricks += rows[g].bricks & 0xFFFFu;
+        faces += rows[g].light_faces;
+        dropped_total += rows[g].bricks >> 16;
+        groups += (rows[g].bricks & 0xFFFFu) != 0 ? 1u : 0u;
+    }
+    control->current = 1u - current;
+    control->light_groups = groups;
+    control->bricks_dropped = uint32_t(std::min(dropped_total, uint64_t(0xFFFFFFFFu)));
+    control->light_bricks = bricks;
+    control->light_faces = faces;
+    control->records_recomputed = recomputed;
+    control->records_copied = copied;
+    control->passes += 1;
+}
+
+// The gather of tree `tree` (caelix_light_gather_count.hlsl, caelix_light_bvh_scan.hlsl,
+// caelix_light_gather_emit.hlsl:36-86), gated by state->building; entity_leaf_base (or null) numbers
+// the bvh_leaf words per entity.
+void oracle_light_gather(const CxLightPass* pass, uint32_t tree, LightBvhLeaf* leaves, uint32_t leaf_capacity,
+                         LightTreeState* state, const uint32_t* entity_leaf_base)
+{
+    if (state->building == 0)
+    {
+        return;
+    }
+    const LightTreeDescriptor* d = pass->descriptor;
+    const LightControl* control = reinterpret_cast<const LightControl*>(d->control);
+    const GroupLightRow* rows = reinterpret_cast<const GroupLightRow*>(d->group_rows);
+    const GroupInfo* infos = reinterpret_cast<const GroupInfo*>(d->group_infos);
+    const EntityInfo* entities = reinterpret_cast<const EntityInfo*>(d->entity_infos);
+    const uint32_t* motion = reinterpret_cast<const uint32_t*>(d->entity_motion);
+    uint32_t* heap = reinterpret_cast<uint32_t*>(d->heap[control->current & 1u]);
+    const uint32_t G = pass->group_count;
+    std::vector<uint32_t> offsets(G + 1, 0);
+    for (uint32_t g = 0; g < G; ++g)
+    {
+        uint32_t count = 0;
+        if (rows[g].block_units != 0 && (rows[g].cpu_flags & kLtCpuReleased) == 0 &&
+            (motion[infos[g].entity] & 1u) == tree)
+        {
+            count = heap[uint64_t(rows[g].block_offset) * 4];
+        }
+        offsets[g + 1] = offsets[g] + count;
+    }
+    state->leaves_found = offsets[G];
+    for (uint32_t g = 0; g < G; ++g)
+    {
+        uint32_t* block = heap + uint64_t(rows[g].block_offset) * 4;
+        const GroupInfo& info = infos[g];
+        const int32_t edge = int32_t(8u << d->group_shift);
+        const float origin[3] = {float(info.coord[0] * edge), float(info.coord[1] * edge), float(info.coord[2] * edge)};
+        const uint32_t base = entity_leaf_base != nullptr ? entity_leaf_base[info.entity] : 0u;
+        for (uint32_t k = 0; k < offsets[g + 1] - offsets[g]; ++k)
+        {
+            const uint32_t leaf = offsets[g] + k;
+            uint32_t* entry = block + 4 + uint64_t(k) * 4;
+            if (leaf >= leaf_capacity)
+            {
+                entry[3] = kNoneU;
+                continue;
+            }
+            const uint32_t* rec = block + uint64_t(entry[2]) * 4;
+            float face[6], lo[3], hi[3];
+            std::memcpy(face, rec, 24);
+            for (int c = 0; c < 3; ++c)
+            {
+                lo[c] = origin[c] + float((rec[6] >> (10 * c)) & 0x3FFu);
+                hi[c] = origin[c] + float((rec[7] >> (10 * c)) & 0x3FFu);
+            }
+            LightBvhLeaf& out = leaves[leaf];
+            out.summary = WorldSummaryOf(face, lo, hi, entities[info.entity].cur_rows);
+            out.group = g;
+            out.entry = k;
+            out.pad = 0;
+            out.parent = kNoneU;
+            entry[3] = (tree << 31) | (leaf - base);
+        }
+    }
 }
 
 void oracle_trace_rays(const CxScene* s, const float* rays, uint32_t count, uint32_t* hits)
@@ -2263,16 +2721,17 @@ void oracle_trace_rays(const CxScene* s, const float* rays, uint32_t count, uint
     }
 }
 
-void oracle_light_sample(const CxScene* s, const CxLightTreeView* L, const float* receivers, const uint32_t* seeds,
-                         uint32_t count, float* samples, uint32_t* ok)
+void oracle_light_sample(const CxLightTreeView* view, const float* receivers, const uint32_t* seeds, uint32_t count,
+                         float* samples, uint32_t* ok)
 {
+    const Lights L = OpenLights(*view);
     for (uint32_t i = 0; i < count; ++i)
     {
         const float* r = receivers + uint64_t(i) * 6;
         uint32_t rng = seeds[i];
         LightPoint p;
-        const bool drawn = LightTreeSample(*s, *L, Mk(r[0], r[1], r[2]), Mk(r[3], r[4], r[5]), rng, p);
-        float* o = samples + uint64_t(i) * 9;
+        const bool drawn = LightTreeSample(L, Mk(r[0], r[1], r[2]), Mk(r[3], r[4], r[5]), rng, p);
+        float* o = samples + uint64_t(i) * 12;
         o[0] = p.position.x;
         o[1] = p.position.y;
         o[2] = p.position.z;
@@ -2282,24 +2741,27 @@ void oracle_light_sample(const CxScene* s, const CxLightTreeView* L, const float
         o[6] = p.pdf;
         std::memcpy(&o[7], &p.block, 4);
         std::memcpy(&o[8], &p.face, 4);
+        std::memcpy(&o[9], &p.group, 4);
+        std::memcpy(&o[10], &p.entry, 4);
+        std::memcpy(&o[11], &p.voxel_in_group, 4);
         ok[i] = drawn ? 1u : 0u;
     }
 }
 
-void oracle_light_face_pdf(const CxScene* s, const CxLightTreeView* L, const uint32_t* queries, uint32_t count, float* pdf)
+void oracle_light_point_pdf(const CxLightTreeView* view, const uint32_t* queries, uint32_t count, float* pdf)
 {
+    const Lights L = OpenLights(*view);
     for (uint32_t i = 0; i < count; ++i)
     {
         const uint32_t* q = queries + uint64_t(i) * 12;
         float f[6];
         std::memcpy(f, q, 24);
-        const int32_t voxel[3] = {int32_t(q[7]), int32_t(q[8]), int32_t(q[9])};
+        const uint32_t v[3] = {q[7], q[8], q[9]};
         bool found;
-        pdf[i] = LightFacePdf(*s, *L, Mk(f[0], f[1], f[2]), Mk(f[3], f[4], f[5]), q[6], voxel, q[10], q[11], found);
+        pdf[i] = LightPointPdf(L, Mk(f[0], f[1], f[2]), Mk(f[3], f[4], f[5]), q[6], v, q[10], q[11], found);
     }
 }
 
-// MisWeight of both sides per query: {p_area, d, cos_l, n_r (3), w (3)} -> {w_nee, w_bsdf}.
 void oracle_mis_weight(const float* queries, uint32_t count, float* weights)
 {
     for (uint32_t i = 0; i < count; ++i)
@@ -2316,10 +2778,12 @@ void oracle_mis_weight(const float* queries, uint32_t count, float* weights)
 // rays (primary + bounce), shadow rays, brick cells visited, hash probes, brick DDA steps, the demand
 // bytes of every global load the kernels make for those pixels, and that total split by kind of work
 // (primary, bounce and shadow traversal, primary-vertex and bounce-vertex shading).
-void oracle_trace_frame(const CxScene* s, const CxLightTreeView* L, const CxCamera* cam, const CxTraceParams* prm,
+void oracle_trace_frame(const CxScene* s, const CxLightTreeView* view, const CxCamera* cam, const CxTraceParams* prm,
                         uint32_t row_begin, uint32_t row_end, double* color, double* radiance_out, uint32_t* attrib_out,
                         uint64_t* counters)
 {
+    const Lights lights = OpenLights(*view);
+    const Lights* L = &lights;
     uint64_t rays = 0, shadow = 0, cells = 0, probes = 0, steps = 0;
     uint64_t d0 = 0, d1 = 0, d2 = 0, d3 = 0, d4 = 0;
     Sky sky = SkyOf(*prm);
@@ -2403,10 +2867,12 @@ void oracle_trace_frame(const CxScene* s, const CxLightTreeView* L, const CxCame
 // The adjoint of oracle_trace_frame over pixel rows [row_begin, row_end): for d_color = dL/dI of the
 // whole image (rgba per pixel, I = color / spp), adds dL/d(albedo of material id) to
 // grad_albedo[id][3] and dL/d(emission scale of brick) to grad_scale[brick], in float64.
-void oracle_trace_frame_adjoint(const CxScene* s, const CxLightTreeView* L, const CxCamera* cam,
+void oracle_trace_frame_adjoint(const CxScene* s, const CxLightTreeView* view, const CxCamera* cam,
                                 const CxTraceParams* prm, uint32_t row_begin, uint32_t row_end, const float* d_color,
                                 double* grad_albedo, double* grad_scale)
 {
+    const Lights lights = OpenLights(*view);
+    const Lights* L = &lights;
     const int threads = omp_get_max_threads();
     std::vector<std::vector<double>> albedo_parts(threads, std::vector<double>(256 * 3, 0.0));
     std::vector<std::vector<double>> scale_parts(threads, std::vector<double>(s->bricks, 0.0));

@@ -29,7 +29,7 @@ def test_batched_trees_equal_single_trees(batch_and_plain):
     datas, batch, plain = batch_and_plain
     for i, (b, ts) in enumerate(zip(batch.builders, plain.scenes)):
         got = b.read()
-        want = ts.tree.read()
+        want = ts.tree_read()
         assert int(batch.light_bricks[i]) == ts.light_bricks
         assert got["header"].tobytes() == want["header"].tobytes(), i
         count = int(want["header"]["node_count"])

@@ -20,6 +20,7 @@ import pytest
 
 from caelix_amd import scene
 from test_glass_gpu import compare
+import lt_structure as lts
 
 pytestmark = pytest.mark.gpu
 
@@ -48,16 +49,12 @@ def test_hd_scene_records_match_oracle(hd):
     got, want = gpu.host_arrays(), ref.a
     assert len(data.keys) == 1 << 20 and gpu.hash.slot_count == 1497966
     assert np.array_equal(got["slots"], want["slots"].reshape(-1))
-    for name in ("info", "occupancy", "brick_bits", "super_bits", "record_offset", "ordinal_of_brick", "payload"):
+    for name in ("info", "occupancy", "brick_bits", "super_bits", "payload", "group_of_brick"):
         assert np.array_equal(got[name], want[name]), name
-    assert np.array_equal(got["totals"][:3], want["totals"][:3])
-    assert int(want["totals"][0]) == 65536
-    units = int(want["totals"][1])
-    assert np.array_equal(got["heap"][: units * 4], want["heap"][: units * 4])
-    assert np.array_equal(got["headers"], want["headers"])
-    nodes = int(want["headers"][:64].view(np.uint32)[14])
-    assert np.array_equal(got["nodes"][: nodes * 224], want["nodes"][: nodes * 224])
-    assert np.array_equal(got["leaves"][: 65536 * 64], want["leaves"][: 65536 * 64])
+    tables = lts.from_gpu({k[3:]: v for k, v in got.items() if k.startswith("lt_")})
+    lts.compare_tables(tables, lts.from_oracle(ref.lt))
+    assert int(tables["control"]["light_bricks"]) == 65536
+    assert int(tables["headers"][0]["leaf_count"]) == 65536
 
 
 def test_hd_frame_matches_oracle(hd):
@@ -102,7 +99,7 @@ def test_chunk_batch_equals_each_scene_alone(chunks):
     p = plain.pixels
     lights = 0
     for i, (b, ts) in enumerate(zip(batch.builders, plain.scenes)):
-        g, w = b.read(), ts.tree.read()
+        g, w = b.read(), ts.tree_read()
         assert g["header"].tobytes() == w["header"].tobytes(), i
         count = int(w["header"]["node_count"])
         assert g["nodes"][:count].tobytes() == w["nodes"][:count].tobytes(), i

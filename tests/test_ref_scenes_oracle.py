@@ -250,7 +250,7 @@ def test_lamp_box_light_nee_matches_path_tracer(oracle, sun):
     pt = build(oracle, plain).trace_frame(cam, 4, 100000, spp_pt, bounces_transparent=4, sky=sky)[0] / spp_pt
     for name, mis, q in LAMP_VARIANTS:
         o = build(oracle, lit, mis=mis)
-        assert int(o.a["totals"][0]) == 5
+        assert int(o.lt.control[0]["light_bricks"]) == 5
         nee = o.trace_frame(cam, 4, 200000, spp_nee, bounces_transparent=4, sky=sky,
                             bounce_nee_probability=q)[0] / spp_nee
         ref_mean, ratio, psnr = compare_light(pt, nee)

@@ -8,6 +8,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
+import lt_structure as lts
 import trace_checks as tc
 from caelix_amd import scene
 
@@ -35,21 +36,22 @@ def test_scene_records_bit_exact(pair):
     want = ref.a
     assert np.array_equal(got["slots"], want["slots"].reshape(-1))
     assert got["hash_info"][0] == want["hash_info"][0]
-    for name in ("info", "occupancy", "brick_bits", "super_bits", "record_offset", "ordinal_of_brick"):
+    for name in ("info", "occupancy", "brick_bits", "super_bits", "group_of_brick"):
         assert np.array_equal(got[name], want[name]), name
-    assert np.array_equal(got["totals"][:3], want["totals"][:3])
-    units = int(want["totals"][1])
-    assert np.array_equal(got["heap"][: units * 4], want["heap"][: units * 4])
 
 
-def test_light_tree_bit_exact(pair):
+def test_light_tables_bit_exact(pair):
+    """The group pass, gathers and builds of the light tree adapter (after its heap and leaf growth)
+    equal the oracle's; and the GPU's tables pass CheckStructure."""
     gpu, got, ref = pair
-    want = ref.a
-    assert np.array_equal(got["headers"], want["headers"])
-    node_count = int(want["headers"][:64].view(np.uint32)[14])
-    assert np.array_equal(got["nodes"][: node_count * 224], want["nodes"][: node_count * 224])
-    leaves = int(want["totals"][0])
-    assert np.array_equal(got["leaves"][: leaves * 64], want["leaves"][: leaves * 64])
+    tables = lts.from_gpu({k[3:]: v for k, v in got.items() if k.startswith("lt_")})
+    lts.compare_tables(tables, lts.from_oracle(ref.lt))
+    totals = lts.check_structure(SMALL, got["group_of_brick"], gpu.group_coords, tables)
+    assert totals["bricks"] == len(SMALL.emissive_bricks)
+    st = gpu.lights.stats()
+    assert st["light_bricks"] == totals["bricks"] and st["light_faces"] == totals["faces"]
+    assert st["tree_leaves"] == [totals["bricks"], 0] and st["tree_leaves_dropped"] == [0, 0]
+    assert st["passes"] >= 1 and st["passes"] == st["failed_passes"] + 1
 
 
 def _rays(solid, rng, count):

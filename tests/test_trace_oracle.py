@@ -8,6 +8,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
+import lt_structure as lts
 import trace_checks as tc
 from caelix_amd import layout, scene
 
@@ -34,56 +35,20 @@ def test_finalize_words_match_payload(small):
     assert np.array_equal(small.a["super_bits"][: len(super_bits)], super_bits)
 
 
-def test_light_records_match_open_faces(small):
+def test_group_pass_structure_matches_open_faces(small):
+    """CheckStructure (light_tree_test.cpp:641-793) on the oracle's group pass, gather and build; record
+    word 1 holds the light face counts (stale bits cleared by the pass)."""
     open_ = tc.open_light_faces(SMALL)
     counts = open_.sum(axis=(1, 2))
     info = small.a["info"].reshape(-1, 2)
     assert np.array_equal(info[:, 1] >> 18, np.minimum(counts, 0xFFF))
-    light = np.nonzero(counts > 0)[0]
-    totals = small.a["totals"]
-    assert totals[0] == len(light) == len(SMALL.emissive_bricks)
-    assert totals[2] == counts.sum()
-    ordinal = small.a["ordinal_of_brick"]
-    assert np.array_equal(ordinal[light], np.arange(len(light)))
-    assert np.all(ordinal[counts == 0] == 0xFFFFFFFF)
-    heap = small.a["heap"]
-    offsets = small.a["record_offset"]
-    units = (48 + counts[light] * 4 + 15) // 16
-    assert np.array_equal(offsets[: len(light)], np.concatenate([[0], np.cumsum(units)[:-1]]))
-    leaves = small.a["leaves"].view(layout.LEAF)
-    g = SMALL.group_shift
-    mask = (1 << g) - 1
-    lum = (SMALL.materials[:, 4] * np.float32(0.2126) + SMALL.materials[:, 5] * np.float32(0.7152)
-           + SMALL.materials[:, 6] * np.float32(0.0722)).astype(np.float64)
-    for k, b in enumerate(light):
-        rec = heap[offsets[k] * 4:]
-        fam = open_[b].sum(axis=0)
-        assert rec[9] == counts[b]
-        assert rec[10] == fam[0] | (fam[1] << 10) | (fam[2] << 20)
-        assert rec[11] == fam[3] | (fam[4] << 10) | (fam[5] << 20)
-        expect_entries = []
-        for f in range(6):
-            vox = np.nonzero(open_[b, :, f])[0]
-            expect_entries.extend((vox | (SMALL.payload[b, vox].astype(np.int64) << 16)).tolist())
-        assert np.array_equal(rec[12:12 + counts[b]], np.array(expect_entries, np.uint32))
-        face = rec[:6].view(np.float32)
-        weights = lum[SMALL.payload[b]]
-        expect_face = (open_[b] * weights[:, None]).sum(axis=0)
-        np.testing.assert_allclose(face, expect_face, rtol=1e-5)
-        vox = np.nonzero(open_[b].any(axis=1))[0]
-        vx, vy, vz = vox & 7, (vox >> 3) & 7, vox >> 6
-        key = SMALL.keys[b]
-        bxyz = key & mask
-        lo = bxyz * 8 + np.array([vx.min(), vy.min(), vz.min()])
-        hi = bxyz * 8 + np.array([vx.max(), vy.max(), vz.max()]) + 1
-        assert rec[6] == lo[0] | (lo[1] << 10) | (lo[2] << 20)
-        assert rec[7] == hi[0] | (hi[1] << 10) | (hi[2] << 20)
-        origin = ((key >> g) << g) * 8
-        leaf = leaves[k]
-        assert leaf["group"] == b and leaf["entry"] == k
-        np.testing.assert_array_equal(leaf["summary"]["lo"], (origin + lo).astype(np.float32))
-        np.testing.assert_array_equal(leaf["summary"]["hi"], (origin + hi).astype(np.float32))
-        np.testing.assert_array_equal(leaf["summary"]["face"], face)
+    coords = small.lt.infos["coord"]
+    totals = lts.check_structure(SMALL, small.a["group_of_brick"], coords, lts.from_oracle(small.lt))
+    assert totals["bricks"] == len(SMALL.emissive_bricks) and totals["faces"] == counts.sum()
+    assert totals["trees"] == [len(SMALL.emissive_bricks), 0]
+    c = small.lt.control[0]
+    assert int(c["records_recomputed"]) == totals["bricks"] and int(c["records_copied"]) == 0
+    assert int(c["passes"]) == 1 and int(c["failed_passes"]) == 0 and int(c["trees_changed"]) == 1
 
 
 def _air_origins(solid, rng, count, lo, hi):
