@@ -208,6 +208,7 @@ SIGNATURES = {
     "cx_hash_build_bricks": (_INT, [_P, _U32, _P, _U32, _P, _P, _P]),
     "cx_hash_get_bricks": (_INT, [_P, _U32, _P, _P, _U32, _P, _P, _P]),
     "cx_scene_finalize": (_INT, [ctypes.POINTER(CxScene), _P]),
+    "cx_scene_finalize_bricks": (_INT, [ctypes.POINTER(CxScene), _P, _U32, _P]),
     "cx_light_pass_scratch_bytes": (_U64, [_U32, _U32]),
     "cx_light_pass": (_INT, [ctypes.POINTER(CxScene), ctypes.POINTER(CxLightPass), _P]),
     "cx_light_gather": (_INT, [ctypes.POINTER(CxLightPass), _U32, _P, _U32, _P, _P]),

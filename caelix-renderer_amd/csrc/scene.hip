@@ -48,6 +48,7 @@ struct SceneArgs
     uint16_t* payload_out;  // = payload: finalize writes the face bits back
     GroupLightRow* light_rows;        // null: no light state in word 1
     const uint32_t* group_of_brick;
+    const uint32_t* list;             // the records to finalize (block x = list index), or null: every record
 };
 
 // The (x, y, z) brick coordinates of a record and its scene (0 for one scene).
@@ -108,7 +109,7 @@ __global__ void FillKernel(uint32_t* p, uint64_t words)
 
 __global__ void __launch_bounds__(64) FinalizeKernel(SceneArgs s)
 {
-    const uint32_t brick = blockIdx.x;
+    const uint32_t brick = s.list != nullptr ? s.list[blockIdx.x] : blockIdx.x;
     const uint32_t lane = threadIdx.x;
     __shared__ unsigned long long lds_occ[8];
     __shared__ uint4 lds_rows[64];
@@ -346,8 +347,12 @@ CX_DEV uint32_t LightFaceCount(uint32_t word1)
 // kLtGpuLightChanged.
 __global__ void __launch_bounds__(64) LightFacesKernel(SceneArgs s, GroupLightRow* rows, const uint32_t* group_of_brick)
 {
-    const uint32_t record = blockIdx.x;
+    const uint32_t record = s.list != nullptr ? s.list[blockIdx.x] : blockIdx.x;
     const uint32_t lane = threadIdx.x;
+    if (group_of_brick[record] == kLtNone)
+    {
+        return;  // a free record
+    }
     uint32_t faces = 0;
     for (uint32_t i = 0; i < 8; ++i)
     {
@@ -1255,6 +1260,7 @@ SceneArgs ArgsOf(const CxScene& c)
     s.box = c.occupied_box;
     s.light_rows = c.light_group_rows;
     s.group_of_brick = c.group_of_brick;
+    s.list = nullptr;
     return s;
 }
 
@@ -1262,6 +1268,40 @@ SceneArgs ArgsOf(const CxScene& c)
 }  // namespace cx
 
 extern "C" {
+
+CX_API int cx_scene_finalize_bricks(const CxScene* scene, const uint32_t* records, uint32_t count, void* stream)
+{
+    if (scene == nullptr || scene->keys == nullptr || scene->info == nullptr || scene->occupancy == nullptr ||
+        scene->payload == nullptr || scene->brick_bits == nullptr || scene->super_bits == nullptr ||
+        scene->slots == nullptr || scene->hash_info == nullptr || (records == nullptr && count > 0))
+    {
+        return cx::ArgError("cx_scene_finalize_bricks: a buffer is NULL");
+    }
+    if ((reinterpret_cast<uintptr_t>(scene->payload) & 15u) != 0 || scene->grid_dim == 0 || scene->grid_dim % 8 != 0 ||
+        scene->group_shift == 0 || scene->group_shift > 7)
+    {
+        return cx::ArgError("cx_scene_finalize_bricks: payload alignment, grid_dim or group_shift invalid");
+    }
+    if (scene->light_group_rows != nullptr &&
+        (scene->light_codes == nullptr || scene->group_of_brick == nullptr || scene->materials == nullptr))
+    {
+        return cx::ArgError("cx_scene_finalize_bricks: the light state needs light_codes, materials and group_of_brick");
+    }
+    if (count == 0)
+    {
+        return CX_OK;
+    }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    cx::SceneArgs a = cx::ArgsOf(*scene);
+    a.list = records;
+    cx::StageScope scope(s, "scene_finalize");
+    cx::FinalizeKernel<<<count, 64, 0, s>>>(a);
+    if (scene->light_group_rows != nullptr)
+    {
+        cx::LightFacesKernel<<<count, 64, 0, s>>>(a, scene->light_group_rows, scene->group_of_brick);
+    }
+    return cx::LaunchResult("cx_scene_finalize_bricks");
+}
 
 CX_API int cx_scene_finalize(const CxScene* scene, void* stream)
 {

@@ -134,7 +134,7 @@ CX_API int cx_radix_sort_keys64(uint64_t* keys0, uint64_t* keys1, const uint32_t
 
 /* ------------------------------------------------------------------------------------------
  * Hash table (lib/src/rhi/hash_table.{h,cpp}, shaders/common/caelix_hash_table.hlsl,
- * shaders/hash/*.hlsl). The table maps 64-bit key fingerprints of 1..8 key words to payload indices
+ * shaders/hash/ (every kernel)). The table maps 64-bit key fingerprints of 1..8 key words to payload indices
  * in 16-byte HashKeySlot slots. The home slot of a key is key_lo mod slot_count (the reference's
  * key_lo & key_mask for a power-of-two slot count); a lookup probes probe_limit slots from home
  * (kHashProbes = 8 in the reference's form).
@@ -256,6 +256,12 @@ typedef struct CxScene {
  * face bit); and the resident bitmaps; with light_group_rows, the light state of word 1. Running it
  * again changes nothing but the stale bits and row flags of bricks with light faces. */
 CX_API int cx_scene_finalize(const CxScene* scene, void* stream);
+/* The finalize of the listed records only (the reference finalizes the records of a flush's uploads and
+ * their face neighbours, caelix_finalize.hlsl:1-29): the same words, the resident bitmaps and the box only
+ * gain bits (a brick emptied keeps its bits until the next cx_scene_finalize). A record whose
+ * group_of_brick is kLtNone is a free record: it gets no light state. No counterpart name in the
+ * reference (its finalize takes the list in its push constants). */
+CX_API int cx_scene_finalize_bricks(const CxScene* scene, const uint32_t* records, uint32_t count, void* stream);
 
 /* The group pass (shaders/light/records/caelix_light_tree_{size,scan,build,commit}.hlsl), over the tables
  * the descriptor names (control, heap[2], group rows, group infos, entity infos and motion words, light

@@ -189,6 +189,8 @@ def _trace_lib():
         S, T = ctypes.POINTER(_Scene), ctypes.POINTER(_TreeView)
         lib.oracle_scene_finalize.restype = None
         lib.oracle_scene_finalize.argtypes = [S]
+        lib.oracle_scene_finalize_bricks.restype = None
+        lib.oracle_scene_finalize_bricks.argtypes = [S, _P, _U32]
         lib.oracle_light_pass.restype = None
         lib.oracle_light_pass.argtypes = [S, ctypes.POINTER(_Pass)]
         lib.oracle_light_gather.restype = None
@@ -391,6 +393,24 @@ class LightTables:
         return self
 
 
+def light_sample(view: _TreeView, receivers: np.ndarray, seeds: np.ndarray):
+    """LightTreeSample per receiver: ([n, 12] float32 as the C ABI's, ok [n])."""
+    r = np.ascontiguousarray(receivers, np.float32)
+    sd = np.ascontiguousarray(seeds, np.uint32)
+    out = np.zeros((len(r), 12), np.float32)
+    ok = np.zeros(len(r), np.uint32)
+    _trace_lib().oracle_light_sample(ctypes.byref(view), _ptr(r), _ptr(sd), len(r), _ptr(out), _ptr(ok))
+    return out, ok
+
+
+def light_point_pdf(view: _TreeView, queries: np.ndarray) -> np.ndarray:
+    """LightTreePointPdf per query (x, n, group, group-local voxel, face, block; 12 u32)."""
+    q = np.ascontiguousarray(queries, np.uint32)
+    pdf = np.zeros(len(q), np.float32)
+    _trace_lib().oracle_light_point_pdf(ctypes.byref(view), _ptr(q), len(q), _ptr(pdf))
+    return pdf
+
+
 class OracleScene:
     """A scene held in host arrays for the oracle. `arrays` holds the same named arrays as the GPU
     scene (keys, info, occupancy, payload, slots, hash_info, brick_bits, super_bits, materials,
@@ -489,18 +509,10 @@ class OracleScene:
         return hits
 
     def light_sample(self, receivers: np.ndarray, seeds: np.ndarray):
-        r = np.ascontiguousarray(receivers, np.float32)
-        sd = np.ascontiguousarray(seeds, np.uint32)
-        out = np.zeros((len(r), 12), np.float32)
-        ok = np.zeros(len(r), np.uint32)
-        _trace_lib().oracle_light_sample(ctypes.byref(self.view), _ptr(r), _ptr(sd), len(r), _ptr(out), _ptr(ok))
-        return out, ok
+        return light_sample(self.view, receivers, seeds)
 
     def light_point_pdf(self, queries: np.ndarray) -> np.ndarray:
-        q = np.ascontiguousarray(queries, np.uint32)
-        pdf = np.zeros(len(q), np.float32)
-        _trace_lib().oracle_light_point_pdf(ctypes.byref(self.view), _ptr(q), len(q), _ptr(pdf))
-        return pdf
+        return light_point_pdf(self.view, queries)
 
     def point_queries(self, queries: np.ndarray) -> np.ndarray:
         """Face queries {x, n, brick record, world voxel, face, block} as point queries {x, n, group,
@@ -558,3 +570,185 @@ class OracleScene:
                                         ctypes.byref(prm), r0, r1, _ptr(color), _ptr(radiance), _ptr(attrib),
                                         _ptr(counters))
         return color.reshape(-1, 4), radiance.reshape(-1, 4), attrib, counters
+
+
+# ------------------------------------------------------------------------------------------------
+# The light tree's CPU half and an editable world, on the oracle.
+
+class LightTreeModel:
+    """The light tree's CPU state as the scene hooks change it (LightTree::On*, Stage and the tree
+    reasons, light_tree.cpp:113-178, 261-346), restated: entity motions, group lives (fresh, listed,
+    alive), the groups staged for the next pass, the trees to rebuild. Called as a hook sink
+    (`model(name, *args)`)."""
+
+    def __init__(self):
+        self.motion = {}
+        self.motion_dirty = set()
+        self.groups = {}  # group -> [fresh, listed, alive]
+        self.staged = []
+        self.tree_reason = [False, False]
+        self.pass_pending = False
+
+    def _group(self, g):
+        return self.groups.setdefault(g, [False, False, False])
+
+    def _list(self, g):
+        s = self._group(g)
+        if not s[1]:
+            s[1] = True
+            self.staged.append(g)
+
+    def __call__(self, name, *args):
+        if name == "on_entity_added":
+            e, m = args
+            self.motion[e] = m
+            self.motion_dirty.add(e)
+        elif name in ("on_entity_removed", "on_entity_moved"):
+            self.tree_reason[self.motion.get(args[0], 0)] = True
+        elif name == "on_entity_motion":
+            e, m = args
+            if self.motion.get(e, 0) != m:
+                self.motion[e] = m
+                self.motion_dirty.add(e)
+                self.tree_reason = [True, True]
+        elif name == "on_group_created":
+            s = self._group(args[0])
+            s[2] = True
+            s[0] = True
+            self._list(args[0])
+        elif name == "on_group_released":
+            g, m = args
+            self._group(g)[2] = False
+            self._list(g)
+            self.tree_reason[m] = True
+        elif name == "on_group_bricks_removed":
+            self._list(args[0])
+        else:
+            raise ValueError(name)
+
+    def stage(self, finalize_ran: bool, group_count: int):
+        """Stage (light_tree.cpp:261-346): (staged flags [group_count], motion words changed {entity: motion},
+        record the pass, record each tree)."""
+        if finalize_ran or self.staged:
+            self.pass_pending = True
+        flags = np.zeros(max(group_count, 1), np.uint32)
+        for g in self.staged:
+            s = self.groups[g]
+            flags[g] = (CPU_REBUILD | (CPU_FRESH if s[0] else 0)) if s[2] else CPU_RELEASED
+            s[0] = False
+            s[1] = False
+        self.staged = []
+        motions = {e: self.motion[e] for e in self.motion_dirty}
+        self.motion_dirty = set()
+        record_pass = self.pass_pending and group_count > 0
+        self.pass_pending = False
+        trees = [self.tree_reason[t] or record_pass for t in range(2)]
+        reasons = list(self.tree_reason)
+        self.tree_reason = [False, False]
+        return flags, motions, record_pass, trees, reasons
+
+
+class OracleWorld:
+    """An editable world on the oracle: the host arrays of a brick store driven by a world's state (keys,
+    payload, groups, frames, the same records as the GPU world's), its finalize of the changed records,
+    and the light tree's tables across flushes (group pass with the model's staged flags, gathers and
+    builds where the begin kernel would build, caelix_light_bvh_begin.hlsl:9-17)."""
+
+    def __init__(self, materials, light_codes, group_shift: int, encoding: int, capacity: int, grid_dim: int,
+                 entities: int, heap_units: int = 1024):
+        self.model = LightTreeModel()
+        self.shift = int(group_shift)
+        self.encoding = int(encoding)
+        n = self.capacity = int(capacity)
+        self.grid_dim = int(grid_dim)
+        self.entities = int(entities)
+        self.slot_count = int(np.ceil(n / 0.7)) + 1
+        cells = self.grid_dim ** 3
+        self.a = {
+            "keys": np.zeros((n, 3), np.int32), "info": np.zeros(2 * n, np.uint32), "occupancy": np.zeros(8 * n, np.uint64),
+            "payload": np.zeros((n, 512), np.uint16), "slots": np.zeros((self.slot_count, 4), np.uint32),
+            "hash_info": np.zeros(4, np.uint32), "brick_bits": np.zeros((cells + 31) // 32, np.uint32),
+            "super_bits": np.zeros((cells // 512 + 31) // 32 + 1, np.uint32),
+            "materials": np.ascontiguousarray(materials, np.float32).reshape(-1),
+            "light_codes": np.ascontiguousarray(light_codes, np.uint32), "group_of_brick": np.zeros(n, np.uint32),
+        }
+        self.lt = LightTables(np.zeros((0, 3), np.int32), np.zeros(0, np.uint32),
+                              np.tile(np.eye(3, 4, dtype=np.float32), (self.entities, 1, 1)),
+                              np.zeros(self.entities, np.uint32), light_codes, group_shift, encoding, heap_units)
+        self.first = True
+        self.refinalize = False
+        self.builds = [0, 0]
+
+    def set_light_codes(self, codes) -> None:
+        """A new light code set: the next flush finalizes every record again (LightTree::Prepare's
+        requeue, light_tree.cpp:198-259)."""
+        self.a["light_codes"][:] = np.asarray(codes, np.uint32)
+        self.lt.codes[:] = np.asarray(codes, np.uint32)
+        self.refinalize = True
+
+    def _scene(self) -> _Scene:
+        a = self.a
+        s = _Scene()
+        s.keys, s.info, s.occupancy, s.payload = _ptr(a["keys"]), _ptr(a["info"]), _ptr(a["occupancy"]), _ptr(a["payload"])
+        s.slots, s.slot_count, s.bricks = _ptr(a["slots"]), self.slot_count, self.capacity
+        s.hash_info, s.brick_bits, s.super_bits = _ptr(a["hash_info"]), _ptr(a["brick_bits"]), _ptr(a["super_bits"])
+        s.grid_min[:] = [0, 0, 0]
+        s.grid_dim, s.group_shift, s.encoding = self.grid_dim, self.shift, self.encoding
+        s.materials, s.light_codes = _ptr(a["materials"]), _ptr(a["light_codes"])
+        s.emission_scale = 1.0
+        s.group_of_brick = _ptr(a["group_of_brick"])
+        s.light_group_rows = self.lt.rows.ctypes.data
+        return s
+
+    def flush(self, state, records, fresh) -> None:
+        """One flush of the world's changes: `records` to finalize (every record on the first flush), `fresh`
+        records whose word 1 starts at 0."""
+        a, lt = self.a, self.lt
+        a["keys"][:] = state.keys
+        a["payload"][records] = state.payload[records]
+        a["payload"][fresh] = state.payload[fresh]
+        if self.first:
+            a["payload"][:] = state.payload
+        a["group_of_brick"][:] = state.group_of_brick
+        for r in fresh:
+            a["info"][2 * r + 1] = 0
+        slots, longest = hash_build_bricks(a["keys"], self.slot_count)
+        a["slots"][:] = slots
+        a["hash_info"][:2] = [longest + 1, longest]
+        coords = state.group_coords()
+        G = len(coords)
+        if G > lt.groups:
+            rows = np.zeros(G, ROW)
+            rows[: lt.groups] = lt.rows
+            lt.rows = rows
+            lt.staged = np.zeros(G, np.uint32)
+        lt.infos = np.zeros(G, GROUP_INFO)
+        lt.infos["coord"] = coords[:, 1:]
+        lt.infos["entity"] = coords[:, 0]
+        lt.entities["cur_rows"] = state.entity_rows
+        lt.entities["prev_rows"] = state.entity_rows
+        lt.refresh()
+        scene = self._scene()
+        lib = _trace_lib()
+        finalize_ran = self.first or self.refinalize or bool(records)
+        if self.first or self.refinalize:
+            lib.oracle_scene_finalize(ctypes.byref(scene))
+            self.first = False
+            self.refinalize = False
+        elif records:
+            rec = np.ascontiguousarray(records, np.uint32)
+            lib.oracle_scene_finalize_bricks(ctypes.byref(scene), _ptr(rec), len(rec))
+        flags, motions, record_pass, trees, reasons = self.model.stage(finalize_ran, G)
+        for e, m in motions.items():
+            lt.motion[e] = m
+        lt.staged[:G] = flags[:G]
+        if record_pass:
+            lt.run_pass(scene)
+        c = lt.control[0]
+        for t in range(2):
+            if not trees[t]:
+                continue
+            committed = record_pass and int(c["pass_result"]) == PASS_OK and (int(c["trees_changed"]) >> t) & 1
+            if reasons[t] or committed:
+                lt.build_tree(t)
+                self.builds[t] += 1

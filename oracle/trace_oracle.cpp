@@ -2299,10 +2299,15 @@ extern "C" {
 // leaves word 1's light bits as it found them): per brick the open faces of its light voxels into word 1
 // bits 18..29; when the old count or the new one is not 0 the stale bit (bit 30) is set and the brick's
 // group row gets kLtGpuLightChanged; a set stale bit is kept.
-void oracle_scene_light_state(CxScene* s)
+void oracle_scene_light_state_bricks(CxScene* s, const uint32_t* records, uint32_t count)
 {
-    for (uint32_t b = 0; b < s->bricks; ++b)
+    for (uint32_t i = 0; i < count; ++i)
     {
+        const uint32_t b = records != nullptr ? records[i] : i;
+        if (s->group_of_brick[b] == kNoneU)
+        {
+            continue;  // a free record
+        }
         uint32_t faces = 0;
         for (uint32_t v = 0; v < 512; ++v)
         {
@@ -2320,17 +2325,8 @@ void oracle_scene_light_state(CxScene* s)
     }
 }
 
-// The finalize kernel (shaders/scene/caelix_finalize.hlsl:205-247, 112-361): per voxel the six
-// neighbours (another brick's through the hash table, air when absent) give the face bits of a
-// transparent or air voxel (a neighbour that is not opaque and of another medium), written into the
-// payload, and the occupancy: alive (fewer than six opaque neighbours), not 0, opaque or a face bit.
-// Then record words 0/1 (bounds), micro occupancy and the bitmaps. The neighbours are read with their
-// face bits masked off, so the order in which bricks are rewritten does not matter.
-void oracle_scene_finalize(CxScene* s)
+void oracle_scene_finalize_bricks(CxScene* s, const uint32_t* records, uint32_t count)
 {
-    const uint64_t cells = uint64_t(s->grid_dim) * s->grid_dim * s->grid_dim;
-    std::memset(s->brick_bits, 0, ((cells + 31) / 32) * 4);
-    std::memset(s->super_bits, 0, ((cells / 512 + 31) / 32) * 4);
     uint64_t* occ_all = static_cast<uint64_t*>(s->occupancy);
     const bool packed = s->encoding == kEncodingPacked;
     const uint32_t opaque_mask = packed ? kPackedOpaqueMask : kBlockIdOpaqueMask;
@@ -2362,8 +2358,9 @@ void oracle_scene_finalize(CxScene* s)
         return s->payload[uint64_t(other) * 512 + p[0] + 8 * p[1] + 64 * p[2]];
     };
     constexpr int kDir[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
-    for (uint32_t b = 0; b < s->bricks; ++b)
+    for (uint32_t i = 0; i < count; ++i)
     {
+        const uint32_t b = records != nullptr ? records[i] : i;
         uint64_t occ[8] = {};
         uint32_t lo[3] = {7, 7, 7}, hi[3] = {0, 0, 0};
         bool any = false;
@@ -2428,8 +2425,17 @@ void oracle_scene_finalize(CxScene* s)
     }
     if (s->light_group_rows != nullptr)
     {
-        oracle_scene_light_state(s);
+        oracle_scene_light_state_bricks(s, records, count);
     }
+}
+
+// The whole scene: the bitmaps cleared, every record finalized.
+void oracle_scene_finalize(CxScene* s)
+{
+    const uint64_t cells = uint64_t(s->grid_dim) * s->grid_dim * s->grid_dim;
+    std::memset(s->brick_bits, 0, ((cells + 31) / 32) * 4);
+    std::memset(s->super_bits, 0, ((cells / 512 + 31) / 32) * 4);
+    oracle_scene_finalize_bricks(s, nullptr, s->bricks);
 }
 
 // The group pass (shaders/light/records/caelix_light_tree_{size,scan,build,commit}.hlsl,
@@ -2456,7 +2462,10 @@ void oracle_light_pass(CxScene* s, const CxLightPass* pass)
     std::vector<std::vector<uint32_t>> members(G);
     for (uint32_t b = 0; b < s->bricks; ++b)
     {
-        members[s->group_of_brick[b]].push_back(b);
+        if (s->group_of_brick[b] < G)  // kLtNone: a free record
+        {
+            members[s->group_of_brick[b]].push_back(b);
+        }
     }
     struct Plan
     {

@@ -11,7 +11,6 @@ from __future__ import annotations
 
 import numpy as np
 
-import trace_checks as tc
 from caelix_amd import layout
 
 ROW = layout.ROW
@@ -61,34 +60,81 @@ def from_gpu(tables: dict) -> dict:
             "headers": [control["headers"][t] for t in range(2)], "control": control}
 
 
-def expected_bricks(data, group_of_brick, group_coords):
+def open_faces_sparse(keys, payload, light_codes, encoding: int = 1) -> dict:
+    """{brick index: open[512, 6] bool} for the bricks with a light voxel: voxel v (x + 8y + 64z) is a light
+    (its material code's bit in light_codes) and its face f borders a voxel that is not opaque (a
+    neighbour brick through the keys; air where there is none)."""
+    keys = np.asarray(keys, np.int64)
+    payload = np.asarray(payload, np.uint16)
+    codes = np.asarray(light_codes, np.uint32)
+    opaque_mask = layout.PACKED_OPAQUE_MASK if encoding == layout.ENCODING_PACKED else layout.BLOCK_ID_OPAQUE_MASK
+    id_mask = layout.PACKED_ID_MASK if encoding == layout.ENCODING_PACKED else layout.BLOCK_ID_ID_MASK
+    code = np.where(payload & opaque_mask, payload, payload & id_mask).astype(np.int64)
+    light = ((codes[code >> 5] >> (code & 31).astype(np.uint32)) & 1).astype(bool)
+    index = {tuple(k): i for i, k in enumerate(keys.tolist())}
+    out = {}
+    dirs = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
+    for b in np.nonzero(light.any(axis=1))[0]:
+        pad = np.zeros((10, 10, 10), bool)  # [z, y, x] opaque, one voxel of border
+        pad[1:9, 1:9, 1:9] = (payload[b].reshape(8, 8, 8) & opaque_mask) != 0
+        k = keys[b]
+        for dx, dy, dz in dirs:
+            n = index.get((int(k[0] + dx), int(k[1] + dy), int(k[2] + dz)))
+            if n is None:
+                continue
+            o = (payload[n].reshape(8, 8, 8) & opaque_mask) != 0
+            if dx:
+                pad[1:9, 1:9, 9 if dx > 0 else 0] = o[:, :, 0 if dx > 0 else 7]
+            elif dy:
+                pad[1:9, 9 if dy > 0 else 0, 1:9] = o[:, 0 if dy > 0 else 7, :]
+            else:
+                pad[9 if dz > 0 else 0, 1:9, 1:9] = o[0 if dz > 0 else 7, :, :]
+        open_ = np.zeros((8, 8, 8, 6), bool)
+        lit = light[b].reshape(8, 8, 8)
+        for f, (dx, dy, dz) in enumerate(dirs):
+            nb = pad[1 + dz:9 + dz, 1 + dy:9 + dy, 1 + dx:9 + dx]
+            open_[..., f] = lit & ~nb
+        out[int(b)] = open_.reshape(512, 6)
+    return out
+
+
+def expected_bricks(data, group_of_brick, group_coords, record_ids=None):
     """Per group, per light brick record: its families [(voxel, block)], face sums (float64), group-local
-    box (lo, hi), face count, pos."""
+    box (lo, hi), face count, pos. record_ids[i]: the record of data brick i (default i)."""
     shift = int(data.group_shift)
-    open_ = tc.open_light_faces(data)
-    counts = open_.sum(axis=(1, 2))
+    faces = open_faces_sparse(data.keys, data.payload, data.light_codes, int(getattr(data, "encoding", 1)))
     e = np.asarray(data.materials, np.float32).reshape(65536, 8)
     lum = (e[:, 4] * np.float32(0.2126) + e[:, 5] * np.float32(0.7152) + e[:, 6] * np.float32(0.0722)).astype(np.float64)
     mask = (1 << shift) - 1
     out = {}
-    for b in np.nonzero(counts > 0)[0]:
+    for b, open_ in faces.items():
+        count = int(open_.sum())
+        if count == 0:
+            continue
         g = int(group_of_brick[b])
         key = np.asarray(data.keys[b], np.int64)
         bxyz = key & mask
         fams = []
         sums = np.zeros(6)
         for f in range(6):
-            vox = np.nonzero(open_[b, :, f])[0]
+            vox = np.nonzero(open_[:, f])[0]
             fams.append([(int(v), int(data.payload[b, v])) for v in vox])
-            sums[f] = lum[data.payload[b, vox]].sum()
-        vox = np.nonzero(open_[b].any(axis=1))[0]
+            sums[f] = lum[canonical_codes(data.payload[b, vox], data)].sum()
+        vox = np.nonzero(open_.any(axis=1))[0]
         vxyz = np.stack([vox & 7, (vox >> 3) & 7, vox >> 6], axis=1)
         lo = bxyz * 8 + vxyz.min(axis=0)
         hi = bxyz * 8 + vxyz.max(axis=0) + 1
         pos = int(bxyz[0] | (bxyz[1] << shift) | (bxyz[2] << (2 * shift)))
-        out.setdefault(g, {})[int(b)] = {"fams": fams, "sums": sums, "lo": lo, "hi": hi, "faces": int(counts[b]),
-                                         "pos": pos}
+        rid = int(b) if record_ids is None else int(record_ids[b])
+        out.setdefault(g, {})[rid] = {"fams": fams, "sums": sums, "lo": lo, "hi": hi, "faces": count, "pos": pos}
     return out
+
+
+def canonical_codes(blocks, data) -> np.ndarray:
+    blocks = np.asarray(blocks, np.int64)
+    if int(getattr(data, "encoding", 1)) == layout.ENCODING_PACKED:
+        return np.where(blocks & layout.PACKED_OPAQUE_MASK, blocks, blocks & layout.PACKED_ID_MASK)
+    return np.where(blocks & layout.BLOCK_ID_OPAQUE_MASK, blocks, blocks & layout.BLOCK_ID_ID_MASK)
 
 
 def canonical(block: int, data) -> int:
@@ -102,13 +148,15 @@ def pack(v) -> int:
     return int(v[0]) | (int(v[1]) << 10) | (int(v[2]) << 20)
 
 
-def check_structure(data, group_of_brick, group_coords, tables: dict, group_entity=None, motion=None) -> dict:
-    """CheckStructure over normalised tables; returns the totals (bricks, faces, per-tree bricks)."""
+def check_structure(data, group_of_brick, group_coords, tables: dict, group_entity=None, motion=None,
+                    record_ids=None) -> dict:
+    """CheckStructure over normalised tables; returns the totals (bricks, faces, per-tree bricks).
+    group_of_brick[i], record_ids[i]: the group and the record of data brick i."""
     shift = int(data.group_shift)
     groups = len(group_coords)
     group_entity = np.zeros(groups, np.int64) if group_entity is None else np.asarray(group_entity)
     motion = np.zeros(int(group_entity.max()) + 1 if groups else 1, np.int64) if motion is None else np.asarray(motion)
-    want = expected_bricks(data, group_of_brick, group_coords)
+    want = expected_bricks(data, group_of_brick, group_coords, record_ids)
     rows, blocks, leaves = tables["rows"], tables["blocks"], tables["leaves"]
     total_bricks = total_faces = 0
     tree_bricks = [0, 0]

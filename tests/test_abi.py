@@ -68,3 +68,14 @@ def test_layout_sizes_match_reference():
     assert layout.HASH_SLOT.itemsize == 16
     assert layout.node_capacity_for(4096) == 4096 // 8 * 5 + 64
     assert np.dtype(layout.SUMMARY).fields["hi"][1] == 36
+
+
+def test_header_is_c99_and_c99_check_touches_every_entry_point(tmp_path):
+    """tests/abi_c99.c compiles as C99 without a warning (the reference's tests/abi_c99.c check) and names
+    every entry point the header declares."""
+    source = REPO / "tests" / "abi_c99.c"
+    text = source.read_text()
+    missing = [n for n in declared_entry_points() if f"(AnyFn){n}," not in text]
+    assert not missing, missing
+    subprocess.run(["gcc", "-std=c99", "-pedantic", "-Wall", "-Wextra", "-Werror", "-c", "-I", str(REPO / "include"),
+                    str(source), "-o", str(tmp_path / "abi_c99.o")], check=True, capture_output=True, text=True)
