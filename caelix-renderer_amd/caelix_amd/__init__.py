@@ -76,7 +76,7 @@ class CxLightPass(ctypes.Structure):
 
 
 class CxLightTreeView(ctypes.Structure):
-    _fields_ = [("descriptor", ctypes.c_void_p), ("mis", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+    _fields_ = [("descriptor", ctypes.c_void_p), ("mis", ctypes.c_uint32), ("select", ctypes.c_uint32)]
 
 
 class CxLightOptions(ctypes.Structure):
@@ -252,6 +252,9 @@ SIGNATURES = {
     "cx_light_sample": (_INT, [ctypes.POINTER(CxLightTreeView), _P, _P, _U32, _P, _P, _P]),
     "cx_light_point_pdf": (_INT, [ctypes.POINTER(CxLightTreeView), _P, _U32, _P, _P]),
     "cx_light_mis_weight": (_INT, [_P, _U32, _P, _P]),
+    "cx_light_sample_brick_box": (_INT, [ctypes.POINTER(CxLightTreeView), _P, _P, _U32, _P, _P]),
+    "cx_light_brick_pdf": (_INT, [ctypes.POINTER(CxLightTreeView), _P, _U32, _U32, _P, _P]),
+    "cx_light_box_support": (_INT, [ctypes.POINTER(CxLightTreeView), _P, _U32, _P, _P]),
     "cx_cxw_open": (_INT, [ctypes.c_char_p, _P, _P]),
     "cx_cxw_close": (None, [_P]),
     "cx_cxw_world": (_INT, [_P, _P]),
@@ -737,6 +740,39 @@ def light_point_pdf(view: CxLightTreeView, queries: np.ndarray, stream=None) -> 
     check(lib.cx_light_point_pdf(ctypes.byref(view), q.data_ptr(), n, pdf.data_ptr(), stream_handle(stream)),
           "cx_light_point_pdf")
     return pdf.cpu().numpy()[:n]
+
+
+def light_sample_brick_box(view: CxLightTreeView, boxes: np.ndarray, seeds: np.ndarray, stream=None) -> np.ndarray:
+    """LightTreeSampleBrickBox per box (lo, hi, n): [n, 8] uint32 (ok, group, entry, generation, leaf,
+    pdf bits, importance bits, 0)."""
+    torch = _torch()
+    n = len(boxes)
+    b = torch.from_numpy(np.ascontiguousarray(boxes, np.float32)).cuda()
+    sd = torch.from_numpy(np.ascontiguousarray(seeds, np.uint32).view(np.int32)).cuda()
+    out = torch.empty(max(n, 1) * 8, dtype=torch.int32, device="cuda")
+    check(load().cx_light_sample_brick_box(ctypes.byref(view), b.data_ptr(), sd.data_ptr(), n, out.data_ptr(),
+                                           stream_handle(stream)), "cx_light_sample_brick_box")
+    return out.cpu().numpy().view(np.uint32).reshape(-1, 8)[:n]
+
+
+def light_brick_pdf(view: CxLightTreeView, queries: np.ndarray, box: bool, stream=None) -> np.ndarray:
+    torch = _torch()
+    n = len(queries)
+    q = torch.from_numpy(np.ascontiguousarray(queries, np.uint32).view(np.int32)).cuda()
+    pdf = torch.empty(max(n, 1), dtype=torch.float32, device="cuda")
+    check(load().cx_light_brick_pdf(ctypes.byref(view), q.data_ptr(), n, 1 if box else 0, pdf.data_ptr(),
+                                    stream_handle(stream)), "cx_light_brick_pdf")
+    return pdf.cpu().numpy()[:n]
+
+
+def light_box_support(view: CxLightTreeView, queries: np.ndarray, stream=None) -> np.ndarray:
+    torch = _torch()
+    n = len(queries)
+    q = torch.from_numpy(np.ascontiguousarray(queries, np.uint32).view(np.int32)).cuda()
+    out = torch.empty(max(n, 1) * 4, dtype=torch.float32, device="cuda")
+    check(load().cx_light_box_support(ctypes.byref(view), q.data_ptr(), n, out.data_ptr(), stream_handle(stream)),
+          "cx_light_box_support")
+    return out.cpu().numpy().reshape(-1, 4)[:n]
 
 
 def point_queries(queries: np.ndarray, group_of_brick: np.ndarray, group_coords: np.ndarray, group_shift: int):

@@ -1,6 +1,7 @@
 // The light tree's device API for gfx950 kernels (shaders/light/caelix_light_tree.hlsl): importance
-// of a summary for a point receiver, the weighted reservoir pick, the walk down the 4-wide nodes to a
-// leaf, the in-brick draw of a light face, and the pdf walk up the parent links. Layouts are the
+// of a summary for a point or a box receiver, the weighted reservoir pick, the walk down the 4-wide nodes
+// to a leaf, the in-brick draw of a light face, the pdf walk up the parent links, brick references and
+// the region (box) sampler. Layouts are the
 // reference's (LightTreeHeader, LightBvhNode, LightBvhLeaf, brick light records).
 //
 // A leaf names its brick as (group, index entry); the group's row gives its light block in the current
@@ -69,11 +70,29 @@ CX_DEV void EmitterCosines(V3 d_lo, V3 d_hi, float c[6])
     }
 }
 
-// LtImportance for a point receiver x with normal n (:118-155, is_point).
-CX_DEV float Importance(V3 x, V3 n, V3 lo, V3 hi, const float face[6])
+// A receiver of the walk (LtReceiver, :368-395): the box [lo, hi] (a point: lo = hi) and the unit normal
+// n (0: no surface). Every walk function takes kPoint as a template constant: the point form's code is
+// the box form with the box terms removed.
+struct Receiver
+{
+    V3 lo;
+    V3 hi;
+    V3 n;
+};
+
+CX_DEV Receiver PointReceiver(V3 x, V3 n)
+{
+    return Receiver{x, x, n};
+}
+
+// LtImportance (:118-155): the importance of a summary (box [lo, hi], face sums) for the receiver: the
+// emitter cosines over D = [r_lo - hi, r_hi - lo], the receiver term cr = min(1, m / dmin), the
+// distance term max(|c_R - c_B|^2, max(1, extent^2)).
+template <bool kPoint>
+CX_DEV float ImportanceR(const Receiver& r, V3 lo, V3 hi, const float face[6])
 {
     float c[6];
-    EmitterCosines(Sub(x, hi), Sub(x, lo), c);
+    EmitterCosines(Sub(r.lo, hi), Sub(r.hi, lo), c);
     float e = 0.0f;
 #pragma unroll
     for (int f = 0; f < 6; ++f)
@@ -86,23 +105,42 @@ CX_DEV float Importance(V3 x, V3 n, V3 lo, V3 hi, const float face[6])
     }
     const V3 centre = Mul(Add(lo, hi), 0.5f);
     const V3 half_size = Mul(Sub(hi, lo), 0.5f);
+    const V3 r_centre = kPoint ? r.lo : Mul(Add(r.lo, r.hi), 0.5f);
+    const V3 n = r.n;
     float cr = 1.0f;
     if (n.x != 0.0f || n.y != 0.0f || n.z != 0.0f)
     {
-        const float m = Dot(n, Sub(centre, x)) + Dot(Make(fabsf(n.x), fabsf(n.y), fabsf(n.z)), half_size);
+        const V3 abs_n = Make(fabsf(n.x), fabsf(n.y), fabsf(n.z));
+        float m = Dot(n, Sub(centre, r_centre)) + Dot(abs_n, half_size);
+        if (!kPoint)
+        {
+            m += Dot(abs_n, Mul(Sub(r.hi, r.lo), 0.5f));
+        }
         if (m <= 0.0f)
         {
             return 0.0f;
         }
-        const V3 gap = Make(fmaxf(fmaxf(lo.x - x.x, x.x - hi.x), 0.0f), fmaxf(fmaxf(lo.y - x.y, x.y - hi.y), 0.0f),
-                            fmaxf(fmaxf(lo.z - x.z, x.z - hi.z), 0.0f));
+        const V3 gap = Make(fmaxf(fmaxf(lo.x - r.hi.x, r.lo.x - hi.x), 0.0f), fmaxf(fmaxf(lo.y - r.hi.y, r.lo.y - hi.y), 0.0f),
+                            fmaxf(fmaxf(lo.z - r.hi.z, r.lo.z - hi.z), 0.0f));
         const float dmin = Sqrt(Dot(gap, gap));
         cr = dmin > 0.0f ? fminf(1.0f, Div(m, dmin)) : 1.0f;
     }
-    const V3 to_centre = Sub(x, centre);
-    const float extent2 = Dot(half_size, half_size);
+    const V3 to_centre = Sub(r_centre, centre);
+    float extent2 = Dot(half_size, half_size);
+    if (!kPoint)
+    {
+        const V3 r_half = Mul(Sub(r.hi, r.lo), 0.5f);
+        const float extent = Sqrt(Dot(half_size, half_size)) + Sqrt(Dot(r_half, r_half));
+        extent2 = extent * extent;
+    }
     const float d2 = fmaxf(Dot(to_centre, to_centre), fmaxf(1.0f, extent2));
     return Div(e * cr, d2);
+}
+
+// LightTreeImportance (:157-160): the point form.
+CX_DEV float Importance(V3 x, V3 n, V3 lo, V3 hi, const float face[6])
+{
+    return ImportanceR<true>(PointReceiver(x, n), lo, hi, face);
 }
 
 struct Pick
@@ -166,10 +204,11 @@ CX_DEV uint32_t ChildCount(const uint32_t* child)
     return n;
 }
 
-// LtTreeSummary (:289-304): the root summary of tree k, power 0 for an empty tree.
+// LtTreeSummary (:289-304): the root summary of tree k, power 0 (root kLtNone) for an empty tree or one
+// LightTreeSelect left out.
 CX_DEV Summ TreeSummary(const LightView& L, uint32_t k, uint32_t& root)
 {
-    root = L.headers[k].root;
+    root = ((L.trees >> k) & 1u) != 0 ? L.headers[k].root : kLtNone;
     if (root == kLtNone)
     {
         Summ e;
@@ -185,25 +224,37 @@ CX_DEV Summ TreeSummary(const LightView& L, uint32_t k, uint32_t& root)
     return HeaderSummary(&L.headers[k]);
 }
 
-// LtChooseTreeR (:417-437) + LtChooseLeafR (:466-512), point receiver.
-CX_DEV uint32_t ChooseLeaf(const LightView& L, V3 x, V3 n, uint32_t& rng, float& pdf, uint32_t& tree)
+// LtChooseTreeR (:417-437): a tree drawn by its root's importance (kLtNone when both are empty); pdf
+// times its probability.
+template <bool kPoint>
+CX_DEV uint32_t ChooseTreeR(const LightView& L, const Receiver& r, uint32_t& rng, float& pdf, uint32_t& root)
 {
     Pick pick = PickBegin();
     uint32_t roots[2];
     for (uint32_t k = 0; k < 2; ++k)
     {
         const Summ s = TreeSummary(L, k, roots[k]);
-        PickAdd(pick, Importance(x, n, s.lo, s.hi, s.face), Power6(s.face), k, LightRandomFloat(rng));
+        PickAdd(pick, ImportanceR<kPoint>(r, s.lo, s.hi, s.face), Power6(s.face), k, LightRandomFloat(rng));
     }
     float probability;
     const uint32_t k = PickEnd(pick, probability);
-    tree = k;
-    if (k == kLtNone)
+    root = k != kLtNone ? roots[k] : kLtNone;
+    if (k != kLtNone)
     {
-        return kLtNone;
+        pdf *= probability;
     }
-    pdf *= probability;
-    uint32_t node = roots[k];
+    return k;
+}
+
+// LtChooseLeafR (:466-512): a leaf of tree k drawn from its root (kLtNone when the draw fails); pdf
+// times each choice's probability; leaf_importance = the importance of the drawn leaf's summary (the
+// slot the last choice took; 0 when it fell back to the power).
+template <bool kPoint>
+CX_DEV uint32_t ChooseLeafR(const LightView& L, uint32_t k, uint32_t root, const Receiver& r, uint32_t& rng, float& pdf,
+                            float& leaf_importance)
+{
+    leaf_importance = 0.0f;
+    uint32_t node = root;
     for (uint32_t depth = 0; depth < 64; ++depth)
     {
         const LightBvhNode* address = L.nodes[k] + node;
@@ -213,6 +264,11 @@ CX_DEV uint32_t ChooseLeaf(const LightView& L, V3 x, V3 n, uint32_t& rng, float&
         if (ChildCount(children) == 1)
         {
             chosen = children[0];
+            if (!kPoint && (chosen & kLtChildLeafBit) != 0)
+            {
+                const Summ slot = NodeSlot(address, 0);
+                leaf_importance = ImportanceR<kPoint>(r, slot.lo, slot.hi, slot.face);
+            }
         }
         else
         {
@@ -224,8 +280,7 @@ CX_DEV uint32_t ChooseLeaf(const LightView& L, V3 x, V3 n, uint32_t& rng, float&
                     continue;
                 }
                 const Summ slot = NodeSlot(address, s);
-                p.sum_i += 0.0f;
-                PickAdd(p, Importance(x, n, slot.lo, slot.hi, slot.face), Power6(slot.face), s, LightRandomFloat(rng));
+                PickAdd(p, ImportanceR<kPoint>(r, slot.lo, slot.hi, slot.face), Power6(slot.face), s, LightRandomFloat(rng));
             }
             float prob;
             const uint32_t s = PickEnd(p, prob);
@@ -235,6 +290,7 @@ CX_DEV uint32_t ChooseLeaf(const LightView& L, V3 x, V3 n, uint32_t& rng, float&
             }
             pdf *= prob;
             chosen = children[s];
+            leaf_importance = p.sum_i > 0.0f ? p.weight_i : 0.0f;
         }
         if ((chosen & kLtChildLeafBit) != 0)
         {
@@ -245,8 +301,23 @@ CX_DEV uint32_t ChooseLeaf(const LightView& L, V3 x, V3 n, uint32_t& rng, float&
     return kLtNone;
 }
 
+// The point form of the two draws (LtChooseTree + LtChooseLeaf, :619-629): a leaf, its tree.
+CX_DEV uint32_t ChooseLeaf(const LightView& L, V3 x, V3 n, uint32_t& rng, float& pdf, uint32_t& tree)
+{
+    const Receiver r = PointReceiver(x, n);
+    uint32_t root;
+    tree = ChooseTreeR<true>(L, r, rng, pdf, root);
+    if (tree == kLtNone)
+    {
+        return kLtNone;
+    }
+    float leaf_importance;
+    return ChooseLeafR<true>(L, tree, root, r, rng, pdf, leaf_importance);
+}
+
 // LtSlotProbabilityR (:515-546)
-CX_DEV float SlotProbability(const LightBvhNode* address, V3 x, V3 n, uint32_t slot)
+template <bool kPoint>
+CX_DEV float SlotProbabilityR(const LightBvhNode* address, const Receiver& r, uint32_t slot)
 {
     uint32_t children[4];
     *reinterpret_cast<uint4*>(children) = *reinterpret_cast<const uint4*>(address->child);
@@ -265,7 +336,7 @@ CX_DEV float SlotProbability(const LightBvhNode* address, V3 x, V3 n, uint32_t s
             continue;
         }
         const Summ summary = NodeSlot(address, s);
-        const float importance = Importance(x, n, summary.lo, summary.hi, summary.face);
+        const float importance = ImportanceR<kPoint>(r, summary.lo, summary.hi, summary.face);
         const float power = Power6(summary.face);
         sum_i += importance > 0.0f ? importance : 0.0f;
         sum_p += power > 0.0f ? power : 0.0f;
@@ -279,7 +350,8 @@ CX_DEV float SlotProbability(const LightBvhNode* address, V3 x, V3 n, uint32_t s
 }
 
 // LtTreeProbabilityR (:439-462)
-CX_DEV float TreeProbability(const LightView& L, V3 x, V3 n, uint32_t target)
+template <bool kPoint>
+CX_DEV float TreeProbabilityR(const LightView& L, const Receiver& r, uint32_t target)
 {
     float sum_i = 0.0f;
     float sum_p = 0.0f;
@@ -289,7 +361,7 @@ CX_DEV float TreeProbability(const LightView& L, V3 x, V3 n, uint32_t target)
     {
         uint32_t root;
         const Summ s = TreeSummary(L, k, root);
-        const float importance = Importance(x, n, s.lo, s.hi, s.face);
+        const float importance = ImportanceR<kPoint>(r, s.lo, s.hi, s.face);
         const float power = Power6(s.face);
         sum_i += importance > 0.0f ? importance : 0.0f;
         sum_p += power > 0.0f ? power : 0.0f;
@@ -302,15 +374,15 @@ CX_DEV float TreeProbability(const LightView& L, V3 x, V3 n, uint32_t target)
     return PickProbability(sum_i, sum_p, importance_t, power_t);
 }
 
-// LtLeafProbabilityR (:549-572): the probability of reaching `leaf` of tree k.
-CX_DEV float LeafProbability(const LightView& L, uint32_t k, uint32_t leaf, V3 x, V3 n)
+// LtLeafProbabilityR (:549-572): the probability of reaching `leaf` of tree k, 0 for an empty or
+// unselected tree.
+template <bool kPoint>
+CX_DEV float LeafProbabilityR(const LightView& L, uint32_t k, uint32_t leaf, const Receiver& r)
 {
-    if (L.headers[k].root == kLtNone)
-    {
-        return 0.0f;
-    }
+    uint32_t root;
+    TreeSummary(L, k, root);
     const uint32_t parent = L.leaves[k][leaf].parent;
-    if (parent == kLtNone)
+    if (root == kLtNone || parent == kLtNone)
     {
         return 0.0f;
     }
@@ -320,16 +392,21 @@ CX_DEV float LeafProbability(const LightView& L, uint32_t k, uint32_t leaf, V3 x
     for (uint32_t depth = 0; depth < 64; ++depth)
     {
         const LightBvhNode* address = L.nodes[k] + node;
-        probability *= SlotProbability(address, x, n, slot);
+        probability *= SlotProbabilityR<kPoint>(address, r, slot);
         const uint32_t up = address->parent;
         if (up == kLtNone)
         {
-            return probability * TreeProbability(L, x, n, k);
+            return probability * TreeProbabilityR<kPoint>(L, r, k);
         }
         slot = address->parent_slot;
         node = up;
     }
     return 0.0f;
+}
+
+CX_DEV float LeafProbability(const LightView& L, uint32_t k, uint32_t leaf, V3 x, V3 n)
+{
+    return LeafProbabilityR<true>(L, k, leaf, PointReceiver(x, n));
 }
 
 // A brick light record (BrickLightRecordHeader + entries, light_tree_layout.h:296-334).
@@ -584,15 +661,25 @@ CX_DEV bool LightTreeSample(const LightView& L, V3 x, V3 n, uint32_t& rng, Light
     return true;
 }
 
-// LightTreePointPdf (:936-949) = LightTreeFindLightFace (:853-932) + LtBrickProbability: the area pdf
-// with which LightTreeSample draws the point on face `face` of the group-local voxel `v` of `group`,
-// whose raw block value is `block`, for the receiver (x, n); 0 when it is not a light face the tree can
-// draw. `found` = a light face of the tree whose brick has a leaf (LightTreeFindLightFace and
-// leaf != kLtNone, the condition of LampHitWeight, caelix_path_trace.hlsl:438-443).
-CX_DEV float LightPointPdf(const LightView& L, V3 x, V3 n, uint32_t group, const uint32_t v[3], uint32_t face,
-                           uint32_t block_value, bool& found)
+// A brick of the tree (LightBrickRef): its group, index entry and the row's generation when it was
+// named; a reference of an older generation is stale.
+struct BrickRef
 {
-    found = false;
+    uint32_t group;
+    uint32_t entry;
+    uint32_t generation;
+};
+
+// LightTreeFindLightFace (:853-932): whether face `face` of the group-local voxel `v` of `group`, with the
+// raw block value `block_value`, is a light face of the tree (a light material code, a brick with an
+// index entry, the voxel in that face family of its record). Then `brick` names the brick, `leaf` is its
+// leaf word (kLtNone: no draw reaches it) and in_brick = P(face | brick, x) / face count.
+CX_DEV bool FindLightFace(const LightView& L, V3 x, uint32_t group, const uint32_t v[3], uint32_t face,
+                          uint32_t block_value, BrickRef& brick, uint32_t& leaf, float& in_brick)
+{
+    brick = BrickRef{group, kLtNone, 0u};
+    leaf = kLtNone;
+    in_brick = 0.0f;
     // BlockMaterialCode (caelix_record.hlsl:209): an opaque value as it is, a transparent one without its
     // face bits
     const uint32_t raw = block_value & 0xFFFFu;
@@ -602,19 +689,21 @@ CX_DEV float LightPointPdf(const LightView& L, V3 x, V3 n, uint32_t group, const
                               : (raw & (packed ? kPackedIdMask : kBlockIdIdMask));
     if (((L.light_codes[code >> 5] >> (code & 31u)) & 1u) == 0u || face > 5u)
     {
-        return 0.0f;
+        return false;
     }
     const uint32_t* block = GroupBlock(L, group);
     if (block == nullptr)
     {
-        return 0.0f;
+        return false;
     }
     const uint32_t key = Morton3(v[0] >> 3, v[1] >> 3, v[2] >> 3);
     const uint32_t entry = FindIndexEntry(block, block[0], key, L.group_shift);
     if (entry == kLtNone)
     {
-        return 0.0f;
+        return false;
     }
+    brick.entry = entry;
+    brick.generation = L.rows[group].generation;
     const Record rec = LoadRecord(block, entry);
     Frame frame;
     const V3 origin = GroupFrame(L, group, frame);
@@ -624,7 +713,7 @@ CX_DEV float LightPointPdf(const LightView& L, V3 x, V3 n, uint32_t group, const
     const uint32_t count = FamilyCount(rec, face);
     if (sum <= 0.0f || w[face] <= 0.0f || count == 0u)
     {
-        return 0.0f;
+        return false;
     }
     uint32_t first = 0;
     for (uint32_t g = 0; g < face; ++g)
@@ -655,16 +744,100 @@ CX_DEV float LightPointPdf(const LightView& L, V3 x, V3 n, uint32_t group, const
     }
     if (!hit)
     {
-        return 0.0f;
+        return false;
     }
-    const float in_brick = Div(Div(w[face], sum), float(count));
-    const uint32_t leaf = BrickLeaf(L, block, group, entry);
-    if (leaf == kLtNone)
+    in_brick = Div(Div(w[face], sum), float(count));
+    leaf = BrickLeaf(L, block, group, entry);
+    return true;
+}
+
+// LightTreePointPdf (:936-949): the area pdf with which LightTreeSample draws the point on face `face` of
+// the group-local voxel `v` of `group`, for the receiver (x, n); 0 when it is not a light face the tree
+// can draw. `found` = a light face of the tree whose brick has a leaf (LampHitWeight's condition,
+// caelix_path_trace.hlsl:438-443).
+CX_DEV float LightPointPdf(const LightView& L, V3 x, V3 n, uint32_t group, const uint32_t v[3], uint32_t face,
+                           uint32_t block_value, bool& found)
+{
+    BrickRef brick;
+    uint32_t leaf;
+    float in_brick;
+    found = FindLightFace(L, x, group, v, face, block_value, brick, leaf, in_brick) && leaf != kLtNone;
+    if (!found)
     {
         return 0.0f;  // no draw reaches it: LampHitWeight gives the full emission
     }
-    found = true;
     return LeafProbability(L, leaf >> kLtBvhLeafTreeShift, leaf & kLtBvhLeafIndexMask, x, n) * in_brick;
+}
+
+// LightBrickRefValid (:640-646): the row still has a block of the reference's generation and the entry
+// lies in it.
+CX_DEV bool BrickRefValid(const LightView& L, const BrickRef& b)
+{
+    const uint4 row = *reinterpret_cast<const uint4*>(&L.rows[b.group].block_offset);
+    return row.y != 0u && row.z == b.generation && b.entry < L.heap[uint64_t(row.x) * 4];
+}
+
+// LtBrickProbabilityR (:575-592) behind LightTreeBrickPdf / LightTreeBrickPdfBox (:769-777, :822-831): the
+// probability that the brick draw takes `brick`, 0 when the reference is stale or the brick has no leaf.
+template <bool kPoint>
+CX_DEV float BrickPdfR(const LightView& L, const Receiver& r, const BrickRef& brick)
+{
+    if (!BrickRefValid(L, brick))
+    {
+        return 0.0f;
+    }
+    const uint32_t* block = GroupBlock(L, brick.group);
+    const uint32_t leaf = BrickLeaf(L, block, brick.group, brick.entry);
+    if (leaf == kLtNone)
+    {
+        return 0.0f;
+    }
+    return LeafProbabilityR<kPoint>(L, leaf >> kLtBvhLeafTreeShift, leaf & kLtBvhLeafIndexMask, r);
+}
+
+// LightTreeSampleBrickBox (:787-818): a brick drawn for every receiver point of the box with the normal n:
+// pdf_brick includes the tree choice, leaf = tree << 31 | leaf index, importance = the box importance of
+// the leaf's summary (0 when the draw fell back to the power).
+CX_DEV bool SampleBrickBox(const LightView& L, V3 box_lo, V3 box_hi, V3 n, uint32_t& rng, BrickRef& brick, float& pdf_brick,
+                           uint32_t& leaf, float& importance)
+{
+    brick = BrickRef{kLtNone, kLtNone, 0u};
+    pdf_brick = 1.0f;
+    leaf = kLtNone;
+    importance = 0.0f;
+    const Receiver r{box_lo, box_hi, n};
+    uint32_t root;
+    const uint32_t k = ChooseTreeR<false>(L, r, rng, pdf_brick, root);
+    if (k == kLtNone)
+    {
+        return false;
+    }
+    const uint32_t index = ChooseLeafR<false>(L, k, root, r, rng, pdf_brick, importance);
+    if (index == kLtNone)
+    {
+        importance = 0.0f;
+        return false;
+    }
+    const uint2 owner = *reinterpret_cast<const uint2*>(&L.leaves[k][index].group);
+    brick = BrickRef{owner.x, owner.y, L.rows[owner.x].generation};
+    leaf = (k << kLtBvhLeafTreeShift) | index;
+    return true;
+}
+
+// LightTreeLeafImportanceBox (:837-850): the box importance of a valid brick's leaf summary (0 when the
+// brick has no leaf of its own).
+CX_DEV float LeafImportanceBox(const LightView& L, const BrickRef& brick, V3 box_lo, V3 box_hi, V3 n)
+{
+    const uint32_t* block = GroupBlock(L, brick.group);
+    const uint32_t leaf = block != nullptr ? BrickLeaf(L, block, brick.group, brick.entry) : kLtNone;
+    if (leaf == kLtNone)
+    {
+        return 0.0f;
+    }
+    const LightBvhLeaf& l = L.leaves[leaf >> kLtBvhLeafTreeShift][leaf & kLtBvhLeafIndexMask];
+    const V3 lo = Make(l.summary.lo[0], l.summary.lo[1], l.summary.lo[2]);
+    const V3 hi = Make(l.summary.hi[0], l.summary.hi[1], l.summary.hi[2]);
+    return ImportanceR<false>(Receiver{box_lo, box_hi, n}, lo, hi, l.summary.face);
 }
 
 // The light face pdf at a hit of the world traversal: brick `record` of the brick store and the world

@@ -47,7 +47,7 @@ LightRef LightsOf(const CxLightTreeView& v)
     LightRef r;
     r.descriptor = v.descriptor;
     r.mis = v.mis;
-    r.pad = 0;
+    r.select = v.select;
     return r;
 }
 
@@ -1165,6 +1165,81 @@ __global__ void __launch_bounds__(256) PointPdfKernel(LightRef lights, const uin
     pdf[i] = LightPointPdf(l, x, n, q[6], voxel, q[10], q[11], found);
 }
 
+// LightTreeSampleBrickBox per box receiver: boxes[i] = lo(3), hi(3), n(3); out[i] = {ok, group, entry,
+// generation, leaf, pdf_brick bits, importance bits, 0}.
+__global__ void __launch_bounds__(256) BrickBoxSampleKernel(LightRef lights, const float* boxes, const uint32_t* seeds,
+                                                            uint32_t count, uint32_t* out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count)
+    {
+        return;
+    }
+    const LightView l = OpenLights(lights);
+    const float* b = boxes + uint64_t(i) * 9;
+    uint32_t rng = seeds[i];
+    BrickRef brick;
+    float pdf, importance;
+    uint32_t leaf;
+    const bool ok = SampleBrickBox(l, Make(b[0], b[1], b[2]), Make(b[3], b[4], b[5]), Make(b[6], b[7], b[8]), rng, brick,
+                                   pdf, leaf, importance);
+    uint4* o = reinterpret_cast<uint4*>(out + uint64_t(i) * 8);
+    o[0] = make_uint4(ok ? 1u : 0u, brick.group, brick.entry, brick.generation);
+    o[1] = make_uint4(leaf, __float_as_uint(pdf), __float_as_uint(importance), 0u);
+}
+
+// LightTreeBrickPdf (point) or LightTreeBrickPdfBox (box) per query: lo(3), hi(3), n(3) as floats; group,
+// entry, generation. A point receiver is lo (hi is not read).
+template <bool kPoint>
+__global__ void __launch_bounds__(256) BrickPdfKernel(LightRef lights, const uint32_t* queries, uint32_t count, float* pdf)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count)
+    {
+        return;
+    }
+    const LightView l = OpenLights(lights);
+    const uint32_t* q = queries + uint64_t(i) * 12;
+    const float* f = reinterpret_cast<const float*>(q);
+    const V3 lo = Make(f[0], f[1], f[2]);
+    const Receiver r{lo, kPoint ? lo : Make(f[3], f[4], f[5]), Make(f[6], f[7], f[8])};
+    pdf[i] = BrickPdfR<kPoint>(l, r, BrickRef{q[9], q[10], q[11]});
+}
+
+// The reference probe's support query (tests/shaders/caelix_light_nee_probe.hlsl mode 0): lo(3), x.x,
+// hi(3), x.y, n(3), x.z as floats; group, voxel in group (x | y << 10 | z << 20), face, block. The face
+// found from the point x in front of it (LightTreeFindLightFace), then its box pdf
+// (LightTreeBrickPdfBox) and target (LightTreeLeafImportanceBox): out[i] = (found, has leaf, pdf, target).
+__global__ void __launch_bounds__(256) BoxSupportKernel(LightRef lights, const uint32_t* queries, uint32_t count,
+                                                        float* out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count)
+    {
+        return;
+    }
+    const LightView l = OpenLights(lights);
+    const uint32_t* q = queries + uint64_t(i) * 16;
+    const float* f = reinterpret_cast<const float*>(q);
+    const V3 lo = Make(f[0], f[1], f[2]);
+    const V3 hi = Make(f[4], f[5], f[6]);
+    const V3 n = Make(f[8], f[9], f[10]);
+    const V3 front = Make(f[3], f[7], f[11]);
+    const uint32_t v[3] = {q[13] & kLtVoxelMask, (q[13] >> kLtVoxelBits) & kLtVoxelMask, q[13] >> (2 * kLtVoxelBits)};
+    BrickRef brick;
+    uint32_t leaf;
+    float in_brick;
+    float4 result = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (FindLightFace(l, front, q[12], v, q[14], q[15], brick, leaf, in_brick))
+    {
+        result.x = 1.0f;
+        result.y = leaf != kLtNone ? 1.0f : 0.0f;
+        result.z = BrickPdfR<false>(l, Receiver{lo, hi, n}, brick);
+        result.w = LeafImportanceBox(l, brick, lo, hi, n);
+    }
+    reinterpret_cast<float4*>(out)[i] = result;
+}
+
 // LightNeeMisWeight of both sides per query: q = {p_area, d, cos_l, n_r (3), w (3)} -> {w_nee, w_bsdf}.
 __global__ void __launch_bounds__(256) MisWeightKernel(const float* queries, uint32_t count, float* weights)
 {
@@ -1683,6 +1758,73 @@ CX_API int cx_light_point_pdf(const CxLightTreeView* lights, const uint32_t* que
     cx::StageScope scope(s, "light_point_pdf");
     cx::PointPdfKernel<<<cx::DivUp(count, 256), 256, 0, s>>>(cx::LightsOf(*lights), queries, count, pdf);
     return cx::LaunchResult("cx_light_point_pdf");
+}
+
+CX_API int cx_light_sample_brick_box(const CxLightTreeView* lights, const float* boxes, const uint32_t* seeds,
+                                     uint32_t count, uint32_t* out, void* stream)
+{
+    if (int r = cx::CheckLights(lights, "cx_light_sample_brick_box"); r != CX_OK)
+    {
+        return r;
+    }
+    if (count == 0)
+    {
+        return CX_OK;
+    }
+    if (boxes == nullptr || seeds == nullptr || out == nullptr)
+    {
+        return cx::ArgError("cx_light_sample_brick_box: a buffer is NULL");
+    }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    cx::BrickBoxSampleKernel<<<cx::DivUp(count, 256), 256, 0, s>>>(cx::LightsOf(*lights), boxes, seeds, count, out);
+    return cx::LaunchResult("cx_light_sample_brick_box");
+}
+
+CX_API int cx_light_brick_pdf(const CxLightTreeView* lights, const uint32_t* queries, uint32_t count, uint32_t box,
+                              float* pdf, void* stream)
+{
+    if (int r = cx::CheckLights(lights, "cx_light_brick_pdf"); r != CX_OK)
+    {
+        return r;
+    }
+    if (count == 0)
+    {
+        return CX_OK;
+    }
+    if (queries == nullptr || pdf == nullptr)
+    {
+        return cx::ArgError("cx_light_brick_pdf: a buffer is NULL");
+    }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    if (box != 0)
+    {
+        cx::BrickPdfKernel<false><<<cx::DivUp(count, 256), 256, 0, s>>>(cx::LightsOf(*lights), queries, count, pdf);
+    }
+    else
+    {
+        cx::BrickPdfKernel<true><<<cx::DivUp(count, 256), 256, 0, s>>>(cx::LightsOf(*lights), queries, count, pdf);
+    }
+    return cx::LaunchResult("cx_light_brick_pdf");
+}
+
+CX_API int cx_light_box_support(const CxLightTreeView* lights, const uint32_t* queries, uint32_t count, float* out,
+                                void* stream)
+{
+    if (int r = cx::CheckLights(lights, "cx_light_box_support"); r != CX_OK)
+    {
+        return r;
+    }
+    if (count == 0)
+    {
+        return CX_OK;
+    }
+    if (queries == nullptr || out == nullptr)
+    {
+        return cx::ArgError("cx_light_box_support: a buffer is NULL");
+    }
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    cx::BoxSupportKernel<<<cx::DivUp(count, 256), 256, 0, s>>>(cx::LightsOf(*lights), queries, count, out);
+    return cx::LaunchResult("cx_light_box_support");
 }
 
 CX_API int cx_light_mis_weight(const float* queries, uint32_t count, float* weights, void* stream)

@@ -71,14 +71,16 @@ struct LightView
     uint32_t group_shift;            // a group is (1 << group_shift)^3 bricks
     uint32_t encoding;               // the block value encoding (kEncoding*)
     uint32_t mis;
+    uint32_t trees;                  // LightTreeSelect: the trees draws and pdfs use (bit k = tree k)
 };
 
-// What a kernel receives of a light tree: its descriptor's device address and the MIS switch.
+// What a kernel receives of a light tree: its descriptor's device address, the MIS switch and the
+// selected trees (kLtSelect*: 1 static, 2 dynamic, 3 both; 0 is taken as both).
 struct LightRef
 {
     const LightTreeDescriptor* descriptor;
     uint32_t mis;
-    uint32_t pad;
+    uint32_t select;
 };
 
 CX_DEV LightView OpenLights(const LightRef& r)
@@ -99,6 +101,7 @@ CX_DEV LightView OpenLights(const LightRef& r)
     l.group_shift = d->group_shift;
     l.encoding = d->encoding;
     l.mis = r.mis;
+    l.trees = r.select != 0u ? (r.select & 3u) : 3u;
     return l;
 }
 

@@ -309,12 +309,14 @@ CX_API int cx_light_gather_entities(const CxLightPass* pass, uint32_t tree, Ligh
 
 /* The light tree as the trace and the light probes read it: the device address of its 128-byte
  * descriptor (LightTreeDescriptor, the reference's LightTreeOpen reads it the same way:
- * shaders/light/caelix_light_tree.hlsl:239-287), and whether the NEE and the BSDF hits weigh each other
- * by the balance heuristic (the reference's light NEE without CAELIX_LIGHT_NEE_FLAG_NO_MIS). */
+ * shaders/light/caelix_light_tree.hlsl:239-287), whether the NEE and the BSDF hits weigh each other
+ * by the balance heuristic (the reference's light NEE without CAELIX_LIGHT_NEE_FLAG_NO_MIS), and the
+ * trees the draws and pdfs use (LightTreeSelect, :266-272: kLtSelect* 1 static, 2 dynamic, 3 both; 0
+ * is taken as both, LightTreeOpen's default). */
 typedef struct CxLightTreeView {
     const LightTreeDescriptor* descriptor;
     uint32_t mis;
-    uint32_t pad;
+    uint32_t select;
 } CxLightTreeView;
 
 /* Pinhole camera (caelix_frame.hlsl:164-171): unit forward/right/up. */
@@ -484,6 +486,27 @@ CX_API int cx_light_sample(const CxLightTreeView* lights, const float* receivers
  * of the tree or its brick has no leaf). */
 CX_API int cx_light_point_pdf(const CxLightTreeView* lights, const uint32_t* queries, uint32_t count, float* pdf,
                               void* stream);
+
+/* The region receiver's draw (LightTreeSampleBrickBox, caelix_light_tree.hlsl:787-818): a brick drawn for
+ * every receiver point of a box with a normal, by the box importance (LightTreeImportanceBox, :162-166) at
+ * every choice. boxes[i] = lo(3), hi(3), n(3) (9 floats; n = 0: no surface), seeds[i] the PCG state;
+ * out[i] = {ok, group, entry, generation, leaf (tree << 31 | index), pdf_brick (float bits), importance
+ * (float bits: the leaf summary's box importance, 0 when the draw fell back to the power), 0}. */
+CX_API int cx_light_sample_brick_box(const CxLightTreeView* lights, const float* boxes, const uint32_t* seeds,
+                                     uint32_t count, uint32_t* out, void* stream);
+/* LightTreeBrickPdf (box = 0, :769-777) or LightTreeBrickPdfBox (box = 1, :822-831) of a brick reference:
+ * queries[i] = lo(3), hi(3), n(3) as floats, then group, entry, generation (12 words; a point receiver is
+ * lo); pdf[i] = the probability of drawing that brick, 0 when the reference is stale (LightBrickRefValid,
+ * :640-646) or the brick has no leaf. */
+CX_API int cx_light_brick_pdf(const CxLightTreeView* lights, const uint32_t* queries, uint32_t count, uint32_t box,
+                              float* pdf, void* stream);
+/* The support query of the reference's light NEE probe (tests/shaders/caelix_light_nee_probe.hlsl mode 0):
+ * queries[i] = lo(3), x.x, hi(3), x.y, n(3), x.z as floats, then group, voxel in group (x | y << 10 |
+ * z << 20), face, block (16 words; x a point in front of the face); out[i] = (found, has leaf, box pdf,
+ * target) as floats: LightTreeFindLightFace from x, LightTreeBrickPdfBox and LightTreeLeafImportanceBox
+ * of the box. */
+CX_API int cx_light_box_support(const CxLightTreeView* lights, const uint32_t* queries, uint32_t count, float* out,
+                                void* stream);
 
 /* LightNeeMisWeight (shaders/render/caelix_light_nee.hlsl:20-34), the balance heuristic both MIS sides
  * share, per query: queries[i] = {p_area, d, cos_l, receiver normal (3), unit direction (3)} (9

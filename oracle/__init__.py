@@ -134,7 +134,7 @@ class _Scene(ctypes.Structure):
 
 
 class _TreeView(ctypes.Structure):
-    _fields_ = [("descriptor", _P), ("mis", _U32), ("pad", _U32)]
+    _fields_ = [("descriptor", _P), ("mis", _U32), ("select", _U32)]
 
 
 class _Pass(ctypes.Structure):
@@ -201,6 +201,12 @@ def _trace_lib():
         lib.oracle_light_sample.argtypes = [T, _P, _P, _U32, _P, _P]
         lib.oracle_light_point_pdf.restype = None
         lib.oracle_light_point_pdf.argtypes = [T, _P, _U32, _P]
+        lib.oracle_light_sample_brick_box.restype = None
+        lib.oracle_light_sample_brick_box.argtypes = [T, _P, _P, _U32, _P]
+        lib.oracle_light_brick_pdf.restype = None
+        lib.oracle_light_brick_pdf.argtypes = [T, _P, _U32, _U32, _P]
+        lib.oracle_light_box_support.restype = None
+        lib.oracle_light_box_support.argtypes = [T, _P, _U32, _P]
         lib.oracle_mis_weight.restype = None
         lib.oracle_mis_weight.argtypes = [_P, _U32, _P]
         lib.oracle_trace_frame.restype = None
@@ -314,8 +320,9 @@ class LightTables:
         d["encoding"] = self.encoding
         d["group_count"] = self.groups
 
-    def view(self, mis: bool) -> _TreeView:
-        return _TreeView(self.desc.ctypes.data, 1 if mis else 0, 0)
+    def view(self, mis: bool, select: int = 0) -> _TreeView:
+        """select: LightTreeSelect's mask (1 static, 2 dynamic, 3 or 0 both)."""
+        return _TreeView(self.desc.ctypes.data, 1 if mis else 0, int(select))
 
     def _pass_struct(self) -> _Pass:
         return _Pass(self.desc.ctypes.data, self.heap_units, self.staged.ctypes.data, None, max(self.groups, 1), 0)
@@ -409,6 +416,33 @@ def light_point_pdf(view: _TreeView, queries: np.ndarray) -> np.ndarray:
     pdf = np.zeros(len(q), np.float32)
     _trace_lib().oracle_light_point_pdf(ctypes.byref(view), _ptr(q), len(q), _ptr(pdf))
     return pdf
+
+
+def light_sample_brick_box(view: _TreeView, boxes: np.ndarray, seeds: np.ndarray) -> np.ndarray:
+    """LightTreeSampleBrickBox per box (lo, hi, n; 9 float32): [n, 8] uint32 (ok, group, entry,
+    generation, leaf, pdf bits, importance bits, 0)."""
+    b = np.ascontiguousarray(boxes, np.float32)
+    sd = np.ascontiguousarray(seeds, np.uint32)
+    out = np.zeros((len(b), 8), np.uint32)
+    _trace_lib().oracle_light_sample_brick_box(ctypes.byref(view), _ptr(b), _ptr(sd), len(b), _ptr(out))
+    return out
+
+
+def light_brick_pdf(view: _TreeView, queries: np.ndarray, box: bool) -> np.ndarray:
+    """LightTreeBrickPdf (box False) or LightTreeBrickPdfBox per query (lo, hi, n as float bits; group,
+    entry, generation; 12 u32)."""
+    q = np.ascontiguousarray(queries, np.uint32)
+    pdf = np.zeros(len(q), np.float32)
+    _trace_lib().oracle_light_brick_pdf(ctypes.byref(view), _ptr(q), len(q), 1 if box else 0, _ptr(pdf))
+    return pdf
+
+
+def light_box_support(view: _TreeView, queries: np.ndarray) -> np.ndarray:
+    """The light NEE probe's support query per 16-word query: (found, has leaf, box pdf, target)."""
+    q = np.ascontiguousarray(queries, np.uint32)
+    out = np.zeros((len(q), 4), np.float32)
+    _trace_lib().oracle_light_box_support(ctypes.byref(view), _ptr(q), len(q), _ptr(out))
+    return out
 
 
 class OracleScene:

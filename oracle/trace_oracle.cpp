@@ -539,10 +539,24 @@ void EmitterCosines(V d_lo, V d_hi, float c[6])
         c[2 * a + 1] = dl < 0.0f ? -dl / std::sqrt(dl * dl + others) : 0.0f;
     }
 }
-float Importance(V x, V n, V lo, V hi, const float face[6])
+// A receiver: the box [lo, hi] (lo = hi for a point) with the unit normal n (LtReceiver,
+// caelix_light_tree.hlsl:368-395); `point` selects the point form of LtImportance.
+struct Recv
+{
+    V lo, hi, n;
+    bool point;
+};
+Recv PointRecv(V x, V n)
+{
+    return Recv{x, x, n, true};
+}
+// LtImportance (caelix_light_tree.hlsl:105-155): emitter cosines over D = [r_lo - hi, r_hi - lo], the
+// receiver bound cr = min(1, m / dmin) with m = n.(c_B - c_R) + |n|.(h_B + h_R), the distance term
+// max(|c_R - c_B|^2, max(1, extent^2)), extent^2 = |h_B|^2 for a point, (|h_B| + |h_R|)^2 for a box.
+float ImportanceR(const Recv& r, V lo, V hi, const float face[6])
 {
     float c[6];
-    EmitterCosines(Sub(x, hi), Sub(x, lo), c);
+    EmitterCosines(Sub(r.lo, hi), Sub(r.hi, lo), c);
     float e = 0.0f;
     for (int f = 0; f < 6; ++f)
     {
@@ -554,22 +568,41 @@ float Importance(V x, V n, V lo, V hi, const float face[6])
     }
     const V centre = Mul(Add(lo, hi), 0.5f);
     const V half = Mul(Sub(hi, lo), 0.5f);
+    const V rc = r.point ? r.lo : Mul(Add(r.lo, r.hi), 0.5f);
+    const V rh = Mul(Sub(r.hi, r.lo), 0.5f);
+    const V n = r.n;
     float cr = 1.0f;
     if (n.x != 0.0f || n.y != 0.0f || n.z != 0.0f)
     {
-        const float m = Dot(n, Sub(centre, x)) + Dot(Mk(std::fabs(n.x), std::fabs(n.y), std::fabs(n.z)), half);
+        const V an = Mk(std::fabs(n.x), std::fabs(n.y), std::fabs(n.z));
+        float m = Dot(n, Sub(centre, rc)) + Dot(an, half);
+        if (!r.point)
+        {
+            m += Dot(an, rh);
+        }
         if (m <= 0.0f)
         {
             return 0.0f;
         }
-        const V gap = Mk(std::fmax(std::fmax(lo.x - x.x, x.x - hi.x), 0.0f), std::fmax(std::fmax(lo.y - x.y, x.y - hi.y), 0.0f),
-                         std::fmax(std::fmax(lo.z - x.z, x.z - hi.z), 0.0f));
+        const V gap = Mk(std::fmax(std::fmax(lo.x - r.hi.x, r.lo.x - hi.x), 0.0f),
+                         std::fmax(std::fmax(lo.y - r.hi.y, r.lo.y - hi.y), 0.0f),
+                         std::fmax(std::fmax(lo.z - r.hi.z, r.lo.z - hi.z), 0.0f));
         const float dmin = std::sqrt(Dot(gap, gap));
         cr = dmin > 0.0f ? std::fmin(1.0f, m / dmin) : 1.0f;
     }
-    const V tc = Sub(x, centre);
-    const float d2 = std::fmax(Dot(tc, tc), std::fmax(1.0f, Dot(half, half)));
+    const V tc = Sub(rc, centre);
+    float ext2 = Dot(half, half);
+    if (!r.point)
+    {
+        const float ext = std::sqrt(Dot(half, half)) + std::sqrt(Dot(rh, rh));
+        ext2 = ext * ext;
+    }
+    const float d2 = std::fmax(Dot(tc, tc), std::fmax(1.0f, ext2));
     return (e * cr) / d2;
+}
+float Importance(V x, V n, V lo, V hi, const float face[6])
+{
+    return ImportanceR(PointRecv(x, n), lo, hi, face);
 }
 struct Pick
 {
@@ -630,6 +663,7 @@ struct Lights
     uint32_t group_shift;
     uint32_t encoding;
     uint32_t mis;
+    uint32_t trees;  // LightTreeSelect's mask: bit k = tree k takes part
 };
 Lights OpenLights(const CxLightTreeView& v)
 {
@@ -650,11 +684,12 @@ Lights OpenLights(const CxLightTreeView& v)
     l.group_shift = d->group_shift;
     l.encoding = d->encoding;
     l.mis = v.mis;
+    l.trees = v.select != 0u ? (v.select & 3u) : 3u;
     return l;
 }
 Summ TreeSummary(const Lights& L, uint32_t k, uint32_t& root)
 {
-    root = L.headers[k].root;
+    root = ((L.trees >> k) & 1u) != 0u ? L.headers[k].root : kNoneU;
     g_demand[g_kind] += 64;
     Summ e;
     if (root == kNoneU)
@@ -675,14 +710,19 @@ uint32_t ChildCount(const uint32_t* c)
     g_demand[g_kind] += 16;
     return uint32_t(c[0] != kNoneU) + uint32_t(c[1] != kNoneU) + uint32_t(c[2] != kNoneU) + uint32_t(c[3] != kNoneU);
 }
-uint32_t ChooseLeaf(const Lights& L, V x, V n, uint32_t& rng, float& pdf, uint32_t& tree)
+// LtChooseTreeR (caelix_light_tree.hlsl:417-437) and LtChooseLeafR (:466-512): a tree, then a leaf of
+// it, each choice a weighted reservoir over importance with power as the fallback; pdf times every
+// choice's probability. leaf_importance = the importance of the summary of the drawn leaf's slot (0 when
+// the last choice fell back to the power; for the point form it is not asked for).
+uint32_t ChooseLeafR(const Lights& L, const Recv& r, uint32_t& rng, float& pdf, uint32_t& tree, float& leaf_importance)
 {
+    leaf_importance = 0.0f;
     Pick pick;
     uint32_t roots[2];
     for (uint32_t k = 0; k < 2; ++k)
     {
         const Summ s = TreeSummary(L, k, roots[k]);
-        PickAdd(pick, Importance(x, n, s.lo, s.hi, s.face), Power6(s.face), k, LightRandomFloat(rng));
+        PickAdd(pick, ImportanceR(r, s.lo, s.hi, s.face), Power6(s.face), k, LightRandomFloat(rng));
     }
     float probability;
     const uint32_t k = PickEnd(pick, probability);
@@ -700,6 +740,11 @@ uint32_t ChooseLeaf(const Lights& L, V x, V n, uint32_t& rng, float& pdf, uint32
         if (ChildCount(nd.child) == 1)
         {
             chosen = nd.child[0];
+            if (!r.point && (chosen & kLtChildLeafBit) != 0)
+            {
+                const Summ slot = SlotOf(nd, 0);
+                leaf_importance = ImportanceR(r, slot.lo, slot.hi, slot.face);
+            }
         }
         else
         {
@@ -711,7 +756,7 @@ uint32_t ChooseLeaf(const Lights& L, V x, V n, uint32_t& rng, float& pdf, uint32
                     continue;
                 }
                 const Summ slot = SlotOf(nd, s);
-                PickAdd(p, Importance(x, n, slot.lo, slot.hi, slot.face), Power6(slot.face), s, LightRandomFloat(rng));
+                PickAdd(p, ImportanceR(r, slot.lo, slot.hi, slot.face), Power6(slot.face), s, LightRandomFloat(rng));
             }
             float prob;
             const uint32_t s = PickEnd(p, prob);
@@ -721,6 +766,7 @@ uint32_t ChooseLeaf(const Lights& L, V x, V n, uint32_t& rng, float& pdf, uint32
             }
             pdf *= prob;
             chosen = nd.child[s];
+            leaf_importance = p.sum_i > 0.0f ? p.weight_i : 0.0f;
         }
         if ((chosen & kLtChildLeafBit) != 0)
         {
@@ -730,7 +776,13 @@ uint32_t ChooseLeaf(const Lights& L, V x, V n, uint32_t& rng, float& pdf, uint32
     }
     return kNoneU;
 }
-float SlotProbability(const LightBvhNode& nd, V x, V n, uint32_t slot)
+uint32_t ChooseLeaf(const Lights& L, V x, V n, uint32_t& rng, float& pdf, uint32_t& tree)
+{
+    float unused;
+    return ChooseLeafR(L, PointRecv(x, n), rng, pdf, tree, unused);
+}
+// LtSlotProbabilityR (:515-546): the reservoir's probability of `slot` as a closed form.
+float SlotProbabilityR(const LightBvhNode& nd, const Recv& r, uint32_t slot)
 {
     if (ChildCount(nd.child) == 1)
     {
@@ -744,7 +796,7 @@ float SlotProbability(const LightBvhNode& nd, V x, V n, uint32_t slot)
             continue;
         }
         const Summ sm = SlotOf(nd, s);
-        const float i = Importance(x, n, sm.lo, sm.hi, sm.face);
+        const float i = ImportanceR(r, sm.lo, sm.hi, sm.face);
         const float p = Power6(sm.face);
         sum_i += i > 0.0f ? i : 0.0f;
         sum_p += p > 0.0f ? p : 0.0f;
@@ -756,14 +808,15 @@ float SlotProbability(const LightBvhNode& nd, V x, V n, uint32_t slot)
     }
     return PickProbability(sum_i, sum_p, it, pt);
 }
-float TreeProbability(const Lights& L, V x, V n, uint32_t target)
+// LtTreeProbabilityR (:439-462)
+float TreeProbabilityR(const Lights& L, const Recv& r, uint32_t target)
 {
     float sum_i = 0.0f, sum_p = 0.0f, it = 0.0f, pt = 0.0f;
     for (uint32_t k = 0; k < 2; ++k)
     {
         uint32_t root;
         const Summ s = TreeSummary(L, k, root);
-        const float i = Importance(x, n, s.lo, s.hi, s.face);
+        const float i = ImportanceR(r, s.lo, s.hi, s.face);
         const float p = Power6(s.face);
         sum_i += i > 0.0f ? i : 0.0f;
         sum_p += p > 0.0f ? p : 0.0f;
@@ -775,9 +828,13 @@ float TreeProbability(const Lights& L, V x, V n, uint32_t target)
     }
     return PickProbability(sum_i, sum_p, it, pt);
 }
-float LeafProbability(const Lights& L, uint32_t k, uint32_t leaf, V x, V n)
+// LtLeafProbabilityR (:549-572): the product of the slot probabilities up the parent links, times the
+// tree's; 0 for an empty or unselected tree.
+float LeafProbabilityR(const Lights& L, uint32_t k, uint32_t leaf, const Recv& r)
 {
-    if (L.headers[k].root == kNoneU)
+    uint32_t root;
+    TreeSummary(L, k, root);
+    if (root == kNoneU)
     {
         return 0.0f;
     }
@@ -793,16 +850,20 @@ float LeafProbability(const Lights& L, uint32_t k, uint32_t leaf, V x, V n)
     for (uint32_t depth = 0; depth < 64; ++depth)
     {
         const LightBvhNode& nd = L.nodes[k][node];
-        probability *= SlotProbability(nd, x, n, slot);
+        probability *= SlotProbabilityR(nd, r, slot);
         if (nd.parent == kNoneU)
         {
-            return probability * TreeProbability(L, x, n, k);
+            return probability * TreeProbabilityR(L, r, k);
         }
         slot = nd.parent_slot;
         node = nd.parent;
         g_demand[g_kind] += 8;
     }
     return 0.0f;
+}
+float LeafProbability(const Lights& L, uint32_t k, uint32_t leaf, V x, V n)
+{
+    return LeafProbabilityR(L, k, leaf, PointRecv(x, n));
 }
 struct Rec
 {
@@ -1032,29 +1093,40 @@ bool LightTreeSample(const Lights& L, V x, V n, uint32_t& rng, LightPoint& p)
     p.pdf = ((w[f] / sum) / float(count)) * pdf_brick;
     return true;
 }
-// LightTreePointPdf (caelix_light_tree.hlsl:936-949) over LightTreeFindLightFace (:853-932): the area
-// pdf of the draw of face `face` of the group-local voxel v of `group`; `found` = a light face whose brick
-// has a leaf (LampHitWeight's condition, caelix_path_trace.hlsl:438-443).
-float LightPointPdf(const Lights& L, V x, V n, uint32_t group, const uint32_t v[3], uint32_t face, uint32_t block_value,
-                    bool& found)
+// A brick of the tree by (group, index entry) and the row generation it was named at (LightBrickRef).
+struct BrickRef
 {
-    found = false;
+    uint32_t group, entry, generation;
+};
+// LightTreeFindLightFace (caelix_light_tree.hlsl:853-932): face `face` of the group-local voxel v of
+// `group` with the raw block value is a light face of the tree when its material code is a light, the
+// group has a block with the voxel's brick in its index and the voxel is in that face family of the
+// record. Then brick, leaf (LtBrickLeaf; kNoneU when no draw reaches the brick) and in_brick =
+// P(face | brick, x) / family count for the receiver x.
+bool FindLightFace(const Lights& L, V x, uint32_t group, const uint32_t v[3], uint32_t face, uint32_t block_value,
+                   BrickRef& brick, uint32_t& leaf, float& in_brick)
+{
+    brick = BrickRef{group, kNoneU, 0u};
+    leaf = kNoneU;
+    in_brick = 0.0f;
     const uint32_t code = MaterialCode(block_value & 0xFFFFu, L.encoding);
     g_demand[g_kind] += 4;
     if (((L.light_codes[code >> 5] >> (code & 31u)) & 1u) == 0u || face > 5u)
     {
-        return 0.0f;
+        return false;
     }
     const uint32_t* block = GroupBlock(L, group);
     if (block == nullptr)
     {
-        return 0.0f;
+        return false;
     }
     const uint32_t entry = FindIndexEntry(block, block[0], Morton3(v[0] >> 3, v[1] >> 3, v[2] >> 3), L.group_shift);
     if (entry == kNoneU)
     {
-        return 0.0f;
+        return false;
     }
+    brick.entry = entry;
+    brick.generation = L.rows[group].generation;
     const Rec rec = LoadRec(block, entry);
     Frame frame;
     const V origin = GroupFrame(L, group, frame);
@@ -1064,7 +1136,7 @@ float LightPointPdf(const Lights& L, V x, V n, uint32_t group, const uint32_t v[
     const uint32_t count = FamilyCount(rec, face);
     if (sum <= 0.0f || w[face] <= 0.0f || count == 0u)
     {
-        return 0.0f;
+        return false;
     }
     uint32_t first = 0;
     for (uint32_t g = 0; g < face; ++g)
@@ -1084,16 +1156,76 @@ float LightPointPdf(const Lights& L, V x, V n, uint32_t group, const uint32_t v[
     }
     if (!hit)
     {
+        return false;
+    }
+    in_brick = (w[face] / sum) / float(count);
+    leaf = BrickLeaf(L, block, group, entry);
+    return true;
+}
+// LightTreePointPdf (caelix_light_tree.hlsl:936-949): the area pdf of the draw of that face for the
+// receiver (x, n); `found` = a light face whose brick has a leaf (LampHitWeight's condition,
+// caelix_path_trace.hlsl:438-443).
+float LightPointPdf(const Lights& L, V x, V n, uint32_t group, const uint32_t v[3], uint32_t face, uint32_t block_value,
+                    bool& found)
+{
+    BrickRef brick;
+    uint32_t leaf;
+    float in_brick;
+    found = FindLightFace(L, x, group, v, face, block_value, brick, leaf, in_brick) && leaf != kNoneU;
+    if (!found)
+    {
         return 0.0f;
     }
-    const float in_brick = (w[face] / sum) / float(count);
-    const uint32_t leaf = BrickLeaf(L, block, group, entry);
+    return LeafProbability(L, leaf >> 31, leaf & 0x7FFFFFFFu, x, n) * in_brick;
+}
+// LightBrickRefValid (caelix_light_tree.hlsl:640-646): the row has a block, of the reference's
+// generation, and the entry is below its record count.
+bool BrickRefValid(const Lights& L, const BrickRef& b)
+{
+    const GroupLightRow& row = L.rows[b.group];
+    return row.block_units != 0u && row.generation == b.generation && b.entry < L.heap[uint64_t(row.block_offset) * 4];
+}
+// LightTreeBrickPdf / LightTreeBrickPdfBox (:769-777, :822-831) over LtBrickProbabilityR (:575-592).
+float BrickPdfR(const Lights& L, const Recv& r, const BrickRef& brick)
+{
+    if (!BrickRefValid(L, brick))
+    {
+        return 0.0f;
+    }
+    const uint32_t leaf = BrickLeaf(L, GroupBlock(L, brick.group), brick.group, brick.entry);
+    return leaf == kNoneU ? 0.0f : LeafProbabilityR(L, leaf >> 31, leaf & 0x7FFFFFFFu, r);
+}
+// LightTreeSampleBrickBox (:787-818): the box-receiver draw of a brick; leaf = tree << 31 | index.
+bool SampleBrickBox(const Lights& L, const Recv& r, uint32_t& rng, BrickRef& brick, float& pdf_brick, uint32_t& leaf,
+                    float& importance)
+{
+    brick = BrickRef{kNoneU, kNoneU, 0u};
+    pdf_brick = 1.0f;
+    leaf = kNoneU;
+    uint32_t tree;
+    const uint32_t index = ChooseLeafR(L, r, rng, pdf_brick, tree, importance);
+    if (index == kNoneU)
+    {
+        importance = 0.0f;
+        return false;
+    }
+    const LightBvhLeaf& lf = L.leaves[tree][index];
+    brick = BrickRef{lf.group, lf.entry, L.rows[lf.group].generation};
+    leaf = (tree << 31) | index;
+    return true;
+}
+// LightTreeLeafImportanceBox (:837-850): the box importance of the brick's own leaf summary.
+float LeafImportanceBox(const Lights& L, const BrickRef& brick, const Recv& r)
+{
+    const uint32_t* block = GroupBlock(L, brick.group);
+    const uint32_t leaf = block != nullptr ? BrickLeaf(L, block, brick.group, brick.entry) : kNoneU;
     if (leaf == kNoneU)
     {
         return 0.0f;
     }
-    found = true;
-    return LeafProbability(L, leaf >> 31, leaf & 0x7FFFFFFFu, x, n) * in_brick;
+    const LightBvhLeaf& lf = L.leaves[leaf >> 31][leaf & 0x7FFFFFFFu];
+    return ImportanceR(r, Mk(lf.summary.lo[0], lf.summary.lo[1], lf.summary.lo[2]),
+                       Mk(lf.summary.hi[0], lf.summary.hi[1], lf.summary.hi[2]), lf.summary.face);
 }
 // A hit of the world traversal (identity entity frames): the group of the brick record, the voxel less
 // the group origin.
@@ -2768,6 +2900,70 @@ void oracle_light_point_pdf(const CxLightTreeView* view, const uint32_t* queries
         const uint32_t v[3] = {q[7], q[8], q[9]};
         bool found;
         pdf[i] = LightPointPdf(L, Mk(f[0], f[1], f[2]), Mk(f[3], f[4], f[5]), q[6], v, q[10], q[11], found);
+    }
+}
+
+void oracle_light_sample_brick_box(const CxLightTreeView* view, const float* boxes, const uint32_t* seeds,
+                                   uint32_t count, uint32_t* out)
+{
+    const Lights L = OpenLights(*view);
+    for (uint32_t i = 0; i < count; ++i)
+    {
+        const float* b = boxes + uint64_t(i) * 9;
+        const Recv r{Mk(b[0], b[1], b[2]), Mk(b[3], b[4], b[5]), Mk(b[6], b[7], b[8]), false};
+        uint32_t rng = seeds[i];
+        BrickRef brick;
+        float pdf, importance;
+        uint32_t leaf;
+        const bool ok = SampleBrickBox(L, r, rng, brick, pdf, leaf, importance);
+        uint32_t* o = out + uint64_t(i) * 8;
+        o[0] = ok ? 1u : 0u;
+        o[1] = brick.group;
+        o[2] = brick.entry;
+        o[3] = brick.generation;
+        o[4] = leaf;
+        std::memcpy(o + 5, &pdf, 4);
+        std::memcpy(o + 6, &importance, 4);
+        o[7] = 0u;
+    }
+}
+
+void oracle_light_brick_pdf(const CxLightTreeView* view, const uint32_t* queries, uint32_t count, uint32_t box, float* pdf)
+{
+    const Lights L = OpenLights(*view);
+    for (uint32_t i = 0; i < count; ++i)
+    {
+        const uint32_t* q = queries + uint64_t(i) * 12;
+        float f[9];
+        std::memcpy(f, q, sizeof(f));
+        const V lo = Mk(f[0], f[1], f[2]);
+        const Recv r{lo, box != 0u ? Mk(f[3], f[4], f[5]) : lo, Mk(f[6], f[7], f[8]), box == 0u};
+        pdf[i] = BrickPdfR(L, r, BrickRef{q[9], q[10], q[11]});
+    }
+}
+
+void oracle_light_box_support(const CxLightTreeView* view, const uint32_t* queries, uint32_t count, float* out)
+{
+    const Lights L = OpenLights(*view);
+    for (uint32_t i = 0; i < count; ++i)
+    {
+        const uint32_t* q = queries + uint64_t(i) * 16;
+        float f[12];
+        std::memcpy(f, q, sizeof(f));
+        const Recv r{Mk(f[0], f[1], f[2]), Mk(f[4], f[5], f[6]), Mk(f[8], f[9], f[10]), false};
+        const uint32_t v[3] = {q[13] & 0x3FFu, (q[13] >> 10) & 0x3FFu, q[13] >> 20};
+        BrickRef brick;
+        uint32_t leaf;
+        float in_brick;
+        float* o = out + uint64_t(i) * 4;
+        o[0] = o[1] = o[2] = o[3] = 0.0f;
+        if (FindLightFace(L, Mk(f[3], f[7], f[11]), q[12], v, q[14], q[15], brick, leaf, in_brick))
+        {
+            o[0] = 1.0f;
+            o[1] = leaf != kNoneU ? 1.0f : 0.0f;
+            o[2] = BrickPdfR(L, r, brick);
+            o[3] = LeafImportanceBox(L, brick, r);
+        }
     }
 }
 

@@ -59,6 +59,8 @@ size_t cx_abi_c99_touch(void)
         (AnyFn)cx_light_bvh_build_static,
         (AnyFn)cx_light_bvh_scratch_bytes,
         (AnyFn)cx_light_bvh_scratch_init,
+        (AnyFn)cx_light_box_support,
+        (AnyFn)cx_light_brick_pdf,
         (AnyFn)cx_light_gather,
         (AnyFn)cx_light_gather_entities,
         (AnyFn)cx_light_mis_weight,
@@ -66,6 +68,7 @@ size_t cx_abi_c99_touch(void)
         (AnyFn)cx_light_pass_scratch_bytes,
         (AnyFn)cx_light_point_pdf,
         (AnyFn)cx_light_sample,
+        (AnyFn)cx_light_sample_brick_box,
         (AnyFn)cx_light_tree_create,
         (AnyFn)cx_light_tree_descriptor,
         (AnyFn)cx_light_tree_destroy,
