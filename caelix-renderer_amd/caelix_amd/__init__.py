@@ -122,7 +122,65 @@ class CxTraceParams(ctypes.Structure):
                 ("pad1", ctypes.c_float), ("sky_ground", ctypes.c_float * 3), ("pad2", ctypes.c_float),
                 ("fog_color", ctypes.c_float * 3), ("fog_density", ctypes.c_float),
                 ("bounces_transparent", ctypes.c_uint32), ("bounce_nee_probability", ctypes.c_float),
-                ("pad3", ctypes.c_uint32 * 2)]
+                ("hint", ctypes.c_void_p)]
+
+
+class CxLightHint(ctypes.Structure):
+    _fields_ = [("previous", ctypes.c_void_p), ("current", ctypes.c_void_p), ("read", ctypes.c_void_p),
+                ("previous_camera", ctypes.POINTER(CxCamera)), ("patch_shift", ctypes.c_uint32),
+                ("reset", ctypes.c_uint32)]
+
+
+assert ctypes.sizeof(CxTraceParams) == 128 and ctypes.sizeof(CxLightHint) == 40
+
+HINT_PATCHES = {1: 0, 2: 1, 4: 2}
+DEFAULT_HINT_PATCH = 4  # the reference renderer's kDefaultHintPatch
+
+
+class LightHint:
+    """The light NEE's visibility hint images of one view, as the reference renderer keeps them
+    (renderer_light.cpp / docs of the light NEE section 12.1): two images of 16-bit words (sun | lamp << 8,
+    k / 255) per patch of `patch` pixels, ping-ponged between frames; the first frame after a creation
+    runs with the reset (every primary vertex reads 0x8080); the previous frame's camera reprojects the
+    primary surfaces. `read` (optional, per pixel): the word each primary vertex read."""
+
+    def __init__(self, width: int, height: int, patch: int = DEFAULT_HINT_PATCH, keep_read: bool = False):
+        if patch not in HINT_PATCHES:
+            raise ValueError("hint patch must be 1, 2 or 4")
+        torch = _torch()
+        self.width, self.height, self.patch = int(width), int(height), int(patch)
+        self.shift = HINT_PATCHES[self.patch]
+        self.columns = (self.width + self.patch - 1) >> self.shift
+        self.rows = (self.height + self.patch - 1) >> self.shift
+        self.images = [torch.full((self.rows * self.columns,), 0x8080 - 0x10000, dtype=torch.int16, device="cuda")
+                       for _ in range(2)]
+        self.read = torch.zeros(self.width * self.height, dtype=torch.int16, device="cuda") if keep_read else None
+        self.parity = 0
+        self.history = False
+        self.camera = None
+        self._keep = None
+
+    def frame_struct(self, camera: dict) -> CxLightHint:
+        """The hint of the next frame (kept alive until `advance`)."""
+        prev = camera_struct(self.camera) if self.camera is not None else None
+        h = CxLightHint(self.images[self.parity].data_ptr(), self.images[1 - self.parity].data_ptr(),
+                        self.read.data_ptr() if self.read is not None else None,
+                        ctypes.pointer(prev) if prev is not None else None, self.shift, 0 if self.history else 1)
+        self._keep = (h, prev)
+        return h
+
+    def advance(self, camera: dict) -> None:
+        self.parity ^= 1
+        self.history = True
+        self.camera = dict(camera)
+
+    def current(self) -> np.ndarray:
+        """[rows, columns, 2] uint8 of the newest image: (sun, lamp)."""
+        w = self.images[self.parity].cpu().numpy().view(np.uint16).reshape(self.rows, self.columns)
+        return np.stack([w & 0xFF, w >> 8], axis=-1).astype(np.uint8)
+
+    def read_words(self) -> np.ndarray:
+        return self.read.cpu().numpy().view(np.uint16).reshape(self.height, self.width)
 
 
 TRACE_NO_RUSSIAN_ROULETTE = 1
@@ -913,7 +971,7 @@ class TraceScene:
 
     def trace_frame(self, camera: dict, bounces: int, seed: int, spp: int = 1, color=None, radiance=None,
                     attrib=None, counters=None, stream=None, flags: int = 0, sky: dict | None = None, fog=None,
-                    bounces_transparent: int = 0, bounce_nee_probability: float = 1.0):
+                    bounces_transparent: int = 0, bounce_nee_probability: float = 1.0, hint: LightHint | None = None):
         torch = _torch()
         w, h = int(camera["width"]), int(camera["height"])
         color = color if color is not None else torch.empty(w * h * 4, dtype=torch.float32, device="cuda")
@@ -925,9 +983,15 @@ class TraceScene:
         apply_sky(params, sky, fog)
         params.bounces_transparent = int(bounces_transparent)
         params.bounce_nee_probability = float(bounce_nee_probability)
+        hint_struct = None
+        if hint is not None:
+            hint_struct = hint.frame_struct(camera)
+            params.hint = ctypes.cast(ctypes.pointer(hint_struct), ctypes.c_void_p)
         out = self._output(camera, color, radiance, attrib, counters)
         check(self.lib.cx_trace_frame(ctypes.byref(self.scene), ctypes.byref(self.view), ctypes.byref(cam),
                                       ctypes.byref(params), ctypes.byref(out), stream_handle(stream)), "cx_trace_frame")
+        if hint is not None:
+            hint.advance(camera)
         return color, radiance, attrib, counters
 
     def trace_rays(self, rays: np.ndarray) -> np.ndarray:

@@ -27,6 +27,27 @@ struct SkyParams
     V3 ground;
 };
 
+// The light NEE's visibility hint per pixel patch (kLightNeeHint, caelix_path_trace.hlsl:229-236,
+// 338-344, 453-497): a word sun | lamp << 8 of 8-bit fractions k / 255 per patch of 1 << shift pixels.
+constexpr float kHintFloor = 0.05f;
+constexpr float kHintAlpha = 0.2f;
+constexpr uint32_t kHintWordOff = 0xFFFFu;    // (1, 1): the choice weights unchanged
+constexpr uint32_t kHintWordReset = 0x8080u;  // (128 / 255, 128 / 255): no history
+constexpr uint32_t kNeeTracedNone = 0u;
+constexpr uint32_t kNeeTracedSun = 1u;
+constexpr uint32_t kNeeTracedLamp = 2u;
+
+struct HintParams
+{
+    const uint16_t* previous;  // [patch rows][columns]; null: no hint
+    uint32_t columns;          // patch columns: (width + patch - 1) >> shift
+    uint32_t shift;
+    uint32_t reset;            // kLightNeeHintReset: every primary vertex reads kHintWordReset
+    uint32_t has_prev;         // a previous camera (prev) to reproject the primary surface into
+    uint32_t sample;           // the frame's sample: its counts start at sample 0
+    CameraView prev;
+};
+
 struct TraceParams
 {
     uint32_t bounces;
@@ -40,6 +61,7 @@ struct TraceParams
     uint32_t transmissive;  // kRenderFlagShadowTransmission: the primary vertex's shadow ray passes media
     uint32_t checkerboard;  // kRenderFlagDeltaCheckerboard: the chain's first interface split by pixel parity
     float nee_q;            // bounce_nee_probability (1: every bounce vertex runs its NEE)
+    HintParams hint;        // the primary vertex's visibility hint (previous null: off)
 };
 
 // Whether the sun stream (nee_rng) is carried from vertex to vertex: the sun's NEE or the bounce NEE
@@ -450,8 +472,16 @@ CX_DEV V3 ShadowTransmissive(const SceneView& s, V3 origin, V3 dir, float t_max,
 // brick (for the adjoint, which runs without the sun).
 CX_DEV bool NeeCandidate(const SceneView& s, const LightView& L, const SkyParams& k, V3 x_r, V3 n_r,
                          uint32_t& light_rng, uint32_t& nee_rng, V3& direction, float& t_max, V3& visible, V3& unscaled,
-                         uint32_t& light_record)
+                         uint32_t& light_record, uint32_t hint_word = kHintWordOff, uint32_t* traced = nullptr)
 {
+    // UnpackHint (caelix_path_trace.hlsl:465-468): the visibility hint that multiplies the choice
+    // weights, floored at kHintFloor; kHintWordOff gives exactly 1.
+    const float hint_sun = float(hint_word & 0xFFu) * (1.0f / 255.0f);
+    const float hint_lamp = float((hint_word >> 8) & 0xFFu) * (1.0f / 255.0f);
+    if (traced != nullptr)
+    {
+        *traced = kNeeTracedNone;
+    }
     const uint32_t point_seed = RandomU32(light_rng);
     V3 contribution = Make(0.0f, 0.0f, 0.0f);
     direction = Make(0.0f, 0.0f, 0.0f);
@@ -485,14 +515,14 @@ CX_DEV bool NeeCandidate(const SceneView& s, const LightView& L, const SkyParams
             }
         }
     }
-    const float w_lamp = Luminance(contribution) * fmaxf(0.05f, 1.0f);
+    const float w_lamp = Luminance(contribution) * fmaxf(kHintFloor, hint_lamp);
     const uint32_t sun_start = nee_rng;
     float w_sun = 0.0f;
     float u_choice = 0.0f;
     if (k.sun != 0u)
     {
         V3 sun_direction;
-        w_sun = Luminance(SunCandidate(k, n_r, nee_rng, sun_direction)) * fmaxf(0.05f, 1.0f);
+        w_sun = Luminance(SunCandidate(k, n_r, nee_rng, sun_direction)) * fmaxf(kHintFloor, hint_sun);
         u_choice = RandomFloat01(nee_rng);
     }
     const float w_sum = w_sun + w_lamp;
@@ -508,9 +538,17 @@ CX_DEV bool NeeCandidate(const SceneView& s, const LightView& L, const SkyParams
         visible = Mul(sun, Div(w_sum, w_sun));
         unscaled = Make(0.0f, 0.0f, 0.0f);
         light_record = kHashNone;
+        if (traced != nullptr)
+        {
+            *traced = kNeeTracedSun;
+        }
         return true;
     }
     visible = Mul(contribution, Div(w_sum, w_lamp));
+    if (traced != nullptr)
+    {
+        *traced = kNeeTracedLamp;
+    }
     return true;
 }
 
@@ -540,6 +578,65 @@ CX_DEV float LampHitWeight(const SceneView& s, const LightView& L, uint32_t segm
     }
     const float cos_l = -Dot(n, direction);
     return MisWeight(p_area, segment_normal, direction, hit.t, cos_l, false);
+}
+
+// PreviousPixel (caelix_path_trace.hlsl:154-172): the pixel (current image size) where the previous
+// camera saw the world point p; false when p was not in front of it.
+CX_DEV bool PreviousPixel(const CameraView& prev, uint32_t width, uint32_t height, V3 p, float& x, float& y)
+{
+    const V3 rel = Sub(p, Make(prev.position[0], prev.position[1], prev.position[2]));
+    const float z = Dot(rel, Make(prev.forward[0], prev.forward[1], prev.forward[2]));
+    x = 0.0f;
+    y = 0.0f;
+    if (!(z > 1e-6f))
+    {
+        return false;
+    }
+    const float ndc_x = Div(Dot(rel, Make(prev.right[0], prev.right[1], prev.right[2])),
+                            (z * prev.tan_half_fov_y) * prev.aspect);
+    const float ndc_y = Div(Dot(rel, Make(prev.up[0], prev.up[1], prev.up[2])), z * prev.tan_half_fov_y);
+    x = ((ndc_x + 1.0f) * 0.5f) * float(width);
+    y = ((1.0f - ndc_y) * 0.5f) * float(height);
+    return fabsf(x) < 1e30f && fabsf(y) < 1e30f;
+}
+
+// ReadLightHint (caelix_path_trace.hlsl:479-498) for the in-image pixel (px, py): kHintWordReset with
+// the reset; else the previous hint at the patch of the previous pixel of the primary surface point p
+// (identity entity frames: the point did not move), or at the pixel's own patch when there is no
+// surface, no previous camera, or p was behind it or outside its image.
+CX_DEV uint32_t ReadHint(const HintParams& h, uint32_t width, uint32_t height, uint32_t px, uint32_t py, bool surface,
+                         V3 p)
+{
+    if (h.reset != 0u)
+    {
+        return kHintWordReset;
+    }
+    uint32_t qx = px >> h.shift;
+    uint32_t qy = py >> h.shift;
+    if (surface && h.has_prev != 0u)
+    {
+        float x, y;
+        if (PreviousPixel(h.prev, width, height, p, x, y) && x >= 0.0f && y >= 0.0f && x < float(width) &&
+            y < float(height))
+        {
+            qx = uint32_t(x) >> h.shift;
+            qy = uint32_t(y) >> h.shift;
+        }
+    }
+    return h.previous[uint64_t(qy) * h.columns + qx];
+}
+
+// One channel of WriteLightHint (caelix_path_trace.hlsl:554-580) in the word's 8 bits: the read value
+// moved by kHintAlpha toward visible / traced (kept without a traced ray), rounded as PackHint does.
+CX_DEV uint32_t HintChannel(uint32_t k, uint32_t traced, uint32_t visible)
+{
+    if (traced == 0u)
+    {
+        return k;
+    }
+    const float h = float(k) * (1.0f / 255.0f);
+    const float cur = h + kHintAlpha * (Div(float(visible), float(traced)) - h);
+    return uint32_t(fminf(fmaxf(cur, 0.0f), 1.0f) * 255.0f + 0.5f);
 }
 
 // CaelixPrimaryDirection (caelix_frame.hlsl:164-171): the pinhole ray through the pixel centre.

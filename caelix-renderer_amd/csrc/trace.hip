@@ -126,6 +126,8 @@ struct Workspace
     uint32_t* queue[2]; // [paths] path queues (ping-pong by bounce parity)
     uint32_t* nee_rng;  // [paths] the sun's random stream, kept whenever something draws from it: the
                         // sun's NEE, or the bounce NEE budget below 1 (KeepsNeeStream)
+    uint32_t* hint_word;  // [paths] the visibility hint the primary vertex read (sample 0) | traced << 16
+    uint2* hint_counts;   // [paths] primary shadow rays over the samples: sun traced | visible << 16, lamp
     uint32_t* ctl;      // [0], [1] queue lengths, [2] shadow queue length
     float* partials;    // [kGradRows][768] adjoint: per-workgroup albedo gradient sums
 };
@@ -161,7 +163,7 @@ uint32_t GradRows(uint32_t paths)
 }
 uint64_t WorkspaceBytes(uint32_t paths)
 {
-    return 13 * Align256(uint64_t(paths) * 16) + 3 * Align256(uint64_t(paths) * 4) + 256 +
+    return 13 * Align256(uint64_t(paths) * 16) + 4 * Align256(uint64_t(paths) * 4) + Align256(uint64_t(paths) * 8) + 256 +
            Align256(uint64_t(GradRows(paths)) * kGradEntries * 4);
 }
 Workspace CarveWorkspace(void* base, uint32_t paths)
@@ -190,6 +192,8 @@ Workspace CarveWorkspace(void* base, uint32_t paths)
     w.queue[0] = reinterpret_cast<uint32_t*>(take(uint64_t(paths) * 4));
     w.queue[1] = reinterpret_cast<uint32_t*>(take(uint64_t(paths) * 4));
     w.nee_rng = reinterpret_cast<uint32_t*>(take(uint64_t(paths) * 4));
+    w.hint_word = reinterpret_cast<uint32_t*>(take(uint64_t(paths) * 4));
+    w.hint_counts = reinterpret_cast<uint2*>(take(uint64_t(paths) * 8));
     w.ctl = reinterpret_cast<uint32_t*>(take(256));
     w.partials = reinterpret_cast<float*>(take(uint64_t(GradRows(paths)) * kGradEntries * 4));
     return w;
@@ -444,6 +448,7 @@ struct ShadowSource
     Views views;
     SceneView sv;
     GradView g;
+    uint32_t hint;  // the primary vertex's shadow rays with the visibility hint: count the visible ones
     CX_DEV uint32_t Count() const { return w.ctl[2]; }
     CX_DEV uint32_t* Cursor() const { return &w.ctl[kCtlShadowCursor]; }
     CX_DEV void Prepare() const {}
@@ -464,6 +469,15 @@ struct ShadowSource
             return;
         }
         const uint32_t path = __float_as_uint(w.shadow_d[item].w);
+        if (hint != 0u)
+        {
+            // one shadow ray per path in this launch: no other lane touches the path's counts
+            const uint32_t traced = (w.hint_word[path] >> 16) & 3u;
+            uint2 counts = w.hint_counts[path];
+            counts.x += traced == kNeeTracedSun ? (1u << 16) : 0u;
+            counts.y += traced == kNeeTracedLamp ? (1u << 16) : 0u;
+            w.hint_counts[path] = counts;
+        }
         const float4 a = w.shadow_add[item];
         float4 r = w.radiance[path];
         r.x = r.x + a.x;
@@ -584,7 +598,8 @@ CX_DEV void PrimaryGlass(const SceneView& s, const TraceParams& prm, const Works
                          const LightView& L, V3 origin, V3 dir, uint4 h, bool even_pixel, uint32_t& rng, uint32_t& light_rng,
                          uint32_t& nee_rng, uint32_t mis_bit, PathCounters& c, bool& shadow, bool& go_on,
                          V3& shadow_origin, V3& shadow_dir, float& shadow_t, V3& shadow_add, uint32_t& shadow_light,
-                         V3& shadow_unscaled)
+                         V3& shadow_unscaled, const CameraView& cam, uint32_t px, uint32_t py, bool hint,
+                         uint32_t& hint_word, uint32_t& traced, uint32_t& seen)
 {
     WorldHit surface{__uint_as_float(h.x), h.y, h.z, {0, 0, 0}, 0u, 0u, 0u};
     uint32_t geometry = h.y;
@@ -635,6 +650,10 @@ CX_DEV void PrimaryGlass(const SceneView& s, const TraceParams& prm, const Works
             break;
         }
     }
+    if (hint)
+    {
+        hint_word = ReadHint(prm.hint, cam.width, cam.height, px, py, surface_hit, Add(ray_o, Mul(ray_d, surface.t)));
+    }
     if (!surface_hit)
     {
         const V3 sky = prm.sky.on != 0u ? SkyRadiance(prm.sky, ray_d) : Make(0.0f, 0.0f, 0.0f);
@@ -661,7 +680,7 @@ CX_DEV void PrimaryGlass(const SceneView& s, const TraceParams& prm, const Works
         {
             V3 visible;
             if (NeeCandidate(s, L, prm.sky, sh.bounce_origin, normal, light_rng, nee_rng, shadow_dir, shadow_t, visible,
-                             shadow_unscaled, shadow_light))
+                             shadow_unscaled, shadow_light, hint_word, &traced))
             {
                 c.shadow_rays += 1;
                 if (prm.transmissive != 0u)
@@ -673,6 +692,7 @@ CX_DEV void PrimaryGlass(const SceneView& s, const TraceParams& prm, const Works
                         const V3 tr = ShadowTransmissive(s, sh.bounce_origin, shadow_dir, shadow_t, medium, c, scene);
                         const V3 add = MulV(visible, tr);
                         w.radiance[path] = make_float4(add.x, add.y, add.z, 0.0f);
+                        seen = tr.x > 0.0f || tr.y > 0.0f || tr.z > 0.0f ? 1u : 0u;
                     }
                 }
                 else
@@ -718,6 +738,14 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
     {
         const uint4 h = w.hit[path];
         w.radiance[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const CameraView& pixel_cam = views.Camera(views.SceneOf(path));
+        uint32_t hx, hy;
+        PathPixel(pixel_cam, views.Local(path), hx, hy);
+        // the visibility hint of the primary vertex (every pixel of the image reads one)
+        const bool hint = prm.hint.previous != nullptr && hx < pixel_cam.width && hy < pixel_cam.height;
+        uint32_t hint_word = kHintWordOff;
+        uint32_t traced = kNeeTracedNone;
+        uint32_t seen = 0u;  // a transmissive shadow ray that reached its light
         if (h.y == 0u)
         {
             // A primary miss: the sky with the sun disk in the sky's frame (the composite's
@@ -725,10 +753,11 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
             V3 sky = Make(0.0f, 0.0f, 0.0f);
             if (prm.sky.on != 0u)
             {
-                const CameraView& cam = views.Camera(views.SceneOf(path));
-                uint32_t px, py;
-                PathPixel(cam, views.Local(path), px, py);
-                sky = SkyRadiance(prm.sky, PrimaryDirection(cam, px, py));
+                sky = SkyRadiance(prm.sky, PrimaryDirection(pixel_cam, hx, hy));
+            }
+            if (hint)
+            {
+                hint_word = ReadHint(prm.hint, pixel_cam.width, pixel_cam.height, hx, hy, false, Make(0.0f, 0.0f, 0.0f));
             }
             w.albedo[path] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             w.emission[path] = make_float4(sky.x, sky.y, sky.z, 0.0f);
@@ -753,7 +782,7 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
             {
                 PrimaryGlass(s, prm, w, path, views.SceneOf(path), L, origin, dir, h, ((px + py) & 1u) == 0u, rng,
                              light_rng, nee_rng, mis_bit, c, shadow, go_on, shadow_origin, shadow_dir, shadow_t,
-                             shadow_add, shadow_light, shadow_unscaled);
+                             shadow_add, shadow_light, shadow_unscaled, cam, px, py, hint, hint_word, traced, seen);
             }
             else
             {
@@ -787,11 +816,15 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
                 }
                 const V3 throughput = Make(1.0f, 1.0f, 1.0f);
                 uint32_t segment_light = 0u;
+                if (hint)
+                {
+                    hint_word = ReadHint(prm.hint, cam.width, cam.height, px, py, true, Add(origin, Mul(dir, t)));
+                }
                 if (prm.bounces >= 1u)
                 {
                     V3 visible;
                     if (NeeCandidate(s, L, prm.sky, sh.bounce_origin, normal, light_rng, nee_rng, shadow_dir, shadow_t, visible,
-                                     shadow_unscaled, shadow_light))
+                                     shadow_unscaled, shadow_light, hint_word, &traced))
                     {
                         c.shadow_rays += 1;
                         shadow = shadow_t > 0.0f;
@@ -810,6 +843,16 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
                 w.ray_o[path] = make_float4(sh.bounce_origin.x, sh.bounce_origin.y, sh.bounce_origin.z, 0.0f);
                 w.ray_d[path] = make_float4(sh.bounce_direction.x, sh.bounce_direction.y, sh.bounce_direction.z, 0.0f);
             }
+        }
+        if (hint)
+        {
+            uint2 counts = prm.hint.sample == 0u ? make_uint2(0u, 0u) : w.hint_counts[path];
+            const uint32_t add = 1u + (seen << 16);
+            counts.x += traced == kNeeTracedSun ? add : 0u;
+            counts.y += traced == kNeeTracedLamp ? add : 0u;
+            w.hint_counts[path] = counts;
+            const uint32_t read = prm.hint.sample == 0u ? hint_word : (w.hint_word[path] & 0xFFFFu);
+            w.hint_word[path] = read | (traced << 16);
         }
     }
     EnqueueShadow(w, path, shadow, shadow_origin, shadow_dir, shadow_t, shadow_add, shadow_light, shadow_unscaled, 1.0f,
@@ -1069,6 +1112,46 @@ __global__ void __launch_bounds__(kWaveThreads) FinishKernel(Views views, Worksp
     {
         GradRowsEnd(w, lds_grad);
     }
+}
+
+// WriteLightHint (caelix_path_trace.hlsl:548-580) after the frame's samples, one thread per patch: the
+// first pixel of the patch in the image (its top-left one, the lowest lane of its 8x8 tile) gives the
+// value read; each channel moves by kHintAlpha toward the fraction of visible primary shadow rays of
+// the patch's pixels over the samples, and keeps its value without one. `read` (optional): the hint
+// each pixel's primary vertex read.
+__global__ void __launch_bounds__(256) HintUpdateKernel(Workspace w, CameraView cam, uint32_t shift, uint32_t columns,
+                                                        uint32_t patch_rows, uint16_t* current, uint16_t* read)
+{
+    const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= columns * patch_rows)
+    {
+        return;
+    }
+    const uint32_t tiles_x = (cam.width + 7) / 8;
+    const uint32_t x0 = (q % columns) << shift;
+    const uint32_t y0 = (q / columns) << shift;
+    const uint32_t x1 = min(x0 + (1u << shift), cam.width);
+    const uint32_t y1 = min(y0 + (1u << shift), cam.height);
+    auto path_of = [&](uint32_t x, uint32_t y) { return ((y >> 3) * tiles_x + (x >> 3)) * 64u + (y & 7u) * 8u + (x & 7u); };
+    const uint32_t first = w.hint_word[path_of(x0, y0)] & 0xFFFFu;
+    uint32_t sun = 0, sun_visible = 0, lamp = 0, lamp_visible = 0;
+    for (uint32_t y = y0; y < y1; ++y)
+    {
+        for (uint32_t x = x0; x < x1; ++x)
+        {
+            const uint32_t path = path_of(x, y);
+            const uint2 c = w.hint_counts[path];
+            sun += c.x & 0xFFFFu;
+            sun_visible += c.x >> 16;
+            lamp += c.y & 0xFFFFu;
+            lamp_visible += c.y >> 16;
+            if (read != nullptr)
+            {
+                read[uint64_t(y) * cam.width + x] = uint16_t(w.hint_word[path] & 0xFFFFu);
+            }
+        }
+    }
+    current[q] = uint16_t(HintChannel(first & 0xFFu, sun, sun_visible) | (HintChannel(first >> 8, lamp, lamp_visible) << 8));
 }
 
 // Zeroes the albedo gradient rows of the workspace.
@@ -1351,7 +1434,8 @@ SkyParams SkyOf(const CxTraceParams* params)
 // scenes of `views` (paths = all of their paths).
 template <class Views>
 void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const CxTraceParams* params, uint32_t mis,
-                 const GradView& g, const Workspace& w, const FrameOut& o, hipStream_t s)
+                 const GradView& g, const Workspace& w, const FrameOut& o, hipStream_t s,
+                 const CameraView* frame_camera = nullptr)
 {
     const bool adjoint = g.d_color != nullptr;
     // The full material model: the scene's extended material table (or a delta chain) asks for it.
@@ -1367,15 +1451,32 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
         const uint64_t n = uint64_t(rows) * kGradEntries;
         ClearRowsKernel<<<uint32_t(DivUp(n, uint64_t(256))), 256, 0, s>>>(w.partials, n);
     }
-    const ShadowSource<Views> shadow_source{w, views, sv, g};
+    const ShadowSource<Views> shadow_source{w, views, sv, g, 0u};
+    // the visibility hint: one scene's frame (frame_camera) only
+    const CxLightHint* hint = frame_camera != nullptr ? params->hint : nullptr;
+    const ShadowSource<Views> primary_shadow_source{w, views, sv, g, hint != nullptr ? 1u : 0u};
     for (uint32_t sample = 0; sample < params->spp; ++sample)
     {
-        const TraceParams p{params->bounces, params->frame_seed + sample, mis,
-                            (params->flags & CX_TRACE_NO_RUSSIAN_ROULETTE) != 0 ? 0u : 1u, SkyOf(params),
-                            V3{params->fog_color[0], params->fog_color[1], params->fog_color[2]}, params->fog_density,
-                            params->bounces_transparent, (params->flags & CX_TRACE_SHADOW_TRANSMISSION) != 0 ? 1u : 0u,
-                            (params->flags & CX_TRACE_DELTA_CHECKERBOARD) != 0 ? 1u : 0u,
-                            params->bounce_nee_probability > 0.0f ? params->bounce_nee_probability : 1.0f};
+        TraceParams p{params->bounces, params->frame_seed + sample, mis,
+                      (params->flags & CX_TRACE_NO_RUSSIAN_ROULETTE) != 0 ? 0u : 1u, SkyOf(params),
+                      V3{params->fog_color[0], params->fog_color[1], params->fog_color[2]}, params->fog_density,
+                      params->bounces_transparent, (params->flags & CX_TRACE_SHADOW_TRANSMISSION) != 0 ? 1u : 0u,
+                      (params->flags & CX_TRACE_DELTA_CHECKERBOARD) != 0 ? 1u : 0u,
+                      params->bounce_nee_probability > 0.0f ? params->bounce_nee_probability : 1.0f, HintParams{}};
+        if (hint != nullptr)
+        {
+            const CameraView& cam = *frame_camera;
+            p.hint.previous = hint->previous;
+            p.hint.shift = hint->patch_shift;
+            p.hint.columns = (cam.width + (1u << hint->patch_shift) - 1u) >> hint->patch_shift;
+            p.hint.reset = hint->reset != 0u ? 1u : 0u;
+            p.hint.has_prev = hint->previous_camera != nullptr ? 1u : 0u;
+            p.hint.sample = sample;
+            if (hint->previous_camera != nullptr)
+            {
+                p.hint.prev = CameraOf(*hint->previous_camera);
+            }
+        }
         ResetKernel<<<1, 64, 0, s>>>(w);
         {
             StageScope scope(s, adjoint ? "adjoint_trace" : "trace_primary");
@@ -1394,8 +1495,8 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
         }
         {
             StageScope scope(s, adjoint ? "adjoint_trace" : "trace_shadow");
-            PersistentTraceKernel<<<shadow_blocks, kWaveThreads, 0, s>>>(sv, shadow_source, o.counters, w, adjoint,
-                                                                         chunk);
+            PersistentTraceKernel<<<shadow_blocks, kWaveThreads, 0, s>>>(sv, primary_shadow_source, o.counters, w,
+                                                                         adjoint, chunk);
         }
         for (uint32_t bounce = 1; bounce <= params->bounces; ++bounce)
         {
@@ -1431,6 +1532,16 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
     if (adjoint)
     {
         ReduceRowsKernel<<<DivUp(kGradEntries, 256u), 256, 0, s>>>(w.partials, rows, g.grad_albedo);
+    }
+    if (hint != nullptr)
+    {
+        const CameraView& cam = *frame_camera;
+        const uint32_t patch = 1u << hint->patch_shift;
+        const uint32_t columns = (cam.width + patch - 1u) >> hint->patch_shift;
+        const uint32_t patch_rows = (cam.height + patch - 1u) >> hint->patch_shift;
+        StageScope scope(s, "trace_hint");
+        HintUpdateKernel<<<DivUp(columns * patch_rows, 256u), 256, 0, s>>>(w, cam, hint->patch_shift, columns, patch_rows,
+                                                                          hint->current, hint->read);
     }
 }
 
@@ -1482,6 +1593,28 @@ int CheckParams(const CxTraceParams* params, const CxTraceOutput* out, uint32_t 
     {
         Log(3, "%s: the adjoint has no sky, sun, fog or delta chain", what);
         return CX_ERROR_UNSUPPORTED;
+    }
+    if (params->hint != nullptr)
+    {
+        const CxLightHint* h = params->hint;
+        if (adjoint)
+        {
+            Log(3, "%s: the adjoint has no visibility hint", what);
+            return CX_ERROR_UNSUPPORTED;
+        }
+        if (h->previous == nullptr || h->current == nullptr || h->patch_shift > 2)
+        {
+            return ArgError("%s: visibility hint without its images, or patch_shift above 2", what);
+        }
+        if (params->spp > 0xFFFFu)
+        {
+            return ArgError("%s: the visibility hint counts at most 65535 samples per pixel", what);
+        }
+        if (h->previous_camera != nullptr &&
+            (h->previous_camera->width != width || h->previous_camera->height != height))
+        {
+            return ArgError("%s: the previous camera's image size differs", what);
+        }
     }
     if ((params->flags & CX_TRACE_SKY) != 0)
     {
@@ -1558,7 +1691,9 @@ int RunFrame(const CxScene* scene, const CxLightTreeView* lights, const CxCamera
     {
         return HipResult(hipErrorInvalidDeviceFunction, "cx_trace_frame: occupancy query");
     }
-    RecordFrame(SceneOf(*scene), OneScene{LightsOf(*lights), CameraOf(*camera)}, paths, params, lights->mis, g, w, o, s);
+    const CameraView frame_camera = CameraOf(*camera);
+    RecordFrame(SceneOf(*scene), OneScene{LightsOf(*lights), frame_camera}, paths, params, lights->mis, g, w, o, s,
+                &frame_camera);
     return LaunchResult(what);
 }
 
@@ -1667,6 +1802,11 @@ CX_API int cx_trace_batch(const CxScene* scene, const void* device_views, uint32
     if (int r = cx::CheckParams(params, out, width, height, false, what); r != CX_OK)
     {
         return r;
+    }
+    if (params->hint != nullptr)
+    {
+        cx::Log(3, "%s: the batch has no visibility hint", what);
+        return CX_ERROR_UNSUPPORTED;
     }
     const uint64_t per_scene = cx::PathCount(width, height);
     if (per_scene * scenes > (uint64_t(1) << 30))

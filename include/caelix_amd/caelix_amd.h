@@ -331,6 +331,8 @@ typedef struct CxCamera {
     uint32_t height;
 } CxCamera;
 
+struct CxLightHint;
+
 typedef struct CxTraceParams {
     uint32_t bounces;    /* bounce_count */
     uint32_t frame_seed;
@@ -365,8 +367,27 @@ typedef struct CxTraceParams {
      * when it does not, its segment still carries the state of a vertex that ran it. 0 or 1: always
      * (no number drawn). In [0, 1]. Not supported by cx_trace_frame_adjoint below 1. */
     float bounce_nee_probability;
-    uint32_t pad3[2];
+    /* The light NEE's visibility hint (below); NULL: off (the choice weights unchanged). */
+    const struct CxLightHint* hint;
 } CxTraceParams;
+
+/* The light NEE's visibility hint per pixel patch (kLightNeeHint; caelix_path_trace.hlsl:229-236,
+ * 338-344, 453-580): the primary vertex multiplies its sun and lamp choice weights by max(0.05, hint)
+ * of its patch in the previous frame, read at the previous pixel of its primary surface (the previous
+ * camera's nearest pixel; its own patch without a surface, a previous camera, or when the point was
+ * behind it or outside its image); after the frame each patch moves its hint by 0.2 toward the fraction
+ * of its primary shadow rays (sun, lamp) that reached their light, and keeps a channel without one.
+ * Only the choice changes: the estimate stays unbiased. Words: sun | lamp << 8, each k / 255. With
+ * spp > 1 every sample reads `previous` and the update pools the samples' shadow rays (the value read
+ * is the patch's first pixel's, sample 0). Not supported by cx_trace_frame_adjoint or the batch. */
+typedef struct CxLightHint {
+    const uint16_t* previous; /* [(height + patch - 1) >> patch_shift][(width + patch - 1) >> patch_shift] */
+    uint16_t* current;        /* the same shape, every patch written; not `previous` */
+    uint16_t* read;           /* nullable: [height][width], the word each primary vertex read (sample 0) */
+    const CxCamera* previous_camera; /* NULL: none; else the same image size as the camera */
+    uint32_t patch_shift;     /* 0, 1 or 2: patches of 1, 2 or 4 pixels square */
+    uint32_t reset;           /* nonzero: no history (kLightNeeHintReset): every vertex reads 0x8080 */
+} CxLightHint;
 
 /* The procedural sky on misses and the sun (caelix_shading.hlsl:364-403, 453-485;
  * caelix_path_trace.hlsl:276-290, 356-413, 811-846): a primary miss shows SkyRadiance (the sky plus

@@ -156,7 +156,7 @@ class _Params(ctypes.Structure):
                 ("sun_solid_angle", ctypes.c_float), ("sun_radiance", _F3), ("sun_cos_radius", ctypes.c_float),
                 ("sky_zenith", _F3), ("pad0", ctypes.c_float), ("sky_horizon", _F3), ("pad1", ctypes.c_float),
                 ("sky_ground", _F3), ("pad2", ctypes.c_float), ("fog_color", _F3), ("fog_density", ctypes.c_float),
-                ("bounces_transparent", _U32), ("bounce_nee_probability", ctypes.c_float), ("pad3", _U32 * 2)]
+                ("bounces_transparent", _U32), ("bounce_nee_probability", ctypes.c_float), ("hint", _P)]
 
 
 NO_RUSSIAN_ROULETTE = 1
@@ -445,6 +445,46 @@ def light_box_support(view: _TreeView, queries: np.ndarray) -> np.ndarray:
     return out
 
 
+class _Hint(ctypes.Structure):
+    _fields_ = [("previous", _P), ("current", _P), ("read", _P), ("previous_camera", ctypes.POINTER(_Camera)),
+                ("patch_shift", _U32), ("reset", _U32)]
+
+
+class HintState:
+    """The visibility hint images of one view on the host (the same rules as the library's LightHint):
+    two word images per patch ping-ponged between frames, the reset until a frame ran, the previous
+    frame's camera for the reprojection."""
+
+    def __init__(self, width: int, height: int, patch: int = 4):
+        self.width, self.height, self.patch = int(width), int(height), int(patch)
+        self.shift = {1: 0, 2: 1, 4: 2}[self.patch]
+        self.columns = (self.width + self.patch - 1) >> self.shift
+        self.rows = (self.height + self.patch - 1) >> self.shift
+        self.images = [np.full(self.rows * self.columns, 0x8080, np.uint16) for _ in range(2)]
+        self.read = np.zeros(self.width * self.height, np.uint16)
+        self.parity = 0
+        self.history = False
+        self.camera = None
+
+    def frame_struct(self):
+        prev = camera_struct(self.camera) if self.camera is not None else None
+        h = _Hint(self.images[self.parity].ctypes.data, self.images[1 - self.parity].ctypes.data, self.read.ctypes.data,
+                  ctypes.pointer(prev) if prev is not None else None, self.shift, 0 if self.history else 1)
+        return h, prev
+
+    def advance(self, camera: dict) -> None:
+        self.parity ^= 1
+        self.history = True
+        self.camera = dict(camera)
+
+    def current(self) -> np.ndarray:
+        w = self.images[self.parity].reshape(self.rows, self.columns)
+        return np.stack([w & 0xFF, w >> 8], axis=-1).astype(np.uint8)
+
+    def read_words(self) -> np.ndarray:
+        return self.read.reshape(self.height, self.width)
+
+
 class OracleScene:
     """A scene held in host arrays for the oracle. `arrays` holds the same named arrays as the GPU
     scene (keys, info, occupancy, payload, slots, hash_info, brick_bits, super_bits, materials,
@@ -589,12 +629,19 @@ class OracleScene:
 
     def trace_frame(self, camera: dict, bounces: int, seed: int, spp: int = 1, rows=None, flags: int = 0,
                     sky: dict | None = None, fog=None, bounces_transparent: int = 0,
-                    bounce_nee_probability: float = 1.0):
+                    bounce_nee_probability: float = 1.0, hint: HintState | None = None):
+        """With `hint`: the rows must hold whole patches (the caller advances the hint after the last
+        rows of a frame)."""
         cam = camera_struct(camera)
         prm = _params(bounces, seed, spp, flags, sky, fog)
         prm.bounces_transparent = int(bounces_transparent)
         prm.bounce_nee_probability = float(bounce_nee_probability)
         r0, r1 = rows if rows is not None else (0, cam.height)
+        keep = None
+        if hint is not None:
+            assert r0 % hint.patch == 0 and (r1 % hint.patch == 0 or r1 == cam.height)
+            keep = hint.frame_struct()
+            prm.hint = ctypes.cast(ctypes.pointer(keep[0]), _P)
         count = (r1 - r0) * cam.width
         color = np.zeros(count * 4, np.float64)
         radiance = np.zeros(count * 4, np.float64)
@@ -603,6 +650,9 @@ class OracleScene:
         _trace_lib().oracle_trace_frame(ctypes.byref(self.scene), ctypes.byref(self.view), ctypes.byref(cam),
                                         ctypes.byref(prm), r0, r1, _ptr(color), _ptr(radiance), _ptr(attrib),
                                         _ptr(counters))
+        del keep
+        if hint is not None and r1 == cam.height:
+            hint.advance(camera)
         return color.reshape(-1, 4), radiance.reshape(-1, 4), attrib, counters
 
 

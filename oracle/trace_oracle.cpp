@@ -1685,9 +1685,24 @@ struct NeeTerm
 
 // NextEventAtVertex (caelix_path_trace.hlsl:356-413): the lamp candidate and, with the sun's NEE,
 // the sun candidate and the choice between them (one-sample RIS), then one shadow ray.
-V NextEvent(const CxScene& s, const Lights& L, const Sky& k, V x_r, V n_r, uint32_t& light_rng,
-            uint32_t& nee_rng, Counters& c, NeeTerm& term, bool transmissive = false, uint32_t medium = 0u)
+// The visibility hint of a primary vertex (kLightNeeHint, caelix_path_trace.hlsl:229-236, 338-344,
+// 453-580): the word read, and which shadow ray the vertex traced (1 sun, 2 lamp) and whether it reached
+// its light.
+struct HintIo
 {
+    const CxLightHint* params = nullptr;  // the frame's hint images (null: no hint)
+    uint32_t word = 0xFFFFu;  // kHintWordOff: the weights unchanged
+    uint32_t traced = 0u;
+    bool visible = false;
+};
+V NextEvent(const CxScene& s, const Lights& L, const Sky& k, V x_r, V n_r, uint32_t& light_rng,
+            uint32_t& nee_rng, Counters& c, NeeTerm& term, bool transmissive = false, uint32_t medium = 0u,
+            HintIo* hint = nullptr)
+{
+    // UnpackHint (:465-468); the choice weights times max(kHintFloor = 0.05, hint)
+    const uint32_t word = hint != nullptr ? hint->word : 0xFFFFu;
+    const float hint_sun = float(word & 0xFFu) * (1.0f / 255.0f);
+    const float hint_lamp = float((word >> 8) & 0xFFu) * (1.0f / 255.0f);
     term = NeeTerm{};
     term.unscaled = Mk(0, 0, 0);
     const uint32_t seed = RandomU32(light_rng);
@@ -1722,14 +1737,14 @@ V NextEvent(const CxScene& s, const Lights& L, const Sky& k, V x_r, V n_r, uint3
             }
         }
     }
-    const float w_lamp = Luminance(contribution) * std::fmax(0.05f, 1.0f);
+    const float w_lamp = Luminance(contribution) * std::fmax(0.05f, hint_lamp);
     const uint32_t sun_start = nee_rng;
     float w_sun = 0.0f;
     float u_choice = 0.0f;
     if (k.sun)
     {
         V sun_direction;
-        w_sun = Luminance(SunCandidate(k, n_r, nee_rng, sun_direction)) * std::fmax(0.05f, 1.0f);
+        w_sun = Luminance(SunCandidate(k, n_r, nee_rng, sun_direction)) * std::fmax(0.05f, hint_sun);
         u_choice = RandomFloat01(nee_rng);
     }
     const float w_sum = w_sun + w_lamp;
@@ -1744,9 +1759,12 @@ V NextEvent(const CxScene& s, const Lights& L, const Sky& k, V x_r, V n_r, uint3
         term.unscaled = Mk(0, 0, 0);
         const int kind = g_kind;
         g_kind = 2;
+        bool reached = false;
         if (transmissive)
         {
-            radiance = MulV(Mul(sun, w_sum / w_sun), ShadowTransmissive(s, x_r, direction, kRayTMax, medium, c));
+            const V tr = ShadowTransmissive(s, x_r, direction, kRayTMax, medium, c);
+            radiance = MulV(Mul(sun, w_sum / w_sun), tr);
+            reached = tr.x > 0.0f || tr.y > 0.0f || tr.z > 0.0f;
         }
         else
         {
@@ -1754,20 +1772,33 @@ V NextEvent(const CxScene& s, const Lights& L, const Sky& k, V x_r, V n_r, uint3
             if (h.attrib == 0u)
             {
                 radiance = Mul(sun, w_sum / w_sun);
+                reached = true;
             }
         }
         g_kind = kind;
+        if (hint != nullptr)
+        {
+            hint->traced = 1u;
+            hint->visible = reached;
+        }
     }
     else if (w_sum > 0.0f)
     {
         c.shadow += 1;
+        if (hint != nullptr)
+        {
+            hint->traced = 2u;
+        }
         if (t_max > 0.0f)
         {
             const int kind = g_kind;
             g_kind = 2;
+            bool reached = false;
             if (transmissive)
             {
-                radiance = MulV(Mul(contribution, w_sum / w_lamp), ShadowTransmissive(s, x_r, direction, t_max, medium, c));
+                const V tr = ShadowTransmissive(s, x_r, direction, t_max, medium, c);
+                radiance = MulV(Mul(contribution, w_sum / w_lamp), tr);
+                reached = tr.x > 0.0f || tr.y > 0.0f || tr.z > 0.0f;
             }
             else
             {
@@ -1776,9 +1807,14 @@ V NextEvent(const CxScene& s, const Lights& L, const Sky& k, V x_r, V n_r, uint3
                 {
                     radiance = Mul(contribution, w_sum / w_lamp);
                     term.visible = true;
+                    reached = true;
                 }
             }
             g_kind = kind;
+            if (hint != nullptr)
+            {
+                hint->visible = reached;
+            }
         }
     }
     return radiance;
@@ -1866,9 +1902,58 @@ void AdjointAlbedoTerms(const CxScene& s, Adjoint& g, const double adj[3], const
 // One pixel sample; radiance accumulated in float64 (the float32 terms are the kernel's). With
 // `adjoint`, also the path-replay gradients of the sample (sampling pdfs, MIS weights and the
 // roulette's normalisation held constant).
+// PreviousPixel (caelix_path_trace.hlsl:154-172): where the previous camera saw the world point p, in
+// pixels of the current image; false when p was not in front of it.
+bool PreviousPixel(const CxCamera& prev, uint32_t width, uint32_t height, V p, float& x, float& y)
+{
+    const V rel = Sub(p, Mk(prev.position[0], prev.position[1], prev.position[2]));
+    const float z = Dot(rel, Mk(prev.forward[0], prev.forward[1], prev.forward[2]));
+    x = y = 0.0f;
+    if (!(z > 1e-6f))
+    {
+        return false;
+    }
+    const float nx = Dot(rel, Mk(prev.right[0], prev.right[1], prev.right[2])) / ((z * prev.tan_half_fov_y) * prev.aspect);
+    const float ny = Dot(rel, Mk(prev.up[0], prev.up[1], prev.up[2])) / (z * prev.tan_half_fov_y);
+    x = ((nx + 1.0f) * 0.5f) * float(width);
+    y = ((1.0f - ny) * 0.5f) * float(height);
+    return std::fabs(x) < 1e30f && std::fabs(y) < 1e30f;
+}
+// ReadLightHint (caelix_path_trace.hlsl:479-498): 0x8080 with the reset; else the previous image at the
+// patch of the previous pixel of the primary surface point (when it has one, there is a previous
+// camera, and the point was in front of it and inside its image), else at the pixel's own patch.
+uint32_t ReadHint(const CxLightHint& h, const CxCamera& cam, uint32_t px, uint32_t py, bool surface, V p)
+{
+    if (h.reset != 0u)
+    {
+        return 0x8080u;
+    }
+    const uint32_t columns = (cam.width + (1u << h.patch_shift) - 1u) >> h.patch_shift;
+    uint32_t qx = px >> h.patch_shift, qy = py >> h.patch_shift;
+    float x, y;
+    if (surface && h.previous_camera != nullptr && PreviousPixel(*h.previous_camera, cam.width, cam.height, p, x, y) &&
+        x >= 0.0f && y >= 0.0f && x < float(cam.width) && y < float(cam.height))
+    {
+        qx = uint32_t(x) >> h.patch_shift;
+        qy = uint32_t(y) >> h.patch_shift;
+    }
+    return h.previous[uint64_t(qy) * columns + qx];
+}
+// One channel of WriteLightHint (:554-580): lerp(read, visible / traced, kHintAlpha = 0.2), rounded to
+// k / 255 as PackHint rounds (:459-463); unchanged without a traced ray.
+uint32_t HintChannel(uint32_t k, uint32_t traced, uint32_t visible)
+{
+    if (traced == 0u)
+    {
+        return k;
+    }
+    const float h = float(k) * (1.0f / 255.0f);
+    const float cur = h + 0.2f * (float(visible) / float(traced) - h);
+    return uint32_t(std::fmin(std::fmax(cur, 0.0f), 1.0f) * 255.0f + 0.5f);
+}
 void TracePixel(const CxScene& s, const Lights& L, const CxCamera& cam, uint32_t bounces, uint32_t seed,
                 uint32_t flags, const Sky& k, uint32_t px, uint32_t py, DV& radiance, V& albedo, V& emission,
-                float& t_hit, uint32_t& attrib, Counters& c, Adjoint* adjoint = nullptr)
+                float& t_hit, uint32_t& attrib, Counters& c, Adjoint* adjoint = nullptr, HintIo* hint = nullptr)
 {
     const bool roulette = (flags & CX_TRACE_NO_RUSSIAN_ROULETTE) == 0u;
     const float sx = float(px) + 0.5f;
@@ -1888,6 +1973,10 @@ void TracePixel(const CxScene& s, const Lights& L, const CxCamera& cam, uint32_t
     g_kind = 0;
     const WorldHit primary = Trace(s, origin, dir, kRayTMax, c);
     g_kind = 3;
+    if (hint != nullptr && hint->params != nullptr)
+    {
+        hint->word = ReadHint(*hint->params, cam, px, py, primary.attrib != 0u, Add(origin, Mul(dir, primary.t)));
+    }
     if (primary.attrib == 0u)
     {
         if (k.on)
@@ -1954,7 +2043,9 @@ void TracePixel(const CxScene& s, const Lights& L, const CxCamera& cam, uint32_t
     if (bounces >= 1u)
     {
         NeeTerm term;
-        AddD(radiance, MulV(NextEvent(s, L, k, ps.origin, primary_normal, light_rng, nee_rng, c, term), throughput));
+        AddD(radiance, MulV(NextEvent(s, L, k, ps.origin, primary_normal, light_rng, nee_rng, c, term, false, 0u,
+                                      hint != nullptr && hint->params != nullptr ? hint : nullptr),
+                            throughput));
         nee_adjoint(term);
         segment_light = 1u | mis_bit;
     }
@@ -2063,7 +2154,7 @@ void TracePixel(const CxScene& s, const Lights& L, const CxCamera& cam, uint32_t
 // its transmission. t_hit = the chain's length, attrib = the last surface or interface it met.
 void TracePixelGlass(const CxScene& s, const Lights& L, const CxCamera& cam, uint32_t bounces, uint32_t seed,
                      uint32_t flags, const Sky& k, uint32_t px, uint32_t py, DV& radiance, V& albedo, V& emission,
-                     float& t_hit, uint32_t& attrib, Counters& c)
+                     float& t_hit, uint32_t& attrib, Counters& c, HintIo* hint = nullptr)
 {
     const bool roulette = (flags & CX_TRACE_NO_RUSSIAN_ROULETTE) == 0u;
     const float sx = float(px) + 0.5f;
@@ -2083,8 +2174,13 @@ void TracePixelGlass(const CxScene& s, const Lights& L, const CxCamera& cam, uin
     g_kind = 0;
     const WorldHit primary = Trace(s, origin, dir, kRayTMax, c);
     g_kind = 3;
+    const bool hinted = hint != nullptr && hint->params != nullptr;
     if (primary.attrib == 0u)
     {
+        if (hinted)
+        {
+            hint->word = ReadHint(*hint->params, cam, px, py, false, origin);
+        }
         if (k.on)
         {
             emission = SkyRadiance(k, dir);
@@ -2146,6 +2242,10 @@ void TracePixelGlass(const CxScene& s, const Lights& L, const CxCamera& cam, uin
     }
     t_hit = t_virtual;
     attrib = geometry;
+    if (hinted)
+    {
+        hint->word = ReadHint(*hint->params, cam, px, py, surface_hit, Add(ro, Mul(rd, surface.t)));
+    }
     if (!surface_hit)
     {
         const V sky = k.on ? SkyRadiance(k, rd) : Mk(0, 0, 0);
@@ -2165,7 +2265,8 @@ void TracePixelGlass(const CxScene& s, const Lights& L, const CxCamera& cam, uin
     if (bounces >= 1u && !ps.specular)
     {
         NeeTerm term;
-        AddD(radiance, MulV(NextEvent(s, L, k, ps.origin, pn, light_rng, nee_rng, c, term, k.transmissive, medium),
+        AddD(radiance, MulV(NextEvent(s, L, k, ps.origin, pn, light_rng, nee_rng, c, term, k.transmissive, medium,
+                                      hinted ? hint : nullptr),
                             throughput));
         segment_light = 1u | mis_bit;
         sees_sun = !k.sun;
@@ -2994,6 +3095,12 @@ void oracle_trace_frame(const CxScene* s, const CxLightTreeView* view, const CxC
     Sky sky = SkyOf(*prm);
     sky.glass = s->materials_ext != nullptr || prm->bounces_transparent > 0u ||
                 (prm->flags & CX_TRACE_SHADOW_TRANSMISSION) != 0u;
+    // The visibility hint: per pixel of the rows, the word read (sample 0) and the primary shadow rays
+    // over the samples (sun traced, sun visible, lamp traced, lamp visible).
+    const CxLightHint* hint = prm->hint;
+    const uint64_t row_pixels = uint64_t(row_end - row_begin) * cam->width;
+    std::vector<uint32_t> hint_words(hint != nullptr ? row_pixels : 0);
+    std::vector<uint32_t> hint_counts(hint != nullptr ? row_pixels * 4 : 0);
 #pragma omp parallel for schedule(dynamic, 1) reduction(+ : rays, shadow, cells, probes, steps, d0, d1, d2, d3, d4)
     for (int64_t py = row_begin; py < int64_t(row_end); ++py)
     {
@@ -3007,19 +3114,35 @@ void oracle_trace_frame(const CxScene* s, const CxLightTreeView* view, const CxC
             DV col, rad;
             float t = 0.0f;
             uint32_t attrib = 0;
+            const uint64_t hi = uint64_t(py - row_begin) * cam->width + px;
             for (uint32_t sample = 0; sample < prm->spp; ++sample)
             {
                 DV r;
                 V albedo, emission;
+                HintIo io;
+                io.params = hint;
                 if (sky.glass)
                 {
                     TracePixelGlass(*s, *L, *cam, prm->bounces, prm->frame_seed + sample, prm->flags, sky, px,
-                                    uint32_t(py), r, albedo, emission, t, attrib, local);
+                                    uint32_t(py), r, albedo, emission, t, attrib, local, &io);
                 }
                 else
                 {
                     TracePixel(*s, *L, *cam, prm->bounces, prm->frame_seed + sample, prm->flags, sky, px, uint32_t(py),
-                               r, albedo, emission, t, attrib, local);
+                               r, albedo, emission, t, attrib, local, nullptr, &io);
+                }
+                if (hint != nullptr)
+                {
+                    if (sample == 0)
+                    {
+                        hint_words[hi] = io.word;
+                    }
+                    const uint32_t base = io.traced == 1u ? 0u : 2u;
+                    if (io.traced != 0u)
+                    {
+                        hint_counts[4 * hi + base] += 1u;
+                        hint_counts[4 * hi + base + 1] += io.visible ? 1u : 0u;
+                    }
                 }
                 col.x += double(emission.x) + double(albedo.x) * r.x;
                 col.y += double(emission.y) + double(albedo.y) * r.y;
@@ -3049,6 +3172,40 @@ void oracle_trace_frame(const CxScene* s, const CxLightTreeView* view, const CxC
         d2 += g_demand[2];
         d3 += g_demand[3];
         d4 += g_demand[4];
+    }
+    if (hint != nullptr)
+    {
+        // WriteLightHint per patch whose rows lie in [row_begin, row_end): the value its first (top-left)
+        // pixel read moved toward the fraction of its pixels' shadow rays that reached their light.
+        const uint32_t shift = hint->patch_shift;
+        const uint32_t columns = (cam->width + (1u << shift) - 1u) >> shift;
+        for (uint32_t qy = (row_begin + (1u << shift) - 1u) >> shift; (qy << shift) < row_end; ++qy)
+        {
+            for (uint32_t qx = 0; qx < columns; ++qx)
+            {
+                uint32_t n[4] = {0, 0, 0, 0};
+                for (uint32_t y = qy << shift; y < std::min(row_end, (qy + 1) << shift); ++y)
+                {
+                    for (uint32_t x = qx << shift; x < std::min(cam->width, (qx + 1) << shift); ++x)
+                    {
+                        for (int k = 0; k < 4; ++k)
+                        {
+                            n[k] += hint_counts[4 * (uint64_t(y - row_begin) * cam->width + x) + k];
+                        }
+                    }
+                }
+                const uint32_t first = hint_words[uint64_t((qy << shift) - row_begin) * cam->width + (qx << shift)];
+                hint->current[uint64_t(qy) * columns + qx] =
+                    uint16_t(HintChannel(first & 0xFFu, n[0], n[1]) | (HintChannel((first >> 8) & 0xFFu, n[2], n[3]) << 8));
+            }
+        }
+        if (hint->read != nullptr)
+        {
+            for (uint64_t i = 0; i < row_pixels; ++i)
+            {
+                hint->read[uint64_t(row_begin) * cam->width + i] = uint16_t(hint_words[i]);
+            }
+        }
     }
     Counters c;
     c.rays = rays;

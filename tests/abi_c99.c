@@ -24,6 +24,9 @@ typedef char entity_info_is_128_bytes[(sizeof(EntityInfo) == 128) ? 1 : -1];
 typedef char light_stats_is_104_bytes[(sizeof(CxLightStats) == 104) ? 1 : -1];
 typedef char light_options_is_16_bytes[(sizeof(CxLightOptions) == 16) ? 1 : -1];
 typedef char tree_view_is_16_bytes[(sizeof(CxLightTreeView) == 16) ? 1 : -1];
+typedef char trace_params_is_128_bytes[(sizeof(CxTraceParams) == 128) ? 1 : -1];
+typedef char trace_params_hint_at_120[(offsetof(CxTraceParams, hint) == 120) ? 1 : -1];
+typedef char light_hint_is_40_bytes[(sizeof(CxLightHint) == 40) ? 1 : -1];
 typedef char hash_slot_is_16_bytes[(sizeof(HashKeySlot) == 16) ? 1 : -1];
 
 size_t cx_abi_c99_touch(void)
