@@ -298,6 +298,11 @@ def lighttree_bench(args, with_cpu: bool) -> dict:
     def build(_):
         builder.build(stream)
 
+    # the sort's digit width and tile, picked for this workload (speed only: the tree is the same)
+    from caelix_amd import autotune
+
+    sort_tuning = autotune.tune_build("lighttree-4m", lambda: build(0), reps=3, use_cache=False)
+
     ms = _time_steps(build, args.steps, args.warmup, stream)
     out = builder.read()
     assert int(out["header"]["leaf_count"]) == n and int(out["state"]["node_overflow"]) == 0
@@ -323,6 +328,8 @@ def lighttree_bench(args, with_cpu: bool) -> dict:
         "pipeline_roofline": {"bytes_per_leaf": BUILD_BYTES_PER_LEAF, "achieved": round(pipeline_gbs, 1),
                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(pipeline_gbs / HBM_PEAK_GBS, 4)},
         "stages": per_stage,
+        "sort_tuning": {"digit_bits": sort_tuning["digit_bits"], "tile": sort_tuning["tile"],
+                        "trials": [{k: t[k] for k in ("digit_bits", "tile", "ms")} for t in sort_tuning["trials"]]},
     }
     # The 64-bit key form of the same rebuild (code << 32 | leaf sorted as one key): the same tree.
     def build64(_):
@@ -475,6 +482,7 @@ def batch_bench(args) -> dict:
     batch_ms = _time_steps(batched, steps, 1, stream)
     from caelix_amd import autotune
 
+    tuned_build = autotune.tune_build("batch256-build", batch.build_trees, reps=3, use_cache=False)
     tuned = autotune.tune("batch256-256x256-b2", lambda: batched(0), reps=2, use_cache=False)
     tuned_ms = _time_steps(batched, steps, 1, stream)
     side = torch.cuda.Stream()
@@ -508,7 +516,10 @@ def batch_bench(args) -> dict:
         "plain_ms": round(plain_ms, 3), "batched_default_ms": round(batch_ms, 3), "batched_ms": round(tuned_ms, 3),
         "graph_ms": round(graph_ms, 3), "plain_graph_ms": round(plain_graph_ms, 3),
         "autotuned": {"queue_chunk": tuned["queue_chunk"], "workgroups_per_cu": tuned["workgroups_per_cu"],
-                      "trials": len(tuned.get("trials", []))},
+                      "trials": len(tuned.get("trials", [])), "digit_bits": tuned_build["digit_bits"],
+                      "tile": tuned_build["tile"],
+                      "build_trials": [{k: t[k] for k in ("digit_bits", "tile", "ms")}
+                                       for t in tuned_build.get("trials", [])]},
         "speedup_batched": round(plain_ms / tuned_ms, 2), "speedup_graph": round(plain_ms / graph_ms, 2),
         "speedup_graph_vs_plain_graph": round(plain_graph_ms / graph_ms, 2),
         "rays_per_step": int(rays), "graph_Mrays_per_s": round(rays / (graph_ms * 1e-3) / 1e6, 1),
@@ -616,7 +627,8 @@ def main() -> int:
         if not args.no_lighttree:
             lt = lighttree_bench(args, with_cpu=not args.no_cpu_baseline)
             result["lighttree"] = {k: lt[k] for k in ("value", "unit", "ms_per_step", "roofline",
-                                                      "pipeline_roofline", "config", "keys64", "cpu_baseline")
+                                                      "pipeline_roofline", "config", "keys64", "cpu_baseline",
+                                                      "stages", "sort_tuning")
                                    if k in lt}
     elif args.workload == "copy":
         result = copy_bench(args)
@@ -641,7 +653,7 @@ def main() -> int:
         "roofline": result["roofline"],
     }
     for key in ("cpu_baseline", "parity", "lighttree", "adjoint", "batch", "pipeline_roofline", "keys64", "stages",
-                "cxw"):
+                "sort_tuning", "cxw"):
         if key in result:
             line[key] = result[key]
     print(json.dumps(line))

@@ -255,6 +255,8 @@ SIGNATURES = {
     "cx_light_bvh_batch_prepare": (_INT, [ctypes.POINTER(CxLightBvhBuild), _U32, _P, _P]),
     "cx_light_bvh_build_batch": (_INT, [_P, _U32, _U32, _P]),
     "cx_light_bvh_build_keys64": (_INT, [ctypes.POINTER(CxLightBvhBuild), _P]),
+    "cx_light_bvh_set_tuning": (_INT, [_P]),
+    "cx_light_bvh_get_tuning": (None, [_P]),
     "cx_radix_sort_state_bytes": (_U64, [_U32]),
     "cx_radix_sort_pairs": (_INT, [_P, _P, _P, _P, _P, _P, _U32, _P, _P]),
     "cx_radix_sort_keys64_state_bytes": (_U64, [_U32]),

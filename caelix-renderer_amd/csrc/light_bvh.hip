@@ -116,6 +116,27 @@ __host__ __device__ inline uint64_t SpillBytes(uint32_t capacity)
     return TreeLevels(capacity) >= 2 ? uint64_t(TreeBlocks(capacity, 2)) * kSpillEntries * kOpenWords * 4 : 0u;
 }
 
+// The sort layouts of the build: the 64-bit key variant and the tunable 32-bit key layouts (digit
+// bits x tile elements, cx_light_bvh_set_tuning).
+using SortMorton6x4096 = SortMorton<6, 8>;
+using SortMorton6x8192 = SortMorton<6, 16>;
+using SortMorton8x4096 = SortMorton<8, 8>;
+using SortMorton8x8192 = SortMorton<8, 16>;
+
+// The sort state region: the largest of the layouts the build may use.
+__host__ __device__ inline uint64_t BuildSortStateBytes(uint32_t capacity)
+{
+    const uint64_t sizes[] = {SortStateBytesOf<SortKeys64>(capacity), SortStateBytesOf<SortMorton6x4096>(capacity),
+                              SortStateBytesOf<SortMorton6x8192>(capacity), SortStateBytesOf<SortMorton8x4096>(capacity),
+                              SortStateBytesOf<SortMorton8x8192>(capacity)};
+    uint64_t m = 0;
+    for (uint64_t v : sizes)
+    {
+        m = v > m ? v : m;
+    }
+    return m;
+}
+
 struct Scratch
 {
     uint8_t* base;
@@ -161,10 +182,7 @@ struct Scratch
     // 24 bytes per leaf of capacity, at a fixed offset (no per-level layout walk).
     __host__ __device__ uint8_t* wide_inputs_base() const
     {
-        const uint64_t state = SortStateBytesOf<SortPairs32>(capacity) > SortStateBytesOf<SortKeys64>(capacity)
-                                   ? SortStateBytesOf<SortPairs32>(capacity)
-                                   : SortStateBytesOf<SortKeys64>(capacity);
-        const uint64_t offset = uint64_t(sort_base() - base) + state;
+        const uint64_t offset = uint64_t(sort_base() - base) + BuildSortStateBytes(capacity);
         return base + ((offset + 255) & ~uint64_t(255));
     }
     __host__ __device__ uint4* quad() const { return reinterpret_cast<uint4*>(wide_inputs_base()); }
@@ -461,7 +479,7 @@ __global__ void __launch_bounds__(kBvhThreads) BoundsKernel(Set set)
 // morton (caelix_light_bvh_morton.hlsl:5-28) + the sort's digit histograms and status clear.
 // One workgroup per 8192 leaves. Every workgroup first reduces the bounds partials (the second half
 // of caelix_light_bvh_bounds); workgroup 0 writes leaf_count, leaves_dropped and bounds. The keys:
-// (code, leaf) pairs in sort buffer 0 (SortPairs32), or code << 32 | leaf in sort buffer 1
+// (code, leaf) pairs in sort buffer passes % 2 (SortMorton), or code << 32 | leaf in sort buffer 1
 // (SortKeys64, whose tiles are half as long: two status rows per workgroup).
 template <class Set, class Keys>
 __global__ void __launch_bounds__(kSortThreads) MortonKernel(Set set)
@@ -530,8 +548,9 @@ __global__ void __launch_bounds__(kSortThreads) MortonKernel(Set set)
     const float e1 = fmaxf(hi[1] - lo1, 1e-6f);
     const float e2 = fmaxf(hi[2] - lo2, 1e-6f);
     const float* centres = b.scratch.centres();
-    uint32_t* keys = b.scratch.keys(0);
-    uint32_t* values = b.scratch.values(0);
+    // the pairs start in the buffer that an even number of passes leaves them in buffer 0 from
+    uint32_t* keys = b.scratch.keys(uint32_t(Keys::kPasses) & 1u);
+    uint32_t* values = b.scratch.values(uint32_t(Keys::kPasses) & 1u);
     uint32_t* counter = b.scratch.u32(kLtScratchCounter);
     int32_t* diff = b.scratch.cover_diff();
     const uint32_t base = blockIdx.x * kSortTile;
@@ -1791,17 +1810,19 @@ __global__ void __launch_bounds__(kWideThreads) WideKernel(Set set)
 
 // caelix_light_bvh_begin.hlsl:9-17
 // One onesweep pass of the build's sort (keys / values ping-pong in the scratch).
-template <class Set>
-__global__ void __launch_bounds__(kSortThreads, kSortWavesPerSimd) BuildSortPassKernel(Set set, uint32_t pass)
+template <class Set, class L>
+__global__ void __launch_bounds__(L::kThreads, kSortWavesPerSimd) BuildSortPassKernel(Set set, uint32_t pass)
 {
     const Build& b = set.Get();
-    if (blockIdx.x >= SortTiles(b.leaf_capacity))
+    if (blockIdx.x >= SortTilesOf<L>(b.leaf_capacity))
     {
         return;
     }
-    const uint32_t in = pass & 1u;
-    SortPassTile(b.scratch.keys(in), b.scratch.values(in), b.scratch.keys(1 - in), b.scratch.values(1 - in),
-                 &b.header->leaf_count, &b.state->building, b.scratch.sort(), pass);
+    // pass p reads buffer (passes + p) % 2: the last pass writes buffer 0
+    const uint32_t in = (uint32_t(L::kPasses) + pass) & 1u;
+    const SortSink<L> out{b.scratch.keys(1 - in), b.scratch.values(1 - in), nullptr, nullptr};
+    SortPassTileOf<L>(b.scratch.keys(in), b.scratch.values(in), out, &b.header->leaf_count, &b.state->building,
+                      SortStateOf<L>(b.scratch.sort_base(), b.leaf_capacity), pass);
 }
 
 // One pass of the 64-bit key variant's sort: the keys (code << 32 | leaf) start in sort buffer 1,
@@ -1825,9 +1846,9 @@ __global__ void __launch_bounds__(SortKeys64::kThreads, kSortWavesPerSimd) Build
 }
 
 // The build sequence (light_tree_record.cpp:271-308) for a set of builds; `capacity` = the largest
-// leaf capacity of the set, `count` = the number of builds (grid y). Keys = SortPairs32 (the
+// leaf capacity of the set, `count` = the number of builds (grid y). Keys = a SortMorton layout (the
 // reference's uint32 Morton keys with leaf values) or SortKeys64 (code << 32 | leaf, sorted as one
-// 64-bit key); both give the same order, hence the same tree.
+// 64-bit key); all give the same order, hence the same tree.
 template <class Keys, class Set>
 void RecordBuild(const Set& set, uint32_t capacity, uint32_t count, hipStream_t s, bool timed = true)
 {
@@ -1842,10 +1863,10 @@ void RecordBuild(const Set& set, uint32_t capacity, uint32_t count, hipStream_t 
     }
     if constexpr (Keys::kValues)
     {
-        for (uint32_t pass = 0; pass < kSortPasses; ++pass)
+        for (uint32_t pass = 0; pass < uint32_t(Keys::kPasses); ++pass)
         {
             StageScope scope(s, "sort_pass", timed);
-            BuildSortPassKernel<<<dim3(tiles, count), kSortThreads, 0, s>>>(set, pass);
+            BuildSortPassKernel<Set, Keys><<<dim3(SortTilesOf<Keys>(capacity), count), Keys::kThreads, 0, s>>>(set, pass);
         }
     }
     else
@@ -1880,6 +1901,27 @@ void RecordBuild(const Set& set, uint32_t capacity, uint32_t count, hipStream_t 
     {
         StageScope scope(s, "lt_wide", timed);
         WideKernel<<<dim3(DivUp(capacity, kWideThreads), count), kWideThreads, 0, s>>>(set);
+    }
+}
+
+// The tunables of the 32-bit key build's sort (cx_light_bvh_set_tuning): digit bits 6 or 8, tiles of
+// 4096 or 8192 elements. They change speed only: every layout sorts stably, so the order and the tree
+// are the same.
+CxLightBvhTuning g_bvh_tuning{8u, 8192u};
+
+template <class Set>
+void RecordBuild32(const Set& set, uint32_t capacity, uint32_t count, hipStream_t s, bool timed = true)
+{
+    const bool small = g_bvh_tuning.tile == 4096u;
+    if (g_bvh_tuning.digit_bits == 6u)
+    {
+        small ? RecordBuild<SortMorton6x4096>(set, capacity, count, s, timed)
+              : RecordBuild<SortMorton6x8192>(set, capacity, count, s, timed);
+    }
+    else
+    {
+        small ? RecordBuild<SortMorton8x4096>(set, capacity, count, s, timed)
+              : RecordBuild<SortMorton8x8192>(set, capacity, count, s, timed);
     }
 }
 
@@ -1969,6 +2011,25 @@ int CheckBuild(const CxLightBvhBuild* build, const char* what, Build& b)
 
 extern "C" {
 
+CX_API int cx_light_bvh_set_tuning(const CxLightBvhTuning* tuning)
+{
+    if (tuning == nullptr || (tuning->digit_bits != 6u && tuning->digit_bits != 8u) ||
+        (tuning->tile != 4096u && tuning->tile != 8192u))
+    {
+        return cx::ArgError("cx_light_bvh_set_tuning: digit bits must be 6 or 8 and the tile 4096 or 8192");
+    }
+    cx::g_bvh_tuning = *tuning;
+    return CX_OK;
+}
+
+CX_API void cx_light_bvh_get_tuning(CxLightBvhTuning* tuning)
+{
+    if (tuning != nullptr)
+    {
+        *tuning = cx::g_bvh_tuning;
+    }
+}
+
 CX_API uint64_t cx_light_bvh_scratch_bytes(uint32_t leaf_capacity)
 {
     return cx::Scratch::Bytes(leaf_capacity);
@@ -2004,7 +2065,7 @@ CX_API int cx_light_bvh_build(const CxLightBvhBuild* build, void* stream)
         return r;
     }
     const hipStream_t s = static_cast<hipStream_t>(stream);
-    cx::RecordBuild<cx::SortPairs32>(cx::OneBuild{b}, build->leaf_capacity, 1, s);
+    cx::RecordBuild32(cx::OneBuild{b}, build->leaf_capacity, 1, s);
     return cx::LaunchResult("cx_light_bvh_build");
 }
 
@@ -2049,7 +2110,7 @@ CX_API int cx_light_bvh_build_static(const CxLightBvhBuild* build, void* stream)
     // The control words sit at an offset that depends on the capacity, over whatever the previous call
     // left there: clear them with a kernel (no memset node if the caller captures the stream).
     cx::ScratchInitKernel<<<1, 64, 0, s>>>(b.scratch.ctl());
-    cx::RecordBuild<cx::SortPairs32>(cx::OneBuild{b}, build->leaf_capacity, 1, s, false);
+    cx::RecordBuild32(cx::OneBuild{b}, build->leaf_capacity, 1, s, false);
     return cx::LaunchResult(what);
 }
 
@@ -2088,7 +2149,7 @@ CX_API int cx_light_bvh_build_batch(const void* descriptors, uint32_t count, uin
         return cx::ArgError("cx_light_bvh_build_batch: largest leaf capacity %u invalid", max_leaf_capacity);
     }
     const hipStream_t s = static_cast<hipStream_t>(stream);
-    cx::RecordBuild<cx::SortPairs32>(cx::BuildArray{static_cast<const cx::Build*>(descriptors)}, max_leaf_capacity,
+    cx::RecordBuild32(cx::BuildArray{static_cast<const cx::Build*>(descriptors)}, max_leaf_capacity,
                                      count, s);
     return cx::LaunchResult("cx_light_bvh_build_batch");
 }

@@ -32,6 +32,21 @@ struct SortKeys64
     static constexpr bool kValues = false;
 };
 
+// The light BVH build's sorts of 30-bit Morton keys with leaf values: kBits-bit digits, as many
+// passes as cover bits 0..29, tiles of 512 x kItemsPerLane elements. The build picks one at run time
+// (cx_light_bvh_set_tuning); every layout gives the same stable order.
+template <int kBits, int kItemsPerLane>
+struct SortMorton
+{
+    using Key = uint32_t;
+    static constexpr int kDigitBits = kBits;
+    static constexpr int kRadix = 1 << kBits;
+    static constexpr int kPasses = (30 + kBits - 1) / kBits;
+    static constexpr int kThreads = 512;
+    static constexpr int kItems = kItemsPerLane;
+    static constexpr bool kValues = true;
+};
+
 // The (uint32 key, uint32 value) layout under its historical names.
 constexpr int kSortThreads = SortPairs32::kThreads;
 constexpr int kSortItems = SortPairs32::kItems;

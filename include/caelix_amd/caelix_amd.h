@@ -106,6 +106,19 @@ CX_API uint64_t cx_light_bvh_batch_bytes(uint32_t count);
 CX_API int cx_light_bvh_batch_prepare(const CxLightBvhBuild* builds, uint32_t count, void* descriptors, void* stream);
 CX_API int cx_light_bvh_build_batch(const void* descriptors, uint32_t count, uint32_t max_leaf_capacity, void* stream);
 
+/* Tunables of the 32-bit key builds' sort (cx_light_bvh_build, _static, _batch and the light tree
+ * adapter; process-wide, read when a build is recorded): digit_bits = 6 (5 passes over the 30-bit
+ * Morton codes) or 8 (4 passes, the default), tile = 4096 or 8192 (the default) elements per
+ * workgroup. The order and the tree are the same for every setting (the passes are stable); only the
+ * speed changes. The Python package's autotuner picks them per workload. No counterpart in the
+ * reference (its sort is fixed). */
+typedef struct CxLightBvhTuning {
+    uint32_t digit_bits;
+    uint32_t tile;
+} CxLightBvhTuning;
+CX_API int cx_light_bvh_set_tuning(const CxLightBvhTuning* tuning);
+CX_API void cx_light_bvh_get_tuning(CxLightBvhTuning* tuning);
+
 /* The build with 64-bit sort keys: code << 32 | leaf index, sorted as one key by
  * cx_radix_sort_keys64's passes (7 x 9 bits) in place of the (code, leaf) pair sort. The order, and
  * so every output (tree, nodes, leaves, header, scratch arrays), is the one cx_light_bvh_build makes.

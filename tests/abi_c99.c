@@ -27,6 +27,7 @@ typedef char tree_view_is_16_bytes[(sizeof(CxLightTreeView) == 16) ? 1 : -1];
 typedef char trace_params_is_128_bytes[(sizeof(CxTraceParams) == 128) ? 1 : -1];
 typedef char trace_params_hint_at_120[(offsetof(CxTraceParams, hint) == 120) ? 1 : -1];
 typedef char light_hint_is_40_bytes[(sizeof(CxLightHint) == 40) ? 1 : -1];
+typedef char bvh_tuning_is_8_bytes[(sizeof(CxLightBvhTuning) == 8) ? 1 : -1];
 typedef char hash_slot_is_16_bytes[(sizeof(HashKeySlot) == 16) ? 1 : -1];
 
 size_t cx_abi_c99_touch(void)
@@ -60,8 +61,10 @@ size_t cx_abi_c99_touch(void)
         (AnyFn)cx_light_bvh_build_batch,
         (AnyFn)cx_light_bvh_build_keys64,
         (AnyFn)cx_light_bvh_build_static,
+        (AnyFn)cx_light_bvh_get_tuning,
         (AnyFn)cx_light_bvh_scratch_bytes,
         (AnyFn)cx_light_bvh_scratch_init,
+        (AnyFn)cx_light_bvh_set_tuning,
         (AnyFn)cx_light_box_support,
         (AnyFn)cx_light_brick_pdf,
         (AnyFn)cx_light_gather,

@@ -297,3 +297,66 @@ def test_degenerate_inputs_at_rebuild_size_match_oracle(cx, oracle, kind):
     gpu, ref = gpu_and_oracle(cx, oracle, leaves, capacity=n)
     compare(gpu, ref, n, n)
     assert int(gpu["header"]["leaf_count"]) == n and int(gpu["header"]["root"]) == 0
+
+
+@pytest.mark.parametrize("bits,tile", [(6, 4096), (6, 8192), (8, 4096)])
+def test_sort_tuning_gives_the_same_tree(cx, oracle, bits, tile):
+    """Every digit width and tile of the build's sort (cx_light_bvh_set_tuning) sorts stably, so the
+    trees are the oracle's: small and odd sizes, a batch of uneven trees, and the 4,194,304-leaf rebuild
+    equal to the default setting's bytes."""
+    from caelix_amd import autotune
+
+    default = autotune.get_build_tuning()
+    assert default == {"digit_bits": 8, "tile": 8192}
+    try:
+        autotune.set_build_tuning(bits, tile)
+        print(f"sort tuning {bits} bits, tile {tile}", flush=True)
+        for n in (2, 4097, 70000):
+            leaves = synthetic.light_leaves(n, 900 + n, extent=4096.0, clusters=8, sigma=16.0)
+            capacity = (n + 63) // 64 * 64
+            gpu, ref = gpu_and_oracle(cx, oracle, leaves, capacity)
+            compare(gpu, ref, capacity, n)
+        sizes = [1, 64, 4095, 9000, 300]
+        builders, leaves_of = [], []
+        for k, n in enumerate(sizes):
+            leaves = synthetic.light_leaves(n, 700 + k, extent=1024.0, clusters=4, sigma=20.0)
+            capacity = max(64, (n + 63) // 64 * 64)
+            b = cx.LightBvhBuilder(capacity)
+            b.upload(leaves)
+            builders.append(b)
+            leaves_of.append((leaves, capacity))
+        cx.LightBvhBatch(builders).build()
+        for b, (leaves, capacity) in zip(builders, leaves_of):
+            state = np.zeros(1, layout.STATE)
+            state["building"] = 1
+            state["leaves_found"] = len(leaves)
+            ref = oracle.light_bvh_build(leaves.view(np.uint8), capacity, layout.node_capacity_for(capacity),
+                                         state.view(np.uint8))
+            compare(b.read(), ref, capacity, len(leaves))
+        n = 1 << 22
+        leaves = synthetic.light_leaves(n, 2024, extent=8192.0, clusters=64, sigma=32.0)
+        outs = []
+        for setting in ((bits, tile), (8, 8192)):
+            autotune.set_build_tuning(*setting)
+            builder = cx.LightBvhBuilder(n)
+            builder.upload(leaves)
+            builder.build()
+            outs.append(builder.read())
+        a, b = outs
+        assert a["header"].tobytes() == b["header"].tobytes() and a["leaves"].tobytes() == b["leaves"].tobytes()
+        count = int(a["header"]["node_count"])
+        assert a["nodes"][:count].tobytes() == b["nodes"][:count].tobytes()
+        for offset in (layout.SCRATCH_KEYS, layout.SCRATCH_VALUES, layout.SCRATCH_LEFT, layout.SCRATCH_WIDE):
+            assert np.array_equal(layout.scratch_u32(a["scratch"], offset, n)[: n - 1],
+                                  layout.scratch_u32(b["scratch"], offset, n)[: n - 1])
+    finally:
+        autotune.set_build_tuning(default["digit_bits"], default["tile"])
+
+
+def test_sort_tuning_rejects_other_settings(cx):
+    from caelix_amd import autotune
+
+    for bits, tile in ((7, 8192), (8, 2048), (10, 8192)):
+        with pytest.raises(cx.CxError):
+            autotune.set_build_tuning(bits, tile)
+    assert autotune.get_build_tuning() == {"digit_bits": 8, "tile": 8192}
