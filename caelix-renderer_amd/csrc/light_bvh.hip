@@ -1682,20 +1682,21 @@ __global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Set set)
 }
 
 // ------------------------------------------------------------------------------------------------
-// wide (caelix_light_bvh_wide.hlsl:10-100), one lane per binary node as in the reference: the
-// odd-depth lanes stop after the first round of loads. Two dependent rounds: the node's depth, 4-wide
-// index and slot list (written by the tree build), all coalesced; then per slot the leaf record, or
-// the 4-wide index and binary aggregate. The 4-wide nodes of a workgroup's binary nodes are
-// consecutive (4-wide index = exclusive scan of the even-depth flags); the first kWideStaged of them
-// are staged in LDS and written out with coalesced stores, any beyond directly. The parent words of a
-// node (bytes 208..215) belong to the lane that builds its parent, so a record write covers bytes
-// 0..207.
-constexpr uint32_t kWideThreads = 256;
-constexpr uint32_t kWideStaged = 192;
-constexpr uint32_t kWideRecordChunks = 13;  // 16-byte chunks of bytes 0..207
+// wide (caelix_light_bvh_wide.hlsl:10-100), chunked: a workgroup owns kWideChunk consecutive binary nodes, whose 4-wide nodes are the
+// consecutive indices [wide[c0], wide[c1]) (the exclusive scan of the even-depth flags). It stages the
+// chunk's slot lists, 4-wide indices and binary parents in LDS (coalesced), then builds each of its
+// 4-wide nodes in one lane: the slot summaries (a leaf's record, or the binary aggregate of an
+// even-depth grandchild) and child words, and the node's own parent word (its binary grandparent's
+// 4-wide index and the slot that holds it there), so every 224-byte record is written whole by the
+// lane that owns it and no lane writes into another chunk's records. Lookups outside the chunk go to
+// global memory. Each wave assembles its 64 records in its LDS slice and writes them out as one
+// contiguous run. Leaf parents are scattered into the leaf records. 4-wide nodes past the node capacity
+// are not built (the overflow leaves the tree empty).
+constexpr uint32_t kWideChunk = 512;
+constexpr uint32_t kWideChunkThreads = 256;
 
 template <class Set>
-__global__ void __launch_bounds__(kWideThreads) WideKernel(Set set)
+__global__ void __launch_bounds__(kWideChunkThreads) WideChunkKernel(Set set)
 {
     const Build& b = set.Get();
     if (!Building(b))
@@ -1704,109 +1705,119 @@ __global__ void __launch_bounds__(kWideThreads) WideKernel(Set set)
     }
     const uint32_t* ctl = b.scratch.ctl();
     const uint32_t n = ctl[kCtlLeafCount];
-    const uint32_t total = min(ctl[kCtlWideNodes], b.node_capacity);
-    const uint32_t i0 = blockIdx.x * kWideThreads;
-    if (total == 0 || i0 + 1 >= n)
+    const uint32_t total = ctl[kCtlWideNodes];
+    const uint32_t c0 = blockIdx.x * kWideChunk;
+    if (total == 0 || uint64_t(c0) + 1 >= n)
     {
         return;
     }
-    __shared__ float4 lds_records[kWideStaged * kWideRecordChunks];
-    __shared__ uint32_t lds_end;
-    const uint32_t t = threadIdx.x;
-    const uint32_t i = i0 + t;
-    const uint32_t at = i + 1 < n ? i : i0;
+    const uint32_t c1 = min(c0 + kWideChunk, n - 1);  // binary nodes [c0, c1)
+    __shared__ uint4 lds_quad[kWideChunk];
+    __shared__ uint32_t lds_wide[kWideChunk];
+    __shared__ uint32_t lds_parent[kWideChunk];
+    __shared__ uint32_t lds_node[kWideChunk];  // binary node of each 4-wide node of the chunk
+    __shared__ float4 lds_rec[kWideChunkThreads / kWave][kWave * 14];  // per wave: 64 records of 14 chunks
     const uint32_t* wide = b.scratch.u32(kLtScratchWide);
-    const uint32_t depth = b.scratch.u32(kLtScratchCounter)[at];
-    const uint32_t w = wide[at];
-    const uint32_t w0 = wide[i0];
-    const uint4 q = b.scratch.quad()[at];
-    if (t == 0)
+    const uint32_t* depth = b.scratch.u32(kLtScratchCounter);
+    const uint32_t* parent = b.scratch.u32(kLtScratchParent);
+    const uint4* quad = b.scratch.quad();
+    const uint32_t w0 = wide[c0];
+    for (uint32_t k = threadIdx.x; k < c1 - c0; k += kWideChunkThreads)
     {
-        lds_end = w0;
-    }
-    __syncthreads();
-    const bool even = i + 1 < n && (depth & 1u) == 0 && w < total;
-    if (even)
-    {
-        atomicMax(&lds_end, w + 1);
-        const uint32_t slot_word[4] = {q.x, q.y, q.z, q.w};
-        // All loads first: per slot the 4-wide index (internal) and the summary.
-        uint32_t child_word[4];
-        Summary summary[4];
-#pragma unroll
-        for (uint32_t s = 0; s < 4; ++s)
+        const uint32_t i = c0 + k;
+        const uint32_t wi = wide[i];
+        lds_quad[k] = quad[i];
+        lds_wide[k] = wi;
+        lds_parent[k] = parent[i];
+        if ((depth[i] & 1u) == 0)
         {
-            const uint32_t word = slot_word[s];
-            const bool used = word != kLtNone;
-            const bool leaf = used && (word & kLtChildLeafBit) != 0;
-            child_word[s] = used && !leaf ? wide[word] : word;
-            const float* p = leaf ? b.leaves[word & ~kLtChildLeafBit].summary.face
-                                  : b.scratch.aggregates() + (used ? uint64_t(word) * 12 : 0u);
-            summary[s] = LoadSummary(p);
-            if (!used)
-            {
-                summary[s] = EmptySummary();
-            }
-        }
-        const uint32_t r = w - w0;
-        float staged[kWideRecordChunks * 4];
-#pragma unroll
-        for (uint32_t s = 0; s < 4; ++s)
-        {
-#pragma unroll
-            for (int f = 0; f < 6; ++f)
-            {
-                staged[6 * s + f] = summary[s].face[f];
-            }
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-            {
-                staged[24 + 3 * s + k] = summary[s].lo[k];
-                staged[36 + 3 * s + k] = summary[s].hi[k];
-            }
-            staged[48 + s] = __uint_as_float(child_word[s]);
-        }
-        float4* dst = r < kWideStaged ? lds_records + r * kWideRecordChunks : reinterpret_cast<float4*>(b.nodes + w);
-#pragma unroll
-        for (uint32_t c = 0; c < kWideRecordChunks; ++c)
-        {
-            dst[c] = make_float4(staged[4 * c], staged[4 * c + 1], staged[4 * c + 2], staged[4 * c + 3]);
-        }
-#pragma unroll
-        for (uint32_t s = 0; s < 4; ++s)
-        {
-            const uint32_t word = child_word[s];
-            if (word == kLtNone)
-            {
-                continue;
-            }
-            if ((word & kLtChildLeafBit) != 0)
-            {
-                b.leaves[word & ~kLtChildLeafBit].parent = (w << kLtParentSlotBits) | s;
-            }
-            else if (word < b.node_capacity)
-            {
-                // parent, parent_slot and the two pad words (zero: the node's bytes do not depend on what
-                // the buffer held)
-                *reinterpret_cast<uint4*>(&b.nodes[word].parent) = make_uint4(w, s, 0u, 0u);
-            }
-        }
-        if (w == 0)
-        {
-            *reinterpret_cast<uint4*>(&b.nodes[0].parent) = make_uint4(kLtNone, 0u, 0u, 0u);  // the root
+            lds_node[wi - w0] = i;
         }
     }
+    const uint32_t w1 = c1 < n - 1 ? wide[c1] : total;
     __syncthreads();
-    const uint32_t records = min(lds_end - w0, kWideStaged);
-    uint8_t* out = reinterpret_cast<uint8_t*>(b.nodes + w0);
-    for (uint32_t c = t; c < records * kWideRecordChunks; c += kWideThreads)
+    auto in_chunk = [&](uint32_t node) { return node >= c0 && node < c1; };
+    auto wide_of = [&](uint32_t node) { return in_chunk(node) ? lds_wide[node - c0] : wide[node]; };
+    auto quad_of = [&](uint32_t node) { return in_chunk(node) ? lds_quad[node - c0] : quad[node]; };
+    auto parent_of = [&](uint32_t node) { return in_chunk(node) ? lds_parent[node - c0] : parent[node]; };
+    const uint32_t built = min(w1, b.node_capacity);
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t lane = threadIdx.x % kWave;
+    for (uint32_t base = w0; base < built; base += kWideChunkThreads)  // uniform: every thread runs each round
     {
-        const uint32_t rec = c / kWideRecordChunks;
-        const uint32_t part = c - rec * kWideRecordChunks;
-        *reinterpret_cast<float4*>(out + uint64_t(rec) * sizeof(LightBvhNode) + part * 16) = lds_records[c];
+        const uint32_t wave_first = base + wave * kWave;
+        const uint32_t w = wave_first + lane;
+        if (w < built)
+        {
+            const uint32_t j = lds_node[w - w0];
+            const uint4 q = lds_quad[j - c0];
+            const uint32_t slot_word[4] = {q.x, q.y, q.z, q.w};
+            // All loads first: per slot the 4-wide index (internal) and the summary; then the parent word.
+            uint32_t child_word[4];
+            Summary summary[4];
+#pragma unroll
+            for (uint32_t s = 0; s < 4; ++s)
+            {
+                const uint32_t word = slot_word[s];
+                const bool used = word != kLtNone;
+                const bool leaf = used && (word & kLtChildLeafBit) != 0;
+                child_word[s] = used && !leaf ? wide_of(word) : word;
+                const float* src = leaf ? b.leaves[word & ~kLtChildLeafBit].summary.face
+                                        : b.scratch.aggregates() + (used ? uint64_t(word) * 12 : 0u);
+                summary[s] = used ? LoadSummary(src) : EmptySummary();
+            }
+            uint32_t up = kLtNone;
+            uint32_t up_slot = 0;
+            if (j != 0)
+            {
+                const uint32_t g = parent_of(parent_of(j));
+                up = wide_of(g);
+                const uint4 gq = quad_of(g);
+                up_slot = gq.x == j ? 0u : gq.y == j ? 1u : gq.z == j ? 2u : 3u;
+            }
+            float4* dst = &lds_rec[wave][lane * 14];
+#pragma unroll
+            for (uint32_t s = 0; s < 4; s += 2)
+            {
+                // child_face[s], child_face[s + 1]: 12 floats = 3 chunks
+                const Summary& x = summary[s];
+                const Summary& y = summary[s + 1];
+                dst[3 * (s / 2) + 0] = make_float4(x.face[0], x.face[1], x.face[2], x.face[3]);
+                dst[3 * (s / 2) + 1] = make_float4(x.face[4], x.face[5], y.face[0], y.face[1]);
+                dst[3 * (s / 2) + 2] = make_float4(y.face[2], y.face[3], y.face[4], y.face[5]);
+            }
+            // child_lo[4][3] (chunks 6..8), child_hi[4][3] (chunks 9..11)
+            dst[6] = make_float4(summary[0].lo[0], summary[0].lo[1], summary[0].lo[2], summary[1].lo[0]);
+            dst[7] = make_float4(summary[1].lo[1], summary[1].lo[2], summary[2].lo[0], summary[2].lo[1]);
+            dst[8] = make_float4(summary[2].lo[2], summary[3].lo[0], summary[3].lo[1], summary[3].lo[2]);
+            dst[9] = make_float4(summary[0].hi[0], summary[0].hi[1], summary[0].hi[2], summary[1].hi[0]);
+            dst[10] = make_float4(summary[1].hi[1], summary[1].hi[2], summary[2].hi[0], summary[2].hi[1]);
+            dst[11] = make_float4(summary[2].hi[2], summary[3].hi[0], summary[3].hi[1], summary[3].hi[2]);
+            reinterpret_cast<uint4*>(dst)[12] = make_uint4(child_word[0], child_word[1], child_word[2], child_word[3]);
+            reinterpret_cast<uint4*>(dst)[13] = make_uint4(up, up_slot, 0u, 0u);
+#pragma unroll
+            for (uint32_t s = 0; s < 4; ++s)
+            {
+                const uint32_t word = child_word[s];
+                if (word != kLtNone && (word & kLtChildLeafBit) != 0)
+                {
+                    b.leaves[word & ~kLtChildLeafBit].parent = (w << kLtParentSlotBits) | s;
+                }
+            }
+        }
+        __syncthreads();
+        if (wave_first < built)
+        {
+            const uint32_t records = min(kWave, built - wave_first);
+            float4* out = reinterpret_cast<float4*>(b.nodes + wave_first);
+            for (uint32_t c = lane; c < records * 14; c += kWave)
+            {
+                out[c] = lds_rec[wave][c];
+            }
+        }
+        __syncthreads();
     }
 }
-
 
 // caelix_light_bvh_begin.hlsl:9-17
 // One onesweep pass of the build's sort (keys / values ping-pong in the scratch).
@@ -1900,7 +1911,7 @@ void RecordBuild(const Set& set, uint32_t capacity, uint32_t count, hipStream_t 
     }
     {
         StageScope scope(s, "lt_wide", timed);
-        WideKernel<<<dim3(DivUp(capacity, kWideThreads), count), kWideThreads, 0, s>>>(set);
+        WideChunkKernel<<<dim3(DivUp(capacity, kWideChunk), count), kWideChunkThreads, 0, s>>>(set);
     }
 }
 
