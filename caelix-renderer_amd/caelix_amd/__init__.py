@@ -122,7 +122,11 @@ class CxTraceParams(ctypes.Structure):
                 ("pad1", ctypes.c_float), ("sky_ground", ctypes.c_float * 3), ("pad2", ctypes.c_float),
                 ("fog_color", ctypes.c_float * 3), ("fog_density", ctypes.c_float),
                 ("bounces_transparent", ctypes.c_uint32), ("bounce_nee_probability", ctypes.c_float),
-                ("hint", ctypes.c_void_p)]
+                ("hint", ctypes.c_void_p), ("sky_cubemap", ctypes.c_void_p)]
+
+
+class CxSkyCubemap(ctypes.Structure):
+    _fields_ = [("texels", ctypes.c_void_p), ("size", ctypes.c_uint32), ("mip_levels", ctypes.c_uint32)]
 
 
 class CxLightHint(ctypes.Structure):
@@ -131,7 +135,38 @@ class CxLightHint(ctypes.Structure):
                 ("reset", ctypes.c_uint32)]
 
 
-assert ctypes.sizeof(CxTraceParams) == 128 and ctypes.sizeof(CxLightHint) == 40
+assert ctypes.sizeof(CxTraceParams) == 136 and ctypes.sizeof(CxLightHint) == 40 and ctypes.sizeof(CxSkyCubemap) == 16
+
+
+def cube_texel_direction(face: int, s, t):
+    """The direction of face coordinates (s, t) in [-1, 1] (the reference's CubeTexelDirection): +X
+    (1, -t, -s), -X (-1, -t, s), +Y (s, 1, t), -Y (s, -1, -t), +Z (s, -t, 1), -Z (-s, -t, -1)."""
+    one = np.ones_like(np.asarray(s, np.float64))
+    return [np.stack([one, -t, -s], -1), np.stack([-one, -t, s], -1), np.stack([s, one, t], -1),
+            np.stack([s, -one, -t], -1), np.stack([s, -t, one], -1), np.stack([-s, -t, -one], -1)][face]
+
+
+def make_sky_cube(size: int, radiance, mip_levels: int = 1) -> np.ndarray:
+    """Cubemap texels in the CxSkyCubemap layout ([texels, 4] float32): mip 0 = radiance(unit direction of
+    each texel centre) (a callable on [..., 3] arrays returning [..., 3]), each further mip the 2 x 2
+    average of the one above."""
+    faces = []
+    c = (np.arange(size) + 0.5) / size * 2.0 - 1.0
+    s, t = np.meshgrid(c, c, indexing="xy")  # [y, x]
+    for f in range(6):
+        d = cube_texel_direction(f, s, t)
+        d = d / np.linalg.norm(d, axis=-1, keepdims=True)
+        faces.append(np.asarray(radiance(d), np.float64))
+    mip = np.stack(faces)  # [6, y, x, 3]
+    out = []
+    for m in range(mip_levels):
+        rgba = np.concatenate([mip, np.ones(mip.shape[:-1] + (1,))], axis=-1)
+        out.append(rgba.reshape(-1, 4))
+        if m + 1 < mip_levels:
+            e = mip.shape[1]
+            if e > 1:
+                mip = mip.reshape(6, e // 2, 2, e // 2, 2, 3).mean(axis=(2, 4))
+    return np.concatenate(out).astype(np.float32)
 
 HINT_PATCHES = {1: 0, 2: 1, 4: 2}
 DEFAULT_HINT_PATCH = 4  # the reference renderer's kDefaultHintPatch
@@ -312,6 +347,8 @@ SIGNATURES = {
     "cx_light_sample": (_INT, [ctypes.POINTER(CxLightTreeView), _P, _P, _U32, _P, _P, _P]),
     "cx_light_point_pdf": (_INT, [ctypes.POINTER(CxLightTreeView), _P, _U32, _P, _P]),
     "cx_light_mis_weight": (_INT, [_P, _U32, _P, _P]),
+    "cx_sky_cube_sample": (_INT, [_P, _P, _U32, _P, _P]),
+    "cx_sky_sh": (_INT, [_P, _P, _P]),
     "cx_light_sample_brick_box": (_INT, [ctypes.POINTER(CxLightTreeView), _P, _P, _U32, _P, _P]),
     "cx_light_brick_pdf": (_INT, [ctypes.POINTER(CxLightTreeView), _P, _U32, _U32, _P, _P]),
     "cx_light_box_support": (_INT, [ctypes.POINTER(CxLightTreeView), _P, _U32, _P, _P]),
@@ -835,6 +872,29 @@ def light_box_support(view: CxLightTreeView, queries: np.ndarray, stream=None) -
     return out.cpu().numpy().reshape(-1, 4)[:n]
 
 
+def sky_cube_sample(texels: np.ndarray, size: int, mip_levels: int, queries: np.ndarray) -> np.ndarray:
+    """The cubemap read per query (direction, lod): [n, 3] float32."""
+    torch = _torch()
+    t = torch.from_numpy(np.ascontiguousarray(texels, np.float32)).cuda()
+    q = torch.from_numpy(np.ascontiguousarray(queries, np.float32)).cuda()
+    n = len(queries)
+    out = torch.empty(max(n, 1) * 3, dtype=torch.float32, device="cuda")
+    cube = CxSkyCubemap(t.data_ptr(), size, mip_levels)
+    check(load().cx_sky_cube_sample(ctypes.byref(cube), q.data_ptr(), n, out.data_ptr(), stream_handle()),
+          "cx_sky_cube_sample")
+    return out.cpu().numpy().reshape(-1, 3)[:n]
+
+
+def sky_sh(texels: np.ndarray, size: int, mip_levels: int) -> np.ndarray:
+    """The cubemap's irradiance SH: [9, 3] float32."""
+    torch = _torch()
+    t = torch.from_numpy(np.ascontiguousarray(texels, np.float32)).cuda()
+    out = torch.empty(36, dtype=torch.float32, device="cuda")
+    cube = CxSkyCubemap(t.data_ptr(), size, mip_levels)
+    check(load().cx_sky_sh(ctypes.byref(cube), out.data_ptr(), stream_handle()), "cx_sky_sh")
+    return out.cpu().numpy().reshape(9, 4)[:, :3]
+
+
 def point_queries(queries: np.ndarray, group_of_brick: np.ndarray, group_coords: np.ndarray, group_shift: int):
     """Face queries {x, n, brick record, world voxel, face, block} (12 u32) as LightTreePointPdf queries
     {x, n, group, group-local voxel, face, block} (identity entity frames)."""
@@ -973,7 +1033,10 @@ class TraceScene:
 
     def trace_frame(self, camera: dict, bounces: int, seed: int, spp: int = 1, color=None, radiance=None,
                     attrib=None, counters=None, stream=None, flags: int = 0, sky: dict | None = None, fog=None,
-                    bounces_transparent: int = 0, bounce_nee_probability: float = 1.0, hint: LightHint | None = None):
+                    bounces_transparent: int = 0, bounce_nee_probability: float = 1.0, hint: LightHint | None = None,
+                    sky_cube: tuple | None = None):
+        """sky_cube = (texels [n, 4] float32 numpy or a device tensor, size, mip_levels): the host cubemap
+        in place of the procedural sky (with `sky`)."""
         torch = _torch()
         w, h = int(camera["width"]), int(camera["height"])
         color = color if color is not None else torch.empty(w * h * 4, dtype=torch.float32, device="cuda")
@@ -985,6 +1048,14 @@ class TraceScene:
         apply_sky(params, sky, fog)
         params.bounces_transparent = int(bounces_transparent)
         params.bounce_nee_probability = float(bounce_nee_probability)
+        cube_keep = None
+        if sky_cube is not None:
+            texels, size, mips = sky_cube
+            if isinstance(texels, np.ndarray):
+                texels = torch.from_numpy(np.ascontiguousarray(texels, np.float32)).cuda()
+            cube = CxSkyCubemap(texels.data_ptr(), int(size), int(mips))
+            cube_keep = (texels, cube)
+            params.sky_cubemap = ctypes.cast(ctypes.pointer(cube), ctypes.c_void_p)
         hint_struct = None
         if hint is not None:
             hint_struct = hint.frame_struct(camera)
@@ -994,6 +1065,8 @@ class TraceScene:
                                       ctypes.byref(params), ctypes.byref(out), stream_handle(stream)), "cx_trace_frame")
         if hint is not None:
             hint.advance(camera)
+        if cube_keep is not None:
+            torch.cuda.current_stream().synchronize()  # the texels may be a temporary upload
         return color, radiance, attrib, counters
 
     def trace_rays(self, rays: np.ndarray) -> np.ndarray:

@@ -25,6 +25,13 @@ struct SkyParams
     V3 zenith;
     V3 horizon;
     V3 ground;
+    // The host's cubemap in place of the procedural sky (null: procedural): mips of 6 faces of
+    // (size >> m)^2 rgba texels, mip-major, faces +X -X +Y -Y +Z -Z, rows from the top; and the
+    // irradiance SH of it (9 float4) that the frame projects first.
+    const float4* cube;
+    uint32_t cube_size;
+    uint32_t cube_mips;
+    const float4* sh;
 };
 
 // The light NEE's visibility hint per pixel patch (kLightNeeHint, caelix_path_trace.hlsl:229-236,
@@ -105,18 +112,156 @@ CX_DEV V3 SunDisk(const SkyParams& k, V3 d)
 {
     return Dot(d, k.sun_dir) > k.sun_cos_radius ? k.sun_radiance : Make(0.0f, 0.0f, 0.0f);
 }
-// SkyRadiance (:400-404): what a primary miss shows.
+// ---- the cubemap sky -------------------------------------------------------------------------------
+// Cube face selection (the Vulkan cube map rules; the inverse of the host's texel directions, the
+// reference test's CubeTexelDirection): the axis of largest magnitude (x, then y, then z on ties),
+// the face coordinates s, t in [0, 1] from (sc / |ma| + 1) / 2 with +X (-z, -y), -X (z, -y),
+// +Y (x, z), -Y (x, -z), +Z (x, -y), -Z (-x, -y).
+CX_DEV uint32_t CubeFace(V3 d, float& s, float& t)
+{
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    uint32_t face;
+    float sc, tc, ma;
+    if (ax >= ay && ax >= az)
+    {
+        face = d.x >= 0.0f ? 0u : 1u;
+        sc = d.x >= 0.0f ? -d.z : d.z;
+        tc = -d.y;
+        ma = ax;
+    }
+    else if (ay >= az)
+    {
+        face = d.y >= 0.0f ? 2u : 3u;
+        sc = d.x;
+        tc = d.y >= 0.0f ? d.z : -d.z;
+        ma = ay;
+    }
+    else
+    {
+        face = d.z >= 0.0f ? 4u : 5u;
+        sc = d.z >= 0.0f ? d.x : -d.x;
+        tc = -d.y;
+        ma = az;
+    }
+    s = (Div(sc, ma) + 1.0f) * 0.5f;
+    t = (Div(tc, ma) + 1.0f) * 0.5f;
+    return face;
+}
+
+// The texels of mip m of the cube.
+CX_DEV const float4* CubeMip(const SkyParams& k, uint32_t m)
+{
+    uint64_t offset = 0;
+    for (uint32_t i = 0; i < m; ++i)
+    {
+        const uint64_t e = max(k.cube_size >> i, 1u);
+        offset += 6 * e * e;
+    }
+    return k.cube + offset;
+}
+
+// Bilinear filtering inside one face of mip m (clamped to the face's edge texels).
+CX_DEV V3 CubeBilinear(const SkyParams& k, uint32_t m, uint32_t face, float s, float t)
+{
+    const uint32_t e = max(k.cube_size >> m, 1u);
+    const float4* texels = CubeMip(k, m) + uint64_t(face) * e * e;
+    const float u = s * float(e) - 0.5f;
+    const float v = t * float(e) - 0.5f;
+    const float fu = floorf(u);
+    const float fv = floorf(v);
+    const float a = u - fu;
+    const float b = v - fv;
+    const int x0 = min(max(int(fu), 0), int(e) - 1);
+    const int x1 = min(max(int(fu) + 1, 0), int(e) - 1);
+    const int y0 = min(max(int(fv), 0), int(e) - 1);
+    const int y1 = min(max(int(fv) + 1, 0), int(e) - 1);
+    const float4 t00 = texels[uint64_t(y0) * e + x0];
+    const float4 t10 = texels[uint64_t(y0) * e + x1];
+    const float4 t01 = texels[uint64_t(y1) * e + x0];
+    const float4 t11 = texels[uint64_t(y1) * e + x1];
+    const V3 top = Lerp(Make(t00.x, t00.y, t00.z), Make(t10.x, t10.y, t10.z), a);
+    const V3 bottom = Lerp(Make(t01.x, t01.y, t01.z), Make(t11.x, t11.y, t11.z), a);
+    return Lerp(top, bottom, b);
+}
+
+// The cubemap in direction d at mip level `lod` (SampleLevel with a linear mipmapped sampler):
+// the lod clamped to [0, mips - 1], bilinear in the two nearest mips, linear between them.
+CX_DEV V3 CubeSample(const SkyParams& k, V3 d, float lod)
+{
+    float s, t;
+    const uint32_t face = CubeFace(d, s, t);
+    const float l = fminf(fmaxf(lod, 0.0f), float(k.cube_mips - 1));
+    const uint32_t m0 = uint32_t(floorf(l));
+    const uint32_t m1 = min(m0 + 1u, k.cube_mips - 1u);
+    const float f = l - float(m0);
+    const V3 c0 = CubeBilinear(k, m0, face, s, t);
+    if (f == 0.0f)
+    {
+        return c0;
+    }
+    return Lerp(c0, CubeBilinear(k, m1, face, s, t), f);
+}
+
+// SkyRadianceNoSunLod (caelix_shading.hlsl:375-386): the cubemap at mip `lod`, else the procedural sky.
+CX_DEV V3 SkyNoSun(const SkyParams& k, V3 d, float lod)
+{
+    return k.cube != nullptr ? CubeSample(k, d, lod) : ProceduralSky(k, d);
+}
+
+// CaelixShBasis (caelix_shading.hlsl:346-359): the 9 real SH of bands 0..2 at the unit direction d.
+CX_DEV void ShBasis(V3 d, float (&basis)[9])
+{
+    basis[0] = 0.282095f;
+    basis[1] = 0.488603f * d.y;
+    basis[2] = 0.488603f * d.z;
+    basis[3] = 0.488603f * d.x;
+    basis[4] = 1.092548f * d.x * d.y;
+    basis[5] = 1.092548f * d.y * d.z;
+    basis[6] = 0.315392f * (3.0f * d.z * d.z - 1.0f);
+    basis[7] = 1.092548f * d.x * d.z;
+    basis[8] = 0.546274f * (d.x * d.x - d.y * d.y);
+}
+
+// SkyIrradianceOverPi (caelix_shading.hlsl:406-419): max(sum c_i Y_i(n), 0) / pi.
+CX_DEV V3 SkyIrradianceOverPi(const SkyParams& k, V3 n)
+{
+    float basis[9];
+    ShBasis(n, basis);
+    V3 sum = Make(0.0f, 0.0f, 0.0f);
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+    {
+        const float4 c = k.sh[i];
+        sum = Add(sum, Mul(Make(c.x, c.y, c.z), basis[i]));
+    }
+    return Mul(Make(fmaxf(sum.x, 0.0f), fmaxf(sum.y, 0.0f), fmaxf(sum.z, 0.0f)), 1.0f / kPi);
+}
+
+// SkyRadiance (:400-404): what a primary miss shows (the cubemap at mip 0).
 CX_DEV V3 SkyRadiance(const SkyParams& k, V3 d)
 {
-    return Add(ProceduralSky(k, d), SunDisk(k, d));
+    return Add(SkyNoSun(k, d, 0.0f), SunDisk(k, d));
 }
-// SkyDiffuseMiss (:428-437) of a bounce ray: every segment leaves a vertex that ran the NEE
+// SkyDiffuseMiss (:428-437): a diffuse bounce ray from a vertex with normal n that misses: the cubemap's
+// SH irradiance / pi at n, else the procedural sky in d; the sun disk when the segment sees the sun.
+CX_DEV V3 SkyDiffuseMiss(const SkyParams& k, V3 n, V3 d, bool sees_sun)
+{
+    const V3 sun = sees_sun ? SunDisk(k, d) : Make(0.0f, 0.0f, 0.0f);
+    return Add(k.cube != nullptr ? SkyIrradianceOverPi(k, n) : ProceduralSky(k, d), sun);
+}
+// SkySpecularMiss (:443-447): a specular bounce ray of a lobe with `roughness`: the cubemap at mip
+// roughness * (mips - 1); the sun disk when the segment sees the sun.
+CX_DEV V3 SkySpecularMiss(const SkyParams& k, V3 d, float roughness, bool sees_sun)
+{
+    const V3 sun = sees_sun ? SunDisk(k, d) : Make(0.0f, 0.0f, 0.0f);
+    return Add(SkyNoSun(k, d, roughness * float(k.cube_mips > 0 ? k.cube_mips - 1 : 0)), sun);
+}
+// A diffuse bounce ray of the opaque path misses: every segment leaves a vertex that ran the NEE
 // (segment_sees_sun = !nee, caelix_path_trace.hlsl:809, 903), so it sees the sun disk exactly when the
 // sun's NEE is off.
-CX_DEV V3 SkyMiss(const SkyParams& k, V3 d)
+CX_DEV V3 SkyMiss(const SkyParams& k, V3 n, V3 d)
 {
-    const V3 sun = k.sun != 0u ? Make(0.0f, 0.0f, 0.0f) : SunDisk(k, d);
-    return Add(ProceduralSky(k, d), sun);
+    return SkyDiffuseMiss(k, n, d, k.sun == 0u);
 }
 // SampleSunDirection (:467-485): uniform in the sun cone, cos theta = 1 - u1 (1 - cos r), phi =
 // 2 pi u2, 1 - cos r from the solid angle; the Duff et al. 2017 basis around sun_dir. sin / cos: the
@@ -336,7 +481,8 @@ struct ShadeFull
     V3 emission;
     V3 bounce_origin;
     V3 bounce_direction;
-    bool specular;  // the bounce took the specular lobe (always at a transparent block): no NEE
+    bool specular;    // the bounce took the specular lobe (always at a transparent block): no NEE
+    float roughness;  // the lobe's roughness: 1 - smoothness, 0 at a transparent block
 };
 // ShadeVoxelHit (caelix_shading.hlsl:267-307). Transparent block: the interface from `medium` into
 // the block's medium, albedo 1, origin pushed to the side of the chosen direction, `medium` updated.
@@ -357,6 +503,7 @@ CX_DEV ShadeFull ShadeVoxel(const SceneView& s, uint32_t block, V3 hit_position,
         sh.bounce_origin = Add(hit_position, Mul(it.normal, it.refracted ? -kPushOff : kPushOff));
         sh.bounce_direction = it.direction;
         sh.specular = true;
+        sh.roughness = 0.0f;
         medium = it.medium;
         return sh;
     }
@@ -370,6 +517,7 @@ CX_DEV ShadeFull ShadeVoxel(const SceneView& s, uint32_t block, V3 hit_position,
     sh.bounce_origin = Add(hit_position, Mul(normal, kPushOff));
     sh.bounce_direction = Lerp(diffuse, specular, do_specular);
     sh.specular = do_specular > 0.5f;
+    sh.roughness = 1.0f - m.smoothness;
     return sh;
 }
 // TraverseSegment (caelix_shading.hlsl:150-175): in the air the fog; inside a medium the throughput
@@ -391,11 +539,12 @@ CX_DEV void MediumSegment(const SceneView& s, const TraceParams& p, uint32_t med
                                  ExpCephes((-(1.0f - m.albedo.z)) * t * m.extinction));
     throughput = MulV(throughput, transmission);
 }
-// The sky a segment that misses receives (SkyDiffuseMiss / SkySpecularMiss with the procedural sky):
-// the sun disk when the segment sees the sun.
-CX_DEV V3 SkySegmentMiss(const SkyParams& k, V3 d, bool sees_sun)
+// The sky a segment that misses receives (caelix_path_trace.hlsl:841-846): SkyDiffuseMiss at the
+// segment's normal after a diffuse vertex, SkySpecularMiss of its lobe's roughness after a specular one
+// (roughness < 0 marks a diffuse segment).
+CX_DEV V3 SkySegmentMiss(const SkyParams& k, V3 n, V3 d, float roughness, bool sees_sun)
 {
-    return Add(ProceduralSky(k, d), sees_sun ? SunDisk(k, d) : Make(0.0f, 0.0f, 0.0f));
+    return roughness < 0.0f ? SkyDiffuseMiss(k, n, d, sees_sun) : SkySpecularMiss(k, d, roughness, sees_sun);
 }
 
 CX_DEV WorldHit Trace(const SceneView& s, V3 o, V3 d, float t_max, PathCounters& c, uint32_t scene = 0)

@@ -128,6 +128,7 @@ struct Workspace
                         // sun's NEE, or the bounce NEE budget below 1 (KeepsNeeStream)
     uint32_t* hint_word;  // [paths] the visibility hint the primary vertex read (sample 0) | traced << 16
     uint2* hint_counts;   // [paths] primary shadow rays over the samples: sun traced | visible << 16, lamp
+    float4* sky_sh;       // [9] the irradiance SH of the frame's cubemap sky
     uint32_t* ctl;      // [0], [1] queue lengths, [2] shadow queue length
     float* partials;    // [kGradRows][768] adjoint: per-workgroup albedo gradient sums
 };
@@ -163,7 +164,7 @@ uint32_t GradRows(uint32_t paths)
 }
 uint64_t WorkspaceBytes(uint32_t paths)
 {
-    return 13 * Align256(uint64_t(paths) * 16) + 4 * Align256(uint64_t(paths) * 4) + Align256(uint64_t(paths) * 8) + 256 +
+    return 13 * Align256(uint64_t(paths) * 16) + 4 * Align256(uint64_t(paths) * 4) + Align256(uint64_t(paths) * 8) + 512 +
            Align256(uint64_t(GradRows(paths)) * kGradEntries * 4);
 }
 Workspace CarveWorkspace(void* base, uint32_t paths)
@@ -194,6 +195,7 @@ Workspace CarveWorkspace(void* base, uint32_t paths)
     w.nee_rng = reinterpret_cast<uint32_t*>(take(uint64_t(paths) * 4));
     w.hint_word = reinterpret_cast<uint32_t*>(take(uint64_t(paths) * 4));
     w.hint_counts = reinterpret_cast<uint2*>(take(uint64_t(paths) * 8));
+    w.sky_sh = reinterpret_cast<float4*>(take(256));
     w.ctl = reinterpret_cast<uint32_t*>(take(256));
     w.partials = reinterpret_cast<float*>(take(uint64_t(GradRows(paths)) * kGradEntries * 4));
     return w;
@@ -714,7 +716,8 @@ CX_DEV void PrimaryGlass(const SceneView& s, const TraceParams& prm, const Works
     {
         w.nee_rng[path] = nee_rng;
     }
-    w.ray_o[path] = make_float4(sh.bounce_origin.x, sh.bounce_origin.y, sh.bounce_origin.z, 0.0f);
+    w.ray_o[path] = make_float4(sh.bounce_origin.x, sh.bounce_origin.y, sh.bounce_origin.z,
+                                sh.specular ? sh.roughness : -1.0f);
     w.ray_d[path] = make_float4(sh.bounce_direction.x, sh.bounce_direction.y, sh.bounce_direction.z, 0.0f);
 }
 
@@ -840,7 +843,7 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryShadeKernel(SceneView s, 
                 {
                     w.nee_rng[path] = nee_rng;
                 }
-                w.ray_o[path] = make_float4(sh.bounce_origin.x, sh.bounce_origin.y, sh.bounce_origin.z, 0.0f);
+                w.ray_o[path] = make_float4(sh.bounce_origin.x, sh.bounce_origin.y, sh.bounce_origin.z, -1.0f);
                 w.ray_d[path] = make_float4(sh.bounce_direction.x, sh.bounce_direction.y, sh.bounce_direction.z, 0.0f);
             }
         }
@@ -906,8 +909,10 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
             const float4 r4 = w.radiance[path];
             // The full material model keeps whether the segment sees the sun (it does after a specular
             // vertex, which ran no NEE); else every segment leaves an NEE vertex.
-            const V3 sky = Glass ? SkySegmentMiss(prm.sky, Make(d4.x, d4.y, d4.z), ((w.state[path].z >> 28) & 1u) != 0u)
-                                 : SkyMiss(prm.sky, Make(d4.x, d4.y, d4.z));
+            const uint32_t z = w.state[path].z;
+            const V3 n = FaceDirection((z >> 16) & 7u);
+            const V3 sky = Glass ? SkySegmentMiss(prm.sky, n, Make(d4.x, d4.y, d4.z), w.ray_o[path].w, ((z >> 28) & 1u) != 0u)
+                                 : SkyMiss(prm.sky, n, Make(d4.x, d4.y, d4.z));
             const V3 add = MulV(sky, Make(t4.x, t4.y, t4.z));
             w.radiance[path] = make_float4(r4.x + add.x, r4.y + add.y, r4.z + add.z, 0.0f);
         }
@@ -956,7 +961,8 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
             else
             {
                 const Shade lambert = ShadeHit(s, block, Add(origin, Mul(dir, hit.t)), vertex_normal, rng);
-                sh = ShadeFull{lambert.albedo, lambert.emission, lambert.bounce_origin, lambert.bounce_direction, false};
+                sh = ShadeFull{lambert.albedo, lambert.emission, lambert.bounce_origin, lambert.bounce_direction, false,
+                               1.0f};
             }
             const V3 emission = Mul(sh.emission, BrickScale(s, h.z));
             bounce_index += 1;
@@ -1022,7 +1028,9 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
                     roulette = roulette * inv_cp;
                 }
                 go_on = bounce_index <= prm.bounces;
-                w.ray_o[path] = make_float4(sh.bounce_origin.x, sh.bounce_origin.y, sh.bounce_origin.z, 0.0f);
+                // w: the segment's lobe for the sky it may miss into (roughness; -1: diffuse)
+                w.ray_o[path] = make_float4(sh.bounce_origin.x, sh.bounce_origin.y, sh.bounce_origin.z,
+                                            sh.specular ? sh.roughness : -1.0f);
                 w.ray_d[path] = make_float4(sh.bounce_direction.x, sh.bounce_direction.y, sh.bounce_direction.z, 0.0f);
             }
             w.radiance[path] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
@@ -1152,6 +1160,104 @@ __global__ void __launch_bounds__(256) HintUpdateKernel(Workspace w, CameraView 
         }
     }
     current[q] = uint16_t(HintChannel(first & 0xFFu, sun, sun_visible) | (HintChannel(first >> 8, lamp, lamp_visible) << 8));
+}
+
+// The sky SH kernel (shaders/render/caelix_sky_sh.hlsl): projects the cubemap at mip `lod` onto the 9
+// SH of bands 0..2 over the 6 x 32 x 32 texel-centre directions (24 per thread) with the solid angles
+// (2 / 32)^2 / (1 + s^2 + t^2)^1.5, renormalized by 4 pi / their sum, and convolved with the clamped
+// cosine (A_l = pi, 2 pi / 3, pi / 4), so that E(n) = sum c_i Y_i(n). One workgroup of 256 threads; the
+// sums of the threads are added by a tree over the workgroup (strides 128 .. 1).
+constexpr uint32_t kShThreads = 256;
+constexpr uint32_t kShFace = 32;
+
+// CubeTexelDirection (caelix_sky_sh.hlsl:41-58): the direction of face coordinates (s, t) in [-1, 1].
+CX_DEV V3 CubeTexelDirection(uint32_t face, float s, float t)
+{
+    switch (face)
+    {
+        case 0u:
+            return Make(1.0f, -t, -s);
+        case 1u:
+            return Make(-1.0f, -t, s);
+        case 2u:
+            return Make(s, 1.0f, t);
+        case 3u:
+            return Make(s, -1.0f, -t);
+        case 4u:
+            return Make(s, -t, 1.0f);
+        default:
+            return Make(-s, -t, -1.0f);
+    }
+}
+
+CX_DEV float ShWorkgroupSum(float* lds, float v)
+{
+    const uint32_t t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+    for (uint32_t stride = kShThreads / 2; stride > 0; stride >>= 1)
+    {
+        if (t < stride)
+        {
+            lds[t] = lds[t] + lds[t + stride];
+        }
+        __syncthreads();
+    }
+    const float total = lds[0];
+    __syncthreads();
+    return total;
+}
+
+__global__ void __launch_bounds__(kShThreads) SkyShKernel(SkyParams k, float lod, float4* sh)
+{
+    __shared__ float lds[kShThreads];
+    const uint32_t t = threadIdx.x;
+    float sums[9][3];
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+    {
+        sums[i][0] = sums[i][1] = sums[i][2] = 0.0f;
+    }
+    float weight_sum = 0.0f;
+    const float texel = 2.0f / float(kShFace);
+    for (uint32_t i = t; i < 6 * kShFace * kShFace; i += kShThreads)
+    {
+        const uint32_t face = i / (kShFace * kShFace);
+        const uint32_t x = i % kShFace;
+        const uint32_t y = (i / kShFace) % kShFace;
+        const float s = (float(x) + 0.5f) * texel - 1.0f;
+        const float tt = (float(y) + 0.5f) * texel - 1.0f;
+        const V3 v = CubeTexelDirection(face, s, tt);
+        const V3 direction = Mul(v, Div(1.0f, Sqrt(Dot(v, v))));
+        const float r2 = 1.0f + s * s + tt * tt;
+        const float weight = Div(texel * texel, r2 * Sqrt(r2));
+        const V3 radiance = CubeSample(k, direction, lod);
+        float basis[9];
+        ShBasis(direction, basis);
+#pragma unroll
+        for (int b = 0; b < 9; ++b)
+        {
+            const float w = basis[b] * weight;
+            sums[b][0] += radiance.x * w;
+            sums[b][1] += radiance.y * w;
+            sums[b][2] += radiance.z * w;
+        }
+        weight_sum += weight;
+    }
+    const float normalization = Div(4.0f * kPi, ShWorkgroupSum(lds, weight_sum));
+    for (int b = 0; b < 9; ++b)
+    {
+        const float band = b == 0 ? kPi : (b < 4 ? 2.0f * kPi / 3.0f : 0.25f * kPi);
+        float c[3];
+        for (int ch = 0; ch < 3; ++ch)
+        {
+            c[ch] = ShWorkgroupSum(lds, sums[b][ch]) * (band * normalization);
+        }
+        if (t == 0)
+        {
+            sh[b] = make_float4(c[0], c[1], c[2], 0.0f);
+        }
+    }
 }
 
 // Zeroes the albedo gradient rows of the workspace.
@@ -1323,6 +1429,22 @@ __global__ void __launch_bounds__(256) BoxSupportKernel(LightRef lights, const u
     reinterpret_cast<float4*>(out)[i] = result;
 }
 
+// The cubemap sky read as the frame reads it, per query (dir(3), lod): out = rgb.
+__global__ void __launch_bounds__(256) SkyCubeSampleKernel(SkyParams k, const float* queries, uint32_t count,
+                                                           float* out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count)
+    {
+        return;
+    }
+    const float* q = queries + uint64_t(i) * 4;
+    const V3 c = CubeSample(k, Make(q[0], q[1], q[2]), q[3]);
+    out[3 * uint64_t(i)] = c.x;
+    out[3 * uint64_t(i) + 1] = c.y;
+    out[3 * uint64_t(i) + 2] = c.z;
+}
+
 // LightNeeMisWeight of both sides per query: q = {p_area, d, cos_l, n_r (3), w (3)} -> {w_nee, w_bsdf}.
 __global__ void __launch_bounds__(256) MisWeightKernel(const float* queries, uint32_t count, float* weights)
 {
@@ -1427,6 +1549,12 @@ SkyParams SkyOf(const CxTraceParams* params)
     k.zenith = v(params->sky_zenith);
     k.horizon = v(params->sky_horizon);
     k.ground = v(params->sky_ground);
+    if (params->sky_cubemap != nullptr)
+    {
+        k.cube = reinterpret_cast<const float4*>(params->sky_cubemap->texels);
+        k.cube_size = params->sky_cubemap->size;
+        k.cube_mips = params->sky_cubemap->mip_levels;
+    }
     return k;
 }
 
@@ -1451,6 +1579,16 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
         const uint64_t n = uint64_t(rows) * kGradEntries;
         ClearRowsKernel<<<uint32_t(DivUp(n, uint64_t(256))), 256, 0, s>>>(w.partials, n);
     }
+    const CxSkyCubemap* cube = (params->flags & CX_TRACE_SKY) != 0 ? params->sky_cubemap : nullptr;
+    if (cube != nullptr)
+    {
+        // the irradiance SH of the cubemap, at the mip whose face is about 32 texels
+        SkyParams k = SkyOf(params);
+        const float max_lod = float(cube->mip_levels - 1);
+        const float sh_lod = std::min(std::max(std::log2(float(cube->size)) - 5.0f, 0.0f), max_lod);
+        StageScope scope(s, "trace_sky_sh");
+        SkyShKernel<<<1, kShThreads, 0, s>>>(k, sh_lod, w.sky_sh);
+    }
     const ShadowSource<Views> shadow_source{w, views, sv, g, 0u};
     // the visibility hint: one scene's frame (frame_camera) only
     const CxLightHint* hint = frame_camera != nullptr ? params->hint : nullptr;
@@ -1463,6 +1601,10 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
                       params->bounces_transparent, (params->flags & CX_TRACE_SHADOW_TRANSMISSION) != 0 ? 1u : 0u,
                       (params->flags & CX_TRACE_DELTA_CHECKERBOARD) != 0 ? 1u : 0u,
                       params->bounce_nee_probability > 0.0f ? params->bounce_nee_probability : 1.0f, HintParams{}};
+        if (cube != nullptr)
+        {
+            p.sky.sh = w.sky_sh;
+        }
         if (hint != nullptr)
         {
             const CameraView& cam = *frame_camera;
@@ -1614,6 +1756,20 @@ int CheckParams(const CxTraceParams* params, const CxTraceOutput* out, uint32_t 
             (h->previous_camera->width != width || h->previous_camera->height != height))
         {
             return ArgError("%s: the previous camera's image size differs", what);
+        }
+    }
+    if (params->sky_cubemap != nullptr)
+    {
+        const CxSkyCubemap* c = params->sky_cubemap;
+        uint32_t levels = 1;
+        while ((c->size >> levels) != 0u)
+        {
+            ++levels;
+        }
+        if ((params->flags & CX_TRACE_SKY) == 0 || c->texels == nullptr || c->size == 0 || c->mip_levels == 0 ||
+            c->mip_levels > levels)
+        {
+            return ArgError("%s: sky cubemap without CX_TRACE_SKY or texels, or a size or mip count out of range", what);
         }
     }
     if ((params->flags & CX_TRACE_SKY) != 0)
@@ -1965,6 +2121,43 @@ CX_API int cx_light_box_support(const CxLightTreeView* lights, const uint32_t* q
     const hipStream_t s = static_cast<hipStream_t>(stream);
     cx::BoxSupportKernel<<<cx::DivUp(count, 256), 256, 0, s>>>(cx::LightsOf(*lights), queries, count, out);
     return cx::LaunchResult("cx_light_box_support");
+}
+
+CX_API int cx_sky_cube_sample(const CxSkyCubemap* cube, const float* queries, uint32_t count, float* out,
+                              void* stream)
+{
+    if (cube == nullptr || cube->texels == nullptr || cube->size == 0 || cube->mip_levels == 0 ||
+        (count > 0 && (queries == nullptr || out == nullptr)))
+    {
+        return cx::ArgError("cx_sky_cube_sample: NULL cubemap, texels or buffer, or an empty cube");
+    }
+    if (count == 0)
+    {
+        return CX_OK;
+    }
+    cx::SkyParams k{};
+    k.cube = reinterpret_cast<const float4*>(cube->texels);
+    k.cube_size = cube->size;
+    k.cube_mips = cube->mip_levels;
+    cx::SkyCubeSampleKernel<<<cx::DivUp(count, 256), 256, 0, static_cast<hipStream_t>(stream)>>>(k, queries, count,
+                                                                                                   out);
+    return cx::LaunchResult("cx_sky_cube_sample");
+}
+
+CX_API int cx_sky_sh(const CxSkyCubemap* cube, float* sh, void* stream)
+{
+    if (cube == nullptr || cube->texels == nullptr || cube->size == 0 || cube->mip_levels == 0 || sh == nullptr)
+    {
+        return cx::ArgError("cx_sky_sh: NULL cubemap, texels or output, or an empty cube");
+    }
+    cx::SkyParams k{};
+    k.cube = reinterpret_cast<const float4*>(cube->texels);
+    k.cube_size = cube->size;
+    k.cube_mips = cube->mip_levels;
+    const float max_lod = float(cube->mip_levels - 1);
+    const float lod = std::min(std::max(std::log2(float(cube->size)) - 5.0f, 0.0f), max_lod);
+    cx::SkyShKernel<<<1, cx::kShThreads, 0, static_cast<hipStream_t>(stream)>>>(k, lod, reinterpret_cast<float4*>(sh));
+    return cx::LaunchResult("cx_sky_sh");
 }
 
 CX_API int cx_light_mis_weight(const float* queries, uint32_t count, float* weights, void* stream)

@@ -346,6 +346,8 @@ typedef struct CxCamera {
 
 struct CxLightHint;
 
+struct CxSkyCubemap;
+
 typedef struct CxTraceParams {
     uint32_t bounces;    /* bounce_count */
     uint32_t frame_seed;
@@ -382,7 +384,24 @@ typedef struct CxTraceParams {
     float bounce_nee_probability;
     /* The light NEE's visibility hint (below); NULL: off (the choice weights unchanged). */
     const struct CxLightHint* hint;
+    /* With CX_TRACE_SKY: the host's cubemap in place of the procedural sky (below); NULL: procedural. */
+    const struct CxSkyCubemap* sky_cubemap;
 } CxTraceParams;
+
+/* The host's sky cubemap (CaelixRenderOptions::sky, caelix.h:1009-1017) in device memory: mip_levels
+ * mips (1 .. log2(size) + 1), each 6 faces (+X, -X, +Y, -Y, +Z, -Z) of (size >> m)^2 texels, rows from
+ * the top of the face, rgba float32 (a ignored), mip-major. The frame samples it as a linear
+ * mipmapped sampler does (the Vulkan face selection; bilinear inside a face, clamped to its edge
+ * texels; linear between mips): primary misses and the delta chain's sky end at mip 0, a specular
+ * bounce that misses at mip roughness * (mip_levels - 1), a diffuse bounce that misses its irradiance
+ * SH / pi at the vertex normal (caelix_shading.hlsl:373-447); each frame first projects the SH
+ * (caelix_sky_sh.hlsl) from the mip whose face is about 32 texels. The sun (disk and NEE) stays as
+ * CxTraceParams gives it. */
+typedef struct CxSkyCubemap {
+    const float* texels;
+    uint32_t size;       /* face width = height of mip 0 */
+    uint32_t mip_levels;
+} CxSkyCubemap;
 
 /* The light NEE's visibility hint per pixel patch (kLightNeeHint; caelix_path_trace.hlsl:229-236,
  * 338-344, 453-580): the primary vertex multiplies its sun and lamp choice weights by max(0.05, hint)
@@ -541,6 +560,14 @@ CX_API int cx_light_brick_pdf(const CxLightTreeView* lights, const uint32_t* que
  * of the box. */
 CX_API int cx_light_box_support(const CxLightTreeView* lights, const uint32_t* queries, uint32_t count, float* out,
                                 void* stream);
+
+/* The cubemap sky as the frame reads it: queries[i] = direction(3), lod; out[i] = rgb (SampleLevel
+ * of caelix_shading.hlsl:375-386 with a linear mipmapped sampler). */
+CX_API int cx_sky_cube_sample(const CxSkyCubemap* cube, const float* queries, uint32_t count, float* out,
+                              void* stream);
+/* The irradiance SH of the cubemap that each frame projects first (shaders/render/caelix_sky_sh.hlsl):
+ * sh = 9 float4 (rgb, 0) of device memory, E(n) = sum sh[i] Y_i(n) (caelix_shading.hlsl:346-419). */
+CX_API int cx_sky_sh(const CxSkyCubemap* cube, float* sh, void* stream);
 
 /* LightNeeMisWeight (shaders/render/caelix_light_nee.hlsl:20-34), the balance heuristic both MIS sides
  * share, per query: queries[i] = {p_area, d, cos_l, receiver normal (3), unit direction (3)} (9

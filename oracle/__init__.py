@@ -156,7 +156,8 @@ class _Params(ctypes.Structure):
                 ("sun_solid_angle", ctypes.c_float), ("sun_radiance", _F3), ("sun_cos_radius", ctypes.c_float),
                 ("sky_zenith", _F3), ("pad0", ctypes.c_float), ("sky_horizon", _F3), ("pad1", ctypes.c_float),
                 ("sky_ground", _F3), ("pad2", ctypes.c_float), ("fog_color", _F3), ("fog_density", ctypes.c_float),
-                ("bounces_transparent", _U32), ("bounce_nee_probability", ctypes.c_float), ("hint", _P)]
+                ("bounces_transparent", _U32), ("bounce_nee_probability", ctypes.c_float), ("hint", _P),
+                ("sky_cubemap", _P)]
 
 
 NO_RUSSIAN_ROULETTE = 1
@@ -207,6 +208,10 @@ def _trace_lib():
         lib.oracle_light_brick_pdf.argtypes = [T, _P, _U32, _U32, _P]
         lib.oracle_light_box_support.restype = None
         lib.oracle_light_box_support.argtypes = [T, _P, _U32, _P]
+        lib.oracle_sky_cube_sample.restype = None
+        lib.oracle_sky_cube_sample.argtypes = [_P, _U32, _U32, _P, _U32, _P]
+        lib.oracle_sky_sh.restype = None
+        lib.oracle_sky_sh.argtypes = [_P, _U32, _U32, _P]
         lib.oracle_mis_weight.restype = None
         lib.oracle_mis_weight.argtypes = [_P, _U32, _P]
         lib.oracle_trace_frame.restype = None
@@ -445,6 +450,10 @@ def light_box_support(view: _TreeView, queries: np.ndarray) -> np.ndarray:
     return out
 
 
+class _Cube(ctypes.Structure):
+    _fields_ = [("texels", _P), ("size", _U32), ("mip_levels", _U32)]
+
+
 class _Hint(ctypes.Structure):
     _fields_ = [("previous", _P), ("current", _P), ("read", _P), ("previous_camera", ctypes.POINTER(_Camera)),
                 ("patch_shift", _U32), ("reset", _U32)]
@@ -483,6 +492,21 @@ class HintState:
 
     def read_words(self) -> np.ndarray:
         return self.read.reshape(self.height, self.width)
+
+
+def sky_cube_sample(texels: np.ndarray, size: int, mip_levels: int, queries: np.ndarray) -> np.ndarray:
+    t = np.ascontiguousarray(texels, np.float32)
+    q = np.ascontiguousarray(queries, np.float32)
+    out = np.zeros((len(q), 3), np.float32)
+    _trace_lib().oracle_sky_cube_sample(_ptr(t), size, mip_levels, _ptr(q), len(q), _ptr(out))
+    return out
+
+
+def sky_sh(texels: np.ndarray, size: int, mip_levels: int) -> np.ndarray:
+    t = np.ascontiguousarray(texels, np.float32)
+    out = np.zeros((9, 3), np.float32)
+    _trace_lib().oracle_sky_sh(_ptr(t), size, mip_levels, _ptr(out))
+    return out
 
 
 class OracleScene:
@@ -629,7 +653,7 @@ class OracleScene:
 
     def trace_frame(self, camera: dict, bounces: int, seed: int, spp: int = 1, rows=None, flags: int = 0,
                     sky: dict | None = None, fog=None, bounces_transparent: int = 0,
-                    bounce_nee_probability: float = 1.0, hint: HintState | None = None):
+                    bounce_nee_probability: float = 1.0, hint: HintState | None = None, sky_cube: tuple | None = None):
         """With `hint`: the rows must hold whole patches (the caller advances the hint after the last
         rows of a frame)."""
         cam = camera_struct(camera)
@@ -638,6 +662,12 @@ class OracleScene:
         prm.bounce_nee_probability = float(bounce_nee_probability)
         r0, r1 = rows if rows is not None else (0, cam.height)
         keep = None
+        cube_keep = None
+        if sky_cube is not None:
+            texels = np.ascontiguousarray(sky_cube[0], np.float32)
+            cube = _Cube(texels.ctypes.data, int(sky_cube[1]), int(sky_cube[2]))
+            cube_keep = (texels, cube)
+            prm.sky_cubemap = ctypes.cast(ctypes.pointer(cube), _P)
         if hint is not None:
             assert r0 % hint.patch == 0 and (r1 % hint.patch == 0 or r1 == cam.height)
             keep = hint.frame_struct()
@@ -650,7 +680,7 @@ class OracleScene:
         _trace_lib().oracle_trace_frame(ctypes.byref(self.scene), ctypes.byref(self.view), ctypes.byref(cam),
                                         ctypes.byref(prm), r0, r1, _ptr(color), _ptr(radiance), _ptr(attrib),
                                         _ptr(counters))
-        del keep
+        del keep, cube_keep
         if hint is not None and r1 == cam.height:
             hint.advance(camera)
         return color.reshape(-1, 4), radiance.reshape(-1, 4), attrib, counters

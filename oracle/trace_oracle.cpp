@@ -1359,6 +1359,10 @@ struct Sky
     bool sun = false;  // the sun's NEE: `on`, kRenderFlagSunNee, a sun radiance above 0 (caelix_path_trace.hlsl:742)
     V sun_dir, sun_radiance, zenith, horizon, ground;
     float sun_solid_angle = 0.0f, sun_cos_radius = 1.0f;
+    // the host's cubemap (CxSkyCubemap; null: the procedural sky) and its irradiance SH
+    const float* cube = nullptr;
+    uint32_t cube_size = 0, cube_mips = 0;
+    float sh[9][3] = {};
 };
 Sky SkyOf(const CxTraceParams& p)
 {
@@ -1384,6 +1388,12 @@ Sky SkyOf(const CxTraceParams& p)
     k.ground = Mk(p.sky_ground[0], p.sky_ground[1], p.sky_ground[2]);
     k.sun_solid_angle = p.sun_solid_angle;
     k.sun_cos_radius = p.sun_cos_radius;
+    if (p.sky_cubemap != nullptr)
+    {
+        k.cube = p.sky_cubemap->texels;
+        k.cube_size = p.sky_cubemap->size;
+        k.cube_mips = p.sky_cubemap->mip_levels;
+    }
     return k;
 }
 float Saturate(float x)
@@ -1409,16 +1419,199 @@ V SunDisk(const Sky& k, V d)
 {
     return Dot(d, k.sun_dir) > k.sun_cos_radius ? k.sun_radiance : Mk(0, 0, 0);
 }
+// The cubemap as a linear mipmapped sampler reads it (SampleLevel, caelix_shading.hlsl:375-386): the
+// face of the largest-magnitude axis (x, y, z on ties), face coordinates s, t = (sc / |ma| + 1) / 2 for
+// +X (-z, -y), -X (z, -y), +Y (x, z), -Y (x, -z), +Z (x, -y), -Z (-x, -y) (the inverse of the reference
+// test's CubeTexelDirection); bilinear at texel (s size - 0.5, t size - 0.5) inside the face, the
+// neighbours clamped to its edge; linear between the two mips around the clamped lod.
+uint32_t CubeFaceOf(V d, float& s, float& t)
+{
+    const float ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+    uint32_t face;
+    float sc, tc, ma;
+    if (ax >= ay && ax >= az)
+    {
+        face = d.x >= 0.0f ? 0u : 1u;
+        sc = d.x >= 0.0f ? -d.z : d.z;
+        tc = -d.y;
+        ma = ax;
+    }
+    else if (ay >= az)
+    {
+        face = d.y >= 0.0f ? 2u : 3u;
+        sc = d.x;
+        tc = d.y >= 0.0f ? d.z : -d.z;
+        ma = ay;
+    }
+    else
+    {
+        face = d.z >= 0.0f ? 4u : 5u;
+        sc = d.z >= 0.0f ? d.x : -d.x;
+        tc = -d.y;
+        ma = az;
+    }
+    s = (sc / ma + 1.0f) * 0.5f;
+    t = (tc / ma + 1.0f) * 0.5f;
+    return face;
+}
+V CubeTexel(const Sky& k, uint32_t m, uint32_t face, int x, int y)
+{
+    uint64_t offset = 0;
+    for (uint32_t i = 0; i < m; ++i)
+    {
+        const uint64_t e = std::max(k.cube_size >> i, 1u);
+        offset += 6 * e * e;
+    }
+    const uint64_t e = std::max(k.cube_size >> m, 1u);
+    const float* p = k.cube + (offset + uint64_t(face) * e * e + uint64_t(y) * e + uint64_t(x)) * 4;
+    return Mk(p[0], p[1], p[2]);
+}
+V CubeBilinear(const Sky& k, uint32_t m, uint32_t face, float s, float t)
+{
+    const int e = int(std::max(k.cube_size >> m, 1u));
+    const float u = s * float(e) - 0.5f;
+    const float v = t * float(e) - 0.5f;
+    const float fu = std::floor(u), fv = std::floor(v);
+    const float a = u - fu, b = v - fv;
+    auto clampi = [&](int i) { return std::min(std::max(i, 0), e - 1); };
+    const int x0 = clampi(int(fu)), x1 = clampi(int(fu) + 1), y0 = clampi(int(fv)), y1 = clampi(int(fv) + 1);
+    const V top = Lerp(CubeTexel(k, m, face, x0, y0), CubeTexel(k, m, face, x1, y0), a);
+    const V bottom = Lerp(CubeTexel(k, m, face, x0, y1), CubeTexel(k, m, face, x1, y1), a);
+    return Lerp(top, bottom, b);
+}
+V CubeSample(const Sky& k, V d, float lod)
+{
+    float s, t;
+    const uint32_t face = CubeFaceOf(d, s, t);
+    const float l = std::fmin(std::fmax(lod, 0.0f), float(k.cube_mips - 1));
+    const uint32_t m0 = uint32_t(std::floor(l));
+    const uint32_t m1 = std::min(m0 + 1u, k.cube_mips - 1u);
+    const float f = l - float(m0);
+    const V c0 = CubeBilinear(k, m0, face, s, t);
+    return f == 0.0f ? c0 : Lerp(c0, CubeBilinear(k, m1, face, s, t), f);
+}
+// CaelixShBasis (caelix_shading.hlsl:346-359).
+void ShBasis(V d, float basis[9])
+{
+    basis[0] = 0.282095f;
+    basis[1] = 0.488603f * d.y;
+    basis[2] = 0.488603f * d.z;
+    basis[3] = 0.488603f * d.x;
+    basis[4] = 1.092548f * d.x * d.y;
+    basis[5] = 1.092548f * d.y * d.z;
+    basis[6] = 0.315392f * (3.0f * d.z * d.z - 1.0f);
+    basis[7] = 1.092548f * d.x * d.z;
+    basis[8] = 0.546274f * (d.x * d.x - d.y * d.y);
+}
+// The sky SH projection (shaders/render/caelix_sky_sh.hlsl): 6 x 32 x 32 texel-centre directions of the
+// reference's CubeTexelDirection at mip `lod`, solid angles (2 / 32)^2 / (1 + s^2 + t^2)^1.5,
+// renormalized by 4 pi / their sum, bands times pi, 2 pi / 3, pi / 4. The sums run as the kernel adds
+// them: per thread of 256 over its directions in order, then a tree over the threads (strides
+// 128 .. 1).
+void ProjectSh(Sky& k, float lod)
+{
+    constexpr uint32_t kThreads = 256, kFace = 32;
+    std::vector<float> sums(kThreads * 28, 0.0f);  // per thread: 9 x 3 sums, the weight
+    const float texel = 2.0f / float(kFace);
+    for (uint32_t th = 0; th < kThreads; ++th)
+    {
+        float* acc = &sums[th * 28];
+        for (uint32_t i = th; i < 6 * kFace * kFace; i += kThreads)
+        {
+            const uint32_t face = i / (kFace * kFace);
+            const uint32_t x = i % kFace, y = (i / kFace) % kFace;
+            const float s = (float(x) + 0.5f) * texel - 1.0f;
+            const float t = (float(y) + 0.5f) * texel - 1.0f;
+            V v;
+            switch (face)
+            {
+                case 0: v = Mk(1.0f, -t, -s); break;
+                case 1: v = Mk(-1.0f, -t, s); break;
+                case 2: v = Mk(s, 1.0f, t); break;
+                case 3: v = Mk(s, -1.0f, -t); break;
+                case 4: v = Mk(s, -t, 1.0f); break;
+                default: v = Mk(-s, -t, -1.0f); break;
+            }
+            const V dir = Mul(v, 1.0f / std::sqrt(Dot(v, v)));
+            const float r2 = 1.0f + s * s + t * t;
+            const float weight = (texel * texel) / (r2 * std::sqrt(r2));
+            const V radiance = CubeSample(k, dir, lod);
+            float basis[9];
+            ShBasis(dir, basis);
+            for (int b = 0; b < 9; ++b)
+            {
+                const float w = basis[b] * weight;
+                acc[3 * b] += radiance.x * w;
+                acc[3 * b + 1] += radiance.y * w;
+                acc[3 * b + 2] += radiance.z * w;
+            }
+            acc[27] += weight;
+        }
+    }
+    auto tree = [&](int column) {
+        std::vector<float> v(kThreads);
+        for (uint32_t th = 0; th < kThreads; ++th)
+        {
+            v[th] = sums[th * 28 + column];
+        }
+        for (uint32_t stride = kThreads / 2; stride > 0; stride >>= 1)
+        {
+            for (uint32_t th = 0; th < stride; ++th)
+            {
+                v[th] = v[th] + v[th + stride];
+            }
+        }
+        return v[0];
+    };
+    const float normalization = (4.0f * kPi) / tree(27);
+    for (int b = 0; b < 9; ++b)
+    {
+        const float band = b == 0 ? kPi : (b < 4 ? 2.0f * kPi / 3.0f : 0.25f * kPi);
+        for (int ch = 0; ch < 3; ++ch)
+        {
+            k.sh[b][ch] = tree(3 * b + ch) * (band * normalization);
+        }
+    }
+}
+// SkyIrradianceOverPi (caelix_shading.hlsl:406-419).
+V SkyIrradianceOverPi(const Sky& k, V n)
+{
+    float basis[9];
+    ShBasis(n, basis);
+    V sum = Mk(0, 0, 0);
+    for (int i = 0; i < 9; ++i)
+    {
+        sum = Add(sum, Mul(Mk(k.sh[i][0], k.sh[i][1], k.sh[i][2]), basis[i]));
+    }
+    return Mul(Mk(std::fmax(sum.x, 0.0f), std::fmax(sum.y, 0.0f), std::fmax(sum.z, 0.0f)), 1.0f / kPi);
+}
+// SkyRadianceNoSunLod (caelix_shading.hlsl:375-386).
+V SkyNoSun(const Sky& k, V d, float lod)
+{
+    return k.cube != nullptr ? CubeSample(k, d, lod) : ProceduralSky(k, d);
+}
 // SkyRadiance (caelix_shading.hlsl:400-404), the primary miss (caelix_composite.hlsl:130-133).
 V SkyRadiance(const Sky& k, V d)
 {
-    return Add(ProceduralSky(k, d), SunDisk(k, d));
+    return Add(SkyNoSun(k, d, 0.0f), SunDisk(k, d));
 }
-// SkyDiffuseMiss (caelix_shading.hlsl:428-437) with the procedural sky; a bounce segment leaves a
-// vertex that ran the NEE, so segment_sees_sun = !sun (caelix_path_trace.hlsl:809, 903).
-V SkyMiss(const Sky& k, V d)
+// SkyDiffuseMiss (caelix_shading.hlsl:428-437): the cubemap's SH irradiance / pi at the normal, else the
+// procedural sky in d; the sun disk when the segment sees it.
+V SkyDiffuseMiss(const Sky& k, V n, V d, bool sees_sun)
 {
-    return Add(ProceduralSky(k, d), k.sun ? Mk(0, 0, 0) : SunDisk(k, d));
+    return Add(k.cube != nullptr ? SkyIrradianceOverPi(k, n) : ProceduralSky(k, d), sees_sun ? SunDisk(k, d) : Mk(0, 0, 0));
+}
+// SkySpecularMiss (caelix_shading.hlsl:443-447).
+V SkySpecularMiss(const Sky& k, V d, float roughness, bool sees_sun)
+{
+    return Add(SkyNoSun(k, d, roughness * float(k.cube_mips > 0 ? k.cube_mips - 1 : 0)),
+               sees_sun ? SunDisk(k, d) : Mk(0, 0, 0));
+}
+// The opaque path's bounce miss: a bounce segment leaves a vertex that ran the NEE, so
+// segment_sees_sun = !sun (caelix_path_trace.hlsl:809, 903).
+V SkyMiss(const Sky& k, V n, V d)
+{
+    return SkyDiffuseMiss(k, n, d, !k.sun);
 }
 // SampleSunDirection (caelix_shading.hlsl:467-485): uniform in the cone of solid angle
 // sun_solid_angle around sun_dir in the branchless basis of Duff et al. 2017.
@@ -1574,6 +1767,7 @@ struct ShadeFull
 {
     V albedo, emission, origin, direction;
     bool specular;
+    float roughness;  // the lobe's: 1 - smoothness, 0 at a transparent block (caelix_shading.hlsl:252)
 };
 // ShadeVoxelHit (caelix_shading.hlsl:267-307); lerp(a, b, t) = a + t (b - a).
 ShadeFull ShadeVoxel(const CxScene& s, uint32_t block, V hp, V dir, V normal, uint32_t& rng, uint32_t& medium)
@@ -1590,6 +1784,7 @@ ShadeFull ShadeVoxel(const CxScene& s, uint32_t block, V hp, V dir, V normal, ui
         sh.origin = Add(hp, Mul(it.normal, it.refracted ? -kPushOff : kPushOff));
         sh.direction = it.direction;
         sh.specular = true;
+        sh.roughness = 0.0f;
         medium = it.medium;
         return sh;
     }
@@ -1603,6 +1798,7 @@ ShadeFull ShadeVoxel(const CxScene& s, uint32_t block, V hp, V dir, V normal, ui
     sh.origin = Add(hp, Mul(normal, kPushOff));
     sh.direction = LerpV(diffuse, specular, do_specular);
     sh.specular = do_specular > 0.5f;
+    sh.roughness = 1.0f - m.smoothness;
     return sh;
 }
 // TraverseSegment (caelix_shading.hlsl:150-175): fog in the air, exp(-(1 - albedo) t extinction)
@@ -2062,7 +2258,7 @@ void TracePixel(const CxScene& s, const Lights& L, const CxCamera& cam, uint32_t
         {
             if (k.on)
             {
-                AddD(radiance, MulV(SkyMiss(k, bd), throughput));
+                AddD(radiance, MulV(SkyMiss(k, segment_normal, bd), throughput));
             }
             break;
         }
@@ -2259,6 +2455,7 @@ void TracePixelGlass(const CxScene& s, const Lights& L, const CxCamera& cam, uin
     emission = Add(MulV(chain, Mul(ps.emission, BrickScale(s, surface.record))), inscatter);
     V throughput = Mk(1, 1, 1);
     V segment_normal = pn;
+    float segment_rough = ps.specular ? ps.roughness : -1.0f;  // the segment's lobe; -1: diffuse
     uint32_t segment_light = 0u;
     bool sees_sun = true;
     bool claim = false;
@@ -2285,7 +2482,9 @@ void TracePixelGlass(const CxScene& s, const Lights& L, const CxCamera& cam, uin
         {
             if (k.on)
             {
-                AddD(radiance, MulV(Add(ProceduralSky(k, bd), sees_sun ? SunDisk(k, bd) : Mk(0, 0, 0)), throughput));
+                AddD(radiance, MulV(segment_rough < 0.0f ? SkyDiffuseMiss(k, segment_normal, bd, sees_sun)
+                                                         : SkySpecularMiss(k, bd, segment_rough, sees_sun),
+                                    throughput));
             }
             break;
         }
@@ -2347,6 +2546,7 @@ void TracePixelGlass(const CxScene& s, const Lights& L, const CxCamera& cam, uin
         bo = sh.origin;
         bd = sh.direction;
         segment_normal = vn;
+        segment_rough = sh.specular ? sh.roughness : -1.0f;
     }
 }
 
@@ -3004,6 +3204,40 @@ void oracle_light_point_pdf(const CxLightTreeView* view, const uint32_t* queries
     }
 }
 
+void oracle_sky_cube_sample(const float* texels, uint32_t size, uint32_t mips, const float* queries, uint32_t count,
+                            float* out)
+{
+    Sky k;
+    k.cube = texels;
+    k.cube_size = size;
+    k.cube_mips = mips;
+    for (uint32_t i = 0; i < count; ++i)
+    {
+        const float* q = queries + uint64_t(i) * 4;
+        const V c = CubeSample(k, Mk(q[0], q[1], q[2]), q[3]);
+        out[3 * i] = c.x;
+        out[3 * i + 1] = c.y;
+        out[3 * i + 2] = c.z;
+    }
+}
+
+void oracle_sky_sh(const float* texels, uint32_t size, uint32_t mips, float* sh)
+{
+    Sky k;
+    k.cube = texels;
+    k.cube_size = size;
+    k.cube_mips = mips;
+    const float max_lod = float(mips - 1);
+    ProjectSh(k, std::min(std::max(std::log2(float(size)) - 5.0f, 0.0f), max_lod));
+    for (int b = 0; b < 9; ++b)
+    {
+        for (int ch = 0; ch < 3; ++ch)
+        {
+            sh[3 * b + ch] = k.sh[b][ch];
+        }
+    }
+}
+
 void oracle_light_sample_brick_box(const CxLightTreeView* view, const float* boxes, const uint32_t* seeds,
                                    uint32_t count, uint32_t* out)
 {
@@ -3095,6 +3329,16 @@ void oracle_trace_frame(const CxScene* s, const CxLightTreeView* view, const CxC
     Sky sky = SkyOf(*prm);
     sky.glass = s->materials_ext != nullptr || prm->bounces_transparent > 0u ||
                 (prm->flags & CX_TRACE_SHADOW_TRANSMISSION) != 0u;
+    if (!sky.on)
+    {
+        sky.cube = nullptr;
+    }
+    if (sky.cube != nullptr)
+    {
+        // the irradiance SH of the cubemap, at the mip whose face is about 32 texels
+        const float max_lod = float(sky.cube_mips - 1);
+        ProjectSh(sky, std::min(std::max(std::log2(float(sky.cube_size)) - 5.0f, 0.0f), max_lod));
+    }
     // The visibility hint: per pixel of the rows, the word read (sample 0) and the primary shadow rays
     // over the samples (sun traced, sun visible, lamp traced, lamp visible).
     const CxLightHint* hint = prm->hint;

@@ -24,7 +24,8 @@ typedef char entity_info_is_128_bytes[(sizeof(EntityInfo) == 128) ? 1 : -1];
 typedef char light_stats_is_104_bytes[(sizeof(CxLightStats) == 104) ? 1 : -1];
 typedef char light_options_is_16_bytes[(sizeof(CxLightOptions) == 16) ? 1 : -1];
 typedef char tree_view_is_16_bytes[(sizeof(CxLightTreeView) == 16) ? 1 : -1];
-typedef char trace_params_is_128_bytes[(sizeof(CxTraceParams) == 128) ? 1 : -1];
+typedef char trace_params_is_136_bytes[(sizeof(CxTraceParams) == 136) ? 1 : -1];
+typedef char sky_cubemap_is_16_bytes[(sizeof(CxSkyCubemap) == 16) ? 1 : -1];
 typedef char trace_params_hint_at_120[(offsetof(CxTraceParams, hint) == 120) ? 1 : -1];
 typedef char light_hint_is_40_bytes[(sizeof(CxLightHint) == 40) ? 1 : -1];
 typedef char bvh_tuning_is_8_bytes[(sizeof(CxLightBvhTuning) == 8) ? 1 : -1];
@@ -108,6 +109,8 @@ size_t cx_abi_c99_touch(void)
         (AnyFn)cx_scene_finalize,
         (AnyFn)cx_scene_finalize_bricks,
         (AnyFn)cx_set_log_callback,
+        (AnyFn)cx_sky_cube_sample,
+        (AnyFn)cx_sky_sh,
         (AnyFn)cx_trace_batch,
         (AnyFn)cx_trace_batch_bytes,
         (AnyFn)cx_trace_batch_prepare,
