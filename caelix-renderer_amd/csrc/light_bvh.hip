@@ -516,6 +516,19 @@ __global__ void __launch_bounds__(kSortThreads) MortonKernel(Set set)
     }
     const uint32_t found = b.state->leaves_found;
     const uint32_t n = min(found, b.leaf_capacity);
+    const float* centres = b.scratch.centres();
+    const uint32_t base = blockIdx.x * kSortTile;
+    // The centres of the lane's items, all loaded before any store (a store between them would
+    // order the later loads behind it), and before the partials are reduced.
+    float cv[kSortItems][3];
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k)
+    {
+        const uint32_t i = min(base + k * kSortThreads + t, n - 1);  // past n: unused
+        cv[k][0] = n > 0 ? centres[3 * uint64_t(i)] : 0.0f;
+        cv[k][1] = n > 0 ? centres[3 * uint64_t(i) + 1] : 0.0f;
+        cv[k][2] = n > 0 ? centres[3 * uint64_t(i) + 2] : 0.0f;
+    }
     float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f};
     float hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
     const float* partials = b.scratch.bounds_partials();
@@ -547,14 +560,12 @@ __global__ void __launch_bounds__(kSortThreads) MortonKernel(Set set)
     const float e0 = fmaxf(hi[0] - lo0, 1e-6f);
     const float e1 = fmaxf(hi[1] - lo1, 1e-6f);
     const float e2 = fmaxf(hi[2] - lo2, 1e-6f);
-    const float* centres = b.scratch.centres();
     // the pairs start in the buffer that an even number of passes leaves them in buffer 0 from
     uint32_t* keys = b.scratch.keys(uint32_t(Keys::kPasses) & 1u);
     uint32_t* values = b.scratch.values(uint32_t(Keys::kPasses) & 1u);
     uint32_t* counter = b.scratch.u32(kLtScratchCounter);
     int32_t* diff = b.scratch.cover_diff();
-    const uint32_t base = blockIdx.x * kSortTile;
-#pragma unroll 4
+#pragma unroll
     for (int k = 0; k < kSortItems; ++k)
     {
         const uint32_t i = base + k * kSortThreads + t;
@@ -571,9 +582,9 @@ __global__ void __launch_bounds__(kSortThreads) MortonKernel(Set set)
         }
         if (i < n)
         {
-            const float c0 = centres[3 * uint64_t(i)];
-            const float c1 = centres[3 * uint64_t(i) + 1];
-            const float c2 = centres[3 * uint64_t(i) + 2];
+            const float c0 = cv[k][0];
+            const float c1 = cv[k][1];
+            const float c2 = cv[k][2];
             // clamp((c - lo) / extent * 1024, 0, 1023) with an IEEE division, truncated to uint.
             const uint32_t x = uint32_t(fminf(fmaxf(__fdiv_rn(c0 - lo0, e0) * 1024.0f, 0.0f), 1023.0f));
             const uint32_t y = uint32_t(fminf(fmaxf(__fdiv_rn(c1 - lo1, e1) * 1024.0f, 0.0f), 1023.0f));

@@ -383,22 +383,37 @@ __global__ void __launch_bounds__(kWaveThreads) PrimaryTraceKernel(SceneView s, 
 {
     const uint32_t path = blockIdx.x * kWaveThreads + threadIdx.x;
     PathCounters c{0, 0, 0, 0, 0};
+    WorldRay r;
+    r.phase = kRayDone;
     if (path < paths)
     {
         const uint32_t scene = views.SceneOf(path);
         const CameraView& cam = views.Camera(scene);
         uint32_t px, py;
         PathPixel(cam, views.Local(path), px, py);
-        uint4 h = make_uint4(0u, 0u, kHashNone, 0u);
         if (px < cam.width && py < cam.height)
         {
             const V3 dir = PrimaryDirection(cam, px, py);
             const V3 origin = Make(cam.position[0], cam.position[1], cam.position[2]);
             c.rays += 1;
-            const WorldHit hit = Trace(s, origin, dir, kRayTMax, c, scene);
-            h = make_uint4(__float_as_uint(hit.t), hit.attrib, hit.record, 0u);
+            WorldRayBegin(s, origin, dir, kRayTMax, scene, r);
         }
-        w.hit[path] = h;
+    }
+    // The wave's rays advance together, their bricks traced in batches (WorldRayAdvance).
+    bool busy = r.phase != kRayDone;
+    WorldHit hit{0.0f, 0u, kHashNone, {0, 0, 0}, 0u, 0u, 0u};
+    while (Ballot(busy) != 0)
+    {
+        WorldHit h;
+        if (WorldRayAdvance(s, r, busy, h, c))
+        {
+            hit = h;
+            busy = false;
+        }
+    }
+    if (path < paths)
+    {
+        w.hit[path] = make_uint4(__float_as_uint(hit.t), hit.attrib, hit.record, 0u);
     }
     FlushCounters(counters, c);
 }
@@ -571,15 +586,11 @@ __global__ void __launch_bounds__(kWaveThreads) PersistentTraceKernel(SceneView 
             }
             continue;
         }
-        if (busy)
+        WorldHit h;
+        if (WorldRayAdvance(s, r, busy, h, c))
         {
-            WorldHit h;
-            WorldRayStep(s, r, h, c);
-            if (r.phase == kRayDone)
-            {
-                src.Finish(item, h, lds_grad);
-                busy = false;
-            }
+            src.Finish(item, h, lds_grad);
+            busy = false;
         }
     }
     if (adjoint)

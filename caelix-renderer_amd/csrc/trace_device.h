@@ -700,6 +700,7 @@ constexpr uint32_t kRayDone = 0;     // finished
 constexpr uint32_t kRaySuper = 1;    // at the top of the super-cell loop for sc
 constexpr uint32_t kRayBricks = 2;   // at brick cell bc inside the resident super cell sc
 constexpr uint32_t kRayAdvance = 3;  // leaving super cell sc
+constexpr uint32_t kRayBrick = 4;    // at the resident brick cell bc, its brick not yet traced
 
 struct WorldRay
 {
@@ -774,14 +775,56 @@ CX_DEV void WorldRayBegin(const SceneView& s, V3 o, V3 dir, float t_max, uint32_
     r.phase = kRaySuper;
 }
 
-// One step of the ray: the super-cell test, at most one brick cell (with its brick DDA) and the
-// super-cell advance when the ray leaves it. On a hit or a miss the phase becomes kRayDone and `hit`
-// holds the result; the counters count this step's work.
-CX_DEV void WorldRayStep(const SceneView& s, WorldRay& r, WorldHit& hit, PathCounters& c)
+// The ray's move past brick cell bc: to the next brick cell of the super cell (phase kRayBricks), out
+// of the super cell (kRayAdvance) or, past t_max, to the end (kRayDone).
+CX_DEV void WorldRayNextCell(const SceneView& s, WorldRay& r)
 {
-    hit = WorldHit{0.0f, 0u, kHashNone, {0, 0, 0}, 0u, 0u, 0u};
-    const int super_dim = int(s.grid_dim >> 3);
     const int st0 = StepOf(r.dir.x), st1 = StepOf(r.dir.y), st2 = StepOf(r.dir.z);
+    const float tb0 = (float((r.bc0 + (st0 > 0 ? 1 : 0)) * 8) + GridOrigin(s, 0) - r.o.x) * r.ri.safe.x;
+    const float tb1 = (float((r.bc1 + (st1 > 0 ? 1 : 0)) * 8) + GridOrigin(s, 1) - r.o.y) * r.ri.safe.y;
+    const float tb2 = (float((r.bc2 + (st2 > 0 ? 1 : 0)) * 8) + GridOrigin(s, 2) - r.o.z) * r.ri.safe.z;
+    int axis = 0;
+    float t_axis = tb0;
+    if (tb1 < t_axis)
+    {
+        axis = 1;
+        t_axis = tb1;
+    }
+    if (tb2 < t_axis)
+    {
+        axis = 2;
+        t_axis = tb2;
+    }
+    if (t_axis > r.t_max)
+    {
+        r.phase = kRayDone;
+        return;
+    }
+    bool left;
+    if (axis == 0)
+    {
+        r.bc0 += st0;
+        left = r.bc0 < r.sc0 * 8 || r.bc0 > r.sc0 * 8 + 7;
+    }
+    else if (axis == 1)
+    {
+        r.bc1 += st1;
+        left = r.bc1 < r.sc1 * 8 || r.bc1 > r.sc1 * 8 + 7;
+    }
+    else
+    {
+        r.bc2 += st2;
+        left = r.bc2 < r.sc2 * 8 || r.bc2 > r.sc2 * 8 + 7;
+    }
+    r.phase = left ? kRayAdvance : kRayBricks;
+}
+
+// One cheap step of the ray: the super-cell test, the bitmap test of one brick cell, or the super-cell
+// advance. A resident brick cell stops the ray in phase kRayBrick (WorldRayBrick traces it); a ray that
+// leaves the grid or passes t_max ends (phase kRayDone: a miss). The counters count this step's work.
+CX_DEV void WorldRayCellStep(const SceneView& s, WorldRay& r, PathCounters& c)
+{
+    const int super_dim = int(s.grid_dim >> 3);
     if (r.phase == kRaySuper)
     {
         const uint64_t super_index =
@@ -814,75 +857,18 @@ CX_DEV void WorldRayStep(const SceneView& s, WorldRay& r, WorldHit& hit, PathCou
             r.scene * s.scene_cells + (uint64_t(r.bc2) * s.grid_dim + r.bc1) * s.grid_dim + r.bc0;
         if (BitSet(s.brick_bits, brick_index))
         {
-            const int32_t key0 = s.grid_min[0] + r.bc0;
-            const int32_t key1 = s.grid_min[1] + r.bc1;
-            const int32_t key2 = s.grid_min[2] + r.bc2;
-            uint32_t probes = 0;
-            const uint32_t record = HashGet(s.slots, s.slot_count, s.pow2, s.hash_info[0],
-                                            SceneBrickFingerprint(s, r.scene, key0, key1, key2), &probes);
-            c.probes += probes;
-            if (record != kHashNone)
-            {
-                const V3 origin = Make(float(key0 * 8), float(key1 * 8), float(key2 * 8));
-                const BrickHit h = TraceBrick(s, record, r.o, r.dir, r.ri, r.t_max, origin);
-                c.steps += h.steps;
-                if (h.attrib != 0u && h.t < r.t_max)
-                {
-                    hit.t = h.t;
-                    hit.attrib = h.attrib;
-                    hit.record = record;
-                    hit.brick[0] = key0;
-                    hit.brick[1] = key1;
-                    hit.brick[2] = key2;
-                    r.phase = kRayDone;
-                    return;
-                }
-            }
-        }
-        const float tb0 = (float((r.bc0 + (st0 > 0 ? 1 : 0)) * 8) + GridOrigin(s, 0) - r.o.x) * r.ri.safe.x;
-        const float tb1 = (float((r.bc1 + (st1 > 0 ? 1 : 0)) * 8) + GridOrigin(s, 1) - r.o.y) * r.ri.safe.y;
-        const float tb2 = (float((r.bc2 + (st2 > 0 ? 1 : 0)) * 8) + GridOrigin(s, 2) - r.o.z) * r.ri.safe.z;
-        int axis = 0;
-        float t_axis = tb0;
-        if (tb1 < t_axis)
-        {
-            axis = 1;
-            t_axis = tb1;
-        }
-        if (tb2 < t_axis)
-        {
-            axis = 2;
-            t_axis = tb2;
-        }
-        if (t_axis > r.t_max)
-        {
-            r.phase = kRayDone;
+            r.phase = kRayBrick;
             return;
         }
-        bool left;
-        if (axis == 0)
-        {
-            r.bc0 += st0;
-            left = r.bc0 < r.sc0 * 8 || r.bc0 > r.sc0 * 8 + 7;
-        }
-        else if (axis == 1)
-        {
-            r.bc1 += st1;
-            left = r.bc1 < r.sc1 * 8 || r.bc1 > r.sc1 * 8 + 7;
-        }
-        else
-        {
-            r.bc2 += st2;
-            left = r.bc2 < r.sc2 * 8 || r.bc2 > r.sc2 * 8 + 7;
-        }
-        if (!left)
+        WorldRayNextCell(s, r);
+        if (r.phase != kRayAdvance)
         {
             return;
         }
-        r.phase = kRayAdvance;
     }
     if (r.phase == kRayAdvance)
     {
+        const int st0 = StepOf(r.dir.x), st1 = StepOf(r.dir.y), st2 = StepOf(r.dir.z);
         const float tn0 = (float((r.sc0 + (st0 > 0 ? 1 : 0)) * 64) + GridOrigin(s, 0) - r.o.x) * r.ri.safe.x;
         const float tn1 = (float((r.sc1 + (st1 > 0 ? 1 : 0)) * 64) + GridOrigin(s, 1) - r.o.y) * r.ri.safe.y;
         const float tn2 = (float((r.sc2 + (st2 > 0 ? 1 : 0)) * 64) + GridOrigin(s, 2) - r.o.z) * r.ri.safe.z;
@@ -925,6 +911,76 @@ CX_DEV void WorldRayStep(const SceneView& s, WorldRay& r, WorldHit& hit, PathCou
         r.t_cell = t_axis;
         r.phase = kRaySuper;
     }
+}
+
+// The resident brick cell bc of a ray in phase kRayBrick: the hash lookup and, for a brick in the table,
+// the brick DDA. A hit ends the ray (phase kRayDone, `hit` set); otherwise the ray moves to the next
+// cell. The counters count the probes and DDA steps.
+CX_DEV void WorldRayBrick(const SceneView& s, WorldRay& r, WorldHit& hit, PathCounters& c)
+{
+    const int32_t key0 = s.grid_min[0] + r.bc0;
+    const int32_t key1 = s.grid_min[1] + r.bc1;
+    const int32_t key2 = s.grid_min[2] + r.bc2;
+    uint32_t probes = 0;
+    const uint32_t record = HashGet(s.slots, s.slot_count, s.pow2, s.hash_info[0],
+                                    SceneBrickFingerprint(s, r.scene, key0, key1, key2), &probes);
+    c.probes += probes;
+    if (record != kHashNone)
+    {
+        const V3 origin = Make(float(key0 * 8), float(key1 * 8), float(key2 * 8));
+        const BrickHit h = TraceBrick(s, record, r.o, r.dir, r.ri, r.t_max, origin);
+        c.steps += h.steps;
+        if (h.attrib != 0u && h.t < r.t_max)
+        {
+            hit.t = h.t;
+            hit.attrib = h.attrib;
+            hit.record = record;
+            hit.brick[0] = key0;
+            hit.brick[1] = key1;
+            hit.brick[2] = key2;
+            r.phase = kRayDone;
+            return;
+        }
+    }
+    WorldRayNextCell(s, r);
+}
+
+// How a wave advances its rays: the lanes waiting at a resident brick (phase kRayBrick) trace their
+// bricks together once at least half the busy lanes wait, or when no busy lane can take a cheap step;
+// otherwise the other busy lanes take up to kCellSteps cheap steps each. The hash lookup and brick DDA
+// are the costly part of a ray's traversal; run only when most lanes need them, they keep most of the
+// wave's lanes busy instead of a few. Each ray still visits its cells in order, so hits and counters
+// are those of TraceWorld. Returns true for the lanes whose ray ended in this call (hit or miss in `hit`).
+constexpr uint32_t kCellSteps = 4;
+CX_DEV bool WorldRayAdvance(const SceneView& s, WorldRay& r, bool busy, WorldHit& hit, PathCounters& c)
+{
+    hit = WorldHit{0.0f, 0u, kHashNone, {0, 0, 0}, 0u, 0u, 0u};
+    const uint64_t busy_lanes = Ballot(busy);
+    const uint64_t waiting = Ballot(busy && r.phase == kRayBrick);
+    const bool bricks = waiting == busy_lanes || 2u * PopC(waiting) >= PopC(busy_lanes);
+    if (!busy)
+    {
+        return false;
+    }
+    if (bricks)
+    {
+        if (r.phase == kRayBrick)
+        {
+            WorldRayBrick(s, r, hit, c);
+        }
+    }
+    else if (r.phase != kRayBrick)
+    {
+        for (uint32_t k = 0; k < kCellSteps; ++k)
+        {
+            WorldRayCellStep(s, r, c);
+            if (r.phase == kRayBrick || r.phase == kRayDone)
+            {
+                break;
+            }
+        }
+    }
+    return r.phase == kRayDone;
 }
 
 }  // namespace cx
