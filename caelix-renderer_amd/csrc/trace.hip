@@ -368,60 +368,56 @@ CX_DEV void GradRowsEnd(const Workspace& w, float* lds)
 
 // ---- persistent traversal ----------------------------------------------------------------------
 // Control words: [0], [1] path queue lengths, [2] shadow queue length, [8] shadow cursor, [9] bounce
-// cursor. A wave takes 64 consecutive items of its queue per atomic and hands
-// them to its idle lanes; every lane advances its ray one cell per iteration and takes the next item
-// when its ray ends, so a wave's lanes stay busy until the queue is drained.
+// cursor, [10] camera-ray cursor. A wave takes a chunk of consecutive items of its queue per atomic
+// and hands them to its idle lanes; the lanes advance their rays together (WorldRayAdvance) and an
+// idle lane takes the next item, so a wave's lanes stay busy until the queue is drained.
 constexpr uint32_t kCtlShadowCursor = 8;
 constexpr uint32_t kCtlBounceCursor = 9;
-
-// Camera rays of every path and their first hit, one path per lane: the rays of a wave are one 8x8
-// pixel tile and stay coherent, so they run in lockstep without refills (faster here than the
-// persistent form).
-template <class Views>
-__global__ void __launch_bounds__(kWaveThreads) PrimaryTraceKernel(SceneView s, Views views, Workspace w,
-                                                                   uint32_t paths, unsigned long long* counters)
-{
-    const uint32_t path = blockIdx.x * kWaveThreads + threadIdx.x;
-    PathCounters c{0, 0, 0, 0, 0};
-    WorldRay r;
-    r.phase = kRayDone;
-    if (path < paths)
-    {
-        const uint32_t scene = views.SceneOf(path);
-        const CameraView& cam = views.Camera(scene);
-        uint32_t px, py;
-        PathPixel(cam, views.Local(path), px, py);
-        if (px < cam.width && py < cam.height)
-        {
-            const V3 dir = PrimaryDirection(cam, px, py);
-            const V3 origin = Make(cam.position[0], cam.position[1], cam.position[2]);
-            c.rays += 1;
-            WorldRayBegin(s, origin, dir, kRayTMax, scene, r);
-        }
-    }
-    // The wave's rays advance together, their bricks traced in batches (WorldRayAdvance).
-    bool busy = r.phase != kRayDone;
-    WorldHit hit{0.0f, 0u, kHashNone, {0, 0, 0}, 0u, 0u, 0u};
-    while (Ballot(busy) != 0)
-    {
-        WorldHit h;
-        if (WorldRayAdvance(s, r, busy, h, c))
-        {
-            hit = h;
-            busy = false;
-        }
-    }
-    if (path < paths)
-    {
-        w.hit[path] = make_uint4(__float_as_uint(hit.t), hit.attrib, hit.record, 0u);
-    }
-    FlushCounters(counters, c);
-}
+constexpr uint32_t kCtlPrimaryCursor = 10;
 
 __global__ void __launch_bounds__(64) ResetKernel(Workspace w)
 {
     w.ctl[threadIdx.x] = 0u;
 }
+
+// Camera rays for the persistent traversal: item = path (its pixel by PathPixel); every path gets its
+// hit, a miss for a path outside its image (no ray: t_max < 0 ends it before the grid). A wave claims
+// kPrimaryChunk paths at a time (one 8x8 pixel tile): the rays a wave holds stay coherent.
+constexpr uint32_t kPrimaryChunk = 64;
+template <class Views>
+struct PrimarySource
+{
+    Workspace w;
+    Views views;
+    uint32_t paths;
+    CX_DEV uint32_t Count() const { return paths; }
+    CX_DEV uint32_t* Cursor() const { return &w.ctl[kCtlPrimaryCursor]; }
+    CX_DEV void Prepare() const {}
+    CX_DEV bool Load(uint32_t item, V3& o, V3& d, float& t_max, uint32_t& scene, PathCounters& c) const
+    {
+        scene = views.SceneOf(item);
+        const CameraView& cam = views.Camera(scene);
+        uint32_t px, py;
+        PathPixel(cam, views.Local(item), px, py);
+        o = Make(cam.position[0], cam.position[1], cam.position[2]);
+        if (px < cam.width && py < cam.height)
+        {
+            d = PrimaryDirection(cam, px, py);
+            t_max = kRayTMax;
+            c.rays += 1;
+        }
+        else
+        {
+            d = Make(0.0f, 0.0f, 1.0f);
+            t_max = -1.0f;
+        }
+        return true;
+    }
+    CX_DEV void Finish(uint32_t item, const WorldHit& h, float*) const
+    {
+        w.hit[item] = make_uint4(__float_as_uint(h.t), h.attrib, h.record, 0u);
+    }
+};
 
 // Bounce rays: item -> path of queue[in].
 template <class Views>
@@ -1584,6 +1580,8 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
     const uint32_t rows = GradRows(paths);
     const uint32_t bounce_blocks = std::min(TunedBlocks(PersistentBlocks(PersistentTraceKernel<BounceSource<Views>>)), rows);
     const uint32_t shadow_blocks = std::min(TunedBlocks(PersistentBlocks(PersistentTraceKernel<ShadowSource<Views>>)), rows);
+    const uint32_t primary_blocks =
+        std::min(TunedBlocks(PersistentBlocks(PersistentTraceKernel<PrimarySource<Views>>)), blocks);
     const uint32_t chunk = g_tuning.queue_chunk;
     if (adjoint)
     {
@@ -1633,7 +1631,8 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
         ResetKernel<<<1, 64, 0, s>>>(w);
         {
             StageScope scope(s, adjoint ? "adjoint_trace" : "trace_primary");
-            PrimaryTraceKernel<<<blocks, kWaveThreads, 0, s>>>(sv, views, w, paths, o.counters);
+            PersistentTraceKernel<<<primary_blocks, kWaveThreads, 0, s>>>(sv, PrimarySource<Views>{w, views, paths},
+                                                                          o.counters, w, false, kPrimaryChunk);
         }
         {
             StageScope scope(s, adjoint ? "adjoint_shade" : "trace_shade_primary");
