@@ -66,7 +66,7 @@ class CxScene(ctypes.Structure):
         ("light_codes", ctypes.c_void_p), ("emission_scale", ctypes.c_float), ("pad", ctypes.c_uint32),
         ("brick_emission_scale", ctypes.c_void_p), ("scene_count", ctypes.c_uint32), ("pad2", ctypes.c_uint32),
         ("occupied_box", ctypes.c_void_p), ("materials_ext", ctypes.c_void_p), ("group_of_brick", ctypes.c_void_p),
-        ("light_group_rows", ctypes.c_void_p),
+        ("light_group_rows", ctypes.c_void_p), ("subsurface", ctypes.c_void_p),
     ]
 
 
@@ -223,6 +223,7 @@ TRACE_SKY = 2
 TRACE_NO_SUN_NEE = 4
 TRACE_SHADOW_TRANSMISSION = 8
 TRACE_DELTA_CHECKERBOARD = 16
+TRACE_SUBSURFACE = 32
 
 
 def sky_params(sun_dir=(0.0, 1.0, 0.0), sun_radiance=(0.0, 0.0, 0.0), sun_radius_deg: float = 0.5,
@@ -977,6 +978,10 @@ class TraceScene:
         sc.emission_scale = 1.0
         sc.occupied_box = self.occupied_box.data_ptr()
         sc.materials_ext = None if self.materials_ext is None else self.materials_ext.data_ptr()
+        sss = getattr(data, "subsurface", None)
+        self.subsurface = None if sss is None else torch.from_numpy(
+            np.ascontiguousarray(sss).view(np.uint8).copy()).to(dev)
+        sc.subsurface = None if self.subsurface is None else self.subsurface.data_ptr()
         # the light tree: one static entity at the identity frame, its groups, the tree's flushes
         coords, gob = groups_of(data.keys, int(data.group_shift))
         self.group_coords = coords
@@ -1122,5 +1127,6 @@ class TraceScene:
             "group_of_brick": self.group_of_brick_host.copy(),
             "group_infos": self.group_infos.cpu().numpy(),
             "entity_infos": self.entity_infos.cpu().numpy(),
+            **({} if self.subsurface is None else {"subsurface": self.subsurface.cpu().numpy()}),
             **{"lt_" + k: v for k, v in self.light_tables().items()},
         }

@@ -35,6 +35,31 @@ class SceneData:
     emissive_bricks: np.ndarray
     encoding: int = 1        # block values: 1 block-ID (opaque 0x8000, id 0x1FF, faces << 9), 0 packed
     materials_ext: np.ndarray | None = None  # float32 [65536, 4] (ior, extinction, 0, 0): the full material model
+    subsurface: np.ndarray | None = None     # SUBSURFACE [65536]: the subsurface entries (CxSubsurface)
+
+
+# CxSubsurface (include/caelix_amd/caelix_amd.h): one per material code, 32 bytes.
+SUBSURFACE = np.dtype([("one_minus_albedo", "<f4", 3), ("anisotropy", "<f4"), ("probability", "<f4"),
+                       ("model", "<u4"), ("steps", "<u4"), ("pad", "<u4")])
+assert SUBSURFACE.itemsize == 32
+SSS_MODEL_HG = 1
+
+
+def subsurface_table(entries: dict) -> np.ndarray:
+    """The subsurface table of {material code: (sss_albedo rgb, anisotropy, probability, steps[, model])},
+    packed as the reference's host packs its GPU subsurface entry (docs/MATERIALS.md: halves of
+    1 - sss_albedo, of g and of p, rounded to nearest even) and widened back to float."""
+    t = np.zeros(65536, SUBSURFACE)
+    half = lambda x: np.float32(np.float16(np.float32(x)))  # noqa: E731
+    for code, e in entries.items():
+        albedo, g, p, steps = e[:4]
+        model = e[4] if len(e) > 4 else SSS_MODEL_HG
+        t[code]["one_minus_albedo"] = [half(np.float32(1.0) - np.float32(a)) for a in albedo]
+        t[code]["anisotropy"] = half(g)
+        t[code]["probability"] = half(p)
+        t[code]["model"] = model
+        t[code]["steps"] = int(steps) & 0xFF
+    return t
 
 
 def value_noise(size: int, seed: int, octaves: int, base_period: int) -> np.ndarray:

@@ -69,6 +69,7 @@ struct TraceParams
     uint32_t checkerboard;  // kRenderFlagDeltaCheckerboard: the chain's first interface split by pixel parity
     float nee_q;            // bounce_nee_probability (1: every bounce vertex runs its NEE)
     HintParams hint;        // the primary vertex's visibility hint (previous null: off)
+    uint32_t subsurface;    // kRenderFlagSubsurface with a subsurface table: the walk at SSS vertices
 };
 
 // Whether the sun stream (nee_rng) is carried from vertex to vertex: the sun's NEE or the bounce NEE
@@ -520,6 +521,257 @@ CX_DEV ShadeFull ShadeVoxel(const SceneView& s, uint32_t block, V3 hit_position,
     sh.roughness = 1.0f - m.smoothness;
     return sh;
 }
+// ---- subsurface scattering (shaders/render/caelix_subsurface.hlsl, docs/SUBSURFACE.md) ---------------
+constexpr float kSubsurfaceMaxAnisotropy = 0.99f;
+constexpr float kSubsurfaceIsotropicG = 1e-3f;
+constexpr uint32_t kSubsurfaceNone = 0;  // not an SSS vertex (or the last vertex): the shade stands
+constexpr uint32_t kSubsurfaceBsdf = 1;  // an SSS material whose draw chose the surface BSDF
+constexpr uint32_t kSubsurfaceExit = 2;  // the walk left the medium: the shade is the exit vertex
+constexpr uint32_t kSubsurfaceDead = 3;  // the walk reached its step cap: the path ends at this vertex
+
+// SubsurfaceVoxel (caelix_scene_trace.hlsl, CAELIX_SUBSURFACE): an opaque block whose code's entry has
+// model HG, probability > 0 and steps > 0; its walk albedo 1 - one_minus_albedo.
+CX_DEV bool SubsurfaceVoxel(const SceneView& s, uint32_t block, V3& albedo)
+{
+    if (!BlockOpaque(block, s.encoding))
+    {
+        return false;
+    }
+    const CxSubsurface& e = s.subsurface[BlockCode(block, s.encoding)];
+    if (e.model != CX_SSS_MODEL_HG || !(e.probability > 0.0f) || (e.steps & 0xFFu) == 0u)
+    {
+        return false;
+    }
+    albedo = Make(1.0f - e.one_minus_albedo[0], 1.0f - e.one_minus_albedo[1], 1.0f - e.one_minus_albedo[2]);
+    return true;
+}
+
+// SampleHenyeyGreenstein (:62-85): the HG cosine by inversion (|g| < 1e-3: the uniform sphere), the
+// azimuth around w in the Duff et al. basis of OrthonormalBasis (caelix_shading.hlsl:454-461). Two numbers.
+CX_DEV V3 SampleHenyeyGreenstein(V3 w, float g, uint32_t& rng)
+{
+    const float u1 = RandomFloat01(rng);
+    const float u2 = RandomFloat01(rng);
+    float cos_theta;
+    if (fabsf(g) < kSubsurfaceIsotropicG)
+    {
+        cos_theta = 1.0f - 2.0f * u1;
+    }
+    else
+    {
+        const float sq = Div(1.0f - g * g, 1.0f - g + 2.0f * g * u1);
+        cos_theta = Div(1.0f + g * g - sq * sq, 2.0f * g);
+    }
+    cos_theta = fminf(fmaxf(cos_theta, -1.0f), 1.0f);
+    const float sin_theta = Sqrt(fmaxf(1.0f - cos_theta * cos_theta, 0.0f));
+    const float phi = kTwoPi * u2;
+    const float sign_z = w.z >= 0.0f ? 1.0f : -1.0f;
+    const float p = Div(-1.0f, sign_z + w.z);
+    const float q = w.x * w.y * p;
+    const V3 t = Make(1.0f + sign_z * w.x * w.x * p, sign_z * q, -sign_z * w.x);
+    const V3 b = Make(q, sign_z + w.y * w.y * p, -w.y);
+    float sp, cp;
+    SinCos(phi, sp, cp);
+    return Add(Add(Mul(t, cp * sin_theta), Mul(b, sp * sin_theta)), Mul(w, cos_theta));
+}
+
+// The voxel cursor of the walk (caelix_voxel_cursor.hlsl): a voxel of the scene's grid (world voxel
+// coordinates: the one entity sits at the identity frame) and its brick record, kHashNone for a brick
+// that is not resident (its voxels are air).
+struct VoxelCursor
+{
+    int32_t v[3];
+    uint32_t record;
+};
+
+// VoxelCursorFromHit: the voxel a hit's attrib names in the brick `record`.
+CX_DEV VoxelCursor CursorFromHit(const SceneView& s, uint32_t record, uint32_t attrib)
+{
+    const int32_t* k = BrickXyz(s, record);
+    const uint32_t cell = (attrib >> 6) & 0x1FFu;
+    VoxelCursor c;
+    c.v[0] = k[0] * 8 + int32_t(cell & 7u);
+    c.v[1] = k[1] * 8 + int32_t((cell >> 3) & 7u);
+    c.v[2] = k[2] * 8 + int32_t(cell >> 6);
+    c.record = record;
+    return c;
+}
+
+// VoxelCursorStep: the cursor moved by (dx, dy, dz), each in -1..1; a new brick is found through the
+// hash table.
+CX_DEV void CursorStep(const SceneView& s, uint32_t scene, VoxelCursor& c, int32_t dx, int32_t dy, int32_t dz)
+{
+    const int32_t n[3] = {c.v[0] + dx, c.v[1] + dy, c.v[2] + dz};
+    if ((n[0] >> 3) != (c.v[0] >> 3) || (n[1] >> 3) != (c.v[1] >> 3) || (n[2] >> 3) != (c.v[2] >> 3))
+    {
+        c.record = HashGet(s.slots, s.slot_count, s.pow2, s.hash_info[0],
+                           SceneBrickFingerprint(s, scene, n[0] >> 3, n[1] >> 3, n[2] >> 3));
+    }
+    c.v[0] = n[0];
+    c.v[1] = n[1];
+    c.v[2] = n[2];
+}
+
+// VoxelCursorBlock: the cursor voxel's block value (0, air, in a brick that is not resident).
+CX_DEV uint32_t CursorBlock(const SceneView& s, const VoxelCursor& c)
+{
+    if (c.record == kHashNone)
+    {
+        return 0u;
+    }
+    return s.payload[uint64_t(c.record) * 512 + uint32_t(c.v[0] & 7) + 8u * uint32_t(c.v[1] & 7) + 64u * uint32_t(c.v[2] & 7)];
+}
+
+// SubsurfaceWalk (:95-190): from the SSS face hit (record, attrib, hit point `hit_point`) of a ray with
+// direction `ray_dir`, with the entry material's g, step cap and albedo. Entry jitter: the voxel of
+// hit_point - 0.5 n + a uniform offset in [-0.5, 0.5] on the face's tangent axes, reached by one cursor
+// step of at most one voxel per tangent axis, if it is an SSS voxel (else the hit voxel). Per step: the
+// weight takes the current voxel's albedo, the direction a new HG draw, then the face neighbour along
+// its dominant axis (ties x, y, z): an SSS voxel is entered, another opaque voxel is a wall (the cursor
+// stays), anything else is the exit: a uniform point on that face of the cursor voxel, pushed off along
+// the face normal, a cosine direction around it. False at the step cap (the walk is absorbed). Two
+// numbers for the jitter, two per step, four at the exit.
+CX_DEV bool SubsurfaceWalk(const SceneView& s, uint32_t scene, uint32_t record, uint32_t attrib, V3 hit_point,
+                           V3 ray_dir, float g, uint32_t steps, V3 entry_albedo, uint32_t& rng, V3& origin, V3& normal,
+                           V3& direction, V3& weight)
+{
+    origin = Make(0.0f, 0.0f, 0.0f);
+    normal = Make(0.0f, 0.0f, 0.0f);
+    direction = Make(0.0f, 0.0f, 0.0f);
+    weight = Make(0.0f, 0.0f, 0.0f);
+    VoxelCursor c = CursorFromHit(s, record, attrib);
+    V3 a = entry_albedo;
+    {
+        const V3 fn = FaceNormal(attrib);
+        const float u1 = RandomFloat01(rng) - 0.5f;
+        const float u2 = RandomFloat01(rng) - 0.5f;
+        const V3 offset = fn.x != 0.0f ? Make(0.0f, u1, u2) : (fn.y != 0.0f ? Make(u1, 0.0f, u2) : Make(u1, u2, 0.0f));
+        const V3 target = Add(Sub(hit_point, Mul(fn, 0.5f)), offset);
+        const int32_t tv[3] = {int32_t(floorf(target.x)), int32_t(floorf(target.y)), int32_t(floorf(target.z))};
+        const float fv[3] = {fn.x, fn.y, fn.z};
+        int32_t delta[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+        {
+            delta[k] = fv[k] == 0.0f ? min(max(tv[k] - c.v[k], -1), 1) : 0;
+        }
+        if (delta[0] != 0 || delta[1] != 0 || delta[2] != 0)
+        {
+            VoxelCursor start = c;
+            CursorStep(s, scene, start, delta[0], delta[1], delta[2]);
+            V3 start_albedo;
+            if (SubsurfaceVoxel(s, CursorBlock(s, start), start_albedo))
+            {
+                c = start;
+                a = start_albedo;
+            }
+        }
+    }
+    V3 d = ray_dir;
+    V3 w = Make(1.0f, 1.0f, 1.0f);
+    for (uint32_t i = 0; i < steps; ++i)
+    {
+        w = MulV(w, a);
+        d = SampleHenyeyGreenstein(d, g, rng);
+        const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+        int32_t dx = 0, dy = 0, dz = 0;
+        if (ax >= ay && ax >= az)
+        {
+            dx = d.x >= 0.0f ? 1 : -1;
+        }
+        else if (ay >= az)
+        {
+            dy = d.y >= 0.0f ? 1 : -1;
+        }
+        else
+        {
+            dz = d.z >= 0.0f ? 1 : -1;
+        }
+        VoxelCursor next = c;
+        CursorStep(s, scene, next, dx, dy, dz);
+        const uint32_t next_block = CursorBlock(s, next);
+        if (BlockOpaque(next_block, s.encoding))
+        {
+            V3 next_albedo;
+            if (SubsurfaceVoxel(s, next_block, next_albedo))
+            {
+                c = next;
+                a = next_albedo;
+            }
+            continue;
+        }
+        const V3 n = Make(float(dx), float(dy), float(dz));
+        const float u1 = RandomFloat01(rng) - 0.5f;
+        const float u2 = RandomFloat01(rng) - 0.5f;
+        const V3 offset = dx != 0 ? Make(0.0f, u1, u2) : (dy != 0 ? Make(u1, 0.0f, u2) : Make(u1, u2, 0.0f));
+        const V3 exit_point =
+            Add(Add(Add(Make(float(c.v[0]), float(c.v[1]), float(c.v[2])), Make(0.5f, 0.5f, 0.5f)), Mul(n, 0.5f)), offset);
+        normal = n;
+        origin = Add(exit_point, Mul(normal, kPushOff));
+        direction = SafeNormalize(Add(normal, RandomUnitVector(rng)));
+        direction = Dot(direction, direction) > 0.0f ? direction : normal;
+        weight = w;
+        return true;
+    }
+    return false;
+}
+
+// SubsurfaceVertex (:192-250): the subsurface choice at a vertex (the hit `record`, `attrib`, hit point,
+// ray direction) after ShadeVoxel gave `shade`. Not an SSS vertex (last_vertex: no next segment; a
+// transparent block; the entry's model, probability or steps): kSubsurfaceNone, nothing drawn.
+// Otherwise guide = lerp(albedo, sss_albedo, p) and one number u: u >= p keeps the shade (Bsdf); else
+// the walk: its exit replaces the vertex (Exit: albedo = the walk's weight, the exit's bounce origin,
+// direction and normal, the diffuse lobe, roughness 1) or the walk died (Dead: albedo 0).
+CX_DEV uint32_t SubsurfaceVertex(const SceneView& s, uint32_t scene, uint32_t record, uint32_t attrib, V3 hit_point,
+                                 V3 ray_dir, uint32_t& rng, ShadeFull& shade, V3& vertex_normal, V3& guide_albedo,
+                                 bool last_vertex)
+{
+    guide_albedo = shade.albedo;
+    const uint32_t block = (attrib >> 16) & 0xFFFFu;
+    if (last_vertex || !BlockOpaque(block, s.encoding))
+    {
+        return kSubsurfaceNone;
+    }
+    const CxSubsurface& e = s.subsurface[BlockCode(block, s.encoding)];
+    const float probability = e.probability;
+    if (e.model != CX_SSS_MODEL_HG || !(probability > 0.0f))
+    {
+        return kSubsurfaceNone;
+    }
+    const uint32_t steps = e.steps & 0xFFu;
+    if (steps == 0u)
+    {
+        return kSubsurfaceNone;
+    }
+    const V3 entry_albedo = Make(1.0f - e.one_minus_albedo[0], 1.0f - e.one_minus_albedo[1], 1.0f - e.one_minus_albedo[2]);
+    guide_albedo = Lerp(shade.albedo, entry_albedo, probability);
+    if (!(RandomFloat01(rng) < probability))
+    {
+        return kSubsurfaceBsdf;
+    }
+    const float g = fminf(fmaxf(e.anisotropy, -kSubsurfaceMaxAnisotropy), kSubsurfaceMaxAnisotropy);
+    V3 origin, normal, direction, weight;
+    if (!SubsurfaceWalk(s, scene, record, attrib, hit_point, ray_dir, g, steps, entry_albedo, rng, origin, normal,
+                        direction, weight))
+    {
+        shade.albedo = Make(0.0f, 0.0f, 0.0f);
+        return kSubsurfaceDead;
+    }
+    shade.albedo = weight;
+    shade.bounce_origin = origin;
+    shade.bounce_direction = direction;
+    shade.specular = false;
+    shade.roughness = 1.0f;
+    vertex_normal = normal;
+    return kSubsurfaceExit;
+}
+
+// The face index (FaceDirection) of an axis-aligned unit normal.
+CX_DEV uint32_t FaceOfNormal(V3 n)
+{
+    return n.x != 0.0f ? (n.x > 0.0f ? 0u : 1u) : (n.y != 0.0f ? (n.y > 0.0f ? 2u : 3u) : (n.z > 0.0f ? 4u : 5u));
+}
+
 // TraverseSegment (caelix_shading.hlsl:150-175): in the air the fog; inside a medium the throughput
 // keeps exp(-(1 - albedo) t extinction) per channel.
 CX_DEV void MediumSegment(const SceneView& s, const TraceParams& p, uint32_t medium, float t, V3& throughput,

@@ -39,6 +39,7 @@ SceneView SceneOf(const CxScene& c)
     s.scene_cells = uint64_t(c.grid_dim) * c.grid_dim * c.grid_dim;
     s.scene_supers = s.scene_cells / 512;
     s.group_of_brick = c.group_of_brick;
+    s.subsurface = c.subsurface;
     return s;
 }
 
@@ -672,14 +673,38 @@ CX_DEV void PrimaryGlass(const SceneView& s, const TraceParams& prm, const Works
         return;
     }
     const uint32_t face = FaceIndex(surface.attrib);
-    const V3 normal = FaceDirection(face);
-    const ShadeFull sh = ShadeVoxel(s, (surface.attrib >> 16) & 0xFFFFu, Add(ray_o, Mul(ray_d, surface.t)), ray_d, normal,
-                                    rng, medium);
+    V3 normal = FaceDirection(face);
+    const V3 hit_point = Add(ray_o, Mul(ray_d, surface.t));
+    ShadeFull sh = ShadeVoxel(s, (surface.attrib >> 16) & 0xFFFFu, hit_point, ray_d, normal, rng, medium);
+    // Subsurface (caelix_path_trace.hlsl:757-781): the walk may replace the vertex; the primary albedo
+    // output is then the guide albedo and the branch weight over it the throughput.
+    V3 throughput = Make(1.0f, 1.0f, 1.0f);
+    uint32_t walk = kSubsurfaceNone;
+    if (prm.subsurface != 0u)
+    {
+        V3 guide;
+        walk = SubsurfaceVertex(s, scene, surface.record, surface.attrib, hit_point, ray_d, rng, sh, normal, guide, false);
+        if (walk != kSubsurfaceNone)
+        {
+            throughput = Make(Div(sh.albedo.x, fmaxf(guide.x, 0.001f)), Div(sh.albedo.y, fmaxf(guide.y, 0.001f)),
+                              Div(sh.albedo.z, fmaxf(guide.z, 0.001f)));
+            sh.albedo = guide;
+            if (walk == kSubsurfaceExit)
+            {
+                medium = 0u;  // the exit vertex lies in the air
+            }
+        }
+    }
+    const uint32_t vertex_face = walk == kSubsurfaceExit ? FaceOfNormal(normal) : face;
     const V3 emission = Mul(sh.emission, BrickScale(s, surface.record));
     const V3 albedo = MulV(chain, sh.albedo);
     const V3 deterministic = Add(MulV(chain, emission), inscatter);
     w.albedo[path] = make_float4(albedo.x, albedo.y, albedo.z, t_virtual);
     w.emission[path] = make_float4(deterministic.x, deterministic.y, deterministic.z, __uint_as_float(geometry));
+    if (walk == kSubsurfaceDead)
+    {
+        return;  // no NEE, no segment: the path ends with its deterministic emission
+    }
     uint32_t segment_light = 0u;
     bool sees_sun = true;
     bool claim = false;
@@ -699,7 +724,7 @@ CX_DEV void PrimaryGlass(const SceneView& s, const TraceParams& prm, const Works
                     if (shadow_t > 0.0f)
                     {
                         const V3 tr = ShadowTransmissive(s, sh.bounce_origin, shadow_dir, shadow_t, medium, c, scene);
-                        const V3 add = MulV(visible, tr);
+                        const V3 add = MulV(MulV(visible, tr), throughput);
                         w.radiance[path] = make_float4(add.x, add.y, add.z, 0.0f);
                         seen = tr.x > 0.0f || tr.y > 0.0f || tr.z > 0.0f ? 1u : 0u;
                     }
@@ -708,7 +733,7 @@ CX_DEV void PrimaryGlass(const SceneView& s, const TraceParams& prm, const Works
                 {
                     shadow = shadow_t > 0.0f;
                     shadow_origin = sh.bounce_origin;
-                    shadow_add = visible;
+                    shadow_add = MulV(visible, throughput);
                 }
             }
             segment_light = kSegmentLampNee | mis_bit;
@@ -717,8 +742,8 @@ CX_DEV void PrimaryGlass(const SceneView& s, const TraceParams& prm, const Works
         }
         go_on = true;
     }
-    w.throughput[path] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
-    w.state[path] = make_uint4(rng, light_rng, PackState(segment_light, 1u, face, medium, sees_sun, claim), 0u);
+    w.throughput[path] = make_float4(throughput.x, throughput.y, throughput.z, 1.0f);
+    w.state[path] = make_uint4(rng, light_rng, PackState(segment_light, 1u, vertex_face, medium, sees_sun, claim), 0u);
     if (KeepsNeeStream(prm))
     {
         w.nee_rng[path] = nee_rng;
@@ -959,11 +984,20 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
             }
             const uint32_t face = FaceIndex(h.y);
             const uint32_t block = (h.y >> 16) & 0xFFFFu;
-            const V3 vertex_normal = FaceDirection(face);
+            V3 vertex_normal = FaceDirection(face);
             ShadeFull sh;
+            uint32_t walk = kSubsurfaceNone;
             if constexpr (Glass)
             {
-                sh = ShadeVoxel(s, block, Add(origin, Mul(dir, hit.t)), dir, vertex_normal, rng, medium);
+                const V3 hit_point = Add(origin, Mul(dir, hit.t));
+                sh = ShadeVoxel(s, block, hit_point, dir, vertex_normal, rng, medium);
+                if (prm.subsurface != 0u)
+                {
+                    // caelix_path_trace.hlsl:861-867: a vertex without a next segment skips the walk
+                    V3 guide;
+                    walk = SubsurfaceVertex(s, views.SceneOf(path), h.z, h.y, hit_point, dir, rng, sh, vertex_normal,
+                                            guide, bounce_index + 1u > prm.bounces);
+                }
             }
             else
             {
@@ -988,6 +1022,12 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
                 const V3 product = AddAlbedoTerms(s, lds_grad, adj, base, verts, VertexCount(verts));
                 ScaleGradAdd(g, h.z, adj, MulV(Mul(sh.emission, emission_weight * roulette), product));
             }
+            // A dead walk ends the path after the hit's emission; an exit vertex lies in the air.
+            const bool dead = walk == kSubsurfaceDead;
+            if (walk == kSubsurfaceExit)
+            {
+                medium = 0u;
+            }
             throughput = MulV(throughput, Make(Div(sh.albedo.x, 1.0f), Div(sh.albedo.y, 1.0f), Div(sh.albedo.z, 1.0f)));
             verts = AppendVertex(verts, block & 0xFFu);
             segment_light = 0u;
@@ -1006,7 +1046,7 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
                     claim = false;
                 }
             }
-            if (!sh.specular && bounce_index <= prm.bounces)
+            if (!sh.specular && bounce_index <= prm.bounces && !dead)
             {
                 segment_light = kSegmentLampNee | (L.mis != 0u ? kSegmentLampMis : 0u);
                 sees_sun = prm.sky.sun == 0u;
@@ -1025,7 +1065,7 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
             }
             const float cp = fmaxf(throughput.x, fmaxf(throughput.y, throughput.z));
             const float u = RandomFloat01(rng);
-            const bool survive = prm.roulette != 0u ? !(cp <= 0.0f || cp < u) : !(cp <= 0.0f);
+            const bool survive = !dead && (prm.roulette != 0u ? !(cp <= 0.0f || cp < u) : !(cp <= 0.0f));
             if (survive)
             {
                 if (prm.roulette != 0u)
@@ -1043,7 +1083,9 @@ __global__ void __launch_bounds__(kWaveThreads) BounceShadeKernel(SceneView s, V
             w.radiance[path] = make_float4(radiance.x, radiance.y, radiance.z, 0.0f);
             w.throughput[path] = make_float4(throughput.x, throughput.y, throughput.z, roulette);
             w.state[path] = make_uint4(rng, light_rng,
-                                       Glass ? PackState(segment_light, bounce_index, face, medium, sees_sun, claim)
+                                       Glass ? PackState(segment_light, bounce_index,
+                                                         walk == kSubsurfaceExit ? FaceOfNormal(vertex_normal) : face,
+                                                         medium, sees_sun, claim)
                                              : PackState(segment_light, bounce_index, face),
                                        verts);
             if (KeepsNeeStream(prm))
@@ -1574,7 +1616,9 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
 {
     const bool adjoint = g.d_color != nullptr;
     // The full material model: the scene's extended material table (or a delta chain) asks for it.
-    const bool glass = sv.materials_ext != nullptr || params->bounces_transparent > 0u ||
+    // Subsurface scattering: the flag with a subsurface table (the walk lives in the full model's kernels).
+    const bool subsurface = (params->flags & CX_TRACE_SUBSURFACE) != 0 && sv.subsurface != nullptr;
+    const bool glass = subsurface || sv.materials_ext != nullptr || params->bounces_transparent > 0u ||
                        (params->flags & CX_TRACE_SHADOW_TRANSMISSION) != 0;
     const uint32_t blocks = DivUp(paths, kWaveThreads);
     const uint32_t rows = GradRows(paths);
@@ -1609,7 +1653,8 @@ void RecordFrame(const SceneView& sv, const Views& views, uint32_t paths, const 
                       V3{params->fog_color[0], params->fog_color[1], params->fog_color[2]}, params->fog_density,
                       params->bounces_transparent, (params->flags & CX_TRACE_SHADOW_TRANSMISSION) != 0 ? 1u : 0u,
                       (params->flags & CX_TRACE_DELTA_CHECKERBOARD) != 0 ? 1u : 0u,
-                      params->bounce_nee_probability > 0.0f ? params->bounce_nee_probability : 1.0f, HintParams{}};
+                      params->bounce_nee_probability > 0.0f ? params->bounce_nee_probability : 1.0f, HintParams{},
+                      subsurface ? 1u : 0u};
         if (cube != nullptr)
         {
             p.sky.sh = w.sky_sh;
@@ -1717,7 +1762,7 @@ int CheckParams(const CxTraceParams* params, const CxTraceOutput* out, uint32_t 
         return ArgError("%s: at most 4 bounces (3 albedo vertices per path)", what);
     }
     if ((params->flags & ~(CX_TRACE_NO_RUSSIAN_ROULETTE | CX_TRACE_SKY | CX_TRACE_NO_SUN_NEE | CX_TRACE_SHADOW_TRANSMISSION |
-                           CX_TRACE_DELTA_CHECKERBOARD)) != 0)
+                           CX_TRACE_DELTA_CHECKERBOARD | CX_TRACE_SUBSURFACE)) != 0)
     {
         return ArgError("%s: unknown flags", what);
     }
@@ -1830,6 +1875,11 @@ int RunFrame(const CxScene* scene, const CxLightTreeView* lights, const CxCamera
     if (grads != nullptr && scene->materials_ext != nullptr)
     {
         Log(3, "%s: the adjoint shades opaque Lambertian voxels only (materials_ext set)", what);
+        return CX_ERROR_UNSUPPORTED;
+    }
+    if (grads != nullptr && (params->flags & CX_TRACE_SUBSURFACE) != 0 && scene->subsurface != nullptr)
+    {
+        Log(3, "%s: the adjoint has no subsurface walk (CX_TRACE_SUBSURFACE)", what);
         return CX_ERROR_UNSUPPORTED;
     }
     if (uint64_t(camera->width) * camera->height > (uint64_t(1) << 28))

@@ -11,6 +11,7 @@
 // polynomials (S. Moshier, Cephes Math Library, sinf.c / cosf.c) with Cody-Waite reduction, not the
 // device library's.
 #pragma once
+#include "caelix_amd/caelix_amd.h"
 #include "device_common.h"
 #include "hash_device.h"
 
@@ -42,6 +43,7 @@ struct SceneView
     uint64_t scene_cells;        // brick-bitmap bits per scene (grid_dim^3)
     uint64_t scene_supers;       // super-bitmap bits per scene ((grid_dim / 8)^3)
     const uint32_t* group_of_brick;  // [bricks] group index (the light tree's group rows and infos)
+    const CxSubsurface* subsurface;  // [65536] subsurface entries, or null (no SSS material)
 };
 
 // The (x, y, z) brick coordinates of a record.

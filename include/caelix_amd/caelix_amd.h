@@ -260,7 +260,25 @@ typedef struct CxScene {
      * when the old or the new count is not 0 (caelix_finalize.hlsl:22-29, 326-340). New bricks start with
      * word 1 = 0. NULL: those bits are 0. */
     GroupLightRow* light_group_rows;
+    /* [65536] CxSubsurface per material code, or NULL (no SSS material): read by cx_trace_frame with
+     * CX_TRACE_SUBSURFACE. */
+    const struct CxSubsurface* subsurface;
 } CxScene;
+
+/* The subsurface entry of a material code (the reference's CaelixMaterial sss_* fields as its GPU
+ * subsurface entry holds them, docs/MATERIALS.md "Subsurface entry" w0..w3, the halves widened to
+ * float): an SSS voxel is an opaque block whose code has model CX_SSS_MODEL_HG, probability > 0 and
+ * steps > 0; the walk's per-step albedo is 1 - one_minus_albedo. 32 bytes. */
+typedef struct CxSubsurface {
+    float one_minus_albedo[3]; /* 1 - sss_albedo (the entry's w0, w1 halves) */
+    float anisotropy;          /* sss_anisotropy g in [-1, 1] (the walk clamps it to [-0.99, 0.99]) */
+    float probability;         /* sss_probability p in [0, 1] */
+    uint32_t model;            /* CX_SSS_MODEL_NONE or CX_SSS_MODEL_HG; other values act as none */
+    uint32_t steps;            /* sss_steps, 0..255 (0: never entered) */
+    uint32_t pad;
+} CxSubsurface;
+#define CX_SSS_MODEL_NONE 0u
+#define CX_SSS_MODEL_HG 1u
 
 /* The record producer (shaders/scene/caelix_finalize.hlsl:112-361) over the bricks and their face
  * neighbours (found through the hash table: slots and hash_info are read): face bits of transparent and
@@ -447,6 +465,17 @@ typedef struct CxLightHint {
  * parity (px + py even refracts, odd reflects) with the weight 2 (1 - F) or 2 F and draws no random
  * number; total internal reflection still reflects with weight 1. */
 #define CX_TRACE_DELTA_CHECKERBOARD 16u
+
+/* Subsurface scattering (kRenderFlagSubsurface; shaders/render/caelix_subsurface.hlsl, docs/SUBSURFACE.md):
+ * with CxScene::subsurface set, a path vertex on an SSS voxel (the primary surface, or a bounce hit that
+ * has a next segment) draws u from the BSDF stream: u >= p keeps the surface shade; else a
+ * Henyey-Greenstein lattice walk through the SSS voxels (entry jitter on the face, per step the voxel's
+ * albedo and a new direction, then the face neighbour along its dominant axis: an SSS voxel is entered,
+ * another opaque voxel is a wall, anything else is the exit) replaces the vertex by a diffuse vertex on
+ * the exit face with the walk's weight as its albedo; a walk that reaches the entry material's step cap
+ * ends the path (dead). The primary vertex's albedo output is then lerp(albedo, sss_albedo, p) and the
+ * branch weight over it its throughput. Implies the full material model. */
+#define CX_TRACE_SUBSURFACE 32u
 
 /* No Russian roulette: paths run all `bounces` (the roulette's random number is still drawn, so the
  * other decisions stay the same). The estimator is then a smooth function of the albedos and the

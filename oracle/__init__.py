@@ -129,7 +129,7 @@ class _Scene(ctypes.Structure):
         ("grid_min", ctypes.c_int32 * 3), ("grid_dim", _U32), ("group_shift", _U32), ("encoding", _U32),
         ("materials", _P), ("light_codes", _P), ("emission_scale", ctypes.c_float), ("pad", _U32),
         ("brick_emission_scale", _P), ("scene_count", _U32), ("pad2", _U32), ("occupied_box", _P),
-        ("materials_ext", _P), ("group_of_brick", _P), ("light_group_rows", _P),
+        ("materials_ext", _P), ("group_of_brick", _P), ("light_group_rows", _P), ("subsurface", _P),
     ]
 
 
@@ -214,6 +214,8 @@ def _trace_lib():
         lib.oracle_sky_sh.argtypes = [_P, _U32, _U32, _P]
         lib.oracle_mis_weight.restype = None
         lib.oracle_mis_weight.argtypes = [_P, _U32, _P]
+        lib.oracle_sss_hg.restype = None
+        lib.oracle_sss_hg.argtypes = [_P, _U32, _U32, _P]
         lib.oracle_trace_frame.restype = None
         lib.oracle_trace_frame.argtypes = [S, T, ctypes.POINTER(_Camera), ctypes.POINTER(_Params), _U32, _U32, _P, _P,
                                            _P, _P]
@@ -230,6 +232,15 @@ def mis_weight(queries: np.ndarray) -> np.ndarray:
     q = np.ascontiguousarray(queries, np.float32)
     out = np.zeros((len(q), 2), np.float32)
     _trace_lib().oracle_mis_weight(_ptr(q), len(q), _ptr(out))
+    return out
+
+
+def sss_hg(queries: np.ndarray, seed: int) -> np.ndarray:
+    """SampleHenyeyGreenstein (the subsurface walk's phase function): queries [n, 4] float32 = (w, g),
+    draw i from the stream seeded PcgHash(seed + i) -> [n, 3] float32 directions."""
+    q = np.ascontiguousarray(queries, np.float32)
+    out = np.zeros((len(q), 3), np.float32)
+    _trace_lib().oracle_sss_hg(_ptr(q), len(q), seed, _ptr(out))
     return out
 
 
@@ -548,6 +559,7 @@ class OracleScene:
         s.emission_scale = 1.0
         s.brick_emission_scale = _ptr(a["brick_scale"]) if "brick_scale" in a else None
         s.materials_ext = _ptr(a["materials_ext"]) if "materials_ext" in a else None
+        s.subsurface = _ptr(a["subsurface"]) if "subsurface" in a else None
         s.group_of_brick = _ptr(a["group_of_brick"]) if "group_of_brick" in a else None
         s.light_group_rows = self.lt.rows.ctypes.data if self.lt is not None else None
         if self.lt is not None:
@@ -588,6 +600,8 @@ class OracleScene:
         }
         if getattr(data, "materials_ext", None) is not None:
             a["materials_ext"] = np.ascontiguousarray(data.materials_ext, np.float32)
+        if getattr(data, "subsurface", None) is not None:
+            a["subsurface"] = np.ascontiguousarray(data.subsurface).view(np.uint8).copy()
         lt = LightTables(coords, np.zeros(len(coords), np.uint32), np.eye(3, 4, dtype=np.float32)[None], [0],
                          data.light_codes, shift, enc, heap_units=max(64, n * 8))
         self = cls(a, data.grid_min, dim, shift, mis, enc, lt)

@@ -1355,6 +1355,7 @@ struct Sky
     bool transmissive = false; // kRenderFlagShadowTransmission at the primary vertex
     bool checkerboard = false; // kRenderFlagDeltaCheckerboard: the chain's first interface by pixel parity
     float nee_q = 1.0f;        // bounce_nee_probability (0 is taken as 1)
+    bool subsurface = false;   // kRenderFlagSubsurface with a subsurface table: the walk at SSS vertices
     bool on = false;
     bool sun = false;  // the sun's NEE: `on`, kRenderFlagSunNee, a sun radiance above 0 (caelix_path_trace.hlsl:742)
     V sun_dir, sun_radiance, zenith, horizon, ground;
@@ -1801,6 +1802,232 @@ ShadeFull ShadeVoxel(const CxScene& s, uint32_t block, V hp, V dir, V normal, ui
     sh.roughness = 1.0f - m.smoothness;
     return sh;
 }
+// ---- subsurface scattering (shaders/render/caelix_subsurface.hlsl, docs/SUBSURFACE.md) ---------------
+// Parity unpinned by the reference's own numbers only where no closed form applies: the walk's
+// decisions are pinned by the furnace (a white-sky SSS cube returns 1 with albedo-1 steps), the step
+// cap's dead fraction, the HG mean cosine g and the reference's lifecycle checks (tests/test_sss_oracle.py).
+enum SssState : uint32_t
+{
+    kSssNone = 0,  // not an SSS vertex (or the last vertex)
+    kSssBsdf = 1,  // the draw kept the surface BSDF
+    kSssExit = 2,  // the walk left: the vertex is its exit
+    kSssDead = 3,  // the walk reached the step cap
+};
+
+// SubsurfaceVoxel: opaque, its code's entry HG with p > 0 and steps > 0; albedo 1 - one_minus_albedo.
+bool SssVoxel(const CxScene& s, uint32_t block, V& albedo)
+{
+    if (!Opaque(block, s.encoding))
+    {
+        return false;
+    }
+    const CxSubsurface& e = s.subsurface[MaterialCode(block, s.encoding)];
+    if (e.model != CX_SSS_MODEL_HG || !(e.probability > 0.0f) || (e.steps & 0xFFu) == 0u)
+    {
+        return false;
+    }
+    albedo = Mk(1.0f - e.one_minus_albedo[0], 1.0f - e.one_minus_albedo[1], 1.0f - e.one_minus_albedo[2]);
+    return true;
+}
+
+// SampleHenyeyGreenstein (caelix_subsurface.hlsl:62-85), the basis of OrthonormalBasis
+// (caelix_shading.hlsl:454-461).
+V SampleHg(V w, float g, uint32_t& rng)
+{
+    const float u1 = RandomFloat01(rng);
+    const float u2 = RandomFloat01(rng);
+    float ct;
+    if (std::fabs(g) < 1e-3f)
+    {
+        ct = 1.0f - 2.0f * u1;
+    }
+    else
+    {
+        const float sq = (1.0f - g * g) / (1.0f - g + 2.0f * g * u1);
+        ct = (1.0f + g * g - sq * sq) / (2.0f * g);
+    }
+    ct = std::fmin(std::fmax(ct, -1.0f), 1.0f);
+    const float st = std::sqrt(std::fmax(1.0f - ct * ct, 0.0f));
+    const float phi = kTwoPi * u2;
+    const float sign_z = w.z >= 0.0f ? 1.0f : -1.0f;
+    const float pz = -1.0f / (sign_z + w.z);
+    const float q = w.x * w.y * pz;
+    const V t = Mk(1.0f + sign_z * w.x * w.x * pz, sign_z * q, -sign_z * w.x);
+    const V b = Mk(q, sign_z + w.y * w.y * pz, -w.y);
+    float sp, cp;
+    SinCos(phi, sp, cp);
+    return Add(Add(Mul(t, cp * st), Mul(b, sp * st)), Mul(w, ct));
+}
+
+// The voxel under the walk: world voxel coordinates and its brick record (kNoneU: not resident, air).
+struct Cursor
+{
+    int32_t v[3];
+    uint32_t record;
+};
+
+int32_t BrickOf(int32_t v)
+{
+    return v >= 0 ? v / 8 : -((7 - v) / 8);
+}
+
+uint32_t CursorValue(const CxScene& s, const Cursor& c)
+{
+    if (c.record == kNoneU)
+    {
+        return 0u;
+    }
+    const int32_t lx = c.v[0] - 8 * BrickOf(c.v[0]);
+    const int32_t ly = c.v[1] - 8 * BrickOf(c.v[1]);
+    const int32_t lz = c.v[2] - 8 * BrickOf(c.v[2]);
+    return s.payload[uint64_t(c.record) * 512 + uint32_t(lx + 8 * ly + 64 * lz)];
+}
+
+Cursor CursorMoved(const CxScene& s, Cursor c, const int32_t d[3])
+{
+    Cursor n = c;
+    bool other = false;
+    for (int a = 0; a < 3; ++a)
+    {
+        n.v[a] = c.v[a] + d[a];
+        other = other || BrickOf(n.v[a]) != BrickOf(c.v[a]);
+    }
+    if (other)
+    {
+        uint32_t probes;
+        n.record = HashGet(s, BrickOf(n.v[0]), BrickOf(n.v[1]), BrickOf(n.v[2]), probes);
+    }
+    return n;
+}
+
+// SubsurfaceWalk (caelix_subsurface.hlsl:95-190) from the hit (record, attrib, point hp) of a ray along
+// `dir`: the entry jitter, then per step albedo, an HG direction and the dominant-axis neighbour (an SSS
+// voxel is entered, another opaque voxel is a wall, anything else is the exit face). False at the cap.
+bool SssWalk(const CxScene& s, uint32_t record, uint32_t attrib, V hp, V dir, float g, uint32_t steps, V entry_albedo,
+             uint32_t& rng, V& origin, V& normal, V& direction, V& weight)
+{
+    origin = normal = direction = weight = Mk(0, 0, 0);
+    const uint32_t cell = (attrib >> 6) & 0x1FFu;
+    Cursor c;
+    c.v[0] = s.keys[3 * uint64_t(record)] * 8 + int32_t(cell & 7u);
+    c.v[1] = s.keys[3 * uint64_t(record) + 1] * 8 + int32_t((cell >> 3) & 7u);
+    c.v[2] = s.keys[3 * uint64_t(record) + 2] * 8 + int32_t(cell >> 6);
+    c.record = record;
+    V a = entry_albedo;
+    {
+        const V fn = FaceDirection(LowBit(attrib & 0x3Fu));
+        const float u1 = RandomFloat01(rng) - 0.5f;
+        const float u2 = RandomFloat01(rng) - 0.5f;
+        const V off = fn.x != 0.0f ? Mk(0, u1, u2) : (fn.y != 0.0f ? Mk(u1, 0, u2) : Mk(u1, u2, 0));
+        const V target = Add(Sub(hp, Mul(fn, 0.5f)), off);
+        int32_t d[3];
+        for (int k = 0; k < 3; ++k)
+        {
+            const int32_t tv = int32_t(std::floor(Comp(target, k)));
+            d[k] = Comp(fn, k) == 0.0f ? std::min(std::max(tv - c.v[k], -1), 1) : 0;
+        }
+        if (d[0] != 0 || d[1] != 0 || d[2] != 0)
+        {
+            const Cursor start = CursorMoved(s, c, d);
+            V sa;
+            if (SssVoxel(s, CursorValue(s, start), sa))
+            {
+                c = start;
+                a = sa;
+            }
+        }
+    }
+    V w = Mk(1, 1, 1);
+    V dd = dir;
+    for (uint32_t i = 0; i < steps; ++i)
+    {
+        w = MulV(w, a);
+        dd = SampleHg(dd, g, rng);
+        const float ax = std::fabs(dd.x), ay = std::fabs(dd.y), az = std::fabs(dd.z);
+        int32_t d[3] = {0, 0, 0};
+        if (ax >= ay && ax >= az)
+        {
+            d[0] = dd.x >= 0.0f ? 1 : -1;
+        }
+        else if (ay >= az)
+        {
+            d[1] = dd.y >= 0.0f ? 1 : -1;
+        }
+        else
+        {
+            d[2] = dd.z >= 0.0f ? 1 : -1;
+        }
+        const Cursor next = CursorMoved(s, c, d);
+        const uint32_t nb = CursorValue(s, next);
+        if (Opaque(nb, s.encoding))
+        {
+            V na;
+            if (SssVoxel(s, nb, na))
+            {
+                c = next;
+                a = na;
+            }
+            continue;
+        }
+        const V n = Mk(float(d[0]), float(d[1]), float(d[2]));
+        const float u1 = RandomFloat01(rng) - 0.5f;
+        const float u2 = RandomFloat01(rng) - 0.5f;
+        const V off = d[0] != 0 ? Mk(0, u1, u2) : (d[1] != 0 ? Mk(u1, 0, u2) : Mk(u1, u2, 0));
+        const V exit_point = Add(Add(Add(Mk(float(c.v[0]), float(c.v[1]), float(c.v[2])), Mk(0.5f, 0.5f, 0.5f)),
+                                     Mul(n, 0.5f)), off);
+        normal = n;
+        origin = Add(exit_point, Mul(normal, kPushOff));
+        direction = SafeNormalize(Add(normal, RandomUnitVector(rng)));
+        direction = Dot(direction, direction) > 0.0f ? direction : normal;
+        weight = w;
+        return true;
+    }
+    return false;
+}
+
+// SubsurfaceVertex (caelix_subsurface.hlsl:192-250).
+uint32_t SssVertex(const CxScene& s, uint32_t record, uint32_t attrib, V hp, V dir, uint32_t& rng, V& albedo,
+                   V& origin, V& direction, bool& specular, float& roughness, V& vertex_normal, V& guide, bool last)
+{
+    guide = albedo;
+    const uint32_t block = (attrib >> 16) & 0xFFFFu;
+    if (last || !Opaque(block, s.encoding))
+    {
+        return kSssNone;
+    }
+    const CxSubsurface& e = s.subsurface[MaterialCode(block, s.encoding)];
+    const float p = e.probability;
+    if (e.model != CX_SSS_MODEL_HG || !(p > 0.0f))
+    {
+        return kSssNone;
+    }
+    const uint32_t steps = e.steps & 0xFFu;
+    if (steps == 0u)
+    {
+        return kSssNone;
+    }
+    const V entry_albedo = Mk(1.0f - e.one_minus_albedo[0], 1.0f - e.one_minus_albedo[1], 1.0f - e.one_minus_albedo[2]);
+    guide = Lerp(albedo, entry_albedo, p);
+    if (!(RandomFloat01(rng) < p))
+    {
+        return kSssBsdf;
+    }
+    const float g = std::fmin(std::fmax(e.anisotropy, -0.99f), 0.99f);
+    V o, n, d, wgt;
+    if (!SssWalk(s, record, attrib, hp, dir, g, steps, entry_albedo, rng, o, n, d, wgt))
+    {
+        albedo = Mk(0, 0, 0);
+        return kSssDead;
+    }
+    albedo = wgt;
+    origin = o;
+    direction = d;
+    specular = false;
+    roughness = 1.0f;
+    vertex_normal = n;
+    return kSssExit;
+}
+
 // TraverseSegment (caelix_shading.hlsl:150-175): fog in the air, exp(-(1 - albedo) t extinction)
 // per channel inside a medium.
 void MediumSegment(const CxScene& s, const Sky& k, uint32_t medium, float t, V& throughput, V& radiance)
@@ -2448,12 +2675,35 @@ void TracePixelGlass(const CxScene& s, const Lights& L, const CxCamera& cam, uin
         emission = Add(MulV(chain, sky), inscatter);
         return;
     }
-    const V pn = FaceDirection(LowBit(surface.attrib & 0x3Fu));
-    const ShadeFull ps =
-        ShadeVoxel(s, (surface.attrib >> 16) & 0xFFFFu, Add(ro, Mul(rd, surface.t)), rd, pn, rng, medium);
+    V pn = FaceDirection(LowBit(surface.attrib & 0x3Fu));
+    const V php = Add(ro, Mul(rd, surface.t));
+    ShadeFull ps = ShadeVoxel(s, (surface.attrib >> 16) & 0xFFFFu, php, rd, pn, rng, medium);
+    // Subsurface at the primary vertex (caelix_path_trace.hlsl:757-781): throughput = branch weight /
+    // guide albedo, the albedo output the guide; a dead walk ends the path after the emission.
+    V throughput = Mk(1, 1, 1);
+    uint32_t walk = kSssNone;
+    if (k.subsurface)
+    {
+        V guide;
+        walk = SssVertex(s, surface.record, surface.attrib, php, rd, rng, ps.albedo, ps.origin, ps.direction, ps.specular,
+                         ps.roughness, pn, guide, false);
+        if (walk != kSssNone)
+        {
+            throughput = Mk(ps.albedo.x / std::fmax(guide.x, 0.001f), ps.albedo.y / std::fmax(guide.y, 0.001f),
+                            ps.albedo.z / std::fmax(guide.z, 0.001f));
+            ps.albedo = guide;
+            if (walk == kSssExit)
+            {
+                medium = 0u;
+            }
+        }
+    }
     albedo = MulV(chain, ps.albedo);
     emission = Add(MulV(chain, Mul(ps.emission, BrickScale(s, surface.record))), inscatter);
-    V throughput = Mk(1, 1, 1);
+    if (walk == kSssDead)
+    {
+        return;
+    }
     V segment_normal = pn;
     float segment_rough = ps.specular ? ps.roughness : -1.0f;  // the segment's lobe; -1: diffuse
     uint32_t segment_light = 0u;
@@ -2491,9 +2741,17 @@ void TracePixelGlass(const CxScene& s, const Lights& L, const CxCamera& cam, uin
         V segment_add = Mk(0, 0, 0);
         MediumSegment(s, k, medium, hit.t, throughput, segment_add);
         AddD(radiance, segment_add);
-        const V vn = FaceDirection(LowBit(hit.attrib & 0x3Fu));
+        V vn = FaceDirection(LowBit(hit.attrib & 0x3Fu));
         const uint32_t block = (hit.attrib >> 16) & 0xFFFFu;
-        const ShadeFull sh = ShadeVoxel(s, block, Add(bo, Mul(bd, hit.t)), bd, vn, rng, medium);
+        const V hp = Add(bo, Mul(bd, hit.t));
+        ShadeFull sh = ShadeVoxel(s, block, hp, bd, vn, rng, medium);
+        uint32_t walk = kSssNone;
+        if (k.subsurface)
+        {
+            V guide;
+            walk = SssVertex(s, hit.record, hit.attrib, hp, bd, rng, sh.albedo, sh.origin, sh.direction, sh.specular,
+                             sh.roughness, vn, guide, bounce_index + 1u > bounces);
+        }
         const V scaled_emission = Mul(sh.emission, BrickScale(s, hit.record));
         bounce_index += 1;
         float ew = 1.0f;
@@ -2502,6 +2760,14 @@ void TracePixelGlass(const CxScene& s, const Lights& L, const CxCamera& cam, uin
             ew = LampHitWeight(s, L, segment_light, hit, bo, bd, segment_normal);
         }
         AddD(radiance, MulV(scaled_emission, Mul(throughput, ew)));
+        if (walk == kSssDead)
+        {
+            break;
+        }
+        if (walk == kSssExit)
+        {
+            medium = 0u;
+        }
         throughput = MulV(throughput, Mk(sh.albedo.x / 1.0f, sh.albedo.y / 1.0f, sh.albedo.z / 1.0f));
         segment_light = 0u;
         sees_sun = true;
@@ -3314,6 +3580,21 @@ void oracle_mis_weight(const float* queries, uint32_t count, float* weights)
     }
 }
 
+// SampleHenyeyGreenstein for `count` draws: queries [count][4] = (w xyz, g), stream i seeded with
+// PcgHash(seed + i); out [count][3] the directions.
+void oracle_sss_hg(const float* queries, uint32_t count, uint32_t seed, float* out)
+{
+    for (uint32_t i = 0; i < count; ++i)
+    {
+        const float* q = queries + uint64_t(i) * 4;
+        uint32_t rng = PcgHash(seed + i);
+        const V d = SampleHg(Mk(q[0], q[1], q[2]), q[3], rng);
+        out[3 * uint64_t(i)] = d.x;
+        out[3 * uint64_t(i) + 1] = d.y;
+        out[3 * uint64_t(i) + 2] = d.z;
+    }
+}
+
 // A frame over pixel rows [row_begin, row_end): color (rgb + t, float64 rgb) and radiance. counters:
 // rays (primary + bounce), shadow rays, brick cells visited, hash probes, brick DDA steps, the demand
 // bytes of every global load the kernels make for those pixels, and that total split by kind of work
@@ -3327,7 +3608,8 @@ void oracle_trace_frame(const CxScene* s, const CxLightTreeView* view, const CxC
     uint64_t rays = 0, shadow = 0, cells = 0, probes = 0, steps = 0;
     uint64_t d0 = 0, d1 = 0, d2 = 0, d3 = 0, d4 = 0;
     Sky sky = SkyOf(*prm);
-    sky.glass = s->materials_ext != nullptr || prm->bounces_transparent > 0u ||
+    sky.subsurface = (prm->flags & CX_TRACE_SUBSURFACE) != 0u && s->subsurface != nullptr;
+    sky.glass = sky.subsurface || s->materials_ext != nullptr || prm->bounces_transparent > 0u ||
                 (prm->flags & CX_TRACE_SHADOW_TRANSMISSION) != 0u;
     if (!sky.on)
     {
