@@ -6,7 +6,8 @@
 // (shaders/render/caelix_light_nee.hlsl:20-34) and the path tracer
 // (shaders/render/caelix_path_trace.hlsl:276-452, 595-978): opaque Lambertian voxels, and with the
 // extended material table the full material model (shaders/render/caelix_shading.hlsl:116-307: media,
-// glass, specular lobes, the delta chain, transmissive shadows of caelix_scene_trace.hlsl:401-477);
+// glass, specular lobes, the delta chain, transmissive shadows of caelix_scene_trace.hlsl:401-477),
+// with the subsurface table the Henyey-Greenstein lattice walk (shaders/render/caelix_subsurface.hlsl);
 // the procedural sky and sun, the fog of the air.
 //
 // Every float operation is one IEEE binary32 operation in source order (-ffp-contract=off), as in the
@@ -27,7 +28,11 @@
 //   path tracer       closed forms: the furnace box, the sun-lit floor (L sin^2 r), constant and graded
 //                     skies, the fogged camera segment and the isothermal fogged furnace, the glass
 //                     slab (Fresnel share, Snell path, Beer transmission, depth), sun through a
-//                     floating slab with transmissive shadows; RIS unbiasedness (sun + lamp); beyond
+//                     floating slab with transmissive shadows; RIS unbiasedness (sun + lamp); the
+//                     subsurface walk: the HG cosine distribution (mean g, KS), the reference's furnace
+//                     (a pixel is exactly 1 or 0, the step cap) and lifecycle checks, a one-step closed
+//                     form (start voxel, entry jitter, dominant axis) and the guide albedo / branch
+//                     weights (tests/test_sss_oracle.py); beyond
 //                     these, radiance at matched streams on terrain scenes is checked against this
 //                     oracle only
 #include <algorithm>
@@ -1803,9 +1808,8 @@ ShadeFull ShadeVoxel(const CxScene& s, uint32_t block, V hp, V dir, V normal, ui
     return sh;
 }
 // ---- subsurface scattering (shaders/render/caelix_subsurface.hlsl, docs/SUBSURFACE.md) ---------------
-// Parity unpinned by the reference's own numbers only where no closed form applies: the walk's
-// decisions are pinned by the furnace (a white-sky SSS cube returns 1 with albedo-1 steps), the step
-// cap's dead fraction, the HG mean cosine g and the reference's lifecycle checks (tests/test_sss_oracle.py).
+// Pinned by the reference's furnace and lifecycle checks, the HG cosine distribution, a one-step closed
+// form over the start voxel and the entry jitter, and the branch weights (tests/test_sss_oracle.py).
 enum SssState : uint32_t
 {
     kSssNone = 0,  // not an SSS vertex (or the last vertex)
