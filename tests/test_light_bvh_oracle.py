@@ -68,6 +68,12 @@ def test_oracle_build_matches_independent_checks(oracle, n, seed):
     lo = out["state_rec"]["bounds"][:3]
     extent = np.maximum(out["state_rec"]["bounds"][3:] - lo, np.float32(1e-6))
     q = np.clip((centre - lo) / extent * np.float32(1024.0), 0.0, 1023.0).astype(np.uint32)
+    # the quantisation against float64 arithmetic (the mathematics the float32 formula rounds): equal
+    # cells except where the float64 position lies within 1e-3 of a cell boundary
+    x64 = (centre.astype(np.float64) - lo.astype(np.float64)) / extent.astype(np.float64) * 1024.0
+    q64 = np.clip(np.floor(x64), 0, 1023).astype(np.uint32)
+    near_edge = np.abs(x64 - np.round(x64)) < 1e-3
+    assert np.all((q == q64) | near_edge) and (q != q64).mean() < 0.01
     codes = lt_checks.morton_bruteforce(q[:, 0], q[:, 1], q[:, 2])
 
     # sorted pairs = std::sort of (code, leaf index)
