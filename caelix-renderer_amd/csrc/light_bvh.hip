@@ -175,21 +175,19 @@ struct Scratch
     {
         return reinterpret_cast<uint64_t*>(u32(kLtScratchKeys + buffer * kLtScratchSortBufferStride));
     }
-    // The wide collapse's inputs, written while the binary tree is built: per binary node its
-    // child words (`kids`: a leaf child as kLtChildLeafBit | leaf id, an internal child as its node
-    // index) and the 4 slots of the 4-wide node it becomes if its depth is even (`quad`: the child
-    // words of its grandchildren, a leaf child in place of its absent children, kLtNone padding).
-    // 24 bytes per leaf of capacity, at a fixed offset (no per-level layout walk).
+    // The wide collapse's input, written while the binary tree is built: per binary node the 4 slots
+    // of the 4-wide node it becomes if its depth is even (`quad`: the child words of its grandchildren,
+    // a leaf child as kLtChildLeafBit | leaf id in place of its absent children, kLtNone padding).
+    // 16 bytes per leaf of capacity, at a fixed offset (no per-level layout walk).
     __host__ __device__ uint8_t* wide_inputs_base() const
     {
         const uint64_t offset = uint64_t(sort_base() - base) + BuildSortStateBytes(capacity);
         return base + ((offset + 255) & ~uint64_t(255));
     }
     __host__ __device__ uint4* quad() const { return reinterpret_cast<uint4*>(wide_inputs_base()); }
-    __host__ __device__ uint2* kids() const { return reinterpret_cast<uint2*>(quad() + capacity); }
     // Open lists of the tree levels: per level-L block, up to kOpenCap elements whose parent the
     // block did not build (kOpenWords words each), and their count.
-    __host__ __device__ uint8_t* open_base() const { return reinterpret_cast<uint8_t*>(kids() + capacity); }
+    __host__ __device__ uint8_t* open_base() const { return reinterpret_cast<uint8_t*>(quad() + capacity); }
     __host__ __device__ uint32_t* open(uint32_t level) const
     {
         uint8_t* p = open_base();
@@ -621,8 +619,8 @@ __global__ void __launch_bounds__(kSortThreads) MortonKernel(Set set)
 // [g + 1, b] is node g + 1, the root is node 0. The first element to arrive at a gap parks and stops;
 // the second builds the parent: range, children, parent links, and the union of the two summaries
 // (left, then right: caelix_light_bvh_aggregate.hlsl:30-55), then climbs on. Also written per node:
-// its split gap, +1/-1 at its range ends in the gap-cover array (for the depths), its child words and
-// its 4-wide slot list (for the wide collapse).
+// its split gap, +1/-1 at its range ends in the gap-cover array (for the depths) and its 4-wide slot
+// list (for the wide collapse).
 //
 // The climb runs in levels of nested blocks of sorted leaves, each level-L workgroup in LDS: every
 // arrival at a gap inside the block happens in the block's workgroup, so the arrivals, parked
@@ -1081,7 +1079,7 @@ CX_DEV void TreeLevel1(const Build& b, int n, uint32_t lo, uint32_t hi, uint32_t
         }
         *open_count = min(total, kOpenCap);
     }
-    // Write-out of the nodes the block built, node lo + t by lane t: children, child words, split gap
+    // Write-out of the nodes the block built, node lo + t by lane t: children, split gap
     // (the left child's index: a leaf at the split, or a left child numbered by its range end), the
     // 4-wide slot list, the parent words of the children and the aggregate (parked at the node's
     // position: node p is the last element to use position p).
@@ -1100,7 +1098,6 @@ CX_DEV void TreeLevel1(const Build& b, int n, uint32_t lo, uint32_t hi, uint32_t
         b.scratch.u32(kLtScratchRight)[g] = right;
         const uint32_t kid0 = ChildWordLds(l, left, lo);
         const uint32_t kid1 = ChildWordLds(l, right, lo);
-        b.scratch.kids()[g] = make_uint2(kid0, kid1);
         b.scratch.split()[g] = left & ~kLtChildLeafBit;
         uint32_t l0 = kid0;
         uint32_t l1 = kLtNone;
@@ -1353,7 +1350,6 @@ __global__ void __launch_bounds__(kUpperThreads) TreeUpperKernel(Set set, uint32
     uint32_t* left_out = b.scratch.u32(kLtScratchLeft);
     uint32_t* right_out = b.scratch.u32(kLtScratchRight);
     uint32_t* parent_out = b.scratch.u32(kLtScratchParent);
-    uint2* kids_out = b.scratch.kids();
     int32_t* diff = b.scratch.cover_diff();
     for (uint32_t k = t; k < kHashEntries; k += kUpperThreads)
     {
@@ -1381,7 +1377,6 @@ __global__ void __launch_bounds__(kUpperThreads) TreeUpperKernel(Set set, uint32
             left, right);
         left_out[node] = left;
         right_out[node] = right;
-        kids_out[node] = make_uint2(parent.kid0, parent.kid1);
     }
     __syncthreads();
     if (t == 0)
