@@ -10,14 +10,15 @@
 //   sort       onesweep passes (radix_sort.h); sorted pairs end in sort buffer 0
 //   tree L1    per block of 512 sorted leaves, in LDS: the binary radix tree as the reference builds
 //              it (Karras 2012 numbering) and its bottom-up aggregation, by one climb from the leaves;
-//              per node its children, parent links, split gap, +1/-1 at the two ends of its range in
-//              the gap-cover array, aggregate and 4-wide slot list; the block's open elements (whose
+//              per node its children, parent links, +1/-1 at the two ends of its range in the
+//              gap-cover array, aggregate and 4-wide slot list; the block's open elements (whose
 //              parent lies outside it) go to its open list
 //   tree L2+   per block of 32 sub-blocks, in LDS: the same climb from the sub-blocks' open elements
 //   cover      inclusive scan of the gap-cover array (decoupled look-back): cover[g] = the number of
 //              binary nodes whose range spans the gap between sorted leaves g and g + 1
-//   flags      depth(i) = cover[split(i)] - 1 (the nodes spanning node i's split gap are i and its
-//              ancestors), even-depth flags, their exclusive scan = 4-wide node indices, and the
+//   flags      depth(i) = cover[split(i)] - 1, split(i) = node i's left child index (the nodes spanning
+//              node i's split gap are i and its ancestors), even-depth flags, their exclusive scan =
+//              4-wide node indices (where it steps: the 4-wide nodes), and the
 //              header (the reference's caelix_light_bvh_finish) in the last tile
 //   wide       4-wide nodes from the even-depth binary nodes
 // The reference computes the depth by walking every node up to the root (O(n * depth) dependent
@@ -147,11 +148,10 @@ struct Scratch
     __host__ __device__ float* aggregates() const { return reinterpret_cast<float*>(base + uint64_t(kLtScratchAggregates) * capacity); }
     // Free arrays reused by this build: the leaf centres live in the aggregates array until the
     // aggregation; the gap-cover diffs in `wide` until the flags scan; after the sort, sort buffer 1
-    // holds the cover counts (keys) and every node's split gap (values).
+    // holds the cover counts (keys).
     __host__ __device__ float* centres() const { return aggregates(); }
     __host__ __device__ int32_t* cover_diff() const { return reinterpret_cast<int32_t*>(u32(kLtScratchWide)); }
     __host__ __device__ int32_t* cover() const { return reinterpret_cast<int32_t*>(keys(1)); }
-    __host__ __device__ uint32_t* split() const { return values(1); }
     __host__ __device__ uint8_t* state_base() const { return base + uint64_t(kLtScratchSortState) * capacity; }
     __host__ __device__ uint32_t* ctl() const { return reinterpret_cast<uint32_t*>(state_base()); }
     __host__ __device__ float* bounds_partials() const { return reinterpret_cast<float*>(state_base() + kCtlWords * 4); }
@@ -619,8 +619,8 @@ __global__ void __launch_bounds__(kSortThreads) MortonKernel(Set set)
 // [g + 1, b] is node g + 1, the root is node 0. The first element to arrive at a gap parks and stops;
 // the second builds the parent: range, children, parent links, and the union of the two summaries
 // (left, then right: caelix_light_bvh_aggregate.hlsl:30-55), then climbs on. Also written per node:
-// its split gap, +1/-1 at its range ends in the gap-cover array (for the depths) and its 4-wide slot
-// list (for the wide collapse).
+// +1/-1 at its range ends in the gap-cover array (for the depths) and its 4-wide slot list (for the
+// wide collapse).
 //
 // The climb runs in levels of nested blocks of sorted leaves, each level-L workgroup in LDS: every
 // arrival at a gap inside the block happens in the block's workgroup, so the arrivals, parked
@@ -719,8 +719,8 @@ CX_DEV uint32_t NodeOf(const Climb& p, int n)
     return p.a == 0 && p.last == uint32_t(n - 1) ? 0u : ElementOf(p.a, p.last, p.d_left, p.d_right);
 }
 
-// The outputs of the node `p` built from `l` and `r` at `gap` that go to global memory: split gap,
-// the leaf children's parent words, aggregate and 4-wide slot list; the internal children's parent
+// The outputs of the node `p` built from `l` and `r` at `gap` that go to global memory: the leaf
+// children's parent words, aggregate and 4-wide slot list; the internal children's parent
 // words and the cover diffs go through the callbacks (LDS at level 1). Returns the node index and the
 // reference-format child words (kLtChildLeafBit | sorted position for a leaf child).
 template <class InternalParent, class CoverAdd>
@@ -751,7 +751,6 @@ CX_DEV uint32_t WriteNode(const Build& b, int n, const Climb& l, const Climb& r,
     {
         internal_parent(right, node);
     }
-    b.scratch.split()[node] = gap;
     cover_add(p.a, p.last);
     StoreSummary(b.scratch.aggregates() + uint64_t(node) * 12, p.own);
     b.scratch.quad()[node] = QuadOf(l.kid0, l.kid1, r.kid0, r.kid1);
@@ -1079,8 +1078,8 @@ CX_DEV void TreeLevel1(const Build& b, int n, uint32_t lo, uint32_t hi, uint32_t
         }
         *open_count = min(total, kOpenCap);
     }
-    // Write-out of the nodes the block built, node lo + t by lane t: children, split gap
-    // (the left child's index: a leaf at the split, or a left child numbered by its range end), the
+    // Write-out of the nodes the block built, node lo + t by lane t: children (the left child's
+    // index is the split gap: a leaf at the split, or a left child numbered by its range end), the
     // 4-wide slot list, the parent words of the children and the aggregate (parked at the node's
     // position: node p is the last element to use position p).
     int32_t* diff = b.scratch.cover_diff();
@@ -1098,7 +1097,6 @@ CX_DEV void TreeLevel1(const Build& b, int n, uint32_t lo, uint32_t hi, uint32_t
         b.scratch.u32(kLtScratchRight)[g] = right;
         const uint32_t kid0 = ChildWordLds(l, left, lo);
         const uint32_t kid1 = ChildWordLds(l, right, lo);
-        b.scratch.split()[g] = left & ~kLtChildLeafBit;
         uint32_t l0 = kid0;
         uint32_t l1 = kLtNone;
         uint32_t r0 = kid1;
@@ -1574,7 +1572,7 @@ __global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Set set)
     const uint32_t capacity = b.leaf_capacity;
     const uint32_t first = tile * kScanTile;
     const int32_t* cover = b.scratch.cover();
-    const uint32_t* split = b.scratch.split();
+    const uint32_t* left = b.scratch.u32(kLtScratchLeft);
     uint32_t* counter = b.scratch.u32(kLtScratchCounter);
     uint32_t* wide = b.scratch.u32(kLtScratchWide);
     // All loads of the thread's items first (splits, then the covers they index), stores after: a
@@ -1585,7 +1583,8 @@ __global__ void __launch_bounds__(kScanThreads) FlagsScanKernel(Set set)
     for (uint32_t k = 0; k < kScanItems; ++k)
     {
         const uint32_t i = base + k * kScanThreads;
-        at[k] = i + 1 < n ? split[i] : 0u;  // gap 0 stands in for items past the gaps
+        // node i's split gap: its left child's index (kLtChildLeafBit | gap for a leaf, else node gap)
+        at[k] = i + 1 < n ? left[i] & ~kLtChildLeafBit : 0u;  // gap 0 stands in for items past the gaps
     }
     uint32_t depth[kScanItems];
 #pragma unroll
@@ -1719,28 +1718,36 @@ __global__ void __launch_bounds__(kWideChunkThreads) WideChunkKernel(Set set)
     }
     const uint32_t c1 = min(c0 + kWideChunk, n - 1);  // binary nodes [c0, c1)
     __shared__ uint4 lds_quad[kWideChunk];
-    __shared__ uint32_t lds_wide[kWideChunk];
+    __shared__ uint32_t lds_wide[kWideChunk + 1];
     __shared__ uint32_t lds_parent[kWideChunk];
     __shared__ uint32_t lds_node[kWideChunk];  // binary node of each 4-wide node of the chunk
     __shared__ float4 lds_rec[kWideChunkThreads / kWave][kWave * 14];  // per wave: 64 records of 14 chunks
     const uint32_t* wide = b.scratch.u32(kLtScratchWide);
-    const uint32_t* depth = b.scratch.u32(kLtScratchCounter);
     const uint32_t* parent = b.scratch.u32(kLtScratchParent);
     const uint4* quad = b.scratch.quad();
     const uint32_t w0 = wide[c0];
     for (uint32_t k = threadIdx.x; k < c1 - c0; k += kWideChunkThreads)
     {
         const uint32_t i = c0 + k;
-        const uint32_t wi = wide[i];
         lds_quad[k] = quad[i];
-        lds_wide[k] = wi;
+        lds_wide[k] = wide[i];
         lds_parent[k] = parent[i];
-        if ((depth[i] & 1u) == 0)
-        {
-            lds_node[wi - w0] = i;
-        }
     }
     const uint32_t w1 = c1 < n - 1 ? wide[c1] : total;
+    if (threadIdx.x == 0)
+    {
+        lds_wide[c1 - c0] = w1;
+    }
+    __syncthreads();
+    // Node i has an even depth (is a 4-wide node) where the exclusive scan of the even-depth flags steps.
+    for (uint32_t k = threadIdx.x; k < c1 - c0; k += kWideChunkThreads)
+    {
+        const uint32_t wi = lds_wide[k];
+        if (lds_wide[k + 1] != wi)
+        {
+            lds_node[wi - w0] = c0 + k;
+        }
+    }
     __syncthreads();
     auto in_chunk = [&](uint32_t node) { return node >= c0 && node < c1; };
     auto wide_of = [&](uint32_t node) { return in_chunk(node) ? lds_wide[node - c0] : wide[node]; };
