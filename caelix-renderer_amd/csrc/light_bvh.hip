@@ -450,7 +450,10 @@ __global__ void __launch_bounds__(kBvhThreads) BoundsKernel(Set set)
     float* partials = b.scratch.bounds_partials();
     if (t < 6)
     {
-        partials[blockIdx.x * 8 + t] = t < 3 ? lo[t] : hi[t - 3];
+        // selects, not lo[t] / hi[t - 3]: a per-lane index would put both arrays in scratch memory
+        const float a = t == 0 ? lo[0] : (t == 1 ? lo[1] : lo[2]);
+        const float z = t == 3 ? hi[0] : (t == 4 ? hi[1] : hi[2]);
+        partials[blockIdx.x * 8 + t] = t < 3 ? a : z;
     }
     if (blockIdx.x == 0)
     {

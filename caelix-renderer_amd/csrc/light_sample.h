@@ -208,7 +208,7 @@ CX_DEV uint32_t ChildCount(const uint32_t* child)
 // LightTreeSelect left out.
 CX_DEV Summ TreeSummary(const LightView& L, uint32_t k, uint32_t& root)
 {
-    root = ((L.trees >> k) & 1u) != 0 ? L.headers[k].root : kLtNone;
+    root = ((L.trees >> k) & 1u) != 0 ? L.Header(k)->root : kLtNone;
     if (root == kLtNone)
     {
         Summ e;
@@ -221,7 +221,7 @@ CX_DEV Summ TreeSummary(const LightView& L, uint32_t k, uint32_t& root)
         e.hi = Make(0.0f, 0.0f, 0.0f);
         return e;
     }
-    return HeaderSummary(&L.headers[k]);
+    return HeaderSummary(L.Header(k));
 }
 
 // LtChooseTreeR (:417-437): a tree drawn by its root's importance (kLtNone when both are empty); pdf
@@ -257,7 +257,7 @@ CX_DEV uint32_t ChooseLeafR(const LightView& L, uint32_t k, uint32_t root, const
     uint32_t node = root;
     for (uint32_t depth = 0; depth < 64; ++depth)
     {
-        const LightBvhNode* address = L.nodes[k] + node;
+        const LightBvhNode* address = L.Nodes(k) + node;
         uint32_t children[4];
         *reinterpret_cast<uint4*>(children) = *reinterpret_cast<const uint4*>(address->child);
         uint32_t chosen;
@@ -381,7 +381,7 @@ CX_DEV float LeafProbabilityR(const LightView& L, uint32_t k, uint32_t leaf, con
 {
     uint32_t root;
     TreeSummary(L, k, root);
-    const uint32_t parent = L.leaves[k][leaf].parent;
+    const uint32_t parent = L.Leaves(k)[leaf].parent;
     if (root == kLtNone || parent == kLtNone)
     {
         return 0.0f;
@@ -391,7 +391,7 @@ CX_DEV float LeafProbabilityR(const LightView& L, uint32_t k, uint32_t leaf, con
     float probability = 1.0f;
     for (uint32_t depth = 0; depth < 64; ++depth)
     {
-        const LightBvhNode* address = L.nodes[k] + node;
+        const LightBvhNode* address = L.Nodes(k) + node;
         probability *= SlotProbabilityR<kPoint>(address, r, slot);
         const uint32_t up = address->parent;
         if (up == kLtNone)
@@ -544,7 +544,7 @@ CX_DEV uint32_t BrickLeaf(const LightView& L, const uint32_t* block, uint32_t gr
     {
         return kLtNone;
     }
-    const LightBvhLeaf& leaf = L.leaves[word >> kLtBvhLeafTreeShift][word & kLtBvhLeafIndexMask];
+    const LightBvhLeaf& leaf = L.Leaves(word >> kLtBvhLeafTreeShift)[word & kLtBvhLeafIndexMask];
     const uint2 owner = *reinterpret_cast<const uint2*>(&leaf.group);
     return owner.x == group && owner.y == entry ? word : kLtNone;
 }
@@ -590,7 +590,7 @@ CX_DEV bool LightTreeSample(const LightView& L, V3 x, V3 n, uint32_t& rng, Light
     {
         return false;
     }
-    const uint2 owner = *reinterpret_cast<const uint2*>(&L.leaves[tree][leaf].group);
+    const uint2 owner = *reinterpret_cast<const uint2*>(&L.Leaves(tree)[leaf].group);
     p.group = owner.x;
     p.entry = owner.y;
     const uint32_t* block = GroupBlock(L, owner.x);
@@ -818,7 +818,7 @@ CX_DEV bool SampleBrickBox(const LightView& L, V3 box_lo, V3 box_hi, V3 n, uint3
         importance = 0.0f;
         return false;
     }
-    const uint2 owner = *reinterpret_cast<const uint2*>(&L.leaves[k][index].group);
+    const uint2 owner = *reinterpret_cast<const uint2*>(&L.Leaves(k)[index].group);
     brick = BrickRef{owner.x, owner.y, L.rows[owner.x].generation};
     leaf = (k << kLtBvhLeafTreeShift) | index;
     return true;
@@ -834,7 +834,7 @@ CX_DEV float LeafImportanceBox(const LightView& L, const BrickRef& brick, V3 box
     {
         return 0.0f;
     }
-    const LightBvhLeaf& l = L.leaves[leaf >> kLtBvhLeafTreeShift][leaf & kLtBvhLeafIndexMask];
+    const LightBvhLeaf& l = L.Leaves(leaf >> kLtBvhLeafTreeShift)[leaf & kLtBvhLeafIndexMask];
     const V3 lo = Make(l.summary.lo[0], l.summary.lo[1], l.summary.lo[2]);
     const V3 hi = Make(l.summary.hi[0], l.summary.hi[1], l.summary.hi[2]);
     return ImportanceR<false>(Receiver{box_lo, box_hi, n}, lo, hi, l.summary.face);

@@ -333,7 +333,7 @@ CX_DEV Shade ShadeHit(const SceneView& s, uint32_t block, V3 hit_position, V3 no
     sh.albedo = BlockAlbedo(s, block);
     (void)RandomFloat01(rng);  // the lobe choice against specular chance 0
     V3 diffuse = SafeNormalize(Add(normal, RandomUnitVector(rng)));
-    diffuse = Dot(diffuse, diffuse) > 0.0f ? diffuse : normal;
+    diffuse = Select(Dot(diffuse, diffuse) > 0.0f, diffuse, normal);
     sh.bounce_origin = Add(hit_position, Mul(normal, kPushOff));
     sh.bounce_direction = diffuse;
     return sh;
@@ -513,7 +513,7 @@ CX_DEV ShadeFull ShadeVoxel(const SceneView& s, uint32_t block, V3 hit_position,
     const float chance = m.metallic + fresnel * m.smoothness * (1.0f - m.metallic);
     const float do_specular = RandomFloat01(rng) < chance ? 1.0f : 0.0f;
     V3 diffuse = SafeNormalize(Add(normal, RandomUnitVector(rng)));
-    diffuse = Dot(diffuse, diffuse) > 0.0f ? diffuse : normal;
+    diffuse = Select(Dot(diffuse, diffuse) > 0.0f, diffuse, normal);
     const V3 specular = SafeNormalize(Lerp(diffuse, Reflect(dir, normal), m.smoothness));
     sh.bounce_origin = Add(hit_position, Mul(normal, kPushOff));
     sh.bounce_direction = Lerp(diffuse, specular, do_specular);
@@ -709,7 +709,7 @@ CX_DEV bool SubsurfaceWalk(const SceneView& s, uint32_t scene, uint32_t record, 
         normal = n;
         origin = Add(exit_point, Mul(normal, kPushOff));
         direction = SafeNormalize(Add(normal, RandomUnitVector(rng)));
-        direction = Dot(direction, direction) > 0.0f ? direction : normal;
+        direction = Select(Dot(direction, direction) > 0.0f, direction, normal);
         weight = w;
         return true;
     }
@@ -843,7 +843,7 @@ CX_DEV V3 ShadowTransmissive(const SceneView& s, V3 origin, V3 dir, float t_max,
             {
                 transmittance = MulV(transmittance, ShadowGlassTransmittance(s, cur, remaining));
             }
-            return ShadowOccluded(transmittance) ? none : transmittance;
+            return Select(ShadowOccluded(transmittance), none, transmittance);
         }
         transmittance = MulV(transmittance, ShadowGlassTransmittance(s, cur, h.t));
         if (ShadowOccluded(transmittance))

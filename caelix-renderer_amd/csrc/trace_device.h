@@ -74,6 +74,12 @@ struct LightView
     uint32_t encoding;               // the block value encoding (kEncoding*)
     uint32_t mis;
     uint32_t trees;                  // LightTreeSelect: the trees draws and pdfs use (bit k = tree k)
+
+    // Tree k's arrays, picked by a select: an index into the arrays themselves with a per-lane k puts
+    // the whole view in per-lane scratch memory.
+    CX_DEV const LightBvhNode* Nodes(uint32_t k) const { return k == 0u ? nodes[0] : nodes[1]; }
+    CX_DEV const LightBvhLeaf* Leaves(uint32_t k) const { return k == 0u ? leaves[0] : leaves[1]; }
+    CX_DEV const LightTreeHeader* Header(uint32_t k) const { return headers + k; }
 };
 
 // What a kernel receives of a light tree: its descriptor's device address, the MIS switch and the
@@ -176,6 +182,11 @@ CX_DEV void Set(V3& v, int i, float s)
 }
 // v * (1 / sqrt(max(FLT_MIN, dot(v, v)))) (SafeNormalize, caelix_shading.hlsl:110-114, with an IEEE
 // reciprocal square root).
+// a ? x : y per component (a select of whole V3 values can become a per-lane scratch copy)
+CX_DEV V3 Select(bool a, V3 x, V3 y)
+{
+    return Make(a ? x.x : y.x, a ? x.y : y.y, a ? x.z : y.z);
+}
 CX_DEV V3 SafeNormalize(V3 v)
 {
     const float dp3 = fmaxf(1.175494351e-38f, Dot(v, v));
